@@ -1,0 +1,351 @@
+/*
+ * pywrf_amd: C ABI of the MI355X (gfx950) kernels for the ARW RK3 step of
+ * PyWRF (the body of WrfSolver.solve's time loop, pywrf/solver.py:454-1910).
+ *
+ * One entry point per reference function on the path.  Arguments keep the
+ * reference's order with tensors as plain device pointers (FP32, contiguous,
+ * (nz, ny, nx) with i fastest; 2-D (ny, nx); level arrays (nz)), Python
+ * floats as double, integer flags as int, staggering strings as one char,
+ * and the WRF index sets packed in pywrf_grid_t.  Every call is asynchronous
+ * on `stream` (a hipStream_t) and returns 0 or an error code: a hipError_t
+ * from the launch, PYWRF_E_ARG for an argument the reference rejects (it
+ * raises NameError for an unknown staggering flag) or PYWRF_E_RANGE for index
+ * sets whose stencils would leave the allocation.
+ *
+ * Where the reference returns a fresh tensor (clone / torch.where / new
+ * arithmetic result) the caller passes the output buffer explicitly
+ * (`*_out`); where it mutates an argument in place the same pointer is both
+ * read and written.
+ */
+#ifndef PYWRF_AMD_H
+#define PYWRF_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* pywrf_stream_t; /* hipStream_t */
+
+#define PYWRF_E_ARG (-1)
+#define PYWRF_E_RANGE (-2)
+
+/* Allocation shape and the WRF index sets (config_params.py:67-99).  Functions
+ * whose reference signature has no patch indices ignore ips..kpe. */
+typedef struct pywrf_grid {
+  int nz, ny, nx;
+  int ids, ide, jds, jde, kds, kde;
+  int ims, ime, jms, jme, kms, kme;
+  int ips, ipe, jps, jpe, kps, kpe;
+  int its, ite, jts, jte, kts, kte;
+} pywrf_grid_t;
+
+/* Library version and the value of a named physical constant (for host-side
+ * consistency checks); returns 0 when the name is known. */
+int pywrf_version(void);
+int pywrf_constant(const char* name, double* value);
+
+/* ---- RK-stage preparation: wrf_dynamics.py:30-159 ---------------------- */
+/* calculate_full, wrf_dynamics.py:30-38 */
+int pywrf_calculate_full(const pywrf_grid_t* g, float* mut, const float* mub, const float* mu, pywrf_stream_t s);
+/* calc_mu_uv, wrf_dynamics.py:41-61 */
+int pywrf_calc_mu_uv(const pywrf_grid_t* g, const float* mu, const float* mub, float* muu, float* muv,
+                     pywrf_stream_t s);
+/* couple_momentum, wrf_dynamics.py:64-86 */
+int pywrf_couple_momentum(const pywrf_grid_t* g, const float* muu, float* ru, const float* u, const float* msfu,
+                          const float* muv, float* rv, const float* v, const float* msfv, const float* msfvx_inv,
+                          const float* mut, float* rw, const float* w, const float* msft, pywrf_stream_t s);
+/* calc_ww_cp, wrf_dynamics.py:89-118 */
+int pywrf_calc_ww_cp(const pywrf_grid_t* g, const float* u, const float* v, const float* muu, const float* muv,
+                     const float* c1h, float* ww, double rdx, double rdy, const float* msftx,
+                     const float* msfuy, const float* msfvx_inv, const float* dnw, pywrf_stream_t s);
+/* calc_cq, wrf_dynamics.py:121-136 (moist: n_moist consecutive 3-D fields) */
+int pywrf_calc_cq(const pywrf_grid_t* g, const float* moist, int n_moist, float* cqu, float* cqv, float* cqw,
+                  pywrf_stream_t s);
+/* calc_alt, wrf_dynamics.py:139-147 */
+int pywrf_calc_alt(const pywrf_grid_t* g, float* alt, const float* al, const float* alb, pywrf_stream_t s);
+/* calc_php, wrf_dynamics.py:150-159 */
+int pywrf_calc_php(const pywrf_grid_t* g, float* php, const float* ph, const float* phb, pywrf_stream_t s);
+
+/* ---- open lateral boundaries: wrf_dynamics.py:164-278 ------------------- */
+/* set_physical_bc2d, wrf_dynamics.py:164-212 (variable: u v w t x y r p, any case) */
+int pywrf_set_physical_bc2d(const pywrf_grid_t* g, float* dat, char variable, pywrf_stream_t s);
+/* set_physical_bc3d, wrf_dynamics.py:216-278 (variable: u v w t d e x y f r p, U V) */
+int pywrf_set_physical_bc3d(const pywrf_grid_t* g, float* dat, char variable, pywrf_stream_t s);
+
+/* ---- first-stage physics prep and Smagorinsky: wrf_dynamics.py:314-1076 - */
+/* phy_prep, wrf_dynamics.py:314-411 (pi_phy_out, t_phy_out: fresh arrays) */
+int pywrf_phy_prep(const pywrf_grid_t* g, const float* mut, const float* u, const float* v, const float* p,
+                   const float* pb, const float* alt, const float* ph, const float* phb, const float* t,
+                   const float* moist, int n_moist, float* rho, float* th_phy, float* p_phy, float* u_phy,
+                   float* v_phy, float* p8w, float* t8w, float* z, float* z_at_w, float* dz8w, float* p_hyd,
+                   float* p_hyd_w, const float* dnw, const float* fzm, const float* fzp, double p_top,
+                   float* pi_phy_out, float* t_phy_out, pywrf_stream_t s);
+/* calculate_phy_tend, wrf_dynamics.py:449-488 (scalar_tend: 3 fields) */
+int pywrf_calculate_phy_tend(const pywrf_grid_t* g, const float* mut, float* rublten, float* rvblten,
+                             float* rthblten, float* rqvblten, float* rqcblten, float* rqiblten,
+                             float* scalar_tend, pywrf_stream_t s);
+/* compute_diff_metrics, wrf_dynamics.py:491-536 */
+int pywrf_compute_diff_metrics(const pywrf_grid_t* g, const float* ph, const float* phb, float* z, float* rdz,
+                               float* rdzw, float* zx, float* zy, double rdx, double rdy, pywrf_stream_t s);
+/* cal_deform_and_div, wrf_dynamics.py:539-813 (cf1..cf3: 0-d arrays) */
+int pywrf_cal_deform_and_div(const pywrf_grid_t* g, const float* u, const float* v, const float* w, float* div,
+                             float* defor11, float* defor22, float* defor33, float* defor12, float* defor13,
+                             float* defor23, const float* msfux, const float* msfuy, const float* msfvx,
+                             const float* msfvy, const float* msftx, const float* msfty, double rdx, double rdy,
+                             const float* dn, const float* dnw, const float* rdz, const float* rdzw,
+                             const float* fnm, const float* fnp, const float* cf, const float* zx,
+                             const float* zy, pywrf_stream_t s);
+/* calculate_km_kh, wrf_dynamics.py:816-995 (BN2_out, xkmh_out: fresh arrays;
+ * xkmh is also updated in place on the interior as the reference does) */
+int pywrf_calculate_km_kh(const pywrf_grid_t* g, float* xkmh, float* xkmv, float* xkhh, float* xkhv,
+                          const float* defor11, const float* defor22, const float* defor12, const float* p8w,
+                          const float* t8w, const float* theta, const float* t, const float* p,
+                          const float* moist, double dx, double dy, const float* rdz, const float* rdzw,
+                          const float* cf, const float* msftx, const float* msfty, float* BN2_out,
+                          float* xkmh_out, pywrf_stream_t s);
+/* update_phy_ten with add_a2a / add_a2c_u / add_a2c_v, wrf_dynamics.py:998-1076 */
+int pywrf_update_phy_ten(const pywrf_grid_t* g, float* rt_tendf, float* ru_tendf, float* rv_tendf,
+                         float* moist_tendf, const float* rthblten, const float* rublten, const float* rvblten,
+                         const float* rqvblten, const float* rqcblten, const float* rqiblten, pywrf_stream_t s);
+
+/* ---- large-step tendencies: wrf_dynamics.py:1112-3148 ------------------- */
+/* advect_u, wrf_dynamics.py:1112-1307 (u_old, c1, c2, mut, time_step unused as in the reference) */
+int pywrf_advect_u(const pywrf_grid_t* g, const float* u, float* tendency, const float* ru, const float* rv,
+                   const float* rom, const float* msfux, const float* fzm, const float* fzp, double rdx, double rdy,
+                   const float* rdzw, pywrf_stream_t s);
+/* advect_v, wrf_dynamics.py:1332-1532 */
+int pywrf_advect_v(const pywrf_grid_t* g, const float* v, float* tendency, const float* ru, const float* rv,
+                   const float* rom, const float* msfvx, const float* msfvy, const float* fzm, const float* fzp,
+                   double rdx, double rdy, const float* rdzw, pywrf_stream_t s);
+/* advect_w, wrf_dynamics.py:1535-1822 */
+int pywrf_advect_w(const pywrf_grid_t* g, const float* w, float* tendency, const float* ru, const float* rv,
+                   const float* rom, const float* msftx, const float* fzm, const float* fzp, double rdx, double rdy,
+                   const float* rdzu, pywrf_stream_t s);
+/* advect_scalar, wrf_dynamics.py:1841-2028 */
+int pywrf_advect_scalar(const pywrf_grid_t* g, const float* field, float* tendency, const float* ru,
+                        const float* rv, const float* rom, const float* msftx, const float* fzm, const float* fzp,
+                        double rdx, double rdy, const float* rdzw, pywrf_stream_t s);
+/* rhs_ph, wrf_dynamics.py:2031-2297 (cfn, cfn1: 0-d arrays) */
+int pywrf_rhs_ph(const pywrf_grid_t* g, float* ph_tend, const float* u, const float* v, const float* ww,
+                 const float* ph, const float* phb, const float* w, const float* mut, const float* muuf,
+                 const float* muvf, const float* fnm, const float* fnp, const float* rdnw, const float* cfn,
+                 const float* cfn1, double rdx, double rdy, const float* msfux, const float* msfvy,
+                 const float* msfty, pywrf_stream_t s);
+/* horizontal_pressure_gradient, wrf_dynamics.py:2300-2421 (cf: cf1, cf2, cf3 as 3 floats) */
+int pywrf_horizontal_pressure_gradient(const pywrf_grid_t* g, float* ru_tend, float* rv_tend, const float* ph,
+                                       const float* alt, const float* p, const float* pb, const float* al,
+                                       const float* php, const float* cqu, const float* cqv, const float* muu,
+                                       const float* muv, const float* mu, const float* fnm, const float* fnp,
+                                       const float* rdnw, const float* cf, double rdx, double rdy,
+                                       const float* msfux, const float* msfuy, const float* msfvx,
+                                       const float* msfvy, pywrf_stream_t s);
+/* pg_buoy_w, wrf_dynamics.py:2424-2460 (cqw overwritten with 1/(1+cqw) as in the reference) */
+int pywrf_pg_buoy_w(const pywrf_grid_t* g, float* rw_tend, const float* p, float* cqw, const float* muf,
+                    const float* mubf, const float* rdnw, const float* rdn, double g_, const float* msfty,
+                    pywrf_stream_t s);
+/* coriolis, wrf_dynamics.py:2463-2532 */
+int pywrf_coriolis(const pywrf_grid_t* g, const float* ru, const float* rv, const float* rw, float* ru_tend,
+                   float* rv_tend, float* rw_tend, const float* msftx, const float* msfty, const float* msfux,
+                   const float* msfuy, const float* msfvx, const float* msfvy, const float* f, const float* e,
+                   const float* sina, const float* cosa, const float* fzm, const float* fzp, pywrf_stream_t s);
+/* curvature, wrf_dynamics.py:2535-2620 */
+int pywrf_curvature(const pywrf_grid_t* g, const float* ru, const float* rv, const float* rw, const float* u,
+                    const float* v, float* ru_tend, float* rv_tend, float* rw_tend, const float* msfuy,
+                    const float* msfvx, const float* msfvy, const float* msftx, const float* msfty,
+                    const float* fzm, const float* fzp, double rdx, double rdy, pywrf_stream_t s);
+/* horizontal_diffusion, wrf_dynamics.py:2623-2838 (name: 'u', 'v', 'w' or a scalar name) */
+int pywrf_horizontal_diffusion(const pywrf_grid_t* g, char name, const float* field, float* tendency,
+                               const float* mut, const float* msfux, const float* msfuy, const float* msfvx,
+                               const float* msfvx_inv, const float* msfvy, const float* msftx, const float* msfty,
+                               const float* xkmhd, double rdx, double rdy, pywrf_stream_t s);
+/* horizontal_diffusion_3dmp, wrf_dynamics.py:2841-2915 */
+int pywrf_horizontal_diffusion_3dmp(const pywrf_grid_t* g, const float* field, float* tendency,
+                                    const float* mut, const float* base_3d, const float* msfux,
+                                    const float* msfuy, const float* msfvx, const float* msfvx_inv,
+                                    const float* msfvy, const float* msftx, const float* msfty,
+                                    const float* xkmhd, double rdx, double rdy, pywrf_stream_t s);
+/* vertical_diffusion, wrf_dynamics.py:2917-2959 (name 'w' or 'm'; others leave tendency untouched) */
+int pywrf_vertical_diffusion(const pywrf_grid_t* g, char name, const float* field, float* tendency,
+                             const float* alt, const float* mut, const float* rdn, const float* rdnw, double kvdif,
+                             pywrf_stream_t s);
+/* vertical_diffusion_u, wrf_dynamics.py:2962-2991 */
+int pywrf_vertical_diffusion_u(const pywrf_grid_t* g, const float* field, float* tendency, const float* u_base,
+                               const float* alt, const float* muu, const float* rdn, const float* rdnw,
+                               double kvdif, pywrf_stream_t s);
+/* vertical_diffusion_v, wrf_dynamics.py:2994-3024 */
+int pywrf_vertical_diffusion_v(const pywrf_grid_t* g, const float* field, float* tendency, const float* v_base,
+                               const float* alt, const float* muv, const float* rdn, const float* rdnw,
+                               double kvdif, pywrf_stream_t s);
+/* vertical_diffusion_mp, wrf_dynamics.py:3026-3055 */
+int pywrf_vertical_diffusion_mp(const pywrf_grid_t* g, const float* field, float* tendency, const float* base,
+                                const float* alt, const float* mut, const float* rdn, const float* rdnw,
+                                double kvdif, pywrf_stream_t s);
+/* vertical_diffusion_3dmp, wrf_dynamics.py:3058-3086 */
+int pywrf_vertical_diffusion_3dmp(const pywrf_grid_t* g, const float* field, float* tendency,
+                                  const float* base_3d, const float* alt, const float* mut, const float* rdn,
+                                  const float* rdnw, double kvdif, pywrf_stream_t s);
+/* rk_addtend_dry, wrf_dynamics.py:3100-3148 */
+int pywrf_rk_addtend_dry(const pywrf_grid_t* g, float* ru_tend, float* rv_tend, float* rw_tend, float* ph_tend,
+                         float* t_tend, float* ru_tendf, float* rv_tendf, float* rw_tendf, float* ph_tendf,
+                         float* t_tendf, const float* u_save, const float* v_save, const float* w_save,
+                         const float* ph_save, const float* t_save, float* mu_tend, const float* mu_tendf,
+                         int rk_step, const float* h_diabatic, const float* mut, const float* msfty,
+                         const float* msfuy, const float* msfvx, const float* msfvx_inv, pywrf_stream_t s);
+
+/* ---- acoustic small step: wrf_dynamics.py:3152-3930 -------------------- */
+/* small_step_prep, wrf_dynamics.py:3152-3266 (u_out..ph_out: the five fresh perturbation arrays) */
+int pywrf_small_step_prep(const pywrf_grid_t* g, float* u_1, const float* u_2, float* v_1, const float* v_2,
+                          float* w_1, const float* w_2, float* t_1, const float* t_2, float* ph_1,
+                          const float* ph_2, const float* mub, float* mu_1, float* mu_2, const float* muu,
+                          float* muus, const float* muv, float* muvs, const float* mut, float* muts, float* mudf,
+                          float* u_save, float* v_save, float* w_save, float* t_save, float* ph_save,
+                          float* mu_save, const float* ww, float* ww_save, float* c2a, const float* pb,
+                          const float* p, const float* alt, const float* msfuy, const float* msfvx_inv,
+                          const float* msfty, int rk_step, float* u_out, float* v_out, float* w_out, float* t_out,
+                          float* ph_out, pywrf_stream_t s);
+/* calc_p_rho, wrf_dynamics.py:3269-3313 (al_out, p_out fresh; pm1 in place, or the fresh copy at step 0) */
+int pywrf_calc_p_rho(const pywrf_grid_t* g, const float* al_in, const float* p_in, const float* ph,
+                     const float* alt, const float* t_2, const float* t_1, const float* c2a, float* pm1,
+                     const float* mu, const float* mut, double t0, const float* rdnw, double smdiv, int step,
+                     float* al_out, float* p_out, pywrf_stream_t s);
+/* calc_coef_w, wrf_dynamics.py:3316-3388 */
+int pywrf_calc_coef_w(const pywrf_grid_t* g, float* a, float* alpha, float* gamma, const float* mut,
+                      const float* cqw, const float* rdn, const float* rdnw, const float* c2a, double dts,
+                      double g_, double epssm, int top_lid, pywrf_stream_t s);
+/* advance_uv, wrf_dynamics.py:3391-3525 (u_out, v_out fresh) */
+int pywrf_advance_uv(const pywrf_grid_t* g, const float* u_in, const float* ru_tend, const float* v_in,
+                     const float* rv_tend, const float* p, const float* pb, const float* ph, const float* php,
+                     const float* alt, const float* al, const float* mu, const float* muu, const float* cqu,
+                     const float* muv, const float* cqv, const float* mudf, const float* msfux, const float* msfuy,
+                     const float* msfvx, const float* msfvx_inv, const float* msfvy, double rdx, double rdy,
+                     double dts, const float* cf, const float* fnm, const float* fnp, double emdiv,
+                     const float* rdnw, int spec_zone, float* u_out, float* v_out, pywrf_stream_t s);
+/* advance_mu_t, wrf_dynamics.py:3528-3638 (the seven *_out arrays fresh) */
+int pywrf_advance_mu_t(const pywrf_grid_t* g, const float* ww_in, const float* ww_1, const float* u,
+                       const float* u_1, const float* v, const float* v_1, const float* mu_in, const float* mut,
+                       const float* muave_in, const float* muts_in, const float* muu, const float* muv,
+                       const float* mudf_in, const float* t_in, const float* t_1, const float* t_ave_in,
+                       const float* ft, const float* mu_tend, double rdx, double rdy, double dts, double epssm,
+                       const float* dnw, const float* fnm, const float* fnp, const float* rdnw,
+                       const float* msfuy, const float* msfvx_inv, const float* msftx, const float* msfty,
+                       float* ww_out, float* t_out, float* t_ave_out, float* muave_out, float* muts_out,
+                       float* mudf_out, float* mu_out, pywrf_stream_t s);
+/* advance_w, wrf_dynamics.py:3641-3796 (t_2ave_out, w_out, ph_out fresh) */
+int pywrf_advance_w(const pywrf_grid_t* g, const float* w_in, const float* rw_tend, const float* ww,
+                    const float* w_save, const float* u, const float* v, const float* mut, const float* muave,
+                    const float* muts, const float* t_2ave_in, const float* t_2, const float* t_1,
+                    const float* ph_in, const float* ph_1, const float* phb, const float* ph_tend,
+                    const float* ht, const float* c2a, const float* cqw, const float* alt, const float* a,
+                    const float* alpha, const float* gamma, double rdx, double rdy, double dts, double t0,
+                    double epssm, const float* fnm, const float* fnp, const float* rdnw, const float* rdn,
+                    const float* cf, const float* msftx, const float* msfty, float* t_2ave_out, float* w_out,
+                    float* ph_out, pywrf_stream_t s);
+/* sumflux, wrf_dynamics.py:3799-3839 */
+int pywrf_sumflux(const pywrf_grid_t* g, const float* ru, const float* rv, const float* ww, const float* u_lin,
+                  const float* v_lin, const float* ww_lin, const float* muu, const float* muv, float* ru_m,
+                  float* rv_m, float* ww_m, const float* msfuy, const float* msfvx_inv, int iteration,
+                  int number_of_small_timesteps, pywrf_stream_t s);
+/* calc_mu_uv_1, wrf_dynamics.py:3844-3860 */
+int pywrf_calc_mu_uv_1(const pywrf_grid_t* g, const float* mu, float* muu, float* muv, pywrf_stream_t s);
+/* small_step_finish, wrf_dynamics.py:3863-3930 (u..t_out fresh; ww, mu_2 in place) */
+int pywrf_small_step_finish(const pywrf_grid_t* g, const float* u_2, const float* v_2, const float* w_2,
+                            const float* t_2, const float* ph_2, float* ww, const float* ww1, float* mu_2,
+                            const float* mut, const float* muts, const float* muu, const float* muus,
+                            const float* muv, const float* muvs, const float* u_save, const float* v_save,
+                            const float* w_save, const float* t_save, const float* ph_save, const float* mu_save,
+                            const float* msfuy, const float* msfvx, const float* msfty, const float* h_diabatic,
+                            int number_of_small_timesteps, double dts, int rk_step, int rk_order, float* u_out,
+                            float* v_out, float* w_out, float* ph_out, float* t_out, pywrf_stream_t s);
+
+/* ---- scalar transport, EOS, surface w: wrf_dynamics.py:3982-4324 ------- */
+/* rk_scalar_tend, wrf_dynamics.py:3982-4051 (scalar, scalar_tends, advect_tend: (n, nz, ny, nx)) */
+int pywrf_rk_scalar_tend(const pywrf_grid_t* g, int scs, int sce, int rk_step, const float* ru, const float* rv,
+                         const float* ww, const float* mut, const float* alt, const float* scalar,
+                         float* scalar_tends, float* advect_tend, float* h_tendency, float* z_tendency,
+                         const float* base, int moist_step, const float* fnm, const float* fnp,
+                         const float* msfux, const float* msfuy, const float* msfvx, const float* msfvx_inv,
+                         const float* msfvy, const float* msftx, const float* msfty, double rdx, double rdy,
+                         const float* rdn, const float* rdnw, double kvdif, const float* xkmhd,
+                         pywrf_stream_t s);
+/* rk_update_scalar, wrf_dynamics.py:4055-4119 */
+int pywrf_rk_update_scalar(const pywrf_grid_t* g, int scs, int sce, float* scalar_1, float* scalar_2,
+                           const float* sc_tend, const float* advect_tend, const float* msfty, const float* mu_old,
+                           const float* mu_new, const float* mu_base, int rk_step, double dt, int spec_zone,
+                           pywrf_stream_t s);
+/* calc_p_rho_phi, wrf_dynamics.py:4124-4164 (phi is not re-integrated, as in the reference) */
+int pywrf_calc_p_rho_phi(const pywrf_grid_t* g, const float* moist, float* al, const float* alb, const float* mu,
+                         const float* muts, const float* ph, float* p, const float* pb, const float* t, double p0,
+                         double t0, const float* rdnw, pywrf_stream_t s);
+/* phy_prep_part2, wrf_dynamics.py:4179-4213 */
+int pywrf_phy_prep_part2(const pywrf_grid_t* g, const float* mut, float* rthblten, float* rublten,
+                         float* rvblten, float* rqvblten, float* rqcblten, float* rqiblten, pywrf_stream_t s);
+/* set_w_surface, wrf_dynamics.py:4218-4324 */
+int pywrf_set_w_surface(const pywrf_grid_t* g, float* w, const float* ht, const float* u, const float* v,
+                        const float* cf, double rdx, double rdy, const float* msftx, const float* msfty,
+                        pywrf_stream_t s);
+
+/* ---- lateral boundaries: wrf_boundary.py:52-587 ------------------------ */
+/* Boundary arrays: x-boundaries (kz, ny, 5), y-boundaries (kz, 5, nx), contiguous. */
+/* relax_bdytend, wrf_boundary.py:52-161 (fcx, gcx: 5 floats each) */
+int pywrf_relax_bdytend(const pywrf_grid_t* g, const float* field, float* field_tend, const float* bxs,
+                        const float* bxe, const float* bys, const float* bye, const float* btxs, const float* btxe,
+                        const float* btys, const float* btye, char variable_in, double dtbc, const float* fcx,
+                        const float* gcx, pywrf_stream_t s);
+/* spec_bdytend, wrf_boundary.py:188-225 */
+int pywrf_spec_bdytend(const pywrf_grid_t* g, float* field_tend, const float* btxs, const float* btxe,
+                       const float* btys, const float* btye, char variable_in, pywrf_stream_t s);
+/* spec_bdyupdate, wrf_boundary.py:228-284 (kds == kde == 0: 2-D field) */
+int pywrf_spec_bdyupdate(const pywrf_grid_t* g, float* field, const float* field_tend, double dt, char variable,
+                         pywrf_stream_t s);
+/* spec_bdyupdate_ph, wrf_boundary.py:287-358 */
+int pywrf_spec_bdyupdate_ph(const pywrf_grid_t* g, const float* ph_save, float* field, const float* field_tend,
+                            const float* mu_tend, const float* muts, double dt, char variable_in,
+                            pywrf_stream_t s);
+/* zero_grad_bdy, wrf_boundary.py:361-405 */
+int pywrf_zero_grad_bdy(const pywrf_grid_t* g, float* field, char variable_in, pywrf_stream_t s);
+/* mass_weight, wrf_boundary.py:408-415 (tile = its..ite x jts..jte x kts..kte of g) */
+int pywrf_mass_weight(const pywrf_grid_t* g, const float* field, const float* mut, float* rfield,
+                      pywrf_stream_t s);
+/* relax_bdy_scalar, wrf_boundary.py:418-451 */
+int pywrf_relax_bdy_scalar(const pywrf_grid_t* g, float* scalar_tend, const float* scalar, const float* mu,
+                           const float* bxs, const float* bxe, const float* bys, const float* bye,
+                           const float* btxs, const float* btxe, const float* btys, const float* btye, double dtbc,
+                           const float* fcx, const float* gcx, pywrf_stream_t s);
+/* spec_bdy_scalar, wrf_boundary.py:454-472 */
+int pywrf_spec_bdy_scalar(const pywrf_grid_t* g, float* scalar_tend, const float* btxs, const float* btxe,
+                          const float* btys, const float* btye, pywrf_stream_t s);
+/* flow_dep_bdy, wrf_boundary.py:475-510 */
+int pywrf_flow_dep_bdy(const pywrf_grid_t* g, float* field, const float* u, const float* v, pywrf_stream_t s);
+/* spec_bdy_final, wrf_boundary.py:513-587 (msf / mu ignored where the reference does not couple) */
+int pywrf_spec_bdy_final(const pywrf_grid_t* g, float* field, const float* mu, const float* msf, const float* bxs,
+                         const float* bxe, const float* bys, const float* bye, const float* btxs,
+                         const float* btxe, const float* btys, const float* btye, char variable_in, double dtbc,
+                         pywrf_stream_t s);
+
+/* ---- moist physics: wrf_physics.py:27-2137 ------------------------------ */
+/* moist_physics_prep_em, wrf_physics.py:27-91 */
+int pywrf_moist_physics_prep_em(const pywrf_grid_t* g, const float* t_new, double t0, float* rho, const float* al,
+                                const float* alb, const float* p, float* p8w, double p0, const float* pb,
+                                const float* ph, const float* phb, float* th_phy, float* pii, float* pf, float* z,
+                                float* z_at_w, float* dz8w, float* h_diabatic, const float* qv, float* qv_diabatic,
+                                const float* qc, float* qc_diabatic, const float* fzm, const float* fzp,
+                                pywrf_stream_t s);
+/* microphysics_driver, wrf_physics.py:94-134, forwarding to wsm6 */
+int pywrf_microphysics_driver(const pywrf_grid_t* g, float* th, const float* rho, const float* pi_phy,
+                              const float* p, const float* dz8w, double dt, float* qv, float* qc, float* qr,
+                              float* qi, float* qs, float* qg, float* rainnc, float* rainncv, float* snownc,
+                              float* snowncv, float* sr, float* graupelnc, float* graupelncv, pywrf_stream_t s);
+/* wsm6 with nislfv_rain_plm / nislfv_rain_plm6 / slope_*, wrf_physics.py:137-1584, 1988-2103 */
+int pywrf_wsm6(const pywrf_grid_t* g, float* th, float* q, float* qc, float* qr, float* qi, float* qs, float* qg,
+               const float* den, const float* pii, const float* p, const float* delz, double delt, float* rain,
+               float* rainncv, float* snow, float* snowncv, float* sr, float* graupel, float* graupelncv,
+               pywrf_stream_t s);
+/* moist_physics_finish_em, wrf_physics.py:2106-2137 */
+int pywrf_moist_physics_finish_em(const pywrf_grid_t* g, float* t_new, const float* th_phy, float* h_diabatic,
+                                  double dt, const float* qv, float* qv_diabatic, const float* qc,
+                                  float* qc_diabatic, pywrf_stream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,351 @@
+// Flux-form 5th/3rd-order advection of u, v, w and mass-point scalars
+// (pywrf/wrf_dynamics.py:1112-2028).
+//
+// One launch per call.  A thread owns one tendency point and evaluates the
+// face fluxes on both sides of it in y, x and z (the reference stores each
+// face-flux plane in a full-grid scratch array and differences it; here the
+// face flux is a pure function of the inputs, so neighbouring threads simply
+// recompute shared faces from cached loads).  The three directional updates
+// are applied to the tendency in the reference's order, y then x then z.
+//
+// Face rules, including the order reductions on the rows and columns next
+// to the domain edge and the sign-source quirks of some 3rd-order faces, are
+// those of the reference; a face the reference never sets carries zero flux.
+// The upwind sign is sign(vel) * sign(time_step) with the reference's
+// module-level time_step = 60 > 0.
+#include "common.h"
+#include "consts.h"
+
+#define AT(p, k, j, i) (p)[D.ix((k), (j), (i))]
+#define AT2(p, j, i) (p)[D.ix2((j), (i))]
+
+int pw_check_grid(const pywrf_grid_t* g, int r);
+#define CHECK(g, r) do { int _e = pw_check_grid((g), (r)); if (_e) return _e; } while (0)
+
+struct AdvIdx {
+  int ids, ide, jds, jde, ktf;
+};
+
+// ------------------------------------------------------------------ advect_u
+struct AdvU {
+  const float *u, *ru, *rv, *rom, *msfux, *fzm, *fzp, *rdzw;
+  float *tend;
+  float rdx, rdy;
+};
+
+__device__ inline float u_yface(const Dims& D, const AdvU& a, const AdvIdx& x, int k, int jf, int i) {
+  auto U = [&](int o) { return AT(a.u, k, jf + o, i); };
+  float rvs = AT(a.rv, k, jf, i) + AT(a.rv, k, jf, i - 1);
+  if (jf >= x.jds + 3 && jf < x.jde - 3) {
+    float vel = 0.5f * rvs;
+    return vel * flux5(U(-3), U(-2), U(-1), U(0), U(1), U(2), vel);
+  }
+  if (jf == x.jds + 1 || jf == x.jde - 2) return 0.25f * rvs * (U(0) + U(-1));
+  if (jf == x.jds + 2) {
+    float vel = 0.5f * rvs;
+    return vel * flux3(U(-2), U(-1), U(0), U(1), vel);
+  }
+  if (jf == x.jde - 3) {  // sign from the velocity scratch at row jde-2, never set (zero)
+    float vel = 0.5f * rvs;
+    return vel * flux3(U(-2), U(-1), U(0), U(1), 0.0f);
+  }
+  return 0.0f;
+}
+
+__device__ inline float u_xface(const Dims& D, const AdvU& a, const AdvIdx& x, int k, int j, int f) {
+  auto U = [&](int o) { return AT(a.u, k, j, f + o); };
+  float rus = AT(a.ru, k, j, f) + AT(a.ru, k, j, f - 1);
+  if (f >= x.ids + 3 && f < x.ide - 2) {
+    float vel = 0.5f * rus;
+    return vel * flux5(U(-3), U(-2), U(-1), U(0), U(1), U(2), vel);
+  }
+  if (f == x.ids + 1) {
+    float ub = U(0) < 0.0f ? U(0) : U(-1);
+    return 0.25f * rus * (U(0) + ub);
+  }
+  if (f == x.ids + 2 || f == x.ide - 2) {
+    float vel = 0.5f * rus;
+    return vel * flux3(U(-2), U(-1), U(0), U(1), vel);
+  }
+  if (f == x.ide - 1) {
+    float ub = U(-1) > 0.0f ? U(-1) : U(0);
+    return 0.25f * rus * (U(-1) + ub);
+  }
+  return 0.0f;
+}
+
+__device__ inline float u_zface(const Dims& D, const AdvU& a, const AdvIdx& x, int kf, int j, int i) {
+  auto U = [&](int o) { return AT(a.u, kf + o, j, i); };
+  if (kf < 1 || kf >= x.ktf) return 0.0f;
+  float vel = 0.5f * (AT(a.rom, kf, j, i - 1) + AT(a.rom, kf, j, i));
+  if (kf >= 3 && kf < x.ktf - 2) return vel * flux5(U(-3), U(-2), U(-1), U(0), U(1), U(2), -vel);
+  if (kf == 1 || kf == x.ktf - 1) return vel * (a.fzm[kf] * U(0) + a.fzp[kf] * U(-1));
+  return vel * flux3(U(-2), U(-1), U(0), U(1), -vel);  // kf == 2 or ktf-2
+}
+
+__global__ void k_advect_u(Box b, Dims D, AdvU a, AdvIdx x, Box by, Box bz) {
+  int k, j, i;
+  if (!box_point(b, k, j, i)) return;
+  long q = D.ix(k, j, i);
+  float t = a.tend[q];
+  if (by.has(k, j, i))
+    t = t - (AT2(a.msfux, j, i) * a.rdy) * (u_yface(D, a, x, k, j + 1, i) - u_yface(D, a, x, k, j, i));
+  t = t - (AT2(a.msfux, j, i) * a.rdx) * (u_xface(D, a, x, k, j, i + 1) - u_xface(D, a, x, k, j, i));
+  t = t - a.rdzw[k] * (u_zface(D, a, x, k + 1, j, i) - u_zface(D, a, x, k, j, i));
+  a.tend[q] = t;
+}
+
+extern "C" int pywrf_advect_u(const pywrf_grid_t* g, const float* u, float* tendency, const float* ru,
+                              const float* rv, const float* rom, const float* msfux, const float* fzm,
+                              const float* fzp, double rdx, double rdy, const float* rdzw, pywrf_stream_t s) {
+  CHECK(g, 4);
+  AdvIdx x = {g->ids, g->ide, g->jds, g->jde, imin(g->kte, g->kde - 1)};
+  if (g->kts != 0 || x.ktf + 1 > g->nz) return PYWRF_E_RANGE;
+  AdvU a = {u, ru, rv, rom, msfux, fzm, fzp, rdzw, tendency, F(rdx), F(rdy)};
+  int i0 = imax(g->ids + 1, g->its), i1 = imin(g->ide - 1, g->ite);
+  int j0 = g->jts, j1 = imin(g->jte, g->jde - 1);
+  Box b = mkbox(g->kts, x.ktf, j0, j1, i0, i1);
+  Box by = mkbox(g->kts, x.ktf, imax(g->jts, g->jds + 1), imin(g->jte, g->jde - 2), i0, i1);
+  Box bz = b;
+  PW_LAUNCH_BOX(k_advect_u, b, s, dims_of(g), a, x, by, bz);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ advect_v
+struct AdvV {
+  const float *v, *ru, *rv, *rom, *msfvx, *msfvy, *fzm, *fzp, *rdzw;
+  float *tend;
+  float rdx, rdy;
+};
+
+__device__ inline float v_yface(const Dims& D, const AdvV& a, const AdvIdx& x, int k, int f, int i) {
+  auto V = [&](int o) { return AT(a.v, k, f + o, i); };
+  float rvs = AT(a.rv, k, f, i) + AT(a.rv, k, f - 1, i);
+  if (f >= x.jds + 3 && f < x.jde - 2) {
+    float vel = 0.5f * rvs;
+    return vel * flux5(V(-3), V(-2), V(-1), V(0), V(1), V(2), vel);
+  }
+  if (f == x.jds + 1) {
+    float vb = V(0) < 0.0f ? V(0) : V(-1);
+    return 0.25f * rvs * (V(0) + vb);
+  }
+  if (f == x.jds + 2 || f == x.jde - 2) {
+    float vel = 0.5f * rvs;
+    return vel * flux3(V(-2), V(-1), V(0), V(1), vel);
+  }
+  if (f == x.jde - 1) {
+    float vb = V(-1) > 0.0f ? V(-1) : V(0);
+    return 0.25f * rvs * (V(-1) + vb);
+  }
+  return 0.0f;
+}
+
+__device__ inline float v_xface(const Dims& D, const AdvV& a, const AdvIdx& x, int k, int j, int f) {
+  auto V = [&](int o) { return AT(a.v, k, j, f + o); };
+  float rus = AT(a.ru, k, j, f) + AT(a.ru, k, j - 1, f);
+  if (f >= x.ids + 3 && f < x.ide - 3) {
+    float vel = 0.5f * rus;
+    return vel * flux5(V(-3), V(-2), V(-1), V(0), V(1), V(2), vel);
+  }
+  if (f == x.ids + 1 || f == x.ide - 2) return 0.25f * rus * (V(0) + V(-1));
+  if (f == x.ids + 2 || f == x.ide - 3) {
+    float vel = 0.5f * rus;
+    return vel * flux3(V(-2), V(-1), V(0), V(1), vel);
+  }
+  return 0.0f;
+}
+
+__device__ inline float v_zface(const Dims& D, const AdvV& a, const AdvIdx& x, int kf, int j, int i) {
+  auto V = [&](int o) { return AT(a.v, kf + o, j, i); };
+  if (kf < 1 || kf >= x.ktf) return 0.0f;
+  float vel = 0.5f * (AT(a.rom, kf, j, i) + AT(a.rom, kf, j - 1, i));
+  if (kf >= 3 && kf < x.ktf - 2) return vel * flux5(V(-3), V(-2), V(-1), V(0), V(1), V(2), -vel);
+  if (kf == 1 || kf == x.ktf - 1) return vel * (a.fzm[kf] * V(0) + a.fzp[kf] * V(-1));
+  if (kf == 2) return vel * flux3(V(-2), V(-1), V(0), V(1), -vel);
+  return vel * flux3(V(-2), V(-1), V(0), V(1), vel);  // kf == ktf-2: +vel in the reference
+}
+
+__global__ void k_advect_v(Box b, Dims D, AdvV a, AdvIdx x, Box bx) {
+  int k, j, i;
+  if (!box_point(b, k, j, i)) return;
+  long q = D.ix(k, j, i);
+  float t = a.tend[q];
+  float my = AT2(a.msfvy, j, i);
+  t = t - (my * a.rdy) * (v_yface(D, a, x, k, j + 1, i) - v_yface(D, a, x, k, j, i));
+  if (bx.has(k, j, i)) t = t - (my * a.rdx) * (v_xface(D, a, x, k, j, i + 1) - v_xface(D, a, x, k, j, i));
+  t = t - (my / AT2(a.msfvx, j, i)) * a.rdzw[k] * (v_zface(D, a, x, k + 1, j, i) - v_zface(D, a, x, k, j, i));
+  a.tend[q] = t;
+}
+
+extern "C" int pywrf_advect_v(const pywrf_grid_t* g, const float* v, float* tendency, const float* ru,
+                              const float* rv, const float* rom, const float* msfvx, const float* msfvy,
+                              const float* fzm, const float* fzp, double rdx, double rdy, const float* rdzw,
+                              pywrf_stream_t s) {
+  CHECK(g, 4);
+  AdvIdx x = {g->ids, g->ide, g->jds, g->jde, imin(g->kte, g->kde - 1)};
+  if (g->kts != 0 || x.ktf + 1 > g->nz) return PYWRF_E_RANGE;
+  AdvV a = {v, ru, rv, rom, msfvx, msfvy, fzm, fzp, rdzw, tendency, F(rdx), F(rdy)};
+  int i0 = g->its, i1 = imin(g->ite, g->ide - 1);
+  int j0 = imax(g->jts, g->jds + 1), j1 = imin(g->jte, g->jde - 1);
+  Box b = mkbox(g->kts, x.ktf, j0, j1, i0, i1);
+  Box bx = mkbox(g->kts, x.ktf, j0, j1, imax(g->ids + 1, g->its), imin(g->ide - 2, g->ite));
+  PW_LAUNCH_BOX(k_advect_v, b, s, dims_of(g), a, x, bx);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ advect_w
+struct AdvW {
+  const float *w, *ru, *rv, *rom, *msftx, *fzm, *fzp, *rdzu;
+  float *tend;
+  float rdx, rdy;
+};
+
+// Horizontal mass flux interpolated to w level k (k < ktf), or extrapolated
+// to the model top (k == ktf).
+__device__ inline float w_level_flux(const Dims& D, const AdvW& a, const float* r, int ktf, int k, int j, int i) {
+  if (k < ktf) return a.fzm[k] * AT(r, k, j, i) + a.fzp[k] * AT(r, k - 1, j, i);
+  return (2.0f - a.fzm[ktf - 1]) * AT(r, ktf - 1, j, i) - a.fzp[ktf - 1] * AT(r, ktf - 2, j, i);
+}
+
+__device__ inline float w_yface(const Dims& D, const AdvW& a, const AdvIdx& x, int k, int f, int i) {
+  auto W = [&](int o) { return AT(a.w, k, f + o, i); };
+  if (f >= x.jds + 3 && f < x.jde - 3) {
+    float vel = w_level_flux(D, a, a.rv, x.ktf, k, f, i);
+    return vel * flux5(W(-3), W(-2), W(-1), W(0), W(1), W(2), vel);
+  }
+  if (f == x.jds + 1 || f == x.jde - 2) return 0.5f * w_level_flux(D, a, a.rv, x.ktf, k, f, i) * (W(0) + W(-1));
+  if (f == x.jds + 2 || f == x.jde - 3) {
+    float vel = w_level_flux(D, a, a.rv, x.ktf, k, f, i);
+    return vel * flux3(W(-2), W(-1), W(0), W(1), vel);
+  }
+  return 0.0f;
+}
+
+__device__ inline float w_xface(const Dims& D, const AdvW& a, const AdvIdx& x, int k, int j, int f) {
+  auto W = [&](int o) { return AT(a.w, k, j, f + o); };
+  if (f >= x.ids + 3 && f < x.ide - 3) {
+    float vel = w_level_flux(D, a, a.ru, x.ktf, k, j, f);
+    return vel * flux5(W(-3), W(-2), W(-1), W(0), W(1), W(2), vel);
+  }
+  if (f == x.ids + 1 || f == x.ide - 2) return 0.5f * w_level_flux(D, a, a.ru, x.ktf, k, j, f) * (W(0) + W(-1));
+  if (f == x.ids + 2 || f == x.ide - 3) {
+    float vel = w_level_flux(D, a, a.ru, x.ktf, k, j, f);
+    return vel * flux3(W(-2), W(-1), W(0), W(1), vel);
+  }
+  return 0.0f;
+}
+
+// Vertical flux at mass level kf (between w[kf-1] and w[kf]), kf in [1, ktf].
+__device__ inline float w_zface(const Dims& D, const AdvW& a, const AdvIdx& x, int kf, int j, int i) {
+  auto W = [&](int o) { return AT(a.w, kf + o, j, i); };
+  if (kf < 1 || kf > x.ktf) return 0.0f;
+  float rs = AT(a.rom, kf, j, i) + AT(a.rom, kf - 1, j, i);
+  if (kf == 1 || kf == x.ktf) return 0.25f * rs * (W(0) + W(-1));
+  float vel = 0.5f * rs;
+  if (kf >= 3 && kf < x.ktf - 1) return vel * flux5(W(-3), W(-2), W(-1), W(0), W(1), W(2), -vel);
+  if (kf == 2) return vel * flux3(W(-2), W(-1), W(0), W(1), -vel);
+  return vel * flux3(W(-2), W(-1), W(0), W(1), vel);  // kf == ktf-1: +vel in the reference
+}
+
+__global__ void k_advect_w(Box b, Dims D, AdvW a, AdvIdx x, Box by, Box bx) {
+  int k, j, i;
+  if (!box_point(b, k, j, i)) return;
+  long q = D.ix(k, j, i);
+  float t = a.tend[q];
+  float mx = AT2(a.msftx, j, i);
+  if (by.has(k, j, i)) t = t - (mx * a.rdy) * (w_yface(D, a, x, k, j + 1, i) - w_yface(D, a, x, k, j, i));
+  if (bx.has(k, j, i)) t = t - (mx * a.rdx) * (w_xface(D, a, x, k, j, i + 1) - w_xface(D, a, x, k, j, i));
+  if (k < x.ktf) t = t - a.rdzu[k] * (w_zface(D, a, x, k + 1, j, i) - w_zface(D, a, x, k, j, i));
+  else t = t + 2.0f * a.rdzu[x.ktf - 1] * w_zface(D, a, x, k, j, i);
+  a.tend[q] = t;
+}
+
+extern "C" int pywrf_advect_w(const pywrf_grid_t* g, const float* w, float* tendency, const float* ru,
+                              const float* rv, const float* rom, const float* msftx, const float* fzm,
+                              const float* fzp, double rdx, double rdy, const float* rdzu, pywrf_stream_t s) {
+  CHECK(g, 4);
+  AdvIdx x = {g->ids, g->ide, g->jds, g->jde, imin(g->kte, g->kde - 1)};
+  if (g->kts != 0 || x.ktf + 1 > g->nz) return PYWRF_E_RANGE;
+  AdvW a = {w, ru, rv, rom, msftx, fzm, fzp, rdzu, tendency, F(rdx), F(rdy)};
+  int i0 = g->its, i1 = imin(g->ite, g->ide - 1);
+  int j0 = g->jts, j1 = imin(g->jte, g->jde - 1);
+  Box b = mkbox(g->kts + 1, x.ktf + 1, j0, j1, i0, i1);
+  Box by = mkbox(g->kts + 1, x.ktf + 1, imax(g->jts, g->jds + 1), imin(g->jte, g->jde - 2), i0, i1);
+  Box bx = mkbox(g->kts + 1, x.ktf + 1, j0, j1, imax(g->ids + 1, g->its), imin(g->ide - 2, g->ite));
+  PW_LAUNCH_BOX(k_advect_w, b, s, dims_of(g), a, x, by, bx);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ advect_scalar
+struct AdvS {
+  const float *f, *ru, *rv, *rom, *msftx, *fzm, *fzp, *rdzw;
+  float *tend;
+  float rdx, rdy;
+};
+
+__device__ inline float s_yface(const Dims& D, const AdvS& a, const AdvIdx& x, int k, int jf, int i) {
+  auto Q = [&](int o) { return AT(a.f, k, jf + o, i); };
+  float vel = AT(a.rv, k, jf, i);
+  if (jf >= x.jds + 3 && jf < x.jde - 3) return vel * flux5(Q(-3), Q(-2), Q(-1), Q(0), Q(1), Q(2), vel);
+  if (jf == x.jds + 1 || jf == x.jde - 2) return 0.5f * vel * (Q(0) + Q(-1));
+  if (jf == x.jds + 2 || jf == x.jde - 3) return vel * flux3(Q(-2), Q(-1), Q(0), Q(1), vel);
+  return 0.0f;
+}
+
+__device__ inline float s_xface(const Dims& D, const AdvS& a, const AdvIdx& x, int k, int j, int f) {
+  auto Q = [&](int o) { return AT(a.f, k, j, f + o); };
+  float vel = AT(a.ru, k, j, f);
+  if (f >= x.ids + 3 && f < x.ide - 3) return vel * flux5(Q(-3), Q(-2), Q(-1), Q(0), Q(1), Q(2), vel);
+  if (f == x.ids + 1 || f == x.ide - 2) return 0.5f * vel * (Q(0) + Q(-1));
+  if (f == x.ids + 2) return vel * flux3(Q(-2), Q(-1), Q(0), Q(1), vel);
+  if (f == x.ide - 3) return vel * flux3(Q(-2), Q(-1), Q(0), Q(1), AT(a.ru, k, j, f + 1));  // sign from ru[ide-2]
+  return 0.0f;
+}
+
+__device__ inline float s_zface(const Dims& D, const AdvS& a, const AdvIdx& x, int kf, int j, int i) {
+  auto Q = [&](int o) { return AT(a.f, kf + o, j, i); };
+  if (kf < 1 || kf >= x.ktf) return 0.0f;
+  float vel = AT(a.rom, kf, j, i);
+  if (kf >= 3 && kf < x.ktf - 2) return vel * flux5(Q(-3), Q(-2), Q(-1), Q(0), Q(1), Q(2), -vel);
+  if (kf == 1 || kf == x.ktf - 1) return vel * (a.fzm[kf] * Q(0) + a.fzp[kf] * Q(-1));
+  return vel * flux3(Q(-2), Q(-1), Q(0), Q(1), -vel);  // kf == 2 or ktf-2
+}
+
+__global__ void k_advect_scalar(Box b, Dims D, AdvS a, AdvIdx x, Box badv, Box by, Box bx, int zero) {
+  int k, j, i;
+  if (!box_point(b, k, j, i)) return;
+  long q = D.ix(k, j, i);
+  float t = zero ? 0.0f : a.tend[q];
+  if (badv.has(k, j, i)) {
+    float mx = AT2(a.msftx, j, i);
+    if (by.has(k, j, i)) t = t - (mx * a.rdy) * (s_yface(D, a, x, k, j + 1, i) - s_yface(D, a, x, k, j, i));
+    if (bx.has(k, j, i)) t = t - (mx * a.rdx) * (s_xface(D, a, x, k, j, i + 1) - s_xface(D, a, x, k, j, i));
+    t = t - a.rdzw[k] * (s_zface(D, a, x, k + 1, j, i) - s_zface(D, a, x, k, j, i));
+  }
+  a.tend[q] = t;
+}
+
+// zero_box: first zero the tendency on [kts, kte) x [jts, jte) x [its, ite)
+// (rk_scalar_tend, wrf_dynamics.py:3995) and accumulate into it.
+int pw_advect_scalar_box(const pywrf_grid_t* g, const float* field, float* tendency, const float* ru,
+                         const float* rv, const float* rom, const float* msftx, const float* fzm, const float* fzp,
+                         float rdx, float rdy, const float* rdzw, int zero_box, pywrf_stream_t s) {
+  CHECK(g, 4);
+  AdvIdx x = {g->ids, g->ide, g->jds, g->jde, imin(g->kte, g->kde - 1)};
+  if (g->kts != 0 || x.ktf + 1 > g->nz || g->kte > g->nz) return PYWRF_E_RANGE;
+  AdvS a = {field, ru, rv, rom, msftx, fzm, fzp, rdzw, tendency, rdx, rdy};
+  int i0 = g->its, i1 = imin(g->ite, g->ide - 1);
+  int j0 = g->jts, j1 = imin(g->jte, g->jde - 1);
+  Box badv = mkbox(g->kts, x.ktf, j0, j1, i0, i1);
+  Box by = mkbox(g->kts, x.ktf, imax(g->jts, g->jds + 1), imin(g->jte, g->jde - 2), i0, i1);
+  Box bx = mkbox(g->kts, x.ktf, j0, j1, imax(g->ids + 1, g->its), imin(g->ide - 2, g->ite));
+  Box b = zero_box ? mkbox(g->kts, g->kte, g->jts, g->jte, g->its, g->ite) : badv;
+  PW_LAUNCH_BOX(k_advect_scalar, b, s, dims_of(g), a, x, badv, by, bx, zero_box);
+  PW_RET();
+}
+
+extern "C" int pywrf_advect_scalar(const pywrf_grid_t* g, const float* field, float* tendency, const float* ru,
+                                   const float* rv, const float* rom, const float* msftx, const float* fzm,
+                                   const float* fzp, double rdx, double rdy, const float* rdzw, pywrf_stream_t s) {
+  return pw_advect_scalar_box(g, field, tendency, ru, rv, rom, msftx, fzm, fzp, F(rdx), F(rdy), rdzw, 0, s);
+}

@@ -1,0 +1,335 @@
+// Lateral-boundary zone kernels (pywrf/wrf_boundary.py:52-587): relaxation
+// and specified-zone tendencies, specified-zone updates, zero-gradient and
+// flow-dependent edges, mass weighting and the end-of-step specified values.
+//
+// Boundary arrays are contiguous: x-boundaries (kz, ny, 5) indexed [k, j, c],
+// y-boundaries (kz, 5, nx) indexed [k, r, i], c / r counting inward from the
+// domain edge.  Each kernel is one launch over the ring of points it writes;
+// where the reference writes a point in several sequential statements the
+// kernel reproduces the last writer (or repeats the update as often as the
+// statements cover the point).
+#include "common.h"
+#include "consts.h"
+
+#define AT(p, k, j, i) (p)[D.ix((k), (j), (i))]
+#define AT2(p, j, i) (p)[D.ix2((j), (i))]
+
+int pw_check_grid(const pywrf_grid_t* g, int r);
+#define CHECK(g, r) do { int _e = pw_check_grid((g), (r)); if (_e) return _e; } while (0)
+
+struct Bdy {
+  const float *xs, *xe, *ys, *ye, *txs, *txe, *tys, *tye;
+};
+
+__device__ inline float bx(const Dims& D, const float* a, int k, int j, int c) { return a[((long)k * D.ny + j) * 5 + c]; }
+__device__ inline float by(const Dims& D, const float* a, int k, int r, int i) { return a[((long)k * 5 + r) * D.nx + i]; }
+
+// staggering of the relax/spec routines (wrf_boundary.py:_bounds logic)
+static void bounds_of(char v, const pywrf_grid_t* g, int& ibe, int& jbe, int& ktf) {
+  ibe = g->ide - 1;
+  jbe = g->jde - 1;
+  ktf = g->kde - 1;
+  if (v == 'u') ibe = g->ide;
+  if (v == 'v') jbe = g->jde;
+  if (v == 'm' || v == 'h') ktf = g->kte;
+}
+
+// ------------------------------------------------------------------ relax_bdytend (A31, A48)
+struct Relax {
+  const float* field;
+  const float* mut;  // non-null: field is the unweighted scalar, weighted by mut on mbox (relax_bdy_scalar)
+  Box mbox;
+  float* tend;
+  Bdy b;
+  const float *fcx, *gcx;
+  float dtbc;
+  int ibs, ibe, jbs, jbe;
+};
+
+__device__ inline float relax_field(const Dims& D, const Relax& a, int k, int j, int i) {
+  if (!a.mut) return AT(a.field, k, j, i);
+  return a.mbox.has(k, j, i) ? AT(a.field, k, j, i) * AT2(a.mut, j, i) : 0.0f;
+}
+
+__global__ void k_relax(Box all, Dims D, Relax a) {
+  int k, j, i;
+  if (!box_point(all, k, j, i)) return;
+  // which strip (y-start, y-end, x-start, x-end) and b_dist owns this point
+  int side = -1, bd = 0;
+  for (int b = 1; b <= 3 && side < 0; ++b) {
+    if (j == a.jbs + b && i >= a.ibs + b && i < a.ibe - b) { side = 0; bd = b; }
+    else if (j == a.jbe - 1 - b && i >= a.ibs + b && i < a.ibe - b) { side = 1; bd = b; }
+    else if (i == a.ibs + b && j >= a.jbs + b + 1 && j < a.jbe - b - 1) { side = 2; bd = b; }
+    else if (i == a.ibe - 1 - b && j >= a.jbs + b + 1 && j < a.jbe - b - 1) { side = 3; bd = b; }
+  }
+  if (side < 0) return;
+  const float dt = a.dtbc;
+  auto FY = [&](const float* bv, const float* bt, int r, int jj, int ii) {
+    return by(D, bv, k, r, ii) + dt * by(D, bt, k, r, ii) - relax_field(D, a, k, jj, ii);
+  };
+  auto FX = [&](const float* bv, const float* bt, int c, int jj, int ii) {
+    return bx(D, bv, k, jj, c) + dt * bx(D, bt, k, jj, c) - relax_field(D, a, k, jj, ii);
+  };
+  float f0, f1, f2, f3, f4;
+  if (side == 0) {
+    f0 = FY(a.b.ys, a.b.tys, bd, j, i); f1 = FY(a.b.ys, a.b.tys, bd, j, i - 1); f2 = FY(a.b.ys, a.b.tys, bd, j, i + 1);
+    f3 = FY(a.b.ys, a.b.tys, bd - 1, j - 1, i); f4 = FY(a.b.ys, a.b.tys, bd + 1, j + 1, i);
+  } else if (side == 1) {
+    f0 = FY(a.b.ye, a.b.tye, bd, j, i); f1 = FY(a.b.ye, a.b.tye, bd, j, i - 1); f2 = FY(a.b.ye, a.b.tye, bd, j, i + 1);
+    f3 = FY(a.b.ye, a.b.tye, bd - 1, j + 1, i); f4 = FY(a.b.ye, a.b.tye, bd + 1, j - 1, i);
+  } else if (side == 2) {
+    f0 = FX(a.b.xs, a.b.txs, bd, j, i); f1 = FX(a.b.xs, a.b.txs, bd, j - 1, i); f2 = FX(a.b.xs, a.b.txs, bd, j + 1, i);
+    f3 = FX(a.b.xs, a.b.txs, bd - 1, j, i - 1); f4 = FX(a.b.xs, a.b.txs, bd + 1, j, i + 1);
+  } else {
+    f0 = FX(a.b.xe, a.b.txe, bd, j, i); f1 = FX(a.b.xe, a.b.txe, bd, j - 1, i); f2 = FX(a.b.xe, a.b.txe, bd, j + 1, i);
+    f3 = FX(a.b.xe, a.b.txe, bd - 1, j, i + 1); f4 = FX(a.b.xe, a.b.txe, bd + 1, j, i - 1);
+  }
+  long q = D.ix(k, j, i);
+  a.tend[q] = a.tend[q] + a.fcx[bd] * f0 - a.gcx[bd] * (f1 + f2 + f3 + f4 - 4.0f * f0);
+}
+
+static int relax_launch(const pywrf_grid_t* g, Relax& a, char v, pywrf_stream_t s) {
+  int ibe, jbe, ktf;
+  bounds_of(v, g, ibe, jbe, ktf);
+  a.ibs = g->ids; a.jbs = g->jds; a.ibe = ibe; a.jbe = jbe;
+  if (g->ids < 0 || ibe + 2 > g->nx || g->jds < 0 || jbe + 2 > g->ny || ktf > g->nz) return PYWRF_E_RANGE;
+  Box all = mkbox(g->kts, ktf, g->jds, jbe, g->ids, ibe);
+  PW_LAUNCH_BOX(k_relax, all, s, dims_of(g), a);
+  PW_RET();
+}
+
+extern "C" int pywrf_relax_bdytend(const pywrf_grid_t* g, const float* field, float* field_tend, const float* bxs,
+                                   const float* bxe, const float* bys, const float* bye, const float* btxs,
+                                   const float* btxe, const float* btys, const float* btye, char variable_in,
+                                   double dtbc, const float* fcx, const float* gcx, pywrf_stream_t s) {
+  Relax a;
+  a.field = field; a.mut = nullptr; a.mbox = mkbox(0, 0, 0, 0, 0, 0); a.tend = field_tend;
+  a.b = {bxs, bxe, bys, bye, btxs, btxe, btys, btye};
+  a.fcx = fcx; a.gcx = gcx; a.dtbc = F(dtbc);
+  return relax_launch(g, a, variable_in, s);
+}
+
+extern "C" int pywrf_relax_bdy_scalar(const pywrf_grid_t* g, float* scalar_tend, const float* scalar,
+                                      const float* mu, const float* bxs, const float* bxe, const float* bys,
+                                      const float* bye, const float* btxs, const float* btxe, const float* btys,
+                                      const float* btye, double dtbc, const float* fcx, const float* gcx,
+                                      pywrf_stream_t s) {
+  Relax a;
+  a.field = scalar; a.mut = mu; a.tend = scalar_tend;
+  a.mbox = mkbox(g->kts, g->kte - 1, imax(g->jts - 1, g->jds), imin(g->jte + 1, g->jde - 1), imax(g->its - 1, g->ids),
+                 imin(g->ite + 1, g->ide - 1));
+  a.b = {bxs, bxe, bys, bye, btxs, btxe, btys, btye};
+  a.fcx = fcx; a.gcx = gcx; a.dtbc = F(dtbc);
+  return relax_launch(g, a, 'q', s);
+}
+
+// ------------------------------------------------------------------ spec_bdytend (A34, A49)
+__global__ void k_spec_bdytend(Box all, Dims D, float* tend, Bdy b, int ibs, int ibe, int jbs, int jbe) {
+  int k, j, i;
+  if (!box_point(all, k, j, i)) return;
+  float v;
+  bool xr = j >= jbs && j < jbe;
+  if (xr && i == ibe - 1) v = bx(D, b.txe, k, j, 0);
+  else if (xr && i == ibs) v = bx(D, b.txs, k, j, 0);
+  else if (j == jbe - 1) v = by(D, b.tye, k, 0, i);
+  else if (j == jbs) v = by(D, b.tys, k, 0, i);
+  else return;
+  AT(tend, k, j, i) = v;
+}
+
+extern "C" int pywrf_spec_bdytend(const pywrf_grid_t* g, float* field_tend, const float* btxs, const float* btxe,
+                                  const float* btys, const float* btye, char variable_in, pywrf_stream_t s) {
+  int ibe, jbe, ktf;
+  bounds_of(variable_in, g, ibe, jbe, ktf);
+  if (g->ids < 0 || ibe > g->nx || g->jds < 0 || jbe > g->ny || ktf > g->nz) return PYWRF_E_RANGE;
+  Bdy b = {nullptr, nullptr, nullptr, nullptr, btxs, btxe, btys, btye};
+  Box all = mkbox(g->kts, ktf, g->jds, jbe, g->ids, ibe);
+  PW_LAUNCH_BOX(k_spec_bdytend, all, s, dims_of(g), field_tend, b, g->ids, ibe, g->jds, jbe);
+  PW_RET();
+}
+
+extern "C" int pywrf_spec_bdy_scalar(const pywrf_grid_t* g, float* scalar_tend, const float* btxs,
+                                     const float* btxe, const float* btys, const float* btye, pywrf_stream_t s) {
+  return pywrf_spec_bdytend(g, scalar_tend, btxs, btxe, btys, btye, 'q', s);
+}
+
+// ------------------------------------------------------------------ spec_bdyupdate (A39)
+__global__ void k_spec_bdyupdate(Box all, Dims D, float* f, const float* t, float dt, int jts, int jtf, int its,
+                                 int itf, int jbs, int jbe) {
+  int k, j, i;
+  if (!box_point(all, k, j, i)) return;
+  bool ic = i >= its && i < itf, jc = j >= jbs + 1 && j < jbe - 1;
+  int n = (j == jts && ic) + (j == jtf - 1 && ic) + (i == its && jc) + (i == itf - 1 && jc);
+  if (!n) return;
+  long q = D.ix(k, j, i);
+  float v = f[q], tv = t[q];
+  for (int m = 0; m < n; ++m) v = v + dt * tv;
+  f[q] = v;
+}
+
+extern "C" int pywrf_spec_bdyupdate(const pywrf_grid_t* g, float* field, const float* field_tend, double dt,
+                                    char variable, pywrf_stream_t s) {
+  char v = variable;
+  if (v == 'U') v = 'u'; else if (v == 'V') v = 'v'; else if (v == 'M') v = 'm'; else if (v == 'H') v = 'h';
+  int itf = imin(g->ite, g->ide - 1), jtf = imin(g->jte, g->jde - 1), ktf = g->kde - 1, jbe = g->jde - 1;
+  if (v == 'u') itf = imin(g->ite, g->ide);
+  if (v == 'v') { jbe = g->jde; jtf = imin(g->jte, g->jde); }
+  if (v == 'm' || v == 'h') ktf = g->kte;
+  int nz = g->nz;
+  if (g->kds == 0 && g->kde == 0) { ktf = 1; nz = 1; }  // 2-D field (the reference's unsqueeze path)
+  if (itf > g->nx || jbe > g->ny || jtf > g->ny || ktf > nz || g->its < 0 || g->jts < 0) return PYWRF_E_RANGE;
+  pywrf_grid_t g2 = *g;
+  g2.nz = nz;
+  Box all = mkbox(g->kts, ktf, imin(g->jts, g->jds), imax(jtf, jbe), g->its, itf);
+  PW_LAUNCH_BOX(k_spec_bdyupdate, all, s, dims_of(&g2), field, field_tend, F(dt), g->jts, jtf, g->its, itf, g->jds,
+                jbe);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ spec_bdyupdate_ph (A43)
+__global__ void k_spec_bdyupdate_ph(Box all, Dims D, const float* ph_save, float* f, const float* ft,
+                                    const float* mu_tend, const float* muts, float dt, int jts, int ibs, int ibe,
+                                    int jbs, int jbe) {
+  int k, j, i;
+  if (!box_point(all, k, j, i)) return;
+  bool ir = i >= ibs && i < ibe, jc = j >= jbs + 1 && j < jbe - 1;
+  int n = (j == jts && ir) + (j == jbe - 1 && ir) + (i == ibs && jc) + (i == ibe - 1 && jc);
+  if (!n) return;
+  long q = D.ix(k, j, i);
+  float mt = AT2(muts, j, i);
+  float mo = mt - dt * AT2(mu_tend, j, i);
+  float v = f[q];
+  for (int m = 0; m < n; ++m) v = v * mo / mt + dt * ft[q] / mt + ph_save[q] * (mo / mt - 1.0f);
+  f[q] = v;
+}
+
+extern "C" int pywrf_spec_bdyupdate_ph(const pywrf_grid_t* g, const float* ph_save, float* field,
+                                       const float* field_tend, const float* mu_tend, const float* muts, double dt,
+                                       char variable_in, pywrf_stream_t s) {
+  char v = variable_in;
+  if (v == 'U') v = 'u'; else if (v == 'V') v = 'v'; else if (v == 'M') v = 'm'; else if (v == 'H') v = 'h';
+  int ibe, jbe, ktf;
+  bounds_of(v, g, ibe, jbe, ktf);
+  if (ibe > g->nx || jbe > g->ny || ktf > g->nz || g->ids < 0 || g->jds < 0) return PYWRF_E_RANGE;
+  Box all = mkbox(g->kts, ktf, imin(g->jts, g->jds), imax(jbe, g->jts + 1), g->ids, ibe);
+  PW_LAUNCH_BOX(k_spec_bdyupdate_ph, all, s, dims_of(g), ph_save, field, field_tend, mu_tend, muts, F(dt), g->jts,
+                g->ids, ibe, g->jds, jbe);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ zero_grad_bdy (A44)
+__global__ void k_zero_grad(Box all, Dims D, float* f, int ibs, int ibe, int jbs, int jbe) {
+  int k, j, i;
+  if (!box_point(all, k, j, i)) return;
+  bool ring = (j == jbs || j == jbe - 1) || (i == ibs || i == ibe - 1);
+  if (!ring) return;
+  int sj = j < jbs + 1 ? jbs + 1 : (j > jbe - 2 ? jbe - 2 : j);
+  int si = i < ibs + 1 ? ibs + 1 : (i > ibe - 2 ? ibe - 2 : i);
+  AT(f, k, j, i) = AT(f, k, sj, si);
+}
+
+extern "C" int pywrf_zero_grad_bdy(const pywrf_grid_t* g, float* field, char variable_in, pywrf_stream_t s) {
+  char v = variable_in;
+  if (v == 'U') v = 'u'; else if (v == 'V') v = 'v';
+  int ibe = g->ide - 1, jbe = g->jde - 1, ktf = g->kde - 1;
+  if (v == 'u') ibe = g->ide;
+  if (v == 'v') jbe = g->jde;
+  if (v == 'w') ktf = g->kde;
+  if (ibe > g->nx || jbe > g->ny || ktf > g->nz || g->ids < 0 || g->jds < 0) return PYWRF_E_RANGE;
+  Box all = mkbox(g->kts, ktf, g->jds, jbe, g->ids, ibe);
+  PW_LAUNCH_BOX(k_zero_grad, all, s, dims_of(g), field, g->ids, ibe, g->jds, jbe);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ mass_weight (A32)
+__global__ void k_mass_weight(Box b, Dims D, const float* field, const float* mut, float* rfield) {
+  int k, j, i;
+  if (!box_point(b, k, j, i)) return;
+  long q = D.ix(k, j, i);
+  rfield[q] = field[q] * AT2(mut, j, i);
+}
+
+extern "C" int pywrf_mass_weight(const pywrf_grid_t* g, const float* field, const float* mut, float* rfield,
+                                 pywrf_stream_t s) {
+  Box b = mkbox(g->kts, g->kte, g->jts, g->jte, g->its, g->ite);
+  if (b.k0 < 0 || b.k1 > g->nz || b.j0 < 0 || b.j1 > g->ny || b.i0 < 0 || b.i1 > g->nx) return PYWRF_E_RANGE;
+  PW_LAUNCH_BOX(k_mass_weight, b, s, dims_of(g), field, mut, rfield);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ flow_dep_bdy (A51)
+__global__ void k_flow_dep(Box all, Dims D, float* f, const float* u, const float* v, int jts, int jtf, int its,
+                           int itf) {
+  int k, j, i;
+  if (!box_point(all, k, j, i)) return;
+  bool top = j == jts, bot = j == jtf - 1, lft = i == its, rgt = i == itf - 1;
+  if (!(top || bot || lft || rgt)) return;
+  float r;
+  if ((top || bot) && (lft || rgt)) {
+    r = AT(f, k, top ? jts + 1 : jtf - 2, lft ? its + 1 : itf - 2) + 0.0f;
+  } else if (top) {
+    r = AT(v, k, jts, i) < 0.0f ? AT(f, k, jts + 1, i) : 0.0f;
+  } else if (bot) {
+    r = AT(v, k, jtf, i) > 0.0f ? AT(f, k, jtf - 2, i) : 0.0f;
+  } else if (lft) {
+    r = AT(u, k, j, its) < 0.0f ? AT(f, k, j, its + 1) : 0.0f;
+  } else {
+    r = AT(u, k, j, itf) > 0.0f ? AT(f, k, j, itf - 2) : 0.0f;
+  }
+  AT(f, k, j, i) = r;
+}
+
+extern "C" int pywrf_flow_dep_bdy(const pywrf_grid_t* g, float* field, const float* u, const float* v,
+                                  pywrf_stream_t s) {
+  CHECK(g, 1);
+  int itf = imin(g->ite, g->ide - 1), jtf = imin(g->jte, g->jde - 1), ktf = g->kde - 1;
+  if (ktf > g->nz) return PYWRF_E_RANGE;
+  Box all = mkbox(g->kts, ktf, g->jts, jtf, g->its, itf);
+  PW_LAUNCH_BOX(k_flow_dep, all, s, dims_of(g), field, u, v, g->jts, jtf, g->its, itf);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ spec_bdy_final (B1)
+__global__ void k_spec_final(Box all, Dims D, float* f, const float* mu, const float* msf, Bdy b, float dtbc,
+                             int jts, int jtf, int its, int itf, int ibe, int jbe) {
+  int k, j, i;
+  if (!box_point(all, k, j, i)) return;
+  bool xr = j >= jts && j < jbe, ir = i >= its && i < ibe;
+  float bv, bt;
+  if (xr && i == itf - 1) { bv = bx(D, b.xe, k, j, 0); bt = bx(D, b.txe, k, j, 0); }
+  else if (xr && i == its) { bv = bx(D, b.xs, k, j, 0); bt = bx(D, b.txs, k, j, 0); }
+  else if (ir && j == jtf - 1) { bv = by(D, b.ye, k, 0, i); bt = by(D, b.tye, k, 0, i); }
+  else if (ir && j == jts) { bv = by(D, b.ys, k, 0, i); bt = by(D, b.tys, k, 0, i); }
+  else return;
+  float sm = msf ? AT2(msf, j, i) : 1.0f;
+  float su = mu ? AT2(mu, j, i) : 1.0f;
+  AT(f, k, j, i) = sm * (bv + dtbc * bt) / su;
+}
+
+extern "C" int pywrf_spec_bdy_final(const pywrf_grid_t* g, float* field, const float* mu, const float* msf,
+                                    const float* bxs, const float* bxe, const float* bys, const float* bye,
+                                    const float* btxs, const float* btxe, const float* btys, const float* btye,
+                                    char variable_in, double dtbc, pywrf_stream_t s) {
+  char v = variable_in;
+  switch (v) {
+    case 'U': v = 'u'; break;
+    case 'V': v = 'v'; break;
+    case 'W': v = 'w'; break;
+    case 'M': v = 'm'; break;
+    case 'T': v = 't'; break;
+    case 'H': v = 'h'; break;
+    default: break;
+  }
+  int ibe = g->ide - 1, jbe = g->jde - 1;
+  int itf = imin(g->ite, g->ide - 1), jtf = imin(g->jte, g->jde - 1), ktf = g->kde - 1;
+  if (v == 'u') { ibe = g->ide; itf = imin(g->ite, g->ide); }
+  if (v == 'v') { jbe = g->jde; jtf = imin(g->jte, g->jde); }
+  if (v == 'm' || v == 'h' || v == 'w') ktf = g->kde;
+  if (ibe > g->nx || jbe > g->ny || jtf > g->ny || ktf > g->nz || g->its < 0 || g->jts < 0) return PYWRF_E_RANGE;
+  const float* xmsf = (v == 'u' || v == 'v' || v == 'w') ? msf : nullptr;
+  const float* xmu = (v != 'm') ? mu : nullptr;
+  Bdy b = {bxs, bxe, bys, bye, btxs, btxe, btys, btye};
+  Box all = mkbox(g->kts, ktf, g->jts, imax(jtf, jbe), g->its, imax(ibe, itf));
+  PW_LAUNCH_BOX(k_spec_final, all, s, dims_of(g), field, xmu, xmsf, b, F(dtbc), g->jts, jtf, g->its, itf, ibe, jbe);
+  PW_RET();
+}

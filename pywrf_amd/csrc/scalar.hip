@@ -1,0 +1,221 @@
+// Scalar transport and update, the moist equation of state, PBL decoupling and
+// the kinematic surface w (pywrf/wrf_dynamics.py:3982-4324).
+#include "common.h"
+#include "consts.h"
+
+#define AT(p, k, j, i) (p)[D.ix((k), (j), (i))]
+#define AT2(p, j, i) (p)[D.ix2((j), (i))]
+
+int pw_check_grid(const pywrf_grid_t* g, int r);
+#define CHECK(g, r) do { int _e = pw_check_grid((g), (r)); if (_e) return _e; } while (0)
+
+int pw_advect_scalar_box(const pywrf_grid_t* g, const float* field, float* tendency, const float* ru,
+                         const float* rv, const float* rom, const float* msftx, const float* fzm, const float* fzp,
+                         float rdx, float rdy, const float* rdzw, int zero_box, pywrf_stream_t s);
+int pw_horizontal_diffusion(const pywrf_grid_t* g, char name, const float* field, float* tendency,
+                            const float* base, const float* mut, const float* msfux, const float* msfuy,
+                            const float* msfvx, const float* msfvx_inv, const float* msfvy, const float* msftx,
+                            const float* msfty, const float* xkmhd, float rdx, float rdy, pywrf_stream_t s);
+extern "C" int pywrf_vertical_diffusion(const pywrf_grid_t* g, char name, const float* field, float* tendency,
+                                        const float* alt, const float* mut, const float* rdn, const float* rdnw,
+                                        double kvdif, pywrf_stream_t s);
+extern "C" int pywrf_vertical_diffusion_mp(const pywrf_grid_t* g, const float* field, float* tendency,
+                                           const float* base, const float* alt, const float* mut, const float* rdn,
+                                           const float* rdnw, double kvdif, pywrf_stream_t s);
+
+// Fill kernel used instead of hipMemsetAsync so that the calls stay safe under
+// stream capture.
+__global__ void k_fill(float* a, long n, float v) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; t < n; t += (long)gridDim.x * blockDim.x) a[t] = v;
+}
+
+int pw_fill(float* a, long n, float v, pywrf_stream_t s) {
+  if (n <= 0) return 0;
+  long blocks = (n + PW_BLOCK - 1) / PW_BLOCK;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(PW_BLOCK), 0, (hipStream_t)s, a, n, v);
+  return (int)hipGetLastError();
+}
+
+// ------------------------------------------------------------------ rk_scalar_tend (A47)
+extern "C" int pywrf_rk_scalar_tend(const pywrf_grid_t* g, int scs, int sce, int rk_step, const float* ru,
+                                    const float* rv, const float* ww, const float* mut, const float* alt,
+                                    const float* scalar, float* scalar_tends, float* advect_tend,
+                                    float* h_tendency, float* z_tendency, const float* base, int moist_step,
+                                    const float* fnm, const float* fnp, const float* msfux, const float* msfuy,
+                                    const float* msfvx, const float* msfvx_inv, const float* msfvy,
+                                    const float* msftx, const float* msfty, double rdx, double rdy,
+                                    const float* rdn, const float* rdnw, double kvdif, const float* xkmhd,
+                                    pywrf_stream_t s) {
+  CHECK(g, 4);
+  long n3 = (long)g->nz * g->ny * g->nx;
+  double kvdq = kvdif / pw::prandtl;
+  int e;
+  // h_tendency and z_tendency are zeroed once per species by the reference; once is the same
+  if ((e = pw_fill(h_tendency, n3, 0.0f, s))) return e;
+  if ((e = pw_fill(z_tendency, n3, 0.0f, s))) return e;
+  for (int im = scs; im <= sce; ++im) {
+    const float* f = scalar + im * n3;
+    if ((e = pw_advect_scalar_box(g, f, advect_tend + im * n3, ru, rv, ww, msftx, fnm, fnp, F(rdx), F(rdy), rdnw, 1,
+                                  s)))
+      return e;
+    if (rk_step == 1) {
+      float* st = scalar_tends + im * n3;
+      if ((e = pw_horizontal_diffusion(g, 'm', f, st, nullptr, mut, msfux, msfuy, msfvx, msfvx_inv, msfvy, msftx,
+                                       msfty, xkmhd, F(rdx), F(rdy), s)))
+        return e;
+      if (moist_step && im == pw::P_QV) e = pywrf_vertical_diffusion_mp(g, f, st, base, alt, mut, rdn, rdnw, kvdq, s);
+      else e = pywrf_vertical_diffusion(g, 'm', f, st, alt, mut, rdn, rdnw, kvdq, s);
+      if (e) return e;
+    }
+  }
+  return 0;
+}
+
+// ------------------------------------------------------------------ rk_update_scalar (A50)
+struct UpdS {
+  float *s1, *s2;
+  const float *sc_tend, *adv, *msfty, *mu_old, *mu_new, *mub;
+  float dt;
+  int rk1;
+  Box bs, bi, bf;
+  int jf0, jf1, if0, if1;  // rows / columns where muold and munew are set
+};
+
+__global__ void k_rk_update_scalar(Box all, Dims D, UpdS a) {
+  int k, j, i;
+  if (!box_point(all, k, j, i)) return;
+  long q = D.ix(k, j, i), q2 = D.ix2(j, i);
+  float s2 = a.s2[q];
+  float s1 = a.s1[q];
+  if (a.rk1) {
+    s1 = s2 + 0.0f;
+    a.s1[q] = s1;
+  }
+  if (!a.bf.has(k, j, i)) return;
+  float t = 0.0f;
+  if (a.bi.has(k, j, i)) t = a.adv[q] * a.msfty[q2];
+  if (a.bs.has(k, j, i)) t = t + a.sc_tend[q];
+  bool hm = j >= a.jf0 && j < a.jf1 && i >= a.if0 && i < a.if1;
+  float muold = hm ? a.mu_old[q2] + a.mub[q2] : 0.0f;
+  float munew = hm ? a.mu_new[q2] + a.mub[q2] : 0.0f;
+  a.s2[q] = (muold * s1 + a.dt * t) / munew;
+}
+
+extern "C" int pywrf_rk_update_scalar(const pywrf_grid_t* g, int scs, int sce, float* scalar_1, float* scalar_2,
+                                      const float* sc_tend, const float* advect_tend, const float* msfty,
+                                      const float* mu_old, const float* mu_new, const float* mu_base, int rk_step,
+                                      double dt, int spec_zone, pywrf_stream_t s) {
+  CHECK(g, 0);
+  if (g->kte > g->nz || g->kde > g->nz) return PYWRF_E_RANGE;
+  long n3 = (long)g->nz * g->ny * g->nx;
+  UpdS a;
+  a.mu_old = mu_old; a.mu_new = mu_new; a.mub = mu_base; a.msfty = msfty; a.dt = F(dt); a.rk1 = rk_step == 1;
+  a.bs = mkbox(g->kts, g->kte - 1, g->jts, imin(g->jte, g->jde - 1), g->its, imin(g->ite, g->ide - 1));
+  a.bi = mkbox(g->kts, imin(g->kte, g->kde - 1), imax(g->jts, g->jds + spec_zone), imin(g->jte, g->jde - spec_zone - 1),
+               imax(g->its, g->ids + spec_zone), imin(g->ite, g->ide - spec_zone - 1));
+  a.bf = mkbox(g->kts, g->kde - 1, g->jts, g->jde - 1, g->its, g->ide - 1);
+  a.jf0 = g->jts; a.jf1 = g->jde - 1; a.if0 = g->its; a.if1 = g->ide - 1;
+  Box all = mkbox(0, g->nz, 0, g->ny, 0, g->nx);
+  for (int im = scs; im <= sce; ++im) {
+    a.s1 = scalar_1 + im * n3;
+    a.s2 = scalar_2 + im * n3;
+    a.sc_tend = sc_tend + im * n3;
+    a.adv = advect_tend + im * n3;
+    Box b = a.rk1 ? all : a.bf;
+    PW_LAUNCH_BOX(k_rk_update_scalar, b, s, dims_of(g), a);
+  }
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ calc_p_rho_phi (A52)
+__global__ void k_calc_p_rho_phi(Box b, Dims D, const float* qv, float* al, const float* alb, const float* mu,
+                                 const float* muts, const float* ph, float* p, const float* pb, const float* t,
+                                 float p0, float t0, const float* rdnw) {
+  int k, j, i;
+  if (!box_point(b, k, j, i)) return;
+  long q = D.ix(k, j, i);
+  float a = -1.0f / AT2(muts, j, i) * (alb[q] * AT2(mu, j, i) + rdnw[k] * (AT(ph, k + 1, j, i) - ph[q]));
+  al[q] = a;
+  float qvf = 1.0f + F(pw::rvovrd) * qv[q];
+  float temp = (F(pw::r_d) * (t0 + t[q]) * qvf) / (p0 * (a + alb[q]));
+  float pp = powf(temp, F(pw::cpovcv));
+  p[q] = pp * p0 - pb[q];
+}
+
+extern "C" int pywrf_calc_p_rho_phi(const pywrf_grid_t* g, const float* moist, float* al, const float* alb,
+                                    const float* mu, const float* muts, const float* ph, float* p, const float* pb,
+                                    const float* t, double p0, double t0, const float* rdnw, pywrf_stream_t s) {
+  CHECK(g, 0);
+  Box b = mkbox(g->kts, imin(g->kte, g->kde - 1), g->jts, imin(g->jte, g->jde - 1), g->its,
+                imin(g->ite, g->ide - 1));
+  if (b.k1 + 1 > g->nz) return PYWRF_E_RANGE;
+  long n3 = (long)g->nz * g->ny * g->nx;
+  PW_LAUNCH_BOX(k_calc_p_rho_phi, b, s, dims_of(g), moist + pw::P_QV * n3, al, alb, mu, muts, ph, p, pb, t, F(p0),
+                F(t0), rdnw);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ phy_prep_part2 (A53)
+__global__ void k_div_by_mut(Box b, Dims D, const float* mut, float* a0, float* a1, float* a2, float* a3,
+                             float* a4, float* a5) {
+  int k, j, i;
+  if (!box_point(b, k, j, i)) return;
+  long q = D.ix(k, j, i);
+  float m = AT2(mut, j, i);
+  a0[q] = a0[q] / m;
+  a1[q] = a1[q] / m;
+  a2[q] = a2[q] / m;
+  a3[q] = a3[q] / m;
+  a4[q] = a4[q] / m;
+  a5[q] = a5[q] / m;
+}
+
+extern "C" int pywrf_phy_prep_part2(const pywrf_grid_t* g, const float* mut, float* rthblten, float* rublten,
+                                    float* rvblten, float* rqvblten, float* rqcblten, float* rqiblten,
+                                    pywrf_stream_t s) {
+  CHECK(g, 0);
+  Box b = mkbox(g->kts, imin(g->kte, g->kde - 1), g->jts, imin(g->jte, g->jde - 1), g->its,
+                imin(g->ite, g->ide - 1));
+  PW_LAUNCH_BOX(k_div_by_mut, b, s, dims_of(g), mut, rublten, rvblten, rthblten, rqvblten, rqcblten, rqiblten);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ set_w_surface (B2)
+// One-sided terrain-slope stencils on the outer rows / columns of [jts, jte-1)
+// x [its, ite-1); at the (jts, ite-2) corner the y slope is the reference's
+// ht[jts+1] - ht[jts+1], and at the (jte-2, ite-2) corner the x term samples
+// msftx at (jts, its).
+__global__ void k_set_w_surface(Box b, Dims D, float* w, const float* ht, const float* u, const float* v,
+                                const float* cf, float rdx, float rdy, const float* msftx, const float* msfty,
+                                int jts, int jte, int its, int ite) {
+  int j, i;
+  if (!box_column(b, j, i)) return;
+  float cf1 = cf[0], cf2 = cf[1], cf3 = cf[2];
+  auto cv = [&](int jj) { return cf1 * AT(v, 0, jj, i) + cf2 * AT(v, 1, jj, i) + cf3 * AT(v, 2, jj, i); };
+  auto cu = [&](int ii) { return cf1 * AT(u, 0, j, ii) + cf2 * AT(u, 1, j, ii) + cf3 * AT(u, 2, j, ii); };
+  auto H = [&](int jj, int ii) { return AT2(ht, jj, ii); };
+  bool jlo = (j == jts), jhi = (j == jte - 2), ilo = (i == its), ihi = (i == ite - 2);
+  float ty, tx;
+  if (jlo && ihi) ty = (H(j + 1, i) - H(j + 1, i)) * cv(j + 1);
+  else if (jlo) ty = (H(j + 1, i) - H(j, i)) * cv(j + 1);
+  else if (jhi) ty = (H(j, i) - H(j - 1, i)) * cv(j);
+  else ty = (H(j + 1, i) - H(j, i)) * cv(j + 1) + (H(j, i) - H(j - 1, i)) * cv(j);
+  if (ilo) tx = (H(j, i + 1) - H(j, i)) * cu(i + 1);
+  else if (ihi) tx = (H(j, i) - H(j, i - 1)) * cu(i);
+  else tx = (H(j, i + 1) - H(j, i)) * cu(i + 1) + (H(j, i) - H(j, i - 1)) * cu(i);
+  float msx = (jhi && ihi) ? AT2(msftx, jts, its) : AT2(msftx, j, i);
+  AT(w, 0, j, i) = AT2(msfty, j, i) * 0.5f * rdy * ty + msx * 0.5f * rdx * tx;
+}
+
+extern "C" int pywrf_set_w_surface(const pywrf_grid_t* g, float* w, const float* ht, const float* u, const float* v,
+                                   const float* cf, double rdx, double rdy, const float* msftx, const float* msfty,
+                                   pywrf_stream_t s) {
+  CHECK(g, 1);
+  if (g->nz < 3) return PYWRF_E_RANGE;
+  Box b = mkbox(0, 1, g->jts, g->jte - 1, g->its, g->ite - 1);
+  PW_LAUNCH_COLS(k_set_w_surface, b, s, dims_of(g), w, ht, u, v, cf, F(rdx), F(rdy), msftx, msfty, g->jts, g->jte,
+                 g->its, g->ite);
+  PW_RET();
+}

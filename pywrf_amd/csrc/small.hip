@@ -1,0 +1,680 @@
+// Acoustic small-step machinery (pywrf/wrf_dynamics.py:3152-3930):
+// small_step_prep, calc_p_rho, calc_coef_w, advance_uv, advance_mu_t,
+// advance_w, sumflux, calc_mu_uv_1, small_step_finish.
+//
+// Where the reference clones an argument and updates a box of the clone, the
+// kernel writes the whole output array (box points computed, the rest copied)
+// in the same pass.  Column recurrences (eta velocity, Thomas solve) run one
+// thread per (j, i) column with k marching; the output buffer doubles as the
+// column scratch, so no full-grid temporaries are allocated.
+#include "common.h"
+#include "consts.h"
+
+#define AT(p, k, j, i) (p)[D.ix((k), (j), (i))]
+#define AT2(p, j, i) (p)[D.ix2((j), (i))]
+
+int pw_check_grid(const pywrf_grid_t* g, int r);
+#define CHECK(g, r) do { int _e = pw_check_grid((g), (r)); if (_e) return _e; } while (0)
+
+static Box whole(const pywrf_grid_t* g) { return mkbox(0, g->nz, 0, g->ny, 0, g->nx); }
+static Box plane(const pywrf_grid_t* g) { return mkbox(0, 1, 0, g->ny, 0, g->nx); }
+
+// ------------------------------------------------------------------ small_step_prep (A35)
+struct SsPrep2 {
+  const float *mub, *muu, *muv;
+  float *mu_1, *mu_2, *muus, *muvs, *muts, *mudf, *mu_save;
+  int rk1, j0, j1, i0, i1, i1u, j1v;
+};
+
+__global__ void k_ssprep_2d(Box b, Dims D, SsPrep2 a) {
+  int j, i;
+  if (!box_column(b, j, i)) return;
+  bool h = j >= a.j0 && j < a.j1 && i >= a.i0 && i < a.i1;
+  bool hu = j >= a.j0 && j < a.j1 && i >= a.i0 && i < a.i1u;
+  long q = D.ix2(j, i);
+  if (a.rk1) {
+    bool hv = j >= a.j0 && j < a.j1v && i >= a.i0 && i < a.i1;
+    float m2 = a.mu_2[q];
+    if (h) {
+      a.mu_1[q] = m2 + 0.0f;
+      a.mudf[q] = 0.0f;
+      a.muts[q] = a.mub[q] + m2;
+      a.mu_save[q] = m2 + 0.0f;
+      a.mu_2[q] = 0.0f;
+    }
+    if (hu) a.muus[q] = a.muu[q] + 0.0f;
+    if (hv) a.muvs[q] = a.muv[q] + 0.0f;
+  } else {
+    bool hv = j >= a.j0 && j < a.j1v && i >= a.i0 && i < a.i1u;  // i-range i1u as in the reference
+    if (h) a.muts[q] = a.mub[q] + a.mu_1[q];
+    if (hu) a.muus[q] = 0.5f * (a.mub[q] + a.mu_1[q] + AT2(a.mub, j, i - 1) + AT2(a.mu_1, j, i - 1));
+    if (hv) a.muvs[q] = 0.5f * (a.mub[q] + a.mu_1[q] + AT2(a.mub, j - 1, i) + AT2(a.mu_1, j - 1, i));
+    if (h) {
+      float m2 = a.mu_2[q];
+      a.mu_save[q] = m2 + 0.0f;
+      a.mu_2[q] = a.mu_1[q] - m2;
+    }
+  }
+}
+
+struct SsPrep3 {
+  float *u_1, *v_1, *w_1, *t_1, *ph_1;
+  const float *u_2, *v_2, *w_2, *t_2, *ph_2;
+  float *u_save, *v_save, *w_save, *t_save, *ph_save, *ww_save, *c2a;
+  const float *ww, *pb, *p, *alt, *muu, *muus, *muv, *muvs, *mut, *muts, *msfuy, *msfvx_inv, *msfty;
+  float *u_out, *v_out, *w_out, *t_out, *ph_out;
+  int rk1, k1, kde, j0, j1, i0, i1, i1u, j1v;
+};
+
+__global__ void k_ssprep_3d(Box all, Dims D, SsPrep3 a) {
+  int k, j, i;
+  if (!box_point(all, k, j, i)) return;
+  long q = D.ix(k, j, i);
+  long q2 = D.ix2(j, i);
+  bool hcol = j >= a.j0 && j < a.j1 && i >= a.i0 && i < a.i1;
+  bool bm = k < a.k1 && hcol;
+  bool bu = k < a.k1 && j >= a.j0 && j < a.j1 && i >= a.i0 && i < a.i1u;
+  bool bv = k < a.k1 && j >= a.j0 && j < a.j1v && i >= a.i0 && i < a.i1;
+  bool bw = k < a.kde && hcol;
+  // u
+  float u2 = a.u_2[q], uo = u2;
+  if (bu) {
+    float u1 = a.u_1[q];
+    if (a.rk1) a.u_1[q] = u1 = u2 + 0.0f;
+    a.u_save[q] = u2 + 0.0f;
+    uo = (a.muus[q2] * u1 - a.muu[q2] * u2) / a.msfuy[q2];
+  }
+  a.u_out[q] = uo;
+  float v2 = a.v_2[q], vo = v2;
+  if (bv) {
+    float v1 = a.v_1[q];
+    if (a.rk1) a.v_1[q] = v1 = v2 + 0.0f;
+    a.v_save[q] = v2 + 0.0f;
+    vo = (a.muvs[q2] * v1 - a.muv[q2] * v2) * a.msfvx_inv[q2];
+  }
+  a.v_out[q] = vo;
+  float t2 = a.t_2[q], to = t2;
+  if (bm) {
+    float t1 = a.t_1[q];
+    if (a.rk1) a.t_1[q] = t1 = t2 + 0.0f;
+    a.c2a[q] = F(pw::cpovcv) * (a.pb[q] + a.p[q]) / a.alt[q];
+    a.t_save[q] = t2 + 0.0f;
+    to = a.muts[q2] * t1 - a.mut[q2] * t2;
+  }
+  a.t_out[q] = to;
+  float w2 = a.w_2[q], wo = w2, p2 = a.ph_2[q], po = p2;
+  if (bw) {
+    float w1 = a.w_1[q], p1 = a.ph_1[q];
+    if (a.rk1) {
+      a.w_1[q] = w1 = w2 + 0.0f;
+      a.ph_1[q] = p1 = p2 + 0.0f;
+    }
+    a.w_save[q] = w2 + 0.0f;
+    wo = (a.muts[q2] * w1 - a.mut[q2] * w2) / a.msfty[q2];
+    a.ph_save[q] = p2 + 0.0f;
+    po = p1 - p2;
+    a.ww_save[q] = a.ww[q];
+  } else if (hcol && (k == 0 || k == a.kde - 1)) {
+    a.ww_save[q] = 0.0f;
+  }
+  a.w_out[q] = wo;
+  a.ph_out[q] = po;
+}
+
+extern "C" int pywrf_small_step_prep(const pywrf_grid_t* g, float* u_1, const float* u_2, float* v_1,
+                                     const float* v_2, float* w_1, const float* w_2, float* t_1, const float* t_2,
+                                     float* ph_1, const float* ph_2, const float* mub, float* mu_1, float* mu_2,
+                                     const float* muu, float* muus, const float* muv, float* muvs, const float* mut,
+                                     float* muts, float* mudf, float* u_save, float* v_save, float* w_save,
+                                     float* t_save, float* ph_save, float* mu_save, const float* ww,
+                                     float* ww_save, float* c2a, const float* pb, const float* p, const float* alt,
+                                     const float* msfuy, const float* msfvx_inv, const float* msfty, int rk_step,
+                                     float* u_out, float* v_out, float* w_out, float* t_out, float* ph_out,
+                                     pywrf_stream_t s) {
+  CHECK(g, 1);
+  if (g->kde > g->nz || g->kts != 0) return PYWRF_E_RANGE;
+  int j0 = g->jts, j1 = imin(g->jte, g->jde - 1), i0 = g->its, i1 = imin(g->ite, g->ide - 1);
+  int k1 = imin(g->kte, g->kde - 1), i1u = g->ite, j1v = g->jte;
+  SsPrep2 a2 = {mub, muu, muv, mu_1, mu_2, muus, muvs, muts, mudf, mu_save, rk_step == 1, j0, j1, i0, i1, i1u, j1v};
+  Dims D = dims_of(g);
+  Box b2 = mkbox(0, 1, j0, imax(j1, j1v), i0, imax(i1, i1u));
+  PW_LAUNCH_COLS(k_ssprep_2d, b2, s, D, a2);
+  SsPrep3 a3 = {u_1, v_1, w_1, t_1, ph_1, u_2, v_2, w_2, t_2, ph_2, u_save, v_save, w_save, t_save, ph_save,
+                ww_save, c2a, ww, pb, p, alt, muu, muus, muv, muvs, mut, muts, msfuy, msfvx_inv, msfty,
+                u_out, v_out, w_out, t_out, ph_out, rk_step == 1, k1, g->kde, j0, j1, i0, i1, i1u, j1v};
+  PW_LAUNCH_BOX(k_ssprep_3d, whole(g), s, D, a3);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ calc_p_rho (A36)
+struct PRho {
+  const float *al_in, *p_in, *ph, *alt, *t_2, *t_1, *c2a, *mu, *mut, *rdnw;
+  float *al, *p, *pm1;
+  float t0, smdiv;
+  int step;
+};
+
+__global__ void k_calc_p_rho(Box all, Dims D, Box b, PRho a) {
+  int k, j, i;
+  if (!box_point(all, k, j, i)) return;
+  long q = D.ix(k, j, i);
+  float al = a.al_in[q], p = a.p_in[q];
+  if (b.has(k, j, i)) {
+    float mut = AT2(a.mut, j, i), mu = AT2(a.mu, j, i);
+    al = -1.0f / mut * (a.alt[q] * mu + a.rdnw[k] * (AT(a.ph, k + 1, j, i) - a.ph[q]));
+    float t1 = a.t_1[q];
+    p = a.c2a[q] * (a.alt[q] * (a.t_2[q] - mu * t1) / (mut * (a.t0 + t1)) - al);
+    if (a.step != 0) {
+      float ptmp = p + 0.0f;
+      p = p + a.smdiv * (p - a.pm1[q]);
+      a.pm1[q] = ptmp;
+    }
+  }
+  a.al[q] = al;
+  a.p[q] = p;
+  if (a.step == 0) a.pm1[q] = p + 0.0f;
+}
+
+extern "C" int pywrf_calc_p_rho(const pywrf_grid_t* g, const float* al_in, const float* p_in, const float* ph,
+                                const float* alt, const float* t_2, const float* t_1, const float* c2a, float* pm1,
+                                const float* mu, const float* mut, double t0, const float* rdnw, double smdiv,
+                                int step, float* al_out, float* p_out, pywrf_stream_t s) {
+  CHECK(g, 0);
+  Box b = mkbox(g->kts, imin(g->kte, g->kde - 1), g->jts, imin(g->jte, g->jde - 1), g->its,
+                imin(g->ite, g->ide - 1));
+  if (b.k1 + 1 > g->nz) return PYWRF_E_RANGE;
+  PRho a = {al_in, p_in, ph, alt, t_2, t_1, c2a, mu, mut, rdnw, al_out, p_out, pm1, F(t0), F(smdiv), step};
+  PW_LAUNCH_BOX(k_calc_p_rho, whole(g), s, dims_of(g), b, a);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ calc_coef_w (A37)
+__global__ void k_calc_coef_w(Box b, Dims D, float* a, float* alpha, float* gamma, const float* mut,
+                              const float* cqw, const float* rdn, const float* rdnw, const float* c2a, float C,
+                              float m2C, float p2C, float lid, int kde) {
+  int j, i;
+  if (!box_column(b, j, i)) return;
+  float m = AT2(mut, j, i);
+  float r2 = rdnw[kde - 2] * rdnw[kde - 2];
+  AT(a, 1, j, i) = 0.0f;
+  float atop = m2C * r2 * AT(c2a, kde - 2, j, i) * lid / (m * m);
+  AT(a, kde - 1, j, i) = atop;
+  AT(gamma, 0, j, i) = 0.0f;
+  float gm = 0.0f;
+  for (int k = 1; k < kde - 1; ++k) {
+    float cq = AT(cqw, k, j, i), c2 = AT(c2a, k, j, i), c2m = AT(c2a, k - 1, j, i);
+    float ak = 0.0f;
+    if (k >= 2) {
+      ak = -cq * C * rdn[k] * rdnw[k - 1] * c2m / (m * m);
+      AT(a, k, j, i) = ak;
+    }
+    float bb = 1.0f + cq * C * rdn[k] * (rdnw[k] * c2 / (m * m) + rdnw[k - 1] * c2m / (m * m));
+    float cc = -cq * C * rdn[k] * rdnw[k] * c2 / (m * m);
+    float al = 1.0f / (bb - ak * gm);
+    gm = cc * al;
+    AT(alpha, k, j, i) = al;
+    AT(gamma, k, j, i) = gm;
+  }
+  float bb = 1.0f + p2C * r2 * AT(c2a, kde - 2, j, i) / (m * m);
+  AT(alpha, kde - 1, j, i) = 1.0f / (bb - atop * gm);
+  AT(gamma, kde - 1, j, i) = 0.0f;
+}
+
+extern "C" int pywrf_calc_coef_w(const pywrf_grid_t* g, float* a, float* alpha, float* gamma, const float* mut,
+                                 const float* cqw, const float* rdn, const float* rdnw, const float* c2a,
+                                 double dts, double g_, double epssm, int top_lid, pywrf_stream_t s) {
+  CHECK(g, 0);
+  if (g->kde > g->nz || g->kde < 3) return PYWRF_E_RANGE;
+  double C = (0.5 * dts * g_ * (1. + epssm));
+  C = C * C;
+  Box b = mkbox(0, 1, g->jts, imin(g->jte, g->jde - 1), g->its, imin(g->ite, g->ide - 1));
+  PW_LAUNCH_COLS(k_calc_coef_w, b, s, dims_of(g), a, alpha, gamma, mut, cqw, rdn, rdnw, c2a, F(C), F(-2. * C),
+                 F(2. * C), top_lid ? 0.0f : 1.0f, g->kde);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ advance_uv (A38)
+struct AdvUV {
+  const float *u_in, *ru_tend, *v_in, *rv_tend, *p, *pb, *ph, *php, *alt, *al, *mu, *muu, *cqu, *muv, *cqv, *mudf;
+  const float *msfux, *msfuy, *msfvx, *msfvx_inv, *msfvy, *cf, *fnm, *fnp, *rdnw;
+  float *u, *v;
+  float rdx, rdy, dts, dfx, dfy;
+  int kde;
+};
+
+__device__ inline float uv_dpn(const Dims& D, const AdvUV& a, int k, int j, int i, int dj, int di) {
+  auto P = [&](int kk, int jj, int ii) { return AT(a.p, kk, jj, ii); };
+  if (k == 0)
+    return 0.5f * (a.cf[0] * (P(0, j, i) + P(0, j + dj, i + di)) + a.cf[1] * (P(1, j, i) + P(1, j + dj, i + di)) +
+                   a.cf[2] * (P(2, j, i) + P(2, j + dj, i + di)));
+  if (k == a.kde - 1) return 0.0f;
+  return 0.5f * (a.fnm[k] * (P(k, j, i) + P(k, j + dj, i + di)) + a.fnp[k] * (P(k - 1, j, i) + P(k - 1, j + dj, i + di)));
+}
+
+__device__ inline float uv_dpxy(const Dims& D, const AdvUV& a, int k, int j, int i, int dj, int di, const float* mv,
+                                const float* mx, const float* my, float rd) {
+  int jd = j + dj, id = i + di;
+  float r = AT2(mx, j, i) / AT2(my, j, i);
+  float d = r * 0.5f * rd * AT2(mv, j, i) *
+            (((AT(a.ph, k + 1, j, i) - AT(a.ph, k + 1, jd, id)) + (AT(a.ph, k, j, i) - AT(a.ph, k, jd, id))) +
+             (AT(a.alt, k, j, i) + AT(a.alt, k, jd, id)) * (AT(a.p, k, j, i) - AT(a.p, k, jd, id)) +
+             (AT(a.al, k, j, i) + AT(a.al, k, jd, id)) * (AT(a.pb, k, j, i) - AT(a.pb, k, jd, id)));
+  return d + r * rd * (AT(a.php, k, j, i) - AT(a.php, k, jd, id)) *
+                 (a.rdnw[k] * (uv_dpn(D, a, k + 1, j, i, dj, di) - uv_dpn(D, a, k, j, i, dj, di)) -
+                  0.5f * (AT2(a.mu, jd, id) + AT2(a.mu, j, i)));
+}
+
+__global__ void k_advance_uv(Box all, Dims D, AdvUV a, Box bu, Box bv) {
+  int k, j, i;
+  if (!box_point(all, k, j, i)) return;
+  long q = D.ix(k, j, i);
+  float u = a.u_in[q];
+  if (bu.has(k, j, i)) {
+    u = u + a.dts * a.ru_tend[q];
+    float mxy = a.dfx * (AT2(a.mudf, j, i) - AT2(a.mudf, j, i - 1)) / AT2(a.msfuy, j, i);
+    float d = uv_dpxy(D, a, k, j, i, 0, -1, a.muu, a.msfux, a.msfuy, a.rdx);
+    u = u - a.dts * a.cqu[q] * d + mxy;
+  }
+  a.u[q] = u;
+  float v = a.v_in[q];
+  if (bv.has(k, j, i)) {
+    v = v + a.dts * a.rv_tend[q];
+    float mxy = a.dfy * (AT2(a.mudf, j, i) - AT2(a.mudf, j - 1, i)) * AT2(a.msfvx_inv, j, i);
+    float d = uv_dpxy(D, a, k, j, i, -1, 0, a.muv, a.msfvy, a.msfvx, a.rdy);
+    v = v - a.dts * a.cqv[q] * d + mxy;
+  }
+  a.v[q] = v;
+}
+
+extern "C" int pywrf_advance_uv(const pywrf_grid_t* g, const float* u_in, const float* ru_tend, const float* v_in,
+                                const float* rv_tend, const float* p, const float* pb, const float* ph,
+                                const float* php, const float* alt, const float* al, const float* mu,
+                                const float* muu, const float* cqu, const float* muv, const float* cqv,
+                                const float* mudf, const float* msfux, const float* msfuy, const float* msfvx,
+                                const float* msfvx_inv, const float* msfvy, double rdx, double rdy, double dts,
+                                const float* cf, const float* fnm, const float* fnp, double emdiv,
+                                const float* rdnw, int spec_zone, float* u_out, float* v_out, pywrf_stream_t s) {
+  CHECK(g, 1);
+  int k1 = imin(g->kte, g->kde - 1);
+  if (k1 + 1 > g->nz || g->kde > g->nz) return PYWRF_E_RANGE;
+  int i0 = imax(g->its, g->ids + spec_zone), i1 = imin(g->ite, g->ide - spec_zone - 1);
+  int j0 = imax(g->jts, g->jds + spec_zone), j1 = imin(g->jte, g->jde - spec_zone - 1);
+  int i1u = imin(g->ite, g->ide - spec_zone), j1v = imin(g->jte, g->jde - spec_zone);
+  double dx = 1. / rdx, dy = 1. / rdy;
+  AdvUV a = {u_in, ru_tend, v_in, rv_tend, p, pb, ph, php, alt, al, mu, muu, cqu, muv, cqv, mudf, msfux, msfuy,
+             msfvx, msfvx_inv, msfvy, cf, fnm, fnp, rdnw, u_out, v_out, F(rdx), F(rdy), F(dts), F(-emdiv * dx),
+             F(-emdiv * dy), g->kde};
+  Box bu = mkbox(g->kts, k1, j0, j1, i0, i1u);
+  Box bv = mkbox(g->kts, k1, j0, j1v, i0, i1);
+  PW_LAUNCH_BOX(k_advance_uv, whole(g), s, dims_of(g), a, bu, bv);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ advance_mu_t (A40)
+struct AdvMuT {
+  const float *ww_in, *ww_1, *u, *u_1, *v, *v_1, *mu_in, *mut, *muave_in, *muts_in, *muu, *muv, *mudf_in;
+  const float *t_in, *t_1, *t_ave_in, *ft, *mu_tend, *dnw, *fnm, *fnp, *rdnw;
+  const float *msfuy, *msfvx_inv, *msftx, *msfty;
+  float *ww, *t, *t_ave, *muave, *muts, *mudf, *mu;
+  float rdx, rdy, dts, ep1, em1, hrdx, hrdy;
+  int k1, kde, j0, j1, i0, i1;
+};
+
+__device__ inline float mut_dvdxi(const Dims& D, const AdvMuT& a, int k, int j, int i) {
+  float vy = (AT(a.v, k, j + 1, i) + AT2(a.muv, j + 1, i) * AT(a.v_1, k, j + 1, i) * AT2(a.msfvx_inv, j + 1, i)) -
+             (AT(a.v, k, j, i) + AT2(a.muv, j, i) * AT(a.v_1, k, j, i) * AT2(a.msfvx_inv, j, i));
+  float ux = (AT(a.u, k, j, i + 1) + AT2(a.muu, j, i + 1) * AT(a.u_1, k, j, i + 1) / AT2(a.msfuy, j, i + 1)) -
+             (AT(a.u, k, j, i) + AT2(a.muu, j, i) * AT(a.u_1, k, j, i) / AT2(a.msfuy, j, i));
+  return AT2(a.msftx, j, i) * AT2(a.msfty, j, i) * (a.rdy * vy + a.rdx * ux);
+}
+
+// one thread per (j, i) column of the whole plane; columns outside the box copy
+__global__ void k_advance_mu_t(Box cols, Dims D, AdvMuT a) {
+  int j, i;
+  if (!box_column(cols, j, i)) return;
+  long q2 = D.ix2(j, i);
+  bool h = j >= a.j0 && j < a.j1 && i >= a.i0 && i < a.i1;
+  if (!h) {
+    a.muave[q2] = a.muave_in[q2];
+    a.muts[q2] = a.muts_in[q2];
+    a.mudf[q2] = a.mudf_in[q2];
+    a.mu[q2] = a.mu_in[q2];
+    for (int k = 0; k < D.nz; ++k) {
+      long q = D.ix(k, j, i);
+      a.ww[q] = a.ww_in[q];
+      a.t[q] = a.t_in[q];
+      a.t_ave[q] = a.t_ave_in[q];
+    }
+    return;
+  }
+  float dmdt = torch_outer_sum(D.nz, [&](int k) { return k < a.k1 ? a.dnw[k] * mut_dvdxi(D, a, k, j, i) : 0.0f; });
+  float mt = a.mu_tend[q2];
+  float mu_old = a.mu_in[q2];
+  float mu = mu_old + a.dts * (dmdt + mt);
+  a.mu[q2] = mu;
+  a.mudf[q2] = dmdt + mt;
+  a.muts[q2] = a.mut[q2] + mu;
+  a.muave[q2] = 0.5f * (a.ep1 * mu + a.em1 * (mu_old + 0.0f));
+  float my = a.msfty[q2], mx = a.msftx[q2];
+  // eta-velocity recurrence, then subtract ww_1 on [0, k1)
+  float wk = a.ww_in[D.ix(0, j, i)];
+  a.ww[D.ix(0, j, i)] = wk - a.ww_1[D.ix(0, j, i)];
+  for (int k = 1; k < a.k1; ++k) {
+    wk = wk - a.dnw[k - 1] * (dmdt + mut_dvdxi(D, a, k - 1, j, i) + mt) / my;
+    long q = D.ix(k, j, i);
+    a.ww[q] = wk - a.ww_1[q];
+  }
+  for (int k = a.k1; k < D.nz; ++k) {
+    long q = D.ix(k, j, i);
+    a.ww[q] = a.ww_in[q];
+  }
+  // theta update with the time-averaged flux divergence
+  for (int k = 0; k < D.nz; ++k) {
+    long q = D.ix(k, j, i);
+    float t = a.t_in[q];
+    if (k >= a.k1) {
+      a.t[q] = t;
+      a.t_ave[q] = a.t_ave_in[q];
+      continue;
+    }
+    a.t_ave[q] = t + 0.0f;
+    t = t + my * a.dts * a.ft[q];
+    auto wdtn = [&](int kk) -> float {
+      if (kk <= 0 || kk >= a.k1) return 0.0f;
+      return a.ww[D.ix(kk, j, i)] * (a.fnm[kk] * AT(a.t_1, kk, j, i) + a.fnp[kk] * AT(a.t_1, kk - 1, j, i));
+    };
+    float t1 = a.t_1[q];
+    float yterm = a.hrdy * (AT(a.v, k, j + 1, i) * (AT(a.t_1, k, j + 1, i) + t1) - AT(a.v, k, j, i) * (t1 + AT(a.t_1, k, j - 1, i)));
+    float xterm = a.hrdx * (AT(a.u, k, j, i + 1) * (AT(a.t_1, k, j, i + 1) + t1) - AT(a.u, k, j, i) * (t1 + AT(a.t_1, k, j, i - 1)));
+    t = t - a.dts * my * (mx * (yterm + xterm) + a.rdnw[k] * (wdtn(k + 1) - wdtn(k)));
+    a.t[q] = t;
+  }
+}
+
+extern "C" int pywrf_advance_mu_t(const pywrf_grid_t* g, const float* ww_in, const float* ww_1, const float* u,
+                                  const float* u_1, const float* v, const float* v_1, const float* mu_in,
+                                  const float* mut, const float* muave_in, const float* muts_in, const float* muu,
+                                  const float* muv, const float* mudf_in, const float* t_in, const float* t_1,
+                                  const float* t_ave_in, const float* ft, const float* mu_tend, double rdx,
+                                  double rdy, double dts, double epssm, const float* dnw, const float* fnm,
+                                  const float* fnp, const float* rdnw, const float* msfuy, const float* msfvx_inv,
+                                  const float* msftx, const float* msfty, float* ww_out, float* t_out,
+                                  float* t_ave_out, float* muave_out, float* muts_out, float* mudf_out,
+                                  float* mu_out, pywrf_stream_t s) {
+  CHECK(g, 1);
+  if (g->kte > g->nz || g->kte < 2) return PYWRF_E_RANGE;
+  AdvMuT a = {ww_in, ww_1, u, u_1, v, v_1, mu_in, mut, muave_in, muts_in, muu, muv, mudf_in, t_in, t_1, t_ave_in,
+              ft, mu_tend, dnw, fnm, fnp, rdnw, msfuy, msfvx_inv, msftx, msfty, ww_out, t_out, t_ave_out,
+              muave_out, muts_out, mudf_out, mu_out, F(rdx), F(rdy), F(dts), F(1. + epssm), F(1. - epssm),
+              F(0.5 * rdx), F(0.5 * rdy), g->kte - 1, g->kde, imax(g->jts, g->jds + 1), imin(g->jte, g->jde - 2),
+              imax(g->its, g->ids + 1), imin(g->ite, g->ide - 2)};
+  PW_LAUNCH_COLS(k_advance_mu_t, plane(g), s, dims_of(g), a);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ advance_w (A41)
+struct AdvW {
+  const float *w_in, *rw_tend, *ww, *w_save, *u, *v, *mut, *muave, *muts, *t_2ave_in, *t_2, *t_1, *ph_in, *ph_1;
+  const float *phb, *ph_tend, *ht, *c2a, *cqw, *alt, *a, *alpha, *gamma, *fnm, *fnp, *rdnw, *rdn, *cf;
+  const float *msftx, *msfty;
+  float *t_2ave, *w, *ph;
+  float dts, t0, ep1, em1, hgem1, rdx, rdy, hdtsg, dtsg, mhdtsg, dampmag, hpi, hdepth;
+  int k1, j0, j1, i0, i1;
+};
+
+__global__ void k_advance_w(Box cols, Dims D, AdvW a) {
+  int j, i;
+  if (!box_column(cols, j, i)) return;
+  long q2 = D.ix2(j, i);
+  bool h = j >= a.j0 && j < a.j1 && i >= a.i0 && i < a.i1;
+  const int k1 = a.k1;
+  if (!h) {
+    for (int k = 0; k < D.nz; ++k) {
+      long q = D.ix(k, j, i);
+      a.t_2ave[q] = a.t_2ave_in[q];
+      a.w[q] = a.w_in[q];
+      a.ph[q] = a.ph_in[q];
+    }
+    return;
+  }
+  const float G = F(pw::g);
+  float mut = a.mut[q2], muts = a.muts[q2], muave = a.muave[q2], my = a.msfty[q2], mx = a.msftx[q2];
+  float mi = 1.0f / my;
+  // t_2ave on [0, k1)
+  for (int k = 0; k < D.nz; ++k) {
+    long q = D.ix(k, j, i);
+    float t = a.t_2ave_in[q];
+    if (k < k1) {
+      t = 0.5f * (a.ep1 * a.t_2[q] + a.em1 * t);
+      t = (t + muave * a.t0) / (muts * (a.t0 + a.t_1[q]));
+    }
+    a.t_2ave[q] = t;
+  }
+  // rhs on [1, k1] held in the ph output; ph[0] copies through
+  auto wdwn = [&](int k) {
+    return 0.5f * (AT(a.ww, k, j, i) + AT(a.ww, k - 1, j, i)) * a.rdnw[k - 1] *
+           (AT(a.ph_1, k, j, i) - AT(a.ph_1, k - 1, j, i) + AT(a.phb, k, j, i) - AT(a.phb, k - 1, j, i));
+  };
+  a.ph[D.ix(0, j, i)] = a.ph_in[D.ix(0, j, i)];
+  for (int k = 1; k <= k1; ++k) {
+    long q = D.ix(k, j, i);
+    float r = a.dts * (a.ph_tend[q] + a.hgem1 * a.w_in[q]);
+    if (k < k1) r = r - a.dts * (a.fnm[k] * wdwn(k + 1) + a.fnp[k] * wdwn(k));
+    a.ph[q] = a.ph_in[q] + my * r / mut;
+  }
+  for (int k = k1 + 1; k < D.nz; ++k) a.ph[D.ix(k, j, i)] = a.ph_in[D.ix(k, j, i)];
+  auto RHS = [&](int k) -> float { return k >= 1 ? AT(a.ph, k, j, i) : 0.0f; };
+  auto PH = [&](int k) { return AT(a.ph_in, k, j, i); };
+  // surface w from the terrain slope
+  {
+    float cf1 = a.cf[0], cf2 = a.cf[1], cf3 = a.cf[2];
+    auto V3 = [&](int jj, int ii) {
+      return cf1 * AT(a.v, 0, jj, ii) + cf2 * AT(a.v, 1, jj, ii) + cf3 * AT(a.v, 2, jj, ii);
+    };
+    auto U3 = [&](int jj, int ii) {
+      return cf1 * AT(a.u, 0, jj, ii) + cf2 * AT(a.u, 1, jj, ii) + cf3 * AT(a.u, 2, jj, ii);
+    };
+    float htc = AT2(a.ht, j, i);
+    float ty = (AT2(a.ht, j + 1, i) - htc) * V3(j + 1, i) + (htc - AT2(a.ht, j - 1, i)) * V3(j, i);
+    float tx = (AT2(a.ht, j, i + 1) - htc) * U3(j, i + 1) + (htc - AT2(a.ht, j, i - 1)) * U3(j, i);
+    a.w[D.ix(0, j, i)] = my * 0.5f * a.rdy * ty + mx * 0.5f * a.rdx * tx;
+  }
+  // explicit predictor on [1, k1)
+  for (int k = 1; k < k1; ++k) {
+    long q = D.ix(k, j, i);
+    float c2 = a.c2a[q], c2m = AT(a.c2a, k - 1, j, i);
+    float X = a.hdtsg * a.rdn[k] *
+              (c2 * a.rdnw[k] / mut * (a.ep1 * (RHS(k + 1) - RHS(k)) + a.em1 * (PH(k + 1) - PH(k))) -
+               c2m * a.rdnw[k - 1] / mut * (a.ep1 * (RHS(k) - RHS(k - 1)) + a.em1 * (PH(k) - PH(k - 1))));
+    float Y = a.rdn[k] * (c2 * a.alt[q] * a.t_2ave[q] - c2m * AT(a.alt, k - 1, j, i) * AT(a.t_2ave, k - 1, j, i)) - muave;
+    a.w[q] = a.w_in[q] + a.dts * a.rw_tend[q] + mi * a.cqw[q] * X + a.dtsg * mi * Y;
+  }
+  {
+    long q = D.ix(k1, j, i);
+    float rn = a.rdnw[k1 - 1];
+    float c2m = AT(a.c2a, k1 - 1, j, i);
+    float Z = a.mhdtsg / mut * (rn * rn) * 2.0f * c2m *
+                  (a.ep1 * (RHS(k1) - RHS(k1 - 1)) + a.em1 * (PH(k1) - PH(k1 - 1))) -
+              a.dtsg * (2.0f * rn * c2m * AT(a.alt, k1 - 1, j, i) * AT(a.t_2ave, k1 - 1, j, i) + muave);
+    a.w[q] = a.w_in[q] + a.dts * a.rw_tend[q] + mi * Z;
+  }
+  for (int k = k1 + 1; k < D.nz; ++k) a.w[D.ix(k, j, i)] = a.w_in[D.ix(k, j, i)];
+  // Thomas forward sweep and back substitution
+  float wm = a.w[D.ix(0, j, i)];
+  for (int k = 1; k <= k1; ++k) {
+    long q = D.ix(k, j, i);
+    wm = (a.w[q] - a.a[q] * wm) * a.alpha[q];
+    a.w[q] = wm;
+  }
+  float wp = wm;
+  for (int k = k1 - 1; k >= 1; --k) {
+    long q = D.ix(k, j, i);
+    wp = a.w[q] - a.gamma[q] * wp;
+    a.w[q] = wp;
+  }
+  // Rayleigh damping in the top hdepth metres, then phi
+  float htop = (AT(a.ph_1, k1, j, i) + AT(a.phb, k1, j, i)) / G;
+  float hbot = htop - a.hdepth;
+  for (int k = 1; k <= k1; ++k) {
+    long q = D.ix(k, j, i);
+    float hk = (a.ph_1[q] + a.phb[q]) / G;
+    float dw = 0.0f;
+    if (hk >= hbot) {
+      float sn = sinf(a.hpi * (hk - hbot) / a.hdepth);
+      dw = a.dampmag * (sn * sn);
+    }
+    float w = (a.w[q] - dw * mut * a.w_save[q]) / (1.0f + dw);
+    a.w[q] = w;
+    a.ph[q] = a.ph[q] + my * 0.5f * a.dts * G * a.ep1 * w / muts;
+  }
+}
+
+extern "C" int pywrf_advance_w(const pywrf_grid_t* g, const float* w_in, const float* rw_tend, const float* ww,
+                               const float* w_save, const float* u, const float* v, const float* mut,
+                               const float* muave, const float* muts, const float* t_2ave_in, const float* t_2,
+                               const float* t_1, const float* ph_in, const float* ph_1, const float* phb,
+                               const float* ph_tend, const float* ht, const float* c2a, const float* cqw,
+                               const float* alt, const float* a, const float* alpha, const float* gamma,
+                               double rdx, double rdy, double dts, double t0, double epssm, const float* fnm,
+                               const float* fnp, const float* rdnw, const float* rdn, const float* cf,
+                               const float* msftx, const float* msfty, float* t_2ave_out, float* w_out,
+                               float* ph_out, pywrf_stream_t s) {
+  CHECK(g, 1);
+  if (g->kte > g->nz || g->kte < 3) return PYWRF_E_RANGE;
+  const double pi_ = 3.14159265359, G = pw::g;
+  AdvW p = {w_in, rw_tend, ww, w_save, u, v, mut, muave, muts, t_2ave_in, t_2, t_1, ph_in, ph_1, phb, ph_tend, ht,
+            c2a, cqw, alt, a, alpha, gamma, fnm, fnp, rdnw, rdn, cf, msftx, msfty, t_2ave_out, w_out, ph_out,
+            F(dts), F(t0), F(1. + epssm), F(1. - epssm), F(0.5 * G * (1. - epssm)), F(rdx), F(rdy),
+            F(0.5 * dts * G), F(dts * G), F(-0.5 * dts * G), F(dts * 0.2), F(0.5 * pi_), F(5000.), g->kte - 1,
+            imax(g->jts, g->jds + 1), imin(g->jte, g->jde - 2), imax(g->its, g->ids + 1), imin(g->ite, g->ide - 2)};
+  PW_LAUNCH_COLS(k_advance_w, plane(g), s, dims_of(g), p);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ sumflux (A42)
+__global__ void k_sumflux(Box b, Dims D, const float* ru, const float* rv, const float* ww, const float* u_lin,
+                          const float* v_lin, const float* ww_lin, const float* muu, const float* muv, float* ru_m,
+                          float* rv_m, float* ww_m, const float* msfuy, const float* msfvx_inv, Box bu, Box bv,
+                          Box bw, int first, int last, float n) {
+  int k, j, i;
+  if (!box_point(b, k, j, i)) return;
+  long q = D.ix(k, j, i);
+  float rm = first ? 0.0f : ru_m[q], vm = first ? 0.0f : rv_m[q], wm = first ? 0.0f : ww_m[q];
+  if (bu.has(k, j, i)) {
+    rm = rm + ru[q];
+    if (last) rm = rm / n + AT2(muu, j, i) * u_lin[q] / AT2(msfuy, j, i);
+  }
+  if (bv.has(k, j, i)) {
+    vm = vm + rv[q];
+    if (last) vm = vm / n + AT2(muv, j, i) * v_lin[q] * AT2(msfvx_inv, j, i);
+  }
+  if (bw.has(k, j, i)) {
+    wm = wm + ww[q];
+    if (last) wm = wm / n + ww_lin[q];
+  }
+  ru_m[q] = rm;
+  rv_m[q] = vm;
+  ww_m[q] = wm;
+}
+
+extern "C" int pywrf_sumflux(const pywrf_grid_t* g, const float* ru, const float* rv, const float* ww,
+                             const float* u_lin, const float* v_lin, const float* ww_lin, const float* muu,
+                             const float* muv, float* ru_m, float* rv_m, float* ww_m, const float* msfuy,
+                             const float* msfvx_inv, int iteration, int number_of_small_timesteps,
+                             pywrf_stream_t s) {
+  CHECK(g, 0);
+  if (g->kte > g->nz || number_of_small_timesteps == 0) return PYWRF_E_RANGE;
+  int mini = imin(g->ide - 1, g->ite), minj = imin(g->jde - 1, g->jte), mink = imin(g->kde - 1, g->kte);
+  Box bu = mkbox(g->kts, mink, g->jts, minj, g->its, g->ite);
+  Box bv = mkbox(g->kts, mink, g->jts, g->jte, g->its, mini);
+  Box bw = mkbox(g->kts, g->kte, g->jts, minj, g->its, mini);
+  Box b = mkbox(g->kts, g->kte, g->jts, g->jte, g->its, g->ite);
+  PW_LAUNCH_BOX(k_sumflux, b, s, dims_of(g), ru, rv, ww, u_lin, v_lin, ww_lin, muu, muv, ru_m, rv_m, ww_m, msfuy,
+                msfvx_inv, bu, bv, bw, iteration == 1, iteration == number_of_small_timesteps,
+                (float)number_of_small_timesteps);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ calc_mu_uv_1 (A45)
+__global__ void k_calc_mu_uv_1(Box b, Dims D, const float* mu, float* muu, float* muv, int jts, int jtf, int jte,
+                               int its, int itf, int ite) {
+  int j, i;
+  if (!box_column(b, j, i)) return;
+  if (j < jtf) {
+    if (i >= its + 1 && i < ite - 1) AT2(muu, j, i) = 0.5f * (AT2(mu, j, i) + AT2(mu, j, i - 1));
+    else if (i == its) AT2(muu, j, i) = AT2(mu, j, i);
+    if (i == ite - 1) AT2(muu, j, i) = AT2(mu, j, ite - 2);
+  }
+  if (i < itf) {
+    if (j >= jts + 1 && j < jte - 1) AT2(muv, j, i) = 0.5f * (AT2(mu, j, i) + AT2(mu, j - 1, i));
+    else if (j == jts) AT2(muv, j, i) = AT2(mu, j, i);
+    if (j == jte - 1) AT2(muv, j, i) = AT2(mu, jte - 2, i);
+  }
+}
+
+extern "C" int pywrf_calc_mu_uv_1(const pywrf_grid_t* g, const float* mu, float* muu, float* muv, pywrf_stream_t s) {
+  CHECK(g, 1);
+  int jtf = imin(g->jte, g->jde - 1), itf = imin(g->ite, g->ide - 1);
+  Box b = mkbox(0, 1, g->jts, g->jte, g->its, g->ite);
+  PW_LAUNCH_COLS(k_calc_mu_uv_1, b, s, dims_of(g), mu, muu, muv, g->jts, jtf, g->jte, g->its, itf, g->ite);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ small_step_finish (A46)
+struct SsFin {
+  const float *u_2, *v_2, *w_2, *t_2, *ph_2, *ww1, *mut, *muts, *muu, *muus, *muv, *muvs;
+  const float *u_save, *v_save, *w_save, *t_save, *ph_save, *mu_save, *msfuy, *msfvx, *msfty, *h_diabatic;
+  float *ww, *mu_2, *u, *v, *w, *ph, *t;
+  float dtsn;
+  int last, kde, j0, j1, i0, i1, jte, ite;
+};
+
+__global__ void k_ssfinish(Box all, Dims D, SsFin a) {
+  int k, j, i;
+  if (!box_point(all, k, j, i)) return;
+  long q = D.ix(k, j, i), q2 = D.ix2(j, i);
+  bool hcol = j >= a.j0 && j < a.j1 && i >= a.i0 && i < a.i1;
+  float v = a.v_2[q];
+  if (k < a.kde - 1 && j >= a.j0 && j < a.jte && i >= a.i0 && i < a.i1)
+    v = (a.msfvx[q2] * v + a.v_save[q] * a.muv[q2]) / a.muvs[q2];
+  a.v[q] = v;
+  float u = a.u_2[q];
+  if (k < a.kde - 1 && j >= a.j0 && j < a.j1 && i >= a.i0 && i < a.ite)
+    u = (a.msfuy[q2] * u + a.u_save[q] * a.muu[q2]) / a.muus[q2];
+  a.u[q] = u;
+  float w = a.w_2[q], ph = a.ph_2[q];
+  if (k < a.kde && hcol) {
+    w = (a.msfty[q2] * w + a.w_save[q] * a.mut[q2]) / a.muts[q2];
+    ph = ph + a.ph_save[q];
+    a.ww[q] = a.ww[q] + a.ww1[q];
+  }
+  a.w[q] = w;
+  a.ph[q] = ph;
+  float t = a.t_2[q];
+  if (k < a.kde - 1 && hcol) {
+    if (!a.last) t = (t + a.t_save[q] * a.mut[q2]) / a.muts[q2];
+    else t = (t - a.dtsn * a.mut[q2] * a.h_diabatic[q] + a.t_save[q] * a.mut[q2]) / a.muts[q2];
+  }
+  a.t[q] = t;
+  if (k == 0 && hcol) a.mu_2[q2] = a.mu_2[q2] + a.mu_save[q2];
+}
+
+extern "C" int pywrf_small_step_finish(const pywrf_grid_t* g, const float* u_2, const float* v_2, const float* w_2,
+                                       const float* t_2, const float* ph_2, float* ww, const float* ww1,
+                                       float* mu_2, const float* mut, const float* muts, const float* muu,
+                                       const float* muus, const float* muv, const float* muvs,
+                                       const float* u_save, const float* v_save, const float* w_save,
+                                       const float* t_save, const float* ph_save, const float* mu_save,
+                                       const float* msfuy, const float* msfvx, const float* msfty,
+                                       const float* h_diabatic, int number_of_small_timesteps, double dts,
+                                       int rk_step, int rk_order, float* u_out, float* v_out, float* w_out,
+                                       float* ph_out, float* t_out, pywrf_stream_t s) {
+  CHECK(g, 0);
+  if (g->kde > g->nz || g->kds != 0) return PYWRF_E_RANGE;
+  SsFin a = {u_2, v_2, w_2, t_2, ph_2, ww1, mut, muts, muu, muus, muv, muvs, u_save, v_save, w_save, t_save,
+             ph_save, mu_save, msfuy, msfvx, msfty, h_diabatic, ww, mu_2, u_out, v_out, w_out, ph_out, t_out,
+             F(dts * number_of_small_timesteps), rk_step >= rk_order, g->kde, g->jts, imin(g->jte, g->jde - 1),
+             g->its, imin(g->ite, g->ide - 1), g->jte, g->ite};
+  PW_LAUNCH_BOX(k_ssfinish, whole(g), s, dims_of(g), a);
+  PW_RET();
+}

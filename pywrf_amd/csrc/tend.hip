@@ -1,0 +1,621 @@
+// Large-step tendencies other than advection: geopotential RHS, horizontal
+// pressure gradient, vertical PGF and buoyancy, Coriolis, curvature,
+// horizontal and vertical diffusion, and the first-stage tendency fold
+// (pywrf/wrf_dynamics.py:2031-3148).  One launch per call; a thread owns one
+// output point and applies the reference's statements for that point in order.
+#include "common.h"
+#include "consts.h"
+
+#define AT(p, k, j, i) (p)[D.ix((k), (j), (i))]
+#define AT2(p, j, i) (p)[D.ix2((j), (i))]
+
+int pw_check_grid(const pywrf_grid_t* g, int r);
+#define CHECK(g, r) do { int _e = pw_check_grid((g), (r)); if (_e) return _e; } while (0)
+
+// ------------------------------------------------------------------ rhs_ph (A26)
+struct RhsPh {
+  const float *u, *v, *ww, *ph, *phb, *w, *mut, *muu, *muv, *fnm, *fnp, *rdnw, *cfn, *cfn1;
+  const float *msfux, *msfvy, *msfty;
+  float* tend;
+  float rdx, rdy, c4x, c4y, c2x, c2y;  // 0.25*rd and 0.5*rd rounded once
+  int kte, ids, ide, jds, jde, jts, jtf, its, itf;
+};
+
+// One direction of the horizontal phi advection at (k, j, i); (dj, di) is the
+// unit step, order 6, 4 or 2 (wrf_dynamics.py:2082-2295).
+__device__ inline float ph_hflux(const Dims& D, const RhsPh& a, int k, int j, int i, int dj, int di, int order) {
+  bool top = (k == a.kte - 1);
+  const float* vel = dj ? a.v : a.u;
+  const float* muf = dj ? a.muv : a.muu;
+  const float* msf = dj ? a.msfvy : a.msfux;
+  float c = (top ? (dj ? a.c2y : a.c2x) : (dj ? a.c4y : a.c4x)) / AT2(a.msfty, j, i);
+  float cfn = *a.cfn, cfn1 = *a.cfn1;
+  auto V = [&](int n) {
+    int jj = j + n * dj, ii = i + n * di;
+    if (top) return cfn * AT(vel, k - 1, jj, ii) + cfn1 * AT(vel, k - 2, jj, ii);
+    return AT(vel, k, jj, ii) + AT(vel, k - 1, jj, ii);
+  };
+  auto P = [&](const float* f, int n) { return AT(f, k, j + n * dj, i + n * di); };
+  float F1 = AT2(muf, j + dj, i + di) * V(1) * AT2(msf, j + dj, i + di);
+  float F0 = AT2(muf, j, i) * V(0) * AT2(msf, j, i);
+  if (order == 6) {
+    float s = 45.0f * (P(a.ph, 1) - P(a.ph, -1)) - 9.0f * (P(a.ph, 2) - P(a.ph, -2)) + (P(a.ph, 3) - P(a.ph, -3)) +
+              45.0f * (P(a.phb, 1) - P(a.phb, -1)) - 9.0f * (P(a.phb, 2) - P(a.phb, -2)) +
+              (P(a.phb, 3) - P(a.phb, -3));
+    return c * ((F1 + F0) * F(1. / 60.) * s);
+  }
+  if (order == 4) {
+    float s = 8.0f * (P(a.ph, 1) - P(a.ph, -1)) - (P(a.ph, 2) - P(a.ph, -2)) + 8.0f * (P(a.phb, 1) - P(a.phb, -1)) -
+              (P(a.phb, 2) - P(a.phb, -2));
+    return c * ((F1 + F0) * F(1. / 12.) * s);
+  }
+  float d1 = P(a.phb, 1) - P(a.phb, 0) + P(a.ph, 1) - P(a.ph, 0);
+  float d0 = P(a.phb, 0) - P(a.phb, -1) + P(a.ph, 0) - P(a.ph, -1);
+  return c * (F1 * d1 + F0 * d0);
+}
+
+__device__ inline float ph_wdwn(const Dims& D, const RhsPh& a, int k, int j, int i) {
+  return 0.5f * (AT(a.ww, k, j, i) + AT(a.ww, k - 1, j, i)) * a.rdnw[k - 1] *
+         (AT(a.ph, k, j, i) - AT(a.ph, k - 1, j, i) + AT(a.phb, k, j, i) - AT(a.phb, k - 1, j, i));
+}
+
+__global__ void k_rhs_ph(Box b, Dims D, RhsPh a) {
+  int k, j, i;
+  if (!box_point(b, k, j, i)) return;
+  long q = D.ix(k, j, i);
+  float t = a.tend[q];
+  if (k < a.kte - 1) t = t - (a.fnm[k] * ph_wdwn(D, a, k + 1, j, i) + a.fnp[k] * ph_wdwn(D, a, k, j, i));
+  else t = 0.0f;
+  t = t + AT2(a.mut, j, i) * F(pw::g) * a.w[q] / AT2(a.msfty, j, i);
+  // y direction: 6th order inside, 4th order at jds+2 and jde-4, 2nd order at jds+1 and jde-3
+  int oy = 0;
+  if (j >= imax(a.jts, a.jds + 3) && j < imin(a.jtf, a.jde - 4)) oy = 6;
+  else if (j == a.jds + 2 || j == a.jde - 4) oy = 4;
+  else if (j == a.jds + 1 || j == a.jde - 3) oy = 2;
+  if (oy) t = t - ph_hflux(D, a, k, j, i, 1, 0, oy);
+  // x direction: 6th order inside, 2nd order at ids+1 and ide-3 (no 4th-order columns)
+  int ox = 0;
+  if (i >= imax(a.its, a.ids + 3) && i < imin(a.itf, a.ide - 4)) ox = 6;
+  else if (i == a.ids + 1 || i == a.ide - 3) ox = 2;
+  if (ox) t = t - ph_hflux(D, a, k, j, i, 0, 1, ox);
+  a.tend[q] = t;
+}
+
+extern "C" int pywrf_rhs_ph(const pywrf_grid_t* g, float* ph_tend, const float* u, const float* v,
+                            const float* ww, const float* ph, const float* phb, const float* w, const float* mut,
+                            const float* muuf, const float* muvf, const float* fnm, const float* fnp,
+                            const float* rdnw, const float* cfn, const float* cfn1, double rdx, double rdy,
+                            const float* msfux, const float* msfvy, const float* msfty, pywrf_stream_t s) {
+  CHECK(g, 4);
+  if (g->kte > g->nz || g->kte < 3) return PYWRF_E_RANGE;
+  RhsPh a;
+  a.u = u; a.v = v; a.ww = ww; a.ph = ph; a.phb = phb; a.w = w; a.mut = mut; a.muu = muuf; a.muv = muvf;
+  a.fnm = fnm; a.fnp = fnp; a.rdnw = rdnw; a.cfn = cfn; a.cfn1 = cfn1; a.msfux = msfux; a.msfvy = msfvy;
+  a.msfty = msfty; a.tend = ph_tend;
+  a.rdx = F(rdx); a.rdy = F(rdy);
+  a.c4x = F(0.25 * rdx); a.c4y = F(0.25 * rdy); a.c2x = F(0.5 * rdx); a.c2y = F(0.5 * rdy);
+  a.kte = g->kte; a.ids = g->ids; a.ide = g->ide; a.jds = g->jds; a.jde = g->jde;
+  a.jts = g->jts; a.jtf = imin(g->jte, g->jde - 1); a.its = g->its; a.itf = imin(g->ite, g->ide - 1);
+  Box b = mkbox(1, g->kte, a.jts, a.jtf, a.its, a.itf);
+  PW_LAUNCH_BOX(k_rhs_ph, b, s, dims_of(g), a);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ horizontal_pressure_gradient (A27)
+struct Hpg {
+  const float *ph, *alt, *p, *pb, *al, *php, *cqu, *cqv, *muu, *muv, *mu, *fnm, *fnp, *rdnw, *cf;
+  const float *msfux, *msfuy, *msfvx, *msfvy;
+  float *ru_tend, *rv_tend;
+  float rdx, rdy;
+  int ktf, kde;
+};
+
+// pressure on w level k averaged over the face between (j, i) and (j+dj, i+di)
+__device__ inline float hpg_dpn(const Dims& D, const Hpg& a, int k, int j, int i, int dj, int di) {
+  auto P = [&](int kk, int jj, int ii) { return AT(a.p, kk, jj, ii); };
+  if (k == 0)
+    return 0.5f * (a.cf[0] * (P(0, j + dj, i + di) + P(0, j, i)) + a.cf[1] * (P(1, j + dj, i + di) + P(1, j, i)) +
+                   a.cf[2] * (P(2, j + dj, i + di) + P(2, j, i)));
+  if (k == a.kde - 1) return 0.0f;
+  return 0.5f * (a.fnm[k] * (P(k, j + dj, i + di) + P(k, j, i)) + a.fnp[k] * (P(k - 1, j + dj, i + di) + P(k - 1, j, i)));
+}
+
+__device__ inline float hpg_dpx(const Dims& D, const Hpg& a, int k, int j, int i, int dj, int di, const float* mv,
+                                const float* mx, const float* my, float rd) {
+  int jd = j + dj, id = i + di;
+  float r = AT2(mx, j, i) / AT2(my, j, i);
+  float dpx = r * 0.5f * rd * AT2(mv, j, i) *
+              ((AT(a.ph, k + 1, j, i) - AT(a.ph, k + 1, jd, id) + AT(a.ph, k, j, i) - AT(a.ph, k, jd, id)) +
+               (AT(a.alt, k, j, i) + AT(a.alt, k, jd, id)) * (AT(a.p, k, j, i) - AT(a.p, k, jd, id)) +
+               (AT(a.al, k, j, i) + AT(a.al, k, jd, id)) * (AT(a.pb, k, j, i) - AT(a.pb, k, jd, id)));
+  dpx = dpx + r * rd * (AT(a.php, k, j, i) - AT(a.php, k, jd, id)) *
+                  (a.rdnw[k] * (hpg_dpn(D, a, k + 1, j, i, dj, di) - hpg_dpn(D, a, k, j, i, dj, di)) -
+                   0.5f * (AT2(a.mu, jd, id) + AT2(a.mu, j, i)));
+  return dpx;
+}
+
+__global__ void k_hpg(Box b, Dims D, Hpg a, Box bv, Box bu) {
+  int k, j, i;
+  if (!box_point(b, k, j, i)) return;
+  long q = D.ix(k, j, i);
+  if (bv.has(k, j, i))
+    a.rv_tend[q] = a.rv_tend[q] - a.cqv[q] * hpg_dpx(D, a, k, j, i, -1, 0, a.muv, a.msfvy, a.msfvx, a.rdy);
+  if (bu.has(k, j, i))
+    a.ru_tend[q] = a.ru_tend[q] - a.cqu[q] * hpg_dpx(D, a, k, j, i, 0, -1, a.muu, a.msfux, a.msfuy, a.rdx);
+}
+
+extern "C" int pywrf_horizontal_pressure_gradient(const pywrf_grid_t* g, float* ru_tend, float* rv_tend,
+                                                  const float* ph, const float* alt, const float* p,
+                                                  const float* pb, const float* al, const float* php,
+                                                  const float* cqu, const float* cqv, const float* muu,
+                                                  const float* muv, const float* mu, const float* fnm,
+                                                  const float* fnp, const float* rdnw, const float* cf, double rdx,
+                                                  double rdy, const float* msfux, const float* msfuy,
+                                                  const float* msfvx, const float* msfvy, pywrf_stream_t s) {
+  CHECK(g, 1);
+  Hpg a;
+  a.ph = ph; a.alt = alt; a.p = p; a.pb = pb; a.al = al; a.php = php; a.cqu = cqu; a.cqv = cqv; a.muu = muu;
+  a.muv = muv; a.mu = mu; a.fnm = fnm; a.fnp = fnp; a.rdnw = rdnw; a.cf = cf; a.msfux = msfux; a.msfuy = msfuy;
+  a.msfvx = msfvx; a.msfvy = msfvy; a.ru_tend = ru_tend; a.rv_tend = rv_tend; a.rdx = F(rdx); a.rdy = F(rdy);
+  a.ktf = imin(g->kte, g->kde - 1);
+  a.kde = g->kde;
+  if (a.ktf + 1 > g->nz || g->kde > g->nz) return PYWRF_E_RANGE;
+  Box bv = mkbox(0, a.ktf, g->jts + 1, g->jte - 1, g->its, imin(g->ite, g->ide - 1));
+  Box bu = mkbox(0, a.ktf, g->jts, imin(g->jte, g->jde - 1), g->its + 1, g->ite - 1);
+  Box b = mkbox(0, a.ktf, g->jts, g->jte, g->its, g->ite);
+  PW_LAUNCH_BOX(k_hpg, b, s, dims_of(g), a, bv, bu);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ pg_buoy_w (A28)
+__global__ void k_pg_buoy_w(Box b, Dims D, float* rw_tend, const float* p, float* cqw, const float* muf,
+                            const float* mubf, const float* rdnw, const float* rdn, float g_, const float* msfty,
+                            int kde) {
+  int k, j, i;
+  if (!box_point(b, k, j, i)) return;
+  long q = D.ix(k, j, i);
+  float cqo = cqw[q];
+  float rmsf = 1.0f / AT2(msfty, j, i);
+  if (k == kde - 2) {  // the top level reads cqw[kde-2] before it is overwritten
+    float cq1 = 1.0f / (1.0f + cqo);
+    float cq2 = cqo * cq1;
+    long t = D.ix(kde - 1, j, i);
+    rw_tend[t] = rw_tend[t] + rmsf * g_ * (cq1 * 2.0f * rdnw[kde - 2] * (-p[q]) - AT2(muf, j, i) -
+                                           cq2 * AT2(mubf, j, i));
+  }
+  float cq1 = 1.0f / (1.0f + cqo);
+  float cq2 = cqo * cq1;
+  cqw[q] = cq1;
+  rw_tend[q] = rw_tend[q] + rmsf * g_ * (cq1 * rdn[k] * (p[q] - AT(p, k - 1, j, i)) - AT2(muf, j, i) -
+                                         cq2 * AT2(mubf, j, i));
+}
+
+extern "C" int pywrf_pg_buoy_w(const pywrf_grid_t* g, float* rw_tend, const float* p, float* cqw, const float* muf,
+                               const float* mubf, const float* rdnw, const float* rdn, double g_,
+                               const float* msfty, pywrf_stream_t s) {
+  CHECK(g, 0);
+  if (g->kde > g->nz || g->kde < 3) return PYWRF_E_RANGE;
+  Box b = mkbox(1, g->kde - 1, g->jts, imin(g->jte, g->jde - 1), g->its, imin(g->ite, g->ide - 1));
+  PW_LAUNCH_BOX(k_pg_buoy_w, b, s, dims_of(g), rw_tend, p, cqw, muf, mubf, rdnw, rdn, F(g_), msfty, g->kde);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ coriolis (A29)
+struct Cor {
+  const float *ru, *rv, *rw, *msftx, *msfty, *msfux, *msfuy, *msfvx, *msfvy, *f, *e, *sina, *cosa, *fzm, *fzp;
+  float *ru_tend, *rv_tend, *rw_tend;
+};
+
+__global__ void k_coriolis(Box b, Dims D, Cor a, Box bu, Box bv, Box bw) {
+  int k, j, i;
+  if (!box_point(b, k, j, i)) return;
+  long q = D.ix(k, j, i);
+  if (bu.has(k, j, i)) {
+    float x = (AT2(a.msfux, j, i) / AT2(a.msfuy, j, i)) * 0.5f * (AT2(a.f, j, i) + AT2(a.f, j, i - 1)) * 0.25f *
+              (AT(a.rv, k, j + 1, i - 1) + AT(a.rv, k, j + 1, i) + AT(a.rv, k, j, i - 1) + AT(a.rv, k, j, i));
+    float y = 0.5f * (AT2(a.e, j, i) + AT2(a.e, j, i - 1)) * 0.5f * (AT2(a.cosa, j, i) + AT2(a.cosa, j, i - 1)) *
+              0.25f * (AT(a.rw, k + 1, j, i - 1) + AT(a.rw, k, j, i - 1) + AT(a.rw, k + 1, j, i) + AT(a.rw, k, j, i));
+    a.ru_tend[q] = a.ru_tend[q] + x - y;
+  }
+  if (bv.has(k, j, i)) {
+    float r = AT2(a.msfvy, j, i) / AT2(a.msfvx, j, i);
+    float x = r * 0.5f * (AT2(a.f, j, i) + AT2(a.f, j - 1, i)) * 0.25f *
+              (AT(a.ru, k, j, i) + AT(a.ru, k, j, i + 1) + AT(a.ru, k, j - 1, i) + AT(a.ru, k, j - 1, i + 1));
+    float y = r * 0.5f * (AT2(a.e, j, i) + AT2(a.e, j - 1, i)) * 0.5f * (AT2(a.sina, j, i) + AT2(a.sina, j - 1, i)) *
+              0.25f * (AT(a.rw, k + 1, j - 1, i) + AT(a.rw, k, j - 1, i) + AT(a.rw, k + 1, j, i) + AT(a.rw, k, j, i));
+    a.rv_tend[q] = a.rv_tend[q] - x + y;
+  }
+  if (bw.has(k, j, i)) {
+    float fzm = a.fzm[k], fzp = a.fzp[k];
+    float cu = AT2(a.cosa, j, i) * 0.5f *
+               (fzm * (AT(a.ru, k, j, i) + AT(a.ru, k, j, i + 1)) + fzp * (AT(a.ru, k - 1, j, i) + AT(a.ru, k - 1, j, i + 1)));
+    float sv = (AT2(a.msftx, j, i) / AT2(a.msfty, j, i)) * AT2(a.sina, j, i) * 0.5f *
+               (fzm * (AT(a.rv, k, j, i) + AT(a.rv, k, j + 1, i)) + fzp * (AT(a.rv, k - 1, j, i) + AT(a.rv, k - 1, j + 1, i)));
+    a.rw_tend[q] = a.rw_tend[q] + AT2(a.e, j, i) * (cu - sv);
+  }
+}
+
+extern "C" int pywrf_coriolis(const pywrf_grid_t* g, const float* ru, const float* rv, const float* rw,
+                              float* ru_tend, float* rv_tend, float* rw_tend, const float* msftx, const float* msfty,
+                              const float* msfux, const float* msfuy, const float* msfvx, const float* msfvy,
+                              const float* f, const float* e, const float* sina, const float* cosa,
+                              const float* fzm, const float* fzp, pywrf_stream_t s) {
+  CHECK(g, 1);
+  int ktf = imin(g->kte, g->kde - 1);
+  if (ktf + 1 > g->nz) return PYWRF_E_RANGE;
+  Cor a = {ru, rv, rw, msftx, msfty, msfux, msfuy, msfvx, msfvy, f, e, sina, cosa, fzm, fzp, ru_tend, rv_tend,
+           rw_tend};
+  Box bu = mkbox(g->kts, ktf, g->jts, g->jte - 1, imax(g->ids + 1, g->its), imin(g->ide - 1, g->ite));
+  Box bv = mkbox(g->kts, ktf, imax(g->jds + 1, g->jts), imin(g->jde - 1, g->jte), g->its, g->ite - 1);
+  Box bw = mkbox(g->kts + 1, ktf, g->jts, g->jte - 1, g->its, g->ite - 1);
+  Box b = mkbox(g->kts, ktf, g->jts, g->jte, g->its, g->ite);
+  PW_LAUNCH_BOX(k_coriolis, b, s, dims_of(g), a, bu, bv, bw);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ curvature (A30)
+struct Curv {
+  const float *ru, *rv, *rw, *u, *v, *msfuy, *msfvx, *msfvy, *msftx, *msfty, *fzm, *fzp;
+  float *ru_tend, *rv_tend, *rw_tend;
+  float rdx, rdy;
+  int jts, jte, its, ite;
+};
+
+// vxgm on [jts-1, jte) x [its-1, ite): the interior value on [jts, jte-1) x
+// [its, ite-1), copied outward to the edge ring (columns first, then rows);
+// zero elsewhere.
+__device__ inline float curv_vxgm(const Dims& D, const Curv& a, int k, int j, int i) {
+  if (j < a.jts - 1 || j >= a.jte || i < a.its - 1 || i >= a.ite) return 0.0f;
+  j = j < a.jts ? a.jts : (j > a.jte - 2 ? a.jte - 2 : j);
+  i = i < a.its ? a.its : (i > a.ite - 2 ? a.ite - 2 : i);
+  return 0.5f * (AT(a.u, k, j, i) + AT(a.u, k, j, i + 1)) * (AT2(a.msfvx, j + 1, i) - AT2(a.msfvx, j, i)) * a.rdy -
+         0.5f * (AT(a.v, k, j, i) + AT(a.v, k, j + 1, i)) * (AT2(a.msfuy, j, i + 1) - AT2(a.msfuy, j, i)) * a.rdx;
+}
+
+__global__ void k_curvature(Box b, Dims D, Curv a, Box bu, Box bv, Box bw) {
+  int k, j, i;
+  if (!box_point(b, k, j, i)) return;
+  long q = D.ix(k, j, i);
+  const float rr = F(pw::reradius);
+  if (bu.has(k, j, i)) {
+    float x = 0.5f * (curv_vxgm(D, a, k, j, i) + curv_vxgm(D, a, k, j, i - 1)) * 0.25f *
+              (AT(a.rv, k, j + 1, i - 1) + AT(a.rv, k, j + 1, i) + AT(a.rv, k, j, i - 1) + AT(a.rv, k, j, i));
+    float y = AT(a.u, k, j, i) * rr * 0.25f *
+              (AT(a.rw, k + 1, j, i - 1) + AT(a.rw, k, j, i - 1) + AT(a.rw, k + 1, j, i) + AT(a.rw, k, j, i));
+    a.ru_tend[q] = a.ru_tend[q] + x - y;
+  }
+  if (bv.has(k, j, i)) {
+    float x = 0.5f * (curv_vxgm(D, a, k, j, i) + curv_vxgm(D, a, k, j - 1, i)) * 0.25f *
+              (AT(a.ru, k, j, i) + AT(a.ru, k, j, i + 1) + AT(a.ru, k, j - 1, i) + AT(a.ru, k, j - 1, i + 1));
+    float y = (AT2(a.msfvy, j, i) / AT2(a.msfvx, j, i)) * AT(a.v, k, j, i) * rr * 0.25f *
+              (AT(a.rw, k + 1, j - 1, i) + AT(a.rw, k, j - 1, i) + AT(a.rw, k + 1, j, i) + AT(a.rw, k, j, i));
+    a.rv_tend[q] = a.rv_tend[q] - x - y;
+  }
+  if (bw.has(k, j, i)) {
+    float fzm = a.fzm[k], fzp = a.fzp[k];
+    auto fz = [&](const float* f, int dj, int di) {
+      return fzm * (AT(f, k, j, i) + AT(f, k, j + dj, i + di)) + fzp * (AT(f, k - 1, j, i) + AT(f, k - 1, j + dj, i + di));
+    };
+    a.rw_tend[q] = a.rw_tend[q] + rr * (0.5f * fz(a.ru, 0, 1) * 0.5f * fz(a.u, 0, 1) +
+                                        (AT2(a.msftx, j, i) / AT2(a.msfty, j, i)) * 0.5f * fz(a.rv, 1, 0) * 0.5f *
+                                            fz(a.v, 1, 0));
+  }
+}
+
+extern "C" int pywrf_curvature(const pywrf_grid_t* g, const float* ru, const float* rv, const float* rw,
+                               const float* u, const float* v, float* ru_tend, float* rv_tend, float* rw_tend,
+                               const float* msfuy, const float* msfvx, const float* msfvy, const float* msftx,
+                               const float* msfty, const float* fzm, const float* fzp, double rdx, double rdy,
+                               pywrf_stream_t s) {
+  CHECK(g, 2);
+  int ktf = imin(g->kte, g->kde - 1);
+  if (ktf + 1 > g->nz) return PYWRF_E_RANGE;
+  Curv a = {ru, rv, rw, u, v, msfuy, msfvx, msfvy, msftx, msfty, fzm, fzp, ru_tend, rv_tend, rw_tend,
+            F(rdx), F(rdy), g->jts, g->jte, g->its, g->ite};
+  Box bu = mkbox(g->kts, ktf, g->jts, g->jte - 1, imax(g->ids + 1, g->its), imin(g->ide - 1, g->ite));
+  Box bv = mkbox(g->kts, ktf, imax(g->jds + 1, g->jts), imin(g->jde - 1, g->jte), g->its, g->ite - 1);
+  Box bw = mkbox(g->kts + 1, ktf, g->jts, g->jte - 1, g->its, g->ite - 1);
+  Box b = mkbox(g->kts, ktf, g->jts, g->jte, g->its, g->ite);
+  PW_LAUNCH_BOX(k_curvature, b, s, dims_of(g), a, bu, bv, bw);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ horizontal_diffusion (A16, A17)
+struct HDiff {
+  const float *f, *base, *mut, *xk, *msfux, *msfuy, *msfvx, *msfvx_inv, *msfvy, *msftx, *msfty;
+  float* tend;
+  float rdx, rdy;
+  char name;
+};
+
+struct HCoef {
+  float mkxm, mkxp, mrdx, mkym, mkyp, mrdy;
+};
+
+__device__ inline HCoef hdiff_coefs(const Dims& D, const HDiff& a, int k, int j, int i) {
+  HCoef c;
+  auto M = [&](const float* f, int dj, int di) { return AT2(f, j + dj, i + di); };
+  auto X = [&](int dk, int dj, int di) { return AT(a.xk, k + dk, j + dj, i + di); };
+  const float rdx = a.rdx, rdy = a.rdy;
+  if (a.name == 'u') {
+    c.mkxm = (M(a.msftx, 0, -1) / M(a.msfty, 0, -1)) * M(a.mut, 0, -1) * X(0, 0, -1) * rdx;
+    c.mkxp = (M(a.msftx, 0, 0) / M(a.msfty, 0, 0)) * M(a.mut, 0, 0) * X(0, 0, 0) * rdx;
+    c.mrdx = M(a.msfux, 0, 0) * M(a.msfuy, 0, 0) * rdx;
+    c.mkym = ((M(a.msfuy, 0, 0) + M(a.msfuy, -1, 0)) / (M(a.msfux, 0, 0) + M(a.msfux, -1, 0))) * 0.25f *
+             (M(a.mut, 0, 0) + M(a.mut, -1, 0) + M(a.mut, -1, -1) + M(a.mut, 0, -1)) * 0.25f *
+             (X(0, 0, 0) + X(0, -1, 0) + X(0, -1, -1) + X(0, 0, -1)) * rdy;
+    c.mkyp = ((M(a.msfuy, 0, 0) + M(a.msfuy, 1, 0)) / (M(a.msfux, 0, 0) + M(a.msfux, 1, 0))) * 0.25f *
+             (M(a.mut, 0, 0) + M(a.mut, 1, 0) + M(a.mut, 1, -1) + M(a.mut, 0, -1)) * 0.25f *
+             (X(0, 0, 0) + X(0, 1, 0) + X(0, 1, -1) + X(0, 0, -1)) * rdy;
+    c.mrdy = M(a.msfux, 0, 0) * M(a.msfuy, 0, 0) * rdy;
+  } else if (a.name == 'v') {
+    c.mkxm = ((M(a.msfvx, 0, 0) + M(a.msfvx, 0, -1)) / (M(a.msfvy, 0, 0) + M(a.msfvy, 0, -1))) * 0.25f *
+             (M(a.mut, 0, 0) + M(a.mut, -1, 0) + M(a.mut, -1, -1) + M(a.mut, 0, -1)) * 0.25f *
+             (X(0, 0, 0) + X(0, -1, 0) + X(0, -1, -1) + X(0, 0, -1)) * rdx;
+    c.mkxp = ((M(a.msfvx, 0, 0) + M(a.msfvx, 0, 1)) / (M(a.msfvy, 0, 0) + M(a.msfvy, 0, 1))) * 0.25f *
+             (M(a.mut, 0, 0) + M(a.mut, -1, 0) + M(a.mut, -1, 1) + M(a.mut, 0, 1)) * 0.25f *
+             (X(0, 0, 0) + X(0, -1, 0) + X(0, -1, 1) + X(0, 0, 1)) * rdx;
+    c.mrdx = M(a.msfvx, 0, 0) * M(a.msfvy, 0, 0) * rdx;
+    c.mkym = (M(a.msfty, -1, 0) / M(a.msftx, -1, 0)) * X(0, -1, 0) * rdy;
+    c.mkyp = (M(a.msfty, 0, 0) / M(a.msftx, 0, 0)) * X(0, 0, 0) * rdy;
+    c.mrdy = M(a.msfvx, 0, 0) * M(a.msfvy, 0, 0) * rdy;
+  } else if (a.name == 'w') {
+    c.mkxm = (M(a.msfux, 0, 0) / M(a.msfuy, 0, 0)) * 0.25f *
+             (M(a.mut, 0, 0) + M(a.mut, 0, -1) + M(a.mut, 0, 0) + M(a.mut, 0, -1)) * 0.25f *
+             (X(0, 0, 0) + X(0, 0, -1) + X(-1, 0, 0) + X(-1, 0, -1)) * rdx;
+    c.mkxp = (M(a.msfux, 0, 1) / M(a.msfuy, 0, 1)) * 0.25f *
+             (M(a.mut, 0, 1) + M(a.mut, 0, 0) + M(a.mut, 0, 1) + M(a.mut, 0, 0)) * 0.25f *
+             (X(0, 0, 1) + X(0, 0, 0) + X(-1, 0, 1) + X(-1, 0, 0)) * rdx;
+    c.mrdx = M(a.msftx, 0, 0) * M(a.msfty, 0, 0) * rdx;
+    c.mkym = (M(a.msfvy, 0, 0) / M(a.msfvx, 0, 0)) * 0.25f *
+             (M(a.mut, 0, 0) + M(a.mut, -1, 0) + M(a.mut, 0, 0) + M(a.mut, -1, 0)) * 0.25f *
+             (X(0, 0, 0) + X(0, -1, 0) + X(-1, 0, 0) + X(-1, -1, 0)) * rdy;
+    c.mkyp = (M(a.msfvy, 1, 0) / M(a.msfvx, 1, 0)) * 0.25f *
+             (M(a.mut, 1, 0) + M(a.mut, 0, 0) + M(a.mut, 1, 0) + M(a.mut, 0, 0)) * 0.25f *
+             (X(0, 1, 0) + X(0, 0, 0) + X(-1, 1, 0) + X(-1, 0, 0)) * rdy;
+    c.mrdy = M(a.msftx, 0, 0) * M(a.msfty, 0, 0) * rdy;
+  } else {
+    c.mkxm = (M(a.msfux, 0, 0) / M(a.msfuy, 0, 0)) * 0.5f * (X(0, 0, 0) + X(0, 0, -1)) * 0.5f *
+             (M(a.mut, 0, 0) + M(a.mut, 0, -1)) * rdx;
+    c.mkxp = (M(a.msfux, 0, 1) / M(a.msfuy, 0, 1)) * 0.5f * (X(0, 0, 1) + X(0, 0, 0)) * 0.5f *
+             (M(a.mut, 0, 1) + M(a.mut, 0, 0)) * rdx;
+    c.mrdx = M(a.msftx, 0, 0) * M(a.msfty, 0, 0) * rdx;
+    c.mkym = (M(a.msfvy, 0, 0) * M(a.msfvx_inv, 0, 0)) * 0.5f * (X(0, 0, 0) + X(0, -1, 0)) * 0.5f *
+             (M(a.mut, 0, 0) + M(a.mut, -1, 0)) * rdy;
+    c.mkyp = (M(a.msfvy, 1, 0) * M(a.msfvx_inv, 1, 0)) * 0.5f * (X(0, 1, 0) + X(0, 0, 0)) * 0.5f *
+             (M(a.mut, 1, 0) + M(a.mut, 0, 0)) * rdy;
+    c.mrdy = M(a.msftx, 0, 0) * M(a.msfty, 0, 0) * rdy;
+  }
+  return c;
+}
+
+__global__ void k_hdiff(Box b, Dims D, HDiff a) {
+  int k, j, i;
+  if (!box_point(b, k, j, i)) return;
+  long q = D.ix(k, j, i);
+  HCoef c = hdiff_coefs(D, a, k, j, i);
+  auto Fv = [&](int dj, int di) { return AT(a.f, k, j + dj, i + di); };
+  if (!a.base) {
+    a.tend[q] = a.tend[q] + (c.mrdx * (c.mkxp * (Fv(0, 1) - Fv(0, 0)) - c.mkxm * (Fv(0, 0) - Fv(0, -1))) +
+                             c.mrdy * (c.mkyp * (Fv(1, 0) - Fv(0, 0)) - c.mkym * (Fv(0, 0) - Fv(-1, 0))));
+  } else {
+    auto S = [&](int dj, int di) { return AT(a.base, k, j + dj, i + di); };
+    a.tend[q] = a.tend[q] +
+                (c.mrdx * (c.mkxp * (Fv(0, 1) - Fv(0, 0) - S(0, 1) + S(0, 0)) -
+                           c.mkxm * (Fv(0, 0) - Fv(0, -1) - S(0, 0) + S(0, -1))) +
+                 c.mrdy * (c.mkyp * (Fv(1, 0) - Fv(0, 0) - S(1, 0) + S(0, 0)) -
+                           c.mkym * (Fv(0, 0) - Fv(-1, 0) - S(0, 0) + S(-1, 0))));
+  }
+}
+
+int pw_horizontal_diffusion(const pywrf_grid_t* g, char name, const float* field, float* tendency,
+                            const float* base, const float* mut, const float* msfux, const float* msfuy,
+                            const float* msfvx, const float* msfvx_inv, const float* msfvy, const float* msftx,
+                            const float* msfty, const float* xkmhd, float rdx, float rdy, pywrf_stream_t s) {
+  CHECK(g, 2);
+  int ktf = imin(g->kte, g->kde - 1);
+  HDiff a = {field, base, mut, xkmhd, msfux, msfuy, msfvx, msfvx_inv, msfvy, msftx, msfty, tendency, rdx, rdy, name};
+  int j0 = imax(g->jds + 1, g->jts), i0 = imax(g->ids + 1, g->its);
+  Box b;
+  if (name == 'u') b = mkbox(g->kts, ktf, j0, imin(g->jde - 2, g->jte), i0, imin(g->ide - 1, g->ite));
+  else if (name == 'v') b = mkbox(g->kts, ktf, j0, imin(g->jde - 1, g->jte), i0, imin(g->ide - 2, g->ite));
+  else if (name == 'w') b = mkbox(g->kts + 1, ktf, j0, imin(g->jde - 2, g->jte), i0, imin(g->ide - 2, g->ite));
+  else b = mkbox(g->kts, ktf, j0, imin(g->jde - 2, g->jte), i0, imin(g->ide - 2, g->ite));
+  PW_LAUNCH_BOX(k_hdiff, b, s, dims_of(g), a);
+  PW_RET();
+}
+
+extern "C" int pywrf_horizontal_diffusion(const pywrf_grid_t* g, char name, const float* field, float* tendency,
+                                          const float* mut, const float* msfux, const float* msfuy,
+                                          const float* msfvx, const float* msfvx_inv, const float* msfvy,
+                                          const float* msftx, const float* msfty, const float* xkmhd, double rdx,
+                                          double rdy, pywrf_stream_t s) {
+  return pw_horizontal_diffusion(g, name, field, tendency, nullptr, mut, msfux, msfuy, msfvx, msfvx_inv, msfvy,
+                                 msftx, msfty, xkmhd, F(rdx), F(rdy), s);
+}
+
+extern "C" int pywrf_horizontal_diffusion_3dmp(const pywrf_grid_t* g, const float* field, float* tendency,
+                                               const float* mut, const float* base_3d, const float* msfux,
+                                               const float* msfuy, const float* msfvx, const float* msfvx_inv,
+                                               const float* msfvy, const float* msftx, const float* msfty,
+                                               const float* xkmhd, double rdx, double rdy, pywrf_stream_t s) {
+  // the 3dmp form always uses the scalar coefficients
+  return pw_horizontal_diffusion(g, 'm', field, tendency, base_3d, mut, msfux, msfuy, msfvx, msfvx_inv, msfvy,
+                                 msftx, msfty, xkmhd, F(rdx), F(rdy), s);
+}
+
+// ------------------------------------------------------------------ vertical diffusion (A18-A20)
+// kind: 'w' (vertical_diffusion 'w'), 'm' (vertical_diffusion 'm'), 'u'/'v'
+// (vertical_diffusion_u/_v, 1-D base), 'p' (vertical_diffusion_mp, 1-D base),
+// '3' (vertical_diffusion_3dmp, 3-D base).
+struct VDiff {
+  const float *f, *alt, *mus, *rdn, *rdnw, *base1, *base3;
+  float* tend;
+  float kvdif;
+  int ktf, kts;
+  char kind;
+};
+
+__device__ inline float vdiff_flux(const Dims& D, const VDiff& a, int k, int j, int i) {
+  auto A = [&](int dk, int dj, int di) { return AT(a.alt, k + dk, j + dj, i + di); };
+  auto Fv = [&](int dk) { return AT(a.f, k + dk, j, i); };
+  if (k < a.kts || k >= a.ktf - 1) return 0.0f;
+  switch (a.kind) {
+    case 'w':
+      return (a.kvdif / A(0, 0, 0)) * a.rdnw[k] * (Fv(1) - Fv(0));
+    case 'm':
+      if (k == a.kts) k = a.kts + 1;  // vflux[kts] = vflux[kts+1]
+      if (k >= a.ktf - 1) return 0.0f;
+      return a.kvdif * a.rdn[k + 1] / (0.5f * (AT(a.alt, k, j, i) + AT(a.alt, k + 1, j, i))) *
+             (AT(a.f, k + 1, j, i) - AT(a.f, k, j, i));
+    case 'u':
+    case 'v': {
+      int dj = a.kind == 'v' ? -1 : 0, di = a.kind == 'u' ? -1 : 0;
+      return a.kvdif * a.rdn[k + 1] / (0.25f * (A(0, 0, 0) + A(0, dj, di) + A(1, 0, 0) + A(1, dj, di))) *
+             (Fv(1) - Fv(0) - a.base1[k + 1] + a.base1[k]);
+    }
+    case 'p':
+      return a.kvdif * a.rdn[k + 1] / (0.5f * (A(0, 0, 0) + A(1, 0, 0))) *
+             (Fv(1) - Fv(0) - a.base1[k + 1] + a.base1[k]);
+    default:
+      return a.kvdif * a.rdn[k + 1] / (0.5f * (A(0, 0, 0) + A(1, 0, 0))) *
+             (Fv(1) - Fv(0) - AT(a.base3, k + 1, j, i) + AT(a.base3, k, j, i));
+  }
+}
+
+__global__ void k_vdiff(Box b, Dims D, VDiff a) {
+  int k, j, i;
+  if (!box_point(b, k, j, i)) return;
+  long q = D.ix(k, j, i);
+  float dv = vdiff_flux(D, a, k, j, i) - vdiff_flux(D, a, k - 1, j, i);
+  const float gg = F(pw::g * pw::g), G = F(pw::g);
+  float m = AT2(a.mus, j, i);
+  switch (a.kind) {
+    case 'w':
+      a.tend[q] = a.tend[q] + a.rdn[k] * G * G / m / (0.5f * (a.alt[q] + AT(a.alt, k - 1, j, i))) * dv;
+      break;
+    case 'm':
+      a.tend[q] = a.tend[q] + gg / m / a.alt[q] * a.rdnw[k] * dv;
+      break;
+    case 'u':
+      a.tend[q] = a.tend[q] + gg * a.rdnw[k] / m / (0.5f * (AT(a.alt, k, j, i - 1) + a.alt[q])) * dv;
+      break;
+    case 'v':
+      a.tend[q] = a.tend[q] + gg * a.rdnw[k] / m / (0.5f * (AT(a.alt, k, j - 1, i) + a.alt[q])) * dv;
+      break;
+    default:
+      a.tend[q] = a.tend[q] + gg * m / a.alt[q] * a.rdnw[k] * dv;
+  }
+}
+
+static int vdiff_launch(const pywrf_grid_t* g, char kind, const float* field, float* tendency, const float* base1,
+                        const float* base3, const float* alt, const float* mus, const float* rdn,
+                        const float* rdnw, double kvdif, pywrf_stream_t s) {
+  CHECK(g, 1);
+  int ktf = imin(g->kte, g->kde - 1);
+  if (ktf + 1 > g->nz) return PYWRF_E_RANGE;
+  VDiff a = {field, alt, mus, rdn, rdnw, base1, base3, tendency, F(kvdif), ktf, g->kts, kind};
+  int jtf = imin(g->jte, g->jde - 1), itf = imin(g->ite, g->ide - 1);
+  Box b;
+  if (kind == 'u') b = mkbox(g->kts + 1, ktf - 1, g->jts, jtf, imax(g->ids + 1, g->its), imin(g->ide - 1, g->ite));
+  else if (kind == 'v') b = mkbox(g->kts + 1, ktf - 1, g->jts, jtf, g->its, itf);
+  else b = mkbox(g->kts + 1, ktf, g->jts, jtf, g->its, itf);
+  PW_LAUNCH_BOX(k_vdiff, b, s, dims_of(g), a);
+  PW_RET();
+}
+
+extern "C" int pywrf_vertical_diffusion(const pywrf_grid_t* g, char name, const float* field, float* tendency,
+                                        const float* alt, const float* mut, const float* rdn, const float* rdnw,
+                                        double kvdif, pywrf_stream_t s) {
+  if (name != 'w' && name != 'm') return 0;  // the reference leaves other names untouched
+  return vdiff_launch(g, name, field, tendency, nullptr, nullptr, alt, mut, rdn, rdnw, kvdif, s);
+}
+
+extern "C" int pywrf_vertical_diffusion_u(const pywrf_grid_t* g, const float* field, float* tendency,
+                                          const float* u_base, const float* alt, const float* muu, const float* rdn,
+                                          const float* rdnw, double kvdif, pywrf_stream_t s) {
+  return vdiff_launch(g, 'u', field, tendency, u_base, nullptr, alt, muu, rdn, rdnw, kvdif, s);
+}
+
+extern "C" int pywrf_vertical_diffusion_v(const pywrf_grid_t* g, const float* field, float* tendency,
+                                          const float* v_base, const float* alt, const float* muv, const float* rdn,
+                                          const float* rdnw, double kvdif, pywrf_stream_t s) {
+  return vdiff_launch(g, 'v', field, tendency, v_base, nullptr, alt, muv, rdn, rdnw, kvdif, s);
+}
+
+extern "C" int pywrf_vertical_diffusion_mp(const pywrf_grid_t* g, const float* field, float* tendency,
+                                           const float* base, const float* alt, const float* mut, const float* rdn,
+                                           const float* rdnw, double kvdif, pywrf_stream_t s) {
+  return vdiff_launch(g, 'p', field, tendency, base, nullptr, alt, mut, rdn, rdnw, kvdif, s);
+}
+
+extern "C" int pywrf_vertical_diffusion_3dmp(const pywrf_grid_t* g, const float* field, float* tendency,
+                                             const float* base_3d, const float* alt, const float* mut,
+                                             const float* rdn, const float* rdnw, double kvdif, pywrf_stream_t s) {
+  return vdiff_launch(g, '3', field, tendency, nullptr, base_3d, alt, mut, rdn, rdnw, kvdif, s);
+}
+
+// ------------------------------------------------------------------ rk_addtend_dry (A33)
+struct AddT {
+  float *ru_tend, *rv_tend, *rw_tend, *ph_tend, *t_tend, *ru_tendf, *rv_tendf, *rw_tendf, *ph_tendf, *t_tendf;
+  const float *u_save, *v_save, *w_save, *ph_save, *t_save, *h_diabatic, *mut, *msfty, *msfuy, *msfvx,
+      *msfvx_inv;
+  float *mu_tend;
+  const float* mu_tendf;
+  int rk1;
+};
+
+__global__ void k_addtend(Box b, Dims D, AddT a, Box bu, Box bv, Box bw, Box bt) {
+  int k, j, i;
+  if (!box_point(b, k, j, i)) return;
+  long q = D.ix(k, j, i);
+  if (bu.has(k, j, i)) {
+    float mu = AT2(a.msfuy, j, i);
+    float tf = a.ru_tendf[q];
+    if (a.rk1) a.ru_tendf[q] = tf = tf + a.u_save[q] * mu;
+    a.ru_tend[q] = a.ru_tend[q] + tf / mu;
+  }
+  if (bv.has(k, j, i)) {
+    float tf = a.rv_tendf[q];
+    if (a.rk1) a.rv_tendf[q] = tf = tf + a.v_save[q] * AT2(a.msfvx, j, i);
+    a.rv_tend[q] = a.rv_tend[q] + tf * AT2(a.msfvx_inv, j, i);
+  }
+  if (bw.has(k, j, i)) {
+    float my = AT2(a.msfty, j, i);
+    float tf = a.rw_tendf[q];
+    if (a.rk1) a.rw_tendf[q] = tf = tf + a.w_save[q] * my;
+    a.rw_tend[q] = a.rw_tend[q] + tf / my;
+    tf = a.ph_tendf[q];
+    if (a.rk1) a.ph_tendf[q] = tf = tf + a.ph_save[q];
+    a.ph_tend[q] = a.ph_tend[q] + tf / my;
+  }
+  if (bt.has(k, j, i)) {
+    float my = AT2(a.msfty, j, i);
+    float tf = a.t_tendf[q];
+    if (a.rk1) a.t_tendf[q] = tf = tf + a.t_save[q];
+    a.t_tend[q] = a.t_tend[q] + tf / my + AT2(a.mut, j, i) * a.h_diabatic[q] / my;
+  }
+  if (k == bt.k0 && bt.has(k, j, i)) AT2(a.mu_tend, j, i) = AT2(a.mu_tend, j, i) + AT2(a.mu_tendf, j, i);
+}
+
+extern "C" int pywrf_rk_addtend_dry(const pywrf_grid_t* g, float* ru_tend, float* rv_tend, float* rw_tend,
+                                    float* ph_tend, float* t_tend, float* ru_tendf, float* rv_tendf,
+                                    float* rw_tendf, float* ph_tendf, float* t_tendf, const float* u_save,
+                                    const float* v_save, const float* w_save, const float* ph_save,
+                                    const float* t_save, float* mu_tend, const float* mu_tendf, int rk_step,
+                                    const float* h_diabatic, const float* mut, const float* msfty,
+                                    const float* msfuy, const float* msfvx, const float* msfvx_inv,
+                                    pywrf_stream_t s) {
+  CHECK(g, 0);
+  if (g->kte > g->nz || g->kte < 2) return PYWRF_E_RANGE;
+  AddT a = {ru_tend, rv_tend, rw_tend, ph_tend, t_tend, ru_tendf, rv_tendf, rw_tendf, ph_tendf, t_tendf,
+            u_save, v_save, w_save, ph_save, t_save, h_diabatic, mut, msfty, msfuy, msfvx, msfvx_inv,
+            mu_tend, mu_tendf, rk_step == 1};
+  Box bu = mkbox(g->kts, g->kte - 1, g->jts, g->jde - 1, g->its, g->ite);
+  Box bv = mkbox(g->kts, g->kte - 1, g->jts, g->jte, g->its, g->ide - 1);
+  Box bw = mkbox(g->kts, g->kte, g->jts, g->jde - 1, g->its, g->ide - 1);
+  Box bt = mkbox(g->kts, g->kte - 1, g->jts, g->jde - 1, g->its, g->ide - 1);
+  Box b = mkbox(g->kts, g->kte, g->jts, imax(g->jde - 1, g->jte), g->its, imax(g->ite, g->ide - 1));
+  PW_LAUNCH_BOX(k_addtend, b, s, dims_of(g), a, bu, bv, bw, bt);
+  PW_RET();
+}

@@ -49,8 +49,7 @@ def allocate_state(nz, ny, nx, device="cpu", dtype=torch.float32):
         setattr(s, n, z(ny, nx))
     for n in FIELDS1:
         setattr(s, n, z(nz))
-    for n in FIELDS0:
-        setattr(s, n, z())
+    _pack_scalars(s, z(len(FIELDS0)))
     for n, c in FIELDS4.items():
         setattr(s, n, z(c, nz, ny, nx))
     s.bdy = {}
@@ -84,7 +83,16 @@ def clone_state(s, device=None, dtype=None):
             setattr(c, n, conv(v))
     c.bdy = {k: tuple(conv(a) for a in arrs) for k, arrs in s.bdy.items()}
     c.grid = s.grid
+    _pack_scalars(c, torch.stack([getattr(c, n) for n in FIELDS0]))
     return c
+
+
+def _pack_scalars(s, pack):
+    """Hold cf1, cf2, cf3, cfn, cfn1 as views of one contiguous vector, so that
+    kernels can read (cf1, cf2, cf3) as three consecutive floats."""
+    s.scalars0 = pack
+    for n, name in enumerate(FIELDS0):
+        setattr(s, name, pack[n])
 
 
 PROGNOSTIC = ("u", "v", "w", "t", "ph", "p", "al", "mu", "moist")

@@ -1,0 +1,183 @@
+"""Drop-in for pywrf/wrf_boundary.py on the ARW step's path (same names,
+positional parameters and returns), forwarding to libpywrf_amd.so.
+
+Boundary arrays are contiguous: x-boundaries (kz, ny, 5), y-boundaries
+(kz, 5, nx) -- the shapes of the reference driver's permuted views
+(solver.py:468-703).  Fields are updated in place, as in the reference.
+"""
+from ._lib import call, grid, ptr, flag, stream
+
+
+def _I24(ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, ips, ipe, jps, jpe, kps, kpe,
+         its, ite, jts, jte, kts, kte):
+    return (ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, ips, ipe, jps, jpe, kps, kpe,
+            its, ite, jts, jte, kts, kte)
+
+
+def relax_bdytend(field, field_tend,
+                  field_bdy_xs, field_bdy_xe,
+                  field_bdy_ys, field_bdy_ye,
+                  field_bdy_tend_xs, field_bdy_tend_xe,
+                  field_bdy_tend_ys, field_bdy_tend_ye,
+                  variable_in,
+                  spec_bdy_width, spec_zone, relax_zone,
+                  dtbc, fcx, gcx,
+                  ids, ide, jds, jde, kds, kde,
+                  ims, ime, jms, jme, kms, kme,
+                  ips, ipe, jps, jpe, kps, kpe,
+                  its, ite, jts, jte, kts, kte):
+    """wrf_boundary.py:52-161"""
+    g = grid(field.shape, _I24(ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, ips, ipe, jps, jpe, kps,
+                               kpe, its, ite, jts, jte, kts, kte))
+    call("pywrf_relax_bdytend", g, ptr(field), ptr(field_tend), ptr(field_bdy_xs), ptr(field_bdy_xe),
+         ptr(field_bdy_ys), ptr(field_bdy_ye), ptr(field_bdy_tend_xs), ptr(field_bdy_tend_xe),
+         ptr(field_bdy_tend_ys), ptr(field_bdy_tend_ye), flag(variable_in), float(dtbc), ptr(fcx), ptr(gcx),
+         stream())
+    return field_tend
+
+
+def spec_bdytend(field_tend,
+                 field_bdy_xs, field_bdy_xe,
+                 field_bdy_ys, field_bdy_ye,
+                 field_bdy_tend_xs, field_bdy_tend_xe,
+                 field_bdy_tend_ys, field_bdy_tend_ye,
+                 variable_in,
+                 spec_bdy_width, spec_zone,
+                 ids, ide, jds, jde, kds, kde,
+                 ims, ime, jms, jme, kms, kme,
+                 ips, ipe, jps, jpe, kps, kpe,
+                 its, ite, jts, jte, kts, kte):
+    """wrf_boundary.py:188-225"""
+    g = grid(field_tend.shape, _I24(ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, ips, ipe, jps, jpe,
+                                    kps, kpe, its, ite, jts, jte, kts, kte))
+    call("pywrf_spec_bdytend", g, ptr(field_tend), ptr(field_bdy_tend_xs), ptr(field_bdy_tend_xe),
+         ptr(field_bdy_tend_ys), ptr(field_bdy_tend_ye), flag(variable_in), stream())
+    return field_tend
+
+
+def spec_bdyupdate(field,
+                   field_tend, dt,
+                   variable,
+                   spec_zone,
+                   ids, ide, jds, jde, kds, kde,
+                   ims, ime, jms, jme, kms, kme,
+                   ips, ipe, jps, jpe, kps, kpe,
+                   its, ite, jts, jte, kts, kte):
+    """wrf_boundary.py:228-284"""
+    g = grid(field.shape, _I24(ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, ips, ipe, jps, jpe, kps,
+                               kpe, its, ite, jts, jte, kts, kte))
+    call("pywrf_spec_bdyupdate", g, ptr(field), ptr(field_tend), float(dt), flag(variable), stream())
+    return field
+
+
+def spec_bdyupdate_ph(ph_save, field,
+                      field_tend, mu_tend, muts,
+                      c1, c2, dt,
+                      variable_in,
+                      spec_zone,
+                      ids, ide, jds, jde, kds, kde,
+                      ims, ime, jms, jme, kms, kme,
+                      ips, ipe, jps, jpe, kps, kpe,
+                      its, ite, jts, jte, kts, kte):
+    """wrf_boundary.py:287-358"""
+    g = grid(field.shape, _I24(ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, ips, ipe, jps, jpe, kps,
+                               kpe, its, ite, jts, jte, kts, kte))
+    call("pywrf_spec_bdyupdate_ph", g, ptr(ph_save), ptr(field), ptr(field_tend), ptr(mu_tend), ptr(muts), float(dt),
+         flag(variable_in), stream())
+    return field
+
+
+def zero_grad_bdy(field,
+                  variable_in,
+                  spec_zone,
+                  ids, ide, jds, jde, kds, kde,
+                  ims, ime, jms, jme, kms, kme,
+                  ips, ipe, jps, jpe, kps, kpe,
+                  its, ite, jts, jte, kts, kte):
+    """wrf_boundary.py:361-405"""
+    g = grid(field.shape, _I24(ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, ips, ipe, jps, jpe, kps,
+                               kpe, its, ite, jts, jte, kts, kte))
+    call("pywrf_zero_grad_bdy", g, ptr(field), flag(variable_in), stream())
+    return field
+
+
+def mass_weight(field, mut, rfield, c1, c2,
+                ids, ide, jds, jde, kds, kde,
+                ims, ime, jms, jme, kms, kme,
+                irs, ire, jrs, jre, krs, kre,
+                its, ite, jts, jte, kts, kte):
+    """wrf_boundary.py:408-415"""
+    g = grid(field.shape, (ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, irs, ire, jrs, jre, krs, kre,
+                           its, ite, jts, jte, kts, kte))
+    call("pywrf_mass_weight", g, ptr(field), ptr(mut), ptr(rfield), stream())
+    return rfield
+
+
+def relax_bdy_scalar(scalar_tend,
+                     scalar, mu, c1h, c2h,
+                     scalar_bxs, scalar_bxe, scalar_bys, scalar_bye,
+                     scalar_btxs, scalar_btxe, scalar_btys, scalar_btye,
+                     spec_bdy_width, spec_zone, relax_zone,
+                     dtbc, fcx, gcx,
+                     ids, ide, jds, jde, kds, kde,
+                     ims, ime, jms, jme, kms, kme,
+                     ips, ipe, jps, jpe, kps, kpe,
+                     its, ite, jts, jte, kts, kte):
+    """wrf_boundary.py:418-451"""
+    g = grid(scalar.shape, _I24(ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, ips, ipe, jps, jpe, kps,
+                                kpe, its, ite, jts, jte, kts, kte))
+    call("pywrf_relax_bdy_scalar", g, ptr(scalar_tend), ptr(scalar), ptr(mu), ptr(scalar_bxs), ptr(scalar_bxe),
+         ptr(scalar_bys), ptr(scalar_bye), ptr(scalar_btxs), ptr(scalar_btxe), ptr(scalar_btys), ptr(scalar_btye),
+         float(dtbc), ptr(fcx), ptr(gcx), stream())
+    return scalar_tend
+
+
+def spec_bdy_scalar(scalar_tend,
+                    scalar_bxs, scalar_bxe, scalar_bys, scalar_bye,
+                    scalar_btxs, scalar_btxe, scalar_btys, scalar_btye,
+                    spec_bdy_width, spec_zone,
+                    ids, ide, jds, jde, kds, kde,
+                    ims, ime, jms, jme, kms, kme,
+                    ips, ipe, jps, jpe, kps, kpe,
+                    its, ite, jts, jte, kts, kte):
+    """wrf_boundary.py:454-472"""
+    g = grid(scalar_tend.shape, _I24(ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, ips, ipe, jps, jpe,
+                                     kps, kpe, its, ite, jts, jte, kts, kte))
+    call("pywrf_spec_bdy_scalar", g, ptr(scalar_tend), ptr(scalar_btxs), ptr(scalar_btxe), ptr(scalar_btys),
+         ptr(scalar_btye), stream())
+    return scalar_tend
+
+
+def flow_dep_bdy(field,
+                 u, v,
+                 spec_zone,
+                 ids, ide, jds, jde, kds, kde,
+                 ims, ime, jms, jme, kms, kme,
+                 ips, ipe, jps, jpe, kps, kpe,
+                 its, ite, jts, jte, kts, kte):
+    """wrf_boundary.py:475-510"""
+    g = grid(field.shape, _I24(ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, ips, ipe, jps, jpe, kps,
+                               kpe, its, ite, jts, jte, kts, kte))
+    call("pywrf_flow_dep_bdy", g, ptr(field), ptr(u), ptr(v), stream())
+    return field
+
+
+def spec_bdy_final(field, mu, c1, c2, msf,
+                   field_bdy_xs, field_bdy_xe,
+                   field_bdy_ys, field_bdy_ye,
+                   field_bdy_tend_xs, field_bdy_tend_xe,
+                   field_bdy_tend_ys, field_bdy_tend_ye,
+                   variable_in,
+                   spec_bdy_width, spec_zone,
+                   dtbc,
+                   ids, ide, jds, jde, kds, kde,
+                   ims, ime, jms, jme, kms, kme,
+                   ips, ipe, jps, jpe, kps, kpe,
+                   its, ite, jts, jte, kts, kte):
+    """wrf_boundary.py:513-587"""
+    g = grid(field.shape, _I24(ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, ips, ipe, jps, jpe, kps,
+                               kpe, its, ite, jts, jte, kts, kte))
+    call("pywrf_spec_bdy_final", g, ptr(field), ptr(mu), ptr(msf), ptr(field_bdy_xs), ptr(field_bdy_xe),
+         ptr(field_bdy_ys), ptr(field_bdy_ye), ptr(field_bdy_tend_xs), ptr(field_bdy_tend_xe),
+         ptr(field_bdy_tend_ys), ptr(field_bdy_tend_ye), flag(variable_in), float(dtbc), stream())
+    return field
