@@ -48,6 +48,8 @@ CONFIGS = {
     # real shape: 240x240x41, 3 seeded hills, 20 m/s jet at 10 km, noise, seeded bubbles, 6 species
     "config2": dict(n=240, hills="random3", jet=(20.0, 10e3, 5e3), u0=0.0, noise=2.0,
                     bubbles="random", clouds=6, seed=0),
+    # flat terrain, air at rest, no perturbations: an invariant case (must stay at rest)
+    "rest": dict(n=40, hills=[], jet=None, u0=0.0, noise=0.0, bubbles=[], clouds=0, seed=0),
     # adjoint-check shape: 120x120x41 with the config2 recipe
     "config4": dict(n=120, hills="random3", jet=(20.0, 10e3, 5e3), u0=0.0, noise=2.0,
                     bubbles="random", clouds=4, seed=0),
