@@ -72,6 +72,14 @@ int pywrf_set_physical_bc2d(const pywrf_grid_t* g, float* dat, char variable, py
 /* set_physical_bc3d, wrf_dynamics.py:216-278 (variable: u v w t d e x y f r p, U V) */
 int pywrf_set_physical_bc3d(const pywrf_grid_t* g, float* dat, char variable, pywrf_stream_t s);
 
+/* A group of n consecutive set_physical_bc3d / set_physical_bc2d calls on
+ * distinct fields (as the reference driver issues them, e.g. solver.py:781-843)
+ * in one launch: fields[f] is a device pointer, variables[f] its flag,
+ * is2d[f] selects the 2-D form (index sets of g2) over the 3-D form (g3).
+ * No counterpart in the reference: a batching of the two calls above. */
+int pywrf_set_physical_bc_group(const pywrf_grid_t* g3, const pywrf_grid_t* g2, int n, void* const* fields,
+                                const char* variables, const int* is2d, pywrf_stream_t s);
+
 /* ---- first-stage physics prep and Smagorinsky: wrf_dynamics.py:314-1076 - */
 /* phy_prep, wrf_dynamics.py:314-411 (pi_phy_out, t_phy_out: fresh arrays) */
 int pywrf_phy_prep(const pywrf_grid_t* g, const float* mut, const float* u, const float* v, const float* p,

@@ -38,6 +38,8 @@ _CTYPES = {
     "pywrf_stream_t": ctypes.c_void_p,
     "const char*": ctypes.c_char_p,
     "double*": ctypes.POINTER(ctypes.c_double),
+    "void* const*": ctypes.POINTER(ctypes.c_void_p),
+    "const int*": ctypes.POINTER(ctypes.c_int),
 }
 
 
@@ -54,6 +56,7 @@ def parse_header(path=HEADER):
                 a = a.strip()
                 t = re.sub(r"\s*\b\w+$", "", a).strip()       # drop the parameter name
                 t = re.sub(r"\s*\*", "*", t)
+                t = re.sub(r"\*\s*const\*", "* const*", t)
                 if t not in _CTYPES:
                     raise ValueError("unsupported C type %r in %s" % (t, name))
                 types.append(_CTYPES[t])

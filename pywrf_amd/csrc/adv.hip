@@ -15,6 +15,7 @@
 // module-level time_step = 60 > 0.
 #include "common.h"
 #include "consts.h"
+#include "diff.h"
 
 #define AT(p, k, j, i) (p)[D.ix((k), (j), (i))]
 #define AT2(p, j, i) (p)[D.ix2((j), (i))]
@@ -348,4 +349,91 @@ extern "C" int pywrf_advect_scalar(const pywrf_grid_t* g, const float* field, fl
                                    const float* rv, const float* rom, const float* msftx, const float* fzm,
                                    const float* fzp, double rdx, double rdy, const float* rdzw, pywrf_stream_t s) {
   return pw_advect_scalar_box(g, field, tendency, ru, rv, rom, msftx, fzm, fzp, F(rdx), F(rdy), rdzw, 0, s);
+}
+
+// ------------------------------------------------------------------ rk_scalar_tend, all species in one pass
+// One thread per point of the whole array: zeroes h_tendency / z_tendency,
+// and for every species of [scs, sce] zeroes its advection tendency on the
+// tile, adds the flux-form advection and (first RK stage) the horizontal and
+// vertical diffusion into scalar_tends -- the per-species sequence of
+// rk_scalar_tend (wrf_dynamics.py:3982-4051), with the mass fluxes ru, rv, rom
+// read once for all species.
+struct ScalarTend {
+  AdvS adv;
+  AdvIdx x;
+  HDiff hd;
+  VDiff vd;
+  const float* scalar;
+  float *adv_tend, *sc_tend, *h_t, *z_t;
+  long n3;
+  int scs, sce, rk1, moist_step;
+  Box bzero, badv, by, bx, bh, bv;
+};
+
+__global__ void k_scalar_tend(Box all, Dims D, ScalarTend a) {
+  int k, j, i;
+  if (!box_point(all, k, j, i)) return;
+  long q = D.ix(k, j, i);
+  a.h_t[q] = 0.0f;
+  a.z_t[q] = 0.0f;
+  if (!a.bzero.has(k, j, i)) return;
+  const bool adv = a.badv.has(k, j, i), ay = a.by.has(k, j, i), ax = a.bx.has(k, j, i);
+  const bool dh = a.rk1 && a.bh.has(k, j, i), dv = a.rk1 && a.bv.has(k, j, i);
+  for (int im = a.scs; im <= a.sce; ++im) {
+    AdvS ad = a.adv;
+    ad.f = a.scalar + im * a.n3;
+    float t = 0.0f;
+    if (adv) {
+      float mx = AT2(ad.msftx, j, i);
+      if (ay) t = t - (mx * ad.rdy) * (s_yface(D, ad, a.x, k, j + 1, i) - s_yface(D, ad, a.x, k, j, i));
+      if (ax) t = t - (mx * ad.rdx) * (s_xface(D, ad, a.x, k, j, i + 1) - s_xface(D, ad, a.x, k, j, i));
+      t = t - ad.rdzw[k] * (s_zface(D, ad, a.x, k + 1, j, i) - s_zface(D, ad, a.x, k, j, i));
+    }
+    a.adv_tend[im * a.n3 + q] = t;
+    if (a.rk1) {
+      float st = a.sc_tend[im * a.n3 + q];
+      if (dh) {
+        HDiff hd = a.hd;
+        hd.f = ad.f;
+        st = hdiff_apply(D, hd, k, j, i, st);
+      }
+      if (dv) {
+        VDiff vd = a.vd;
+        vd.f = ad.f;
+        vd.kind = (a.moist_step && im == pw::P_QV) ? 'p' : 'm';
+        st = vdiff_apply(D, vd, k, j, i, st);
+      }
+      a.sc_tend[im * a.n3 + q] = st;
+    }
+  }
+}
+
+int pw_rk_scalar_tend_fused(const pywrf_grid_t* g, int scs, int sce, int rk_step, const float* ru, const float* rv,
+                            const float* ww, const float* mut, const float* alt, const float* scalar,
+                            float* scalar_tends, float* advect_tend, float* h_tendency, float* z_tendency,
+                            const float* base, int moist_step, const float* fnm, const float* fnp,
+                            const float* msfux, const float* msfuy, const float* msfvx, const float* msfvx_inv,
+                            const float* msfvy, const float* msftx, const float* msfty, float rdx, float rdy,
+                            const float* rdn, const float* rdnw, float kvdq, const float* xkmhd, pywrf_stream_t s) {
+  CHECK(g, 4);
+  ScalarTend a;
+  a.x = {g->ids, g->ide, g->jds, g->jde, imin(g->kte, g->kde - 1)};
+  if (g->kts != 0 || a.x.ktf + 1 > g->nz || g->kte > g->nz || scs < 0 || sce < scs) return PYWRF_E_RANGE;
+  a.adv = {nullptr, ru, rv, ww, msftx, fnm, fnp, rdnw, nullptr, rdx, rdy};
+  a.hd = {nullptr, nullptr, mut, xkmhd, msfux, msfuy, msfvx, msfvx_inv, msfvy, msftx, msfty, nullptr, rdx, rdy, 'm'};
+  a.vd = {nullptr, alt, mut, rdn, rdnw, base, nullptr, nullptr, kvdq, a.x.ktf, g->kts, 'm'};
+  a.scalar = scalar; a.adv_tend = advect_tend; a.sc_tend = scalar_tends; a.h_t = h_tendency; a.z_t = z_tendency;
+  a.n3 = (long)g->nz * g->ny * g->nx;
+  a.scs = scs; a.sce = sce; a.rk1 = rk_step == 1; a.moist_step = moist_step;
+  int ktf = a.x.ktf, jtf = imin(g->jte, g->jde - 1), itf = imin(g->ite, g->ide - 1);
+  a.bzero = mkbox(g->kts, g->kte, g->jts, g->jte, g->its, g->ite);
+  a.badv = mkbox(g->kts, ktf, g->jts, jtf, g->its, itf);
+  a.by = mkbox(g->kts, ktf, imax(g->jts, g->jds + 1), imin(g->jte, g->jde - 2), g->its, itf);
+  a.bx = mkbox(g->kts, ktf, g->jts, jtf, imax(g->ids + 1, g->its), imin(g->ide - 2, g->ite));
+  a.bh = mkbox(g->kts, ktf, imax(g->jds + 1, g->jts), imin(g->jde - 2, g->jte), imax(g->ids + 1, g->its),
+               imin(g->ide - 2, g->ite));
+  a.bv = mkbox(g->kts + 1, ktf, g->jts, jtf, g->its, itf);
+  Box all = mkbox(0, g->nz, 0, g->ny, 0, g->nx);
+  PW_LAUNCH_BOX(k_scalar_tend, all, s, dims_of(g), a);
+  PW_RET();
 }

@@ -34,7 +34,7 @@ static void bounds_of(char v, const pywrf_grid_t* g, int& ibe, int& jbe, int& kt
   if (v == 'm' || v == 'h') ktf = g->kte;
 }
 
-// ------------------------------------------------------------------ relax_bdytend (A31, A48)
+// ------------------------------------------------------------------ relax_bdytend
 struct Relax {
   const float* field;
   const float* mut;  // non-null: field is the unweighted scalar, weighted by mut on mbox (relax_bdy_scalar)
@@ -123,7 +123,7 @@ extern "C" int pywrf_relax_bdy_scalar(const pywrf_grid_t* g, float* scalar_tend,
   return relax_launch(g, a, 'q', s);
 }
 
-// ------------------------------------------------------------------ spec_bdytend (A34, A49)
+// ------------------------------------------------------------------ spec_bdytend
 __global__ void k_spec_bdytend(Box all, Dims D, float* tend, Bdy b, int ibs, int ibe, int jbs, int jbe) {
   int k, j, i;
   if (!box_point(all, k, j, i)) return;
@@ -153,7 +153,7 @@ extern "C" int pywrf_spec_bdy_scalar(const pywrf_grid_t* g, float* scalar_tend, 
   return pywrf_spec_bdytend(g, scalar_tend, btxs, btxe, btys, btye, 'q', s);
 }
 
-// ------------------------------------------------------------------ spec_bdyupdate (A39)
+// ------------------------------------------------------------------ spec_bdyupdate
 __global__ void k_spec_bdyupdate(Box all, Dims D, float* f, const float* t, float dt, int jts, int jtf, int its,
                                  int itf, int jbs, int jbe) {
   int k, j, i;
@@ -186,7 +186,7 @@ extern "C" int pywrf_spec_bdyupdate(const pywrf_grid_t* g, float* field, const f
   PW_RET();
 }
 
-// ------------------------------------------------------------------ spec_bdyupdate_ph (A43)
+// ------------------------------------------------------------------ spec_bdyupdate_ph
 __global__ void k_spec_bdyupdate_ph(Box all, Dims D, const float* ph_save, float* f, const float* ft,
                                     const float* mu_tend, const float* muts, float dt, int jts, int ibs, int ibe,
                                     int jbs, int jbe) {
@@ -217,7 +217,7 @@ extern "C" int pywrf_spec_bdyupdate_ph(const pywrf_grid_t* g, const float* ph_sa
   PW_RET();
 }
 
-// ------------------------------------------------------------------ zero_grad_bdy (A44)
+// ------------------------------------------------------------------ zero_grad_bdy
 __global__ void k_zero_grad(Box all, Dims D, float* f, int ibs, int ibe, int jbs, int jbe) {
   int k, j, i;
   if (!box_point(all, k, j, i)) return;
@@ -241,7 +241,7 @@ extern "C" int pywrf_zero_grad_bdy(const pywrf_grid_t* g, float* field, char var
   PW_RET();
 }
 
-// ------------------------------------------------------------------ mass_weight (A32)
+// ------------------------------------------------------------------ mass_weight
 __global__ void k_mass_weight(Box b, Dims D, const float* field, const float* mut, float* rfield) {
   int k, j, i;
   if (!box_point(b, k, j, i)) return;
@@ -257,7 +257,7 @@ extern "C" int pywrf_mass_weight(const pywrf_grid_t* g, const float* field, cons
   PW_RET();
 }
 
-// ------------------------------------------------------------------ flow_dep_bdy (A51)
+// ------------------------------------------------------------------ flow_dep_bdy
 __global__ void k_flow_dep(Box all, Dims D, float* f, const float* u, const float* v, int jts, int jtf, int its,
                            int itf) {
   int k, j, i;
@@ -289,7 +289,7 @@ extern "C" int pywrf_flow_dep_bdy(const pywrf_grid_t* g, float* field, const flo
   PW_RET();
 }
 
-// ------------------------------------------------------------------ spec_bdy_final (B1)
+// ------------------------------------------------------------------ spec_bdy_final
 __global__ void k_spec_final(Box all, Dims D, float* f, const float* mu, const float* msf, Bdy b, float dtbc,
                              int jts, int jtf, int its, int itf, int ibe, int jbe) {
   int k, j, i;

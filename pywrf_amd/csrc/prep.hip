@@ -26,7 +26,7 @@ int pw_check_grid(const pywrf_grid_t* g, int r) {
 }
 #define CHECK(g, r) do { int _e = pw_check_grid((g), (r)); if (_e) return _e; } while (0)
 
-// ---------------------------------------------------------------- A1, A2, A3
+// ---------------------------------------------------------------- calculate_full, calc_mu_uv, couple_momentum
 __global__ void k_calculate_full(Box b, Dims D, float* mut, const float* mub, const float* mu) {
   int j, i;
   if (!box_column(b, j, i)) return;
@@ -99,7 +99,7 @@ extern "C" int pywrf_couple_momentum(const pywrf_grid_t* g, const float* muu, fl
   PW_RET();
 }
 
-// ---------------------------------------------------------------- A4
+// ---------------------------------------------------------------- calc_ww_cp
 __device__ inline float ww_divv(const Dims& D, int k, int j, int i, const float* u, const float* v,
                                 const float* muu, const float* muv, const float* msftx, const float* msfuy,
                                 const float* msfvx_inv, const float* dnw, float rdx, float rdy) {
@@ -141,7 +141,7 @@ extern "C" int pywrf_calc_ww_cp(const pywrf_grid_t* g, const float* u, const flo
   PW_RET();
 }
 
-// ---------------------------------------------------------------- A5, A6, A7
+// ---------------------------------------------------------------- calc_cq, calc_alt, calc_php
 __device__ inline float qsum(const Dims& D, const float* moist, int n, int k, int j, int i) {
   long p = D.ix(k, j, i), st = (long)D.nz * D.plane();
   float s = moist[p];
@@ -206,7 +206,7 @@ extern "C" int pywrf_calc_php(const pywrf_grid_t* g, float* php, const float* ph
   PW_RET();
 }
 
-// ---------------------------------------------------------------- A8, A9
+// ---------------------------------------------------------------- set_physical_bc3d / bc2d
 // One workgroup per level: the x-direction copies, a barrier, then the
 // y-direction copies (which read cells the x-copies may have written).
 struct BcSpec {
@@ -301,7 +301,114 @@ extern "C" int pywrf_set_physical_bc3d(const pywrf_grid_t* g, float* dat, char v
   return bc_launch(g, dat, g->kts, k1, c, s);
 }
 
-// ---------------------------------------------------------------- A10
+// Several open-boundary fills in one launch: the reference applies
+// set_physical_bc3d / set_physical_bc2d to a group of distinct fields in a
+// row; each (field, level) pair here is one workgroup running the same
+// x-then-y copy as k_open_bc.
+#define PW_BC_GROUP_MAX 32
+struct BcJob {
+  float* p;
+  int k0, nk;
+  BcSpec c;
+};
+struct BcJobs {
+  int n;
+  BcJob job[PW_BC_GROUP_MAX];
+};
+
+__global__ void k_open_bc_group(long plane, int nx, BcJobs J) {
+  if ((int)blockIdx.y >= J.n) return;
+  const BcJob& jb = J.job[blockIdx.y];
+  if ((int)blockIdx.x >= jb.nk) return;
+  float* a = jb.p + (long)(jb.k0 + blockIdx.x) * plane;
+  const BcSpec& c = jb.c;
+  int r0 = c.ustag ? c.ur0 : c.xr0, r1 = c.ustag ? c.ur1 : c.xr1;
+  int nstart = c.xr1 - c.xr0, nend = r1 - r0;
+  for (int t = threadIdx.x; t < 3 * (nstart + nend); t += blockDim.x) {
+    int h = t % 3, r = t / 3;
+    if (r < nstart) {
+      int row = c.xr0 + r;
+      a[(long)row * nx + c.ids - 1 - h] = a[(long)row * nx + c.ids];
+    } else {
+      int row = r0 + (r - nstart);
+      if (c.ustag) a[(long)row * nx + c.ide + h] = a[(long)row * nx + c.ide - 1];
+      else a[(long)row * nx + c.ide - 1 + h] = a[(long)row * nx + c.ide - 2];
+    }
+  }
+  __syncthreads();
+  int nc = c.yc1 - c.yc0;
+  for (int t = threadIdx.x; t < 6 * nc; t += blockDim.x) {
+    int h = t % 3, side = (t / 3) & 1, col = c.yc0 + t / 6;
+    if (side == 0) a[(long)(c.jds - 1 - h) * nx + col] = a[(long)c.jds * nx + col];
+    else if (c.vstag) a[(long)(c.jde + h) * nx + col] = a[(long)(c.jde - 1) * nx + col];
+    else a[(long)(c.jde - 1 + h) * nx + col] = a[(long)(c.jde - 2) * nx + col];
+  }
+}
+
+static int bc_spec3(const pywrf_grid_t* g, char variable, BcSpec& c, int& k1) {
+  char v = variable;
+  if (v == 'U') v = 'u';
+  else if (v == 'V') v = 'v';
+  if (!in(v, "uvwtdexyfrp")) return PYWRF_E_ARG;
+  int istag = in(v, "uxde") ? 0 : -1, jstag = in(v, "vydf") ? 0 : -1;
+  k1 = in(v, "efw") ? imin(g->kde, g->kte) : imin(g->kde - 1, g->kte);
+  c.ids = g->ids; c.ide = g->ide; c.jds = g->jds; c.jde = g->jde;
+  c.xr0 = g->jts - 4; c.xr1 = g->jde + jstag + 4;
+  c.ur0 = g->jts - 5; c.ur1 = g->jde + jstag + 4;
+  c.ustag = in(v, "ux"); c.vstag = in(v, "vy");
+  c.yc0 = g->ids; c.yc1 = g->ide + istag;
+  return 0;
+}
+
+static int bc_spec2(const pywrf_grid_t* g, char variable, BcSpec& c) {
+  char v = lower(variable);
+  if (!in(v, "uvwtxyrp")) return PYWRF_E_ARG;
+  int istag = in(v, "uxde") ? 0 : -1, jstag = in(v, "vydf") ? 0 : -1;
+  c.ids = g->ids; c.ide = g->ide; c.jds = g->jds; c.jde = g->jde;
+  c.xr0 = c.ur0 = g->jds; c.xr1 = c.ur1 = g->jde + jstag;
+  c.ustag = in(v, "ux"); c.vstag = in(v, "vy");
+  c.yc0 = imax(g->ids, g->its - 1); c.yc1 = imin(g->ite + 1, g->ide + istag);
+  return 0;
+}
+
+static int bc_spec_ok(int ny, int nx, int nz, int k0, int k1, const BcSpec& c) {
+  int rlo = imin(c.xr0, c.ustag ? c.ur0 : c.xr0), rhi = imax(c.xr1, c.ustag ? c.ur1 : c.xr1);
+  if (rlo < 0 || rhi > ny || c.ids - 3 < 0 || c.ide + 3 > nx) return 0;
+  if (c.yc0 < 0 || c.yc1 > nx || c.jds - 3 < 0 || c.jde + 3 > ny) return 0;
+  return k0 >= 0 && k1 <= nz;
+}
+
+extern "C" int pywrf_set_physical_bc_group(const pywrf_grid_t* g3, const pywrf_grid_t* g2, int n, void* const* fields,
+                                           const char* variables, const int* is2d, pywrf_stream_t s) {
+  if (n < 0 || n > PW_BC_GROUP_MAX || g3->ny != g2->ny || g3->nx != g2->nx) return PYWRF_E_ARG;
+  BcJobs J;
+  J.n = n;
+  int maxk = 1;
+  for (int f = 0; f < n; ++f) {
+    BcJob& jb = J.job[f];
+    jb.p = (float*)fields[f];
+    int e, k1 = 1;
+    if (is2d[f]) {
+      e = bc_spec2(g2, variables[f], jb.c);
+      jb.k0 = 0;
+      k1 = 1;
+      if (!e && !bc_spec_ok(g2->ny, g2->nx, 1, 0, 1, jb.c)) e = PYWRF_E_RANGE;
+    } else {
+      e = bc_spec3(g3, variables[f], jb.c, k1);
+      jb.k0 = g3->kts;
+      if (!e && !bc_spec_ok(g3->ny, g3->nx, g3->nz, g3->kts, k1, jb.c)) e = PYWRF_E_RANGE;
+    }
+    if (e) return e;
+    jb.nk = k1 - jb.k0 > 0 ? k1 - jb.k0 : 0;
+    maxk = imax(maxk, jb.nk);
+  }
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_open_bc_group, dim3(maxk, n), dim3(PW_BLOCK), 0, (hipStream_t)s, (long)g3->ny * g3->nx,
+                     g3->nx, J);
+  PW_RET();
+}
+
+// ---------------------------------------------------------------- phy_prep
 // Launch 1 (whole array): th_phy, p_phy on the box; pi_phy, t_phy everywhere
 // (fresh arrays); rho, u_phy, v_phy, z_at_w on the box.
 __global__ void k_phy_prep_pt(Box all, Dims D, Box b, const float* u, const float* v, const float* p,
@@ -393,7 +500,7 @@ extern "C" int pywrf_phy_prep(const pywrf_grid_t* g, const float* mut, const flo
   PW_RET();
 }
 
-// ---------------------------------------------------------------- A11, A53
+// ---------------------------------------------------------------- calculate_phy_tend
 __global__ void k_scale_by_mut(Box b, Dims D, const float* mut, float* a0, float* a1, float* a2, float* a3,
                                float* a4, float* a5, float* sc, int nsc, int divide) {
   int k, j, i;
@@ -416,7 +523,7 @@ extern "C" int pywrf_calculate_phy_tend(const pywrf_grid_t* g, const float* mut,
   PW_RET();
 }
 
-// ---------------------------------------------------------------- A12
+// ---------------------------------------------------------------- compute_diff_metrics
 __device__ inline float zw_at(const Dims& D, const float* ph, const float* phb, int k, int j, int i) {
   return (AT(ph, k, j, i) + AT(phb, k, j, i)) / F(pw::g);
 }
@@ -456,7 +563,7 @@ extern "C" int pywrf_compute_diff_metrics(const pywrf_grid_t* g, const float* ph
   PW_RET();
 }
 
-// ---------------------------------------------------------------- A13
+// ---------------------------------------------------------------- cal_deform_and_div
 struct DefCoef {
   const float *cf1, *cf2, *cf3, *dn, *dnw;
   int ktes1, ktes2, kte, ktf;
@@ -626,7 +733,7 @@ extern "C" int pywrf_cal_deform_and_div(const pywrf_grid_t* g, const float* u, c
   PW_RET();
 }
 
-// ---------------------------------------------------------------- A14
+// ---------------------------------------------------------------- calculate_km_kh
 __device__ inline float sat_qvs(float t, float p) {
   float es = F(1000.0 * pw::SVP1) * expf(F(pw::SVP2) * (t - F(pw::SVPT0)) / (t - F(pw::SVP3)));
   return F(pw::EP_2) * es / (p - es);
@@ -741,7 +848,7 @@ extern "C" int pywrf_calculate_km_kh(const pywrf_grid_t* g, float* xkmh, float* 
   PW_RET();
 }
 
-// ---------------------------------------------------------------- A15
+// ---------------------------------------------------------------- update_phy_ten
 __global__ void k_update_phy_ten(Box b, Dims D, Box ba, Box bu, Box bv, float* rt_tendf, float* ru_tendf,
                                  float* rv_tendf, float* mqv, float* mqc, float* mqi, const float* rth,
                                  const float* ru, const float* rv, const float* rqv, const float* rqc,

@@ -9,19 +9,13 @@
 int pw_check_grid(const pywrf_grid_t* g, int r);
 #define CHECK(g, r) do { int _e = pw_check_grid((g), (r)); if (_e) return _e; } while (0)
 
-int pw_advect_scalar_box(const pywrf_grid_t* g, const float* field, float* tendency, const float* ru,
-                         const float* rv, const float* rom, const float* msftx, const float* fzm, const float* fzp,
-                         float rdx, float rdy, const float* rdzw, int zero_box, pywrf_stream_t s);
-int pw_horizontal_diffusion(const pywrf_grid_t* g, char name, const float* field, float* tendency,
-                            const float* base, const float* mut, const float* msfux, const float* msfuy,
-                            const float* msfvx, const float* msfvx_inv, const float* msfvy, const float* msftx,
-                            const float* msfty, const float* xkmhd, float rdx, float rdy, pywrf_stream_t s);
-extern "C" int pywrf_vertical_diffusion(const pywrf_grid_t* g, char name, const float* field, float* tendency,
-                                        const float* alt, const float* mut, const float* rdn, const float* rdnw,
-                                        double kvdif, pywrf_stream_t s);
-extern "C" int pywrf_vertical_diffusion_mp(const pywrf_grid_t* g, const float* field, float* tendency,
-                                           const float* base, const float* alt, const float* mut, const float* rdn,
-                                           const float* rdnw, double kvdif, pywrf_stream_t s);
+int pw_rk_scalar_tend_fused(const pywrf_grid_t* g, int scs, int sce, int rk_step, const float* ru, const float* rv,
+                            const float* ww, const float* mut, const float* alt, const float* scalar,
+                            float* scalar_tends, float* advect_tend, float* h_tendency, float* z_tendency,
+                            const float* base, int moist_step, const float* fnm, const float* fnp,
+                            const float* msfux, const float* msfuy, const float* msfvx, const float* msfvx_inv,
+                            const float* msfvy, const float* msftx, const float* msfty, float rdx, float rdy,
+                            const float* rdn, const float* rdnw, float kvdq, const float* xkmhd, pywrf_stream_t s);
 
 // Fill kernel used instead of hipMemsetAsync so that the calls stay safe under
 // stream capture.
@@ -38,7 +32,7 @@ int pw_fill(float* a, long n, float v, pywrf_stream_t s) {
   return (int)hipGetLastError();
 }
 
-// ------------------------------------------------------------------ rk_scalar_tend (A47)
+// ------------------------------------------------------------------ rk_scalar_tend
 extern "C" int pywrf_rk_scalar_tend(const pywrf_grid_t* g, int scs, int sce, int rk_step, const float* ru,
                                     const float* rv, const float* ww, const float* mut, const float* alt,
                                     const float* scalar, float* scalar_tends, float* advect_tend,
@@ -48,37 +42,20 @@ extern "C" int pywrf_rk_scalar_tend(const pywrf_grid_t* g, int scs, int sce, int
                                     const float* msftx, const float* msfty, double rdx, double rdy,
                                     const float* rdn, const float* rdnw, double kvdif, const float* xkmhd,
                                     pywrf_stream_t s) {
-  CHECK(g, 4);
-  long n3 = (long)g->nz * g->ny * g->nx;
-  double kvdq = kvdif / pw::prandtl;
-  int e;
-  // h_tendency and z_tendency are zeroed once per species by the reference; once is the same
-  if ((e = pw_fill(h_tendency, n3, 0.0f, s))) return e;
-  if ((e = pw_fill(z_tendency, n3, 0.0f, s))) return e;
-  for (int im = scs; im <= sce; ++im) {
-    const float* f = scalar + im * n3;
-    if ((e = pw_advect_scalar_box(g, f, advect_tend + im * n3, ru, rv, ww, msftx, fnm, fnp, F(rdx), F(rdy), rdnw, 1,
-                                  s)))
-      return e;
-    if (rk_step == 1) {
-      float* st = scalar_tends + im * n3;
-      if ((e = pw_horizontal_diffusion(g, 'm', f, st, nullptr, mut, msfux, msfuy, msfvx, msfvx_inv, msfvy, msftx,
-                                       msfty, xkmhd, F(rdx), F(rdy), s)))
-        return e;
-      if (moist_step && im == pw::P_QV) e = pywrf_vertical_diffusion_mp(g, f, st, base, alt, mut, rdn, rdnw, kvdq, s);
-      else e = pywrf_vertical_diffusion(g, 'm', f, st, alt, mut, rdn, rdnw, kvdq, s);
-      if (e) return e;
-    }
-  }
-  return 0;
+  // kvdif / prandtl is the reference's kvdq (formed in double, rounded once)
+  return pw_rk_scalar_tend_fused(g, scs, sce, rk_step, ru, rv, ww, mut, alt, scalar, scalar_tends, advect_tend,
+                                 h_tendency, z_tendency, base, moist_step, fnm, fnp, msfux, msfuy, msfvx, msfvx_inv,
+                                 msfvy, msftx, msfty, F(rdx), F(rdy), rdn, rdnw, F(kvdif / pw::prandtl), xkmhd, s);
 }
 
-// ------------------------------------------------------------------ rk_update_scalar (A50)
+// ------------------------------------------------------------------ rk_update_scalar
+// All species of [scs, sce] in one pass; muold / munew formed once per point.
 struct UpdS {
   float *s1, *s2;
   const float *sc_tend, *adv, *msfty, *mu_old, *mu_new, *mub;
+  long n3;
   float dt;
-  int rk1;
+  int rk1, scs, sce;
   Box bs, bi, bf;
   int jf0, jf1, if0, if1;  // rows / columns where muold and munew are set
 };
@@ -87,20 +64,23 @@ __global__ void k_rk_update_scalar(Box all, Dims D, UpdS a) {
   int k, j, i;
   if (!box_point(all, k, j, i)) return;
   long q = D.ix(k, j, i), q2 = D.ix2(j, i);
-  float s2 = a.s2[q];
-  float s1 = a.s1[q];
-  if (a.rk1) {
-    s1 = s2 + 0.0f;
-    a.s1[q] = s1;
-  }
-  if (!a.bf.has(k, j, i)) return;
-  float t = 0.0f;
-  if (a.bi.has(k, j, i)) t = a.adv[q] * a.msfty[q2];
-  if (a.bs.has(k, j, i)) t = t + a.sc_tend[q];
+  const bool inf = a.bf.has(k, j, i), ini = a.bi.has(k, j, i), ins = a.bs.has(k, j, i);
   bool hm = j >= a.jf0 && j < a.jf1 && i >= a.if0 && i < a.if1;
   float muold = hm ? a.mu_old[q2] + a.mub[q2] : 0.0f;
   float munew = hm ? a.mu_new[q2] + a.mub[q2] : 0.0f;
-  a.s2[q] = (muold * s1 + a.dt * t) / munew;
+  for (int im = a.scs; im <= a.sce; ++im) {
+    long qq = q + im * a.n3;
+    float s1 = a.s1[qq];
+    if (a.rk1) {
+      s1 = a.s2[qq] + 0.0f;
+      a.s1[qq] = s1;
+    }
+    if (!inf) continue;
+    float t = 0.0f;
+    if (ini) t = a.adv[qq] * a.msfty[q2];
+    if (ins) t = t + a.sc_tend[qq];
+    a.s2[qq] = (muold * s1 + a.dt * t) / munew;
+  }
 }
 
 extern "C" int pywrf_rk_update_scalar(const pywrf_grid_t* g, int scs, int sce, float* scalar_1, float* scalar_2,
@@ -108,28 +88,22 @@ extern "C" int pywrf_rk_update_scalar(const pywrf_grid_t* g, int scs, int sce, f
                                       const float* mu_old, const float* mu_new, const float* mu_base, int rk_step,
                                       double dt, int spec_zone, pywrf_stream_t s) {
   CHECK(g, 0);
-  if (g->kte > g->nz || g->kde > g->nz) return PYWRF_E_RANGE;
-  long n3 = (long)g->nz * g->ny * g->nx;
+  if (g->kte > g->nz || g->kde > g->nz || scs < 0 || sce < scs) return PYWRF_E_RANGE;
   UpdS a;
+  a.s1 = scalar_1; a.s2 = scalar_2; a.sc_tend = sc_tend; a.adv = advect_tend;
   a.mu_old = mu_old; a.mu_new = mu_new; a.mub = mu_base; a.msfty = msfty; a.dt = F(dt); a.rk1 = rk_step == 1;
+  a.n3 = (long)g->nz * g->ny * g->nx; a.scs = scs; a.sce = sce;
   a.bs = mkbox(g->kts, g->kte - 1, g->jts, imin(g->jte, g->jde - 1), g->its, imin(g->ite, g->ide - 1));
   a.bi = mkbox(g->kts, imin(g->kte, g->kde - 1), imax(g->jts, g->jds + spec_zone), imin(g->jte, g->jde - spec_zone - 1),
                imax(g->its, g->ids + spec_zone), imin(g->ite, g->ide - spec_zone - 1));
   a.bf = mkbox(g->kts, g->kde - 1, g->jts, g->jde - 1, g->its, g->ide - 1);
   a.jf0 = g->jts; a.jf1 = g->jde - 1; a.if0 = g->its; a.if1 = g->ide - 1;
-  Box all = mkbox(0, g->nz, 0, g->ny, 0, g->nx);
-  for (int im = scs; im <= sce; ++im) {
-    a.s1 = scalar_1 + im * n3;
-    a.s2 = scalar_2 + im * n3;
-    a.sc_tend = sc_tend + im * n3;
-    a.adv = advect_tend + im * n3;
-    Box b = a.rk1 ? all : a.bf;
-    PW_LAUNCH_BOX(k_rk_update_scalar, b, s, dims_of(g), a);
-  }
+  Box b = a.rk1 ? mkbox(0, g->nz, 0, g->ny, 0, g->nx) : a.bf;
+  PW_LAUNCH_BOX(k_rk_update_scalar, b, s, dims_of(g), a);
   PW_RET();
 }
 
-// ------------------------------------------------------------------ calc_p_rho_phi (A52)
+// ------------------------------------------------------------------ calc_p_rho_phi
 __global__ void k_calc_p_rho_phi(Box b, Dims D, const float* qv, float* al, const float* alb, const float* mu,
                                  const float* muts, const float* ph, float* p, const float* pb, const float* t,
                                  float p0, float t0, const float* rdnw) {
@@ -157,7 +131,7 @@ extern "C" int pywrf_calc_p_rho_phi(const pywrf_grid_t* g, const float* moist, f
   PW_RET();
 }
 
-// ------------------------------------------------------------------ phy_prep_part2 (A53)
+// ------------------------------------------------------------------ phy_prep_part2
 __global__ void k_div_by_mut(Box b, Dims D, const float* mut, float* a0, float* a1, float* a2, float* a3,
                              float* a4, float* a5) {
   int k, j, i;
@@ -182,7 +156,7 @@ extern "C" int pywrf_phy_prep_part2(const pywrf_grid_t* g, const float* mut, flo
   PW_RET();
 }
 
-// ------------------------------------------------------------------ set_w_surface (B2)
+// ------------------------------------------------------------------ set_w_surface
 // One-sided terrain-slope stencils on the outer rows / columns of [jts, jte-1)
 // x [its, ite-1); at the (jts, ite-2) corner the y slope is the reference's
 // ht[jts+1] - ht[jts+1], and at the (jte-2, ite-2) corner the x term samples

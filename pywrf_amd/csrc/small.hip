@@ -19,7 +19,7 @@ int pw_check_grid(const pywrf_grid_t* g, int r);
 static Box whole(const pywrf_grid_t* g) { return mkbox(0, g->nz, 0, g->ny, 0, g->nx); }
 static Box plane(const pywrf_grid_t* g) { return mkbox(0, 1, 0, g->ny, 0, g->nx); }
 
-// ------------------------------------------------------------------ small_step_prep (A35)
+// ------------------------------------------------------------------ small_step_prep
 struct SsPrep2 {
   const float *mub, *muu, *muv;
   float *mu_1, *mu_2, *muus, *muvs, *muts, *mudf, *mu_save;
@@ -146,7 +146,7 @@ extern "C" int pywrf_small_step_prep(const pywrf_grid_t* g, float* u_1, const fl
   PW_RET();
 }
 
-// ------------------------------------------------------------------ calc_p_rho (A36)
+// ------------------------------------------------------------------ calc_p_rho
 struct PRho {
   const float *al_in, *p_in, *ph, *alt, *t_2, *t_1, *c2a, *mu, *mut, *rdnw;
   float *al, *p, *pm1;
@@ -188,7 +188,7 @@ extern "C" int pywrf_calc_p_rho(const pywrf_grid_t* g, const float* al_in, const
   PW_RET();
 }
 
-// ------------------------------------------------------------------ calc_coef_w (A37)
+// ------------------------------------------------------------------ calc_coef_w
 __global__ void k_calc_coef_w(Box b, Dims D, float* a, float* alpha, float* gamma, const float* mut,
                               const float* cqw, const float* rdn, const float* rdnw, const float* c2a, float C,
                               float m2C, float p2C, float lid, int kde) {
@@ -233,7 +233,7 @@ extern "C" int pywrf_calc_coef_w(const pywrf_grid_t* g, float* a, float* alpha, 
   PW_RET();
 }
 
-// ------------------------------------------------------------------ advance_uv (A38)
+// ------------------------------------------------------------------ advance_uv
 struct AdvUV {
   const float *u_in, *ru_tend, *v_in, *rv_tend, *p, *pb, *ph, *php, *alt, *al, *mu, *muu, *cqu, *muv, *cqv, *mudf;
   const float *msfux, *msfuy, *msfvx, *msfvx_inv, *msfvy, *cf, *fnm, *fnp, *rdnw;
@@ -310,7 +310,7 @@ extern "C" int pywrf_advance_uv(const pywrf_grid_t* g, const float* u_in, const 
   PW_RET();
 }
 
-// ------------------------------------------------------------------ advance_mu_t (A40)
+// ------------------------------------------------------------------ advance_mu_t
 struct AdvMuT {
   const float *ww_in, *ww_1, *u, *u_1, *v, *v_1, *mu_in, *mut, *muave_in, *muts_in, *muu, *muv, *mudf_in;
   const float *t_in, *t_1, *t_ave_in, *ft, *mu_tend, *dnw, *fnm, *fnp, *rdnw;
@@ -328,60 +328,80 @@ __device__ inline float mut_dvdxi(const Dims& D, const AdvMuT& a, int k, int j, 
   return AT2(a.msftx, j, i) * AT2(a.msfty, j, i) * (a.rdy * vy + a.rdx * ux);
 }
 
-// one thread per (j, i) column of the whole plane; columns outside the box copy
-__global__ void k_advance_mu_t(Box cols, Dims D, AdvMuT a) {
-  int j, i;
-  if (!box_column(cols, j, i)) return;
-  long q2 = D.ix2(j, i);
-  bool h = j >= a.j0 && j < a.j1 && i >= a.i0 && i < a.i1;
-  if (!h) {
+// Column block: a 256-thread workgroup owns TI consecutive i-columns of one
+// row j and all levels.  The divergence dvdxi of every (k, i) is computed by
+// all threads and staged in LDS; one thread per column then forms the column
+// sum, the column-mass update and the eta-velocity recurrence from LDS; all
+// threads finally apply the theta update level by level (coalesced in i).
+// Columns outside the update box copy their inputs (clone semantics).
+#define TI 64
+#define TB 256
+#define TG (TB / TI)
+
+__global__ void __launch_bounds__(TB) k_advance_mu_t(Dims D, AdvMuT a) {
+  extern __shared__ float sm[];
+  const int nz = D.nz;
+  float* s_dv = sm;             // [nz][TI] dvdxi
+  float* s_ww = sm + nz * TI;   // [nz][TI] eta velocity before subtracting ww_1
+  const int lane = threadIdx.x % TI, grp = threadIdx.x / TI;
+  const int j = blockIdx.y, i = blockIdx.x * TI + lane;
+  const bool vi = i < D.nx;
+  const bool h = vi && j >= a.j0 && j < a.j1 && i >= a.i0 && i < a.i1;
+  const long q2 = vi ? D.ix2(j, i) : 0;
+  for (int k = grp; k < nz; k += TG) {
+    if (!vi) continue;
+    long q = D.ix(k, j, i);
+    if (!h) {
+      a.ww[q] = a.ww_in[q];
+      a.t[q] = a.t_in[q];
+      a.t_ave[q] = a.t_ave_in[q];
+      continue;
+    }
+    s_dv[k * TI + lane] = k < a.k1 ? mut_dvdxi(D, a, k, j, i) : 0.0f;
+  }
+  if (vi && !h && grp == 0) {
     a.muave[q2] = a.muave_in[q2];
     a.muts[q2] = a.muts_in[q2];
     a.mudf[q2] = a.mudf_in[q2];
     a.mu[q2] = a.mu_in[q2];
-    for (int k = 0; k < D.nz; ++k) {
-      long q = D.ix(k, j, i);
+  }
+  __syncthreads();
+  if (grp == 0 && h) {
+    float dmdt = torch_outer_sum(nz, [&](int k) { return k < a.k1 ? a.dnw[k] * s_dv[k * TI + lane] : 0.0f; });
+    float mt = a.mu_tend[q2];
+    float mu_old = a.mu_in[q2];
+    float mu = mu_old + a.dts * (dmdt + mt);
+    a.mu[q2] = mu;
+    a.mudf[q2] = dmdt + mt;
+    a.muts[q2] = a.mut[q2] + mu;
+    a.muave[q2] = 0.5f * (a.ep1 * mu + a.em1 * (mu_old + 0.0f));
+    float my = a.msfty[q2];
+    float wk = a.ww_in[D.ix(0, j, i)];
+    s_ww[lane] = wk;
+    for (int k = 1; k < a.k1; ++k) {
+      wk = wk - a.dnw[k - 1] * (dmdt + s_dv[(k - 1) * TI + lane] + mt) / my;
+      s_ww[k * TI + lane] = wk;
+    }
+  }
+  __syncthreads();
+  if (!h) return;
+  const float my = a.msfty[q2], mx = a.msftx[q2];
+  for (int k = grp; k < nz; k += TG) {
+    long q = D.ix(k, j, i);
+    if (k >= a.k1) {
       a.ww[q] = a.ww_in[q];
       a.t[q] = a.t_in[q];
       a.t_ave[q] = a.t_ave_in[q];
-    }
-    return;
-  }
-  float dmdt = torch_outer_sum(D.nz, [&](int k) { return k < a.k1 ? a.dnw[k] * mut_dvdxi(D, a, k, j, i) : 0.0f; });
-  float mt = a.mu_tend[q2];
-  float mu_old = a.mu_in[q2];
-  float mu = mu_old + a.dts * (dmdt + mt);
-  a.mu[q2] = mu;
-  a.mudf[q2] = dmdt + mt;
-  a.muts[q2] = a.mut[q2] + mu;
-  a.muave[q2] = 0.5f * (a.ep1 * mu + a.em1 * (mu_old + 0.0f));
-  float my = a.msfty[q2], mx = a.msftx[q2];
-  // eta-velocity recurrence, then subtract ww_1 on [0, k1)
-  float wk = a.ww_in[D.ix(0, j, i)];
-  a.ww[D.ix(0, j, i)] = wk - a.ww_1[D.ix(0, j, i)];
-  for (int k = 1; k < a.k1; ++k) {
-    wk = wk - a.dnw[k - 1] * (dmdt + mut_dvdxi(D, a, k - 1, j, i) + mt) / my;
-    long q = D.ix(k, j, i);
-    a.ww[q] = wk - a.ww_1[q];
-  }
-  for (int k = a.k1; k < D.nz; ++k) {
-    long q = D.ix(k, j, i);
-    a.ww[q] = a.ww_in[q];
-  }
-  // theta update with the time-averaged flux divergence
-  for (int k = 0; k < D.nz; ++k) {
-    long q = D.ix(k, j, i);
-    float t = a.t_in[q];
-    if (k >= a.k1) {
-      a.t[q] = t;
-      a.t_ave[q] = a.t_ave_in[q];
       continue;
     }
+    auto wwn = [&](int kk) { return s_ww[kk * TI + lane] - AT(a.ww_1, kk, j, i); };
+    a.ww[q] = wwn(k);
+    float t = a.t_in[q];
     a.t_ave[q] = t + 0.0f;
     t = t + my * a.dts * a.ft[q];
     auto wdtn = [&](int kk) -> float {
       if (kk <= 0 || kk >= a.k1) return 0.0f;
-      return a.ww[D.ix(kk, j, i)] * (a.fnm[kk] * AT(a.t_1, kk, j, i) + a.fnp[kk] * AT(a.t_1, kk - 1, j, i));
+      return wwn(kk) * (a.fnm[kk] * AT(a.t_1, kk, j, i) + a.fnp[kk] * AT(a.t_1, kk - 1, j, i));
     };
     float t1 = a.t_1[q];
     float yterm = a.hrdy * (AT(a.v, k, j + 1, i) * (AT(a.t_1, k, j + 1, i) + t1) - AT(a.v, k, j, i) * (t1 + AT(a.t_1, k, j - 1, i)));
@@ -389,6 +409,11 @@ __global__ void k_advance_mu_t(Box cols, Dims D, AdvMuT a) {
     t = t - a.dts * my * (mx * (yterm + xterm) + a.rdnw[k] * (wdtn(k + 1) - wdtn(k)));
     a.t[q] = t;
   }
+}
+
+static int column_block_launch_check(const pywrf_grid_t* g, int narrays) {
+  size_t bytes = (size_t)narrays * g->nz * TI * sizeof(float);
+  return bytes <= 64 * 1024 ? 0 : PYWRF_E_RANGE;
 }
 
 extern "C" int pywrf_advance_mu_t(const pywrf_grid_t* g, const float* ww_in, const float* ww_1, const float* u,
@@ -408,11 +433,13 @@ extern "C" int pywrf_advance_mu_t(const pywrf_grid_t* g, const float* ww_in, con
               muave_out, muts_out, mudf_out, mu_out, F(rdx), F(rdy), F(dts), F(1. + epssm), F(1. - epssm),
               F(0.5 * rdx), F(0.5 * rdy), g->kte - 1, g->kde, imax(g->jts, g->jds + 1), imin(g->jte, g->jde - 2),
               imax(g->its, g->ids + 1), imin(g->ite, g->ide - 2)};
-  PW_LAUNCH_COLS(k_advance_mu_t, plane(g), s, dims_of(g), a);
+  if (column_block_launch_check(g, 2)) return PYWRF_E_RANGE;
+  dim3 grid((g->nx + TI - 1) / TI, g->ny);
+  hipLaunchKernelGGL(k_advance_mu_t, grid, dim3(TB), 2 * g->nz * TI * sizeof(float), (hipStream_t)s, dims_of(g), a);
   PW_RET();
 }
 
-// ------------------------------------------------------------------ advance_w (A41)
+// ------------------------------------------------------------------ advance_w
 struct AdvW {
   const float *w_in, *rw_tend, *ww, *w_save, *u, *v, *mut, *muave, *muts, *t_2ave_in, *t_2, *t_1, *ph_in, *ph_1;
   const float *phb, *ph_tend, *ht, *c2a, *cqw, *alt, *a, *alpha, *gamma, *fnm, *fnp, *rdnw, *rdn, *cf;
@@ -422,110 +449,143 @@ struct AdvW {
   int k1, j0, j1, i0, i1;
 };
 
-__global__ void k_advance_w(Box cols, Dims D, AdvW a) {
-  int j, i;
-  if (!box_column(cols, j, i)) return;
-  long q2 = D.ix2(j, i);
-  bool h = j >= a.j0 && j < a.j1 && i >= a.i0 && i < a.i1;
-  const int k1 = a.k1;
-  if (!h) {
-    for (int k = 0; k < D.nz; ++k) {
-      long q = D.ix(k, j, i);
+// Column block as in advance_mu_t.  Phase 1 (all threads): t_2ave and the
+// phi right-hand side rhs for every (k, i), staged in LDS with the
+// tridiagonal factors a, alpha, gamma.  Phase 2 (all threads): surface w and
+// the explicit w predictor from the LDS rhs / t_2ave.  Phase 3 (one thread
+// per column): Thomas forward sweep and back substitution in LDS.  Phase 4
+// (all threads): Rayleigh damping and the new phi, written back coalesced.
+__global__ void __launch_bounds__(TB) k_advance_w(Dims D, AdvW a) {
+  extern __shared__ float sm[];
+  const int nz = D.nz, k1 = a.k1;
+  float* s_t2a = sm;
+  float* s_rhs = sm + 1 * nz * TI;
+  float* s_w = sm + 2 * nz * TI;
+  float* s_a = sm + 3 * nz * TI;
+  float* s_al = sm + 4 * nz * TI;
+  float* s_ga = sm + 5 * nz * TI;
+  const int lane = threadIdx.x % TI, grp = threadIdx.x / TI;
+  const int j = blockIdx.y, i = blockIdx.x * TI + lane;
+  const bool vi = i < D.nx;
+  const bool h = vi && j >= a.j0 && j < a.j1 && i >= a.i0 && i < a.i1;
+  const long q2 = vi ? D.ix2(j, i) : 0;
+  const float G = F(pw::g);
+  float mut = 0.0f, muts = 0.0f, muave = 0.0f, my = 1.0f, mx = 1.0f;
+  if (h) {
+    mut = a.mut[q2]; muts = a.muts[q2]; muave = a.muave[q2]; my = a.msfty[q2]; mx = a.msftx[q2];
+  }
+  const float mi = 1.0f / my;
+  auto wdwn = [&](int k) {
+    return 0.5f * (AT(a.ww, k, j, i) + AT(a.ww, k - 1, j, i)) * a.rdnw[k - 1] *
+           (AT(a.ph_1, k, j, i) - AT(a.ph_1, k - 1, j, i) + AT(a.phb, k, j, i) - AT(a.phb, k - 1, j, i));
+  };
+  // phase 1
+  for (int k = grp; k < nz; k += TG) {
+    if (!vi) continue;
+    long q = D.ix(k, j, i);
+    if (!h) {
       a.t_2ave[q] = a.t_2ave_in[q];
       a.w[q] = a.w_in[q];
       a.ph[q] = a.ph_in[q];
+      continue;
     }
-    return;
-  }
-  const float G = F(pw::g);
-  float mut = a.mut[q2], muts = a.muts[q2], muave = a.muave[q2], my = a.msfty[q2], mx = a.msftx[q2];
-  float mi = 1.0f / my;
-  // t_2ave on [0, k1)
-  for (int k = 0; k < D.nz; ++k) {
-    long q = D.ix(k, j, i);
     float t = a.t_2ave_in[q];
     if (k < k1) {
       t = 0.5f * (a.ep1 * a.t_2[q] + a.em1 * t);
       t = (t + muave * a.t0) / (muts * (a.t0 + a.t_1[q]));
     }
     a.t_2ave[q] = t;
+    s_t2a[k * TI + lane] = t;
+    float r = 0.0f;
+    if (k >= 1 && k <= k1) {
+      r = a.dts * (a.ph_tend[q] + a.hgem1 * a.w_in[q]);
+      if (k < k1) r = r - a.dts * (a.fnm[k] * wdwn(k + 1) + a.fnp[k] * wdwn(k));
+      r = a.ph_in[q] + my * r / mut;
+    }
+    s_rhs[k * TI + lane] = r;
+    s_a[k * TI + lane] = a.a[q];
+    s_al[k * TI + lane] = a.alpha[q];
+    s_ga[k * TI + lane] = a.gamma[q];
   }
-  // rhs on [1, k1] held in the ph output; ph[0] copies through
-  auto wdwn = [&](int k) {
-    return 0.5f * (AT(a.ww, k, j, i) + AT(a.ww, k - 1, j, i)) * a.rdnw[k - 1] *
-           (AT(a.ph_1, k, j, i) - AT(a.ph_1, k - 1, j, i) + AT(a.phb, k, j, i) - AT(a.phb, k - 1, j, i));
-  };
-  a.ph[D.ix(0, j, i)] = a.ph_in[D.ix(0, j, i)];
-  for (int k = 1; k <= k1; ++k) {
-    long q = D.ix(k, j, i);
-    float r = a.dts * (a.ph_tend[q] + a.hgem1 * a.w_in[q]);
-    if (k < k1) r = r - a.dts * (a.fnm[k] * wdwn(k + 1) + a.fnp[k] * wdwn(k));
-    a.ph[q] = a.ph_in[q] + my * r / mut;
-  }
-  for (int k = k1 + 1; k < D.nz; ++k) a.ph[D.ix(k, j, i)] = a.ph_in[D.ix(k, j, i)];
-  auto RHS = [&](int k) -> float { return k >= 1 ? AT(a.ph, k, j, i) : 0.0f; };
+  __syncthreads();
+  auto RHS = [&](int k) -> float { return s_rhs[k * TI + lane]; };   // rhs[0] is 0
   auto PH = [&](int k) { return AT(a.ph_in, k, j, i); };
-  // surface w from the terrain slope
-  {
-    float cf1 = a.cf[0], cf2 = a.cf[1], cf3 = a.cf[2];
-    auto V3 = [&](int jj, int ii) {
-      return cf1 * AT(a.v, 0, jj, ii) + cf2 * AT(a.v, 1, jj, ii) + cf3 * AT(a.v, 2, jj, ii);
-    };
-    auto U3 = [&](int jj, int ii) {
-      return cf1 * AT(a.u, 0, jj, ii) + cf2 * AT(a.u, 1, jj, ii) + cf3 * AT(a.u, 2, jj, ii);
-    };
-    float htc = AT2(a.ht, j, i);
-    float ty = (AT2(a.ht, j + 1, i) - htc) * V3(j + 1, i) + (htc - AT2(a.ht, j - 1, i)) * V3(j, i);
-    float tx = (AT2(a.ht, j, i + 1) - htc) * U3(j, i + 1) + (htc - AT2(a.ht, j, i - 1)) * U3(j, i);
-    a.w[D.ix(0, j, i)] = my * 0.5f * a.rdy * ty + mx * 0.5f * a.rdx * tx;
-  }
-  // explicit predictor on [1, k1)
-  for (int k = 1; k < k1; ++k) {
+  // phase 2
+  for (int k = grp; k < nz; k += TG) {
+    if (!h) continue;
     long q = D.ix(k, j, i);
-    float c2 = a.c2a[q], c2m = AT(a.c2a, k - 1, j, i);
-    float X = a.hdtsg * a.rdn[k] *
-              (c2 * a.rdnw[k] / mut * (a.ep1 * (RHS(k + 1) - RHS(k)) + a.em1 * (PH(k + 1) - PH(k))) -
-               c2m * a.rdnw[k - 1] / mut * (a.ep1 * (RHS(k) - RHS(k - 1)) + a.em1 * (PH(k) - PH(k - 1))));
-    float Y = a.rdn[k] * (c2 * a.alt[q] * a.t_2ave[q] - c2m * AT(a.alt, k - 1, j, i) * AT(a.t_2ave, k - 1, j, i)) - muave;
-    a.w[q] = a.w_in[q] + a.dts * a.rw_tend[q] + mi * a.cqw[q] * X + a.dtsg * mi * Y;
+    float w;
+    if (k == 0) {
+      float cf1 = a.cf[0], cf2 = a.cf[1], cf3 = a.cf[2];
+      auto V3 = [&](int jj, int ii) {
+        return cf1 * AT(a.v, 0, jj, ii) + cf2 * AT(a.v, 1, jj, ii) + cf3 * AT(a.v, 2, jj, ii);
+      };
+      auto U3 = [&](int jj, int ii) {
+        return cf1 * AT(a.u, 0, jj, ii) + cf2 * AT(a.u, 1, jj, ii) + cf3 * AT(a.u, 2, jj, ii);
+      };
+      float htc = AT2(a.ht, j, i);
+      float ty = (AT2(a.ht, j + 1, i) - htc) * V3(j + 1, i) + (htc - AT2(a.ht, j - 1, i)) * V3(j, i);
+      float tx = (AT2(a.ht, j, i + 1) - htc) * U3(j, i + 1) + (htc - AT2(a.ht, j, i - 1)) * U3(j, i);
+      w = my * 0.5f * a.rdy * ty + mx * 0.5f * a.rdx * tx;
+    } else if (k < k1) {
+      float c2 = a.c2a[q], c2m = AT(a.c2a, k - 1, j, i);
+      float X = a.hdtsg * a.rdn[k] *
+                (c2 * a.rdnw[k] / mut * (a.ep1 * (RHS(k + 1) - RHS(k)) + a.em1 * (PH(k + 1) - PH(k))) -
+                 c2m * a.rdnw[k - 1] / mut * (a.ep1 * (RHS(k) - RHS(k - 1)) + a.em1 * (PH(k) - PH(k - 1))));
+      float Y = a.rdn[k] * (c2 * a.alt[q] * s_t2a[k * TI + lane] -
+                            c2m * AT(a.alt, k - 1, j, i) * s_t2a[(k - 1) * TI + lane]) - muave;
+      w = a.w_in[q] + a.dts * a.rw_tend[q] + mi * a.cqw[q] * X + a.dtsg * mi * Y;
+    } else if (k == k1) {
+      float rn = a.rdnw[k1 - 1];
+      float c2m = AT(a.c2a, k1 - 1, j, i);
+      float Z = a.mhdtsg / mut * (rn * rn) * 2.0f * c2m *
+                    (a.ep1 * (RHS(k1) - RHS(k1 - 1)) + a.em1 * (PH(k1) - PH(k1 - 1))) -
+                a.dtsg * (2.0f * rn * c2m * AT(a.alt, k1 - 1, j, i) * s_t2a[(k1 - 1) * TI + lane] + muave);
+      w = a.w_in[q] + a.dts * a.rw_tend[q] + mi * Z;
+    } else {
+      w = a.w_in[q];
+    }
+    s_w[k * TI + lane] = w;
   }
-  {
-    long q = D.ix(k1, j, i);
-    float rn = a.rdnw[k1 - 1];
-    float c2m = AT(a.c2a, k1 - 1, j, i);
-    float Z = a.mhdtsg / mut * (rn * rn) * 2.0f * c2m *
-                  (a.ep1 * (RHS(k1) - RHS(k1 - 1)) + a.em1 * (PH(k1) - PH(k1 - 1))) -
-              a.dtsg * (2.0f * rn * c2m * AT(a.alt, k1 - 1, j, i) * AT(a.t_2ave, k1 - 1, j, i) + muave);
-    a.w[q] = a.w_in[q] + a.dts * a.rw_tend[q] + mi * Z;
+  __syncthreads();
+  // phase 3
+  if (grp == 0 && h) {
+    float wm = s_w[lane];
+    for (int k = 1; k <= k1; ++k) {
+      wm = (s_w[k * TI + lane] - s_a[k * TI + lane] * wm) * s_al[k * TI + lane];
+      s_w[k * TI + lane] = wm;
+    }
+    float wp = wm;
+    for (int k = k1 - 1; k >= 1; --k) {
+      wp = s_w[k * TI + lane] - s_ga[k * TI + lane] * wp;
+      s_w[k * TI + lane] = wp;
+    }
   }
-  for (int k = k1 + 1; k < D.nz; ++k) a.w[D.ix(k, j, i)] = a.w_in[D.ix(k, j, i)];
-  // Thomas forward sweep and back substitution
-  float wm = a.w[D.ix(0, j, i)];
-  for (int k = 1; k <= k1; ++k) {
-    long q = D.ix(k, j, i);
-    wm = (a.w[q] - a.a[q] * wm) * a.alpha[q];
-    a.w[q] = wm;
-  }
-  float wp = wm;
-  for (int k = k1 - 1; k >= 1; --k) {
-    long q = D.ix(k, j, i);
-    wp = a.w[q] - a.gamma[q] * wp;
-    a.w[q] = wp;
-  }
-  // Rayleigh damping in the top hdepth metres, then phi
+  __syncthreads();
+  if (!h) return;
+  // phase 4
   float htop = (AT(a.ph_1, k1, j, i) + AT(a.phb, k1, j, i)) / G;
   float hbot = htop - a.hdepth;
-  for (int k = 1; k <= k1; ++k) {
+  for (int k = grp; k < nz; k += TG) {
     long q = D.ix(k, j, i);
-    float hk = (a.ph_1[q] + a.phb[q]) / G;
-    float dw = 0.0f;
-    if (hk >= hbot) {
-      float sn = sinf(a.hpi * (hk - hbot) / a.hdepth);
-      dw = a.dampmag * (sn * sn);
+    float w = s_w[k * TI + lane];
+    if (k == 0) {
+      a.w[q] = w;
+      a.ph[q] = a.ph_in[q];
+    } else if (k <= k1) {
+      float hk = (a.ph_1[q] + a.phb[q]) / G;
+      float dw = 0.0f;
+      if (hk >= hbot) {
+        float sn = sinf(a.hpi * (hk - hbot) / a.hdepth);
+        dw = a.dampmag * (sn * sn);
+      }
+      w = (w - dw * mut * a.w_save[q]) / (1.0f + dw);
+      a.w[q] = w;
+      a.ph[q] = s_rhs[k * TI + lane] + my * 0.5f * a.dts * G * a.ep1 * w / muts;
+    } else {
+      a.w[q] = w;
+      a.ph[q] = a.ph_in[q];
     }
-    float w = (a.w[q] - dw * mut * a.w_save[q]) / (1.0f + dw);
-    a.w[q] = w;
-    a.ph[q] = a.ph[q] + my * 0.5f * a.dts * G * a.ep1 * w / muts;
   }
 }
 
@@ -547,11 +607,13 @@ extern "C" int pywrf_advance_w(const pywrf_grid_t* g, const float* w_in, const f
             F(dts), F(t0), F(1. + epssm), F(1. - epssm), F(0.5 * G * (1. - epssm)), F(rdx), F(rdy),
             F(0.5 * dts * G), F(dts * G), F(-0.5 * dts * G), F(dts * 0.2), F(0.5 * pi_), F(5000.), g->kte - 1,
             imax(g->jts, g->jds + 1), imin(g->jte, g->jde - 2), imax(g->its, g->ids + 1), imin(g->ite, g->ide - 2)};
-  PW_LAUNCH_COLS(k_advance_w, plane(g), s, dims_of(g), p);
+  if (column_block_launch_check(g, 6)) return PYWRF_E_RANGE;
+  dim3 grid((g->nx + TI - 1) / TI, g->ny);
+  hipLaunchKernelGGL(k_advance_w, grid, dim3(TB), 6 * g->nz * TI * sizeof(float), (hipStream_t)s, dims_of(g), p);
   PW_RET();
 }
 
-// ------------------------------------------------------------------ sumflux (A42)
+// ------------------------------------------------------------------ sumflux
 __global__ void k_sumflux(Box b, Dims D, const float* ru, const float* rv, const float* ww, const float* u_lin,
                           const float* v_lin, const float* ww_lin, const float* muu, const float* muv, float* ru_m,
                           float* rv_m, float* ww_m, const float* msfuy, const float* msfvx_inv, Box bu, Box bv,
@@ -595,7 +657,7 @@ extern "C" int pywrf_sumflux(const pywrf_grid_t* g, const float* ru, const float
   PW_RET();
 }
 
-// ------------------------------------------------------------------ calc_mu_uv_1 (A45)
+// ------------------------------------------------------------------ calc_mu_uv_1
 __global__ void k_calc_mu_uv_1(Box b, Dims D, const float* mu, float* muu, float* muv, int jts, int jtf, int jte,
                                int its, int itf, int ite) {
   int j, i;
@@ -620,7 +682,7 @@ extern "C" int pywrf_calc_mu_uv_1(const pywrf_grid_t* g, const float* mu, float*
   PW_RET();
 }
 
-// ------------------------------------------------------------------ small_step_finish (A46)
+// ------------------------------------------------------------------ small_step_finish
 struct SsFin {
   const float *u_2, *v_2, *w_2, *t_2, *ph_2, *ww1, *mut, *muts, *muu, *muus, *muv, *muvs;
   const float *u_save, *v_save, *w_save, *t_save, *ph_save, *mu_save, *msfuy, *msfvx, *msfty, *h_diabatic;

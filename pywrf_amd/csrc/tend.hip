@@ -5,6 +5,7 @@
 // output point and applies the reference's statements for that point in order.
 #include "common.h"
 #include "consts.h"
+#include "diff.h"
 
 #define AT(p, k, j, i) (p)[D.ix((k), (j), (i))]
 #define AT2(p, j, i) (p)[D.ix2((j), (i))]
@@ -12,7 +13,7 @@
 int pw_check_grid(const pywrf_grid_t* g, int r);
 #define CHECK(g, r) do { int _e = pw_check_grid((g), (r)); if (_e) return _e; } while (0)
 
-// ------------------------------------------------------------------ rhs_ph (A26)
+// ------------------------------------------------------------------ rhs_ph
 struct RhsPh {
   const float *u, *v, *ww, *ph, *phb, *w, *mut, *muu, *muv, *fnm, *fnp, *rdnw, *cfn, *cfn1;
   const float *msfux, *msfvy, *msfty;
@@ -101,7 +102,7 @@ extern "C" int pywrf_rhs_ph(const pywrf_grid_t* g, float* ph_tend, const float* 
   PW_RET();
 }
 
-// ------------------------------------------------------------------ horizontal_pressure_gradient (A27)
+// ------------------------------------------------------------------ horizontal_pressure_gradient
 struct Hpg {
   const float *ph, *alt, *p, *pb, *al, *php, *cqu, *cqv, *muu, *muv, *mu, *fnm, *fnp, *rdnw, *cf;
   const float *msfux, *msfuy, *msfvx, *msfvy;
@@ -167,7 +168,7 @@ extern "C" int pywrf_horizontal_pressure_gradient(const pywrf_grid_t* g, float* 
   PW_RET();
 }
 
-// ------------------------------------------------------------------ pg_buoy_w (A28)
+// ------------------------------------------------------------------ pg_buoy_w
 __global__ void k_pg_buoy_w(Box b, Dims D, float* rw_tend, const float* p, float* cqw, const float* muf,
                             const float* mubf, const float* rdnw, const float* rdn, float g_, const float* msfty,
                             int kde) {
@@ -200,7 +201,7 @@ extern "C" int pywrf_pg_buoy_w(const pywrf_grid_t* g, float* rw_tend, const floa
   PW_RET();
 }
 
-// ------------------------------------------------------------------ coriolis (A29)
+// ------------------------------------------------------------------ coriolis
 struct Cor {
   const float *ru, *rv, *rw, *msftx, *msfty, *msfux, *msfuy, *msfvx, *msfvy, *f, *e, *sina, *cosa, *fzm, *fzp;
   float *ru_tend, *rv_tend, *rw_tend;
@@ -253,7 +254,7 @@ extern "C" int pywrf_coriolis(const pywrf_grid_t* g, const float* ru, const floa
   PW_RET();
 }
 
-// ------------------------------------------------------------------ curvature (A30)
+// ------------------------------------------------------------------ curvature
 struct Curv {
   const float *ru, *rv, *rw, *u, *v, *msfuy, *msfvx, *msfvy, *msftx, *msfty, *fzm, *fzp;
   float *ru_tend, *rv_tend, *rw_tend;
@@ -320,92 +321,12 @@ extern "C" int pywrf_curvature(const pywrf_grid_t* g, const float* ru, const flo
   PW_RET();
 }
 
-// ------------------------------------------------------------------ horizontal_diffusion (A16, A17)
-struct HDiff {
-  const float *f, *base, *mut, *xk, *msfux, *msfuy, *msfvx, *msfvx_inv, *msfvy, *msftx, *msfty;
-  float* tend;
-  float rdx, rdy;
-  char name;
-};
-
-struct HCoef {
-  float mkxm, mkxp, mrdx, mkym, mkyp, mrdy;
-};
-
-__device__ inline HCoef hdiff_coefs(const Dims& D, const HDiff& a, int k, int j, int i) {
-  HCoef c;
-  auto M = [&](const float* f, int dj, int di) { return AT2(f, j + dj, i + di); };
-  auto X = [&](int dk, int dj, int di) { return AT(a.xk, k + dk, j + dj, i + di); };
-  const float rdx = a.rdx, rdy = a.rdy;
-  if (a.name == 'u') {
-    c.mkxm = (M(a.msftx, 0, -1) / M(a.msfty, 0, -1)) * M(a.mut, 0, -1) * X(0, 0, -1) * rdx;
-    c.mkxp = (M(a.msftx, 0, 0) / M(a.msfty, 0, 0)) * M(a.mut, 0, 0) * X(0, 0, 0) * rdx;
-    c.mrdx = M(a.msfux, 0, 0) * M(a.msfuy, 0, 0) * rdx;
-    c.mkym = ((M(a.msfuy, 0, 0) + M(a.msfuy, -1, 0)) / (M(a.msfux, 0, 0) + M(a.msfux, -1, 0))) * 0.25f *
-             (M(a.mut, 0, 0) + M(a.mut, -1, 0) + M(a.mut, -1, -1) + M(a.mut, 0, -1)) * 0.25f *
-             (X(0, 0, 0) + X(0, -1, 0) + X(0, -1, -1) + X(0, 0, -1)) * rdy;
-    c.mkyp = ((M(a.msfuy, 0, 0) + M(a.msfuy, 1, 0)) / (M(a.msfux, 0, 0) + M(a.msfux, 1, 0))) * 0.25f *
-             (M(a.mut, 0, 0) + M(a.mut, 1, 0) + M(a.mut, 1, -1) + M(a.mut, 0, -1)) * 0.25f *
-             (X(0, 0, 0) + X(0, 1, 0) + X(0, 1, -1) + X(0, 0, -1)) * rdy;
-    c.mrdy = M(a.msfux, 0, 0) * M(a.msfuy, 0, 0) * rdy;
-  } else if (a.name == 'v') {
-    c.mkxm = ((M(a.msfvx, 0, 0) + M(a.msfvx, 0, -1)) / (M(a.msfvy, 0, 0) + M(a.msfvy, 0, -1))) * 0.25f *
-             (M(a.mut, 0, 0) + M(a.mut, -1, 0) + M(a.mut, -1, -1) + M(a.mut, 0, -1)) * 0.25f *
-             (X(0, 0, 0) + X(0, -1, 0) + X(0, -1, -1) + X(0, 0, -1)) * rdx;
-    c.mkxp = ((M(a.msfvx, 0, 0) + M(a.msfvx, 0, 1)) / (M(a.msfvy, 0, 0) + M(a.msfvy, 0, 1))) * 0.25f *
-             (M(a.mut, 0, 0) + M(a.mut, -1, 0) + M(a.mut, -1, 1) + M(a.mut, 0, 1)) * 0.25f *
-             (X(0, 0, 0) + X(0, -1, 0) + X(0, -1, 1) + X(0, 0, 1)) * rdx;
-    c.mrdx = M(a.msfvx, 0, 0) * M(a.msfvy, 0, 0) * rdx;
-    c.mkym = (M(a.msfty, -1, 0) / M(a.msftx, -1, 0)) * X(0, -1, 0) * rdy;
-    c.mkyp = (M(a.msfty, 0, 0) / M(a.msftx, 0, 0)) * X(0, 0, 0) * rdy;
-    c.mrdy = M(a.msfvx, 0, 0) * M(a.msfvy, 0, 0) * rdy;
-  } else if (a.name == 'w') {
-    c.mkxm = (M(a.msfux, 0, 0) / M(a.msfuy, 0, 0)) * 0.25f *
-             (M(a.mut, 0, 0) + M(a.mut, 0, -1) + M(a.mut, 0, 0) + M(a.mut, 0, -1)) * 0.25f *
-             (X(0, 0, 0) + X(0, 0, -1) + X(-1, 0, 0) + X(-1, 0, -1)) * rdx;
-    c.mkxp = (M(a.msfux, 0, 1) / M(a.msfuy, 0, 1)) * 0.25f *
-             (M(a.mut, 0, 1) + M(a.mut, 0, 0) + M(a.mut, 0, 1) + M(a.mut, 0, 0)) * 0.25f *
-             (X(0, 0, 1) + X(0, 0, 0) + X(-1, 0, 1) + X(-1, 0, 0)) * rdx;
-    c.mrdx = M(a.msftx, 0, 0) * M(a.msfty, 0, 0) * rdx;
-    c.mkym = (M(a.msfvy, 0, 0) / M(a.msfvx, 0, 0)) * 0.25f *
-             (M(a.mut, 0, 0) + M(a.mut, -1, 0) + M(a.mut, 0, 0) + M(a.mut, -1, 0)) * 0.25f *
-             (X(0, 0, 0) + X(0, -1, 0) + X(-1, 0, 0) + X(-1, -1, 0)) * rdy;
-    c.mkyp = (M(a.msfvy, 1, 0) / M(a.msfvx, 1, 0)) * 0.25f *
-             (M(a.mut, 1, 0) + M(a.mut, 0, 0) + M(a.mut, 1, 0) + M(a.mut, 0, 0)) * 0.25f *
-             (X(0, 1, 0) + X(0, 0, 0) + X(-1, 1, 0) + X(-1, 0, 0)) * rdy;
-    c.mrdy = M(a.msftx, 0, 0) * M(a.msfty, 0, 0) * rdy;
-  } else {
-    c.mkxm = (M(a.msfux, 0, 0) / M(a.msfuy, 0, 0)) * 0.5f * (X(0, 0, 0) + X(0, 0, -1)) * 0.5f *
-             (M(a.mut, 0, 0) + M(a.mut, 0, -1)) * rdx;
-    c.mkxp = (M(a.msfux, 0, 1) / M(a.msfuy, 0, 1)) * 0.5f * (X(0, 0, 1) + X(0, 0, 0)) * 0.5f *
-             (M(a.mut, 0, 1) + M(a.mut, 0, 0)) * rdx;
-    c.mrdx = M(a.msftx, 0, 0) * M(a.msfty, 0, 0) * rdx;
-    c.mkym = (M(a.msfvy, 0, 0) * M(a.msfvx_inv, 0, 0)) * 0.5f * (X(0, 0, 0) + X(0, -1, 0)) * 0.5f *
-             (M(a.mut, 0, 0) + M(a.mut, -1, 0)) * rdy;
-    c.mkyp = (M(a.msfvy, 1, 0) * M(a.msfvx_inv, 1, 0)) * 0.5f * (X(0, 1, 0) + X(0, 0, 0)) * 0.5f *
-             (M(a.mut, 1, 0) + M(a.mut, 0, 0)) * rdy;
-    c.mrdy = M(a.msftx, 0, 0) * M(a.msfty, 0, 0) * rdy;
-  }
-  return c;
-}
-
+// ------------------------------------------------------------------ horizontal_diffusion
 __global__ void k_hdiff(Box b, Dims D, HDiff a) {
   int k, j, i;
   if (!box_point(b, k, j, i)) return;
   long q = D.ix(k, j, i);
-  HCoef c = hdiff_coefs(D, a, k, j, i);
-  auto Fv = [&](int dj, int di) { return AT(a.f, k, j + dj, i + di); };
-  if (!a.base) {
-    a.tend[q] = a.tend[q] + (c.mrdx * (c.mkxp * (Fv(0, 1) - Fv(0, 0)) - c.mkxm * (Fv(0, 0) - Fv(0, -1))) +
-                             c.mrdy * (c.mkyp * (Fv(1, 0) - Fv(0, 0)) - c.mkym * (Fv(0, 0) - Fv(-1, 0))));
-  } else {
-    auto S = [&](int dj, int di) { return AT(a.base, k, j + dj, i + di); };
-    a.tend[q] = a.tend[q] +
-                (c.mrdx * (c.mkxp * (Fv(0, 1) - Fv(0, 0) - S(0, 1) + S(0, 0)) -
-                           c.mkxm * (Fv(0, 0) - Fv(0, -1) - S(0, 0) + S(0, -1))) +
-                 c.mrdy * (c.mkyp * (Fv(1, 0) - Fv(0, 0) - S(1, 0) + S(0, 0)) -
-                           c.mkym * (Fv(0, 0) - Fv(-1, 0) - S(0, 0) + S(-1, 0))));
-  }
+  a.tend[q] = hdiff_apply(D, a, k, j, i, a.tend[q]);
 }
 
 int pw_horizontal_diffusion(const pywrf_grid_t* g, char name, const float* field, float* tendency,
@@ -444,68 +365,15 @@ extern "C" int pywrf_horizontal_diffusion_3dmp(const pywrf_grid_t* g, const floa
                                  msftx, msfty, xkmhd, F(rdx), F(rdy), s);
 }
 
-// ------------------------------------------------------------------ vertical diffusion (A18-A20)
+// ------------------------------------------------------------------ vertical diffusion
 // kind: 'w' (vertical_diffusion 'w'), 'm' (vertical_diffusion 'm'), 'u'/'v'
 // (vertical_diffusion_u/_v, 1-D base), 'p' (vertical_diffusion_mp, 1-D base),
 // '3' (vertical_diffusion_3dmp, 3-D base).
-struct VDiff {
-  const float *f, *alt, *mus, *rdn, *rdnw, *base1, *base3;
-  float* tend;
-  float kvdif;
-  int ktf, kts;
-  char kind;
-};
-
-__device__ inline float vdiff_flux(const Dims& D, const VDiff& a, int k, int j, int i) {
-  auto A = [&](int dk, int dj, int di) { return AT(a.alt, k + dk, j + dj, i + di); };
-  auto Fv = [&](int dk) { return AT(a.f, k + dk, j, i); };
-  if (k < a.kts || k >= a.ktf - 1) return 0.0f;
-  switch (a.kind) {
-    case 'w':
-      return (a.kvdif / A(0, 0, 0)) * a.rdnw[k] * (Fv(1) - Fv(0));
-    case 'm':
-      if (k == a.kts) k = a.kts + 1;  // vflux[kts] = vflux[kts+1]
-      if (k >= a.ktf - 1) return 0.0f;
-      return a.kvdif * a.rdn[k + 1] / (0.5f * (AT(a.alt, k, j, i) + AT(a.alt, k + 1, j, i))) *
-             (AT(a.f, k + 1, j, i) - AT(a.f, k, j, i));
-    case 'u':
-    case 'v': {
-      int dj = a.kind == 'v' ? -1 : 0, di = a.kind == 'u' ? -1 : 0;
-      return a.kvdif * a.rdn[k + 1] / (0.25f * (A(0, 0, 0) + A(0, dj, di) + A(1, 0, 0) + A(1, dj, di))) *
-             (Fv(1) - Fv(0) - a.base1[k + 1] + a.base1[k]);
-    }
-    case 'p':
-      return a.kvdif * a.rdn[k + 1] / (0.5f * (A(0, 0, 0) + A(1, 0, 0))) *
-             (Fv(1) - Fv(0) - a.base1[k + 1] + a.base1[k]);
-    default:
-      return a.kvdif * a.rdn[k + 1] / (0.5f * (A(0, 0, 0) + A(1, 0, 0))) *
-             (Fv(1) - Fv(0) - AT(a.base3, k + 1, j, i) + AT(a.base3, k, j, i));
-  }
-}
-
 __global__ void k_vdiff(Box b, Dims D, VDiff a) {
   int k, j, i;
   if (!box_point(b, k, j, i)) return;
   long q = D.ix(k, j, i);
-  float dv = vdiff_flux(D, a, k, j, i) - vdiff_flux(D, a, k - 1, j, i);
-  const float gg = F(pw::g * pw::g), G = F(pw::g);
-  float m = AT2(a.mus, j, i);
-  switch (a.kind) {
-    case 'w':
-      a.tend[q] = a.tend[q] + a.rdn[k] * G * G / m / (0.5f * (a.alt[q] + AT(a.alt, k - 1, j, i))) * dv;
-      break;
-    case 'm':
-      a.tend[q] = a.tend[q] + gg / m / a.alt[q] * a.rdnw[k] * dv;
-      break;
-    case 'u':
-      a.tend[q] = a.tend[q] + gg * a.rdnw[k] / m / (0.5f * (AT(a.alt, k, j, i - 1) + a.alt[q])) * dv;
-      break;
-    case 'v':
-      a.tend[q] = a.tend[q] + gg * a.rdnw[k] / m / (0.5f * (AT(a.alt, k, j - 1, i) + a.alt[q])) * dv;
-      break;
-    default:
-      a.tend[q] = a.tend[q] + gg * m / a.alt[q] * a.rdnw[k] * dv;
-  }
+  a.tend[q] = vdiff_apply(D, a, k, j, i, a.tend[q]);
 }
 
 static int vdiff_launch(const pywrf_grid_t* g, char kind, const float* field, float* tendency, const float* base1,
@@ -555,7 +423,7 @@ extern "C" int pywrf_vertical_diffusion_3dmp(const pywrf_grid_t* g, const float*
   return vdiff_launch(g, '3', field, tendency, nullptr, base_3d, alt, mut, rdn, rdnw, kvdif, s);
 }
 
-// ------------------------------------------------------------------ rk_addtend_dry (A33)
+// ------------------------------------------------------------------ rk_addtend_dry
 struct AddT {
   float *ru_tend, *rv_tend, *rw_tend, *ph_tend, *t_tend, *ru_tendf, *rv_tendf, *rw_tendf, *ph_tendf, *t_tendf;
   const float *u_save, *v_save, *w_save, *ph_save, *t_save, *h_diabatic, *mut, *msfty, *msfuy, *msfvx,

@@ -634,7 +634,7 @@ extern "C" int pywrf_microphysics_driver(const pywrf_grid_t* g, float* th, const
                     graupelnc, graupelncv, s);
 }
 
-// ------------------------------------------------------------------ moist_physics_prep_em (P1)
+// ------------------------------------------------------------------ moist_physics_prep_em
 struct MpPrep {
   const float *t_new, *al, *alb, *p, *pb, *ph, *phb, *qv, *qc, *fzm, *fzp;
   float *rho, *th_phy, *pii, *pf, *z, *z_at_w, *dz8w, *p8w, *h_diabatic, *qv_diabatic, *qc_diabatic;
@@ -693,7 +693,7 @@ extern "C" int pywrf_moist_physics_prep_em(const pywrf_grid_t* g, const float* t
   PW_RET();
 }
 
-// ------------------------------------------------------------------ moist_physics_finish_em (P7)
+// ------------------------------------------------------------------ moist_physics_finish_em
 __global__ void k_mp_finish(Box b, Dims D, float* t_new, const float* th_phy, float* h_diabatic, const float* qv,
                             float* qv_diabatic, const float* qc, float* qc_diabatic, float dt) {
   int k, j, i;

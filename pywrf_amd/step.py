@@ -106,6 +106,15 @@ def rk3_step(s, ops, cfg, check_nan=False):
     b = s.bdy
     bc3 = lambda a, v: o.set_physical_bc3d(a, v, *I24)
     bc2 = lambda a, v: o.set_physical_bc2d(a, v, *I2)
+    grouped = hasattr(o, "set_physical_bc_group")
+
+    def bcs(*items):
+        """A run of consecutive open-boundary fills, items (tensor, flag, is_2d);
+        one batched launch when the implementation offers it."""
+        if grouped:
+            o.set_physical_bc_group(list(items), I24, I2)
+            return [t for t, _, _ in items]
+        return [bc2(t, v) if two else bc3(t, v) for t, v, two in items]
 
     s.step += 1
     if s.step % cfg.bdy_interval == 1:
@@ -133,18 +142,9 @@ def rk3_step(s, ops, cfg, check_nan=False):
         s.alt = o.calc_alt(s.alt, s.al, s.alb, *I18)
         s.php = o.calc_php(s.php, s.ph, s.phb, *I18)
         # rk_phys_bc_dry_1 (solver.py:781-843)
-        s.ru = bc3(s.ru, 'u')
-        s.rv = bc3(s.rv, 'v')
-        s.rw = bc3(s.rw, 'w')
-        s.ww = bc3(s.ww, 'w')
-        s.php = bc3(s.php, 'w')
-        s.alt = bc3(s.alt, 't')
-        s.p = bc3(s.p, 'p')
-        s.muu = bc2(s.muu, 'u')
-        s.muv = bc2(s.muv, 'v')
-        s.mut = bc2(s.mut, 't')
-        s.al = bc3(s.al, 'p')
-        s.ph = bc3(s.ph, 'w')
+        (s.ru, s.rv, s.rw, s.ww, s.php, s.alt, s.p, s.muu, s.muv, s.mut, s.al, s.ph) = bcs(
+            (s.ru, 'u', 0), (s.rv, 'v', 0), (s.rw, 'w', 0), (s.ww, 'w', 0), (s.php, 'w', 0), (s.alt, 't', 0),
+            (s.p, 'p', 0), (s.muu, 'u', 1), (s.muv, 'v', 1), (s.mut, 't', 1), (s.al, 'p', 0), (s.ph, 'w', 0))
         if rk_step == 1:
             # first_rk_step_part1/2 (solver.py:846-968)
             for a in (s.ru_tendf, s.rv_tendf, s.rw_tendf, s.ph_tendf, s.t_tendf, s.tke_tend, s.mu_tendf,
@@ -160,11 +160,8 @@ def rk3_step(s, ops, cfg, check_nan=False):
                 s.rqcblten, s.rqiblten, s.scalar, s.scalar_tend, N_SCALAR, *I18)
             s.rdzw, s.rdz, s.zx, s.zy, s.z = o.compute_diff_metrics(s.ph, s.phb, s.z, s.rdz, s.rdzw, s.zx, s.zy,
                                                                     rdx, rdy, *I18)
-            s.rdzw = bc3(s.rdzw, 'w')
-            s.rdz = bc3(s.rdz, 'w')
-            s.z = bc3(s.z, 'w')
-            s.zx = bc3(s.zx, 'e')
-            s.zy = bc3(s.zy, 'f')
+            s.rdzw, s.rdz, s.z, s.zx, s.zy = bcs((s.rdzw, 'w', 0), (s.rdz, 'w', 0), (s.z, 'w', 0), (s.zx, 'e', 0),
+                                                 (s.zy, 'f', 0))
             (s.defor11, s.defor12, s.defor13, s.defor22, s.defor23, s.defor33, s.div) = o.cal_deform_and_div(
                 s.u, s.v, s.w, s.div, s.defor11, s.defor22, s.defor33, s.defor12, s.defor13, s.defor23,
                 s.u_base, s.v_base, s.msfux, s.msfuy, s.msfvx, s.msfvy, s.msftx, s.msfty, rdx, rdy, s.dn, s.dnw,
@@ -174,10 +171,8 @@ def rk3_step(s, ops, cfg, check_nan=False):
                 s.defor11, s.defor22, s.defor33, s.defor12, s.defor13, s.defor23, s.tke, s.p8w, s.t8w,
                 s.th_phy, s.t_phy, s.p_phy, s.moist, s.dn, s.dnw, cfg.dx, cfg.dy, s.rdz, s.rdzw, 0, N_MOIST,
                 s.cf1, s.cf2, s.cf3, 0, 0.1, s.msftx, s.msfty, s.zx, s.zy, *I18)
-            s.rublten = bc3(s.rublten, 't')
-            s.rvblten = bc3(s.rvblten, 't')
-            s.xkmh = bc3(s.xkmh, 't')
-            s.xkhh = bc3(s.xkhh, 't')
+            s.rublten, s.rvblten, s.xkmh, s.xkhh = bcs((s.rublten, 't', 0), (s.rvblten, 't', 0), (s.xkmh, 't', 0),
+                                                       (s.xkhh, 't', 0))
             s.t_tendf, s.ru_tendf, s.rv_tendf, s.moist_tend = o.update_phy_ten(
                 s.ph_tendf, s.t_tendf, s.ru_tendf, s.rv_tendf, s.moist_tend, s.scalar_tend, s.mu_tendf,
                 s.rthblten, s.rublten, s.rvblten, s.rqvblten, s.rqcblten, s.rqiblten, N_MOIST, N_SCALAR,
@@ -272,16 +267,9 @@ def rk3_step(s, ops, cfg, check_nan=False):
                                               s.c4h, s.c3f, s.c4f, s.cqw, s.rdn, s.rdnw, s.c2a, dts_rk, G, EPSSM,
                                               False, *I18)
         # set_phys_bc2_tim (solver.py:1296-1345)
-        s.ru_tend = bc3(s.ru_tend, 'u')
-        s.rv_tend = bc3(s.rv_tend, 'v')
-        s.ph = bc3(s.ph, 'w')
-        s.al = bc3(s.al, 'p')
-        s.p = bc3(s.p, 'p')
-        s.t_1 = bc3(s.t_1, 't')
-        s.t_save = bc3(s.t_save, 't')
-        s.mu_1 = bc2(s.mu_1, 't')
-        s.mu = bc2(s.mu, 't')
-        s.mudf = bc2(s.mudf, 't')
+        (s.ru_tend, s.rv_tend, s.ph, s.al, s.p, s.t_1, s.t_save, s.mu_1, s.mu, s.mudf) = bcs(
+            (s.ru_tend, 'u', 0), (s.rv_tend, 'v', 0), (s.ph, 'w', 0), (s.al, 'p', 0), (s.p, 'p', 0), (s.t_1, 't', 0),
+            (s.t_save, 't', 0), (s.mu_1, 't', 1), (s.mu, 't', 1), (s.mudf, 't', 1))
         ix_2d = (ids, ide, jds, jde, 0, 0, ims, ime, jms, jme, 0, 0, ips, ipe, jps, jpe, 0, 0,
                  its, ite, jts, jte, 0, 0)
         # acoustic small steps (solver.py:1349-1493)
@@ -313,12 +301,8 @@ def rk3_step(s, ops, cfg, check_nan=False):
                                        SPEC_ZONE, *I24)
             s.w = o.zero_grad_bdy(s.w, 'w', SPEC_ZONE, *I24)
             s.al, s.p, s.pm1 = cpr(iteration)
-            s.ph = bc3(s.ph, 'w')
-            s.al = bc3(s.al, 'p')
-            s.p = bc3(s.p, 'p')
-            s.muts = bc2(s.muts, 't')
-            s.mu = bc2(s.mu, 't')
-            s.mudf = bc2(s.mudf, 't')
+            s.ph, s.al, s.p, s.muts, s.mu, s.mudf = bcs((s.ph, 'w', 0), (s.al, 'p', 0), (s.p, 'p', 0),
+                                                        (s.muts, 't', 1), (s.mu, 't', 1), (s.mudf, 't', 1))
         # back to full variables (solver.py:1496-1543)
         s.muus, s.muvs = o.calc_mu_uv_1(s.muts, s.muus, s.muvs, *I18)
         s.u, s.v, s.w, s.ph, s.ww, s.t, s.mu = o.small_step_finish(
@@ -327,11 +311,8 @@ def rk3_step(s, ops, cfg, check_nan=False):
             s.w_save, s.t_save, s.ph_save, s.mu_save, s.msfux, s.msfuy, s.msfvx, s.msfvy, s.msftx, s.msfty,
             s.h_diabatic, nsmall, dts_rk, rk_step, cfg.rk_order, *I18)
         if rk_step == cfg.rk_order:
-            s.ru_m = bc3(s.ru_m, 'u')
-            s.rv_m = bc3(s.rv_m, 'v')
-            s.ww_m = bc3(s.ww_m, 'w')
-            s.mut = bc2(s.mut, 't')
-            s.muts = bc2(s.muts, 't')
+            s.ru_m, s.rv_m, s.ww_m, s.mut, s.muts = bcs((s.ru_m, 'u', 0), (s.rv_m, 'v', 0), (s.ww_m, 'w', 0),
+                                                        (s.mut, 't', 1), (s.muts, 't', 1))
         # moist and scalar transport (solver.py:1548-1636)
         sct = (rdx, rdy, s.rdn, s.rdnw, KHDIF, KVDIF, s.xkhh, DIFF_6TH_OPT, DIFF_6TH_FACTOR)
         mf = (s.msfux, s.msfuy, s.msfvx, s.msfvx_inv, s.msfvy, s.msftx, s.msfty)
@@ -364,16 +345,10 @@ def rk3_step(s, ops, cfg, check_nan=False):
         if check_nan:
             _nan_check(s.p, "rk_step %d" % rk_step)
         if rk_step < cfg.rk_order:
-            s.u = bc3(s.u, 'U')
-            s.v = bc3(s.v, 'V')
-            s.w = bc3(s.w, 'w')
-            s.t = bc3(s.t, 'p')
-            s.ph = bc3(s.ph, 'w')
-            s.mu = bc2(s.mu, 't')
-            for im in range(1, 7):
-                s.moist[im, :, :, :] = bc3(s.moist[im, :, :, :], 'p')
-            for im in range(1, 3):
-                s.scalar[im, :, :, :] = bc3(s.scalar[im, :, :, :], 'p')
+            (s.u, s.v, s.w, s.t, s.ph, s.mu) = bcs(
+                (s.u, 'U', 0), (s.v, 'V', 0), (s.w, 'w', 0), (s.t, 'p', 0), (s.ph, 'w', 0), (s.mu, 't', 1),
+                *[(s.moist[im, :, :, :], 'p', 0) for im in range(1, 7)],
+                *[(s.scalar[im, :, :, :], 'p', 0) for im in range(1, 3)])[:6]
     # end of the RK loop: phy_prep_part2 (solver.py:1702-1709)
     (s.rublten, s.rvblten, s.rthblten, s.rqvblten, s.rqcblten, s.rqiblten) = o.phy_prep_part2(
         s.mut, s.muu, s.muv, s.c1h, s.c2h, s.c1f, s.c2f, s.rthblten, s.rublten, s.rvblten, s.rqvblten,
@@ -399,24 +374,12 @@ def rk3_step(s, ops, cfg, check_nan=False):
     if check_nan:
         _nan_check(s.p, "physics")
     # set_phys_bc_dry2 (solver.py:1776-1858)
-    s.u_1 = bc3(s.u_1, 'U')
-    s.u = bc3(s.u, 'U')
-    s.v_1 = bc3(s.v_1, 'V')
-    s.v = bc3(s.v, 'V')
-    s.w_1 = bc3(s.w_1, 'w')
-    s.w = bc3(s.w, 'w')
-    s.t_1 = bc3(s.t_1, 'p')
-    s.t = bc3(s.t, 'p')
-    s.ph_1 = bc3(s.ph_1, 'w')
-    s.ph = bc3(s.ph, 'w')
-    s.mu_1 = bc2(s.mu_1, 't')
-    s.mu = bc2(s.mu, 't')
-    s.tke_1 = bc3(s.tke_1, 'p')
-    s.tke = bc3(s.tke, 'p')
-    for im in range(1, 7):
-        s.moist[im, :, :, :] = bc3(s.moist[im, :, :, :], 'p')
-    for im in range(1, 3):
-        s.scalar[im, :, :, :] = bc3(s.scalar[im, :, :, :], 'p')
+    (s.u_1, s.u, s.v_1, s.v, s.w_1, s.w, s.t_1, s.t, s.ph_1, s.ph, s.mu_1, s.mu, s.tke_1, s.tke) = bcs(
+        (s.u_1, 'U', 0), (s.u, 'U', 0), (s.v_1, 'V', 0), (s.v, 'V', 0), (s.w_1, 'w', 0), (s.w, 'w', 0),
+        (s.t_1, 'p', 0), (s.t, 'p', 0), (s.ph_1, 'w', 0), (s.ph, 'w', 0), (s.mu_1, 't', 1), (s.mu, 't', 1),
+        (s.tke_1, 'p', 0), (s.tke, 'p', 0),
+        *[(s.moist[im, :, :, :], 'p', 0) for im in range(1, 7)],
+        *[(s.scalar[im, :, :, :], 'p', 0) for im in range(1, 3)])[:14]
     # spec_bdy_final and surface w (solver.py:1863-1910)
     sf = lambda fld, mu, msf, key, var: o.spec_bdy_final(fld, mu, s.c1h, s.c2h, msf, *b[key], var, SPEC_BDY_WIDTH,
                                                           SPEC_ZONE, dtbc, *I24)

@@ -814,3 +814,27 @@ def set_w_surface(znw, fill_w_flag,
     call("pywrf_set_w_surface", g, ptr(w), ptr(ht), ptr(u), ptr(v), cf, float(rdx), float(rdy), ptr(msftx),
          ptr(msfty), stream())
     return w
+
+
+# ----------------------------------------------------------------- batched open boundaries
+def set_physical_bc_group(items, idx24, idx2d):
+    """A run of consecutive set_physical_bc3d / set_physical_bc2d calls on
+    distinct fields, in one launch.  items: [(tensor, variable, is_2d)];
+    idx24 / idx2d: the index tuples the individual calls would take.  Same
+    result as calling the two functions in order (the fields are distinct, so
+    the fills are independent).  Not part of the reference API."""
+    import ctypes
+    n = len(items)
+    if n == 0:
+        return
+    if len({t.data_ptr() for t, _, _ in items}) != n:
+        raise ValueError("set_physical_bc_group: a field appears twice")
+    t3 = next((t for t, _, two in items if not two), None)
+    t2 = next((t for t, _, two in items if two), None)
+    shape3 = t3.shape if t3 is not None else (idx24[5],) + tuple(items[0][0].shape[-2:])
+    shape2 = t2.shape if t2 is not None else tuple(items[0][0].shape[-2:])
+    g3, g2 = grid(shape3, idx24), grid(shape2, idx2d)
+    ptrs = (ctypes.c_void_p * n)(*[ptr(t).value for t, _, _ in items])
+    flags = b"".join(v.encode() for _, v, _ in items)
+    kinds = (ctypes.c_int * n)(*[1 if two else 0 for _, _, two in items])
+    call("pywrf_set_physical_bc_group", g3, g2, n, ptrs, flags, kinds, stream())

