@@ -120,6 +120,9 @@ def main():
     ap.add_argument("--eager", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="run each measured kernel --kernel-reps times and a plain copy of one 3-D field "
+                         "(the reference for counter scaling), print nothing else; for rocprofv3 --pmc runs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (the domain is not sharded across GPUs); "
@@ -141,6 +144,17 @@ def main():
     _log("bench: eager warmup step")
     rk3_step(s, ops, cfg)
     torch.cuda.synchronize()
+    if args.kernels_only:
+        src = s.u.clone()
+        dst = torch.empty_like(src)
+        for name, _, nbytes, fn in kernel_candidates(s, cfg):
+            for _ in range(args.kernel_reps):
+                fn()
+        for _ in range(args.kernel_reps):
+            dst.copy_(src)                  # reads and writes one 3-D field: 9.45 MB each way at 240^2
+        torch.cuda.synchronize()
+        _log("bench: kernels-only run done (copy of %d bytes)" % (src.numel() * 4))
+        return
     if args.eager:
         step = lambda: rk3_step(s, ops, cfg)
     else:

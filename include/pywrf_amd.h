@@ -80,6 +80,11 @@ int pywrf_set_physical_bc3d(const pywrf_grid_t* g, float* dat, char variable, py
 int pywrf_set_physical_bc_group(const pywrf_grid_t* g3, const pywrf_grid_t* g2, int n, void* const* fields,
                                 const char* variables, const int* is2d, pywrf_stream_t s);
 
+/* Zero n distinct device arrays of counts[f] floats each in one launch: the
+ * runs of tensor.zero_() calls of the reference driver (solver.py:848-851,
+ * 975-981).  No counterpart in the reference API: a batching of zero_(). */
+int pywrf_zero_group(int n, void* const* fields, const long long* counts, pywrf_stream_t s);
+
 /* ---- first-stage physics prep and Smagorinsky: wrf_dynamics.py:314-1076 - */
 /* phy_prep, wrf_dynamics.py:314-411 (pi_phy_out, t_phy_out: fresh arrays) */
 int pywrf_phy_prep(const pywrf_grid_t* g, const float* mut, const float* u, const float* v, const float* p,

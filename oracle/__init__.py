@@ -9,7 +9,7 @@ as the checker; the product path (pywrf_amd) never imports it.
 
 Pinning: every function is checked against golden vectors written by driving
 the real reference (tests/golden/make_golden.py) and, where mathematics fixes
-the answer, against closed forms (tests/test_oracle_math.py).
+the answer, against closed forms (tests/test_oracle_invariants.py).
 """
 from types import SimpleNamespace
 

@@ -40,6 +40,7 @@ _CTYPES = {
     "double*": ctypes.POINTER(ctypes.c_double),
     "void* const*": ctypes.POINTER(ctypes.c_void_p),
     "const int*": ctypes.POINTER(ctypes.c_int),
+    "const long long*": ctypes.POINTER(ctypes.c_longlong),
 }
 
 

@@ -370,7 +370,7 @@ struct ScalarTend {
   Box bzero, badv, by, bx, bh, bv;
 };
 
-__global__ void k_scalar_tend(Box all, Dims D, ScalarTend a) {
+__global__ void k_scalar_adv(Box all, Dims D, ScalarTend a) {
   int k, j, i;
   if (!box_point(all, k, j, i)) return;
   long q = D.ix(k, j, i);
@@ -378,7 +378,7 @@ __global__ void k_scalar_tend(Box all, Dims D, ScalarTend a) {
   a.z_t[q] = 0.0f;
   if (!a.bzero.has(k, j, i)) return;
   const bool adv = a.badv.has(k, j, i), ay = a.by.has(k, j, i), ax = a.bx.has(k, j, i);
-  const bool dh = a.rk1 && a.bh.has(k, j, i), dv = a.rk1 && a.bv.has(k, j, i);
+#pragma unroll 1
   for (int im = a.scs; im <= a.sce; ++im) {
     AdvS ad = a.adv;
     ad.f = a.scalar + im * a.n3;
@@ -390,21 +390,33 @@ __global__ void k_scalar_tend(Box all, Dims D, ScalarTend a) {
       t = t - ad.rdzw[k] * (s_zface(D, ad, a.x, k + 1, j, i) - s_zface(D, ad, a.x, k, j, i));
     }
     a.adv_tend[im * a.n3 + q] = t;
-    if (a.rk1) {
-      float st = a.sc_tend[im * a.n3 + q];
-      if (dh) {
-        HDiff hd = a.hd;
-        hd.f = ad.f;
-        st = hdiff_apply(D, hd, k, j, i, st);
-      }
-      if (dv) {
-        VDiff vd = a.vd;
-        vd.f = ad.f;
-        vd.kind = (a.moist_step && im == pw::P_QV) ? 'p' : 'm';
-        st = vdiff_apply(D, vd, k, j, i, st);
-      }
-      a.sc_tend[im * a.n3 + q] = st;
+  }
+}
+
+// First RK stage only: horizontal ('m') and vertical diffusion of every
+// species into scalar_tends (wrf_dynamics.py:4018-4046).
+__global__ void __launch_bounds__(PW_TX * PW_TY) k_scalar_diff(Box all, Dims D, ScalarTend a) {
+  int k, j, i;
+  if (!box_point(all, k, j, i)) return;
+  long q = D.ix(k, j, i);
+  const bool dh = a.bh.has(k, j, i), dv = a.bv.has(k, j, i);
+  if (!dh && !dv) return;
+#pragma unroll 1
+  for (int im = a.scs; im <= a.sce; ++im) {
+    const float* f = a.scalar + im * a.n3;
+    float st = a.sc_tend[im * a.n3 + q];
+    if (dh) {
+      HDiff hd = a.hd;
+      hd.f = f;
+      st = hdiff_apply(D, hd, k, j, i, st);
     }
+    if (dv) {
+      VDiff vd = a.vd;
+      vd.f = f;
+      vd.kind = (a.moist_step && im == pw::P_QV) ? 'p' : 'm';
+      st = vdiff_apply(D, vd, k, j, i, st);
+    }
+    a.sc_tend[im * a.n3 + q] = st;
   }
 }
 
@@ -434,6 +446,7 @@ int pw_rk_scalar_tend_fused(const pywrf_grid_t* g, int scs, int sce, int rk_step
                imin(g->ide - 2, g->ite));
   a.bv = mkbox(g->kts + 1, ktf, g->jts, jtf, g->its, itf);
   Box all = mkbox(0, g->nz, 0, g->ny, 0, g->nx);
-  PW_LAUNCH_BOX(k_scalar_tend, all, s, dims_of(g), a);
+  PW_LAUNCH_BOX(k_scalar_adv, all, s, dims_of(g), a);
+  if (a.rk1) PW_LAUNCH_BOX(k_scalar_diff, a.bzero, s, dims_of(g), a);
   PW_RET();
 }

@@ -51,9 +51,9 @@ __device__ inline float relax_field(const Dims& D, const Relax& a, int k, int j,
   return a.mbox.has(k, j, i) ? AT(a.field, k, j, i) * AT2(a.mut, j, i) : 0.0f;
 }
 
-__global__ void k_relax(Box all, Dims D, Relax a) {
+__global__ void k_relax(Box all, int w, Dims D, Relax a) {
   int k, j, i;
-  if (!box_point(all, k, j, i)) return;
+  if (!edge_point(all, w, k, j, i)) return;
   // which strip (y-start, y-end, x-start, x-end) and b_dist owns this point
   int side = -1, bd = 0;
   for (int b = 1; b <= 3 && side < 0; ++b) {
@@ -94,7 +94,7 @@ static int relax_launch(const pywrf_grid_t* g, Relax& a, char v, pywrf_stream_t 
   a.ibs = g->ids; a.jbs = g->jds; a.ibe = ibe; a.jbe = jbe;
   if (g->ids < 0 || ibe + 2 > g->nx || g->jds < 0 || jbe + 2 > g->ny || ktf > g->nz) return PYWRF_E_RANGE;
   Box all = mkbox(g->kts, ktf, g->jds, jbe, g->ids, ibe);
-  PW_LAUNCH_BOX(k_relax, all, s, dims_of(g), a);
+  pw_launch_edges(k_relax, all, 4, s, dims_of(g), a);  // strips b_dist = 1..3 lie in a band of 4
   PW_RET();
 }
 
@@ -124,9 +124,9 @@ extern "C" int pywrf_relax_bdy_scalar(const pywrf_grid_t* g, float* scalar_tend,
 }
 
 // ------------------------------------------------------------------ spec_bdytend
-__global__ void k_spec_bdytend(Box all, Dims D, float* tend, Bdy b, int ibs, int ibe, int jbs, int jbe) {
+__global__ void k_spec_bdytend(Box all, int w, Dims D, float* tend, Bdy b, int ibs, int ibe, int jbs, int jbe) {
   int k, j, i;
-  if (!box_point(all, k, j, i)) return;
+  if (!edge_point(all, w, k, j, i)) return;
   float v;
   bool xr = j >= jbs && j < jbe;
   if (xr && i == ibe - 1) v = bx(D, b.txe, k, j, 0);
@@ -144,7 +144,7 @@ extern "C" int pywrf_spec_bdytend(const pywrf_grid_t* g, float* field_tend, cons
   if (g->ids < 0 || ibe > g->nx || g->jds < 0 || jbe > g->ny || ktf > g->nz) return PYWRF_E_RANGE;
   Bdy b = {nullptr, nullptr, nullptr, nullptr, btxs, btxe, btys, btye};
   Box all = mkbox(g->kts, ktf, g->jds, jbe, g->ids, ibe);
-  PW_LAUNCH_BOX(k_spec_bdytend, all, s, dims_of(g), field_tend, b, g->ids, ibe, g->jds, jbe);
+  pw_launch_edges(k_spec_bdytend, all, 1, s, dims_of(g), field_tend, b, g->ids, ibe, g->jds, jbe);
   PW_RET();
 }
 
@@ -154,10 +154,10 @@ extern "C" int pywrf_spec_bdy_scalar(const pywrf_grid_t* g, float* scalar_tend, 
 }
 
 // ------------------------------------------------------------------ spec_bdyupdate
-__global__ void k_spec_bdyupdate(Box all, Dims D, float* f, const float* t, float dt, int jts, int jtf, int its,
+__global__ void k_spec_bdyupdate(Box all, int w, Dims D, float* f, const float* t, float dt, int jts, int jtf, int its,
                                  int itf, int jbs, int jbe) {
   int k, j, i;
-  if (!box_point(all, k, j, i)) return;
+  if (!edge_point(all, w, k, j, i)) return;
   bool ic = i >= its && i < itf, jc = j >= jbs + 1 && j < jbe - 1;
   int n = (j == jts && ic) + (j == jtf - 1 && ic) + (i == its && jc) + (i == itf - 1 && jc);
   if (!n) return;
@@ -181,17 +181,19 @@ extern "C" int pywrf_spec_bdyupdate(const pywrf_grid_t* g, float* field, const f
   pywrf_grid_t g2 = *g;
   g2.nz = nz;
   Box all = mkbox(g->kts, ktf, imin(g->jts, g->jds), imax(jtf, jbe), g->its, itf);
-  PW_LAUNCH_BOX(k_spec_bdyupdate, all, s, dims_of(&g2), field, field_tend, F(dt), g->jts, jtf, g->its, itf, g->jds,
-                jbe);
+  // the updated rows jts, jtf-1 and columns its, itf-1 are the box's sides when jts and jtf bound it
+  int w = (g->jts == all.j0 && jtf == all.j1) ? 1 : 0;
+  pw_launch_edges(k_spec_bdyupdate, all, w, s, dims_of(&g2), field, field_tend, F(dt), g->jts, jtf, g->its, itf,
+                  g->jds, jbe);
   PW_RET();
 }
 
 // ------------------------------------------------------------------ spec_bdyupdate_ph
-__global__ void k_spec_bdyupdate_ph(Box all, Dims D, const float* ph_save, float* f, const float* ft,
+__global__ void k_spec_bdyupdate_ph(Box all, int w, Dims D, const float* ph_save, float* f, const float* ft,
                                     const float* mu_tend, const float* muts, float dt, int jts, int ibs, int ibe,
                                     int jbs, int jbe) {
   int k, j, i;
-  if (!box_point(all, k, j, i)) return;
+  if (!edge_point(all, w, k, j, i)) return;
   bool ir = i >= ibs && i < ibe, jc = j >= jbs + 1 && j < jbe - 1;
   int n = (j == jts && ir) + (j == jbe - 1 && ir) + (i == ibs && jc) + (i == ibe - 1 && jc);
   if (!n) return;
@@ -212,15 +214,16 @@ extern "C" int pywrf_spec_bdyupdate_ph(const pywrf_grid_t* g, const float* ph_sa
   bounds_of(v, g, ibe, jbe, ktf);
   if (ibe > g->nx || jbe > g->ny || ktf > g->nz || g->ids < 0 || g->jds < 0) return PYWRF_E_RANGE;
   Box all = mkbox(g->kts, ktf, imin(g->jts, g->jds), imax(jbe, g->jts + 1), g->ids, ibe);
-  PW_LAUNCH_BOX(k_spec_bdyupdate_ph, all, s, dims_of(g), ph_save, field, field_tend, mu_tend, muts, F(dt), g->jts,
-                g->ids, ibe, g->jds, jbe);
+  int w = (g->jts == all.j0 && jbe == all.j1) ? 1 : 0;
+  pw_launch_edges(k_spec_bdyupdate_ph, all, w, s, dims_of(g), ph_save, field, field_tend, mu_tend, muts, F(dt),
+                  g->jts, g->ids, ibe, g->jds, jbe);
   PW_RET();
 }
 
 // ------------------------------------------------------------------ zero_grad_bdy
-__global__ void k_zero_grad(Box all, Dims D, float* f, int ibs, int ibe, int jbs, int jbe) {
+__global__ void k_zero_grad(Box all, int w, Dims D, float* f, int ibs, int ibe, int jbs, int jbe) {
   int k, j, i;
-  if (!box_point(all, k, j, i)) return;
+  if (!edge_point(all, w, k, j, i)) return;
   bool ring = (j == jbs || j == jbe - 1) || (i == ibs || i == ibe - 1);
   if (!ring) return;
   int sj = j < jbs + 1 ? jbs + 1 : (j > jbe - 2 ? jbe - 2 : j);
@@ -237,7 +240,7 @@ extern "C" int pywrf_zero_grad_bdy(const pywrf_grid_t* g, float* field, char var
   if (v == 'w') ktf = g->kde;
   if (ibe > g->nx || jbe > g->ny || ktf > g->nz || g->ids < 0 || g->jds < 0) return PYWRF_E_RANGE;
   Box all = mkbox(g->kts, ktf, g->jds, jbe, g->ids, ibe);
-  PW_LAUNCH_BOX(k_zero_grad, all, s, dims_of(g), field, g->ids, ibe, g->jds, jbe);
+  pw_launch_edges(k_zero_grad, all, 1, s, dims_of(g), field, g->ids, ibe, g->jds, jbe);
   PW_RET();
 }
 
@@ -258,10 +261,10 @@ extern "C" int pywrf_mass_weight(const pywrf_grid_t* g, const float* field, cons
 }
 
 // ------------------------------------------------------------------ flow_dep_bdy
-__global__ void k_flow_dep(Box all, Dims D, float* f, const float* u, const float* v, int jts, int jtf, int its,
+__global__ void k_flow_dep(Box all, int w, Dims D, float* f, const float* u, const float* v, int jts, int jtf, int its,
                            int itf) {
   int k, j, i;
-  if (!box_point(all, k, j, i)) return;
+  if (!edge_point(all, w, k, j, i)) return;
   bool top = j == jts, bot = j == jtf - 1, lft = i == its, rgt = i == itf - 1;
   if (!(top || bot || lft || rgt)) return;
   float r;
@@ -285,15 +288,15 @@ extern "C" int pywrf_flow_dep_bdy(const pywrf_grid_t* g, float* field, const flo
   int itf = imin(g->ite, g->ide - 1), jtf = imin(g->jte, g->jde - 1), ktf = g->kde - 1;
   if (ktf > g->nz) return PYWRF_E_RANGE;
   Box all = mkbox(g->kts, ktf, g->jts, jtf, g->its, itf);
-  PW_LAUNCH_BOX(k_flow_dep, all, s, dims_of(g), field, u, v, g->jts, jtf, g->its, itf);
+  pw_launch_edges(k_flow_dep, all, 1, s, dims_of(g), field, u, v, g->jts, jtf, g->its, itf);
   PW_RET();
 }
 
 // ------------------------------------------------------------------ spec_bdy_final
-__global__ void k_spec_final(Box all, Dims D, float* f, const float* mu, const float* msf, Bdy b, float dtbc,
+__global__ void k_spec_final(Box all, int w, Dims D, float* f, const float* mu, const float* msf, Bdy b, float dtbc,
                              int jts, int jtf, int its, int itf, int ibe, int jbe) {
   int k, j, i;
-  if (!box_point(all, k, j, i)) return;
+  if (!edge_point(all, w, k, j, i)) return;
   bool xr = j >= jts && j < jbe, ir = i >= its && i < ibe;
   float bv, bt;
   if (xr && i == itf - 1) { bv = bx(D, b.xe, k, j, 0); bt = bx(D, b.txe, k, j, 0); }
@@ -330,6 +333,8 @@ extern "C" int pywrf_spec_bdy_final(const pywrf_grid_t* g, float* field, const f
   const float* xmu = (v != 'm') ? mu : nullptr;
   Bdy b = {bxs, bxe, bys, bye, btxs, btxe, btys, btye};
   Box all = mkbox(g->kts, ktf, g->jts, imax(jtf, jbe), g->its, imax(ibe, itf));
-  PW_LAUNCH_BOX(k_spec_final, all, s, dims_of(g), field, xmu, xmsf, b, F(dtbc), g->jts, jtf, g->its, itf, ibe, jbe);
+  int w = (itf >= ibe && jtf >= jbe) ? 1 : 0;  // rows jts, jtf-1 and columns its, itf-1 are the box's sides
+  pw_launch_edges(k_spec_final, all, w, s, dims_of(g), field, xmu, xmsf, b, F(dtbc), g->jts, jtf, g->its, itf, ibe,
+                  jbe);
   PW_RET();
 }

@@ -31,44 +31,82 @@ __host__ __device__ inline Box mkbox(int k0, int k1, int j0, int j1, int i0, int
   Box b; b.k0 = k0; b.k1 = k1; b.j0 = j0; b.j1 = j1; b.i0 = i0; b.i1 = i1; return b;
 }
 
-// Launch helper: one thread per point of a box, i fastest.
+// Launch helpers.  A box launch is a 3-D grid: x covers i in 64-lane rows (one
+// wavefront per row segment, so every load and store of a wave is one
+// contiguous run of a row), y covers j four rows per workgroup, z covers k.
+// Decoding needs no integer division.
+#define PW_TX 64
+#define PW_TY 4
 #define PW_LAUNCH_BOX(kern, box, stream, ...)                                              \
   do {                                                                                     \
-    long _n = (box).size();                                                                \
-    if (_n > 0) {                                                                          \
-      unsigned _g = (unsigned)((_n + PW_BLOCK - 1) / PW_BLOCK);                            \
-      hipLaunchKernelGGL(kern, dim3(_g), dim3(PW_BLOCK), 0, (hipStream_t)(stream), (box), __VA_ARGS__); \
+    if ((box).size() > 0) {                                                                \
+      dim3 _g((unsigned)(((box).ni() + PW_TX - 1) / PW_TX), (unsigned)(((box).nj() + PW_TY - 1) / PW_TY), \
+              (unsigned)(box).nk());                                                       \
+      hipLaunchKernelGGL(kern, _g, dim3(PW_TX, PW_TY), 0, (hipStream_t)(stream), (box), __VA_ARGS__); \
     }                                                                                      \
   } while (0)
 
-// Decode the flat thread id of a PW_LAUNCH_BOX launch; returns false past the end.
+// Point of a PW_LAUNCH_BOX launch; false outside the box.
 __device__ inline bool box_point(const Box& b, int& k, int& j, int& i) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= b.size()) return false;
-  int ni = b.ni(), nj = b.nj();
-  i = b.i0 + (int)(t % ni);
-  long r = t / ni;
-  j = b.j0 + (int)(r % nj);
-  k = b.k0 + (int)(r / nj);
-  return true;
+  i = b.i0 + (int)(blockIdx.x * PW_TX + threadIdx.x);
+  j = b.j0 + (int)(blockIdx.y * PW_TY + threadIdx.y);
+  k = b.k0 + (int)blockIdx.z;
+  return i < b.i1 && j < b.j1;
 }
 
 // Launch helper: one thread per (j, i) column of a box (column kernels).
 #define PW_LAUNCH_COLS(kern, box, stream, ...)                                             \
   do {                                                                                     \
-    long _n = (long)(box).nj() * (box).ni();                                               \
-    if (_n > 0) {                                                                          \
-      unsigned _g = (unsigned)((_n + PW_BLOCK - 1) / PW_BLOCK);                            \
-      hipLaunchKernelGGL(kern, dim3(_g), dim3(PW_BLOCK), 0, (hipStream_t)(stream), (box), __VA_ARGS__); \
+    if ((box).nj() > 0 && (box).ni() > 0) {                                                \
+      dim3 _g((unsigned)(((box).ni() + PW_TX - 1) / PW_TX), (unsigned)(((box).nj() + PW_TY - 1) / PW_TY)); \
+      hipLaunchKernelGGL(kern, _g, dim3(PW_TX, PW_TY), 0, (hipStream_t)(stream), (box), __VA_ARGS__); \
     }                                                                                      \
   } while (0)
 
 __device__ inline bool box_column(const Box& b, int& j, int& i) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)b.nj() * b.ni()) return false;
-  i = b.i0 + (int)(t % b.ni());
-  j = b.j0 + (int)(t / b.ni());
-  return true;
+  i = b.i0 + (int)(blockIdx.x * PW_TX + threadIdx.x);
+  j = b.j0 + (int)(blockIdx.y * PW_TY + threadIdx.y);
+  return i < b.i1 && j < b.j1;
+}
+
+// Edge launches, for kernels that act only on a band of width w along the
+// four sides of a box (boundary rows and columns): one thread per band point,
+// each point visited exactly once (corners belong to the j-rows).  Per level
+// the band holds 2*w*ni points of the first and last w rows, then 2*w*(nj-2w)
+// points of the first and last w columns.  With w = 0 the launch is an
+// ordinary box launch (used when the band would not fit in the box, or when
+// the acting set of the kernel is not known to lie in the band).
+#define PW_EDGE_BLOCK 256
+__host__ __device__ inline long edge_count(const Box& b, int w) {
+  return 2L * w * b.ni() + 2L * w * (b.nj() - 2 * w);
+}
+
+__device__ inline bool edge_point(const Box& b, int w, int& k, int& j, int& i) {
+  if (w == 0) return box_point(b, k, j, i);
+  int t = (int)(blockIdx.x * PW_EDGE_BLOCK + threadIdx.x);
+  k = b.k0 + (int)blockIdx.z;
+  const int ni = b.ni(), nj = b.nj(), rows = w * ni, h = nj - 2 * w;
+  if (t < rows) { j = b.j0 + t / ni; i = b.i0 + t % ni; return true; }
+  t -= rows;
+  if (t < rows) { j = b.j1 - w + t / ni; i = b.i0 + t % ni; return true; }
+  t -= rows;
+  if (t < w * h) { j = b.j0 + w + t / w; i = b.i0 + t % w; return true; }
+  t -= w * h;
+  if (t < w * h) { j = b.j0 + w + t / w; i = b.i1 - w + t % w; return true; }
+  return false;
+}
+
+template <class Kern, class... Args>
+inline void pw_launch_edges(Kern kern, const Box& b, int w, pywrf_stream_t s, Args... args) {
+  if (b.size() <= 0) return;
+  if (w > 0 && 2 * w <= b.nj() && 2 * w <= b.ni()) {
+    long n = edge_count(b, w);
+    dim3 g((unsigned)((n + PW_EDGE_BLOCK - 1) / PW_EDGE_BLOCK), 1, (unsigned)b.nk());
+    hipLaunchKernelGGL(kern, g, dim3(PW_EDGE_BLOCK), 0, (hipStream_t)s, b, w, args...);
+  } else {
+    dim3 g((unsigned)((b.ni() + PW_TX - 1) / PW_TX), (unsigned)((b.nj() + PW_TY - 1) / PW_TY), (unsigned)b.nk());
+    hipLaunchKernelGGL(kern, g, dim3(PW_TX, PW_TY), 0, (hipStream_t)s, b, 0, args...);
+  }
 }
 
 // Fortran SIGN(1., x) (the reference's fortran_sign): +1 for x >= 0, else -1.

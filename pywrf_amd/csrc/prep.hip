@@ -110,21 +110,36 @@ __device__ inline float ww_divv(const Dims& D, int k, int j, int i, const float*
   return AT2(msftx, j, i) * dnw[k] * (rdx * (a - b) + rdy * (c - d));
 }
 
-__global__ void k_calc_ww_cp(Box b, Dims D, const float* u, const float* v, const float* muu, const float* muv,
-                             const float* c1h, float* ww, float rdx, float rdy, const float* msftx,
-                             const float* msfuy, const float* msfvx_inv, const float* dnw) {
-  int j, i;
-  if (!box_column(b, j, i)) return;
+// Column block (a 512-thread workgroup owns 64 i-columns of one row j): the
+// divergence of every level is computed level-parallel into LDS; one wave then
+// forms the column sum in torch's order and runs the eta-velocity recurrence.
+#define WW_TI 64
+#define WW_TB 512
+#define WW_TG (WW_TB / WW_TI)
+struct WwCp {
+  const float *u, *v, *muu, *muv, *c1h, *msftx, *msfuy, *msfvx_inv, *dnw;
+  float* ww;
+  float rdx, rdy;
+};
+
+__global__ void __launch_bounds__(WW_TB) k_calc_ww_cp(Box b, Dims D, WwCp a) {
+  extern __shared__ float s_dv[];   // [nz][WW_TI]
+  const int lane = threadIdx.x % WW_TI, grp = threadIdx.x / WW_TI;
+  const int j = b.j0 + (int)blockIdx.y, i = b.i0 + (int)blockIdx.x * WW_TI + lane;
+  const bool ok = i < b.i1;
+  if (ok)
+    for (int k = grp; k < D.nz; k += WW_TG)
+      s_dv[k * WW_TI + lane] = (k >= b.k0 && k < b.k1) ? ww_divv(D, k, j, i, a.u, a.v, a.muu, a.muv, a.msftx,
+                                                                 a.msfuy, a.msfvx_inv, a.dnw, a.rdx, a.rdy)
+                                                       : 0.0f;
+  __syncthreads();
+  if (!ok || grp != 0) return;
   // column sum over all nz levels of the divergence (zero outside [k0, k1))
-  float dmdt = torch_outer_sum(D.nz, [&](int k) {
-    return (k >= b.k0 && k < b.k1) ? ww_divv(D, k, j, i, u, v, muu, muv, msftx, msfuy, msfvx_inv, dnw, rdx, rdy)
-                                   : 0.0f;
-  });
-  float wk = AT(ww, 0, j, i);
+  const float dmdt = torch_outer_sum(D.nz, [&](int k) { return s_dv[k * WW_TI + lane]; });
+  float wk = AT(a.ww, 0, j, i);
   for (int k = 0; k < b.k1 - 1; ++k) {
-    float dv = ww_divv(D, k, j, i, u, v, muu, muv, msftx, msfuy, msfvx_inv, dnw, rdx, rdy);
-    wk = wk - dnw[k] * c1h[k] * dmdt - dv;
-    AT(ww, k + 1, j, i) = wk;
+    wk = wk - a.dnw[k] * a.c1h[k] * dmdt - s_dv[k * WW_TI + lane];
+    AT(a.ww, k + 1, j, i) = wk;
   }
 }
 
@@ -136,8 +151,11 @@ extern "C" int pywrf_calc_ww_cp(const pywrf_grid_t* g, const float* u, const flo
   int ktf = imin(g->kte, g->kde - 1), jtf = imin(g->jte, g->jde - 1), itf = imin(g->ite, g->ide - 1);
   Box b = mkbox(g->kts, ktf, g->jts, jtf, g->its, itf);
   if (g->kts != 0) return PYWRF_E_RANGE;
-  PW_LAUNCH_COLS(k_calc_ww_cp, b, s, dims_of(g), u, v, muu, muv, c1h, ww, F(rdx), F(rdy), msftx, msfuy,
-                 msfvx_inv, dnw);
+  if ((size_t)g->nz * WW_TI * sizeof(float) > 64 * 1024) return PYWRF_E_RANGE;
+  if (b.nj() <= 0 || b.ni() <= 0) PW_RET();
+  WwCp a = {u, v, muu, muv, c1h, msftx, msfuy, msfvx_inv, dnw, ww, F(rdx), F(rdy)};
+  hipLaunchKernelGGL(k_calc_ww_cp, dim3((unsigned)((b.ni() + WW_TI - 1) / WW_TI), (unsigned)b.nj()), dim3(WW_TB),
+                     (size_t)g->nz * WW_TI * sizeof(float), (hipStream_t)s, b, dims_of(g), a);
   PW_RET();
 }
 
@@ -440,43 +458,75 @@ __global__ void k_phy_prep_pt(Box all, Dims D, Box b, const float* u, const floa
   }
 }
 
-// Launch 2 (one thread per column): dz8w, z, p8w, t8w, hydrostatic pressure.
-__global__ void k_phy_prep_col(Box b, Dims D, int kte, int kde, const float* mut, const float* moist, int n,
-                               const float* p_phy, const float* t_phy, float* z_at_w, float* dz8w, float* z,
-                               float* p8w, float* t8w, float* p_hyd, float* p_hyd_w, const float* dnw,
-                               const float* fzm, const float* fzp, float p_top) {
-  int j, i;
-  if (!box_column(b, j, i)) return;
-  int k0 = b.k0, k1 = b.k1;
-  for (int k = k0; k < kte - 1; ++k) AT(dz8w, k, j, i) = AT(z_at_w, k + 1, j, i) - AT(z_at_w, k, j, i);
-  AT(dz8w, kte - 1, j, i) = 0.0f;
-  for (int k = k0; k < k1; ++k) AT(z, k, j, i) = 0.5f * (AT(z_at_w, k, j, i) + AT(z_at_w, k + 1, j, i));
-  for (int k = k0 + 1; k < k1; ++k) {
-    AT(p8w, k, j, i) = fzm[k] * AT(p_phy, k, j, i) + fzp[k] * AT(p_phy, k - 1, j, i);
-    AT(t8w, k, j, i) = fzm[k] * AT(t_phy, k, j, i) + fzp[k] * AT(t_phy, k - 1, j, i);
+// Launch 2 (column block: a 512-thread workgroup owns 64 i-columns of one
+// row j): dz8w, z, p8w, t8w and the per-level moisture sums level-parallel
+// (sums staged in LDS), then one wave runs the downward hydrostatic
+// recurrence p_hyd_w per column from LDS, then all waves write p_hyd.
+#define PP_TI 64
+#define PP_TB 512
+#define PP_TG (PP_TB / PP_TI)
+struct PhyCol {
+  const float *mut, *moist, *p_phy, *t_phy, *z_at_w, *dnw, *fzm, *fzp;
+  float *dz8w, *z, *p8w, *t8w, *p_hyd, *p_hyd_w;
+  float p_top;
+  int n, kte, kde;
+};
+
+__global__ void __launch_bounds__(PP_TB) k_phy_prep_col(Box b, Dims D, PhyCol a) {
+  extern __shared__ float sm[];
+  float* s_qt = sm;                  // [nz][PP_TI]
+  float* s_pw = sm + D.nz * PP_TI;   // [nz][PP_TI]
+  const int lane = threadIdx.x % PP_TI, grp = threadIdx.x / PP_TI;
+  const int j = b.j0 + (int)blockIdx.y, i = b.i0 + (int)blockIdx.x * PP_TI + lane;
+  const bool ok = i < b.i1;
+  const int k0 = b.k0, k1 = b.k1, kte = a.kte, kde = a.kde;
+  auto ZW = [&](int kk) { return AT(a.z_at_w, kk, j, i); };
+  auto ZM = [&](int kk) { return 0.5f * (ZW(kk) + ZW(kk + 1)); };
+  auto PP = [&](int kk) { return AT(a.p_phy, kk, j, i); };
+  auto TP = [&](int kk) { return AT(a.t_phy, kk, j, i); };
+  if (ok) {
+    for (int k = grp; k < D.nz; k += PP_TG) {
+      const long q = D.ix(k, j, i);
+      if (k >= k0 && k < kte - 1) a.dz8w[q] = ZW(k + 1) - ZW(k);
+      if (k == kte - 1) a.dz8w[q] = 0.0f;
+      if (k >= k0 && k < k1) a.z[q] = ZM(k);
+      if (k >= k0 + 1 && k < k1) {
+        a.p8w[q] = a.fzm[k] * PP(k) + a.fzp[k] * PP(k - 1);
+        a.t8w[q] = a.fzm[k] * TP(k) + a.fzp[k] * TP(k - 1);
+      }
+      if (k >= k0 && k < kte - 1) s_qt[k * PP_TI + lane] = (k < k1 - 1) ? qsum(D, a.moist, a.n, k, j, i) : 0.0f;
+    }
+    if (grp == 1) {
+      float z0 = ZW(0), z1 = ZM(0), z2 = ZM(1);
+      float w1 = (z0 - z2) / (z1 - z2);
+      float w2 = 1.0f - w1;
+      AT(a.p8w, 0, j, i) = w1 * PP(0) + w2 * PP(1);
+      AT(a.t8w, 0, j, i) = w1 * TP(0) + w2 * TP(1);
+    }
+    if (grp == 2) {
+      float z0 = ZW(kte - 1), z1 = ZM(k1 - 1), z2 = ZM(k1 - 2);
+      float w1 = (z0 - z2) / (z1 - z2);
+      float w2 = 1.0f - w1;
+      AT(a.p8w, kde - 1, j, i) = expf(w1 * logf(PP(kde - 2)) + w2 * logf(PP(kde - 3)));
+      AT(a.t8w, kde - 1, j, i) = w1 * TP(kde - 2) + w2 * TP(kde - 3);
+    }
   }
-  {
-    float z0 = AT(z_at_w, 0, j, i), z1 = AT(z, 0, j, i), z2 = AT(z, 1, j, i);
-    float w1 = (z0 - z2) / (z1 - z2);
-    float w2 = 1.0f - w1;
-    AT(p8w, 0, j, i) = w1 * AT(p_phy, 0, j, i) + w2 * AT(p_phy, 1, j, i);
-    AT(t8w, 0, j, i) = w1 * AT(t_phy, 0, j, i) + w2 * AT(t_phy, 1, j, i);
+  __syncthreads();
+  if (ok && grp == 0) {
+    const float m = AT2(a.mut, j, i);
+    float pw_ = a.p_top;
+    s_pw[(kte - 1) * PP_TI + lane] = pw_;
+    AT(a.p_hyd_w, kte - 1, j, i) = pw_;
+    for (int k = kte - 2; k >= k0; --k) {
+      pw_ = pw_ - (1.0f + s_qt[k * PP_TI + lane]) * m * a.dnw[k];
+      s_pw[k * PP_TI + lane] = pw_;
+      AT(a.p_hyd_w, k, j, i) = pw_;
+    }
   }
-  {
-    float z0 = AT(z_at_w, kte - 1, j, i), z1 = AT(z, k1 - 1, j, i), z2 = AT(z, k1 - 2, j, i);
-    float w1 = (z0 - z2) / (z1 - z2);
-    float w2 = 1.0f - w1;
-    AT(p8w, kde - 1, j, i) = expf(w1 * logf(AT(p_phy, kde - 2, j, i)) + w2 * logf(AT(p_phy, kde - 3, j, i)));
-    AT(t8w, kde - 1, j, i) = w1 * AT(t_phy, kde - 2, j, i) + w2 * AT(t_phy, kde - 3, j, i);
-  }
-  float pw_ = p_top;
-  AT(p_hyd_w, kte - 1, j, i) = pw_;
-  for (int k = kte - 2; k >= k0; --k) {
-    float qt = (k < k1 - 1) ? qsum(D, moist, n, k, j, i) : 0.0f;
-    pw_ = pw_ - (1.0f + qt) * AT2(mut, j, i) * dnw[k];
-    AT(p_hyd_w, k, j, i) = pw_;
-  }
-  for (int k = k0; k < k1; ++k) AT(p_hyd, k, j, i) = 0.5f * (AT(p_hyd_w, k, j, i) + AT(p_hyd_w, k + 1, j, i));
+  __syncthreads();
+  if (!ok) return;
+  for (int k = k0 + grp; k < k1; k += PP_TG)
+    AT(a.p_hyd, k, j, i) = 0.5f * (s_pw[k * PP_TI + lane] + s_pw[(k + 1) * PP_TI + lane]);
 }
 
 extern "C" int pywrf_phy_prep(const pywrf_grid_t* g, const float* mut, const float* u, const float* v,
@@ -488,6 +538,8 @@ extern "C" int pywrf_phy_prep(const pywrf_grid_t* g, const float* mut, const flo
                               float* t_phy_out, pywrf_stream_t s) {
   CHECK(g, 1);
   if (n_moist < 2 || g->kte != g->nz || g->kde != g->nz || g->kts != 0) return PYWRF_E_RANGE;
+  const size_t lds = 2 * (size_t)g->nz * PP_TI * sizeof(float);
+  if (lds > 64 * 1024) return PYWRF_E_RANGE;
   Dims D = dims_of(g);
   int k1 = imin(g->kte, g->kde - 1);
   Box b = mkbox(g->kts, k1, g->jts, imin(g->jte, g->jde - 1), g->its, imin(g->ite, g->ide - 1));
@@ -495,8 +547,11 @@ extern "C" int pywrf_phy_prep(const pywrf_grid_t* g, const float* mut, const flo
   const float* qv = moist + (long)pw::P_QV * D.nz * D.plane();
   PW_LAUNCH_BOX(k_phy_prep_pt, all, s, D, b, u, v, p, pb, alt, ph, phb, t, qv, rho, th_phy, p_phy, u_phy, v_phy,
                 z_at_w, pi_phy_out, t_phy_out);
-  PW_LAUNCH_COLS(k_phy_prep_col, b, s, D, g->kte, g->kde, mut, moist, n_moist, p_phy, t_phy_out, z_at_w, dz8w,
-                 z, p8w, t8w, p_hyd, p_hyd_w, dnw, fzm, fzp, F(p_top));
+  PhyCol c = {mut, moist, p_phy, t_phy_out, z_at_w, dnw, fzm, fzp, dz8w, z, p8w, t8w, p_hyd, p_hyd_w, F(p_top),
+              n_moist, g->kte, g->kde};
+  if (b.nj() <= 0 || b.ni() <= 0) PW_RET();
+  hipLaunchKernelGGL(k_phy_prep_col, dim3((unsigned)((b.ni() + PP_TI - 1) / PP_TI), (unsigned)b.nj()), dim3(PP_TB),
+                     lds, (hipStream_t)s, b, D, c);
   PW_RET();
 }
 

@@ -32,6 +32,37 @@ int pw_fill(float* a, long n, float v, pywrf_stream_t s) {
   return (int)hipGetLastError();
 }
 
+// Grouped zero fill: blockIdx.y selects the array, a grid-stride loop over x
+// covers it.
+#define PW_ZERO_MAX 32
+struct ZeroJobs {
+  float* p[PW_ZERO_MAX];
+  long n[PW_ZERO_MAX];
+};
+
+__global__ void k_zero_group(ZeroJobs J) {
+  float* p = J.p[blockIdx.y];
+  const long n = J.n[blockIdx.y];
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) p[t] = 0.0f;
+}
+
+extern "C" int pywrf_zero_group(int n, void* const* fields, const long long* counts, pywrf_stream_t s) {
+  if (n < 0 || n > PW_ZERO_MAX || (n > 0 && (!fields || !counts))) return PYWRF_E_ARG;
+  ZeroJobs J;
+  long most = 0;
+  for (int f = 0; f < n; ++f) {
+    if (!fields[f] || counts[f] < 0) return PYWRF_E_ARG;
+    J.p[f] = (float*)fields[f];
+    J.n[f] = (long)counts[f];
+    most = J.n[f] > most ? J.n[f] : most;
+  }
+  if (n == 0 || most == 0) return 0;
+  long blocks = (most + PW_BLOCK - 1) / PW_BLOCK;
+  if (blocks > 2048) blocks = 2048;
+  hipLaunchKernelGGL(k_zero_group, dim3((unsigned)blocks, (unsigned)n), dim3(PW_BLOCK), 0, (hipStream_t)s, J);
+  return (int)hipGetLastError();
+}
+
 // ------------------------------------------------------------------ rk_scalar_tend
 extern "C" int pywrf_rk_scalar_tend(const pywrf_grid_t* g, int scs, int sce, int rk_step, const float* ru,
                                     const float* rv, const float* ww, const float* mut, const float* alt,

@@ -189,35 +189,61 @@ extern "C" int pywrf_calc_p_rho(const pywrf_grid_t* g, const float* al_in, const
 }
 
 // ------------------------------------------------------------------ calc_coef_w
-__global__ void k_calc_coef_w(Box b, Dims D, float* a, float* alpha, float* gamma, const float* mut,
-                              const float* cqw, const float* rdn, const float* rdnw, const float* c2a, float C,
-                              float m2C, float p2C, float lid, int kde) {
-  int j, i;
-  if (!box_column(b, j, i)) return;
-  float m = AT2(mut, j, i);
-  float r2 = rdnw[kde - 2] * rdnw[kde - 2];
-  AT(a, 1, j, i) = 0.0f;
-  float atop = m2C * r2 * AT(c2a, kde - 2, j, i) * lid / (m * m);
-  AT(a, kde - 1, j, i) = atop;
-  AT(gamma, 0, j, i) = 0.0f;
+// Column block (a 512-thread workgroup owns 64 i-columns of one row j): the
+// tridiagonal coefficients of every level are formed level-parallel (a written
+// out, the diagonal and upper coefficients staged in LDS); one wave then runs
+// the forward elimination alpha/gamma per column from LDS.
+#define CW_TI 64
+#define CW_TB 512
+#define CW_TG (CW_TB / CW_TI)
+struct CoefW {
+  float *a, *alpha, *gamma;
+  const float *mut, *cqw, *rdn, *rdnw, *c2a;
+  float C, m2C, p2C, lid;
+  int kde;
+};
+
+__global__ void __launch_bounds__(CW_TB) k_calc_coef_w(Box b, Dims D, CoefW p) {
+  extern __shared__ float sm[];
+  float* s_a = sm;                  // [nz][CW_TI] sub-diagonal
+  float* s_b = sm + D.nz * CW_TI;   // diagonal
+  float* s_c = sm + 2 * D.nz * CW_TI;   // super-diagonal
+  const int lane = threadIdx.x % CW_TI, grp = threadIdx.x / CW_TI;
+  const int j = b.j0 + (int)blockIdx.y, i = b.i0 + (int)blockIdx.x * CW_TI + lane;
+  const bool ok = i < b.i1;
+  const int kde = p.kde;
+  float m = 0.0f;
+  if (ok) {
+    m = AT2(p.mut, j, i);
+    for (int k = 1 + grp; k < kde - 1; k += CW_TG) {
+      float cq = AT(p.cqw, k, j, i), c2 = AT(p.c2a, k, j, i), c2m = AT(p.c2a, k - 1, j, i);
+      float ak = 0.0f;
+      if (k >= 2) {
+        ak = -cq * p.C * p.rdn[k] * p.rdnw[k - 1] * c2m / (m * m);
+        AT(p.a, k, j, i) = ak;
+      }
+      s_a[k * CW_TI + lane] = ak;
+      s_b[k * CW_TI + lane] = 1.0f + cq * p.C * p.rdn[k] * (p.rdnw[k] * c2 / (m * m) + p.rdnw[k - 1] * c2m / (m * m));
+      s_c[k * CW_TI + lane] = -cq * p.C * p.rdn[k] * p.rdnw[k] * c2 / (m * m);
+    }
+  }
+  __syncthreads();
+  if (!ok || grp != 0) return;
+  float r2 = p.rdnw[kde - 2] * p.rdnw[kde - 2];
+  AT(p.a, 1, j, i) = 0.0f;
+  float atop = p.m2C * r2 * AT(p.c2a, kde - 2, j, i) * p.lid / (m * m);
+  AT(p.a, kde - 1, j, i) = atop;
+  AT(p.gamma, 0, j, i) = 0.0f;
   float gm = 0.0f;
   for (int k = 1; k < kde - 1; ++k) {
-    float cq = AT(cqw, k, j, i), c2 = AT(c2a, k, j, i), c2m = AT(c2a, k - 1, j, i);
-    float ak = 0.0f;
-    if (k >= 2) {
-      ak = -cq * C * rdn[k] * rdnw[k - 1] * c2m / (m * m);
-      AT(a, k, j, i) = ak;
-    }
-    float bb = 1.0f + cq * C * rdn[k] * (rdnw[k] * c2 / (m * m) + rdnw[k - 1] * c2m / (m * m));
-    float cc = -cq * C * rdn[k] * rdnw[k] * c2 / (m * m);
-    float al = 1.0f / (bb - ak * gm);
-    gm = cc * al;
-    AT(alpha, k, j, i) = al;
-    AT(gamma, k, j, i) = gm;
+    float al = 1.0f / (s_b[k * CW_TI + lane] - s_a[k * CW_TI + lane] * gm);
+    gm = s_c[k * CW_TI + lane] * al;
+    AT(p.alpha, k, j, i) = al;
+    AT(p.gamma, k, j, i) = gm;
   }
-  float bb = 1.0f + p2C * r2 * AT(c2a, kde - 2, j, i) / (m * m);
-  AT(alpha, kde - 1, j, i) = 1.0f / (bb - atop * gm);
-  AT(gamma, kde - 1, j, i) = 0.0f;
+  float bb = 1.0f + p.p2C * r2 * AT(p.c2a, kde - 2, j, i) / (m * m);
+  AT(p.alpha, kde - 1, j, i) = 1.0f / (bb - atop * gm);
+  AT(p.gamma, kde - 1, j, i) = 0.0f;
 }
 
 extern "C" int pywrf_calc_coef_w(const pywrf_grid_t* g, float* a, float* alpha, float* gamma, const float* mut,
@@ -228,8 +254,12 @@ extern "C" int pywrf_calc_coef_w(const pywrf_grid_t* g, float* a, float* alpha, 
   double C = (0.5 * dts * g_ * (1. + epssm));
   C = C * C;
   Box b = mkbox(0, 1, g->jts, imin(g->jte, g->jde - 1), g->its, imin(g->ite, g->ide - 1));
-  PW_LAUNCH_COLS(k_calc_coef_w, b, s, dims_of(g), a, alpha, gamma, mut, cqw, rdn, rdnw, c2a, F(C), F(-2. * C),
-                 F(2. * C), top_lid ? 0.0f : 1.0f, g->kde);
+  const size_t lds = 3 * (size_t)g->nz * CW_TI * sizeof(float);
+  if (lds > 64 * 1024) return PYWRF_E_RANGE;
+  if (b.nj() <= 0 || b.ni() <= 0) PW_RET();
+  CoefW p = {a, alpha, gamma, mut, cqw, rdn, rdnw, c2a, F(C), F(-2. * C), F(2. * C), top_lid ? 0.0f : 1.0f, g->kde};
+  hipLaunchKernelGGL(k_calc_coef_w, dim3((unsigned)((b.ni() + CW_TI - 1) / CW_TI), (unsigned)b.nj()), dim3(CW_TB),
+                     lds, (hipStream_t)s, b, dims_of(g), p);
   PW_RET();
 }
 
@@ -328,14 +358,14 @@ __device__ inline float mut_dvdxi(const Dims& D, const AdvMuT& a, int k, int j, 
   return AT2(a.msftx, j, i) * AT2(a.msfty, j, i) * (a.rdy * vy + a.rdx * ux);
 }
 
-// Column block: a 256-thread workgroup owns TI consecutive i-columns of one
+// Column block: a 512-thread workgroup owns TI consecutive i-columns of one
 // row j and all levels.  The divergence dvdxi of every (k, i) is computed by
 // all threads and staged in LDS; one thread per column then forms the column
 // sum, the column-mass update and the eta-velocity recurrence from LDS; all
 // threads finally apply the theta update level by level (coalesced in i).
 // Columns outside the update box copy their inputs (clone semantics).
 #define TI 64
-#define TB 256
+#define TB 512
 #define TG (TB / TI)
 
 __global__ void __launch_bounds__(TB) k_advance_mu_t(Dims D, AdvMuT a) {

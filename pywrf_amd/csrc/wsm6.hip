@@ -642,37 +642,44 @@ struct MpPrep {
   int kte, kde, k1;
 };
 
+// One thread per point of levels [0, kte): every output is local in k except
+// the neighbours' z_at_w and full pressure, which the thread recomputes with
+// the same operations (so the values are those the column sequence of the
+// reference produces).  The k = 0 thread also writes the extrapolated surface
+// and top rows of p8w.
 __global__ void k_mp_prep(Box b, Dims D, MpPrep a) {
-  int j, i;
-  if (!box_column(b, j, i)) return;
+  int k, j, i;
+  if (!box_point(b, k, j, i)) return;
   const float G = F(pw::g);
-  for (int k = 0; k < a.kte; ++k) AT(a.z_at_w, k, j, i) = (AT(a.ph, k, j, i) + AT(a.phb, k, j, i)) / G;
-  for (int k = 0; k < a.kte - 1; ++k) AT(a.dz8w, k, j, i) = AT(a.z_at_w, k + 1, j, i) - AT(a.z_at_w, k, j, i);
-  AT(a.dz8w, a.kte - 1, j, i) = 0.0f;
-  for (int k = 0; k < a.k1; ++k) {
-    long q = D.ix(k, j, i);
+  // levels this call writes are recomputed; others (only for unusual index
+  // sets, kde > kte + 1) are read as stored, as the reference reads them
+  auto ZW = [&](int kk) { return kk < a.kte ? (AT(a.ph, kk, j, i) + AT(a.phb, kk, j, i)) / G : AT(a.z_at_w, kk, j, i); };
+  auto PF = [&](int kk) { return kk < a.k1 ? AT(a.p, kk, j, i) + AT(a.pb, kk, j, i) : AT(a.pf, kk, j, i); };
+  auto ZM = [&](int kk) { return 0.5f * (ZW(kk) + ZW(kk + 1)); };
+  const long q = D.ix(k, j, i);
+  const float zw = ZW(k);
+  a.z_at_w[q] = zw;
+  a.dz8w[q] = k < a.kte - 1 ? ZW(k + 1) - zw : 0.0f;
+  if (k < a.k1) {
     float th = a.t_new[q] + a.t0;
     a.th_phy[q] = th;
     a.h_diabatic[q] = th + 0.0f;
     a.qv_diabatic[q] = a.qv[q] + 0.0f;
     a.qc_diabatic[q] = a.qc[q] + 0.0f;
     a.rho[q] = 1.0f / (a.al[q] + a.alb[q]);
-    a.pii[q] = powf((a.p[q] + a.pb[q]) / a.p0, F(pw::rcp));
-    a.z[q] = 0.5f * (a.z_at_w[q] + AT(a.z_at_w, k + 1, j, i));
-    a.pf[q] = a.p[q] + a.pb[q];
+    float pf = PF(k);
+    a.pii[q] = powf(pf / a.p0, F(pw::rcp));
+    a.z[q] = 0.5f * (zw + ZW(k + 1));
+    a.pf[q] = pf;
+    if (k >= 1) a.p8w[q] = a.fzm[k] * pf + a.fzp[k] * PF(k - 1);
   }
-  for (int k = 1; k < a.k1; ++k) AT(a.p8w, k, j, i) = a.fzm[k] * AT(a.pf, k, j, i) + a.fzp[k] * AT(a.pf, k - 1, j, i);
-  {
-    float w1 = (AT(a.z_at_w, 0, j, i) - AT(a.z, 1, j, i)) / (AT(a.z, 0, j, i) - AT(a.z, 1, j, i));
+  if (k == 0) {
+    float w1 = (zw - ZM(1)) / (ZM(0) - ZM(1));
     float w2 = 1.0f - w1;
-    AT(a.p8w, 0, j, i) = w1 * AT(a.pf, 0, j, i) + w2 * AT(a.pf, 1, j, i);
-  }
-  {
-    float w1 = (AT(a.z_at_w, a.kte - 1, j, i) - AT(a.z, a.k1 - 2, j, i)) /
-               (AT(a.z, a.k1 - 1, j, i) - AT(a.z, a.k1 - 2, j, i));
-    float w2 = 1.0f - w1;
-    AT(a.p8w, a.kde - 1, j, i) =
-        expf(w1 * logf(AT(a.pf, a.kde - 2, j, i)) + w2 * logf(AT(a.pf, a.kde - 3, j, i)));
+    AT(a.p8w, 0, j, i) = w1 * PF(0) + w2 * PF(1);
+    float u1 = (ZW(a.kte - 1) - ZM(a.k1 - 2)) / (ZM(a.k1 - 1) - ZM(a.k1 - 2));
+    float u2 = 1.0f - u1;
+    AT(a.p8w, a.kde - 1, j, i) = expf(u1 * logf(PF(a.kde - 2)) + u2 * logf(PF(a.kde - 3)));
   }
 }
 
@@ -688,8 +695,8 @@ extern "C" int pywrf_moist_physics_prep_em(const pywrf_grid_t* g, const float* t
   if (g->kts != 0 || g->kte > g->nz || g->kde > g->nz || g->kte < 4) return PYWRF_E_RANGE;
   MpPrep a = {t_new, al, alb, p, pb, ph, phb, qv, qc, fzm, fzp, rho, th_phy, pii, pf, z, z_at_w, dz8w, p8w,
               h_diabatic, qv_diabatic, qc_diabatic, F(t0), F(p0), g->kte, g->kde, imin(g->kte, g->kde - 1)};
-  Box b = mkbox(0, 1, g->jts, imin(g->jte, g->jde - 1), g->its, imin(g->ite, g->ide - 1));
-  PW_LAUNCH_COLS(k_mp_prep, b, s, dims_of(g), a);
+  Box b = mkbox(0, g->kte, g->jts, imin(g->jte, g->jde - 1), g->its, imin(g->ite, g->ide - 1));
+  PW_LAUNCH_BOX(k_mp_prep, b, s, dims_of(g), a);
   PW_RET();
 }
 

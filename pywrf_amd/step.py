@@ -116,6 +116,14 @@ def rk3_step(s, ops, cfg, check_nan=False):
             return [t for t, _, _ in items]
         return [bc2(t, v) if two else bc3(t, v) for t, v, two in items]
 
+    def zeros(*arrays):
+        """A run of zero_() calls; one launch when the implementation offers it."""
+        if hasattr(o, "zero_group"):
+            o.zero_group(list(arrays))
+        else:
+            for a in arrays:
+                a.zero_()
+
     s.step += 1
     if s.step % cfg.bdy_interval == 1:
         s.it_bdy = int(s.step / cfg.bdy_interval)
@@ -147,9 +155,8 @@ def rk3_step(s, ops, cfg, check_nan=False):
             (s.p, 'p', 0), (s.muu, 'u', 1), (s.muv, 'v', 1), (s.mut, 't', 1), (s.al, 'p', 0), (s.ph, 'w', 0))
         if rk_step == 1:
             # first_rk_step_part1/2 (solver.py:846-968)
-            for a in (s.ru_tendf, s.rv_tendf, s.rw_tendf, s.ph_tendf, s.t_tendf, s.tke_tend, s.mu_tendf,
-                      s.moist_tend, s.scalar_tend):
-                a.zero_()
+            zeros(s.ru_tendf, s.rv_tendf, s.rw_tendf, s.ph_tendf, s.t_tendf, s.tke_tend, s.mu_tendf,
+                  s.moist_tend, s.scalar_tend)
             (s.th_phy, s.p_phy, s.pi_phy, s.t_phy, s.rho, s.u_phy, s.v_phy, s.z_at_w, s.dz8w, s.z, s.p8w, s.t8w,
              s.p_hyd_w, s.p_hyd) = o.phy_prep(
                 s.mut, s.muu, s.muv, s.c1h, s.c2h, s.c1f, s.c2f, s.u, s.v, s.p, s.pb, s.alt, s.ph, s.phb, s.t,
@@ -178,9 +185,8 @@ def rk3_step(s, ops, cfg, check_nan=False):
                 s.rthblten, s.rublten, s.rvblten, s.rqvblten, s.rqcblten, s.rqiblten, N_MOIST, N_SCALAR,
                 rk_step, True, *I18)
         # rk_tendency (solver.py:971-1216)
-        for a in (s.ru_tend, s.rv_tend, s.rw_tend, s.t_tend, s.ph_tend, s.u_save, s.v_save, s.w_save,
-                  s.ph_save, s.t_save, s.mu_tend, s.mu_save):
-            a.zero_()
+        zeros(s.ru_tend, s.rv_tend, s.rw_tend, s.t_tend, s.ph_tend, s.u_save, s.v_save, s.w_save,
+              s.ph_save, s.t_save, s.mu_tend, s.mu_save)
         mu_tend_e = s.mu_tend.unsqueeze(0)
         mu_e = s.mu.unsqueeze(0)
         adv = (s.c1h, s.c2h, s.mut, TIME_STEP, s.msfux, s.msfuy, s.msfvx, s.msfvy, s.msftx, s.msfty, s.fnm, s.fnp,

@@ -838,3 +838,20 @@ def set_physical_bc_group(items, idx24, idx2d):
     flags = b"".join(v.encode() for _, v, _ in items)
     kinds = (ctypes.c_int * n)(*[1 if two else 0 for _, _, two in items])
     call("pywrf_set_physical_bc_group", g3, g2, n, ptrs, flags, kinds, stream())
+
+
+def zero_group(tensors):
+    """tensor.zero_() on each of up to 32 distinct contiguous FP32 device
+    tensors, in one launch (the driver's runs of zero_ calls,
+    solver.py:848-851, 975-981).  Not part of the reference API."""
+    import ctypes
+    n = len(tensors)
+    if n == 0:
+        return
+    if n > 32:
+        raise ValueError("zero_group: at most 32 tensors per call")
+    if len({t.data_ptr() for t in tensors}) != n:
+        raise ValueError("zero_group: a tensor appears twice")
+    ptrs = (ctypes.c_void_p * n)(*[ptr(t).value for t in tensors])
+    counts = (ctypes.c_longlong * n)(*[t.numel() for t in tensors])
+    call("pywrf_zero_group", n, ptrs, counts, stream())
