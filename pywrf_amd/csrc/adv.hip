@@ -394,30 +394,31 @@ __global__ void k_scalar_adv(Box all, Dims D, ScalarTend a) {
 }
 
 // First RK stage only: horizontal ('m') and vertical diffusion of every
-// species into scalar_tends (wrf_dynamics.py:4018-4046).
-__global__ void __launch_bounds__(PW_TX * PW_TY) k_scalar_diff(Box all, Dims D, ScalarTend a) {
-  int k, j, i;
-  if (!box_point(all, k, j, i)) return;
-  long q = D.ix(k, j, i);
+// species into scalar_tends (wrf_dynamics.py:4018-4046).  One thread per
+// (species, point): blockIdx.z = species * nk + level.
+__global__ void k_scalar_diff(Box b, Dims D, ScalarTend a) {
+  const int nk = b.nk();
+  const int im = a.scs + (int)blockIdx.z / nk;
+  const int k = b.k0 + (int)blockIdx.z % nk;
+  const int i = b.i0 + (int)(blockIdx.x * PW_TX + threadIdx.x), j = b.j0 + (int)(blockIdx.y * PW_TY + threadIdx.y);
+  if (i >= b.i1 || j >= b.j1) return;
   const bool dh = a.bh.has(k, j, i), dv = a.bv.has(k, j, i);
   if (!dh && !dv) return;
-#pragma unroll 1
-  for (int im = a.scs; im <= a.sce; ++im) {
-    const float* f = a.scalar + im * a.n3;
-    float st = a.sc_tend[im * a.n3 + q];
-    if (dh) {
-      HDiff hd = a.hd;
-      hd.f = f;
-      st = hdiff_apply(D, hd, k, j, i, st);
-    }
-    if (dv) {
-      VDiff vd = a.vd;
-      vd.f = f;
-      vd.kind = (a.moist_step && im == pw::P_QV) ? 'p' : 'm';
-      st = vdiff_apply(D, vd, k, j, i, st);
-    }
-    a.sc_tend[im * a.n3 + q] = st;
+  const long q = im * a.n3 + D.ix(k, j, i);
+  const float* f = a.scalar + im * a.n3;
+  float st = a.sc_tend[q];
+  if (dh) {
+    HDiff hd = a.hd;
+    hd.f = f;
+    st = hdiff_apply(D, hd, k, j, i, st);
   }
+  if (dv) {
+    VDiff vd = a.vd;
+    vd.f = f;
+    vd.kind = (a.moist_step && im == pw::P_QV) ? 'p' : 'm';
+    st = vdiff_apply(D, vd, k, j, i, st);
+  }
+  a.sc_tend[q] = st;
 }
 
 int pw_rk_scalar_tend_fused(const pywrf_grid_t* g, int scs, int sce, int rk_step, const float* ru, const float* rv,
@@ -447,6 +448,11 @@ int pw_rk_scalar_tend_fused(const pywrf_grid_t* g, int scs, int sce, int rk_step
   a.bv = mkbox(g->kts + 1, ktf, g->jts, jtf, g->its, itf);
   Box all = mkbox(0, g->nz, 0, g->ny, 0, g->nx);
   PW_LAUNCH_BOX(k_scalar_adv, all, s, dims_of(g), a);
-  if (a.rk1) PW_LAUNCH_BOX(k_scalar_diff, a.bzero, s, dims_of(g), a);
+  if (a.rk1 && a.bzero.size() > 0) {
+    const Box& bz = a.bzero;
+    dim3 grid((unsigned)((bz.ni() + PW_TX - 1) / PW_TX), (unsigned)((bz.nj() + PW_TY - 1) / PW_TY),
+              (unsigned)(bz.nk() * (sce - scs + 1)));
+    hipLaunchKernelGGL(k_scalar_diff, grid, dim3(PW_TX, PW_TY), 0, (hipStream_t)s, bz, dims_of(g), a);
+  }
   PW_RET();
 }

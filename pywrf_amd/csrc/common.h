@@ -14,6 +14,9 @@
 #include "../../include/pywrf_amd.h"
 
 #define PW_BLOCK 256
+// MI355X has 8 XCDs, each with its own L2; consecutive workgroups of a launch
+// are dealt to them round-robin.
+#define PW_XCDS 8
 
 struct Box {
   int k0, k1, j0, j1, i0, i1;

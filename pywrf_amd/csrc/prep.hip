@@ -124,6 +124,8 @@ struct WwCp {
 
 __global__ void __launch_bounds__(WW_TB) k_calc_ww_cp(Box b, Dims D, WwCp a) {
   extern __shared__ float s_dv[];   // [nz][WW_TI]
+  __shared__ float s_dc[256];       // dnw[k] * c1h[k] (nz <= 256: the LDS check bounds nz by 256)
+  for (int t = threadIdx.x; t < D.nz; t += WW_TB) s_dc[t] = a.dnw[t] * a.c1h[t];
   const int lane = threadIdx.x % WW_TI, grp = threadIdx.x / WW_TI;
   const int j = b.j0 + (int)blockIdx.y, i = b.i0 + (int)blockIdx.x * WW_TI + lane;
   const bool ok = i < b.i1;
@@ -138,7 +140,7 @@ __global__ void __launch_bounds__(WW_TB) k_calc_ww_cp(Box b, Dims D, WwCp a) {
   const float dmdt = torch_outer_sum(D.nz, [&](int k) { return s_dv[k * WW_TI + lane]; });
   float wk = AT(a.ww, 0, j, i);
   for (int k = 0; k < b.k1 - 1; ++k) {
-    wk = wk - a.dnw[k] * a.c1h[k] * dmdt - s_dv[k * WW_TI + lane];
+    wk = wk - s_dc[k] * dmdt - s_dv[k * WW_TI + lane];
     AT(a.ww, k + 1, j, i) = wk;
   }
 }
@@ -151,7 +153,7 @@ extern "C" int pywrf_calc_ww_cp(const pywrf_grid_t* g, const float* u, const flo
   int ktf = imin(g->kte, g->kde - 1), jtf = imin(g->jte, g->jde - 1), itf = imin(g->ite, g->ide - 1);
   Box b = mkbox(g->kts, ktf, g->jts, jtf, g->its, itf);
   if (g->kts != 0) return PYWRF_E_RANGE;
-  if ((size_t)g->nz * WW_TI * sizeof(float) > 64 * 1024) return PYWRF_E_RANGE;
+  if ((size_t)g->nz * WW_TI * sizeof(float) > 64 * 1024 || g->nz > 256) return PYWRF_E_RANGE;
   if (b.nj() <= 0 || b.ni() <= 0) PW_RET();
   WwCp a = {u, v, muu, muv, c1h, msftx, msfuy, msfvx_inv, dnw, ww, F(rdx), F(rdy)};
   hipLaunchKernelGGL(k_calc_ww_cp, dim3((unsigned)((b.ni() + WW_TI - 1) / WW_TI), (unsigned)b.nj()), dim3(WW_TB),
@@ -476,6 +478,8 @@ __global__ void __launch_bounds__(PP_TB) k_phy_prep_col(Box b, Dims D, PhyCol a)
   extern __shared__ float sm[];
   float* s_qt = sm;                  // [nz][PP_TI]
   float* s_pw = sm + D.nz * PP_TI;   // [nz][PP_TI]
+  __shared__ float s_dnw[128];       // nz <= 128 (two LDS arrays of nz x 64 floats within 64 KB)
+  for (int t = threadIdx.x; t < D.nz; t += PP_TB) s_dnw[t] = a.dnw[t];
   const int lane = threadIdx.x % PP_TI, grp = threadIdx.x / PP_TI;
   const int j = b.j0 + (int)blockIdx.y, i = b.i0 + (int)blockIdx.x * PP_TI + lane;
   const bool ok = i < b.i1;
@@ -518,7 +522,7 @@ __global__ void __launch_bounds__(PP_TB) k_phy_prep_col(Box b, Dims D, PhyCol a)
     s_pw[(kte - 1) * PP_TI + lane] = pw_;
     AT(a.p_hyd_w, kte - 1, j, i) = pw_;
     for (int k = kte - 2; k >= k0; --k) {
-      pw_ = pw_ - (1.0f + s_qt[k * PP_TI + lane]) * m * a.dnw[k];
+      pw_ = pw_ - (1.0f + s_qt[k * PP_TI + lane]) * m * s_dnw[k];
       s_pw[k * PP_TI + lane] = pw_;
       AT(a.p_hyd_w, k, j, i) = pw_;
     }

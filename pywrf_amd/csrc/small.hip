@@ -350,12 +350,19 @@ struct AdvMuT {
   int k1, kde, j0, j1, i0, i1;
 };
 
+// Operands are loaded into registers before the arithmetic so that the loads
+// of one level are in flight together.
 __device__ inline float mut_dvdxi(const Dims& D, const AdvMuT& a, int k, int j, int i) {
-  float vy = (AT(a.v, k, j + 1, i) + AT2(a.muv, j + 1, i) * AT(a.v_1, k, j + 1, i) * AT2(a.msfvx_inv, j + 1, i)) -
-             (AT(a.v, k, j, i) + AT2(a.muv, j, i) * AT(a.v_1, k, j, i) * AT2(a.msfvx_inv, j, i));
-  float ux = (AT(a.u, k, j, i + 1) + AT2(a.muu, j, i + 1) * AT(a.u_1, k, j, i + 1) / AT2(a.msfuy, j, i + 1)) -
-             (AT(a.u, k, j, i) + AT2(a.muu, j, i) * AT(a.u_1, k, j, i) / AT2(a.msfuy, j, i));
-  return AT2(a.msftx, j, i) * AT2(a.msfty, j, i) * (a.rdy * vy + a.rdx * ux);
+  const float vn = AT(a.v, k, j + 1, i), mvn = AT2(a.muv, j + 1, i), v1n = AT(a.v_1, k, j + 1, i);
+  const float xn = AT2(a.msfvx_inv, j + 1, i);
+  const float vs = AT(a.v, k, j, i), mvs = AT2(a.muv, j, i), v1s = AT(a.v_1, k, j, i), xs = AT2(a.msfvx_inv, j, i);
+  const float ue = AT(a.u, k, j, i + 1), mue = AT2(a.muu, j, i + 1), u1e = AT(a.u_1, k, j, i + 1);
+  const float ye = AT2(a.msfuy, j, i + 1);
+  const float uw = AT(a.u, k, j, i), muw = AT2(a.muu, j, i), u1w = AT(a.u_1, k, j, i), yw = AT2(a.msfuy, j, i);
+  const float mx = AT2(a.msftx, j, i), my = AT2(a.msfty, j, i);
+  float vy = (vn + mvn * v1n * xn) - (vs + mvs * v1s * xs);
+  float ux = (ue + mue * u1e / ye) - (uw + muw * u1w / yw);
+  return mx * my * (a.rdy * vy + a.rdx * ux);
 }
 
 // Column block: a 512-thread workgroup owns TI consecutive i-columns of one
@@ -368,9 +375,15 @@ __device__ inline float mut_dvdxi(const Dims& D, const AdvMuT& a, int k, int j, 
 #define TB 512
 #define TG (TB / TI)
 
+#define CB_MAXNZ 128   // column blocks hold at most 64 KB of per-level LDS, so nz <= 128
+
 __global__ void __launch_bounds__(TB) k_advance_mu_t(Dims D, AdvMuT a) {
   extern __shared__ float sm[];
+  __shared__ float s_dnw[CB_MAXNZ], s_fnm[CB_MAXNZ], s_fnp[CB_MAXNZ], s_rdnw[CB_MAXNZ];
   const int nz = D.nz;
+  for (int t = threadIdx.x; t < nz; t += TB) {
+    s_dnw[t] = a.dnw[t]; s_fnm[t] = a.fnm[t]; s_fnp[t] = a.fnp[t]; s_rdnw[t] = a.rdnw[t];
+  }
   float* s_dv = sm;             // [nz][TI] dvdxi
   float* s_ww = sm + nz * TI;   // [nz][TI] eta velocity before subtracting ww_1
   const int lane = threadIdx.x % TI, grp = threadIdx.x / TI;
@@ -397,7 +410,7 @@ __global__ void __launch_bounds__(TB) k_advance_mu_t(Dims D, AdvMuT a) {
   }
   __syncthreads();
   if (grp == 0 && h) {
-    float dmdt = torch_outer_sum(nz, [&](int k) { return k < a.k1 ? a.dnw[k] * s_dv[k * TI + lane] : 0.0f; });
+    float dmdt = torch_outer_sum(nz, [&](int k) { return k < a.k1 ? s_dnw[k] * s_dv[k * TI + lane] : 0.0f; });
     float mt = a.mu_tend[q2];
     float mu_old = a.mu_in[q2];
     float mu = mu_old + a.dts * (dmdt + mt);
@@ -409,7 +422,7 @@ __global__ void __launch_bounds__(TB) k_advance_mu_t(Dims D, AdvMuT a) {
     float wk = a.ww_in[D.ix(0, j, i)];
     s_ww[lane] = wk;
     for (int k = 1; k < a.k1; ++k) {
-      wk = wk - a.dnw[k - 1] * (dmdt + s_dv[(k - 1) * TI + lane] + mt) / my;
+      wk = wk - s_dnw[k - 1] * (dmdt + s_dv[(k - 1) * TI + lane] + mt) / my;
       s_ww[k * TI + lane] = wk;
     }
   }
@@ -424,19 +437,22 @@ __global__ void __launch_bounds__(TB) k_advance_mu_t(Dims D, AdvMuT a) {
       a.t_ave[q] = a.t_ave_in[q];
       continue;
     }
-    auto wwn = [&](int kk) { return s_ww[kk * TI + lane] - AT(a.ww_1, kk, j, i); };
-    a.ww[q] = wwn(k);
-    float t = a.t_in[q];
-    a.t_ave[q] = t + 0.0f;
-    t = t + my * a.dts * a.ft[q];
-    auto wdtn = [&](int kk) -> float {
-      if (kk <= 0 || kk >= a.k1) return 0.0f;
-      return wwn(kk) * (a.fnm[kk] * AT(a.t_1, kk, j, i) + a.fnp[kk] * AT(a.t_1, kk - 1, j, i));
-    };
-    float t1 = a.t_1[q];
-    float yterm = a.hrdy * (AT(a.v, k, j + 1, i) * (AT(a.t_1, k, j + 1, i) + t1) - AT(a.v, k, j, i) * (t1 + AT(a.t_1, k, j - 1, i)));
-    float xterm = a.hrdx * (AT(a.u, k, j, i + 1) * (AT(a.t_1, k, j, i + 1) + t1) - AT(a.u, k, j, i) * (t1 + AT(a.t_1, k, j, i - 1)));
-    t = t - a.dts * my * (mx * (yterm + xterm) + a.rdnw[k] * (wdtn(k + 1) - wdtn(k)));
+    const float tin = a.t_in[q], ftq = a.ft[q], t1 = a.t_1[q], w1k = a.ww_1[q];
+    const float vn = AT(a.v, k, j + 1, i), vs = AT(a.v, k, j, i), ue = AT(a.u, k, j, i + 1), uw = AT(a.u, k, j, i);
+    const float tn = AT(a.t_1, k, j + 1, i), ts = AT(a.t_1, k, j - 1, i);
+    const float te = AT(a.t_1, k, j, i + 1), tw = AT(a.t_1, k, j, i - 1);
+    const bool up = k + 1 < a.k1, dn = k > 0;   // where the eta fluxes wdtn(k + 1), wdtn(k) are nonzero
+    const float w1u = up ? AT(a.ww_1, k + 1, j, i) : 0.0f, t1u = up ? AT(a.t_1, k + 1, j, i) : 0.0f;
+    const float t1d = dn ? AT(a.t_1, k - 1, j, i) : 0.0f;
+    const float wwk = s_ww[k * TI + lane] - w1k;
+    a.ww[q] = wwk;
+    a.t_ave[q] = tin + 0.0f;
+    float t = tin + my * a.dts * ftq;
+    const float wdu = up ? (s_ww[(k + 1) * TI + lane] - w1u) * (s_fnm[k + 1] * t1u + s_fnp[k + 1] * t1) : 0.0f;
+    const float wdk = dn ? wwk * (s_fnm[k] * t1 + s_fnp[k] * t1d) : 0.0f;
+    float yterm = a.hrdy * (vn * (tn + t1) - vs * (t1 + ts));
+    float xterm = a.hrdx * (ue * (te + t1) - uw * (t1 + tw));
+    t = t - a.dts * my * (mx * (yterm + xterm) + s_rdnw[k] * (wdu - wdk));
     a.t[q] = t;
   }
 }
@@ -487,7 +503,11 @@ struct AdvW {
 // (all threads): Rayleigh damping and the new phi, written back coalesced.
 __global__ void __launch_bounds__(TB) k_advance_w(Dims D, AdvW a) {
   extern __shared__ float sm[];
+  __shared__ float s_fnm[CB_MAXNZ], s_fnp[CB_MAXNZ], s_rdnw[CB_MAXNZ], s_rdn[CB_MAXNZ];
   const int nz = D.nz, k1 = a.k1;
+  for (int t = threadIdx.x; t < nz; t += TB) {
+    s_fnm[t] = a.fnm[t]; s_fnp[t] = a.fnp[t]; s_rdnw[t] = a.rdnw[t]; s_rdn[t] = a.rdn[t];
+  }
   float* s_t2a = sm;
   float* s_rhs = sm + 1 * nz * TI;
   float* s_w = sm + 2 * nz * TI;
@@ -505,11 +525,8 @@ __global__ void __launch_bounds__(TB) k_advance_w(Dims D, AdvW a) {
     mut = a.mut[q2]; muts = a.muts[q2]; muave = a.muave[q2]; my = a.msfty[q2]; mx = a.msftx[q2];
   }
   const float mi = 1.0f / my;
-  auto wdwn = [&](int k) {
-    return 0.5f * (AT(a.ww, k, j, i) + AT(a.ww, k - 1, j, i)) * a.rdnw[k - 1] *
-           (AT(a.ph_1, k, j, i) - AT(a.ph_1, k - 1, j, i) + AT(a.phb, k, j, i) - AT(a.phb, k - 1, j, i));
-  };
-  // phase 1
+  __syncthreads();   // level arrays staged
+  // phase 1 (operands loaded before the arithmetic, so one level's loads are in flight together)
   for (int k = grp; k < nz; k += TG) {
     if (!vi) continue;
     long q = D.ix(k, j, i);
@@ -519,27 +536,41 @@ __global__ void __launch_bounds__(TB) k_advance_w(Dims D, AdvW a) {
       a.ph[q] = a.ph_in[q];
       continue;
     }
-    float t = a.t_2ave_in[q];
-    if (k < k1) {
-      t = 0.5f * (a.ep1 * a.t_2[q] + a.em1 * t);
-      t = (t + muave * a.t0) / (muts * (a.t0 + a.t_1[q]));
+    const bool lo = k < k1, rk = k >= 1 && k <= k1, wd = rk && k < k1;
+    const float t2i = a.t_2ave_in[q], t2 = lo ? a.t_2[q] : 0.0f, t1 = lo ? a.t_1[q] : 0.0f;
+    const float pht = rk ? a.ph_tend[q] : 0.0f, win = rk ? a.w_in[q] : 0.0f, phi = rk ? a.ph_in[q] : 0.0f;
+    const float ca = a.a[q], cal = a.alpha[q], cga = a.gamma[q];
+    float wwp = 0.0f, ww0 = 0.0f, wwm = 0.0f, p1p = 0.0f, p10 = 0.0f, p1m = 0.0f, pbp = 0.0f, pb0 = 0.0f, pbm = 0.0f;
+    if (wd) {
+      wwp = AT(a.ww, k + 1, j, i); ww0 = AT(a.ww, k, j, i); wwm = AT(a.ww, k - 1, j, i);
+      p1p = AT(a.ph_1, k + 1, j, i); p10 = AT(a.ph_1, k, j, i); p1m = AT(a.ph_1, k - 1, j, i);
+      pbp = AT(a.phb, k + 1, j, i); pb0 = AT(a.phb, k, j, i); pbm = AT(a.phb, k - 1, j, i);
+    }
+    float t = t2i;
+    if (lo) {
+      t = 0.5f * (a.ep1 * t2 + a.em1 * t);
+      t = (t + muave * a.t0) / (muts * (a.t0 + t1));
     }
     a.t_2ave[q] = t;
     s_t2a[k * TI + lane] = t;
     float r = 0.0f;
-    if (k >= 1 && k <= k1) {
-      r = a.dts * (a.ph_tend[q] + a.hgem1 * a.w_in[q]);
-      if (k < k1) r = r - a.dts * (a.fnm[k] * wdwn(k + 1) + a.fnp[k] * wdwn(k));
-      r = a.ph_in[q] + my * r / mut;
+    if (rk) {
+      r = a.dts * (pht + a.hgem1 * win);
+      if (wd) {
+        // wdwn(kk) = 0.5 (ww[kk] + ww[kk-1]) rdnw[kk-1] (ph_1[kk] - ph_1[kk-1] + phb[kk] - phb[kk-1])
+        const float wup = 0.5f * (wwp + ww0) * s_rdnw[k] * (p1p - p10 + pbp - pb0);
+        const float wdn = 0.5f * (ww0 + wwm) * s_rdnw[k - 1] * (p10 - p1m + pb0 - pbm);
+        r = r - a.dts * (s_fnm[k] * wup + s_fnp[k] * wdn);
+      }
+      r = phi + my * r / mut;
     }
     s_rhs[k * TI + lane] = r;
-    s_a[k * TI + lane] = a.a[q];
-    s_al[k * TI + lane] = a.alpha[q];
-    s_ga[k * TI + lane] = a.gamma[q];
+    s_a[k * TI + lane] = ca;
+    s_al[k * TI + lane] = cal;
+    s_ga[k * TI + lane] = cga;
   }
   __syncthreads();
   auto RHS = [&](int k) -> float { return s_rhs[k * TI + lane]; };   // rhs[0] is 0
-  auto PH = [&](int k) { return AT(a.ph_in, k, j, i); };
   // phase 2
   for (int k = grp; k < nz; k += TG) {
     if (!h) continue;
@@ -558,20 +589,22 @@ __global__ void __launch_bounds__(TB) k_advance_w(Dims D, AdvW a) {
       float tx = (AT2(a.ht, j, i + 1) - htc) * U3(j, i + 1) + (htc - AT2(a.ht, j, i - 1)) * U3(j, i);
       w = my * 0.5f * a.rdy * ty + mx * 0.5f * a.rdx * tx;
     } else if (k < k1) {
-      float c2 = a.c2a[q], c2m = AT(a.c2a, k - 1, j, i);
-      float X = a.hdtsg * a.rdn[k] *
-                (c2 * a.rdnw[k] / mut * (a.ep1 * (RHS(k + 1) - RHS(k)) + a.em1 * (PH(k + 1) - PH(k))) -
-                 c2m * a.rdnw[k - 1] / mut * (a.ep1 * (RHS(k) - RHS(k - 1)) + a.em1 * (PH(k) - PH(k - 1))));
-      float Y = a.rdn[k] * (c2 * a.alt[q] * s_t2a[k * TI + lane] -
-                            c2m * AT(a.alt, k - 1, j, i) * s_t2a[(k - 1) * TI + lane]) - muave;
-      w = a.w_in[q] + a.dts * a.rw_tend[q] + mi * a.cqw[q] * X + a.dtsg * mi * Y;
+      const float c2 = a.c2a[q], c2m = AT(a.c2a, k - 1, j, i), al0 = a.alt[q], alm = AT(a.alt, k - 1, j, i);
+      const float win = a.w_in[q], rwt = a.rw_tend[q], cq = a.cqw[q];
+      const float php = AT(a.ph_in, k + 1, j, i), ph0 = a.ph_in[q], phm = AT(a.ph_in, k - 1, j, i);
+      float X = a.hdtsg * s_rdn[k] *
+                (c2 * s_rdnw[k] / mut * (a.ep1 * (RHS(k + 1) - RHS(k)) + a.em1 * (php - ph0)) -
+                 c2m * s_rdnw[k - 1] / mut * (a.ep1 * (RHS(k) - RHS(k - 1)) + a.em1 * (ph0 - phm)));
+      float Y = s_rdn[k] * (c2 * al0 * s_t2a[k * TI + lane] - c2m * alm * s_t2a[(k - 1) * TI + lane]) - muave;
+      w = win + a.dts * rwt + mi * cq * X + a.dtsg * mi * Y;
     } else if (k == k1) {
-      float rn = a.rdnw[k1 - 1];
-      float c2m = AT(a.c2a, k1 - 1, j, i);
-      float Z = a.mhdtsg / mut * (rn * rn) * 2.0f * c2m *
-                    (a.ep1 * (RHS(k1) - RHS(k1 - 1)) + a.em1 * (PH(k1) - PH(k1 - 1))) -
-                a.dtsg * (2.0f * rn * c2m * AT(a.alt, k1 - 1, j, i) * s_t2a[(k1 - 1) * TI + lane] + muave);
-      w = a.w_in[q] + a.dts * a.rw_tend[q] + mi * Z;
+      const float c2m = AT(a.c2a, k1 - 1, j, i), alm = AT(a.alt, k1 - 1, j, i);
+      const float win = a.w_in[q], rwt = a.rw_tend[q];
+      const float ph0 = a.ph_in[q], phm = AT(a.ph_in, k1 - 1, j, i);
+      float rn = s_rdnw[k1 - 1];
+      float Z = a.mhdtsg / mut * (rn * rn) * 2.0f * c2m * (a.ep1 * (RHS(k1) - RHS(k1 - 1)) + a.em1 * (ph0 - phm)) -
+                a.dtsg * (2.0f * rn * c2m * alm * s_t2a[(k1 - 1) * TI + lane] + muave);
+      w = win + a.dts * rwt + mi * Z;
     } else {
       w = a.w_in[q];
     }
@@ -599,22 +632,20 @@ __global__ void __launch_bounds__(TB) k_advance_w(Dims D, AdvW a) {
   for (int k = grp; k < nz; k += TG) {
     long q = D.ix(k, j, i);
     float w = s_w[k * TI + lane];
-    if (k == 0) {
+    if (k == 0 || k > k1) {
       a.w[q] = w;
       a.ph[q] = a.ph_in[q];
-    } else if (k <= k1) {
-      float hk = (a.ph_1[q] + a.phb[q]) / G;
+    } else {
+      const float p1 = a.ph_1[q], pb = a.phb[q], ws = a.w_save[q];
+      float hk = (p1 + pb) / G;
       float dw = 0.0f;
       if (hk >= hbot) {
         float sn = sinf(a.hpi * (hk - hbot) / a.hdepth);
         dw = a.dampmag * (sn * sn);
       }
-      w = (w - dw * mut * a.w_save[q]) / (1.0f + dw);
+      w = (w - dw * mut * ws) / (1.0f + dw);
       a.w[q] = w;
       a.ph[q] = s_rhs[k * TI + lane] + my * 0.5f * a.dts * G * a.ep1 * w / muts;
-    } else {
-      a.w[q] = w;
-      a.ph[q] = a.ph_in[q];
     }
   }
 }

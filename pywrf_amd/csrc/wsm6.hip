@@ -220,7 +220,7 @@ struct Wsm6 {
   const float *den, *pii, *p, *delz;
   float *rain, *rainncv, *snow, *snowncv, *graupel, *graupelncv, *sr;
   float delt;
-  int km, j0, j1, i0, i1;
+  int km, j0, j1, i0, i1, ncol, per;
 };
 
 // rates limited so that a sink cannot remove more than value
@@ -261,8 +261,13 @@ __device__ inline float venfac(float a, float b, float c) {
 
 __global__ void __launch_bounds__(WAVE) k_wsm6(Dims D, Wsm6 a) {
   __shared__ SedLds L;
-  const int ncol = a.i1 - a.i0;
-  const int j = a.j0 + blockIdx.x / ncol, i = a.i0 + blockIdx.x % ncol;
+  // XCD-aware column order: workgroups are dealt to the 8 XCDs round-robin,
+  // so XCD x takes the contiguous run [x * per, (x + 1) * per) of columns and
+  // the columns that share a cache line are served by one L2.
+  const int ni = a.i1 - a.i0;
+  const int c = (int)(blockIdx.x % PW_XCDS) * a.per + (int)(blockIdx.x / PW_XCDS);
+  if (c >= a.ncol) return;
+  const int j = a.j0 + c / ni, i = a.i0 + c % ni;
   const int k = threadIdx.x, km = a.km;
   const bool act = k < km;
   const float qmin = F(epsilon), dtcld = a.delt;
@@ -621,7 +626,9 @@ extern "C" int pywrf_wsm6(const pywrf_grid_t* g, float* th, float* q, float* qc,
   if (g->kts != 0 || a.km < 4 || a.km + 1 > WAVE || g->kte > g->nz) return PYWRF_E_RANGE;
   long ncol = (long)(a.j1 - a.j0) * (a.i1 - a.i0);
   if (ncol <= 0) return 0;
-  hipLaunchKernelGGL(k_wsm6, dim3((unsigned)ncol), dim3(WAVE), 0, (hipStream_t)s, dims_of(g), a);
+  a.ncol = (int)ncol;
+  a.per = (int)((ncol + PW_XCDS - 1) / PW_XCDS);
+  hipLaunchKernelGGL(k_wsm6, dim3((unsigned)(a.per * PW_XCDS)), dim3(WAVE), 0, (hipStream_t)s, dims_of(g), a);
   return (int)hipGetLastError();
 }
 
