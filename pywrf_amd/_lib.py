@@ -14,7 +14,7 @@ import torch
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-LIB_PATH = os.path.join(HERE, "libpywrf_amd.so")
+LIB_PATH = os.environ.get("PYWRF_AMD_LIB") or os.path.join(HERE, "libpywrf_amd.so")   # override: another build
 HEADER = os.path.join(ROOT, "include", "pywrf_amd.h")
 
 PYWRF_E_ARG = -1

@@ -169,3 +169,30 @@ __device__ inline float torch_outer_sum(int n, Fn term) {
   for (int j = 1; j < 4; ++j) acc[0] += acc[j];
   return acc[0];
 }
+
+// torch_outer_sum for a sum whose terms are +0 outside [lo, hi]: only the
+// terms inside are evaluated, in their block positions.  Skipping +0 terms
+// leaves every partial sum unchanged (the accumulators start at +0 and only
+// non-negative or NaN terms are added), so the result is the same float.
+template <class Fn>
+__device__ inline float torch_outer_sum_range(int n, int lo, int hi, Fn term) {
+  int lg = 0;
+  while ((1 << lg) < n) ++lg;
+  const int lp = (lg / 4) > 4 ? (lg / 4) : 4;
+  const int step = 1 << lp, mask = step - 1;
+  float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  int i = 0;
+  while (i + step <= n) {
+    const int e = i + step;
+    for (int m = imax(i, lo), me = imin(e, hi + 1); m < me; ++m) acc[0] += term(m);
+    i = e;
+    for (int j = 1; j < 4; ++j) {
+      acc[j] += acc[j - 1];
+      acc[j - 1] = 0.0f;
+      if ((i & (mask << (j * lp))) != 0) break;
+    }
+  }
+  for (int m = imax(i, lo), me = imin(n, hi + 1); m < me; ++m) acc[0] += term(m);
+  for (int j = 1; j < 4; ++j) acc[0] += acc[j];
+  return acc[0];
+}

@@ -85,34 +85,38 @@ struct SedLds {
   float red[2];
 };
 
+__device__ inline float lane_value(float v, int m) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), m));
+}
+
 // nislfv_rain_plm (two == false) and nislfv_rain_plm6 (two == true) for one
 // column; lane k holds level k, km levels are active.  Inputs dz, den,
 // denfac, tk, ww (fall speed), q1 (and q2) are per-lane values (zero for
 // k >= km); returns the new q1 (q2) in place and the surface precipitation
 // on lane 0 in prec1 (prec2).  wrf_physics.py:1216-1584.
+//
+// The serial parts run as wave-wide operations: the sign test of the column
+// sums is a ballot (the inputs are non-negative or NaN), the interface heights
+// are each lane's in-order sum of the layers below it, the backward CFL
+// limiter walks the lanes top-down with readlane, and the overlap remap sums,
+// in the reference's summation order, only the source cells that can overlap
+// the target cell when the departure heights are increasing (otherwise, or
+// with non-finite values, every cell, as the reference does).
 __device__ void sediment(SedLds& L, int k, int km, float dz, float den, float denfac, float tk, float ww, float& q1,
                          float& q2, bool two, float dt, int iter, float& prec1, float& prec2) {
-  // all-level column sums (only their sign is used)
-  if (k == 0) {
-    L.red[0] = 0.0f;
-  }
+  // sign of the all-level column sums: > 0 iff no NaN and some value > 0
+  const bool anynan = __ballot((q1 != q1) || (two && q2 != q2)) != 0;
+  const bool anypos = __ballot((q1 > 0.0f) || (two && q2 > 0.0f)) != 0;
+  const bool cond = !anynan && anypos;
   L.dz[k] = dz;
-  L.qa[k] = q1;
-  L.qa2[k] = two ? q2 : 0.0f;
   __syncthreads();
-  if (k == 0) {
-    float s = 0.0f, s2 = 0.0f;
-    for (int m = 0; m <= km; ++m) s += L.qa[m];
-    if (two)
-      for (int m = 0; m <= km; ++m) s2 += L.qa2[m];
-    L.red[0] = two ? s + s2 : s;
-  }
   // heights of the level interfaces, summed bottom-up in order
   float zi = 0.0f;
-  if (k <= km)
-    for (int m = 0; m < k; ++m) zi = zi + L.dz[m];
-  __syncthreads();
-  bool cond = L.red[0] > 0.0f;
+  if (k <= km) {
+#pragma unroll
+    for (int m = 0; m < WAVE; ++m)
+      if (m < k) zi = zi + L.dz[m];
+  }
   L.zi[k] = zi;
   const float wd = ww;
   float was = 0.0f;
@@ -132,16 +136,22 @@ __device__ void sediment(SedLds& L, int k, int km, float dz, float den, float de
       else wi = L.ww[km - 1] + 0.0f;
       if (k >= 1 && k < km && L.ww[k] == 0.0f) wi = L.ww[k - 1];
     }
-    L.wi[k] = wi;
-    __syncthreads();
-    if (k == 0) {  // backward CFL limiter
+    // backward CFL limiter from the top: level m takes the already limited
+    // speed of level m + 1 (every lane runs the same uniform chain)
+    {
+      float wn = lane_value(wi, km), wnew = wi;
       for (int m = km - 1; m >= 0; --m) {
-        float decfl = (L.wi[m + 1] - L.wi[m]) * dt / L.dz[m];
-        if (decfl > 0.05f) L.wi[m] = L.wi[m + 1] - 0.05f * L.dz[m] / dt;
+        float wm = lane_value(wi, m);
+        const float dzm = lane_value(dz, m);
+        float decfl = (wn - wm) * dt / dzm;
+        if (decfl > 0.05f) wm = wn - 0.05f * dzm / dt;
+        wnew = k == m ? wm : wnew;
+        wn = wm;
       }
+      wi = wnew;
     }
+    za = zi - wi * dt;
     __syncthreads();
-    za = zi - L.wi[k] * dt;
     L.za[k] = za;
     __syncthreads();
     if (k < km) dza = L.za[k + 1] - za;
@@ -181,18 +191,31 @@ __device__ void sediment(SedLds& L, int k, int km, float dz, float den, float de
   // remap onto the fixed grid; the top source cell is dropped
   __syncthreads();
   L.dza[k] = dza;
-  L.qa[k] = (k == km - 1) ? 0.0f : qa;
-  L.qa2[k] = (k == km - 1) ? 0.0f : qa2;
+  const float qs1 = (k == km - 1) ? 0.0f : qa, qs2 = (k == km - 1) ? 0.0f : qa2;
+  L.qa[k] = qs1;
+  L.qa2[k] = qs2;
   __syncthreads();
+  // departure heights increasing and all values finite: the overlaps of a
+  // target cell are a contiguous run of source cells, the rest exactly +0
+  const bool bad = (k < km - 1 && !(L.za[k + 1] > za)) || !isfinite(za) || !isfinite(qs1) || !isfinite(qs2);
+  const bool ordered = __ballot(bad) == 0;
+  auto zah_of = [&](int m) { return (m < km - 1) ? L.za[m + 1] : L.za[km - 1] + 3800.0f; };
   if (k < km) {
-    float zlo = L.zi[k];
-    float zhi = (k < km - 1) ? L.zi[k + 1] : L.zi[km - 1] + 3800.0f;
-    auto ov = [&](int m) {
-      float zah = (m < km - 1) ? L.za[m + 1] : L.za[km - 1] + 3800.0f;
-      return tmax(0.0f, tmin(zhi, zah) - tmax(zlo, L.za[m]));
-    };
-    float s1 = torch_outer_sum(km, [&](int m) { return L.qa[m] * ov(m); });
-    float s2 = two ? torch_outer_sum(km, [&](int m) { return L.qa2[m] * ov(m); }) : 0.0f;
+    const float zlo = L.zi[k];
+    const float zhi = (k < km - 1) ? L.zi[k + 1] : L.zi[km - 1] + 3800.0f;
+    auto ov = [&](int m) { return tmax(0.0f, tmin(zhi, zah_of(m)) - tmax(zlo, L.za[m])); };
+    int lo = 0, hi = km - 1;
+    if (ordered) {
+      // first source cell whose top is above zlo, last whose bottom is below zhi
+      int a = 0, b = km;
+      while (a < b) { int c = (a + b) >> 1; if (zah_of(c) > zlo) b = c; else a = c + 1; }
+      lo = a;
+      a = 0; b = km;
+      while (a < b) { int c = (a + b) >> 1; if (L.za[c] < zhi) a = c + 1; else b = c; }
+      hi = a - 1;
+    }
+    float s1 = torch_outer_sum_range(km, lo, hi, [&](int m) { return L.qa[m] * ov(m); });
+    float s2 = two ? torch_outer_sum_range(km, lo, hi, [&](int m) { return L.qa2[m] * ov(m); }) : 0.0f;
     q1 = cond ? s1 / dz : 0.0f;
     if (two) q2 = cond ? s2 / dz : 0.0f;
   }
@@ -200,6 +223,7 @@ __device__ void sediment(SedLds& L, int k, int km, float dz, float den, float de
     float p1 = prec1, p2 = prec2;
     for (int m = 0; m < km; ++m) {
       float z0 = L.za[m], z1 = L.za[m + 1];
+      if (ordered && !(z0 < 0.0f)) break;   // increasing heights: no later cell reaches the ground
       if (z0 < 0.0f && z1 < 0.0f) {
         p1 = p1 + L.qa[m] * L.dza[m];
         if (two) p2 = p2 + L.qa2[m] * L.dza[m];
