@@ -99,6 +99,7 @@ __global__ void k_advect_u(Box b, Dims D, AdvU a, AdvIdx x, Box by, Box bz) {
 extern "C" int pywrf_advect_u(const pywrf_grid_t* g, const float* u, float* tendency, const float* ru,
                               const float* rv, const float* rom, const float* msfux, const float* fzm,
                               const float* fzp, double rdx, double rdy, const float* rdzw, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 4);
   AdvIdx x = {g->ids, g->ide, g->jds, g->jde, imin(g->kte, g->kde - 1)};
   if (g->kts != 0 || x.ktf + 1 > g->nz) return PYWRF_E_RANGE;
@@ -182,6 +183,7 @@ extern "C" int pywrf_advect_v(const pywrf_grid_t* g, const float* v, float* tend
                               const float* rv, const float* rom, const float* msfvx, const float* msfvy,
                               const float* fzm, const float* fzp, double rdx, double rdy, const float* rdzw,
                               pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 4);
   AdvIdx x = {g->ids, g->ide, g->jds, g->jde, imin(g->kte, g->kde - 1)};
   if (g->kts != 0 || x.ktf + 1 > g->nz) return PYWRF_E_RANGE;
@@ -264,6 +266,7 @@ __global__ void k_advect_w(Box b, Dims D, AdvW a, AdvIdx x, Box by, Box bx) {
 extern "C" int pywrf_advect_w(const pywrf_grid_t* g, const float* w, float* tendency, const float* ru,
                               const float* rv, const float* rom, const float* msftx, const float* fzm,
                               const float* fzp, double rdx, double rdy, const float* rdzu, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 4);
   AdvIdx x = {g->ids, g->ide, g->jds, g->jde, imin(g->kte, g->kde - 1)};
   if (g->kts != 0 || x.ktf + 1 > g->nz) return PYWRF_E_RANGE;
@@ -348,6 +351,7 @@ int pw_advect_scalar_box(const pywrf_grid_t* g, const float* field, float* tende
 extern "C" int pywrf_advect_scalar(const pywrf_grid_t* g, const float* field, float* tendency, const float* ru,
                                    const float* rv, const float* rom, const float* msftx, const float* fzm,
                                    const float* fzp, double rdx, double rdy, const float* rdzw, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   return pw_advect_scalar_box(g, field, tendency, ru, rv, rom, msftx, fzm, fzp, F(rdx), F(rdy), rdzw, 0, s);
 }
 

@@ -102,6 +102,7 @@ extern "C" int pywrf_relax_bdytend(const pywrf_grid_t* g, const float* field, fl
                                    const float* bxe, const float* bys, const float* bye, const float* btxs,
                                    const float* btxe, const float* btys, const float* btye, char variable_in,
                                    double dtbc, const float* fcx, const float* gcx, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   Relax a;
   a.field = field; a.mut = nullptr; a.mbox = mkbox(0, 0, 0, 0, 0, 0); a.tend = field_tend;
   a.b = {bxs, bxe, bys, bye, btxs, btxe, btys, btye};
@@ -114,6 +115,7 @@ extern "C" int pywrf_relax_bdy_scalar(const pywrf_grid_t* g, float* scalar_tend,
                                       const float* bye, const float* btxs, const float* btxe, const float* btys,
                                       const float* btye, double dtbc, const float* fcx, const float* gcx,
                                       pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   Relax a;
   a.field = scalar; a.mut = mu; a.tend = scalar_tend;
   a.mbox = mkbox(g->kts, g->kte - 1, imax(g->jts - 1, g->jds), imin(g->jte + 1, g->jde - 1), imax(g->its - 1, g->ids),
@@ -139,6 +141,7 @@ __global__ void k_spec_bdytend(Box all, int w, Dims D, float* tend, Bdy b, int i
 
 extern "C" int pywrf_spec_bdytend(const pywrf_grid_t* g, float* field_tend, const float* btxs, const float* btxe,
                                   const float* btys, const float* btye, char variable_in, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   int ibe, jbe, ktf;
   bounds_of(variable_in, g, ibe, jbe, ktf);
   if (g->ids < 0 || ibe > g->nx || g->jds < 0 || jbe > g->ny || ktf > g->nz) return PYWRF_E_RANGE;
@@ -150,6 +153,7 @@ extern "C" int pywrf_spec_bdytend(const pywrf_grid_t* g, float* field_tend, cons
 
 extern "C" int pywrf_spec_bdy_scalar(const pywrf_grid_t* g, float* scalar_tend, const float* btxs,
                                      const float* btxe, const float* btys, const float* btye, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   return pywrf_spec_bdytend(g, scalar_tend, btxs, btxe, btys, btye, 'q', s);
 }
 
@@ -169,6 +173,7 @@ __global__ void k_spec_bdyupdate(Box all, int w, Dims D, float* f, const float* 
 
 extern "C" int pywrf_spec_bdyupdate(const pywrf_grid_t* g, float* field, const float* field_tend, double dt,
                                     char variable, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   char v = variable;
   if (v == 'U') v = 'u'; else if (v == 'V') v = 'v'; else if (v == 'M') v = 'm'; else if (v == 'H') v = 'h';
   int itf = imin(g->ite, g->ide - 1), jtf = imin(g->jte, g->jde - 1), ktf = g->kde - 1, jbe = g->jde - 1;
@@ -208,6 +213,7 @@ __global__ void k_spec_bdyupdate_ph(Box all, int w, Dims D, const float* ph_save
 extern "C" int pywrf_spec_bdyupdate_ph(const pywrf_grid_t* g, const float* ph_save, float* field,
                                        const float* field_tend, const float* mu_tend, const float* muts, double dt,
                                        char variable_in, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   char v = variable_in;
   if (v == 'U') v = 'u'; else if (v == 'V') v = 'v'; else if (v == 'M') v = 'm'; else if (v == 'H') v = 'h';
   int ibe, jbe, ktf;
@@ -232,6 +238,7 @@ __global__ void k_zero_grad(Box all, int w, Dims D, float* f, int ibs, int ibe, 
 }
 
 extern "C" int pywrf_zero_grad_bdy(const pywrf_grid_t* g, float* field, char variable_in, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   char v = variable_in;
   if (v == 'U') v = 'u'; else if (v == 'V') v = 'v';
   int ibe = g->ide - 1, jbe = g->jde - 1, ktf = g->kde - 1;
@@ -254,6 +261,7 @@ __global__ void k_mass_weight(Box b, Dims D, const float* field, const float* mu
 
 extern "C" int pywrf_mass_weight(const pywrf_grid_t* g, const float* field, const float* mut, float* rfield,
                                  pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   Box b = mkbox(g->kts, g->kte, g->jts, g->jte, g->its, g->ite);
   if (b.k0 < 0 || b.k1 > g->nz || b.j0 < 0 || b.j1 > g->ny || b.i0 < 0 || b.i1 > g->nx) return PYWRF_E_RANGE;
   PW_LAUNCH_BOX(k_mass_weight, b, s, dims_of(g), field, mut, rfield);
@@ -284,6 +292,7 @@ __global__ void k_flow_dep(Box all, int w, Dims D, float* f, const float* u, con
 
 extern "C" int pywrf_flow_dep_bdy(const pywrf_grid_t* g, float* field, const float* u, const float* v,
                                   pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 1);
   int itf = imin(g->ite, g->ide - 1), jtf = imin(g->jte, g->jde - 1), ktf = g->kde - 1;
   if (ktf > g->nz) return PYWRF_E_RANGE;
@@ -313,6 +322,7 @@ extern "C" int pywrf_spec_bdy_final(const pywrf_grid_t* g, float* field, const f
                                     const float* bxs, const float* bxe, const float* bys, const float* bye,
                                     const float* btxs, const float* btxe, const float* btys, const float* btye,
                                     char variable_in, double dtbc, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   char v = variable_in;
   switch (v) {
     case 'U': v = 'u'; break;

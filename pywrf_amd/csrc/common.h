@@ -128,12 +128,18 @@ __device__ inline float flux3(float qm2, float qm1, float q0, float qp1, float u
   return f4 + (sgn1(ua) * d) / 12.0f;
 }
 
+// Offsets are 32-bit: every entry point rejects arrays of 2^31 or more points
+// (pw_size_ok), which keeps the address arithmetic of the stencils short.
 struct Dims {
   int nz, ny, nx;
-  __host__ __device__ long ix(int k, int j, int i) const { return ((long)k * ny + j) * nx + i; }
-  __host__ __device__ long ix2(int j, int i) const { return (long)j * nx + i; }
-  __host__ __device__ long plane() const { return (long)ny * nx; }
+  __host__ __device__ int ix(int k, int j, int i) const { return (k * ny + j) * nx + i; }
+  __host__ __device__ int ix2(int j, int i) const { return j * nx + i; }
+  __host__ __device__ int plane() const { return ny * nx; }
 };
+
+__host__ inline bool pw_size_ok(const pywrf_grid_t* g) {
+  return g && g->nz > 0 && g->ny > 0 && g->nx > 0 && (long)g->nz * g->ny * g->nx < 0x7fffffffL;
+}
 
 __host__ inline Dims dims_of(const pywrf_grid_t* g) { Dims d; d.nz = g->nz; d.ny = g->ny; d.nx = g->nx; return d; }
 

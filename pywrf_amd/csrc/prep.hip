@@ -16,7 +16,7 @@
 // Generic index-set check: every index in range and the stencil reach `r`
 // (cells beyond the outermost index used) inside the allocation.
 int pw_check_grid(const pywrf_grid_t* g, int r) {
-  if (!g || g->nz <= 0 || g->ny <= 0 || g->nx <= 0) return PYWRF_E_RANGE;
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   int ilo = imin(g->ids, g->its), ihi = imax(g->ide, g->ite);
   int jlo = imin(g->jds, g->jts), jhi = imax(g->jde, g->jte);
   if (ilo - r < 0 || ihi + r > g->nx || jlo - r < 0 || jhi + r > g->ny) return PYWRF_E_RANGE;
@@ -35,6 +35,7 @@ __global__ void k_calculate_full(Box b, Dims D, float* mut, const float* mub, co
 
 extern "C" int pywrf_calculate_full(const pywrf_grid_t* g, float* mut, const float* mub, const float* mu,
                                     pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 0);
   Box b = mkbox(0, 1, g->jts, imin(g->jte, g->jde - 1), g->its, imin(g->ite, g->ide - 1));
   PW_LAUNCH_COLS(k_calculate_full, b, s, dims_of(g), mut, mub, mu);
@@ -64,6 +65,7 @@ __global__ void k_calc_mu_uv(Box b, Dims D, const float* mu, const float* mub, f
 
 extern "C" int pywrf_calc_mu_uv(const pywrf_grid_t* g, const float* mu, const float* mub, float* muu, float* muv,
                                 pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 1);
   int jtf = imin(g->jte, g->jde - 1), itf = imin(g->ite, g->ide - 1);
   Box b = mkbox(0, 1, g->jts, g->jte, g->its, g->ite);
@@ -87,6 +89,7 @@ extern "C" int pywrf_couple_momentum(const pywrf_grid_t* g, const float* muu, fl
                                      const float* msfu, const float* muv, float* rv, const float* v,
                                      const float* msfv, const float* msfvx_inv, const float* mut, float* rw,
                                      const float* w, const float* msft, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   (void)msfv;
   CHECK(g, 0);
   int ktf = imin(g->kte, g->kde - 1), jtf = imin(g->jte, g->jde - 1), itf = imin(g->ite, g->ide - 1);
@@ -149,6 +152,7 @@ extern "C" int pywrf_calc_ww_cp(const pywrf_grid_t* g, const float* u, const flo
                                 const float* muv, const float* c1h, float* ww, double rdx, double rdy,
                                 const float* msftx, const float* msfuy, const float* msfvx_inv, const float* dnw,
                                 pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 1);
   int ktf = imin(g->kte, g->kde - 1), jtf = imin(g->jte, g->jde - 1), itf = imin(g->ite, g->ide - 1);
   Box b = mkbox(g->kts, ktf, g->jts, jtf, g->its, itf);
@@ -182,6 +186,7 @@ __global__ void k_calc_cq(Box b, Dims D, Box bu, Box bv, Box bw, const float* mo
 
 extern "C" int pywrf_calc_cq(const pywrf_grid_t* g, const float* moist, int n_moist, float* cqu, float* cqv,
                              float* cqw, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 1);
   if (n_moist < 1) return PYWRF_E_ARG;
   int ktf = imin(g->kte, g->kde - 1), jtf = imin(g->jte, g->jde - 1), itf = imin(g->ite, g->ide - 1);
@@ -202,6 +207,7 @@ __global__ void k_calc_alt(Box b, Dims D, float* alt, const float* al, const flo
 
 extern "C" int pywrf_calc_alt(const pywrf_grid_t* g, float* alt, const float* al, const float* alb,
                               pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 0);
   Box b = mkbox(g->kts, imin(g->kte, g->kde - 1), g->jts, imin(g->jte, g->jde - 1), g->its,
                 imin(g->ite, g->ide - 1));
@@ -218,6 +224,7 @@ __global__ void k_calc_php(Box b, Dims D, float* php, const float* ph, const flo
 
 extern "C" int pywrf_calc_php(const pywrf_grid_t* g, float* php, const float* ph, const float* phb,
                               pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 0);
   Box b = mkbox(g->kts, imin(g->kte, g->kde - 1), g->jts, imin(g->jte, g->jde - 1), g->its,
                 imin(g->ite, g->ide - 1));
@@ -284,6 +291,7 @@ static char lower(char v) { return (v >= 'A' && v <= 'Z') ? (char)(v - 'A' + 'a'
 static bool in(char v, const char* set) { for (; *set; ++set) if (*set == v) return true; return false; }
 
 extern "C" int pywrf_set_physical_bc2d(const pywrf_grid_t* g, float* dat, char variable, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   char v = lower(variable);
   if (!in(v, "uvwtxyrp")) return PYWRF_E_ARG;
   int istag = in(v, "uxde") ? 0 : -1, jstag = in(v, "vydf") ? 0 : -1;
@@ -302,6 +310,7 @@ extern "C" int pywrf_set_physical_bc2d(const pywrf_grid_t* g, float* dat, char v
 }
 
 extern "C" int pywrf_set_physical_bc3d(const pywrf_grid_t* g, float* dat, char variable, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   char v = variable;
   if (v == 'U') v = 'u';
   else if (v == 'V') v = 'v';
@@ -400,6 +409,7 @@ static int bc_spec_ok(int ny, int nx, int nz, int k0, int k1, const BcSpec& c) {
 
 extern "C" int pywrf_set_physical_bc_group(const pywrf_grid_t* g3, const pywrf_grid_t* g2, int n, void* const* fields,
                                            const char* variables, const int* is2d, pywrf_stream_t s) {
+  if (!pw_size_ok(g3) || !pw_size_ok(g2)) return PYWRF_E_RANGE;
   if (n < 0 || n > PW_BC_GROUP_MAX || g3->ny != g2->ny || g3->nx != g2->nx) return PYWRF_E_ARG;
   BcJobs J;
   J.n = n;
@@ -540,6 +550,7 @@ extern "C" int pywrf_phy_prep(const pywrf_grid_t* g, const float* mut, const flo
                               float* z_at_w, float* dz8w, float* p_hyd, float* p_hyd_w, const float* dnw,
                               const float* fzm, const float* fzp, double p_top, float* pi_phy_out,
                               float* t_phy_out, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 1);
   if (n_moist < 2 || g->kte != g->nz || g->kde != g->nz || g->kts != 0) return PYWRF_E_RANGE;
   const size_t lds = 2 * (size_t)g->nz * PP_TI * sizeof(float);
@@ -574,6 +585,7 @@ __global__ void k_scale_by_mut(Box b, Dims D, const float* mut, float* a0, float
 extern "C" int pywrf_calculate_phy_tend(const pywrf_grid_t* g, const float* mut, float* rublten, float* rvblten,
                                         float* rthblten, float* rqvblten, float* rqcblten, float* rqiblten,
                                         float* scalar_tend, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 0);
   Box b = mkbox(g->kts, imin(g->kte, g->kde - 1), g->jts, imin(g->jte, g->jde - 1), g->its,
                 imin(g->ite, g->ide - 1));
@@ -613,6 +625,7 @@ __global__ void k_diff_metrics(Box all, Dims D, const float* ph, const float* ph
 extern "C" int pywrf_compute_diff_metrics(const pywrf_grid_t* g, const float* ph, const float* phb, float* z,
                                           float* rdz, float* rdzw, float* zx, float* zy, double rdx, double rdy,
                                           pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 2);
   int ktf = imin(g->kte, g->kde - 1), jtf = imin(g->jte, g->jde - 1), itf = imin(g->ite, g->ide - 1);
   if (ktf + 1 > g->nz) return PYWRF_E_RANGE;
@@ -767,6 +780,7 @@ extern "C" int pywrf_cal_deform_and_div(const pywrf_grid_t* g, const float* u, c
                                         const float* dnw, const float* rdz, const float* rdzw, const float* fnm,
                                         const float* fnp, const float* cf, const float* zx, const float* zy,
                                         pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 2);
   if (g->kts != 0 || g->kte != g->nz || g->kde != g->nz) return PYWRF_E_RANGE;
   DefCoef c;
@@ -889,6 +903,7 @@ extern "C" int pywrf_calculate_km_kh(const pywrf_grid_t* g, float* xkmh, float* 
                                      const float* p, const float* moist, double dx, double dy, const float* rdz,
                                      const float* rdzw, const float* cf, const float* msftx, const float* msfty,
                                      float* BN2_out, float* xkmh_out, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 1);
   if (g->kts != 0 || g->kte > g->nz) return PYWRF_E_RANGE;
   Dims D = dims_of(g);
@@ -929,6 +944,7 @@ extern "C" int pywrf_update_phy_ten(const pywrf_grid_t* g, float* rt_tendf, floa
                                     float* moist_tendf, const float* rthblten, const float* rublten,
                                     const float* rvblten, const float* rqvblten, const float* rqcblten,
                                     const float* rqiblten, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 1);
   Dims D = dims_of(g);
   long st = (long)D.nz * D.plane();

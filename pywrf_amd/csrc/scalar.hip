@@ -73,6 +73,7 @@ extern "C" int pywrf_rk_scalar_tend(const pywrf_grid_t* g, int scs, int sce, int
                                     const float* msftx, const float* msfty, double rdx, double rdy,
                                     const float* rdn, const float* rdnw, double kvdif, const float* xkmhd,
                                     pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   // kvdif / prandtl is the reference's kvdq (formed in double, rounded once)
   return pw_rk_scalar_tend_fused(g, scs, sce, rk_step, ru, rv, ww, mut, alt, scalar, scalar_tends, advect_tend,
                                  h_tendency, z_tendency, base, moist_step, fnm, fnp, msfux, msfuy, msfvx, msfvx_inv,
@@ -118,6 +119,7 @@ extern "C" int pywrf_rk_update_scalar(const pywrf_grid_t* g, int scs, int sce, f
                                       const float* sc_tend, const float* advect_tend, const float* msfty,
                                       const float* mu_old, const float* mu_new, const float* mu_base, int rk_step,
                                       double dt, int spec_zone, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 0);
   if (g->kte > g->nz || g->kde > g->nz || scs < 0 || sce < scs) return PYWRF_E_RANGE;
   UpdS a;
@@ -152,6 +154,7 @@ __global__ void k_calc_p_rho_phi(Box b, Dims D, const float* qv, float* al, cons
 extern "C" int pywrf_calc_p_rho_phi(const pywrf_grid_t* g, const float* moist, float* al, const float* alb,
                                     const float* mu, const float* muts, const float* ph, float* p, const float* pb,
                                     const float* t, double p0, double t0, const float* rdnw, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 0);
   Box b = mkbox(g->kts, imin(g->kte, g->kde - 1), g->jts, imin(g->jte, g->jde - 1), g->its,
                 imin(g->ite, g->ide - 1));
@@ -180,6 +183,7 @@ __global__ void k_div_by_mut(Box b, Dims D, const float* mut, float* a0, float* 
 extern "C" int pywrf_phy_prep_part2(const pywrf_grid_t* g, const float* mut, float* rthblten, float* rublten,
                                     float* rvblten, float* rqvblten, float* rqcblten, float* rqiblten,
                                     pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 0);
   Box b = mkbox(g->kts, imin(g->kte, g->kde - 1), g->jts, imin(g->jte, g->jde - 1), g->its,
                 imin(g->ite, g->ide - 1));
@@ -217,6 +221,7 @@ __global__ void k_set_w_surface(Box b, Dims D, float* w, const float* ht, const 
 extern "C" int pywrf_set_w_surface(const pywrf_grid_t* g, float* w, const float* ht, const float* u, const float* v,
                                    const float* cf, double rdx, double rdy, const float* msftx, const float* msfty,
                                    pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 1);
   if (g->nz < 3) return PYWRF_E_RANGE;
   Box b = mkbox(0, 1, g->jts, g->jte - 1, g->its, g->ite - 1);

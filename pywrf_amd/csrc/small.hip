@@ -131,6 +131,7 @@ extern "C" int pywrf_small_step_prep(const pywrf_grid_t* g, float* u_1, const fl
                                      const float* msfuy, const float* msfvx_inv, const float* msfty, int rk_step,
                                      float* u_out, float* v_out, float* w_out, float* t_out, float* ph_out,
                                      pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 1);
   if (g->kde > g->nz || g->kts != 0) return PYWRF_E_RANGE;
   int j0 = g->jts, j1 = imin(g->jte, g->jde - 1), i0 = g->its, i1 = imin(g->ite, g->ide - 1);
@@ -179,6 +180,7 @@ extern "C" int pywrf_calc_p_rho(const pywrf_grid_t* g, const float* al_in, const
                                 const float* alt, const float* t_2, const float* t_1, const float* c2a, float* pm1,
                                 const float* mu, const float* mut, double t0, const float* rdnw, double smdiv,
                                 int step, float* al_out, float* p_out, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 0);
   Box b = mkbox(g->kts, imin(g->kte, g->kde - 1), g->jts, imin(g->jte, g->jde - 1), g->its,
                 imin(g->ite, g->ide - 1));
@@ -249,6 +251,7 @@ __global__ void __launch_bounds__(CW_TB) k_calc_coef_w(Box b, Dims D, CoefW p) {
 extern "C" int pywrf_calc_coef_w(const pywrf_grid_t* g, float* a, float* alpha, float* gamma, const float* mut,
                                  const float* cqw, const float* rdn, const float* rdnw, const float* c2a,
                                  double dts, double g_, double epssm, int top_lid, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 0);
   if (g->kde > g->nz || g->kde < 3) return PYWRF_E_RANGE;
   double C = (0.5 * dts * g_ * (1. + epssm));
@@ -324,6 +327,7 @@ extern "C" int pywrf_advance_uv(const pywrf_grid_t* g, const float* u_in, const 
                                 const float* msfvx_inv, const float* msfvy, double rdx, double rdy, double dts,
                                 const float* cf, const float* fnm, const float* fnp, double emdiv,
                                 const float* rdnw, int spec_zone, float* u_out, float* v_out, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 1);
   int k1 = imin(g->kte, g->kde - 1);
   if (k1 + 1 > g->nz || g->kde > g->nz) return PYWRF_E_RANGE;
@@ -472,6 +476,7 @@ extern "C" int pywrf_advance_mu_t(const pywrf_grid_t* g, const float* ww_in, con
                                   const float* msftx, const float* msfty, float* ww_out, float* t_out,
                                   float* t_ave_out, float* muave_out, float* muts_out, float* mudf_out,
                                   float* mu_out, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 1);
   if (g->kte > g->nz || g->kte < 2) return PYWRF_E_RANGE;
   AdvMuT a = {ww_in, ww_1, u, u_1, v, v_1, mu_in, mut, muave_in, muts_in, muu, muv, mudf_in, t_in, t_1, t_ave_in,
@@ -660,6 +665,7 @@ extern "C" int pywrf_advance_w(const pywrf_grid_t* g, const float* w_in, const f
                                const float* fnp, const float* rdnw, const float* rdn, const float* cf,
                                const float* msftx, const float* msfty, float* t_2ave_out, float* w_out,
                                float* ph_out, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 1);
   if (g->kte > g->nz || g->kte < 3) return PYWRF_E_RANGE;
   const double pi_ = 3.14159265359, G = pw::g;
@@ -705,6 +711,7 @@ extern "C" int pywrf_sumflux(const pywrf_grid_t* g, const float* ru, const float
                              const float* muv, float* ru_m, float* rv_m, float* ww_m, const float* msfuy,
                              const float* msfvx_inv, int iteration, int number_of_small_timesteps,
                              pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 0);
   if (g->kte > g->nz || number_of_small_timesteps == 0) return PYWRF_E_RANGE;
   int mini = imin(g->ide - 1, g->ite), minj = imin(g->jde - 1, g->jte), mink = imin(g->kde - 1, g->kte);
@@ -736,6 +743,7 @@ __global__ void k_calc_mu_uv_1(Box b, Dims D, const float* mu, float* muu, float
 }
 
 extern "C" int pywrf_calc_mu_uv_1(const pywrf_grid_t* g, const float* mu, float* muu, float* muv, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 1);
   int jtf = imin(g->jte, g->jde - 1), itf = imin(g->ite, g->ide - 1);
   Box b = mkbox(0, 1, g->jts, g->jte, g->its, g->ite);
@@ -792,6 +800,7 @@ extern "C" int pywrf_small_step_finish(const pywrf_grid_t* g, const float* u_2, 
                                        const float* h_diabatic, int number_of_small_timesteps, double dts,
                                        int rk_step, int rk_order, float* u_out, float* v_out, float* w_out,
                                        float* ph_out, float* t_out, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 0);
   if (g->kde > g->nz || g->kds != 0) return PYWRF_E_RANGE;
   SsFin a = {u_2, v_2, w_2, t_2, ph_2, ww1, mut, muts, muu, muus, muv, muvs, u_save, v_save, w_save, t_save,

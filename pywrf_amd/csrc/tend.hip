@@ -87,6 +87,7 @@ extern "C" int pywrf_rhs_ph(const pywrf_grid_t* g, float* ph_tend, const float* 
                             const float* muuf, const float* muvf, const float* fnm, const float* fnp,
                             const float* rdnw, const float* cfn, const float* cfn1, double rdx, double rdy,
                             const float* msfux, const float* msfvy, const float* msfty, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 4);
   if (g->kte > g->nz || g->kte < 3) return PYWRF_E_RANGE;
   RhsPh a;
@@ -153,6 +154,7 @@ extern "C" int pywrf_horizontal_pressure_gradient(const pywrf_grid_t* g, float* 
                                                   const float* fnp, const float* rdnw, const float* cf, double rdx,
                                                   double rdy, const float* msfux, const float* msfuy,
                                                   const float* msfvx, const float* msfvy, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 1);
   Hpg a;
   a.ph = ph; a.alt = alt; a.p = p; a.pb = pb; a.al = al; a.php = php; a.cqu = cqu; a.cqv = cqv; a.muu = muu;
@@ -194,6 +196,7 @@ __global__ void k_pg_buoy_w(Box b, Dims D, float* rw_tend, const float* p, float
 extern "C" int pywrf_pg_buoy_w(const pywrf_grid_t* g, float* rw_tend, const float* p, float* cqw, const float* muf,
                                const float* mubf, const float* rdnw, const float* rdn, double g_,
                                const float* msfty, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 0);
   if (g->kde > g->nz || g->kde < 3) return PYWRF_E_RANGE;
   Box b = mkbox(1, g->kde - 1, g->jts, imin(g->jte, g->jde - 1), g->its, imin(g->ite, g->ide - 1));
@@ -241,6 +244,7 @@ extern "C" int pywrf_coriolis(const pywrf_grid_t* g, const float* ru, const floa
                               const float* msfux, const float* msfuy, const float* msfvx, const float* msfvy,
                               const float* f, const float* e, const float* sina, const float* cosa,
                               const float* fzm, const float* fzp, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 1);
   int ktf = imin(g->kte, g->kde - 1);
   if (ktf + 1 > g->nz) return PYWRF_E_RANGE;
@@ -308,6 +312,7 @@ extern "C" int pywrf_curvature(const pywrf_grid_t* g, const float* ru, const flo
                                const float* msfuy, const float* msfvx, const float* msfvy, const float* msftx,
                                const float* msfty, const float* fzm, const float* fzp, double rdx, double rdy,
                                pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 2);
   int ktf = imin(g->kte, g->kde - 1);
   if (ktf + 1 > g->nz) return PYWRF_E_RANGE;
@@ -351,6 +356,7 @@ extern "C" int pywrf_horizontal_diffusion(const pywrf_grid_t* g, char name, cons
                                           const float* msfvx, const float* msfvx_inv, const float* msfvy,
                                           const float* msftx, const float* msfty, const float* xkmhd, double rdx,
                                           double rdy, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   return pw_horizontal_diffusion(g, name, field, tendency, nullptr, mut, msfux, msfuy, msfvx, msfvx_inv, msfvy,
                                  msftx, msfty, xkmhd, F(rdx), F(rdy), s);
 }
@@ -360,6 +366,7 @@ extern "C" int pywrf_horizontal_diffusion_3dmp(const pywrf_grid_t* g, const floa
                                                const float* msfuy, const float* msfvx, const float* msfvx_inv,
                                                const float* msfvy, const float* msftx, const float* msfty,
                                                const float* xkmhd, double rdx, double rdy, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   // the 3dmp form always uses the scalar coefficients
   return pw_horizontal_diffusion(g, 'm', field, tendency, base_3d, mut, msfux, msfuy, msfvx, msfvx_inv, msfvy,
                                  msftx, msfty, xkmhd, F(rdx), F(rdy), s);
@@ -395,6 +402,7 @@ static int vdiff_launch(const pywrf_grid_t* g, char kind, const float* field, fl
 extern "C" int pywrf_vertical_diffusion(const pywrf_grid_t* g, char name, const float* field, float* tendency,
                                         const float* alt, const float* mut, const float* rdn, const float* rdnw,
                                         double kvdif, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   if (name != 'w' && name != 'm') return 0;  // the reference leaves other names untouched
   return vdiff_launch(g, name, field, tendency, nullptr, nullptr, alt, mut, rdn, rdnw, kvdif, s);
 }
@@ -402,24 +410,28 @@ extern "C" int pywrf_vertical_diffusion(const pywrf_grid_t* g, char name, const 
 extern "C" int pywrf_vertical_diffusion_u(const pywrf_grid_t* g, const float* field, float* tendency,
                                           const float* u_base, const float* alt, const float* muu, const float* rdn,
                                           const float* rdnw, double kvdif, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   return vdiff_launch(g, 'u', field, tendency, u_base, nullptr, alt, muu, rdn, rdnw, kvdif, s);
 }
 
 extern "C" int pywrf_vertical_diffusion_v(const pywrf_grid_t* g, const float* field, float* tendency,
                                           const float* v_base, const float* alt, const float* muv, const float* rdn,
                                           const float* rdnw, double kvdif, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   return vdiff_launch(g, 'v', field, tendency, v_base, nullptr, alt, muv, rdn, rdnw, kvdif, s);
 }
 
 extern "C" int pywrf_vertical_diffusion_mp(const pywrf_grid_t* g, const float* field, float* tendency,
                                            const float* base, const float* alt, const float* mut, const float* rdn,
                                            const float* rdnw, double kvdif, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   return vdiff_launch(g, 'p', field, tendency, base, nullptr, alt, mut, rdn, rdnw, kvdif, s);
 }
 
 extern "C" int pywrf_vertical_diffusion_3dmp(const pywrf_grid_t* g, const float* field, float* tendency,
                                              const float* base_3d, const float* alt, const float* mut,
                                              const float* rdn, const float* rdnw, double kvdif, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   return vdiff_launch(g, '3', field, tendency, nullptr, base_3d, alt, mut, rdn, rdnw, kvdif, s);
 }
 
@@ -474,6 +486,7 @@ extern "C" int pywrf_rk_addtend_dry(const pywrf_grid_t* g, float* ru_tend, float
                                     const float* h_diabatic, const float* mut, const float* msfty,
                                     const float* msfuy, const float* msfvx, const float* msfvx_inv,
                                     pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 0);
   if (g->kte > g->nz || g->kte < 2) return PYWRF_E_RANGE;
   AddT a = {ru_tend, rv_tend, rw_tend, ph_tend, t_tend, ru_tendf, rv_tendf, rw_tendf, ph_tendf, t_tendf,

@@ -635,6 +635,7 @@ extern "C" int pywrf_wsm6(const pywrf_grid_t* g, float* th, float* q, float* qc,
                           float* qg, const float* den, const float* pii, const float* p, const float* delz,
                           double delt, float* rain, float* rainncv, float* snow, float* snowncv, float* sr,
                           float* graupel, float* graupelncv, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   int e = pw_check_grid(g, 0);
   if (e) return e;
   Wsm6 a;
@@ -661,6 +662,7 @@ extern "C" int pywrf_microphysics_driver(const pywrf_grid_t* g, float* th, const
                                          float* qr, float* qi, float* qs, float* qg, float* rainnc, float* rainncv,
                                          float* snownc, float* snowncv, float* sr, float* graupelnc,
                                          float* graupelncv, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   return pywrf_wsm6(g, th, qv, qc, qr, qi, qs, qg, rho, pi_phy, p, dz8w, dt, rainnc, rainncv, snownc, snowncv, sr,
                     graupelnc, graupelncv, s);
 }
@@ -721,6 +723,7 @@ extern "C" int pywrf_moist_physics_prep_em(const pywrf_grid_t* g, const float* t
                                            float* h_diabatic, const float* qv, float* qv_diabatic, const float* qc,
                                            float* qc_diabatic, const float* fzm, const float* fzp,
                                            pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   int e = pw_check_grid(g, 0);
   if (e) return e;
   if (g->kts != 0 || g->kte > g->nz || g->kde > g->nz || g->kte < 4) return PYWRF_E_RANGE;
@@ -749,6 +752,7 @@ __global__ void k_mp_finish(Box b, Dims D, float* t_new, const float* th_phy, fl
 extern "C" int pywrf_moist_physics_finish_em(const pywrf_grid_t* g, float* t_new, const float* th_phy,
                                              float* h_diabatic, double dt, const float* qv, float* qv_diabatic,
                                              const float* qc, float* qc_diabatic, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   int e = pw_check_grid(g, 0);
   if (e) return e;
   Box b = mkbox(g->kts, imin(g->kte, g->kde - 1), g->jts, imin(g->jte, g->jde - 1), g->its,
