@@ -310,6 +310,14 @@ int pywrf_spec_bdytend(const pywrf_grid_t* g, float* field_tend, const float* bt
 /* spec_bdyupdate, wrf_boundary.py:228-284 (kds == kde == 0: 2-D field) */
 int pywrf_spec_bdyupdate(const pywrf_grid_t* g, float* field, const float* field_tend, double dt, char variable,
                          pywrf_stream_t s);
+
+/* n consecutive spec_bdyupdate calls on distinct fields (as the acoustic step
+ * issues them, solver.py:1366-1406) in one launch: fields[f], tends[f],
+ * variables[f] as in pywrf_spec_bdyupdate, is2d[f] selects the 2-D index set
+ * g2 (kds = kde = 0) over g3.  No counterpart in the reference: a batching. */
+int pywrf_spec_bdyupdate_group(const pywrf_grid_t* g3, const pywrf_grid_t* g2, int n, void* const* fields,
+                               void* const* tends, const char* variables, const int* is2d, double dt,
+                               pywrf_stream_t s);
 /* spec_bdyupdate_ph, wrf_boundary.py:287-358 */
 int pywrf_spec_bdyupdate_ph(const pywrf_grid_t* g, const float* ph_save, float* field, const float* field_tend,
                             const float* mu_tend, const float* muts, double dt, char variable_in,
@@ -329,6 +337,12 @@ int pywrf_spec_bdy_scalar(const pywrf_grid_t* g, float* scalar_tend, const float
                           const float* btys, const float* btye, pywrf_stream_t s);
 /* flow_dep_bdy, wrf_boundary.py:475-510 */
 int pywrf_flow_dep_bdy(const pywrf_grid_t* g, float* field, const float* u, const float* v, pywrf_stream_t s);
+
+/* flow_dep_bdy on n distinct fields with the same u, v (the moisture species
+ * loop, solver.py:1599-1602) in one launch.  No counterpart in the reference:
+ * a batching. */
+int pywrf_flow_dep_bdy_group(const pywrf_grid_t* g, int n, void* const* fields, const float* u, const float* v,
+                             pywrf_stream_t s);
 /* spec_bdy_final, wrf_boundary.py:513-587 (msf / mu ignored where the reference does not couple) */
 int pywrf_spec_bdy_final(const pywrf_grid_t* g, float* field, const float* mu, const float* msf, const float* bxs,
                          const float* bxe, const float* bys, const float* bye, const float* btxs,

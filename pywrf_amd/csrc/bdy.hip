@@ -158,21 +158,36 @@ extern "C" int pywrf_spec_bdy_scalar(const pywrf_grid_t* g, float* scalar_tend, 
 }
 
 // ------------------------------------------------------------------ spec_bdyupdate
-__global__ void k_spec_bdyupdate(Box all, int w, Dims D, float* f, const float* t, float dt, int jts, int jtf, int its,
-                                 int itf, int jbs, int jbe) {
-  int k, j, i;
-  if (!edge_point(all, w, k, j, i)) return;
-  bool ic = i >= its && i < itf, jc = j >= jbs + 1 && j < jbe - 1;
-  int n = (j == jts && ic) + (j == jtf - 1 && ic) + (i == its && jc) + (i == itf - 1 && jc);
+struct UpdJob {
+  Box all;
+  Dims D;
+  float* f;
+  const float* t;
+  float dt;
+  int jts, jtf, its, itf, jbs, jbe;
+};
+
+__device__ inline void spec_upd_point(const UpdJob& b, int k, int j, int i) {
+  const Dims& D = b.D;
+  bool ic = i >= b.its && i < b.itf, jc = j >= b.jbs + 1 && j < b.jbe - 1;
+  int n = (j == b.jts && ic) + (j == b.jtf - 1 && ic) + (i == b.its && jc) + (i == b.itf - 1 && jc);
   if (!n) return;
   long q = D.ix(k, j, i);
-  float v = f[q], tv = t[q];
-  for (int m = 0; m < n; ++m) v = v + dt * tv;
-  f[q] = v;
+  float v = b.f[q], tv = b.t[q];
+  for (int m = 0; m < n; ++m) v = v + b.dt * tv;
+  b.f[q] = v;
 }
 
-extern "C" int pywrf_spec_bdyupdate(const pywrf_grid_t* g, float* field, const float* field_tend, double dt,
-                                    char variable, pywrf_stream_t s) {
+__global__ void k_spec_bdyupdate(Box all, int w, UpdJob b) {
+  int k, j, i;
+  if (!edge_point(all, w, k, j, i)) return;
+  spec_upd_point(b, k, j, i);
+}
+
+// Host side of spec_bdyupdate: the job and its edge-band width (1 when the
+// updated rows jts, jtf-1 and columns its, itf-1 are the box's sides, else 0).
+static int spec_upd_job(const pywrf_grid_t* g, float* field, const float* field_tend, double dt, char variable,
+                        UpdJob& b, int& w) {
   if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   char v = variable;
   if (v == 'U') v = 'u'; else if (v == 'V') v = 'v'; else if (v == 'M') v = 'm'; else if (v == 'H') v = 'h';
@@ -185,11 +200,65 @@ extern "C" int pywrf_spec_bdyupdate(const pywrf_grid_t* g, float* field, const f
   if (itf > g->nx || jbe > g->ny || jtf > g->ny || ktf > nz || g->its < 0 || g->jts < 0) return PYWRF_E_RANGE;
   pywrf_grid_t g2 = *g;
   g2.nz = nz;
-  Box all = mkbox(g->kts, ktf, imin(g->jts, g->jds), imax(jtf, jbe), g->its, itf);
-  // the updated rows jts, jtf-1 and columns its, itf-1 are the box's sides when jts and jtf bound it
-  int w = (g->jts == all.j0 && jtf == all.j1) ? 1 : 0;
-  pw_launch_edges(k_spec_bdyupdate, all, w, s, dims_of(&g2), field, field_tend, F(dt), g->jts, jtf, g->its, itf,
-                  g->jds, jbe);
+  b.all = mkbox(g->kts, ktf, imin(g->jts, g->jds), imax(jtf, jbe), g->its, itf);
+  b.D = dims_of(&g2);
+  b.f = field; b.t = field_tend; b.dt = F(dt);
+  b.jts = g->jts; b.jtf = jtf; b.its = g->its; b.itf = itf; b.jbs = g->jds; b.jbe = jbe;
+  w = (g->jts == b.all.j0 && jtf == b.all.j1) ? 1 : 0;
+  return 0;
+}
+
+extern "C" int pywrf_spec_bdyupdate(const pywrf_grid_t* g, float* field, const float* field_tend, double dt,
+                                    char variable, pywrf_stream_t s) {
+  UpdJob b;
+  int w, e = spec_upd_job(g, field, field_tend, dt, variable, b, w);
+  if (e) return e;
+  pw_launch_edges(k_spec_bdyupdate, b.all, w, s, b);
+  PW_RET();
+}
+
+// Several spec_bdyupdate calls on distinct fields in one launch (blockIdx.y
+// selects the job); a job whose update rows are not its box's sides runs on
+// its own as in pywrf_spec_bdyupdate.
+#define PW_UPD_GROUP_MAX 8
+struct UpdJobs {
+  int n;
+  UpdJob j[PW_UPD_GROUP_MAX];
+};
+
+__global__ void k_spec_bdyupdate_group(UpdJobs J) {
+  const UpdJob& b = J.j[blockIdx.y];
+  if ((int)blockIdx.z >= b.all.nk()) return;
+  int k, j, i;
+  if (!edge_point(b.all, 1, k, j, i)) return;
+  spec_upd_point(b, k, j, i);
+}
+
+extern "C" int pywrf_spec_bdyupdate_group(const pywrf_grid_t* g3, const pywrf_grid_t* g2, int n, void* const* fields,
+                                          void* const* tends, const char* variables, const int* is2d, double dt,
+                                          pywrf_stream_t s) {
+  if (!pw_size_ok(g3) || !pw_size_ok(g2)) return PYWRF_E_RANGE;
+  if (n < 0 || n > PW_UPD_GROUP_MAX) return PYWRF_E_ARG;
+  UpdJobs J;
+  J.n = 0;
+  long most = 0;
+  int maxk = 0;
+  for (int f = 0; f < n; ++f) {
+    UpdJob b;
+    int w, e = spec_upd_job(is2d[f] ? g2 : g3, (float*)fields[f], (const float*)tends[f], dt, variables[f], b, w);
+    if (e) return e;
+    if (b.all.size() <= 0) continue;
+    if (w == 1 && 2 <= b.all.nj() && 2 <= b.all.ni()) {
+      J.j[J.n++] = b;
+      most = edge_count(b.all, 1) > most ? edge_count(b.all, 1) : most;
+      maxk = b.all.nk() > maxk ? b.all.nk() : maxk;
+    } else {
+      pw_launch_edges(k_spec_bdyupdate, b.all, w, s, b);
+    }
+  }
+  if (J.n > 0)
+    hipLaunchKernelGGL(k_spec_bdyupdate_group, dim3((unsigned)((most + PW_EDGE_BLOCK - 1) / PW_EDGE_BLOCK),
+                       (unsigned)J.n, (unsigned)maxk), dim3(PW_EDGE_BLOCK), 0, (hipStream_t)s, J);
   PW_RET();
 }
 
@@ -269,10 +338,8 @@ extern "C" int pywrf_mass_weight(const pywrf_grid_t* g, const float* field, cons
 }
 
 // ------------------------------------------------------------------ flow_dep_bdy
-__global__ void k_flow_dep(Box all, int w, Dims D, float* f, const float* u, const float* v, int jts, int jtf, int its,
-                           int itf) {
-  int k, j, i;
-  if (!edge_point(all, w, k, j, i)) return;
+__device__ inline void flow_point(const Dims& D, float* f, const float* u, const float* v, int jts, int jtf, int its,
+                                  int itf, int k, int j, int i) {
   bool top = j == jts, bot = j == jtf - 1, lft = i == its, rgt = i == itf - 1;
   if (!(top || bot || lft || rgt)) return;
   float r;
@@ -290,6 +357,13 @@ __global__ void k_flow_dep(Box all, int w, Dims D, float* f, const float* u, con
   AT(f, k, j, i) = r;
 }
 
+__global__ void k_flow_dep(Box all, int w, Dims D, float* f, const float* u, const float* v, int jts, int jtf, int its,
+                           int itf) {
+  int k, j, i;
+  if (!edge_point(all, w, k, j, i)) return;
+  flow_point(D, f, u, v, jts, jtf, its, itf, k, j, i);
+}
+
 extern "C" int pywrf_flow_dep_bdy(const pywrf_grid_t* g, float* field, const float* u, const float* v,
                                   pywrf_stream_t s) {
   if (!pw_size_ok(g)) return PYWRF_E_RANGE;
@@ -298,6 +372,47 @@ extern "C" int pywrf_flow_dep_bdy(const pywrf_grid_t* g, float* field, const flo
   if (ktf > g->nz) return PYWRF_E_RANGE;
   Box all = mkbox(g->kts, ktf, g->jts, jtf, g->its, itf);
   pw_launch_edges(k_flow_dep, all, 1, s, dims_of(g), field, u, v, g->jts, jtf, g->its, itf);
+  PW_RET();
+}
+
+// flow_dep_bdy on up to 8 distinct fields of one grid with the same u, v, in
+// one launch: blockIdx.y selects the field.  The fields are independent (each
+// point reads only its own field), so this equals the calls in sequence.
+#define PW_FLOW_GROUP_MAX 8
+struct FlowFields {
+  float* f[PW_FLOW_GROUP_MAX];
+};
+
+__global__ void k_flow_dep_group(Box all, Dims D, FlowFields ff, const float* u, const float* v, int jts, int jtf,
+                                 int its, int itf) {
+  int k, j, i;
+  if (!edge_point(all, 1, k, j, i)) return;
+  flow_point(D, ff.f[blockIdx.y], u, v, jts, jtf, its, itf, k, j, i);
+}
+
+extern "C" int pywrf_flow_dep_bdy_group(const pywrf_grid_t* g, int n, void* const* fields, const float* u,
+                                        const float* v, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
+  CHECK(g, 1);
+  if (n < 0 || n > PW_FLOW_GROUP_MAX) return PYWRF_E_ARG;
+  int itf = imin(g->ite, g->ide - 1), jtf = imin(g->jte, g->jde - 1), ktf = g->kde - 1;
+  if (ktf > g->nz) return PYWRF_E_RANGE;
+  Box all = mkbox(g->kts, ktf, g->jts, jtf, g->its, itf);
+  if (n == 0 || all.size() <= 0) return 0;
+  FlowFields ff;
+  for (int f = 0; f < n; ++f) {
+    if (!fields[f]) return PYWRF_E_ARG;
+    ff.f[f] = (float*)fields[f];
+  }
+  if (2 > all.nj() || 2 > all.ni()) {
+    for (int f = 0; f < n; ++f)
+      pw_launch_edges(k_flow_dep, all, 1, s, dims_of(g), ff.f[f], u, v, g->jts, jtf, g->its, itf);
+    PW_RET();
+  }
+  const long cnt = edge_count(all, 1);
+  hipLaunchKernelGGL(k_flow_dep_group, dim3((unsigned)((cnt + PW_EDGE_BLOCK - 1) / PW_EDGE_BLOCK), (unsigned)n,
+                     (unsigned)all.nk()), dim3(PW_EDGE_BLOCK), 0, (hipStream_t)s, all, dims_of(g), ff, u, v, g->jts, jtf,
+                     g->its, itf);
   PW_RET();
 }
 

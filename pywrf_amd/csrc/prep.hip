@@ -245,35 +245,76 @@ struct BcSpec {
   int ids, ide, jds, jde;
 };
 
-__global__ void k_open_bc(Dims D, float* dat, int k0, BcSpec c) {
-  int k = k0 + blockIdx.x;
-  float* a = dat + (long)k * D.plane();
-  int r0 = c.ustag ? c.ur0 : c.xr0, r1 = c.ustag ? c.ur1 : c.xr1;
+// One plane's open-boundary fill by one workgroup.  In each phase the reads
+// (edge cells) and writes (halo cells) are disjoint, so every thread first
+// loads all of its items and then stores them: the loads of a phase are in
+// flight together instead of one round trip per item.
+#define BC_ITEMS 8
+__device__ inline void open_bc_plane(float* a, int nx, const BcSpec& c) {
+  const int r0 = c.ustag ? c.ur0 : c.xr0, r1 = c.ustag ? c.ur1 : c.xr1;
+  const int nstart = c.xr1 - c.xr0, nend = r1 - r0;
+  const int nxi = 3 * (nstart + nend), chunk = BC_ITEMS * (int)blockDim.x;
   // x-direction: 3 start-halo cells from column ids (rows xr), 3 end cells per row
-  int nstart = c.xr1 - c.xr0, nend = r1 - r0;
-  for (int t = threadIdx.x; t < 3 * (nstart + nend); t += blockDim.x) {
-    int h = t % 3, r = t / 3;
-    if (r < nstart) {
-      int row = c.xr0 + r;
-      a[(long)row * D.nx + c.ids - 1 - h] = a[(long)row * D.nx + c.ids];
-    } else {
-      int row = r0 + (r - nstart);
-      if (c.ustag) a[(long)row * D.nx + c.ide + h] = a[(long)row * D.nx + c.ide - 1];
-      else a[(long)row * D.nx + c.ide - 1 + h] = a[(long)row * D.nx + c.ide - 2];
+  for (int base = 0; base < nxi; base += chunk) {
+    float v[BC_ITEMS];
+    int dst[BC_ITEMS];
+#pragma unroll
+    for (int u = 0; u < BC_ITEMS; ++u) {
+      const int t = base + u * (int)blockDim.x + (int)threadIdx.x;
+      dst[u] = -1;
+      if (t < nxi) {
+        const int h = t % 3, r = t / 3;
+        int src;
+        if (r < nstart) {
+          const int row = c.xr0 + r;
+          src = row * nx + c.ids;
+          dst[u] = row * nx + c.ids - 1 - h;
+        } else {
+          const int row = r0 + (r - nstart);
+          src = c.ustag ? row * nx + c.ide - 1 : row * nx + c.ide - 2;
+          dst[u] = c.ustag ? row * nx + c.ide + h : row * nx + c.ide - 1 + h;
+        }
+        v[u] = a[src];
+      }
     }
+#pragma unroll
+    for (int u = 0; u < BC_ITEMS; ++u)
+      if (dst[u] >= 0) a[dst[u]] = v[u];
   }
   __syncthreads();
-  int nc = c.yc1 - c.yc0;
-  for (int t = threadIdx.x; t < 6 * nc; t += blockDim.x) {
-    int h = t % 3, side = (t / 3) & 1, col = c.yc0 + t / 6;
-    if (side == 0) {
-      a[(long)(c.jds - 1 - h) * D.nx + col] = a[(long)c.jds * D.nx + col];
-    } else if (c.vstag) {
-      a[(long)(c.jde + h) * D.nx + col] = a[(long)(c.jde - 1) * D.nx + col];
-    } else {
-      a[(long)(c.jde - 1 + h) * D.nx + col] = a[(long)(c.jde - 2) * D.nx + col];
+  // y-direction (reads rows the x-copies may have written)
+  const int nyi = 6 * (c.yc1 - c.yc0);
+  for (int base = 0; base < nyi; base += chunk) {
+    float v[BC_ITEMS];
+    int dst[BC_ITEMS];
+#pragma unroll
+    for (int u = 0; u < BC_ITEMS; ++u) {
+      const int t = base + u * (int)blockDim.x + (int)threadIdx.x;
+      dst[u] = -1;
+      if (t < nyi) {
+        const int h = t % 3, side = (t / 3) & 1, col = c.yc0 + t / 6;
+        int src;
+        if (side == 0) {
+          src = c.jds * nx + col;
+          dst[u] = (c.jds - 1 - h) * nx + col;
+        } else if (c.vstag) {
+          src = (c.jde - 1) * nx + col;
+          dst[u] = (c.jde + h) * nx + col;
+        } else {
+          src = (c.jde - 2) * nx + col;
+          dst[u] = (c.jde - 1 + h) * nx + col;
+        }
+        v[u] = a[src];
+      }
     }
+#pragma unroll
+    for (int u = 0; u < BC_ITEMS; ++u)
+      if (dst[u] >= 0) a[dst[u]] = v[u];
   }
+}
+
+__global__ void k_open_bc(Dims D, float* dat, int k0, BcSpec c) {
+  open_bc_plane(dat + (long)(k0 + (int)blockIdx.x) * D.plane(), D.nx, c);
 }
 
 static int bc_launch(const pywrf_grid_t* g, float* dat, int k0, int k1, const BcSpec& c, pywrf_stream_t s) {
@@ -349,29 +390,7 @@ __global__ void k_open_bc_group(long plane, int nx, BcJobs J) {
   if ((int)blockIdx.y >= J.n) return;
   const BcJob& jb = J.job[blockIdx.y];
   if ((int)blockIdx.x >= jb.nk) return;
-  float* a = jb.p + (long)(jb.k0 + blockIdx.x) * plane;
-  const BcSpec& c = jb.c;
-  int r0 = c.ustag ? c.ur0 : c.xr0, r1 = c.ustag ? c.ur1 : c.xr1;
-  int nstart = c.xr1 - c.xr0, nend = r1 - r0;
-  for (int t = threadIdx.x; t < 3 * (nstart + nend); t += blockDim.x) {
-    int h = t % 3, r = t / 3;
-    if (r < nstart) {
-      int row = c.xr0 + r;
-      a[(long)row * nx + c.ids - 1 - h] = a[(long)row * nx + c.ids];
-    } else {
-      int row = r0 + (r - nstart);
-      if (c.ustag) a[(long)row * nx + c.ide + h] = a[(long)row * nx + c.ide - 1];
-      else a[(long)row * nx + c.ide - 1 + h] = a[(long)row * nx + c.ide - 2];
-    }
-  }
-  __syncthreads();
-  int nc = c.yc1 - c.yc0;
-  for (int t = threadIdx.x; t < 6 * nc; t += blockDim.x) {
-    int h = t % 3, side = (t / 3) & 1, col = c.yc0 + t / 6;
-    if (side == 0) a[(long)(c.jds - 1 - h) * nx + col] = a[(long)c.jds * nx + col];
-    else if (c.vstag) a[(long)(c.jde + h) * nx + col] = a[(long)(c.jde - 1) * nx + col];
-    else a[(long)(c.jde - 1 + h) * nx + col] = a[(long)(c.jde - 2) * nx + col];
-  }
+  open_bc_plane(jb.p + (long)(jb.k0 + (int)blockIdx.x) * plane, nx, jb.c);
 }
 
 static int bc_spec3(const pywrf_grid_t* g, char variable, BcSpec& c, int& k1) {

@@ -107,6 +107,7 @@ def rk3_step(s, ops, cfg, check_nan=False):
     bc3 = lambda a, v: o.set_physical_bc3d(a, v, *I24)
     bc2 = lambda a, v: o.set_physical_bc2d(a, v, *I2)
     grouped = hasattr(o, "set_physical_bc_group")
+    grouped_upd = hasattr(o, "spec_bdyupdate_group")
 
     def bcs(*items):
         """A run of consecutive open-boundary fills, items (tensor, flag, is_2d);
@@ -284,16 +285,23 @@ def rk3_step(s, ops, cfg, check_nan=False):
                                     s.muu, s.cqu, s.muv, s.cqv, s.mudf, s.c1h, s.c2h, s.c1f, s.c2f, s.c3h, s.c4h,
                                     s.c3f, s.c4f, s.msfux, s.msfuy, s.msfvx, s.msfvx_inv, s.msfvy, rdx, rdy, dts_rk,
                                     s.cf1, s.cf2, s.cf3, s.fnm, s.fnp, EMDIV, s.rdnw, SPEC_ZONE, True, False, *I18)
-            s.u = o.spec_bdyupdate(s.u, s.ru_tend, dts_rk, 'u', SPEC_ZONE, *I24)
-            s.v = o.spec_bdyupdate(s.v, s.rv_tend, dts_rk, 'v', SPEC_ZONE, *I24)
+            if grouped_upd:
+                o.spec_bdyupdate_group([(s.u, s.ru_tend, 'u', 0), (s.v, s.rv_tend, 'v', 0)], dts_rk, I24, ix_2d)
+            else:
+                s.u = o.spec_bdyupdate(s.u, s.ru_tend, dts_rk, 'u', SPEC_ZONE, *I24)
+                s.v = o.spec_bdyupdate(s.v, s.rv_tend, dts_rk, 'v', SPEC_ZONE, *I24)
             (s.ww, s.ww1, s.t, s.t_2save, s.ru_m, s.rv_m, s.ww_m, s.muave, s.muts, s.mudf, s.mu) = o.advance_mu_t(
                 s.ww, s.ww1, s.u, s.u_save, s.v, s.v_save, s.mu, s.mut, s.muave, s.muts, s.muu, s.muv, s.mudf,
                 s.c1h, s.c2h, s.c1f, s.c2f, s.c3h, s.c4h, s.c3f, s.c4f, s.ru_m, s.rv_m, s.ww_m, s.t, s.t_save,
                 s.t_2save, s.t_tend, s.mu_tend, rdx, rdy, dts_rk, EPSSM, s.dnw, s.fnm, s.fnp, s.rdnw, s.msfux,
                 s.msfuy, s.msfvx, s.msfvx_inv, s.msfvy, s.msftx, s.msfty, iteration, *I18)
-            s.t = o.spec_bdyupdate(s.t, s.t_tend, dts_rk, 't', SPEC_ZONE, *I24)
-            s.mu = o.spec_bdyupdate(s.mu, s.mu_tend, dts_rk, 'm', SPEC_ZONE, *ix_2d)
-            s.muts = o.spec_bdyupdate(s.muts, s.mu_tend, dts_rk, 'm', SPEC_ZONE, *ix_2d)
+            if grouped_upd:
+                o.spec_bdyupdate_group([(s.t, s.t_tend, 't', 0), (s.mu, s.mu_tend, 'm', 1),
+                                        (s.muts, s.mu_tend, 'm', 1)], dts_rk, I24, ix_2d)
+            else:
+                s.t = o.spec_bdyupdate(s.t, s.t_tend, dts_rk, 't', SPEC_ZONE, *I24)
+                s.mu = o.spec_bdyupdate(s.mu, s.mu_tend, dts_rk, 'm', SPEC_ZONE, *ix_2d)
+                s.muts = o.spec_bdyupdate(s.muts, s.mu_tend, dts_rk, 'm', SPEC_ZONE, *ix_2d)
             s.t_2save, s.w, s.ph = o.advance_w(
                 s.w, s.rw_tend, s.ww, s.w_save, s.u, s.v, s.mu, s.mut, s.muave, s.muts, s.c1h, s.c2h, s.c1f, s.c2f,
                 s.c3h, s.c4h, s.c3f, s.c4f, s.t_2save, s.t, s.t_save, s.ph, s.ph_save, s.phb, s.ph_tend, s.ht, s.c2a,
@@ -335,8 +343,11 @@ def rk3_step(s, ops, cfg, check_nan=False):
         s.moist_old, s.moist = o.rk_update_scalar(1, 6, s.moist_old, s.moist, s.moist_tend, s.advect_tend,
                                                   s.h_tendency, s.z_tendency, s.msftx, s.msfty, s.c1h, s.c2h,
                                                   s.mu_1, s.mu, s.mub, rk_step, dt_rk, SPEC_ZONE, False, *I18)
-        for im in range(2, 7):
-            s.moist[im, :, :, :] = o.flow_dep_bdy(s.moist[im, :, :, :], s.ru_m, s.rv_m, 1, *I24)
+        if hasattr(o, "flow_dep_bdy_group"):
+            o.flow_dep_bdy_group([s.moist[im] for im in range(2, 7)], s.ru_m, s.rv_m, I24)
+        else:
+            for im in range(2, 7):
+                s.moist[im, :, :, :] = o.flow_dep_bdy(s.moist[im, :, :, :], s.ru_m, s.rv_m, 1, *I24)
         s.scalar_tend, s.advect_tend_sca, s.h_tendency, s.z_tendency = o.rk_scalar_tend(
             1, 2, False, rk_step, dt_rk, s.ru_m, s.rv_m, s.ww_m, s.muts, s.mub, s.mu_1, s.c1h, s.c2h, s.alt,
             s.scalar_old, s.scalar, s.scalar_tend, s.advect_tend_sca, s.h_tendency, s.z_tendency, s.RQVFTEN,

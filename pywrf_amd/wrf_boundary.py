@@ -181,3 +181,43 @@ def spec_bdy_final(field, mu, c1, c2, msf,
          ptr(field_bdy_ys), ptr(field_bdy_ye), ptr(field_bdy_tend_xs), ptr(field_bdy_tend_xe),
          ptr(field_bdy_tend_ys), ptr(field_bdy_tend_ye), flag(variable_in), float(dtbc), stream())
     return field
+
+
+# ----------------------------------------------------------------- batched boundary updates
+def spec_bdyupdate_group(items, dt, idx24, idx2d):
+    """A run of spec_bdyupdate calls on distinct fields in one launch.  items:
+    [(field, field_tend, variable, is_2d)], all with time step dt; idx24 /
+    idx2d: the index tuples the individual calls would take (idx2d with
+    kds = kde = 0).  Same result as the calls in order.  Not part of the
+    reference API."""
+    import ctypes
+    n = len(items)
+    if n == 0:
+        return
+    if len({f.data_ptr() for f, _, _, _ in items}) != n:
+        raise ValueError("spec_bdyupdate_group: a field appears twice")
+    f3 = next((f for f, _, _, two in items if not two), None)
+    f2 = next((f for f, _, _, two in items if two), None)
+    shape3 = f3.shape if f3 is not None else (idx24[5],) + tuple(items[0][0].shape[-2:])
+    shape2 = f2.shape if f2 is not None else tuple(items[0][0].shape[-2:])
+    g3 = grid(shape3, _I24(*idx24))
+    g2 = grid(shape2, _I24(*idx2d))
+    fields = (ctypes.c_void_p * n)(*[ptr(f).value for f, _, _, _ in items])
+    tends = (ctypes.c_void_p * n)(*[ptr(t).value for _, t, _, _ in items])
+    flags = b"".join(v.encode() for _, _, v, _ in items)
+    kinds = (ctypes.c_int * n)(*[1 if two else 0 for _, _, _, two in items])
+    call("pywrf_spec_bdyupdate_group", g3, g2, n, fields, tends, flags, kinds, float(dt), stream())
+
+
+def flow_dep_bdy_group(fields, u, v, idx24):
+    """flow_dep_bdy on several distinct fields with the same u, v in one
+    launch (the moisture species loop).  Not part of the reference API."""
+    import ctypes
+    n = len(fields)
+    if n == 0:
+        return
+    if len({f.data_ptr() for f in fields}) != n:
+        raise ValueError("flow_dep_bdy_group: a field appears twice")
+    g = grid(fields[0].shape, _I24(*idx24))
+    arr = (ctypes.c_void_p * n)(*[ptr(f).value for f in fields])
+    call("pywrf_flow_dep_bdy_group", g, n, arr, ptr(u), ptr(v), stream())

@@ -39,3 +39,43 @@ def test_bc_group_equals_individual_calls(n):
     torch.cuda.synchronize()
     for r, (t, v, two) in zip(ref, items):
         assert torch.equal(r, t), (v, two)
+
+
+def _ix2d(g):
+    (ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, ips, ipe, jps, jpe, kps, kpe,
+     its, ite, jts, jte, kts, kte) = g.ix24
+    return (ids, ide, jds, jde, 0, 0, ims, ime, jms, jme, 0, 0, ips, ipe, jps, jpe, 0, 0, its, ite, jts, jte, 0, 0)
+
+
+@pytest.mark.parametrize("n", [40, 120])
+def test_spec_bdyupdate_group_equals_individual_calls(n):
+    import pywrf_amd.wrf_boundary as bdy
+    from pywrf_amd.step import Grid
+    g = Grid(41, n, n)
+    ix2 = _ix2d(g)
+    torch.manual_seed(2)
+    items = [(torch.randn(41, n, n, device="cuda"), torch.randn(41, n, n, device="cuda"), v, 0) for v in "uvth"]
+    items += [(torch.randn(n, n, device="cuda"), torch.randn(n, n, device="cuda"), "m", 1) for _ in range(2)]
+    ref = [f.clone() for f, _, _, _ in items]
+    for r, (_, t, v, two) in zip(ref, items):
+        bdy.spec_bdyupdate(r, t, 13.5, v, 1, *(ix2 if two else g.ix24))
+    bdy.spec_bdyupdate_group(items, 13.5, g.ix24, ix2)
+    torch.cuda.synchronize()
+    for r, (f, _, v, two) in zip(ref, items):
+        assert torch.equal(r, f), (v, two)
+
+
+@pytest.mark.parametrize("n", [40, 120])
+def test_flow_dep_group_equals_individual_calls(n):
+    import pywrf_amd.wrf_boundary as bdy
+    from pywrf_amd.step import Grid
+    g = Grid(41, n, n)
+    torch.manual_seed(3)
+    q = torch.rand(7, 41, n, n, device="cuda")
+    u, v = torch.randn(41, n, n, device="cuda"), torch.randn(41, n, n, device="cuda")
+    ref = q.clone()
+    for im in range(2, 7):
+        bdy.flow_dep_bdy(ref[im], u, v, 1, *g.ix24)
+    bdy.flow_dep_bdy_group([q[im] for im in range(2, 7)], u, v, g.ix24)
+    torch.cuda.synchronize()
+    assert torch.equal(ref, q)
