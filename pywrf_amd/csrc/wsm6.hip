@@ -31,6 +31,14 @@ __device__ inline float tmin(float a, float b) { return (a != a) ? a : ((b != b)
 
 #define WAVE 64
 
+// Fractional powers of the scheme.  x^(1/4) is two square roots and x^y is
+// exp2(y log2 x): a few instructions instead of the general powf, with a
+// relative error of a few 1e-7 for the magnitudes here (x^y and the
+// reference's pow agree to far below the parity tolerance).  Zero, infinity,
+// NaN and negative x behave as pow does for these positive, non-integer y.
+__device__ inline float qroot(float x) { return sqrtf(sqrtf(x)); }
+__device__ inline float fpow(float x, float y) { return exp2f(y * log2f(x)); }
+
 struct Slope3 {
   float rs[3], rb[3], r2[3], r3[3], vt[3];
 };
@@ -44,12 +52,12 @@ __device__ inline void slope3(float q0, float q1, float q2, float den, float den
   for (int n = 0; n < 3; ++n) {
     bool small = q[n] <= F(qcrmin);
     float lam;
-    if (n == 0) lam = powf(F(pidn0r) / (q[n] * den), 0.25f);
-    else if (n == 1) lam = powf(F(pidn0s) * n0sfac / (q[n] * den), 0.25f);
-    else lam = powf(F(pidn0g) / (q[n] * den), 0.25f);
+    if (n == 0) lam = qroot(F(pidn0r) / (q[n] * den));
+    else if (n == 1) lam = qroot(F(pidn0s) * n0sfac / (q[n] * den));
+    else lam = qroot(F(pidn0g) / (q[n] * den));
     float r = small ? F(rslopermax) : 1.0f / lam;
     s.rs[n] = r;
-    s.rb[n] = small ? F(rsloperbmax) : powf(r, bexp[n]);
+    s.rb[n] = small ? F(rsloperbmax) : fpow(r, bexp[n]);
     s.r2[n] = small ? F(rsloper2max) : r * r;
     s.r3[n] = small ? F(rsloper3max) : r * r * r;
     float v = pv[n] * s.rb[n] * denfac;
@@ -62,18 +70,18 @@ __device__ inline float slope1(int kind, float qr, float den, float denfac, floa
   bool small = qr <= F(qcrmin);
   float r, rb, v;
   if (kind == 0) {
-    r = small ? F(rslopermax) : 1.0f / powf(F(pidn0r) / (qr * den), 0.25f);
-    rb = small ? F(rsloperbmax) : powf(r, F(bvtr));
+    r = small ? F(rslopermax) : 1.0f / qroot(F(pidn0r) / (qr * den));
+    rb = small ? F(rsloperbmax) : fpow(r, F(bvtr));
     v = F(pvtr) * rb * denfac;
   } else if (kind == 1) {
     float supcol = F(t0c) - t;
     float n0sfac = tmax(tmin(expf(F(alpha) * supcol), F(n0smax / n0s)), 1.0f);
-    r = small ? F(rslopermax) : 1.0f / powf(F(pidn0s) * n0sfac / (qr * den), 0.25f);
-    rb = small ? F(rsloperbmax) : powf(r, F(bvts));
+    r = small ? F(rslopermax) : 1.0f / qroot(F(pidn0s) * n0sfac / (qr * den));
+    rb = small ? F(rsloperbmax) : fpow(r, F(bvts));
     v = F(pvts) * rb * denfac;
   } else {
-    r = small ? F(rslopermax) : 1.0f / powf(F(pidn0g) / (qr * den), 0.25f);
-    rb = small ? F(rsloperbmax) : powf(r, F(bvtg));
+    r = small ? F(rslopermax) : 1.0f / qroot(F(pidn0g) / (qr * den));
+    rb = small ? F(rsloperbmax) : fpow(r, F(bvtg));
     v = F(pvtg) * rb * denfac;
   }
   return qr <= 0.0f ? 0.0f : v;
@@ -311,7 +319,7 @@ __global__ void __launch_bounds__(WAVE) k_wsm6(Dims D, Wsm6 a) {
   float qs0_, qs1_;
   fpvs(t, pp, qs0_, qs1_);
   float rh0 = tmax(qv / qs0_, qmin), rh1 = tmax(qv / qs1_, qmin);
-  float xni = tmin(tmax(F(5.387e7) * powf(d * tmax(qci1, qmin), 0.75f), 1.e3f), 1.e6f);
+  float xni = tmin(tmax(F(5.387e7) * fpow(d * tmax(qci1, qmin), 0.75f), 1.e3f), 1.e6f);
   (void)xni;
   Slope3 s;
   slope3(qrs0, qrs1, qrs2, d, denfac, t, s);
@@ -357,7 +365,7 @@ __global__ void __launch_bounds__(WAVE) k_wsm6(Dims D, Wsm6 a) {
   }
   // cloud-ice fall (one pass with the ice fall speed)
   {
-    float xni0 = tmin(tmax(F(5.387e7) * powf(d * tmax(qci1, qmin), 0.75f), 1.e3f), 1.e6f);
+    float xni0 = tmin(tmax(F(5.387e7) * fpow(d * tmax(qci1, qmin), 0.75f), 1.e3f), 1.e6f);
     float wt = tmax(tmin(F(dicon) * sqrtf(d * qci1 / xni0), F(dimax)), 1.e-25f);
     wt = 1.49e4f * expf(logf(wt) * 1.31f);
     float work1c = qci1 <= 0.0f ? 0.0f : wt;
@@ -432,7 +440,7 @@ __global__ void __launch_bounds__(WAVE) k_wsm6(Dims D, Wsm6 a) {
   work2 = venfac(pp, t, d);
   // warm rain
   float supsat = tmax(qv, qmin) - qs0_;
-  float praut = F(qck1) * powf(qci0, F(7. / 3.));
+  float praut = F(qck1) * fpow(qci0, F(7. / 3.));
   praut = qci0 > F(qc0) ? tmin(praut, qci0 / dtcld) : 0.0f;
   float pracw = 0.0f;
   if (qrs0 > F(qcrmin) && qci0 > qmin) pracw = tmin(F(pacrr) * s.r3[0] * s.rb[0] * qci0 * denfac, qci0 / dtcld);
@@ -452,11 +460,11 @@ __global__ void __launch_bounds__(WAVE) k_wsm6(Dims D, Wsm6 a) {
   supsat = tmax(qv, qmin) - qs1_;
   float satdt = supsat / dtcld;
   int ifsat = 0;
-  xni = tmin(tmax(F(5.38e7) * powf(d * tmax(qci1, qmin), 0.75f), 1.e3f), 1.e6f);
+  xni = tmin(tmax(F(5.38e7) * fpow(d * tmax(qci1, qmin), 0.75f), 1.e3f), 1.e6f);
   float eacrs = expf(0.07f * (-supcol));
   float xmi = d * qci1 / xni;
   float diameter = tmin(F(dicon) * sqrtf(xmi), F(dimax));
-  float vt2i = 1.49e4f * powf(diameter, 1.31f);
+  float vt2i = 1.49e4f * fpow(diameter, 1.31f);
   float vt2r = F(pvtr) * s.rb[0] * denfac, vt2s = F(pvts) * s.rb[1] * denfac, vt2g = F(pvtg) * s.rb[2] * denfac;
   qsum = tmax(qrs1 + qrs2, 1.e-15f);
   float vt2ave = qsum > 1.e-15f ? (vt2s * qrs1 + vt2g * qrs2) / qsum : 0.0f;
@@ -539,7 +547,7 @@ __global__ void __launch_bounds__(WAVE) k_wsm6(Dims D, Wsm6 a) {
     }
     c1 = supsat > 0.0f && ifsat != 1 && cc;
     float supice = satdt - prevp - pidep - psdep - pgdep;
-    float roqi0 = F(4.92e-11) * powf(1.e3f * expf(0.1f * supcol), 1.33f);
+    float roqi0 = F(4.92e-11) * fpow(1.e3f * expf(0.1f * supcol), 1.33f);
     float pg = tmax(0.0f, (roqi0 / d - tmax(qci1, 0.0f)) / dtcld);
     pigen = c1 ? tmin(tmin(pg, satdt), supice) : 0.0f;
     if (qci1 > 0.0f && cc) psaut = tmax(0.0f, (qci1 - F(roqimax) / d) / dtcld);
@@ -701,7 +709,7 @@ __global__ void k_mp_prep(Box b, Dims D, MpPrep a) {
     a.qc_diabatic[q] = a.qc[q] + 0.0f;
     a.rho[q] = 1.0f / (a.al[q] + a.alb[q]);
     float pf = PF(k);
-    a.pii[q] = powf(pf / a.p0, F(pw::rcp));
+    a.pii[q] = fpow(pf / a.p0, F(pw::rcp));
     a.z[q] = 0.5f * (zw + ZW(k + 1));
     a.pf[q] = pf;
     if (k >= 1) a.p8w[q] = a.fzm[k] * pf + a.fzp[k] * PF(k - 1);
