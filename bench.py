@@ -27,6 +27,10 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
 COPY_REF_FLOATS = 64 * 1024 * 1024   # 256 MiB copy used to scale the HBM byte counters
+# Measured bytes per launch at 240x240x41 (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, the scale of a
+# 256 MiB copy), from profiles/measured_kernels_240x240x41_hbm_bytes.md.  Only valid at that size.
+TRAFFIC_240 = {"advance_w": 267.8e6, "advance_uv": 191.9e6, "advance_mu_t": 203.8e6, "wsm6": 159.0e6,
+               "rk_scalar_tend_moist": 290.0e6}
 FIELD_PASSES_PER_STEP = 700          # algorithmic 3-D field passes per RK3 step
 
 
@@ -194,8 +198,10 @@ def main():
     kern.sort(reverse=True)
     _, kname, ksec, kbytes = kern[0]
     achieved = kbytes / ksec / 1e9
+    traffic = TRAFFIC_240.get(kname) if (g.nz, g.ny, g.nx) == (41, 240, 240) else None
     roofline = {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
-                "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+                "traffic": None if traffic is None else round(traffic),
                 "bytes_per_launch": kbytes, "us_per_launch": round(ksec * 1e6, 2)}
     step_bytes = FIELD_PASSES_PER_STEP * n3 * 4
     out = {
