@@ -1,12 +1,14 @@
 // Flux-form 5th/3rd-order advection of u, v, w and mass-point scalars
 // (pywrf/wrf_dynamics.py:1112-2028).
 //
-// One launch per call.  A thread owns one tendency point and evaluates the
-// face fluxes on both sides of it in y, x and z (the reference stores each
-// face-flux plane in a full-grid scratch array and differences it; here the
-// face flux is a pure function of the inputs, so neighbouring threads simply
-// recompute shared faces from cached loads).  The three directional updates
-// are applied to the tendency in the reference's order, y then x then z.
+// One launch per call.  For u, v and w a thread owns one tendency point and
+// evaluates the face fluxes on both sides of it in y, x and z (the reference
+// stores each face-flux plane in a full-grid scratch array and differences
+// it; here the face flux is a pure function of the inputs, so neighbouring
+// threads recompute shared faces from cached loads).  Scalars share x-faces
+// between neighbouring lanes and carry z-faces up the column.  The three
+// directional updates are applied to the tendency in the reference's order,
+// y then x then z.
 //
 // Face rules, including the order reductions on the rows and columns next
 // to the domain edge and the sign-source quirks of some 3rd-order faces, are
@@ -315,18 +317,67 @@ __device__ inline float s_zface(const Dims& D, const AdvS& a, const AdvIdx& x, i
   return vel * flux3(Q(-2), Q(-1), Q(0), Q(1), -vel);  // kf == 2 or ktf-2
 }
 
-__global__ void k_advect_scalar(Box b, Dims D, AdvS a, AdvIdx x, Box badv, Box by, Box bx, int zero) {
-  int k, j, i;
-  if (!box_point(b, k, j, i)) return;
-  long q = D.ix(k, j, i);
-  float t = zero ? 0.0f : a.tend[q];
-  if (badv.has(k, j, i)) {
-    float mx = AT2(a.msftx, j, i);
-    if (by.has(k, j, i)) t = t - (mx * a.rdy) * (s_yface(D, a, x, k, j + 1, i) - s_yface(D, a, x, k, j, i));
-    if (bx.has(k, j, i)) t = t - (mx * a.rdx) * (s_xface(D, a, x, k, j, i + 1) - s_xface(D, a, x, k, j, i));
-    t = t - a.rdzw[k] * (s_zface(D, a, x, k + 1, j, i) - s_zface(D, a, x, k, j, i));
+
+// ------------------------------------------------------------------ rk_scalar_tend, all species in one pass
+// Zeroes h_tendency / z_tendency (one grouped fill), then for every species of
+// [scs, sce] writes its advection tendency on the tile (zero outside the
+// advection box) and, on the first RK stage, adds the horizontal and vertical
+// diffusion into scalar_tends -- the per-species sequence of rk_scalar_tend
+// (wrf_dynamics.py:3982-4051) -- with the mass fluxes read once for all species.
+struct ScalarTend {
+  AdvS adv;
+  AdvIdx x;
+  HDiff hd;
+  VDiff vd;
+  const float* scalar;
+  float *adv_tend, *sc_tend, *h_t, *z_t;
+  long n3;
+  int scs, sce, rk1, moist_step, accumulate;
+  Box bzero, badv, by, bx, bh, bv;
+};
+
+// Tile of 63 i x 4 j points per workgroup (lane l of a row evaluates the x-face
+// at i0 + l; the point takes its right-hand face from lane l + 1 by a lane
+// shuffle), marching SA_KC levels up the column and carrying each z-face to
+// the level above.  Face values and the tendency arithmetic are those of the
+// one-thread-per-point form (s_yface, s_xface, s_zface in the reference's
+// y, x, z order), so the results are the same floats; each x- and z-face is
+// evaluated about once instead of by both neighbours.
+#define SA_OX 63
+#define SA_KC 4
+__global__ void __launch_bounds__(PW_TX * PW_TY) k_scalar_adv(Dims D, ScalarTend a) {
+  const Box& b = a.bzero;
+  const int i = b.i0 + (int)(blockIdx.x * SA_OX + threadIdx.x);
+  const int j = b.j0 + (int)(blockIdx.y * PW_TY + threadIdx.y);
+  if (j >= D.ny) return;                      // a whole wave (one row) leaves together
+  const int k0 = b.k0 + (int)blockIdx.z * SA_KC, k1 = imin(k0 + SA_KC, b.k1);
+  const bool live = i < D.nx;                 // may evaluate faces
+  const bool own = threadIdx.x < SA_OX && i < b.i1 && j < b.j1;
+  const float mx = (own && a.badv.has2(j, i)) ? AT2(a.adv.msftx, j, i) : 0.0f;
+#pragma unroll 1
+  for (int im = a.scs; im <= a.sce; ++im) {
+    AdvS ad = a.adv;
+    ad.f = a.scalar + im * a.n3;
+    float zlo = live ? s_zface(D, ad, a.x, k0, j, i) : 0.0f;
+#pragma unroll 1
+    for (int k = k0; k < k1; ++k) {
+      const float xf = (live && j < b.j1) ? s_xface(D, ad, a.x, k, j, i) : 0.0f;
+      const float xr = __shfl_down(xf, 1, PW_TX);
+      const float zhi = live ? s_zface(D, ad, a.x, k + 1, j, i) : 0.0f;
+      if (own) {
+        const int q = im * a.n3 + D.ix(k, j, i);
+        float t = a.accumulate ? a.adv_tend[q] : 0.0f;
+        if (a.badv.has(k, j, i)) {
+          if (a.by.has(k, j, i))
+            t = t - (mx * ad.rdy) * (s_yface(D, ad, a.x, k, j + 1, i) - s_yface(D, ad, a.x, k, j, i));
+          if (a.bx.has(k, j, i)) t = t - (mx * ad.rdx) * (xr - xf);
+          t = t - ad.rdzw[k] * (zhi - zlo);
+        }
+        a.adv_tend[q] = t;
+      }
+      zlo = zhi;
+    }
   }
-  a.tend[q] = t;
 }
 
 // zero_box: first zero the tendency on [kts, kte) x [jts, jte) x [its, ite)
@@ -343,8 +394,16 @@ int pw_advect_scalar_box(const pywrf_grid_t* g, const float* field, float* tende
   Box badv = mkbox(g->kts, x.ktf, j0, j1, i0, i1);
   Box by = mkbox(g->kts, x.ktf, imax(g->jts, g->jds + 1), imin(g->jte, g->jde - 2), i0, i1);
   Box bx = mkbox(g->kts, x.ktf, j0, j1, imax(g->ids + 1, g->its), imin(g->ide - 2, g->ite));
-  Box b = zero_box ? mkbox(g->kts, g->kte, g->jts, g->jte, g->its, g->ite) : badv;
-  PW_LAUNCH_BOX(k_advect_scalar, b, s, dims_of(g), a, x, badv, by, bx, zero_box);
+  ScalarTend t;
+  t.adv = a; t.x = x;
+  t.scalar = field; t.adv_tend = tendency; t.n3 = 0; t.scs = 0; t.sce = 0; t.accumulate = zero_box ? 0 : 1;
+  t.bzero = zero_box ? mkbox(g->kts, g->kte, g->jts, g->jte, g->its, g->ite) : badv;
+  t.badv = badv; t.by = by; t.bx = bx;
+  if (t.bzero.size() > 0) {
+    dim3 grid((unsigned)((t.bzero.ni() + SA_OX - 1) / SA_OX), (unsigned)((t.bzero.nj() + PW_TY - 1) / PW_TY),
+              (unsigned)((t.bzero.nk() + SA_KC - 1) / SA_KC));
+    hipLaunchKernelGGL(k_scalar_adv, grid, dim3(PW_TX, PW_TY), 0, (hipStream_t)s, dims_of(g), t);
+  }
   PW_RET();
 }
 
@@ -353,48 +412,6 @@ extern "C" int pywrf_advect_scalar(const pywrf_grid_t* g, const float* field, fl
                                    const float* fzp, double rdx, double rdy, const float* rdzw, pywrf_stream_t s) {
   if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   return pw_advect_scalar_box(g, field, tendency, ru, rv, rom, msftx, fzm, fzp, F(rdx), F(rdy), rdzw, 0, s);
-}
-
-// ------------------------------------------------------------------ rk_scalar_tend, all species in one pass
-// One thread per point of the whole array: zeroes h_tendency / z_tendency,
-// and for every species of [scs, sce] zeroes its advection tendency on the
-// tile, adds the flux-form advection and (first RK stage) the horizontal and
-// vertical diffusion into scalar_tends -- the per-species sequence of
-// rk_scalar_tend (wrf_dynamics.py:3982-4051), with the mass fluxes ru, rv, rom
-// read once for all species.
-struct ScalarTend {
-  AdvS adv;
-  AdvIdx x;
-  HDiff hd;
-  VDiff vd;
-  const float* scalar;
-  float *adv_tend, *sc_tend, *h_t, *z_t;
-  long n3;
-  int scs, sce, rk1, moist_step;
-  Box bzero, badv, by, bx, bh, bv;
-};
-
-__global__ void k_scalar_adv(Box all, Dims D, ScalarTend a) {
-  int k, j, i;
-  if (!box_point(all, k, j, i)) return;
-  long q = D.ix(k, j, i);
-  a.h_t[q] = 0.0f;
-  a.z_t[q] = 0.0f;
-  if (!a.bzero.has(k, j, i)) return;
-  const bool adv = a.badv.has(k, j, i), ay = a.by.has(k, j, i), ax = a.bx.has(k, j, i);
-#pragma unroll 1
-  for (int im = a.scs; im <= a.sce; ++im) {
-    AdvS ad = a.adv;
-    ad.f = a.scalar + im * a.n3;
-    float t = 0.0f;
-    if (adv) {
-      float mx = AT2(ad.msftx, j, i);
-      if (ay) t = t - (mx * ad.rdy) * (s_yface(D, ad, a.x, k, j + 1, i) - s_yface(D, ad, a.x, k, j, i));
-      if (ax) t = t - (mx * ad.rdx) * (s_xface(D, ad, a.x, k, j, i + 1) - s_xface(D, ad, a.x, k, j, i));
-      t = t - ad.rdzw[k] * (s_zface(D, ad, a.x, k + 1, j, i) - s_zface(D, ad, a.x, k, j, i));
-    }
-    a.adv_tend[im * a.n3 + q] = t;
-  }
 }
 
 // First RK stage only: horizontal ('m') and vertical diffusion of every
@@ -441,7 +458,7 @@ int pw_rk_scalar_tend_fused(const pywrf_grid_t* g, int scs, int sce, int rk_step
   a.vd = {nullptr, alt, mut, rdn, rdnw, base, nullptr, nullptr, kvdq, a.x.ktf, g->kts, 'm'};
   a.scalar = scalar; a.adv_tend = advect_tend; a.sc_tend = scalar_tends; a.h_t = h_tendency; a.z_t = z_tendency;
   a.n3 = (long)g->nz * g->ny * g->nx;
-  a.scs = scs; a.sce = sce; a.rk1 = rk_step == 1; a.moist_step = moist_step;
+  a.scs = scs; a.sce = sce; a.rk1 = rk_step == 1; a.moist_step = moist_step; a.accumulate = 0;
   int ktf = a.x.ktf, jtf = imin(g->jte, g->jde - 1), itf = imin(g->ite, g->ide - 1);
   a.bzero = mkbox(g->kts, g->kte, g->jts, g->jte, g->its, g->ite);
   a.badv = mkbox(g->kts, ktf, g->jts, jtf, g->its, itf);
@@ -450,8 +467,18 @@ int pw_rk_scalar_tend_fused(const pywrf_grid_t* g, int scs, int sce, int rk_step
   a.bh = mkbox(g->kts, ktf, imax(g->jds + 1, g->jts), imin(g->jde - 2, g->jte), imax(g->ids + 1, g->its),
                imin(g->ide - 2, g->ite));
   a.bv = mkbox(g->kts + 1, ktf, g->jts, jtf, g->its, itf);
-  Box all = mkbox(0, g->nz, 0, g->ny, 0, g->nx);
-  PW_LAUNCH_BOX(k_scalar_adv, all, s, dims_of(g), a);
+  {
+    void* hz[2] = {h_tendency, z_tendency};
+    const long long cnt[2] = {(long long)a.n3, (long long)a.n3};
+    int e = pywrf_zero_group(2, hz, cnt, s);
+    if (e) return e;
+  }
+  if (a.bzero.size() > 0) {
+    const Box& bz = a.bzero;
+    dim3 grid((unsigned)((bz.ni() + SA_OX - 1) / SA_OX), (unsigned)((bz.nj() + PW_TY - 1) / PW_TY),
+              (unsigned)((bz.nk() + SA_KC - 1) / SA_KC));
+    hipLaunchKernelGGL(k_scalar_adv, grid, dim3(PW_TX, PW_TY), 0, (hipStream_t)s, dims_of(g), a);
+  }
   if (a.rk1 && a.bzero.size() > 0) {
     const Box& bz = a.bzero;
     dim3 grid((unsigned)((bz.ni() + PW_TX - 1) / PW_TX), (unsigned)((bz.nj() + PW_TY - 1) / PW_TY),
