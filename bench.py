@@ -2,6 +2,7 @@
 """RK3 steps/s of the HIP ARW step on one MI355X at 240 x 240 x 41 FP32.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--eager] [--no-cpu-baseline]
+    python bench.py --drift 60        # RMS drift of 60 steps against the oracle in FP64
 
 Workload: the seeded 240 x 240 x 41 state (3 Gaussian hills, 20 m/s jet at
 10 km with N(0, 2 m/s) noise, 6 warm bubbles, 6 moisture species; dt = 54 s,
@@ -121,6 +122,58 @@ def cpu_baseline(s_cpu, cfg):
                       "(torch CPU, %d threads), %.1f s" % (threads, dt)}
 
 
+def drift(config, nsteps, cfg):
+    """RMS drift after nsteps: rms(x - x64) / rms(x64) per prognostic field, for
+    the HIP run (FP32, device) and for the oracle's FP32 run (the reference's
+    arithmetic on the host), both against the oracle's FP64 run of the same
+    seeded state."""
+    import torch
+    import oracle
+    import pywrf_amd
+    from pywrf_amd.gen import synthetic_state
+    from pywrf_amd.state import clone_state
+    from pywrf_amd.step import rk3_step
+    torch.set_num_threads(min(16, os.cpu_count() or 1))
+    names = ("u", "v", "w", "t", "ph", "p", "al", "mu")
+
+    def fields(s):
+        out = {n: getattr(s, n).detach().double().cpu() for n in names}
+        out["qv"] = s.moist[1].detach().double().cpu()
+        out["qc+qr+qi+qs+qg"] = s.moist[2:7].sum(0).detach().double().cpu()
+        return out
+
+    s32 = synthetic_state(config)
+    gpu = clone_state(s32, device="cuda")
+    o = pywrf_amd.ops()
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        rk3_step(gpu, o, cfg)
+    torch.cuda.synchronize()
+    _log("drift: %d GPU steps in %.1f s" % (nsteps, time.perf_counter() - t0))
+    hip = fields(gpu)
+    ops = oracle.ops()
+    ref = {}
+    for dt in (torch.float64, torch.float32):
+        s = synthetic_state(config, dtype=dt)
+        t0 = time.perf_counter()
+        for n in range(nsteps):
+            rk3_step(s, ops, cfg)
+            if (n + 1) % 10 == 0:
+                _log("drift: oracle %s step %d (%.1f s)" % (dt, n + 1, time.perf_counter() - t0))
+        ref[dt] = fields(s)
+    rms = lambda x: float(torch.sqrt(torch.mean(x * x)))
+    out = {"drift_steps": nsteps, "config": config, "grid": [s32.grid.nz, s32.grid.ny, s32.grid.nx],
+           "metric": "rms(x - x_fp64) / rms(x_fp64) after %d RK3 steps" % nsteps,
+           "hip_fp32": {}, "oracle_fp32": {}}
+    r64 = ref[torch.float64]
+    for n in r64:
+        den = rms(r64[n]) or 1.0
+        out["hip_fp32"][n] = float("%.3e" % (rms(hip[n] - r64[n]) / den))
+        out["oracle_fp32"][n] = float("%.3e" % (rms(ref[torch.float32][n] - r64[n]) / den))
+    out["finite"] = all(bool(torch.isfinite(v).all()) for v in hip.values())
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,6 +183,9 @@ def main():
     ap.add_argument("--eager", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--drift", type=int, default=0, metavar="N",
+                    help="instead of timing: run N steps on the GPU and the oracle in FP64 and FP32 on the host, "
+                         "print the RMS drift of each field from the FP64 run")
     ap.add_argument("--kernels-only", action="store_true",
                     help="run each measured kernel --kernel-reps times and a plain copy of one 3-D field "
                          "(the reference for counter scaling), print nothing else; for rocprofv3 --pmc runs")
@@ -146,6 +202,9 @@ def main():
     from pywrf_amd.state import clone_state
 
     cfg = RunConfig()
+    if args.drift > 0:
+        drift(args.config, args.drift, cfg)
+        return
     _log("bench: generating %s state" % args.config)
     s_cpu = synthetic_state(args.config)
     s = clone_state(s_cpu, device="cuda")
