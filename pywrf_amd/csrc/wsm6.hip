@@ -145,10 +145,17 @@ __device__ void sediment(SedLds& L, int k, int km, float dz, float den, float de
       if (k >= 1 && k < km && L.ww[k] == 0.0f) wi = L.ww[k - 1];
     }
     // backward CFL limiter from the top: level m takes the already limited
-    // speed of level m + 1 (every lane runs the same uniform chain)
-    {
-      float wn = lane_value(wi, km), wnew = wi;
-      for (int m = km - 1; m >= 0; --m) {
+    // speed of level m + 1 (every lane runs the same uniform chain).  Levels
+    // above the highest level that the limiter changes keep their speeds, and
+    // that level is the highest one whose test already fires with the
+    // unchanged speeds; so the test is first made on all levels at once and
+    // the chain runs from there down (not at all in columns it never touches).
+    const float wup = __shfl_down(wi, 1, WAVE);
+    const unsigned long long fire = __ballot(k < km && (wup - wi) * dt / dz > 0.05f);
+    if (fire) {
+      const int mtop = 63 - __clzll(fire);
+      float wn = lane_value(wi, mtop + 1), wnew = wi;
+      for (int m = mtop; m >= 0; --m) {
         float wm = lane_value(wi, m);
         const float dzm = lane_value(dz, m);
         float decfl = (wn - wm) * dt / dzm;
