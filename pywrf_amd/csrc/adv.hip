@@ -319,7 +319,7 @@ __device__ inline float s_zface(const Dims& D, const AdvS& a, const AdvIdx& x, i
 
 
 // ------------------------------------------------------------------ rk_scalar_tend, all species in one pass
-// Zeroes h_tendency / z_tendency (one grouped fill), then for every species of
+// Zeroes h_tendency / z_tendency (whole array, in the same launch); for every species of
 // [scs, sce] writes its advection tendency on the tile (zero outside the
 // advection box) and, on the first RK stage, adds the horizontal and vertical
 // diffusion into scalar_tends -- the per-species sequence of rk_scalar_tend
@@ -333,7 +333,7 @@ struct ScalarTend {
   float *adv_tend, *sc_tend, *h_t, *z_t;
   long n3;
   int scs, sce, rk1, moist_step, accumulate;
-  Box bzero, badv, by, bx, bh, bv;
+  Box bzero, badv, by, bx, bh, bv, span;
 };
 
 // Tile of 63 i x 4 j points per workgroup (lane l of a row evaluates the x-face
@@ -346,13 +346,22 @@ struct ScalarTend {
 #define SA_OX 63
 #define SA_KC 4
 __global__ void __launch_bounds__(PW_TX * PW_TY) k_scalar_adv(Dims D, ScalarTend a) {
-  const Box& b = a.bzero;
-  const int i = b.i0 + (int)(blockIdx.x * SA_OX + threadIdx.x);
-  const int j = b.j0 + (int)(blockIdx.y * PW_TY + threadIdx.y);
-  if (j >= D.ny) return;                      // a whole wave (one row) leaves together
-  const int k0 = b.k0 + (int)blockIdx.z * SA_KC, k1 = imin(k0 + SA_KC, b.k1);
+  const Box& b = a.bzero;   // points that receive a tendency
+  const Box& sp = a.span;   // launch region: b, or the whole array when h_t / z_t are zeroed as well
+  const int i = sp.i0 + (int)(blockIdx.x * SA_OX + threadIdx.x);
+  const int j = sp.j0 + (int)(blockIdx.y * PW_TY + threadIdx.y);
+  if (j >= sp.j1) return;                     // a whole wave (one row) leaves together
+  const int k0 = sp.k0 + (int)blockIdx.z * SA_KC, k1 = imin(k0 + SA_KC, sp.k1);
+  const bool mine = threadIdx.x < SA_OX && i < sp.i1;
+  if (a.h_t && mine)
+    for (int k = k0; k < k1; ++k) {
+      const int q = D.ix(k, j, i);
+      a.h_t[q] = 0.0f;
+      a.z_t[q] = 0.0f;
+    }
+  if (j < b.j0 || j >= b.j1) return;          // whole wave: no tendency points in this row
   const bool live = i < D.nx;                 // may evaluate faces
-  const bool own = threadIdx.x < SA_OX && i < b.i1 && j < b.j1;
+  const bool own = mine && i >= b.i0 && i < b.i1;
   const float mx = (own && a.badv.has2(j, i)) ? AT2(a.adv.msftx, j, i) : 0.0f;
 #pragma unroll 1
   for (int im = a.scs; im <= a.sce; ++im) {
@@ -361,10 +370,10 @@ __global__ void __launch_bounds__(PW_TX * PW_TY) k_scalar_adv(Dims D, ScalarTend
     float zlo = live ? s_zface(D, ad, a.x, k0, j, i) : 0.0f;
 #pragma unroll 1
     for (int k = k0; k < k1; ++k) {
-      const float xf = (live && j < b.j1) ? s_xface(D, ad, a.x, k, j, i) : 0.0f;
+      const float xf = live ? s_xface(D, ad, a.x, k, j, i) : 0.0f;
       const float xr = __shfl_down(xf, 1, PW_TX);
       const float zhi = live ? s_zface(D, ad, a.x, k + 1, j, i) : 0.0f;
-      if (own) {
+      if (own && k >= b.k0 && k < b.k1) {
         const int q = im * a.n3 + D.ix(k, j, i);
         float t = a.accumulate ? a.adv_tend[q] : 0.0f;
         if (a.badv.has(k, j, i)) {
@@ -378,6 +387,14 @@ __global__ void __launch_bounds__(PW_TX * PW_TY) k_scalar_adv(Dims D, ScalarTend
       zlo = zhi;
     }
   }
+}
+
+static void pw_launch_scalar_adv(const pywrf_grid_t* g, const ScalarTend& a, pywrf_stream_t s) {
+  const Box& sp = a.span;
+  if (sp.size() <= 0) return;
+  dim3 grid((unsigned)((sp.ni() + SA_OX - 1) / SA_OX), (unsigned)((sp.nj() + PW_TY - 1) / PW_TY),
+            (unsigned)((sp.nk() + SA_KC - 1) / SA_KC));
+  hipLaunchKernelGGL(k_scalar_adv, grid, dim3(PW_TX, PW_TY), 0, (hipStream_t)s, dims_of(g), a);
 }
 
 // zero_box: first zero the tendency on [kts, kte) x [jts, jte) x [its, ite)
@@ -399,11 +416,10 @@ int pw_advect_scalar_box(const pywrf_grid_t* g, const float* field, float* tende
   t.scalar = field; t.adv_tend = tendency; t.n3 = 0; t.scs = 0; t.sce = 0; t.accumulate = zero_box ? 0 : 1;
   t.bzero = zero_box ? mkbox(g->kts, g->kte, g->jts, g->jte, g->its, g->ite) : badv;
   t.badv = badv; t.by = by; t.bx = bx;
-  if (t.bzero.size() > 0) {
-    dim3 grid((unsigned)((t.bzero.ni() + SA_OX - 1) / SA_OX), (unsigned)((t.bzero.nj() + PW_TY - 1) / PW_TY),
-              (unsigned)((t.bzero.nk() + SA_KC - 1) / SA_KC));
-    hipLaunchKernelGGL(k_scalar_adv, grid, dim3(PW_TX, PW_TY), 0, (hipStream_t)s, dims_of(g), t);
-  }
+  t.span = t.bzero;
+  t.h_t = nullptr;
+  t.z_t = nullptr;
+  pw_launch_scalar_adv(g, t, s);
   PW_RET();
 }
 
@@ -467,18 +483,8 @@ int pw_rk_scalar_tend_fused(const pywrf_grid_t* g, int scs, int sce, int rk_step
   a.bh = mkbox(g->kts, ktf, imax(g->jds + 1, g->jts), imin(g->jde - 2, g->jte), imax(g->ids + 1, g->its),
                imin(g->ide - 2, g->ite));
   a.bv = mkbox(g->kts + 1, ktf, g->jts, jtf, g->its, itf);
-  {
-    void* hz[2] = {h_tendency, z_tendency};
-    const long long cnt[2] = {(long long)a.n3, (long long)a.n3};
-    int e = pywrf_zero_group(2, hz, cnt, s);
-    if (e) return e;
-  }
-  if (a.bzero.size() > 0) {
-    const Box& bz = a.bzero;
-    dim3 grid((unsigned)((bz.ni() + SA_OX - 1) / SA_OX), (unsigned)((bz.nj() + PW_TY - 1) / PW_TY),
-              (unsigned)((bz.nk() + SA_KC - 1) / SA_KC));
-    hipLaunchKernelGGL(k_scalar_adv, grid, dim3(PW_TX, PW_TY), 0, (hipStream_t)s, dims_of(g), a);
-  }
+  a.span = mkbox(0, g->nz, 0, g->ny, 0, g->nx);   // h_tendency and z_tendency are zeroed everywhere
+  pw_launch_scalar_adv(g, a, s);
   if (a.rk1 && a.bzero.size() > 0) {
     const Box& bz = a.bzero;
     dim3 grid((unsigned)((bz.ni() + PW_TX - 1) / PW_TX), (unsigned)((bz.nj() + PW_TY - 1) / PW_TY),

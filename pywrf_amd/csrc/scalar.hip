@@ -43,7 +43,15 @@ struct ZeroJobs {
 __global__ void k_zero_group(ZeroJobs J) {
   float* p = J.p[blockIdx.y];
   const long n = J.n[blockIdx.y];
-  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) p[t] = 0.0f;
+  const long stride = (long)gridDim.x * blockDim.x, t0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long done = 0;
+  if (((uintptr_t)p & 15) == 0) {   // 16-byte stores over the aligned body
+    const long n4 = n / 4;
+    float4* p4 = (float4*)p;
+    for (long t = t0; t < n4; t += stride) p4[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    done = n4 * 4;
+  }
+  for (long t = done + t0; t < n; t += stride) p[t] = 0.0f;
 }
 
 extern "C" int pywrf_zero_group(int n, void* const* fields, const long long* counts, pywrf_stream_t s) {
@@ -57,8 +65,9 @@ extern "C" int pywrf_zero_group(int n, void* const* fields, const long long* cou
     most = J.n[f] > most ? J.n[f] : most;
   }
   if (n == 0 || most == 0) return 0;
-  long blocks = (most + PW_BLOCK - 1) / PW_BLOCK;
+  long blocks = (most / 4 + PW_BLOCK - 1) / PW_BLOCK;
   if (blocks > 2048) blocks = 2048;
+  if (blocks < 1) blocks = 1;
   hipLaunchKernelGGL(k_zero_group, dim3((unsigned)blocks, (unsigned)n), dim3(PW_BLOCK), 0, (hipStream_t)s, J);
   return (int)hipGetLastError();
 }
