@@ -11,9 +11,9 @@ boundary values), synthetic data from pywrf_amd.gen.  A "step" is one full
 RK3 step (3 stages, 7 acoustic substeps, moisture transport, WSM6, boundary
 updates), all on the device; the state is resident in HBM before timing.
 
-The step is replayed from a HIP graph (pywrf_amd.graph) unless --eager.  The
-first step runs eagerly (it loads the code objects) and counts as a warmup
-step.  One JSON line is printed with the steps/s, the HBM roofline of the
+The timed steps are replays of HIP graphs of up to 10 steps (pywrf_amd.graph)
+unless --eager.  The warmup steps run eagerly (the first loads the code
+objects).  One JSON line is printed with the steps/s, the HBM roofline of the
 dominant kernel (timed with HIP events on the launch stream) and the CPU
 baseline (the oracle restatement of the reference, one step on the host).
 """
@@ -228,20 +228,27 @@ def main():
         torch.cuda.synchronize()
         _log("bench: kernels-only run done (copies of %d and %d bytes)" % (src.numel() * 4, big.numel() * 4))
         return
-    if args.eager:
-        step = lambda: rk3_step(s, ops, cfg)
-    else:
-        _log("bench: capturing the step as a HIP graph")
-        from pywrf_amd.graph import StepGraph
-        graph = StepGraph(s, cfg, ops)
-        step = graph.replay
+    step = lambda: rk3_step(s, ops, cfg)
     for _ in range(max(args.warmup - 1, 0)):
-        step()
+        rk3_step(s, ops, cfg)
+    torch.cuda.synchronize()
+    if args.eager:
+        runs = [(step, args.steps)]
+    else:
+        # the timed steps as replays of graphs of up to 10 steps each
+        from pywrf_amd.graph import StepGraph
+        per = min(10, args.steps)
+        full, rest = divmod(args.steps, per)
+        _log("bench: capturing a %d-step HIP graph%s" % (per, " (+ a %d-step one)" % rest if rest else ""))
+        runs = [(StepGraph(s, cfg, ops, steps=per).replay, full)]
+        if rest:
+            runs.append((StepGraph(s, cfg, ops, steps=rest).replay, 1))
     torch.cuda.synchronize()
     _log("bench: timing %d steps" % args.steps)
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
+    for fn, times in runs:
+        for _ in range(times):
+            fn()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ms = elapsed / args.steps * 1e3

@@ -20,7 +20,10 @@ from .step import rk3_step
 
 
 class StepGraph:
-    def __init__(self, s, cfg, ops=None):
+    """`steps` consecutive RK3 steps captured as one graph (the rebound fields
+    are copied back once per replay, so longer graphs amortise the copies)."""
+
+    def __init__(self, s, cfg, ops=None, steps=1):
         from . import ops as _ops
         ops = ops or _ops()
         for var, arrs in s.bdy.items():
@@ -28,14 +31,17 @@ class StepGraph:
                 if bool((a != 0).any()):
                     raise ValueError("boundary tendency of %r is not zero: dtbc changes every step, "
                                      "capture is only valid for constant boundaries" % var)
-        self.s, self.cfg, self.ops = s, cfg, ops
+        if steps < 1:
+            raise ValueError("steps must be >= 1")
+        self.s, self.cfg, self.ops, self.steps = s, cfg, ops, steps
         names = [n for n, v in vars(s).items() if isinstance(v, torch.Tensor)]
         orig = {n: getattr(s, n) for n in names}
         step0, it0 = s.step, s.it_bdy
         torch.cuda.synchronize()
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
-            rk3_step(s, ops, cfg)
+            for _ in range(steps):
+                rk3_step(s, ops, cfg)
             for n in names:
                 t = getattr(s, n)
                 if t is not orig[n]:
@@ -45,10 +51,12 @@ class StepGraph:
         s.step, s.it_bdy = step0, it0
 
     def replay(self, n=1):
-        """Advance the state by n RK3 steps (asynchronous)."""
+        """Advance the state by n replays of the graph, n * steps RK3 steps
+        (asynchronous)."""
         s, cfg = self.s, self.cfg
         for _ in range(n):
-            s.step += 1
-            if s.step % cfg.bdy_interval == 1:
-                s.it_bdy = int(s.step / cfg.bdy_interval)
+            for _ in range(self.steps):
+                s.step += 1
+                if s.step % cfg.bdy_interval == 1:
+                    s.it_bdy = int(s.step / cfg.bdy_interval)
             self.graph.replay()
