@@ -243,24 +243,6 @@ int pywrf_advance_mu_t(const pywrf_grid_t* g, const float* ww_in, const float* w
                        const float* msfuy, const float* msfvx_inv, const float* msftx, const float* msfty,
                        float* ww_out, float* t_out, float* t_ave_out, float* muave_out, float* muts_out,
                        float* mudf_out, float* mu_out, pywrf_stream_t s);
-/* advance_mu_t followed by the same acoustic step's sumflux (wrf_dynamics.py:
- * 3799-3839) in one launch: the arguments of pywrf_advance_mu_t, then
- * sumflux's u_lin, v_lin, ww_lin, ru_m, rv_m, ww_m (updated in place),
- * iteration and number_of_small_timesteps; sumflux reads u, v and the new ww.
- * Equal to the two calls in the reference's order (advance_w, the ph boundary
- * update and the w zero-gradient fill in between touch none of sumflux's
- * inputs or outputs).  No counterpart in the reference: a fusion. */
-int pywrf_advance_mu_t_sumflux(const pywrf_grid_t* g, const float* ww_in, const float* ww_1, const float* u,
-                               const float* u_1, const float* v, const float* v_1, const float* mu_in,
-                               const float* mut, const float* muave_in, const float* muts_in, const float* muu,
-                               const float* muv, const float* mudf_in, const float* t_in, const float* t_1,
-                               const float* t_ave_in, const float* ft, const float* mu_tend, double rdx, double rdy,
-                               double dts, double epssm, const float* dnw, const float* fnm, const float* fnp,
-                               const float* rdnw, const float* msfuy, const float* msfvx_inv, const float* msftx,
-                               const float* msfty, float* ww_out, float* t_out, float* t_ave_out, float* muave_out,
-                               float* muts_out, float* mudf_out, float* mu_out, const float* u_lin,
-                               const float* v_lin, const float* ww_lin, float* ru_m, float* rv_m, float* ww_m,
-                               int iteration, int number_of_small_timesteps, pywrf_stream_t s);
 /* advance_w, wrf_dynamics.py:3641-3796 (t_2ave_out, w_out, ph_out fresh) */
 int pywrf_advance_w(const pywrf_grid_t* g, const float* w_in, const float* rw_tend, const float* ww,
                     const float* w_save, const float* u, const float* v, const float* mut, const float* muave,

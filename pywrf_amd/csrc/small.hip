@@ -352,35 +352,7 @@ struct AdvMuT {
   float *ww, *t, *t_ave, *muave, *muts, *mudf, *mu;
   float rdx, rdy, dts, ep1, em1, hrdx, hrdy;
   int k1, kde, j0, j1, i0, i1;
-  // sumflux of the same acoustic step, fused (sf != 0): accumulates u, v and the
-  // new ww into ru_m, rv_m, ww_m (wrf_dynamics.py:3799-3839)
-  const float *u_lin, *v_lin, *ww_lin;
-  float *ru_m, *rv_m, *ww_m;
-  Box sfb, sfu, sfv, sfw;
-  int sf, sf_first, sf_last;
-  float sf_n;
 };
-
-__device__ inline void mut_sumflux(const Dims& D, const AdvMuT& a, int k, int j, int i, long q, float uq, float vq,
-                                   float wq) {
-  if (!a.sfb.has(k, j, i)) return;
-  float rm = a.sf_first ? 0.0f : a.ru_m[q], vm = a.sf_first ? 0.0f : a.rv_m[q], wm = a.sf_first ? 0.0f : a.ww_m[q];
-  if (a.sfu.has(k, j, i)) {
-    rm = rm + uq;
-    if (a.sf_last) rm = rm / a.sf_n + AT2(a.muu, j, i) * a.u_lin[q] / AT2(a.msfuy, j, i);
-  }
-  if (a.sfv.has(k, j, i)) {
-    vm = vm + vq;
-    if (a.sf_last) vm = vm / a.sf_n + AT2(a.muv, j, i) * a.v_lin[q] * AT2(a.msfvx_inv, j, i);
-  }
-  if (a.sfw.has(k, j, i)) {
-    wm = wm + wq;
-    if (a.sf_last) wm = wm / a.sf_n + a.ww_lin[q];
-  }
-  a.ru_m[q] = rm;
-  a.rv_m[q] = vm;
-  a.ww_m[q] = wm;
-}
 
 // Operands are loaded into registers before the arithmetic so that the loads
 // of one level are in flight together.
@@ -427,11 +399,9 @@ __global__ void __launch_bounds__(TB) k_advance_mu_t(Dims D, AdvMuT a) {
     if (!vi) continue;
     long q = D.ix(k, j, i);
     if (!h) {
-      const float wq = a.ww_in[q];
-      a.ww[q] = wq;
+      a.ww[q] = a.ww_in[q];
       a.t[q] = a.t_in[q];
       a.t_ave[q] = a.t_ave_in[q];
-      if (a.sf) mut_sumflux(D, a, k, j, i, q, a.u[q], a.v[q], wq);
       continue;
     }
     s_dv[k * TI + lane] = k < a.k1 ? mut_dvdxi(D, a, k, j, i) : 0.0f;
@@ -466,11 +436,9 @@ __global__ void __launch_bounds__(TB) k_advance_mu_t(Dims D, AdvMuT a) {
   for (int k = grp; k < nz; k += TG) {
     long q = D.ix(k, j, i);
     if (k >= a.k1) {
-      const float wq = a.ww_in[q];
-      a.ww[q] = wq;
+      a.ww[q] = a.ww_in[q];
       a.t[q] = a.t_in[q];
       a.t_ave[q] = a.t_ave_in[q];
-      if (a.sf) mut_sumflux(D, a, k, j, i, q, a.u[q], a.v[q], wq);
       continue;
     }
     const float tin = a.t_in[q], ftq = a.ft[q], t1 = a.t_1[q], w1k = a.ww_1[q];
@@ -490,7 +458,6 @@ __global__ void __launch_bounds__(TB) k_advance_mu_t(Dims D, AdvMuT a) {
     float xterm = a.hrdx * (ue * (te + t1) - uw * (t1 + tw));
     t = t - a.dts * my * (mx * (yterm + xterm) + s_rdnw[k] * (wdu - wdk));
     a.t[q] = t;
-    if (a.sf) mut_sumflux(D, a, k, j, i, q, uw, vs, wwk);
   }
 }
 
@@ -499,63 +466,28 @@ static int column_block_launch_check(const pywrf_grid_t* g, int narrays) {
   return bytes <= 64 * 1024 ? 0 : PYWRF_E_RANGE;
 }
 
-static int advance_mu_t_launch(const pywrf_grid_t* g, AdvMuT& a, pywrf_stream_t s) {
+extern "C" int pywrf_advance_mu_t(const pywrf_grid_t* g, const float* ww_in, const float* ww_1, const float* u,
+                                  const float* u_1, const float* v, const float* v_1, const float* mu_in,
+                                  const float* mut, const float* muave_in, const float* muts_in, const float* muu,
+                                  const float* muv, const float* mudf_in, const float* t_in, const float* t_1,
+                                  const float* t_ave_in, const float* ft, const float* mu_tend, double rdx,
+                                  double rdy, double dts, double epssm, const float* dnw, const float* fnm,
+                                  const float* fnp, const float* rdnw, const float* msfuy, const float* msfvx_inv,
+                                  const float* msftx, const float* msfty, float* ww_out, float* t_out,
+                                  float* t_ave_out, float* muave_out, float* muts_out, float* mudf_out,
+                                  float* mu_out, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
+  CHECK(g, 1);
+  if (g->kte > g->nz || g->kte < 2) return PYWRF_E_RANGE;
+  AdvMuT a = {ww_in, ww_1, u, u_1, v, v_1, mu_in, mut, muave_in, muts_in, muu, muv, mudf_in, t_in, t_1, t_ave_in,
+              ft, mu_tend, dnw, fnm, fnp, rdnw, msfuy, msfvx_inv, msftx, msfty, ww_out, t_out, t_ave_out,
+              muave_out, muts_out, mudf_out, mu_out, F(rdx), F(rdy), F(dts), F(1. + epssm), F(1. - epssm),
+              F(0.5 * rdx), F(0.5 * rdy), g->kte - 1, g->kde, imax(g->jts, g->jds + 1), imin(g->jte, g->jde - 2),
+              imax(g->its, g->ids + 1), imin(g->ite, g->ide - 2)};
   if (column_block_launch_check(g, 2)) return PYWRF_E_RANGE;
   dim3 grid((g->nx + TI - 1) / TI, g->ny);
   hipLaunchKernelGGL(k_advance_mu_t, grid, dim3(TB), 2 * g->nz * TI * sizeof(float), (hipStream_t)s, dims_of(g), a);
   PW_RET();
-}
-
-#define ADV_MU_T_PARAMS                                                                                            \
-  const pywrf_grid_t *g, const float *ww_in, const float *ww_1, const float *u, const float *u_1, const float *v,   \
-      const float *v_1, const float *mu_in, const float *mut, const float *muave_in, const float *muts_in,         \
-      const float *muu, const float *muv, const float *mudf_in, const float *t_in, const float *t_1,              \
-      const float *t_ave_in, const float *ft, const float *mu_tend, double rdx, double rdy, double dts,           \
-      double epssm, const float *dnw, const float *fnm, const float *fnp, const float *rdnw, const float *msfuy, \
-      const float *msfvx_inv, const float *msftx, const float *msfty, float *ww_out, float *t_out,               \
-      float *t_ave_out, float *muave_out, float *muts_out, float *mudf_out, float *mu_out
-
-static int advance_mu_t_args(ADV_MU_T_PARAMS, AdvMuT& a) {
-  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
-  CHECK(g, 1);
-  if (g->kte > g->nz || g->kte < 2) return PYWRF_E_RANGE;
-  a = {ww_in, ww_1, u, u_1, v, v_1, mu_in, mut, muave_in, muts_in, muu, muv, mudf_in, t_in, t_1, t_ave_in,
-       ft, mu_tend, dnw, fnm, fnp, rdnw, msfuy, msfvx_inv, msftx, msfty, ww_out, t_out, t_ave_out,
-       muave_out, muts_out, mudf_out, mu_out, F(rdx), F(rdy), F(dts), F(1. + epssm), F(1. - epssm),
-       F(0.5 * rdx), F(0.5 * rdy), g->kte - 1, g->kde, imax(g->jts, g->jds + 1), imin(g->jte, g->jde - 2),
-       imax(g->its, g->ids + 1), imin(g->ite, g->ide - 2)};
-  a.sf = 0;
-  return 0;
-}
-
-extern "C" int pywrf_advance_mu_t(ADV_MU_T_PARAMS, pywrf_stream_t s) {
-  AdvMuT a;
-  int e = advance_mu_t_args(g, ww_in, ww_1, u, u_1, v, v_1, mu_in, mut, muave_in, muts_in, muu, muv, mudf_in, t_in,
-                            t_1, t_ave_in, ft, mu_tend, rdx, rdy, dts, epssm, dnw, fnm, fnp, rdnw, msfuy, msfvx_inv,
-                            msftx, msfty, ww_out, t_out, t_ave_out, muave_out, muts_out, mudf_out, mu_out, a);
-  if (e) return e;
-  return advance_mu_t_launch(g, a, s);
-}
-
-extern "C" int pywrf_advance_mu_t_sumflux(ADV_MU_T_PARAMS, const float* u_lin, const float* v_lin,
-                                          const float* ww_lin, float* ru_m, float* rv_m, float* ww_m, int iteration,
-                                          int number_of_small_timesteps, pywrf_stream_t s) {
-  AdvMuT a;
-  int e = advance_mu_t_args(g, ww_in, ww_1, u, u_1, v, v_1, mu_in, mut, muave_in, muts_in, muu, muv, mudf_in, t_in,
-                            t_1, t_ave_in, ft, mu_tend, rdx, rdy, dts, epssm, dnw, fnm, fnp, rdnw, msfuy, msfvx_inv,
-                            msftx, msfty, ww_out, t_out, t_ave_out, muave_out, muts_out, mudf_out, mu_out, a);
-  if (e) return e;
-  if (number_of_small_timesteps == 0) return PYWRF_E_RANGE;
-  // sumflux's index sets (the same as pywrf_sumflux)
-  int mini = imin(g->ide - 1, g->ite), minj = imin(g->jde - 1, g->jte), mink = imin(g->kde - 1, g->kte);
-  a.sfu = mkbox(g->kts, mink, g->jts, minj, g->its, g->ite);
-  a.sfv = mkbox(g->kts, mink, g->jts, g->jte, g->its, mini);
-  a.sfw = mkbox(g->kts, g->kte, g->jts, minj, g->its, mini);
-  a.sfb = mkbox(g->kts, g->kte, g->jts, g->jte, g->its, g->ite);
-  a.u_lin = u_lin; a.v_lin = v_lin; a.ww_lin = ww_lin; a.ru_m = ru_m; a.rv_m = rv_m; a.ww_m = ww_m;
-  a.sf = 1; a.sf_first = iteration == 1; a.sf_last = iteration == number_of_small_timesteps;
-  a.sf_n = (float)number_of_small_timesteps;
-  return advance_mu_t_launch(g, a, s);
 }
 
 // ------------------------------------------------------------------ advance_w

@@ -108,7 +108,6 @@ def rk3_step(s, ops, cfg, check_nan=False):
     bc2 = lambda a, v: o.set_physical_bc2d(a, v, *I2)
     grouped = hasattr(o, "set_physical_bc_group")
     grouped_upd = hasattr(o, "spec_bdyupdate_group")
-    fused_sumflux = hasattr(o, "advance_mu_t_sumflux")
 
     def bcs(*items):
         """A run of consecutive open-boundary fills, items (tensor, flag, is_2d);
@@ -291,17 +290,11 @@ def rk3_step(s, ops, cfg, check_nan=False):
             else:
                 s.u = o.spec_bdyupdate(s.u, s.ru_tend, dts_rk, 'u', SPEC_ZONE, *I24)
                 s.v = o.spec_bdyupdate(s.v, s.rv_tend, dts_rk, 'v', SPEC_ZONE, *I24)
-            amt = (s.ww, s.ww1, s.u, s.u_save, s.v, s.v_save, s.mu, s.mut, s.muave, s.muts, s.muu, s.muv, s.mudf,
-                   s.c1h, s.c2h, s.c1f, s.c2f, s.c3h, s.c4h, s.c3f, s.c4f, s.ru_m, s.rv_m, s.ww_m, s.t, s.t_save,
-                   s.t_2save, s.t_tend, s.mu_tend, rdx, rdy, dts_rk, EPSSM, s.dnw, s.fnm, s.fnp, s.rdnw, s.msfux,
-                   s.msfuy, s.msfvx, s.msfvx_inv, s.msfvy, s.msftx, s.msfty, iteration)
-            if fused_sumflux:
-                # sumflux (below, after advance_w) reads u, v and this ww only: done in the same launch
-                (s.ww, s.ww1, s.t, s.t_2save, s.ru_m, s.rv_m, s.ww_m, s.muave, s.muts, s.mudf,
-                 s.mu) = o.advance_mu_t_sumflux(*amt, s.u_save, s.v_save, s.ww1, nsmall, *I18)
-            else:
-                (s.ww, s.ww1, s.t, s.t_2save, s.ru_m, s.rv_m, s.ww_m, s.muave, s.muts, s.mudf,
-                 s.mu) = o.advance_mu_t(*amt, *I18)
+            (s.ww, s.ww1, s.t, s.t_2save, s.ru_m, s.rv_m, s.ww_m, s.muave, s.muts, s.mudf, s.mu) = o.advance_mu_t(
+                s.ww, s.ww1, s.u, s.u_save, s.v, s.v_save, s.mu, s.mut, s.muave, s.muts, s.muu, s.muv, s.mudf,
+                s.c1h, s.c2h, s.c1f, s.c2f, s.c3h, s.c4h, s.c3f, s.c4f, s.ru_m, s.rv_m, s.ww_m, s.t, s.t_save,
+                s.t_2save, s.t_tend, s.mu_tend, rdx, rdy, dts_rk, EPSSM, s.dnw, s.fnm, s.fnp, s.rdnw, s.msfux,
+                s.msfuy, s.msfvx, s.msfvx_inv, s.msfvy, s.msftx, s.msfty, iteration, *I18)
             if grouped_upd:
                 o.spec_bdyupdate_group([(s.t, s.t_tend, 't', 0), (s.mu, s.mu_tend, 'm', 1),
                                         (s.muts, s.mu_tend, 'm', 1)], dts_rk, I24, ix_2d)
@@ -314,11 +307,10 @@ def rk3_step(s, ops, cfg, check_nan=False):
                 s.c3h, s.c4h, s.c3f, s.c4f, s.t_2save, s.t, s.t_save, s.ph, s.ph_save, s.phb, s.ph_tend, s.ht, s.c2a,
                 s.cqw, s.alt, s.alb, s.a, s.alpha, s.gamma, rdx, rdy, dts_rk, T0, EPSSM, s.dnw, s.fnm, s.fnp, s.rdnw,
                 s.rdn, s.cf1, s.cf2, s.cf3, s.msftx, s.msfty, False, *I18)
-            if not fused_sumflux:
-                s.ru_m, s.rv_m, s.ww_m = o.sumflux(s.u, s.v, s.ww, s.u_save, s.v_save, s.ww1, s.muu, s.muv, s.c1h,
-                                                   s.c2h, s.c1f, s.c2f, s.c3h, s.c4h, s.c3f, s.c4f, s.ru_m, s.rv_m,
-                                                   s.ww_m, EPSSM, s.msfux, s.msfuy, s.msfvx, s.msfvx_inv, s.msfvy,
-                                                   iteration, nsmall, *I18)
+            s.ru_m, s.rv_m, s.ww_m = o.sumflux(s.u, s.v, s.ww, s.u_save, s.v_save, s.ww1, s.muu, s.muv, s.c1h, s.c2h,
+                                               s.c1f, s.c2f, s.c3h, s.c4h, s.c3f, s.c4f, s.ru_m, s.rv_m, s.ww_m, EPSSM,
+                                               s.msfux, s.msfuy, s.msfvx, s.msfvx_inv, s.msfvy, iteration, nsmall,
+                                               *I18)
             s.ph = o.spec_bdyupdate_ph(s.ph_save, s.ph, s.ph_tend, s.mu_tend, s.muts, s.c1f, s.c2f, dts_rk, 'h',
                                        SPEC_ZONE, *I24)
             s.w = o.zero_grad_bdy(s.w, 'w', SPEC_ZONE, *I24)

@@ -639,37 +639,6 @@ def advance_mu_t(ww_ori, ww_1, u, u_1, v, v_1,
     return ww, ww_1, t, t_ave, uam, vam, wwam, muave, muts, mudf, mu
 
 
-def advance_mu_t_sumflux(ww_ori, ww_1, u, u_1, v, v_1,
-                         mu_ori, mut, muave_ori, muts_ori, muu, muv, mudf_ori,
-                         c1h, c2h, c1f, c2f,
-                         c3h, c4h, c3f, c4f,
-                         uam, vam, wwam, t_ori, t_1,
-                         t_ave_ori, ft, mu_tend,
-                         rdx, rdy, dts, epssm,
-                         dnw, fnm, fnp, rdnw,
-                         msfux, msfuy, msfvx, msfvx_inv,
-                         msfvy, msftx, msfty,
-                         step,
-                         u_lin, v_lin, ww_lin, number_of_small_timesteps,
-                         ids, ide, jds, jde, kds, kde,
-                         ims, ime, jms, jme, kms, kme,
-                         its, ite, jts, jte, kts, kte):
-    """advance_mu_t (wrf_dynamics.py:3528-3638) fused with the same acoustic
-    step's sumflux (wrf_dynamics.py:3799-3839): returns advance_mu_t's 11-tuple;
-    uam, vam, wwam (ru_m, rv_m, ww_m) are accumulated in place from u, v and the
-    new ww with iteration = step.  Not part of the reference API."""
-    g = grid(u.shape, (ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, its, ite, jts, jte, kts, kte))
-    ww, t, t_ave = fresh(ww_ori), fresh(t_ori), fresh(t_ave_ori)
-    muave, muts, mudf, mu = fresh(muave_ori), fresh(muts_ori), fresh(mudf_ori), fresh(mu_ori)
-    call("pywrf_advance_mu_t_sumflux", g, ptr(ww_ori), ptr(ww_1), ptr(u), ptr(u_1), ptr(v), ptr(v_1), ptr(mu_ori),
-         ptr(mut), ptr(muave_ori), ptr(muts_ori), ptr(muu), ptr(muv), ptr(mudf_ori), ptr(t_ori), ptr(t_1),
-         ptr(t_ave_ori), ptr(ft), ptr(mu_tend), float(rdx), float(rdy), float(dts), float(epssm), ptr(dnw), ptr(fnm),
-         ptr(fnp), ptr(rdnw), ptr(msfuy), ptr(msfvx_inv), ptr(msftx), ptr(msfty), ptr(ww), ptr(t), ptr(t_ave),
-         ptr(muave), ptr(muts), ptr(mudf), ptr(mu), ptr(u_lin), ptr(v_lin), ptr(ww_lin), ptr(uam), ptr(vam),
-         ptr(wwam), int(step), int(number_of_small_timesteps), stream())
-    return ww, ww_1, t, t_ave, uam, vam, wwam, muave, muts, mudf, mu
-
-
 def advance_w(w_ori, rw_tend, ww, w_save, u, v,
               mu1, mut, muave, muts,
               c1h, c2h, c1f, c2f,

@@ -79,33 +79,3 @@ def test_flow_dep_group_equals_individual_calls(n):
     bdy.flow_dep_bdy_group([q[im] for im in range(2, 7)], u, v, g.ix24)
     torch.cuda.synchronize()
     assert torch.equal(ref, q)
-
-
-@pytest.mark.parametrize("iteration,nsmall", [(1, 4), (2, 4), (4, 4), (1, 1)])
-def test_advance_mu_t_sumflux_equals_the_two_calls(iteration, nsmall):
-    """The fused entry point against advance_mu_t followed by sumflux, on the
-    state after one RK3 step of the 40x40 moist workload (bit for bit)."""
-    import pywrf_amd
-    from pywrf_amd.step import RunConfig
-    from parity import state, run
-    cfg = RunConfig()
-    o = pywrf_amd.ops()
-    s = state("config1c", device="cuda")
-    run(s, o, 1)
-    I18 = s.grid.ix18
-    args = lambda: (s.ww, s.ww1, s.u, s.u_save, s.v, s.v_save, s.mu, s.mut, s.muave, s.muts, s.muu, s.muv, s.mudf,
-                    s.c1h, s.c2h, s.c1f, s.c2f, s.c3h, s.c4h, s.c3f, s.c4f, rm, vm, wm, s.t, s.t_save, s.t_2save,
-                    s.t_tend, s.mu_tend, cfg.rdx, cfg.rdy, 13.5, 0.1, s.dnw, s.fnm, s.fnp, s.rdnw, s.msfux, s.msfuy,
-                    s.msfvx, s.msfvx_inv, s.msfvy, s.msftx, s.msfty, iteration)
-    rm, vm, wm = s.ru_m.clone(), s.rv_m.clone(), s.ww_m.clone()
-    ref = o.advance_mu_t(*args(), *I18)
-    o.sumflux(s.u, s.v, ref[0], s.u_save, s.v_save, s.ww1, s.muu, s.muv, s.c1h, s.c2h, s.c1f, s.c2f, s.c3h, s.c4h,
-              s.c3f, s.c4f, rm, vm, wm, 0.1, s.msfux, s.msfuy, s.msfvx, s.msfvx_inv, s.msfvy, iteration, nsmall, *I18)
-    ref_m = (rm, vm, wm)
-    rm, vm, wm = s.ru_m.clone(), s.rv_m.clone(), s.ww_m.clone()
-    got = o.advance_mu_t_sumflux(*args(), s.u_save, s.v_save, s.ww1, nsmall, *I18)
-    torch.cuda.synchronize()
-    for n, (a, b) in enumerate(zip(got, ref)):
-        assert torch.equal(a, b), n
-    for a, b in zip((rm, vm, wm), ref_m):
-        assert torch.equal(a, b)
