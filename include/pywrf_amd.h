@@ -166,6 +166,17 @@ int pywrf_curvature(const pywrf_grid_t* g, const float* ru, const float* rv, con
                     const float* v, float* ru_tend, float* rv_tend, float* rw_tend, const float* msfuy,
                     const float* msfvx, const float* msfvy, const float* msftx, const float* msfty,
                     const float* fzm, const float* fzp, double rdx, double rdy, pywrf_stream_t s);
+/* coriolis then curvature, as the driver calls them back to back
+ * (solver.py:1063-1079), in one pass over the tendencies: the arguments of
+ * both (ru, rv, rw, u, v, the three tendencies updated in place, the map
+ * factors, f, e, sina, cosa, fzm, fzp, rdx, rdy).  Equal to the two calls.
+ * No counterpart in the reference: a fusion. */
+int pywrf_coriolis_curvature(const pywrf_grid_t* g, const float* ru, const float* rv, const float* rw,
+                             const float* u, const float* v, float* ru_tend, float* rv_tend, float* rw_tend,
+                             const float* msftx, const float* msfty, const float* msfux, const float* msfuy,
+                             const float* msfvx, const float* msfvy, const float* f, const float* e,
+                             const float* sina, const float* cosa, const float* fzm, const float* fzp, double rdx,
+                             double rdy, pywrf_stream_t s);
 /* horizontal_diffusion, wrf_dynamics.py:2623-2838 (name: 'u', 'v', 'w' or a scalar name) */
 int pywrf_horizontal_diffusion(const pywrf_grid_t* g, char name, const float* field, float* tendency,
                                const float* mut, const float* msfux, const float* msfuy, const float* msfvx,

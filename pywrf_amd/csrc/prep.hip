@@ -173,15 +173,32 @@ __device__ inline float qsum(const Dims& D, const float* moist, int n, int k, in
   return s;
 }
 
+// Tiles of 63 i x 4 j points marching CQ_KC levels: lane l sums the species
+// at i0 - 1 + l, so the point at i takes the sum at i - 1 from the lane to
+// its left, and the sum at k - 1 is carried from the level below; only the
+// sum at j - 1 is evaluated again.  Each sum is the same in-order float sum.
+#define CQ_OX 63
+#define CQ_KC 4
 __global__ void k_calc_cq(Box b, Dims D, Box bu, Box bv, Box bw, const float* moist, int n, float* cqu,
                           float* cqv, float* cqw) {
-  int k, j, i;
-  if (!box_point(b, k, j, i)) return;
-  long p = D.ix(k, j, i);
-  float s0 = qsum(D, moist, n, k, j, i);
-  if (bu.has(k, j, i)) cqu[p] = 1.0f / (1.0f + 0.5f * (s0 + qsum(D, moist, n, k, j, i - 1)));
-  if (bv.has(k, j, i)) cqv[p] = 1.0f / (1.0f + 0.5f * (s0 + qsum(D, moist, n, k, j - 1, i)));
-  if (bw.has(k, j, i)) cqw[p] = 0.5f * (s0 + qsum(D, moist, n, k - 1, j, i));
+  const int i = b.i0 - 1 + (int)(blockIdx.x * CQ_OX + threadIdx.x);
+  const int j = b.j0 + (int)(blockIdx.y * PW_TY + threadIdx.y);
+  if (j >= b.j1) return;                      // a whole wave (one row) leaves together
+  const int k0 = b.k0 + (int)blockIdx.z * CQ_KC, k1 = imin(k0 + CQ_KC, b.k1);
+  const bool live = i < b.i1;
+  const bool own = threadIdx.x >= 1 && live;
+  float sdown = (live && k0 >= 1) ? qsum(D, moist, n, k0 - 1, j, i) : 0.0f;
+  for (int k = k0; k < k1; ++k) {
+    const float s0 = live ? qsum(D, moist, n, k, j, i) : 0.0f;
+    const float sw = __shfl_up(s0, 1, PW_TX);
+    if (own) {
+      const int p = D.ix(k, j, i);
+      if (bu.has(k, j, i)) cqu[p] = 1.0f / (1.0f + 0.5f * (s0 + sw));
+      if (bv.has(k, j, i)) cqv[p] = 1.0f / (1.0f + 0.5f * (s0 + qsum(D, moist, n, k, j - 1, i)));
+      if (bw.has(k, j, i)) cqw[p] = 0.5f * (s0 + sdown);
+    }
+    sdown = s0;
+  }
 }
 
 extern "C" int pywrf_calc_cq(const pywrf_grid_t* g, const float* moist, int n_moist, float* cqu, float* cqv,
@@ -194,7 +211,12 @@ extern "C" int pywrf_calc_cq(const pywrf_grid_t* g, const float* moist, int n_mo
   Box bv = mkbox(g->kts, ktf, g->jts, g->jte, g->its, itf);
   Box bw = mkbox(g->kts + 1, ktf, g->jts, jtf, g->its, itf);
   Box b = mkbox(g->kts, ktf, g->jts, g->jte, g->its, g->ite);
-  PW_LAUNCH_BOX(k_calc_cq, b, s, dims_of(g), bu, bv, bw, moist, n_moist, cqu, cqv, cqw);
+  if (b.size() > 0) {
+    dim3 grid((unsigned)((b.ni() + CQ_OX - 1) / CQ_OX), (unsigned)((b.nj() + PW_TY - 1) / PW_TY),
+              (unsigned)((b.nk() + CQ_KC - 1) / CQ_KC));
+    hipLaunchKernelGGL(k_calc_cq, grid, dim3(PW_TX, PW_TY), 0, (hipStream_t)s, b, dims_of(g), bu, bv, bw, moist,
+                       n_moist, cqu, cqv, cqw);
+  }
   PW_RET();
 }
 

@@ -326,6 +326,73 @@ extern "C" int pywrf_curvature(const pywrf_grid_t* g, const float* ru, const flo
   PW_RET();
 }
 
+// ------------------------------------------------------------------ coriolis, then curvature
+// The two calls the driver makes back to back (solver.py:1063-1079) in one
+// pass: same boxes, each tendency point updated by the Coriolis terms and then
+// by the curvature terms, in that order (the same floats as the two kernels).
+__global__ void k_cor_curv(Box b, Dims D, Cor a, Curv c, Box bu, Box bv, Box bw) {
+  int k, j, i;
+  if (!box_point(b, k, j, i)) return;
+  const int q = D.ix(k, j, i);
+  const float rr = F(pw::reradius);
+  if (bu.has(k, j, i)) {
+    const float rv4 = AT(a.rv, k, j + 1, i - 1) + AT(a.rv, k, j + 1, i) + AT(a.rv, k, j, i - 1) + AT(a.rv, k, j, i);
+    const float rw4 = AT(a.rw, k + 1, j, i - 1) + AT(a.rw, k, j, i - 1) + AT(a.rw, k + 1, j, i) + AT(a.rw, k, j, i);
+    float x = (AT2(a.msfux, j, i) / AT2(a.msfuy, j, i)) * 0.5f * (AT2(a.f, j, i) + AT2(a.f, j, i - 1)) * 0.25f * rv4;
+    float y = 0.5f * (AT2(a.e, j, i) + AT2(a.e, j, i - 1)) * 0.5f * (AT2(a.cosa, j, i) + AT2(a.cosa, j, i - 1)) *
+              0.25f * rw4;
+    float t = a.ru_tend[q] + x - y;
+    x = 0.5f * (curv_vxgm(D, c, k, j, i) + curv_vxgm(D, c, k, j, i - 1)) * 0.25f * rv4;
+    y = AT(c.u, k, j, i) * rr * 0.25f * rw4;
+    a.ru_tend[q] = t + x - y;
+  }
+  if (bv.has(k, j, i)) {
+    const float ru4 = AT(a.ru, k, j, i) + AT(a.ru, k, j, i + 1) + AT(a.ru, k, j - 1, i) + AT(a.ru, k, j - 1, i + 1);
+    const float rw4 = AT(a.rw, k + 1, j - 1, i) + AT(a.rw, k, j - 1, i) + AT(a.rw, k + 1, j, i) + AT(a.rw, k, j, i);
+    float r = AT2(a.msfvy, j, i) / AT2(a.msfvx, j, i);
+    float x = r * 0.5f * (AT2(a.f, j, i) + AT2(a.f, j - 1, i)) * 0.25f * ru4;
+    float y = r * 0.5f * (AT2(a.e, j, i) + AT2(a.e, j - 1, i)) * 0.5f * (AT2(a.sina, j, i) + AT2(a.sina, j - 1, i)) *
+              0.25f * rw4;
+    float t = a.rv_tend[q] - x + y;
+    x = 0.5f * (curv_vxgm(D, c, k, j, i) + curv_vxgm(D, c, k, j - 1, i)) * 0.25f * ru4;
+    y = r * AT(c.v, k, j, i) * rr * 0.25f * rw4;
+    a.rv_tend[q] = t - x - y;
+  }
+  if (bw.has(k, j, i)) {
+    const float fzm = a.fzm[k], fzp = a.fzp[k];
+    auto fz = [&](const float* f, int dj, int di) {
+      return fzm * (AT(f, k, j, i) + AT(f, k, j + dj, i + di)) + fzp * (AT(f, k - 1, j, i) + AT(f, k - 1, j + dj, i + di));
+    };
+    const float mr = AT2(a.msftx, j, i) / AT2(a.msfty, j, i);
+    float cu = AT2(a.cosa, j, i) * 0.5f * fz(a.ru, 0, 1);
+    float sv = mr * AT2(a.sina, j, i) * 0.5f * fz(a.rv, 1, 0);
+    float t = a.rw_tend[q] + AT2(a.e, j, i) * (cu - sv);
+    a.rw_tend[q] = t + rr * (0.5f * fz(a.ru, 0, 1) * 0.5f * fz(c.u, 0, 1) + mr * 0.5f * fz(a.rv, 1, 0) * 0.5f * fz(c.v, 1, 0));
+  }
+}
+
+extern "C" int pywrf_coriolis_curvature(const pywrf_grid_t* g, const float* ru, const float* rv, const float* rw,
+                                        const float* u, const float* v, float* ru_tend, float* rv_tend,
+                                        float* rw_tend, const float* msftx, const float* msfty, const float* msfux,
+                                        const float* msfuy, const float* msfvx, const float* msfvy, const float* f,
+                                        const float* e, const float* sina, const float* cosa, const float* fzm,
+                                        const float* fzp, double rdx, double rdy, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
+  CHECK(g, 2);
+  int ktf = imin(g->kte, g->kde - 1);
+  if (ktf + 1 > g->nz) return PYWRF_E_RANGE;
+  Cor a = {ru, rv, rw, msftx, msfty, msfux, msfuy, msfvx, msfvy, f, e, sina, cosa, fzm, fzp, ru_tend, rv_tend,
+           rw_tend};
+  Curv c = {ru, rv, rw, u, v, msfuy, msfvx, msfvy, msftx, msfty, fzm, fzp, ru_tend, rv_tend, rw_tend,
+            F(rdx), F(rdy), g->jts, g->jte, g->its, g->ite};
+  Box bu = mkbox(g->kts, ktf, g->jts, g->jte - 1, imax(g->ids + 1, g->its), imin(g->ide - 1, g->ite));
+  Box bv = mkbox(g->kts, ktf, imax(g->jds + 1, g->jts), imin(g->jde - 1, g->jte), g->its, g->ite - 1);
+  Box bw = mkbox(g->kts + 1, ktf, g->jts, g->jte - 1, g->its, g->ite - 1);
+  Box b = mkbox(g->kts, ktf, g->jts, g->jte, g->its, g->ite);
+  PW_LAUNCH_BOX(k_cor_curv, b, s, dims_of(g), a, c, bu, bv, bw);
+  PW_RET();
+}
+
 // ------------------------------------------------------------------ horizontal_diffusion
 __global__ void k_hdiff(Box b, Dims D, HDiff a) {
   int k, j, i;

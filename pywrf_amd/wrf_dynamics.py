@@ -396,6 +396,22 @@ def curvature(ru, rv, rw, u, v, w, ru_tend, rv_tend, rw_tend,
     return ru_tend, rv_tend, rw_tend
 
 
+def coriolis_curvature(ru, rv, rw, u, v, w, ru_tend, rv_tend, rw_tend,
+                       msfux, msfuy, msfvx, msfvy, msftx, msfty,
+                       f, e, sina, cosa, xlat, fzm, fzp, rdx, rdy,
+                       ids, ide, jds, jde, kds, kde,
+                       ims, ime, jms, jme, kms, kme,
+                       its, ite, jts, jte, kts, kte):
+    """coriolis (wrf_dynamics.py:2463-2532) then curvature (:2535-2620) in one
+    launch; the tendencies are updated in place.  Not part of the reference
+    API."""
+    g = grid(ru.shape, (ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, its, ite, jts, jte, kts, kte))
+    call("pywrf_coriolis_curvature", g, ptr(ru), ptr(rv), ptr(rw), ptr(u), ptr(v), ptr(ru_tend), ptr(rv_tend),
+         ptr(rw_tend), ptr(msftx), ptr(msfty), ptr(msfux), ptr(msfuy), ptr(msfvx), ptr(msfvy), ptr(f), ptr(e),
+         ptr(sina), ptr(cosa), ptr(fzm), ptr(fzp), float(rdx), float(rdy), stream())
+    return ru_tend, rv_tend, rw_tend
+
+
 def horizontal_diffusion(name, field, tendency, mut, c1, c2,
                          msfux, msfuy, msfvx, msfvx_inv,
                          msfvy, msftx, msfty,

@@ -79,3 +79,29 @@ def test_flow_dep_group_equals_individual_calls(n):
     bdy.flow_dep_bdy_group([q[im] for im in range(2, 7)], u, v, g.ix24)
     torch.cuda.synchronize()
     assert torch.equal(ref, q)
+
+
+def test_coriolis_curvature_equals_the_two_calls():
+    """The fused entry point against coriolis followed by curvature, on the
+    state after one RK3 step of the 40x40 moist workload (bit for bit)."""
+    import pywrf_amd
+    from pywrf_amd.step import RunConfig
+    from parity import state, run
+    cfg = RunConfig()
+    o = pywrf_amd.ops()
+    s = state("config1c", device="cuda")
+    run(s, o, 1)
+    I18 = s.grid.ix18
+    torch.manual_seed(4)
+    t0 = [torch.randn_like(s.ru_tend) for _ in range(3)]
+    ref = [t.clone() for t in t0]
+    o.coriolis(s.ru, s.rv, s.rw, *ref, s.msftx, s.msfty, s.msfux, s.msfuy, s.msfvx, s.msfvy, s.f, s.e, s.sina,
+               s.cosa, s.fnm, s.fnp, *I18)
+    o.curvature(s.ru, s.rv, s.rw, s.u, s.v, s.w, *ref, s.msfux, s.msfuy, s.msfvx, s.msfvy, s.msftx, s.msfty, s.clat,
+                s.fnm, s.fnp, cfg.rdx, cfg.rdy, *I18)
+    got = [t.clone() for t in t0]
+    o.coriolis_curvature(s.ru, s.rv, s.rw, s.u, s.v, s.w, *got, s.msfux, s.msfuy, s.msfvx, s.msfvy, s.msftx, s.msfty,
+                         s.f, s.e, s.sina, s.cosa, s.clat, s.fnm, s.fnp, cfg.rdx, cfg.rdy, *I18)
+    torch.cuda.synchronize()
+    for a, b in zip(got, ref):
+        assert torch.equal(a, b)
