@@ -267,16 +267,77 @@ struct BcSpec {
   int ids, ide, jds, jde;
 };
 
-// One plane's open-boundary fill by one workgroup.  In each phase the reads
-// (edge cells) and writes (halo cells) are disjoint, so every thread first
-// loads all of its items and then stores them: the loads of a phase are in
-// flight together instead of one round trip per item.
+// One plane's open-boundary fill by one workgroup of BC_THREADS threads.  The
+// reference copies the x-halos and then the y-halos (whose sources, the edge
+// rows, include cells the x-copies wrote).  Here every value is first loaded
+// -- a y-copy reads the cell the x-copies take its value from -- and, after
+// one barrier, stored; an x-copy whose target a y-copy overwrites later is not
+// stored.  The result equals the two passes in order, with one round of loads.
+// Planes with more copies than BC_THREADS * BC_ITEMS take the two passes.
+#define BC_THREADS 512
 #define BC_ITEMS 8
 __device__ inline void open_bc_plane(float* a, int nx, const BcSpec& c) {
   const int r0 = c.ustag ? c.ur0 : c.xr0, r1 = c.ustag ? c.ur1 : c.xr1;
   const int nstart = c.xr1 - c.xr0, nend = r1 - r0;
-  const int nxi = 3 * (nstart + nend), chunk = BC_ITEMS * (int)blockDim.x;
-  // x-direction: 3 start-halo cells from column ids (rows xr), 3 end cells per row
+  const int nxi = 3 * (nstart + nend), nyi = 6 * (c.yc1 - c.yc0);
+  const int yl0 = c.jds - 3, yh0 = c.vstag ? c.jde : c.jde - 1;   // y-copy target rows [yl0, jds), [yh0, yh0 + 3)
+  auto xitem = [&](int t, int& src, int& dst) {
+    const int h = t % 3, r = t / 3;
+    if (r < nstart) {
+      const int row = c.xr0 + r;
+      src = row * nx + c.ids;
+      dst = row * nx + c.ids - 1 - h;
+    } else {
+      const int row = r0 + (r - nstart);
+      src = c.ustag ? row * nx + c.ide - 1 : row * nx + c.ide - 2;
+      dst = c.ustag ? row * nx + c.ide + h : row * nx + c.ide - 1 + h;
+    }
+  };
+  auto yitem = [&](int t, int& srow, int& col, int& dst) {
+    const int h = t % 3, side = (t / 3) & 1;
+    col = c.yc0 + t / 6;
+    if (side == 0) { srow = c.jds; dst = (c.jds - 1 - h) * nx + col; }
+    else if (c.vstag) { srow = c.jde - 1; dst = (c.jde + h) * nx + col; }
+    else { srow = c.jde - 2; dst = (c.jde - 1 + h) * nx + col; }
+  };
+  const int chunk = BC_ITEMS * (int)blockDim.x;
+  if (nxi + nyi <= chunk) {
+    float v[BC_ITEMS];
+    int dst[BC_ITEMS];
+#pragma unroll
+    for (int u = 0; u < BC_ITEMS; ++u) {
+      const int t = u * (int)blockDim.x + (int)threadIdx.x;
+      dst[u] = -1;
+      if (t < nxi) {
+        int src, d;
+        xitem(t, src, d);
+        const int row = d / nx, col = d - row * nx;
+        const bool ylater = ((row >= yl0 && row < c.jds) || (row >= yh0 && row < yh0 + 3)) && col >= c.yc0 &&
+                            col < c.yc1;
+        if (!ylater) {
+          v[u] = a[src];
+          dst[u] = d;
+        }
+      } else if (t < nxi + nyi) {
+        int srow, col, d;
+        yitem(t - nxi, srow, col, d);
+        // the value of (srow, col) once the x-copies are done
+        int src = srow * nx + col;
+        if (srow >= c.xr0 && srow < c.xr1 && col >= c.ids - 3 && col <= c.ids - 1) src = srow * nx + c.ids;
+        else if (srow >= r0 && srow < r1 && c.ustag && col >= c.ide && col <= c.ide + 2) src = srow * nx + c.ide - 1;
+        else if (srow >= r0 && srow < r1 && !c.ustag && col >= c.ide - 1 && col <= c.ide + 1)
+          src = srow * nx + c.ide - 2;
+        v[u] = a[src];
+        dst[u] = d;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < BC_ITEMS; ++u)
+      if (dst[u] >= 0) a[dst[u]] = v[u];
+    return;
+  }
+  // two passes: x-copies, then y-copies
   for (int base = 0; base < nxi; base += chunk) {
     float v[BC_ITEMS];
     int dst[BC_ITEMS];
@@ -285,17 +346,8 @@ __device__ inline void open_bc_plane(float* a, int nx, const BcSpec& c) {
       const int t = base + u * (int)blockDim.x + (int)threadIdx.x;
       dst[u] = -1;
       if (t < nxi) {
-        const int h = t % 3, r = t / 3;
         int src;
-        if (r < nstart) {
-          const int row = c.xr0 + r;
-          src = row * nx + c.ids;
-          dst[u] = row * nx + c.ids - 1 - h;
-        } else {
-          const int row = r0 + (r - nstart);
-          src = c.ustag ? row * nx + c.ide - 1 : row * nx + c.ide - 2;
-          dst[u] = c.ustag ? row * nx + c.ide + h : row * nx + c.ide - 1 + h;
-        }
+        xitem(t, src, dst[u]);
         v[u] = a[src];
       }
     }
@@ -304,8 +356,6 @@ __device__ inline void open_bc_plane(float* a, int nx, const BcSpec& c) {
       if (dst[u] >= 0) a[dst[u]] = v[u];
   }
   __syncthreads();
-  // y-direction (reads rows the x-copies may have written)
-  const int nyi = 6 * (c.yc1 - c.yc0);
   for (int base = 0; base < nyi; base += chunk) {
     float v[BC_ITEMS];
     int dst[BC_ITEMS];
@@ -314,19 +364,9 @@ __device__ inline void open_bc_plane(float* a, int nx, const BcSpec& c) {
       const int t = base + u * (int)blockDim.x + (int)threadIdx.x;
       dst[u] = -1;
       if (t < nyi) {
-        const int h = t % 3, side = (t / 3) & 1, col = c.yc0 + t / 6;
-        int src;
-        if (side == 0) {
-          src = c.jds * nx + col;
-          dst[u] = (c.jds - 1 - h) * nx + col;
-        } else if (c.vstag) {
-          src = (c.jde - 1) * nx + col;
-          dst[u] = (c.jde + h) * nx + col;
-        } else {
-          src = (c.jde - 2) * nx + col;
-          dst[u] = (c.jde - 1 + h) * nx + col;
-        }
-        v[u] = a[src];
+        int srow, col;
+        yitem(t, srow, col, dst[u]);
+        v[u] = a[srow * nx + col];
       }
     }
 #pragma unroll
@@ -346,7 +386,7 @@ static int bc_launch(const pywrf_grid_t* g, float* dat, int k0, int k1, const Bc
   if (c.yc0 < 0 || c.yc1 > g->nx || c.jds - 3 < 0 || c.jde + 3 > g->ny) return PYWRF_E_RANGE;
   if (k0 < 0 || k1 > g->nz) return PYWRF_E_RANGE;
   if (k1 <= k0) return 0;
-  hipLaunchKernelGGL(k_open_bc, dim3(k1 - k0), dim3(PW_BLOCK), 0, (hipStream_t)s, dims_of(g), dat, k0, c);
+  hipLaunchKernelGGL(k_open_bc, dim3(k1 - k0), dim3(BC_THREADS), 0, (hipStream_t)s, dims_of(g), dat, k0, c);
   PW_RET();
 }
 
@@ -474,7 +514,7 @@ extern "C" int pywrf_set_physical_bc_group(const pywrf_grid_t* g3, const pywrf_g
     maxk = imax(maxk, jb.nk);
   }
   if (n == 0) return 0;
-  hipLaunchKernelGGL(k_open_bc_group, dim3(maxk, n), dim3(PW_BLOCK), 0, (hipStream_t)s, (long)g3->ny * g3->nx,
+  hipLaunchKernelGGL(k_open_bc_group, dim3(maxk, n), dim3(BC_THREADS), 0, (hipStream_t)s, (long)g3->ny * g3->nx,
                      g3->nx, J);
   PW_RET();
 }
