@@ -318,6 +318,14 @@ int pywrf_relax_bdytend(const pywrf_grid_t* g, const float* field, float* field_
 /* spec_bdytend, wrf_boundary.py:188-225 */
 int pywrf_spec_bdytend(const pywrf_grid_t* g, float* field_tend, const float* btxs, const float* btxe,
                        const float* btys, const float* btye, char variable_in, pywrf_stream_t s);
+
+/* n consecutive spec_bdytend calls on distinct tendencies (solver.py:
+ * 1220-1254) in one launch: tends[f], variables[f] as in pywrf_spec_bdytend,
+ * bdy_tends[4f .. 4f+3] its btxs, btxe, btys, btye, is2d[f] selects the index
+ * set g2 (a one-level field) over g3.  No counterpart in the reference: a
+ * batching. */
+int pywrf_spec_bdytend_group(const pywrf_grid_t* g3, const pywrf_grid_t* g2, int n, void* const* tends,
+                             void* const* bdy_tends, const char* variables, const int* is2d, pywrf_stream_t s);
 /* spec_bdyupdate, wrf_boundary.py:228-284 (kds == kde == 0: 2-D field) */
 int pywrf_spec_bdyupdate(const pywrf_grid_t* g, float* field, const float* field_tend, double dt, char variable,
                          pywrf_stream_t s);
@@ -359,6 +367,14 @@ int pywrf_spec_bdy_final(const pywrf_grid_t* g, float* field, const float* mu, c
                          const float* bxe, const float* bys, const float* bye, const float* btxs,
                          const float* btxe, const float* btys, const float* btye, char variable_in, double dtbc,
                          pywrf_stream_t s);
+
+/* n consecutive spec_bdy_final calls on distinct fields (solver.py:1863-1903)
+ * in one launch: fields[f], mus[f], msfs[f], variables[f] as in
+ * pywrf_spec_bdy_final, bdys[8f .. 8f+7] its bxs, bxe, bys, bye, btxs, btxe,
+ * btys, btye; one dtbc.  No counterpart in the reference: a batching. */
+int pywrf_spec_bdy_final_group(const pywrf_grid_t* g, int n, void* const* fields, void* const* mus,
+                               void* const* msfs, void* const* bdys, const char* variables, double dtbc,
+                               pywrf_stream_t s);
 
 /* ---- moist physics: wrf_physics.py:27-2137 ------------------------------ */
 /* moist_physics_prep_em, wrf_physics.py:27-91 */

@@ -126,28 +126,96 @@ extern "C" int pywrf_relax_bdy_scalar(const pywrf_grid_t* g, float* scalar_tend,
 }
 
 // ------------------------------------------------------------------ spec_bdytend
-__global__ void k_spec_bdytend(Box all, int w, Dims D, float* tend, Bdy b, int ibs, int ibe, int jbs, int jbe) {
-  int k, j, i;
-  if (!edge_point(all, w, k, j, i)) return;
+struct TendJob {
+  Box all;
+  Dims D;
+  float* tend;
+  Bdy b;
+  int ibs, ibe, jbs, jbe;
+};
+
+__device__ inline void spec_tend_point(const TendJob& a, int k, int j, int i) {
+  const Dims& D = a.D;
   float v;
-  bool xr = j >= jbs && j < jbe;
-  if (xr && i == ibe - 1) v = bx(D, b.txe, k, j, 0);
-  else if (xr && i == ibs) v = bx(D, b.txs, k, j, 0);
-  else if (j == jbe - 1) v = by(D, b.tye, k, 0, i);
-  else if (j == jbs) v = by(D, b.tys, k, 0, i);
+  bool xr = j >= a.jbs && j < a.jbe;
+  if (xr && i == a.ibe - 1) v = bx(D, a.b.txe, k, j, 0);
+  else if (xr && i == a.ibs) v = bx(D, a.b.txs, k, j, 0);
+  else if (j == a.jbe - 1) v = by(D, a.b.tye, k, 0, i);
+  else if (j == a.jbs) v = by(D, a.b.tys, k, 0, i);
   else return;
-  AT(tend, k, j, i) = v;
+  AT(a.tend, k, j, i) = v;
 }
 
-extern "C" int pywrf_spec_bdytend(const pywrf_grid_t* g, float* field_tend, const float* btxs, const float* btxe,
-                                  const float* btys, const float* btye, char variable_in, pywrf_stream_t s) {
+__global__ void k_spec_bdytend(Box all, int w, TendJob a) {
+  int k, j, i;
+  if (!edge_point(all, w, k, j, i)) return;
+  spec_tend_point(a, k, j, i);
+}
+
+static int spec_tend_job(const pywrf_grid_t* g, float* field_tend, const float* btxs, const float* btxe,
+                         const float* btys, const float* btye, char variable_in, TendJob& a) {
   if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   int ibe, jbe, ktf;
   bounds_of(variable_in, g, ibe, jbe, ktf);
   if (g->ids < 0 || ibe > g->nx || g->jds < 0 || jbe > g->ny || ktf > g->nz) return PYWRF_E_RANGE;
-  Bdy b = {nullptr, nullptr, nullptr, nullptr, btxs, btxe, btys, btye};
-  Box all = mkbox(g->kts, ktf, g->jds, jbe, g->ids, ibe);
-  pw_launch_edges(k_spec_bdytend, all, 1, s, dims_of(g), field_tend, b, g->ids, ibe, g->jds, jbe);
+  a.all = mkbox(g->kts, ktf, g->jds, jbe, g->ids, ibe);
+  a.D = dims_of(g);
+  a.tend = field_tend;
+  a.b = {nullptr, nullptr, nullptr, nullptr, btxs, btxe, btys, btye};
+  a.ibs = g->ids; a.ibe = ibe; a.jbs = g->jds; a.jbe = jbe;
+  return 0;
+}
+
+extern "C" int pywrf_spec_bdytend(const pywrf_grid_t* g, float* field_tend, const float* btxs, const float* btxe,
+                                  const float* btys, const float* btye, char variable_in, pywrf_stream_t s) {
+  TendJob a;
+  int e = spec_tend_job(g, field_tend, btxs, btxe, btys, btye, variable_in, a);
+  if (e) return e;
+  pw_launch_edges(k_spec_bdytend, a.all, 1, s, a);
+  PW_RET();
+}
+
+// Several spec_bdytend calls on distinct tendencies in one launch
+// (blockIdx.y selects the job).
+#define PW_TEND_GROUP_MAX 8
+struct TendJobs {
+  TendJob j[PW_TEND_GROUP_MAX];
+};
+
+__global__ void k_spec_bdytend_group(TendJobs J) {
+  const TendJob& a = J.j[blockIdx.y];
+  if ((int)blockIdx.z >= a.all.nk()) return;
+  int k, j, i;
+  if (!edge_point(a.all, 1, k, j, i)) return;
+  spec_tend_point(a, k, j, i);
+}
+
+extern "C" int pywrf_spec_bdytend_group(const pywrf_grid_t* g3, const pywrf_grid_t* g2, int n, void* const* tends,
+                                        void* const* bdy_tends, const char* variables, const int* is2d,
+                                        pywrf_stream_t s) {
+  if (!pw_size_ok(g3) || !pw_size_ok(g2)) return PYWRF_E_RANGE;
+  if (n < 0 || n > PW_TEND_GROUP_MAX) return PYWRF_E_ARG;
+  TendJobs J;
+  int m = 0, maxk = 0;
+  long most = 0;
+  for (int f = 0; f < n; ++f) {
+    TendJob a;
+    int e = spec_tend_job(is2d[f] ? g2 : g3, (float*)tends[f], (const float*)bdy_tends[4 * f],
+                          (const float*)bdy_tends[4 * f + 1], (const float*)bdy_tends[4 * f + 2],
+                          (const float*)bdy_tends[4 * f + 3], variables[f], a);
+    if (e) return e;
+    if (a.all.size() <= 0) continue;
+    if (a.all.nj() < 2 || a.all.ni() < 2) {
+      pw_launch_edges(k_spec_bdytend, a.all, 1, s, a);
+      continue;
+    }
+    J.j[m++] = a;
+    most = edge_count(a.all, 1) > most ? edge_count(a.all, 1) : most;
+    maxk = a.all.nk() > maxk ? a.all.nk() : maxk;
+  }
+  if (m > 0)
+    hipLaunchKernelGGL(k_spec_bdytend_group, dim3((unsigned)((most + PW_EDGE_BLOCK - 1) / PW_EDGE_BLOCK), (unsigned)m,
+                       (unsigned)maxk), dim3(PW_EDGE_BLOCK), 0, (hipStream_t)s, J);
   PW_RET();
 }
 
@@ -417,26 +485,41 @@ extern "C" int pywrf_flow_dep_bdy_group(const pywrf_grid_t* g, int n, void* cons
 }
 
 // ------------------------------------------------------------------ spec_bdy_final
-__global__ void k_spec_final(Box all, int w, Dims D, float* f, const float* mu, const float* msf, Bdy b, float dtbc,
-                             int jts, int jtf, int its, int itf, int ibe, int jbe) {
-  int k, j, i;
-  if (!edge_point(all, w, k, j, i)) return;
-  bool xr = j >= jts && j < jbe, ir = i >= its && i < ibe;
+struct FinalJob {
+  Box all;
+  int w;
+  Dims D;
+  float* f;
+  const float *mu, *msf;
+  Bdy b;
+  float dtbc;
+  int jts, jtf, its, itf, ibe, jbe;
+};
+
+__device__ inline void spec_final_point(const FinalJob& a, int k, int j, int i) {
+  const Dims& D = a.D;
+  bool xr = j >= a.jts && j < a.jbe, ir = i >= a.its && i < a.ibe;
   float bv, bt;
-  if (xr && i == itf - 1) { bv = bx(D, b.xe, k, j, 0); bt = bx(D, b.txe, k, j, 0); }
-  else if (xr && i == its) { bv = bx(D, b.xs, k, j, 0); bt = bx(D, b.txs, k, j, 0); }
-  else if (ir && j == jtf - 1) { bv = by(D, b.ye, k, 0, i); bt = by(D, b.tye, k, 0, i); }
-  else if (ir && j == jts) { bv = by(D, b.ys, k, 0, i); bt = by(D, b.tys, k, 0, i); }
+  if (xr && i == a.itf - 1) { bv = bx(D, a.b.xe, k, j, 0); bt = bx(D, a.b.txe, k, j, 0); }
+  else if (xr && i == a.its) { bv = bx(D, a.b.xs, k, j, 0); bt = bx(D, a.b.txs, k, j, 0); }
+  else if (ir && j == a.jtf - 1) { bv = by(D, a.b.ye, k, 0, i); bt = by(D, a.b.tye, k, 0, i); }
+  else if (ir && j == a.jts) { bv = by(D, a.b.ys, k, 0, i); bt = by(D, a.b.tys, k, 0, i); }
   else return;
-  float sm = msf ? AT2(msf, j, i) : 1.0f;
-  float su = mu ? AT2(mu, j, i) : 1.0f;
-  AT(f, k, j, i) = sm * (bv + dtbc * bt) / su;
+  float sm = a.msf ? AT2(a.msf, j, i) : 1.0f;
+  float su = a.mu ? AT2(a.mu, j, i) : 1.0f;
+  AT(a.f, k, j, i) = sm * (bv + a.dtbc * bt) / su;
 }
 
-extern "C" int pywrf_spec_bdy_final(const pywrf_grid_t* g, float* field, const float* mu, const float* msf,
-                                    const float* bxs, const float* bxe, const float* bys, const float* bye,
-                                    const float* btxs, const float* btxe, const float* btys, const float* btye,
-                                    char variable_in, double dtbc, pywrf_stream_t s) {
+__global__ void k_spec_final(Box all, int w, FinalJob a) {
+  int k, j, i;
+  if (!edge_point(all, w, k, j, i)) return;
+  spec_final_point(a, k, j, i);
+}
+
+static int spec_final_job(const pywrf_grid_t* g, float* field, const float* mu, const float* msf, const float* bxs,
+                          const float* bxe, const float* bys, const float* bye, const float* btxs,
+                          const float* btxe, const float* btys, const float* btye, char variable_in, double dtbc,
+                          FinalJob& a) {
   if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   char v = variable_in;
   switch (v) {
@@ -454,12 +537,68 @@ extern "C" int pywrf_spec_bdy_final(const pywrf_grid_t* g, float* field, const f
   if (v == 'v') { jbe = g->jde; jtf = imin(g->jte, g->jde); }
   if (v == 'm' || v == 'h' || v == 'w') ktf = g->kde;
   if (ibe > g->nx || jbe > g->ny || jtf > g->ny || ktf > g->nz || g->its < 0 || g->jts < 0) return PYWRF_E_RANGE;
-  const float* xmsf = (v == 'u' || v == 'v' || v == 'w') ? msf : nullptr;
-  const float* xmu = (v != 'm') ? mu : nullptr;
-  Bdy b = {bxs, bxe, bys, bye, btxs, btxe, btys, btye};
-  Box all = mkbox(g->kts, ktf, g->jts, imax(jtf, jbe), g->its, imax(ibe, itf));
-  int w = (itf >= ibe && jtf >= jbe) ? 1 : 0;  // rows jts, jtf-1 and columns its, itf-1 are the box's sides
-  pw_launch_edges(k_spec_final, all, w, s, dims_of(g), field, xmu, xmsf, b, F(dtbc), g->jts, jtf, g->its, itf, ibe,
-                  jbe);
+  a.msf = (v == 'u' || v == 'v' || v == 'w') ? msf : nullptr;
+  a.mu = (v != 'm') ? mu : nullptr;
+  a.b = {bxs, bxe, bys, bye, btxs, btxe, btys, btye};
+  a.all = mkbox(g->kts, ktf, g->jts, imax(jtf, jbe), g->its, imax(ibe, itf));
+  a.w = (itf >= ibe && jtf >= jbe) ? 1 : 0;  // rows jts, jtf-1 and columns its, itf-1 are the box's sides
+  a.D = dims_of(g);
+  a.f = field; a.dtbc = F(dtbc);
+  a.jts = g->jts; a.jtf = jtf; a.its = g->its; a.itf = itf; a.ibe = ibe; a.jbe = jbe;
+  return 0;
+}
+
+extern "C" int pywrf_spec_bdy_final(const pywrf_grid_t* g, float* field, const float* mu, const float* msf,
+                                    const float* bxs, const float* bxe, const float* bys, const float* bye,
+                                    const float* btxs, const float* btxe, const float* btys, const float* btye,
+                                    char variable_in, double dtbc, pywrf_stream_t s) {
+  FinalJob a;
+  int e = spec_final_job(g, field, mu, msf, bxs, bxe, bys, bye, btxs, btxe, btys, btye, variable_in, dtbc, a);
+  if (e) return e;
+  pw_launch_edges(k_spec_final, a.all, a.w, s, a);
+  PW_RET();
+}
+
+// Several spec_bdy_final calls on distinct fields in one launch (blockIdx.y
+// selects the job); a job whose rows are not its box's sides runs on its own.
+#define PW_FINAL_GROUP_MAX 8
+struct FinalJobs {
+  FinalJob j[PW_FINAL_GROUP_MAX];
+};
+
+__global__ void k_spec_final_group(FinalJobs J) {
+  const FinalJob& a = J.j[blockIdx.y];
+  if ((int)blockIdx.z >= a.all.nk()) return;
+  int k, j, i;
+  if (!edge_point(a.all, 1, k, j, i)) return;
+  spec_final_point(a, k, j, i);
+}
+
+extern "C" int pywrf_spec_bdy_final_group(const pywrf_grid_t* g, int n, void* const* fields, void* const* mus,
+                                          void* const* msfs, void* const* bdys, const char* variables, double dtbc,
+                                          pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
+  if (n < 0 || n > PW_FINAL_GROUP_MAX) return PYWRF_E_ARG;
+  FinalJobs J;
+  int m = 0, maxk = 0;
+  long most = 0;
+  for (int f = 0; f < n; ++f) {
+    FinalJob a;
+    const float* const* bd = (const float* const*)bdys + 8 * f;
+    int e = spec_final_job(g, (float*)fields[f], (const float*)mus[f], (const float*)msfs[f], bd[0], bd[1], bd[2],
+                           bd[3], bd[4], bd[5], bd[6], bd[7], variables[f], dtbc, a);
+    if (e) return e;
+    if (a.all.size() <= 0) continue;
+    if (a.w != 1 || a.all.nj() < 2 || a.all.ni() < 2) {
+      pw_launch_edges(k_spec_final, a.all, a.w, s, a);
+      continue;
+    }
+    J.j[m++] = a;
+    most = edge_count(a.all, 1) > most ? edge_count(a.all, 1) : most;
+    maxk = a.all.nk() > maxk ? a.all.nk() : maxk;
+  }
+  if (m > 0)
+    hipLaunchKernelGGL(k_spec_final_group, dim3((unsigned)((most + PW_EDGE_BLOCK - 1) / PW_EDGE_BLOCK), (unsigned)m,
+                       (unsigned)maxk), dim3(PW_EDGE_BLOCK), 0, (hipStream_t)s, J);
   PW_RET();
 }

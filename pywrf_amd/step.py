@@ -256,13 +256,18 @@ def rk3_step(s, ops, cfg, check_nan=False):
             s.t_tendf, s.u_save, s.v_save, s.w_save, s.ph_save, s.t_save, s.mu_tend, s.mu_tendf, rk_step, s.c1h,
             s.c2h, s.h_diabatic, s.mut, s.msftx, s.msfty, s.msfux, s.msfuy, s.msfvx, s.msfvx_inv, s.msfvy, *I24)
         # spec_bdy_dry (solver.py:1220-1255)
-        sb = lambda tend, key, var, ix=I24: o.spec_bdytend(tend, *b[key], var, SPEC_BDY_WIDTH, SPEC_ZONE, *ix)
-        s.ru_tend = sb(s.ru_tend, 'u', 'u')
-        s.rv_tend = sb(s.rv_tend, 'v', 'v')
-        s.ph_tend = sb(s.ph_tend, 'ph', 'h')
-        s.t_tend = sb(s.t_tend, 't', 't')
         ix_mu = (ids, ide, jds, jde, 0, 1, ims, ime, jms, jme, 0, 1, ips, ipe, jps, jpe, 0, 1, its, ite, jts, jte, 0, 1)
-        mu_tend_e = sb(mu_tend_e, 'mu', 'm', ix_mu)
+        if hasattr(o, "spec_bdytend_group"):
+            o.spec_bdytend_group([(s.ru_tend, b['u'][4:], 'u', False), (s.rv_tend, b['v'][4:], 'v', False),
+                                  (s.ph_tend, b['ph'][4:], 'h', False), (s.t_tend, b['t'][4:], 't', False),
+                                  (mu_tend_e, b['mu'][4:], 'm', True)], I24, ix_mu)
+        else:
+            sb = lambda tend, key, var, ix=I24: o.spec_bdytend(tend, *b[key], var, SPEC_BDY_WIDTH, SPEC_ZONE, *ix)
+            s.ru_tend = sb(s.ru_tend, 'u', 'u')
+            s.rv_tend = sb(s.rv_tend, 'v', 'v')
+            s.ph_tend = sb(s.ph_tend, 'ph', 'h')
+            s.t_tend = sb(s.t_tend, 't', 't')
+            mu_tend_e = sb(mu_tend_e, 'mu', 'm', ix_mu)
         s.mu_tend = mu_tend_e[0, :, :] + 0.
         # small_step_prep, calc_p_rho, calc_coef_w (solver.py:1258-1294)
         (s.u_1, s.v_1, s.w_1, s.t_1, s.ph_1, s.u_save, s.v_save, s.w_save, s.t_save, s.ph_save, s.u, s.v, s.w,
@@ -403,13 +408,18 @@ def rk3_step(s, ops, cfg, check_nan=False):
         *[(s.moist[im, :, :, :], 'p', 0) for im in range(1, 7)],
         *[(s.scalar[im, :, :, :], 'p', 0) for im in range(1, 3)])[:14]
     # spec_bdy_final and surface w (solver.py:1863-1910)
-    sf = lambda fld, mu, msf, key, var: o.spec_bdy_final(fld, mu, s.c1h, s.c2h, msf, *b[key], var, SPEC_BDY_WIDTH,
-                                                          SPEC_ZONE, dtbc, *I24)
-    s.u = sf(s.u, s.muus, s.msfuy, 'u', 'u')
-    s.v = sf(s.v, s.muvs, s.msfvx, 'v', 'v')
-    s.t = sf(s.t, s.muts, s.msfty, 't', 't')
-    s.ph = sf(s.ph, s.muts, s.msfty, 'ph', 'h')
-    s.moist[P_QV, :, :, :] = sf(s.moist[P_QV, :, :, :], s.muts, s.msfty, 'qv', 't')
+    if hasattr(o, "spec_bdy_final_group"):
+        o.spec_bdy_final_group([(s.u, s.muus, s.msfuy, b['u'], 'u'), (s.v, s.muvs, s.msfvx, b['v'], 'v'),
+                                (s.t, s.muts, s.msfty, b['t'], 't'), (s.ph, s.muts, s.msfty, b['ph'], 'h'),
+                                (s.moist[P_QV, :, :, :], s.muts, s.msfty, b['qv'], 't')], dtbc, I24)
+    else:
+        sf = lambda fld, mu, msf, key, var: o.spec_bdy_final(fld, mu, s.c1h, s.c2h, msf, *b[key], var,
+                                                              SPEC_BDY_WIDTH, SPEC_ZONE, dtbc, *I24)
+        s.u = sf(s.u, s.muus, s.msfuy, 'u', 'u')
+        s.v = sf(s.v, s.muvs, s.msfvx, 'v', 'v')
+        s.t = sf(s.t, s.muts, s.msfty, 't', 't')
+        s.ph = sf(s.ph, s.muts, s.msfty, 'ph', 'h')
+        s.moist[P_QV, :, :, :] = sf(s.moist[P_QV, :, :, :], s.muts, s.msfty, 'qv', 't')
     s.w = o.set_w_surface(s.znw, False, s.w, s.ht, s.u, s.v, s.cf1, s.cf2, s.cf3, rdx, rdy, s.msftx, s.msfty,
                           *I18)
     return s

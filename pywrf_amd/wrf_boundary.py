@@ -221,3 +221,48 @@ def flow_dep_bdy_group(fields, u, v, idx24):
     g = grid(fields[0].shape, _I24(*idx24))
     arr = (ctypes.c_void_p * n)(*[ptr(f).value for f in fields])
     call("pywrf_flow_dep_bdy_group", g, n, arr, ptr(u), ptr(v), stream())
+
+
+def spec_bdytend_group(items, idx24, idx2d):
+    """A run of spec_bdytend calls on distinct tendencies in one launch.
+    items: [(field_tend, (btxs, btxe, btys, btye), variable, is_2d)]; idx24 /
+    idx2d: the index tuples the individual calls would take (is_2d selects
+    idx2d).  Same result as the calls in order.  Not part of the reference
+    API."""
+    import ctypes
+    n = len(items)
+    if n == 0:
+        return
+    if len({t.data_ptr() for t, _, _, _ in items}) != n:
+        raise ValueError("spec_bdytend_group: a tendency appears twice")
+    t3 = next((t for t, _, _, two in items if not two), None)
+    t2 = next((t for t, _, _, two in items if two), None)
+    shape3 = t3.shape if t3 is not None else (idx24[5],) + tuple(items[0][0].shape[-2:])
+    shape2 = t2.shape if t2 is not None else (1,) + tuple(items[0][0].shape[-2:])
+    g3, g2 = grid(shape3, _I24(*idx24)), grid(shape2, _I24(*idx2d))
+    tends = (ctypes.c_void_p * n)(*[ptr(t).value for t, _, _, _ in items])
+    bt = (ctypes.c_void_p * (4 * n))(*[ptr(a).value for _, arrs, _, _ in items for a in arrs])
+    flags = b"".join(v.encode() for _, _, v, _ in items)
+    kinds = (ctypes.c_int * n)(*[1 if two else 0 for _, _, _, two in items])
+    call("pywrf_spec_bdytend_group", g3, g2, n, tends, bt, flags, kinds, stream())
+
+
+def spec_bdy_final_group(items, dtbc, idx24):
+    """A run of spec_bdy_final calls on distinct fields in one launch.
+    items: [(field, mu, msf, (bxs, bxe, bys, bye, btxs, btxe, btys, btye),
+    variable)].  Same result as the calls in order.  Not part of the reference
+    API."""
+    import ctypes
+    n = len(items)
+    if n == 0:
+        return
+    if len({f.data_ptr() for f, _, _, _, _ in items}) != n:
+        raise ValueError("spec_bdy_final_group: a field appears twice")
+    g = grid(items[0][0].shape, _I24(*idx24))
+    vp = lambda xs: (ctypes.c_void_p * len(xs))(*[ptr(x).value for x in xs])
+    fields = vp([f for f, _, _, _, _ in items])
+    mus = vp([m for _, m, _, _, _ in items])
+    msfs = vp([m for _, _, m, _, _ in items])
+    bdys = vp([a for _, _, _, arrs, _ in items for a in arrs])
+    flags = b"".join(v.encode() for _, _, _, _, v in items)
+    call("pywrf_spec_bdy_final_group", g, n, fields, mus, msfs, bdys, flags, float(dtbc), stream())

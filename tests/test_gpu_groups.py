@@ -1,5 +1,5 @@
-"""The batched entry points (pywrf_zero_group, pywrf_set_physical_bc_group)
-give exactly what the individual calls they replace give."""
+"""The batched entry points (pywrf_*_group, pywrf_coriolis_curvature) give
+exactly what the individual calls they replace give."""
 import pytest
 import torch
 
@@ -105,3 +105,53 @@ def test_coriolis_curvature_equals_the_two_calls():
     torch.cuda.synchronize()
     for a, b in zip(got, ref):
         assert torch.equal(a, b)
+
+
+def _bdy_arrays(kz, n, gen):
+    return tuple(torch.randn((kz, n, 5) if m % 4 < 2 else (kz, 5, n), generator=gen).cuda() for m in range(8))
+
+
+@pytest.mark.parametrize("n", [40, 120])
+def test_spec_bdytend_group_equals_individual_calls(n):
+    import pywrf_amd.wrf_boundary as bdy
+    from pywrf_amd.step import Grid
+    g = Grid(41, n, n)
+    (ids, ide, jds, jde, _, _, ims, ime, jms, jme, _, _, ips, ipe, jps, jpe, _, _, its, ite, jts, jte, _, _) = g.ix24
+    ix_mu = (ids, ide, jds, jde, 0, 1, ims, ime, jms, jme, 0, 1, ips, ipe, jps, jpe, 0, 1, its, ite, jts, jte, 0, 1)
+    gen = torch.Generator().manual_seed(5)
+    items = [(torch.randn(41, n, n, generator=gen).cuda(), _bdy_arrays(41, n, gen), v, False) for v in "uvht"]
+    items.append((torch.randn(1, n, n, generator=gen).cuda(), _bdy_arrays(1, n, gen), "m", True))
+    ref = [t.clone() for t, _, _, _ in items]
+    for r, (_, arrs, v, two) in zip(ref, items):
+        bdy.spec_bdytend(r, *arrs, v, 5, 1, *(ix_mu if two else g.ix24))
+    bdy.spec_bdytend_group([(t, arrs[4:], v, two) for t, arrs, v, two in items], g.ix24, ix_mu)
+    torch.cuda.synchronize()
+    for r, (t, _, v, two) in zip(ref, items):
+        assert torch.equal(r, t), (v, two)
+
+
+@pytest.mark.parametrize("n", [40, 120])
+def test_spec_bdy_final_group_equals_individual_calls(n):
+    import pywrf_amd.wrf_boundary as bdy
+    from pywrf_amd.step import Grid
+    g = Grid(41, n, n)
+    gen = torch.Generator().manual_seed(6)
+    c1 = torch.rand(41, generator=gen).cuda()
+    c2 = torch.rand(41, generator=gen).cuda()
+    mus = [torch.rand(n, n, generator=gen).cuda() + 1.0 for _ in range(3)]
+    msfs = [torch.rand(n, n, generator=gen).cuda() + 0.5 for _ in range(3)]
+    q = torch.randn(7, 41, n, n, generator=gen).cuda()
+    items = [(torch.randn(41, n, n, generator=gen).cuda(), mus[0], msfs[0], _bdy_arrays(41, n, gen), "u"),
+             (torch.randn(41, n, n, generator=gen).cuda(), mus[1], msfs[1], _bdy_arrays(41, n, gen), "v"),
+             (torch.randn(41, n, n, generator=gen).cuda(), mus[2], msfs[2], _bdy_arrays(41, n, gen), "t"),
+             (torch.randn(41, n, n, generator=gen).cuda(), mus[2], msfs[2], _bdy_arrays(41, n, gen), "h"),
+             (q[0], mus[2], msfs[2], _bdy_arrays(41, n, gen), "t")]
+    ref = [f.clone() for f, _, _, _, _ in items]
+    q0 = q.clone()
+    for r, (_, mu, msf, arrs, v) in zip(ref, items):
+        bdy.spec_bdy_final(r, mu, c1, c2, msf, *arrs, v, 5, 1, 36.0, *g.ix24)
+    bdy.spec_bdy_final_group(items, 36.0, g.ix24)
+    torch.cuda.synchronize()
+    for r, (f, _, _, _, v) in zip(ref, items):
+        assert torch.equal(r, f), v
+    assert torch.equal(q[1:], q0[1:])
