@@ -19,8 +19,6 @@
 #include "consts.h"
 #include "diff.h"
 
-#define AT(p, k, j, i) (p)[D.ix((k), (j), (i))]
-#define AT2(p, j, i) (p)[D.ix2((j), (i))]
 
 int pw_check_grid(const pywrf_grid_t* g, int r);
 #define CHECK(g, r) do { int _e = pw_check_grid((g), (r)); if (_e) return _e; } while (0)
@@ -344,16 +342,24 @@ struct ScalarTend {
 // y, x, z order), so the results are the same floats; each x- and z-face is
 // evaluated about once instead of by both neighbours.
 #define SA_OX 63
+#ifndef SA_KC
 #define SA_KC 4
+#endif
+#ifndef SA_SPW
+#define SA_SPW 1   // species per wave
+#endif
 __global__ void __launch_bounds__(PW_TX * PW_TY) k_scalar_adv(Dims D, ScalarTend a) {
   const Box& b = a.bzero;   // points that receive a tendency
   const Box& sp = a.span;   // launch region: b, or the whole array when h_t / z_t are zeroed as well
   const int i = sp.i0 + (int)(blockIdx.x * SA_OX + threadIdx.x);
   const int j = sp.j0 + (int)(blockIdx.y * PW_TY + threadIdx.y);
   if (j >= sp.j1) return;                     // a whole wave (one row) leaves together
-  const int k0 = sp.k0 + (int)blockIdx.z * SA_KC, k1 = imin(k0 + SA_KC, sp.k1);
+  const int nkc = (sp.nk() + SA_KC - 1) / SA_KC;
+  const int kc = (int)blockIdx.z % nkc, sg = (int)blockIdx.z / nkc;   // level chunk, species group
+  const int k0 = sp.k0 + kc * SA_KC, k1 = imin(k0 + SA_KC, sp.k1);
+  const int ms = a.scs + sg * SA_SPW, me = imin(a.sce, ms + SA_SPW - 1);
   const bool mine = threadIdx.x < SA_OX && i < sp.i1;
-  if (a.h_t && mine)
+  if (a.h_t && mine && sg == 0)
     for (int k = k0; k < k1; ++k) {
       const int q = D.ix(k, j, i);
       a.h_t[q] = 0.0f;
@@ -364,7 +370,7 @@ __global__ void __launch_bounds__(PW_TX * PW_TY) k_scalar_adv(Dims D, ScalarTend
   const bool own = mine && i >= b.i0 && i < b.i1;
   const float mx = (own && a.badv.has2(j, i)) ? AT2(a.adv.msftx, j, i) : 0.0f;
 #pragma unroll 1
-  for (int im = a.scs; im <= a.sce; ++im) {
+  for (int im = ms; im <= me; ++im) {
     AdvS ad = a.adv;
     ad.f = a.scalar + im * a.n3;
     float zlo = live ? s_zface(D, ad, a.x, k0, j, i) : 0.0f;
@@ -393,7 +399,7 @@ static void pw_launch_scalar_adv(const pywrf_grid_t* g, const ScalarTend& a, pyw
   const Box& sp = a.span;
   if (sp.size() <= 0) return;
   dim3 grid((unsigned)((sp.ni() + SA_OX - 1) / SA_OX), (unsigned)((sp.nj() + PW_TY - 1) / PW_TY),
-            (unsigned)((sp.nk() + SA_KC - 1) / SA_KC));
+            (unsigned)((sp.nk() + SA_KC - 1) / SA_KC * ((a.sce - a.scs + SA_SPW) / SA_SPW)));
   hipLaunchKernelGGL(k_scalar_adv, grid, dim3(PW_TX, PW_TY), 0, (hipStream_t)s, dims_of(g), a);
 }
 
@@ -474,6 +480,7 @@ int pw_rk_scalar_tend_fused(const pywrf_grid_t* g, int scs, int sce, int rk_step
   a.vd = {nullptr, alt, mut, rdn, rdnw, base, nullptr, nullptr, kvdq, a.x.ktf, g->kts, 'm'};
   a.scalar = scalar; a.adv_tend = advect_tend; a.sc_tend = scalar_tends; a.h_t = h_tendency; a.z_t = z_tendency;
   a.n3 = (long)g->nz * g->ny * g->nx;
+  if ((long)(sce + 1) * a.n3 >= (1L << 31)) return PYWRF_E_RANGE;   // species offsets are 32-bit
   a.scs = scs; a.sce = sce; a.rk1 = rk_step == 1; a.moist_step = moist_step; a.accumulate = 0;
   int ktf = a.x.ktf, jtf = imin(g->jte, g->jde - 1), itf = imin(g->ite, g->ide - 1);
   a.bzero = mkbox(g->kts, g->kte, g->jts, g->jte, g->its, g->ite);

@@ -151,17 +151,35 @@ __device__ inline float flux3(float qm2, float qm1, float q0, float qp1, float u
   return f4 + div_const<12>(sgn1(ua) * d);
 }
 
-// Offsets are 32-bit: every entry point rejects arrays of 2^31 or more points
-// (pw_size_ok), which keeps the address arithmetic of the stencils short.
+// Offsets are 32-bit: every entry point rejects arrays of 2^30 or more points
+// (pw_size_ok), so an element's byte offset fits in 32 bits.  AT / AT2 index
+// through that unsigned byte offset added to the (wave-uniform) array base, the
+// form global loads and stores take as a scalar base plus one 32-bit vector
+// offset, with constant displacements folded into the instruction; unsigned
+// offset arithmetic is exact modulo 2^32, so the compiler may reassociate it.
 struct Dims {
   int nz, ny, nx;
   __host__ __device__ int ix(int k, int j, int i) const { return (k * ny + j) * nx + i; }
   __host__ __device__ int ix2(int j, int i) const { return j * nx + i; }
   __host__ __device__ int plane() const { return ny * nx; }
+  // byte offsets of element (k, j, i) / (j, i)
+  __device__ unsigned ob(int k, int j, int i) const {
+    return (((unsigned)k * (unsigned)ny + (unsigned)j) * (unsigned)nx + (unsigned)i) * 4u;
+  }
+  __device__ unsigned ob2(int j, int i) const { return ((unsigned)j * (unsigned)nx + (unsigned)i) * 4u; }
 };
 
+template <class T>
+__device__ __forceinline__ T& at_b(T* p, unsigned off) {
+  return *(T*)((char*)p + off);
+}
+template <class T>
+__device__ __forceinline__ const T& at_b(const T* p, unsigned off) {
+  return *(const T*)((const char*)p + off);
+}
+
 __host__ inline bool pw_size_ok(const pywrf_grid_t* g) {
-  return g && g->nz > 0 && g->ny > 0 && g->nx > 0 && (long)g->nz * g->ny * g->nx < 0x7fffffffL;
+  return g && g->nz > 0 && g->ny > 0 && g->nx > 0 && (long)g->nz * g->ny * g->nx < (1L << 30);
 }
 
 __host__ inline Dims dims_of(const pywrf_grid_t* g) { Dims d; d.nz = g->nz; d.ny = g->ny; d.nx = g->nx; return d; }

@@ -6,8 +6,8 @@
 #include "consts.h"
 
 #ifndef AT
-#define AT(p, k, j, i) (p)[D.ix((k), (j), (i))]
-#define AT2(p, j, i) (p)[D.ix2((j), (i))]
+#define AT(p, k, j, i) at_b((p), D.ob((k), (j), (i)))
+#define AT2(p, j, i) at_b((p), D.ob2((j), (i)))
 #endif
 
 struct HDiff {

@@ -10,8 +10,8 @@
 #include "common.h"
 #include "consts.h"
 
-#define AT(p, k, j, i) (p)[D.ix((k), (j), (i))]
-#define AT2(p, j, i) (p)[D.ix2((j), (i))]
+#define AT(p, k, j, i) at_b((p), D.ob((k), (j), (i)))
+#define AT2(p, j, i) at_b((p), D.ob2((j), (i)))
 
 // Generic index-set check: every index in range and the stencil reach `r`
 // (cells beyond the outermost index used) inside the allocation.

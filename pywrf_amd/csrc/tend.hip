@@ -7,8 +7,6 @@
 #include "consts.h"
 #include "diff.h"
 
-#define AT(p, k, j, i) (p)[D.ix((k), (j), (i))]
-#define AT2(p, j, i) (p)[D.ix2((j), (i))]
 
 int pw_check_grid(const pywrf_grid_t* g, int r);
 #define CHECK(g, r) do { int _e = pw_check_grid((g), (r)); if (_e) return _e; } while (0)
