@@ -437,31 +437,85 @@ extern "C" int pywrf_advect_scalar(const pywrf_grid_t* g, const float* field, fl
 }
 
 // First RK stage only: horizontal ('m') and vertical diffusion of every
-// species into scalar_tends (wrf_dynamics.py:4018-4046).  One thread per
-// (species, point): blockIdx.z = species * nk + level.
+// species into scalar_tends (wrf_dynamics.py:4018-4046), one thread per point
+// for all species of the call.  The diffusion coefficients depend only on the
+// point: the horizontal ones (hdiff_coefs), the vertical flux factor
+// kvdif rdn[k+1] / (0.5 (alt[k] + alt[k+1])) of the two faces and the factor
+// the flux difference is applied with are formed once, each with the
+// reference's operation order up to the product with the species' own
+// difference, and applied to every species: the floats are those of the
+// per-species form (hdiff_apply, vdiff_apply with kind 'm', or 'p' for the
+// water vapour of the moist step).
+struct VFace {
+  float c;     // flux factor, 0 where the flux is zero
+  int k;       // level of the face's lower point (after the kts rule of kind 'm')
+  bool live;
+};
+
+// Face below level kf+1 (the reference's vflux[kf]) of kind 'm' or 'p'.
+__device__ inline VFace vface(const Dims& D, const VDiff& v, char kind, int kf, int j, int i) {
+  VFace r = {0.0f, kf, false};
+  if (kf < v.kts || kf >= v.ktf - 1) return r;
+  if (kind == 'm' && kf == v.kts) kf = v.kts + 1;   // vflux[kts] = vflux[kts+1]
+  if (kf >= v.ktf - 1) return r;
+  r.k = kf;
+  r.live = true;
+  r.c = v.kvdif * v.rdn[kf + 1] / (0.5f * (AT(v.alt, kf, j, i) + AT(v.alt, kf + 1, j, i)));
+  return r;
+}
+
+__device__ inline float vface_flux(const Dims& D, const VFace& fc, const float* f, const float* base1, int j,
+                                   int i) {
+  if (!fc.live) return 0.0f;
+  const float df = AT(f, fc.k + 1, j, i) - AT(f, fc.k, j, i);
+  return base1 ? fc.c * (df - base1[fc.k + 1] + base1[fc.k]) : fc.c * df;
+}
+
 __global__ void k_scalar_diff(Box b, Dims D, ScalarTend a) {
-  const int nk = b.nk();
-  const int im = a.scs + (int)blockIdx.z / nk;
-  const int k = b.k0 + (int)blockIdx.z % nk;
-  const int i = b.i0 + (int)(blockIdx.x * PW_TX + threadIdx.x), j = b.j0 + (int)(blockIdx.y * PW_TY + threadIdx.y);
-  if (i >= b.i1 || j >= b.j1) return;
+  int k, j, i;
+  if (!box_point(b, k, j, i)) return;
   const bool dh = a.bh.has(k, j, i), dv = a.bv.has(k, j, i);
   if (!dh && !dv) return;
-  const long q = im * a.n3 + D.ix(k, j, i);
-  const float* f = a.scalar + im * a.n3;
-  float st = a.sc_tend[q];
-  if (dh) {
-    HDiff hd = a.hd;
-    hd.f = f;
-    st = hdiff_apply(D, hd, k, j, i, st);
-  }
+  const int q0 = D.ix(k, j, i);
+  HCoef c = {};
+  if (dh) c = hdiff_coefs(D, a.hd, k, j, i);
+  const bool has_p = a.moist_step && a.scs <= pw::P_QV && pw::P_QV <= a.sce;
+  const bool has_m = !has_p || a.sce > a.scs;
+  VFace um = {}, dm = {}, up = {}, dp = {};
+  float am = 0.0f, ap = 0.0f;
   if (dv) {
-    VDiff vd = a.vd;
-    vd.f = f;
-    vd.kind = (a.moist_step && im == pw::P_QV) ? 'p' : 'm';
-    st = vdiff_apply(D, vd, k, j, i, st);
+    const VDiff& v = a.vd;
+    const float gg = F(pw::g * pw::g), m = AT2(v.mus, j, i), al = v.alt[q0];
+    if (has_m) {
+      um = vface(D, v, 'm', k, j, i);
+      dm = vface(D, v, 'm', k - 1, j, i);
+      am = gg / m / al * v.rdnw[k];
+    }
+    if (has_p) {
+      up = vface(D, v, 'p', k, j, i);
+      dp = vface(D, v, 'p', k - 1, j, i);
+      ap = gg * m / al * v.rdnw[k];
+    }
   }
-  a.sc_tend[q] = st;
+#pragma unroll 1
+  for (int im = a.scs; im <= a.sce; ++im) {
+    const float* f = a.scalar + im * a.n3;
+    const int q = im * (int)a.n3 + q0;
+    float st = a.sc_tend[q];
+    if (dh) {
+      auto Fv = [&](int dj, int di) { return AT(f, k, j + dj, i + di); };
+      const float f0 = Fv(0, 0);
+      st = st + (c.mrdx * (c.mkxp * (Fv(0, 1) - f0) - c.mkxm * (f0 - Fv(0, -1))) +
+                 c.mrdy * (c.mkyp * (Fv(1, 0) - f0) - c.mkym * (f0 - Fv(-1, 0))));
+    }
+    if (dv) {
+      if (a.moist_step && im == pw::P_QV)
+        st = st + ap * (vface_flux(D, up, f, a.vd.base1, j, i) - vface_flux(D, dp, f, a.vd.base1, j, i));
+      else
+        st = st + am * (vface_flux(D, um, f, nullptr, j, i) - vface_flux(D, dm, f, nullptr, j, i));
+    }
+    a.sc_tend[q] = st;
+  }
 }
 
 int pw_rk_scalar_tend_fused(const pywrf_grid_t* g, int scs, int sce, int rk_step, const float* ru, const float* rv,
@@ -495,7 +549,7 @@ int pw_rk_scalar_tend_fused(const pywrf_grid_t* g, int scs, int sce, int rk_step
   if (a.rk1 && a.bzero.size() > 0) {
     const Box& bz = a.bzero;
     dim3 grid((unsigned)((bz.ni() + PW_TX - 1) / PW_TX), (unsigned)((bz.nj() + PW_TY - 1) / PW_TY),
-              (unsigned)(bz.nk() * (sce - scs + 1)));
+              (unsigned)bz.nk());
     hipLaunchKernelGGL(k_scalar_diff, grid, dim3(PW_TX, PW_TY), 0, (hipStream_t)s, bz, dims_of(g), a);
   }
   PW_RET();
