@@ -166,8 +166,20 @@ extern "C" int pywrf_calc_ww_cp(const pywrf_grid_t* g, const float* u, const flo
 }
 
 // ---------------------------------------------------------------- calc_cq, calc_alt, calc_php
+// In-order float sum of the n species at (k, j, i); up to 8 species the loads
+// are issued together.
 __device__ inline float qsum(const Dims& D, const float* moist, int n, int k, int j, int i) {
-  long p = D.ix(k, j, i), st = (long)D.nz * D.plane();
+  const long p = D.ix(k, j, i), st = (long)D.nz * D.plane();
+  if (n <= 8) {
+    float v[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) v[m] = m < n ? moist[p + m * st] : 0.0f;
+    float s = v[0];
+#pragma unroll
+    for (int m = 1; m < 8; ++m)
+      if (m < n) s += v[m];
+    return s;
+  }
   float s = moist[p];
   for (int m = 1; m < n; ++m) s += moist[p + m * st];
   return s;
@@ -187,14 +199,25 @@ __global__ void k_calc_cq(Box b, Dims D, Box bu, Box bv, Box bw, const float* mo
   const int k0 = b.k0 + (int)blockIdx.z * CQ_KC, k1 = imin(k0 + CQ_KC, b.k1);
   const bool live = i < b.i1;
   const bool own = threadIdx.x >= 1 && live;
+  // every sum of the tile first (no store in between, so all loads can be in flight)
   float sdown = (live && k0 >= 1) ? qsum(D, moist, n, k0 - 1, j, i) : 0.0f;
-  for (int k = k0; k < k1; ++k) {
-    const float s0 = live ? qsum(D, moist, n, k, j, i) : 0.0f;
+  float s[CQ_KC], sn[CQ_KC];
+#pragma unroll
+  for (int kk = 0; kk < CQ_KC; ++kk) {
+    const int k = k0 + kk;
+    s[kk] = (live && k < k1) ? qsum(D, moist, n, k, j, i) : 0.0f;
+    sn[kk] = (own && k < k1 && bv.has(k, j, i)) ? qsum(D, moist, n, k, j - 1, i) : 0.0f;
+  }
+#pragma unroll
+  for (int kk = 0; kk < CQ_KC; ++kk) {
+    const int k = k0 + kk;
+    if (k >= k1) break;                       // uniform across the wave
+    const float s0 = s[kk];
     const float sw = __shfl_up(s0, 1, PW_TX);
     if (own) {
       const int p = D.ix(k, j, i);
       if (bu.has(k, j, i)) cqu[p] = 1.0f / (1.0f + 0.5f * (s0 + sw));
-      if (bv.has(k, j, i)) cqv[p] = 1.0f / (1.0f + 0.5f * (s0 + qsum(D, moist, n, k, j - 1, i)));
+      if (bv.has(k, j, i)) cqv[p] = 1.0f / (1.0f + 0.5f * (s0 + sn[kk]));
       if (bw.has(k, j, i)) cqw[p] = 0.5f * (s0 + sdown);
     }
     sdown = s0;

@@ -101,6 +101,7 @@ struct UpdS {
   int jf0, jf1, if0, if1;  // rows / columns where muold and munew are set
 };
 
+#define UPD_CHUNK 8
 __global__ void k_rk_update_scalar(Box all, Dims D, UpdS a) {
   int k, j, i;
   if (!box_point(all, k, j, i)) return;
@@ -109,18 +110,30 @@ __global__ void k_rk_update_scalar(Box all, Dims D, UpdS a) {
   bool hm = j >= a.jf0 && j < a.jf1 && i >= a.if0 && i < a.if1;
   float muold = hm ? a.mu_old[q2] + a.mub[q2] : 0.0f;
   float munew = hm ? a.mu_new[q2] + a.mub[q2] : 0.0f;
-  for (int im = a.scs; im <= a.sce; ++im) {
-    long qq = q + im * a.n3;
-    float s1 = a.s1[qq];
-    if (a.rk1) {
-      s1 = a.s2[qq] + 0.0f;
-      a.s1[qq] = s1;
+  const float my = (inf && ini) ? a.msfty[q2] : 0.0f;
+  // species in chunks of UPD_CHUNK: every load of a chunk is issued before its
+  // stores (the arrays are distinct, which the compiler cannot assume)
+  for (int m0 = a.scs; m0 <= a.sce; m0 += UPD_CHUNK) {
+    float v1[UPD_CHUNK], va[UPD_CHUNK], vt[UPD_CHUNK];
+#pragma unroll
+    for (int m = 0; m < UPD_CHUNK; ++m) {
+      if (m0 + m > a.sce) break;
+      const long qq = q + (m0 + m) * a.n3;
+      v1[m] = a.rk1 ? a.s2[qq] + 0.0f : a.s1[qq];
+      va[m] = (inf && ini) ? a.adv[qq] : 0.0f;
+      vt[m] = (inf && ins) ? a.sc_tend[qq] : 0.0f;
     }
-    if (!inf) continue;
-    float t = 0.0f;
-    if (ini) t = a.adv[qq] * a.msfty[q2];
-    if (ins) t = t + a.sc_tend[qq];
-    a.s2[qq] = (muold * s1 + a.dt * t) / munew;
+#pragma unroll
+    for (int m = 0; m < UPD_CHUNK; ++m) {
+      if (m0 + m > a.sce) break;
+      const long qq = q + (m0 + m) * a.n3;
+      if (a.rk1) a.s1[qq] = v1[m];
+      if (!inf) continue;
+      float t = 0.0f;
+      if (ini) t = va[m] * my;
+      if (ins) t = t + vt[m];
+      a.s2[qq] = (muold * v1[m] + a.dt * t) / munew;
+    }
   }
 }
 

@@ -506,6 +506,9 @@ struct AdvW {
 // the explicit w predictor from the LDS rhs / t_2ave.  Phase 3 (one thread
 // per column): Thomas forward sweep and back substitution in LDS.  Phase 4
 // (all threads): Rayleigh damping and the new phi, written back coalesced.
+#ifndef AW_UNROLL
+#define AW_UNROLL 2
+#endif
 __global__ void __launch_bounds__(TB) k_advance_w(Dims D, AdvW a) {
   extern __shared__ float sm[];
   __shared__ float s_fnm[CB_MAXNZ], s_fnp[CB_MAXNZ], s_rdnw[CB_MAXNZ], s_rdn[CB_MAXNZ];
@@ -531,7 +534,9 @@ __global__ void __launch_bounds__(TB) k_advance_w(Dims D, AdvW a) {
   }
   const float mi = 1.0f / my;
   __syncthreads();   // level arrays staged
-  // phase 1 (operands loaded before the arithmetic, so one level's loads are in flight together)
+  // phase 1 (operands loaded before the arithmetic, so one level's loads are in
+  // flight together; no global store in the loop, so the levels can overlap)
+#pragma unroll AW_UNROLL
   for (int k = grp; k < nz; k += TG) {
     if (!vi) continue;
     long q = D.ix(k, j, i);
@@ -556,7 +561,6 @@ __global__ void __launch_bounds__(TB) k_advance_w(Dims D, AdvW a) {
       t = 0.5f * (a.ep1 * t2 + a.em1 * t);
       t = (t + muave * a.t0) / (muts * (a.t0 + t1));
     }
-    a.t_2ave[q] = t;
     s_t2a[k * TI + lane] = t;
     float r = 0.0f;
     if (rk) {
@@ -577,6 +581,7 @@ __global__ void __launch_bounds__(TB) k_advance_w(Dims D, AdvW a) {
   __syncthreads();
   auto RHS = [&](int k) -> float { return s_rhs[k * TI + lane]; };   // rhs[0] is 0
   // phase 2
+#pragma unroll AW_UNROLL
   for (int k = grp; k < nz; k += TG) {
     if (!h) continue;
     long q = D.ix(k, j, i);
@@ -631,15 +636,16 @@ __global__ void __launch_bounds__(TB) k_advance_w(Dims D, AdvW a) {
   }
   __syncthreads();
   if (!h) return;
-  // phase 4
+  // phase 4: the new w and phi of this thread's levels into LDS (s_w, s_rhs),
+  // then written out with t_2ave in a loop of stores only
   float htop = (AT(a.ph_1, k1, j, i) + AT(a.phb, k1, j, i)) / G;
   float hbot = htop - a.hdepth;
+#pragma unroll AW_UNROLL
   for (int k = grp; k < nz; k += TG) {
     long q = D.ix(k, j, i);
     float w = s_w[k * TI + lane];
     if (k == 0 || k > k1) {
-      a.w[q] = w;
-      a.ph[q] = a.ph_in[q];
+      s_rhs[k * TI + lane] = a.ph_in[q];
     } else {
       const float p1 = a.ph_1[q], pb = a.phb[q], ws = a.w_save[q];
       float hk = (p1 + pb) / G;
@@ -649,9 +655,15 @@ __global__ void __launch_bounds__(TB) k_advance_w(Dims D, AdvW a) {
         dw = a.dampmag * (sn * sn);
       }
       w = (w - dw * mut * ws) / (1.0f + dw);
-      a.w[q] = w;
-      a.ph[q] = s_rhs[k * TI + lane] + my * 0.5f * a.dts * G * a.ep1 * w / muts;
+      s_w[k * TI + lane] = w;
+      s_rhs[k * TI + lane] = s_rhs[k * TI + lane] + my * 0.5f * a.dts * G * a.ep1 * w / muts;
     }
+  }
+  for (int k = grp; k < nz; k += TG) {
+    long q = D.ix(k, j, i);
+    a.w[q] = s_w[k * TI + lane];
+    a.ph[q] = s_rhs[k * TI + lane];
+    a.t_2ave[q] = s_t2a[k * TI + lane];
   }
 }
 
