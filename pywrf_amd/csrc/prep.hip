@@ -80,9 +80,13 @@ __global__ void k_couple_momentum(Box b, Dims D, Box bu, Box bv, Box bw, const f
   int k, j, i;
   if (!box_point(b, k, j, i)) return;
   long p = D.ix(k, j, i);
-  if (bu.has(k, j, i)) ru[p] = u[p] * AT2(muu, j, i) / AT2(msfu, j, i);
-  if (bv.has(k, j, i)) rv[p] = v[p] * AT2(muv, j, i) * AT2(msfvx_inv, j, i);
-  if (bw.has(k, j, i)) rw[p] = w[p] * AT2(mut, j, i) / AT2(msft, j, i);
+  const bool hu = bu.has(k, j, i), hv = bv.has(k, j, i), hw = bw.has(k, j, i);
+  const float xu = hu ? u[p] * AT2(muu, j, i) / AT2(msfu, j, i) : 0.0f;
+  const float xv = hv ? v[p] * AT2(muv, j, i) * AT2(msfvx_inv, j, i) : 0.0f;
+  const float xw = hw ? w[p] * AT2(mut, j, i) / AT2(msft, j, i) : 0.0f;
+  if (hu) ru[p] = xu;   // stores last: no load waits on them
+  if (hv) rv[p] = xv;
+  if (hw) rw[p] = xw;
 }
 
 extern "C" int pywrf_couple_momentum(const pywrf_grid_t* g, const float* muu, float* ru, const float* u,
@@ -682,8 +686,17 @@ __global__ void k_scale_by_mut(Box b, Dims D, const float* mut, float* a0, float
   long q = D.ix(k, j, i), st = (long)D.nz * D.plane();
   float m = AT2(mut, j, i);
   float* arr[6] = {a0, a1, a2, a3, a4, a5};
-  for (int n = 0; n < 6; ++n) arr[n][q] = divide ? arr[n][q] / m : m * arr[n][q];
-  for (int n = 0; n < nsc; ++n) sc[q + n * st] = divide ? sc[q + n * st] / m : m * sc[q + n * st];
+  float v[6 + 8];   // loads before stores (up to 8 scalar species together)
+#pragma unroll
+  for (int n = 0; n < 6; ++n) v[n] = arr[n][q];
+#pragma unroll
+  for (int n = 0; n < 8; ++n) v[6 + n] = n < nsc ? sc[q + n * st] : 0.0f;
+#pragma unroll
+  for (int n = 0; n < 6; ++n) arr[n][q] = divide ? v[n] / m : m * v[n];
+#pragma unroll
+  for (int n = 0; n < 8; ++n)
+    if (n < nsc) sc[q + n * st] = divide ? v[6 + n] / m : m * v[6 + n];
+  for (int n = 8; n < nsc; ++n) sc[q + n * st] = divide ? sc[q + n * st] / m : m * sc[q + n * st];
 }
 
 extern "C" int pywrf_calculate_phy_tend(const pywrf_grid_t* g, const float* mut, float* rublten, float* rvblten,

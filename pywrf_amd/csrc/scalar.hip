@@ -194,12 +194,13 @@ __global__ void k_div_by_mut(Box b, Dims D, const float* mut, float* a0, float* 
   if (!box_point(b, k, j, i)) return;
   long q = D.ix(k, j, i);
   float m = AT2(mut, j, i);
-  a0[q] = a0[q] / m;
-  a1[q] = a1[q] / m;
-  a2[q] = a2[q] / m;
-  a3[q] = a3[q] / m;
-  a4[q] = a4[q] / m;
-  a5[q] = a5[q] / m;
+  const float v0 = a0[q], v1 = a1[q], v2 = a2[q], v3 = a3[q], v4 = a4[q], v5 = a5[q];   // loads before stores
+  a0[q] = v0 / m;
+  a1[q] = v1 / m;
+  a2[q] = v2 / m;
+  a3[q] = v3 / m;
+  a4[q] = v4 / m;
+  a5[q] = v5 / m;
 }
 
 extern "C" int pywrf_phy_prep_part2(const pywrf_grid_t* g, const float* mut, float* rthblten, float* rublten,

@@ -76,45 +76,58 @@ __global__ void k_ssprep_3d(Box all, Dims D, SsPrep3 a) {
   bool bu = k < a.k1 && j >= a.j0 && j < a.j1 && i >= a.i0 && i < a.i1u;
   bool bv = k < a.k1 && j >= a.j0 && j < a.j1v && i >= a.i0 && i < a.i1;
   bool bw = k < a.kde && hcol;
-  // u
-  float u2 = a.u_2[q], uo = u2;
+  // every value is formed first and stored last, so that no load waits on a store
+  const float u2 = a.u_2[q], v2 = a.v_2[q], t2 = a.t_2[q], w2 = a.w_2[q], p2 = a.ph_2[q];
+  float uo = u2, u1 = 0.0f;
   if (bu) {
-    float u1 = a.u_1[q];
-    if (a.rk1) a.u_1[q] = u1 = u2 + 0.0f;
-    a.u_save[q] = u2 + 0.0f;
+    u1 = a.rk1 ? u2 + 0.0f : a.u_1[q];
     uo = (a.muus[q2] * u1 - a.muu[q2] * u2) / a.msfuy[q2];
   }
-  a.u_out[q] = uo;
-  float v2 = a.v_2[q], vo = v2;
+  float vo = v2, v1 = 0.0f;
   if (bv) {
-    float v1 = a.v_1[q];
-    if (a.rk1) a.v_1[q] = v1 = v2 + 0.0f;
-    a.v_save[q] = v2 + 0.0f;
+    v1 = a.rk1 ? v2 + 0.0f : a.v_1[q];
     vo = (a.muvs[q2] * v1 - a.muv[q2] * v2) * a.msfvx_inv[q2];
   }
-  a.v_out[q] = vo;
-  float t2 = a.t_2[q], to = t2;
+  float to = t2, t1 = 0.0f, c2 = 0.0f;
   if (bm) {
-    float t1 = a.t_1[q];
-    if (a.rk1) a.t_1[q] = t1 = t2 + 0.0f;
-    a.c2a[q] = F(pw::cpovcv) * (a.pb[q] + a.p[q]) / a.alt[q];
-    a.t_save[q] = t2 + 0.0f;
+    t1 = a.rk1 ? t2 + 0.0f : a.t_1[q];
+    c2 = F(pw::cpovcv) * (a.pb[q] + a.p[q]) / a.alt[q];
     to = a.muts[q2] * t1 - a.mut[q2] * t2;
   }
-  a.t_out[q] = to;
-  float w2 = a.w_2[q], wo = w2, p2 = a.ph_2[q], po = p2;
+  float wo = w2, po = p2, w1 = 0.0f, p1 = 0.0f, wws = 0.0f;
+  const bool wwz = !bw && hcol && (k == 0 || k == a.kde - 1);
   if (bw) {
-    float w1 = a.w_1[q], p1 = a.ph_1[q];
+    w1 = a.rk1 ? w2 + 0.0f : a.w_1[q];
+    p1 = a.rk1 ? p2 + 0.0f : a.ph_1[q];
+    wo = (a.muts[q2] * w1 - a.mut[q2] * w2) / a.msfty[q2];
+    po = p1 - p2;
+    wws = a.ww[q];
+  }
+  if (bu) {
+    if (a.rk1) a.u_1[q] = u1;
+    a.u_save[q] = u2 + 0.0f;
+  }
+  a.u_out[q] = uo;
+  if (bv) {
+    if (a.rk1) a.v_1[q] = v1;
+    a.v_save[q] = v2 + 0.0f;
+  }
+  a.v_out[q] = vo;
+  if (bm) {
+    if (a.rk1) a.t_1[q] = t1;
+    a.c2a[q] = c2;
+    a.t_save[q] = t2 + 0.0f;
+  }
+  a.t_out[q] = to;
+  if (bw) {
     if (a.rk1) {
-      a.w_1[q] = w1 = w2 + 0.0f;
-      a.ph_1[q] = p1 = p2 + 0.0f;
+      a.w_1[q] = w1;
+      a.ph_1[q] = p1;
     }
     a.w_save[q] = w2 + 0.0f;
-    wo = (a.muts[q2] * w1 - a.mut[q2] * w2) / a.msfty[q2];
     a.ph_save[q] = p2 + 0.0f;
-    po = p1 - p2;
-    a.ww_save[q] = a.ww[q];
-  } else if (hcol && (k == 0 || k == a.kde - 1)) {
+    a.ww_save[q] = wws;
+  } else if (wwz) {
     a.ww_save[q] = 0.0f;
   }
   a.w_out[q] = wo;
@@ -308,7 +321,6 @@ __global__ void k_advance_uv(Box all, Dims D, AdvUV a, Box bu, Box bv) {
     float d = uv_dpxy(D, a, k, j, i, 0, -1, a.muu, a.msfux, a.msfuy, a.rdx);
     u = u - a.dts * a.cqu[q] * d + mxy;
   }
-  a.u[q] = u;
   float v = a.v_in[q];
   if (bv.has(k, j, i)) {
     v = v + a.dts * a.rv_tend[q];
@@ -316,6 +328,7 @@ __global__ void k_advance_uv(Box all, Dims D, AdvUV a, Box bu, Box bv) {
     float d = uv_dpxy(D, a, k, j, i, -1, 0, a.muv, a.msfvy, a.msfvx, a.rdy);
     v = v - a.dts * a.cqv[q] * d + mxy;
   }
+  a.u[q] = u;   // stores last: no load waits on them
   a.v[q] = v;
 }
 
@@ -777,29 +790,34 @@ __global__ void k_ssfinish(Box all, Dims D, SsFin a) {
   if (!box_point(all, k, j, i)) return;
   long q = D.ix(k, j, i), q2 = D.ix2(j, i);
   bool hcol = j >= a.j0 && j < a.j1 && i >= a.i0 && i < a.i1;
+  // every value is formed first and stored last, so that no load waits on a store
   float v = a.v_2[q];
   if (k < a.kde - 1 && j >= a.j0 && j < a.jte && i >= a.i0 && i < a.i1)
     v = (a.msfvx[q2] * v + a.v_save[q] * a.muv[q2]) / a.muvs[q2];
-  a.v[q] = v;
   float u = a.u_2[q];
   if (k < a.kde - 1 && j >= a.j0 && j < a.j1 && i >= a.i0 && i < a.ite)
     u = (a.msfuy[q2] * u + a.u_save[q] * a.muu[q2]) / a.muus[q2];
-  a.u[q] = u;
-  float w = a.w_2[q], ph = a.ph_2[q];
-  if (k < a.kde && hcol) {
+  float w = a.w_2[q], ph = a.ph_2[q], wwn = 0.0f;
+  const bool hw = k < a.kde && hcol;
+  if (hw) {
     w = (a.msfty[q2] * w + a.w_save[q] * a.mut[q2]) / a.muts[q2];
     ph = ph + a.ph_save[q];
-    a.ww[q] = a.ww[q] + a.ww1[q];
+    wwn = a.ww[q] + a.ww1[q];
   }
-  a.w[q] = w;
-  a.ph[q] = ph;
   float t = a.t_2[q];
   if (k < a.kde - 1 && hcol) {
     if (!a.last) t = (t + a.t_save[q] * a.mut[q2]) / a.muts[q2];
     else t = (t - a.dtsn * a.mut[q2] * a.h_diabatic[q] + a.t_save[q] * a.mut[q2]) / a.muts[q2];
   }
+  const bool hm = k == 0 && hcol;
+  const float mu = hm ? a.mu_2[q2] + a.mu_save[q2] : 0.0f;
+  a.v[q] = v;
+  a.u[q] = u;
+  if (hw) a.ww[q] = wwn;
+  a.w[q] = w;
+  a.ph[q] = ph;
   a.t[q] = t;
-  if (k == 0 && hcol) a.mu_2[q2] = a.mu_2[q2] + a.mu_save[q2];
+  if (hm) a.mu_2[q2] = mu;
 }
 
 extern "C" int pywrf_small_step_finish(const pywrf_grid_t* g, const float* u_2, const float* v_2, const float* w_2,

@@ -333,7 +333,10 @@ __global__ void k_cor_curv(Box b, Dims D, Cor a, Curv c, Box bu, Box bv, Box bw)
   if (!box_point(b, k, j, i)) return;
   const int q = D.ix(k, j, i);
   const float rr = F(pw::reradius);
-  if (bu.has(k, j, i)) {
+  // the three tendencies are formed first and stored last, so that no load waits on a store
+  const bool hu = bu.has(k, j, i), hv = bv.has(k, j, i), hw = bw.has(k, j, i);
+  float tu = 0.0f, tv = 0.0f, tw = 0.0f;
+  if (hu) {
     const float rv4 = AT(a.rv, k, j + 1, i - 1) + AT(a.rv, k, j + 1, i) + AT(a.rv, k, j, i - 1) + AT(a.rv, k, j, i);
     const float rw4 = AT(a.rw, k + 1, j, i - 1) + AT(a.rw, k, j, i - 1) + AT(a.rw, k + 1, j, i) + AT(a.rw, k, j, i);
     float x = (AT2(a.msfux, j, i) / AT2(a.msfuy, j, i)) * 0.5f * (AT2(a.f, j, i) + AT2(a.f, j, i - 1)) * 0.25f * rv4;
@@ -342,9 +345,9 @@ __global__ void k_cor_curv(Box b, Dims D, Cor a, Curv c, Box bu, Box bv, Box bw)
     float t = a.ru_tend[q] + x - y;
     x = 0.5f * (curv_vxgm(D, c, k, j, i) + curv_vxgm(D, c, k, j, i - 1)) * 0.25f * rv4;
     y = AT(c.u, k, j, i) * rr * 0.25f * rw4;
-    a.ru_tend[q] = t + x - y;
+    tu = t + x - y;
   }
-  if (bv.has(k, j, i)) {
+  if (hv) {
     const float ru4 = AT(a.ru, k, j, i) + AT(a.ru, k, j, i + 1) + AT(a.ru, k, j - 1, i) + AT(a.ru, k, j - 1, i + 1);
     const float rw4 = AT(a.rw, k + 1, j - 1, i) + AT(a.rw, k, j - 1, i) + AT(a.rw, k + 1, j, i) + AT(a.rw, k, j, i);
     float r = AT2(a.msfvy, j, i) / AT2(a.msfvx, j, i);
@@ -354,9 +357,9 @@ __global__ void k_cor_curv(Box b, Dims D, Cor a, Curv c, Box bu, Box bv, Box bw)
     float t = a.rv_tend[q] - x + y;
     x = 0.5f * (curv_vxgm(D, c, k, j, i) + curv_vxgm(D, c, k, j - 1, i)) * 0.25f * ru4;
     y = r * AT(c.v, k, j, i) * rr * 0.25f * rw4;
-    a.rv_tend[q] = t - x - y;
+    tv = t - x - y;
   }
-  if (bw.has(k, j, i)) {
+  if (hw) {
     const float fzm = a.fzm[k], fzp = a.fzp[k];
     auto fz = [&](const float* f, int dj, int di) {
       return fzm * (AT(f, k, j, i) + AT(f, k, j + dj, i + di)) + fzp * (AT(f, k - 1, j, i) + AT(f, k - 1, j + dj, i + di));
@@ -365,8 +368,11 @@ __global__ void k_cor_curv(Box b, Dims D, Cor a, Curv c, Box bu, Box bv, Box bw)
     float cu = AT2(a.cosa, j, i) * 0.5f * fz(a.ru, 0, 1);
     float sv = mr * AT2(a.sina, j, i) * 0.5f * fz(a.rv, 1, 0);
     float t = a.rw_tend[q] + AT2(a.e, j, i) * (cu - sv);
-    a.rw_tend[q] = t + rr * (0.5f * fz(a.ru, 0, 1) * 0.5f * fz(c.u, 0, 1) + mr * 0.5f * fz(a.rv, 1, 0) * 0.5f * fz(c.v, 1, 0));
+    tw = t + rr * (0.5f * fz(a.ru, 0, 1) * 0.5f * fz(c.u, 0, 1) + mr * 0.5f * fz(a.rv, 1, 0) * 0.5f * fz(c.v, 1, 0));
   }
+  if (hu) a.ru_tend[q] = tu;
+  if (hv) a.rv_tend[q] = tv;
+  if (hw) a.rw_tend[q] = tw;
 }
 
 extern "C" int pywrf_coriolis_curvature(const pywrf_grid_t* g, const float* ru, const float* rv, const float* rw,
@@ -514,33 +520,58 @@ __global__ void k_addtend(Box b, Dims D, AddT a, Box bu, Box bv, Box bw, Box bt)
   int k, j, i;
   if (!box_point(b, k, j, i)) return;
   long q = D.ix(k, j, i);
-  if (bu.has(k, j, i)) {
+  // every value is formed first and stored last, so that no load waits on a store
+  const bool hu = bu.has(k, j, i), hv = bv.has(k, j, i), hw = bw.has(k, j, i), ht = bt.has(k, j, i);
+  float fu = 0.0f, tu = 0.0f, fv = 0.0f, tv = 0.0f, fw = 0.0f, tw = 0.0f, fp = 0.0f, tp = 0.0f, ft = 0.0f, tt = 0.0f;
+  if (hu) {
     float mu = AT2(a.msfuy, j, i);
-    float tf = a.ru_tendf[q];
-    if (a.rk1) a.ru_tendf[q] = tf = tf + a.u_save[q] * mu;
-    a.ru_tend[q] = a.ru_tend[q] + tf / mu;
+    fu = a.ru_tendf[q];
+    if (a.rk1) fu = fu + a.u_save[q] * mu;
+    tu = a.ru_tend[q] + fu / mu;
   }
-  if (bv.has(k, j, i)) {
-    float tf = a.rv_tendf[q];
-    if (a.rk1) a.rv_tendf[q] = tf = tf + a.v_save[q] * AT2(a.msfvx, j, i);
-    a.rv_tend[q] = a.rv_tend[q] + tf * AT2(a.msfvx_inv, j, i);
+  if (hv) {
+    fv = a.rv_tendf[q];
+    if (a.rk1) fv = fv + a.v_save[q] * AT2(a.msfvx, j, i);
+    tv = a.rv_tend[q] + fv * AT2(a.msfvx_inv, j, i);
   }
-  if (bw.has(k, j, i)) {
+  if (hw) {
     float my = AT2(a.msfty, j, i);
-    float tf = a.rw_tendf[q];
-    if (a.rk1) a.rw_tendf[q] = tf = tf + a.w_save[q] * my;
-    a.rw_tend[q] = a.rw_tend[q] + tf / my;
-    tf = a.ph_tendf[q];
-    if (a.rk1) a.ph_tendf[q] = tf = tf + a.ph_save[q];
-    a.ph_tend[q] = a.ph_tend[q] + tf / my;
+    fw = a.rw_tendf[q];
+    if (a.rk1) fw = fw + a.w_save[q] * my;
+    tw = a.rw_tend[q] + fw / my;
+    fp = a.ph_tendf[q];
+    if (a.rk1) fp = fp + a.ph_save[q];
+    tp = a.ph_tend[q] + fp / my;
   }
-  if (bt.has(k, j, i)) {
+  if (ht) {
     float my = AT2(a.msfty, j, i);
-    float tf = a.t_tendf[q];
-    if (a.rk1) a.t_tendf[q] = tf = tf + a.t_save[q];
-    a.t_tend[q] = a.t_tend[q] + tf / my + AT2(a.mut, j, i) * a.h_diabatic[q] / my;
+    ft = a.t_tendf[q];
+    if (a.rk1) ft = ft + a.t_save[q];
+    tt = a.t_tend[q] + ft / my + AT2(a.mut, j, i) * a.h_diabatic[q] / my;
   }
-  if (k == bt.k0 && bt.has(k, j, i)) AT2(a.mu_tend, j, i) = AT2(a.mu_tend, j, i) + AT2(a.mu_tendf, j, i);
+  const bool hm = k == bt.k0 && ht;
+  const float tm = hm ? AT2(a.mu_tend, j, i) + AT2(a.mu_tendf, j, i) : 0.0f;
+  if (hu) {
+    if (a.rk1) a.ru_tendf[q] = fu;
+    a.ru_tend[q] = tu;
+  }
+  if (hv) {
+    if (a.rk1) a.rv_tendf[q] = fv;
+    a.rv_tend[q] = tv;
+  }
+  if (hw) {
+    if (a.rk1) {
+      a.rw_tendf[q] = fw;
+      a.ph_tendf[q] = fp;
+    }
+    a.rw_tend[q] = tw;
+    a.ph_tend[q] = tp;
+  }
+  if (ht) {
+    if (a.rk1) a.t_tendf[q] = ft;
+    a.t_tend[q] = tt;
+  }
+  if (hm) AT2(a.mu_tend, j, i) = tm;
 }
 
 extern "C" int pywrf_rk_addtend_dry(const pywrf_grid_t* g, float* ru_tend, float* rv_tend, float* rw_tend,
