@@ -18,6 +18,21 @@
 // are dealt to them round-robin.
 #define PW_XCDS 8
 
+// Block (bx, by) of a 2-D grid, renumbered in XCD order: the hardware hands
+// consecutive workgroups to the PW_XCDS XCDs in turn, so neighbouring rows of
+// a row-per-block grid would sit in different L2 caches.  Renumbering gives
+// each XCD a contiguous range of the row-major block order (XCD x < rem takes
+// q + 1 blocks, the others q, for n = q PW_XCDS + rem blocks): a bijection of
+// the blocks, so every block is still run exactly once.
+__device__ inline void xcd_block_2d(int& bx, int& by) {
+  const int gx = (int)gridDim.x, n = gx * (int)gridDim.y;
+  const int lin = (int)blockIdx.y * gx + (int)blockIdx.x;
+  const int q = n / PW_XCDS, rem = n % PW_XCDS, x = lin % PW_XCDS, r = lin / PW_XCDS;
+  const int ord = x < rem ? x * (q + 1) + r : rem * (q + 1) + (x - rem) * q + r;
+  bx = ord % gx;
+  by = ord / gx;
+}
+
 struct Box {
   int k0, k1, j0, j1, i0, i1;
   __host__ __device__ int nk() const { return k1 > k0 ? k1 - k0 : 0; }

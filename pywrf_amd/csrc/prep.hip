@@ -134,7 +134,9 @@ __global__ void __launch_bounds__(WW_TB) k_calc_ww_cp(Box b, Dims D, WwCp a) {
   __shared__ float s_dc[256];       // dnw[k] * c1h[k] (nz <= 256: the LDS check bounds nz by 256)
   for (int t = threadIdx.x; t < D.nz; t += WW_TB) s_dc[t] = a.dnw[t] * a.c1h[t];
   const int lane = threadIdx.x % WW_TI, grp = threadIdx.x / WW_TI;
-  const int j = b.j0 + (int)blockIdx.y, i = b.i0 + (int)blockIdx.x * WW_TI + lane;
+  int bx, by;
+  xcd_block_2d(bx, by);   // rows of one XCD contiguous (neighbouring rows share an L2)
+  const int j = b.j0 + by, i = b.i0 + bx * WW_TI + lane;
   const bool ok = i < b.i1;
   if (ok)
     for (int k = grp; k < D.nz; k += WW_TG)
@@ -599,7 +601,9 @@ __global__ void __launch_bounds__(PP_TB) k_phy_prep_col(Box b, Dims D, PhyCol a)
   __shared__ float s_dnw[128];       // nz <= 128 (two LDS arrays of nz x 64 floats within 64 KB)
   for (int t = threadIdx.x; t < D.nz; t += PP_TB) s_dnw[t] = a.dnw[t];
   const int lane = threadIdx.x % PP_TI, grp = threadIdx.x / PP_TI;
-  const int j = b.j0 + (int)blockIdx.y, i = b.i0 + (int)blockIdx.x * PP_TI + lane;
+  int bx, by;
+  xcd_block_2d(bx, by);   // rows of one XCD contiguous (neighbouring rows share an L2)
+  const int j = b.j0 + by, i = b.i0 + bx * PP_TI + lane;
   const bool ok = i < b.i1;
   const int k0 = b.k0, k1 = b.k1, kte = a.kte, kde = a.kde;
   auto ZW = [&](int kk) { return AT(a.z_at_w, kk, j, i); };

@@ -8,6 +8,10 @@
 // thread per (j, i) column with k marching; the output buffer doubles as the
 // column scratch, so no full-grid temporaries are allocated.
 #include "common.h"
+
+#ifndef PW_COL_XCD
+#define PW_COL_XCD 1
+#endif
 #include "consts.h"
 
 #define AT(p, k, j, i) at_b((p), D.ob((k), (j), (i)))
@@ -224,7 +228,9 @@ __global__ void __launch_bounds__(CW_TB) k_calc_coef_w(Box b, Dims D, CoefW p) {
   float* s_b = sm + D.nz * CW_TI;   // diagonal
   float* s_c = sm + 2 * D.nz * CW_TI;   // super-diagonal
   const int lane = threadIdx.x % CW_TI, grp = threadIdx.x / CW_TI;
-  const int j = b.j0 + (int)blockIdx.y, i = b.i0 + (int)blockIdx.x * CW_TI + lane;
+  int bx, by;
+  xcd_block_2d(bx, by);   // rows of one XCD contiguous (neighbouring rows share an L2)
+  const int j = b.j0 + by, i = b.i0 + bx * CW_TI + lane;
   const bool ok = i < b.i1;
   const int kde = p.kde;
   float m = 0.0f;
@@ -404,7 +410,13 @@ __global__ void __launch_bounds__(TB) k_advance_mu_t(Dims D, AdvMuT a) {
   float* s_dv = sm;             // [nz][TI] dvdxi
   float* s_ww = sm + nz * TI;   // [nz][TI] eta velocity before subtracting ww_1
   const int lane = threadIdx.x % TI, grp = threadIdx.x / TI;
-  const int j = blockIdx.y, i = blockIdx.x * TI + lane;
+  int bx, by;
+#if PW_COL_XCD
+  xcd_block_2d(bx, by);
+#else
+  bx = (int)blockIdx.x; by = (int)blockIdx.y;
+#endif
+  const int j = by, i = bx * TI + lane;
   const bool vi = i < D.nx;
   const bool h = vi && j >= a.j0 && j < a.j1 && i >= a.i0 && i < a.i1;
   const long q2 = vi ? D.ix2(j, i) : 0;
@@ -536,7 +548,13 @@ __global__ void __launch_bounds__(TB) k_advance_w(Dims D, AdvW a) {
   float* s_al = sm + 4 * nz * TI;
   float* s_ga = sm + 5 * nz * TI;
   const int lane = threadIdx.x % TI, grp = threadIdx.x / TI;
-  const int j = blockIdx.y, i = blockIdx.x * TI + lane;
+  int bx, by;
+#if PW_COL_XCD
+  xcd_block_2d(bx, by);
+#else
+  bx = (int)blockIdx.x; by = (int)blockIdx.y;
+#endif
+  const int j = by, i = bx * TI + lane;
   const bool vi = i < D.nx;
   const bool h = vi && j >= a.j0 && j < a.j1 && i >= a.i0 && i < a.i1;
   const long q2 = vi ? D.ix2(j, i) : 0;
