@@ -316,6 +316,22 @@ __device__ inline float s_zface(const Dims& D, const AdvS& a, const AdvIdx& x, i
 }
 
 
+// The same faces from operand values: Q[0..5] are the field at offsets -3..2
+// from the face (rows for y, levels for z), vel the face velocity.
+__device__ inline float s_yface_v(const AdvIdx& x, int jf, float vel, const float* Q) {
+  if (jf >= x.jds + 3 && jf < x.jde - 3) return vel * flux5(Q[0], Q[1], Q[2], Q[3], Q[4], Q[5], vel);
+  if (jf == x.jds + 1 || jf == x.jde - 2) return 0.5f * vel * (Q[3] + Q[2]);
+  if (jf == x.jds + 2 || jf == x.jde - 3) return vel * flux3(Q[1], Q[2], Q[3], Q[4], vel);
+  return 0.0f;
+}
+
+__device__ inline float s_zface_v(const AdvS& a, const AdvIdx& x, int kf, float vel, const float* Q) {
+  if (kf < 1 || kf >= x.ktf) return 0.0f;
+  if (kf >= 3 && kf < x.ktf - 2) return vel * flux5(Q[0], Q[1], Q[2], Q[3], Q[4], Q[5], -vel);
+  if (kf == 1 || kf == x.ktf - 1) return vel * (a.fzm[kf] * Q[3] + a.fzp[kf] * Q[2]);
+  return vel * flux3(Q[1], Q[2], Q[3], Q[4], -vel);  // kf == 2 or ktf-2
+}
+
 // ------------------------------------------------------------------ rk_scalar_tend, all species in one pass
 // Zeroes h_tendency / z_tendency (whole array, in the same launch); for every species of
 // [scs, sce] writes its advection tendency on the tile (zero outside the
@@ -373,18 +389,39 @@ __global__ void __launch_bounds__(PW_TX * PW_TY) k_scalar_adv(Dims D, ScalarTend
   for (int im = ms; im <= me; ++im) {
     AdvS ad = a.adv;
     ad.f = a.scalar + im * a.n3;
-    float zlo = live ? s_zface(D, ad, a.x, k0, j, i) : 0.0f;
+    // the column's values at levels kf-3 .. kf+2 around the current z-face kf,
+    // slid up one level per step (one new load per level); levels outside the
+    // array are never used by the face formulas and are not loaded
+    // (indices clamped into the array so that every load is unconditional; the
+    // clamped values are never used)
+    const int ic = imin(i, D.nx - 1);
+    auto ZQ = [&](int kk) { return AT(ad.f, imin(imax(kk, 0), D.nz - 1), j, ic); };
+    float wz[6];
+#pragma unroll
+    for (int o = 0; o < 6; ++o) wz[o] = ZQ(k0 - 3 + o);
+    float zlo = live ? s_zface_v(ad, a.x, k0, AT(ad.rom, imin(imax(k0, 0), D.nz - 1), j, ic), wz) : 0.0f;
 #pragma unroll 1
     for (int k = k0; k < k1; ++k) {
       const float xf = live ? s_xface(D, ad, a.x, k, j, i) : 0.0f;
       const float xr = __shfl_down(xf, 1, PW_TX);
-      const float zhi = live ? s_zface(D, ad, a.x, k + 1, j, i) : 0.0f;
+#pragma unroll
+      for (int o = 0; o < 5; ++o) wz[o] = wz[o + 1];
+      wz[5] = ZQ(k + 3);
+      const int kf = k + 1;
+      const float zhi = live ? s_zface_v(ad, a.x, kf, AT(ad.rom, imin(kf, D.nz - 1), j, ic), wz) : 0.0f;
       if (own && k >= b.k0 && k < b.k1) {
         const int q = im * a.n3 + D.ix(k, j, i);
         float t = a.accumulate ? a.adv_tend[q] : 0.0f;
         if (a.badv.has(k, j, i)) {
-          if (a.by.has(k, j, i))
-            t = t - (mx * ad.rdy) * (s_yface(D, ad, a.x, k, j + 1, i) - s_yface(D, ad, a.x, k, j, i));
+          if (a.by.has(k, j, i)) {
+            // rows j-3 .. j+3 once for the faces at j (rows j-3 .. j+2) and j+1 (rows j-2 .. j+3)
+            float r[7];
+#pragma unroll
+            for (int o = 0; o < 7; ++o) r[o] = AT(ad.f, k, imin(imax(j - 3 + o, 0), D.ny - 1), i);
+            const float yhi = s_yface_v(a.x, j + 1, AT(ad.rv, k, j + 1, i), r + 1);
+            const float ylo = s_yface_v(a.x, j, AT(ad.rv, k, j, i), r);
+            t = t - (mx * ad.rdy) * (yhi - ylo);
+          }
           if (a.bx.has(k, j, i)) t = t - (mx * ad.rdx) * (xr - xf);
           t = t - ad.rdzw[k] * (zhi - zlo);
         }
