@@ -108,9 +108,12 @@ __global__ void k_rk_update_scalar(Box all, Dims D, UpdS a) {
   long q = D.ix(k, j, i), q2 = D.ix2(j, i);
   const bool inf = a.bf.has(k, j, i), ini = a.bi.has(k, j, i), ins = a.bs.has(k, j, i);
   bool hm = j >= a.jf0 && j < a.jf1 && i >= a.if0 && i < a.if1;
-  float muold = hm ? a.mu_old[q2] + a.mub[q2] : 0.0f;
-  float munew = hm ? a.mu_new[q2] + a.mub[q2] : 0.0f;
-  const float my = (inf && ini) ? a.msfty[q2] : 0.0f;
+  // the point's operands are loaded whatever the masks say (a load under a
+  // divergent branch is waited for before the branch ends); the masks select
+  const float mo = a.mu_old[q2], mn = a.mu_new[q2], mb = a.mub[q2], msf = a.msfty[q2];
+  float muold = hm ? mo + mb : 0.0f;
+  float munew = hm ? mn + mb : 0.0f;
+  const float my = (inf && ini) ? msf : 0.0f;
   // species in chunks of UPD_CHUNK: every load of a chunk is issued before its
   // stores (the arrays are distinct, which the compiler cannot assume)
   for (int m0 = a.scs; m0 <= a.sce; m0 += UPD_CHUNK) {
@@ -119,9 +122,10 @@ __global__ void k_rk_update_scalar(Box all, Dims D, UpdS a) {
     for (int m = 0; m < UPD_CHUNK; ++m) {
       if (m0 + m > a.sce) break;
       const long qq = q + (m0 + m) * a.n3;
-      v1[m] = a.rk1 ? a.s2[qq] + 0.0f : a.s1[qq];
-      va[m] = (inf && ini) ? a.adv[qq] : 0.0f;
-      vt[m] = (inf && ins) ? a.sc_tend[qq] : 0.0f;
+      v1[m] = a.rk1 ? a.s2[qq] + 0.0f : a.s1[qq];   // rk1 is uniform
+      const float av = a.adv[qq], tv = a.sc_tend[qq];
+      va[m] = (inf && ini) ? av : 0.0f;
+      vt[m] = (inf && ins) ? tv : 0.0f;
     }
 #pragma unroll
     for (int m = 0; m < UPD_CHUNK; ++m) {

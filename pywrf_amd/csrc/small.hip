@@ -80,32 +80,38 @@ __global__ void k_ssprep_3d(Box all, Dims D, SsPrep3 a) {
   bool bu = k < a.k1 && j >= a.j0 && j < a.j1 && i >= a.i0 && i < a.i1u;
   bool bv = k < a.k1 && j >= a.j0 && j < a.j1v && i >= a.i0 && i < a.i1;
   bool bw = k < a.kde && hcol;
-  // every value is formed first and stored last, so that no load waits on a store
+  // every value is formed first and stored last, so that no load waits on a
+  // store, and every operand (all at this point) is loaded before the
+  // branches, so that no branch waits on its loads
   const float u2 = a.u_2[q], v2 = a.v_2[q], t2 = a.t_2[q], w2 = a.w_2[q], p2 = a.ph_2[q];
+  const float u1l = a.u_1[q], v1l = a.v_1[q], t1l = a.t_1[q], w1l = a.w_1[q], p1l = a.ph_1[q];
+  const float pbq = a.pb[q], pq = a.p[q], altq = a.alt[q], wwq = a.ww[q];
+  const float muus = a.muus[q2], muu = a.muu[q2], msfuy = a.msfuy[q2], muvs = a.muvs[q2], muv = a.muv[q2];
+  const float msfvx_inv = a.msfvx_inv[q2], muts = a.muts[q2], mut = a.mut[q2], msfty = a.msfty[q2];
   float uo = u2, u1 = 0.0f;
   if (bu) {
-    u1 = a.rk1 ? u2 + 0.0f : a.u_1[q];
-    uo = (a.muus[q2] * u1 - a.muu[q2] * u2) / a.msfuy[q2];
+    u1 = a.rk1 ? u2 + 0.0f : u1l;
+    uo = (muus * u1 - muu * u2) / msfuy;
   }
   float vo = v2, v1 = 0.0f;
   if (bv) {
-    v1 = a.rk1 ? v2 + 0.0f : a.v_1[q];
-    vo = (a.muvs[q2] * v1 - a.muv[q2] * v2) * a.msfvx_inv[q2];
+    v1 = a.rk1 ? v2 + 0.0f : v1l;
+    vo = (muvs * v1 - muv * v2) * msfvx_inv;
   }
   float to = t2, t1 = 0.0f, c2 = 0.0f;
   if (bm) {
-    t1 = a.rk1 ? t2 + 0.0f : a.t_1[q];
-    c2 = F(pw::cpovcv) * (a.pb[q] + a.p[q]) / a.alt[q];
-    to = a.muts[q2] * t1 - a.mut[q2] * t2;
+    t1 = a.rk1 ? t2 + 0.0f : t1l;
+    c2 = F(pw::cpovcv) * (pbq + pq) / altq;
+    to = muts * t1 - mut * t2;
   }
   float wo = w2, po = p2, w1 = 0.0f, p1 = 0.0f, wws = 0.0f;
   const bool wwz = !bw && hcol && (k == 0 || k == a.kde - 1);
   if (bw) {
-    w1 = a.rk1 ? w2 + 0.0f : a.w_1[q];
-    p1 = a.rk1 ? p2 + 0.0f : a.ph_1[q];
-    wo = (a.muts[q2] * w1 - a.mut[q2] * w2) / a.msfty[q2];
+    w1 = a.rk1 ? w2 + 0.0f : w1l;
+    p1 = a.rk1 ? p2 + 0.0f : p1l;
+    wo = (muts * w1 - mut * w2) / msfty;
     po = p1 - p2;
-    wws = a.ww[q];
+    wws = wwq;
   }
   if (bu) {
     if (a.rk1) a.u_1[q] = u1;
@@ -731,18 +737,27 @@ __global__ void k_sumflux(Box b, Dims D, const float* ru, const float* rv, const
   int k, j, i;
   if (!box_point(b, k, j, i)) return;
   long q = D.ix(k, j, i);
+  // every operand is at this point, so all loads are issued before the
+  // branches (a load inside a divergent branch is waited for before the
+  // branch ends); `first` and `last` are uniform
   float rm = first ? 0.0f : ru_m[q], vm = first ? 0.0f : rv_m[q], wm = first ? 0.0f : ww_m[q];
+  const float rq = ru[q], vq = rv[q], wq = ww[q];
+  float ul = 0.0f, vl = 0.0f, wl = 0.0f, mu = 0.0f, my = 0.0f, mv = 0.0f, mx = 0.0f;
+  if (last) {
+    ul = u_lin[q]; vl = v_lin[q]; wl = ww_lin[q];
+    mu = AT2(muu, j, i); my = AT2(msfuy, j, i); mv = AT2(muv, j, i); mx = AT2(msfvx_inv, j, i);
+  }
   if (bu.has(k, j, i)) {
-    rm = rm + ru[q];
-    if (last) rm = rm / n + AT2(muu, j, i) * u_lin[q] / AT2(msfuy, j, i);
+    rm = rm + rq;
+    if (last) rm = rm / n + mu * ul / my;
   }
   if (bv.has(k, j, i)) {
-    vm = vm + rv[q];
-    if (last) vm = vm / n + AT2(muv, j, i) * v_lin[q] * AT2(msfvx_inv, j, i);
+    vm = vm + vq;
+    if (last) vm = vm / n + mv * vl * mx;
   }
   if (bw.has(k, j, i)) {
-    wm = wm + ww[q];
-    if (last) wm = wm / n + ww_lin[q];
+    wm = wm + wq;
+    if (last) wm = wm / n + wl;
   }
   ru_m[q] = rm;
   rv_m[q] = vm;
@@ -808,27 +823,33 @@ __global__ void k_ssfinish(Box all, Dims D, SsFin a) {
   if (!box_point(all, k, j, i)) return;
   long q = D.ix(k, j, i), q2 = D.ix2(j, i);
   bool hcol = j >= a.j0 && j < a.j1 && i >= a.i0 && i < a.i1;
-  // every value is formed first and stored last, so that no load waits on a store
-  float v = a.v_2[q];
-  if (k < a.kde - 1 && j >= a.j0 && j < a.jte && i >= a.i0 && i < a.i1)
-    v = (a.msfvx[q2] * v + a.v_save[q] * a.muv[q2]) / a.muvs[q2];
-  float u = a.u_2[q];
-  if (k < a.kde - 1 && j >= a.j0 && j < a.j1 && i >= a.i0 && i < a.ite)
-    u = (a.msfuy[q2] * u + a.u_save[q] * a.muu[q2]) / a.muus[q2];
-  float w = a.w_2[q], ph = a.ph_2[q], wwn = 0.0f;
+  // every value is formed first and stored last, so that no load waits on a
+  // store, and every operand (all at this point) is loaded before the
+  // branches, so that no branch waits on its loads
+  const float v2 = a.v_2[q], vs = a.v_save[q], u2 = a.u_2[q], us = a.u_save[q];
+  const float w2 = a.w_2[q], ws = a.w_save[q], p2 = a.ph_2[q], ps = a.ph_save[q], wwq = a.ww[q], ww1 = a.ww1[q];
+  const float t2 = a.t_2[q], ts = a.t_save[q], hd = a.last ? a.h_diabatic[q] : 0.0f;
+  const float msfvx = a.msfvx[q2], muv = a.muv[q2], muvs = a.muvs[q2], msfuy = a.msfuy[q2], muu = a.muu[q2];
+  const float muus = a.muus[q2], msfty = a.msfty[q2], mut = a.mut[q2], muts = a.muts[q2];
+  const float mu2 = a.mu_2[q2], musv = a.mu_save[q2];
+  float v = v2;
+  if (k < a.kde - 1 && j >= a.j0 && j < a.jte && i >= a.i0 && i < a.i1) v = (msfvx * v + vs * muv) / muvs;
+  float u = u2;
+  if (k < a.kde - 1 && j >= a.j0 && j < a.j1 && i >= a.i0 && i < a.ite) u = (msfuy * u + us * muu) / muus;
+  float w = w2, ph = p2, wwn = 0.0f;
   const bool hw = k < a.kde && hcol;
   if (hw) {
-    w = (a.msfty[q2] * w + a.w_save[q] * a.mut[q2]) / a.muts[q2];
-    ph = ph + a.ph_save[q];
-    wwn = a.ww[q] + a.ww1[q];
+    w = (msfty * w + ws * mut) / muts;
+    ph = ph + ps;
+    wwn = wwq + ww1;
   }
-  float t = a.t_2[q];
+  float t = t2;
   if (k < a.kde - 1 && hcol) {
-    if (!a.last) t = (t + a.t_save[q] * a.mut[q2]) / a.muts[q2];
-    else t = (t - a.dtsn * a.mut[q2] * a.h_diabatic[q] + a.t_save[q] * a.mut[q2]) / a.muts[q2];
+    if (!a.last) t = (t + ts * mut) / muts;
+    else t = (t - a.dtsn * mut * hd + ts * mut) / muts;
   }
   const bool hm = k == 0 && hcol;
-  const float mu = hm ? a.mu_2[q2] + a.mu_save[q2] : 0.0f;
+  const float mu = hm ? mu2 + musv : 0.0f;
   a.v[q] = v;
   a.u[q] = u;
   if (hw) a.ww[q] = wwn;

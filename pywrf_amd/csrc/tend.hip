@@ -522,35 +522,44 @@ __global__ void k_addtend(Box b, Dims D, AddT a, Box bu, Box bv, Box bw, Box bt)
   long q = D.ix(k, j, i);
   // every value is formed first and stored last, so that no load waits on a store
   const bool hu = bu.has(k, j, i), hv = bv.has(k, j, i), hw = bw.has(k, j, i), ht = bt.has(k, j, i);
+  // the point's operands are loaded before the branches (a load under a
+  // divergent branch is waited for before the branch ends); rk1 is uniform
+  const float rut = a.ru_tend[q], rutf = a.ru_tendf[q], rvt = a.rv_tend[q], rvtf = a.rv_tendf[q];
+  const float rwt = a.rw_tend[q], rwtf = a.rw_tendf[q], pht = a.ph_tend[q], phtf = a.ph_tendf[q];
+  const float tt0 = a.t_tend[q], ttf = a.t_tendf[q], hd = a.h_diabatic[q];
+  float us = 0.0f, vs = 0.0f, ws = 0.0f, ps = 0.0f, tsv = 0.0f;
+  if (a.rk1) {
+    us = a.u_save[q]; vs = a.v_save[q]; ws = a.w_save[q]; ps = a.ph_save[q]; tsv = a.t_save[q];
+  }
+  const float msfuy = AT2(a.msfuy, j, i), msfvx = AT2(a.msfvx, j, i), msfvx_inv = AT2(a.msfvx_inv, j, i);
+  const float msfty = AT2(a.msfty, j, i), mut = AT2(a.mut, j, i);
+  const float mt0 = AT2(a.mu_tend, j, i), mtf = AT2(a.mu_tendf, j, i);
   float fu = 0.0f, tu = 0.0f, fv = 0.0f, tv = 0.0f, fw = 0.0f, tw = 0.0f, fp = 0.0f, tp = 0.0f, ft = 0.0f, tt = 0.0f;
   if (hu) {
-    float mu = AT2(a.msfuy, j, i);
-    fu = a.ru_tendf[q];
-    if (a.rk1) fu = fu + a.u_save[q] * mu;
-    tu = a.ru_tend[q] + fu / mu;
+    fu = rutf;
+    if (a.rk1) fu = fu + us * msfuy;
+    tu = rut + fu / msfuy;
   }
   if (hv) {
-    fv = a.rv_tendf[q];
-    if (a.rk1) fv = fv + a.v_save[q] * AT2(a.msfvx, j, i);
-    tv = a.rv_tend[q] + fv * AT2(a.msfvx_inv, j, i);
+    fv = rvtf;
+    if (a.rk1) fv = fv + vs * msfvx;
+    tv = rvt + fv * msfvx_inv;
   }
   if (hw) {
-    float my = AT2(a.msfty, j, i);
-    fw = a.rw_tendf[q];
-    if (a.rk1) fw = fw + a.w_save[q] * my;
-    tw = a.rw_tend[q] + fw / my;
-    fp = a.ph_tendf[q];
-    if (a.rk1) fp = fp + a.ph_save[q];
-    tp = a.ph_tend[q] + fp / my;
+    fw = rwtf;
+    if (a.rk1) fw = fw + ws * msfty;
+    tw = rwt + fw / msfty;
+    fp = phtf;
+    if (a.rk1) fp = fp + ps;
+    tp = pht + fp / msfty;
   }
   if (ht) {
-    float my = AT2(a.msfty, j, i);
-    ft = a.t_tendf[q];
-    if (a.rk1) ft = ft + a.t_save[q];
-    tt = a.t_tend[q] + ft / my + AT2(a.mut, j, i) * a.h_diabatic[q] / my;
+    ft = ttf;
+    if (a.rk1) ft = ft + tsv;
+    tt = tt0 + ft / msfty + mut * hd / msfty;
   }
   const bool hm = k == bt.k0 && ht;
-  const float tm = hm ? AT2(a.mu_tend, j, i) + AT2(a.mu_tendf, j, i) : 0.0f;
+  const float tm = hm ? mt0 + mtf : 0.0f;
   if (hu) {
     if (a.rk1) a.ru_tendf[q] = fu;
     a.ru_tend[q] = tu;
