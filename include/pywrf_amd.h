@@ -291,11 +291,35 @@ int pywrf_rk_scalar_tend(const pywrf_grid_t* g, int scs, int sce, int rk_step, c
                          const float* msfvy, const float* msftx, const float* msfty, double rdx, double rdy,
                          const float* rdn, const float* rdnw, double kvdif, const float* xkmhd,
                          pywrf_stream_t s);
+/* Two rk_scalar_tend calls that share everything but the species array, its
+ * range, its two tendency arrays and the moist flag (the moist and the
+ * tracer call of solver.py:1548-1610), in the same launches; h_tendency and
+ * z_tendency are zeroed once.  Same results as the two calls.  No
+ * counterpart in the reference: a batching. */
+int pywrf_rk_scalar_tend_pair(const pywrf_grid_t* g, int rk_step, int scs, int sce, const float* scalar,
+                              float* scalar_tends, float* advect_tend, int moist_step, int scs2, int sce2,
+                              const float* scalar2, float* scalar_tends2, float* advect_tend2, int moist_step2,
+                              const float* ru, const float* rv, const float* ww, const float* mut,
+                              const float* alt, float* h_tendency, float* z_tendency, const float* base,
+                              const float* fnm, const float* fnp, const float* msfux, const float* msfuy,
+                              const float* msfvx, const float* msfvx_inv, const float* msfvy, const float* msftx,
+                              const float* msfty, double rdx, double rdy, const float* rdn, const float* rdnw,
+                              double kvdif, const float* xkmhd, pywrf_stream_t s);
 /* rk_update_scalar, wrf_dynamics.py:4055-4119 */
 int pywrf_rk_update_scalar(const pywrf_grid_t* g, int scs, int sce, float* scalar_1, float* scalar_2,
                            const float* sc_tend, const float* advect_tend, const float* msfty, const float* mu_old,
                            const float* mu_new, const float* mu_base, int rk_step, double dt, int spec_zone,
                            pywrf_stream_t s);
+/* Two rk_update_scalar calls that share everything but the two species
+ * arrays, their range and their tendencies (the moist and the tracer update
+ * of solver.py:1548-1636), in one launch.  Same results as the two calls.  No
+ * counterpart in the reference: a batching. */
+int pywrf_rk_update_scalar_pair(const pywrf_grid_t* g, int rk_step, int scs, int sce, float* scalar_1,
+                                float* scalar_2, const float* sc_tend, const float* advect_tend, int scs2, int sce2,
+                                float* scalar_1b, float* scalar_2b, const float* sc_tend_b,
+                                const float* advect_tend_b, const float* msfty, const float* mu_old,
+                                const float* mu_new, const float* mu_base, double dt, int spec_zone,
+                                pywrf_stream_t s);
 /* calc_p_rho_phi, wrf_dynamics.py:4124-4164 (phi is not re-integrated, as in the reference) */
 int pywrf_calc_p_rho_phi(const pywrf_grid_t* g, const float* moist, float* al, const float* alb, const float* mu,
                          const float* muts, const float* ph, float* p, const float* pb, const float* t, double p0,

@@ -348,6 +348,11 @@ struct ScalarTend {
   long n3;
   int scs, sce, rk1, moist_step, accumulate;
   Box bzero, badv, by, bx, bh, bv, span;
+  // a second species array over the same fluxes and index set (scalar2 null:
+  // none); species groups [0, nsg1) are the first array's, the rest the second's
+  const float* scalar2;
+  float *adv_tend2, *sc_tend2;
+  int scs2, sce2, moist_step2, nsg1;
 };
 
 // Tile of 63 i x 4 j points per workgroup (lane l of a row evaluates the x-face
@@ -373,7 +378,11 @@ __global__ void __launch_bounds__(PW_TX * PW_TY) k_scalar_adv(Dims D, ScalarTend
   const int nkc = (sp.nk() + SA_KC - 1) / SA_KC;
   const int kc = (int)blockIdx.z % nkc, sg = (int)blockIdx.z / nkc;   // level chunk, species group
   const int k0 = sp.k0 + kc * SA_KC, k1 = imin(k0 + SA_KC, sp.k1);
-  const int ms = a.scs + sg * SA_SPW, me = imin(a.sce, ms + SA_SPW - 1);
+  const bool second = sg >= a.nsg1;                                   // the second species array
+  const int scs = second ? a.scs2 : a.scs, sce = second ? a.sce2 : a.sce;
+  const int ms = scs + (second ? sg - a.nsg1 : sg) * SA_SPW, me = imin(sce, ms + SA_SPW - 1);
+  const float* const sbase = second ? a.scalar2 : a.scalar;
+  float* const tbase = second ? a.adv_tend2 : a.adv_tend;
   const bool mine = threadIdx.x < SA_OX && i < sp.i1;
   if (a.h_t && mine && sg == 0)
     for (int k = k0; k < k1; ++k) {
@@ -388,7 +397,7 @@ __global__ void __launch_bounds__(PW_TX * PW_TY) k_scalar_adv(Dims D, ScalarTend
 #pragma unroll 1
   for (int im = ms; im <= me; ++im) {
     AdvS ad = a.adv;
-    ad.f = a.scalar + im * a.n3;
+    ad.f = sbase + im * a.n3;
     // the column's values at levels kf-3 .. kf+2 around the current z-face kf,
     // slid up one level per step (one new load per level); levels outside the
     // array are never used by the face formulas and are not loaded
@@ -411,7 +420,7 @@ __global__ void __launch_bounds__(PW_TX * PW_TY) k_scalar_adv(Dims D, ScalarTend
       const float zhi = live ? s_zface_v(ad, a.x, kf, AT(ad.rom, imin(kf, D.nz - 1), j, ic), wz) : 0.0f;
       if (own && k >= b.k0 && k < b.k1) {
         const int q = im * a.n3 + D.ix(k, j, i);
-        float t = a.accumulate ? a.adv_tend[q] : 0.0f;
+        float t = a.accumulate ? tbase[q] : 0.0f;
         if (a.badv.has(k, j, i)) {
           if (a.by.has(k, j, i)) {
             // rows j-3 .. j+3 once for the faces at j (rows j-3 .. j+2) and j+1 (rows j-2 .. j+3)
@@ -425,7 +434,7 @@ __global__ void __launch_bounds__(PW_TX * PW_TY) k_scalar_adv(Dims D, ScalarTend
           if (a.bx.has(k, j, i)) t = t - (mx * ad.rdx) * (xr - xf);
           t = t - ad.rdzw[k] * (zhi - zlo);
         }
-        a.adv_tend[q] = t;
+        tbase[q] = t;
       }
       zlo = zhi;
     }
@@ -436,7 +445,8 @@ static void pw_launch_scalar_adv(const pywrf_grid_t* g, const ScalarTend& a, pyw
   const Box& sp = a.span;
   if (sp.size() <= 0) return;
   dim3 grid((unsigned)((sp.ni() + SA_OX - 1) / SA_OX), (unsigned)((sp.nj() + PW_TY - 1) / PW_TY),
-            (unsigned)((sp.nk() + SA_KC - 1) / SA_KC * ((a.sce - a.scs + SA_SPW) / SA_SPW)));
+            (unsigned)((sp.nk() + SA_KC - 1) / SA_KC *
+                       (a.nsg1 + (a.scalar2 ? (a.sce2 - a.scs2 + SA_SPW) / SA_SPW : 0))));
   hipLaunchKernelGGL(k_scalar_adv, grid, dim3(PW_TX, PW_TY), 0, (hipStream_t)s, dims_of(g), a);
 }
 
@@ -457,6 +467,8 @@ int pw_advect_scalar_box(const pywrf_grid_t* g, const float* field, float* tende
   ScalarTend t;
   t.adv = a; t.x = x;
   t.scalar = field; t.adv_tend = tendency; t.n3 = 0; t.scs = 0; t.sce = 0; t.accumulate = zero_box ? 0 : 1;
+  t.scalar2 = nullptr; t.adv_tend2 = t.sc_tend2 = nullptr; t.scs2 = 0; t.sce2 = -1; t.moist_step2 = 0;
+  t.nsg1 = 1;
   t.bzero = zero_box ? mkbox(g->kts, g->kte, g->jts, g->jte, g->its, g->ite) : badv;
   t.badv = badv; t.by = by; t.bx = bx;
   t.span = t.bzero;
@@ -516,8 +528,11 @@ __global__ void k_scalar_diff(Box b, Dims D, ScalarTend a) {
   const int q0 = D.ix(k, j, i);
   HCoef c = {};
   if (dh) c = hdiff_coefs(D, a.hd, k, j, i);
-  const bool has_p = a.moist_step && a.scs <= pw::P_QV && pw::P_QV <= a.sce;
-  const bool has_m = !has_p || a.sce > a.scs;
+  // kinds needed: 'p' for the water vapour of a moist array, 'm' for the rest
+  const bool p1 = a.moist_step && a.scs <= pw::P_QV && pw::P_QV <= a.sce;
+  const bool p2 = a.scalar2 && a.moist_step2 && a.scs2 <= pw::P_QV && pw::P_QV <= a.sce2;
+  const bool has_p = p1 || p2;
+  const bool has_m = !p1 || a.sce > a.scs || (a.scalar2 && (!p2 || a.sce2 > a.scs2));
   VFace um = {}, dm = {}, up = {}, dp = {};
   float am = 0.0f, ap = 0.0f;
   if (dv) {
@@ -534,34 +549,44 @@ __global__ void k_scalar_diff(Box b, Dims D, ScalarTend a) {
       ap = gg * m / al * v.rdnw[k];
     }
   }
+  for (int arr = 0; arr < (a.scalar2 ? 2 : 1); ++arr) {
+    const float* const sbase = arr ? a.scalar2 : a.scalar;
+    float* const st_base = arr ? a.sc_tend2 : a.sc_tend;
+    const int scs = arr ? a.scs2 : a.scs, sce = arr ? a.sce2 : a.sce, moist = arr ? a.moist_step2 : a.moist_step;
 #pragma unroll 1
-  for (int im = a.scs; im <= a.sce; ++im) {
-    const float* f = a.scalar + im * a.n3;
-    const int q = im * (int)a.n3 + q0;
-    float st = a.sc_tend[q];
-    if (dh) {
-      auto Fv = [&](int dj, int di) { return AT(f, k, j + dj, i + di); };
-      const float f0 = Fv(0, 0);
-      st = st + (c.mrdx * (c.mkxp * (Fv(0, 1) - f0) - c.mkxm * (f0 - Fv(0, -1))) +
-                 c.mrdy * (c.mkyp * (Fv(1, 0) - f0) - c.mkym * (f0 - Fv(-1, 0))));
+    for (int im = scs; im <= sce; ++im) {
+      const float* f = sbase + im * a.n3;
+      const int q = im * (int)a.n3 + q0;
+      float st = st_base[q];
+      if (dh) {
+        auto Fv = [&](int dj, int di) { return AT(f, k, j + dj, i + di); };
+        const float f0 = Fv(0, 0);
+        st = st + (c.mrdx * (c.mkxp * (Fv(0, 1) - f0) - c.mkxm * (f0 - Fv(0, -1))) +
+                   c.mrdy * (c.mkyp * (Fv(1, 0) - f0) - c.mkym * (f0 - Fv(-1, 0))));
+      }
+      if (dv) {
+        if (moist && im == pw::P_QV)
+          st = st + ap * (vface_flux(D, up, f, a.vd.base1, j, i) - vface_flux(D, dp, f, a.vd.base1, j, i));
+        else
+          st = st + am * (vface_flux(D, um, f, nullptr, j, i) - vface_flux(D, dm, f, nullptr, j, i));
+      }
+      st_base[q] = st;
     }
-    if (dv) {
-      if (a.moist_step && im == pw::P_QV)
-        st = st + ap * (vface_flux(D, up, f, a.vd.base1, j, i) - vface_flux(D, dp, f, a.vd.base1, j, i));
-      else
-        st = st + am * (vface_flux(D, um, f, nullptr, j, i) - vface_flux(D, dm, f, nullptr, j, i));
-    }
-    a.sc_tend[q] = st;
   }
 }
 
+// scalar2 non-null: a second species array [scs2, sce2] (its own tendencies
+// and moist flag) over the same fluxes, index set and diffusion, in the same
+// launches.
 int pw_rk_scalar_tend_fused(const pywrf_grid_t* g, int scs, int sce, int rk_step, const float* ru, const float* rv,
                             const float* ww, const float* mut, const float* alt, const float* scalar,
                             float* scalar_tends, float* advect_tend, float* h_tendency, float* z_tendency,
                             const float* base, int moist_step, const float* fnm, const float* fnp,
                             const float* msfux, const float* msfuy, const float* msfvx, const float* msfvx_inv,
                             const float* msfvy, const float* msftx, const float* msfty, float rdx, float rdy,
-                            const float* rdn, const float* rdnw, float kvdq, const float* xkmhd, pywrf_stream_t s) {
+                            const float* rdn, const float* rdnw, float kvdq, const float* xkmhd, pywrf_stream_t s,
+                            const float* scalar2, float* scalar_tends2, float* advect_tend2, int scs2, int sce2,
+                            int moist_step2) {
   CHECK(g, 4);
   ScalarTend a;
   a.x = {g->ids, g->ide, g->jds, g->jde, imin(g->kte, g->kde - 1)};
@@ -573,6 +598,12 @@ int pw_rk_scalar_tend_fused(const pywrf_grid_t* g, int scs, int sce, int rk_step
   a.n3 = (long)g->nz * g->ny * g->nx;
   if ((long)(sce + 1) * a.n3 >= (1L << 31)) return PYWRF_E_RANGE;   // species offsets are 32-bit
   a.scs = scs; a.sce = sce; a.rk1 = rk_step == 1; a.moist_step = moist_step; a.accumulate = 0;
+  a.nsg1 = (sce - scs + SA_SPW) / SA_SPW;
+  a.scalar2 = scalar2; a.sc_tend2 = scalar_tends2; a.adv_tend2 = advect_tend2;
+  a.scs2 = scs2; a.sce2 = sce2; a.moist_step2 = moist_step2;
+  if (scalar2 && (scs2 < 0 || sce2 < scs2 || (long)(sce2 + 1) * a.n3 >= (1L << 31) || !scalar_tends2 ||
+                  !advect_tend2))
+    return PYWRF_E_RANGE;
   int ktf = a.x.ktf, jtf = imin(g->jte, g->jde - 1), itf = imin(g->ite, g->ide - 1);
   a.bzero = mkbox(g->kts, g->kte, g->jts, g->jte, g->its, g->ite);
   a.badv = mkbox(g->kts, ktf, g->jts, jtf, g->its, itf);

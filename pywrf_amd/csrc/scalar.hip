@@ -15,7 +15,9 @@ int pw_rk_scalar_tend_fused(const pywrf_grid_t* g, int scs, int sce, int rk_step
                             const float* base, int moist_step, const float* fnm, const float* fnp,
                             const float* msfux, const float* msfuy, const float* msfvx, const float* msfvx_inv,
                             const float* msfvy, const float* msftx, const float* msfty, float rdx, float rdy,
-                            const float* rdn, const float* rdnw, float kvdq, const float* xkmhd, pywrf_stream_t s);
+                            const float* rdn, const float* rdnw, float kvdq, const float* xkmhd, pywrf_stream_t s,
+                            const float* scalar2 = nullptr, float* scalar_tends2 = nullptr,
+                            float* advect_tend2 = nullptr, int scs2 = 0, int sce2 = -1, int moist_step2 = 0);
 
 // Fill kernel used instead of hipMemsetAsync so that the calls stay safe under
 // stream capture.
@@ -89,6 +91,25 @@ extern "C" int pywrf_rk_scalar_tend(const pywrf_grid_t* g, int scs, int sce, int
                                  msfvy, msftx, msfty, F(rdx), F(rdy), rdn, rdnw, F(kvdif / pw::prandtl), xkmhd, s);
 }
 
+extern "C" int pywrf_rk_scalar_tend_pair(const pywrf_grid_t* g, int rk_step, int scs, int sce, const float* scalar,
+                                         float* scalar_tends, float* advect_tend, int moist_step, int scs2,
+                                         int sce2, const float* scalar2, float* scalar_tends2, float* advect_tend2,
+                                         int moist_step2, const float* ru, const float* rv, const float* ww,
+                                         const float* mut, const float* alt, float* h_tendency, float* z_tendency,
+                                         const float* base, const float* fnm, const float* fnp, const float* msfux,
+                                         const float* msfuy, const float* msfvx, const float* msfvx_inv,
+                                         const float* msfvy, const float* msftx, const float* msfty, double rdx,
+                                         double rdy, const float* rdn, const float* rdnw, double kvdif,
+                                         const float* xkmhd, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
+  if (!scalar2 || scalar2 == scalar || scalar_tends2 == scalar_tends || advect_tend2 == advect_tend)
+    return PYWRF_E_ARG;
+  return pw_rk_scalar_tend_fused(g, scs, sce, rk_step, ru, rv, ww, mut, alt, scalar, scalar_tends, advect_tend,
+                                 h_tendency, z_tendency, base, moist_step, fnm, fnp, msfux, msfuy, msfvx, msfvx_inv,
+                                 msfvy, msftx, msfty, F(rdx), F(rdy), rdn, rdnw, F(kvdif / pw::prandtl), xkmhd, s,
+                                 scalar2, scalar_tends2, advect_tend2, scs2, sce2, moist_step2);
+}
+
 // ------------------------------------------------------------------ rk_update_scalar
 // All species of [scs, sce] in one pass; muold / munew formed once per point.
 struct UpdS {
@@ -97,6 +118,10 @@ struct UpdS {
   long n3;
   float dt;
   int rk1, scs, sce;
+  // a second species array over the same mass and index set (s1b null: none)
+  float *s1b, *s2b;
+  const float *sc_tend_b, *adv_b;
+  int scs2, sce2;
   Box bs, bi, bf;
   int jf0, jf1, if0, if1;  // rows / columns where muold and munew are set
 };
@@ -116,35 +141,45 @@ __global__ void k_rk_update_scalar(Box all, Dims D, UpdS a) {
   const float my = (inf && ini) ? msf : 0.0f;
   // species in chunks of UPD_CHUNK: every load of a chunk is issued before its
   // stores (the arrays are distinct, which the compiler cannot assume)
-  for (int m0 = a.scs; m0 <= a.sce; m0 += UPD_CHUNK) {
-    float v1[UPD_CHUNK], va[UPD_CHUNK], vt[UPD_CHUNK];
+  for (int arr = 0; arr < (a.s1b ? 2 : 1); ++arr) {
+    float* const s1 = arr ? a.s1b : a.s1;
+    float* const s2 = arr ? a.s2b : a.s2;
+    const float* const adv = arr ? a.adv_b : a.adv;
+    const float* const sct = arr ? a.sc_tend_b : a.sc_tend;
+    const int scs = arr ? a.scs2 : a.scs, sce = arr ? a.sce2 : a.sce;
+    for (int m0 = scs; m0 <= sce; m0 += UPD_CHUNK) {
+      float v1[UPD_CHUNK], va[UPD_CHUNK], vt[UPD_CHUNK];
 #pragma unroll
-    for (int m = 0; m < UPD_CHUNK; ++m) {
-      if (m0 + m > a.sce) break;
-      const long qq = q + (m0 + m) * a.n3;
-      v1[m] = a.rk1 ? a.s2[qq] + 0.0f : a.s1[qq];   // rk1 is uniform
-      const float av = a.adv[qq], tv = a.sc_tend[qq];
-      va[m] = (inf && ini) ? av : 0.0f;
-      vt[m] = (inf && ins) ? tv : 0.0f;
-    }
+      for (int m = 0; m < UPD_CHUNK; ++m) {
+        if (m0 + m > sce) break;
+        const long qq = q + (m0 + m) * a.n3;
+        v1[m] = a.rk1 ? s2[qq] + 0.0f : s1[qq];   // rk1 is uniform
+        const float av = adv[qq], tv = sct[qq];
+        va[m] = (inf && ini) ? av : 0.0f;
+        vt[m] = (inf && ins) ? tv : 0.0f;
+      }
 #pragma unroll
-    for (int m = 0; m < UPD_CHUNK; ++m) {
-      if (m0 + m > a.sce) break;
-      const long qq = q + (m0 + m) * a.n3;
-      if (a.rk1) a.s1[qq] = v1[m];
-      if (!inf) continue;
-      float t = 0.0f;
-      if (ini) t = va[m] * my;
-      if (ins) t = t + vt[m];
-      a.s2[qq] = (muold * v1[m] + a.dt * t) / munew;
+      for (int m = 0; m < UPD_CHUNK; ++m) {
+        if (m0 + m > sce) break;
+        const long qq = q + (m0 + m) * a.n3;
+        if (a.rk1) s1[qq] = v1[m];
+        if (!inf) continue;
+        float t = 0.0f;
+        if (ini) t = va[m] * my;
+        if (ins) t = t + vt[m];
+        s2[qq] = (muold * v1[m] + a.dt * t) / munew;
+      }
     }
   }
 }
 
-extern "C" int pywrf_rk_update_scalar(const pywrf_grid_t* g, int scs, int sce, float* scalar_1, float* scalar_2,
-                                      const float* sc_tend, const float* advect_tend, const float* msfty,
-                                      const float* mu_old, const float* mu_new, const float* mu_base, int rk_step,
-                                      double dt, int spec_zone, pywrf_stream_t s) {
+// scalar_1b non-null: a second species array [scs2, sce2] with its own
+// tendencies over the same mass, map factors and index set, in the same launch.
+static int rk_update_scalar(const pywrf_grid_t* g, int scs, int sce, float* scalar_1, float* scalar_2,
+                            const float* sc_tend, const float* advect_tend, const float* msfty, const float* mu_old,
+                            const float* mu_new, const float* mu_base, int rk_step, double dt, int spec_zone,
+                            pywrf_stream_t s, float* scalar_1b, float* scalar_2b, const float* sc_tend_b,
+                            const float* advect_tend_b, int scs2, int sce2) {
   if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 0);
   if (g->kte > g->nz || g->kde > g->nz || scs < 0 || sce < scs) return PYWRF_E_RANGE;
@@ -152,6 +187,9 @@ extern "C" int pywrf_rk_update_scalar(const pywrf_grid_t* g, int scs, int sce, f
   a.s1 = scalar_1; a.s2 = scalar_2; a.sc_tend = sc_tend; a.adv = advect_tend;
   a.mu_old = mu_old; a.mu_new = mu_new; a.mub = mu_base; a.msfty = msfty; a.dt = F(dt); a.rk1 = rk_step == 1;
   a.n3 = (long)g->nz * g->ny * g->nx; a.scs = scs; a.sce = sce;
+  a.s1b = scalar_1b; a.s2b = scalar_2b; a.sc_tend_b = sc_tend_b; a.adv_b = advect_tend_b;
+  a.scs2 = scs2; a.sce2 = sce2;
+  if (scalar_1b && (scs2 < 0 || sce2 < scs2 || !scalar_2b || !sc_tend_b || !advect_tend_b)) return PYWRF_E_RANGE;
   a.bs = mkbox(g->kts, g->kte - 1, g->jts, imin(g->jte, g->jde - 1), g->its, imin(g->ite, g->ide - 1));
   a.bi = mkbox(g->kts, imin(g->kte, g->kde - 1), imax(g->jts, g->jds + spec_zone), imin(g->jte, g->jde - spec_zone - 1),
                imax(g->its, g->ids + spec_zone), imin(g->ite, g->ide - spec_zone - 1));
@@ -160,6 +198,25 @@ extern "C" int pywrf_rk_update_scalar(const pywrf_grid_t* g, int scs, int sce, f
   Box b = a.rk1 ? mkbox(0, g->nz, 0, g->ny, 0, g->nx) : a.bf;
   PW_LAUNCH_BOX(k_rk_update_scalar, b, s, dims_of(g), a);
   PW_RET();
+}
+
+extern "C" int pywrf_rk_update_scalar(const pywrf_grid_t* g, int scs, int sce, float* scalar_1, float* scalar_2,
+                                      const float* sc_tend, const float* advect_tend, const float* msfty,
+                                      const float* mu_old, const float* mu_new, const float* mu_base, int rk_step,
+                                      double dt, int spec_zone, pywrf_stream_t s) {
+  return rk_update_scalar(g, scs, sce, scalar_1, scalar_2, sc_tend, advect_tend, msfty, mu_old, mu_new, mu_base,
+                          rk_step, dt, spec_zone, s, nullptr, nullptr, nullptr, nullptr, 0, -1);
+}
+
+extern "C" int pywrf_rk_update_scalar_pair(const pywrf_grid_t* g, int rk_step, int scs, int sce, float* scalar_1,
+                                           float* scalar_2, const float* sc_tend, const float* advect_tend,
+                                           int scs2, int sce2, float* scalar_1b, float* scalar_2b,
+                                           const float* sc_tend_b, const float* advect_tend_b, const float* msfty,
+                                           const float* mu_old, const float* mu_new, const float* mu_base,
+                                           double dt, int spec_zone, pywrf_stream_t s) {
+  if (!scalar_1b || scalar_1b == scalar_1 || scalar_2b == scalar_2) return PYWRF_E_ARG;
+  return rk_update_scalar(g, scs, sce, scalar_1, scalar_2, sc_tend, advect_tend, msfty, mu_old, mu_new, mu_base,
+                          rk_step, dt, spec_zone, s, scalar_1b, scalar_2b, sc_tend_b, advect_tend_b, scs2, sce2);
 }
 
 // ------------------------------------------------------------------ calc_p_rho_phi

@@ -182,21 +182,26 @@ __global__ void k_calc_p_rho(Box all, Dims D, Box b, PRho a) {
   int k, j, i;
   if (!box_point(all, k, j, i)) return;
   long q = D.ix(k, j, i);
-  float al = a.al_in[q], p = a.p_in[q];
-  if (b.has(k, j, i)) {
-    float mut = AT2(a.mut, j, i), mu = AT2(a.mu, j, i);
-    al = -1.0f / mut * (a.alt[q] * mu + a.rdnw[k] * (AT(a.ph, k + 1, j, i) - a.ph[q]));
-    float t1 = a.t_1[q];
-    p = a.c2a[q] * (a.alt[q] * (a.t_2[q] - mu * t1) / (mut * (a.t0 + t1)) - al);
+  // every operand is loaded before the branch (a load under a branch is waited
+  // for before the branch ends); the level above is clamped into the array,
+  // its value being used only inside the box, where k + 1 is a level
+  const float al_in = a.al_in[q], p_in = a.p_in[q], ph0 = a.ph[q], ph1 = AT(a.ph, imin(k + 1, D.nz - 1), j, i);
+  const float altq = a.alt[q], t1 = a.t_1[q], t2 = a.t_2[q], c2 = a.c2a[q], pm = a.pm1[q];
+  const float mut = AT2(a.mut, j, i), mu = AT2(a.mu, j, i), rd = a.rdnw[k];
+  float al = al_in, p = p_in, pmo = 0.0f;
+  const bool in = b.has(k, j, i);
+  if (in) {
+    al = -1.0f / mut * (altq * mu + rd * (ph1 - ph0));
+    p = c2 * (altq * (t2 - mu * t1) / (mut * (a.t0 + t1)) - al);
     if (a.step != 0) {
-      float ptmp = p + 0.0f;
-      p = p + a.smdiv * (p - a.pm1[q]);
-      a.pm1[q] = ptmp;
+      pmo = p + 0.0f;
+      p = p + a.smdiv * (p - pm);
     }
   }
   a.al[q] = al;
   a.p[q] = p;
   if (a.step == 0) a.pm1[q] = p + 0.0f;
+  else if (in) a.pm1[q] = pmo;
 }
 
 extern "C" int pywrf_calc_p_rho(const pywrf_grid_t* g, const float* al_in, const float* p_in, const float* ph,

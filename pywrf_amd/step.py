@@ -340,31 +340,58 @@ def rk3_step(s, ops, cfg, check_nan=False):
         # moist and scalar transport (solver.py:1548-1636)
         sct = (rdx, rdy, s.rdn, s.rdnw, KHDIF, KVDIF, s.xkhh, DIFF_6TH_OPT, DIFF_6TH_FACTOR)
         mf = (s.msfux, s.msfuy, s.msfvx, s.msfvx_inv, s.msfvy, s.msftx, s.msfty)
-        s.moist_tend, s.advect_tend, s.h_tendency, s.z_tendency = o.rk_scalar_tend(
-            1, 6, False, rk_step, dt_rk, s.ru_m, s.rv_m, s.ww_m, s.muts, s.mub, s.mu_1, s.c1h, s.c2h, s.alt,
-            s.moist_old, s.moist, s.moist_tend, s.advect_tend, s.h_tendency, s.z_tendency, s.RQVFTEN, s.qv_base,
-            True, s.fnm, s.fnp, *mf, *sct, MOIST_ADV_OPT, *I18)
-        if rk_step == 1:
-            s.moist_tend[P_QV, :, :, :] = o.relax_bdy_scalar(
-                s.moist_tend[P_QV, :, :, :], s.moist[P_QV, :, :, :], s.mut, s.c1h, s.c2h, *b['qv'],
-                SPEC_BDY_WIDTH, SPEC_ZONE, RELAX_ZONE, dtbc, s.fcx, s.gcx, *I24)
-            s.moist_tend[P_QV, :, :, :] = o.spec_bdy_scalar(s.moist_tend[P_QV, :, :, :], *b['qv'], SPEC_BDY_WIDTH,
-                                                           SPEC_ZONE, *I24)
-        s.moist_old, s.moist = o.rk_update_scalar(1, 6, s.moist_old, s.moist, s.moist_tend, s.advect_tend,
-                                                  s.h_tendency, s.z_tendency, s.msftx, s.msfty, s.c1h, s.c2h,
-                                                  s.mu_1, s.mu, s.mub, rk_step, dt_rk, SPEC_ZONE, False, *I18)
-        if hasattr(o, "flow_dep_bdy_group"):
+        paired = hasattr(o, "rk_scalar_tend_pair") and hasattr(o, "rk_update_scalar_pair")
+        if paired:
+            # the moist and the tracer calls in the same launches (they share the
+            # fluxes, the index set and h/z_tendency, which both only zero); the
+            # water-vapour boundary terms come between tendency and update as in
+            # the reference, and the tracer calls touch neither
+            (s.moist_tend, s.advect_tend, s.h_tendency, s.z_tendency), (s.scalar_tend, s.advect_tend_sca, _h, _z) = (
+                o.rk_scalar_tend_pair(
+                    (1, 6, False, rk_step, dt_rk, s.ru_m, s.rv_m, s.ww_m, s.muts, s.mub, s.mu_1, s.c1h, s.c2h, s.alt,
+                     s.moist_old, s.moist, s.moist_tend, s.advect_tend, s.h_tendency, s.z_tendency, s.RQVFTEN,
+                     s.qv_base, True, s.fnm, s.fnp, *mf, *sct, MOIST_ADV_OPT, *I18),
+                    (1, 2, False, rk_step, dt_rk, s.ru_m, s.rv_m, s.ww_m, s.muts, s.mub, s.mu_1, s.c1h, s.c2h, s.alt,
+                     s.scalar_old, s.scalar, s.scalar_tend, s.advect_tend_sca, s.h_tendency, s.z_tendency,
+                     s.RQVFTEN, s.qv_base, False, s.fnm, s.fnp, *mf, *sct, SCALAR_ADV_OPT, *I18)))
+            if rk_step == 1:
+                s.moist_tend[P_QV, :, :, :] = o.relax_bdy_scalar(
+                    s.moist_tend[P_QV, :, :, :], s.moist[P_QV, :, :, :], s.mut, s.c1h, s.c2h, *b['qv'],
+                    SPEC_BDY_WIDTH, SPEC_ZONE, RELAX_ZONE, dtbc, s.fcx, s.gcx, *I24)
+                s.moist_tend[P_QV, :, :, :] = o.spec_bdy_scalar(s.moist_tend[P_QV, :, :, :], *b['qv'],
+                                                               SPEC_BDY_WIDTH, SPEC_ZONE, *I24)
+            (s.moist_old, s.moist), (s.scalar_old, s.scalar) = o.rk_update_scalar_pair(
+                (1, 6, s.moist_old, s.moist, s.moist_tend, s.advect_tend, s.h_tendency, s.z_tendency, s.msftx,
+                 s.msfty, s.c1h, s.c2h, s.mu_1, s.mu, s.mub, rk_step, dt_rk, SPEC_ZONE, False, *I18),
+                (1, 2, s.scalar_old, s.scalar, s.scalar_tend, s.advect_tend_sca, s.h_tendency, s.z_tendency,
+                 s.msftx, s.msfty, s.c1h, s.c2h, s.mu_1, s.mu, s.mub, rk_step, dt_rk, SPEC_ZONE, False, *I18))
             o.flow_dep_bdy_group([s.moist[im] for im in range(2, 7)], s.ru_m, s.rv_m, I24)
         else:
-            for im in range(2, 7):
-                s.moist[im, :, :, :] = o.flow_dep_bdy(s.moist[im, :, :, :], s.ru_m, s.rv_m, 1, *I24)
-        s.scalar_tend, s.advect_tend_sca, s.h_tendency, s.z_tendency = o.rk_scalar_tend(
-            1, 2, False, rk_step, dt_rk, s.ru_m, s.rv_m, s.ww_m, s.muts, s.mub, s.mu_1, s.c1h, s.c2h, s.alt,
-            s.scalar_old, s.scalar, s.scalar_tend, s.advect_tend_sca, s.h_tendency, s.z_tendency, s.RQVFTEN,
-            s.qv_base, False, s.fnm, s.fnp, *mf, *sct, SCALAR_ADV_OPT, *I18)
-        s.scalar_old, s.scalar = o.rk_update_scalar(1, 2, s.scalar_old, s.scalar, s.scalar_tend, s.advect_tend_sca,
-                                                    s.h_tendency, s.z_tendency, s.msftx, s.msfty, s.c1h, s.c2h,
-                                                    s.mu_1, s.mu, s.mub, rk_step, dt_rk, SPEC_ZONE, False, *I18)
+            s.moist_tend, s.advect_tend, s.h_tendency, s.z_tendency = o.rk_scalar_tend(
+                1, 6, False, rk_step, dt_rk, s.ru_m, s.rv_m, s.ww_m, s.muts, s.mub, s.mu_1, s.c1h, s.c2h, s.alt,
+                s.moist_old, s.moist, s.moist_tend, s.advect_tend, s.h_tendency, s.z_tendency, s.RQVFTEN, s.qv_base,
+                True, s.fnm, s.fnp, *mf, *sct, MOIST_ADV_OPT, *I18)
+            if rk_step == 1:
+                s.moist_tend[P_QV, :, :, :] = o.relax_bdy_scalar(
+                    s.moist_tend[P_QV, :, :, :], s.moist[P_QV, :, :, :], s.mut, s.c1h, s.c2h, *b['qv'],
+                    SPEC_BDY_WIDTH, SPEC_ZONE, RELAX_ZONE, dtbc, s.fcx, s.gcx, *I24)
+                s.moist_tend[P_QV, :, :, :] = o.spec_bdy_scalar(s.moist_tend[P_QV, :, :, :], *b['qv'], SPEC_BDY_WIDTH,
+                                                               SPEC_ZONE, *I24)
+            s.moist_old, s.moist = o.rk_update_scalar(1, 6, s.moist_old, s.moist, s.moist_tend, s.advect_tend,
+                                                      s.h_tendency, s.z_tendency, s.msftx, s.msfty, s.c1h, s.c2h,
+                                                      s.mu_1, s.mu, s.mub, rk_step, dt_rk, SPEC_ZONE, False, *I18)
+            if hasattr(o, "flow_dep_bdy_group"):
+                o.flow_dep_bdy_group([s.moist[im] for im in range(2, 7)], s.ru_m, s.rv_m, I24)
+            else:
+                for im in range(2, 7):
+                    s.moist[im, :, :, :] = o.flow_dep_bdy(s.moist[im, :, :, :], s.ru_m, s.rv_m, 1, *I24)
+            s.scalar_tend, s.advect_tend_sca, s.h_tendency, s.z_tendency = o.rk_scalar_tend(
+                1, 2, False, rk_step, dt_rk, s.ru_m, s.rv_m, s.ww_m, s.muts, s.mub, s.mu_1, s.c1h, s.c2h, s.alt,
+                s.scalar_old, s.scalar, s.scalar_tend, s.advect_tend_sca, s.h_tendency, s.z_tendency, s.RQVFTEN,
+                s.qv_base, False, s.fnm, s.fnp, *mf, *sct, SCALAR_ADV_OPT, *I18)
+            s.scalar_old, s.scalar = o.rk_update_scalar(1, 2, s.scalar_old, s.scalar, s.scalar_tend, s.advect_tend_sca,
+                                                        s.h_tendency, s.z_tendency, s.msftx, s.msfty, s.c1h, s.c2h,
+                                                        s.mu_1, s.mu, s.mub, rk_step, dt_rk, SPEC_ZONE, False, *I18)
         # EOS (solver.py:1639-1654)
         s.al, s.p, s.ph = o.calc_p_rho_phi(s.moist, N_MOIST, HYPSOMETRIC_OPT, s.al, s.alb, s.mu, s.muts, s.c1h,
                                            s.c2h, s.c3h, s.c4h, s.c3f, s.c4f, s.ph, s.phb, s.p, s.pb, s.t, P0, T0,

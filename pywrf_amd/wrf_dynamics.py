@@ -11,6 +11,9 @@ torch's current stream; nothing here synchronises the device.
 Tensors are contiguous FP32 CUDA tensors, 3-D (nz, ny, nx), 2-D (ny, nx),
 level arrays (nz,); 0-d tensors for cf1..cf3, cfn, cfn1.
 """
+import inspect
+from types import SimpleNamespace
+
 import torch
 
 from ._lib import call, grid, ptr, flag, fresh, stream
@@ -767,6 +770,48 @@ def rk_scalar_tend(scs, sce,
     return scalar_tends, advect_tend, h_tendency, z_tendency
 
 
+def _bind(fn, args):
+    """The positional arguments of a call of `fn`, by parameter name."""
+    names = list(inspect.signature(fn).parameters)
+    if len(args) != len(names):
+        raise TypeError("%s takes %d positional arguments, got %d" % (fn.__name__, len(names), len(args)))
+    return SimpleNamespace(**dict(zip(names, args)))
+
+
+def _shared(fn, a, b, own):
+    """Raise unless the two bound calls of `fn` agree on every argument outside `own`."""
+    for n in inspect.signature(fn).parameters:
+        if n in own:
+            continue
+        x, y = getattr(a, n), getattr(b, n)
+        same = x is y if isinstance(x, torch.Tensor) or isinstance(y, torch.Tensor) else x == y
+        if not same:
+            raise ValueError("%s pair: argument %r differs between the two calls" % (fn.__name__, n))
+
+
+def rk_scalar_tend_pair(args1, args2):
+    """Two rk_scalar_tend calls, each given as the tuple of its positional
+    arguments, that share every argument but the species range, the species
+    arrays, the two tendency arrays, the moist flag and the advection option
+    (the moist and the tracer call of solver.py:1548-1610), in the same
+    launches.  Returns the two calls' results.  Not part of the reference API."""
+    a, b = _bind(rk_scalar_tend, args1), _bind(rk_scalar_tend, args2)
+    _shared(rk_scalar_tend, a, b, ("scs", "sce", "scalar_old", "scalar", "scalar_tends", "advect_tend",
+                                   "moist_step", "adv_opt"))
+    if a.scalar is b.scalar or a.scalar_tends is b.scalar_tends or a.advect_tend is b.advect_tend:
+        raise ValueError("rk_scalar_tend pair: the two calls must have distinct species and tendency arrays")
+    g = grid(a.ru.shape, (a.ids, a.ide, a.jds, a.jde, a.kds, a.kde, a.ims, a.ime, a.jms, a.jme, a.kms, a.kme,
+                          a.its, a.ite, a.jts, a.jte, a.kts, a.kte))
+    call("pywrf_rk_scalar_tend_pair", g, int(a.rk_step), int(a.scs), int(a.sce), ptr(a.scalar), ptr(a.scalar_tends),
+         ptr(a.advect_tend), int(bool(a.moist_step)), int(b.scs), int(b.sce), ptr(b.scalar), ptr(b.scalar_tends),
+         ptr(b.advect_tend), int(bool(b.moist_step)), ptr(a.ru), ptr(a.rv), ptr(a.ww), ptr(a.mut), ptr(a.alt),
+         ptr(a.h_tendency), ptr(a.z_tendency), ptr(a.base), ptr(a.fnm), ptr(a.fnp), ptr(a.msfux), ptr(a.msfuy),
+         ptr(a.msfvx), ptr(a.msfvx_inv), ptr(a.msfvy), ptr(a.msftx), ptr(a.msfty), float(a.rdx), float(a.rdy),
+         ptr(a.rdn), ptr(a.rdnw), float(a.kvdif), ptr(a.xkmhd), stream())
+    return ((a.scalar_tends, a.advect_tend, a.h_tendency, a.z_tendency),
+            (b.scalar_tends, b.advect_tend, b.h_tendency, b.z_tendency))
+
+
 def rk_update_scalar(scs, sce,
                      scalar_1, scalar_2, sc_tend,
                      advect_tend,
@@ -785,6 +830,27 @@ def rk_update_scalar(scs, sce,
          ptr(advect_tend), ptr(msfty), ptr(mu_old), ptr(mu_new), ptr(mu_base), int(rk_step), float(dt),
          int(spec_zone), stream())
     return scalar_1, scalar_2
+
+
+def rk_update_scalar_pair(args1, args2):
+    """Two rk_update_scalar calls, each given as the tuple of its positional
+    arguments, that share every argument but the species range, the two
+    species arrays and their tendencies (the moist and the tracer update of
+    solver.py:1548-1636), in one launch.  Returns the two calls' results.  Not
+    part of the reference API."""
+    a, b = _bind(rk_update_scalar, args1), _bind(rk_update_scalar, args2)
+    _shared(rk_update_scalar, a, b, ("scs", "sce", "scalar_1", "scalar_2", "sc_tend", "advect_tend"))
+    if a.scalar_1 is b.scalar_1 or a.scalar_2 is b.scalar_2:
+        raise ValueError("rk_update_scalar pair: the two calls must have distinct species arrays")
+    if a.scalar_2.shape[1:] != b.scalar_2.shape[1:]:
+        raise ValueError("rk_update_scalar pair: the two species arrays differ in shape per species")
+    g = grid(a.scalar_2.shape[1:], (a.ids, a.ide, a.jds, a.jde, a.kds, a.kde, a.ims, a.ime, a.jms, a.jme, a.kms,
+                                    a.kme, a.its, a.ite, a.jts, a.jte, a.kts, a.kte))
+    call("pywrf_rk_update_scalar_pair", g, int(a.rk_step), int(a.scs), int(a.sce), ptr(a.scalar_1), ptr(a.scalar_2),
+         ptr(a.sc_tend), ptr(a.advect_tend), int(b.scs), int(b.sce), ptr(b.scalar_1), ptr(b.scalar_2),
+         ptr(b.sc_tend), ptr(b.advect_tend), ptr(a.msfty), ptr(a.mu_old), ptr(a.mu_new), ptr(a.mu_base),
+         float(a.dt), int(a.spec_zone), stream())
+    return (a.scalar_1, a.scalar_2), (b.scalar_1, b.scalar_2)
 
 
 def calc_p_rho_phi(moist, n_moist, hypsometric_opt,

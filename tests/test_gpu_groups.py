@@ -155,3 +155,76 @@ def test_spec_bdy_final_group_equals_individual_calls(n):
     for r, (f, _, _, _, v) in zip(ref, items):
         assert torch.equal(r, f), v
     assert torch.equal(q[1:], q0[1:])
+
+
+def _transport_args(s, rk_step, moist, arrs):
+    from pywrf_amd.step import (RunConfig, KHDIF, DIFF_6TH_OPT, DIFF_6TH_FACTOR, MOIST_ADV_OPT,
+                                SCALAR_ADV_OPT)
+    cfg = RunConfig()
+    old, cur, tend, adv, h, z = arrs
+    mf = (s.msfux, s.msfuy, s.msfvx, s.msfvx_inv, s.msfvy, s.msftx, s.msfty)
+    sct = (cfg.rdx, cfg.rdy, s.rdn, s.rdnw, KHDIF, 5.0, s.xkhh, DIFF_6TH_OPT, DIFF_6TH_FACTOR)
+    n = 6 if moist else 2
+    return (1, n, False, rk_step, cfg.dt / 3, s.ru_m, s.rv_m, s.ww_m, s.muts, s.mub, s.mu_1, s.c1h, s.c2h, s.alt,
+            old, cur, tend, adv, h, z, s.RQVFTEN, s.qv_base, moist, s.fnm, s.fnp, *mf, *sct,
+            MOIST_ADV_OPT if moist else SCALAR_ADV_OPT, *s.grid.ix18)
+
+
+def _update_args(s, rk_step, n, arrs, h, z):
+    s1, s2, tend, adv = arrs
+    return (1, n, s1, s2, tend, adv, h, z, s.msftx, s.msfty, s.c1h, s.c2h, s.mu_1, s.mu, s.mub, rk_step, 18.0, 1,
+            False, *s.grid.ix18)
+
+
+@pytest.mark.parametrize("rk_step", [1, 2])
+def test_transport_pairs_equal_the_two_calls(rk_step):
+    """rk_scalar_tend_pair and rk_update_scalar_pair against the moist call
+    followed by the tracer call (bit for bit), on the state after one step,
+    with random tracer species and tendencies and a nonzero vertical
+    diffusion coefficient."""
+    s, o = _stepped()
+    torch.manual_seed(9)
+    sc = torch.rand_like(s.scalar)
+    base = {"m": [s.moist_old, s.moist, torch.randn_like(s.moist), torch.randn_like(s.moist)],
+            "s": [torch.rand_like(s.scalar), sc, torch.randn_like(sc), torch.randn_like(sc)]}
+    hz = [torch.randn_like(s.h_tendency), torch.randn_like(s.z_tendency)]
+    ref = {k: [x.clone() for x in v] for k, v in base.items()}
+    got = {k: [x.clone() for x in v] for k, v in base.items()}
+    rh, gh = [x.clone() for x in hz], [x.clone() for x in hz]
+    o.rk_scalar_tend(*_transport_args(s, rk_step, True, ref["m"] + rh))
+    o.rk_scalar_tend(*_transport_args(s, rk_step, False, ref["s"] + rh))
+    o.rk_scalar_tend_pair(_transport_args(s, rk_step, True, got["m"] + gh),
+                          _transport_args(s, rk_step, False, got["s"] + gh))
+    torch.cuda.synchronize()
+    for k in ("m", "s"):
+        for n, (a, b) in enumerate(zip(ref[k], got[k])):
+            assert torch.equal(a, b), (k, n)
+    assert torch.equal(rh[0], gh[0]) and torch.equal(rh[1], gh[1])
+    o.rk_update_scalar(*_update_args(s, rk_step, 6, ref["m"], *rh))
+    o.rk_update_scalar(*_update_args(s, rk_step, 2, ref["s"], *rh))
+    o.rk_update_scalar_pair(_update_args(s, rk_step, 6, got["m"], *gh), _update_args(s, rk_step, 2, got["s"], *gh))
+    torch.cuda.synchronize()
+    for k in ("m", "s"):
+        for n, (a, b) in enumerate(zip(ref[k], got[k])):
+            assert torch.equal(a, b), (k, n)
+
+
+def test_transport_pair_refuses_calls_that_differ_in_shared_arguments():
+    s, o = _stepped()
+    a = _transport_args(s, 2, True, [s.moist_old, s.moist, s.moist_tend.clone(), s.advect_tend.clone(),
+                                     s.h_tendency, s.z_tendency])
+    b = list(_transport_args(s, 2, False, [s.scalar_old, s.scalar, s.scalar_tend.clone(),
+                                           s.advect_tend_sca.clone(), s.h_tendency, s.z_tendency]))
+    b[5] = s.ru_m.clone()                          # a different ru
+    with pytest.raises(ValueError):
+        o.rk_scalar_tend_pair(a, tuple(b))
+
+
+def _stepped(config="config1c"):
+    import pywrf_amd
+    from parity import state, run
+    o = pywrf_amd.ops()
+    s = state(config, device="cuda")
+    run(s, o, 1)
+    torch.cuda.synchronize()
+    return s, o
