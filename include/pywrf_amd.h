@@ -144,6 +144,29 @@ int pywrf_rhs_ph(const pywrf_grid_t* g, float* ph_tend, const float* u, const fl
                  const float* muvf, const float* fnm, const float* fnp, const float* rdnw, const float* cfn,
                  const float* cfn1, double rdx, double rdy, const float* msfux, const float* msfvy,
                  const float* msfty, pywrf_stream_t s);
+/* The five calls above into a tendency array the caller would have zeroed
+ * first (solver.py:971-975 zero ru, rv, rw, t and ph_tend before the calls
+ * of :987-1036): the whole array is written, 0 + the tendency inside the
+ * call's index box and 0 elsewhere, and the array is not read.  Same
+ * arguments and results as zeroing and then making the call.  No counterpart
+ * in the reference: the zeroing folded into the first writer. */
+int pywrf_advect_u_fresh(const pywrf_grid_t* g, const float* u, float* tendency, const float* ru, const float* rv,
+                         const float* rom, const float* msfux, const float* fzm, const float* fzp, double rdx,
+                         double rdy, const float* rdzw, pywrf_stream_t s);
+int pywrf_advect_v_fresh(const pywrf_grid_t* g, const float* v, float* tendency, const float* ru, const float* rv,
+                         const float* rom, const float* msfvx, const float* msfvy, const float* fzm,
+                         const float* fzp, double rdx, double rdy, const float* rdzw, pywrf_stream_t s);
+int pywrf_advect_w_fresh(const pywrf_grid_t* g, const float* w, float* tendency, const float* ru, const float* rv,
+                         const float* rom, const float* msftx, const float* fzm, const float* fzp, double rdx,
+                         double rdy, const float* rdzu, pywrf_stream_t s);
+int pywrf_advect_scalar_fresh(const pywrf_grid_t* g, const float* field, float* tendency, const float* ru,
+                              const float* rv, const float* rom, const float* msftx, const float* fzm,
+                              const float* fzp, double rdx, double rdy, const float* rdzw, pywrf_stream_t s);
+int pywrf_rhs_ph_fresh(const pywrf_grid_t* g, float* ph_tend, const float* u, const float* v, const float* ww,
+                       const float* ph, const float* phb, const float* w, const float* mut, const float* muuf,
+                       const float* muvf, const float* fnm, const float* fnp, const float* rdnw, const float* cfn,
+                       const float* cfn1, double rdx, double rdy, const float* msfux, const float* msfvy,
+                       const float* msfty, pywrf_stream_t s);
 /* horizontal_pressure_gradient, wrf_dynamics.py:2300-2421 (cf: cf1, cf2, cf3 as 3 floats) */
 int pywrf_horizontal_pressure_gradient(const pywrf_grid_t* g, float* ru_tend, float* rv_tend, const float* ph,
                                        const float* alt, const float* p, const float* pb, const float* al,

@@ -84,11 +84,18 @@ __device__ inline float u_zface(const Dims& D, const AdvU& a, const AdvIdx& x, i
   return vel * flux3(U(-2), U(-1), U(0), U(1), -vel);  // kf == 2 or ktf-2
 }
 
-__global__ void k_advect_u(Box b, Dims D, AdvU a, AdvIdx x, Box by, Box bz) {
+// fresh: the tendency array is taken as zero (the caller's zeroing is folded
+// in): the launch covers the whole array, points outside the box get 0, and the
+// box's points start from 0 instead of the stored value.
+__global__ void k_advect_u(Box all, Dims D, AdvU a, AdvIdx x, Box by, Box bz, Box b, int fresh) {
   int k, j, i;
-  if (!box_point(b, k, j, i)) return;
+  if (!box_point(all, k, j, i)) return;
   long q = D.ix(k, j, i);
-  float t = a.tend[q];
+  if (!b.has(k, j, i)) {
+    if (fresh) a.tend[q] = 0.0f;
+    return;
+  }
+  float t = fresh ? 0.0f : a.tend[q];
   if (by.has(k, j, i))
     t = t - (AT2(a.msfux, j, i) * a.rdy) * (u_yface(D, a, x, k, j + 1, i) - u_yface(D, a, x, k, j, i));
   t = t - (AT2(a.msfux, j, i) * a.rdx) * (u_xface(D, a, x, k, j, i + 1) - u_xface(D, a, x, k, j, i));
@@ -96,9 +103,9 @@ __global__ void k_advect_u(Box b, Dims D, AdvU a, AdvIdx x, Box by, Box bz) {
   a.tend[q] = t;
 }
 
-extern "C" int pywrf_advect_u(const pywrf_grid_t* g, const float* u, float* tendency, const float* ru,
-                              const float* rv, const float* rom, const float* msfux, const float* fzm,
-                              const float* fzp, double rdx, double rdy, const float* rdzw, pywrf_stream_t s) {
+static int advect_u(int fresh, const pywrf_grid_t* g, const float* u, float* tendency, const float* ru,
+                    const float* rv, const float* rom, const float* msfux, const float* fzm, const float* fzp,
+                    double rdx, double rdy, const float* rdzw, pywrf_stream_t s) {
   if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 4);
   AdvIdx x = {g->ids, g->ide, g->jds, g->jde, imin(g->kte, g->kde - 1)};
@@ -109,8 +116,20 @@ extern "C" int pywrf_advect_u(const pywrf_grid_t* g, const float* u, float* tend
   Box b = mkbox(g->kts, x.ktf, j0, j1, i0, i1);
   Box by = mkbox(g->kts, x.ktf, imax(g->jts, g->jds + 1), imin(g->jte, g->jde - 2), i0, i1);
   Box bz = b;
-  PW_LAUNCH_BOX(k_advect_u, b, s, dims_of(g), a, x, by, bz);
+  PW_LAUNCH_BOX(k_advect_u, fresh ? whole(g) : b, s, dims_of(g), a, x, by, bz, b, fresh);
   PW_RET();
+}
+
+extern "C" int pywrf_advect_u(const pywrf_grid_t* g, const float* u, float* tendency, const float* ru,
+                              const float* rv, const float* rom, const float* msfux, const float* fzm,
+                              const float* fzp, double rdx, double rdy, const float* rdzw, pywrf_stream_t s) {
+  return advect_u(0, g, u, tendency, ru, rv, rom, msfux, fzm, fzp, rdx, rdy, rdzw, s);
+}
+
+extern "C" int pywrf_advect_u_fresh(const pywrf_grid_t* g, const float* u, float* tendency, const float* ru,
+                                    const float* rv, const float* rom, const float* msfux, const float* fzm,
+                                    const float* fzp, double rdx, double rdy, const float* rdzw, pywrf_stream_t s) {
+  return advect_u(1, g, u, tendency, ru, rv, rom, msfux, fzm, fzp, rdx, rdy, rdzw, s);
 }
 
 // ------------------------------------------------------------------ advect_v
@@ -167,11 +186,15 @@ __device__ inline float v_zface(const Dims& D, const AdvV& a, const AdvIdx& x, i
   return vel * flux3(V(-2), V(-1), V(0), V(1), vel);  // kf == ktf-2: +vel in the reference
 }
 
-__global__ void k_advect_v(Box b, Dims D, AdvV a, AdvIdx x, Box bx) {
+__global__ void k_advect_v(Box all, Dims D, AdvV a, AdvIdx x, Box bx, Box b, int fresh) {
   int k, j, i;
-  if (!box_point(b, k, j, i)) return;
+  if (!box_point(all, k, j, i)) return;
   long q = D.ix(k, j, i);
-  float t = a.tend[q];
+  if (!b.has(k, j, i)) {
+    if (fresh) a.tend[q] = 0.0f;
+    return;
+  }
+  float t = fresh ? 0.0f : a.tend[q];
   float my = AT2(a.msfvy, j, i);
   t = t - (my * a.rdy) * (v_yface(D, a, x, k, j + 1, i) - v_yface(D, a, x, k, j, i));
   if (bx.has(k, j, i)) t = t - (my * a.rdx) * (v_xface(D, a, x, k, j, i + 1) - v_xface(D, a, x, k, j, i));
@@ -179,10 +202,9 @@ __global__ void k_advect_v(Box b, Dims D, AdvV a, AdvIdx x, Box bx) {
   a.tend[q] = t;
 }
 
-extern "C" int pywrf_advect_v(const pywrf_grid_t* g, const float* v, float* tendency, const float* ru,
-                              const float* rv, const float* rom, const float* msfvx, const float* msfvy,
-                              const float* fzm, const float* fzp, double rdx, double rdy, const float* rdzw,
-                              pywrf_stream_t s) {
+static int advect_v(int fresh, const pywrf_grid_t* g, const float* v, float* tendency, const float* ru,
+                    const float* rv, const float* rom, const float* msfvx, const float* msfvy, const float* fzm,
+                    const float* fzp, double rdx, double rdy, const float* rdzw, pywrf_stream_t s) {
   if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 4);
   AdvIdx x = {g->ids, g->ide, g->jds, g->jde, imin(g->kte, g->kde - 1)};
@@ -192,8 +214,22 @@ extern "C" int pywrf_advect_v(const pywrf_grid_t* g, const float* v, float* tend
   int j0 = imax(g->jts, g->jds + 1), j1 = imin(g->jte, g->jde - 1);
   Box b = mkbox(g->kts, x.ktf, j0, j1, i0, i1);
   Box bx = mkbox(g->kts, x.ktf, j0, j1, imax(g->ids + 1, g->its), imin(g->ide - 2, g->ite));
-  PW_LAUNCH_BOX(k_advect_v, b, s, dims_of(g), a, x, bx);
+  PW_LAUNCH_BOX(k_advect_v, fresh ? whole(g) : b, s, dims_of(g), a, x, bx, b, fresh);
   PW_RET();
+}
+
+extern "C" int pywrf_advect_v(const pywrf_grid_t* g, const float* v, float* tendency, const float* ru,
+                              const float* rv, const float* rom, const float* msfvx, const float* msfvy,
+                              const float* fzm, const float* fzp, double rdx, double rdy, const float* rdzw,
+                              pywrf_stream_t s) {
+  return advect_v(0, g, v, tendency, ru, rv, rom, msfvx, msfvy, fzm, fzp, rdx, rdy, rdzw, s);
+}
+
+extern "C" int pywrf_advect_v_fresh(const pywrf_grid_t* g, const float* v, float* tendency, const float* ru,
+                                    const float* rv, const float* rom, const float* msfvx, const float* msfvy,
+                                    const float* fzm, const float* fzp, double rdx, double rdy, const float* rdzw,
+                                    pywrf_stream_t s) {
+  return advect_v(1, g, v, tendency, ru, rv, rom, msfvx, msfvy, fzm, fzp, rdx, rdy, rdzw, s);
 }
 
 // ------------------------------------------------------------------ advect_w
@@ -250,11 +286,15 @@ __device__ inline float w_zface(const Dims& D, const AdvW& a, const AdvIdx& x, i
   return vel * flux3(W(-2), W(-1), W(0), W(1), vel);  // kf == ktf-1: +vel in the reference
 }
 
-__global__ void k_advect_w(Box b, Dims D, AdvW a, AdvIdx x, Box by, Box bx) {
+__global__ void k_advect_w(Box all, Dims D, AdvW a, AdvIdx x, Box by, Box bx, Box b, int fresh) {
   int k, j, i;
-  if (!box_point(b, k, j, i)) return;
+  if (!box_point(all, k, j, i)) return;
   long q = D.ix(k, j, i);
-  float t = a.tend[q];
+  if (!b.has(k, j, i)) {
+    if (fresh) a.tend[q] = 0.0f;
+    return;
+  }
+  float t = fresh ? 0.0f : a.tend[q];
   float mx = AT2(a.msftx, j, i);
   if (by.has(k, j, i)) t = t - (mx * a.rdy) * (w_yface(D, a, x, k, j + 1, i) - w_yface(D, a, x, k, j, i));
   if (bx.has(k, j, i)) t = t - (mx * a.rdx) * (w_xface(D, a, x, k, j, i + 1) - w_xface(D, a, x, k, j, i));
@@ -263,9 +303,9 @@ __global__ void k_advect_w(Box b, Dims D, AdvW a, AdvIdx x, Box by, Box bx) {
   a.tend[q] = t;
 }
 
-extern "C" int pywrf_advect_w(const pywrf_grid_t* g, const float* w, float* tendency, const float* ru,
-                              const float* rv, const float* rom, const float* msftx, const float* fzm,
-                              const float* fzp, double rdx, double rdy, const float* rdzu, pywrf_stream_t s) {
+static int advect_w(int fresh, const pywrf_grid_t* g, const float* w, float* tendency, const float* ru,
+                    const float* rv, const float* rom, const float* msftx, const float* fzm, const float* fzp,
+                    double rdx, double rdy, const float* rdzu, pywrf_stream_t s) {
   if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 4);
   AdvIdx x = {g->ids, g->ide, g->jds, g->jde, imin(g->kte, g->kde - 1)};
@@ -276,8 +316,20 @@ extern "C" int pywrf_advect_w(const pywrf_grid_t* g, const float* w, float* tend
   Box b = mkbox(g->kts + 1, x.ktf + 1, j0, j1, i0, i1);
   Box by = mkbox(g->kts + 1, x.ktf + 1, imax(g->jts, g->jds + 1), imin(g->jte, g->jde - 2), i0, i1);
   Box bx = mkbox(g->kts + 1, x.ktf + 1, j0, j1, imax(g->ids + 1, g->its), imin(g->ide - 2, g->ite));
-  PW_LAUNCH_BOX(k_advect_w, b, s, dims_of(g), a, x, by, bx);
+  PW_LAUNCH_BOX(k_advect_w, fresh ? whole(g) : b, s, dims_of(g), a, x, by, bx, b, fresh);
   PW_RET();
+}
+
+extern "C" int pywrf_advect_w(const pywrf_grid_t* g, const float* w, float* tendency, const float* ru,
+                              const float* rv, const float* rom, const float* msftx, const float* fzm,
+                              const float* fzp, double rdx, double rdy, const float* rdzu, pywrf_stream_t s) {
+  return advect_w(0, g, w, tendency, ru, rv, rom, msftx, fzm, fzp, rdx, rdy, rdzu, s);
+}
+
+extern "C" int pywrf_advect_w_fresh(const pywrf_grid_t* g, const float* w, float* tendency, const float* ru,
+                                    const float* rv, const float* rom, const float* msftx, const float* fzm,
+                                    const float* fzp, double rdx, double rdy, const float* rdzu, pywrf_stream_t s) {
+  return advect_w(1, g, w, tendency, ru, rv, rom, msftx, fzm, fzp, rdx, rdy, rdzu, s);
 }
 
 // ------------------------------------------------------------------ advect_scalar
@@ -450,8 +502,9 @@ static void pw_launch_scalar_adv(const pywrf_grid_t* g, const ScalarTend& a, pyw
   hipLaunchKernelGGL(k_scalar_adv, grid, dim3(PW_TX, PW_TY), 0, (hipStream_t)s, dims_of(g), a);
 }
 
-// zero_box: first zero the tendency on [kts, kte) x [jts, jte) x [its, ite)
-// (rk_scalar_tend, wrf_dynamics.py:3995) and accumulate into it.
+// zero_box 1: first zero the tendency on [kts, kte) x [jts, jte) x [its, ite)
+// (rk_scalar_tend, wrf_dynamics.py:3995) and accumulate into it; 2: the whole
+// tendency array is taken as zero (the caller's zeroing folded in).
 int pw_advect_scalar_box(const pywrf_grid_t* g, const float* field, float* tendency, const float* ru,
                          const float* rv, const float* rom, const float* msftx, const float* fzm, const float* fzp,
                          float rdx, float rdy, const float* rdzw, int zero_box, pywrf_stream_t s) {
@@ -469,7 +522,7 @@ int pw_advect_scalar_box(const pywrf_grid_t* g, const float* field, float* tende
   t.scalar = field; t.adv_tend = tendency; t.n3 = 0; t.scs = 0; t.sce = 0; t.accumulate = zero_box ? 0 : 1;
   t.scalar2 = nullptr; t.adv_tend2 = t.sc_tend2 = nullptr; t.scs2 = 0; t.sce2 = -1; t.moist_step2 = 0;
   t.nsg1 = 1;
-  t.bzero = zero_box ? mkbox(g->kts, g->kte, g->jts, g->jte, g->its, g->ite) : badv;
+  t.bzero = zero_box == 2 ? whole(g) : zero_box ? mkbox(g->kts, g->kte, g->jts, g->jte, g->its, g->ite) : badv;
   t.badv = badv; t.by = by; t.bx = bx;
   t.span = t.bzero;
   t.h_t = nullptr;
@@ -483,6 +536,14 @@ extern "C" int pywrf_advect_scalar(const pywrf_grid_t* g, const float* field, fl
                                    const float* fzp, double rdx, double rdy, const float* rdzw, pywrf_stream_t s) {
   if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   return pw_advect_scalar_box(g, field, tendency, ru, rv, rom, msftx, fzm, fzp, F(rdx), F(rdy), rdzw, 0, s);
+}
+
+extern "C" int pywrf_advect_scalar_fresh(const pywrf_grid_t* g, const float* field, float* tendency,
+                                         const float* ru, const float* rv, const float* rom, const float* msftx,
+                                         const float* fzm, const float* fzp, double rdx, double rdy,
+                                         const float* rdzw, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
+  return pw_advect_scalar_box(g, field, tendency, ru, rv, rom, msftx, fzm, fzp, F(rdx), F(rdy), rdzw, 2, s);
 }
 
 // First RK stage only: horizontal ('m') and vertical diffusion of every

@@ -193,6 +193,9 @@ __device__ __forceinline__ const T& at_b(const T* p, unsigned off) {
   return *(const T*)((const char*)p + off);
 }
 
+// The whole array of a grid, as a box.
+static inline Box whole(const pywrf_grid_t* g) { return mkbox(0, g->nz, 0, g->ny, 0, g->nx); }
+
 __host__ inline bool pw_size_ok(const pywrf_grid_t* g) {
   return g && g->nz > 0 && g->ny > 0 && g->nx > 0 && (long)g->nz * g->ny * g->nx < (1L << 30);
 }

@@ -20,7 +20,6 @@
 int pw_check_grid(const pywrf_grid_t* g, int r);
 #define CHECK(g, r) do { int _e = pw_check_grid((g), (r)); if (_e) return _e; } while (0)
 
-static Box whole(const pywrf_grid_t* g) { return mkbox(0, g->nz, 0, g->ny, 0, g->nx); }
 static Box plane(const pywrf_grid_t* g) { return mkbox(0, 1, 0, g->ny, 0, g->nx); }
 
 // ------------------------------------------------------------------ small_step_prep

@@ -58,11 +58,18 @@ __device__ inline float ph_wdwn(const Dims& D, const RhsPh& a, int k, int j, int
          (AT(a.ph, k, j, i) - AT(a.ph, k - 1, j, i) + AT(a.phb, k, j, i) - AT(a.phb, k - 1, j, i));
 }
 
-__global__ void k_rhs_ph(Box b, Dims D, RhsPh a) {
+// fresh: the tendency array is taken as zero (the caller's zeroing is folded
+// in): the launch covers the whole array, points outside the box get 0, and the
+// box's points start from 0 instead of the stored value.
+__global__ void k_rhs_ph(Box all, Dims D, RhsPh a, Box b, int fresh) {
   int k, j, i;
-  if (!box_point(b, k, j, i)) return;
+  if (!box_point(all, k, j, i)) return;
   long q = D.ix(k, j, i);
-  float t = a.tend[q];
+  if (!b.has(k, j, i)) {
+    if (fresh) a.tend[q] = 0.0f;
+    return;
+  }
+  float t = fresh ? 0.0f : a.tend[q];
   if (k < a.kte - 1) t = t - (a.fnm[k] * ph_wdwn(D, a, k + 1, j, i) + a.fnp[k] * ph_wdwn(D, a, k, j, i));
   else t = 0.0f;
   t = t + AT2(a.mut, j, i) * F(pw::g) * a.w[q] / AT2(a.msfty, j, i);
@@ -80,11 +87,11 @@ __global__ void k_rhs_ph(Box b, Dims D, RhsPh a) {
   a.tend[q] = t;
 }
 
-extern "C" int pywrf_rhs_ph(const pywrf_grid_t* g, float* ph_tend, const float* u, const float* v,
-                            const float* ww, const float* ph, const float* phb, const float* w, const float* mut,
-                            const float* muuf, const float* muvf, const float* fnm, const float* fnp,
-                            const float* rdnw, const float* cfn, const float* cfn1, double rdx, double rdy,
-                            const float* msfux, const float* msfvy, const float* msfty, pywrf_stream_t s) {
+static int rhs_ph(int fresh, const pywrf_grid_t* g, float* ph_tend, const float* u, const float* v,
+                  const float* ww, const float* ph, const float* phb, const float* w, const float* mut,
+                  const float* muuf, const float* muvf, const float* fnm, const float* fnp, const float* rdnw,
+                  const float* cfn, const float* cfn1, double rdx, double rdy, const float* msfux,
+                  const float* msfvy, const float* msfty, pywrf_stream_t s) {
   if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 4);
   if (g->kte > g->nz || g->kte < 3) return PYWRF_E_RANGE;
@@ -97,8 +104,27 @@ extern "C" int pywrf_rhs_ph(const pywrf_grid_t* g, float* ph_tend, const float* 
   a.kte = g->kte; a.ids = g->ids; a.ide = g->ide; a.jds = g->jds; a.jde = g->jde;
   a.jts = g->jts; a.jtf = imin(g->jte, g->jde - 1); a.its = g->its; a.itf = imin(g->ite, g->ide - 1);
   Box b = mkbox(1, g->kte, a.jts, a.jtf, a.its, a.itf);
-  PW_LAUNCH_BOX(k_rhs_ph, b, s, dims_of(g), a);
+  PW_LAUNCH_BOX(k_rhs_ph, fresh ? whole(g) : b, s, dims_of(g), a, b, fresh);
   PW_RET();
+}
+
+extern "C" int pywrf_rhs_ph(const pywrf_grid_t* g, float* ph_tend, const float* u, const float* v,
+                            const float* ww, const float* ph, const float* phb, const float* w, const float* mut,
+                            const float* muuf, const float* muvf, const float* fnm, const float* fnp,
+                            const float* rdnw, const float* cfn, const float* cfn1, double rdx, double rdy,
+                            const float* msfux, const float* msfvy, const float* msfty, pywrf_stream_t s) {
+  return rhs_ph(0, g, ph_tend, u, v, ww, ph, phb, w, mut, muuf, muvf, fnm, fnp, rdnw, cfn, cfn1, rdx, rdy, msfux,
+                msfvy, msfty, s);
+}
+
+extern "C" int pywrf_rhs_ph_fresh(const pywrf_grid_t* g, float* ph_tend, const float* u, const float* v,
+                                  const float* ww, const float* ph, const float* phb, const float* w,
+                                  const float* mut, const float* muuf, const float* muvf, const float* fnm,
+                                  const float* fnp, const float* rdnw, const float* cfn, const float* cfn1,
+                                  double rdx, double rdy, const float* msfux, const float* msfvy,
+                                  const float* msfty, pywrf_stream_t s) {
+  return rhs_ph(1, g, ph_tend, u, v, ww, ph, phb, w, mut, muuf, muvf, fnm, fnp, rdnw, cfn, cfn1, rdx, rdy, msfux,
+                msfvy, msfty, s);
 }
 
 // ------------------------------------------------------------------ horizontal_pressure_gradient

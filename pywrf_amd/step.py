@@ -186,19 +186,28 @@ def rk3_step(s, ops, cfg, check_nan=False):
                 s.rthblten, s.rublten, s.rvblten, s.rqvblten, s.rqcblten, s.rqiblten, N_MOIST, N_SCALAR,
                 rk_step, True, *I18)
         # rk_tendency (solver.py:971-1216)
-        zeros(s.ru_tend, s.rv_tend, s.rw_tend, s.t_tend, s.ph_tend, s.u_save, s.v_save, s.w_save,
-              s.ph_save, s.t_save, s.mu_tend, s.mu_save)
+        # with the *_fresh calls the five tendencies are zeroed by their first
+        # writer (advect_u/v/w, advect_scalar, rhs_ph below), which nothing before reads
+        fresh = hasattr(o, "advect_u_fresh")
+        if fresh:
+            zeros(s.u_save, s.v_save, s.w_save, s.ph_save, s.t_save, s.mu_tend, s.mu_save)
+        else:
+            zeros(s.ru_tend, s.rv_tend, s.rw_tend, s.t_tend, s.ph_tend, s.u_save, s.v_save, s.w_save,
+                  s.ph_save, s.t_save, s.mu_tend, s.mu_save)
         mu_tend_e = s.mu_tend.unsqueeze(0)
         mu_e = s.mu.unsqueeze(0)
         adv = (s.c1h, s.c2h, s.mut, TIME_STEP, s.msfux, s.msfuy, s.msfvx, s.msfvy, s.msftx, s.msfty, s.fnm, s.fnp,
                rdx, rdy)
-        s.ru_tend = o.advect_u(s.u, s.u, s.ru_tend, s.ru, s.rv, s.ww, *adv, s.rdnw, *I18)
-        s.rv_tend = o.advect_v(s.v, s.v, s.rv_tend, s.ru, s.rv, s.ww, *adv, s.rdnw, *I18)
-        s.rw_tend = o.advect_w(s.w, s.w, s.rw_tend, s.ru, s.rv, s.ww, *adv, s.rdn, *I18)
-        s.t_tend = o.advect_scalar(s.t, s.t, s.t_tend, s.ru, s.rv, s.ww, *adv, s.rdnw, *I18)
-        s.ph_tend = o.rhs_ph(s.ph_tend, s.u, s.v, s.ww, s.ph, s.ph, s.phb, s.w, s.mut, s.muu, s.muv, s.c1f, s.c2f,
-                             s.fnm, s.fnp, s.rdnw, s.cfn, s.cfn1, rdx, rdy, s.msfux, s.msfuy, s.msfvx, s.msfvx_inv,
-                             s.msfvy, s.msftx, s.msfty, True, *I18)
+        adv_u, adv_v, adv_w, adv_s, rhs = ((o.advect_u_fresh, o.advect_v_fresh, o.advect_w_fresh,
+                                            o.advect_scalar_fresh, o.rhs_ph_fresh) if fresh else
+                                           (o.advect_u, o.advect_v, o.advect_w, o.advect_scalar, o.rhs_ph))
+        s.ru_tend = adv_u(s.u, s.u, s.ru_tend, s.ru, s.rv, s.ww, *adv, s.rdnw, *I18)
+        s.rv_tend = adv_v(s.v, s.v, s.rv_tend, s.ru, s.rv, s.ww, *adv, s.rdnw, *I18)
+        s.rw_tend = adv_w(s.w, s.w, s.rw_tend, s.ru, s.rv, s.ww, *adv, s.rdn, *I18)
+        s.t_tend = adv_s(s.t, s.t, s.t_tend, s.ru, s.rv, s.ww, *adv, s.rdnw, *I18)
+        s.ph_tend = rhs(s.ph_tend, s.u, s.v, s.ww, s.ph, s.ph, s.phb, s.w, s.mut, s.muu, s.muv, s.c1f, s.c2f,
+                        s.fnm, s.fnp, s.rdnw, s.cfn, s.cfn1, rdx, rdy, s.msfux, s.msfuy, s.msfvx, s.msfvx_inv,
+                        s.msfvy, s.msftx, s.msfty, True, *I18)
         s.ru_tend, s.rv_tend = o.horizontal_pressure_gradient(
             s.ru_tend, s.rv_tend, s.ph, s.alt, s.p, s.pb, s.al, s.php, s.cqu, s.cqv, s.muu, s.muv, s.mu, s.c1h,
             s.c2h, s.fnm, s.fnp, s.rdnw, s.cf1, s.cf2, s.cf3, s.cfn, s.cfn1, rdx, rdy, s.msfux, s.msfuy, s.msfvx,

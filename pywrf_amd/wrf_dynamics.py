@@ -255,10 +255,25 @@ def advect_u(u, u_old, tendency,
              ims, ime, jms, jme, kms, kme,
              its, ite, jts, jte, kts, kte):
     """wrf_dynamics.py:1112-1307"""
+    return _advect_u("pywrf_advect_u", **locals())
+
+
+def _advect_u(entry, u, u_old, tendency, ru, rv, rom, c1, c2, mut, time_step, msfux, msfuy, msfvx, msfvy, msftx,
+              msfty, fzm, fzp, rdx, rdy, rdzw, ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, its, ite,
+              jts, jte, kts, kte):
     g = grid(u.shape, (ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, its, ite, jts, jte, kts, kte))
-    call("pywrf_advect_u", g, ptr(u), ptr(tendency), ptr(ru), ptr(rv), ptr(rom), ptr(msfux), ptr(fzm), ptr(fzp),
+    call(entry, g, ptr(u), ptr(tendency), ptr(ru), ptr(rv), ptr(rom), ptr(msfux), ptr(fzm), ptr(fzp),
          float(rdx), float(rdy), ptr(rdzw), stream())
     return tendency
+
+
+def advect_u_fresh(*args):
+    """advect_u (same positional arguments) into a tendency the caller would have
+    zeroed first (the zeroing at solver.py:971-975, then advect_u): the
+    kernel writes 0 + the tendency inside its index box and 0 everywhere else,
+    so the array needs no separate zeroing and is not read.  Same result as
+    zeroing and calling advect_u.  Not part of the reference API."""
+    return _advect_u("pywrf_advect_u_fresh", **vars(_bind(advect_u, args)))
 
 
 def advect_v(v, v_old, tendency,
@@ -273,10 +288,25 @@ def advect_v(v, v_old, tendency,
              ims, ime, jms, jme, kms, kme,
              its, ite, jts, jte, kts, kte):
     """wrf_dynamics.py:1332-1532"""
+    return _advect_v("pywrf_advect_v", **locals())
+
+
+def _advect_v(entry, v, v_old, tendency, ru, rv, rom, c1, c2, mut, time_step, msfux, msfuy, msfvx, msfvy, msftx,
+              msfty, fzm, fzp, rdx, rdy, rdzw, ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, its, ite,
+              jts, jte, kts, kte):
     g = grid(v.shape, (ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, its, ite, jts, jte, kts, kte))
-    call("pywrf_advect_v", g, ptr(v), ptr(tendency), ptr(ru), ptr(rv), ptr(rom), ptr(msfvx), ptr(msfvy), ptr(fzm),
+    call(entry, g, ptr(v), ptr(tendency), ptr(ru), ptr(rv), ptr(rom), ptr(msfvx), ptr(msfvy), ptr(fzm),
          ptr(fzp), float(rdx), float(rdy), ptr(rdzw), stream())
     return tendency
+
+
+def advect_v_fresh(*args):
+    """advect_v (same positional arguments) into a tendency the caller would have
+    zeroed first (the zeroing at solver.py:971-975, then advect_v): the
+    kernel writes 0 + the tendency inside its index box and 0 everywhere else,
+    so the array needs no separate zeroing and is not read.  Same result as
+    zeroing and calling advect_v.  Not part of the reference API."""
+    return _advect_v("pywrf_advect_v_fresh", **vars(_bind(advect_v, args)))
 
 
 def advect_w(w, w_old, tendency,
@@ -291,10 +321,25 @@ def advect_w(w, w_old, tendency,
              ims, ime, jms, jme, kms, kme,
              its, ite, jts, jte, kts, kte):
     """wrf_dynamics.py:1535-1822"""
+    return _advect_w("pywrf_advect_w", **locals())
+
+
+def _advect_w(entry, w, w_old, tendency, ru, rv, rom, c1, c2, mut, time_step, msfux, msfuy, msfvx, msfvy, msftx,
+              msfty, fzm, fzp, rdx, rdy, rdzu, ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, its, ite,
+              jts, jte, kts, kte):
     g = grid(w.shape, (ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, its, ite, jts, jte, kts, kte))
-    call("pywrf_advect_w", g, ptr(w), ptr(tendency), ptr(ru), ptr(rv), ptr(rom), ptr(msftx), ptr(fzm), ptr(fzp),
+    call(entry, g, ptr(w), ptr(tendency), ptr(ru), ptr(rv), ptr(rom), ptr(msftx), ptr(fzm), ptr(fzp),
          float(rdx), float(rdy), ptr(rdzu), stream())
     return tendency
+
+
+def advect_w_fresh(*args):
+    """advect_w (same positional arguments) into a tendency the caller would have
+    zeroed first (the zeroing at solver.py:971-975, then advect_w): the
+    kernel writes 0 + the tendency inside its index box and 0 everywhere else,
+    so the array needs no separate zeroing and is not read.  Same result as
+    zeroing and calling advect_w.  Not part of the reference API."""
+    return _advect_w("pywrf_advect_w_fresh", **vars(_bind(advect_w, args)))
 
 
 def advect_scalar(field, field_old, tendency,
@@ -309,10 +354,25 @@ def advect_scalar(field, field_old, tendency,
                   ims, ime, jms, jme, kms, kme,
                   its, ite, jts, jte, kts, kte):
     """wrf_dynamics.py:1841-2028"""
+    return _advect_scalar("pywrf_advect_scalar", **locals())
+
+
+def _advect_scalar(entry, field, field_old, tendency, ru, rv, rom, c1, c2, mut, time_step, msfux, msfuy, msfvx, msfvy,
+                   msftx, msfty, fzm, fzp, rdx, rdy, rdzw, ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme,
+                   its, ite, jts, jte, kts, kte):
     g = grid(field.shape, (ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, its, ite, jts, jte, kts, kte))
-    call("pywrf_advect_scalar", g, ptr(field), ptr(tendency), ptr(ru), ptr(rv), ptr(rom), ptr(msftx), ptr(fzm),
+    call(entry, g, ptr(field), ptr(tendency), ptr(ru), ptr(rv), ptr(rom), ptr(msftx), ptr(fzm),
          ptr(fzp), float(rdx), float(rdy), ptr(rdzw), stream())
     return tendency
+
+
+def advect_scalar_fresh(*args):
+    """advect_scalar (same positional arguments) into a tendency the caller would have
+    zeroed first (the zeroing at solver.py:971-975, then advect_scalar): the
+    kernel writes 0 + the tendency inside its index box and 0 everywhere else,
+    so the array needs no separate zeroing and is not read.  Same result as
+    zeroing and calling advect_scalar.  Not part of the reference API."""
+    return _advect_scalar("pywrf_advect_scalar_fresh", **vars(_bind(advect_scalar, args)))
 
 
 # ----------------------------------------------------------------- other large-step tendencies
@@ -330,11 +390,26 @@ def rhs_ph(ph_tend, u, v, ww,
            ims, ime, jms, jme, kms, kme,
            its, ite, jts, jte, kts, kte):
     """wrf_dynamics.py:2031-2297"""
+    return _rhs_ph("pywrf_rhs_ph", **locals())
+
+
+def _rhs_ph(entry, ph_tend, u, v, ww, ph, ph_old, phb, w, mut, muuf, muvf, c1f, c2f, fnm, fnp, rdnw, cfn, cfn1, rdx,
+            rdy, msfux, msfuy, msfvx, msfvx_inv, msfvy, msftx, msfty, non_hydrostatic, ids, ide, jds, jde, kds, kde,
+            ims, ime, jms, jme, kms, kme, its, ite, jts, jte, kts, kte):
     g = grid(ph.shape, (ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, its, ite, jts, jte, kts, kte))
-    call("pywrf_rhs_ph", g, ptr(ph_tend), ptr(u), ptr(v), ptr(ww), ptr(ph), ptr(phb), ptr(w), ptr(mut), ptr(muuf),
+    call(entry, g, ptr(ph_tend), ptr(u), ptr(v), ptr(ww), ptr(ph), ptr(phb), ptr(w), ptr(mut), ptr(muuf),
          ptr(muvf), ptr(fnm), ptr(fnp), ptr(rdnw), ptr(cfn), ptr(cfn1), float(rdx), float(rdy), ptr(msfux),
          ptr(msfvy), ptr(msfty), stream())
     return ph_tend
+
+
+def rhs_ph_fresh(*args):
+    """rhs_ph (same positional arguments) into a tendency the caller would have
+    zeroed first (the zeroing at solver.py:971-975, then rhs_ph): the
+    kernel writes 0 + the tendency inside its index box and 0 everywhere else,
+    so the array needs no separate zeroing and is not read.  Same result as
+    zeroing and calling rhs_ph.  Not part of the reference API."""
+    return _rhs_ph("pywrf_rhs_ph_fresh", **vars(_bind(rhs_ph, args)))
 
 
 def horizontal_pressure_gradient(ru_tend, rv_tend,
