@@ -200,6 +200,23 @@ int pywrf_coriolis_curvature(const pywrf_grid_t* g, const float* ru, const float
                              const float* msfvx, const float* msfvy, const float* f, const float* e,
                              const float* sina, const float* cosa, const float* fzm, const float* fzp, double rdx,
                              double rdy, pywrf_stream_t s);
+/* horizontal_pressure_gradient (ru_tend, rv_tend), pg_buoy_w (rw_tend, cqw)
+ * and coriolis_curvature (the three tendencies) of one RK stage
+ * (solver.py:1038-1080) in one launch, each point in that order: same results
+ * as the three calls.  The buoyancy's mu is the gradient's mu, as in the
+ * reference's calls; fnm, fnp serve as fzm, fzp.  No counterpart in the
+ * reference: a fusion. */
+int pywrf_pgf_buoy_coriolis_curvature(const pywrf_grid_t* g, float* ru_tend, float* rv_tend, float* rw_tend,
+                                      const float* ph, const float* alt, const float* p, const float* pb,
+                                      const float* al, const float* php, const float* cqu, const float* cqv,
+                                      float* cqw, const float* muu, const float* muv, const float* mu,
+                                      const float* mub, const float* fnm, const float* fnp, const float* rdnw,
+                                      const float* rdn, const float* cf, double g_, const float* ru,
+                                      const float* rv, const float* rw, const float* u, const float* v,
+                                      const float* msftx, const float* msfty, const float* msfux,
+                                      const float* msfuy, const float* msfvx, const float* msfvy, const float* f,
+                                      const float* e, const float* sina, const float* cosa, double rdx,
+                                      double rdy, pywrf_stream_t s);
 /* horizontal_diffusion, wrf_dynamics.py:2623-2838 (name: 'u', 'v', 'w' or a scalar name) */
 int pywrf_horizontal_diffusion(const pywrf_grid_t* g, char name, const float* field, float* tendency,
                                const float* mut, const float* msfux, const float* msfuy, const float* msfvx,

@@ -354,21 +354,20 @@ extern "C" int pywrf_curvature(const pywrf_grid_t* g, const float* ru, const flo
 // The two calls the driver makes back to back (solver.py:1063-1079) in one
 // pass: same boxes, each tendency point updated by the Coriolis terms and then
 // by the curvature terms, in that order (the same floats as the two kernels).
-__global__ void k_cor_curv(Box b, Dims D, Cor a, Curv c, Box bu, Box bv, Box bw) {
-  int k, j, i;
-  if (!box_point(b, k, j, i)) return;
-  const int q = D.ix(k, j, i);
+// Coriolis then curvature of the three momentum tendencies at (k, j, i)
+// (the fused coriolis + curvature), from their incoming values u0, v0, w0;
+// each result is meaningful where its box flag is set.
+__device__ inline void cor_curv_point(const Dims& D, const Cor& a, const Curv& c, bool hu, bool hv, bool hw, int k,
+                                      int j, int i, float u0, float v0, float w0, float& tu, float& tv, float& tw) {
   const float rr = F(pw::reradius);
-  // the three tendencies are formed first and stored last, so that no load waits on a store
-  const bool hu = bu.has(k, j, i), hv = bv.has(k, j, i), hw = bw.has(k, j, i);
-  float tu = 0.0f, tv = 0.0f, tw = 0.0f;
+  tu = 0.0f; tv = 0.0f; tw = 0.0f;
   if (hu) {
     const float rv4 = AT(a.rv, k, j + 1, i - 1) + AT(a.rv, k, j + 1, i) + AT(a.rv, k, j, i - 1) + AT(a.rv, k, j, i);
     const float rw4 = AT(a.rw, k + 1, j, i - 1) + AT(a.rw, k, j, i - 1) + AT(a.rw, k + 1, j, i) + AT(a.rw, k, j, i);
     float x = (AT2(a.msfux, j, i) / AT2(a.msfuy, j, i)) * 0.5f * (AT2(a.f, j, i) + AT2(a.f, j, i - 1)) * 0.25f * rv4;
     float y = 0.5f * (AT2(a.e, j, i) + AT2(a.e, j, i - 1)) * 0.5f * (AT2(a.cosa, j, i) + AT2(a.cosa, j, i - 1)) *
               0.25f * rw4;
-    float t = a.ru_tend[q] + x - y;
+    float t = u0 + x - y;
     x = 0.5f * (curv_vxgm(D, c, k, j, i) + curv_vxgm(D, c, k, j, i - 1)) * 0.25f * rv4;
     y = AT(c.u, k, j, i) * rr * 0.25f * rw4;
     tu = t + x - y;
@@ -380,7 +379,7 @@ __global__ void k_cor_curv(Box b, Dims D, Cor a, Curv c, Box bu, Box bv, Box bw)
     float x = r * 0.5f * (AT2(a.f, j, i) + AT2(a.f, j - 1, i)) * 0.25f * ru4;
     float y = r * 0.5f * (AT2(a.e, j, i) + AT2(a.e, j - 1, i)) * 0.5f * (AT2(a.sina, j, i) + AT2(a.sina, j - 1, i)) *
               0.25f * rw4;
-    float t = a.rv_tend[q] - x + y;
+    float t = v0 - x + y;
     x = 0.5f * (curv_vxgm(D, c, k, j, i) + curv_vxgm(D, c, k, j - 1, i)) * 0.25f * ru4;
     y = r * AT(c.v, k, j, i) * rr * 0.25f * rw4;
     tv = t - x - y;
@@ -393,12 +392,70 @@ __global__ void k_cor_curv(Box b, Dims D, Cor a, Curv c, Box bu, Box bv, Box bw)
     const float mr = AT2(a.msftx, j, i) / AT2(a.msfty, j, i);
     float cu = AT2(a.cosa, j, i) * 0.5f * fz(a.ru, 0, 1);
     float sv = mr * AT2(a.sina, j, i) * 0.5f * fz(a.rv, 1, 0);
-    float t = a.rw_tend[q] + AT2(a.e, j, i) * (cu - sv);
+    float t = w0 + AT2(a.e, j, i) * (cu - sv);
     tw = t + rr * (0.5f * fz(a.ru, 0, 1) * 0.5f * fz(c.u, 0, 1) + mr * 0.5f * fz(a.rv, 1, 0) * 0.5f * fz(c.v, 1, 0));
   }
+}
+
+__global__ void k_cor_curv(Box b, Dims D, Cor a, Curv c, Box bu, Box bv, Box bw) {
+  int k, j, i;
+  if (!box_point(b, k, j, i)) return;
+  const int q = D.ix(k, j, i);
+  // the three tendencies are formed first and stored last, so that no load waits on a store
+  const bool hu = bu.has(k, j, i), hv = bv.has(k, j, i), hw = bw.has(k, j, i);
+  const float u0 = a.ru_tend[q], v0 = a.rv_tend[q], w0 = a.rw_tend[q];
+  float tu, tv, tw;
+  cor_curv_point(D, a, c, hu, hv, hw, k, j, i, u0, v0, w0, tu, tv, tw);
   if (hu) a.ru_tend[q] = tu;
   if (hv) a.rv_tend[q] = tv;
   if (hw) a.rw_tend[q] = tw;
+}
+
+// horizontal_pressure_gradient, pg_buoy_w and the fused coriolis + curvature
+// of one RK stage in one pass (each point in the calls' order: the pressure
+// gradient on ru, rv, the buoyancy on rw and cqw, then Coriolis and curvature
+// on the three).  The calls' boxes nest in the launch box; only the buoyancy
+// thread at kde-2 touches level kde-1 (the top rw tendency, from the cqw it is
+// about to overwrite), which no other part updates.
+struct PgBuoy {
+  float *rw_tend, *cqw;
+  const float *p, *muf, *mubf, *rdnw, *rdn, *msfty;
+  float g;
+  int kde;
+};
+
+__global__ void k_pgf_buoy_cor(Box b, Dims D, Hpg h, Box hbv, Box hbu, PgBuoy pg, Box pbox, Cor a, Curv c, Box bu,
+                               Box bv, Box bw) {
+  int k, j, i;
+  if (!box_point(b, k, j, i)) return;
+  const int q = D.ix(k, j, i);
+  float u0 = a.ru_tend[q], v0 = a.rv_tend[q], w0 = a.rw_tend[q];
+  const bool gv = hbv.has(k, j, i), gu = hbu.has(k, j, i), gb = pbox.has(k, j, i);
+  if (gv) v0 = v0 - h.cqv[q] * hpg_dpx(D, h, k, j, i, -1, 0, h.muv, h.msfvy, h.msfvx, h.rdy);
+  if (gu) u0 = u0 - h.cqu[q] * hpg_dpx(D, h, k, j, i, 0, -1, h.muu, h.msfux, h.msfuy, h.rdx);
+  float cq1 = 0.0f, wtop = 0.0f;
+  const bool top = gb && k == pg.kde - 2;
+  if (gb) {
+    const float cqo = pg.cqw[q];
+    const float rmsf = 1.0f / AT2(pg.msfty, j, i);
+    if (top) {
+      const float c1 = 1.0f / (1.0f + cqo), c2 = cqo * c1;
+      wtop = AT(pg.rw_tend, pg.kde - 1, j, i) + rmsf * pg.g * (c1 * 2.0f * pg.rdnw[pg.kde - 2] * (-pg.p[q]) -
+                                                               AT2(pg.muf, j, i) - c2 * AT2(pg.mubf, j, i));
+    }
+    cq1 = 1.0f / (1.0f + cqo);
+    const float cq2 = cqo * cq1;
+    w0 = w0 + rmsf * pg.g * (cq1 * pg.rdn[k] * (pg.p[q] - AT(pg.p, k - 1, j, i)) - AT2(pg.muf, j, i) -
+                             cq2 * AT2(pg.mubf, j, i));
+  }
+  const bool hu = bu.has(k, j, i), hv = bv.has(k, j, i), hw = bw.has(k, j, i);
+  float tu, tv, tw;
+  cor_curv_point(D, a, c, hu, hv, hw, k, j, i, u0, v0, w0, tu, tv, tw);
+  if (hu || gu) a.ru_tend[q] = hu ? tu : u0;
+  if (hv || gv) a.rv_tend[q] = hv ? tv : v0;
+  if (hw || gb) a.rw_tend[q] = hw ? tw : w0;
+  if (gb) pg.cqw[q] = cq1;
+  if (top) AT(pg.rw_tend, pg.kde - 1, j, i) = wtop;
 }
 
 extern "C" int pywrf_coriolis_curvature(const pywrf_grid_t* g, const float* ru, const float* rv, const float* rw,
@@ -629,5 +686,44 @@ extern "C" int pywrf_rk_addtend_dry(const pywrf_grid_t* g, float* ru_tend, float
   Box bt = mkbox(g->kts, g->kte - 1, g->jts, g->jde - 1, g->its, g->ide - 1);
   Box b = mkbox(g->kts, g->kte, g->jts, imax(g->jde - 1, g->jte), g->its, imax(g->ite, g->ide - 1));
   PW_LAUNCH_BOX(k_addtend, b, s, dims_of(g), a, bu, bv, bw, bt);
+  PW_RET();
+}
+
+extern "C" int pywrf_pgf_buoy_coriolis_curvature(
+    const pywrf_grid_t* g, float* ru_tend, float* rv_tend, float* rw_tend, const float* ph, const float* alt,
+    const float* p, const float* pb, const float* al, const float* php, const float* cqu, const float* cqv,
+    float* cqw, const float* muu, const float* muv, const float* mu, const float* mub, const float* fnm,
+    const float* fnp, const float* rdnw, const float* rdn, const float* cf, double g_, const float* ru,
+    const float* rv, const float* rw, const float* u, const float* v, const float* msftx, const float* msfty,
+    const float* msfux, const float* msfuy, const float* msfvx, const float* msfvy, const float* f, const float* e,
+    const float* sina, const float* cosa, double rdx, double rdy, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
+  CHECK(g, 2);
+  Hpg h;
+  h.ph = ph; h.alt = alt; h.p = p; h.pb = pb; h.al = al; h.php = php; h.cqu = cqu; h.cqv = cqv; h.muu = muu;
+  h.muv = muv; h.mu = mu; h.fnm = fnm; h.fnp = fnp; h.rdnw = rdnw; h.cf = cf; h.msfux = msfux; h.msfuy = msfuy;
+  h.msfvx = msfvx; h.msfvy = msfvy; h.ru_tend = ru_tend; h.rv_tend = rv_tend; h.rdx = F(rdx); h.rdy = F(rdy);
+  h.ktf = imin(g->kte, g->kde - 1);
+  h.kde = g->kde;
+  if (h.ktf + 1 > g->nz || g->kde > g->nz || g->kde < 3) return PYWRF_E_RANGE;
+  Box hbv = mkbox(0, h.ktf, g->jts + 1, g->jte - 1, g->its, imin(g->ite, g->ide - 1));
+  Box hbu = mkbox(0, h.ktf, g->jts, imin(g->jte, g->jde - 1), g->its + 1, g->ite - 1);
+  Box hb = mkbox(0, h.ktf, g->jts, g->jte, g->its, g->ite);
+  PgBuoy pg = {rw_tend, cqw, p, mu, mub, rdnw, rdn, msfty, F(g_), g->kde};
+  Box pbox = mkbox(1, g->kde - 1, g->jts, imin(g->jte, g->jde - 1), g->its, imin(g->ite, g->ide - 1));
+  Cor a = {ru, rv, rw, msftx, msfty, msfux, msfuy, msfvx, msfvy, f, e, sina, cosa, fnm, fnp, ru_tend, rv_tend,
+           rw_tend};
+  Curv c = {ru, rv, rw, u, v, msfuy, msfvx, msfvy, msftx, msfty, fnm, fnp, ru_tend, rv_tend, rw_tend,
+            F(rdx), F(rdy), g->jts, g->jte, g->its, g->ite};
+  const int ktf = h.ktf;
+  Box bu = mkbox(g->kts, ktf, g->jts, g->jte - 1, imax(g->ids + 1, g->its), imin(g->ide - 1, g->ite));
+  Box bv = mkbox(g->kts, ktf, imax(g->jds + 1, g->jts), imin(g->jde - 1, g->jte), g->its, g->ite - 1);
+  Box bw = mkbox(g->kts + 1, ktf, g->jts, g->jte - 1, g->its, g->ite - 1);
+  Box cb = mkbox(g->kts, ktf, g->jts, g->jte, g->its, g->ite);
+  // the launch box holds the three calls' boxes (pg_buoy_w's top-level write is made by its kde-2 thread)
+  Box b = mkbox(imin(hb.k0, imin(pbox.k0, cb.k0)), imax(hb.k1, imax(pbox.k1, cb.k1)),
+                imin(hb.j0, imin(pbox.j0, cb.j0)), imax(hb.j1, imax(pbox.j1, cb.j1)),
+                imin(hb.i0, imin(pbox.i0, cb.i0)), imax(hb.i1, imax(pbox.i1, cb.i1)));
+  PW_LAUNCH_BOX(k_pgf_buoy_cor, b, s, dims_of(g), h, hbv, hbu, pg, pbox, a, c, bu, bv, bw);
   PW_RET();
 }

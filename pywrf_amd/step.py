@@ -208,23 +208,31 @@ def rk3_step(s, ops, cfg, check_nan=False):
         s.ph_tend = rhs(s.ph_tend, s.u, s.v, s.ww, s.ph, s.ph, s.phb, s.w, s.mut, s.muu, s.muv, s.c1f, s.c2f,
                         s.fnm, s.fnp, s.rdnw, s.cfn, s.cfn1, rdx, rdy, s.msfux, s.msfuy, s.msfvx, s.msfvx_inv,
                         s.msfvy, s.msftx, s.msfty, True, *I18)
-        s.ru_tend, s.rv_tend = o.horizontal_pressure_gradient(
-            s.ru_tend, s.rv_tend, s.ph, s.alt, s.p, s.pb, s.al, s.php, s.cqu, s.cqv, s.muu, s.muv, s.mu, s.c1h,
-            s.c2h, s.fnm, s.fnp, s.rdnw, s.cf1, s.cf2, s.cf3, s.cfn, s.cfn1, rdx, rdy, s.msfux, s.msfuy, s.msfvx,
-            s.msfvy, s.msftx, s.msfty, True, False, *I18)
-        s.rw_tend, s.cqw = o.pg_buoy_w(s.rw_tend, s.p, s.cqw, s.mu, s.mub, s.c1f, s.c2f, s.rdnw, s.rdn, G,
-                                       s.msftx, s.msfty, *I18)
-        if hasattr(o, "coriolis_curvature"):
-            o.coriolis_curvature(s.ru, s.rv, s.rw, s.u, s.v, s.w, s.ru_tend, s.rv_tend, s.rw_tend, s.msfux,
-                                 s.msfuy, s.msfvx, s.msfvy, s.msftx, s.msfty, s.f, s.e, s.sina, s.cosa, s.clat,
-                                 s.fnm, s.fnp, rdx, rdy, *I18)
+        hpg_args = (s.ru_tend, s.rv_tend, s.ph, s.alt, s.p, s.pb, s.al, s.php, s.cqu, s.cqv, s.muu, s.muv, s.mu,
+                    s.c1h, s.c2h, s.fnm, s.fnp, s.rdnw, s.cf1, s.cf2, s.cf3, s.cfn, s.cfn1, rdx, rdy, s.msfux,
+                    s.msfuy, s.msfvx, s.msfvy, s.msftx, s.msfty, True, False, *I18)
+        buoy_args = (s.rw_tend, s.p, s.cqw, s.mu, s.mub, s.c1f, s.c2f, s.rdnw, s.rdn, G, s.msftx, s.msfty, *I18)
+        if hasattr(o, "pgf_buoy_coriolis_curvature"):
+            # the three calls in one launch, each point in the same order
+            o.pgf_buoy_coriolis_curvature(
+                hpg_args, buoy_args,
+                (s.ru, s.rv, s.rw, s.u, s.v, s.w, s.ru_tend, s.rv_tend, s.rw_tend, s.msfux, s.msfuy, s.msfvx,
+                 s.msfvy, s.msftx, s.msfty, s.f, s.e, s.sina, s.cosa, s.clat, s.fnm, s.fnp, rdx, rdy, *I18))
         else:
-            s.ru_tend, s.rv_tend, s.rw_tend = o.coriolis(s.ru, s.rv, s.rw, s.ru_tend, s.rv_tend, s.rw_tend,
-                                                         s.msftx, s.msfty, s.msfux, s.msfuy, s.msfvx, s.msfvy, s.f,
-                                                         s.e, s.sina, s.cosa, s.fnm, s.fnp, *I18)
-            s.ru_tend, s.rv_tend, s.rw_tend = o.curvature(s.ru, s.rv, s.rw, s.u, s.v, s.w, s.ru_tend, s.rv_tend,
-                                                          s.rw_tend, s.msfux, s.msfuy, s.msfvx, s.msfvy, s.msftx,
-                                                          s.msfty, s.clat, s.fnm, s.fnp, rdx, rdy, *I18)
+            s.ru_tend, s.rv_tend = o.horizontal_pressure_gradient(*hpg_args)
+            s.rw_tend, s.cqw = o.pg_buoy_w(*buoy_args)
+            if hasattr(o, "coriolis_curvature"):
+                o.coriolis_curvature(s.ru, s.rv, s.rw, s.u, s.v, s.w, s.ru_tend, s.rv_tend, s.rw_tend, s.msfux,
+                                     s.msfuy, s.msfvx, s.msfvy, s.msftx, s.msfty, s.f, s.e, s.sina, s.cosa, s.clat,
+                                     s.fnm, s.fnp, rdx, rdy, *I18)
+            else:
+                s.ru_tend, s.rv_tend, s.rw_tend = o.coriolis(s.ru, s.rv, s.rw, s.ru_tend, s.rv_tend, s.rw_tend,
+                                                             s.msftx, s.msfty, s.msfux, s.msfuy, s.msfvx, s.msfvy,
+                                                             s.f, s.e, s.sina, s.cosa, s.fnm, s.fnp, *I18)
+                s.ru_tend, s.rv_tend, s.rw_tend = o.curvature(s.ru, s.rv, s.rw, s.u, s.v, s.w, s.ru_tend,
+                                                              s.rv_tend, s.rw_tend, s.msfux, s.msfuy, s.msfvx,
+                                                              s.msfvy, s.msftx, s.msfty, s.clat, s.fnm, s.fnp, rdx,
+                                                              rdy, *I18)
         if rk_step == 1:
             hd = (s.msfux, s.msfuy, s.msfvx, s.msfvx_inv, s.msfvy, s.msftx, s.msfty)
             s.ru_tendf = o.horizontal_diffusion('u', s.u, s.ru_tendf, s.mut, s.c1h, s.c2h, *hd, KHDIF, s.xkmh,

@@ -490,6 +490,36 @@ def coriolis_curvature(ru, rv, rw, u, v, w, ru_tend, rv_tend, rw_tend,
     return ru_tend, rv_tend, rw_tend
 
 
+def pgf_buoy_coriolis_curvature(hpg_args, buoy_args, cc_args):
+    """horizontal_pressure_gradient, pg_buoy_w and coriolis_curvature, each
+    given as the tuple of its positional arguments, in one launch (the
+    sequence of solver.py:1038-1080).  The calls must act on the same
+    tendencies (ru_tend, rv_tend of the first and the third, rw_tend of the
+    second and the third), pressure, mu, rdnw, map factors, index set and grid
+    spacing, with the third's fzm, fzp the first's fnm, fnp.  Returns the
+    three calls' results.  Not part of the reference API."""
+    h = _bind(horizontal_pressure_gradient, hpg_args)
+    b = _bind(pg_buoy_w, buoy_args)
+    c = _bind(coriolis_curvature, cc_args)
+    same = [(c.ru_tend, h.ru_tend), (c.rv_tend, h.rv_tend), (c.rw_tend, b.rw_tend), (b.p, h.p), (b.muf, h.mu),
+            (b.rdnw, h.rdnw), (b.msfty, c.msfty), (c.fzm, h.fnm), (c.fzp, h.fnp), (c.msfux, h.msfux),
+            (c.msfuy, h.msfuy), (c.msfvx, h.msfvx), (c.msfvy, h.msfvy)]
+    idx = ("ids", "ide", "jds", "jde", "kds", "kde", "ims", "ime", "jms", "jme", "kms", "kme", "its", "ite", "jts",
+           "jte", "kts", "kte")
+    if (not all(x is y for x, y in same) or h.rdx != c.rdx or h.rdy != c.rdy
+            or any(getattr(h, n) != getattr(b, n) or getattr(h, n) != getattr(c, n) for n in idx)):
+        raise ValueError("pgf_buoy_coriolis_curvature: the three calls do not share their common arguments")
+    g = grid(h.p.shape, tuple(getattr(h, n) for n in idx))
+    cf, _keep = _cf(h.cf1, h.cf2, h.cf3)
+    call("pywrf_pgf_buoy_coriolis_curvature", g, ptr(h.ru_tend), ptr(h.rv_tend), ptr(b.rw_tend), ptr(h.ph),
+         ptr(h.alt), ptr(h.p), ptr(h.pb), ptr(h.al), ptr(h.php), ptr(h.cqu), ptr(h.cqv), ptr(b.cqw), ptr(h.muu),
+         ptr(h.muv), ptr(h.mu), ptr(b.mubf), ptr(h.fnm), ptr(h.fnp), ptr(h.rdnw), ptr(b.rdn), cf, float(b.g_),
+         ptr(c.ru), ptr(c.rv), ptr(c.rw), ptr(c.u), ptr(c.v), ptr(c.msftx), ptr(c.msfty), ptr(c.msfux),
+         ptr(c.msfuy), ptr(c.msfvx), ptr(c.msfvy), ptr(c.f), ptr(c.e), ptr(c.sina), ptr(c.cosa), float(h.rdx),
+         float(h.rdy), stream())
+    return (h.ru_tend, h.rv_tend), (b.rw_tend, b.cqw), (c.ru_tend, c.rv_tend, c.rw_tend)
+
+
 def horizontal_diffusion(name, field, tendency, mut, c1, c2,
                          msfux, msfuy, msfvx, msfvx_inv,
                          msfvy, msftx, msfty,

@@ -259,3 +259,35 @@ def test_fresh_first_writers_equal_zeroing_then_the_call():
         torch.cuda.synchronize()
         assert int(torch.count_nonzero(ref)) > 0, name
         assert torch.equal(ref, got), name
+
+
+def test_pgf_buoy_coriolis_curvature_equals_the_three_calls():
+    """The fused pressure-gradient / buoyancy / Coriolis-curvature entry point
+    against horizontal_pressure_gradient, pg_buoy_w and coriolis_curvature in
+    sequence, bit for bit (tendencies and cqw), on the state after one step
+    with random incoming tendencies."""
+    from pywrf_amd.step import RunConfig, G
+    cfg = RunConfig()
+    s, o = _stepped()
+    I18 = s.grid.ix18
+    torch.manual_seed(13)
+    t0 = [torch.randn_like(s.ru_tend) for _ in range(3)]
+    cq0 = s.cqw.clone()
+
+    def args(ru_t, rv_t, rw_t, cqw):
+        return ((ru_t, rv_t, s.ph, s.alt, s.p, s.pb, s.al, s.php, s.cqu, s.cqv, s.muu, s.muv, s.mu, s.c1h, s.c2h,
+                 s.fnm, s.fnp, s.rdnw, s.cf1, s.cf2, s.cf3, s.cfn, s.cfn1, cfg.rdx, cfg.rdy, s.msfux, s.msfuy,
+                 s.msfvx, s.msfvy, s.msftx, s.msfty, True, False, *I18),
+                (rw_t, s.p, cqw, s.mu, s.mub, s.c1f, s.c2f, s.rdnw, s.rdn, G, s.msftx, s.msfty, *I18),
+                (s.ru, s.rv, s.rw, s.u, s.v, s.w, ru_t, rv_t, rw_t, s.msfux, s.msfuy, s.msfvx, s.msfvy, s.msftx,
+                 s.msfty, s.f, s.e, s.sina, s.cosa, s.clat, s.fnm, s.fnp, cfg.rdx, cfg.rdy, *I18))
+    ref = [t.clone() for t in t0] + [cq0.clone()]
+    ha, ba, ca = args(*ref)
+    o.horizontal_pressure_gradient(*ha)
+    o.pg_buoy_w(*ba)
+    o.coriolis_curvature(*ca)
+    got = [t.clone() for t in t0] + [cq0.clone()]
+    o.pgf_buoy_coriolis_curvature(*args(*got))
+    torch.cuda.synchronize()
+    for n, (a, b) in enumerate(zip(ref, got)):
+        assert torch.equal(a, b), n
