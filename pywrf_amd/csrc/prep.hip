@@ -558,25 +558,32 @@ __global__ void k_phy_prep_pt(Box all, Dims D, Box b, const float* u, const floa
   int k, j, i;
   if (!box_point(all, k, j, i)) return;
   long q = D.ix(k, j, i);
-  float th, pp;
+  // every value is formed first and stored last, so that no load waits on a store
+  float th, pp, ro = 0.0f, up = 0.0f, vp = 0.0f, zw = 0.0f;
   bool inb = b.has(k, j, i);
   if (inb) {
     th = t[q] + F(pw::t0);
     pp = p[q] + pb[q];
-    th_phy[q] = th;
-    p_phy[q] = pp;
+    ro = 1.0f / alt[q] * (1.0f + qv[q]);
+    up = 0.5f * (u[q] + AT(u, k, j, i + 1));
+    vp = 0.5f * (v[q] + AT(v, k, j + 1, i));
+    zw = (phb[q] + ph[q]) / F(pw::g);
   } else {
     th = th_phy[q];
     pp = p_phy[q];
   }
   float pi = powf(pp / F(pw::p1000mb), F(pw::rcp));
+  if (inb) {
+    th_phy[q] = th;
+    p_phy[q] = pp;
+  }
   pi_phy[q] = pi;
   t_phy[q] = th * pi;
   if (inb) {
-    rho[q] = 1.0f / alt[q] * (1.0f + qv[q]);
-    u_phy[q] = 0.5f * (u[q] + AT(u, k, j, i + 1));
-    v_phy[q] = 0.5f * (v[q] + AT(v, k, j + 1, i));
-    z_at_w[q] = (phb[q] + ph[q]) / F(pw::g);
+    rho[q] = ro;
+    u_phy[q] = up;
+    v_phy[q] = vp;
+    z_at_w[q] = zw;
   }
 }
 
@@ -801,8 +808,13 @@ __global__ void k_deform(Box all, Dims D, const float* u, const float* v, const 
   DefW cw = def_weights(c);
   long q = D.ix(k, j, i);
   const int kts = 0, ktf = c.ktf;
+  // every value is formed first and stored last, so that no load waits on a store
+  float o11 = 0.0f, o22 = 0.0f, o33 = 0.0f, odv = 0.0f, o12 = 0.0f, o13 = 0.0f, o23 = 0.0f;
   // D11, D22, D33 and div on [kts, ktf) x [jts, jtf) x [its, itf)
-  if (k < ktf && j >= jts && j < jtf && i >= its && i < itf) {
+  const bool s1 = k < ktf && j >= jts && j < jtf && i >= its && i < itf;
+  const bool s2 = k < ktf && j >= jts + 1 && j < jte - 1 && i >= its + 1 && i < ite - 1;
+  const bool s3 = k <= ktf && j >= jts && j < jde - 1 && i >= its && i < ide - 1;
+  if (s1) {
     float mm = AT2(msftx, j, i) * AT2(msfty, j, i);
     float ha1 = hat_pair_avg(D, u, msfuy, fnm, fnp, c, cw, k + 1, j, i, j, i + 1);
     float ha0 = hat_pair_avg(D, u, msfuy, fnm, fnp, c, cw, k, j, i, j, i + 1);
@@ -810,7 +822,7 @@ __global__ void k_deform(Box all, Dims D, const float* u, const float* v, const 
                                        AT(zx, k + 1, j, i + 1)) * rdzw[q]);
     float hp = AT(u, k, j, i + 1) / AT2(msfuy, j, i + 1), h0 = u[q] / AT2(msfuy, j, i);
     t1 = mm * (rdx * (hp - h0) - t1);
-    d11[q] = 2.0f * t1;
+    o11 = 2.0f * t1;
     float dv = t1 + 0.0f;
     ha1 = hat_pair_avg(D, v, msfvx, fnm, fnp, c, cw, k + 1, j, i, j + 1, i);
     ha0 = hat_pair_avg(D, v, msfvx, fnm, fnp, c, cw, k, j, i, j + 1, i);
@@ -819,14 +831,14 @@ __global__ void k_deform(Box all, Dims D, const float* u, const float* v, const 
     t1 = mm * (rdy * (hp - h0) - (ha1 - ha0) * (0.25f * (AT(zy, k, j, i) + AT(zy, k, j + 1, i) +
                                                          AT(zy, k + 1, j, i) + AT(zy, k + 1, j + 1, i)) *
                                                 rdzw[q]));
-    d22[q] = 2.0f * t1;
+    o22 = 2.0f * t1;
     dv = t1 + dv;
     t1 = (AT(w, k + 1, j, i) - w[q]) * rdzw[q];
-    d33[q] = 2.0f * t1;
-    div[q] = t1 + dv;
+    o33 = 2.0f * t1;
+    odv = t1 + dv;
   }
   // D12 on [kts, ktf) x [jts+1, jte-1) x [its+1, ite-1)
-  if (k < ktf && j >= jts + 1 && j < jte - 1 && i >= its + 1 && i < ite - 1) {
+  if (s2) {
     float mm = 0.25f * (AT2(msfux, j - 1, i) + AT2(msfux, j, i)) * (AT2(msfvy, j, i - 1) + AT2(msfvy, j, i));
     float ha1 = hat_pair_avg(D, u, msfux, fnm, fnp, c, cw, k + 1, j - 1, i, j, i);
     float ha0 = hat_pair_avg(D, u, msfux, fnm, fnp, c, cw, k, j - 1, i, j, i);
@@ -842,13 +854,13 @@ __global__ void k_deform(Box all, Dims D, const float* u, const float* v, const 
                         (rdzw[q] + AT(rdzw, k, j - 1, i) + AT(rdzw, k, j - 1, i - 1) + AT(rdzw, k, j, i - 1)));
     h0 = v[q] / AT2(msfvy, j, i);
     hm = AT(v, k, j, i - 1) / AT2(msfvy, j, i - 1);
-    d12[q] = r + mm * (rdx * (h0 - hm) - t1);
+    o12 = r + mm * (rdx * (h0 - hm) - t1);
   }
   // D13, D23 on [kts, ktf] x [jts, jde-1) x [its, ide-1)
-  if (k <= ktf && j >= jts && j < jde - 1 && i >= its && i < ide - 1) {
+  if (s3) {
     if (k == kts || k == ktf) {
-      d13[q] = 0.0f;
-      d23[q] = 0.0f;
+      o13 = 0.0f;
+      o23 = 0.0f;
     } else {
       float mm = AT2(msfux, j, i) * AT2(msfuy, j, i);
       auto ht = [&](int kk, int jj, int ii) { return AT(w, kk, jj, ii) / AT2(msfty, jj, ii); };
@@ -857,7 +869,7 @@ __global__ void k_deform(Box all, Dims D, const float* u, const float* v, const 
       float t1 = (hv0 - hv1) * AT(zx, k, j, i) * 0.5f * (rdz[q] + AT(rdz, k, j, i - 1));
       float r = mm * (rdx * (ht(k, j, i) - ht(k, j, i - 1)) - t1);
       t1 = (u[q] - AT(u, k - 1, j, i)) * 0.5f * (rdz[q] + AT(rdz, k, j, i - 1));
-      d13[q] = r + t1;
+      o13 = r + t1;
       mm = AT2(msfvx, j, i) * AT2(msfvy, j, i);
       auto hx = [&](int kk, int jj, int ii) { return AT(w, kk, jj, ii) / AT2(msftx, jj, ii); };
       hv0 = 0.25f * (hx(k, j, i) + hx(k + 1, j, i) + hx(k, j - 1, i) + hx(k + 1, j - 1, i));
@@ -865,8 +877,19 @@ __global__ void k_deform(Box all, Dims D, const float* u, const float* v, const 
       t1 = (hv0 - hv1) * AT(zy, k, j, i) * 0.5f * (rdz[q] + AT(rdz, k, j - 1, i));
       r = mm * (rdy * (hx(k, j, i) - hx(k, j - 1, i)) - t1);
       t1 = (v[q] - AT(v, k - 1, j, i)) * 0.5f * (rdz[q] + AT(rdz, k, j - 1, i));
-      d23[q] = r + t1;
+      o23 = r + t1;
     }
+  }
+  if (s1) {
+    d11[q] = o11;
+    d22[q] = o22;
+    d33[q] = o33;
+    div[q] = odv;
+  }
+  if (s2) d12[q] = o12;
+  if (s3) {
+    d13[q] = o13;
+    d23[q] = o23;
   }
 }
 
@@ -993,7 +1016,8 @@ __global__ void k_km_kh(Box all, Dims D, KmArgs a) {
   // BN2: level ktf-1 repeats level ktf-2 over the whole plane
   int kb = (k == a.ktf - 1) ? a.ktf - 2 : k;
   bool incol = j >= a.j0 && j < a.j1 && i >= a.i0 && i < a.i1;
-  a.bn2[q] = (incol && kb < a.ktf - 1) ? bn2_at(D, a, kb, j, i) : 0.0f;
+  // every value is formed first and stored last, so that no load waits on a store
+  const float bn = (incol && kb < a.ktf - 1) ? bn2_at(D, a, kb, j, i) : 0.0f;
   // 2-D Smagorinsky on the box, then the cap against 10 * mlen_h everywhere
   bool inb = k < a.ktf && j >= a.sj0 && j < a.sj1 && i >= a.si0 && i < a.si1;
   float cap = 0.0f, xk = a.xkmh[q];
@@ -1008,10 +1032,11 @@ __global__ void k_km_kh(Box all, Dims D, KmArgs a) {
       def2 = def2 + tmp * tmp;
       tmp = sqrtf(def2);
       xk = F(pw::c_s * pw::c_s) * mlen * mlen * tmp;
-      a.xkmh[q] = xk;
     }
   }
   float xn = xk < cap ? xk : cap;
+  a.bn2[q] = bn;
+  if (inb) a.xkmh[q] = xk;
   a.xkmh_out[q] = xn;
   a.xkmv[q] = 0.0f;
   a.xkhv[q] = 0.0f;
@@ -1051,14 +1076,25 @@ __global__ void k_update_phy_ten(Box b, Dims D, Box ba, Box bu, Box bv, float* r
   int k, j, i;
   if (!box_point(b, k, j, i)) return;
   long q = D.ix(k, j, i);
-  if (ba.has(k, j, i)) {
-    rt_tendf[q] = rt_tendf[q] + rth[q];
-    mqv[q] = mqv[q] + rqv[q];
-    mqc[q] = mqc[q] + rqc[q];
-    mqi[q] = mqi[q] + rqi[q];
+  // every value is formed first and stored last, so that no load waits on a store
+  const bool ha = ba.has(k, j, i), hu = bu.has(k, j, i), hv = bv.has(k, j, i);
+  float nt = 0.0f, nqv = 0.0f, nqc = 0.0f, nqi = 0.0f, nu = 0.0f, nv = 0.0f;
+  if (ha) {
+    nt = rt_tendf[q] + rth[q];
+    nqv = mqv[q] + rqv[q];
+    nqc = mqc[q] + rqc[q];
+    nqi = mqi[q] + rqi[q];
   }
-  if (bu.has(k, j, i)) ru_tendf[q] = ru_tendf[q] + 0.5f * ru[q] + 0.5f * AT(ru, k, j, i - 1);
-  if (bv.has(k, j, i)) rv_tendf[q] = rv_tendf[q] + 0.5f * rv[q] + 0.5f * AT(rv, k, j - 1, i);
+  if (hu) nu = ru_tendf[q] + 0.5f * ru[q] + 0.5f * AT(ru, k, j, i - 1);
+  if (hv) nv = rv_tendf[q] + 0.5f * rv[q] + 0.5f * AT(rv, k, j - 1, i);
+  if (ha) {
+    rt_tendf[q] = nt;
+    mqv[q] = nqv;
+    mqc[q] = nqc;
+    mqi[q] = nqi;
+  }
+  if (hu) ru_tendf[q] = nu;
+  if (hv) rv_tendf[q] = nv;
 }
 
 extern "C" int pywrf_update_phy_ten(const pywrf_grid_t* g, float* rt_tendf, float* ru_tendf, float* rv_tendf,
