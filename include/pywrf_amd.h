@@ -248,6 +248,20 @@ int pywrf_vertical_diffusion_mp(const pywrf_grid_t* g, const float* field, float
 int pywrf_vertical_diffusion_3dmp(const pywrf_grid_t* g, const float* field, float* tendency,
                                   const float* base_3d, const float* alt, const float* mut, const float* rdn,
                                   const float* rdnw, double kvdif, pywrf_stream_t s);
+/* The eight diffusion calls of the first stage (solver.py:1079-1141):
+ * horizontal_diffusion 'u', 'v', 'w' (coefficient xkmh), horizontal_diffusion_3dmp
+ * of t about t_init (xkhh), vertical_diffusion_u, _v, vertical_diffusion 'w'
+ * (mu: mut) with kvdif_m and vertical_diffusion_3dmp of t about t_init (mut)
+ * with kvdif_t, in one launch with the same result as the calls in that
+ * order.  The four tendencies are distinct arrays that no input aliases.  No
+ * counterpart in the reference: a fusion. */
+int pywrf_diffusion_dry(const pywrf_grid_t* g, const float* u, float* ru_tendf, const float* v, float* rv_tendf,
+                        const float* w, float* rw_tendf, const float* t, float* t_tendf, const float* t_init,
+                        const float* mut, const float* msfux, const float* msfuy, const float* msfvx,
+                        const float* msfvx_inv, const float* msfvy, const float* msftx, const float* msfty,
+                        const float* xkmh, const float* xkhh, const float* u_base, const float* v_base,
+                        const float* alt, const float* muu, const float* muv, const float* rdn, const float* rdnw,
+                        double kvdif_m, double kvdif_t, double rdx, double rdy, pywrf_stream_t s);
 /* rk_addtend_dry, wrf_dynamics.py:3100-3148 */
 int pywrf_rk_addtend_dry(const pywrf_grid_t* g, float* ru_tend, float* rv_tend, float* rw_tend, float* ph_tend,
                          float* t_tend, float* ru_tendf, float* rv_tendf, float* rw_tendf, float* ph_tendf,

@@ -488,19 +488,31 @@ __global__ void k_hdiff(Box b, Dims D, HDiff a) {
   a.tend[q] = hdiff_apply(D, a, k, j, i, a.tend[q]);
 }
 
-int pw_horizontal_diffusion(const pywrf_grid_t* g, char name, const float* field, float* tendency,
-                            const float* base, const float* mut, const float* msfux, const float* msfuy,
-                            const float* msfvx, const float* msfvx_inv, const float* msfvy, const float* msftx,
-                            const float* msfty, const float* xkmhd, float rdx, float rdy, pywrf_stream_t s) {
+// the struct and box of one horizontal_diffusion call
+static int hdiff_setup(const pywrf_grid_t* g, char name, const float* field, float* tendency, const float* base,
+                       const float* mut, const float* msfux, const float* msfuy, const float* msfvx,
+                       const float* msfvx_inv, const float* msfvy, const float* msftx, const float* msfty,
+                       const float* xkmhd, float rdx, float rdy, HDiff& a, Box& b) {
   CHECK(g, 2);
   int ktf = imin(g->kte, g->kde - 1);
-  HDiff a = {field, base, mut, xkmhd, msfux, msfuy, msfvx, msfvx_inv, msfvy, msftx, msfty, tendency, rdx, rdy, name};
+  a = {field, base, mut, xkmhd, msfux, msfuy, msfvx, msfvx_inv, msfvy, msftx, msfty, tendency, rdx, rdy, name};
   int j0 = imax(g->jds + 1, g->jts), i0 = imax(g->ids + 1, g->its);
-  Box b;
   if (name == 'u') b = mkbox(g->kts, ktf, j0, imin(g->jde - 2, g->jte), i0, imin(g->ide - 1, g->ite));
   else if (name == 'v') b = mkbox(g->kts, ktf, j0, imin(g->jde - 1, g->jte), i0, imin(g->ide - 2, g->ite));
   else if (name == 'w') b = mkbox(g->kts + 1, ktf, j0, imin(g->jde - 2, g->jte), i0, imin(g->ide - 2, g->ite));
   else b = mkbox(g->kts, ktf, j0, imin(g->jde - 2, g->jte), i0, imin(g->ide - 2, g->ite));
+  return 0;
+}
+
+int pw_horizontal_diffusion(const pywrf_grid_t* g, char name, const float* field, float* tendency,
+                            const float* base, const float* mut, const float* msfux, const float* msfuy,
+                            const float* msfvx, const float* msfvx_inv, const float* msfvy, const float* msftx,
+                            const float* msfty, const float* xkmhd, float rdx, float rdy, pywrf_stream_t s) {
+  HDiff a;
+  Box b;
+  int e = hdiff_setup(g, name, field, tendency, base, mut, msfux, msfuy, msfvx, msfvx_inv, msfvy, msftx, msfty,
+                      xkmhd, rdx, rdy, a, b);
+  if (e) return e;
   PW_LAUNCH_BOX(k_hdiff, b, s, dims_of(g), a);
   PW_RET();
 }
@@ -537,18 +549,28 @@ __global__ void k_vdiff(Box b, Dims D, VDiff a) {
   a.tend[q] = vdiff_apply(D, a, k, j, i, a.tend[q]);
 }
 
-static int vdiff_launch(const pywrf_grid_t* g, char kind, const float* field, float* tendency, const float* base1,
-                        const float* base3, const float* alt, const float* mus, const float* rdn,
-                        const float* rdnw, double kvdif, pywrf_stream_t s) {
+// the struct and box of one vertical diffusion call
+static int vdiff_setup(const pywrf_grid_t* g, char kind, const float* field, float* tendency, const float* base1,
+                       const float* base3, const float* alt, const float* mus, const float* rdn, const float* rdnw,
+                       double kvdif, VDiff& a, Box& b) {
   CHECK(g, 1);
   int ktf = imin(g->kte, g->kde - 1);
   if (ktf + 1 > g->nz) return PYWRF_E_RANGE;
-  VDiff a = {field, alt, mus, rdn, rdnw, base1, base3, tendency, F(kvdif), ktf, g->kts, kind};
+  a = {field, alt, mus, rdn, rdnw, base1, base3, tendency, F(kvdif), ktf, g->kts, kind};
   int jtf = imin(g->jte, g->jde - 1), itf = imin(g->ite, g->ide - 1);
-  Box b;
   if (kind == 'u') b = mkbox(g->kts + 1, ktf - 1, g->jts, jtf, imax(g->ids + 1, g->its), imin(g->ide - 1, g->ite));
   else if (kind == 'v') b = mkbox(g->kts + 1, ktf - 1, g->jts, jtf, g->its, itf);
   else b = mkbox(g->kts + 1, ktf, g->jts, jtf, g->its, itf);
+  return 0;
+}
+
+static int vdiff_launch(const pywrf_grid_t* g, char kind, const float* field, float* tendency, const float* base1,
+                        const float* base3, const float* alt, const float* mus, const float* rdn,
+                        const float* rdnw, double kvdif, pywrf_stream_t s) {
+  VDiff a;
+  Box b;
+  int e = vdiff_setup(g, kind, field, tendency, base1, base3, alt, mus, rdn, rdnw, kvdif, a, b);
+  if (e) return e;
   PW_LAUNCH_BOX(k_vdiff, b, s, dims_of(g), a);
   PW_RET();
 }
@@ -587,6 +609,80 @@ extern "C" int pywrf_vertical_diffusion_3dmp(const pywrf_grid_t* g, const float*
                                              const float* rdn, const float* rdnw, double kvdif, pywrf_stream_t s) {
   if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   return vdiff_launch(g, '3', field, tendency, nullptr, base_3d, alt, mut, rdn, rdnw, kvdif, s);
+}
+
+// ------------------------------------------------------------------ the first stage's diffusion
+// Array A of the four (u, v, w, theta): its tendency at the point gains the
+// horizontal then the vertical term, as the two calls add them one after the
+// other; neither term reads a tendency.
+struct DiffDry {
+  HDiff h[4];
+  VDiff v[4];
+  Box hb[4], vb[4];
+  int nk;
+};
+
+template <int A>
+__device__ inline void diff_dry_point(const Dims& D, const DiffDry& a, int k, int j, int i) {
+  const bool hh = a.hb[A].has(k, j, i), hv = a.vb[A].has(k, j, i);
+  if (!hh && !hv) return;
+  const long q = D.ix(k, j, i);
+  float t = a.h[A].tend[q];
+  if (hh) t = hdiff_apply(D, a.h[A], k, j, i, t);
+  if (hv) t = vdiff_apply(D, a.v[A], k, j, i, t);
+  a.h[A].tend[q] = t;
+}
+
+// blockIdx.z = k - b.k0 + nk * array, so that every block works on one array
+__global__ void k_diff_dry(Box b, Dims D, DiffDry a) {
+  const int i = b.i0 + (int)(blockIdx.x * PW_TX + threadIdx.x), j = b.j0 + (int)(blockIdx.y * PW_TY + threadIdx.y);
+  const int arr = (int)blockIdx.z / a.nk, k = b.k0 + (int)blockIdx.z - arr * a.nk;
+  if (i >= b.i1 || j >= b.j1) return;
+  switch (arr) {
+    case 0: diff_dry_point<0>(D, a, k, j, i); break;
+    case 1: diff_dry_point<1>(D, a, k, j, i); break;
+    case 2: diff_dry_point<2>(D, a, k, j, i); break;
+    default: diff_dry_point<3>(D, a, k, j, i); break;
+  }
+}
+
+extern "C" int pywrf_diffusion_dry(const pywrf_grid_t* g, const float* u, float* ru_tendf, const float* v,
+                                   float* rv_tendf, const float* w, float* rw_tendf, const float* t,
+                                   float* t_tendf, const float* t_init, const float* mut, const float* msfux,
+                                   const float* msfuy, const float* msfvx, const float* msfvx_inv,
+                                   const float* msfvy, const float* msftx, const float* msfty, const float* xkmh,
+                                   const float* xkhh, const float* u_base, const float* v_base, const float* alt,
+                                   const float* muu, const float* muv, const float* rdn, const float* rdnw,
+                                   double kvdif_m, double kvdif_t, double rdx, double rdy, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
+  DiffDry a;
+  int e = 0;
+  const float* fld[4] = {u, v, w, t};
+  float* tend[4] = {ru_tendf, rv_tendf, rw_tendf, t_tendf};
+  const char hn[4] = {'u', 'v', 'w', 'm'};
+  for (int n = 0; n < 4 && !e; ++n)
+    e = hdiff_setup(g, hn[n], fld[n], tend[n], n == 3 ? t_init : nullptr, mut, msfux, msfuy, msfvx, msfvx_inv,
+                    msfvy, msftx, msfty, n == 3 ? xkhh : xkmh, F(rdx), F(rdy), a.h[n], a.hb[n]);
+  if (!e) e = vdiff_setup(g, 'u', u, ru_tendf, u_base, nullptr, alt, muu, rdn, rdnw, kvdif_m, a.v[0], a.vb[0]);
+  if (!e) e = vdiff_setup(g, 'v', v, rv_tendf, v_base, nullptr, alt, muv, rdn, rdnw, kvdif_m, a.v[1], a.vb[1]);
+  if (!e) e = vdiff_setup(g, 'w', w, rw_tendf, nullptr, nullptr, alt, mut, rdn, rdnw, kvdif_m, a.v[2], a.vb[2]);
+  if (!e) e = vdiff_setup(g, '3', t, t_tendf, nullptr, t_init, alt, mut, rdn, rdnw, kvdif_t, a.v[3], a.vb[3]);
+  if (e) return e;
+  // the launch covers the union of the eight boxes (empty boxes add nothing)
+  Box b = mkbox(g->nz, 0, g->ny, 0, g->nx, 0);
+  for (int n = 0; n < 8; ++n) {
+    const Box& c = n < 4 ? a.hb[n] : a.vb[n - 4];
+    if (c.size() == 0) continue;
+    b.k0 = imin(b.k0, c.k0); b.k1 = imax(b.k1, c.k1);
+    b.j0 = imin(b.j0, c.j0); b.j1 = imax(b.j1, c.j1);
+    b.i0 = imin(b.i0, c.i0); b.i1 = imax(b.i1, c.i1);
+  }
+  if (b.size() == 0) return 0;
+  a.nk = b.nk();
+  if ((long)a.nk * 4 > 65535) return PYWRF_E_RANGE;
+  dim3 grd((unsigned)((b.ni() + PW_TX - 1) / PW_TX), (unsigned)((b.nj() + PW_TY - 1) / PW_TY), (unsigned)(4 * a.nk));
+  hipLaunchKernelGGL(k_diff_dry, grd, dim3(PW_TX, PW_TY), 0, (hipStream_t)s, b, dims_of(g), a);
+  PW_RET();
 }
 
 // ------------------------------------------------------------------ rk_addtend_dry

@@ -235,22 +235,27 @@ def rk3_step(s, ops, cfg, check_nan=False):
                                                               rdy, *I18)
         if rk_step == 1:
             hd = (s.msfux, s.msfuy, s.msfvx, s.msfvx_inv, s.msfvy, s.msftx, s.msfty)
-            s.ru_tendf = o.horizontal_diffusion('u', s.u, s.ru_tendf, s.mut, s.c1h, s.c2h, *hd, KHDIF, s.xkmh,
-                                                rdx, rdy, *I18)
-            s.rv_tendf = o.horizontal_diffusion('v', s.v, s.rv_tendf, s.mut, s.c1h, s.c2h, *hd, KHDIF, s.xkmh,
-                                                rdx, rdy, *I18)
-            s.rw_tendf = o.horizontal_diffusion('w', s.w, s.rw_tendf, s.mut, s.c1f, s.c2f, *hd, KHDIF, s.xkmh,
-                                                rdx, rdy, *I18)
-            s.t_tendf = o.horizontal_diffusion_3dmp('m', s.t, s.t_tendf, s.mut, s.c1h, s.c2h, s.t_init, *hd,
-                                                    3. * KHDIF, s.xkhh, rdx, rdy, *I18)
-            s.ru_tendf = o.vertical_diffusion_u(s.u, s.ru_tendf, s.u_base, s.c1h, s.c2h, s.alt, s.muu, s.rdn,
-                                                s.rdnw, KVDIF, *I18)
-            s.rv_tendf = o.vertical_diffusion_v(s.v, s.rv_tendf, s.v_base, s.c1h, s.c2h, s.alt, s.muv, s.rdn,
-                                                s.rdnw, KVDIF, *I18)
-            s.rw_tendf = o.vertical_diffusion('w', s.w, s.rw_tendf, s.c1f, s.c2f, s.alt, s.mut, s.rdn, s.rdnw,
-                                              KVDIF, *I18)
-            s.t_tendf = o.vertical_diffusion_3dmp(s.t, s.t_tendf, s.t_init, s.c1h, s.c2h, s.alt, s.mut, s.rdn,
-                                                  s.rdnw, 3. * KVDIF, *I18)
+            diff_args = (
+                ('u', s.u, s.ru_tendf, s.mut, s.c1h, s.c2h, *hd, KHDIF, s.xkmh, rdx, rdy, *I18),
+                ('v', s.v, s.rv_tendf, s.mut, s.c1h, s.c2h, *hd, KHDIF, s.xkmh, rdx, rdy, *I18),
+                ('w', s.w, s.rw_tendf, s.mut, s.c1f, s.c2f, *hd, KHDIF, s.xkmh, rdx, rdy, *I18),
+                ('m', s.t, s.t_tendf, s.mut, s.c1h, s.c2h, s.t_init, *hd, 3. * KHDIF, s.xkhh, rdx, rdy, *I18),
+                (s.u, s.ru_tendf, s.u_base, s.c1h, s.c2h, s.alt, s.muu, s.rdn, s.rdnw, KVDIF, *I18),
+                (s.v, s.rv_tendf, s.v_base, s.c1h, s.c2h, s.alt, s.muv, s.rdn, s.rdnw, KVDIF, *I18),
+                ('w', s.w, s.rw_tendf, s.c1f, s.c2f, s.alt, s.mut, s.rdn, s.rdnw, KVDIF, *I18),
+                (s.t, s.t_tendf, s.t_init, s.c1h, s.c2h, s.alt, s.mut, s.rdn, s.rdnw, 3. * KVDIF, *I18))
+            if hasattr(o, "diffusion_dry"):
+                o.diffusion_dry(*diff_args)
+            else:
+                hu, hv, hw, ht, vu, vv, vw, vt = diff_args
+                s.ru_tendf = o.horizontal_diffusion(*hu)
+                s.rv_tendf = o.horizontal_diffusion(*hv)
+                s.rw_tendf = o.horizontal_diffusion(*hw)
+                s.t_tendf = o.horizontal_diffusion_3dmp(*ht)
+                s.ru_tendf = o.vertical_diffusion_u(*vu)
+                s.rv_tendf = o.vertical_diffusion_v(*vv)
+                s.rw_tendf = o.vertical_diffusion(*vw)
+                s.t_tendf = o.vertical_diffusion_3dmp(*vt)
             # relax_bdy_dry (solver.py:1146-1203)
             rb = lambda fld, tend, key, var, ix=I24: o.relax_bdytend(
                 fld, tend, *b[key], var, SPEC_BDY_WIDTH, SPEC_ZONE, RELAX_ZONE, dtbc, s.fcx, s.gcx, *ix)

@@ -619,6 +619,52 @@ def vertical_diffusion_3dmp(field, tendency,
     return tendency
 
 
+def diffusion_dry(hu_args, hv_args, hw_args, ht_args, vu_args, vv_args, vw_args, vt_args):
+    """The first stage's eight diffusion calls (solver.py:1079-1141), each given
+    as the tuple of its positional arguments: horizontal_diffusion 'u', 'v',
+    'w', horizontal_diffusion_3dmp, vertical_diffusion_u, vertical_diffusion_v,
+    vertical_diffusion 'w' and vertical_diffusion_3dmp, in one launch.  Each
+    vertical call acts on its horizontal call's field and tendency; the calls
+    share mut, map factors, alt, rdn, rdnw, the index set and grid spacing,
+    the momentum calls their xkmhd and kvdif; the four tendencies are distinct
+    and alias no input.  Returns the four tendencies.  Not part of the
+    reference API."""
+    hs = [_bind(horizontal_diffusion, a) for a in (hu_args, hv_args, hw_args)] + [
+        _bind(horizontal_diffusion_3dmp, ht_args)]
+    vu, vv = _bind(vertical_diffusion_u, vu_args), _bind(vertical_diffusion_v, vv_args)
+    vw, vt = _bind(vertical_diffusion, vw_args), _bind(vertical_diffusion_3dmp, vt_args)
+    vs = [vu, vv, vw, vt]
+    hu = hs[0]
+    idx = ("ids", "ide", "jds", "jde", "kds", "kde", "ims", "ime", "jms", "jme", "kms", "kme", "its", "ite", "jts",
+           "jte", "kts", "kte")
+    shared = ("mut", "msfux", "msfuy", "msfvx", "msfvx_inv", "msfvy", "msftx", "msfty")
+    same = [(h.field, v.field) for h, v in zip(hs, vs)] + [(h.tendency, v.tendency) for h, v in zip(hs, vs)]
+    same += [(getattr(h, n), getattr(hu, n)) for h in hs[1:] for n in shared]
+    same += [(h.xkmhd, hu.xkmhd) for h in hs[1:3]] + [(hs[3].base_3d, vt.base_3d), (vw.MUT, hu.mut),
+                                                      (vt.MUT, hu.mut)]
+    same += [(getattr(v, n), getattr(vu, n)) for v in vs[1:] for n in ("alt", "rdn", "rdnw")]
+    tends = [h.tendency for h in hs]
+    inputs = [h.field for h in hs] + [hu.mut, hs[3].base_3d, hu.xkmhd, hs[3].xkmhd, vu.u_base, vv.v_base, vu.alt,
+                                      vu.muu, vv.muv] + [getattr(hu, n) for n in shared]
+    if (hs[0].name != "u" or hs[1].name != "v" or hs[2].name != "w" or vw.name != "w"
+            or not all(x is y for x, y in same)
+            or any(h.rdx != hu.rdx or h.rdy != hu.rdy for h in hs)
+            or vv.kvdif != vu.kvdif or vw.kvdif != vu.kvdif
+            or any(getattr(c, n) != getattr(hu, n) for c in hs[1:] + vs for n in idx)):
+        raise ValueError("diffusion_dry: the eight calls do not share their common arguments")
+    ptrs = {t.data_ptr() for t in tends}
+    if len(ptrs) != 4 or any(x.data_ptr() in ptrs for x in inputs):
+        raise ValueError("diffusion_dry: the tendencies must be four distinct arrays that alias no input")
+    g = grid(hu.field.shape, tuple(getattr(hu, n) for n in idx))
+    call("pywrf_diffusion_dry", g, ptr(hs[0].field), ptr(tends[0]), ptr(hs[1].field), ptr(tends[1]),
+         ptr(hs[2].field), ptr(tends[2]), ptr(hs[3].field), ptr(tends[3]), ptr(hs[3].base_3d), ptr(hu.mut),
+         ptr(hu.msfux), ptr(hu.msfuy), ptr(hu.msfvx), ptr(hu.msfvx_inv), ptr(hu.msfvy), ptr(hu.msftx),
+         ptr(hu.msfty), ptr(hu.xkmhd), ptr(hs[3].xkmhd), ptr(vu.u_base), ptr(vv.v_base), ptr(vu.alt), ptr(vu.muu),
+         ptr(vv.muv), ptr(vu.rdn), ptr(vu.rdnw), float(vu.kvdif), float(vt.kvdif), float(hu.rdx), float(hu.rdy),
+         stream())
+    return tuple(tends)
+
+
 def rk_addtend_dry(ru_tend, rv_tend, rw_tend, ph_tend, t_tend,
                    ru_tendf, rv_tendf, rw_tendf, ph_tendf, t_tendf,
                    u_save, v_save, w_save, ph_save, t_save,

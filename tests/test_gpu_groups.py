@@ -291,3 +291,51 @@ def test_pgf_buoy_coriolis_curvature_equals_the_three_calls():
     torch.cuda.synchronize()
     for n, (a, b) in enumerate(zip(ref, got)):
         assert torch.equal(a, b), n
+
+
+@pytest.mark.parametrize("config", ["config1c", "config4"])
+def test_diffusion_dry_equals_the_eight_calls(config):
+    """The first stage's fused diffusion entry point against the eight
+    horizontal/vertical diffusion calls in the order solver.py:1079-1141 makes
+    them, bit for bit, with random tendencies, positive random eddy
+    coefficients and non-zero vertical coefficients."""
+    from pywrf_amd.step import RunConfig
+    cfg = RunConfig()
+    s, o = _stepped(config)
+    I18 = s.grid.ix18
+    gen = torch.Generator(device="cuda").manual_seed(29)
+    t0 = [torch.randn(x.shape, device="cuda", generator=gen) for x in (s.ru_tendf, s.rv_tendf, s.rw_tendf, s.t_tendf)]
+    xkmh = torch.rand(s.xkmh.shape, device="cuda", generator=gen) * 50.
+    xkhh = torch.rand(s.xkhh.shape, device="cuda", generator=gen) * 150.
+    hd = (s.msfux, s.msfuy, s.msfvx, s.msfvx_inv, s.msfvy, s.msftx, s.msfty)
+    rdx, rdy, kv = cfg.rdx, cfg.rdy, 0.7
+
+    def args(ru, rv, rw, tt):
+        return (('u', s.u, ru, s.mut, s.c1h, s.c2h, *hd, 0., xkmh, rdx, rdy, *I18),
+                ('v', s.v, rv, s.mut, s.c1h, s.c2h, *hd, 0., xkmh, rdx, rdy, *I18),
+                ('w', s.w, rw, s.mut, s.c1f, s.c2f, *hd, 0., xkmh, rdx, rdy, *I18),
+                ('m', s.t, tt, s.mut, s.c1h, s.c2h, s.t_init, *hd, 0., xkhh, rdx, rdy, *I18),
+                (s.u, ru, s.u_base, s.c1h, s.c2h, s.alt, s.muu, s.rdn, s.rdnw, kv, *I18),
+                (s.v, rv, s.v_base, s.c1h, s.c2h, s.alt, s.muv, s.rdn, s.rdnw, kv, *I18),
+                ('w', s.w, rw, s.c1f, s.c2f, s.alt, s.mut, s.rdn, s.rdnw, kv, *I18),
+                (s.t, tt, s.t_init, s.c1h, s.c2h, s.alt, s.mut, s.rdn, s.rdnw, 3. * kv, *I18))
+    ref = [t.clone() for t in t0]
+    hu, hv, hw, ht, vu, vv, vw, vt = args(*ref)
+    o.horizontal_diffusion(*hu)
+    o.horizontal_diffusion(*hv)
+    o.horizontal_diffusion(*hw)
+    o.horizontal_diffusion_3dmp(*ht)
+    o.vertical_diffusion_u(*vu)
+    o.vertical_diffusion_v(*vv)
+    o.vertical_diffusion(*vw)
+    o.vertical_diffusion_3dmp(*vt)
+    got = [t.clone() for t in t0]
+    o.diffusion_dry(*args(*got))
+    torch.cuda.synchronize()
+    for n, (a, b) in enumerate(zip(ref, got)):
+        assert not torch.equal(a, t0[n]), n
+        assert torch.equal(a, b), n
+    bad = list(args(*got))
+    bad[5] = (s.u,) + bad[5][1:]  # the v vertical call on another field
+    with pytest.raises(ValueError):
+        o.diffusion_dry(*bad)
