@@ -261,14 +261,15 @@ def test_fresh_first_writers_equal_zeroing_then_the_call():
         assert torch.equal(ref, got), name
 
 
-def test_pgf_buoy_coriolis_curvature_equals_the_three_calls():
+@pytest.mark.parametrize("config", ["config1c", "config4"])
+def test_pgf_buoy_coriolis_curvature_equals_the_three_calls(config):
     """The fused pressure-gradient / buoyancy / Coriolis-curvature entry point
     against horizontal_pressure_gradient, pg_buoy_w and coriolis_curvature in
     sequence, bit for bit (tendencies and cqw), on the state after one step
     with random incoming tendencies."""
     from pywrf_amd.step import RunConfig, G
     cfg = RunConfig()
-    s, o = _stepped()
+    s, o = _stepped(config)
     I18 = s.grid.ix18
     torch.manual_seed(13)
     t0 = [torch.randn_like(s.ru_tend) for _ in range(3)]
