@@ -413,6 +413,7 @@ __device__ inline float mut_dvdxi(const Dims& D, const AdvMuT& a, int k, int j, 
 __global__ void __launch_bounds__(TB) k_advance_mu_t(Dims D, AdvMuT a) {
   extern __shared__ float sm[];
   __shared__ float s_dnw[CB_MAXNZ], s_fnm[CB_MAXNZ], s_fnp[CB_MAXNZ], s_rdnw[CB_MAXNZ];
+  __shared__ float s_part[(CB_MAXNZ / 16 + 1) * TI];
   const int nz = D.nz;
   for (int t = threadIdx.x; t < nz; t += TB) {
     s_dnw[t] = a.dnw[t]; s_fnm[t] = a.fnm[t]; s_fnp[t] = a.fnp[t]; s_rdnw[t] = a.rdnw[t];
@@ -448,8 +449,26 @@ __global__ void __launch_bounds__(TB) k_advance_mu_t(Dims D, AdvMuT a) {
     a.mu[q2] = a.mu_in[q2];
   }
   __syncthreads();
+  // The column sum dmdt in torch's order (torch_outer_sum with nz <= 128:
+  // blocks of 16 levels summed from +0 one after the other, the tail from +0,
+  // tail + (sum of the block sums) + 0 + 0): each block's chain and the
+  // tail's are independent, so the groups form them side by side.
+  const int nb = nz >> 4;
+  for (int c = grp; c <= nb; c += TG) {
+    float acc = 0.0f;
+    if (h) {
+      const int k0 = c << 4, k1 = imin(k0 + 16, nz);
+      for (int k = k0; k < k1; ++k) acc += k < a.k1 ? s_dnw[k] * s_dv[k * TI + lane] : 0.0f;
+    }
+    s_part[c * TI + lane] = acc;
+  }
+  __syncthreads();
   if (grp == 0 && h) {
-    float dmdt = torch_outer_sum(nz, [&](int k) { return k < a.k1 ? s_dnw[k] * s_dv[k * TI + lane] : 0.0f; });
+    float blocks = 0.0f;
+    for (int c = 0; c < nb; ++c) blocks += s_part[c * TI + lane];
+    float dmdt = s_part[nb * TI + lane] + blocks;
+    dmdt = dmdt + 0.0f;
+    dmdt = dmdt + 0.0f;
     float mt = a.mu_tend[q2];
     float mu_old = a.mu_in[q2];
     float mu = mu_old + a.dts * (dmdt + mt);
@@ -457,11 +476,22 @@ __global__ void __launch_bounds__(TB) k_advance_mu_t(Dims D, AdvMuT a) {
     a.mudf[q2] = dmdt + mt;
     a.muts[q2] = a.mut[q2] + mu;
     a.muave[q2] = 0.5f * (a.ep1 * mu + a.em1 * (mu_old + 0.0f));
-    float my = a.msfty[q2];
+    s_part[lane] = dmdt;
+  }
+  __syncthreads();
+  // the eta-velocity decrements dnw(k-1) (dmdt + dvdxi(k-1) + mu_tend) / msfty
+  // of every level at once (in place of dvdxi(k-1), which nothing reads again)
+  if (h) {
+    const float dmdt = s_part[lane], mt = a.mu_tend[q2], my = a.msfty[q2];
+    for (int k = 1 + grp; k < a.k1; k += TG)
+      s_dv[(k - 1) * TI + lane] = s_dnw[k - 1] * (dmdt + s_dv[(k - 1) * TI + lane] + mt) / my;
+  }
+  __syncthreads();
+  if (grp == 0 && h) {
     float wk = a.ww_in[D.ix(0, j, i)];
     s_ww[lane] = wk;
     for (int k = 1; k < a.k1; ++k) {
-      wk = wk - s_dnw[k - 1] * (dmdt + s_dv[(k - 1) * TI + lane] + mt) / my;
+      wk = wk - s_dv[(k - 1) * TI + lane];
       s_ww[k * TI + lane] = wk;
     }
   }
