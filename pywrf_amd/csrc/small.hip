@@ -488,9 +488,22 @@ __global__ void __launch_bounds__(TB) k_advance_mu_t(Dims D, AdvMuT a) {
   }
   __syncthreads();
   if (grp == 0 && h) {
+    // the decrements 8 levels at a time into registers first (s_dv and s_ww
+    // share one LDS buffer, so a read after a store to s_ww would wait for it)
     float wk = a.ww_in[D.ix(0, j, i)];
     s_ww[lane] = wk;
-    for (int k = 1; k < a.k1; ++k) {
+    int k = 1;
+    for (; k + 7 < a.k1; k += 8) {
+      float dv[8];
+#pragma unroll
+      for (int m = 0; m < 8; ++m) dv[m] = s_dv[(k - 1 + m) * TI + lane];
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        wk = wk - dv[m];
+        s_ww[(k + m) * TI + lane] = wk;
+      }
+    }
+    for (; k < a.k1; ++k) {
       wk = wk - s_dv[(k - 1) * TI + lane];
       s_ww[k * TI + lane] = wk;
     }
@@ -693,14 +706,42 @@ __global__ void __launch_bounds__(TB) k_advance_w(Dims D, AdvW a) {
   }
   __syncthreads();
   // phase 3
+  // The sweeps take their operands 8 levels at a time into registers before
+  // the dependent arithmetic: s_w, s_a, s_al, s_ga are runtime offsets into
+  // one LDS buffer, so a read placed after a store to s_w would be made to
+  // wait for it.
   if (grp == 0 && h) {
     float wm = s_w[lane];
-    for (int k = 1; k <= k1; ++k) {
+    int k = 1;
+    for (; k + 7 <= k1; k += 8) {
+      float wv[8], av[8], lv[8];
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        wv[m] = s_w[(k + m) * TI + lane]; av[m] = s_a[(k + m) * TI + lane]; lv[m] = s_al[(k + m) * TI + lane];
+      }
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        wm = (wv[m] - av[m] * wm) * lv[m];
+        s_w[(k + m) * TI + lane] = wm;
+      }
+    }
+    for (; k <= k1; ++k) {
       wm = (s_w[k * TI + lane] - s_a[k * TI + lane] * wm) * s_al[k * TI + lane];
       s_w[k * TI + lane] = wm;
     }
     float wp = wm;
-    for (int k = k1 - 1; k >= 1; --k) {
+    k = k1 - 1;
+    for (; k - 7 >= 1; k -= 8) {
+      float wv[8], gv[8];
+#pragma unroll
+      for (int m = 0; m < 8; ++m) { wv[m] = s_w[(k - m) * TI + lane]; gv[m] = s_ga[(k - m) * TI + lane]; }
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        wp = wv[m] - gv[m] * wp;
+        s_w[(k - m) * TI + lane] = wp;
+      }
+    }
+    for (; k >= 1; --k) {
       wp = s_w[k * TI + lane] - s_ga[k * TI + lane] * wp;
       s_w[k * TI + lane] = wp;
     }
