@@ -377,6 +377,14 @@ __device__ inline float s_yface_v(const AdvIdx& x, int jf, float vel, const floa
   return 0.0f;
 }
 
+__device__ inline float s_xface_v(const AdvIdx& x, int f, float vel, float vel1, const float* Q) {
+  if (f >= x.ids + 3 && f < x.ide - 3) return vel * flux5(Q[0], Q[1], Q[2], Q[3], Q[4], Q[5], vel);
+  if (f == x.ids + 1 || f == x.ide - 2) return 0.5f * vel * (Q[3] + Q[2]);
+  if (f == x.ids + 2) return vel * flux3(Q[1], Q[2], Q[3], Q[4], vel);
+  if (f == x.ide - 3) return vel * flux3(Q[1], Q[2], Q[3], Q[4], vel1);  // sign from ru[ide-2]
+  return 0.0f;
+}
+
 __device__ inline float s_zface_v(const AdvS& a, const AdvIdx& x, int kf, float vel, const float* Q) {
   if (kf < 1 || kf >= x.ktf) return 0.0f;
   if (kf >= 3 && kf < x.ktf - 2) return vel * flux5(Q[0], Q[1], Q[2], Q[3], Q[4], Q[5], -vel);
@@ -461,30 +469,48 @@ __global__ void __launch_bounds__(PW_TX * PW_TY) k_scalar_adv(Dims D, ScalarTend
 #pragma unroll
     for (int o = 0; o < 6; ++o) wz[o] = ZQ(k0 - 3 + o);
     float zlo = live ? s_zface_v(ad, a.x, k0, AT(ad.rom, imin(imax(k0, 0), D.nz - 1), j, ic), wz) : 0.0f;
+    // every operand of a level is loaded before its arithmetic and outside
+    // the branches (indices clamped into the array; values at clamped indices
+    // are never used), so that a level waits on memory once
+    const int jn = imin(j + 1, D.ny - 1);
+    int xi[6], yj[7];
+#pragma unroll
+    for (int o = 0; o < 6; ++o) xi[o] = imin(imax(ic - 3 + o, 0), D.nx - 1);
+#pragma unroll
+    for (int o = 0; o < 7; ++o) yj[o] = imin(imax(j - 3 + o, 0), D.ny - 1);
+    const int ic1 = imin(ic + 1, D.nx - 1);
 #pragma unroll 1
     for (int k = k0; k < k1; ++k) {
-      const float xf = live ? s_xface(D, ad, a.x, k, j, i) : 0.0f;
+      float xq[6], r[7];
+#pragma unroll
+      for (int o = 0; o < 6; ++o) xq[o] = AT(ad.f, k, j, xi[o]);
+#pragma unroll
+      for (int o = 0; o < 7; ++o) r[o] = AT(ad.f, k, yj[o], ic);
+      const float xvel = AT(ad.ru, k, j, ic), xvel1 = AT(ad.ru, k, j, ic1);
+      const float rvl = AT(ad.rv, k, j, ic), rvh = AT(ad.rv, k, jn, ic);
+      const int kf = k + 1;
+      const float romv = AT(ad.rom, imin(kf, D.nz - 1), j, ic);
+      const float wnew = ZQ(k + 3);
+      const float told = a.accumulate ? tbase[im * a.n3 + D.ix(k, j, ic)] : 0.0f;
+      const float rdz = k < b.k1 ? ad.rdzw[k] : 0.0f;
+      const float xf = live ? s_xface_v(a.x, i, xvel, xvel1, xq) : 0.0f;
       const float xr = __shfl_down(xf, 1, PW_TX);
 #pragma unroll
       for (int o = 0; o < 5; ++o) wz[o] = wz[o + 1];
-      wz[5] = ZQ(k + 3);
-      const int kf = k + 1;
-      const float zhi = live ? s_zface_v(ad, a.x, kf, AT(ad.rom, imin(kf, D.nz - 1), j, ic), wz) : 0.0f;
+      wz[5] = wnew;
+      const float zhi = live ? s_zface_v(ad, a.x, kf, romv, wz) : 0.0f;
       if (own && k >= b.k0 && k < b.k1) {
         const int q = im * a.n3 + D.ix(k, j, i);
-        float t = a.accumulate ? tbase[q] : 0.0f;
+        float t = told;
         if (a.badv.has(k, j, i)) {
           if (a.by.has(k, j, i)) {
-            // rows j-3 .. j+3 once for the faces at j (rows j-3 .. j+2) and j+1 (rows j-2 .. j+3)
-            float r[7];
-#pragma unroll
-            for (int o = 0; o < 7; ++o) r[o] = AT(ad.f, k, imin(imax(j - 3 + o, 0), D.ny - 1), i);
-            const float yhi = s_yface_v(a.x, j + 1, AT(ad.rv, k, j + 1, i), r + 1);
-            const float ylo = s_yface_v(a.x, j, AT(ad.rv, k, j, i), r);
+            // rows j-3 .. j+3 serve the faces at j (rows j-3 .. j+2) and j+1 (rows j-2 .. j+3)
+            const float yhi = s_yface_v(a.x, j + 1, rvh, r + 1);
+            const float ylo = s_yface_v(a.x, j, rvl, r);
             t = t - (mx * ad.rdy) * (yhi - ylo);
           }
           if (a.bx.has(k, j, i)) t = t - (mx * ad.rdx) * (xr - xf);
-          t = t - ad.rdzw[k] * (zhi - zlo);
+          t = t - rdz * (zhi - zlo);
         }
         tbase[q] = t;
       }
