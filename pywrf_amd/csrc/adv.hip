@@ -34,54 +34,51 @@ struct AdvU {
   float rdx, rdy;
 };
 
-__device__ inline float u_yface(const Dims& D, const AdvU& a, const AdvIdx& x, int k, int jf, int i) {
-  auto U = [&](int o) { return AT(a.u, k, jf + o, i); };
-  float rvs = AT(a.rv, k, jf, i) + AT(a.rv, k, jf, i - 1);
+// The u faces from operand values: Q[0..5] are u at offsets -3..2 from the
+// face, rvs / rus the sum of the two mass fluxes beside the face, vel the
+// z-face velocity.
+__device__ inline float u_yface_v(const AdvIdx& x, int jf, float rvs, const float* Q) {
   if (jf >= x.jds + 3 && jf < x.jde - 3) {
     float vel = 0.5f * rvs;
-    return vel * flux5(U(-3), U(-2), U(-1), U(0), U(1), U(2), vel);
+    return vel * flux5(Q[0], Q[1], Q[2], Q[3], Q[4], Q[5], vel);
   }
-  if (jf == x.jds + 1 || jf == x.jde - 2) return 0.25f * rvs * (U(0) + U(-1));
+  if (jf == x.jds + 1 || jf == x.jde - 2) return 0.25f * rvs * (Q[3] + Q[2]);
   if (jf == x.jds + 2) {
     float vel = 0.5f * rvs;
-    return vel * flux3(U(-2), U(-1), U(0), U(1), vel);
+    return vel * flux3(Q[1], Q[2], Q[3], Q[4], vel);
   }
-  if (jf == x.jde - 3) {  // sign from the velocity scratch at row jde-2, never set (zero)
+  if (jf == x.jde - 3) {
     float vel = 0.5f * rvs;
-    return vel * flux3(U(-2), U(-1), U(0), U(1), 0.0f);
+    return vel * flux3(Q[1], Q[2], Q[3], Q[4], 0.0f);
   }
   return 0.0f;
 }
 
-__device__ inline float u_xface(const Dims& D, const AdvU& a, const AdvIdx& x, int k, int j, int f) {
-  auto U = [&](int o) { return AT(a.u, k, j, f + o); };
-  float rus = AT(a.ru, k, j, f) + AT(a.ru, k, j, f - 1);
+__device__ inline float u_xface_v(const AdvIdx& x, int f, float rus, const float* Q) {
   if (f >= x.ids + 3 && f < x.ide - 2) {
     float vel = 0.5f * rus;
-    return vel * flux5(U(-3), U(-2), U(-1), U(0), U(1), U(2), vel);
+    return vel * flux5(Q[0], Q[1], Q[2], Q[3], Q[4], Q[5], vel);
   }
   if (f == x.ids + 1) {
-    float ub = U(0) < 0.0f ? U(0) : U(-1);
-    return 0.25f * rus * (U(0) + ub);
+    float ub = Q[3] < 0.0f ? Q[3] : Q[2];
+    return 0.25f * rus * (Q[3] + ub);
   }
   if (f == x.ids + 2 || f == x.ide - 2) {
     float vel = 0.5f * rus;
-    return vel * flux3(U(-2), U(-1), U(0), U(1), vel);
+    return vel * flux3(Q[1], Q[2], Q[3], Q[4], vel);
   }
   if (f == x.ide - 1) {
-    float ub = U(-1) > 0.0f ? U(-1) : U(0);
-    return 0.25f * rus * (U(-1) + ub);
+    float ub = Q[2] > 0.0f ? Q[2] : Q[3];
+    return 0.25f * rus * (Q[2] + ub);
   }
   return 0.0f;
 }
 
-__device__ inline float u_zface(const Dims& D, const AdvU& a, const AdvIdx& x, int kf, int j, int i) {
-  auto U = [&](int o) { return AT(a.u, kf + o, j, i); };
+__device__ inline float u_zface_v(const AdvU& a, const AdvIdx& x, int kf, float vel, const float* Q) {
   if (kf < 1 || kf >= x.ktf) return 0.0f;
-  float vel = 0.5f * (AT(a.rom, kf, j, i - 1) + AT(a.rom, kf, j, i));
-  if (kf >= 3 && kf < x.ktf - 2) return vel * flux5(U(-3), U(-2), U(-1), U(0), U(1), U(2), -vel);
-  if (kf == 1 || kf == x.ktf - 1) return vel * (a.fzm[kf] * U(0) + a.fzp[kf] * U(-1));
-  return vel * flux3(U(-2), U(-1), U(0), U(1), -vel);  // kf == 2 or ktf-2
+  if (kf >= 3 && kf < x.ktf - 2) return vel * flux5(Q[0], Q[1], Q[2], Q[3], Q[4], Q[5], -vel);
+  if (kf == 1 || kf == x.ktf - 1) return vel * (a.fzm[kf] * Q[3] + a.fzp[kf] * Q[2]);
+  return vel * flux3(Q[1], Q[2], Q[3], Q[4], -vel);  // kf == 2 or ktf-2
 }
 
 // fresh: the tendency array is taken as zero (the caller's zeroing is folded
@@ -95,11 +92,25 @@ __global__ void k_advect_u(Box all, Dims D, AdvU a, AdvIdx x, Box by, Box bz, Bo
     if (fresh) a.tend[q] = 0.0f;
     return;
   }
+  // every operand is loaded first, outside the branches, at indices clamped
+  // into the array (the halo check puts every value a face formula uses in
+  // range, so clamping changes only values that are never used)
+  const int jn = imin(j + 1, D.ny - 1), iw = imax(i - 1, 0), ie = imin(i + 1, D.nx - 1), kn = imin(k + 1, D.nz - 1);
+  float uy[7], ux[7], uz[7];
+#pragma unroll
+  for (int o = 0; o < 7; ++o) {
+    uy[o] = AT(a.u, k, imin(imax(j - 3 + o, 0), D.ny - 1), i);
+    ux[o] = AT(a.u, k, j, imin(imax(i - 3 + o, 0), D.nx - 1));
+    uz[o] = AT(a.u, imin(imax(k - 3 + o, 0), D.nz - 1), j, i);
+  }
+  const float rv0 = AT(a.rv, k, j, i), rv0w = AT(a.rv, k, j, iw), rv1 = AT(a.rv, k, jn, i), rv1w = AT(a.rv, k, jn, iw);
+  const float ruw = AT(a.ru, k, j, iw), ru0 = AT(a.ru, k, j, i), rue = AT(a.ru, k, j, ie);
+  const float rmw0 = AT(a.rom, k, j, iw), rm0 = AT(a.rom, k, j, i), rmw1 = AT(a.rom, kn, j, iw), rm1 = AT(a.rom, kn, j, i);
+  const float msf = AT2(a.msfux, j, i), rdz = a.rdzw[k];
   float t = fresh ? 0.0f : a.tend[q];
-  if (by.has(k, j, i))
-    t = t - (AT2(a.msfux, j, i) * a.rdy) * (u_yface(D, a, x, k, j + 1, i) - u_yface(D, a, x, k, j, i));
-  t = t - (AT2(a.msfux, j, i) * a.rdx) * (u_xface(D, a, x, k, j, i + 1) - u_xface(D, a, x, k, j, i));
-  t = t - a.rdzw[k] * (u_zface(D, a, x, k + 1, j, i) - u_zface(D, a, x, k, j, i));
+  if (by.has(k, j, i)) t = t - (msf * a.rdy) * (u_yface_v(x, j + 1, rv1 + rv1w, uy + 1) - u_yface_v(x, j, rv0 + rv0w, uy));
+  t = t - (msf * a.rdx) * (u_xface_v(x, i + 1, rue + ru0, ux + 1) - u_xface_v(x, i, ru0 + ruw, ux));
+  t = t - rdz * (u_zface_v(a, x, k + 1, 0.5f * (rmw1 + rm1), uz + 1) - u_zface_v(a, x, k, 0.5f * (rmw0 + rm0), uz));
   a.tend[q] = t;
 }
 
@@ -139,51 +150,46 @@ struct AdvV {
   float rdx, rdy;
 };
 
-__device__ inline float v_yface(const Dims& D, const AdvV& a, const AdvIdx& x, int k, int f, int i) {
-  auto V = [&](int o) { return AT(a.v, k, f + o, i); };
-  float rvs = AT(a.rv, k, f, i) + AT(a.rv, k, f - 1, i);
+// The v faces from operand values (as the u faces above).
+__device__ inline float v_yface_v(const AdvIdx& x, int f, float rvs, const float* Q) {
   if (f >= x.jds + 3 && f < x.jde - 2) {
     float vel = 0.5f * rvs;
-    return vel * flux5(V(-3), V(-2), V(-1), V(0), V(1), V(2), vel);
+    return vel * flux5(Q[0], Q[1], Q[2], Q[3], Q[4], Q[5], vel);
   }
   if (f == x.jds + 1) {
-    float vb = V(0) < 0.0f ? V(0) : V(-1);
-    return 0.25f * rvs * (V(0) + vb);
+    float vb = Q[3] < 0.0f ? Q[3] : Q[2];
+    return 0.25f * rvs * (Q[3] + vb);
   }
   if (f == x.jds + 2 || f == x.jde - 2) {
     float vel = 0.5f * rvs;
-    return vel * flux3(V(-2), V(-1), V(0), V(1), vel);
+    return vel * flux3(Q[1], Q[2], Q[3], Q[4], vel);
   }
   if (f == x.jde - 1) {
-    float vb = V(-1) > 0.0f ? V(-1) : V(0);
-    return 0.25f * rvs * (V(-1) + vb);
+    float vb = Q[2] > 0.0f ? Q[2] : Q[3];
+    return 0.25f * rvs * (Q[2] + vb);
   }
   return 0.0f;
 }
 
-__device__ inline float v_xface(const Dims& D, const AdvV& a, const AdvIdx& x, int k, int j, int f) {
-  auto V = [&](int o) { return AT(a.v, k, j, f + o); };
-  float rus = AT(a.ru, k, j, f) + AT(a.ru, k, j - 1, f);
+__device__ inline float v_xface_v(const AdvIdx& x, int f, float rus, const float* Q) {
   if (f >= x.ids + 3 && f < x.ide - 3) {
     float vel = 0.5f * rus;
-    return vel * flux5(V(-3), V(-2), V(-1), V(0), V(1), V(2), vel);
+    return vel * flux5(Q[0], Q[1], Q[2], Q[3], Q[4], Q[5], vel);
   }
-  if (f == x.ids + 1 || f == x.ide - 2) return 0.25f * rus * (V(0) + V(-1));
+  if (f == x.ids + 1 || f == x.ide - 2) return 0.25f * rus * (Q[3] + Q[2]);
   if (f == x.ids + 2 || f == x.ide - 3) {
     float vel = 0.5f * rus;
-    return vel * flux3(V(-2), V(-1), V(0), V(1), vel);
+    return vel * flux3(Q[1], Q[2], Q[3], Q[4], vel);
   }
   return 0.0f;
 }
 
-__device__ inline float v_zface(const Dims& D, const AdvV& a, const AdvIdx& x, int kf, int j, int i) {
-  auto V = [&](int o) { return AT(a.v, kf + o, j, i); };
+__device__ inline float v_zface_v(const AdvV& a, const AdvIdx& x, int kf, float vel, const float* Q) {
   if (kf < 1 || kf >= x.ktf) return 0.0f;
-  float vel = 0.5f * (AT(a.rom, kf, j, i) + AT(a.rom, kf, j - 1, i));
-  if (kf >= 3 && kf < x.ktf - 2) return vel * flux5(V(-3), V(-2), V(-1), V(0), V(1), V(2), -vel);
-  if (kf == 1 || kf == x.ktf - 1) return vel * (a.fzm[kf] * V(0) + a.fzp[kf] * V(-1));
-  if (kf == 2) return vel * flux3(V(-2), V(-1), V(0), V(1), -vel);
-  return vel * flux3(V(-2), V(-1), V(0), V(1), vel);  // kf == ktf-2: +vel in the reference
+  if (kf >= 3 && kf < x.ktf - 2) return vel * flux5(Q[0], Q[1], Q[2], Q[3], Q[4], Q[5], -vel);
+  if (kf == 1 || kf == x.ktf - 1) return vel * (a.fzm[kf] * Q[3] + a.fzp[kf] * Q[2]);
+  if (kf == 2) return vel * flux3(Q[1], Q[2], Q[3], Q[4], -vel);
+  return vel * flux3(Q[1], Q[2], Q[3], Q[4], vel);  // kf == ktf-2: +vel in the reference
 }
 
 __global__ void k_advect_v(Box all, Dims D, AdvV a, AdvIdx x, Box bx, Box b, int fresh) {
@@ -194,11 +200,24 @@ __global__ void k_advect_v(Box all, Dims D, AdvV a, AdvIdx x, Box bx, Box b, int
     if (fresh) a.tend[q] = 0.0f;
     return;
   }
+  // every operand is loaded first, outside the branches, at clamped indices
+  // (as in k_advect_u)
+  const int jn = imin(j + 1, D.ny - 1), js = imax(j - 1, 0), ie = imin(i + 1, D.nx - 1), kn = imin(k + 1, D.nz - 1);
+  float vy[7], vx[7], vz[7];
+#pragma unroll
+  for (int o = 0; o < 7; ++o) {
+    vy[o] = AT(a.v, k, imin(imax(j - 3 + o, 0), D.ny - 1), i);
+    vx[o] = AT(a.v, k, j, imin(imax(i - 3 + o, 0), D.nx - 1));
+    vz[o] = AT(a.v, imin(imax(k - 3 + o, 0), D.nz - 1), j, i);
+  }
+  const float rvs_ = AT(a.rv, k, js, i), rv0 = AT(a.rv, k, j, i), rvn = AT(a.rv, k, jn, i);
+  const float ru0 = AT(a.ru, k, j, i), ru0s = AT(a.ru, k, js, i), rue = AT(a.ru, k, j, ie), rues = AT(a.ru, k, js, ie);
+  const float rm0 = AT(a.rom, k, j, i), rm0s = AT(a.rom, k, js, i), rm1 = AT(a.rom, kn, j, i), rm1s = AT(a.rom, kn, js, i);
+  const float my = AT2(a.msfvy, j, i), mvx = AT2(a.msfvx, j, i), rdz = a.rdzw[k];
   float t = fresh ? 0.0f : a.tend[q];
-  float my = AT2(a.msfvy, j, i);
-  t = t - (my * a.rdy) * (v_yface(D, a, x, k, j + 1, i) - v_yface(D, a, x, k, j, i));
-  if (bx.has(k, j, i)) t = t - (my * a.rdx) * (v_xface(D, a, x, k, j, i + 1) - v_xface(D, a, x, k, j, i));
-  t = t - (my / AT2(a.msfvx, j, i)) * a.rdzw[k] * (v_zface(D, a, x, k + 1, j, i) - v_zface(D, a, x, k, j, i));
+  t = t - (my * a.rdy) * (v_yface_v(x, j + 1, rvn + rv0, vy + 1) - v_yface_v(x, j, rv0 + rvs_, vy));
+  if (bx.has(k, j, i)) t = t - (my * a.rdx) * (v_xface_v(x, i + 1, rue + rues, vx + 1) - v_xface_v(x, i, ru0 + ru0s, vx));
+  t = t - (my / mvx) * rdz * (v_zface_v(a, x, k + 1, 0.5f * (rm1 + rm1s), vz + 1) - v_zface_v(a, x, k, 0.5f * (rm0 + rm0s), vz));
   a.tend[q] = t;
 }
 
@@ -239,51 +258,31 @@ struct AdvW {
   float rdx, rdy;
 };
 
-// Horizontal mass flux interpolated to w level k (k < ktf), or extrapolated
-// to the model top (k == ktf).
-__device__ inline float w_level_flux(const Dims& D, const AdvW& a, const float* r, int ktf, int k, int j, int i) {
-  if (k < ktf) return a.fzm[k] * AT(r, k, j, i) + a.fzp[k] * AT(r, k - 1, j, i);
-  return (2.0f - a.fzm[ktf - 1]) * AT(r, ktf - 1, j, i) - a.fzp[ktf - 1] * AT(r, ktf - 2, j, i);
+// Horizontal mass flux interpolated to w level k (k < ktf) from its values rA
+// at level k and rB at k-1, or extrapolated to the model top (k == ktf) from
+// rA at ktf-1 and rB at ktf-2.
+__device__ inline float w_level_flux_v(const AdvW& a, int ktf, int k, float rA, float rB) {
+  if (k < ktf) return a.fzm[k] * rA + a.fzp[k] * rB;
+  return (2.0f - a.fzm[ktf - 1]) * rA - a.fzp[ktf - 1] * rB;
 }
 
-__device__ inline float w_yface(const Dims& D, const AdvW& a, const AdvIdx& x, int k, int f, int i) {
-  auto W = [&](int o) { return AT(a.w, k, f + o, i); };
-  if (f >= x.jds + 3 && f < x.jde - 3) {
-    float vel = w_level_flux(D, a, a.rv, x.ktf, k, f, i);
-    return vel * flux5(W(-3), W(-2), W(-1), W(0), W(1), W(2), vel);
-  }
-  if (f == x.jds + 1 || f == x.jde - 2) return 0.5f * w_level_flux(D, a, a.rv, x.ktf, k, f, i) * (W(0) + W(-1));
-  if (f == x.jds + 2 || f == x.jde - 3) {
-    float vel = w_level_flux(D, a, a.rv, x.ktf, k, f, i);
-    return vel * flux3(W(-2), W(-1), W(0), W(1), vel);
-  }
-  return 0.0f;
-}
-
-__device__ inline float w_xface(const Dims& D, const AdvW& a, const AdvIdx& x, int k, int j, int f) {
-  auto W = [&](int o) { return AT(a.w, k, j, f + o); };
-  if (f >= x.ids + 3 && f < x.ide - 3) {
-    float vel = w_level_flux(D, a, a.ru, x.ktf, k, j, f);
-    return vel * flux5(W(-3), W(-2), W(-1), W(0), W(1), W(2), vel);
-  }
-  if (f == x.ids + 1 || f == x.ide - 2) return 0.5f * w_level_flux(D, a, a.ru, x.ktf, k, j, f) * (W(0) + W(-1));
-  if (f == x.ids + 2 || f == x.ide - 3) {
-    float vel = w_level_flux(D, a, a.ru, x.ktf, k, j, f);
-    return vel * flux3(W(-2), W(-1), W(0), W(1), vel);
-  }
+// The w faces from operand values: Q[0..5] are w at offsets -3..2 from the
+// face, vel the face's mass flux (rs for z: the sum of the two rom values).
+__device__ inline float w_hface_v(int f, int lo, int hi, float vel, const float* Q) {
+  if (f >= lo + 3 && f < hi - 3) return vel * flux5(Q[0], Q[1], Q[2], Q[3], Q[4], Q[5], vel);
+  if (f == lo + 1 || f == hi - 2) return 0.5f * vel * (Q[3] + Q[2]);
+  if (f == lo + 2 || f == hi - 3) return vel * flux3(Q[1], Q[2], Q[3], Q[4], vel);
   return 0.0f;
 }
 
 // Vertical flux at mass level kf (between w[kf-1] and w[kf]), kf in [1, ktf].
-__device__ inline float w_zface(const Dims& D, const AdvW& a, const AdvIdx& x, int kf, int j, int i) {
-  auto W = [&](int o) { return AT(a.w, kf + o, j, i); };
+__device__ inline float w_zface_v(const AdvIdx& x, int kf, float rs, const float* Q) {
   if (kf < 1 || kf > x.ktf) return 0.0f;
-  float rs = AT(a.rom, kf, j, i) + AT(a.rom, kf - 1, j, i);
-  if (kf == 1 || kf == x.ktf) return 0.25f * rs * (W(0) + W(-1));
+  if (kf == 1 || kf == x.ktf) return 0.25f * rs * (Q[3] + Q[2]);
   float vel = 0.5f * rs;
-  if (kf >= 3 && kf < x.ktf - 1) return vel * flux5(W(-3), W(-2), W(-1), W(0), W(1), W(2), -vel);
-  if (kf == 2) return vel * flux3(W(-2), W(-1), W(0), W(1), -vel);
-  return vel * flux3(W(-2), W(-1), W(0), W(1), vel);  // kf == ktf-1: +vel in the reference
+  if (kf >= 3 && kf < x.ktf - 1) return vel * flux5(Q[0], Q[1], Q[2], Q[3], Q[4], Q[5], -vel);
+  if (kf == 2) return vel * flux3(Q[1], Q[2], Q[3], Q[4], -vel);
+  return vel * flux3(Q[1], Q[2], Q[3], Q[4], vel);  // kf == ktf-1: +vel in the reference
 }
 
 __global__ void k_advect_w(Box all, Dims D, AdvW a, AdvIdx x, Box by, Box bx, Box b, int fresh) {
@@ -294,12 +293,32 @@ __global__ void k_advect_w(Box all, Dims D, AdvW a, AdvIdx x, Box by, Box bx, Bo
     if (fresh) a.tend[q] = 0.0f;
     return;
   }
+  // every operand is loaded first, outside the branches, at clamped indices
+  // (as in k_advect_u); the two levels of the horizontal mass fluxes are k
+  // and k-1 below the top, ktf-1 and ktf-2 at it (k is one per workgroup)
+  const int ktf = x.ktf, top = k >= ktf;
+  const int kA = top ? ktf - 1 : k, kB = imax(top ? ktf - 2 : k - 1, 0);
+  const int jn = imin(j + 1, D.ny - 1), ie = imin(i + 1, D.nx - 1);
+  float wy[7], wx[7], wz[7];
+#pragma unroll
+  for (int o = 0; o < 7; ++o) {
+    wy[o] = AT(a.w, k, imin(imax(j - 3 + o, 0), D.ny - 1), i);
+    wx[o] = AT(a.w, k, j, imin(imax(i - 3 + o, 0), D.nx - 1));
+    wz[o] = AT(a.w, imin(imax(k - 3 + o, 0), D.nz - 1), j, i);
+  }
+  const float rvA0 = AT(a.rv, kA, j, i), rvB0 = AT(a.rv, kB, j, i), rvA1 = AT(a.rv, kA, jn, i), rvB1 = AT(a.rv, kB, jn, i);
+  const float ruA0 = AT(a.ru, kA, j, i), ruB0 = AT(a.ru, kB, j, i), ruA1 = AT(a.ru, kA, j, ie), ruB1 = AT(a.ru, kB, j, ie);
+  const float rmn = AT(a.rom, imin(k + 1, D.nz - 1), j, i), rm0 = AT(a.rom, k, j, i), rmb = AT(a.rom, imax(k - 1, 0), j, i);
+  const float mx = AT2(a.msftx, j, i), rdz = a.rdzu[top ? ktf - 1 : k];
   float t = fresh ? 0.0f : a.tend[q];
-  float mx = AT2(a.msftx, j, i);
-  if (by.has(k, j, i)) t = t - (mx * a.rdy) * (w_yface(D, a, x, k, j + 1, i) - w_yface(D, a, x, k, j, i));
-  if (bx.has(k, j, i)) t = t - (mx * a.rdx) * (w_xface(D, a, x, k, j, i + 1) - w_xface(D, a, x, k, j, i));
-  if (k < x.ktf) t = t - a.rdzu[k] * (w_zface(D, a, x, k + 1, j, i) - w_zface(D, a, x, k, j, i));
-  else t = t + 2.0f * a.rdzu[x.ktf - 1] * w_zface(D, a, x, k, j, i);
+  if (by.has(k, j, i))
+    t = t - (mx * a.rdy) * (w_hface_v(j + 1, x.jds, x.jde, w_level_flux_v(a, ktf, k, rvA1, rvB1), wy + 1) -
+                            w_hface_v(j, x.jds, x.jde, w_level_flux_v(a, ktf, k, rvA0, rvB0), wy));
+  if (bx.has(k, j, i))
+    t = t - (mx * a.rdx) * (w_hface_v(i + 1, x.ids, x.ide, w_level_flux_v(a, ktf, k, ruA1, ruB1), wx + 1) -
+                            w_hface_v(i, x.ids, x.ide, w_level_flux_v(a, ktf, k, ruA0, ruB0), wx));
+  if (!top) t = t - rdz * (w_zface_v(x, k + 1, rmn + rm0, wz + 1) - w_zface_v(x, k, rm0 + rmb, wz));
+  else t = t + 2.0f * rdz * w_zface_v(x, k, rm0 + rmb, wz);
   a.tend[q] = t;
 }
 
