@@ -443,7 +443,7 @@ struct ScalarTend {
 // evaluated about once instead of by both neighbours.
 #define SA_OX 63
 #ifndef SA_KC
-#define SA_KC 4
+#define SA_KC 8
 #endif
 #ifndef SA_SPW
 #define SA_SPW 1   // species per wave
