@@ -511,31 +511,49 @@ __global__ void __launch_bounds__(TB) k_advance_mu_t(Dims D, AdvMuT a) {
   __syncthreads();
   if (!h) return;
   const float my = a.msfty[q2], mx = a.msftx[q2];
-  for (int k = grp; k < nz; k += TG) {
-    long q = D.ix(k, j, i);
-    if (k >= a.k1) {
-      a.ww[q] = a.ww_in[q];
-      a.t[q] = a.t_in[q];
-      a.t_ave[q] = a.t_ave_in[q];
-      continue;
-    }
+  // theta and eta velocity of level k < k1 from loads only (indices clamped;
+  // the values taken at clamped indices are not used)
+  auto level = [&](int k, float& o_ww, float& o_t) {
+    const long q = D.ix(k, j, i);
+    const int ku = imin(k + 1, nz - 1), kd = imax(k - 1, 0);
     const float tin = a.t_in[q], ftq = a.ft[q], t1 = a.t_1[q], w1k = a.ww_1[q];
     const float vn = AT(a.v, k, j + 1, i), vs = AT(a.v, k, j, i), ue = AT(a.u, k, j, i + 1), uw = AT(a.u, k, j, i);
     const float tn = AT(a.t_1, k, j + 1, i), ts = AT(a.t_1, k, j - 1, i);
     const float te = AT(a.t_1, k, j, i + 1), tw = AT(a.t_1, k, j, i - 1);
     const bool up = k + 1 < a.k1, dn = k > 0;   // where the eta fluxes wdtn(k + 1), wdtn(k) are nonzero
-    const float w1u = up ? AT(a.ww_1, k + 1, j, i) : 0.0f, t1u = up ? AT(a.t_1, k + 1, j, i) : 0.0f;
-    const float t1d = dn ? AT(a.t_1, k - 1, j, i) : 0.0f;
+    const float w1u = AT(a.ww_1, ku, j, i), t1u = AT(a.t_1, ku, j, i), t1d = AT(a.t_1, kd, j, i);
     const float wwk = s_ww[k * TI + lane] - w1k;
-    a.ww[q] = wwk;
-    a.t_ave[q] = tin + 0.0f;
+    o_ww = wwk;
     float t = tin + my * a.dts * ftq;
-    const float wdu = up ? (s_ww[(k + 1) * TI + lane] - w1u) * (s_fnm[k + 1] * t1u + s_fnp[k + 1] * t1) : 0.0f;
+    const float wdu = up ? (s_ww[ku * TI + lane] - w1u) * (s_fnm[ku] * t1u + s_fnp[ku] * t1) : 0.0f;
     const float wdk = dn ? wwk * (s_fnm[k] * t1 + s_fnp[k] * t1d) : 0.0f;
     float yterm = a.hrdy * (vn * (tn + t1) - vs * (t1 + ts));
     float xterm = a.hrdx * (ue * (te + t1) - uw * (t1 + tw));
-    t = t - a.dts * my * (mx * (yterm + xterm) + s_rdnw[k] * (wdu - wdk));
-    a.t[q] = t;
+    o_t = t - a.dts * my * (mx * (yterm + xterm) + s_rdnw[k] * (wdu - wdk));
+  };
+  // two of the thread's levels at a time, every load before the stores (the
+  // second level clamped to the last one below k1 and stored only if real)
+  for (int k = grp; k < a.k1; k += 2 * TG) {
+    const int k2 = imin(k + TG, a.k1 - 1);
+    float w0, t0, w1, t1;
+    level(k, w0, t0);
+    level(k2, w1, t1);
+    const long q = D.ix(k, j, i), qb = D.ix(k2, j, i);
+    const float ta0 = a.t_in[q] + 0.0f, ta1 = a.t_in[qb] + 0.0f;
+    a.ww[q] = w0;
+    a.t_ave[q] = ta0;
+    a.t[q] = t0;
+    if (k + TG < a.k1) {
+      a.ww[qb] = w1;
+      a.t_ave[qb] = ta1;
+      a.t[qb] = t1;
+    }
+  }
+  for (int k = a.k1 + grp; k < nz; k += TG) {   // levels at and above k1 copy their inputs
+    const long q = D.ix(k, j, i);
+    a.ww[q] = a.ww_in[q];
+    a.t[q] = a.t_in[q];
+    a.t_ave[q] = a.t_ave_in[q];
   }
 }
 
