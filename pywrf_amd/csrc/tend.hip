@@ -20,42 +20,32 @@ struct RhsPh {
   int kte, ids, ide, jds, jde, jts, jtf, its, itf;
 };
 
-// One direction of the horizontal phi advection at (k, j, i); (dj, di) is the
-// unit step, order 6, 4 or 2 (wrf_dynamics.py:2082-2295).
-__device__ inline float ph_hflux(const Dims& D, const RhsPh& a, int k, int j, int i, int dj, int di, int order) {
-  bool top = (k == a.kte - 1);
-  const float* vel = dj ? a.v : a.u;
-  const float* muf = dj ? a.muv : a.muu;
-  const float* msf = dj ? a.msfvy : a.msfux;
-  float c = (top ? (dj ? a.c2y : a.c2x) : (dj ? a.c4y : a.c4x)) / AT2(a.msfty, j, i);
-  float cfn = *a.cfn, cfn1 = *a.cfn1;
-  auto V = [&](int n) {
-    int jj = j + n * dj, ii = i + n * di;
-    if (top) return cfn * AT(vel, k - 1, jj, ii) + cfn1 * AT(vel, k - 2, jj, ii);
-    return AT(vel, k, jj, ii) + AT(vel, k - 1, jj, ii);
-  };
-  auto P = [&](const float* f, int n) { return AT(f, k, j + n * dj, i + n * di); };
-  float F1 = AT2(muf, j + dj, i + di) * V(1) * AT2(msf, j + dj, i + di);
-  float F0 = AT2(muf, j, i) * V(0) * AT2(msf, j, i);
+// One direction of the horizontal phi advection, order 6, 4 or 2
+// (wrf_dynamics.py:2082-2295), from operand values: P[0..6] and B[0..6] are ph
+// and phb at -3..3 steps from the point; vA / vB the normal velocity at the two
+// levels the face value uses (k and k-1, or k-1 and k-2 at the top level, where
+// they are extrapolated with cfn, cfn1) at the point (0) and one step on (1);
+// mu0 / mu1 and m0 / m1 the mass and map factor there.
+__device__ inline float ph_hflux_v(const RhsPh& a, bool top, int order, float c, float vA0, float vB0,
+                                   float vA1, float vB1, float mu0, float mu1, float m0, float m1, const float* P,
+                                   const float* B) {
+  const float cfn = *a.cfn, cfn1 = *a.cfn1;
+  const float V0 = top ? cfn * vA0 + cfn1 * vB0 : vA0 + vB0;
+  const float V1 = top ? cfn * vA1 + cfn1 * vB1 : vA1 + vB1;
+  float F1 = mu1 * V1 * m1;
+  float F0 = mu0 * V0 * m0;
   if (order == 6) {
-    float s = 45.0f * (P(a.ph, 1) - P(a.ph, -1)) - 9.0f * (P(a.ph, 2) - P(a.ph, -2)) + (P(a.ph, 3) - P(a.ph, -3)) +
-              45.0f * (P(a.phb, 1) - P(a.phb, -1)) - 9.0f * (P(a.phb, 2) - P(a.phb, -2)) +
-              (P(a.phb, 3) - P(a.phb, -3));
+    float s = 45.0f * (P[4] - P[2]) - 9.0f * (P[5] - P[1]) + (P[6] - P[0]) + 45.0f * (B[4] - B[2]) -
+              9.0f * (B[5] - B[1]) + (B[6] - B[0]);
     return c * ((F1 + F0) * F(1. / 60.) * s);
   }
   if (order == 4) {
-    float s = 8.0f * (P(a.ph, 1) - P(a.ph, -1)) - (P(a.ph, 2) - P(a.ph, -2)) + 8.0f * (P(a.phb, 1) - P(a.phb, -1)) -
-              (P(a.phb, 2) - P(a.phb, -2));
+    float s = 8.0f * (P[4] - P[2]) - (P[5] - P[1]) + 8.0f * (B[4] - B[2]) - (B[5] - B[1]);
     return c * ((F1 + F0) * F(1. / 12.) * s);
   }
-  float d1 = P(a.phb, 1) - P(a.phb, 0) + P(a.ph, 1) - P(a.ph, 0);
-  float d0 = P(a.phb, 0) - P(a.phb, -1) + P(a.ph, 0) - P(a.ph, -1);
+  float d1 = B[4] - B[3] + P[4] - P[3];
+  float d0 = B[3] - B[2] + P[3] - P[2];
   return c * (F1 * d1 + F0 * d0);
-}
-
-__device__ inline float ph_wdwn(const Dims& D, const RhsPh& a, int k, int j, int i) {
-  return 0.5f * (AT(a.ww, k, j, i) + AT(a.ww, k - 1, j, i)) * a.rdnw[k - 1] *
-         (AT(a.ph, k, j, i) - AT(a.ph, k - 1, j, i) + AT(a.phb, k, j, i) - AT(a.phb, k - 1, j, i));
 }
 
 // fresh: the tendency array is taken as zero (the caller's zeroing is folded
@@ -69,21 +59,48 @@ __global__ void k_rhs_ph(Box all, Dims D, RhsPh a, Box b, int fresh) {
     if (fresh) a.tend[q] = 0.0f;
     return;
   }
+  // every operand is loaded first, outside the branches, at indices clamped
+  // into the array (k is one per workgroup; the halo check puts every value a
+  // flux formula uses in range, so clamping changes only unused values)
+  const bool top = k == a.kte - 1;
+  const int ku = imin(k + 1, D.nz - 1), kA = top ? k - 1 : k, kB = imax(top ? k - 2 : k - 1, 0);
+  const int jn = imin(j + 1, D.ny - 1), ie = imin(i + 1, D.nx - 1);
+  float py[7], by_[7], px[7], bx_[7];
+#pragma unroll
+  for (int o = 0; o < 7; ++o) {
+    const int jj = imin(imax(j - 3 + o, 0), D.ny - 1), ii = imin(imax(i - 3 + o, 0), D.nx - 1);
+    py[o] = AT(a.ph, k, jj, i); by_[o] = AT(a.phb, k, jj, i);
+    px[o] = AT(a.ph, k, j, ii); bx_[o] = AT(a.phb, k, j, ii);
+  }
+  const float wwu = AT(a.ww, ku, j, i), ww0 = AT(a.ww, k, j, i), wwd = AT(a.ww, k - 1, j, i);
+  const float phu = AT(a.ph, ku, j, i), phd = AT(a.ph, k - 1, j, i);
+  const float pbu = AT(a.phb, ku, j, i), pbd = AT(a.phb, k - 1, j, i);
+  const float vA0 = AT(a.v, kA, j, i), vB0 = AT(a.v, kB, j, i), vA1 = AT(a.v, kA, jn, i), vB1 = AT(a.v, kB, jn, i);
+  const float uA0 = AT(a.u, kA, j, i), uB0 = AT(a.u, kB, j, i), uA1 = AT(a.u, kA, j, ie), uB1 = AT(a.u, kB, j, ie);
+  const float mv0 = AT2(a.muv, j, i), mv1 = AT2(a.muv, jn, i), sv0 = AT2(a.msfvy, j, i), sv1 = AT2(a.msfvy, jn, i);
+  const float mu0 = AT2(a.muu, j, i), mu1 = AT2(a.muu, j, ie), su0 = AT2(a.msfux, j, i), su1 = AT2(a.msfux, j, ie);
+  const float mut = AT2(a.mut, j, i), my = AT2(a.msfty, j, i), wq = a.w[q];
   float t = fresh ? 0.0f : a.tend[q];
-  if (k < a.kte - 1) t = t - (a.fnm[k] * ph_wdwn(D, a, k + 1, j, i) + a.fnp[k] * ph_wdwn(D, a, k, j, i));
+  // the vertical phi advection terms at w levels k + 1 and k
+  const float w1 = 0.5f * (wwu + ww0) * a.rdnw[top ? k - 1 : k] * (phu - py[3] + pbu - by_[3]);
+  const float w0 = 0.5f * (ww0 + wwd) * a.rdnw[k - 1] * (py[3] - phd + by_[3] - pbd);
+  if (!top) t = t - (a.fnm[k] * w1 + a.fnp[k] * w0);
   else t = 0.0f;
-  t = t + AT2(a.mut, j, i) * F(pw::g) * a.w[q] / AT2(a.msfty, j, i);
+  t = t + mut * F(pw::g) * wq / my;
   // y direction: 6th order inside, 4th order at jds+2 and jde-4, 2nd order at jds+1 and jde-3
   int oy = 0;
   if (j >= imax(a.jts, a.jds + 3) && j < imin(a.jtf, a.jde - 4)) oy = 6;
   else if (j == a.jds + 2 || j == a.jde - 4) oy = 4;
   else if (j == a.jds + 1 || j == a.jde - 3) oy = 2;
-  if (oy) t = t - ph_hflux(D, a, k, j, i, 1, 0, oy);
+  const float cy = (top ? a.c2y : a.c4y) / my, cx = (top ? a.c2x : a.c4x) / my;
+  const float fy = ph_hflux_v(a, top, oy, cy, vA0, vB0, vA1, vB1, mv0, mv1, sv0, sv1, py, by_);
+  if (oy) t = t - fy;
   // x direction: 6th order inside, 2nd order at ids+1 and ide-3 (no 4th-order columns)
   int ox = 0;
   if (i >= imax(a.its, a.ids + 3) && i < imin(a.itf, a.ide - 4)) ox = 6;
   else if (i == a.ids + 1 || i == a.ide - 3) ox = 2;
-  if (ox) t = t - ph_hflux(D, a, k, j, i, 0, 1, ox);
+  const float fx = ph_hflux_v(a, top, ox, cx, uA0, uB0, uA1, uB1, mu0, mu1, su0, su1, px, bx_);
+  if (ox) t = t - fx;
   a.tend[q] = t;
 }
 
