@@ -30,8 +30,8 @@ PEAK_HBM_GBS = 8000.0
 COPY_REF_FLOATS = 64 * 1024 * 1024   # 256 MiB copy used to scale the HBM byte counters
 # Measured bytes per launch at 240x240x41 (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, the scale of a
 # 256 MiB copy), from profiles/measured_kernels_240x240x41_hbm_bytes.md.  Only valid at that size.
-TRAFFIC_240 = {"advance_w": 214.1e6, "advance_uv": 194.7e6, "advance_mu_t": 132.2e6, "wsm6": 159.1e6,
-               "rk_scalar_tend_moist": 528.6e6}
+TRAFFIC_240 = {"advance_w": 213.9e6, "advance_uv": 194.8e6, "advance_mu_t": 131.8e6, "wsm6": 159.1e6,
+               "rk_scalar_tend_moist": 588.0e6}
 FIELD_PASSES_PER_STEP = 700          # algorithmic 3-D field passes per RK3 step (7 acoustic substeps included)
 
 
