@@ -387,12 +387,25 @@ int pywrf_set_w_surface(const pywrf_grid_t* g, float* w, const float* ht, const 
                         pywrf_stream_t s);
 
 /* ---- lateral boundaries: wrf_boundary.py:52-587 ------------------------ */
-/* Boundary arrays: x-boundaries (kz, ny, 5), y-boundaries (kz, 5, nx), contiguous. */
+/* Boundary arrays in the reference driver's layout (solver.py:468-703): an
+ * x-boundary array (bxs, bxe, btxs, btxe) is a contiguous (5, kz, ny) array,
+ * the storage of the driver's (kz, ny, 5) view permute(1, 2, 0); a y-boundary
+ * array (bys, bye, btys, btye) a contiguous (5, kz, nx) array, the storage of
+ * its (kz, 5, nx) view permute(1, 0, 2).  kz is the level count of the field
+ * (g->nz); the strip width is the reference's spec_bdy_width = 5. */
 /* relax_bdytend, wrf_boundary.py:52-161 (fcx, gcx: 5 floats each) */
 int pywrf_relax_bdytend(const pywrf_grid_t* g, const float* field, float* field_tend, const float* bxs,
                         const float* bxe, const float* bys, const float* bye, const float* btxs, const float* btxe,
                         const float* btys, const float* btye, char variable_in, double dtbc, const float* fcx,
                         const float* gcx, pywrf_stream_t s);
+/* relax_bdytend with the boundary time dtbc read from device memory (one float)
+ * when the kernel runs, so that a captured HIP graph of the step stays valid
+ * as dtbc changes from step to step.  Same arithmetic as pywrf_relax_bdytend.
+ * No counterpart in the reference (its dtbc is a Python float, solver.py:713). */
+int pywrf_relax_bdytend_dtbc_dev(const pywrf_grid_t* g, const float* field, float* field_tend, const float* bxs,
+                                 const float* bxe, const float* bys, const float* bye, const float* btxs,
+                                 const float* btxe, const float* btys, const float* btye, char variable_in,
+                                 const float* dtbc, const float* fcx, const float* gcx, pywrf_stream_t s);
 /* spec_bdytend, wrf_boundary.py:188-225 */
 int pywrf_spec_bdytend(const pywrf_grid_t* g, float* field_tend, const float* btxs, const float* btxe,
                        const float* btys, const float* btye, char variable_in, pywrf_stream_t s);
@@ -429,6 +442,11 @@ int pywrf_relax_bdy_scalar(const pywrf_grid_t* g, float* scalar_tend, const floa
                            const float* bxs, const float* bxe, const float* bys, const float* bye,
                            const float* btxs, const float* btxe, const float* btys, const float* btye, double dtbc,
                            const float* fcx, const float* gcx, pywrf_stream_t s);
+/* relax_bdy_scalar with dtbc in device memory (see pywrf_relax_bdytend_dtbc_dev). */
+int pywrf_relax_bdy_scalar_dtbc_dev(const pywrf_grid_t* g, float* scalar_tend, const float* scalar, const float* mu,
+                                    const float* bxs, const float* bxe, const float* bys, const float* bye,
+                                    const float* btxs, const float* btxe, const float* btys, const float* btye,
+                                    const float* dtbc, const float* fcx, const float* gcx, pywrf_stream_t s);
 /* spec_bdy_scalar, wrf_boundary.py:454-472 */
 int pywrf_spec_bdy_scalar(const pywrf_grid_t* g, float* scalar_tend, const float* btxs, const float* btxe,
                           const float* btys, const float* btye, pywrf_stream_t s);
@@ -453,6 +471,11 @@ int pywrf_spec_bdy_final(const pywrf_grid_t* g, float* field, const float* mu, c
 int pywrf_spec_bdy_final_group(const pywrf_grid_t* g, int n, void* const* fields, void* const* mus,
                                void* const* msfs, void* const* bdys, const char* variables, double dtbc,
                                pywrf_stream_t s);
+/* pywrf_spec_bdy_final_group with dtbc in device memory (see
+ * pywrf_relax_bdytend_dtbc_dev). */
+int pywrf_spec_bdy_final_group_dtbc_dev(const pywrf_grid_t* g, int n, void* const* fields, void* const* mus,
+                                        void* const* msfs, void* const* bdys, const char* variables,
+                                        const float* dtbc, pywrf_stream_t s);
 
 /* ---- moist physics: wrf_physics.py:27-2137 ------------------------------ */
 /* moist_physics_prep_em, wrf_physics.py:27-91 */
@@ -476,6 +499,76 @@ int pywrf_wsm6(const pywrf_grid_t* g, float* th, float* q, float* qc, float* qr,
 int pywrf_moist_physics_finish_em(const pywrf_grid_t* g, float* t_new, const float* th_phy, float* h_diabatic,
                                   double dt, const float* qv, float* qv_diabatic, const float* qc,
                                   float* qc_diabatic, pywrf_stream_t s);
+
+/* ---- WSM6 column surrogate (a learned replacement for wsm6, see below) --- */
+/* wsm6 on a device list of the scheme's tile columns, c = (j - jts) *
+ * (ite - 1 - its) + (i - its) for c in cols[0 .. *ncols): the other columns are
+ * untouched.  The surrogate's fallback; no counterpart in the reference. */
+int pywrf_wsm6_columns(const pywrf_grid_t* g, float* th, float* q, float* qc, float* qr, float* qi, float* qs,
+                       float* qg, const float* den, const float* pii, const float* p, const float* delz, double delt,
+                       float* rain, float* rainncv, float* snow, float* snowncv, float* sr, float* graupel,
+                       float* graupelncv, const int* cols, const int* ncols, pywrf_stream_t s);
+
+/* Dense layers on the matrix cores: C[M x N] = A[M x K] B[N x K]^T with BF16
+ * A and B (row-major, K contiguous, K a multiple of 32, lda / ldb multiples
+ * of 8, 16-byte aligned) and FP32 accumulation.  epi: 0 FP32 C (+ bias[N] if
+ * given), 1 BF16 relu(C + bias), 2 BF16 C * [H > 0] (H BF16 [M x ldh]),
+ * 3 FP32 atomic accumulation into C (K split over ksplit workgroups; C must
+ * start at zero).  No counterpart in the reference (the surrogate is new). */
+int pywrf_mlp_gemm(int epi, int M, int N, int K, const void* A, int lda, const void* B, int ldb, void* C, int ldc,
+                   const float* bias, const void* H, int ldh, int ksplit, pywrf_stream_t s);
+/* out[c][r] = in[r][c], BF16. */
+int pywrf_mlp_transpose(int R, int C, const void* in, int ldi, void* out, int ldo, pywrf_stream_t s);
+/* out[r] = sum_{c < C} in[r][c] (BF16 in, FP32 out): bias gradients. */
+int pywrf_mlp_rowsum(int R, int C, const void* in, int ld, float* out, pywrf_stream_t s);
+/* Weighted squared error of Y (FP32 [B x ldy]) against T over N outputs:
+ * *loss += sum w[n] (Y - T)^2 scale, dY (BF16 [B x lddy]) = 2 w[n] (Y - T) scale
+ * (0 in columns N .. lddy). */
+int pywrf_mlp_mse(int B, int N, const float* Y, int ldy, const float* T, int ldt, const float* w, double scale,
+                  void* dY, int lddy, float* loss, pywrf_stream_t s);
+/* One bias-corrected Adam step of an FP32 [rows x cols] parameter with
+ * gradient g * gscale (step number t >= 1); wb / wtb (optional) receive its
+ * BF16 copy and transposed BF16 copy. */
+int pywrf_mlp_adam(int rows, int cols, float* w, const float* g, float* m, float* v, double lr, double beta1,
+                   double beta2, double eps, double gscale, int t, void* wb, void* wtb, pywrf_stream_t s);
+
+/* The trained network: 369 -> 512 -> 512 -> 287, ReLU.  Inputs per column:
+ * T = th * pii, p, rho, qv, qc, qr, qi, qs, qg over 41 levels (feature
+ * var * 41 + k, padded to 384), standardised with in_mean / in_rstd; outputs:
+ * standardised tendencies of T, qv, qc, qr, qi, qs, qg (feature var * 41 + k,
+ * 287 used of 288), de-standardised with out_mean / out_std.  qtol: the most
+ * negative mixing ratio accepted (smaller ones are clipped to 0); wtol: the
+ * largest accepted relative gain of the column's water. */
+typedef struct pywrf_surrogate {
+  const void* w1; /* BF16 [512 x 384] */
+  const void* w2; /* BF16 [512 x 512] */
+  const void* w3; /* BF16 [288 x 512] (row 287 zero) */
+  const float* b1;
+  const float* b2;
+  const float* b3;
+  const float* in_mean;
+  const float* in_rstd;
+  const float* out_mean;
+  const float* out_std;
+  float qtol, wtol;
+} pywrf_surrogate_t;
+
+/* Bytes of workspace pywrf_wsm6_surrogate needs for grid g (its physics tile)
+ * when it processes `chunk` columns at a time. */
+int pywrf_wsm6_surrogate_workspace(const pywrf_grid_t* g, int chunk, long long* bytes);
+/* microphysics_driver / wsm6 (wrf_physics.py:94-134, 137-1214) replaced by the
+ * surrogate: same fields, tile and outputs as pywrf_wsm6; every column that
+ * fails the checks (negative mixing ratio, water gain, non-finite value) is
+ * recomputed by the plain scheme, and their count is written to *nfallback
+ * (device int).  workspace: device memory of at least the size above for
+ * some chunk >= 128 (larger chunks, fewer launches).  No counterpart in the
+ * reference (its README, README.md:170-173, describes but does not ship the
+ * learned coupling). */
+int pywrf_wsm6_surrogate(const pywrf_grid_t* g, const pywrf_surrogate_t* net, float* th, const float* rho,
+                         const float* pii, const float* p, const float* delz, double dt, float* qv, float* qc,
+                         float* qr, float* qi, float* qs, float* qg, float* rain, float* rainncv, float* snow,
+                         float* snowncv, float* sr, float* graupel, float* graupelncv, void* workspace,
+                         long long workspace_bytes, int* nfallback, pywrf_stream_t s);
 
 #ifdef __cplusplus
 }

@@ -28,6 +28,17 @@ class Grid(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in _GRID_FIELDS]
 
 
+_GridPtr = ctypes.POINTER(Grid)
+
+
+class Surrogate(ctypes.Structure):
+    """pywrf_surrogate_t: device pointers of the surrogate network and its
+    standardisation, and the fallback tolerances."""
+    _fields_ = [(n, ctypes.c_void_p) for n in
+                "w1 w2 w3 b1 b2 b3 in_mean in_rstd out_mean out_std".split()] + [
+               ("qtol", ctypes.c_float), ("wtol", ctypes.c_float)]
+
+
 _CTYPES = {
     "const pywrf_grid_t*": ctypes.POINTER(Grid),
     "const float*": ctypes.c_void_p,
@@ -41,17 +52,24 @@ _CTYPES = {
     "void* const*": ctypes.POINTER(ctypes.c_void_p),
     "const int*": ctypes.POINTER(ctypes.c_int),
     "const long long*": ctypes.POINTER(ctypes.c_longlong),
+    "long long*": ctypes.POINTER(ctypes.c_longlong),
+    "long long": ctypes.c_longlong,
+    "const void*": ctypes.c_void_p,
+    "void*": ctypes.c_void_p,
+    "int*": ctypes.c_void_p,
+    "const pywrf_surrogate_t*": ctypes.c_void_p,
 }
 
 
-def parse_header(path=HEADER):
-    """{name: [ctypes argtypes]} for every `int pywrf_*(...)` declaration."""
+def parse_header(path=HEADER, with_names=False):
+    """{name: [ctypes argtypes]} for every `int pywrf_*(...)` declaration
+    (with_names: {name: [parameter names]} as well)."""
     text = open(path).read()
     text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
-    decls = {}
+    decls, pnames = {}, {}
     for m in re.finditer(r"\bint\s+(pywrf_\w+)\s*\(([^)]*)\)\s*;", text, flags=re.S):
         name, args = m.group(1), " ".join(m.group(2).split())
-        types = []
+        types, names = [], []
         if args not in ("", "void"):
             for a in args.split(","):
                 a = a.strip()
@@ -61,23 +79,25 @@ def parse_header(path=HEADER):
                 if t not in _CTYPES:
                     raise ValueError("unsupported C type %r in %s" % (t, name))
                 types.append(_CTYPES[t])
+                names.append(re.search(r"(\w+)$", a).group(1))
         decls[name] = types
-    return decls
+        pnames[name] = names
+    return (decls, pnames) if with_names else decls
 
 
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError("%s is missing: build the HIP kernels first (python -m pywrf_amd.build)" % LIB_PATH)
     lib = ctypes.CDLL(LIB_PATH)
-    decls = parse_header()
+    decls, pnames = parse_header(with_names=True)
     for name, types in decls.items():
         f = getattr(lib, name)          # AttributeError here means the library lacks a declared symbol
         f.argtypes = types
         f.restype = ctypes.c_int
-    return lib, decls
+    return lib, decls, pnames
 
 
-LIB, DECLS = _load()
+LIB, DECLS, PARAMS = _load()
 
 
 class KernelError(RuntimeError):
@@ -94,7 +114,64 @@ def check(name, rc):
     raise KernelError("%s: HIP error %d" % (name, rc))
 
 
+# Operand roles by C parameter name (include/pywrf_amd.h).  A kernel indexes
+# a 2-D operand over the grid's (ny, nx) plane, a level array over nz entries
+# (or the fixed count given here) and every other array operand over the
+# whole (nz, ny, nx) grid (species arrays: over at least that).
+PLANE_PARAMS = frozenset((
+    "mu mub mut muu muv mu_1 mu_2 mu_in mu_new mu_old mu_out mu_save mu_tend mu_tendf muave muave_in "
+    "muave_out mubf mudf mudf_in mudf_out muf muts muts_in muts_out muuf muus muvf muvs msf msft msftx "
+    "msfty msfu msfux msfuy msfv msfvx msfvx_inv msfvy ht f e sina cosa rain rainnc rainncv snow snownc "
+    "snowncv graupel graupelnc graupelncv sr mu_base").split())
+LEVEL_PARAMS = dict.fromkeys("c1h dn dnw fnm fnp fzm fzp rdn rdnw rdzu u_base v_base base".split())
+LEVEL_PARAMS.update(fcx=5, gcx=5, cf=3, cfn=1, cfn1=1, dtbc=1)
+# names whose role depends on the function: the advection operators' rdzw is
+# the level array rdnw / rdn (wrf_dynamics.py:1112, 1332, 1535, 1841), the
+# diffusion metrics' rdzw a 3-D field
+LEVEL_IN = {"rdzw": "pywrf_advect_"}
+
+
+class TensorPtr(ctypes.c_void_p):
+    """Device pointer of a tensor argument; `shape` is checked against the
+    call's grid before the launch."""
+
+
+class BoundaryPtr(ctypes.c_void_p):
+    """Device pointer of a lateral-boundary array (checked by bptr)."""
+
+
+def check_operands(name, g, params, args):
+    """Raise IndexError when an array operand is smaller than the part of it
+    the kernel addresses for grid g (a Grid), so that no mismatched operand
+    reaches a kernel.  params: the C parameter names, args: the arguments."""
+    nz, ny, nx = g.nz, g.ny, g.nx
+    for pname, a in zip(params, args):
+        if not isinstance(a, TensorPtr):
+            continue
+        shape = a.shape
+        if pname in LEVEL_PARAMS or (pname in LEVEL_IN and name.startswith(LEVEL_IN[pname])):
+            need = LEVEL_PARAMS.get(pname) or nz
+            if pname == "cf" and len(shape) == 0:
+                continue                      # (cf1, cf2, cf3) checked adjacent by the caller
+            n = 1
+            for d in shape:
+                n *= d
+            if n < need:
+                raise IndexError("%s: %s has %d entries, the grid needs %d" % (name, pname, n, need))
+            continue
+        if len(shape) < 2 or tuple(shape[-2:]) != (ny, nx):
+            raise IndexError("%s: %s has shape %s, the grid is (%d, %d, %d)" % (name, pname, tuple(shape), nz, ny, nx))
+        if pname in PLANE_PARAMS:
+            continue
+        lev = shape[-3] if len(shape) >= 3 else 1
+        if lev < nz:
+            raise IndexError("%s: %s has %d levels, the grid needs %d (shape %s)" % (name, pname, lev, nz,
+                                                                                    tuple(shape)))
+
+
 def call(name, *args):
+    if args and isinstance(args[0], _GridPtr):
+        check_operands(name, args[0].contents, PARAMS[name], args)
     check(name, getattr(LIB, name)(*args))
 
 
@@ -137,14 +214,65 @@ def stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
-def ptr(t):
-    """Device pointer of a contiguous FP32 CUDA tensor (raises otherwise)."""
+def _require_device(t):
     if not isinstance(t, torch.Tensor):
         raise TypeError("expected a tensor, got %r" % type(t))
-    if t.device.type != "cuda" or t.dtype != torch.float32 or not t.is_contiguous():
-        raise TypeError("expected a contiguous float32 CUDA tensor, got %s %s contiguous=%s"
-                        % (t.device, t.dtype, t.is_contiguous()))
-    return ctypes.c_void_p(t.data_ptr())
+    if t.device.type != "cuda" or t.dtype != torch.float32:
+        raise TypeError("expected a float32 CUDA tensor, got %s %s" % (t.device, t.dtype))
+
+
+def ptr(t):
+    """Device pointer of a contiguous FP32 CUDA tensor (raises otherwise)."""
+    _require_device(t)
+    if not t.is_contiguous():
+        raise TypeError("expected a contiguous float32 CUDA tensor, got strides %s" % (t.stride(),))
+    p = TensorPtr(t.data_ptr())
+    p.shape = tuple(t.shape)
+    return p
+
+
+_bdy_copies = {}
+
+
+def boundary_layout(t, kind, kz, n):
+    """The boundary array t of the reference driver, as a tensor whose storage
+    has the kernels' layout: an x-boundary (kind 'x', shape (kz, n, 5)) is
+    stored as a contiguous (5, kz, n) array, a y-boundary (kind 'y', shape
+    (kz, 5, n)) as a contiguous (5, kz, n) array -- the layout of the
+    driver's own permuted views (solver.py:468-703), which are used in place.
+    Arrays in any other layout are copied into that layout; the copy is kept
+    while the array's data and version are unchanged."""
+    _require_device(t)
+    want = (kz, n, 5) if kind == "x" else (kz, 5, n)
+    if tuple(t.shape) != want:
+        raise IndexError("boundary array has shape %s, expected %s (spec_bdy_width 5)" % (tuple(t.shape), want))
+    strides = (n, 1, kz * n) if kind == "x" else (n, kz * n, 1)
+    if tuple(t.stride()) == strides or t.numel() == 0:
+        return t
+    key = (t.data_ptr(), tuple(t.stride()), kind)
+    hit = _bdy_copies.get(key)
+    if hit is not None and hit[0] == t._version:
+        return hit[1]
+    c = (t.permute(2, 0, 1) if kind == "x" else t.permute(1, 0, 2)).contiguous()
+    c = c.permute(1, 2, 0) if kind == "x" else c.permute(1, 0, 2)
+    if len(_bdy_copies) > 256:
+        _bdy_copies.clear()
+    _bdy_copies[key] = (t._version, c)
+    return c
+
+
+def bptr(t, kind, kz, n):
+    """Device pointer of a boundary array in the kernels' layout (see
+    boundary_layout)."""
+    c = boundary_layout(t, kind, kz, n)
+    p = BoundaryPtr(c.data_ptr())
+    p.keep = c
+    return p
+
+
+def bdy_ptrs(arrs, kz, ny, nx):
+    """Pointers of (xs, xe, ys, ye) or (xs, xe, ys, ye, txs, txe, tys, tye)."""
+    return [bptr(a, "x" if n % 4 < 2 else "y", kz, ny if n % 4 < 2 else nx) for n, a in enumerate(arrs)]
 
 
 def flag(s):

@@ -2,9 +2,13 @@
 // and specified-zone tendencies, specified-zone updates, zero-gradient and
 // flow-dependent edges, mass weighting and the end-of-step specified values.
 //
-// Boundary arrays are contiguous: x-boundaries (kz, ny, 5) indexed [k, j, c],
-// y-boundaries (kz, 5, nx) indexed [k, r, i], c / r counting inward from the
-// domain edge.  Each kernel is one launch over the ring of points it writes;
+// Boundary arrays have the reference driver's layout (solver.py:468-703):
+// an x-boundary array is the (kz, ny, 5) view permute(1, 2, 0) of a
+// contiguous (5, kz, ny) array, a y-boundary array the (kz, 5, nx) view
+// permute(1, 0, 2) of a contiguous (5, kz, nx) array, so element [k, j, c] of
+// an x-boundary lies at (c kz + k) ny + j and element [k, r, i] of a
+// y-boundary at (r kz + k) nx + i; c / r count inward from the domain edge and
+// kz is the level count of the field the array belongs to.  Each kernel is one launch over the ring of points it writes;
 // where the reference writes a point in several sequential statements the
 // kernel reproduces the last writer (or repeats the update as often as the
 // statements cover the point).
@@ -21,8 +25,8 @@ struct Bdy {
   const float *xs, *xe, *ys, *ye, *txs, *txe, *tys, *tye;
 };
 
-__device__ inline float bx(const Dims& D, const float* a, int k, int j, int c) { return a[((long)k * D.ny + j) * 5 + c]; }
-__device__ inline float by(const Dims& D, const float* a, int k, int r, int i) { return a[((long)k * 5 + r) * D.nx + i]; }
+__device__ inline float bx(const Dims& D, const float* a, int k, int j, int c) { return a[((long)c * D.nz + k) * D.ny + j]; }
+__device__ inline float by(const Dims& D, const float* a, int k, int r, int i) { return a[((long)r * D.nz + k) * D.nx + i]; }
 
 // staggering of the relax/spec routines (wrf_boundary.py:_bounds logic)
 static void bounds_of(char v, const pywrf_grid_t* g, int& ibe, int& jbe, int& ktf) {
@@ -43,6 +47,7 @@ struct Relax {
   Bdy b;
   const float *fcx, *gcx;
   float dtbc;
+  const float* dtbc_dev;  // non-null: the boundary time is read from device memory (graph replays)
   int ibs, ibe, jbs, jbe;
 };
 
@@ -63,7 +68,7 @@ __global__ void k_relax(Box all, int w, Dims D, Relax a) {
     else if (i == a.ibe - 1 - b && j >= a.jbs + b + 1 && j < a.jbe - b - 1) { side = 3; bd = b; }
   }
   if (side < 0) return;
-  const float dt = a.dtbc;
+  const float dt = a.dtbc_dev ? *a.dtbc_dev : a.dtbc;
   auto FY = [&](const float* bv, const float* bt, int r, int jj, int ii) {
     return by(D, bv, k, r, ii) + dt * by(D, bt, k, r, ii) - relax_field(D, a, k, jj, ii);
   };
@@ -106,7 +111,20 @@ extern "C" int pywrf_relax_bdytend(const pywrf_grid_t* g, const float* field, fl
   Relax a;
   a.field = field; a.mut = nullptr; a.mbox = mkbox(0, 0, 0, 0, 0, 0); a.tend = field_tend;
   a.b = {bxs, bxe, bys, bye, btxs, btxe, btys, btye};
-  a.fcx = fcx; a.gcx = gcx; a.dtbc = F(dtbc);
+  a.fcx = fcx; a.gcx = gcx; a.dtbc = F(dtbc); a.dtbc_dev = nullptr;
+  return relax_launch(g, a, variable_in, s);
+}
+
+extern "C" int pywrf_relax_bdytend_dtbc_dev(const pywrf_grid_t* g, const float* field, float* field_tend,
+                                            const float* bxs, const float* bxe, const float* bys, const float* bye,
+                                            const float* btxs, const float* btxe, const float* btys,
+                                            const float* btye, char variable_in, const float* dtbc, const float* fcx,
+                                            const float* gcx, pywrf_stream_t s) {
+  if (!pw_size_ok(g) || !dtbc) return PYWRF_E_RANGE;
+  Relax a;
+  a.field = field; a.mut = nullptr; a.mbox = mkbox(0, 0, 0, 0, 0, 0); a.tend = field_tend;
+  a.b = {bxs, bxe, bys, bye, btxs, btxe, btys, btye};
+  a.fcx = fcx; a.gcx = gcx; a.dtbc = 0.0f; a.dtbc_dev = dtbc;
   return relax_launch(g, a, variable_in, s);
 }
 
@@ -121,7 +139,22 @@ extern "C" int pywrf_relax_bdy_scalar(const pywrf_grid_t* g, float* scalar_tend,
   a.mbox = mkbox(g->kts, g->kte - 1, imax(g->jts - 1, g->jds), imin(g->jte + 1, g->jde - 1), imax(g->its - 1, g->ids),
                  imin(g->ite + 1, g->ide - 1));
   a.b = {bxs, bxe, bys, bye, btxs, btxe, btys, btye};
-  a.fcx = fcx; a.gcx = gcx; a.dtbc = F(dtbc);
+  a.fcx = fcx; a.gcx = gcx; a.dtbc = F(dtbc); a.dtbc_dev = nullptr;
+  return relax_launch(g, a, 'q', s);
+}
+
+extern "C" int pywrf_relax_bdy_scalar_dtbc_dev(const pywrf_grid_t* g, float* scalar_tend, const float* scalar,
+                                               const float* mu, const float* bxs, const float* bxe, const float* bys,
+                                               const float* bye, const float* btxs, const float* btxe,
+                                               const float* btys, const float* btye, const float* dtbc,
+                                               const float* fcx, const float* gcx, pywrf_stream_t s) {
+  if (!pw_size_ok(g) || !dtbc) return PYWRF_E_RANGE;
+  Relax a;
+  a.field = scalar; a.mut = mu; a.tend = scalar_tend;
+  a.mbox = mkbox(g->kts, g->kte - 1, imax(g->jts - 1, g->jds), imin(g->jte + 1, g->jde - 1), imax(g->its - 1, g->ids),
+                 imin(g->ite + 1, g->ide - 1));
+  a.b = {bxs, bxe, bys, bye, btxs, btxe, btys, btye};
+  a.fcx = fcx; a.gcx = gcx; a.dtbc = 0.0f; a.dtbc_dev = dtbc;
   return relax_launch(g, a, 'q', s);
 }
 
@@ -493,6 +526,7 @@ struct FinalJob {
   const float *mu, *msf;
   Bdy b;
   float dtbc;
+  const float* dtbc_dev;  // non-null: the boundary time is read from device memory (graph replays)
   int jts, jtf, its, itf, ibe, jbe;
 };
 
@@ -507,7 +541,8 @@ __device__ inline void spec_final_point(const FinalJob& a, int k, int j, int i) 
   else return;
   float sm = a.msf ? AT2(a.msf, j, i) : 1.0f;
   float su = a.mu ? AT2(a.mu, j, i) : 1.0f;
-  AT(a.f, k, j, i) = sm * (bv + a.dtbc * bt) / su;
+  const float dtbc = a.dtbc_dev ? *a.dtbc_dev : a.dtbc;
+  AT(a.f, k, j, i) = sm * (bv + dtbc * bt) / su;
 }
 
 __global__ void k_spec_final(Box all, int w, FinalJob a) {
@@ -543,7 +578,7 @@ static int spec_final_job(const pywrf_grid_t* g, float* field, const float* mu, 
   a.all = mkbox(g->kts, ktf, g->jts, imax(jtf, jbe), g->its, imax(ibe, itf));
   a.w = (itf >= ibe && jtf >= jbe) ? 1 : 0;  // rows jts, jtf-1 and columns its, itf-1 are the box's sides
   a.D = dims_of(g);
-  a.f = field; a.dtbc = F(dtbc);
+  a.f = field; a.dtbc = F(dtbc); a.dtbc_dev = nullptr;
   a.jts = g->jts; a.jtf = jtf; a.its = g->its; a.itf = itf; a.ibe = ibe; a.jbe = jbe;
   return 0;
 }
@@ -574,9 +609,9 @@ __global__ void k_spec_final_group(FinalJobs J) {
   spec_final_point(a, k, j, i);
 }
 
-extern "C" int pywrf_spec_bdy_final_group(const pywrf_grid_t* g, int n, void* const* fields, void* const* mus,
-                                          void* const* msfs, void* const* bdys, const char* variables, double dtbc,
-                                          pywrf_stream_t s) {
+static int spec_final_group(const pywrf_grid_t* g, int n, void* const* fields, void* const* mus, void* const* msfs,
+                            void* const* bdys, const char* variables, double dtbc, const float* dtbc_dev,
+                            pywrf_stream_t s) {
   if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   if (n < 0 || n > PW_FINAL_GROUP_MAX) return PYWRF_E_ARG;
   FinalJobs J;
@@ -588,6 +623,7 @@ extern "C" int pywrf_spec_bdy_final_group(const pywrf_grid_t* g, int n, void* co
     int e = spec_final_job(g, (float*)fields[f], (const float*)mus[f], (const float*)msfs[f], bd[0], bd[1], bd[2],
                            bd[3], bd[4], bd[5], bd[6], bd[7], variables[f], dtbc, a);
     if (e) return e;
+    a.dtbc_dev = dtbc_dev;
     if (a.all.size() <= 0) continue;
     if (a.w != 1 || a.all.nj() < 2 || a.all.ni() < 2) {
       pw_launch_edges(k_spec_final, a.all, a.w, s, a);
@@ -601,4 +637,17 @@ extern "C" int pywrf_spec_bdy_final_group(const pywrf_grid_t* g, int n, void* co
     hipLaunchKernelGGL(k_spec_final_group, dim3((unsigned)((most + PW_EDGE_BLOCK - 1) / PW_EDGE_BLOCK), (unsigned)m,
                        (unsigned)maxk), dim3(PW_EDGE_BLOCK), 0, (hipStream_t)s, J);
   PW_RET();
+}
+
+extern "C" int pywrf_spec_bdy_final_group(const pywrf_grid_t* g, int n, void* const* fields, void* const* mus,
+                                          void* const* msfs, void* const* bdys, const char* variables, double dtbc,
+                                          pywrf_stream_t s) {
+  return spec_final_group(g, n, fields, mus, msfs, bdys, variables, dtbc, nullptr, s);
+}
+
+extern "C" int pywrf_spec_bdy_final_group_dtbc_dev(const pywrf_grid_t* g, int n, void* const* fields,
+                                                   void* const* mus, void* const* msfs, void* const* bdys,
+                                                   const char* variables, const float* dtbc, pywrf_stream_t s) {
+  if (!dtbc) return PYWRF_E_RANGE;
+  return spec_final_group(g, n, fields, mus, msfs, bdys, variables, 0.0, dtbc, s);
 }

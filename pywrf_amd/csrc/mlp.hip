@@ -1,0 +1,304 @@
+// Dense layers of the WSM6 column surrogate on the matrix cores (gfx950):
+// BF16 operands, FP32 accumulation, v_mfma_f32_32x32x16_bf16.
+//
+// One GEMM form serves the forward and backward passes:
+//   C[M x N] = A[M x K] . B[N x K]^T
+// with both operands K-contiguous (row-major A, row-major B = the layer's
+// weight [out x in]); the backward products use transposed copies (the
+// optimiser keeps W^T next to W, activations and output gradients are
+// transposed by k_transpose), so every operand is read along K.
+//
+// Tiling: a 256-thread workgroup (4 waves, 2 x 2) owns a 128 x 128 tile of C;
+// each wave owns 64 x 64 = 2 x 2 MFMA tiles of 32 x 32.  K advances 32 at a
+// time through double-buffered LDS tiles of A and B (rows padded to 40 bf16,
+// 80 bytes, so the 16-byte fragment reads of neighbouring rows fall in
+// different banks); the next K-slab is loaded from global into registers
+// while the current one feeds the MFMAs, and one barrier per slab separates
+// the LDS writes of one buffer from the reads of the other.  Rows of A and B
+// beyond M and N load as zero; stores are predicated.  Blocks are numbered so
+// that consecutive M-tiles of one N-tile go to one XCD (they share B).
+//
+// Epilogues: FP32 C (+ bias), BF16 relu(C + bias), BF16 C * [H > 0] (the
+// ReLU derivative, H the layer's forward output) and FP32 atomic accumulation
+// for split-K weight gradients (grid.z splits K; C must start at zero).
+#include "common.h"
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+#define MLP_BM 128
+#define MLP_BN 128
+#define MLP_BK 32
+#define MLP_LDS_ROW 40  // bf16 per LDS row (32 + 8 padding)
+
+enum { EPI_F32 = 0, EPI_BF16_RELU = 1, EPI_BF16_DRELU = 2, EPI_F32_ATOMIC = 3 };
+
+struct GemmArgs {
+  int M, N, K, lda, ldb, ldc, ldh, kchunk;
+  const bf16* A;
+  const bf16* B;
+  void* C;
+  const float* bias;
+  const bf16* H;
+};
+
+// 16 bytes of a row-major bf16 matrix, zero outside [0, rows)
+__device__ inline uint4 load16(const bf16* base, int ld, int row, int rows, int col) {
+  if (row >= rows) return make_uint4(0u, 0u, 0u, 0u);
+  return *(const uint4*)(base + (long)row * ld + col);
+}
+
+template <int EPI>
+__global__ void __launch_bounds__(256) k_mlp_gemm(GemmArgs g) {
+  __shared__ __attribute__((aligned(16))) bf16 As[2][MLP_BM * MLP_LDS_ROW];
+  __shared__ __attribute__((aligned(16))) bf16 Bs[2][MLP_BN * MLP_LDS_ROW];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  // tile order: the M-tiles of one N-tile in a contiguous run per XCD
+  int bx = (int)blockIdx.x, by = (int)blockIdx.y;
+  xcd_block_2d(bx, by);
+  const int m0 = bx * MLP_BM, n0 = by * MLP_BN;
+  const int kbeg = (int)blockIdx.z * g.kchunk;
+  const int kend = imin(g.K, kbeg + g.kchunk);
+  const int nk = (kend - kbeg) / MLP_BK;
+
+  // each thread moves two 16-byte chunks of A and of B per K-slab: chunk c =
+  // tid + 256 r covers row c / 4, columns 8 (c % 4) .. + 8
+  uint4 ra[2], rb[2];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int c = tid + 256 * r, row = c >> 2, col = k0 + (c & 3) * 8;
+      ra[r] = load16(g.A, g.lda, m0 + row, g.M, col);
+      rb[r] = load16(g.B, g.ldb, n0 + row, g.N, col);
+    }
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int c = tid + 256 * r, row = c >> 2, col = (c & 3) * 8;
+      *(uint4*)&As[buf][row * MLP_LDS_ROW + col] = ra[r];
+      *(uint4*)&Bs[buf][row * MLP_LDS_ROW + col] = rb[r];
+    }
+  };
+
+  f32x16 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[a][b][e] = 0.0f;
+
+  if (nk > 0) {
+    gload(kbeg);
+    lstore(0);
+  }
+  __syncthreads();
+  const int lr = lane & 31, lk = (lane >> 5) * 8;
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) gload(kbeg + (kt + 1) * MLP_BK);
+#pragma unroll
+    for (int ks = 0; ks < MLP_BK; ks += 16) {
+      bf16x8 af[2], bfr[2];
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        af[t] = *(const bf16x8*)&As[cur][(wm * 64 + t * 32 + lr) * MLP_LDS_ROW + ks + lk];
+        bfr[t] = *(const bf16x8*)&Bs[cur][(wn * 64 + t * 32 + lr) * MLP_LDS_ROW + ks + lk];
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
+    }
+    if (kt + 1 < nk) lstore(cur ^ 1);
+    __syncthreads();
+  }
+
+  // accumulator element e of lane l holds C[8 (e / 4) + 4 (l / 32) + e % 4][l % 32] of the 32 x 32 tile
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int n = n0 + wn * 64 + b * 32 + lr;
+      if (n >= g.N) continue;
+      const float bv = (EPI == EPI_F32 || EPI == EPI_BF16_RELU) && g.bias ? g.bias[n] : 0.0f;
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int m = m0 + wm * 64 + a * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+        if (m >= g.M) continue;
+        const float v = acc[a][b][e];
+        const long q = (long)m * g.ldc + n;
+        if (EPI == EPI_F32) {
+          ((float*)g.C)[q] = v + bv;
+        } else if (EPI == EPI_BF16_RELU) {
+          const float r = v + bv;
+          ((bf16*)g.C)[q] = (bf16)(r > 0.0f ? r : 0.0f);
+        } else if (EPI == EPI_BF16_DRELU) {
+          const float h = (float)g.H[(long)m * g.ldh + n];
+          ((bf16*)g.C)[q] = (bf16)(h > 0.0f ? v : 0.0f);
+        } else {
+          atomicAdd(&((float*)g.C)[q], v);
+        }
+      }
+    }
+}
+
+extern "C" int pywrf_mlp_gemm(int epi, int M, int N, int K, const void* A, int lda, const void* B, int ldb, void* C,
+                              int ldc, const float* bias, const void* H, int ldh, int ksplit, pywrf_stream_t s) {
+  if (M < 0 || N < 0 || K < 0 || K % MLP_BK || lda < K || ldb < K || ldc < N || !A || !B || !C) return PYWRF_E_RANGE;
+  if (lda % 8 || ldb % 8 || ((uintptr_t)A & 15) || ((uintptr_t)B & 15)) return PYWRF_E_RANGE;  // 16-byte rows
+  if (epi == EPI_BF16_DRELU && (!H || ldh < N)) return PYWRF_E_RANGE;
+  if (epi < 0 || epi > 3 || ksplit < 1 || (ksplit > 1 && epi != EPI_F32_ATOMIC)) return PYWRF_E_ARG;
+  if ((long)M * ldc >= (1L << 31) || (long)M * lda >= (1L << 31) || (long)N * ldb >= (1L << 31)) return PYWRF_E_RANGE;
+  if (M == 0 || N == 0) return 0;
+  GemmArgs g;
+  g.M = M; g.N = N; g.K = K; g.lda = lda; g.ldb = ldb; g.ldc = ldc; g.ldh = ldh;
+  g.A = (const bf16*)A; g.B = (const bf16*)B; g.C = C; g.bias = bias; g.H = (const bf16*)H;
+  const int slabs = K / MLP_BK;
+  const int per = (slabs + ksplit - 1) / ksplit;
+  g.kchunk = per * MLP_BK;
+  const int splits = K == 0 ? 1 : (slabs + per - 1) / per;
+  dim3 grid((unsigned)((M + MLP_BM - 1) / MLP_BM), (unsigned)((N + MLP_BN - 1) / MLP_BN), (unsigned)splits);
+  switch (epi) {
+    case EPI_F32: hipLaunchKernelGGL(k_mlp_gemm<EPI_F32>, grid, dim3(256), 0, (hipStream_t)s, g); break;
+    case EPI_BF16_RELU: hipLaunchKernelGGL(k_mlp_gemm<EPI_BF16_RELU>, grid, dim3(256), 0, (hipStream_t)s, g); break;
+    case EPI_BF16_DRELU: hipLaunchKernelGGL(k_mlp_gemm<EPI_BF16_DRELU>, grid, dim3(256), 0, (hipStream_t)s, g); break;
+    default: hipLaunchKernelGGL(k_mlp_gemm<EPI_F32_ATOMIC>, grid, dim3(256), 0, (hipStream_t)s, g); break;
+  }
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ transpose
+// out[c][r] = in[r][c] for a bf16 matrix, through 64 x 64 LDS tiles (reads and
+// writes along rows of 64 contiguous elements).  Entries of out's rows beyond
+// R (up to ldo) are left untouched.
+__global__ void __launch_bounds__(256) k_mlp_transpose(int R, int C, const bf16* in, int ldi, bf16* out, int ldo) {
+  __shared__ bf16 t[64][65];
+  const int r0 = (int)blockIdx.y * 64, c0 = (int)blockIdx.x * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int y = ty; y < 64; y += 4) {
+    const int r = r0 + y, c = c0 + tx;
+    t[y][tx] = (r < R && c < C) ? in[(long)r * ldi + c] : (bf16)0.0f;
+  }
+  __syncthreads();
+  for (int y = ty; y < 64; y += 4) {
+    const int c = c0 + y, r = r0 + tx;
+    if (c < C && r < R) out[(long)c * ldo + r] = t[tx][y];
+  }
+}
+
+extern "C" int pywrf_mlp_transpose(int R, int C, const void* in, int ldi, void* out, int ldo, pywrf_stream_t s) {
+  if (R < 0 || C < 0 || ldi < C || ldo < R || !in || !out) return PYWRF_E_RANGE;
+  if ((long)R * ldi >= (1L << 31) || (long)C * ldo >= (1L << 31)) return PYWRF_E_RANGE;
+  if (R == 0 || C == 0) return 0;
+  hipLaunchKernelGGL(k_mlp_transpose, dim3((unsigned)((C + 63) / 64), (unsigned)((R + 63) / 64)), dim3(256), 0,
+                     (hipStream_t)s, R, C, (const bf16*)in, ldi, (bf16*)out, ldo);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ row sums (bias gradients)
+// out[r] = sum over c < C of in[r][c] (bf16 in, FP32 sums): one workgroup per row.
+__global__ void __launch_bounds__(256) k_mlp_rowsum(int C, const bf16* in, int ld, float* out) {
+  __shared__ float part[256];
+  const int r = (int)blockIdx.x;
+  float acc = 0.0f;
+  for (int c = threadIdx.x; c < C; c += 256) acc += (float)in[(long)r * ld + c];
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[r] = part[0];
+}
+
+extern "C" int pywrf_mlp_rowsum(int R, int C, const void* in, int ld, float* out, pywrf_stream_t s) {
+  if (R < 0 || C < 0 || ld < C || !in || !out || (long)R * ld >= (1L << 31)) return PYWRF_E_RANGE;
+  if (R == 0) return 0;
+  hipLaunchKernelGGL(k_mlp_rowsum, dim3((unsigned)R), dim3(256), 0, (hipStream_t)s, C, (const bf16*)in, ld, out);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ loss
+// Weighted mean squared error over B rows of N outputs and its gradient:
+//   loss += sum_{b,n} w[n] (Y - T)^2 * scale,   dY = bf16(2 w[n] (Y - T) * scale)
+// (scale = 1 / (B * sum w) gives the mean).  loss accumulates atomically
+// (start it at zero); dY entries of columns n in [N, lddy) are set to zero.
+__global__ void __launch_bounds__(256) k_mlp_mse(int B, int N, const float* Y, int ldy, const float* T, int ldt,
+                                                 const float* w, float scale, bf16* dY, int lddy, float* loss) {
+  __shared__ float part[256];
+  const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+  const long total = (long)B * lddy;
+  float l = 0.0f;
+  if (idx < total) {
+    const int b = (int)(idx / lddy), n = (int)(idx % lddy);
+    float g = 0.0f;
+    if (n < N) {
+      const float d = Y[(long)b * ldy + n] - T[(long)b * ldt + n];
+      const float wn = w[n];
+      l = wn * d * d * scale;
+      g = 2.0f * wn * d * scale;
+    }
+    dY[idx] = (bf16)g;
+  }
+  part[threadIdx.x] = l;
+  __syncthreads();
+  for (int s2 = 128; s2 > 0; s2 >>= 1) {
+    if ((int)threadIdx.x < s2) part[threadIdx.x] += part[threadIdx.x + s2];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && loss) atomicAdd(loss, part[0]);
+}
+
+extern "C" int pywrf_mlp_mse(int B, int N, const float* Y, int ldy, const float* T, int ldt, const float* w,
+                             double scale, void* dY, int lddy, float* loss, pywrf_stream_t s) {
+  if (B < 0 || N < 0 || ldy < N || ldt < N || lddy < N || !Y || !T || !w || !dY) return PYWRF_E_RANGE;
+  const long total = (long)B * lddy;
+  if (total >= (1L << 31)) return PYWRF_E_RANGE;
+  if (total == 0) return 0;
+  hipLaunchKernelGGL(k_mlp_mse, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, (hipStream_t)s, B, N, Y, ldy, T,
+                     ldt, w, (float)scale, (bf16*)dY, lddy, loss);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ Adam
+// One Adam step (Kingma & Ba, bias-corrected) on an FP32 master array of
+// rows x cols, gradient g * gscale; writes the updated master, and when wb /
+// wtb are given its BF16 copy (rows x cols, row-major) and transposed BF16
+// copy (cols x rows).
+__global__ void __launch_bounds__(256) k_mlp_adam(int n, int cols, float* w, const float* g, float* m, float* v,
+                                                  float lr_t, float b1, float b2, float eps, float gscale, bf16* wb,
+                                                  bf16* wtb, int rows) {
+  const int idx = (int)(blockIdx.x * 256 + threadIdx.x);
+  if (idx >= n) return;
+  const float gr = g[idx] * gscale;
+  const float mm = b1 * m[idx] + (1.0f - b1) * gr;
+  const float vv = b2 * v[idx] + (1.0f - b2) * gr * gr;
+  m[idx] = mm;
+  v[idx] = vv;
+  const float nw = w[idx] - lr_t * mm / (sqrtf(vv) + eps);
+  w[idx] = nw;
+  if (wb) wb[idx] = (bf16)nw;
+  if (wtb) {
+    const int r = idx / cols, c = idx % cols;
+    wtb[(long)c * rows + r] = (bf16)nw;
+  }
+}
+
+extern "C" int pywrf_mlp_adam(int rows, int cols, float* w, const float* g, float* m, float* v, double lr,
+                              double beta1, double beta2, double eps, double gscale, int t, void* wb, void* wtb,
+                              pywrf_stream_t s) {
+  if (rows < 0 || cols < 0 || t < 1 || !w || !g || !m || !v || (long)rows * cols >= (1L << 31)) return PYWRF_E_RANGE;
+  const int n = rows * cols;
+  if (n == 0) return 0;
+  // bias corrections folded into the step size (double on the host)
+  const double c1 = 1.0 - pow(beta1, t), c2 = 1.0 - pow(beta2, t);
+  const double lr_t = lr * sqrt(c2) / c1;
+  const double eps_t = eps * sqrt(c2);
+  hipLaunchKernelGGL(k_mlp_adam, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)s, n, cols, w, g, m, v,
+                     (float)lr_t, (float)beta1, (float)beta2, (float)eps_t, (float)gscale, (bf16*)wb, (bf16*)wtb, rows);
+  PW_RET();
+}

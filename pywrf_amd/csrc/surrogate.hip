@@ -1,0 +1,237 @@
+// WSM6 column surrogate: a 369 -> 512 -> 512 -> 287 ReLU network over each
+// microphysics column, run inline in place of wsm6 (pywrf/wrf_physics.py:
+// 137-1214, called at solver.py:1714-1756), with a per-column fallback to the
+// plain scheme.
+//
+// Per column (the scheme's tile, the same columns and levels as wsm6):
+//   input  41 levels x 9 fields  T = th * pii, p, rho, qv, qc, qr, qi, qs, qg
+//          (feature var * 41 + k; level 40 and features 369..383 are padding)
+//   output 41 levels x 7 tendencies  T, qv, qc, qr, qi, qs, qg  (feature
+//          var * 41 + k, per second; 287 used, 288 stored)
+// Inputs are standardised (x - mean) * rstd and rounded to BF16; outputs are
+// standardised tendencies, out * std + mean.  The three layers run on the
+// matrix cores (mlp.hip, BF16 operands, FP32 accumulation).
+//
+// Checks (a column that fails any of them gets the plain scheme instead):
+//   * no mixing ratio below -qtol after the update (values in [-qtol, 0) are
+//     set to 0);
+//   * no water-mass gain: the column's total water sum_k rho dz (qv + qc + qr
+//     + qi + qs + qg) may not rise by more than wtol times its old value
+//     (the scheme only removes water, by precipitation);
+//   * every updated value finite.
+// A passing column's water loss sum_k rho dz (old - new) [kg m-2 = mm] is its
+// surface precipitation of the step (rainncv, added to rainnc; the surrogate
+// does not split it into snow and graupel, so snowncv = graupelncv = sr = 0).
+//
+// Columns are processed in chunks through a workspace: packed inputs X
+// (bf16 [chunk x 384]), hidden layers (bf16 [chunk x 512] x 2), outputs
+// (f32 [chunk x 288]), then the fallback list (int [ncol]) and its counter.
+#include "common.h"
+
+typedef __bf16 bf16;
+
+#define SG_LEV 41
+#define SG_NIN 9
+#define SG_NOUT 7
+#define SG_KIN 384   // 369 padded to a multiple of 32
+#define SG_H 512
+#define SG_NO 288    // 287 padded
+#define SG_COLS 64   // columns per pack workgroup
+
+extern "C" int pywrf_mlp_gemm(int epi, int M, int N, int K, const void* A, int lda, const void* B, int ldb, void* C,
+                              int ldc, const float* bias, const void* H, int ldh, int ksplit, pywrf_stream_t s);
+extern "C" int pywrf_wsm6_columns(const pywrf_grid_t* g, float* th, float* q, float* qc, float* qr, float* qi,
+                                  float* qs, float* qg, const float* den, const float* pii, const float* p,
+                                  const float* delz, double delt, float* rain, float* rainncv, float* snow,
+                                  float* snowncv, float* sr, float* graupel, float* graupelncv, const int* cols,
+                                  const int* ncols, pywrf_stream_t s);
+
+struct Tile {
+  Dims D;
+  int j0, i0, ni, ncol, km;
+};
+
+struct Fields {
+  float *th, *qv, *qc, *qr, *qi, *qs, *qg;
+  const float *rho, *pii, *p, *dz;
+  float *rain, *rainncv, *snowncv, *sr, *graupelncv;
+};
+
+__device__ inline float field_in(const Fields& f, int var, long q) {
+  switch (var) {
+    case 0: return f.th[q] * f.pii[q];
+    case 1: return f.p[q];
+    case 2: return f.rho[q];
+    case 3: return f.qv[q];
+    case 4: return f.qc[q];
+    case 5: return f.qr[q];
+    case 6: return f.qi[q];
+    case 7: return f.qs[q];
+    default: return f.qg[q];
+  }
+}
+
+// X[c - c_base][feature] for the columns c_base + [0, m) of the chunk: one
+// workgroup per 64 columns; features are read column-contiguous (rows of
+// the field) into LDS, then written as whole rows of X.
+__global__ void __launch_bounds__(256) k_sg_pack(Tile T, Fields f, const float* mean, const float* rstd, int c_base,
+                                                 int m, bf16* X) {
+  __shared__ bf16 lds[SG_KIN][SG_COLS + 2];
+  const int c0 = (int)blockIdx.x * SG_COLS;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int cl = c0 + tx, c = c_base + cl;
+  const bool live = cl < m;
+  const int j = T.j0 + (live ? c / T.ni : 0), i = T.i0 + (live ? c % T.ni : 0);
+  for (int feat = ty; feat < SG_KIN; feat += 4) {
+    const int var = feat / SG_LEV, k = feat % SG_LEV;
+    float v = 0.0f;
+    if (live && var < SG_NIN && k < T.km) v = (field_in(f, var, T.D.ix(k, j, i)) - mean[feat]) * rstd[feat];
+    lds[feat][tx] = (bf16)v;
+  }
+  __syncthreads();
+  // 64 rows of 384 bf16: 4 threads per row, 96 features each
+  const int row = threadIdx.x >> 2, seg = (threadIdx.x & 3) * 96;
+  if (c0 + row >= m) return;
+  bf16* dst = X + (long)(c0 + row) * SG_KIN + seg;
+  for (int e = 0; e < 96; ++e) dst[e] = lds[seg + e][row];
+}
+
+__device__ inline float wave_sum(float v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// One wave per column of the chunk, lane = level: apply the predicted
+// tendencies if the column passes the checks, else append it to the
+// fallback list.
+__global__ void __launch_bounds__(64) k_sg_apply(Tile T, Fields f, const float* Y, const float* omean,
+                                                 const float* ostd, float dt, float qtol, float wtol, int c_base, int m,
+                                                 int* fall, int* nfall) {
+  const int cl = (int)blockIdx.x;
+  if (cl >= m) return;
+  const int c = c_base + cl, k = threadIdx.x;
+  const int j = T.j0 + c / T.ni, i = T.i0 + c % T.ni;
+  const bool act = k < T.km;
+  const long q = T.D.ix(act ? k : 0, j, i);
+  const float* y = Y + (long)cl * SG_NO;
+  float old[SG_NOUT], nw[SG_NOUT];
+  float pii = act ? f.pii[q] : 1.0f;
+  old[0] = act ? f.th[q] * pii : 0.0f;
+  old[1] = act ? f.qv[q] : 0.0f;
+  old[2] = act ? f.qc[q] : 0.0f;
+  old[3] = act ? f.qr[q] : 0.0f;
+  old[4] = act ? f.qi[q] : 0.0f;
+  old[5] = act ? f.qs[q] : 0.0f;
+  old[6] = act ? f.qg[q] : 0.0f;
+  bool bad = false;
+#pragma unroll
+  for (int v = 0; v < SG_NOUT; ++v) {
+    const int o = v * SG_LEV + (act ? k : 0);
+    const float tend = act ? y[o] * ostd[o] + omean[o] : 0.0f;
+    float x = old[v] + dt * tend;
+    if (!(x == x) || x - x != 0.0f) bad = true;                 // NaN or infinite
+    if (v > 0) {
+      if (x < -qtol) bad = true;
+      if (x < 0.0f) x = 0.0f;
+    }
+    nw[v] = act ? x : 0.0f;
+  }
+  const float mass = act ? f.rho[q] * f.dz[q] : 0.0f;
+  const float w_old = wave_sum(mass * (old[1] + old[2] + old[3] + old[4] + old[5] + old[6]));
+  const float w_new = wave_sum(mass * (nw[1] + nw[2] + nw[3] + nw[4] + nw[5] + nw[6]));
+  if (w_new - w_old > wtol * w_old) bad = true;
+  bad = __any(bad);
+  if (bad) {
+    if (k == 0) fall[atomicAdd(nfall, 1)] = c;
+    return;
+  }
+  if (act) {
+    f.th[q] = nw[0] / pii;
+    f.qv[q] = nw[1];
+    f.qc[q] = nw[2];
+    f.qr[q] = nw[3];
+    f.qi[q] = nw[4];
+    f.qs[q] = nw[5];
+    f.qg[q] = nw[6];
+  }
+  if (k == 0) {
+    const long q2 = T.D.ix2(j, i);
+    const float pr = w_old - w_new > 0.0f ? w_old - w_new : 0.0f;
+    f.rainncv[q2] = pr;
+    f.rain[q2] = f.rain[q2] + pr;
+    f.snowncv[q2] = 0.0f;
+    f.graupelncv[q2] = 0.0f;
+    f.sr[q2] = 0.0f;
+  }
+}
+
+__global__ void k_sg_zero_int(int* p) { *p = 0; }
+
+static long sg_chunk_bytes(long chunk) {
+  return chunk * (SG_KIN * 2L + 2L * SG_H * 2L + SG_NO * 4L);
+}
+
+static int sg_run(const pywrf_grid_t* g, const pywrf_surrogate_t* net, float* th, const float* rho,
+                  const float* pii, const float* p, const float* delz, double dt, float* qv, float* qc, float* qr,
+                  float* qi, float* qs, float* qg, float* rain, float* rainncv, float* snow, float* snowncv, float* sr,
+                  float* graupel, float* graupelncv, char* ws, long ws_bytes, int* nfall, pywrf_stream_t s) {
+  if (!pw_size_ok(g) || !net || !ws || !nfall) return PYWRF_E_RANGE;
+  const int km = (g->kte - 1) - g->kts;
+  if (g->kts != 0 || km < 4 || km > SG_LEV || g->kte > g->nz) return PYWRF_E_RANGE;
+  if (g->jts < 0 || g->its < 0 || g->jte - 1 > g->ny || g->ite - 1 > g->nx) return PYWRF_E_RANGE;
+  const int ni = (g->ite - 1) - g->its, nj = (g->jte - 1) - g->jts;
+  if (ni <= 0 || nj <= 0) return 0;
+  const long ncol = (long)ni * nj;
+  // workspace: fallback list and counter first, then the chunk buffers
+  const long list_bytes = ((ncol + 1) * 4 + 255) / 256 * 256;
+  const long avail = ws_bytes - list_bytes;
+  long chunk = avail > 0 ? avail / sg_chunk_bytes(1) : 0;
+  chunk = chunk / 128 * 128;
+  if (chunk < 128) return PYWRF_E_RANGE;
+  if (chunk > ncol) chunk = (ncol + 127) / 128 * 128;
+  int* fall = (int*)ws;
+  char* b = ws + list_bytes;
+  bf16* X = (bf16*)b;
+  bf16* H1 = X + chunk * SG_KIN;
+  bf16* H2 = H1 + chunk * SG_H;
+  float* Y = (float*)(H2 + chunk * SG_H);
+  Tile T;
+  T.D = dims_of(g); T.j0 = g->jts; T.i0 = g->its; T.ni = ni; T.ncol = (int)ncol; T.km = km;
+  Fields f;
+  f.th = th; f.qv = qv; f.qc = qc; f.qr = qr; f.qi = qi; f.qs = qs; f.qg = qg;
+  f.rho = rho; f.pii = pii; f.p = p; f.dz = delz;
+  f.rain = rain; f.rainncv = rainncv; f.snowncv = snowncv; f.sr = sr; f.graupelncv = graupelncv;
+  (void)snow; (void)graupel;
+  hipLaunchKernelGGL(k_sg_zero_int, dim3(1), dim3(1), 0, (hipStream_t)s, nfall);
+  for (long c0 = 0; c0 < ncol; c0 += chunk) {
+    const int m = (int)(ncol - c0 < chunk ? ncol - c0 : chunk);
+    hipLaunchKernelGGL(k_sg_pack, dim3((unsigned)((m + SG_COLS - 1) / SG_COLS)), dim3(256), 0, (hipStream_t)s, T, f,
+                       net->in_mean, net->in_rstd, (int)c0, m, X);
+    int e = pywrf_mlp_gemm(1, m, SG_H, SG_KIN, X, SG_KIN, net->w1, SG_KIN, H1, SG_H, net->b1, nullptr, 0, 1, s);
+    if (!e) e = pywrf_mlp_gemm(1, m, SG_H, SG_H, H1, SG_H, net->w2, SG_H, H2, SG_H, net->b2, nullptr, 0, 1, s);
+    if (!e) e = pywrf_mlp_gemm(0, m, SG_NO, SG_H, H2, SG_H, net->w3, SG_H, Y, SG_NO, net->b3, nullptr, 0, 1, s);
+    if (e) return e;
+    hipLaunchKernelGGL(k_sg_apply, dim3((unsigned)m), dim3(64), 0, (hipStream_t)s, T, f, Y, net->out_mean,
+                       net->out_std, (float)dt, net->qtol, net->wtol, (int)c0, m, fall, nfall);
+  }
+  // the plain scheme on the columns that failed a check
+  return pywrf_wsm6_columns(g, th, qv, qc, qr, qi, qs, qg, rho, pii, p, delz, dt, rain, rainncv, snow, snowncv, sr,
+                            graupel, graupelncv, fall, nfall, s);
+}
+
+extern "C" int pywrf_wsm6_surrogate_workspace(const pywrf_grid_t* g, int chunk, long long* bytes) {
+  if (!g || chunk < 128 || !bytes) return PYWRF_E_ARG;
+  const long ncol = (long)imax((g->ite - 1) - g->its, 0) * imax((g->jte - 1) - g->jts, 0);
+  *bytes = ((ncol + 1) * 4 + 255) / 256 * 256 + sg_chunk_bytes(chunk / 128 * 128);
+  return 0;
+}
+
+extern "C" int pywrf_wsm6_surrogate(const pywrf_grid_t* g, const pywrf_surrogate_t* net, float* th,
+                                    const float* rho, const float* pii, const float* p, const float* delz, double dt,
+                                    float* qv, float* qc, float* qr, float* qi, float* qs, float* qg, float* rain,
+                                    float* rainncv, float* snow, float* snowncv, float* sr, float* graupel,
+                                    float* graupelncv, void* workspace, long long workspace_bytes, int* nfallback,
+                                    pywrf_stream_t s) {
+  return sg_run(g, net, th, rho, pii, p, delz, dt, qv, qc, qr, qi, qs, qg, rain, rainncv, snow, snowncv, sr, graupel,
+                graupelncv, (char*)workspace, (long)workspace_bytes, nfallback, s);
+}

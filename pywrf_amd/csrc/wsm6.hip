@@ -260,6 +260,8 @@ struct Wsm6 {
   float *rain, *rainncv, *snow, *snowncv, *graupel, *graupelncv, *sr;
   float delt;
   int km, j0, j1, i0, i1, ncol, per;
+  const int* cols;   // non-null: run only the columns cols[0 .. *ncols) (indices into the tile's column order)
+  const int* ncols;
 };
 
 // rates limited so that a sink cannot remove more than value
@@ -304,8 +306,13 @@ __global__ void __launch_bounds__(WAVE) k_wsm6(Dims D, Wsm6 a) {
   // so XCD x takes the contiguous run [x * per, (x + 1) * per) of columns and
   // the columns that share a cache line are served by one L2.
   const int ni = a.i1 - a.i0;
-  const int c = (int)(blockIdx.x % PW_XCDS) * a.per + (int)(blockIdx.x / PW_XCDS);
+  int c = (int)(blockIdx.x % PW_XCDS) * a.per + (int)(blockIdx.x / PW_XCDS);
   if (c >= a.ncol) return;
+  if (a.cols) {
+    if (c >= *a.ncols) return;
+    c = a.cols[c];
+    if (c < 0 || c >= a.ncol) return;
+  }
   const int j = a.j0 + c / ni, i = a.i0 + c % ni;
   const int k = threadIdx.x, km = a.km;
   const bool act = k < km;
@@ -646,10 +653,10 @@ __global__ void __launch_bounds__(WAVE) k_wsm6(Dims D, Wsm6 a) {
   a.qg[q] = qrs2;
 }
 
-extern "C" int pywrf_wsm6(const pywrf_grid_t* g, float* th, float* q, float* qc, float* qr, float* qi, float* qs,
-                          float* qg, const float* den, const float* pii, const float* p, const float* delz,
-                          double delt, float* rain, float* rainncv, float* snow, float* snowncv, float* sr,
-                          float* graupel, float* graupelncv, pywrf_stream_t s) {
+static int wsm6_launch(const pywrf_grid_t* g, float* th, float* q, float* qc, float* qr, float* qi, float* qs,
+                       float* qg, const float* den, const float* pii, const float* p, const float* delz, double delt,
+                       float* rain, float* rainncv, float* snow, float* snowncv, float* sr, float* graupel,
+                       float* graupelncv, const int* cols, const int* ncols, pywrf_stream_t s) {
   if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   int e = pw_check_grid(g, 0);
   if (e) return e;
@@ -668,8 +675,31 @@ extern "C" int pywrf_wsm6(const pywrf_grid_t* g, float* th, float* q, float* qc,
   if (ncol <= 0) return 0;
   a.ncol = (int)ncol;
   a.per = (int)((ncol + PW_XCDS - 1) / PW_XCDS);
+  a.cols = cols;
+  a.ncols = ncols;
   hipLaunchKernelGGL(k_wsm6, dim3((unsigned)(a.per * PW_XCDS)), dim3(WAVE), 0, (hipStream_t)s, dims_of(g), a);
   return (int)hipGetLastError();
+}
+
+extern "C" int pywrf_wsm6(const pywrf_grid_t* g, float* th, float* q, float* qc, float* qr, float* qi, float* qs,
+                          float* qg, const float* den, const float* pii, const float* p, const float* delz,
+                          double delt, float* rain, float* rainncv, float* snow, float* snowncv, float* sr,
+                          float* graupel, float* graupelncv, pywrf_stream_t s) {
+  return wsm6_launch(g, th, q, qc, qr, qi, qs, qg, den, pii, p, delz, delt, rain, rainncv, snow, snowncv, sr, graupel,
+                     graupelncv, nullptr, nullptr, s);
+}
+
+// WSM6 on a device list of the tile's columns (c = (j - jts) * (ite - 1 - its) +
+// (i - its)), cols[0 .. *ncols): the surrogate's fallback.  Columns outside
+// the list are untouched; each listed column gets exactly the plain scheme.
+extern "C" int pywrf_wsm6_columns(const pywrf_grid_t* g, float* th, float* q, float* qc, float* qr, float* qi,
+                                  float* qs, float* qg, const float* den, const float* pii, const float* p,
+                                  const float* delz, double delt, float* rain, float* rainncv, float* snow,
+                                  float* snowncv, float* sr, float* graupel, float* graupelncv, const int* cols,
+                                  const int* ncols, pywrf_stream_t s) {
+  if (!cols || !ncols) return PYWRF_E_ARG;
+  return wsm6_launch(g, th, q, qc, qr, qi, qs, qg, den, pii, p, delz, delt, rain, rainncv, snow, snowncv, sr, graupel,
+                     graupelncv, cols, ncols, s);
 }
 
 extern "C" int pywrf_microphysics_driver(const pywrf_grid_t* g, float* th, const float* rho, const float* pi_phy,

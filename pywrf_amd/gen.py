@@ -23,8 +23,12 @@ analytically so every workload is reproducible from a seed:
 * map factors close to 1 with a small x/y anisotropy (so x/y swaps show), a
   beta-plane Coriolis f, e and a small grid rotation sin(alpha);
 * lateral boundary values = the mass-coupled initial state in the 5-point
-  strips (u*muu/msfuy, v*muv/msfvx, t*mut, ph*mut, mu, qv*mut) with zero
-  tendencies, zero outside the ranges the reference fills (solver.py:468-703).
+  strips (u*muu/msfuy, v*muv/msfvx, t*mut, ph*mut, mu, qv*mut), zero outside
+  the ranges the reference fills (solver.py:468-703); boundary tendencies
+  zero, or (key "bdy_tend") the coupled difference between a seeded smooth
+  perturbation of the initial state and the initial state, divided by the
+  boundary period bdy_interval * dt (a boundary that drifts towards another
+  state, as consecutive wrfbdy records do).
 """
 import math
 
@@ -53,7 +57,14 @@ CONFIGS = {
     # adjoint-check shape: 120x120x41 with the config2 recipe
     "config4": dict(n=120, hills="random3", jet=(20.0, 10e3, 5e3), u0=0.0, noise=2.0,
                     bubbles="random", clouds=4, seed=0),
+    # time-varying lateral boundaries: 24x24x41 (14 interior mass points), clouds and noise, and
+    # boundary tendencies towards a perturbed state (amplitudes of the uncoupled perturbation:
+    # u, v in m/s, t in K, qv relative, mu in Pa, ph in m^2/s^2)
+    "bdy24": dict(n=24, hills=[(0.35, 0.6, 400.0, 40e3)], jet=None, u0=8.0, noise=0.5,
+                  bubbles=[(0.6, 0.4, 1500.0, 2.0, 15e3)], clouds=6, seed=3,
+                  bdy_tend=dict(u=6.0, v=4.0, t=2.0, qv=0.15, mu=300.0, ph=300.0)),
 }
+BDY_PERIOD = 360 * 54.0     # bdy_interval * dt of the step's default run configuration (seconds)
 
 
 def _exner_of_z(z, ths=300.0, nbv=0.01):
@@ -277,13 +288,30 @@ def synthetic_state(config="config1", seed=None, device="cpu", dtype=torch.float
     s.mub[:, g.ite - 1] = s.mub[:, g.ite - 2]
     s.mub[g.jts - 1, :] = s.mub[g.jts, :]
     s.mub[:, g.its - 1] = s.mub[:, g.its]
-    _boundary_arrays(s)
+    _boundary_arrays(s, cfg.get("bdy_tend"), seed)
     s = _cast(s, device, dtype)
     return s
 
 
-def _boundary_arrays(s):
-    """Mass-coupled boundary values of the initial state, zero tendencies."""
+def _pattern(rng, nz, ny, nx):
+    """Smooth seeded field in [-1, 1]: three horizontal waves of non-integer
+    wavenumber (so that no two strips of the domain see equal values) with a
+    vertical half-sine envelope plus a level-wise tilt."""
+    y, x = np.meshgrid(np.arange(ny) / ny, np.arange(nx) / nx, indexing="ij")
+    kk = np.arange(nz)[:, None, None] / max(nz - 1, 1)
+    out = np.zeros((nz, ny, nx))
+    for _ in range(3):
+        ax, ay = rng.uniform(0.6, 2.4, size=2)
+        px, py, a = rng.uniform(0, 2 * np.pi), rng.uniform(0, 2 * np.pi), rng.uniform(0.5, 1.0)
+        out += a * np.sin(2 * np.pi * ax * x + px)[None] * np.cos(2 * np.pi * ay * y + py)[None] \
+            * (0.6 + 0.4 * np.sin(np.pi * kk + rng.uniform(0, 1)))
+    return out / max(np.abs(out).max(), 1e-30)
+
+
+def _boundary_arrays(s, bdy_tend=None, seed=0):
+    """Mass-coupled boundary values of the initial state; tendencies zero, or
+    towards a perturbed state over BDY_PERIOD when bdy_tend gives the
+    perturbation amplitudes."""
     g = s.grid
     nz, ny, nx = g.nz, g.ny, g.nx
     ids, ide, jds, jde = g.ids, g.ide, g.jds, g.jde
@@ -304,16 +332,28 @@ def _boundary_arrays(s):
         "ph": ((0, nz), (5, ny - 6), (5, nx - 6)), "t": ((0, nz - 1), (5, ny - 6), (5, nx - 6)),
         "qv": ((0, nz - 1), (5, ny - 6), (5, nx - 6)), "mu": ((0, 1), (5, ny - 6), (5, nx - 6)),
     }
+    tends = {}
+    if bdy_tend:
+        rng = np.random.default_rng(1000 + seed)
+        T = lambda a: torch.as_tensor(a, dtype=torch.float64)
+        P = {var: T(_pattern(rng, c.shape[0], ny, nx)) for var, c in coupled.items()}
+        scale = {"u": muu / s.msfuy, "v": muv / s.msfvx, "t": mu_t, "ph": mu_t, "qv": s.moist[1] * mu_t,
+                 "mu": torch.ones_like(mu_t)}
+        for var, c in coupled.items():
+            tends[var] = bdy_tend[var] * P[var] * scale[var] / BDY_PERIOD
     for var, c in coupled.items():
         (k0, k1), (j0, j1), (i0, i1) = fill[var]
         ibe = ide if var == "u" else ide - 1
         jbe = jde if var == "v" else jde - 1
-        bxs, bxe, bys, bye = s.bdy[var][:4]
-        for b in range(5):
-            bxs[k0:k1, j0:j1, b] = c[k0:k1, j0:j1, ids + b]
-            bxe[k0:k1, j0:j1, b] = c[k0:k1, j0:j1, ibe - 1 - b]
-            bys[k0:k1, b, i0:i1] = c[k0:k1, jds + b, i0:i1]
-            bye[k0:k1, b, i0:i1] = c[k0:k1, jbe - 1 - b, i0:i1]
+        for arrs, src in ((s.bdy[var][:4], c), (s.bdy[var][4:], tends.get(var))):
+            if src is None:
+                continue
+            bxs, bxe, bys, bye = arrs
+            for b in range(5):
+                bxs[k0:k1, j0:j1, b] = src[k0:k1, j0:j1, ids + b]
+                bxe[k0:k1, j0:j1, b] = src[k0:k1, j0:j1, ibe - 1 - b]
+                bys[k0:k1, b, i0:i1] = src[k0:k1, jds + b, i0:i1]
+                bye[k0:k1, b, i0:i1] = src[k0:k1, jbe - 1 - b, i0:i1]
 
 
 def _cast(s, device, dtype):

@@ -7,16 +7,17 @@ a step (as the reference does); inside the capture every rebound field is
 copied back into the tensor it replaced, so each replay reads and writes the
 same buffers and replays can follow each other.
 
-Scalars the driver computes on the host are baked into the graph: dt, the
-acoustic step and the boundary time dtbc = dt * (step - it_bdy *
-bdy_interval).  dtbc multiplies only the boundary tendencies, so a captured
-step is valid for every step when those tendencies are zero (the case for a
-constant lateral boundary); StepGraph checks this at capture time and refuses
-otherwise -- run such cases eagerly.
+Scalars that stay fixed from step to step (dt, the acoustic step, the
+coefficients) are baked into the graph.  The one that changes, the boundary
+time dtbc = dt * (step - it_bdy * bdy_interval), is formed on the device at
+the start of each captured step from a device step counter (DeviceClock),
+which is set from the host's counters before every replay; the kernels that
+use it (relax_bdytend, relax_bdy_scalar, spec_bdy_final) read it from device
+memory, so replays are exact for time-varying lateral boundaries too.
 """
 import torch
 
-from .step import rk3_step
+from .step import rk3_step, DeviceClock
 
 
 class StepGraph:
@@ -26,22 +27,18 @@ class StepGraph:
     def __init__(self, s, cfg, ops=None, steps=1):
         from . import ops as _ops
         ops = ops or _ops()
-        for var, arrs in s.bdy.items():
-            for a in arrs[4:]:
-                if bool((a != 0).any()):
-                    raise ValueError("boundary tendency of %r is not zero: dtbc changes every step, "
-                                     "capture is only valid for constant boundaries" % var)
         if steps < 1:
             raise ValueError("steps must be >= 1")
         self.s, self.cfg, self.ops, self.steps = s, cfg, ops, steps
         names = [n for n, v in vars(s).items() if isinstance(v, torch.Tensor)]
         orig = {n: getattr(s, n) for n in names}
         step0, it0 = s.step, s.it_bdy
+        self.clock = DeviceClock(s)
         torch.cuda.synchronize()
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
             for _ in range(steps):
-                rk3_step(s, ops, cfg)
+                rk3_step(s, ops, cfg, clock=self.clock)
             for n in names:
                 t = getattr(s, n)
                 if t is not orig[n]:
@@ -55,6 +52,7 @@ class StepGraph:
         (asynchronous)."""
         s, cfg = self.s, self.cfg
         for _ in range(n):
+            self.clock.sync_from(s)
             for _ in range(self.steps):
                 s.step += 1
                 if s.step % cfg.bdy_interval == 1:

@@ -2,10 +2,11 @@
 (pywrf/solver.py:140-447) under the reference's names, in (k, j, i) layout,
 i fastest, FP32 (or FP64), with a 5-point halo.
 
-Lateral-boundary arrays are held contiguous: x-boundaries as (nz, ny, 5),
-y-boundaries as (nz, 5, nx) -- the shapes of the reference's permuted views
-(solver.py:468-703) -- under `s.bdy[var] = (bxs, bxe, bys, bye, btxs, btxe,
-btys, btye)` for var in u, v, ph, t, mu, qv.
+Lateral-boundary arrays are held as the reference driver holds them
+(solver.py:468-703): x-boundaries as the (nz, ny, 5) view permute(1, 2, 0) of
+a (5, nz, ny) array, y-boundaries as the (nz, 5, nx) view permute(1, 0, 2) of
+a (5, nz, nx) array, under `s.bdy[var] = (bxs, bxe, bys, bye, btxs, btxe,
+btys, btye)` for var in u, v, ph, t, mu, qv (nz = 1 for mu).
 """
 from types import SimpleNamespace
 
@@ -55,11 +56,20 @@ def allocate_state(nz, ny, nx, device="cpu", dtype=torch.float32):
     s.bdy = {}
     for v in BDY_VARS:
         kz = 1 if v == "mu" else nz
-        s.bdy[v] = tuple(z(kz, ny, 5) if n % 4 < 2 else z(kz, 5, nx) for n in range(8))
+        s.bdy[v] = tuple(boundary_array(kz, ny, nx, n % 4 < 2, dtype, device) for n in range(8))
     s.gcx = torch.tensor([0., 1.1111111e-3, 7.4074074e-4, 3.7037037e-4, 0.], dtype=dtype, device=device)
     s.fcx = torch.tensor([0., 5.5555557e-3, 3.7037039e-3, 1.8518519e-3, 0.], dtype=dtype, device=device)
     s.step, s.it_bdy = 0, 0
     return s
+
+
+def boundary_array(kz, ny, nx, x_side, dtype=torch.float32, device="cpu"):
+    """A zero lateral-boundary array in the reference driver's layout: for an
+    x-side, the (kz, ny, 5) view of a (5, kz, ny) array; for a y-side, the
+    (kz, 5, nx) view of a (5, kz, nx) array (solver.py:468-470, 478-480)."""
+    if x_side:
+        return torch.zeros((5, kz, ny), dtype=dtype, device=device).permute(1, 2, 0)
+    return torch.zeros((5, kz, nx), dtype=dtype, device=device).permute(1, 0, 2)
 
 
 def state_tensors(s):

@@ -87,15 +87,44 @@ class Grid:
                 self.its + 1, self.ite - 1, self.jts + 1, self.jte - 1, self.kts, self.kte)
 
 
+class DeviceClock:
+    """The step counter, the boundary record index it_bdy and the boundary
+    time dtbc = dt (step - it_bdy bdy_interval) (solver.py:457-458, 713) held
+    as device tensors and advanced by device operations at the start of each
+    step, so that a captured graph of steps forms each replay's dtbc itself.
+    dtbc is formed in double and rounded to float once, the value the
+    kernels take for the host's Python float."""
+
+    def __init__(self, s, device="cuda"):
+        import torch
+        self.step = torch.tensor(s.step, dtype=torch.int64, device=device)
+        self.it_bdy = torch.tensor(s.it_bdy, dtype=torch.int64, device=device)
+        self.dtbc = torch.zeros((), dtype=torch.float32, device=device)
+
+    def sync_from(self, s):
+        """Set the device counters to the host's (asynchronous fills)."""
+        self.step.fill_(s.step)
+        self.it_bdy.fill_(s.it_bdy)
+
+    def advance(self, cfg):
+        import torch
+        self.step.add_(1)
+        rec = torch.div(self.step, cfg.bdy_interval, rounding_mode="floor")
+        self.it_bdy.copy_(torch.where(torch.remainder(self.step, cfg.bdy_interval) == 1, rec, self.it_bdy))
+        self.dtbc.copy_(((self.step - self.it_bdy * cfg.bdy_interval).double() * cfg.dt).float())
+
+
 def _nan_check(p, where):
     import torch
     if bool(torch.isnan(p).any()):
         raise FloatingPointError("NaN in p (%s)" % where)
 
 
-def rk3_step(s, ops, cfg, check_nan=False):
+def rk3_step(s, ops, cfg, check_nan=False, clock=None):
     """Advance state `s` (a namespace of tensors, see pywrf_amd.state) by one
-    RK3 step using the functions in `ops`.  solver.py:454-1910."""
+    RK3 step using the functions in `ops`.  solver.py:454-1910.  With a
+    DeviceClock the boundary time dtbc is its device tensor (the form a
+    captured graph needs); otherwise it is computed on the host."""
     gr = s.grid
     I18, I24, I2 = gr.ix18, gr.ix24, gr.ix2d
     ids, ide, jds, jde, kds, kde = gr.ids, gr.ide, gr.jds, gr.jde, gr.kds, gr.kde
@@ -130,7 +159,11 @@ def rk3_step(s, ops, cfg, check_nan=False):
         s.it_bdy = int(s.step / cfg.bdy_interval)
     dtm = cfg.dt
     dts = cfg.dt / cfg.soundsteps
-    dtbc = cfg.dt * (s.step - s.it_bdy * cfg.bdy_interval)
+    if clock is not None:
+        clock.advance(cfg)
+        dtbc = clock.dtbc
+    else:
+        dtbc = cfg.dt * (s.step - s.it_bdy * cfg.bdy_interval)
     rdx, rdy = cfg.rdx, cfg.rdy
 
     for rk_step in range(1, 4):

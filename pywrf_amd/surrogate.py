@@ -1,0 +1,467 @@
+"""WSM6 column surrogate: data, training and inline use.
+
+A 369 -> 512 -> 512 -> 287 ReLU network maps one microphysics column (41
+levels x T, p, rho, qv, qc, qr, qi, qs, qg) to the tendencies the WSM6 scheme
+(pywrf/wrf_physics.py:137-1214) gives that column over one step (41 levels x
+T, qv, qc, qr, qi, qs, qg).  Everything that touches the network runs in the
+library's own kernels: BF16 matrix-core GEMMs with FP32 accumulation for the
+forward and backward passes, the loss, the bias gradients and Adam
+(csrc/mlp.hip); the inline step packs the columns, runs the three layers,
+applies the tendencies with the fallback checks and re-runs the plain scheme
+on the columns that fail them (csrc/surrogate.hip).  torch provides the
+device memory, the shuffling of the training set and the bookkeeping.
+
+Training data are column pairs written by the plain WSM6 kernel inside the
+step: the state of every tile column just before microphysics_driver and the
+change it made, over seeded runs of the 40 x 40 x 41 case (TRAIN_RECIPE).
+
+    from pywrf_amd import surrogate
+    data = surrogate.collect_pairs(runs=50, steps=140)       # about 5.1 M pairs
+    net = surrogate.Surrogate.fit(data, epochs=8)
+    ops = surrogate.surrogate_ops(net)                       # the step's functions, WSM6 replaced
+"""
+import ctypes
+import math
+import time
+
+import torch
+
+from . import _lib
+from ._lib import call, grid, ptr, stream
+
+NLEV, NIN, NOUT = 41, 9, 7
+FIN, FOUT = NLEV * NIN, NLEV * NOUT          # 369, 287
+KIN, HID, NO = 384, 512, 288                 # padded widths
+IN_VARS = ("T", "p", "rho", "qv", "qc", "qr", "qi", "qs", "qg")
+OUT_VARS = ("T", "qv", "qc", "qr", "qi", "qs", "qg")
+
+# seeded 40 x 40 x 41 runs of the reference's CPU case (one 500 m hill, 10 m/s
+# flow) with the seed choosing six warm bubbles of 1-3 K, u/v noise and cloud /
+# precipitation blobs of all five condensate species, so that every process of
+# the scheme occurs in the training set
+TRAIN_RECIPE = dict(n=40, hills=[(0.5, 0.5, 500.0, 45e3)], jet=None, u0=10.0, noise=0.5, bubbles="random",
+                    clouds=6)
+
+
+def _vp(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def _check_bf16(t, rows, cols):
+    if t.dtype != torch.bfloat16 or not t.is_cuda or not t.is_contiguous() or tuple(t.shape) != (rows, cols):
+        raise TypeError("expected a contiguous bf16 CUDA tensor of shape %s, got %s %s %s"
+                        % ((rows, cols), t.dtype, t.device, tuple(t.shape)))
+
+
+def gemm(epi, A, B, C, bias=None, H=None, ksplit=1):
+    """C = A B^T with the library's matrix-core kernel (see pywrf_mlp_gemm):
+    A bf16 [M x K], B bf16 [N x K]; C f32 or bf16 [M x N]."""
+    M, K = A.shape
+    N = B.shape[0]
+    if B.shape[1] != K or C.shape[0] != M or C.shape[1] < N:
+        raise IndexError("gemm: shapes %s x %s^T -> %s" % (tuple(A.shape), tuple(B.shape), tuple(C.shape)))
+    _check_bf16(A, M, K)
+    _check_bf16(B, N, K)
+    if bias is not None and (bias.numel() < N or bias.dtype != torch.float32):
+        raise IndexError("gemm: bias too short")
+    ldh = 0
+    if H is not None:
+        _check_bf16(H, M, H.shape[1])
+        ldh = H.shape[1]
+    call("pywrf_mlp_gemm", epi, M, N, K, _vp(A), K, _vp(B), K, _vp(C), C.shape[1], _vp(bias), _vp(H), ldh, ksplit,
+         stream())
+
+
+def transpose(x, out):
+    R, C = x.shape
+    if tuple(out.shape) != (C, R):
+        raise IndexError("transpose: %s -> %s" % (tuple(x.shape), tuple(out.shape)))
+    call("pywrf_mlp_transpose", R, C, _vp(x), C, _vp(out), R, stream())
+
+
+def rowsum(x, out):
+    R, C = x.shape
+    call("pywrf_mlp_rowsum", R, C, _vp(x), C, _vp(out), stream())
+
+
+def _to_bf16(x):
+    return x.to(torch.bfloat16).contiguous()
+
+
+class MLP:
+    """Parameters of the network: FP32 masters, their BF16 copies (W and
+    W^T) and the Adam moments.  Padded input columns (369..383) and the padded
+    output row (287) are zero and stay zero (their gradients are zero)."""
+
+    LAYERS = ((HID, KIN, FIN), (HID, HID, HID), (NO, HID, HID))
+
+    def __init__(self, seed=0, device="cuda"):
+        gen = torch.Generator().manual_seed(seed)
+        self.W, self.b = [], []
+        for (rows, cols, fan_in), valid in zip(self.LAYERS, (HID, HID, FOUT)):
+            w = torch.randn(rows, cols, generator=gen) * math.sqrt(2.0 / fan_in)
+            w[:, fan_in:] = 0.0
+            w[valid:] = 0.0
+            self.W.append(w.to(device))
+            self.b.append(torch.zeros(rows, device=device))
+        self.mW = [torch.zeros_like(w) for w in self.W]
+        self.vW = [torch.zeros_like(w) for w in self.W]
+        self.mb = [torch.zeros_like(b) for b in self.b]
+        self.vb = [torch.zeros_like(b) for b in self.b]
+        self.t = 0
+        self.sync()
+
+    def sync(self):
+        """BF16 copies of the FP32 masters."""
+        self.Wb = [_to_bf16(w) for w in self.W]
+        self.Wt = [_to_bf16(w.t()) for w in self.W]
+
+    def state_dict(self):
+        return {"W%d" % n: w for n, w in enumerate(self.W)} | {"b%d" % n: b for n, b in enumerate(self.b)}
+
+    def load_state_dict(self, d):
+        for n in range(3):
+            self.W[n].copy_(d["W%d" % n])
+            self.b[n].copy_(d["b%d" % n])
+        self.sync()
+
+
+class Workspace:
+    """Activation buffers of one batch of M columns (M a multiple of 32)."""
+
+    def __init__(self, M, device="cuda"):
+        if M % 32:
+            raise ValueError("batch size must be a multiple of 32")
+        bf = dict(dtype=torch.bfloat16, device=device)
+        self.M = M
+        self.H1 = torch.empty(M, HID, **bf)
+        self.H2 = torch.empty(M, HID, **bf)
+        self.Y = torch.empty(M, NO, dtype=torch.float32, device=device)
+        self.dY = torch.empty(M, NO, **bf)
+        self.dH2 = torch.empty(M, HID, **bf)
+        self.dH1 = torch.empty(M, HID, **bf)
+        self.Xt = torch.empty(KIN, M, **bf)
+        self.H1t = torch.empty(HID, M, **bf)
+        self.H2t = torch.empty(HID, M, **bf)
+        self.dYt = torch.empty(NO, M, **bf)
+        self.dH1t = torch.empty(HID, M, **bf)
+        self.dH2t = torch.empty(HID, M, **bf)
+        self.gW = [torch.zeros(r, c, device=device) for r, c, _ in MLP.LAYERS]
+        self.gb = [torch.zeros(r, device=device) for r, _, _ in MLP.LAYERS]
+        self.loss = torch.zeros(1, device=device)
+
+
+def forward(net, X, ws):
+    """Y (FP32 [M x 288], standardised tendencies) of the BF16 inputs X [M x 384]."""
+    M = X.shape[0]
+    gemm(1, X, net.Wb[0], ws.H1[:M], net.b[0])
+    gemm(1, ws.H1[:M], net.Wb[1], ws.H2[:M], net.b[1])
+    gemm(0, ws.H2[:M], net.Wb[2], ws.Y[:M], net.b[2])
+    return ws.Y[:M]
+
+
+def backward(net, X, T, w, ws, wsum, ksplit=None):
+    """Loss and parameter gradients (ws.gW, ws.gb) of one batch: the weighted
+    mean squared error sum_n w[n] (Y - T)^2 / (M wsum) of the forward pass
+    just run on X (standardised targets T, FP32 [M x 288]; wsum = sum w)."""
+    M = ws.M
+    ks = ksplit or max(1, min(64, M // 256))
+    ws.loss.zero_()
+    call("pywrf_mlp_mse", M, FOUT, _vp(ws.Y), NO, _vp(T), NO, _vp(w), 1.0 / (M * wsum), _vp(ws.dY), NO,
+         _vp(ws.loss), stream())
+    # layer 3: dW3 = dY^T H2, db3 = sum dY, dH2 = (dY W3) [H2 > 0]
+    transpose(ws.dY, ws.dYt)
+    transpose(ws.H2, ws.H2t)
+    for g in ws.gW:
+        g.zero_()
+    gemm(3, ws.dYt, ws.H2t, ws.gW[2], ksplit=ks)
+    rowsum(ws.dYt, ws.gb[2])
+    gemm(2, ws.dY, net.Wt[2], ws.dH2, H=ws.H2)
+    # layer 2
+    transpose(ws.dH2, ws.dH2t)
+    transpose(ws.H1, ws.H1t)
+    gemm(3, ws.dH2t, ws.H1t, ws.gW[1], ksplit=ks)
+    rowsum(ws.dH2t, ws.gb[1])
+    gemm(2, ws.dH2, net.Wt[1], ws.dH1, H=ws.H1)
+    # layer 1
+    transpose(ws.dH1, ws.dH1t)
+    transpose(X, ws.Xt)
+    gemm(3, ws.dH1t, ws.Xt, ws.gW[0], ksplit=ks)
+    rowsum(ws.dH1t, ws.gb[0])
+    return ws.loss
+
+
+def loss_weights(device="cuda"):
+    """Per-output weights of the loss: 1 on the 7 x 40 tendencies, 0 on the
+    padding (level 40 of each field, output 287)."""
+    w = torch.ones(NO, device=device)
+    w[FOUT:] = 0.0
+    for v in range(NOUT):
+        w[v * NLEV + NLEV - 1] = 0.0
+    return w
+
+
+def adam_step(net, ws, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+    net.t += 1
+    for n in range(3):
+        W = net.W[n]
+        call("pywrf_mlp_adam", W.shape[0], W.shape[1], _vp(W), _vp(ws.gW[n]), _vp(net.mW[n]), _vp(net.vW[n]),
+             float(lr), beta1, beta2, eps, 1.0, net.t, _vp(net.Wb[n]), _vp(net.Wt[n]), stream())
+        b = net.b[n]
+        call("pywrf_mlp_adam", 1, b.shape[0], _vp(b), _vp(ws.gb[n]), _vp(net.mb[n]), _vp(net.vb[n]), float(lr),
+             beta1, beta2, eps, 1.0, net.t, None, None, stream())
+
+
+# ----------------------------------------------------------------- data
+class Pairs:
+    """Column pairs on the device: raw inputs (FP32 [N x 369], feature
+    var * 41 + k of IN_VARS, level 40 zero) and WSM6 tendencies (FP32
+    [N x 287], var * 41 + k of OUT_VARS, per second)."""
+
+    def __init__(self, X, Y):
+        self.X, self.Y = X, Y
+
+    def __len__(self):
+        return self.X.shape[0]
+
+
+def _tile(th):
+    """The scheme's tile slices of a (41, ny, nx) field (wsm6's columns: the
+    physics index set, one point in from the step's tile on each side, and
+    levels [0, 40))."""
+    return slice(6, th.shape[1] - 7), slice(6, th.shape[2] - 7)
+
+
+def _columns(field, js, is_, km=NLEV - 1):
+    """(ncol, 41) view-copy of a field's tile columns, level 40 zero."""
+    c = field[:km, js, is_].permute(1, 2, 0).reshape(-1, km)
+    return torch.nn.functional.pad(c, (0, NLEV - km))
+
+
+class _Recorder:
+    """Wraps microphysics_driver: the inputs and the change of each call are
+    appended to the lists."""
+
+    def __init__(self, ops):
+        self._ops = ops
+        self.inputs, self.tends = [], []
+
+    def __getattr__(self, name):
+        return getattr(self._ops, name)
+
+    def microphysics_driver(self, th, rho, pi_phy, p, ht, dz8w, p8w, dt, *rest):
+        js, is_ = _tile(th)
+        q = rest[2:8]
+        before_T = th * pi_phy
+        ins = [before_T, p, rho] + list(q)
+        X = torch.cat([_columns(f, js, is_) for f in ins], dim=1)
+        old = [before_T.clone()] + [x.clone() for x in q]
+        out = self._ops.microphysics_driver(th, rho, pi_phy, p, ht, dz8w, p8w, dt, *rest)
+        new = [th * pi_phy] + list(q)
+        Y = torch.cat([_columns((a - b) / dt, js, is_) for a, b in zip(new, old)], dim=1)
+        self.inputs.append(X)
+        self.tends.append(Y)
+        return out
+
+
+def collect_pairs(runs=50, steps=140, seed0=100, log=None):
+    """Column pairs of the plain WSM6 kernel over `runs` seeded runs of
+    TRAIN_RECIPE, `steps` RK3 steps each (729 columns per step at 40 x 40)."""
+    import pywrf_amd
+    from .gen import synthetic_state
+    from .step import rk3_step, RunConfig
+    cfg = RunConfig()
+    Xs, Ys = [], []
+    t0 = time.perf_counter()
+    for r in range(runs):
+        s = synthetic_state(dict(TRAIN_RECIPE, seed=seed0 + r), device="cuda")
+        rec = _Recorder(pywrf_amd.ops())
+        for _ in range(steps):
+            rk3_step(s, rec, cfg)
+        Xs.append(torch.cat(rec.inputs))
+        Ys.append(torch.cat(rec.tends))
+        if log:
+            log("collect: run %d/%d, %d pairs, %.1f s" % (r + 1, runs, sum(x.shape[0] for x in Xs),
+                                                        time.perf_counter() - t0))
+    return Pairs(torch.cat(Xs), torch.cat(Ys))
+
+
+# ----------------------------------------------------------------- the trained network
+class Surrogate:
+    """The network with its standardisation and fallback tolerances."""
+
+    QTOL = 1e-7      # kg/kg: smaller negative mixing ratios fail the column
+    WTOL = 1e-5      # relative column water gain that fails the column
+
+    def __init__(self, net, in_mean, in_std, out_mean, out_std):
+        self.net = net
+        dev = net.W[0].device
+        self.in_mean = torch.zeros(KIN, device=dev)
+        self.in_rstd = torch.zeros(KIN, device=dev)
+        self.in_mean[:FIN] = in_mean
+        self.in_rstd[:FIN] = 1.0 / in_std
+        self.out_mean = torch.zeros(NO, device=dev)
+        self.out_std = torch.zeros(NO, device=dev)
+        self.out_mean[:FOUT] = out_mean
+        self.out_std[:FOUT] = out_std
+        self.history = []
+
+    @staticmethod
+    def stats(x, floor):
+        mean = x.mean(0)
+        std = x.std(0)
+        return mean, torch.clamp(std, min=floor)
+
+    @classmethod
+    def untrained(cls, data, seed=0):
+        in_mean, in_std = cls.stats(data.X, 1e-12)
+        in_std[(data.X.std(0) == 0)] = 1.0
+        out_mean, out_std = cls.stats(data.Y, 1e-12)
+        out_std[(data.Y.std(0) == 0)] = 1.0
+        return cls(MLP(seed, device=data.X.device), in_mean, in_std, out_mean, out_std)
+
+    def standardise(self, data):
+        X = torch.zeros(len(data), KIN, dtype=torch.bfloat16, device=data.X.device)
+        X[:, :FIN] = ((data.X - self.in_mean[:FIN]) * self.in_rstd[:FIN]).to(torch.bfloat16)
+        T = torch.zeros(len(data), NO, device=data.X.device)
+        T[:, :FOUT] = (data.Y - self.out_mean[:FOUT]) / self.out_std[:FOUT]
+        return X, T
+
+    @classmethod
+    def fit(cls, data, epochs=8, batch=8192, lr=1e-3, seed=0, log=None, valid=None):
+        """Train on the GPU with the library's kernels (Adam, cosine decay)."""
+        self = cls.untrained(data, seed)
+        X, T = self.standardise(data)
+        w = loss_weights(X.device)
+        wsum = float(w.sum())
+        self.loss_weight = w
+        ws = Workspace(batch, device=X.device)
+        n = X.shape[0]
+        nb = n // batch
+        total = epochs * nb
+        gen = torch.Generator(device=X.device).manual_seed(seed)
+        step = 0
+        t0 = time.perf_counter()
+        for ep in range(epochs):
+            perm = torch.randperm(n, device=X.device, generator=gen)
+            acc = torch.zeros((), device=X.device)
+            for bi in range(nb):
+                idx = perm[bi * batch:(bi + 1) * batch]
+                xb = X.index_select(0, idx)
+                tb = T.index_select(0, idx)
+                forward(self.net, xb, ws)
+                acc += backward(self.net, xb, tb, w, ws, wsum)[0]
+                lr_t = lr * 0.5 * (1.0 + math.cos(math.pi * step / total))
+                adam_step(self.net, ws, lr_t)
+                step += 1
+            loss = float(acc) / max(nb, 1)
+            rec = {"epoch": ep + 1, "train_loss": loss, "seconds": time.perf_counter() - t0}
+            if valid is not None:
+                rec["valid_loss"] = self.loss(valid, batch)
+            self.history.append(rec)
+            if log:
+                log("fit: " + " ".join("%s %.4g" % kv for kv in rec.items()))
+        self.train_seconds = time.perf_counter() - t0
+        self.train_steps = step
+        return self
+
+    def loss(self, data, batch=8192):
+        """Weighted mean squared error (standardised) over a set of pairs."""
+        X, T = self.standardise(data)
+        ws = Workspace(batch, device=X.device)
+        w = loss_weights(X.device)
+        wsum = float(w.sum())
+        tot, cnt = 0.0, 0
+        for b0 in range(0, X.shape[0] - batch + 1, batch):
+            forward(self.net, X[b0:b0 + batch], ws)
+            tot += float(backward(self.net, X[b0:b0 + batch], T[b0:b0 + batch], w, ws, wsum)[0]) * batch
+            cnt += batch
+        return tot / max(cnt, 1)
+
+    def predict(self, data, batch=8192):
+        """De-standardised tendencies (FP32 [N x 287]) of a set of pairs."""
+        X, _ = self.standardise(data)
+        ws = Workspace(batch, device=X.device)
+        out = []
+        for b0 in range(0, X.shape[0], batch):
+            xb = X[b0:b0 + batch]
+            Y = forward(self.net, xb, ws)
+            out.append(Y[:, :FOUT] * self.out_std[:FOUT] + self.out_mean[:FOUT])
+        return torch.cat(out)
+
+    def struct(self):
+        s = _lib.Surrogate()
+        net = self.net
+        s.w1, s.w2, s.w3 = (t.data_ptr() for t in net.Wb)
+        s.b1, s.b2, s.b3 = (t.data_ptr() for t in net.b)
+        s.in_mean, s.in_rstd = self.in_mean.data_ptr(), self.in_rstd.data_ptr()
+        s.out_mean, s.out_std = self.out_mean.data_ptr(), self.out_std.data_ptr()
+        s.qtol, s.wtol = self.QTOL, self.WTOL
+        return s
+
+    def save(self, path):
+        torch.save({"net": {k: v.cpu() for k, v in self.net.state_dict().items()},
+                    "in_mean": self.in_mean.cpu(), "in_rstd": self.in_rstd.cpu(),
+                    "out_mean": self.out_mean.cpu(), "out_std": self.out_std.cpu()}, path)
+
+    @classmethod
+    def load(cls, path, device="cuda"):
+        d = torch.load(path, map_location=device, weights_only=True)
+        self = cls.__new__(cls)
+        self.net = MLP(0, device=device)
+        self.net.load_state_dict(d["net"])
+        self.in_mean, self.in_rstd = d["in_mean"], d["in_rstd"]
+        self.out_mean, self.out_std = d["out_mean"], d["out_std"]
+        self.history = []
+        return self
+
+
+# ----------------------------------------------------------------- inline use
+class _SurrogateOps:
+    def __init__(self, ops, sur, chunk, qtol=None, wtol=None):
+        self._ops, self.sur, self.chunk = ops, sur, chunk
+        self._ws = {}
+        self.nfallback = torch.zeros(1, dtype=torch.int32, device="cuda")
+        self._struct = sur.struct()
+        if qtol is not None:
+            self._struct.qtol = qtol
+        if wtol is not None:
+            self._struct.wtol = wtol
+
+    def __getattr__(self, name):
+        return getattr(self._ops, name)
+
+    def microphysics_driver(self, th, rho, pi_phy, p, ht,
+                            dz8w, p8w, dt, dx, dy,
+                            qv_curr, qc_curr, qr_curr, qi_curr, qs_curr, qg_curr,
+                            rainnc, rainncv, snownc, snowncv, sr,
+                            refl_10cm, diagflag, do_radar_ref,
+                            graupelnc, graupelncv,
+                            has_reqc, has_reqi, has_reqs,
+                            re_cloud, re_ice, re_snow,
+                            ids, ide, jds, jde, kds, kde,
+                            ims, ime, jms, jme, kms, kme,
+                            its, ite, jts, jte, kts, kte):
+        """microphysics_driver's signature and results (wrf_physics.py:94-134),
+        the columns computed by the surrogate with per-column WSM6 fallback."""
+        g = grid(th.shape, (ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, its, ite, jts, jte, kts, kte))
+        key = tuple(th.shape) + (its, ite, jts, jte)
+        ws = self._ws.get(key)
+        if ws is None:
+            nbytes = ctypes.c_longlong()
+            call("pywrf_wsm6_surrogate_workspace", g, self.chunk, ctypes.byref(nbytes))
+            ws = self._ws[key] = torch.empty(nbytes.value, dtype=torch.uint8, device="cuda")
+        call("pywrf_wsm6_surrogate", g, ctypes.addressof(self._struct), ptr(th), ptr(rho), ptr(pi_phy), ptr(p), ptr(dz8w),
+             float(dt), ptr(qv_curr), ptr(qc_curr), ptr(qr_curr), ptr(qi_curr), ptr(qs_curr), ptr(qg_curr),
+             ptr(rainnc), ptr(rainncv), ptr(snownc), ptr(snowncv), ptr(sr), ptr(graupelnc), ptr(graupelncv),
+             _vp(ws), ws.numel(), _vp(self.nfallback), stream())
+        return (th, qv_curr, qc_curr, qr_curr, qi_curr, qs_curr, qg_curr, rainnc, rainncv, snownc, snowncv,
+                graupelnc, graupelncv, sr, refl_10cm)
+
+
+def surrogate_ops(sur, ops=None, chunk=16384, qtol=None, wtol=None):
+    """The step's functions with microphysics_driver served by the surrogate
+    (ops: the namespace to wrap, pywrf_amd.ops() by default; qtol / wtol
+    override the fallback tolerances)."""
+    import pywrf_amd
+    return _SurrogateOps(ops or pywrf_amd.ops(), sur, chunk, qtol, wtol)

@@ -1,11 +1,37 @@
 """Drop-in for pywrf/wrf_boundary.py on the ARW step's path (same names,
 positional parameters and returns), forwarding to libpywrf_amd.so.
 
-Boundary arrays are contiguous: x-boundaries (kz, ny, 5), y-boundaries
-(kz, 5, nx) -- the shapes of the reference driver's permuted views
-(solver.py:468-703).  Fields are updated in place, as in the reference.
+Boundary arrays are the reference driver's: x-boundaries (kz, ny, 5) and
+y-boundaries (kz, 5, nx), as the permuted views of (5, kz, ny) / (5, kz, nx)
+arrays it builds (solver.py:468-703), which the kernels read in place; arrays
+in another layout are copied into that one (see _lib.boundary_layout).  kz is
+the level count of the field they belong to.  Fields are updated in place, as
+in the reference.
 """
-from ._lib import call, grid, ptr, flag, stream
+import torch
+
+from ._lib import call, grid, ptr, flag, stream, bdy_ptrs, check_operands
+
+
+def _width(spec_bdy_width):
+    if int(spec_bdy_width) != 5:
+        raise IndexError("spec_bdy_width %r: the boundary arrays of the kernels are 5 points wide"
+                         % (spec_bdy_width,))
+
+
+def _bdy(g, arrs):
+    gc = g.contents
+    return bdy_ptrs(arrs, gc.nz, gc.ny, gc.nx)
+
+
+def _on_device(dtbc):
+    """True when the boundary time is a device tensor (one float, read by the
+    kernel when it runs: the form a captured graph of the step uses)."""
+    if isinstance(dtbc, torch.Tensor):
+        if dtbc.numel() != 1:
+            raise TypeError("dtbc: expected one value, got shape %s" % (tuple(dtbc.shape),))
+        return True
+    return False
 
 
 def _I24(ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, ips, ipe, jps, jpe, kps, kpe,
@@ -27,12 +53,17 @@ def relax_bdytend(field, field_tend,
                   ips, ipe, jps, jpe, kps, kpe,
                   its, ite, jts, jte, kts, kte):
     """wrf_boundary.py:52-161"""
+    _width(spec_bdy_width)
     g = grid(field.shape, _I24(ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, ips, ipe, jps, jpe, kps,
                                kpe, its, ite, jts, jte, kts, kte))
-    call("pywrf_relax_bdytend", g, ptr(field), ptr(field_tend), ptr(field_bdy_xs), ptr(field_bdy_xe),
-         ptr(field_bdy_ys), ptr(field_bdy_ye), ptr(field_bdy_tend_xs), ptr(field_bdy_tend_xe),
-         ptr(field_bdy_tend_ys), ptr(field_bdy_tend_ye), flag(variable_in), float(dtbc), ptr(fcx), ptr(gcx),
-         stream())
+    bp = _bdy(g, (field_bdy_xs, field_bdy_xe, field_bdy_ys, field_bdy_ye, field_bdy_tend_xs, field_bdy_tend_xe,
+                  field_bdy_tend_ys, field_bdy_tend_ye))
+    if _on_device(dtbc):
+        call("pywrf_relax_bdytend_dtbc_dev", g, ptr(field), ptr(field_tend), *bp, flag(variable_in), ptr(dtbc),
+             ptr(fcx), ptr(gcx), stream())
+    else:
+        call("pywrf_relax_bdytend", g, ptr(field), ptr(field_tend), *bp, flag(variable_in), float(dtbc), ptr(fcx),
+             ptr(gcx), stream())
     return field_tend
 
 
@@ -48,10 +79,12 @@ def spec_bdytend(field_tend,
                  ips, ipe, jps, jpe, kps, kpe,
                  its, ite, jts, jte, kts, kte):
     """wrf_boundary.py:188-225"""
+    _width(spec_bdy_width)
     g = grid(field_tend.shape, _I24(ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, ips, ipe, jps, jpe,
                                     kps, kpe, its, ite, jts, jte, kts, kte))
-    call("pywrf_spec_bdytend", g, ptr(field_tend), ptr(field_bdy_tend_xs), ptr(field_bdy_tend_xe),
-         ptr(field_bdy_tend_ys), ptr(field_bdy_tend_ye), flag(variable_in), stream())
+    _bdy(g, (field_bdy_xs, field_bdy_xe, field_bdy_ys, field_bdy_ye))           # shape checks only
+    bp = _bdy(g, (field_bdy_tend_xs, field_bdy_tend_xe, field_bdy_tend_ys, field_bdy_tend_ye))
+    call("pywrf_spec_bdytend", g, ptr(field_tend), *bp, flag(variable_in), stream())
     return field_tend
 
 
@@ -124,11 +157,17 @@ def relax_bdy_scalar(scalar_tend,
                      ips, ipe, jps, jpe, kps, kpe,
                      its, ite, jts, jte, kts, kte):
     """wrf_boundary.py:418-451"""
+    _width(spec_bdy_width)
     g = grid(scalar.shape, _I24(ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, ips, ipe, jps, jpe, kps,
                                 kpe, its, ite, jts, jte, kts, kte))
-    call("pywrf_relax_bdy_scalar", g, ptr(scalar_tend), ptr(scalar), ptr(mu), ptr(scalar_bxs), ptr(scalar_bxe),
-         ptr(scalar_bys), ptr(scalar_bye), ptr(scalar_btxs), ptr(scalar_btxe), ptr(scalar_btys), ptr(scalar_btye),
-         float(dtbc), ptr(fcx), ptr(gcx), stream())
+    bp = _bdy(g, (scalar_bxs, scalar_bxe, scalar_bys, scalar_bye, scalar_btxs, scalar_btxe, scalar_btys,
+                  scalar_btye))
+    if _on_device(dtbc):
+        call("pywrf_relax_bdy_scalar_dtbc_dev", g, ptr(scalar_tend), ptr(scalar), ptr(mu), *bp, ptr(dtbc), ptr(fcx),
+             ptr(gcx), stream())
+    else:
+        call("pywrf_relax_bdy_scalar", g, ptr(scalar_tend), ptr(scalar), ptr(mu), *bp, float(dtbc), ptr(fcx),
+             ptr(gcx), stream())
     return scalar_tend
 
 
@@ -141,10 +180,12 @@ def spec_bdy_scalar(scalar_tend,
                     ips, ipe, jps, jpe, kps, kpe,
                     its, ite, jts, jte, kts, kte):
     """wrf_boundary.py:454-472"""
+    _width(spec_bdy_width)
     g = grid(scalar_tend.shape, _I24(ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, ips, ipe, jps, jpe,
                                      kps, kpe, its, ite, jts, jte, kts, kte))
-    call("pywrf_spec_bdy_scalar", g, ptr(scalar_tend), ptr(scalar_btxs), ptr(scalar_btxe), ptr(scalar_btys),
-         ptr(scalar_btye), stream())
+    _bdy(g, (scalar_bxs, scalar_bxe, scalar_bys, scalar_bye))                   # shape checks only
+    bp = _bdy(g, (scalar_btxs, scalar_btxe, scalar_btys, scalar_btye))
+    call("pywrf_spec_bdy_scalar", g, ptr(scalar_tend), *bp, stream())
     return scalar_tend
 
 
@@ -175,11 +216,19 @@ def spec_bdy_final(field, mu, c1, c2, msf,
                    ips, ipe, jps, jpe, kps, kpe,
                    its, ite, jts, jte, kts, kte):
     """wrf_boundary.py:513-587"""
+    _width(spec_bdy_width)
+    if _on_device(dtbc):
+        spec_bdy_final_group([(field, mu, msf, (field_bdy_xs, field_bdy_xe, field_bdy_ys, field_bdy_ye,
+                                                field_bdy_tend_xs, field_bdy_tend_xe, field_bdy_tend_ys,
+                                                field_bdy_tend_ye), variable_in)], dtbc,
+                             (ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, ips, ipe, jps, jpe, kps,
+                              kpe, its, ite, jts, jte, kts, kte))
+        return field
     g = grid(field.shape, _I24(ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, ips, ipe, jps, jpe, kps,
                                kpe, its, ite, jts, jte, kts, kte))
-    call("pywrf_spec_bdy_final", g, ptr(field), ptr(mu), ptr(msf), ptr(field_bdy_xs), ptr(field_bdy_xe),
-         ptr(field_bdy_ys), ptr(field_bdy_ye), ptr(field_bdy_tend_xs), ptr(field_bdy_tend_xe),
-         ptr(field_bdy_tend_ys), ptr(field_bdy_tend_ye), flag(variable_in), float(dtbc), stream())
+    bp = _bdy(g, (field_bdy_xs, field_bdy_xe, field_bdy_ys, field_bdy_ye, field_bdy_tend_xs, field_bdy_tend_xe,
+                  field_bdy_tend_ys, field_bdy_tend_ye))
+    call("pywrf_spec_bdy_final", g, ptr(field), ptr(mu), ptr(msf), *bp, flag(variable_in), float(dtbc), stream())
     return field
 
 
@@ -240,8 +289,11 @@ def spec_bdytend_group(items, idx24, idx2d):
     shape3 = t3.shape if t3 is not None else (idx24[5],) + tuple(items[0][0].shape[-2:])
     shape2 = t2.shape if t2 is not None else (1,) + tuple(items[0][0].shape[-2:])
     g3, g2 = grid(shape3, _I24(*idx24)), grid(shape2, _I24(*idx2d))
+    for t, _, _, two in items:
+        check_operands("spec_bdytend_group", (g2 if two else g3).contents, ("tend",), (ptr(t),))
     tends = (ctypes.c_void_p * n)(*[ptr(t).value for t, _, _, _ in items])
-    bt = (ctypes.c_void_p * (4 * n))(*[ptr(a).value for _, arrs, _, _ in items for a in arrs])
+    bps = [p for _, arrs, _, two in items for p in _bdy(g2 if two else g3, arrs)]
+    bt = (ctypes.c_void_p * (4 * n))(*[p.value for p in bps])
     flags = b"".join(v.encode() for _, _, v, _ in items)
     kinds = (ctypes.c_int * n)(*[1 if two else 0 for _, _, _, two in items])
     call("pywrf_spec_bdytend_group", g3, g2, n, tends, bt, flags, kinds, stream())
@@ -250,8 +302,8 @@ def spec_bdytend_group(items, idx24, idx2d):
 def spec_bdy_final_group(items, dtbc, idx24):
     """A run of spec_bdy_final calls on distinct fields in one launch.
     items: [(field, mu, msf, (bxs, bxe, bys, bye, btxs, btxe, btys, btye),
-    variable)].  Same result as the calls in order.  Not part of the reference
-    API."""
+    variable)]; dtbc a number or a one-value device tensor.  Same result as
+    the calls in order.  Not part of the reference API."""
     import ctypes
     n = len(items)
     if n == 0:
@@ -259,10 +311,17 @@ def spec_bdy_final_group(items, dtbc, idx24):
     if len({f.data_ptr() for f, _, _, _, _ in items}) != n:
         raise ValueError("spec_bdy_final_group: a field appears twice")
     g = grid(items[0][0].shape, _I24(*idx24))
+    for f, mu, msf, _, _ in items:
+        check_operands("spec_bdy_final_group", g.contents, ("field", "mu", "msf"), (ptr(f), ptr(mu), ptr(msf)))
     vp = lambda xs: (ctypes.c_void_p * len(xs))(*[ptr(x).value for x in xs])
     fields = vp([f for f, _, _, _, _ in items])
     mus = vp([m for _, m, _, _, _ in items])
     msfs = vp([m for _, _, m, _, _ in items])
-    bdys = vp([a for _, _, _, arrs, _ in items for a in arrs])
+    bps = [p for _, _, _, arrs, _ in items for p in _bdy(g, arrs)]
+    bdys = (ctypes.c_void_p * len(bps))(*[p.value for p in bps])
     flags = b"".join(v.encode() for _, _, _, _, v in items)
-    call("pywrf_spec_bdy_final_group", g, n, fields, mus, msfs, bdys, flags, float(dtbc), stream())
+    if _on_device(dtbc):
+        check_operands("spec_bdy_final_group", g.contents, ("dtbc",), (ptr(dtbc),))
+        call("pywrf_spec_bdy_final_group_dtbc_dev", g, n, fields, mus, msfs, bdys, flags, ptr(dtbc), stream())
+    else:
+        call("pywrf_spec_bdy_final_group", g, n, fields, mus, msfs, bdys, flags, float(dtbc), stream())

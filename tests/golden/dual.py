@@ -42,12 +42,33 @@ def rel_err(x, y):
     return d / s if s > 0 else d
 
 
+def local_err(x, y, before):
+    """rel_err over the points either implementation changed (x or y differs
+    from `before`): the error of a function that writes only a strip or a
+    ring, measured on that strip against its own values.  0.0 when neither
+    changed anything."""
+    x = x.double().cpu()
+    y = y.double().cpu()
+    b = before.double().cpu()
+    m = (x != b) | (y != b)
+    m &= torch.isfinite(y) & torch.isfinite(x)
+    if not bool(m.any()):
+        return 0.0
+    d = (x[m] - y[m]).abs().max().item()
+    s = y[m].abs().max().item()
+    return d / s if s > 0 else d
+
+
 class DualOps:
-    def __init__(self, a, b, b_device=None, on_call=None):
+    """local: names of functions whose updated arguments are also compared on
+    the points they changed (local_err, recorded as "local_arg<n>")."""
+
+    def __init__(self, a, b, b_device=None, on_call=None, local=()):
         self._a, self._b, self._bdev = a, b, b_device
         self.records = []
         self._on_call = on_call
         self._count = {}
+        self._local = set(local)
 
     def __getattr__(self, name):
         fa = getattr(self._a, name)
@@ -58,6 +79,7 @@ class DualOps:
             self._count[name] = n + 1
             args_a = _clone_args(args)
             args_b = _clone_args(args, self._bdev)
+            pre = _clone_args(args) if name in self._local else None
             ra = _as_tuple(fa(*args_a))
             rb = _as_tuple(fb(*args_b))
             errs = []
@@ -67,6 +89,8 @@ class DualOps:
             for i, (x, y) in enumerate(zip(args_b, args_a)):
                 if isinstance(y, torch.Tensor):
                     errs.append(("arg%d" % i, rel_err(x, y)))
+                    if pre is not None and pre[i].shape == y.shape:
+                        errs.append(("local_arg%d" % i, local_err(x, y, pre[i])))
             rec = dict(name=name, call=n, errs=errs)
             self.records.append(rec)
             if self._on_call is not None:

@@ -14,6 +14,13 @@ the oracle (dual.py).  Written, per workload:
   tensor argument after the call, [sum, sum|x|, max|x|, count of non-finite];
 * state_<name>_s<N>.npz -- the prognostic fields (u v w t ph p al mu moist)
   of the reference trajectory after N steps;
+* for the workload with time-varying lateral boundaries (bdy24), the call
+  statistics of steps 1-3 (calls_bdy24.npz, with the step of each row) and
+  bdycalls_bdy24.npz: the full argument arrays (before the call) and the
+  returned array of every lateral-boundary call of step 2 that reads the
+  boundary arrays (relax_bdytend, spec_bdytend of the first RK stage,
+  relax_bdy_scalar, spec_bdy_scalar, spec_bdy_final), with bdycalls_bdy24.json listing each
+  call's scalar arguments;
 * meta.json -- workload list, torch version, and the largest call-level
   difference seen between the oracle and the reference.
 
@@ -40,7 +47,10 @@ import oracle                                # noqa: E402
 from pywrf_amd.gen import synthetic_state   # noqa: E402
 from pywrf_amd.step import rk3_step, RunConfig  # noqa: E402
 
-WORKLOADS = [("config1", 10, (1, 10)), ("config1c", 2, (1,))]
+# (name, steps run, state snapshots, steps whose call statistics are kept)
+WORKLOADS = [("config1", 10, (1, 10), (1,)), ("config1c", 2, (1,), (1,)), ("bdy24", 3, (1, 3), (1, 2, 3))]
+BDY_FUNCS = ("relax_bdytend", "spec_bdytend", "relax_bdy_scalar", "spec_bdy_scalar", "spec_bdy_final")
+BDY_STEP = 2
 PROG = ("u", "v", "w", "t", "ph", "p", "al", "mu")
 
 
@@ -52,20 +62,68 @@ def save_state(path, s):
     np.savez_compressed(path, **arrs)
 
 
+def record_boundary_calls(ref, r_step, store):
+    """Wrap the reference's boundary functions so that every call of step
+    BDY_STEP saves its arguments (before the call) and its result."""
+    count = {}
+    for fname in BDY_FUNCS:
+        f = getattr(ref, fname)
+
+        def wrapped(*args, _f=f, _n=fname):
+            if r_step[0] != BDY_STEP:
+                return _f(*args)
+            c = count.get(_n, 0)
+            count[_n] = c + 1
+            if _n == "spec_bdytend" and c >= 5:
+                return _f(*args)          # stages 2 and 3 repeat the first stage's five copies
+            key = "%s.%d" % (_n, c)
+            ins = [a.detach().clone() if isinstance(a, torch.Tensor) else a for a in args]
+            out = _f(*args)
+            store[key] = (ins, out.detach().clone())
+            return out
+        setattr(ref, fname, wrapped)
+
+
+def save_boundary_calls(name, store):
+    arrs, calls = {}, {}
+    for key, (ins, out) in store.items():
+        scalars = []
+        for n, a in enumerate(ins):
+            if isinstance(a, torch.Tensor):
+                arrs["%s.arg%d" % (key, n)] = a.numpy().astype(np.float32)
+                scalars.append(None)
+            else:
+                scalars.append(a)
+        arrs["%s.out" % key] = out.numpy().astype(np.float32)
+        calls[key] = scalars
+    np.savez_compressed(os.path.join(HERE, "bdycalls_%s.npz" % name), **arrs)
+    with open(os.path.join(HERE, "bdycalls_%s.json" % name), "w") as f:
+        json.dump(calls, f, indent=0, sort_keys=True)
+
+
 def main():
     torch.set_num_threads(int(os.environ.get("GOLDEN_THREADS", "8")))
     meta = {"torch": torch.__version__, "workloads": {}}
-    for name, nsteps, snaps in WORKLOADS:
+    only = sys.argv[1:]
+    mpath = os.path.join(HERE, "meta.json")
+    if only and os.path.exists(mpath):
+        meta = json.load(open(mpath))
+    for name, nsteps, snaps, stat_steps in WORKLOADS:
+        if only and name not in only:
+            continue
         s = synthetic_state(name)
         ref = load_reference(s.grid.ny, s.grid.nx)
         rows = []
 
         def rec(r, args_a, ret_a):
-            if r_step[0] == 1:
+            if r_step[0] in stat_steps:
                 rows.append((r["name"], [stats_of(x) for x in ret_a if isinstance(x, torch.Tensor)],
-                             [stats_of(x) for x in args_a if isinstance(x, torch.Tensor)]))
+                             [stats_of(x) for x in args_a if isinstance(x, torch.Tensor)], r_step[0]))
 
         r_step = [0]
+        store = {}
+        if name == "bdy24":
+            record_boundary_calls(ref, r_step, store)
         ops = DualOps(ref, oracle.ops(), on_call=rec)
         worst = {}
         t0 = time.time()
@@ -81,13 +139,17 @@ def main():
         nret = np.array([len(r[1]) for r in rows])
         narg = np.array([len(r[2]) for r in rows])
         stats = np.array([x for r in rows for x in (r[1] + r[2])], dtype=np.float64)
+        steps = np.array([r[3] for r in rows])
         np.savez_compressed(os.path.join(HERE, "calls_%s.npz" % name), names=names, nret=nret, narg=narg,
-                            stats=stats)
-        meta["workloads"][name] = dict(steps=nsteps, snapshots=list(snaps), calls_step1=len(rows),
+                            stats=stats, step=steps)
+        if store:
+            save_boundary_calls(name, store)
+        meta["workloads"][name] = dict(steps=nsteps, snapshots=list(snaps), stat_steps=list(stat_steps),
+                                       calls_step1=int((steps == 1).sum()),
                                        seconds=time.time() - t0,
                                        oracle_vs_reference_max_rel=worst)
         print(name, "done in %.1fs" % (time.time() - t0), "worst", max(worst.values()))
-    with open(os.path.join(HERE, "meta.json"), "w") as f:
+    with open(mpath, "w") as f:
         json.dump(meta, f, indent=1, sort_keys=True)
 
 
