@@ -293,32 +293,30 @@ class Surrogate:
     QTOL = 1e-7      # kg/kg: smaller negative mixing ratios fail the column
     WTOL = 1e-5      # relative column water gain that fails the column
 
-    def __init__(self, net, in_mean, in_std, out_mean, out_std):
+    def __init__(self, net, in_mean, in_rstd, out_mean, out_std):
         self.net = net
         dev = net.W[0].device
         self.in_mean = torch.zeros(KIN, device=dev)
         self.in_rstd = torch.zeros(KIN, device=dev)
         self.in_mean[:FIN] = in_mean
-        self.in_rstd[:FIN] = 1.0 / in_std
+        self.in_rstd[:FIN] = in_rstd
         self.out_mean = torch.zeros(NO, device=dev)
         self.out_std = torch.zeros(NO, device=dev)
         self.out_mean[:FOUT] = out_mean
         self.out_std[:FOUT] = out_std
         self.history = []
 
-    @staticmethod
-    def stats(x, floor):
-        mean = x.mean(0)
-        std = x.std(0)
-        return mean, torch.clamp(std, min=floor)
-
     @classmethod
     def untrained(cls, data, seed=0):
-        in_mean, in_std = cls.stats(data.X, 1e-12)
-        in_std[(data.X.std(0) == 0)] = 1.0
-        out_mean, out_std = cls.stats(data.Y, 1e-12)
-        out_std[(data.Y.std(0) == 0)] = 1.0
-        return cls(MLP(seed, device=data.X.device), in_mean, in_std, out_mean, out_std)
+        """The network at its seeded initialisation, with the standardisation
+        of `data`: features that never vary are ignored (rstd 0), outputs that
+        never vary are de-standardised with a negligible scale (1e-12), so
+        that a process absent from the training set yields no tendency."""
+        in_mean, in_std = data.X.mean(0), data.X.std(0)
+        in_rstd = torch.where(in_std > 0, 1.0 / torch.where(in_std > 0, in_std, 1.0), 0.0)
+        out_mean, out_std = data.Y.mean(0), data.Y.std(0)
+        out_std = torch.where(out_std > 0, out_std, 1e-12)
+        return cls(MLP(seed, device=data.X.device), in_mean, in_rstd, out_mean, out_std)
 
     def standardise(self, data):
         X = torch.zeros(len(data), KIN, dtype=torch.bfloat16, device=data.X.device)
@@ -465,3 +463,87 @@ def surrogate_ops(sur, ops=None, chunk=16384, qtol=None, wtol=None):
     override the fallback tolerances)."""
     import pywrf_amd
     return _SurrogateOps(ops or pywrf_amd.ops(), sur, chunk, qtol, wtol)
+
+
+# ----------------------------------------------------------------- evaluation
+def compare_runs(sur, config="config2", nsteps=60, chunk=16384, log=None):
+    """The surrogate step against the plain step on the same seeded state:
+    steps/s of each (HIP graphs of the step), the RMS difference of qv and T
+    (theta) after nsteps relative to the RMS of the plain run's field, and the
+    mean fraction of columns that fell back to WSM6."""
+    import pywrf_amd
+    from .gen import synthetic_state
+    from .graph import StepGraph
+    from .state import clone_state
+    from .step import RunConfig
+    cfg = RunConfig()
+    base = synthetic_state(config)
+    out = {}
+    finals = {}
+    for name in ("wsm6", "surrogate"):
+        s = clone_state(base, device="cuda")
+        ops = pywrf_amd.ops() if name == "wsm6" else surrogate_ops(sur, chunk=chunk)
+        per = 10 if nsteps % 10 == 0 else 1
+        g = StepGraph(s, cfg, ops, steps=per)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fb = []
+        for _ in range(nsteps // per):
+            g.replay()
+            if name == "surrogate":
+                fb.append(ops.nfallback.clone())
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out[name + "_steps_per_s"] = nsteps / dt
+        if name == "surrogate":
+            gr = s.grid
+            ncol = (gr.nx - 13) * (gr.ny - 13)
+            out["fallback_fraction"] = float(torch.cat(fb).float().mean()) / ncol
+        finals[name] = (s.moist[1].double().cpu(), s.t.double().cpu())
+        if log:
+            log("compare: %s %.1f steps/s" % (name, nsteps / dt))
+    rms = lambda x: float(torch.sqrt(torch.mean(x * x)))
+    (qv0, t0_), (qv1, t1) = finals["wsm6"], finals["surrogate"]
+    out["qv_rms_rel"] = rms(qv1 - qv0) / rms(qv0)
+    out["theta_rms_rel"] = rms(t1 - t0_) / rms(t0_ + 300.0)
+    out["theta_rms_K"] = rms(t1 - t0_)
+    out["speedup"] = out["surrogate_steps_per_s"] / out["wsm6_steps_per_s"]
+    out["steps"] = nsteps
+    return out
+
+
+def main(argv=None):
+    import argparse
+    import json
+    import sys
+    ap = argparse.ArgumentParser(description="train the WSM6 column surrogate on the GPU")
+    ap.add_argument("--runs", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=140)
+    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--batch", type=int, default=8192)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--compare", type=int, default=60, help="steps of the 240x240x41 comparison (0: none)")
+    a = ap.parse_args(argv)
+    log = lambda *m: print(*m, file=sys.stderr, flush=True)
+    data = collect_pairs(a.runs, a.steps, log=log)
+    nval = max(a.batch, len(data) // 50)
+    perm = torch.randperm(len(data), device=data.X.device, generator=torch.Generator(device="cuda").manual_seed(1))
+    valid = Pairs(data.X[perm[:nval]], data.Y[perm[:nval]])
+    train = Pairs(data.X[perm[nval:]], data.Y[perm[nval:]])
+    del data
+    untrained = Surrogate.untrained(train)
+    untrained.loss_weight = loss_weights()
+    sur = Surrogate.fit(train, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log, valid=valid)
+    sur.save(a.out)
+    rec = {"pairs": len(train) + len(valid), "train_pairs": len(train), "valid_pairs": len(valid),
+           "epochs": a.epochs, "batch": a.batch, "train_seconds": round(sur.train_seconds, 2),
+           "train_steps": sur.train_steps, "valid_loss": sur.history[-1]["valid_loss"],
+           "valid_loss_untrained": untrained.loss(valid, a.batch), "history": sur.history}
+    if a.compare:
+        rec["compare_240"] = compare_runs(sur, nsteps=a.compare, log=log)
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

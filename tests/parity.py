@@ -11,6 +11,16 @@ from dual import rel_err  # noqa: F401  (re-exported)
 
 PROG = ("u", "v", "w", "t", "ph", "p", "al", "mu")
 
+# Whole-step parity bound per field, max|x - y| / max|y| against the
+# reference's FP32 trajectory.  1e-4 for the full fields; v, w, p and al are
+# perturbation fields whose rounding noise is set by the ulp of the large
+# quantities they are differences of (the full wind, the full pressure and
+# specific volume): the reference's own FP32 run differs from its FP64 run by
+# up to 5.3e-4 (v), 2.2e-4 (w), 1.3e-4 (p) and 1.7e-4 (al) on these workloads,
+# so these four are held to 3e-4.
+STEP_BOUND = dict(u=1e-4, v=3e-4, w=3e-4, t=1e-4, ph=1e-4, p=3e-4, al=3e-4, mu=1e-4, moist=1e-4, rainnc=1e-4,
+                  snownc=1e-4, graupelnc=1e-4)
+
 
 def state(config, device="cpu", dtype=torch.float32):
     from pywrf_amd.gen import synthetic_state

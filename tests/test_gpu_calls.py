@@ -7,14 +7,13 @@ normwise (max|hip - oracle| / max|oracle|).  The oracle's results then carry
 the step on, so each call is judged on identical inputs.
 
 Bounds:
-* every output of every call: normwise error <= 1e-4 (the differences come
-  only from transcendental functions -- exp, log, pow, sin -- and a few
-  reordered column sums);
+* every function not listed in INEXACT: bit-exact, every output of every
+  call (error exactly 0) -- the kernels keep the reference's operation order,
+  rounding and summation order;
+* the functions in INEXACT (transcendentals from the device library, or a
+  sum the kernel forms in another order): normwise error <= 1e-4;
 * the lateral-boundary functions are also judged on the points they change
-  (the strips), against the strip's own values: <= 1e-5;
-* functions that only copy, select or zero values (open-boundary fills,
-  specified-zone tendencies, zero-gradient and flow-dependent edges) and the
-  single-operation ones (full mass, alt, mass weighting): error exactly 0.
+  (the strips), against the strip's own values: bit-exact.
 
 Workloads: 40 x 40 x 41 (config1, config1c: one step), 24 x 24 x 41 with
 time-varying lateral boundaries (bdy24: three steps, so the boundary
@@ -29,11 +28,19 @@ from parity import state, run
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-4
-TOL_STRIP = 1e-5
 BOUNDARY = ("relax_bdytend", "relax_bdy_scalar", "spec_bdytend", "spec_bdy_scalar", "spec_bdy_final",
             "spec_bdyupdate", "spec_bdyupdate_ph", "zero_grad_bdy", "flow_dep_bdy")
-EXACT = ("set_physical_bc3d", "set_physical_bc2d", "spec_bdytend", "spec_bdy_scalar", "zero_grad_bdy",
-         "flow_dep_bdy", "calculate_full", "calc_alt", "mass_weight")
+# the functions whose results may differ from the oracle's in the last bits, and why
+INEXACT = {
+    "advance_w": "sinf of the damping profile (device library)",
+    "calc_p_rho_phi": "powf of the equation of state (device library)",
+    "calculate_km_kh": "expf of the stability limiter (device library)",
+    "microphysics_driver": "expf / logf / fractional powers of WSM6 (device library)",
+    "moist_physics_prep_em": "powf of the Exner function (device library)",
+    "phy_prep": "expf / logf of the hydrostatic pressure (device library)",
+    "calc_cq": "moisture sum over the species in another order",
+    "rhs_ph": "the vertical advection terms summed in another order",
+}
 CALLS_PER_STEP = 401
 
 
@@ -69,12 +76,12 @@ def _check(config, records, nsteps):
         print("  %-32s %.2e (call %d, %s)" % ((name,) + worst[name]))
     bad = {n: w for n, w in worst.items() if not w[0] <= TOL}
     assert not bad, bad
-    inexact = {n: w for n, w in worst.items() if n in EXACT and w[0] != 0.0}
+    inexact = {n: w for n, w in worst.items() if n not in INEXACT and w[0] != 0.0}
     assert not inexact, inexact
     strips = _worst(records, local=True)
     for name in sorted(strips):
         print("  strip %-26s %.2e (call %d, %s)" % ((name,) + strips[name]))
-    bad = {n: w for n, w in strips.items() if not w[0] <= TOL_STRIP}
+    bad = {n: w for n, w in strips.items() if w[0] != 0.0}
     assert not bad, bad
     assert len(records) == CALLS_PER_STEP * nsteps
 

@@ -2,28 +2,23 @@
 reference's FP32 trajectory (golden fixtures written by the reference under
 tests/golden/make_golden.py), judged against an FP64 yardstick.
 
-Two faithful FP32 implementations differ by rounding (transcendentals, the
-ulp of full pressure inside the perturbation pressure p); fields of small
-magnitude (v, w at the first steps) amplify that into normwise differences
-above 1e-4 even between the reference and its CPU restatement.  So each
-field must satisfy
-
-    err(hip32, ref32) <= max(1e-4, 3 * err(ref32, oracle64))
-
-where oracle64 is the oracle run in FP64 on the same seeded state -- the
-reference's own FP32 distance to FP64 sets the scale.  Mass-like fields
-(t, ph, mu, moist) are held to 1e-4 regardless.
+Metric: max|hip - ref32| / max|ref32| per field.  Bound: fixed per field
+(STEP_BOUND in parity.py): 1e-4 for the full fields (u, t, ph, mu, moisture,
+precipitation), 3e-4 for the perturbation fields v, w, p, al, whose FP32
+rounding noise is set by the large quantities they are differences of (the
+reference's own FP32 run is 1.3e-4 to 5.3e-4 from its FP64 run on them).
+The distances of both FP32 runs to the FP64 run of the oracle are printed
+beside it as a diagnostic.
 """
 import os
 
 import pytest
 import torch
 
-from parity import state, run, fields, golden_fields, errors
+from parity import state, run, fields, golden_fields, errors, STEP_BOUND
 
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-STRICT = ("t", "ph", "mu", "moist")
 
 
 def _hip(config, nsteps):
@@ -42,7 +37,8 @@ def _oracle64(config, nsteps):
     return fields(s)
 
 
-@pytest.mark.parametrize("config,nsteps", [("config1", 1), ("config1c", 1), ("config1", 10)])
+@pytest.mark.parametrize("config,nsteps", [("config1", 1), ("config1c", 1), ("config1", 10), ("bdy24", 1),
+                                            ("bdy24", 3)])
 def test_step_matches_reference(config, nsteps):
     ref32 = golden_fields(os.path.join(GOLD, "state_%s_s%d.npz" % (config, nsteps)))
     hip = _hip(config, nsteps)
@@ -54,5 +50,4 @@ def test_step_matches_reference(config, nsteps):
     for n in ref32:
         print("  %-10s hip-ref32 %.2e   ref32-fp64 %.2e   hip-fp64 %.2e" % (n, e_hip[n], e_ref[n], e_hip64[n]))
     for n in ref32:
-        tol = 1e-4 if n in STRICT else max(1e-4, 3 * e_ref[n])
-        assert e_hip[n] <= tol, (n, e_hip[n], tol)
+        assert e_hip[n] <= STEP_BOUND[n], (n, e_hip[n], STEP_BOUND[n])
