@@ -57,43 +57,48 @@ struct Fields {
   float *rain, *rainncv, *snowncv, *sr, *graupelncv;
 };
 
-__device__ inline float field_in(const Fields& f, int var, long q) {
-  switch (var) {
-    case 0: return f.th[q] * f.pii[q];
-    case 1: return f.p[q];
-    case 2: return f.rho[q];
-    case 3: return f.qv[q];
-    case 4: return f.qc[q];
-    case 5: return f.qr[q];
-    case 6: return f.qi[q];
-    case 7: return f.qs[q];
-    default: return f.qg[q];
-  }
-}
-
 // X[c - c_base][feature] for the columns c_base + [0, m) of the chunk: one
-// workgroup per 64 columns; features are read column-contiguous (rows of
-// the field) into LDS, then written as whole rows of X.
+// workgroup per 64 columns, lane = column (rows of the fields are read
+// contiguously), the four waves taking every fourth level; all ten fields of
+// a level (th, pii and the eight others) are loaded before any is used.  The
+// standardised values go to LDS as [feature][column] and leave as 16-byte
+// pieces of whole rows of X.
 __global__ void __launch_bounds__(256) k_sg_pack(Tile T, Fields f, const float* mean, const float* rstd, int c_base,
                                                  int m, bf16* X) {
-  __shared__ bf16 lds[SG_KIN][SG_COLS + 2];
+  __shared__ __attribute__((aligned(16))) bf16 lds[SG_KIN][SG_COLS + 2];
   const int c0 = (int)blockIdx.x * SG_COLS;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   const int cl = c0 + tx, c = c_base + cl;
   const bool live = cl < m;
   const int j = T.j0 + (live ? c / T.ni : 0), i = T.i0 + (live ? c % T.ni : 0);
-  for (int feat = ty; feat < SG_KIN; feat += 4) {
-    const int var = feat / SG_LEV, k = feat % SG_LEV;
-    float v = 0.0f;
-    if (live && var < SG_NIN && k < T.km) v = (field_in(f, var, T.D.ix(k, j, i)) - mean[feat]) * rstd[feat];
-    lds[feat][tx] = (bf16)v;
+  for (int k = ty; k < SG_LEV; k += 4) {
+    const bool on = live && k < T.km;
+    const long q = T.D.ix(on ? k : 0, j, i);
+    float v[SG_NIN];
+    if (on) {
+      const float th = f.th[q], pii = f.pii[q];
+      v[1] = f.p[q]; v[2] = f.rho[q]; v[3] = f.qv[q]; v[4] = f.qc[q]; v[5] = f.qr[q];
+      v[6] = f.qi[q]; v[7] = f.qs[q]; v[8] = f.qg[q];
+      v[0] = th * pii;
+    }
+#pragma unroll
+    for (int var = 0; var < SG_NIN; ++var) {
+      const int feat = var * SG_LEV + k;
+      lds[feat][tx] = (bf16)(on ? (v[var] - mean[feat]) * rstd[feat] : 0.0f);
+    }
   }
+  for (int feat = SG_NIN * SG_LEV + ty; feat < SG_KIN; feat += 4) lds[feat][tx] = (bf16)0.0f;
   __syncthreads();
-  // 64 rows of 384 bf16: 4 threads per row, 96 features each
-  const int row = threadIdx.x >> 2, seg = (threadIdx.x & 3) * 96;
-  if (c0 + row >= m) return;
-  bf16* dst = X + (long)(c0 + row) * SG_KIN + seg;
-  for (int e = 0; e < 96; ++e) dst[e] = lds[seg + e][row];
+  // 64 rows of 48 pieces of 8 features; consecutive threads write consecutive pieces
+  for (int id = threadIdx.x; id < SG_COLS * (SG_KIN / 8); id += 256) {
+    const int row = id / (SG_KIN / 8), f8 = (id % (SG_KIN / 8)) * 8;
+    if (c0 + row >= m) continue;
+    typedef __bf16 bf16x8v __attribute__((ext_vector_type(8)));
+    bf16x8v piece;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) piece[e] = lds[f8 + e][row];
+    *(bf16x8v*)(X + (long)(c0 + row) * SG_KIN + f8) = piece;
+  }
 }
 
 __device__ inline float wave_sum(float v) {

@@ -278,11 +278,16 @@ def collect_pairs(runs=50, steps=140, seed0=100, log=None):
         rec = _Recorder(pywrf_amd.ops())
         for _ in range(steps):
             rk3_step(s, rec, cfg)
-        Xs.append(torch.cat(rec.inputs))
-        Ys.append(torch.cat(rec.tends))
+        X, Y = torch.cat(rec.inputs), torch.cat(rec.tends)
+        ok = torch.isfinite(X).all(1) & torch.isfinite(Y).all(1)
+        dropped = int((~ok).sum())
+        if dropped:
+            X, Y = X[ok], Y[ok]
+        Xs.append(X)
+        Ys.append(Y)
         if log:
-            log("collect: run %d/%d, %d pairs, %.1f s" % (r + 1, runs, sum(x.shape[0] for x in Xs),
-                                                        time.perf_counter() - t0))
+            log("collect: run %d/%d, %d pairs (%d non-finite dropped), %.1f s"
+                % (r + 1, runs, sum(x.shape[0] for x in Xs), dropped, time.perf_counter() - t0))
     return Pairs(torch.cat(Xs), torch.cat(Ys))
 
 
@@ -306,17 +311,23 @@ class Surrogate:
         self.out_std[:FOUT] = out_std
         self.history = []
 
+    # smallest standard deviations used to standardise, per field: a feature
+    # that (nearly) never varies in the training set must not be scaled up
+    IN_FLOOR = dict(T=1e-2, p=1.0, rho=1e-4, qv=1e-6, qc=1e-7, qr=1e-7, qi=1e-7, qs=1e-7, qg=1e-7)
+    OUT_FLOOR = dict(T=1e-6, qv=1e-10, qc=1e-10, qr=1e-10, qi=1e-10, qs=1e-10, qg=1e-10)
+
     @classmethod
     def untrained(cls, data, seed=0):
         """The network at its seeded initialisation, with the standardisation
-        of `data`: features that never vary are ignored (rstd 0), outputs that
-        never vary are de-standardised with a negligible scale (1e-12), so
-        that a process absent from the training set yields no tendency."""
-        in_mean, in_std = data.X.mean(0), data.X.std(0)
-        in_rstd = torch.where(in_std > 0, 1.0 / torch.where(in_std > 0, in_std, 1.0), 0.0)
-        out_mean, out_std = data.Y.mean(0), data.Y.std(0)
-        out_std = torch.where(out_std > 0, out_std, 1e-12)
-        return cls(MLP(seed, device=data.X.device), in_mean, in_rstd, out_mean, out_std)
+        of `data` (per-feature mean and standard deviation, the deviation
+        floored per field by IN_FLOOR / OUT_FLOOR)."""
+        dev = data.X.device
+        fin = torch.tensor([cls.IN_FLOOR[v] for v in IN_VARS for _ in range(NLEV)], device=dev)
+        fout = torch.tensor([cls.OUT_FLOOR[v] for v in OUT_VARS for _ in range(NLEV)], device=dev)
+        X, Y = data.X.double(), data.Y.double()
+        in_mean, in_std = X.mean(0).float(), torch.maximum(X.std(0).float(), fin)
+        out_mean, out_std = Y.mean(0).float(), torch.maximum(Y.std(0).float(), fout)
+        return cls(MLP(seed, device=dev), in_mean, 1.0 / in_std, out_mean, out_std)
 
     def standardise(self, data):
         X = torch.zeros(len(data), KIN, dtype=torch.bfloat16, device=data.X.device)
