@@ -514,7 +514,8 @@ int pywrf_wsm6_columns(const pywrf_grid_t* g, float* th, float* q, float* qc, fl
  * of 8, 16-byte aligned) and FP32 accumulation.  epi: 0 FP32 C (+ bias[N] if
  * given), 1 BF16 relu(C + bias), 2 BF16 C * [H > 0] (H BF16 [M x ldh]),
  * 3 FP32 atomic accumulation into C (K split over ksplit workgroups; C must
- * start at zero).  No counterpart in the reference (the surrogate is new). */
+ * start at zero), 4 FP32 C^T (+ bias) into C [N x ldc], ldc >= M.  No
+ * counterpart in the reference (the surrogate is new). */
 int pywrf_mlp_gemm(int epi, int M, int N, int K, const void* A, int lda, const void* B, int ldb, void* C, int ldc,
                    const float* bias, const void* H, int ldh, int ksplit, pywrf_stream_t s);
 /* out[c][r] = in[r][c], BF16. */

@@ -9,16 +9,19 @@
 // transposed by k_transpose), so every operand is read along K.
 //
 // Tiling: a 256-thread workgroup (4 waves, 2 x 2) owns a 128 x 128 tile of C;
-// each wave owns 64 x 64 = 2 x 2 MFMA tiles of 32 x 32.  K advances 32 at a
-// time through double-buffered LDS tiles of A and B (rows padded to 40 bf16,
-// 80 bytes, so the 16-byte fragment reads of neighbouring rows fall in
-// different banks); the next K-slab is loaded from global into registers
-// while the current one feeds the MFMAs, and one barrier per slab separates
-// the LDS writes of one buffer from the reads of the other.  Rows of A and B
-// beyond M and N load as zero; stores are predicated.  Blocks are numbered so
-// that consecutive M-tiles of one N-tile go to one XCD (they share B).
+// each wave owns 64 x 64 = 2 x 2 MFMA tiles of 32 x 32.  K advances 128 at a
+// time: the slab's A and B tiles (128 x 128 bf16 each, rows padded to 136
+// bf16 so that the 16-byte fragment reads of neighbouring rows fall in
+// different banks) are loaded from global into registers one slab ahead, so
+// the loads of slab t + 1 are in flight while the 32 MFMAs per wave of slab t
+// run from LDS; a slab this deep keeps the matrix cores busy across the
+// global-load latency with two workgroups per CU.  Columns of A and B at or
+// beyond K, and rows beyond M and N, load as zero; stores are predicated.
+// Blocks are numbered so that consecutive M-tiles of one N-tile go to one
+// XCD (they share B).
 //
-// Epilogues: FP32 C (+ bias), BF16 relu(C + bias), BF16 C * [H > 0] (the
+// Epilogues: FP32 C (+ bias), FP32 C^T (+ bias; the surrogate's output, read
+// per feature across columns), BF16 relu(C + bias), BF16 C * [H > 0] (the
 // ReLU derivative, H the layer's forward output) and FP32 atomic accumulation
 // for split-K weight gradients (grid.z splits K; C must start at zero).
 #include "common.h"
@@ -29,10 +32,12 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 #define MLP_BM 128
 #define MLP_BN 128
-#define MLP_BK 32
-#define MLP_LDS_ROW 40  // bf16 per LDS row (32 + 8 padding)
+#define MLP_BK 128
+#define MLP_LDS_ROW (MLP_BK + 8)  // bf16 per LDS row (8 padding)
+#define MLP_KQ 32                 // K must be a multiple of this
+#define MLP_CH (MLP_BM * MLP_BK / 8 / 256)  // 16-byte pieces of A (and of B) per thread per slab
 
-enum { EPI_F32 = 0, EPI_BF16_RELU = 1, EPI_BF16_DRELU = 2, EPI_F32_ATOMIC = 3 };
+enum { EPI_F32 = 0, EPI_BF16_RELU = 1, EPI_BF16_DRELU = 2, EPI_F32_ATOMIC = 3, EPI_F32_T = 4 };
 
 struct GemmArgs {
   int M, N, K, lda, ldb, ldc, ldh, kchunk;
@@ -43,16 +48,17 @@ struct GemmArgs {
   const bf16* H;
 };
 
-// 16 bytes of a row-major bf16 matrix, zero outside [0, rows)
-__device__ inline uint4 load16(const bf16* base, int ld, int row, int rows, int col) {
-  if (row >= rows) return make_uint4(0u, 0u, 0u, 0u);
+// 16 bytes of a row-major bf16 matrix, zero outside rows [0, rows) and
+// columns [0, cols)
+__device__ inline uint4 load16(const bf16* base, int ld, int row, int rows, int col, int cols) {
+  if (row >= rows || col >= cols) return make_uint4(0u, 0u, 0u, 0u);
   return *(const uint4*)(base + (long)row * ld + col);
 }
 
 template <int EPI>
 __global__ void __launch_bounds__(256) k_mlp_gemm(GemmArgs g) {
-  __shared__ __attribute__((aligned(16))) bf16 As[2][MLP_BM * MLP_LDS_ROW];
-  __shared__ __attribute__((aligned(16))) bf16 Bs[2][MLP_BN * MLP_LDS_ROW];
+  __shared__ __attribute__((aligned(16))) bf16 As[MLP_BM * MLP_LDS_ROW];
+  __shared__ __attribute__((aligned(16))) bf16 Bs[MLP_BN * MLP_LDS_ROW];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   // tile order: the M-tiles of one N-tile in a contiguous run per XCD
@@ -61,25 +67,24 @@ __global__ void __launch_bounds__(256) k_mlp_gemm(GemmArgs g) {
   const int m0 = bx * MLP_BM, n0 = by * MLP_BN;
   const int kbeg = (int)blockIdx.z * g.kchunk;
   const int kend = imin(g.K, kbeg + g.kchunk);
-  const int nk = (kend - kbeg) / MLP_BK;
+  const int nk = kend > kbeg ? (kend - kbeg + MLP_BK - 1) / MLP_BK : 0;
 
-  // each thread moves two 16-byte chunks of A and of B per K-slab: chunk c =
-  // tid + 256 r covers row c / 4, columns 8 (c % 4) .. + 8
-  uint4 ra[2], rb[2];
+  // piece p = tid + 256 r of a slab covers row p / 16, columns 8 (p % 16) .. + 8
+  uint4 ra[MLP_CH], rb[MLP_CH];
   auto gload = [&](int k0) {
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      const int c = tid + 256 * r, row = c >> 2, col = k0 + (c & 3) * 8;
-      ra[r] = load16(g.A, g.lda, m0 + row, g.M, col);
-      rb[r] = load16(g.B, g.ldb, n0 + row, g.N, col);
+    for (int r = 0; r < MLP_CH; ++r) {
+      const int p = tid + 256 * r, row = p >> 4, col = k0 + (p & 15) * 8;
+      ra[r] = load16(g.A, g.lda, m0 + row, g.M, col, kend);
+      rb[r] = load16(g.B, g.ldb, n0 + row, g.N, col, kend);
     }
   };
-  auto lstore = [&](int buf) {
+  auto lstore = [&]() {
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      const int c = tid + 256 * r, row = c >> 2, col = (c & 3) * 8;
-      *(uint4*)&As[buf][row * MLP_LDS_ROW + col] = ra[r];
-      *(uint4*)&Bs[buf][row * MLP_LDS_ROW + col] = rb[r];
+    for (int r = 0; r < MLP_CH; ++r) {
+      const int p = tid + 256 * r, row = p >> 4, col = (p & 15) * 8;
+      *(uint4*)&As[row * MLP_LDS_ROW + col] = ra[r];
+      *(uint4*)&Bs[row * MLP_LDS_ROW + col] = rb[r];
     }
   };
 
@@ -91,30 +96,26 @@ __global__ void __launch_bounds__(256) k_mlp_gemm(GemmArgs g) {
 #pragma unroll
       for (int e = 0; e < 16; ++e) acc[a][b][e] = 0.0f;
 
-  if (nk > 0) {
-    gload(kbeg);
-    lstore(0);
-  }
-  __syncthreads();
+  if (nk > 0) gload(kbeg);
   const int lr = lane & 31, lk = (lane >> 5) * 8;
   for (int kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
+    __syncthreads();                 // every wave is done reading the previous slab
+    lstore();
+    __syncthreads();
     if (kt + 1 < nk) gload(kbeg + (kt + 1) * MLP_BK);
 #pragma unroll
     for (int ks = 0; ks < MLP_BK; ks += 16) {
       bf16x8 af[2], bfr[2];
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
-        af[t] = *(const bf16x8*)&As[cur][(wm * 64 + t * 32 + lr) * MLP_LDS_ROW + ks + lk];
-        bfr[t] = *(const bf16x8*)&Bs[cur][(wn * 64 + t * 32 + lr) * MLP_LDS_ROW + ks + lk];
+        af[t] = *(const bf16x8*)&As[(wm * 64 + t * 32 + lr) * MLP_LDS_ROW + ks + lk];
+        bfr[t] = *(const bf16x8*)&Bs[(wn * 64 + t * 32 + lr) * MLP_LDS_ROW + ks + lk];
       }
 #pragma unroll
       for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
     }
-    if (kt + 1 < nk) lstore(cur ^ 1);
-    __syncthreads();
   }
 
   // accumulator element e of lane l holds C[8 (e / 4) + 4 (l / 32) + e % 4][l % 32] of the 32 x 32 tile
@@ -124,7 +125,7 @@ __global__ void __launch_bounds__(256) k_mlp_gemm(GemmArgs g) {
     for (int b = 0; b < 2; ++b) {
       const int n = n0 + wn * 64 + b * 32 + lr;
       if (n >= g.N) continue;
-      const float bv = (EPI == EPI_F32 || EPI == EPI_BF16_RELU) && g.bias ? g.bias[n] : 0.0f;
+      const float bv = (EPI == EPI_F32 || EPI == EPI_BF16_RELU || EPI == EPI_F32_T) && g.bias ? g.bias[n] : 0.0f;
 #pragma unroll
       for (int e = 0; e < 16; ++e) {
         const int m = m0 + wm * 64 + a * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
@@ -133,6 +134,8 @@ __global__ void __launch_bounds__(256) k_mlp_gemm(GemmArgs g) {
         const long q = (long)m * g.ldc + n;
         if (EPI == EPI_F32) {
           ((float*)g.C)[q] = v + bv;
+        } else if (EPI == EPI_F32_T) {
+          ((float*)g.C)[(long)n * g.ldc + m] = v + bv;
         } else if (EPI == EPI_BF16_RELU) {
           const float r = v + bv;
           ((bf16*)g.C)[q] = (bf16)(r > 0.0f ? r : 0.0f);
@@ -148,16 +151,19 @@ __global__ void __launch_bounds__(256) k_mlp_gemm(GemmArgs g) {
 
 extern "C" int pywrf_mlp_gemm(int epi, int M, int N, int K, const void* A, int lda, const void* B, int ldb, void* C,
                               int ldc, const float* bias, const void* H, int ldh, int ksplit, pywrf_stream_t s) {
-  if (M < 0 || N < 0 || K < 0 || K % MLP_BK || lda < K || ldb < K || ldc < N || !A || !B || !C) return PYWRF_E_RANGE;
+  if (M < 0 || N < 0 || K < 0 || K % MLP_KQ || lda < K || ldb < K || !A || !B || !C) return PYWRF_E_RANGE;
+  if (epi == EPI_F32_T ? ldc < M : ldc < N) return PYWRF_E_RANGE;
   if (lda % 8 || ldb % 8 || ((uintptr_t)A & 15) || ((uintptr_t)B & 15)) return PYWRF_E_RANGE;  // 16-byte rows
   if (epi == EPI_BF16_DRELU && (!H || ldh < N)) return PYWRF_E_RANGE;
-  if (epi < 0 || epi > 3 || ksplit < 1 || (ksplit > 1 && epi != EPI_F32_ATOMIC)) return PYWRF_E_ARG;
-  if ((long)M * ldc >= (1L << 31) || (long)M * lda >= (1L << 31) || (long)N * ldb >= (1L << 31)) return PYWRF_E_RANGE;
+  if (epi < 0 || epi > 4 || ksplit < 1 || (ksplit > 1 && epi != EPI_F32_ATOMIC)) return PYWRF_E_ARG;
+  if ((long)(epi == EPI_F32_T ? N : M) * ldc >= (1L << 31) || (long)M * lda >= (1L << 31) ||
+      (long)N * ldb >= (1L << 31))
+    return PYWRF_E_RANGE;
   if (M == 0 || N == 0) return 0;
   GemmArgs g;
   g.M = M; g.N = N; g.K = K; g.lda = lda; g.ldb = ldb; g.ldc = ldc; g.ldh = ldh;
   g.A = (const bf16*)A; g.B = (const bf16*)B; g.C = C; g.bias = bias; g.H = (const bf16*)H;
-  const int slabs = K / MLP_BK;
+  const int slabs = (K + MLP_BK - 1) / MLP_BK;
   const int per = (slabs + ksplit - 1) / ksplit;
   g.kchunk = per * MLP_BK;
   const int splits = K == 0 ? 1 : (slabs + per - 1) / per;
@@ -166,6 +172,7 @@ extern "C" int pywrf_mlp_gemm(int epi, int M, int N, int K, const void* A, int l
     case EPI_F32: hipLaunchKernelGGL(k_mlp_gemm<EPI_F32>, grid, dim3(256), 0, (hipStream_t)s, g); break;
     case EPI_BF16_RELU: hipLaunchKernelGGL(k_mlp_gemm<EPI_BF16_RELU>, grid, dim3(256), 0, (hipStream_t)s, g); break;
     case EPI_BF16_DRELU: hipLaunchKernelGGL(k_mlp_gemm<EPI_BF16_DRELU>, grid, dim3(256), 0, (hipStream_t)s, g); break;
+    case EPI_F32_T: hipLaunchKernelGGL(k_mlp_gemm<EPI_F32_T>, grid, dim3(256), 0, (hipStream_t)s, g); break;
     default: hipLaunchKernelGGL(k_mlp_gemm<EPI_F32_ATOMIC>, grid, dim3(256), 0, (hipStream_t)s, g); break;
   }
   PW_RET();

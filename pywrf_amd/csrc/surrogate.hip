@@ -23,9 +23,9 @@
 // surface precipitation of the step (rainncv, added to rainnc; the surrogate
 // does not split it into snow and graupel, so snowncv = graupelncv = sr = 0).
 //
-// Columns are processed in chunks through a workspace: packed inputs X
-// (bf16 [chunk x 384]), hidden layers (bf16 [chunk x 512] x 2), outputs
-// (f32 [chunk x 288]), then the fallback list (int [ncol]) and its counter.
+// Columns are processed in chunks through a workspace: the fallback list
+// (int [ncol]), then packed inputs X (bf16 [chunk x 384]), hidden layers
+// (bf16 [chunk x 512] x 2) and outputs (f32, feature-major [288 x chunk]).
 #include "common.h"
 
 typedef __bf16 bf16;
@@ -101,73 +101,70 @@ __global__ void __launch_bounds__(256) k_sg_pack(Tile T, Fields f, const float* 
   }
 }
 
-__device__ inline float wave_sum(float v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
-}
-
-// One wave per column of the chunk, lane = level: apply the predicted
-// tendencies if the column passes the checks, else append it to the
-// fallback list.
-__global__ void __launch_bounds__(64) k_sg_apply(Tile T, Fields f, const float* Y, const float* omean,
-                                                 const float* ostd, float dt, float qtol, float wtol, int c_base, int m,
-                                                 int* fall, int* nfall) {
-  const int cl = (int)blockIdx.x;
+// One thread per column of the chunk (consecutive threads take consecutive
+// columns, so every load and store of a level is a row segment of the
+// field), in two passes over the levels: the first forms the updated values
+// only to run the checks and the column water sums, the second -- for a
+// column that passes -- forms them again and stores them.  A failing column
+// is appended to the fallback list.  Yt holds the network's outputs feature-
+// major, [288][ldy].
+__global__ void __launch_bounds__(256) k_sg_apply(Tile T, Fields f, const float* Yt, int ldy, const float* omean,
+                                                  const float* ostd, float dt, float qtol, float wtol, int c_base,
+                                                  int m, int* fall, int* nfall) {
+  const int cl = (int)(blockIdx.x * 256 + threadIdx.x);
   if (cl >= m) return;
-  const int c = c_base + cl, k = threadIdx.x;
+  const int c = c_base + cl;
   const int j = T.j0 + c / T.ni, i = T.i0 + c % T.ni;
-  const bool act = k < T.km;
-  const long q = T.D.ix(act ? k : 0, j, i);
-  const float* y = Y + (long)cl * SG_NO;
-  float old[SG_NOUT], nw[SG_NOUT];
-  float pii = act ? f.pii[q] : 1.0f;
-  old[0] = act ? f.th[q] * pii : 0.0f;
-  old[1] = act ? f.qv[q] : 0.0f;
-  old[2] = act ? f.qc[q] : 0.0f;
-  old[3] = act ? f.qr[q] : 0.0f;
-  old[4] = act ? f.qi[q] : 0.0f;
-  old[5] = act ? f.qs[q] : 0.0f;
-  old[6] = act ? f.qg[q] : 0.0f;
+  auto upd = [&](int v, int k, float old) {
+    const int o = v * SG_LEV + k;
+    return old + dt * (Yt[(long)o * ldy + cl] * ostd[o] + omean[o]);
+  };
   bool bad = false;
+  float w_old = 0.0f, w_new = 0.0f;
+  for (int k = 0; k < T.km; ++k) {
+    const long q = T.D.ix(k, j, i);
+    const float th = f.th[q], pii = f.pii[q], mass = f.rho[q] * f.dz[q];
+    const float old[SG_NOUT - 1] = {f.qv[q], f.qc[q], f.qr[q], f.qi[q], f.qs[q], f.qg[q]};
+    const float tn = upd(0, k, th * pii);
+    bad |= !(tn - tn == 0.0f);                           // NaN or infinite
+    float so = 0.0f, sn = 0.0f;
 #pragma unroll
-  for (int v = 0; v < SG_NOUT; ++v) {
-    const int o = v * SG_LEV + (act ? k : 0);
-    const float tend = act ? y[o] * ostd[o] + omean[o] : 0.0f;
-    float x = old[v] + dt * tend;
-    if (!(x == x) || x - x != 0.0f) bad = true;                 // NaN or infinite
-    if (v > 0) {
-      if (x < -qtol) bad = true;
-      if (x < 0.0f) x = 0.0f;
+    for (int v = 1; v < SG_NOUT; ++v) {
+      float x = upd(v, k, old[v - 1]);
+      bad |= !(x - x == 0.0f) || x < -qtol;
+      so += old[v - 1];
+      sn += x > 0.0f ? x : 0.0f;
     }
-    nw[v] = act ? x : 0.0f;
+    w_old += mass * so;
+    w_new += mass * sn;
   }
-  const float mass = act ? f.rho[q] * f.dz[q] : 0.0f;
-  const float w_old = wave_sum(mass * (old[1] + old[2] + old[3] + old[4] + old[5] + old[6]));
-  const float w_new = wave_sum(mass * (nw[1] + nw[2] + nw[3] + nw[4] + nw[5] + nw[6]));
   if (w_new - w_old > wtol * w_old) bad = true;
-  bad = __any(bad);
   if (bad) {
-    if (k == 0) fall[atomicAdd(nfall, 1)] = c;
+    fall[atomicAdd(nfall, 1)] = c;
     return;
   }
-  if (act) {
-    f.th[q] = nw[0] / pii;
-    f.qv[q] = nw[1];
-    f.qc[q] = nw[2];
-    f.qr[q] = nw[3];
-    f.qi[q] = nw[4];
-    f.qs[q] = nw[5];
-    f.qg[q] = nw[6];
+  for (int k = 0; k < T.km; ++k) {
+    const long q = T.D.ix(k, j, i);
+    const float pii = f.pii[q];
+    float* dst[SG_NOUT - 1] = {f.qv, f.qc, f.qr, f.qi, f.qs, f.qg};
+    float nw[SG_NOUT - 1];
+#pragma unroll
+    for (int v = 1; v < SG_NOUT; ++v) {
+      const float x = upd(v, k, dst[v - 1][q]);
+      nw[v - 1] = x > 0.0f ? x : 0.0f;
+    }
+    const float tn = upd(0, k, f.th[q] * pii);
+    f.th[q] = tn / pii;
+#pragma unroll
+    for (int v = 1; v < SG_NOUT; ++v) dst[v - 1][q] = nw[v - 1];
   }
-  if (k == 0) {
-    const long q2 = T.D.ix2(j, i);
-    const float pr = w_old - w_new > 0.0f ? w_old - w_new : 0.0f;
-    f.rainncv[q2] = pr;
-    f.rain[q2] = f.rain[q2] + pr;
-    f.snowncv[q2] = 0.0f;
-    f.graupelncv[q2] = 0.0f;
-    f.sr[q2] = 0.0f;
-  }
+  const long q2 = T.D.ix2(j, i);
+  const float pr = w_old - w_new > 0.0f ? w_old - w_new : 0.0f;
+  f.rainncv[q2] = pr;
+  f.rain[q2] = f.rain[q2] + pr;
+  f.snowncv[q2] = 0.0f;
+  f.graupelncv[q2] = 0.0f;
+  f.sr[q2] = 0.0f;
 }
 
 __global__ void k_sg_zero_int(int* p) { *p = 0; }
@@ -214,10 +211,12 @@ static int sg_run(const pywrf_grid_t* g, const pywrf_surrogate_t* net, float* th
                        net->in_mean, net->in_rstd, (int)c0, m, X);
     int e = pywrf_mlp_gemm(1, m, SG_H, SG_KIN, X, SG_KIN, net->w1, SG_KIN, H1, SG_H, net->b1, nullptr, 0, 1, s);
     if (!e) e = pywrf_mlp_gemm(1, m, SG_H, SG_H, H1, SG_H, net->w2, SG_H, H2, SG_H, net->b2, nullptr, 0, 1, s);
-    if (!e) e = pywrf_mlp_gemm(0, m, SG_NO, SG_H, H2, SG_H, net->w3, SG_H, Y, SG_NO, net->b3, nullptr, 0, 1, s);
+    // the outputs feature-major (Y^T, [288][chunk]) for the column-per-thread update
+    if (!e) e = pywrf_mlp_gemm(4, m, SG_NO, SG_H, H2, SG_H, net->w3, SG_H, Y, (int)chunk, net->b3, nullptr, 0, 1, s);
     if (e) return e;
-    hipLaunchKernelGGL(k_sg_apply, dim3((unsigned)m), dim3(64), 0, (hipStream_t)s, T, f, Y, net->out_mean,
-                       net->out_std, (float)dt, net->qtol, net->wtol, (int)c0, m, fall, nfall);
+    hipLaunchKernelGGL(k_sg_apply, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, (hipStream_t)s, T, f, Y,
+                       (int)chunk, net->out_mean, net->out_std, (float)dt, net->qtol, net->wtol, (int)c0, m, fall,
+                       nfall);
   }
   // the plain scheme on the columns that failed a check
   return pywrf_wsm6_columns(g, th, qv, qc, qr, qi, qs, qg, rho, pii, p, delz, dt, rain, rainncv, snow, snowncv, sr,

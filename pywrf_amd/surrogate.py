@@ -58,7 +58,8 @@ def gemm(epi, A, B, C, bias=None, H=None, ksplit=1):
     A bf16 [M x K], B bf16 [N x K]; C f32 or bf16 [M x N]."""
     M, K = A.shape
     N = B.shape[0]
-    if B.shape[1] != K or C.shape[0] != M or C.shape[1] < N:
+    rows, cols = (N, M) if epi == 4 else (M, N)          # epilogue 4 writes C^T
+    if B.shape[1] != K or C.shape[0] != rows or C.shape[1] < cols:
         raise IndexError("gemm: shapes %s x %s^T -> %s" % (tuple(A.shape), tuple(B.shape), tuple(C.shape)))
     _check_bf16(A, M, K)
     _check_bf16(B, N, K)
@@ -468,7 +469,7 @@ class _SurrogateOps:
                 graupelnc, graupelncv, sr, refl_10cm)
 
 
-def surrogate_ops(sur, ops=None, chunk=16384, qtol=None, wtol=None):
+def surrogate_ops(sur, ops=None, chunk=65536, qtol=None, wtol=None):
     """The step's functions with microphysics_driver served by the surrogate
     (ops: the namespace to wrap, pywrf_amd.ops() by default; qtol / wtol
     override the fallback tolerances)."""

@@ -25,7 +25,8 @@ def _rel(a, b):
 
 
 @pytest.mark.parametrize("M,N,K,ksplit", [(256, 128, 64, 1), (300, 200, 96, 1), (1000, 288, 512, 1),
-                                          (96, 512, 4096, 8), (51529 % 4096 + 7, 384, 384, 1)])
+                                          (96, 512, 4096, 8), (51529 % 4096 + 7, 384, 384, 1),
+                                          (700, 512, 288, 1), (512, 288, 8192, 16)])
 def test_gemm_matches_cpu(M, N, K, ksplit):
     from pywrf_amd import surrogate as sg
     gen = torch.Generator().manual_seed(M * 7 + N)
@@ -49,6 +50,10 @@ def test_gemm_matches_cpu(M, N, K, ksplit):
         sg.gemm(2, Ad, Bd, Cd, H=H.cuda())
         want = ref * (H.float() > 0)
         assert _rel(Cd.float(), want) < 1e-2
+        Ct = torch.full((N, M + 5), 3.0, device="cuda")
+        sg.gemm(4, Ad, Bd, Ct, biasd)
+        assert _rel(Ct[:, :M].t(), ref + bias) < 1e-5
+        assert bool((Ct[:, M:] == 3.0).all())
     C = torch.zeros(M, N, device="cuda")
     sg.gemm(3, Ad, Bd, C, ksplit=ksplit)
     assert _rel(C, ref) < 1e-5
