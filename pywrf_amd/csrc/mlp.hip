@@ -49,10 +49,16 @@ struct GemmArgs {
 };
 
 // 16 bytes of a row-major bf16 matrix, zero outside rows [0, rows) and
-// columns [0, cols)
+// columns [0, cols).  The load itself is unconditional, from an address
+// clamped into the matrix, and the zero is selected afterwards: a load under
+// a branch that not every lane takes is waited for where the branch closes,
+// which would serialise the slab's sixteen loads.
 __device__ inline uint4 load16(const bf16* base, int ld, int row, int rows, int col, int cols) {
-  if (row >= rows || col >= cols) return make_uint4(0u, 0u, 0u, 0u);
-  return *(const uint4*)(base + (long)row * ld + col);
+  const bool in = row < rows && col < cols;
+  const int r = row < rows ? row : rows - 1, c = col < cols ? col : cols - 8;
+  uint4 v = *(const uint4*)(base + (long)r * ld + c);
+  if (!in) v = make_uint4(0u, 0u, 0u, 0u);
+  return v;
 }
 
 template <int EPI>

@@ -74,13 +74,11 @@ __global__ void __launch_bounds__(256) k_sg_pack(Tile T, Fields f, const float* 
   for (int k = ty; k < SG_LEV; k += 4) {
     const bool on = live && k < T.km;
     const long q = T.D.ix(on ? k : 0, j, i);
-    float v[SG_NIN];
-    if (on) {
-      const float th = f.th[q], pii = f.pii[q];
-      v[1] = f.p[q]; v[2] = f.rho[q]; v[3] = f.qv[q]; v[4] = f.qc[q]; v[5] = f.qr[q];
-      v[6] = f.qi[q]; v[7] = f.qs[q]; v[8] = f.qg[q];
-      v[0] = th * pii;
-    }
+    float v[SG_NIN];                                    // loads at a valid (clamped) point, no branch
+    const float th = f.th[q], pii = f.pii[q];
+    v[1] = f.p[q]; v[2] = f.rho[q]; v[3] = f.qv[q]; v[4] = f.qc[q]; v[5] = f.qr[q];
+    v[6] = f.qi[q]; v[7] = f.qs[q]; v[8] = f.qg[q];
+    v[0] = th * pii;
 #pragma unroll
     for (int var = 0; var < SG_NIN; ++var) {
       const int feat = var * SG_LEV + k;
