@@ -539,7 +539,12 @@ int pywrf_mlp_adam(int rows, int cols, float* w, const float* g, float* m, float
  * standardised tendencies of T, qv, qc, qr, qi, qs, qg (feature var * 41 + k,
  * 287 used of 288), de-standardised with out_mean / out_std.  qtol: the most
  * negative mixing ratio accepted (smaller ones are clipped to 0); wtol: the
- * largest accepted relative gain of the column's water. */
+ * largest accepted relative gain of the column's water.  The training
+ * envelope: in_lo / in_hi (384 each) bound every input feature the column
+ * may show, out_absmax (288) every tendency the network may return, as seen
+ * in the training set; a column outside it is not trusted to the network.
+ * envelope scales the bounds (the input range is widened by (envelope - 1)
+ * times its width on both sides, the tendency bound multiplied by it). */
 typedef struct pywrf_surrogate {
   const void* w1; /* BF16 [512 x 384] */
   const void* w2; /* BF16 [512 x 512] */
@@ -551,7 +556,10 @@ typedef struct pywrf_surrogate {
   const float* in_rstd;
   const float* out_mean;
   const float* out_std;
-  float qtol, wtol;
+  const float* in_lo;
+  const float* in_hi;
+  const float* out_absmax;
+  float qtol, wtol, envelope;
 } pywrf_surrogate_t;
 
 /* Bytes of workspace pywrf_wsm6_surrogate needs for grid g (its physics tile)
@@ -559,8 +567,9 @@ typedef struct pywrf_surrogate {
 int pywrf_wsm6_surrogate_workspace(const pywrf_grid_t* g, int chunk, long long* bytes);
 /* microphysics_driver / wsm6 (wrf_physics.py:94-134, 137-1214) replaced by the
  * surrogate: same fields, tile and outputs as pywrf_wsm6; every column that
- * fails the checks (negative mixing ratio, water gain, non-finite value) is
- * recomputed by the plain scheme, and their count is written to *nfallback
+ * fails the checks (negative mixing ratio, water gain, non-finite value, an
+ * input or a tendency outside the training envelope) is recomputed by the
+ * plain scheme, and their count is written to *nfallback
  * (device int).  workspace: device memory of at least the size above for
  * some chunk >= 128 (larger chunks, fewer launches).  No counterpart in the
  * reference (its README, README.md:170-173, describes but does not ship the

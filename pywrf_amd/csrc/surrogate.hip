@@ -18,7 +18,12 @@
 //   * no water-mass gain: the column's total water sum_k rho dz (qv + qc + qr
 //     + qi + qs + qg) may not rise by more than wtol times its old value
 //     (the scheme only removes water, by precipitation);
-//   * every updated value finite.
+//   * every updated value finite;
+//   * the column inside the training envelope: each input the update reads
+//     (T, rho and the six mixing ratios at every level) within the range the
+//     training set showed for that feature, widened by (envelope - 1) times
+//     its width on each side, and each tendency within envelope times the
+//     largest magnitude the training set showed for it.
 // A passing column's water loss sum_k rho dz (old - new) [kg m-2 = mm] is its
 // surface precipitation of the step (rainncv, added to rainnc; the surrogate
 // does not split it into snow and graupel, so snowncv = graupelncv = sr = 0).
@@ -106,30 +111,46 @@ __global__ void __launch_bounds__(256) k_sg_pack(Tile T, Fields f, const float* 
 // column that passes -- forms them again and stores them.  A failing column
 // is appended to the fallback list.  Yt holds the network's outputs feature-
 // major, [288][ldy].
+struct Envelope {
+  const float *lo, *hi, *amax;
+  float scale;
+};
+
+// input feature var * 41 + k inside the widened training range
+__device__ inline bool in_range(const Envelope& e, int feat, float x) {
+  const float w = (e.hi[feat] - e.lo[feat]) * (e.scale - 1.0f);
+  return x >= e.lo[feat] - w && x <= e.hi[feat] + w;
+}
+
 __global__ void __launch_bounds__(256) k_sg_apply(Tile T, Fields f, const float* Yt, int ldy, const float* omean,
-                                                  const float* ostd, float dt, float qtol, float wtol, int c_base,
-                                                  int m, int* fall, int* nfall) {
+                                                  const float* ostd, Envelope env, float dt, float qtol, float wtol,
+                                                  int c_base, int m, int* fall, int* nfall) {
   const int cl = (int)(blockIdx.x * 256 + threadIdx.x);
   if (cl >= m) return;
   const int c = c_base + cl;
   const int j = T.j0 + c / T.ni, i = T.i0 + c % T.ni;
-  auto upd = [&](int v, int k, float old) {
+  auto tend = [&](int v, int k) {
     const int o = v * SG_LEV + k;
-    return old + dt * (Yt[(long)o * ldy + cl] * ostd[o] + omean[o]);
+    return Yt[(long)o * ldy + cl] * ostd[o] + omean[o];
   };
+  auto upd = [&](int v, int k, float old) { return old + dt * tend(v, k); };
   bool bad = false;
   float w_old = 0.0f, w_new = 0.0f;
   for (int k = 0; k < T.km; ++k) {
     const long q = T.D.ix(k, j, i);
-    const float th = f.th[q], pii = f.pii[q], mass = f.rho[q] * f.dz[q];
+    const float th = f.th[q], pii = f.pii[q], rho = f.rho[q], mass = rho * f.dz[q];
     const float old[SG_NOUT - 1] = {f.qv[q], f.qc[q], f.qr[q], f.qi[q], f.qs[q], f.qg[q]};
-    const float tn = upd(0, k, th * pii);
+    const float t0 = th * pii, tt = tend(0, k);
+    const float tn = t0 + dt * tt;
     bad |= !(tn - tn == 0.0f);                           // NaN or infinite
+    bad |= !in_range(env, k, t0) || !in_range(env, 2 * SG_LEV + k, rho) || !(fabsf(tt) <= env.scale * env.amax[k]);
     float so = 0.0f, sn = 0.0f;
 #pragma unroll
     for (int v = 1; v < SG_NOUT; ++v) {
-      float x = upd(v, k, old[v - 1]);
+      const float tv = tend(v, k);
+      float x = old[v - 1] + dt * tv;
       bad |= !(x - x == 0.0f) || x < -qtol;
+      bad |= !in_range(env, (v + 2) * SG_LEV + k, old[v - 1]) || !(fabsf(tv) <= env.scale * env.amax[v * SG_LEV + k]);
       so += old[v - 1];
       sn += x > 0.0f ? x : 0.0f;
     }
@@ -202,6 +223,9 @@ static int sg_run(const pywrf_grid_t* g, const pywrf_surrogate_t* net, float* th
   f.rho = rho; f.pii = pii; f.p = p; f.dz = delz;
   f.rain = rain; f.rainncv = rainncv; f.snowncv = snowncv; f.sr = sr; f.graupelncv = graupelncv;
   (void)snow; (void)graupel;
+  if (!net->in_lo || !net->in_hi || !net->out_absmax || !(net->envelope >= 1.0f)) return PYWRF_E_ARG;
+  Envelope env;
+  env.lo = net->in_lo; env.hi = net->in_hi; env.amax = net->out_absmax; env.scale = net->envelope;
   hipLaunchKernelGGL(k_sg_zero_int, dim3(1), dim3(1), 0, (hipStream_t)s, nfall);
   for (long c0 = 0; c0 < ncol; c0 += chunk) {
     const int m = (int)(ncol - c0 < chunk ? ncol - c0 : chunk);
@@ -213,8 +237,8 @@ static int sg_run(const pywrf_grid_t* g, const pywrf_surrogate_t* net, float* th
     if (!e) e = pywrf_mlp_gemm(4, m, SG_NO, SG_H, H2, SG_H, net->w3, SG_H, Y, (int)chunk, net->b3, nullptr, 0, 1, s);
     if (e) return e;
     hipLaunchKernelGGL(k_sg_apply, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, (hipStream_t)s, T, f, Y,
-                       (int)chunk, net->out_mean, net->out_std, (float)dt, net->qtol, net->wtol, (int)c0, m, fall,
-                       nfall);
+                       (int)chunk, net->out_mean, net->out_std, env, (float)dt, net->qtol, net->wtol, (int)c0, m,
+                       fall, nfall);
   }
   // the plain scheme on the columns that failed a check
   return pywrf_wsm6_columns(g, th, qv, qc, qr, qi, qs, qg, rho, pii, p, delz, dt, rain, rainncv, snow, snowncv, sr,

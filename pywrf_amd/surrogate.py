@@ -35,12 +35,24 @@ KIN, HID, NO = 384, 512, 288                 # padded widths
 IN_VARS = ("T", "p", "rho", "qv", "qc", "qr", "qi", "qs", "qg")
 OUT_VARS = ("T", "qv", "qc", "qr", "qi", "qs", "qg")
 
-# seeded 40 x 40 x 41 runs of the reference's CPU case (one 500 m hill, 10 m/s
-# flow) with the seed choosing six warm bubbles of 1-3 K, u/v noise and cloud /
-# precipitation blobs of all five condensate species, so that every process of
-# the scheme occurs in the training set
+# seeded 40 x 40 x 41 runs of the reference's CPU case (one 500 m hill) with
+# the seed choosing six warm bubbles of 1-3 K and cloud / precipitation blobs
+# of all five condensate species, so that every process of the scheme occurs
+# in the training set; train_recipe() also draws the flow per seed (uniform
+# wind, an upper-level jet, u/v noise) to cover the flows of the 240 x 240
+# case
 TRAIN_RECIPE = dict(n=40, hills=[(0.5, 0.5, 500.0, 45e3)], jet=None, u0=10.0, noise=0.5, bubbles="random",
                     clouds=6)
+
+
+def train_recipe(seed):
+    """TRAIN_RECIPE with the seed's flow: u0 ~ U(0, 12) m/s, with probability
+    1/2 a jet of U(5, 25) m/s at 10 km, noise ~ U(0.3, 2) m/s."""
+    import numpy as np
+    rng = np.random.default_rng(10_000 + seed)
+    u0, jet_amp, noise = rng.uniform(0, 12), rng.uniform(5, 25), rng.uniform(0.3, 2.0)
+    jet = (float(jet_amp), 10e3, 5e3) if rng.uniform() < 0.5 else None
+    return dict(TRAIN_RECIPE, seed=seed, u0=float(u0), jet=jet, noise=float(noise))
 
 
 def _vp(t):
@@ -275,7 +287,7 @@ def collect_pairs(runs=50, steps=140, seed0=100, log=None):
     Xs, Ys = [], []
     t0 = time.perf_counter()
     for r in range(runs):
-        s = synthetic_state(dict(TRAIN_RECIPE, seed=seed0 + r), device="cuda")
+        s = synthetic_state(train_recipe(seed0 + r), device="cuda")
         rec = _Recorder(pywrf_amd.ops())
         for _ in range(steps):
             rk3_step(s, rec, cfg)
@@ -298,10 +310,16 @@ class Surrogate:
 
     QTOL = 1e-7      # kg/kg: smaller negative mixing ratios fail the column
     WTOL = 1e-5      # relative column water gain that fails the column
+    ENVELOPE = 1.05  # widening of the training envelope (see pywrf_surrogate_t)
 
-    def __init__(self, net, in_mean, in_rstd, out_mean, out_std):
+    def __init__(self, net, in_mean, in_rstd, out_mean, out_std, in_lo=None, in_hi=None, out_absmax=None):
         self.net = net
         dev = net.W[0].device
+        self.in_lo = torch.zeros(KIN, device=dev)
+        self.in_hi = torch.zeros(KIN, device=dev)
+        self.out_absmax = torch.zeros(NO, device=dev)
+        if in_lo is not None:
+            self.in_lo[:FIN], self.in_hi[:FIN], self.out_absmax[:FOUT] = in_lo, in_hi, out_absmax
         self.in_mean = torch.zeros(KIN, device=dev)
         self.in_rstd = torch.zeros(KIN, device=dev)
         self.in_mean[:FIN] = in_mean
@@ -328,7 +346,11 @@ class Surrogate:
         X, Y = data.X.double(), data.Y.double()
         in_mean, in_std = X.mean(0).float(), torch.maximum(X.std(0).float(), fin)
         out_mean, out_std = Y.mean(0).float(), torch.maximum(Y.std(0).float(), fout)
-        return cls(MLP(seed, device=dev), in_mean, 1.0 / in_std, out_mean, out_std)
+        # the envelope: observed ranges, widened by the floors (a feature that never
+        # varies still admits values within its floor of the one it showed)
+        lo, hi = data.X.min(0).values - fin, data.X.max(0).values + fin
+        amax = torch.maximum(data.Y.abs().max(0).values, 10.0 * fout)
+        return cls(MLP(seed, device=dev), in_mean, 1.0 / in_std, out_mean, out_std, lo, hi, amax)
 
     def standardise(self, data):
         X = torch.zeros(len(data), KIN, dtype=torch.bfloat16, device=data.X.device)
@@ -406,13 +428,15 @@ class Surrogate:
         s.b1, s.b2, s.b3 = (t.data_ptr() for t in net.b)
         s.in_mean, s.in_rstd = self.in_mean.data_ptr(), self.in_rstd.data_ptr()
         s.out_mean, s.out_std = self.out_mean.data_ptr(), self.out_std.data_ptr()
-        s.qtol, s.wtol = self.QTOL, self.WTOL
+        s.in_lo, s.in_hi, s.out_absmax = self.in_lo.data_ptr(), self.in_hi.data_ptr(), self.out_absmax.data_ptr()
+        s.qtol, s.wtol, s.envelope = self.QTOL, self.WTOL, self.ENVELOPE
         return s
 
     def save(self, path):
         torch.save({"net": {k: v.cpu() for k, v in self.net.state_dict().items()},
                     "in_mean": self.in_mean.cpu(), "in_rstd": self.in_rstd.cpu(),
-                    "out_mean": self.out_mean.cpu(), "out_std": self.out_std.cpu()}, path)
+                    "out_mean": self.out_mean.cpu(), "out_std": self.out_std.cpu(), "in_lo": self.in_lo.cpu(),
+                    "in_hi": self.in_hi.cpu(), "out_absmax": self.out_absmax.cpu()}, path)
 
     @classmethod
     def load(cls, path, device="cuda"):
@@ -422,13 +446,14 @@ class Surrogate:
         self.net.load_state_dict(d["net"])
         self.in_mean, self.in_rstd = d["in_mean"], d["in_rstd"]
         self.out_mean, self.out_std = d["out_mean"], d["out_std"]
+        self.in_lo, self.in_hi, self.out_absmax = d["in_lo"], d["in_hi"], d["out_absmax"]
         self.history = []
         return self
 
 
 # ----------------------------------------------------------------- inline use
 class _SurrogateOps:
-    def __init__(self, ops, sur, chunk, qtol=None, wtol=None):
+    def __init__(self, ops, sur, chunk, qtol=None, wtol=None, envelope=None):
         self._ops, self.sur, self.chunk = ops, sur, chunk
         self._ws = {}
         self.nfallback = torch.zeros(1, dtype=torch.int32, device="cuda")
@@ -437,6 +462,8 @@ class _SurrogateOps:
             self._struct.qtol = qtol
         if wtol is not None:
             self._struct.wtol = wtol
+        if envelope is not None:
+            self._struct.envelope = envelope
 
     def __getattr__(self, name):
         return getattr(self._ops, name)
@@ -469,12 +496,12 @@ class _SurrogateOps:
                 graupelnc, graupelncv, sr, refl_10cm)
 
 
-def surrogate_ops(sur, ops=None, chunk=65536, qtol=None, wtol=None):
+def surrogate_ops(sur, ops=None, chunk=65536, qtol=None, wtol=None, envelope=None):
     """The step's functions with microphysics_driver served by the surrogate
-    (ops: the namespace to wrap, pywrf_amd.ops() by default; qtol / wtol
-    override the fallback tolerances)."""
+    (ops: the namespace to wrap, pywrf_amd.ops() by default; qtol / wtol /
+    envelope override the fallback tolerances)."""
     import pywrf_amd
-    return _SurrogateOps(ops or pywrf_amd.ops(), sur, chunk, qtol, wtol)
+    return _SurrogateOps(ops or pywrf_amd.ops(), sur, chunk, qtol, wtol, envelope)
 
 
 # ----------------------------------------------------------------- evaluation
