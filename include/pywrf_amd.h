@@ -544,7 +544,10 @@ int pywrf_mlp_adam(int rows, int cols, float* w, const float* g, float* m, float
  * may show, out_absmax (288) every tendency the network may return, as seen
  * in the training set; a column outside it is not trusted to the network.
  * envelope scales the bounds (the input range is widened by (envelope - 1)
- * times its width on both sides, the tendency bound multiplied by it). */
+ * times its width on both sides, the tendency bound multiplied by it).
+ * dtmax: the largest temperature change (K) the network may make at a level
+ * in one call; columns it would heat or cool more (deep convection) are left
+ * to the plain scheme. */
 typedef struct pywrf_surrogate {
   const void* w1; /* BF16 [512 x 384] */
   const void* w2; /* BF16 [512 x 512] */
@@ -559,7 +562,7 @@ typedef struct pywrf_surrogate {
   const float* in_lo;
   const float* in_hi;
   const float* out_absmax;
-  float qtol, wtol, envelope;
+  float qtol, wtol, envelope, dtmax;
 } pywrf_surrogate_t;
 
 /* Bytes of workspace pywrf_wsm6_surrogate needs for grid g (its physics tile)

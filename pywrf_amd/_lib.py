@@ -36,7 +36,8 @@ class Surrogate(ctypes.Structure):
     standardisation, and the fallback tolerances."""
     _fields_ = [(n, ctypes.c_void_p) for n in
                 "w1 w2 w3 b1 b2 b3 in_mean in_rstd out_mean out_std in_lo in_hi out_absmax".split()] + [
-               ("qtol", ctypes.c_float), ("wtol", ctypes.c_float), ("envelope", ctypes.c_float)]
+               ("qtol", ctypes.c_float), ("wtol", ctypes.c_float), ("envelope", ctypes.c_float),
+               ("dtmax", ctypes.c_float)]
 
 
 _CTYPES = {

@@ -23,7 +23,10 @@
 //     (T, rho and the six mixing ratios at every level) within the range the
 //     training set showed for that feature, widened by (envelope - 1) times
 //     its width on each side, and each tendency within envelope times the
-//     largest magnitude the training set showed for it.
+//     largest magnitude the training set showed for it;
+//   * no level heated or cooled by more than dtmax in the call (the deep
+//     convective columns, where errors of the network feed back fastest,
+//     are left to the plain scheme).
 // A passing column's water loss sum_k rho dz (old - new) [kg m-2 = mm] is its
 // surface precipitation of the step (rainncv, added to rainnc; the surrogate
 // does not split it into snow and graupel, so snowncv = graupelncv = sr = 0).
@@ -113,7 +116,7 @@ __global__ void __launch_bounds__(256) k_sg_pack(Tile T, Fields f, const float* 
 // major, [288][ldy].
 struct Envelope {
   const float *lo, *hi, *amax;
-  float scale;
+  float scale, dtmax;
 };
 
 // input feature var * 41 + k inside the widened training range
@@ -142,7 +145,7 @@ __global__ void __launch_bounds__(256) k_sg_apply(Tile T, Fields f, const float*
     const float old[SG_NOUT - 1] = {f.qv[q], f.qc[q], f.qr[q], f.qi[q], f.qs[q], f.qg[q]};
     const float t0 = th * pii, tt = tend(0, k);
     const float tn = t0 + dt * tt;
-    bad |= !(tn - tn == 0.0f);                           // NaN or infinite
+    bad |= !(tn - tn == 0.0f) || !(fabsf(tn - t0) <= env.dtmax);   // NaN, infinite or too large
     bad |= !in_range(env, k, t0) || !in_range(env, 2 * SG_LEV + k, rho) || !(fabsf(tt) <= env.scale * env.amax[k]);
     float so = 0.0f, sn = 0.0f;
 #pragma unroll
@@ -226,6 +229,7 @@ static int sg_run(const pywrf_grid_t* g, const pywrf_surrogate_t* net, float* th
   if (!net->in_lo || !net->in_hi || !net->out_absmax || !(net->envelope >= 1.0f)) return PYWRF_E_ARG;
   Envelope env;
   env.lo = net->in_lo; env.hi = net->in_hi; env.amax = net->out_absmax; env.scale = net->envelope;
+  env.dtmax = net->dtmax;
   hipLaunchKernelGGL(k_sg_zero_int, dim3(1), dim3(1), 0, (hipStream_t)s, nfall);
   for (long c0 = 0; c0 < ncol; c0 += chunk) {
     const int m = (int)(ncol - c0 < chunk ? ncol - c0 : chunk);

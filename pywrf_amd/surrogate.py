@@ -311,6 +311,7 @@ class Surrogate:
     QTOL = 1e-7      # kg/kg: smaller negative mixing ratios fail the column
     WTOL = 1e-5      # relative column water gain that fails the column
     ENVELOPE = 1.05  # widening of the training envelope (see pywrf_surrogate_t)
+    DTMAX = 0.25     # K per call: larger heating or cooling at a level fails the column
 
     def __init__(self, net, in_mean, in_rstd, out_mean, out_std, in_lo=None, in_hi=None, out_absmax=None):
         self.net = net
@@ -429,7 +430,7 @@ class Surrogate:
         s.in_mean, s.in_rstd = self.in_mean.data_ptr(), self.in_rstd.data_ptr()
         s.out_mean, s.out_std = self.out_mean.data_ptr(), self.out_std.data_ptr()
         s.in_lo, s.in_hi, s.out_absmax = self.in_lo.data_ptr(), self.in_hi.data_ptr(), self.out_absmax.data_ptr()
-        s.qtol, s.wtol, s.envelope = self.QTOL, self.WTOL, self.ENVELOPE
+        s.qtol, s.wtol, s.envelope, s.dtmax = self.QTOL, self.WTOL, self.ENVELOPE, self.DTMAX
         return s
 
     def save(self, path):
@@ -453,7 +454,7 @@ class Surrogate:
 
 # ----------------------------------------------------------------- inline use
 class _SurrogateOps:
-    def __init__(self, ops, sur, chunk, qtol=None, wtol=None, envelope=None):
+    def __init__(self, ops, sur, chunk, qtol=None, wtol=None, envelope=None, dtmax=None):
         self._ops, self.sur, self.chunk = ops, sur, chunk
         self._ws = {}
         self.nfallback = torch.zeros(1, dtype=torch.int32, device="cuda")
@@ -464,6 +465,8 @@ class _SurrogateOps:
             self._struct.wtol = wtol
         if envelope is not None:
             self._struct.envelope = envelope
+        if dtmax is not None:
+            self._struct.dtmax = dtmax
 
     def __getattr__(self, name):
         return getattr(self._ops, name)
@@ -496,12 +499,12 @@ class _SurrogateOps:
                 graupelnc, graupelncv, sr, refl_10cm)
 
 
-def surrogate_ops(sur, ops=None, chunk=65536, qtol=None, wtol=None, envelope=None):
+def surrogate_ops(sur, ops=None, chunk=65536, qtol=None, wtol=None, envelope=None, dtmax=None):
     """The step's functions with microphysics_driver served by the surrogate
     (ops: the namespace to wrap, pywrf_amd.ops() by default; qtol / wtol /
-    envelope override the fallback tolerances)."""
+    envelope / dtmax override the fallback tolerances)."""
     import pywrf_amd
-    return _SurrogateOps(ops or pywrf_amd.ops(), sur, chunk, qtol, wtol, envelope)
+    return _SurrogateOps(ops or pywrf_amd.ops(), sur, chunk, qtol, wtol, envelope, dtmax)
 
 
 # ----------------------------------------------------------------- evaluation
@@ -546,6 +549,9 @@ def compare_runs(sur, config="config2", nsteps=60, chunk=16384, log=None):
     out["qv_rms_rel"] = rms(qv1 - qv0) / rms(qv0)
     out["theta_rms_rel"] = rms(t1 - t0_) / rms(t0_ + 300.0)
     out["theta_rms_K"] = rms(t1 - t0_)
+    out["theta_maxabs_K"] = float((t1 - t0_).abs().max())
+    out["qv_maxabs"] = float((qv1 - qv0).abs().max())
+    out["finite"] = bool(torch.isfinite(t1).all()) and bool(torch.isfinite(qv1).all())
     out["speedup"] = out["surrogate_steps_per_s"] / out["wsm6_steps_per_s"]
     out["steps"] = nsteps
     return out

@@ -68,7 +68,7 @@ def test_fallback_counts_and_inline_accuracy(trained):
     import pywrf_amd
     sg, sur, _, _ = trained
     plain = _step("config1c", pywrf_amd.ops(), 5)
-    none = sg.surrogate_ops(sur, qtol=1e30, wtol=1e30, envelope=1e30)
+    none = sg.surrogate_ops(sur, qtol=1e30, wtol=1e30, envelope=1e30, dtmax=1e30)
     s_none = _step("config1c", none, 5)
     assert int(none.nfallback) == 0
     assert not torch.equal(s_none.moist, plain.moist)
