@@ -17,8 +17,8 @@
 // run from LDS; a slab this deep keeps the matrix cores busy across the
 // global-load latency with two workgroups per CU.  Columns of A and B at or
 // beyond K, and rows beyond M and N, load as zero; stores are predicated.
-// Blocks are numbered so that consecutive M-tiles of one N-tile go to one
-// XCD (they share B).
+// Blocks are numbered so that the N-tiles of one M-tile go to one XCD (they
+// share A, the large operand).
 //
 // Epilogues: FP32 C (+ bias), FP32 C^T (+ bias; the surrogate's output, read
 // per feature across columns), BF16 relu(C + bias), BF16 C * [H > 0] (the
@@ -67,10 +67,14 @@ __global__ void __launch_bounds__(256) k_mlp_gemm(GemmArgs g) {
   __shared__ __attribute__((aligned(16))) bf16 Bs[MLP_BN * MLP_LDS_ROW];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  // tile order: the M-tiles of one N-tile in a contiguous run per XCD
+  // tile order: blockIdx.x counts the N-tiles, blockIdx.y the M-tiles, and
+  // each XCD takes a contiguous run of that order, so all N-tiles of one
+  // M-tile run on one XCD and read their A tile (the activations, the large
+  // operand) once from memory into that XCD's L2; B (a weight matrix) fits
+  // in every L2
   int bx = (int)blockIdx.x, by = (int)blockIdx.y;
   xcd_block_2d(bx, by);
-  const int m0 = bx * MLP_BM, n0 = by * MLP_BN;
+  const int m0 = by * MLP_BM, n0 = bx * MLP_BN;
   const int kbeg = (int)blockIdx.z * g.kchunk;
   const int kend = imin(g.K, kbeg + g.kchunk);
   const int nk = kend > kbeg ? (kend - kbeg + MLP_BK - 1) / MLP_BK : 0;
@@ -173,7 +177,7 @@ extern "C" int pywrf_mlp_gemm(int epi, int M, int N, int K, const void* A, int l
   const int per = (slabs + ksplit - 1) / ksplit;
   g.kchunk = per * MLP_BK;
   const int splits = K == 0 ? 1 : (slabs + per - 1) / per;
-  dim3 grid((unsigned)((M + MLP_BM - 1) / MLP_BM), (unsigned)((N + MLP_BN - 1) / MLP_BN), (unsigned)splits);
+  dim3 grid((unsigned)((N + MLP_BN - 1) / MLP_BN), (unsigned)((M + MLP_BM - 1) / MLP_BM), (unsigned)splits);
   switch (epi) {
     case EPI_F32: hipLaunchKernelGGL(k_mlp_gemm<EPI_F32>, grid, dim3(256), 0, (hipStream_t)s, g); break;
     case EPI_BF16_RELU: hipLaunchKernelGGL(k_mlp_gemm<EPI_BF16_RELU>, grid, dim3(256), 0, (hipStream_t)s, g); break;
