@@ -16,7 +16,8 @@
 // the loads of slab t + 1 are in flight while the 32 MFMAs per wave of slab t
 // run from LDS; a slab this deep keeps the matrix cores busy across the
 // global-load latency with two workgroups per CU.  Columns of A and B at or
-// beyond K, and rows beyond M and N, load as zero; stores are predicated.
+// beyond K enter LDS as zeros; rows beyond M and N are clamped loads whose
+// products are never stored.
 // Blocks are numbered so that the N-tiles of one M-tile go to one XCD (they
 // share A, the large operand).
 //
@@ -48,17 +49,15 @@ struct GemmArgs {
   const bf16* H;
 };
 
-// 16 bytes of a row-major bf16 matrix, zero outside rows [0, rows) and
-// columns [0, cols).  The load itself is unconditional, from an address
-// clamped into the matrix, and the zero is selected afterwards: a load under
-// a branch that not every lane takes is waited for where the branch closes,
-// which would serialise the slab's sixteen loads.
+// 16 bytes of a row-major bf16 matrix at (row, col), both clamped into the
+// matrix ([0, rows) x [0, cols - 8]).  The load is unconditional: a load the
+// compiler may predicate is waited for before the predicated region closes,
+// which would serialise a slab's sixteen loads.  Rows clamped beyond M or N
+// only feed outputs that are not stored; columns beyond K are replaced by
+// zeros when the slab is written to LDS.
 __device__ inline uint4 load16(const bf16* base, int ld, int row, int rows, int col, int cols) {
-  const bool in = row < rows && col < cols;
-  const int r = row < rows ? row : rows - 1, c = col < cols ? col : cols - 8;
-  uint4 v = *(const uint4*)(base + (long)r * ld + c);
-  if (!in) v = make_uint4(0u, 0u, 0u, 0u);
-  return v;
+  const int r = row < rows ? row : rows - 1, c = col < cols - 8 ? col : cols - 8;
+  return *(const uint4*)(base + (long)r * ld + c);
 }
 
 template <int EPI>
@@ -89,12 +88,15 @@ __global__ void __launch_bounds__(256) k_mlp_gemm(GemmArgs g) {
       rb[r] = load16(g.B, g.ldb, n0 + row, g.N, col, kend);
     }
   };
-  auto lstore = [&]() {
+  // the slab at k0 into LDS, columns at or beyond K as zeros
+  auto lstore = [&](int k0) {
+    const int lim = kend - k0;
+    const uint4 zero = make_uint4(0u, 0u, 0u, 0u);
 #pragma unroll
     for (int r = 0; r < MLP_CH; ++r) {
       const int p = tid + 256 * r, row = p >> 4, col = (p & 15) * 8;
-      *(uint4*)&As[row * MLP_LDS_ROW + col] = ra[r];
-      *(uint4*)&Bs[row * MLP_LDS_ROW + col] = rb[r];
+      *(uint4*)&As[row * MLP_LDS_ROW + col] = col < lim ? ra[r] : zero;
+      *(uint4*)&Bs[row * MLP_LDS_ROW + col] = col < lim ? rb[r] : zero;
     }
   };
 
@@ -110,7 +112,7 @@ __global__ void __launch_bounds__(256) k_mlp_gemm(GemmArgs g) {
   const int lr = lane & 31, lk = (lane >> 5) * 8;
   for (int kt = 0; kt < nk; ++kt) {
     __syncthreads();                 // every wave is done reading the previous slab
-    lstore();
+    lstore(kbeg + kt * MLP_BK);
     __syncthreads();
     if (kt + 1 < nk) gload(kbeg + (kt + 1) * MLP_BK);
 #pragma unroll
