@@ -80,25 +80,28 @@ __global__ void __launch_bounds__(256) k_mlp_gemm(GemmArgs g) {
 
   // piece p = tid + 256 r of a slab covers row p / 16, columns 8 (p % 16) .. + 8
   uint4 ra[MLP_CH], rb[MLP_CH];
-  auto gload = [&](int k0) {
-#pragma unroll
-    for (int r = 0; r < MLP_CH; ++r) {
-      const int p = tid + 256 * r, row = p >> 4, col = k0 + (p & 15) * 8;
-      ra[r] = load16(g.A, g.lda, m0 + row, g.M, col, kend);
-      rb[r] = load16(g.B, g.ldb, n0 + row, g.N, col, kend);
-    }
-  };
+#define MLP_GLOAD(k0)                                                                   \
+  do {                                                                                  \
+    _Pragma("unroll") for (int r = 0; r < MLP_CH; ++r) {                                \
+      const int p = tid + 256 * r, row = p >> 4, col = (k0) + (p & 15) * 8;             \
+      ra[r] = load16(g.A, g.lda, m0 + row, g.M, col, kend);                             \
+      rb[r] = load16(g.B, g.ldb, n0 + row, g.N, col, kend);                             \
+    }                                                                                   \
+  } while (0)
   // the slab at k0 into LDS, columns at or beyond K as zeros
-  auto lstore = [&](int k0) {
-    const int lim = kend - k0;
-    const uint4 zero = make_uint4(0u, 0u, 0u, 0u);
-#pragma unroll
-    for (int r = 0; r < MLP_CH; ++r) {
-      const int p = tid + 256 * r, row = p >> 4, col = (p & 15) * 8;
-      *(uint4*)&As[row * MLP_LDS_ROW + col] = col < lim ? ra[r] : zero;
-      *(uint4*)&Bs[row * MLP_LDS_ROW + col] = col < lim ? rb[r] : zero;
-    }
-  };
+#define MLP_LSTORE(k0)                                                                  \
+  do {                                                                                  \
+    const int lim = kend - (k0);                                                        \
+    _Pragma("unroll") for (int r = 0; r < MLP_CH; ++r) {                                \
+      const int p = tid + 256 * r, row = p >> 4, col = (p & 15) * 8;                    \
+      const bool in = col < lim;                                                        \
+      uint4 a = ra[r], b = rb[r];                                                       \
+      a.x = in ? a.x : 0u; a.y = in ? a.y : 0u; a.z = in ? a.z : 0u; a.w = in ? a.w : 0u; \
+      b.x = in ? b.x : 0u; b.y = in ? b.y : 0u; b.z = in ? b.z : 0u; b.w = in ? b.w : 0u; \
+      *(uint4*)&As[row * MLP_LDS_ROW + col] = a;                                        \
+      *(uint4*)&Bs[row * MLP_LDS_ROW + col] = b;                                        \
+    }                                                                                   \
+  } while (0)
 
   f32x16 acc[2][2];
 #pragma unroll
@@ -108,13 +111,13 @@ __global__ void __launch_bounds__(256) k_mlp_gemm(GemmArgs g) {
 #pragma unroll
       for (int e = 0; e < 16; ++e) acc[a][b][e] = 0.0f;
 
-  if (nk > 0) gload(kbeg);
+  if (nk > 0) MLP_GLOAD(kbeg);
   const int lr = lane & 31, lk = (lane >> 5) * 8;
   for (int kt = 0; kt < nk; ++kt) {
     __syncthreads();                 // every wave is done reading the previous slab
-    lstore(kbeg + kt * MLP_BK);
+    MLP_LSTORE(kbeg + kt * MLP_BK);
     __syncthreads();
-    if (kt + 1 < nk) gload(kbeg + (kt + 1) * MLP_BK);
+    if (kt + 1 < nk) MLP_GLOAD(kbeg + (kt + 1) * MLP_BK);
 #pragma unroll
     for (int ks = 0; ks < MLP_BK; ks += 16) {
       bf16x8 af[2], bfr[2];
@@ -129,31 +132,69 @@ __global__ void __launch_bounds__(256) k_mlp_gemm(GemmArgs g) {
         for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
     }
   }
+#undef MLP_GLOAD
+#undef MLP_LSTORE
 
-  // accumulator element e of lane l holds C[8 (e / 4) + 4 (l / 32) + e % 4][l % 32] of the 32 x 32 tile
+  // accumulator element e of lane l holds C[8 (e / 4) + 4 (l / 32) + e % 4][l % 32] of the 32 x 32 tile.
+  // Every operand of the epilogue (bias, ReLU mask) is loaded, at clamped
+  // indices, before the first store: stores and loads share one counter, and
+  // a load needed after predicated stores would make each store wait for all
+  // earlier ones.
+  float bv[2];
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    const int n = imin(n0 + wn * 64 + b * 32 + lr, g.N - 1);
+    bv[b] = (EPI == EPI_F32 || EPI == EPI_BF16_RELU || EPI == EPI_F32_T) && g.bias ? g.bias[n] : 0.0f;
+  }
+  if (EPI == EPI_BF16_DRELU) {
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const int n = imin(n0 + wn * 64 + b * 32 + lr, g.N - 1);
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+          const int m = imin(m0 + wm * 64 + a * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3), g.M - 1);
+          if (!((float)g.H[(long)m * g.ldh + n] > 0.0f)) acc[a][b][e] = 0.0f;
+        }
+      }
+  }
+  // the values, formed unconditionally (so the operand loads are waited for
+  // once, here, and none is pending at the stores)
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        float v = acc[a][b][e] + bv[b];
+        if (EPI == EPI_BF16_RELU) v = v > 0.0f ? v : 0.0f;
+        acc[a][b][e] = v;
+      }
+  // wait here for every load (s_waitcnt vmcnt(0)): the compiler sinks the
+  // arithmetic above into the predicated stores below and would otherwise
+  // wait for the operand loads -- and so for every earlier store -- in each
+  // of them
+  __builtin_amdgcn_s_waitcnt(0x0F70);
+  // a tile wholly inside C stores without per-element predicates
+  const bool full = m0 + MLP_BM <= g.M && n0 + MLP_BN <= g.N;
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
       const int n = n0 + wn * 64 + b * 32 + lr;
-      if (n >= g.N) continue;
-      const float bv = (EPI == EPI_F32 || EPI == EPI_BF16_RELU || EPI == EPI_F32_T) && g.bias ? g.bias[n] : 0.0f;
 #pragma unroll
       for (int e = 0; e < 16; ++e) {
         const int m = m0 + wm * 64 + a * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
-        if (m >= g.M) continue;
+        if (!full && (m >= g.M || n >= g.N)) continue;
         const float v = acc[a][b][e];
         const long q = (long)m * g.ldc + n;
         if (EPI == EPI_F32) {
-          ((float*)g.C)[q] = v + bv;
+          ((float*)g.C)[q] = v;
         } else if (EPI == EPI_F32_T) {
-          ((float*)g.C)[(long)n * g.ldc + m] = v + bv;
-        } else if (EPI == EPI_BF16_RELU) {
-          const float r = v + bv;
-          ((bf16*)g.C)[q] = (bf16)(r > 0.0f ? r : 0.0f);
-        } else if (EPI == EPI_BF16_DRELU) {
-          const float h = (float)g.H[(long)m * g.ldh + n];
-          ((bf16*)g.C)[q] = (bf16)(h > 0.0f ? v : 0.0f);
+          ((float*)g.C)[(long)n * g.ldc + m] = v;
+        } else if (EPI == EPI_BF16_RELU || EPI == EPI_BF16_DRELU) {
+          ((bf16*)g.C)[q] = (bf16)v;
         } else {
           atomicAdd(&((float*)g.C)[q], v);
         }
