@@ -33,7 +33,10 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 #define MLP_BM 128
 #define MLP_BN 128
+#ifndef MLP_BK
 #define MLP_BK 128
+#endif
+#define MLP_NBUF (MLP_BK >= 128 ? 1 : 2)  // LDS buffers: one for deep slabs, two (ping-pong) for shallow ones
 #define MLP_LDS_ROW (MLP_BK + 8)  // bf16 per LDS row (8 padding)
 #define MLP_KQ 32                 // K must be a multiple of this
 #define MLP_CH (MLP_BM * MLP_BK / 8 / 256)  // 16-byte pieces of A (and of B) per thread per slab
@@ -62,8 +65,8 @@ __device__ inline uint4 load16(const bf16* base, int ld, int row, int rows, int 
 
 template <int EPI>
 __global__ void __launch_bounds__(256) k_mlp_gemm(GemmArgs g) {
-  __shared__ __attribute__((aligned(16))) bf16 As[MLP_BM * MLP_LDS_ROW];
-  __shared__ __attribute__((aligned(16))) bf16 Bs[MLP_BN * MLP_LDS_ROW];
+  __shared__ __attribute__((aligned(16))) bf16 As[MLP_NBUF][MLP_BM * MLP_LDS_ROW];
+  __shared__ __attribute__((aligned(16))) bf16 Bs[MLP_NBUF][MLP_BN * MLP_LDS_ROW];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   // tile order: blockIdx.x counts the N-tiles, blockIdx.y the M-tiles, and
@@ -78,28 +81,28 @@ __global__ void __launch_bounds__(256) k_mlp_gemm(GemmArgs g) {
   const int kend = imin(g.K, kbeg + g.kchunk);
   const int nk = kend > kbeg ? (kend - kbeg + MLP_BK - 1) / MLP_BK : 0;
 
-  // piece p = tid + 256 r of a slab covers row p / 16, columns 8 (p % 16) .. + 8
+  // piece p = tid + 256 r of a slab covers row p / (BK / 8), columns 8 (p % (BK / 8)) .. + 8
   uint4 ra[MLP_CH], rb[MLP_CH];
 #define MLP_GLOAD(k0)                                                                   \
   do {                                                                                  \
     _Pragma("unroll") for (int r = 0; r < MLP_CH; ++r) {                                \
-      const int p = tid + 256 * r, row = p >> 4, col = (k0) + (p & 15) * 8;             \
+      const int p = tid + 256 * r, row = p / (MLP_BK / 8), col = (k0) + (p % (MLP_BK / 8)) * 8; \
       ra[r] = load16(g.A, g.lda, m0 + row, g.M, col, kend);                             \
       rb[r] = load16(g.B, g.ldb, n0 + row, g.N, col, kend);                             \
     }                                                                                   \
   } while (0)
   // the slab at k0 into LDS, columns at or beyond K as zeros
-#define MLP_LSTORE(k0)                                                                  \
+#define MLP_LSTORE(k0, buf)                                                             \
   do {                                                                                  \
     const int lim = kend - (k0);                                                        \
     _Pragma("unroll") for (int r = 0; r < MLP_CH; ++r) {                                \
-      const int p = tid + 256 * r, row = p >> 4, col = (p & 15) * 8;                    \
+      const int p = tid + 256 * r, row = p / (MLP_BK / 8), col = (p % (MLP_BK / 8)) * 8; \
       const bool in = col < lim;                                                        \
       uint4 a = ra[r], b = rb[r];                                                       \
       a.x = in ? a.x : 0u; a.y = in ? a.y : 0u; a.z = in ? a.z : 0u; a.w = in ? a.w : 0u; \
       b.x = in ? b.x : 0u; b.y = in ? b.y : 0u; b.z = in ? b.z : 0u; b.w = in ? b.w : 0u; \
-      *(uint4*)&As[row * MLP_LDS_ROW + col] = a;                                        \
-      *(uint4*)&Bs[row * MLP_LDS_ROW + col] = b;                                        \
+      *(uint4*)&As[buf][row * MLP_LDS_ROW + col] = a;                                   \
+      *(uint4*)&Bs[buf][row * MLP_LDS_ROW + col] = b;                                   \
     }                                                                                   \
   } while (0)
 
@@ -111,27 +114,46 @@ __global__ void __launch_bounds__(256) k_mlp_gemm(GemmArgs g) {
 #pragma unroll
       for (int e = 0; e < 16; ++e) acc[a][b][e] = 0.0f;
 
-  if (nk > 0) MLP_GLOAD(kbeg);
   const int lr = lane & 31, lk = (lane >> 5) * 8;
-  for (int kt = 0; kt < nk; ++kt) {
-    __syncthreads();                 // every wave is done reading the previous slab
-    MLP_LSTORE(kbeg + kt * MLP_BK);
+#define MLP_COMPUTE(buf)                                                                \
+  _Pragma("unroll") for (int ks = 0; ks < MLP_BK; ks += 16) {                           \
+    bf16x8 af[2], bfr[2];                                                               \
+    _Pragma("unroll") for (int t = 0; t < 2; ++t) {                                     \
+      af[t] = *(const bf16x8*)&As[buf][(wm * 64 + t * 32 + lr) * MLP_LDS_ROW + ks + lk]; \
+      bfr[t] = *(const bf16x8*)&Bs[buf][(wn * 64 + t * 32 + lr) * MLP_LDS_ROW + ks + lk]; \
+    }                                                                                   \
+    _Pragma("unroll") for (int a = 0; a < 2; ++a)                                       \
+      _Pragma("unroll") for (int b = 0; b < 2; ++b)                                     \
+        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0); \
+  }
+  if (MLP_NBUF == 1) {
+    // one buffer: store the prefetched slab, then fetch the next one while
+    // the MFMAs of this one run (two barriers per slab)
+    if (nk > 0) MLP_GLOAD(kbeg);
+    for (int kt = 0; kt < nk; ++kt) {
+      __syncthreads();               // every wave is done reading the previous slab
+      MLP_LSTORE(kbeg + kt * MLP_BK, 0);
+      __syncthreads();
+      if (kt + 1 < nk) MLP_GLOAD(kbeg + (kt + 1) * MLP_BK);
+      MLP_COMPUTE(0)
+    }
+  } else {
+    // two buffers: fetch slab t + 1, run slab t from one buffer, store slab
+    // t + 1 into the other (one barrier per slab)
+    if (nk > 0) {
+      MLP_GLOAD(kbeg);
+      MLP_LSTORE(kbeg, 0);
+    }
     __syncthreads();
-    if (kt + 1 < nk) MLP_GLOAD(kbeg + (kt + 1) * MLP_BK);
-#pragma unroll
-    for (int ks = 0; ks < MLP_BK; ks += 16) {
-      bf16x8 af[2], bfr[2];
-#pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        af[t] = *(const bf16x8*)&As[(wm * 64 + t * 32 + lr) * MLP_LDS_ROW + ks + lk];
-        bfr[t] = *(const bf16x8*)&Bs[(wn * 64 + t * 32 + lr) * MLP_LDS_ROW + ks + lk];
-      }
-#pragma unroll
-      for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
+    for (int kt = 0; kt < nk; ++kt) {
+      const int cur = kt & 1;
+      if (kt + 1 < nk) MLP_GLOAD(kbeg + (kt + 1) * MLP_BK);
+      MLP_COMPUTE(cur)
+      if (kt + 1 < nk) MLP_LSTORE(kbeg + (kt + 1) * MLP_BK, cur ^ 1);
+      __syncthreads();
     }
   }
+#undef MLP_COMPUTE
 #undef MLP_GLOAD
 #undef MLP_LSTORE
 
