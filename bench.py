@@ -13,9 +13,12 @@ updates), all on the device; the state is resident in HBM before timing.
 
 The timed steps are replays of HIP graphs of up to 10 steps (pywrf_amd.graph)
 unless --eager.  The warmup steps run eagerly (the first loads the code
-objects).  One JSON line is printed with the steps/s, the HBM roofline of the
-dominant kernel (timed with HIP events on the launch stream) and the CPU
-baseline (the oracle restatement of the reference, one step on the host).
+objects).  One JSON line is printed with the steps/s; the HBM roofline of
+the step (700 field passes per step) and of the dominant kernel, timed inside
+a step with HIP events on the launch stream; the WSM6 surrogate against the
+plain scheme (steps/s, 60-step error, fallback fraction, the matrix-core
+roofline of its layers); and the CPU baseline (the oracle restatement of the
+reference on the host's cores).
 """
 import argparse
 import json
@@ -30,8 +33,11 @@ PEAK_HBM_GBS = 8000.0
 COPY_REF_FLOATS = 64 * 1024 * 1024   # 256 MiB copy used to scale the HBM byte counters
 # Measured bytes per launch at 240x240x41 (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, the scale of a
 # 256 MiB copy), from profiles/measured_kernels_240x240x41_hbm_bytes.md.  Only valid at that size.
-TRAFFIC_240 = {"advance_w": 213.9e6, "advance_uv": 194.8e6, "advance_mu_t": 131.8e6, "wsm6": 159.1e6,
-               "rk_scalar_tend_moist": 588.0e6}
+TRAFFIC_240 = {"advance_w": 213.9e6, "advance_uv": 194.8e6, "advance_mu_t": 131.8e6, "microphysics_driver": 159.1e6}
+# the HIP kernel each function launches
+KERNEL_OF = {"advance_w": "k_advance_w", "advance_uv": "k_advance_uv", "advance_mu_t": "k_advance_mu_t",
+             "microphysics_driver": "k_wsm6", "pgf_buoy_coriolis_curvature": "k_pgf_buoy_cor",
+             "advect_scalar_fresh": "k_scalar_adv", "rk_scalar_tend_pair": "k_scalar_adv"}
 FIELD_PASSES_PER_STEP = 700          # algorithmic 3-D field passes per RK3 step (7 acoustic substeps included)
 
 
@@ -95,31 +101,122 @@ def kernel_candidates(s, cfg):
     ]
 
 
-def time_kernel(fn, reps):
+def call_bytes(name, rk_call, g):
+    """Algorithmic bytes of one call of a hot-path function at grid g: each
+    distinct array it must read or write once, 4 bytes per point (3-D fields
+    n3, 2-D fields n2).  None for functions outside the table (and for the
+    first-stage scalar transport, which also runs the diffusion)."""
+    n3 = g.nz * g.ny * g.nx * 4
+    n2 = g.ny * g.nx * 4
+    table = {
+        "advance_w": 20 * n3 + 6 * n2,
+        "advance_uv": 14 * n3 + 6 * n2,
+        "advance_mu_t": 13 * n3 + 12 * n2,
+        "microphysics_driver": 18 * n3 + 7 * n2,
+        # pressure gradient + buoyancy + Coriolis + curvature: 15 fields read, 3 tendencies read and written
+        "pgf_buoy_coriolis_curvature": 21 * n3 + 12 * n2,
+        # theta advection: t, ru, rv, ww read, t_tend written
+        "advect_scalar_fresh": 5 * n3 + n2,
+        # 6 moisture + 2 tracer species read, 8 advection tendencies written, ru_m, rv_m, ww_m read,
+        # h_tendency and z_tendency zeroed (stages 2 and 3: advection only)
+        "rk_scalar_tend_pair": None if rk_call == 0 else 21 * n3 + n2,
+    }
+    return table.get(name)
+
+
+def in_step_times(s, ops, cfg, spin_cycles=200_000):
+    """One eager RK3 step with every call bracketed by HIP events on the
+    launching stream, each call preceded by a GPU spin (torch.cuda._sleep) so
+    that the events and the call are enqueued before the GPU reaches them:
+    the events then time the call's kernels alone, with the cache contents
+    the step itself leaves (no flush, no host gap).  Returns
+    {name: [(call index, seconds)]}."""
     import torch
-    fn()
+    from pywrf_amd.step import rk3_step
+    pending = []
+    counts = {}
+
+    class Timed:
+        def __getattr__(self, name):
+            f = getattr(ops, name)
+            if not callable(f):
+                return f
+
+            def call(*args):
+                torch.cuda._sleep(spin_cycles)
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                r = f(*args)
+                e1.record()
+                n = counts.get(name, 0)
+                counts[name] = n + 1
+                pending.append((name, n, e0, e1))
+                return r
+            return call
+
+    rk3_step(s, Timed(), cfg)
     torch.cuda.synchronize()
-    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    start.record()
-    for _ in range(reps):
-        fn()
-    end.record()
-    end.synchronize()
-    return start.elapsed_time(end) / reps * 1e-3        # seconds per launch (host-side marshalling overlaps)
+    out = {}
+    for name, n, e0, e1 in pending:
+        out.setdefault(name, []).append((n, e0.elapsed_time(e1) * 1e-3))
+    return out
 
 
-def cpu_baseline(s_cpu, cfg):
+def dominant_roofline(times, g):
+    """The roofline object for the function with the largest in-step time
+    among those with a byte count: its bytes over all its (counted) calls of
+    the step divided by their summed time."""
+    best = None
+    for name, calls in times.items():
+        counted = [(n, t) for n, t in calls if call_bytes(name, n, g)]
+        if not counted:
+            continue
+        tsum = sum(t for _, t in counted)
+        bsum = sum(call_bytes(name, n, g) for n, _ in counted)
+        if best is None or tsum > best[1]:
+            best = (name, tsum, bsum, len(counted))
+    if best is None:
+        return None
+    name, tsum, bsum, ncalls = best
+    achieved = bsum / tsum / 1e9
+    return {"bound": "hbm", "kernel": KERNEL_OF.get(name, name), "function": name, "achieved": round(achieved, 1),
+            "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+            "traffic": TRAFFIC_240.get(name) if (g.nz, g.ny, g.nx) == (41, 240, 240) else None,
+            "bytes_per_launch": round(bsum / ncalls), "us_per_launch": round(tsum / ncalls * 1e6, 2),
+            "launches_per_step": ncalls, "timing": "in-step (eager step, each call spin-separated, HIP events)"}
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(s_cpu, cfg, steps=3):
+    """The oracle (the CPU restatement of the reference's step, torch on the
+    host) on the same seeded state, with every core this process may run on:
+    one warm-up step, then the median of `steps` timed steps."""
+    import statistics
     import torch
     import oracle
     from pywrf_amd.step import rk3_step
-    threads = min(16, os.cpu_count() or 1)
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     torch.set_num_threads(threads)
-    t0 = time.perf_counter()
     rk3_step(s_cpu, oracle.ops(), cfg)
-    dt = time.perf_counter() - t0
+    times = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        rk3_step(s_cpu, oracle.ops(), cfg)
+        times.append(time.perf_counter() - t0)
+    dt = statistics.median(times)
     return {"value": round(1.0 / dt, 5), "unit": "RK3 steps/s", "cores": threads, "kind": "port",
-            "sample": "1 RK3 step of the same 240x240x41 seeded state, FP32, oracle restatement of the reference "
-                      "(torch CPU, %d threads), %.1f s" % (threads, dt)}
+            "sample": "median of %d RK3 steps (after 1 warm-up) of the same 240x240x41 seeded state, FP32, oracle "
+                      "restatement of the reference (torch CPU, %d threads, %s), %.2f s/step"
+                      % (steps, threads, _cpu_model(), dt)}
 
 
 def drift(config, nsteps, cfg):
@@ -174,6 +271,41 @@ def drift(config, nsteps, cfg):
     print(json.dumps(out), flush=True)
 
 
+def surrogate_leg(args):
+    """The learned WSM6 replacement against the plain scheme: steps/s of the
+    whole step with each (HIP graphs), 60-step qv / theta RMS difference, the
+    fraction of columns that fell back, and the matrix-core roofline of the
+    network's three layers over all 227^2 columns."""
+    import torch
+    from pywrf_amd import surrogate as sg
+    if not os.path.exists(args.surrogate):
+        _log("bench: no trained surrogate at %s, surrogate leg skipped" % args.surrogate)
+        return None
+    sur = sg.Surrogate.load(args.surrogate)
+    _log("bench: surrogate vs WSM6, 60 steps each")
+    rec = sg.compare_runs(sur, nsteps=60)
+    # the forward pass alone (pack excluded) on 227^2 columns of standardised inputs
+    M = 227 * 227
+    X = torch.randn(M, sg.KIN, device="cuda").to(torch.bfloat16)
+    ws = sg.Workspace((M + 31) // 32 * 32)
+    for _ in range(3):
+        sg.forward(sur.net, X, ws)
+    torch.cuda._sleep(200_000)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    reps = 10
+    for _ in range(reps):
+        sg.forward(sur.net, X, ws)
+    e1.record()
+    e1.synchronize()
+    sec = e0.elapsed_time(e1) * 1e-3 / reps
+    flops = 2.0 * M * (sg.FIN * sg.HID + sg.HID * sg.HID + sg.HID * sg.FOUT)
+    rec["mfma_roofline"] = {"bound": "mfma", "kernel": "k_mlp_gemm (3 layers)", "achieved": round(flops / sec / 1e12, 1),
+                            "peak": 2500.0, "unit": "TFLOP/s", "frac": round(flops / sec / 1e12 / 2500.0, 4),
+                            "flops_per_call": flops, "us_per_call": round(sec * 1e6, 1)}
+    return {k: (round(v, 6) if isinstance(v, float) else v) for k, v in rec.items()}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,6 +314,8 @@ def main():
     ap.add_argument("--config", default="config2")
     ap.add_argument("--eager", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--surrogate", default=os.path.join(ROOT, "pywrf_amd", "data", "wsm6_surrogate.pt"),
+                    help="trained WSM6 surrogate to compare with the plain scheme over 60 steps ('' to skip)")
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--drift", type=int, default=0, metavar="N",
                     help="instead of timing: run N steps on the GPU and the oracle in FP64 and FP32 on the host, "
@@ -254,24 +388,16 @@ def main():
     ms = elapsed / args.steps * 1e3
     finite = bool(torch.isfinite(s.p).all()) and bool(torch.isfinite(s.u).all())
 
-    # dominant kernel: largest (launch time x launches per step), timed with HIP events
+    # the dominant kernel, timed inside a step (see in_step_times)
     n3 = g.nz * g.ny * g.nx
-    kern = []
-    for name, calls, nbytes, fn in kernel_candidates(s, cfg):
-        sec = time_kernel(fn, args.kernel_reps)
-        kern.append((sec * calls, name, sec, nbytes))
-        _log("bench: %-14s %8.1f us/launch  %7.1f GB/s  x%d per step" % (name, sec * 1e6, nbytes / sec / 1e9, calls))
-    kern.sort(reverse=True)
-    _, kname, ksec, kbytes = kern[0]
-    achieved = kbytes / ksec / 1e9
-    traffic = TRAFFIC_240.get(kname) if (g.nz, g.ny, g.nx) == (41, 240, 240) else None
-    roofline = {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
-                "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                "traffic": None if traffic is None else round(traffic),
-                "bytes_per_launch": kbytes, "us_per_launch": round(ksec * 1e6, 2)}
+    times = in_step_times(s, ops, cfg)
+    per_fn = sorted(((sum(t for _, t in c), n, len(c)) for n, c in times.items()), reverse=True)
+    for tsum, name, ncalls in per_fn[:8]:
+        _log("bench: in-step %-30s %8.1f us/step  %2d calls" % (name, tsum * 1e6, ncalls))
+    roofline = dominant_roofline(times, g)
     step_bytes = FIELD_PASSES_PER_STEP * n3 * 4
     out = {
-        "metric": "RK3 steps/s @240x240x41 FP32",
+        "metric": "RK3 steps/s @240x240x41 FP32 + HBM GB/s; surrogate vs WSM6 speedup at fixed err",
         "value": round(1e3 / ms, 3),
         "unit": "RK3 steps/s",
         "n_gpus": 1,
@@ -291,8 +417,11 @@ def main():
         "finite": finite,
         "roofline": roofline,
     }
+    out["in_step_top"] = {n: round(t * 1e6, 1) for t, n, _ in per_fn[:6]}
+    if args.surrogate and (g.nz, g.ny, g.nx) == (41, 240, 240):
+        out["surrogate"] = surrogate_leg(args)
     if not args.no_cpu_baseline:
-        _log("bench: CPU baseline (oracle, one step)")
+        _log("bench: CPU baseline (oracle, 1 + 3 steps)")
         out["cpu_baseline"] = cpu_baseline(s_cpu, cfg)
     print(json.dumps(out), flush=True)
 
