@@ -155,10 +155,19 @@ def in_step_times(s, ops, cfg, spin_cycles=200_000):
             return call
 
     rk3_step(s, Timed(), cfg)
+    # the cost of the event pair itself, subtracted from every call
+    empty = []
+    for _ in range(8):
+        torch.cuda._sleep(spin_cycles)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        e1.record()
+        empty.append((e0, e1))
     torch.cuda.synchronize()
+    base = sorted(e0.elapsed_time(e1) for e0, e1 in empty)[len(empty) // 2] * 1e-3
     out = {}
     for name, n, e0, e1 in pending:
-        out.setdefault(name, []).append((n, e0.elapsed_time(e1) * 1e-3))
+        out.setdefault(name, []).append((n, max(e0.elapsed_time(e1) * 1e-3 - base, 0.0)))
     return out
 
 
