@@ -583,6 +583,21 @@ int pywrf_wsm6_surrogate(const pywrf_grid_t* g, const pywrf_surrogate_t* net, fl
                          float* snowncv, float* sr, float* graupel, float* graupelncv, void* workspace,
                          long long workspace_bytes, int* nfallback, pywrf_stream_t s);
 
+/* pywrf_wsm6_surrogate without a workspace: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation): the
+ * buffers are a static device array of 65.3 MB held on every GPU the library
+ * runs on until the process ends.  Limit: at most 65536 tile columns (the
+ * 240 x 240 domain has 51529); more return PYWRF_E_RANGE.  It calls no
+ * allocator.  Calls that overlap on one GPU race (they share its array). */
+int pywrf_wsm6_surrogate_noalloc(const pywrf_grid_t* g, const pywrf_surrogate_t* net, float* th, const float* rho,
+                                 const float* pii, const float* p, const float* delz, double dt, float* qv,
+                                 float* qc, float* qr, float* qi, float* qs, float* qg, float* rain, float* rainncv,
+                                 float* snow, float* snowncv, float* sr, float* graupel, float* graupelncv,
+                                 int* nfallback, pywrf_stream_t s);
+/* The limits of pywrf_wsm6_surrogate_noalloc: columns per call, bytes of its array. */
+int pywrf_wsm6_surrogate_noalloc_limits(int* max_columns, long long* scratch_bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -34,6 +34,7 @@
 // Columns are processed in chunks through a workspace: the fallback list
 // (int [ncol]), then packed inputs X (bf16 [chunk x 384]), hidden layers
 // (bf16 [chunk x 512] x 2) and outputs (f32, feature-major [288 x chunk]).
+#include <mutex>
 #include "common.h"
 
 typedef __bf16 bf16;
@@ -264,4 +265,62 @@ extern "C" int pywrf_wsm6_surrogate(const pywrf_grid_t* g, const pywrf_surrogate
                                     pywrf_stream_t s) {
   return sg_run(g, net, th, rho, pii, p, delz, dt, qv, qc, qr, qi, qs, qg, rain, rainncv, snow, snowncv, sr, graupel,
                 graupelncv, (char*)workspace, (long)workspace_bytes, nfallback, s);
+}
+
+// ------------------------------------------------------------------ allocation-free variant
+// For callers that cannot allocate device memory (code under HIP graph
+// capture, serving loops): the same computation through a file-scope device
+// array instead of a caller's workspace.  It holds the fallback list of up to
+// SG_NA_MAX_COLS columns and the buffers of one chunk of SG_NA_CHUNK columns
+// (65.3 MB on every GPU the library runs on, for the life of the process).
+// Every call on a GPU uses that GPU's copy, so calls that overlap on one GPU
+// race; the fallback counter (the caller's *nfallback) is zeroed by a kernel
+// at the start of each call and the list is written before it is read.
+#define SG_NA_MAX_COLS 65536
+#define SG_NA_CHUNK 16384
+#define SG_NA_LIST_BYTES ((((long)SG_NA_MAX_COLS + 1) * 4 + 255) / 256 * 256)
+#define SG_NA_BYTES (SG_NA_LIST_BYTES + (long)SG_NA_CHUNK * (SG_KIN * 2 + 2 * SG_H * 2 + SG_NO * 4))
+static __device__ __align__(256) char scratch[SG_NA_BYTES];
+
+#define SG_NA_MAX_DEVICES 64
+static void* sg_scratch_ptr[SG_NA_MAX_DEVICES];
+static std::mutex sg_scratch_lock;
+
+// The device address of `scratch` on the current device (the array's name is
+// a host-side stand-in, not a device address); looked up once per device.
+static int sg_scratch(char** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= SG_NA_MAX_DEVICES) return PYWRF_E_RANGE;
+  std::lock_guard<std::mutex> guard(sg_scratch_lock);
+  if (!sg_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) return e != hipSuccess ? (int)e : PYWRF_E_RANGE;
+    sg_scratch_ptr[dev] = ptr;
+  }
+  *out = (char*)sg_scratch_ptr[dev];
+  return 0;
+}
+
+extern "C" int pywrf_wsm6_surrogate_noalloc(const pywrf_grid_t* g, const pywrf_surrogate_t* net, float* th,
+                                            const float* rho, const float* pii, const float* p, const float* delz,
+                                            double dt, float* qv, float* qc, float* qr, float* qi, float* qs,
+                                            float* qg, float* rain, float* rainncv, float* snow, float* snowncv,
+                                            float* sr, float* graupel, float* graupelncv, int* nfallback,
+                                            pywrf_stream_t s) {
+  if (!g) return PYWRF_E_RANGE;
+  const long ncol = (long)imax((g->ite - 1) - g->its, 0) * imax((g->jte - 1) - g->jts, 0);
+  if (ncol > SG_NA_MAX_COLS) return PYWRF_E_RANGE;     // more columns than the static list holds
+  char* ws = nullptr;
+  int e = sg_scratch(&ws);
+  if (e) return e;
+  return sg_run(g, net, th, rho, pii, p, delz, dt, qv, qc, qr, qi, qs, qg, rain, rainncv, snow, snowncv, sr, graupel,
+                graupelncv, ws, SG_NA_BYTES, nfallback, s);
+}
+
+extern "C" int pywrf_wsm6_surrogate_noalloc_limits(int* max_columns, long long* scratch_bytes) {
+  if (!max_columns || !scratch_bytes) return PYWRF_E_ARG;
+  *max_columns = SG_NA_MAX_COLS;
+  *scratch_bytes = SG_NA_BYTES;
+  return 0;
 }

@@ -454,8 +454,8 @@ class Surrogate:
 
 # ----------------------------------------------------------------- inline use
 class _SurrogateOps:
-    def __init__(self, ops, sur, chunk, qtol=None, wtol=None, envelope=None, dtmax=None):
-        self._ops, self.sur, self.chunk = ops, sur, chunk
+    def __init__(self, ops, sur, chunk, qtol=None, wtol=None, envelope=None, dtmax=None, noalloc=False):
+        self._ops, self.sur, self.chunk, self.noalloc = ops, sur, chunk, noalloc
         self._ws = {}
         self.nfallback = torch.zeros(1, dtype=torch.int32, device="cuda")
         self._struct = sur.struct()
@@ -485,6 +485,13 @@ class _SurrogateOps:
         """microphysics_driver's signature and results (wrf_physics.py:94-134),
         the columns computed by the surrogate with per-column WSM6 fallback."""
         g = grid(th.shape, (ids, ide, jds, jde, kds, kde, ims, ime, jms, jme, kms, kme, its, ite, jts, jte, kts, kte))
+        if self.noalloc:
+            call("pywrf_wsm6_surrogate_noalloc", g, ctypes.addressof(self._struct), ptr(th), ptr(rho), ptr(pi_phy),
+                 ptr(p), ptr(dz8w), float(dt), ptr(qv_curr), ptr(qc_curr), ptr(qr_curr), ptr(qi_curr), ptr(qs_curr),
+                 ptr(qg_curr), ptr(rainnc), ptr(rainncv), ptr(snownc), ptr(snowncv), ptr(sr), ptr(graupelnc),
+                 ptr(graupelncv), _vp(self.nfallback), stream())
+            return (th, qv_curr, qc_curr, qr_curr, qi_curr, qs_curr, qg_curr, rainnc, rainncv, snownc, snowncv,
+                    graupelnc, graupelncv, sr, refl_10cm)
         key = tuple(th.shape) + (its, ite, jts, jte)
         ws = self._ws.get(key)
         if ws is None:
@@ -499,12 +506,13 @@ class _SurrogateOps:
                 graupelnc, graupelncv, sr, refl_10cm)
 
 
-def surrogate_ops(sur, ops=None, chunk=65536, qtol=None, wtol=None, envelope=None, dtmax=None):
+def surrogate_ops(sur, ops=None, chunk=65536, qtol=None, wtol=None, envelope=None, dtmax=None, noalloc=False):
     """The step's functions with microphysics_driver served by the surrogate
     (ops: the namespace to wrap, pywrf_amd.ops() by default; qtol / wtol /
-    envelope / dtmax override the fallback tolerances)."""
+    envelope / dtmax override the fallback tolerances; noalloc: use the
+    library's allocation-free entry point, pywrf_wsm6_surrogate_noalloc)."""
     import pywrf_amd
-    return _SurrogateOps(ops or pywrf_amd.ops(), sur, chunk, qtol, wtol, envelope, dtmax)
+    return _SurrogateOps(ops or pywrf_amd.ops(), sur, chunk, qtol, wtol, envelope, dtmax, noalloc)
 
 
 # ----------------------------------------------------------------- evaluation
