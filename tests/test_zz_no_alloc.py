@@ -40,7 +40,7 @@ def _args(s):
 def _call(name, g, net, a, extra, nfall):
     from pywrf_amd import _lib
     ptrs = [ctypes.c_void_p(t.data_ptr()) for t in a]
-    rc = _lib.LIB[name] if False else getattr(_lib.LIB, name)(
+    rc = getattr(_lib.LIB, name)(
         g, ctypes.addressof(net), *ptrs[:5], ctypes.c_double(54.0), *ptrs[5:], *extra,
         ctypes.c_void_p(nfall.data_ptr()), ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
     return rc
@@ -52,7 +52,7 @@ def test_noalloc_equals_workspace_entry_point():
     sur = sg.Surrogate.load(os.path.join(ROOT, "pywrf_amd", "data", "wsm6_surrogate.pt"))
     net = sur.struct()
     maxc, nbytes = ctypes.c_int(), ctypes.c_longlong()
-    assert _lib.LIB.pywrf_wsm6_surrogate_noalloc_limits(ctypes.addressof(maxc), ctypes.addressof(nbytes)) == 0
+    assert _lib.LIB.pywrf_wsm6_surrogate_noalloc_limits(ctypes.addressof(maxc), ctypes.byref(nbytes)) == 0
     print("noalloc limits: %d columns, %d bytes" % (maxc.value, nbytes.value))
     for config, ncol in (("config2", 227 * 227), ("config4", 107 * 107)):
         s = _inputs(config)
