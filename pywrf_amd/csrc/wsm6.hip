@@ -300,33 +300,34 @@ __device__ inline float venfac(float a, float b, float c) {
   return expf(logf(viscos(b, c) / diffus(b, a)) * 0.3333333f) / sqrtf(viscos(b, c)) * sqrtf(sqrtf(F(rhoair0) / c));
 }
 
-__global__ void __launch_bounds__(WAVE) k_wsm6(Dims D, Wsm6 a) {
-  __shared__ SedLds L;
-  // XCD-aware column order: workgroups are dealt to the 8 XCDs round-robin,
-  // so XCD x takes the contiguous run [x * per, (x + 1) * per) of columns and
-  // the columns that share a cache line are served by one L2.
-  const int ni = a.i1 - a.i0;
-  int c = (int)(blockIdx.x % PW_XCDS) * a.per + (int)(blockIdx.x / PW_XCDS);
-  if (c >= a.ncol) return;
-  if (a.cols) {
-    if (c >= *a.ncols) return;
-    c = a.cols[c];
-    if (c < 0 || c >= a.ncol) return;
-  }
-  const int j = a.j0 + c / ni, i = a.i0 + c % ni;
-  const int k = threadIdx.x, km = a.km;
+// One column of the scheme: lane k = level k (km levels active), the
+// column's inputs per lane in `in` (the fields th, pii, den, delz, p, q, qc,
+// qi, qr, qs, qg), its outputs per lane in `out` (th, q, qc, qi, qr, qs, qg;
+// out.done is set on the lanes that produced them).  The surface
+// precipitation of the column is written by lane 0.  L: the wave's LDS for
+// the sedimentation.  Every lane of the wave must enter (the sedimentation
+// uses wave-wide operations).
+enum { CI_TH = 0, CI_PII, CI_DEN, CI_DZ, CI_P, CI_QV, CI_QC, CI_QI, CI_QR, CI_QS, CI_QG, CI_N };
+struct ColVals {
+  float v[CI_N];
+  bool done;
+};
+
+__device__ __forceinline__ void wsm6_column(SedLds& L, const Dims& D, const Wsm6& a, int j, int i, int k,
+                                            const ColVals& in, ColVals& out) {
+  const int km = a.km;
   const bool act = k < km;
+  out.done = false;
   const float qmin = F(epsilon), dtcld = a.delt;
   const float cpd = F(Cp), cpv = F(Cpv), xls = F(XLS), xlv0 = F(XLV), xlf0 = F(XLF), denr = F(rhowater);
-  long q = act ? D.ix(k, j, i) : D.ix(0, j, i);
   // column inputs (inactive lanes carry zeros, like the reference's zero-filled tiles)
-  float th = act ? a.th[q] : 0.0f, pii = act ? a.pii[q] : 1.0f;
-  float d = act ? a.den[q] : 0.0f, dz = act ? a.delz[q] : 0.0f, pp = act ? a.p[q] : 0.0f;
-  float qv = act ? a.q[q] : 0.0f;
+  float th = act ? in.v[CI_TH] : 0.0f, pii = act ? in.v[CI_PII] : 1.0f;
+  float d = act ? in.v[CI_DEN] : 0.0f, dz = act ? in.v[CI_DZ] : 0.0f, pp = act ? in.v[CI_P] : 0.0f;
+  float qv = act ? in.v[CI_QV] : 0.0f;
   float t = th * pii;
-  float qci0 = act ? tmax(a.qc[q] + 0.0f, 0.0f) : 0.0f, qci1 = act ? tmax(a.qi[q] + 0.0f, 0.0f) : 0.0f;
-  float qrs0 = act ? tmax(a.qr[q] + 0.0f, 0.0f) : 0.0f, qrs1 = act ? tmax(a.qs[q] + 0.0f, 0.0f) : 0.0f;
-  float qrs2 = act ? tmax(a.qg[q] + 0.0f, 0.0f) : 0.0f;
+  float qci0 = act ? tmax(in.v[CI_QC] + 0.0f, 0.0f) : 0.0f, qci1 = act ? tmax(in.v[CI_QI] + 0.0f, 0.0f) : 0.0f;
+  float qrs0 = act ? tmax(in.v[CI_QR] + 0.0f, 0.0f) : 0.0f, qrs1 = act ? tmax(in.v[CI_QS] + 0.0f, 0.0f) : 0.0f;
+  float qrs2 = act ? tmax(in.v[CI_QG] + 0.0f, 0.0f) : 0.0f;
   float cpm = qv >= qmin ? cpv * qv + (1.0f - qv) * cpd : F(Cp * (1 - epsilon) + Cpv * epsilon);
   float xl = xlv0 - F(xlv1) * (t - F(t0c));
   float denfac = sqrtf(1.0f / d * F(rhoair0));
@@ -644,13 +645,114 @@ __global__ void __launch_bounds__(WAVE) k_wsm6(Dims D, Wsm6 a) {
   t = t + pcond * xl / cpm * dtcld;
   qci0 = qci0 <= qmin ? 0.0f : qci0;
   qci1 = qci1 <= qmin ? 0.0f : qci1;
-  a.q[q] = qvn;
-  a.th[q] = t / pii;
-  a.qc[q] = qci0;
-  a.qi[q] = qci1;
-  a.qr[q] = qrs0;
-  a.qs[q] = qrs1;
-  a.qg[q] = qrs2;
+  out.v[CI_QV] = qvn;
+  out.v[CI_TH] = t / pii;
+  out.v[CI_QC] = qci0;
+  out.v[CI_QI] = qci1;
+  out.v[CI_QR] = qrs0;
+  out.v[CI_QS] = qrs1;
+  out.v[CI_QG] = qrs2;
+  out.done = true;
+}
+
+
+// Global index of the c-th column of a launch (the tile's column order, or
+// the device list); -1 when there is none.
+__device__ inline int wsm6_col_index(const Wsm6& a, int c) {
+  if (c >= a.ncol) return -1;
+  if (a.cols) {
+    if (c >= *a.ncols) return -1;
+    c = a.cols[c];
+    if (c < 0 || c >= a.ncol) return -1;
+  }
+  return c;
+}
+
+// One 64-lane wave per column, reading and writing the fields in place
+// (used when the column has more levels than the staged kernel holds).
+__global__ void __launch_bounds__(WAVE) k_wsm6(Dims D, Wsm6 a) {
+  __shared__ SedLds L;
+  // XCD-aware column order: workgroups are dealt to the 8 XCDs round-robin,
+  // so XCD x takes the contiguous run [x * per, (x + 1) * per) of columns and
+  // the columns that share a cache line are served by one L2.
+  const int ni = a.i1 - a.i0;
+  const int c = wsm6_col_index(a, (int)(blockIdx.x % PW_XCDS) * a.per + (int)(blockIdx.x / PW_XCDS));
+  if (c < 0) return;
+  const int j = a.j0 + c / ni, i = a.i0 + c % ni;
+  const int k = threadIdx.x;
+  const bool act = k < a.km;
+  const long q = act ? D.ix(k, j, i) : D.ix(0, j, i);
+  const float* src[CI_N] = {a.th, a.pii, a.den, a.delz, a.p, a.q, a.qc, a.qi, a.qr, a.qs, a.qg};
+  ColVals in, out;
+#pragma unroll
+  for (int f = 0; f < CI_N; ++f) in.v[f] = act ? src[f][q] : 0.0f;
+  wsm6_column(L, D, a, j, i, k, in, out);
+  if (out.done) {
+    a.q[q] = out.v[CI_QV];
+    a.th[q] = out.v[CI_TH];
+    a.qc[q] = out.v[CI_QC];
+    a.qi[q] = out.v[CI_QI];
+    a.qr[q] = out.v[CI_QR];
+    a.qs[q] = out.v[CI_QS];
+    a.qg[q] = out.v[CI_QG];
+  }
+}
+
+// Columns staged through LDS: a 512-thread workgroup owns WSM_CT columns (in
+// the tile's order, so consecutive columns are consecutive i of a row).  The
+// eleven input fields of all its columns are read into LDS with consecutive
+// threads on consecutive columns (each level of a field is a row segment),
+// its eight waves then take its columns in turn (lane = level, inputs and
+// outputs through LDS), and the seven output fields leave the same way.  A
+// wave reading one column straight from the fields would touch a different
+// cache line on every lane of every load.
+#define WSM_CT 32
+#define WSM_WAVES 8
+#define WSM_KMAX 40
+__global__ void __launch_bounds__(WAVE * WSM_WAVES) k_wsm6_tiles(Dims D, Wsm6 a) {
+  __shared__ float tile[CI_N][WSM_KMAX][WSM_CT + 1];
+  __shared__ SedLds Ls[WSM_WAVES];
+  const int ni = a.i1 - a.i0, km = a.km;
+  // tiles dealt to the XCDs in contiguous runs
+  const int tix = (int)(blockIdx.x % PW_XCDS) * a.per + (int)(blockIdx.x / PW_XCDS);
+  const int c0 = tix * WSM_CT;
+  const int tid = threadIdx.x, wave = tid / WAVE, k = tid % WAVE;
+  float* const fld[CI_N] = {a.th, (float*)a.pii, (float*)a.den, (float*)a.delz, (float*)a.p, a.q, a.qc, a.qi,
+                            a.qr, a.qs, a.qg};
+  // read: element e = (field, level, column) with the column fastest
+  const int per_field = km * WSM_CT;
+  for (int e = tid; e < CI_N * per_field; e += WAVE * WSM_WAVES) {
+    const int f = e / per_field, r = e % per_field, kk = r / WSM_CT, cc = r % WSM_CT;
+    const int c = wsm6_col_index(a, c0 + cc);
+    if (c >= 0) tile[f][kk][cc] = fld[f][D.ix(kk, a.j0 + c / ni, a.i0 + c % ni)];
+  }
+  __syncthreads();
+  for (int cc = wave; cc < WSM_CT; cc += WSM_WAVES) {
+    const int c = wsm6_col_index(a, c0 + cc);
+    if (c < 0) continue;                         // uniform across the wave
+    const bool act = k < km;
+    ColVals in, out;
+#pragma unroll
+    for (int f = 0; f < CI_N; ++f) in.v[f] = act ? tile[f][k][cc] : 0.0f;
+    wsm6_column(Ls[wave], D, a, a.j0 + c / ni, a.i0 + c % ni, k, in, out);
+    if (out.done) {
+      tile[CI_TH][k][cc] = out.v[CI_TH];
+      tile[CI_QV][k][cc] = out.v[CI_QV];
+      tile[CI_QC][k][cc] = out.v[CI_QC];
+      tile[CI_QI][k][cc] = out.v[CI_QI];
+      tile[CI_QR][k][cc] = out.v[CI_QR];
+      tile[CI_QS][k][cc] = out.v[CI_QS];
+      tile[CI_QG][k][cc] = out.v[CI_QG];
+    }
+  }
+  __syncthreads();
+  // write back the seven outputs (th and the six mixing ratios)
+  const int outs[7] = {CI_TH, CI_QV, CI_QC, CI_QI, CI_QR, CI_QS, CI_QG};
+  for (int e = tid; e < 7 * per_field; e += WAVE * WSM_WAVES) {
+    const int f = outs[e / per_field], r = e % per_field, kk = r / WSM_CT, cc = r % WSM_CT;
+    const int c = wsm6_col_index(a, c0 + cc);
+    if (c >= 0) fld[f][D.ix(kk, a.j0 + c / ni, a.i0 + c % ni)] = tile[f][kk][cc];
+  }
 }
 
 static int wsm6_launch(const pywrf_grid_t* g, float* th, float* q, float* qc, float* qr, float* qi, float* qs,
@@ -677,7 +779,14 @@ static int wsm6_launch(const pywrf_grid_t* g, float* th, float* q, float* qc, fl
   a.per = (int)((ncol + PW_XCDS - 1) / PW_XCDS);
   a.cols = cols;
   a.ncols = ncols;
-  hipLaunchKernelGGL(k_wsm6, dim3((unsigned)(a.per * PW_XCDS)), dim3(WAVE), 0, (hipStream_t)s, dims_of(g), a);
+  if (a.km <= WSM_KMAX) {
+    const long tiles = (ncol + WSM_CT - 1) / WSM_CT;
+    a.per = (int)((tiles + PW_XCDS - 1) / PW_XCDS);
+    hipLaunchKernelGGL(k_wsm6_tiles, dim3((unsigned)(a.per * PW_XCDS)), dim3(WAVE * WSM_WAVES), 0, (hipStream_t)s,
+                       dims_of(g), a);
+  } else {
+    hipLaunchKernelGGL(k_wsm6, dim3((unsigned)(a.per * PW_XCDS)), dim3(WAVE), 0, (hipStream_t)s, dims_of(g), a);
+  }
   return (int)hipGetLastError();
 }
 
