@@ -93,6 +93,17 @@ struct SedLds {
   float red[2];
 };
 
+// Ordering point for an exchange through LDS among the lanes of one wave:
+// a wave's LDS operations execute in issue order, so only the compiler must
+// be kept from moving them across this point.  (A workgroup barrier would
+// pair with the other waves' barriers when several waves each work on their
+// own column.)
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ inline float lane_value(float v, int m) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), m));
 }
@@ -117,7 +128,7 @@ __device__ void sediment(SedLds& L, int k, int km, float dz, float den, float de
   const bool anypos = __ballot((q1 > 0.0f) || (two && q2 > 0.0f)) != 0;
   const bool cond = !anynan && anypos;
   L.dz[k] = dz;
-  __syncthreads();
+  wave_sync();
   // heights of the level interfaces, summed bottom-up in order
   float zi = 0.0f;
   if (k <= km) {
@@ -132,9 +143,9 @@ __device__ void sediment(SedLds& L, int k, int km, float dz, float den, float de
   int n = iter == 0 ? 0 : 1;
   while (n <= iter + 1) {
     // arrival-interface fall speeds
-    __syncthreads();
+    wave_sync();
     L.ww[k] = ww;
-    __syncthreads();
+    wave_sync();
     float wi = 0.0f;
     if (k <= km) {
       if (k == 0) wi = L.ww[0] + 0.0f;
@@ -166,9 +177,9 @@ __device__ void sediment(SedLds& L, int k, int km, float dz, float den, float de
       wi = wnew;
     }
     za = zi - wi * dt;
-    __syncthreads();
+    wave_sync();
     L.za[k] = za;
-    __syncthreads();
+    wave_sync();
     if (k < km) dza = L.za[k + 1] - za;
     else if (k == km) dza = zi - za;
     float qr = 0.0f, qr2 = 0.0f;
@@ -204,12 +215,12 @@ __device__ void sediment(SedLds& L, int k, int km, float dz, float den, float de
     if (iter == 0) n = 2;
   }
   // remap onto the fixed grid; the top source cell is dropped
-  __syncthreads();
+  wave_sync();
   L.dza[k] = dza;
   const float qs1 = (k == km - 1) ? 0.0f : qa, qs2 = (k == km - 1) ? 0.0f : qa2;
   L.qa[k] = qs1;
   L.qa2[k] = qs2;
-  __syncthreads();
+  wave_sync();
   // departure heights increasing and all values finite: the overlaps of a
   // target cell are a contiguous run of source cells, the rest exactly +0
   const bool bad = (k < km - 1 && !(L.za[k + 1] > za)) || !isfinite(za) || !isfinite(qs1) || !isfinite(qs2);
@@ -251,7 +262,7 @@ __device__ void sediment(SedLds& L, int k, int km, float dz, float den, float de
     prec1 = cond ? p1 : 0.0f;
     prec2 = cond ? p2 : 0.0f;
   }
-  __syncthreads();
+  wave_sync();
 }
 
 struct Wsm6 {
