@@ -679,8 +679,12 @@ __device__ inline int wsm6_col_index(const Wsm6& a, int c) {
   return c;
 }
 
-// One 64-lane wave per column, reading and writing the fields in place
-// (used when the column has more levels than the staged kernel holds).
+// One 64-lane wave per column (lane = level), reading and writing the fields
+// in place.  Staging the eleven inputs of 8 or 16 columns through LDS with
+// row-contiguous loads was measured slower at 240 x 240 x 41 (420 and 504 us
+// against 283 us per call): the scheme is bound by its per-column arithmetic
+// and the serial sedimentation loops, not by the strided loads, and fewer
+// resident columns per CU hide less of that latency.
 __global__ void __launch_bounds__(WAVE) k_wsm6(Dims D, Wsm6 a) {
   __shared__ SedLds L;
   // XCD-aware column order: workgroups are dealt to the 8 XCDs round-robin,
@@ -709,63 +713,6 @@ __global__ void __launch_bounds__(WAVE) k_wsm6(Dims D, Wsm6 a) {
   }
 }
 
-// Columns staged through LDS: a 512-thread workgroup owns WSM_CT columns (in
-// the tile's order, so consecutive columns are consecutive i of a row).  The
-// eleven input fields of all its columns are read into LDS with consecutive
-// threads on consecutive columns (each level of a field is a row segment),
-// its eight waves then take its columns in turn (lane = level, inputs and
-// outputs through LDS), and the seven output fields leave the same way.  A
-// wave reading one column straight from the fields would touch a different
-// cache line on every lane of every load.
-#define WSM_CT 32
-#define WSM_WAVES 8
-#define WSM_KMAX 40
-__global__ void __launch_bounds__(WAVE * WSM_WAVES) k_wsm6_tiles(Dims D, Wsm6 a) {
-  __shared__ float tile[CI_N][WSM_KMAX][WSM_CT + 1];
-  __shared__ SedLds Ls[WSM_WAVES];
-  const int ni = a.i1 - a.i0, km = a.km;
-  // tiles dealt to the XCDs in contiguous runs
-  const int tix = (int)(blockIdx.x % PW_XCDS) * a.per + (int)(blockIdx.x / PW_XCDS);
-  const int c0 = tix * WSM_CT;
-  const int tid = threadIdx.x, wave = tid / WAVE, k = tid % WAVE;
-  float* const fld[CI_N] = {a.th, (float*)a.pii, (float*)a.den, (float*)a.delz, (float*)a.p, a.q, a.qc, a.qi,
-                            a.qr, a.qs, a.qg};
-  // read: element e = (field, level, column) with the column fastest
-  const int per_field = km * WSM_CT;
-  for (int e = tid; e < CI_N * per_field; e += WAVE * WSM_WAVES) {
-    const int f = e / per_field, r = e % per_field, kk = r / WSM_CT, cc = r % WSM_CT;
-    const int c = wsm6_col_index(a, c0 + cc);
-    if (c >= 0) tile[f][kk][cc] = fld[f][D.ix(kk, a.j0 + c / ni, a.i0 + c % ni)];
-  }
-  __syncthreads();
-  for (int cc = wave; cc < WSM_CT; cc += WSM_WAVES) {
-    const int c = wsm6_col_index(a, c0 + cc);
-    if (c < 0) continue;                         // uniform across the wave
-    const bool act = k < km;
-    ColVals in, out;
-#pragma unroll
-    for (int f = 0; f < CI_N; ++f) in.v[f] = act ? tile[f][k][cc] : 0.0f;
-    wsm6_column(Ls[wave], D, a, a.j0 + c / ni, a.i0 + c % ni, k, in, out);
-    if (out.done) {
-      tile[CI_TH][k][cc] = out.v[CI_TH];
-      tile[CI_QV][k][cc] = out.v[CI_QV];
-      tile[CI_QC][k][cc] = out.v[CI_QC];
-      tile[CI_QI][k][cc] = out.v[CI_QI];
-      tile[CI_QR][k][cc] = out.v[CI_QR];
-      tile[CI_QS][k][cc] = out.v[CI_QS];
-      tile[CI_QG][k][cc] = out.v[CI_QG];
-    }
-  }
-  __syncthreads();
-  // write back the seven outputs (th and the six mixing ratios)
-  const int outs[7] = {CI_TH, CI_QV, CI_QC, CI_QI, CI_QR, CI_QS, CI_QG};
-  for (int e = tid; e < 7 * per_field; e += WAVE * WSM_WAVES) {
-    const int f = outs[e / per_field], r = e % per_field, kk = r / WSM_CT, cc = r % WSM_CT;
-    const int c = wsm6_col_index(a, c0 + cc);
-    if (c >= 0) fld[f][D.ix(kk, a.j0 + c / ni, a.i0 + c % ni)] = tile[f][kk][cc];
-  }
-}
-
 static int wsm6_launch(const pywrf_grid_t* g, float* th, float* q, float* qc, float* qr, float* qi, float* qs,
                        float* qg, const float* den, const float* pii, const float* p, const float* delz, double delt,
                        float* rain, float* rainncv, float* snow, float* snowncv, float* sr, float* graupel,
@@ -790,14 +737,7 @@ static int wsm6_launch(const pywrf_grid_t* g, float* th, float* q, float* qc, fl
   a.per = (int)((ncol + PW_XCDS - 1) / PW_XCDS);
   a.cols = cols;
   a.ncols = ncols;
-  if (a.km <= WSM_KMAX) {
-    const long tiles = (ncol + WSM_CT - 1) / WSM_CT;
-    a.per = (int)((tiles + PW_XCDS - 1) / PW_XCDS);
-    hipLaunchKernelGGL(k_wsm6_tiles, dim3((unsigned)(a.per * PW_XCDS)), dim3(WAVE * WSM_WAVES), 0, (hipStream_t)s,
-                       dims_of(g), a);
-  } else {
-    hipLaunchKernelGGL(k_wsm6, dim3((unsigned)(a.per * PW_XCDS)), dim3(WAVE), 0, (hipStream_t)s, dims_of(g), a);
-  }
+  hipLaunchKernelGGL(k_wsm6, dim3((unsigned)(a.per * PW_XCDS)), dim3(WAVE), 0, (hipStream_t)s, dims_of(g), a);
   return (int)hipGetLastError();
 }
 
