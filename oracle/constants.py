@@ -3,8 +3,10 @@
 TEST INFRASTRUCTURE ONLY.  These values are data that the arithmetic of the
 reference depends on, transcribed from pywrf/config_params.py:29-253 (flags
 29-49, timing 55-65, species 101-112, diffusion 114-118, physics 120-162,
-WSM6 164-253).  The product keeps its own copy (pywrf_amd/constants.py and the
-HIP header); tests/test_constants.py checks the two agree.
+WSM6 164-253).  The product keeps its own copy in the HIP header
+pywrf_amd/csrc/consts.h, read back through the library's pywrf_constant
+entry point; tests/test_abi.py::test_constants_agree_with_oracle checks that
+the two agree.
 """
 
 # namelist-style flags (config_params.py:29-49)
