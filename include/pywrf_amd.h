@@ -598,6 +598,43 @@ int pywrf_wsm6_surrogate_noalloc(const pywrf_grid_t* g, const pywrf_surrogate_t*
 /* The limits of pywrf_wsm6_surrogate_noalloc: columns per call, bytes of its array. */
 int pywrf_wsm6_surrogate_noalloc_limits(int* max_columns, long long* scratch_bytes);
 
+/* ---------------------------------------------------------------- adjoints
+ * Transposes of linear parts of the step, for gradients through it.  The
+ * reference has no hand-written adjoints (it relies on torch autograd, and
+ * runs its step under no_grad, solver.py:456, 710, 1748); each entry point
+ * names the reference function whose linear map it transposes. */
+
+/* Adjoint of advect_scalar (wrf_dynamics.py:1841-2028) with respect to the
+ * field, the mass fluxes held fixed: lam_field (+)= A^T lam_tend, where A maps
+ * the field to the tendency pywrf_advect_scalar_fresh writes (zero outside
+ * the advection box).  accumulate 0: lam_field is overwritten on the whole
+ * array; 1: added to. */
+int pywrf_advect_scalar_adjoint(const pywrf_grid_t* g, const float* lam_tend, const float* ru, const float* rv,
+                                const float* rom, const float* msftx, const float* fzm, const float* fzp, double rdx,
+                                double rdy, const float* rdzw, float* lam_field, int accumulate, pywrf_stream_t s);
+/* Adjoint of one species of rk_update_scalar (wrf_dynamics.py:4055-4119):
+ * given lam_new, the sensitivity of scalar_2 after the update, adds the
+ * sensitivity of scalar_1 to lam_n, and writes that of advect_tend (lam_adv),
+ * of sc_tend (lam_sc; may be null) and of scalar_2 before the update
+ * (lam_prev: lam_new where the update leaves scalar_2 alone, zero where it
+ * overwrites it). */
+int pywrf_rk_update_scalar_adjoint(const pywrf_grid_t* g, const float* lam_new, const float* msfty,
+                                   const float* mu_old, const float* mu_new, const float* mu_base, double dt,
+                                   int spec_zone, float* lam_n, float* lam_adv, float* lam_sc, float* lam_prev,
+                                   pywrf_stream_t s);
+/* The column tridiagonal solve of advance_w (wrf_dynamics.py:3758-3765), in
+ * place on w over advance_w's columns, levels 1..kte-1 (level 0 is the lower
+ * boundary value): the same operations in the same order. */
+int pywrf_tridiag_solve(const pywrf_grid_t* g, const float* a, const float* alpha, const float* gamma, float* w,
+                        pywrf_stream_t s);
+/* Its adjoint: r is the right-hand side the solve was given, x its solution,
+ * lam the sensitivity of x, replaced in place by that of r; grad_a,
+ * grad_alpha, grad_gamma receive the sensitivities of the coefficients on
+ * the solved columns; work: a 3-D scratch array of the grid's size. */
+int pywrf_tridiag_solve_adjoint(const pywrf_grid_t* g, const float* a, const float* alpha, const float* gamma,
+                                const float* r, const float* x, float* lam, float* grad_a, float* grad_alpha,
+                                float* grad_gamma, float* work, pywrf_stream_t s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,241 @@
+"""Gradients through parts of the step: hand-written adjoint kernels behind
+torch.autograd.Function.
+
+The reference claims differentiability through torch autograd (README.md:37-40)
+but runs its step under no_grad (solver.py:456, 710, 1748).  Here gradients
+come from the library's own adjoint kernels (csrc/adjoint.hip), each the
+transpose of a linear part of the step:
+
+* AdvectScalar -- advect_scalar (wrf_dynamics.py:1841-2028) as a function of
+  the field, the mass fluxes held fixed.  Backward: pywrf_advect_scalar_adjoint.
+* TridiagSolve -- the column tridiagonal solve of advance_w
+  (wrf_dynamics.py:3758-3765) as a function of its coefficients a, alpha,
+  gamma and right-hand side.  Backward: pywrf_tridiag_solve_adjoint
+  (the transposed Thomas sweeps).
+* TracerStep -- one RK3 step of the scalar transport of a tracer
+  (rk_scalar_tend's advection, wrf_dynamics.py:3982-4051, and
+  rk_update_scalar, 4055-4119, for the three stages of solver.py:1548-1636)
+  with the step's mass fluxes and column masses recorded from a model run
+  (record_transport).  The map is linear in the tracer, so its backward needs
+  no stored tracer states: per stage in reverse, pywrf_rk_update_scalar_adjoint
+  then pywrf_advect_scalar_adjoint.
+* SurrogateNet -- the WSM6 column surrogate (pywrf_amd.surrogate) as a
+  function of its standardised inputs: the backward products on the matrix
+  cores (the same GEMM kernel as training, the ReLU derivative in its
+  epilogue).
+
+    from pywrf_amd import adjoint
+    rec = adjoint.record_transport(state, ops, cfg, nsteps=6)
+    q = q0 + bias                                  # bias.requires_grad_()
+    for step in rec:
+        q = adjoint.TracerStep.apply(q, step)
+    loss = ((q - target) ** 2).mean(); loss.backward()
+"""
+import torch
+
+from ._lib import call, grid, ptr, stream
+from . import surrogate as _sg
+
+
+def _idx18(g):
+    return (g.ids, g.ide, g.jds, g.jde, g.kds, g.kde, g.ims, g.ime, g.jms, g.jme, g.kms, g.kme,
+            g.its, g.ite, g.jts, g.jte, g.kts, g.kte)
+
+
+def _f32(t):
+    if not (t.is_cuda and t.dtype == torch.float32):
+        raise TypeError("expected a float32 CUDA tensor")
+    return t.contiguous()
+
+
+# ----------------------------------------------------------------------- advection
+def advect_scalar_tendency(field, ru, rv, rom, msftx, fzm, fzp, rdx, rdy, rdzw, idx):
+    """The tendency advect_scalar adds to a zeroed array (HIP)."""
+    g = grid(field.shape, idx)
+    tend = torch.empty_like(field)
+    call("pywrf_advect_scalar_fresh", g, ptr(field), ptr(tend), ptr(ru), ptr(rv), ptr(rom), ptr(msftx), ptr(fzm),
+         ptr(fzp), float(rdx), float(rdy), ptr(rdzw), stream())
+    return tend
+
+
+def advect_scalar_adjoint(lam_tend, ru, rv, rom, msftx, fzm, fzp, rdx, rdy, rdzw, idx, out=None):
+    """A^T lam_tend for the advection map A of advect_scalar_tendency; added to
+    `out` when given, else a fresh array."""
+    g = grid(lam_tend.shape, idx)
+    acc = 1 if out is not None else 0
+    if out is None:
+        out = torch.empty_like(lam_tend)
+    call("pywrf_advect_scalar_adjoint", g, ptr(lam_tend), ptr(ru), ptr(rv), ptr(rom), ptr(msftx), ptr(fzm), ptr(fzp),
+         float(rdx), float(rdy), ptr(rdzw), ptr(out), acc, stream())
+    return out
+
+
+class AdvectScalar(torch.autograd.Function):
+    """tendency = advect_scalar(field) with fixed fluxes; gradient w.r.t. field."""
+
+    @staticmethod
+    def forward(ctx, field, ru, rv, rom, msftx, fzm, fzp, rdx, rdy, rdzw, idx):
+        field = _f32(field)
+        ctx.args = (ru, rv, rom, msftx, fzm, fzp, rdx, rdy, rdzw, tuple(idx))
+        return advect_scalar_tendency(field, ru, rv, rom, msftx, fzm, fzp, rdx, rdy, rdzw, idx)
+
+    @staticmethod
+    def backward(ctx, lam):
+        g = advect_scalar_adjoint(_f32(lam), *ctx.args)
+        return (g,) + (None,) * 10
+
+
+# ----------------------------------------------------------------------- tridiagonal solve
+def tridiag_solve(a, alpha, gamma, r, idx):
+    """x with the column solve of advance_w applied to r (HIP; r unchanged)."""
+    x = r.clone()
+    g = grid(r.shape, idx)
+    call("pywrf_tridiag_solve", g, ptr(a), ptr(alpha), ptr(gamma), ptr(x), stream())
+    return x
+
+
+class TridiagSolve(torch.autograd.Function):
+    """x = solve(a, alpha, gamma; r) over advance_w's columns (levels
+    1..kte-1; other points pass r through); gradients w.r.t. all four."""
+
+    @staticmethod
+    def forward(ctx, a, alpha, gamma, r, idx):
+        a, alpha, gamma, r = (_f32(t) for t in (a, alpha, gamma, r))
+        x = tridiag_solve(a, alpha, gamma, r, idx)
+        ctx.save_for_backward(a, alpha, gamma, r, x)
+        ctx.idx = tuple(idx)
+        return x
+
+    @staticmethod
+    def backward(ctx, lam):
+        a, alpha, gamma, r, x = ctx.saved_tensors
+        g = grid(r.shape, ctx.idx)
+        lr = _f32(lam).clone()
+        ga, galpha, ggamma = (torch.zeros_like(r) for _ in range(3))
+        work = torch.empty_like(r)
+        call("pywrf_tridiag_solve_adjoint", g, ptr(a), ptr(alpha), ptr(gamma), ptr(r), ptr(x), ptr(lr), ptr(ga),
+             ptr(galpha), ptr(ggamma), ptr(work), stream())
+        return ga, galpha, ggamma, lr, None
+
+
+# ----------------------------------------------------------------------- tracer transport
+class _TransportRecorder:
+    """The step's functions, recording for each RK stage the arguments the
+    scalar transport of that stage uses (the mass fluxes ru_m, rv_m, ww_m, the
+    column masses mu_1, mu and mub, the map factors, the stage's dt)."""
+
+    def __init__(self, ops):
+        self._ops = ops
+        self.stages = []
+        self._pending = None
+
+    def __getattr__(self, name):
+        return getattr(self._ops, name)
+
+    def rk_scalar_tend_pair(self, moist_args, scalar_args):
+        a = moist_args
+        # (scs, sce, tenddec, rk_step, dt, ru, rv, ww, mut, mub, mu_old, c1h, c2h, alt, scalar_old, scalar,
+        #  scalar_tends, advect_tend, h_tendency, z_tendency, RQVFTEN, base, moist_step, fnm, fnp,
+        #  msfux, msfuy, msfvx, msfvx_inv, msfvy, msftx, msfty, rdx, rdy, rdn, rdnw, ...)
+        self._pending = dict(ru=a[5].clone(), rv=a[6].clone(), ww=a[7].clone(), fnm=a[23], fnp=a[24],
+                             msftx=a[30], rdx=float(a[32]), rdy=float(a[33]), rdnw=a[35], idx=tuple(a[-18:]))
+        return self._ops.rk_scalar_tend_pair(moist_args, scalar_args)
+
+    def rk_update_scalar_pair(self, moist_args, scalar_args):
+        a = moist_args
+        # (scs, sce, scalar_1, scalar_2, sc_tend, advect_tend, h_tendency, z_tendency, msftx, msfty, c1, c2,
+        #  mu_old, mu_new, mu_base, rk_step, dt, spec_zone, tenddec, *idx18)
+        st = dict(self._pending, msfty=a[9], mu_old=a[12].clone(), mu_new=a[13].clone(), mu_base=a[14],
+                  rk_step=int(a[15]), dt=float(a[16]), spec_zone=int(a[17]))
+        self.stages.append(st)
+        return self._ops.rk_update_scalar_pair(moist_args, scalar_args)
+
+
+def record_transport(s, ops, cfg, nsteps):
+    """Run nsteps RK3 steps of state s with ops and return, per step, the list
+    of its three transport stages (see _TransportRecorder)."""
+    from .step import rk3_step
+    rec = _TransportRecorder(ops)
+    steps = []
+    for _ in range(nsteps):
+        rec.stages = []
+        rk3_step(s, rec, cfg)
+        if len(rec.stages) != cfg.rk_order:
+            raise RuntimeError("expected %d transport stages, recorded %d" % (cfg.rk_order, len(rec.stages)))
+        steps.append(rec.stages)
+    return steps
+
+
+def tracer_step(q, stages):
+    """One RK3 step of the transport of tracer q (3-D) with recorded stages
+    (HIP): per stage the advection tendency of the current tracer, then
+    rk_update_scalar from the step's starting tracer."""
+    g = grid(q.shape, stages[0]["idx"])
+    q1 = q.clone()
+    q2 = q.clone()
+    sc = torch.zeros_like(q)
+    for st in stages:
+        t = advect_scalar_tendency(q2, st["ru"], st["rv"], st["ww"], st["msftx"], st["fnm"], st["fnp"], st["rdx"],
+                                   st["rdy"], st["rdnw"], st["idx"])
+        call("pywrf_rk_update_scalar", g, 0, 0, ptr(q1), ptr(q2), ptr(sc), ptr(t), ptr(st["msfty"]),
+             ptr(st["mu_old"]), ptr(st["mu_new"]), ptr(st["mu_base"]), st["rk_step"], st["dt"], st["spec_zone"],
+             stream())
+    return q2
+
+
+def tracer_step_adjoint(lam, stages):
+    """The sensitivity of the step's starting tracer given that of its result."""
+    g = grid(lam.shape, stages[0]["idx"])
+    lam = lam.clone()
+    lam_n = torch.zeros_like(lam)
+    lam_adv = torch.empty_like(lam)
+    for st in reversed(stages):
+        prev = torch.empty_like(lam)
+        call("pywrf_rk_update_scalar_adjoint", g, ptr(lam), ptr(st["msfty"]), ptr(st["mu_old"]), ptr(st["mu_new"]),
+             ptr(st["mu_base"]), st["dt"], st["spec_zone"], ptr(lam_n), ptr(lam_adv), None, ptr(prev), stream())
+        advect_scalar_adjoint(lam_adv, st["ru"], st["rv"], st["ww"], st["msftx"], st["fnm"], st["fnp"], st["rdx"],
+                              st["rdy"], st["rdnw"], st["idx"], out=prev)
+        lam = prev
+    return lam_n + lam
+
+
+class TracerStep(torch.autograd.Function):
+    """q_next = one RK3 transport step of q (recorded stages)."""
+
+    @staticmethod
+    def forward(ctx, q, stages):
+        ctx.stages = stages
+        return tracer_step(_f32(q), stages)
+
+    @staticmethod
+    def backward(ctx, lam):
+        return tracer_step_adjoint(_f32(lam), ctx.stages), None
+
+
+# ----------------------------------------------------------------------- surrogate
+class SurrogateNet(torch.autograd.Function):
+    """Standardised tendencies Y [M x 287] of standardised column inputs
+    X [M x 369] (FP32; BF16 on the matrix cores, FP32 accumulation), with the
+    gradient w.r.t. X.  M must be a multiple of 32."""
+
+    @staticmethod
+    def forward(ctx, X, net):
+        M = X.shape[0]
+        ws = _sg.Workspace(M, device=X.device)
+        Xb = torch.zeros(M, _sg.KIN, dtype=torch.bfloat16, device=X.device)
+        Xb[:, :_sg.FIN] = X.to(torch.bfloat16)
+        Y = _sg.forward(net, Xb, ws)
+        ctx.net, ctx.ws = net, ws
+        return Y[:, :_sg.FOUT].clone()
+
+    @staticmethod
+    def backward(ctx, dY):
+        net, ws = ctx.net, ctx.ws
+        M = ws.M
+        dYb = torch.zeros(M, _sg.NO, dtype=torch.bfloat16, device=dY.device)
+        dYb[:, :_sg.FOUT] = dY.to(torch.bfloat16)
+        _sg.gemm(2, dYb, net.Wt[2], ws.dH2, H=ws.H2)        # (dY W3) [H2 > 0]
+        _sg.gemm(2, ws.dH2, net.Wt[1], ws.dH1, H=ws.H1)     # (dH2 W2) [H1 > 0]
+        dX = torch.empty(M, _sg.KIN, dtype=torch.float32, device=dY.device)
+        _sg.gemm(0, ws.dH1, net.Wt[0], dX)                  # dH1 W1
+        return dX[:, :_sg.FIN].clone(), None

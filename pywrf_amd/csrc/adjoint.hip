@@ -1,0 +1,304 @@
+// Adjoints (transposes) of linear parts of the step, for gradients through it.
+//
+// * advect_scalar (wrf_dynamics.py:1841-2028): with the mass fluxes fixed the
+//   tendency is linear in the field, T = A q, the face value of each face
+//   being vel * sum_o c_o(face, sign) q(face + o).  k_advect_scalar_adj forms
+//   A^T lam by gathering, for every point, the faces whose stencils hold it
+//   (six in each direction), each face's sensitivity being formed from the
+//   adjoint tendencies of the two points it separates.  No atomics; every
+//   output point is written by one thread.
+// * rk_update_scalar (wrf_dynamics.py:4055-4119): pointwise.
+// * The column tridiagonal solve of advance_w (wrf_dynamics.py:3758-3765):
+//   forward elimination w[k] = (w[k] - a[k] w[k-1]) alpha[k], k = 1..K, then
+//   back substitution w[k] = w[k] - gamma[k] w[k+1], k = K-1..1.  The
+//   adjoint runs the transposed sweeps in the opposite order (the back
+//   substitution's transpose upwards, the elimination's downwards) and also
+//   gives the sensitivities to a, alpha and gamma.
+#include "common.h"
+#include "consts.h"
+#include "diff.h"
+
+int pw_check_grid(const pywrf_grid_t* g, int r);
+#define CHECK(g, r) do { int _e = pw_check_grid((g), (r)); if (_e) return _e; } while (0)
+
+// ------------------------------------------------------------------ advect_scalar^T
+struct AdjS {
+  const float *lam, *ru, *rv, *rom, *msftx, *fzm, *fzp, *rdzw;
+  float* out;
+  float rdx, rdy;
+  int ids, ide, jds, jde, ktf, accumulate;
+  Box badv, by, bx;   // the forward's boxes: all terms, the y-term, the x-term
+};
+
+// d(face value) / d(q at face + o), o = -3..2, without the velocity factor:
+// the 5th-order face (flux5) with upwind sign s = +-1.
+__device__ inline float c5(int o, float s) {
+  switch (o) {
+    case -3: return (1.0f + s) / 60.0f;
+    case -2: return (-8.0f - 5.0f * s) / 60.0f;
+    case -1: return (37.0f + 10.0f * s) / 60.0f;
+    case 0: return (37.0f - 10.0f * s) / 60.0f;
+    case 1: return (-8.0f + 5.0f * s) / 60.0f;
+    case 2: return (1.0f - s) / 60.0f;
+  }
+  return 0.0f;
+}
+// the 3rd-order face (flux3), o = -2..1
+__device__ inline float c3(int o, float s) {
+  switch (o) {
+    case -2: return (-1.0f - s) / 12.0f;
+    case -1: return (7.0f + 3.0f * s) / 12.0f;
+    case 0: return (7.0f - 3.0f * s) / 12.0f;
+    case 1: return (-1.0f + s) / 12.0f;
+  }
+  return 0.0f;
+}
+
+// Coefficient of q(face + o) in the value of the horizontal face f of a row
+// or column whose domain is [d0, d1) (y: jds, jde; x: ids, ide): the
+// reference's face rules (5th order inside, 2nd order next to the edge, 3rd
+// order one further in; zero flux elsewhere).  s_lo / s_hi: the upwind signs
+// the 3rd-order faces at d0 + 2 and d1 - 3 use.
+__device__ inline float hface_coef(int f, int o, int d0, int d1, float s, float s_lo, float s_hi) {
+  if (f >= d0 + 3 && f < d1 - 3) return c5(o, s);
+  if (f == d0 + 1 || f == d1 - 2) return (o == 0 || o == -1) ? 0.5f : 0.0f;
+  if (f == d0 + 2) return c3(o, s_lo);
+  if (f == d1 - 3) return c3(o, s_hi);
+  return 0.0f;
+}
+
+__global__ void k_advect_scalar_adj(Box all, Dims D, AdjS a) {
+  int k, j, i;
+  if (!box_point(all, k, j, i)) return;
+  float acc = 0.0f;
+  // y-faces jf = j - o hold q(k, j, i) at offset o
+  if (k >= a.badv.k0 && k < a.badv.k1 && i >= a.badv.i0 && i < a.badv.i1) {
+#pragma unroll
+    for (int o = -3; o <= 2; ++o) {
+      const int jf = j - o;
+      if (jf < 1 || jf >= D.ny) continue;
+      // sensitivity of the face value: the points jf (face on its low side)
+      // and jf - 1 (face on its high side) that take the y-term
+      float gF = 0.0f;
+      if (a.by.has(k, jf, i)) gF += (AT2(a.msftx, jf, i) * a.rdy) * AT(a.lam, k, jf, i);
+      if (a.by.has(k, jf - 1, i)) gF -= (AT2(a.msftx, jf - 1, i) * a.rdy) * AT(a.lam, k, jf - 1, i);
+      if (gF == 0.0f) continue;
+      const float vel = AT(a.rv, k, jf, i);
+      const float s = sgn1(vel);
+      acc += gF * vel * hface_coef(jf, o, a.jds, a.jde, s, s, s);
+    }
+  }
+  // x-faces f = i - o
+  if (k >= a.badv.k0 && k < a.badv.k1 && j >= a.badv.j0 && j < a.badv.j1) {
+#pragma unroll
+    for (int o = -3; o <= 2; ++o) {
+      const int f = i - o;
+      if (f < 1 || f >= D.nx) continue;
+      float gF = 0.0f;
+      if (a.bx.has(k, j, f)) gF += (AT2(a.msftx, j, f) * a.rdx) * AT(a.lam, k, j, f);
+      if (a.bx.has(k, j, f - 1)) gF -= (AT2(a.msftx, j, f - 1) * a.rdx) * AT(a.lam, k, j, f - 1);
+      if (gF == 0.0f) continue;
+      const float vel = AT(a.ru, k, j, f);
+      const float s = sgn1(vel);
+      // the face at ide - 3 takes its upwind sign from ru at ide - 2
+      const float s_hi = (f == a.ide - 3 && f + 1 < D.nx) ? sgn1(AT(a.ru, k, j, f + 1)) : s;
+      acc += gF * vel * hface_coef(f, o, a.ids, a.ide, s, s, s_hi);
+    }
+  }
+  // z-faces kf = k - o (upwind sign from -rom)
+  if (j >= a.badv.j0 && j < a.badv.j1 && i >= a.badv.i0 && i < a.badv.i1) {
+#pragma unroll
+    for (int o = -3; o <= 2; ++o) {
+      const int kf = k - o;
+      if (kf < 1 || kf >= a.ktf || kf >= D.nz) continue;
+      float gF = 0.0f;
+      if (kf < a.badv.k1) gF += a.rdzw[kf] * AT(a.lam, kf, j, i);
+      if (kf - 1 >= a.badv.k0) gF -= a.rdzw[kf - 1] * AT(a.lam, kf - 1, j, i);
+      if (gF == 0.0f) continue;
+      const float vel = AT(a.rom, kf, j, i);
+      const float s = sgn1(-vel);
+      float c;
+      if (kf >= 3 && kf < a.ktf - 2) c = c5(o, s);
+      else if (kf == 1 || kf == a.ktf - 1) c = o == 0 ? a.fzm[kf] : (o == -1 ? a.fzp[kf] : 0.0f);
+      else c = c3(o, s);   // kf == 2 or ktf - 2
+      acc += gF * vel * c;
+    }
+  }
+  // (each face sensitivity above already carries the sign with which the
+  // forward subtracts the face difference from the tendency)
+  float* out = &AT(a.out, k, j, i);
+  *out = a.accumulate ? *out + acc : acc;
+}
+
+extern "C" int pywrf_advect_scalar_adjoint(const pywrf_grid_t* g, const float* lam_tend, const float* ru,
+                                           const float* rv, const float* rom, const float* msftx, const float* fzm,
+                                           const float* fzp, double rdx, double rdy, const float* rdzw,
+                                           float* lam_field, int accumulate, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
+  CHECK(g, 4);
+  AdjS a;
+  a.lam = lam_tend; a.ru = ru; a.rv = rv; a.rom = rom; a.msftx = msftx; a.fzm = fzm; a.fzp = fzp; a.rdzw = rdzw;
+  a.out = lam_field; a.rdx = F(rdx); a.rdy = F(rdy); a.accumulate = accumulate;
+  a.ids = g->ids; a.ide = g->ide; a.jds = g->jds; a.jde = g->jde; a.ktf = imin(g->kte, g->kde - 1);
+  if (g->kts != 0 || a.ktf + 1 > g->nz || g->kte > g->nz) return PYWRF_E_RANGE;
+  const int i0 = g->its, i1 = imin(g->ite, g->ide - 1), j0 = g->jts, j1 = imin(g->jte, g->jde - 1);
+  a.badv = mkbox(g->kts, a.ktf, j0, j1, i0, i1);
+  a.by = mkbox(g->kts, a.ktf, imax(g->jts, g->jds + 1), imin(g->jte, g->jde - 2), i0, i1);
+  a.bx = mkbox(g->kts, a.ktf, j0, j1, imax(g->ids + 1, g->its), imin(g->ide - 2, g->ite));
+  Box all = whole(g);
+  PW_LAUNCH_BOX(k_advect_scalar_adj, all, s, dims_of(g), a);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ rk_update_scalar^T
+struct AdjU {
+  const float *lam, *msfty, *mu_old, *mu_new, *mu_base;
+  float *lam_n, *lam_adv, *lam_sc, *lam_prev;
+  float dt;
+  Box bf, bi, bs;   // the updated box, the advection box (spec zone excluded), the sc_tend box
+};
+
+__global__ void k_rk_update_scalar_adj(Box all, Dims D, AdjU a) {
+  int k, j, i;
+  if (!box_point(all, k, j, i)) return;
+  const float l = AT(a.lam, k, j, i);
+  if (a.bf.has(k, j, i)) {
+    const float muold = AT2(a.mu_old, j, i) + AT2(a.mu_base, j, i);
+    const float munew = AT2(a.mu_new, j, i) + AT2(a.mu_base, j, i);
+    const float lt = (a.dt * l) / munew;        // d q_new / d tendency
+    AT(a.lam_n, k, j, i) += (muold * l) / munew;
+    AT(a.lam_adv, k, j, i) = a.bi.has(k, j, i) ? lt * AT2(a.msfty, j, i) : 0.0f;
+    if (a.lam_sc) AT(a.lam_sc, k, j, i) = a.bs.has(k, j, i) ? lt : 0.0f;
+    AT(a.lam_prev, k, j, i) = 0.0f;
+  } else {
+    AT(a.lam_adv, k, j, i) = 0.0f;
+    if (a.lam_sc) AT(a.lam_sc, k, j, i) = 0.0f;
+    AT(a.lam_prev, k, j, i) = l;                // points the update leaves alone
+  }
+}
+
+extern "C" int pywrf_rk_update_scalar_adjoint(const pywrf_grid_t* g, const float* lam_new, const float* msfty,
+                                              const float* mu_old, const float* mu_new, const float* mu_base,
+                                              double dt, int spec_zone, float* lam_n, float* lam_adv, float* lam_sc,
+                                              float* lam_prev, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
+  CHECK(g, 0);
+  if (g->kde - 1 > g->nz || g->jde - 1 > g->ny || g->ide - 1 > g->nx) return PYWRF_E_RANGE;
+  AdjU a;
+  a.lam = lam_new; a.msfty = msfty; a.mu_old = mu_old; a.mu_new = mu_new; a.mu_base = mu_base;
+  a.lam_n = lam_n; a.lam_adv = lam_adv; a.lam_sc = lam_sc; a.lam_prev = lam_prev;
+  a.dt = F(dt);
+  a.bf = mkbox(g->kts, g->kde - 1, g->jts, g->jde - 1, g->its, g->ide - 1);
+  a.bi = mkbox(g->kts, imin(g->kte, g->kde - 1), imax(g->jts, g->jds + spec_zone), imin(g->jte, g->jde - spec_zone - 1),
+               imax(g->its, g->ids + spec_zone), imin(g->ite, g->ide - spec_zone - 1));
+  a.bs = mkbox(g->kts, g->kte - 1, g->jts, imin(g->jte, g->jde - 1), g->its, imin(g->ite, g->ide - 1));
+  Box all = whole(g);
+  PW_LAUNCH_BOX(k_rk_update_scalar_adj, all, s, dims_of(g), a);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ tridiagonal solve and its adjoint
+// One thread per column of advance_w's box; consecutive threads take
+// consecutive i, so every level's access is a contiguous row segment.
+struct Tri {
+  const float *a, *alpha, *gamma;
+  float* w;
+  int K;   // levels 1..K are solved; level 0 holds the lower boundary value
+};
+
+__global__ void k_tridiag_solve(Box b, Dims D, Tri t) {
+  int j, i;
+  if (!box_column(b, j, i)) return;
+  float prev = AT(t.w, 0, j, i);
+  for (int k = 1; k <= t.K; ++k) {
+    const float y = (AT(t.w, k, j, i) - AT(t.a, k, j, i) * prev) * AT(t.alpha, k, j, i);
+    AT(t.w, k, j, i) = y;
+    prev = y;
+  }
+  float next = prev;   // w[K]
+  for (int k = t.K - 1; k >= 1; --k) {
+    const float x = AT(t.w, k, j, i) - AT(t.gamma, k, j, i) * next;
+    AT(t.w, k, j, i) = x;
+    next = x;
+  }
+}
+
+static bool tri_box(const pywrf_grid_t* g, Box& b, int& K) {
+  K = g->kte - 1;
+  b = mkbox(0, K + 1, imax(g->jts, g->jds + 1), imin(g->jte, g->jde - 2), imax(g->its, g->ids + 1),
+            imin(g->ite, g->ide - 2));
+  return g->kts == 0 && K >= 1 && K < g->nz;
+}
+
+extern "C" int pywrf_tridiag_solve(const pywrf_grid_t* g, const float* a, const float* alpha, const float* gamma,
+                                   float* w, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
+  CHECK(g, 1);
+  Box b;
+  Tri t;
+  if (!tri_box(g, b, t.K)) return PYWRF_E_RANGE;
+  t.a = a; t.alpha = alpha; t.gamma = gamma; t.w = w;
+  PW_LAUNCH_COLS(k_tridiag_solve, b, s, dims_of(g), t);
+  PW_RET();
+}
+
+// Adjoint of the solve: given the right-hand side r (the input of the
+// solve), the solution x and the sensitivity lam of x, writes the
+// sensitivities of r (over lam, in place), a, alpha and gamma.  The forward
+// elimination's intermediate y is recomputed from r.
+struct TriAdj {
+  const float *a, *alpha, *gamma, *r, *x;
+  float *lam, *ga, *galpha, *ggamma, *y;
+  int K;
+};
+
+__global__ void k_tridiag_adj(Box b, Dims D, TriAdj t) {
+  int j, i;
+  if (!box_column(b, j, i)) return;
+  // recompute y (the elimination's output) into t.y
+  float prev = AT(t.r, 0, j, i);
+  AT(t.y, 0, j, i) = prev;
+  for (int k = 1; k <= t.K; ++k) {
+    const float y = (AT(t.r, k, j, i) - AT(t.a, k, j, i) * prev) * AT(t.alpha, k, j, i);
+    AT(t.y, k, j, i) = y;
+    prev = y;
+  }
+  // back substitution x[k] = y[k] - gamma[k] x[k+1], run for k = K-1..1:
+  // transposed upwards; lam becomes the sensitivity of y
+  AT(t.ggamma, 0, j, i) = 0.0f;
+  AT(t.ggamma, t.K, j, i) = 0.0f;
+  for (int k = 1; k <= t.K - 1; ++k) {
+    const float lk = AT(t.lam, k, j, i);
+    AT(t.ggamma, k, j, i) = -lk * AT(t.x, k + 1, j, i);
+    AT(t.lam, k + 1, j, i) -= AT(t.gamma, k, j, i) * lk;
+  }
+  // elimination y[k] = (r[k] - a[k] y[k-1]) alpha[k], k = 1..K: transposed
+  // downwards; lam becomes the sensitivity of r
+  AT(t.ga, 0, j, i) = 0.0f;
+  AT(t.galpha, 0, j, i) = 0.0f;
+  for (int k = t.K; k >= 1; --k) {
+    const float ly = AT(t.lam, k, j, i);
+    const float al = AT(t.alpha, k, j, i);
+    const float ym = AT(t.y, k - 1, j, i);
+    const float lr = ly * al;
+    AT(t.galpha, k, j, i) = ly * (AT(t.r, k, j, i) - AT(t.a, k, j, i) * ym);
+    AT(t.ga, k, j, i) = -lr * ym;
+    AT(t.lam, k, j, i) = lr;
+    AT(t.lam, k - 1, j, i) -= AT(t.a, k, j, i) * lr;
+  }
+}
+
+extern "C" int pywrf_tridiag_solve_adjoint(const pywrf_grid_t* g, const float* a, const float* alpha,
+                                           const float* gamma, const float* r, const float* x, float* lam,
+                                           float* grad_a, float* grad_alpha, float* grad_gamma, float* work,
+                                           pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
+  CHECK(g, 1);
+  TriAdj t;
+  Box b;
+  if (!tri_box(g, b, t.K)) return PYWRF_E_RANGE;
+  t.a = a; t.alpha = alpha; t.gamma = gamma; t.r = r; t.x = x; t.lam = lam;
+  t.ga = grad_a; t.galpha = grad_alpha; t.ggamma = grad_gamma; t.y = work;
+  PW_LAUNCH_COLS(k_tridiag_adj, b, s, dims_of(g), t);
+  PW_RET();
+}

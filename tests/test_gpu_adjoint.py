@@ -1,0 +1,242 @@
+"""The hand-written adjoint kernels (pywrf_amd.adjoint) on the device.
+
+Checkers, none of them the adjoint kernels' own formulas:
+* the transpose property <A q, lam> = <q, A^T lam> on random vectors, with A
+  the device's forward advection (FP64 sums);
+* torch autograd through the oracle (the CPU restatement of advect_scalar and
+  rk_update_scalar) in FP64, on the same mass fluxes -- the gradient the
+  reference's own functions give under autograd;
+* FP64 central differences of the oracle's transport, eps = 1e-3, along 20
+  random directions, on the 120 x 120 x 41 workload over 6 RK3 steps;
+* for the tridiagonal solve, the textbook factorisation: the sweeps solve
+  L U x = r with L lower bidiagonal (1/alpha on the diagonal, a below) and U
+  unit upper bidiagonal (gamma above), so x and its gradients are checked
+  against torch.linalg.solve on that matrix, differentiated by autograd.
+* for the surrogate, CPU autograd in FP32 of the same network on the BF16
+  values the device uses.
+Tolerances are stated per test.
+"""
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+N = None
+
+
+def _recorded(config, nsteps):
+    import pywrf_amd
+    from pywrf_amd import adjoint
+    from pywrf_amd.gen import synthetic_state
+    from pywrf_amd.step import RunConfig
+    s = synthetic_state(config, device="cuda")
+    steps = adjoint.record_transport(s, pywrf_amd.ops(), RunConfig(), nsteps)
+    torch.cuda.synchronize()
+    return s, steps
+
+
+def _cpu64(st):
+    return {k: (v.detach().double().cpu() if isinstance(v, torch.Tensor) else v) for k, v in st.items()}
+
+
+def _oracle_advect(q, st):
+    import oracle
+    return oracle.ops().advect_scalar(q, q, torch.zeros_like(q), st["ru"], st["rv"], st["ww"], N, N, N, N, N, N, N, N,
+                                      st["msftx"], N, st["fnm"], st["fnp"], st["rdx"], st["rdy"], st["rdnw"],
+                                      *st["idx"])
+
+
+def _oracle_tracer_step(q, stages):
+    """One RK3 transport step of q by the oracle's functions (CPU)."""
+    import oracle
+    O = oracle.ops()
+    s1 = q.unsqueeze(0).clone()
+    s2 = q.unsqueeze(0).clone()
+    zero = torch.zeros_like(s2)
+    for st in stages:
+        t = _oracle_advect(s2[0], st)
+        O.rk_update_scalar(0, 0, s1, s2, zero, t.unsqueeze(0), N, N, N, st["msfty"], N, N, st["mu_old"],
+                           st["mu_new"], st["mu_base"], st["rk_step"], st["dt"], st["spec_zone"], False, *st["idx"])
+    return s2[0]
+
+
+def _rel(x, y):
+    return float((x - y).abs().max() / y.abs().max())
+
+
+@pytest.mark.parametrize("config", ["config1c", "config4"])
+def test_advection_adjoint_is_the_transpose(config):
+    from pywrf_amd import adjoint
+    s, steps = _recorded(config, 1)
+    gen = torch.Generator(device="cuda").manual_seed(1)
+    for st in steps[0]:
+        q = torch.randn(s.t.shape, device="cuda", generator=gen)
+        lam = torch.randn(s.t.shape, device="cuda", generator=gen)
+        args = (st["ru"], st["rv"], st["ww"], st["msftx"], st["fnm"], st["fnp"], st["rdx"], st["rdy"], st["rdnw"],
+                st["idx"])
+        Aq = adjoint.advect_scalar_tendency(q, *args)
+        ATl = adjoint.advect_scalar_adjoint(lam, *args)
+        lhs = float((Aq.double() * lam.double()).sum())
+        rhs = float((q.double() * ATl.double()).sum())
+        scale = float((Aq.double().abs() * lam.double().abs()).sum())
+        print("<Aq, l> %.9e  <q, A^T l> %.9e" % (lhs, rhs))
+        assert abs(lhs - rhs) <= 1e-6 * scale
+
+
+@pytest.mark.parametrize("config", ["config1c", "config4"])
+def test_advection_adjoint_matches_oracle_autograd(config):
+    from pywrf_amd import adjoint
+    s, steps = _recorded(config, 1)
+    st = steps[0][1]
+    lam = torch.randn(s.t.shape, device="cuda", generator=torch.Generator(device="cuda").manual_seed(2))
+    g = adjoint.advect_scalar_adjoint(lam, st["ru"], st["rv"], st["ww"], st["msftx"], st["fnm"], st["fnp"],
+                                      st["rdx"], st["rdy"], st["rdnw"], st["idx"])
+    st64 = _cpu64(st)
+    q = s.moist[1].detach().double().cpu().requires_grad_()
+    (_oracle_advect(q, st64) * lam.double().cpu()).sum().backward()
+    err = _rel(g.double().cpu(), q.grad)
+    print("A^T lam vs oracle autograd (FP64): %.2e" % err)
+    assert err <= 1e-5
+
+
+def test_tracer_step_forward_equals_oracle():
+    """The device transport step is the oracle's arithmetic: bit for bit in FP32."""
+    from pywrf_amd import adjoint
+    s, steps = _recorded("config1c", 1)
+    q = s.moist[1].clone()
+    mine = adjoint.tracer_step(q, steps[0])
+    st32 = [{k: (v.detach().cpu() if isinstance(v, torch.Tensor) else v) for k, v in st.items()} for st in steps[0]]
+    ref = _oracle_tracer_step(q.cpu(), st32)
+    assert torch.equal(mine.cpu(), ref)
+
+
+def test_tracer_gradient_matches_fp64_differences_and_autograd():
+    """Gradient of J(b) = mean((q_6(q_0 + b) - target)^2) at b = 0 on the
+    120 x 120 x 41 workload over 6 RK3 steps; the target is the same
+    transport of a perturbed start."""
+    from pywrf_amd import adjoint
+    nsteps = 6
+    s, steps = _recorded("config4", nsteps)
+    q0 = s.moist[1].clone()
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    pert = 0.05 * float(q0.abs().max()) * torch.randn(q0.shape, device="cuda", generator=gen)
+    with torch.no_grad():
+        target = q0 + pert
+        for st in steps:
+            target = adjoint.tracer_step(target, st)
+    b = torch.zeros_like(q0, requires_grad=True)
+    q = q0 + b
+    for st in steps:
+        q = adjoint.TracerStep.apply(q, st)
+    loss = ((q - target) ** 2).mean()
+    loss.backward()
+    grad = b.grad.double().cpu()
+
+    st64 = [[_cpu64(st) for st in stp] for stp in steps]
+    q064, tgt64 = q0.double().cpu(), target.double().cpu()
+
+    def J(bias):
+        x = q064 + bias
+        for stp in st64:
+            x = _oracle_tracer_step(x, stp)
+        return ((x - tgt64) ** 2).mean()
+
+    b64 = torch.zeros_like(q064, requires_grad=True)
+    J(b64).backward()
+    err = _rel(grad, b64.grad)
+    print("gradient vs oracle autograd (FP64): %.2e" % err)
+    assert err <= 1e-4
+    g2 = torch.Generator().manual_seed(4)
+    eps = 1e-3
+    worst = 0.0
+    with torch.no_grad():
+        for _ in range(20):
+            d = torch.randn(q064.shape, generator=g2, dtype=torch.float64)
+            d *= float(q064.abs().max()) / float(d.abs().max())
+            fd = float(J(eps * d) - J(-eps * d)) / (2 * eps)
+            ad = float((grad * d).sum())
+            worst = max(worst, abs(ad - fd) / max(abs(fd), 1e-300))
+    print("20 directions: worst |<g,d> - FD| / |FD| = %.2e" % worst)
+    assert worst <= 1e-4
+
+
+def _bidiag_matrix(a, alpha, gamma, K):
+    """M = L U for one batch of columns [..., K+1] (level 0 an identity row)."""
+    n = K + 1
+    shape = a.shape[:-1] + (n, n)
+    L = torch.zeros(shape, dtype=a.dtype)
+    U = torch.zeros(shape, dtype=a.dtype)
+    idx = torch.arange(n)
+    U[..., idx, idx] = 1.0
+    L[..., 0, 0] = 1.0
+    k = torch.arange(1, n)
+    L[..., k, k] = 1.0 / alpha[..., 1:n]
+    L[..., k, k - 1] = a[..., 1:n]
+    k2 = torch.arange(1, K)
+    U[..., k2, k2 + 1] = gamma[..., 1:K]
+    return L @ U
+
+
+def test_tridiag_solve_and_adjoint_against_the_factorisation():
+    from pywrf_amd import adjoint
+    from pywrf_amd.gen import synthetic_state
+    s = synthetic_state("config1c", device="cuda")
+    g = s.grid
+    idx = g.ix18
+    K = g.kte - 1
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    # coefficients of the kind calc_coef_w forms: diagonally dominant
+    a = 0.3 * torch.rand(s.w.shape, device="cuda", generator=gen) - 0.15
+    gamma = 0.3 * torch.rand(s.w.shape, device="cuda", generator=gen) - 0.15
+    alpha = 0.8 + 0.4 * torch.rand(s.w.shape, device="cuda", generator=gen)
+    r = torch.randn(s.w.shape, device="cuda", generator=gen)
+    ts = [x.clone().requires_grad_() for x in (a, alpha, gamma, r)]
+    x = adjoint.TridiagSolve.apply(*ts, idx)
+    lam = torch.randn(s.w.shape, device="cuda", generator=gen)
+    (x * lam).sum().backward()
+    j0, j1 = max(g.jts, g.jds + 1), min(g.jte, g.jde - 2)
+    i0, i1 = max(g.its, g.ids + 1), min(g.ite, g.ide - 2)
+    col = lambda t: t.detach().double().cpu()[:K + 1, j0:j1, i0:i1].permute(1, 2, 0)
+    c64 = [col(t).requires_grad_() for t in (a, alpha, gamma, r)]
+    M = _bidiag_matrix(c64[0], c64[1], c64[2], K)
+    xr = torch.linalg.solve(M, c64[3].unsqueeze(-1)).squeeze(-1)
+    ex = _rel(col(x), xr.detach())
+    print("solve vs factorisation: %.2e" % ex)
+    assert ex <= 1e-5
+    (xr * col(lam)).sum().backward()
+    for name, t, ref in zip(("a", "alpha", "gamma", "r"), ts, c64):
+        e = _rel(col(t.grad), ref.grad)
+        print("d/d%s: %.2e" % (name, e))
+        assert e <= 1e-4, name
+    # outside the solved columns r passes through, the coefficients get nothing
+    m = torch.ones(s.w.shape, dtype=torch.bool, device="cuda")
+    m[:K + 1, j0:j1, i0:i1] = False
+    assert torch.equal(ts[3].grad[m], lam[m])
+    assert float(ts[0].grad[m].abs().max()) == 0.0
+
+
+def test_surrogate_input_gradient_matches_cpu_autograd():
+    from pywrf_amd import adjoint, surrogate as sg
+    net = sg.MLP(seed=3)
+    M = 256
+    gen = torch.Generator(device="cuda").manual_seed(6)
+    X = torch.randn(M, sg.FIN, device="cuda", generator=gen).requires_grad_()
+    Y = adjoint.SurrogateNet.apply(X, net)
+    dY = torch.randn(M, sg.FOUT, device="cuda", generator=gen)
+    (Y * dY).sum().backward()
+    # CPU FP32 reference on the BF16 values the kernels see (inputs, weights,
+    # hidden activations and output gradient rounded to BF16)
+    bf = lambda t: t.detach().cpu().to(torch.bfloat16).float()
+    W = [bf(w) for w in net.W]
+    b = [x.detach().cpu() for x in net.b]
+    Xc = bf(X).requires_grad_()
+    xp = torch.nn.functional.pad(Xc, (0, sg.KIN - sg.FIN))
+    h1 = torch.relu(xp @ W[0].t() + b[0])
+    h1b = h1 + (bf(h1) - h1).detach()
+    h2 = torch.relu(h1b @ W[1].t() + b[1])
+    h2b = h2 + (bf(h2) - h2).detach()
+    y = (h2b @ W[2].t() + b[2])[:, :sg.FOUT]
+    assert _rel(Y.detach().cpu(), y.detach()) <= 2e-2
+    (y * bf(dY)).sum().backward()
+    err = _rel(X.grad.cpu(), Xc.grad)
+    print("dX vs CPU autograd: %.2e" % err)
+    assert err <= 3e-2
