@@ -108,13 +108,15 @@ __global__ void __launch_bounds__(256) k_sg_pack(Tile T, Fields f, const float* 
   }
 }
 
-// One thread per column of the chunk (consecutive threads take consecutive
-// columns, so every load and store of a level is a row segment of the
-// field), in two passes over the levels: the first forms the updated values
-// only to run the checks and the column water sums, the second -- for a
-// column that passes -- forms them again and stores them.  A failing column
-// is appended to the fallback list.  Yt holds the network's outputs feature-
-// major, [288][ldy].
+// The network's tendencies applied with the checks, on a workgroup of 64
+// columns x SA_LG level groups: lane = column (every load and store of a
+// level is a row segment of the field), wave = a group of SA_LPG consecutive
+// levels.  Each thread forms the updated values of its levels in registers,
+// runs the per-level checks and sums its levels' column water; the groups'
+// flags and sums meet in LDS (added in group order), and a column that
+// passes every check stores its values, one that fails any is appended to
+// the fallback list.  Yt holds the network's outputs feature-major,
+// [288][ldy].
 struct Envelope {
   const float *lo, *hi, *amax;
   float scale, dtmax;
@@ -126,21 +128,31 @@ __device__ inline bool in_range(const Envelope& e, int feat, float x) {
   return x >= e.lo[feat] - w && x <= e.hi[feat] + w;
 }
 
-__global__ void __launch_bounds__(256) k_sg_apply(Tile T, Fields f, const float* Yt, int ldy, const float* omean,
-                                                  const float* ostd, Envelope env, float dt, float qtol, float wtol,
-                                                  int c_base, int m, int* fall, int* nfall) {
-  const int cl = (int)(blockIdx.x * 256 + threadIdx.x);
-  if (cl >= m) return;
-  const int c = c_base + cl;
+#define SA_LG 8                                    // level groups (waves) per workgroup
+#define SA_LPG ((SG_LEV + SA_LG - 1) / SA_LG)      // levels per group
+__global__ void __launch_bounds__(64 * SA_LG) k_sg_apply(Tile T, Fields f, const float* Yt, int ldy,
+                                                          const float* omean, const float* ostd, Envelope env,
+                                                          float dt, float qtol, float wtol, int c_base, int m,
+                                                          int* fall, int* nfall) {
+  __shared__ float s_wo[SA_LG][64], s_wn[SA_LG][64];
+  __shared__ int s_bad[SA_LG][64];
+  const int tx = threadIdx.x, lg = threadIdx.y;
+  const int cl = (int)blockIdx.x * 64 + tx;
+  const bool live = cl < m;
+  const int c = c_base + (live ? cl : 0);
   const int j = T.j0 + c / T.ni, i = T.i0 + c % T.ni;
+  const int k0 = lg * SA_LPG;
   auto tend = [&](int v, int k) {
     const int o = v * SG_LEV + k;
-    return Yt[(long)o * ldy + cl] * ostd[o] + omean[o];
+    return Yt[(long)o * ldy + (live ? cl : 0)] * ostd[o] + omean[o];
   };
-  auto upd = [&](int v, int k, float old) { return old + dt * tend(v, k); };
   bool bad = false;
-  float w_old = 0.0f, w_new = 0.0f;
-  for (int k = 0; k < T.km; ++k) {
+  float wo = 0.0f, wn = 0.0f;
+  float nth[SA_LPG], nq[SA_LPG][SG_NOUT - 1];
+#pragma unroll
+  for (int kk = 0; kk < SA_LPG; ++kk) {
+    const int k = k0 + kk;
+    if (k >= T.km) break;                      // uniform across the wave
     const long q = T.D.ix(k, j, i);
     const float th = f.th[q], pii = f.pii[q], rho = f.rho[q], mass = rho * f.dz[q];
     const float old[SG_NOUT - 1] = {f.qv[q], f.qc[q], f.qr[q], f.qi[q], f.qs[q], f.qg[q]};
@@ -148,46 +160,57 @@ __global__ void __launch_bounds__(256) k_sg_apply(Tile T, Fields f, const float*
     const float tn = t0 + dt * tt;
     bad |= !(tn - tn == 0.0f) || !(fabsf(tn - t0) <= env.dtmax);   // NaN, infinite or too large
     bad |= !in_range(env, k, t0) || !in_range(env, 2 * SG_LEV + k, rho) || !(fabsf(tt) <= env.scale * env.amax[k]);
+    nth[kk] = tn / pii;
     float so = 0.0f, sn = 0.0f;
 #pragma unroll
     for (int v = 1; v < SG_NOUT; ++v) {
       const float tv = tend(v, k);
-      float x = old[v - 1] + dt * tv;
+      const float x = old[v - 1] + dt * tv;
       bad |= !(x - x == 0.0f) || x < -qtol;
       bad |= !in_range(env, (v + 2) * SG_LEV + k, old[v - 1]) || !(fabsf(tv) <= env.scale * env.amax[v * SG_LEV + k]);
+      nq[kk][v - 1] = x > 0.0f ? x : 0.0f;
       so += old[v - 1];
-      sn += x > 0.0f ? x : 0.0f;
+      sn += nq[kk][v - 1];
     }
-    w_old += mass * so;
-    w_new += mass * sn;
+    wo += mass * so;
+    wn += mass * sn;
   }
-  if (w_new - w_old > wtol * w_old) bad = true;
-  if (bad) {
-    fall[atomicAdd(nfall, 1)] = c;
+  s_wo[lg][tx] = wo;
+  s_wn[lg][tx] = wn;
+  s_bad[lg][tx] = bad ? 1 : 0;
+  __syncthreads();
+  float w_old = 0.0f, w_new = 0.0f;
+  int nbad = 0;
+#pragma unroll
+  for (int gq = 0; gq < SA_LG; ++gq) {
+    w_old += s_wo[gq][tx];
+    w_new += s_wn[gq][tx];
+    nbad += s_bad[gq][tx];
+  }
+  if (!live) return;
+  if (nbad || w_new - w_old > wtol * w_old) {
+    if (lg == 0) fall[atomicAdd(nfall, 1)] = c;
     return;
   }
-  for (int k = 0; k < T.km; ++k) {
+  float* dst[SG_NOUT - 1] = {f.qv, f.qc, f.qr, f.qi, f.qs, f.qg};
+#pragma unroll
+  for (int kk = 0; kk < SA_LPG; ++kk) {
+    const int k = k0 + kk;
+    if (k >= T.km) break;
     const long q = T.D.ix(k, j, i);
-    const float pii = f.pii[q];
-    float* dst[SG_NOUT - 1] = {f.qv, f.qc, f.qr, f.qi, f.qs, f.qg};
-    float nw[SG_NOUT - 1];
+    f.th[q] = nth[kk];
 #pragma unroll
-    for (int v = 1; v < SG_NOUT; ++v) {
-      const float x = upd(v, k, dst[v - 1][q]);
-      nw[v - 1] = x > 0.0f ? x : 0.0f;
-    }
-    const float tn = upd(0, k, f.th[q] * pii);
-    f.th[q] = tn / pii;
-#pragma unroll
-    for (int v = 1; v < SG_NOUT; ++v) dst[v - 1][q] = nw[v - 1];
+    for (int v = 0; v < SG_NOUT - 1; ++v) dst[v][q] = nq[kk][v];
   }
-  const long q2 = T.D.ix2(j, i);
-  const float pr = w_old - w_new > 0.0f ? w_old - w_new : 0.0f;
-  f.rainncv[q2] = pr;
-  f.rain[q2] = f.rain[q2] + pr;
-  f.snowncv[q2] = 0.0f;
-  f.graupelncv[q2] = 0.0f;
-  f.sr[q2] = 0.0f;
+  if (lg == 0) {
+    const long q2 = T.D.ix2(j, i);
+    const float pr = w_old - w_new > 0.0f ? w_old - w_new : 0.0f;
+    f.rainncv[q2] = pr;
+    f.rain[q2] = f.rain[q2] + pr;
+    f.snowncv[q2] = 0.0f;
+    f.graupelncv[q2] = 0.0f;
+    f.sr[q2] = 0.0f;
+  }
 }
 
 __global__ void k_sg_zero_int(int* p) { *p = 0; }
@@ -241,7 +264,7 @@ static int sg_run(const pywrf_grid_t* g, const pywrf_surrogate_t* net, float* th
     // the outputs feature-major (Y^T, [288][chunk]) for the column-per-thread update
     if (!e) e = pywrf_mlp_gemm(4, m, SG_NO, SG_H, H2, SG_H, net->w3, SG_H, Y, (int)chunk, net->b3, nullptr, 0, 1, s);
     if (e) return e;
-    hipLaunchKernelGGL(k_sg_apply, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, (hipStream_t)s, T, f, Y,
+    hipLaunchKernelGGL(k_sg_apply, dim3((unsigned)((m + 63) / 64)), dim3(64, SA_LG), 0, (hipStream_t)s, T, f, Y,
                        (int)chunk, net->out_mean, net->out_std, env, (float)dt, net->qtol, net->wtol, (int)c0, m,
                        fall, nfall);
   }

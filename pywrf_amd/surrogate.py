@@ -516,7 +516,7 @@ def surrogate_ops(sur, ops=None, chunk=65536, qtol=None, wtol=None, envelope=Non
 
 
 # ----------------------------------------------------------------- evaluation
-def compare_runs(sur, config="config2", nsteps=60, chunk=16384, log=None):
+def compare_runs(sur, config="config2", nsteps=60, chunk=65536, log=None):
     """The surrogate step against the plain step on the same seeded state:
     steps/s of each (HIP graphs of the step), the RMS difference of qv and T
     (theta) after nsteps relative to the RMS of the plain run's field, and the
