@@ -431,3 +431,19 @@ def rk_addtend_dry(ru_tend, rv_tend, rw_tend, ph_tend, t_tend,
     b.put(t_tend, b(t_tend) + b(t_tendf) / b.m(msfty) + b.m(mut) * b(h_diabatic) / b.m(msfty))
     mu_tend[jts:jde - 1, its:ide - 1] = mu_tend[jts:jde - 1, its:ide - 1] + mu_tendf[jts:jde - 1, its:ide - 1]
     return ru_tend, rv_tend, rw_tend, ph_tend, t_tend, mu_tend
+
+
+def pbl_driver():
+    """The boundary-layer hook: no arithmetic.  wrf_dynamics.py:413-415."""
+    return None
+
+
+def sixth_order_diffusion(name, field, tendency, MUT, dt,
+                          config_flags, c1, c2,
+                          diff_6th_opt, diff_6th_factor,
+                          ids, ide, jds, jde, kds, kde,
+                          ims, ime, jms, jme, kms, kme,
+                          its, ite, jts, jte, kts, kte):
+    """The 6th-order diffusion hook: forms its coefficient and returns None,
+    leaving the tendency unchanged.  wrf_dynamics.py:3088-3097."""
+    return None

@@ -1088,3 +1088,23 @@ def zero_group(tensors):
     ptrs = (ctypes.c_void_p * n)(*[ptr(t).value for t in tensors])
     counts = (ctypes.c_longlong * n)(*[t.numel() for t in tensors])
     call("pywrf_zero_group", n, ptrs, counts, stream())
+
+
+# ----------------------------------------------------------------- physics hooks
+def pbl_driver():
+    """The boundary-layer hook of the first RK stage (wrf_dynamics.py:413-415,
+    marked at solver.py:868): no work in the reference, none here; the PBL
+    tendencies rublten .. rqiblten stay as the driver set them."""
+    return None
+
+
+def sixth_order_diffusion(name, field, tendency, MUT, dt,
+                          config_flags, c1, c2,
+                          diff_6th_opt, diff_6th_factor,
+                          ids, ide, jds, jde, kds, kde,
+                          ims, ime, jms, jme, kms, kme,
+                          its, ite, jts, jte, kts, kte):
+    """The 6th-order diffusion hook (wrf_dynamics.py:3088-3097): the reference
+    forms diff_6th_coef and returns None without touching the tendency; so
+    does this, with no device work."""
+    return None

@@ -95,3 +95,19 @@ def test_grid_packing():
     assert (g.nz, g.ny, g.nx, g.ids, g.ide, g.kte, g.its, g.ite) == (41, 240, 240, 5, 235, 41, 5, 235)
     g2 = _lib.grid((240, 240), gr.ix2d).contents
     assert (g2.nz, g2.kds, g2.kde, g2.jte) == (1, 0, 1, 235)
+
+
+def test_physics_hooks_leave_the_state_alone():
+    """pbl_driver and sixth_order_diffusion are hooks with no arithmetic in the
+    reference (wrf_dynamics.py:413-415, 3088-3097: None returned, tendency
+    untouched); the mirror and the oracle keep that."""
+    import torch
+    import oracle
+    import pywrf_amd
+    for o in (pywrf_amd.ops(), oracle.ops()):
+        assert o.pbl_driver() is None
+        tend = torch.arange(24.0).reshape(2, 3, 4)
+        before = tend.clone()
+        idx = (0, 4, 0, 3, 0, 2) * 3 + (0, 4, 0, 3, 0, 2)
+        assert o.sixth_order_diffusion("m", tend.clone(), tend, None, 54.0, None, None, None, 2, 0.12, *idx) is None
+        assert torch.equal(tend, before)
