@@ -108,6 +108,6 @@ def test_physics_hooks_leave_the_state_alone():
         assert o.pbl_driver() is None
         tend = torch.arange(24.0).reshape(2, 3, 4)
         before = tend.clone()
-        idx = (0, 4, 0, 3, 0, 2) * 3 + (0, 4, 0, 3, 0, 2)
+        idx = (0, 4, 0, 3, 0, 2) * 3
         assert o.sixth_order_diffusion("m", tend.clone(), tend, None, 54.0, None, None, None, 2, 0.12, *idx) is None
         assert torch.equal(tend, before)
