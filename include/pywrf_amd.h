@@ -476,6 +476,24 @@ int pywrf_spec_bdy_final_group(const pywrf_grid_t* g, int n, void* const* fields
 int pywrf_spec_bdy_final_group_dtbc_dev(const pywrf_grid_t* g, int n, void* const* fields, void* const* mus,
                                         void* const* msfs, void* const* bdys, const char* variables,
                                         const float* dtbc, pywrf_stream_t s);
+/* The end-of-step boundary epilogue (solver.py:1776-1910) in one launch:
+ * set_physical_bc3d / set_physical_bc2d on n distinct fields (as
+ * pywrf_set_physical_bc_group; n <= 24), then spec_bdy_final on nfinal <= 6
+ * of them: final_of[m] is the index in fields of the m-th, with mus[m],
+ * msfs[m], the eight boundary arrays bdys[8m .. 8m+7] and final_variables[m]
+ * as in pywrf_spec_bdy_final_group.  Each (field, level) is one workgroup that
+ * fills the plane's halo and then sets its boundary rows: the same results as
+ * the two grouped calls in that order.  No counterpart in the reference: a
+ * fusion of its calls. */
+int pywrf_boundary_epilogue(const pywrf_grid_t* g3, const pywrf_grid_t* g2, int n, void* const* fields,
+                            const char* variables, const int* is2d, int nfinal, const int* final_of, void* const* mus,
+                            void* const* msfs, void* const* bdys, const char* final_variables, double dtbc,
+                            pywrf_stream_t s);
+/* pywrf_boundary_epilogue with dtbc in device memory. */
+int pywrf_boundary_epilogue_dtbc_dev(const pywrf_grid_t* g3, const pywrf_grid_t* g2, int n, void* const* fields,
+                                     const char* variables, const int* is2d, int nfinal, const int* final_of,
+                                     void* const* mus, void* const* msfs, void* const* bdys,
+                                     const char* final_variables, const float* dtbc, pywrf_stream_t s);
 
 /* ---- moist physics: wrf_physics.py:27-2137 ------------------------------ */
 /* moist_physics_prep_em, wrf_physics.py:27-91 */

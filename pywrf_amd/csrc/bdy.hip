@@ -14,6 +14,7 @@
 // statements cover the point).
 #include "common.h"
 #include "consts.h"
+#include "openbc.h"
 
 #define AT(p, k, j, i) at_b((p), D.ob((k), (j), (i)))
 #define AT2(p, j, i) at_b((p), D.ob2((j), (i)))
@@ -650,4 +651,112 @@ extern "C" int pywrf_spec_bdy_final_group_dtbc_dev(const pywrf_grid_t* g, int n,
                                                    const char* variables, const float* dtbc, pywrf_stream_t s) {
   if (!dtbc) return PYWRF_E_RANGE;
   return spec_final_group(g, n, fields, mus, msfs, bdys, variables, 0.0, dtbc, s);
+}
+
+// ------------------------------------------------------------------ end-of-step boundary epilogue
+// The end of a step (solver.py:1776-1910) fills the open lateral halos of a
+// group of fields (set_physical_bc3d / bc2d) and then sets the specified
+// boundary rows of five of them (spec_bdy_final).  Both act plane by plane on
+// their own field, and spec_bdy_final reads only the boundary arrays, mu and
+// the map factor, so one workgroup per (field, level) does the plane's halo
+// fill, waits for its own writes, and then sets the plane's boundary rows:
+// the order of the calls, in one launch.
+#define PW_EPI_MAX 24
+#define PW_EPI_FINAL_MAX 6
+struct EpiJobs {
+  int n;
+  BcJob bc[PW_EPI_MAX];
+  int fin[PW_EPI_MAX];           // index into F of the field's spec_bdy_final, or -1
+  FinalJob F[PW_EPI_FINAL_MAX];
+};
+
+__global__ void __launch_bounds__(BC_THREADS) k_bdy_epilogue(long plane, int nx, EpiJobs J) {
+  const int f = (int)blockIdx.y;
+  if (f >= J.n) return;
+  const BcJob& jb = J.bc[f];
+  const int lev = (int)blockIdx.x;
+  if (lev < jb.nk) open_bc_plane(jb.p + (long)(jb.k0 + lev) * plane, nx, jb.c);
+  if (J.fin[f] < 0) return;                       // uniform across the workgroup
+  __syncthreads();                                // the halo fill's stores before the rows it read change
+  const FinalJob& a = J.F[J.fin[f]];
+  const int k = a.all.k0 + lev;
+  if (k >= a.all.k1) return;
+  const int ni = a.all.ni(), nj = a.all.nj();
+  const long per_level = edge_count(a.all, 1);
+  for (long t = threadIdx.x; t < per_level; t += blockDim.x) {
+    int j, i;
+    const int rows = ni;                          // rows j0 and j1 - 1, then columns i0 and i1 - 1
+    if (t < rows) { j = a.all.j0; i = a.all.i0 + (int)t; }
+    else if (t < 2 * rows) { j = a.all.j1 - 1; i = a.all.i0 + (int)(t - rows); }
+    else if (t < 2 * rows + (nj - 2)) { j = a.all.j0 + 1 + (int)(t - 2 * rows); i = a.all.i0; }
+    else { j = a.all.j0 + 1 + (int)(t - 2 * rows - (nj - 2)); i = a.all.i1 - 1; }
+    spec_final_point(a, k, j, i);
+  }
+}
+
+static int bdy_epilogue(const pywrf_grid_t* g3, const pywrf_grid_t* g2, int n, void* const* fields,
+                        const char* variables, const int* is2d, int nfinal, const int* final_of, void* const* mus,
+                        void* const* msfs, void* const* bdys, const char* final_variables, double dtbc,
+                        const float* dtbc_dev, pywrf_stream_t s) {
+  if (!pw_size_ok(g3) || !pw_size_ok(g2)) return PYWRF_E_RANGE;
+  if (n < 0 || n > PW_EPI_MAX || nfinal < 0 || nfinal > PW_EPI_FINAL_MAX || g3->ny != g2->ny || g3->nx != g2->nx)
+    return PYWRF_E_ARG;
+  EpiJobs J;
+  J.n = n;
+  int maxk = 1;
+  for (int f = 0; f < n; ++f) {
+    BcJob& jb = J.bc[f];
+    jb.p = (float*)fields[f];
+    int e, k1 = 1;
+    if (is2d[f]) {
+      e = bc_spec2(g2, variables[f], jb.c);
+      jb.k0 = 0;
+      if (!e && !bc_spec_ok(g2->ny, g2->nx, 1, 0, 1, jb.c)) e = PYWRF_E_RANGE;
+    } else {
+      e = bc_spec3(g3, variables[f], jb.c, k1);
+      jb.k0 = g3->kts;
+      if (!e && !bc_spec_ok(g3->ny, g3->nx, g3->nz, g3->kts, k1, jb.c)) e = PYWRF_E_RANGE;
+    }
+    if (e) return e;
+    jb.nk = k1 - jb.k0 > 0 ? k1 - jb.k0 : 0;
+    maxk = imax(maxk, jb.nk);
+    J.fin[f] = -1;
+  }
+  for (int m = 0; m < nfinal; ++m) {
+    const int f = final_of[m];
+    if (f < 0 || f >= n || is2d[f] || J.fin[f] >= 0) return PYWRF_E_ARG;
+    const float* const* bd = (const float* const*)bdys + 8 * m;
+    FinalJob& a = J.F[m];
+    int e = spec_final_job(g3, (float*)fields[f], (const float*)mus[m], (const float*)msfs[m], bd[0], bd[1], bd[2],
+                           bd[3], bd[4], bd[5], bd[6], bd[7], final_variables[m], dtbc, a);
+    if (e) return e;
+    a.dtbc_dev = dtbc_dev;
+    // the rows the workgroup sets must be its box's sides (as in the grouped launch)
+    if (a.all.size() > 0 && (a.w != 1 || a.all.nj() < 2 || a.all.ni() < 2)) return PYWRF_E_RANGE;
+    J.fin[f] = m;
+    maxk = imax(maxk, a.all.nk());
+  }
+  if (n == 0) return 0;
+  static_assert(sizeof(EpiJobs) <= 4096, "kernel arguments of the epilogue exceed 4 KiB");
+  hipLaunchKernelGGL(k_bdy_epilogue, dim3(maxk, n), dim3(BC_THREADS), 0, (hipStream_t)s, (long)g3->ny * g3->nx,
+                     g3->nx, J);
+  PW_RET();
+}
+
+extern "C" int pywrf_boundary_epilogue(const pywrf_grid_t* g3, const pywrf_grid_t* g2, int n, void* const* fields,
+                                       const char* variables, const int* is2d, int nfinal, const int* final_of,
+                                       void* const* mus, void* const* msfs, void* const* bdys,
+                                       const char* final_variables, double dtbc, pywrf_stream_t s) {
+  return bdy_epilogue(g3, g2, n, fields, variables, is2d, nfinal, final_of, mus, msfs, bdys, final_variables, dtbc,
+                      nullptr, s);
+}
+
+extern "C" int pywrf_boundary_epilogue_dtbc_dev(const pywrf_grid_t* g3, const pywrf_grid_t* g2, int n,
+                                                void* const* fields, const char* variables, const int* is2d,
+                                                int nfinal, const int* final_of, void* const* mus, void* const* msfs,
+                                                void* const* bdys, const char* final_variables, const float* dtbc,
+                                                pywrf_stream_t s) {
+  if (!dtbc) return PYWRF_E_RANGE;
+  return bdy_epilogue(g3, g2, n, fields, variables, is2d, nfinal, final_of, mus, msfs, bdys, final_variables, 0.0,
+                      dtbc, s);
 }

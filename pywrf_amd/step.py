@@ -482,26 +482,32 @@ def rk3_step(s, ops, cfg, check_nan=False, clock=None):
                                        s.znu, s.znw, s.dnw, s.rdnw, s.rdn, True, *I18)
     if check_nan:
         _nan_check(s.p, "physics")
-    # set_phys_bc_dry2 (solver.py:1776-1858)
-    (s.u_1, s.u, s.v_1, s.v, s.w_1, s.w, s.t_1, s.t, s.ph_1, s.ph, s.mu_1, s.mu, s.tke_1, s.tke) = bcs(
-        (s.u_1, 'U', 0), (s.u, 'U', 0), (s.v_1, 'V', 0), (s.v, 'V', 0), (s.w_1, 'w', 0), (s.w, 'w', 0),
-        (s.t_1, 'p', 0), (s.t, 'p', 0), (s.ph_1, 'w', 0), (s.ph, 'w', 0), (s.mu_1, 't', 1), (s.mu, 't', 1),
-        (s.tke_1, 'p', 0), (s.tke, 'p', 0),
-        *[(s.moist[im, :, :, :], 'p', 0) for im in range(1, 7)],
-        *[(s.scalar[im, :, :, :], 'p', 0) for im in range(1, 3)])[:14]
-    # spec_bdy_final and surface w (solver.py:1863-1910)
-    if hasattr(o, "spec_bdy_final_group"):
-        o.spec_bdy_final_group([(s.u, s.muus, s.msfuy, b['u'], 'u'), (s.v, s.muvs, s.msfvx, b['v'], 'v'),
-                                (s.t, s.muts, s.msfty, b['t'], 't'), (s.ph, s.muts, s.msfty, b['ph'], 'h'),
-                                (s.moist[P_QV, :, :, :], s.muts, s.msfty, b['qv'], 't')], dtbc, I24)
+    # set_phys_bc_dry2 (solver.py:1776-1858), then spec_bdy_final (solver.py:1863-1904)
+    end_bcs = ((s.u_1, 'U', 0), (s.u, 'U', 0), (s.v_1, 'V', 0), (s.v, 'V', 0), (s.w_1, 'w', 0), (s.w, 'w', 0),
+               (s.t_1, 'p', 0), (s.t, 'p', 0), (s.ph_1, 'w', 0), (s.ph, 'w', 0), (s.mu_1, 't', 1), (s.mu, 't', 1),
+               (s.tke_1, 'p', 0), (s.tke, 'p', 0),
+               *[(s.moist[im, :, :, :], 'p', 0) for im in range(1, 7)],
+               *[(s.scalar[im, :, :, :], 'p', 0) for im in range(1, 3)])
+    finals = [(s.u, s.muus, s.msfuy, b['u'], 'u'), (s.v, s.muvs, s.msfvx, b['v'], 'v'),
+              (s.t, s.muts, s.msfty, b['t'], 't'), (s.ph, s.muts, s.msfty, b['ph'], 'h'),
+              (s.moist[P_QV, :, :, :], s.muts, s.msfty, b['qv'], 't')]
+    if hasattr(o, "boundary_epilogue"):
+        # both in one launch: each (field, level) fills its halo, then its boundary rows
+        o.boundary_epilogue(list(end_bcs), finals, dtbc, I24, I2)
     else:
-        sf = lambda fld, mu, msf, key, var: o.spec_bdy_final(fld, mu, s.c1h, s.c2h, msf, *b[key], var,
-                                                              SPEC_BDY_WIDTH, SPEC_ZONE, dtbc, *I24)
-        s.u = sf(s.u, s.muus, s.msfuy, 'u', 'u')
-        s.v = sf(s.v, s.muvs, s.msfvx, 'v', 'v')
-        s.t = sf(s.t, s.muts, s.msfty, 't', 't')
-        s.ph = sf(s.ph, s.muts, s.msfty, 'ph', 'h')
-        s.moist[P_QV, :, :, :] = sf(s.moist[P_QV, :, :, :], s.muts, s.msfty, 'qv', 't')
+        (s.u_1, s.u, s.v_1, s.v, s.w_1, s.w, s.t_1, s.t, s.ph_1, s.ph, s.mu_1, s.mu, s.tke_1, s.tke) = \
+            bcs(*end_bcs)[:14]
+        if hasattr(o, "spec_bdy_final_group"):
+            o.spec_bdy_final_group(finals, dtbc, I24)
+        else:
+            sf = lambda fld, mu, msf, key, var: o.spec_bdy_final(fld, mu, s.c1h, s.c2h, msf, *b[key], var,
+                                                                  SPEC_BDY_WIDTH, SPEC_ZONE, dtbc, *I24)
+            s.u = sf(s.u, s.muus, s.msfuy, 'u', 'u')
+            s.v = sf(s.v, s.muvs, s.msfvx, 'v', 'v')
+            s.t = sf(s.t, s.muts, s.msfty, 't', 't')
+            s.ph = sf(s.ph, s.muts, s.msfty, 'ph', 'h')
+            s.moist[P_QV, :, :, :] = sf(s.moist[P_QV, :, :, :], s.muts, s.msfty, 'qv', 't')
+    # surface w (solver.py:1906-1910)
     s.w = o.set_w_surface(s.znw, False, s.w, s.ht, s.u, s.v, s.cf1, s.cf2, s.cf3, rdx, rdy, s.msftx, s.msfty,
                           *I18)
     return s

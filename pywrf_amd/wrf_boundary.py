@@ -325,3 +325,49 @@ def spec_bdy_final_group(items, dtbc, idx24):
         call("pywrf_spec_bdy_final_group_dtbc_dev", g, n, fields, mus, msfs, bdys, flags, ptr(dtbc), stream())
     else:
         call("pywrf_spec_bdy_final_group", g, n, fields, mus, msfs, bdys, flags, float(dtbc), stream())
+
+
+def boundary_epilogue(bc_items, final_items, dtbc, idx24, idx2d):
+    """The end-of-step boundary calls (solver.py:1776-1910) in one launch:
+    set_physical_bc3d / bc2d on bc_items [(tensor, variable, is_2d)] (distinct
+    fields), then spec_bdy_final on final_items [(field, mu, msf, (bxs, bxe,
+    bys, bye, btxs, btxe, btys, btye), variable)], each field of which must be
+    one of the bc_items.  Same results as set_physical_bc_group followed by
+    spec_bdy_final_group.  dtbc: a number or a one-value device tensor.  Not
+    part of the reference API."""
+    import ctypes
+    n = len(bc_items)
+    if n == 0:
+        return
+    if len({t.data_ptr() for t, _, _ in bc_items}) != n:
+        raise ValueError("boundary_epilogue: a field appears twice")
+    t3 = next((t for t, _, two in bc_items if not two), None)
+    t2 = next((t for t, _, two in bc_items if two), None)
+    shape3 = t3.shape if t3 is not None else (idx24[5],) + tuple(bc_items[0][0].shape[-2:])
+    shape2 = t2.shape if t2 is not None else tuple(bc_items[0][0].shape[-2:])
+    g3, g2 = grid(shape3, _I24(*idx24)), grid(shape2, idx2d)
+    where = {t.data_ptr(): k for k, (t, _, _) in enumerate(bc_items)}
+    final_of = []
+    for f, mu, msf, _, _ in final_items:
+        if f.data_ptr() not in where:
+            raise ValueError("boundary_epilogue: a spec_bdy_final field is not among the filled fields")
+        check_operands("boundary_epilogue", g3.contents, ("field", "mu", "msf"), (ptr(f), ptr(mu), ptr(msf)))
+        final_of.append(where[f.data_ptr()])
+    vp = lambda xs: (ctypes.c_void_p * max(len(xs), 1))(*[ptr(x).value for x in xs])
+    fields = vp([t for t, _, _ in bc_items])
+    flags = b"".join(v.encode() for _, v, _ in bc_items)
+    kinds = (ctypes.c_int * n)(*[1 if two else 0 for _, _, two in bc_items])
+    m = len(final_items)
+    fin = (ctypes.c_int * max(m, 1))(*final_of)
+    mus = vp([x for _, x, _, _, _ in final_items])
+    msfs = vp([x for _, _, x, _, _ in final_items])
+    bps = [p for _, _, _, arrs, _ in final_items for p in _bdy(g3, arrs)]
+    bdys = (ctypes.c_void_p * max(len(bps), 1))(*[p.value for p in bps])
+    fflags = b"".join(v.encode() for _, _, _, _, v in final_items)
+    if _on_device(dtbc):
+        check_operands("boundary_epilogue", g3.contents, ("dtbc",), (ptr(dtbc),))
+        call("pywrf_boundary_epilogue_dtbc_dev", g3, g2, n, fields, flags, kinds, m, fin, mus, msfs, bdys, fflags,
+             ptr(dtbc), stream())
+    else:
+        call("pywrf_boundary_epilogue", g3, g2, n, fields, flags, kinds, m, fin, mus, msfs, bdys, fflags,
+             float(dtbc), stream())

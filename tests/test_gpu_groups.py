@@ -340,3 +340,48 @@ def test_diffusion_dry_equals_the_eight_calls(config):
     bad[5] = (s.u,) + bad[5][1:]  # the v vertical call on another field
     with pytest.raises(ValueError):
         o.diffusion_dry(*bad)
+
+
+def _epilogue_items(s):
+    """The end-of-step fills and spec_bdy_final calls of the step (solver.py:1776-1904)."""
+    b = s.bdy
+    bcs = [(s.u_1, 'U', 0), (s.u, 'U', 0), (s.v_1, 'V', 0), (s.v, 'V', 0), (s.w_1, 'w', 0), (s.w, 'w', 0),
+           (s.t_1, 'p', 0), (s.t, 'p', 0), (s.ph_1, 'w', 0), (s.ph, 'w', 0), (s.mu_1, 't', 1), (s.mu, 't', 1),
+           (s.tke_1, 'p', 0), (s.tke, 'p', 0)] + [(s.moist[im], 'p', 0) for im in range(1, 7)] + \
+        [(s.scalar[im], 'p', 0) for im in range(1, 3)]
+    finals = [(s.u, s.muus, s.msfuy, b['u'], 'u'), (s.v, s.muvs, s.msfvx, b['v'], 'v'),
+              (s.t, s.muts, s.msfty, b['t'], 't'), (s.ph, s.muts, s.msfty, b['ph'], 'h'),
+              (s.moist[1], s.muts, s.msfty, b['qv'], 't')]
+    return bcs, finals
+
+
+@pytest.mark.parametrize("on_device", [False, True])
+def test_boundary_epilogue_equals_the_grouped_calls(on_device):
+    """One launch for the end-of-step fills and boundary rows equals
+    set_physical_bc_group then spec_bdy_final_group, bit for bit, on a state
+    with live boundary tendencies (24 x 24 x 41 after two steps), every field
+    and its halo randomised first so that each copy and each row is seen."""
+    import pywrf_amd
+    from pywrf_amd.gen import synthetic_state
+    from pywrf_amd.state import clone_state
+    from pywrf_amd.step import rk3_step, RunConfig
+    s0 = synthetic_state("bdy24", device="cuda")
+    for _ in range(2):
+        rk3_step(s0, pywrf_amd.ops(), RunConfig())
+    gen = torch.Generator(device="cuda").manual_seed(9)
+    bcs0, _ = _epilogue_items(s0)
+    for t, _, _ in bcs0:
+        t.add_(torch.randn(t.shape, device="cuda", generator=gen) * (t.abs().max() * 0.1 + 1e-3))
+    a, c = clone_state(s0, device="cuda"), clone_state(s0, device="cuda")
+    g = s0.grid
+    dtbc = torch.full((1,), 108.0, device="cuda") if on_device else 108.0
+    bcs_a, fin_a = _epilogue_items(a)
+    pywrf_amd.ops().set_physical_bc_group(bcs_a, g.ix24, g.ix2d)
+    pywrf_amd.ops().spec_bdy_final_group(fin_a, dtbc, g.ix24)
+    bcs_c, fin_c = _epilogue_items(c)
+    pywrf_amd.ops().boundary_epilogue(bcs_c, fin_c, dtbc, g.ix24, g.ix2d)
+    torch.cuda.synchronize()
+    for (ta, v, _), (tc, _, _) in zip(bcs_a, bcs_c):
+        assert torch.equal(ta, tc), v
+    changed = sum(int((ta != t0).sum()) for (ta, _, _), (t0, _, _) in zip(bcs_a, bcs0))
+    assert changed > 0

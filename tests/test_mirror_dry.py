@@ -44,7 +44,7 @@ def test_every_call_of_a_step_passes_the_operand_checks(dry, config):
     rk3_step(s, pywrf_amd.ops(), RunConfig())
     names = {n for n, _ in dry}
     assert len(dry) > 150
-    for fn in ("pywrf_relax_bdytend", "pywrf_spec_bdytend_group", "pywrf_spec_bdy_final_group",
+    for fn in ("pywrf_relax_bdytend", "pywrf_spec_bdytend_group", "pywrf_boundary_epilogue",
                "pywrf_microphysics_driver",
                "pywrf_advance_w", "pywrf_rk_scalar_tend_pair"):
         assert any(n.startswith(fn) for n in names), fn
