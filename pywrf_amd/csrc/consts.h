@@ -21,6 +21,7 @@ constexpr double SVP1 = 0.611199975013732910, SVP2 = 17.670000076293945312, SVP3
 constexpr double SVPT0 = 273.149993896484375, t0c = 273.149993896484375, EP_2 = 0.621750414371490479;
 constexpr double XLV = 2500000., prandtl = 1. / 3., c_s = 0.2500000, Cp = 1004.500, r_v = 461.6000;
 constexpr double cpovcv = 1.40000, reradius = 1.5698588e-7, epssm = 0.1, rvovrd = 1.608362, Cpv = 1846.400;
+constexpr double cvpm = -0.7142857;   // config_params.py:124 (-cv / cp, as written there)
 constexpr double ep_1 = 0.608362436294555664, ep_2 = 0.621750414371490479, epsilon = 1.0000000e-15;
 constexpr double XLS = 2850000., XLF = 350000.0, rhoair0 = 1.280000, rhowater = 1000.000;
 constexpr double cliq = 4190.000, cice = 2106.000, psat = 610.7800;

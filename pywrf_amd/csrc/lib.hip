@@ -14,7 +14,7 @@ extern "C" int pywrf_constant(const char* name, double* value) {
       {"r_d", pw::r_d}, {"p1000mb", pw::p1000mb}, {"t0", pw::t0}, {"g", pw::g}, {"rcp", pw::rcp},
       {"SVP1", pw::SVP1}, {"SVP2", pw::SVP2}, {"SVP3", pw::SVP3}, {"SVPT0", pw::SVPT0}, {"EP_2", pw::EP_2},
       {"XLV", pw::XLV}, {"prandtl", pw::prandtl}, {"c_s", pw::c_s}, {"Cp", pw::Cp}, {"r_v", pw::r_v},
-      {"cpovcv", pw::cpovcv}, {"reradius", pw::reradius}, {"rvovrd", pw::rvovrd}, {"Cpv", pw::Cpv},
+      {"cpovcv", pw::cpovcv}, {"cvpm", pw::cvpm}, {"reradius", pw::reradius}, {"rvovrd", pw::rvovrd}, {"Cpv", pw::Cpv},
       {"ep_2", pw::ep_2}, {"epsilon", pw::epsilon}, {"XLS", pw::XLS}, {"XLF", pw::XLF},
       {"rhoair0", pw::rhoair0}, {"rhowater", pw::rhowater}, {"cliq", pw::cliq}, {"cice", pw::cice},
       {"psat", pw::psat}, {"dtcldcr", pw::dtcldcr}, {"n0r", pw::n0r}, {"avtr", pw::avtr}, {"bvtr", pw::bvtr},
