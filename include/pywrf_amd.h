@@ -616,6 +616,18 @@ int pywrf_wsm6_surrogate_noalloc(const pywrf_grid_t* g, const pywrf_surrogate_t*
 /* The limits of pywrf_wsm6_surrogate_noalloc: columns per call, bytes of its array. */
 int pywrf_wsm6_surrogate_noalloc_limits(int* max_columns, long long* scratch_bytes);
 
+/* The NaN check of the reference's loop (solver.py:1648-1654, 1768-1774:
+ * torch.isnan(p).any() after each RK stage and after the physics, then exit)
+ * kept on the device: scans the grid's nz*ny*nx field and, if it holds a NaN,
+ * sets flag[0] to `code` (> 0) unless an earlier check already set it, then
+ * flag[1] to the linear index of one NaN point of the check that set flag[0],
+ * and adds the NaN count to flag[2].  flag: 4 device ints, cleared by
+ * pywrf_nan_flag_clear (a kernel, so that both can sit in a HIP graph).  The
+ * host reads the flag when it chooses.  A replacement of the reference's
+ * host-side check, not of a function. */
+int pywrf_nan_scan(const pywrf_grid_t* g, const float* field, int code, int* flag, pywrf_stream_t s);
+int pywrf_nan_flag_clear(int* flag, pywrf_stream_t s);
+
 /* ---------------------------------------------------------------- adjoints
  * Transposes of linear parts of the step, for gradients through it.  The
  * reference has no hand-written adjoints (it relies on torch autograd, and

@@ -313,3 +313,49 @@ extern "C" int pywrf_set_w_surface(const pywrf_grid_t* g, float* w, const float*
                  g->its, g->ite);
   PW_RET();
 }
+
+// ------------------------------------------------------------------ device NaN flag
+// The reference stops when p holds a NaN after an RK stage or after the
+// physics (solver.py:1648-1654, 1768-1774: torch.isnan(p).any(), the indices
+// printed, sys.exit).  Here the check stays on the device: a scan of the
+// field records, in a 4-int flag array, the code of the first check that saw
+// a NaN (flag[0], 0 while none has), the linear index of one NaN point of
+// that check (flag[1]) and the number of NaN points over all checks
+// (flag[2]).  The host reads the flag when it chooses (after a graph replay,
+// a run), so the step needs no synchronisation.
+__global__ void k_nan_scan(const float* f, long n, int code, int* flag) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long stride = (long)gridDim.x * blockDim.x;
+  int cnt = 0;
+  long first = -1;
+  for (long q = i; q < n; q += stride) {
+    const float v = f[q];
+    if (v != v) {
+      ++cnt;
+      if (first < 0) first = q;
+    }
+  }
+  if (cnt) {
+    atomicAdd(flag + 2, cnt);
+    if (atomicCAS(flag, 0, code) == 0) atomicExch(flag + 1, (int)first);
+  }
+}
+
+__global__ void k_nan_flag_clear(int* flag) {
+  if (threadIdx.x < 4) flag[threadIdx.x] = 0;
+}
+
+extern "C" int pywrf_nan_scan(const pywrf_grid_t* g, const float* field, int code, int* flag, pywrf_stream_t s) {
+  if (!pw_size_ok(g) || !flag || code <= 0) return PYWRF_E_ARG;
+  const long n = (long)g->nz * g->ny * g->nx;
+  const int blocks = (int)((n + 256L * 8 - 1) / (256L * 8));
+  hipLaunchKernelGGL(k_nan_scan, dim3((unsigned)(blocks > 0 ? blocks : 1)), dim3(256), 0, (hipStream_t)s, field, n,
+                     code, flag);
+  PW_RET();
+}
+
+extern "C" int pywrf_nan_flag_clear(int* flag, pywrf_stream_t s) {
+  if (!flag) return PYWRF_E_ARG;
+  hipLaunchKernelGGL(k_nan_flag_clear, dim3(1), dim3(64), 0, (hipStream_t)s, flag);
+  PW_RET();
+}

@@ -24,7 +24,9 @@ class StepGraph:
     """`steps` consecutive RK3 steps captured as one graph (the rebound fields
     are copied back once per replay, so longer graphs amortise the copies)."""
 
-    def __init__(self, s, cfg, ops=None, steps=1):
+    def __init__(self, s, cfg, ops=None, steps=1, nan_flag=None):
+        """nan_flag: a step.NanFlag whose scans are captured with the steps
+        (read it with nan_flag.check after replays)."""
         from . import ops as _ops
         ops = ops or _ops()
         if steps < 1:
@@ -38,7 +40,7 @@ class StepGraph:
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
             for _ in range(steps):
-                rk3_step(s, ops, cfg, clock=self.clock)
+                rk3_step(s, ops, cfg, clock=self.clock, nan_flag=nan_flag)
             for n in names:
                 t = getattr(s, n)
                 if t is not orig[n]:

@@ -120,7 +120,48 @@ def _nan_check(p, where):
         raise FloatingPointError("NaN in p (%s)" % where)
 
 
-def rk3_step(s, ops, cfg, check_nan=False, clock=None):
+class NanFlag:
+    """The reference's NaN exits (solver.py:1648-1654, 1768-1774) kept on the
+    device: rk3_step(..., nan_flag=f) scans p after each RK stage (codes 1-3)
+    and after the physics (code 4) with pywrf_nan_scan, which records the
+    first check that saw a NaN, one NaN point of it and the NaN count; no
+    synchronisation inside the step, so the scans can be captured in a HIP
+    graph.  check() reads the record (one small copy) and raises
+    FloatingPointError naming the check and the point."""
+
+    WHERE = {1: "rk_step 1", 2: "rk_step 2", 3: "rk_step 3", 4: "physics"}
+
+    def __init__(self, device="cuda"):
+        import torch
+        self.flag = torch.zeros(4, dtype=torch.int32, device=device)
+
+    def _ptr(self):
+        import ctypes
+        return ctypes.c_void_p(self.flag.data_ptr())
+
+    def clear(self):
+        from ._lib import call, stream
+        call("pywrf_nan_flag_clear", self._ptr(), stream())
+
+    def scan(self, field, code, idx):
+        from ._lib import call, grid, ptr, stream
+        call("pywrf_nan_scan", grid(field.shape, idx), ptr(field), int(code), self._ptr(), stream())
+
+    def record(self):
+        code, first, count, _ = (int(x) for x in self.flag.cpu())
+        return code, first, count
+
+    def check(self, shape):
+        code, first, count = self.record()
+        if code:
+            nz, ny, nx = shape
+            k, rem = divmod(first, ny * nx)
+            j, i = divmod(rem, nx)
+            raise FloatingPointError("NaN in p after %s: %d points, one at (k, j, i) = (%d, %d, %d)"
+                                     % (self.WHERE.get(code, code), count, k, j, i))
+
+
+def rk3_step(s, ops, cfg, check_nan=False, clock=None, nan_flag=None):
     """Advance state `s` (a namespace of tensors, see pywrf_amd.state) by one
     RK3 step using the functions in `ops`.  solver.py:454-1910.  With a
     DeviceClock the boundary time dtbc is its device tensor (the form a
@@ -453,6 +494,8 @@ def rk3_step(s, ops, cfg, check_nan=False, clock=None):
                                            cfg.p_top, s.znu, s.znw, s.dnw, s.rdnw, s.rdn, True, *I18)
         if check_nan:
             _nan_check(s.p, "rk_step %d" % rk_step)
+        if nan_flag is not None:
+            nan_flag.scan(s.p, rk_step, I18)
         if rk_step < cfg.rk_order:
             (s.u, s.v, s.w, s.t, s.ph, s.mu) = bcs(
                 (s.u, 'U', 0), (s.v, 'V', 0), (s.w, 'w', 0), (s.t, 'p', 0), (s.ph, 'w', 0), (s.mu, 't', 1),
@@ -482,6 +525,8 @@ def rk3_step(s, ops, cfg, check_nan=False, clock=None):
                                        s.znu, s.znw, s.dnw, s.rdnw, s.rdn, True, *I18)
     if check_nan:
         _nan_check(s.p, "physics")
+    if nan_flag is not None:
+        nan_flag.scan(s.p, 4, I18)
     # set_phys_bc_dry2 (solver.py:1776-1858), then spec_bdy_final (solver.py:1863-1904)
     end_bcs = ((s.u_1, 'U', 0), (s.u, 'U', 0), (s.v_1, 'V', 0), (s.v, 'V', 0), (s.w_1, 'w', 0), (s.w, 'w', 0),
                (s.t_1, 'p', 0), (s.t, 'p', 0), (s.ph_1, 'w', 0), (s.ph, 'w', 0), (s.mu_1, 't', 1), (s.mu, 't', 1),
