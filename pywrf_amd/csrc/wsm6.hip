@@ -667,34 +667,30 @@ __device__ __forceinline__ void wsm6_column(SedLds& L, const Dims& D, const Wsm6
 }
 
 
-// Global index of the c-th column of a launch (the tile's column order, or
-// the device list); -1 when there is none.
-__device__ inline int wsm6_col_index(const Wsm6& a, int c) {
-  if (c >= a.ncol) return -1;
-  if (a.cols) {
-    if (c >= *a.ncols) return -1;
-    c = a.cols[c];
-    if (c < 0 || c >= a.ncol) return -1;
-  }
-  return c;
-}
-
 // One 64-lane wave per column (lane = level), reading and writing the fields
-// in place.  Staging the eleven inputs of 8 or 16 columns through LDS with
-// row-contiguous loads was measured slower at 240 x 240 x 41 (420 and 504 us
-// against 283 us per call): the scheme is bound by its per-column arithmetic
-// and the serial sedimentation loops, not by the strided loads, and fewer
-// resident columns per CU hide less of that latency.
+// in place.  Columns are dealt to the XCDs in contiguous runs: XCD x takes
+// [x * per, (x + 1) * per) of the tile's order -- or of the device list, with
+// per formed from the list's length on the device, so that a short list
+// (the surrogate's fallback columns) is spread over all XCDs, not left to the
+// first -- and the columns sharing a cache line are served by one L2.  The
+// launch is sized for the whole tile; waves beyond the run return at once.
+// (A wave walking several columns in a loop needs 107 VGPRs instead of 64
+// and halves the occupancy: 408 against 283 us per call at 240 x 240.)
 __global__ void __launch_bounds__(WAVE) k_wsm6(Dims D, Wsm6 a) {
   __shared__ SedLds L;
-  // XCD-aware column order: workgroups are dealt to the 8 XCDs round-robin,
-  // so XCD x takes the contiguous run [x * per, (x + 1) * per) of columns and
-  // the columns that share a cache line are served by one L2.
+  const int k = (int)threadIdx.x;
+  const int total = a.cols ? imin(*a.ncols, a.ncol) : a.ncol;
+  const int per = a.cols ? (total + PW_XCDS - 1) / PW_XCDS : a.per;
+  const int x = (int)(blockIdx.x % PW_XCDS);
+  const int cc = x * per + (int)(blockIdx.x / PW_XCDS);
+  if (cc >= imin(x * per + per, total)) return;
+  int c = cc;
+  if (a.cols) {
+    c = a.cols[cc];
+    if (c < 0 || c >= a.ncol) return;
+  }
   const int ni = a.i1 - a.i0;
-  const int c = wsm6_col_index(a, (int)(blockIdx.x % PW_XCDS) * a.per + (int)(blockIdx.x / PW_XCDS));
-  if (c < 0) return;
   const int j = a.j0 + c / ni, i = a.i0 + c % ni;
-  const int k = threadIdx.x;
   const bool act = k < a.km;
   const long q = act ? D.ix(k, j, i) : D.ix(0, j, i);
   const float* src[CI_N] = {a.th, a.pii, a.den, a.delz, a.p, a.q, a.qc, a.qi, a.qr, a.qs, a.qg};
