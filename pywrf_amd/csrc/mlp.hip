@@ -200,6 +200,28 @@ __global__ void __launch_bounds__(256) k_mlp_gemm(GemmArgs g) {
   __builtin_amdgcn_s_waitcnt(0x0F70);
   // a tile wholly inside C stores without per-element predicates
   const bool full = m0 + MLP_BM <= g.M && n0 + MLP_BN <= g.N;
+  if (EPI == EPI_F32_T && full && (g.ldc & 3) == 0 && ((uintptr_t)g.C & 15) == 0) {
+    // C^T: a lane's elements 4r .. 4r + 3 are four consecutive rows m of one
+    // column n, i.e. four consecutive floats of C^T's row n: one 16-byte store
+    // (m is a multiple of 4, ldc too)
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const int n = n0 + wn * 64 + b * 32 + lr;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int m = m0 + wm * 64 + a * 32 + 8 * r + 4 * (lane >> 5);
+          float4 v;
+          v.x = acc[a][b][4 * r];
+          v.y = acc[a][b][4 * r + 1];
+          v.z = acc[a][b][4 * r + 2];
+          v.w = acc[a][b][4 * r + 3];
+          *(float4*)((float*)g.C + (long)n * g.ldc + m) = v;
+        }
+      }
+    return;
+  }
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
