@@ -652,6 +652,19 @@ int pywrf_rk_update_scalar_adjoint(const pywrf_grid_t* g, const float* lam_new, 
                                    const float* mu_old, const float* mu_new, const float* mu_base, double dt,
                                    int spec_zone, float* lam_n, float* lam_adv, float* lam_sc, float* lam_prev,
                                    pywrf_stream_t s);
+/* Adjoint of advance_w (wrf_dynamics.py:3641-3796) with respect to its
+ * prognostic inputs w, ph, t_2 and t_2ave, every coefficient (column masses,
+ * c2a, cqw, alt, a, alpha, gamma, the damping from ph_1 and phb) held at the
+ * values given: from the sensitivities of the three outputs (t_2ave, w, ph)
+ * writes those of w, ph, t_2 and t_2ave on the whole array.  work, work2:
+ * 3-D scratch arrays of the grid's size. */
+int pywrf_advance_w_adjoint(const pywrf_grid_t* g, const float* lam_t2ave_out, const float* lam_w_out,
+                            const float* lam_ph_out, const float* mut, const float* muts, const float* t_1,
+                            const float* ph_1, const float* phb, const float* c2a, const float* cqw,
+                            const float* alt, const float* a, const float* alpha, const float* gamma, double dts,
+                            double t0, double epssm, const float* rdnw, const float* rdn, const float* msfty,
+                            float* lam_w, float* lam_ph, float* lam_t_2, float* lam_t2ave, float* work,
+                            float* work2, pywrf_stream_t s);
 /* The column tridiagonal solve of advance_w (wrf_dynamics.py:3758-3765), in
  * place on w over advance_w's columns, levels 1..kte-1 (level 0 is the lower
  * boundary value): the same operations in the same order. */

@@ -8,6 +8,10 @@ transpose of a linear part of the step:
 
 * AdvectScalar -- advect_scalar (wrf_dynamics.py:1841-2028) as a function of
   the field, the mass fluxes held fixed.  Backward: pywrf_advect_scalar_adjoint.
+* AdvanceW -- the vertical acoustic step advance_w (wrf_dynamics.py:3641-3796)
+  as a function of w, ph, t_2 and t_2ave.  Backward: pywrf_advance_w_adjoint
+  (the damping, the transposed column solve, the explicit predictor and the
+  geopotential right-hand side, in reverse).
 * TridiagSolve -- the column tridiagonal solve of advance_w
   (wrf_dynamics.py:3758-3765) as a function of its coefficients a, alpha,
   gamma and right-hand side.  Backward: pywrf_tridiag_solve_adjoint
@@ -116,6 +120,44 @@ class TridiagSolve(torch.autograd.Function):
         call("pywrf_tridiag_solve_adjoint", g, ptr(a), ptr(alpha), ptr(gamma), ptr(r), ptr(x), ptr(lr), ptr(ga),
              ptr(galpha), ptr(ggamma), ptr(work), stream())
         return ga, galpha, ggamma, lr, None
+
+
+# ----------------------------------------------------------------------- acoustic step: advance_w
+class AdvanceW(torch.autograd.Function):
+    """(t_2ave, w, ph) = advance_w(...) (wrf_dynamics.py:3641-3796) with the
+    gradient with respect to w, ph, t_2 and t_2ave (the prognostic inputs;
+    every other argument is a coefficient held fixed).  args: advance_w's
+    positional arguments in the reference's order."""
+
+    # positions in advance_w's signature
+    W, T2AVE, T2, PH = 0, 18, 19, 21
+
+    @staticmethod
+    def forward(ctx, w, ph, t_2, t_2ave, args):
+        from . import wrf_dynamics
+        args = list(args)
+        args[AdvanceW.W], args[AdvanceW.PH], args[AdvanceW.T2], args[AdvanceW.T2AVE] = (
+            _f32(w), _f32(ph), _f32(t_2), _f32(t_2ave))
+        ctx.args = args
+        t2a, wo, pho = wrf_dynamics.advance_w(*args)
+        return t2a, wo, pho
+
+    @staticmethod
+    def backward(ctx, lT, lW, lP):
+        a = ctx.args
+        g = grid(a[0].shape, tuple(a[-18:]))
+        z = lambda t: torch.zeros_like(a[0]) if t is None else _f32(t)
+        lT, lW, lP = z(lT), z(lW), z(lP)
+        out = [torch.empty_like(a[0]) for _ in range(6)]
+        # advance_w(w, rw_tend, ww, w_save, u, v, mu1, mut, muave, muts, c1h, c2h, c1f, c2f, c3h, c4h, c3f, c4f,
+        #           t_2ave, t_2, t_1, ph, ph_1, phb, ph_tend, ht, c2a, cqw, alt, alb, a, alpha, gamma,
+        #           rdx, rdy, dts, t0, epssm, dnw, fnm, fnp, rdnw, rdn, cf1, cf2, cf3, msftx, msfty, top_lid, idx)
+        call("pywrf_advance_w_adjoint", g, ptr(lT), ptr(lW), ptr(lP), ptr(a[7]), ptr(a[9]), ptr(a[20]),
+             ptr(a[22]), ptr(a[23]), ptr(a[26]), ptr(a[27]), ptr(a[28]), ptr(a[30]), ptr(a[31]), ptr(a[32]),
+             float(a[35]), float(a[36]), float(a[37]), ptr(a[41]), ptr(a[42]), ptr(a[47]), *[ptr(t) for t in out],
+             stream())
+        lw, lph, lt2, lt2a = out[:4]
+        return lw, lph, lt2, lt2a, None
 
 
 # ----------------------------------------------------------------------- tracer transport

@@ -302,3 +302,139 @@ extern "C" int pywrf_tridiag_solve_adjoint(const pywrf_grid_t* g, const float* a
   PW_LAUNCH_COLS(k_tridiag_adj, b, s, dims_of(g), t);
   PW_RET();
 }
+
+// ------------------------------------------------------------------ advance_w^T
+// advance_w (wrf_dynamics.py:3641-3796) is linear in its prognostic inputs
+// w, ph, t_2 and t_2ave once the coefficients (column masses, c2a, cqw, alt,
+// the tridiagonal factors, the damping weights from ph_1) are fixed:
+//   t2a[k]  = (0.5 ((1+e) t_2[k] + (1-e) t2a_in[k]) + muave t0) / (muts (t0 + t_1[k])),  k < K
+//   rhs[k]  = ph[k] + msfty dts (ph_tend[k] + 0.5 g (1-e) w[k] - ...) / mut,            1 <= k <= K
+//   wp[k]   = w[k] + ... (rhs, ph at k-1, k, k+1; t2a at k-1, k),                      1 <= k <= K
+//   ws      = the column solve of wp (w[0] from the terrain, fixed),
+//   w_out   = (ws - dampwt mut w_save) / (1 + dampwt),
+//   ph_out  = rhs + msfty 0.5 dts g (1+e) w_out / muts,
+// with K = kte - 1, on the columns [jds+1, jde-2) x [ids+1, ide-2); every
+// other point passes w, ph and t_2ave through.  One thread per column walks
+// these in reverse.  work, work2: 3-D scratch (the sensitivities of w and of
+// rhs along the column).
+struct AdjW {
+  const float *lT, *lW, *lP;                      // sensitivities of the outputs t_2ave, w, ph
+  const float *mut, *muts, *t_1, *ph_1, *phb, *c2a, *cqw, *alt, *a, *alpha, *gamma, *rdnw, *rdn, *msfty;
+  float *lw_in, *lph_in, *lt2, *lt2a_in, *work, *work2;
+  float dts, t0, ep1, em1, dampmag, hpi, hdepth;
+  int K, j0, j1, i0, i1;
+};
+
+__global__ void k_advance_w_adj(Box all, Dims D, AdjW a) {
+  int j, i;
+  if (!box_column(all, j, i)) return;
+  const int nz = D.nz, K = a.K;
+  const bool in = j >= a.j0 && j < a.j1 && i >= a.i0 && i < a.i1;
+  if (!in) {
+    for (int k = 0; k < nz; ++k) {
+      AT(a.lw_in, k, j, i) = AT(a.lW, k, j, i);
+      AT(a.lph_in, k, j, i) = AT(a.lP, k, j, i);
+      AT(a.lt2a_in, k, j, i) = AT(a.lT, k, j, i);
+      AT(a.lt2, k, j, i) = 0.0f;
+    }
+    return;
+  }
+  const float G = F(pw::g);
+  const float my = AT2(a.msfty, j, i), mi = 1.0f / my, mut = AT2(a.mut, j, i), muts = AT2(a.muts, j, i);
+  // damping weights (as the forward forms them) and the sensitivity of the solved w
+  const float htop = (AT(a.ph_1, K, j, i) + AT(a.phb, K, j, i)) / G, hbot = htop - a.hdepth;
+  for (int k = 1; k <= K; ++k) {
+    const float hk = (AT(a.ph_1, k, j, i) + AT(a.phb, k, j, i)) / G;
+    float dw = 0.0f;
+    if (hk >= hbot) {
+      const float sn = sinf(a.hpi * (hk - hbot) / a.hdepth);
+      dw = a.dampmag * (sn * sn);
+    }
+    const float lwd = AT(a.lW, k, j, i) + my * 0.5f * a.dts * G * a.ep1 / muts * AT(a.lP, k, j, i);
+    AT(a.work, k, j, i) = lwd / (1.0f + dw);
+  }
+  // the column solve, transposed: back substitution upwards, elimination downwards
+  for (int k = 1; k <= K - 1; ++k) AT(a.work, k + 1, j, i) -= AT(a.gamma, k, j, i) * AT(a.work, k, j, i);
+  for (int k = K; k >= 1; --k) {
+    const float v = AT(a.work, k, j, i) * AT(a.alpha, k, j, i);
+    AT(a.work, k, j, i) = v;
+    if (k - 1 >= 1) AT(a.work, k - 1, j, i) -= AT(a.a, k, j, i) * v;
+  }
+  // accumulators: rhs (work2), ph_in, t2a (in lt2a_in), w_in
+  for (int k = 0; k < nz; ++k) {
+    const bool upd = k >= 1 && k <= K;
+    AT(a.work2, k, j, i) = upd ? AT(a.lP, k, j, i) : 0.0f;
+    AT(a.lph_in, k, j, i) = upd ? 0.0f : AT(a.lP, k, j, i);
+    AT(a.lt2a_in, k, j, i) = AT(a.lT, k, j, i);
+    AT(a.lw_in, k, j, i) = upd ? AT(a.work, k, j, i) : 0.0f;    // w[0] is overwritten from the terrain
+  }
+  // the explicit predictor wp[k]
+  for (int k = 1; k <= K; ++k) {
+    const float L = AT(a.work, k, j, i);
+    if (k < K) {
+      const float C = mi * AT(a.cqw, k, j, i) * (0.5f * a.dts * G * a.rdn[k]);
+      const float P = AT(a.c2a, k, j, i) * a.rdnw[k] / mut, M = AT(a.c2a, k - 1, j, i) * a.rdnw[k - 1] / mut;
+      AT(a.work2, k + 1, j, i) += C * P * a.ep1 * L;
+      AT(a.work2, k, j, i) -= C * (P + M) * a.ep1 * L;
+      if (k - 1 >= 1) AT(a.work2, k - 1, j, i) += C * M * a.ep1 * L;
+      AT(a.lph_in, k + 1, j, i) += C * P * a.em1 * L;
+      AT(a.lph_in, k, j, i) -= C * (P + M) * a.em1 * L;
+      AT(a.lph_in, k - 1, j, i) += C * M * a.em1 * L;
+      const float tg = a.dts * G * mi * a.rdn[k];
+      AT(a.lt2a_in, k, j, i) += tg * AT(a.c2a, k, j, i) * AT(a.alt, k, j, i) * L;
+      AT(a.lt2a_in, k - 1, j, i) -= tg * AT(a.c2a, k - 1, j, i) * AT(a.alt, k - 1, j, i) * L;
+    } else {
+      const float rk = a.rdnw[K - 1];
+      const float Dk = mi * (-0.5f * a.dts * G / mut * rk * rk * 2.0f * AT(a.c2a, K - 1, j, i));
+      AT(a.work2, K, j, i) += Dk * a.ep1 * L;
+      if (K - 1 >= 1) AT(a.work2, K - 1, j, i) -= Dk * a.ep1 * L;
+      AT(a.lph_in, K, j, i) += Dk * a.em1 * L;
+      AT(a.lph_in, K - 1, j, i) -= Dk * a.em1 * L;
+      AT(a.lt2a_in, K - 1, j, i) -= mi * a.dts * G * 2.0f * rk * AT(a.c2a, K - 1, j, i) * AT(a.alt, K - 1, j, i) * L;
+    }
+  }
+  // rhs[k] = ph[k] + msfty dts 0.5 g (1-e) w[k] / mut + (terms without w, ph)
+  const float gam = my * a.dts * 0.5f * G * a.em1 / mut;
+  for (int k = 1; k <= K; ++k) {
+    const float lr = AT(a.work2, k, j, i);
+    AT(a.lph_in, k, j, i) += lr;
+    AT(a.lw_in, k, j, i) += gam * lr;
+  }
+  // t2a[k] for k < K from t_2 and t2a_in; levels from K on pass through
+  for (int k = 0; k < nz; ++k) {
+    const float l = AT(a.lt2a_in, k, j, i);
+    if (k < K) {
+      const float den = muts * (a.t0 + AT(a.t_1, k, j, i));
+      AT(a.lt2, k, j, i) = 0.5f * a.ep1 / den * l;
+      AT(a.lt2a_in, k, j, i) = 0.5f * a.em1 / den * l;
+    } else {
+      AT(a.lt2, k, j, i) = 0.0f;
+    }
+  }
+}
+
+extern "C" int pywrf_advance_w_adjoint(const pywrf_grid_t* g, const float* lam_t2ave_out, const float* lam_w_out,
+                                       const float* lam_ph_out, const float* mut, const float* muts,
+                                       const float* t_1, const float* ph_1, const float* phb, const float* c2a,
+                                       const float* cqw, const float* alt, const float* a, const float* alpha,
+                                       const float* gamma, double dts, double t0, double epssm, const float* rdnw,
+                                       const float* rdn, const float* msfty, float* lam_w, float* lam_ph,
+                                       float* lam_t_2, float* lam_t2ave, float* work, float* work2,
+                                       pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
+  CHECK(g, 1);
+  if (g->kts != 0 || g->kte > g->nz || g->kte < 3) return PYWRF_E_RANGE;
+  AdjW p;
+  p.lT = lam_t2ave_out; p.lW = lam_w_out; p.lP = lam_ph_out;
+  p.mut = mut; p.muts = muts; p.t_1 = t_1; p.ph_1 = ph_1; p.phb = phb; p.c2a = c2a; p.cqw = cqw; p.alt = alt;
+  p.a = a; p.alpha = alpha; p.gamma = gamma; p.rdnw = rdnw; p.rdn = rdn; p.msfty = msfty;
+  p.lw_in = lam_w; p.lph_in = lam_ph; p.lt2 = lam_t_2; p.lt2a_in = lam_t2ave; p.work = work; p.work2 = work2;
+  p.dts = F(dts); p.t0 = F(t0); p.ep1 = F(1. + epssm); p.em1 = F(1. - epssm);
+  p.dampmag = F(dts * 0.2); p.hpi = F(0.5 * 3.14159265359); p.hdepth = 5000.0f;
+  p.K = g->kte - 1;
+  p.j0 = imax(g->jts, g->jds + 1); p.j1 = imin(g->jte, g->jde - 2);
+  p.i0 = imax(g->its, g->ids + 1); p.i1 = imin(g->ite, g->ide - 2);
+  Box all = whole(g);
+  PW_LAUNCH_COLS(k_advance_w_adj, all, s, dims_of(g), p);
+  PW_RET();
+}

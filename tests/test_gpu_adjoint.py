@@ -240,3 +240,54 @@ def test_surrogate_input_gradient_matches_cpu_autograd():
     err = _rel(X.grad.cpu(), Xc.grad)
     print("dX vs CPU autograd: %.2e" % err)
     assert err <= 3e-2
+
+
+class _FirstCall:
+    """The step's functions, keeping a copy of the arguments of the first call of one of them."""
+
+    def __init__(self, ops, name):
+        self._ops, self._name, self.args = ops, name, None
+
+    def __getattr__(self, n):
+        f = getattr(self._ops, n)
+        if n != self._name:
+            return f
+
+        def rec(*args):
+            if self.args is None:
+                self.args = [a.clone() if isinstance(a, torch.Tensor) else a for a in args]
+            return f(*args)
+        return rec
+
+
+@pytest.mark.parametrize("config", ["config1c", "bdy24"])
+def test_advance_w_adjoint_matches_oracle_autograd(config):
+    """Gradients of <lT, t_2ave> + <lW, w> + <lP, ph> after advance_w with
+    respect to w, ph, t_2, t_2ave, at the arguments of the step's first
+    acoustic substep, against torch autograd through the oracle's advance_w
+    in FP64 (the same coefficients).  Bound: 1e-4 normwise per gradient (the
+    coefficients are FP32 values in both; the FP32 adjoint's rounding)."""
+    import oracle
+    import pywrf_amd
+    from pywrf_amd import adjoint
+    from pywrf_amd.gen import synthetic_state
+    from pywrf_amd.step import rk3_step, RunConfig
+    s = synthetic_state(config, device="cuda")
+    rec = _FirstCall(pywrf_amd.ops(), "advance_w")
+    rk3_step(s, rec, RunConfig())
+    args = rec.args
+    gen = torch.Generator(device="cuda").manual_seed(7)
+    lams = [torch.randn(s.w.shape, device="cuda", generator=gen) for _ in range(3)]
+    W, T2A, T2, PH = adjoint.AdvanceW.W, adjoint.AdvanceW.T2AVE, adjoint.AdvanceW.T2, adjoint.AdvanceW.PH
+    x = [args[n].clone().requires_grad_() for n in (W, PH, T2, T2A)]
+    outs = adjoint.AdvanceW.apply(*x, args)
+    sum(float(0) + (o * l).sum() for o, l in zip(outs, lams)).backward()
+    a64 = [a.detach().double().cpu() if isinstance(a, torch.Tensor) else a for a in args]
+    x64 = [a64[n].clone().requires_grad_() for n in (W, PH, T2, T2A)]
+    a64[W], a64[PH], a64[T2], a64[T2A] = x64
+    o64 = oracle.ops().advance_w(*a64)
+    sum((o * l.double().cpu()).sum() for o, l in zip(o64, lams)).backward()
+    for name, mine, ref in zip(("w", "ph", "t_2", "t_2ave"), x, x64):
+        err = _rel(mine.grad.double().cpu(), ref.grad)
+        print("d/d%s: %.2e" % (name, err))
+        assert err <= 1e-4, name
