@@ -46,6 +46,9 @@ typedef __bf16 bf16;
 #define SG_H 512
 #define SG_NO 288    // 287 padded
 #define SG_COLS 64   // columns per pack workgroup
+#ifndef SG_PW
+#define SG_PW 8      // waves per pack workgroup (each takes every SG_PW-th level)
+#endif
 
 extern "C" int pywrf_mlp_gemm(int epi, int M, int N, int K, const void* A, int lda, const void* B, int ldb, void* C,
                               int ldc, const float* bias, const void* H, int ldh, int ksplit, pywrf_stream_t s);
@@ -68,11 +71,13 @@ struct Fields {
 
 // X[c - c_base][feature] for the columns c_base + [0, m) of the chunk: one
 // workgroup per 64 columns, lane = column (rows of the fields are read
-// contiguously), the four waves taking every fourth level; all ten fields of
-// a level (th, pii and the eight others) are loaded before any is used.  The
+// contiguously), its SG_PW waves taking every SG_PW-th level (8: five levels
+// a wave; 4 left too few waves in flight, 16 gained nothing more); all ten
+// fields of a level (th, pii and the eight others) are loaded before any is
+// used.  The
 // standardised values go to LDS as [feature][column] and leave as 16-byte
 // pieces of whole rows of X.
-__global__ void __launch_bounds__(256) k_sg_pack(Tile T, Fields f, const float* mean, const float* rstd, int c_base,
+__global__ void __launch_bounds__(64 * SG_PW) k_sg_pack(Tile T, Fields f, const float* mean, const float* rstd, int c_base,
                                                  int m, bf16* X) {
   __shared__ __attribute__((aligned(16))) bf16 lds[SG_KIN][SG_COLS + 2];
   const int c0 = (int)blockIdx.x * SG_COLS;
@@ -80,7 +85,7 @@ __global__ void __launch_bounds__(256) k_sg_pack(Tile T, Fields f, const float* 
   const int cl = c0 + tx, c = c_base + cl;
   const bool live = cl < m;
   const int j = T.j0 + (live ? c / T.ni : 0), i = T.i0 + (live ? c % T.ni : 0);
-  for (int k = ty; k < SG_LEV; k += 4) {
+  for (int k = ty; k < SG_LEV; k += SG_PW) {
     const bool on = live && k < T.km;
     const long q = T.D.ix(on ? k : 0, j, i);
     float v[SG_NIN];                                    // loads at a valid (clamped) point, no branch
@@ -94,10 +99,10 @@ __global__ void __launch_bounds__(256) k_sg_pack(Tile T, Fields f, const float* 
       lds[feat][tx] = (bf16)(on ? (v[var] - mean[feat]) * rstd[feat] : 0.0f);
     }
   }
-  for (int feat = SG_NIN * SG_LEV + ty; feat < SG_KIN; feat += 4) lds[feat][tx] = (bf16)0.0f;
+  for (int feat = SG_NIN * SG_LEV + ty; feat < SG_KIN; feat += SG_PW) lds[feat][tx] = (bf16)0.0f;
   __syncthreads();
   // 64 rows of 48 pieces of 8 features; consecutive threads write consecutive pieces
-  for (int id = threadIdx.x; id < SG_COLS * (SG_KIN / 8); id += 256) {
+  for (int id = threadIdx.x; id < SG_COLS * (SG_KIN / 8); id += 64 * SG_PW) {
     const int row = id / (SG_KIN / 8), f8 = (id % (SG_KIN / 8)) * 8;
     if (c0 + row >= m) continue;
     typedef __bf16 bf16x8v __attribute__((ext_vector_type(8)));
@@ -109,7 +114,8 @@ __global__ void __launch_bounds__(256) k_sg_pack(Tile T, Fields f, const float* 
 }
 
 // The network's tendencies applied with the checks, on a workgroup of 64
-// columns x SA_LG level groups: lane = column (every load and store of a
+// columns x SA_LG level groups (16: three levels a wave; 8 groups ran 35 us
+// longer at 240 x 240 for want of waves in flight): lane = column (every load and store of a
 // level is a row segment of the field), wave = a group of SA_LPG consecutive
 // levels.  Each thread forms the updated values of its levels in registers,
 // runs the per-level checks and sums its levels' column water; the groups'
@@ -128,7 +134,9 @@ __device__ inline bool in_range(const Envelope& e, int feat, float x) {
   return x >= e.lo[feat] - w && x <= e.hi[feat] + w;
 }
 
-#define SA_LG 8                                    // level groups (waves) per workgroup
+#ifndef SA_LG
+#define SA_LG 16                                   // level groups (waves) per workgroup
+#endif
 #define SA_LPG ((SG_LEV + SA_LG - 1) / SA_LG)      // levels per group
 __global__ void __launch_bounds__(64 * SA_LG) k_sg_apply(Tile T, Fields f, const float* Yt, int ldy,
                                                           const float* omean, const float* ostd, Envelope env,
@@ -257,7 +265,7 @@ static int sg_run(const pywrf_grid_t* g, const pywrf_surrogate_t* net, float* th
   hipLaunchKernelGGL(k_sg_zero_int, dim3(1), dim3(1), 0, (hipStream_t)s, nfall);
   for (long c0 = 0; c0 < ncol; c0 += chunk) {
     const int m = (int)(ncol - c0 < chunk ? ncol - c0 : chunk);
-    hipLaunchKernelGGL(k_sg_pack, dim3((unsigned)((m + SG_COLS - 1) / SG_COLS)), dim3(256), 0, (hipStream_t)s, T, f,
+    hipLaunchKernelGGL(k_sg_pack, dim3((unsigned)((m + SG_COLS - 1) / SG_COLS)), dim3(64 * SG_PW), 0, (hipStream_t)s, T, f,
                        net->in_mean, net->in_rstd, (int)c0, m, X);
     int e = pywrf_mlp_gemm(1, m, SG_H, SG_KIN, X, SG_KIN, net->w1, SG_KIN, H1, SG_H, net->b1, nullptr, 0, 1, s);
     if (!e) e = pywrf_mlp_gemm(1, m, SG_H, SG_H, H1, SG_H, net->w2, SG_H, H2, SG_H, net->b2, nullptr, 0, 1, s);
