@@ -200,6 +200,30 @@ __global__ void __launch_bounds__(256) k_mlp_gemm(GemmArgs g) {
   __builtin_amdgcn_s_waitcnt(0x0F70);
   // a tile wholly inside C stores without per-element predicates
   const bool full = m0 + MLP_BM <= g.M && n0 + MLP_BN <= g.N;
+  if ((EPI == EPI_BF16_RELU || EPI == EPI_BF16_DRELU) && full && (g.ldc & 7) == 0 &&
+      ((uintptr_t)g.C & 15) == 0) {
+    // the 128 x 128 BF16 tile through LDS (the A slab buffer, free once every
+    // wave is past its last MFMA), then out as 16-byte pieces of whole rows:
+    // 8 stores a thread instead of 64 two-byte ones
+    bf16* const T = As[0];
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+          const int ml = wm * 64 + a * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+          T[ml * MLP_LDS_ROW + wn * 64 + b * 32 + lr] = (bf16)acc[a][b][e];
+        }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < MLP_BM * MLP_BN / 8 / 256; ++r) {
+      const int p = tid + 256 * r, row = p / (MLP_BN / 8), col = (p % (MLP_BN / 8)) * 8;
+      *(uint4*)((bf16*)g.C + (long)(m0 + row) * g.ldc + n0 + col) = *(const uint4*)&T[row * MLP_LDS_ROW + col];
+    }
+    return;
+  }
   if (EPI == EPI_F32_T && full && (g.ldc & 3) == 0 && ((uintptr_t)g.C & 15) == 0) {
     // C^T: a lane's elements 4r .. 4r + 3 are four consecutive rows m of one
     // column n, i.e. four consecutive floats of C^T's row n: one 16-byte store
