@@ -36,7 +36,9 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 #ifndef MLP_BK
 #define MLP_BK 128
 #endif
+#ifndef MLP_NBUF
 #define MLP_NBUF (MLP_BK >= 128 ? 1 : 2)  // LDS buffers: one for deep slabs, two (ping-pong) for shallow ones
+#endif
 #define MLP_LDS_ROW (MLP_BK + 8)  // bf16 per LDS row (8 padding)
 #define MLP_KQ 32                 // K must be a multiple of this
 #define MLP_CH (MLP_BM * MLP_BK / 8 / 256)  // 16-byte pieces of A (and of B) per thread per slab
@@ -200,7 +202,8 @@ __global__ void __launch_bounds__(256) k_mlp_gemm(GemmArgs g) {
   __builtin_amdgcn_s_waitcnt(0x0F70);
   // a tile wholly inside C stores without per-element predicates
   const bool full = m0 + MLP_BM <= g.M && n0 + MLP_BN <= g.N;
-  if ((EPI == EPI_BF16_RELU || EPI == EPI_BF16_DRELU) && full && (g.ldc & 7) == 0 &&
+  // (the A buffer holds the 128 x 128 tile only when a slab row is at least 128 wide)
+  if ((EPI == EPI_BF16_RELU || EPI == EPI_BF16_DRELU) && MLP_BK >= MLP_BN && full && (g.ldc & 7) == 0 &&
       ((uintptr_t)g.C & 15) == 0) {
     // the 128 x 128 BF16 tile through LDS (the A slab buffer, free once every
     // wave is past its last MFMA), then out as 16-byte pieces of whole rows:
