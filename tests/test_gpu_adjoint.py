@@ -16,6 +16,8 @@ Checkers, none of them the adjoint kernels' own formulas:
   values the device uses.
 Tolerances are stated per test.
 """
+import os
+
 import pytest
 import torch
 
@@ -291,3 +293,42 @@ def test_advance_w_adjoint_matches_oracle_autograd(config):
         err = _rel(mine.grad.double().cpu(), ref.grad)
         print("d/d%s: %.2e" % (name, err))
         assert err <= 1e-4, name
+
+
+def test_adjoints_match_the_reference_autograd_gradients():
+    """Against the reference's own functions differentiated by torch autograd
+    (tests/golden/adjoint_golden.npz, make_adjoint_golden.py): advance_w's
+    gradient with respect to w, ph, t_2, t_2ave and advect_scalar's with
+    respect to the field, at the arguments of the first call of each in a
+    step of the 40 x 40 x 41 state with clouds, for the same seeded
+    sensitivities.  The arguments here come from the device's own step (the
+    reference's were formed by its functions on the host; they agree to the
+    last bits), so the bound is normwise 1e-4."""
+    import numpy as np
+    import pywrf_amd
+    from pywrf_amd import adjoint
+    from pywrf_amd.gen import synthetic_state
+    from pywrf_amd.step import rk3_step, RunConfig
+    gold = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "adjoint_golden.npz"))
+    s = synthetic_state("config1c", device="cuda")
+    rec = _FirstCall(_FirstCall(pywrf_amd.ops(), "advance_w"), "advect_scalar_fresh")
+    rk3_step(s, rec, RunConfig())
+    w_args = rec._ops.args
+    g = torch.Generator().manual_seed(7)
+    lT, lW, lP = (torch.randn(tuple(s.w.shape), generator=g).cuda() for _ in range(3))
+    W, T2A, T2, PH = adjoint.AdvanceW.W, adjoint.AdvanceW.T2AVE, adjoint.AdvanceW.T2, adjoint.AdvanceW.PH
+    x = [w_args[n].clone().requires_grad_() for n in (W, PH, T2, T2A)]
+    outs = adjoint.AdvanceW.apply(*x, w_args)
+    ((outs[0] * lT).sum() + (outs[1] * lW).sum() + (outs[2] * lP).sum()).backward()
+    for name, t in zip(("w", "ph", "t_2", "t_2ave"), x):
+        ref = torch.from_numpy(gold["advance_w.d" + name]).double()
+        err = _rel(t.grad.double().cpu(), ref)
+        print("advance_w d/d%s vs reference autograd: %.2e" % (name, err))
+        assert err <= 1e-4, name
+    a = rec.args
+    (lam,) = [torch.randn(tuple(s.t.shape), generator=torch.Generator().manual_seed(2)).cuda()]
+    gq = adjoint.advect_scalar_adjoint(lam, a[3], a[4], a[5], a[14], a[16], a[17], a[18], a[19], a[20],
+                                       tuple(a[21:39]))
+    err = _rel(gq.double().cpu(), torch.from_numpy(gold["advect_scalar.dfield"]).double())
+    print("advect_scalar d/dfield vs reference autograd: %.2e" % err)
+    assert err <= 1e-4
