@@ -665,6 +665,20 @@ int pywrf_advance_w_adjoint(const pywrf_grid_t* g, const float* lam_t2ave_out, c
                             double t0, double epssm, const float* rdnw, const float* rdn, const float* msfty,
                             float* lam_w, float* lam_ph, float* lam_t_2, float* lam_t2ave, float* work,
                             float* work2, pywrf_stream_t s);
+/* The adjoint of advance_uv (wrf_dynamics.py:3391-3525) with respect to u,
+ * v, ru_tend, rv_tend, p, ph, al, mu and mudf (alt, pb, php, cqu, cqv, muu,
+ * muv and the map factors held fixed): lam_u_out / lam_v_out are the
+ * sensitivities of the returned u and v; the nine outputs are overwritten
+ * (lam_mu, lam_mudf are planes).  cf: cf1, cf2, cf3 as three consecutive
+ * floats, as for pywrf_advance_uv. */
+int pywrf_advance_uv_adjoint(const pywrf_grid_t* g, const float* lam_u_out, const float* lam_v_out,
+                             const float* pb, const float* php, const float* alt, const float* muu,
+                             const float* cqu, const float* muv, const float* cqv, const float* msfux,
+                             const float* msfuy, const float* msfvx, const float* msfvx_inv, const float* msfvy,
+                             double rdx, double rdy, double dts, const float* cf, const float* fnm,
+                             const float* fnp, double emdiv, const float* rdnw, int spec_zone, float* lam_u,
+                             float* lam_v, float* lam_p, float* lam_ph, float* lam_al, float* lam_mu,
+                             float* lam_mudf, float* lam_ru_tend, float* lam_rv_tend, pywrf_stream_t s);
 /* The column tridiagonal solve of advance_w (wrf_dynamics.py:3758-3765), in
  * place on w over advance_w's columns, levels 1..kte-1 (level 0 is the lower
  * boundary value): the same operations in the same order. */

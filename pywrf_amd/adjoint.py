@@ -12,6 +12,10 @@ transpose of a linear part of the step:
   as a function of w, ph, t_2 and t_2ave.  Backward: pywrf_advance_w_adjoint
   (the damping, the transposed column solve, the explicit predictor and the
   geopotential right-hand side, in reverse).
+* AdvanceUV -- the horizontal acoustic step advance_uv
+  (wrf_dynamics.py:3391-3525) as a function of u, v, ru_tend, rv_tend, p, ph,
+  al, mu and mudf.  Backward: pywrf_advance_uv_adjoint (each u or v column
+  adds its sensitivities into the two columns its differences read).
 * TridiagSolve -- the column tridiagonal solve of advance_w
   (wrf_dynamics.py:3758-3765) as a function of its coefficients a, alpha,
   gamma and right-hand side.  Backward: pywrf_tridiag_solve_adjoint
@@ -158,6 +162,42 @@ class AdvanceW(torch.autograd.Function):
              stream())
         lw, lph, lt2, lt2a = out[:4]
         return lw, lph, lt2, lt2a, None
+
+
+class AdvanceUV(torch.autograd.Function):
+    """(u, v) = advance_uv(...) (wrf_dynamics.py:3391-3525) with the gradient
+    with respect to the inputs at DIFF (u, ru_tend, v, rv_tend, p, ph, al, mu,
+    mudf); every other argument is a coefficient held fixed.  args:
+    advance_uv's positional arguments in the reference's order."""
+
+    # positions in advance_uv's signature
+    DIFF = (0, 1, 2, 3, 4, 6, 9, 10, 15)
+
+    @staticmethod
+    def forward(ctx, u, ru_tend, v, rv_tend, p, ph, al, mu, mudf, args):
+        from . import wrf_dynamics
+        args = list(args)
+        for pos, t in zip(AdvanceUV.DIFF, (u, ru_tend, v, rv_tend, p, ph, al, mu, mudf)):
+            args[pos] = _f32(t)
+        ctx.args = args
+        return wrf_dynamics.advance_uv(*args)
+
+    @staticmethod
+    def backward(ctx, lu, lv):
+        from .wrf_dynamics import _cf
+        a = ctx.args
+        g = grid(a[0].shape, tuple(a[-18:]))
+        lu = torch.zeros_like(a[0]) if lu is None else _f32(lu)
+        lv = torch.zeros_like(a[2]) if lv is None else _f32(lv)
+        cf, _keep = _cf(a[32], a[33], a[34])
+        # outputs in the order u, v, p, ph, al, mu, mudf, ru_tend, rv_tend
+        out = [torch.empty_like(a[n]) for n in (0, 2, 4, 6, 9, 10, 15, 1, 3)]
+        call("pywrf_advance_uv_adjoint", g, ptr(lu), ptr(lv), ptr(a[5]), ptr(a[7]), ptr(a[8]), ptr(a[11]),
+             ptr(a[12]), ptr(a[13]), ptr(a[14]), ptr(a[24]), ptr(a[25]), ptr(a[26]), ptr(a[27]), ptr(a[28]),
+             float(a[29]), float(a[30]), float(a[31]), cf, ptr(a[35]), ptr(a[36]), float(a[37]), ptr(a[38]),
+             int(a[39]), *[ptr(t) for t in out], stream())
+        gu, gv, gp, gph, gal, gmu, gmudf, gru, grv = out
+        return gu, gru, gv, grv, gp, gph, gal, gmu, gmudf, None
 
 
 # ----------------------------------------------------------------------- tracer transport

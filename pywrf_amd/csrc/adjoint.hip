@@ -438,3 +438,131 @@ extern "C" int pywrf_advance_w_adjoint(const pywrf_grid_t* g, const float* lam_t
   PW_LAUNCH_COLS(k_advance_w_adj, all, s, dims_of(g), p);
   PW_RET();
 }
+
+// ------------------------------------------------------------------ advance_uv^T
+// advance_uv (wrf_dynamics.py:3391-3525) is linear in u, v, ru_tend, rv_tend,
+// p, ph, al, mu and mudf once alt, pb, php, cqu / cqv, the column masses
+// muu / muv and the map factors are fixed.  For a u point (k, j, i) of its box
+//   u += dts ru_tend - dts cqu dpx + mudf_x,
+//   dpx = A (ph[k+1,i] - ph[k+1,i-1] + ph[k,i] - ph[k,i-1]
+//            + (alt[k,i] + alt[k,i-1]) (p[k,i] - p[k,i-1]) + (al[k,i] + al[k,i-1]) (pb[k,i] - pb[k,i-1]))
+//       + B[k] (rdnw[k] (dpn[k+1] - dpn[k]) - 0.5 (mu[i-1] + mu[i])),
+//   A = msfux / msfuy 0.5 rdx muu,  B[k] = msfux / msfuy rdx (php[k,i] - php[k,i-1]),
+//   dpn[0] from p[0..2] with cf1..cf3, dpn[k] = 0.5 (fnm[k] (p[k,i] + p[k,i-1]) + fnp[k] (p[k-1,i] + p[k-1,i-1])),
+//   dpn[kde-1] = 0,  mudf_x = -emdiv dx (mudf[i] - mudf[i-1]) / msfuy;
+// v the same along j.  One thread per u (or v) column of the box walks its
+// levels and adds its sensitivities into the two columns it reads (atomic
+// adds: neighbouring columns share points); u, v pass through.
+struct AdjUV {
+  const float *lu, *lv, *alt, *pb, *php, *cqu, *cqv, *muu, *muv, *msfux, *msfuy, *msfvx, *msfvy, *msfvx_inv;
+  const float *fnm, *fnp, *rdnw, *cf;
+  float *gp, *gph, *gal, *gmu, *gmudf, *grut, *grvt;
+  float dts, rdx, rdy, emdiv, dx, dy;
+  int k0, k1, kde1;              // levels [k0, k1) of the box; dpn is zero at kde1 = kde - 1
+  Box bu, bv;                    // the u and v boxes (j, i ranges)
+};
+
+__device__ inline void atomic_add(float* p, float v) { if (v != 0.0f) atomicAdd(p, v); }
+
+template <bool U>
+__global__ void k_advance_uv_adj(Box b, Dims D, AdjUV a) {
+  int j, i;
+  if (!box_column(b, j, i)) return;
+  // the column this point's differences reach back to: i - 1 for u, j - 1 for v
+  const int jm = U ? j : j - 1, im = U ? i - 1 : i;
+  const float* lam = U ? a.lu : a.lv;
+  const float mf = U ? AT2(a.msfux, j, i) / AT2(a.msfuy, j, i) : AT2(a.msfvy, j, i) / AT2(a.msfvx, j, i);
+  const float rd = U ? a.rdx : a.rdy;
+  const float Acoef = mf * 0.5f * rd * (U ? AT2(a.muu, j, i) : AT2(a.muv, j, i));
+  const float* cq = U ? a.cqu : a.cqv;
+  float* grt = U ? a.grut : a.grvt;
+  float smu = 0.0f, sl = 0.0f, c_prev = 0.0f;
+  for (int k = a.k0; k < a.k1; ++k) {
+    const float l = AT(lam, k, j, i);
+    sl += l;
+    AT(grt, k, j, i) = a.dts * l;
+    const float G = -a.dts * AT(cq, k, j, i) * l;             // sensitivity of dp at this point
+    // the A terms: ph at k and k + 1, p with the alt sum, al with the pb difference
+    const float AG = Acoef * G;
+    atomic_add(&AT(a.gph, k + 1, j, i), AG);
+    atomic_add(&AT(a.gph, k + 1, jm, im), -AG);
+    atomic_add(&AT(a.gph, k, j, i), AG);
+    atomic_add(&AT(a.gph, k, jm, im), -AG);
+    const float salt = AT(a.alt, k, j, i) + AT(a.alt, k, jm, im);
+    atomic_add(&AT(a.gp, k, j, i), AG * salt);
+    atomic_add(&AT(a.gp, k, jm, im), -AG * salt);
+    const float dpb = AT(a.pb, k, j, i) - AT(a.pb, k, jm, im);
+    atomic_add(&AT(a.gal, k, j, i), AG * dpb);
+    atomic_add(&AT(a.gal, k, jm, im), AG * dpb);
+    // the B terms: rdnw[k] (dpn[k + 1] - dpn[k]) and the column mass
+    const float BG = mf * rd * (AT(a.php, k, j, i) - AT(a.php, k, jm, im)) * G;
+    smu += -0.5f * BG;
+    const float c_k = a.rdnw[k] * BG;
+    const float ldpn = c_prev - c_k;                          // sensitivity of dpn[k]
+    if (k == 0) {
+      for (int c = 0; c < 3; ++c) {                           // dpn[0] from p[0..2] with cf1..cf3
+        const float h = 0.5f * a.cf[c] * ldpn;
+        atomic_add(&AT(a.gp, c, j, i), h);
+        atomic_add(&AT(a.gp, c, jm, im), h);
+      }
+    } else if (k < a.kde1) {
+      const float hf = 0.5f * a.fnm[k] * ldpn, hb = 0.5f * a.fnp[k] * ldpn;
+      atomic_add(&AT(a.gp, k, j, i), hf);
+      atomic_add(&AT(a.gp, k, jm, im), hf);
+      atomic_add(&AT(a.gp, k - 1, j, i), hb);
+      atomic_add(&AT(a.gp, k - 1, jm, im), hb);
+    }
+    c_prev = c_k;                                             // dpn[k + 1] gets c_k (zero at the top)
+  }
+  atomic_add(&AT2(a.gmu, j, i), smu);
+  atomic_add(&AT2(a.gmu, jm, im), smu);
+  const float xy = -a.emdiv * (U ? a.dx / AT2(a.msfuy, j, i) : a.dy * AT2(a.msfvx_inv, j, i)) * sl;
+  atomic_add(&AT2(a.gmudf, j, i), xy);
+  atomic_add(&AT2(a.gmudf, jm, im), -xy);
+}
+
+// Outputs: lam_u / lam_v (= the sensitivities of the outputs: u and v pass
+// through), lam_p, lam_ph, lam_al, lam_mu, lam_mudf, lam_ru_tend,
+// lam_rv_tend; the library zeroes them first.
+__global__ void k_zero_n(float* p, long n) {
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) p[q] = 0.0f;
+}
+
+extern "C" int pywrf_advance_uv_adjoint(const pywrf_grid_t* g, const float* lam_u_out, const float* lam_v_out,
+                                        const float* pb, const float* php, const float* alt, const float* muu,
+                                        const float* cqu, const float* muv, const float* cqv, const float* msfux,
+                                        const float* msfuy, const float* msfvx, const float* msfvx_inv,
+                                        const float* msfvy, double rdx, double rdy, double dts, const float* cf,
+                                        const float* fnm, const float* fnp, double emdiv, const float* rdnw,
+                                        int spec_zone, float* lam_u, float* lam_v, float* lam_p, float* lam_ph,
+                                        float* lam_al, float* lam_mu, float* lam_mudf, float* lam_ru_tend,
+                                        float* lam_rv_tend, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
+  CHECK(g, 1);
+  if (g->kts != 0 || g->nz < 3 || g->kde > g->nz || g->kte > g->nz) return PYWRF_E_RANGE;
+  const long n3 = (long)g->nz * g->ny * g->nx, n2 = (long)g->ny * g->nx;
+  float* z3[] = {lam_p, lam_ph, lam_al, lam_ru_tend, lam_rv_tend};
+  for (float* z : z3) hipLaunchKernelGGL(k_zero_n, dim3(1024), dim3(256), 0, (hipStream_t)s, z, n3);
+  hipLaunchKernelGGL(k_zero_n, dim3(256), dim3(256), 0, (hipStream_t)s, lam_mu, n2);
+  hipLaunchKernelGGL(k_zero_n, dim3(256), dim3(256), 0, (hipStream_t)s, lam_mudf, n2);
+  if (hipError_t e = hipMemcpyAsync(lam_u, lam_u_out, n3 * sizeof(float), hipMemcpyDeviceToDevice, (hipStream_t)s))
+    return (int)e;
+  if (hipError_t e = hipMemcpyAsync(lam_v, lam_v_out, n3 * sizeof(float), hipMemcpyDeviceToDevice, (hipStream_t)s))
+    return (int)e;
+  AdjUV a;
+  a.lu = lam_u_out; a.lv = lam_v_out; a.alt = alt; a.pb = pb; a.php = php; a.cqu = cqu; a.cqv = cqv;
+  a.muu = muu; a.muv = muv; a.msfux = msfux; a.msfuy = msfuy; a.msfvx = msfvx; a.msfvy = msfvy;
+  a.msfvx_inv = msfvx_inv; a.fnm = fnm; a.fnp = fnp; a.rdnw = rdnw; a.cf = cf;
+  a.gp = lam_p; a.gph = lam_ph; a.gal = lam_al; a.gmu = lam_mu; a.gmudf = lam_mudf;
+  a.grut = lam_ru_tend; a.grvt = lam_rv_tend;
+  a.dts = F(dts); a.rdx = F(rdx); a.rdy = F(rdy); a.emdiv = F(emdiv); a.dx = F(1.0 / rdx); a.dy = F(1.0 / rdy);
+  a.k0 = g->kts; a.k1 = imin(g->kte, g->kde - 1); a.kde1 = g->kde - 1;
+  const int i0 = imax(g->its, g->ids + spec_zone), i1 = imin(g->ite, g->ide - spec_zone - 1);
+  const int j0 = imax(g->jts, g->jds + spec_zone), j1 = imin(g->jte, g->jde - spec_zone - 1);
+  const int iu1 = imin(g->ite, g->ide - spec_zone), jv1 = imin(g->jte, g->jde - spec_zone);
+  if (i0 < 1 || j0 < 1 || iu1 > g->nx || jv1 > g->ny || a.k1 + 1 > g->nz) return PYWRF_E_RANGE;
+  Box bu = mkbox(0, 1, j0, j1, i0, iu1), bv = mkbox(0, 1, j0, jv1, i0, i1);
+  PW_LAUNCH_COLS(k_advance_uv_adj<true>, bu, s, dims_of(g), a);
+  PW_LAUNCH_COLS(k_advance_uv_adj<false>, bv, s, dims_of(g), a);
+  PW_RET();
+}

@@ -295,6 +295,44 @@ def test_advance_w_adjoint_matches_oracle_autograd(config):
         assert err <= 1e-4, name
 
 
+@pytest.mark.parametrize("config", ["config1c", "bdy24"])
+def test_advance_uv_adjoint_matches_oracle_autograd(config):
+    """Gradients of <lU, u> + <lV, v> after advance_uv with respect to u,
+    ru_tend, v, rv_tend, p, ph, al, mu and mudf, at the arguments of the
+    step's first acoustic substep, against torch autograd through the
+    oracle's advance_uv in FP64 (the same coefficients).  The map is affine
+    in these inputs, so the check covers every term whatever their values.
+    Bound: 1e-4 normwise per gradient (FP32 rounding and the atomic adds'
+    order)."""
+    import oracle
+    import pywrf_amd
+    from pywrf_amd import adjoint
+    from pywrf_amd.gen import synthetic_state
+    from pywrf_amd.step import rk3_step, RunConfig
+    s = synthetic_state(config, device="cuda")
+    rec = _FirstCall(pywrf_amd.ops(), "advance_uv")
+    rk3_step(s, rec, RunConfig())
+    args = rec.args
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    lams = [torch.randn(args[n].shape, device="cuda", generator=gen) for n in (0, 2)]
+    D = adjoint.AdvanceUV.DIFF
+    x = [args[n].clone().requires_grad_() for n in D]
+    outs = adjoint.AdvanceUV.apply(*x, args)
+    ((outs[0] * lams[0]).sum() + (outs[1] * lams[1]).sum()).backward()
+    a64 = [a.detach().double().cpu() if isinstance(a, torch.Tensor) else a for a in args]
+    x64 = [a64[n].clone().requires_grad_() for n in D]
+    for n, t in zip(D, x64):
+        a64[n] = t
+    o64 = oracle.ops().advance_uv(*a64)
+    sum((o * l.double().cpu()).sum() for o, l in zip(o64, lams)).backward()
+    names = ("u", "ru_tend", "v", "rv_tend", "p", "ph", "al", "mu", "mudf")
+    for name, mine, ref in zip(names, x, x64):
+        assert float(ref.grad.abs().max()) > 0, name
+        err = _rel(mine.grad.double().cpu(), ref.grad)
+        print("d/d%s: %.2e" % (name, err))
+        assert err <= 1e-4, name
+
+
 def test_adjoints_match_the_reference_autograd_gradients():
     """Against the reference's own functions differentiated by torch autograd
     (tests/golden/adjoint_golden.npz, make_adjoint_golden.py): advance_w's
