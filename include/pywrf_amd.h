@@ -679,6 +679,19 @@ int pywrf_advance_uv_adjoint(const pywrf_grid_t* g, const float* lam_u_out, cons
                              const float* fnp, double emdiv, const float* rdnw, int spec_zone, float* lam_u,
                              float* lam_v, float* lam_p, float* lam_ph, float* lam_al, float* lam_mu,
                              float* lam_mudf, float* lam_ru_tend, float* lam_rv_tend, pywrf_stream_t s);
+/* The adjoint of advance_mu_t (wrf_dynamics.py:3528-3638) with respect to
+ * u, v, ww, mu, muave, muts, mudf, t, t_ave, ft and mu_tend (t_1 and the map
+ * factors held fixed; u_1, v_1, ww_1 enter additively): lam_ww .. lam_mu are
+ * the sensitivities of the returned ww, t, t_ave, muave, muts, mudf and mu;
+ * the eleven outputs are overwritten (the mu-like ones are planes). */
+int pywrf_advance_mu_t_adjoint(const pywrf_grid_t* g, const float* lam_ww, const float* lam_t,
+                               const float* lam_t_ave, const float* lam_muave, const float* lam_muts,
+                               const float* lam_mudf, const float* lam_mu, const float* t_1, const float* msftx,
+                               const float* msfty, const float* dnw, const float* fnm, const float* fnp,
+                               const float* rdnw, double rdx, double rdy, double dts, double epssm,
+                               float* lam_ww_ori, float* lam_u, float* lam_v, float* lam_mu_ori,
+                               float* lam_muave_ori, float* lam_muts_ori, float* lam_mudf_ori, float* lam_t_ori,
+                               float* lam_t_ave_ori, float* lam_ft, float* lam_mu_tend, pywrf_stream_t s);
 /* The column tridiagonal solve of advance_w (wrf_dynamics.py:3758-3765), in
  * place on w over advance_w's columns, levels 1..kte-1 (level 0 is the lower
  * boundary value): the same operations in the same order. */

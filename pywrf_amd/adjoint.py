@@ -16,6 +16,10 @@ transpose of a linear part of the step:
   (wrf_dynamics.py:3391-3525) as a function of u, v, ru_tend, rv_tend, p, ph,
   al, mu and mudf.  Backward: pywrf_advance_uv_adjoint (each u or v column
   adds its sensitivities into the two columns its differences read).
+* AdvanceMuT -- the column-mass and theta acoustic step advance_mu_t
+  (wrf_dynamics.py:3528-3638) as a function of u, v, ww, mu, muave, muts,
+  mudf, t, t_ave, ft and mu_tend.  Backward: pywrf_advance_mu_t_adjoint (the
+  eta-velocity recurrence as suffix sums down each column).
 * TridiagSolve -- the column tridiagonal solve of advance_w
   (wrf_dynamics.py:3758-3765) as a function of its coefficients a, alpha,
   gamma and right-hand side.  Backward: pywrf_tridiag_solve_adjoint
@@ -198,6 +202,47 @@ class AdvanceUV(torch.autograd.Function):
              int(a[39]), *[ptr(t) for t in out], stream())
         gu, gv, gp, gph, gal, gmu, gmudf, gru, grv = out
         return gu, gru, gv, grv, gp, gph, gal, gmu, gmudf, None
+
+
+class AdvanceMuT(torch.autograd.Function):
+    """The seven fresh results of advance_mu_t (wrf_dynamics.py:3528-3638):
+    (ww, t, t_ave, muave, muts, mudf, mu), with the gradient with respect to
+    the inputs at DIFF (u, v, ww, mu, muave, muts, mudf, t, t_ave, ft,
+    mu_tend); t_1, u_1, v_1, ww_1, mut and the map factors are held fixed.
+    The reference runs the mass updates under no_grad (its autograd would
+    leave them out); this is the full derivative.  args: advance_mu_t's
+    positional arguments in the reference's order."""
+
+    # positions in advance_mu_t's signature
+    DIFF = (2, 4, 0, 6, 8, 9, 12, 24, 26, 27, 28)
+    OUT = (0, 2, 3, 7, 8, 9, 10)            # positions of the fresh results in its 11-tuple
+
+    @staticmethod
+    def forward(ctx, u, v, ww, mu, muave, muts, mudf, t, t_ave, ft, mu_tend, args):
+        from . import wrf_dynamics
+        args = list(args)
+        for pos, x in zip(AdvanceMuT.DIFF, (u, v, ww, mu, muave, muts, mudf, t, t_ave, ft, mu_tend)):
+            args[pos] = _f32(x)
+        ctx.args = args
+        r = wrf_dynamics.advance_mu_t(*args)
+        return tuple(r[n] for n in AdvanceMuT.OUT)
+
+    @staticmethod
+    def backward(ctx, lww, lt, ltave, lmuave, lmuts, lmudf, lmu):
+        a = ctx.args
+        g = grid(a[0].shape, tuple(a[-18:]))
+        like = (a[0], a[24], a[26], a[8], a[9], a[12], a[6])
+        lam = [torch.zeros_like(x) if l is None else _f32(l)
+               for l, x in zip((lww, lt, ltave, lmuave, lmuts, lmudf, lmu), like)]
+        # (ww_ori, ww_1, u, u_1, v, v_1, mu_ori, mut, muave_ori, muts_ori, muu, muv, mudf_ori, c1h, c2h, c1f, c2f,
+        #  c3h, c4h, c3f, c4f, uam, vam, wwam, t_ori, t_1, t_ave_ori, ft, mu_tend, rdx, rdy, dts, epssm,
+        #  dnw, fnm, fnp, rdnw, msfux, msfuy, msfvx, msfvx_inv, msfvy, msftx, msfty, step, idx)
+        out = [torch.empty_like(a[n]) for n in (0, 2, 4, 6, 8, 9, 12, 24, 26, 27, 28)]
+        call("pywrf_advance_mu_t_adjoint", g, *[ptr(x) for x in lam], ptr(a[25]), ptr(a[42]), ptr(a[43]),
+             ptr(a[33]), ptr(a[34]), ptr(a[35]), ptr(a[36]), float(a[29]), float(a[30]), float(a[31]),
+             float(a[32]), *[ptr(x) for x in out], stream())
+        gww, gu, gv, gmu, gmuave, gmuts, gmudf, gt, gtave, gft, gmut = out
+        return gu, gv, gww, gmu, gmuave, gmuts, gmudf, gt, gtave, gft, gmut, None
 
 
 # ----------------------------------------------------------------------- tracer transport

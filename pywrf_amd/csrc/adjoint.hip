@@ -566,3 +566,119 @@ extern "C" int pywrf_advance_uv_adjoint(const pywrf_grid_t* g, const float* lam_
   PW_LAUNCH_COLS(k_advance_uv_adj<false>, bv, s, dims_of(g), a);
   PW_RET();
 }
+
+// ------------------------------------------------------------ advance_mu_t adjoint
+// The transpose of advance_mu_t (wrf_dynamics.py:3528-3638) with respect to u,
+// v, ww, mu, muave, muts, mudf, t, t_ave, ft and mu_tend, with t_1 and the
+// map factors held fixed (u_1, v_1 and ww_1 enter additively).  On a column
+// (j, i) of the box, levels k < k1 = kte - 1:
+//   dvdxi[k] = msftx msfty (rdy (v[k,j+1] - v[k,j]) + rdx (u[k,i+1] - u[k,i]) + terms in u_1, v_1),
+//   S = sum_k dnw[k] dvdxi[k] + mu_tend,  mu' = mu + dts S,  mudf' = S,  muts' = mut + mu',
+//   muave' = 0.5 ((1 + epssm) mu' + (1 - epssm) mu),
+//   ww'[0] = ww[0] - ww_1[0],  ww'[k] = ww[0] - sum_{m<k} dnw[m] (S + dvdxi[m]) / msfty - ww_1[k],
+//   wdtn[k] = ww'[k] (fnm[k] t_1[k] + fnp[k] t_1[k-1]) (0 < k < k1; 0 at 0 and k1),
+//   t_ave' = t,  t' = t + msfty dts ft - dts msfty (msftx (0.5 rdy (v[k,j+1] (t_1[j+1] + t_1) - v (t_1 + t_1[j-1]))
+//                    + 0.5 rdx (u[i+1] (t_1[i+1] + t_1) - u (t_1 + t_1[i-1]))) + rdnw[k] (wdtn[k+1] - wdtn[k])).
+// Outside the box every output is its input.  One thread per column: a first
+// pass down the column forms the suffix sums of the ww sensitivities (through
+// wdtn) and the total sensitivity of S, a second pass the sensitivities of
+// dvdxi and t, added into u and v with atomics (neighbouring columns share
+// faces).
+struct AdjMuT {
+  const float *lww, *lt, *ltave, *lmuave, *lmuts, *lmudf, *lmu;
+  const float *t1, *msftx, *msfty, *dnw, *fnm, *fnp, *rdnw;
+  float *gww, *gu, *gv, *gmu, *gmuave, *gmuts, *gmudf, *gt, *gtave, *gft, *gmut;
+  float rdx, rdy, dts, epssm;
+  int k1;
+};
+
+__device__ inline float mut_lw(const AdjMuT& a, const Dims& D, int k, int j, int i, float c) {
+  // the sensitivity of ww'[k]: its own and, through wdtn[k] (0 < k < k1), t's
+  float lw = AT(a.lww, k, j, i);
+  if (k >= 1) {
+    const float lwdtn = c * (a.rdnw[k] * AT(a.lt, k, j, i) - a.rdnw[k - 1] * AT(a.lt, k - 1, j, i));
+    lw += lwdtn * (a.fnm[k] * AT(a.t1, k, j, i) + a.fnp[k] * AT(a.t1, k - 1, j, i));
+  }
+  return lw;
+}
+
+__global__ void k_advance_mu_t_adj(Box b, Dims D, AdjMuT a) {
+  int j, i;
+  if (!box_column(b, j, i)) return;
+  const float mfx = AT2(a.msftx, j, i), mfy = AT2(a.msfty, j, i);
+  const float e = a.epssm;
+  const float lmn = AT2(a.lmu, j, i) + AT2(a.lmuts, j, i) + 0.5f * (1.0f + e) * AT2(a.lmuave, j, i);
+  AT2(a.gmu, j, i) = lmn + 0.5f * (1.0f - e) * AT2(a.lmuave, j, i);
+  AT2(a.gmuave, j, i) = 0.0f;
+  AT2(a.gmuts, j, i) = 0.0f;
+  AT2(a.gmudf, j, i) = 0.0f;
+  const float c = a.dts * mfy;
+  float R = 0.0f, A = 0.0f;
+  for (int k = a.k1 - 1; k >= 0; --k) {
+    A += a.dnw[k] * R;                           // R = sum of the ww' sensitivities above k
+    R += mut_lw(a, D, k, j, i, c);
+  }
+  AT(a.gww, 0, j, i) = R;
+  for (int k = 1; k < a.k1; ++k) AT(a.gww, k, j, i) = 0.0f;
+  const float lS = a.dts * lmn + AT2(a.lmudf, j, i) - A / mfy;
+  AT2(a.gmut, j, i) = lS;
+  R = 0.0f;
+  for (int k = a.k1 - 1; k >= 0; --k) {
+    const float ldv = a.dnw[k] * lS - a.dnw[k] / mfy * R;
+    R += mut_lw(a, D, k, j, i, c);
+    const float lt = AT(a.lt, k, j, i);
+    AT(a.gt, k, j, i) = lt + AT(a.ltave, k, j, i);
+    AT(a.gtave, k, j, i) = 0.0f;
+    AT(a.gft, k, j, i) = mfy * a.dts * lt;
+    const float q = -a.dts * mfy * mfx * lt;
+    const float du = mfx * mfy * a.rdx * ldv, dv = mfx * mfy * a.rdy * ldv;
+    const float t0 = AT(a.t1, k, j, i);
+    atomic_add(&AT(a.gu, k, j, i + 1), du + q * 0.5f * a.rdx * (AT(a.t1, k, j, i + 1) + t0));
+    atomic_add(&AT(a.gu, k, j, i), -du - q * 0.5f * a.rdx * (t0 + AT(a.t1, k, j, i - 1)));
+    atomic_add(&AT(a.gv, k, j + 1, i), dv + q * 0.5f * a.rdy * (AT(a.t1, k, j + 1, i) + t0));
+    atomic_add(&AT(a.gv, k, j, i), -dv - q * 0.5f * a.rdy * (t0 + AT(a.t1, k, j - 1, i)));
+  }
+}
+
+extern "C" int pywrf_advance_mu_t_adjoint(const pywrf_grid_t* g, const float* lam_ww, const float* lam_t,
+                                          const float* lam_t_ave, const float* lam_muave, const float* lam_muts,
+                                          const float* lam_mudf, const float* lam_mu, const float* t_1,
+                                          const float* msftx, const float* msfty, const float* dnw,
+                                          const float* fnm, const float* fnp, const float* rdnw, double rdx,
+                                          double rdy, double dts, double epssm, float* lam_ww_ori, float* lam_u,
+                                          float* lam_v, float* lam_mu_ori, float* lam_muave_ori,
+                                          float* lam_muts_ori, float* lam_mudf_ori, float* lam_t_ori,
+                                          float* lam_t_ave_ori, float* lam_ft, float* lam_mu_tend,
+                                          pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
+  CHECK(g, 1);
+  if (g->kts != 0 || g->kte > g->nz || g->kte < 2) return PYWRF_E_RANGE;
+  const long n3 = (long)g->nz * g->ny * g->nx, n2 = (long)g->ny * g->nx;
+  const hipStream_t st = (hipStream_t)s;
+  // outside the box every output passes its input's sensitivity through
+  const struct { float* dst; const float* src; long n; } pass[] = {
+      {lam_ww_ori, lam_ww, n3}, {lam_t_ori, lam_t, n3}, {lam_t_ave_ori, lam_t_ave, n3},
+      {lam_mu_ori, lam_mu, n2}, {lam_muave_ori, lam_muave, n2}, {lam_muts_ori, lam_muts, n2},
+      {lam_mudf_ori, lam_mudf, n2}};
+  for (const auto& p : pass)
+    if (hipError_t e = hipMemcpyAsync(p.dst, p.src, p.n * sizeof(float), hipMemcpyDeviceToDevice, st)) return (int)e;
+  hipLaunchKernelGGL(k_zero_n, dim3(1024), dim3(256), 0, st, lam_u, n3);
+  hipLaunchKernelGGL(k_zero_n, dim3(1024), dim3(256), 0, st, lam_v, n3);
+  hipLaunchKernelGGL(k_zero_n, dim3(1024), dim3(256), 0, st, lam_ft, n3);
+  hipLaunchKernelGGL(k_zero_n, dim3(256), dim3(256), 0, st, lam_mu_tend, n2);
+  AdjMuT a;
+  a.lww = lam_ww; a.lt = lam_t; a.ltave = lam_t_ave; a.lmuave = lam_muave; a.lmuts = lam_muts;
+  a.lmudf = lam_mudf; a.lmu = lam_mu; a.t1 = t_1; a.msftx = msftx; a.msfty = msfty; a.dnw = dnw;
+  a.fnm = fnm; a.fnp = fnp; a.rdnw = rdnw;
+  a.gww = lam_ww_ori; a.gu = lam_u; a.gv = lam_v; a.gmu = lam_mu_ori; a.gmuave = lam_muave_ori;
+  a.gmuts = lam_muts_ori; a.gmudf = lam_mudf_ori; a.gt = lam_t_ori; a.gtave = lam_t_ave_ori; a.gft = lam_ft;
+  a.gmut = lam_mu_tend;
+  a.rdx = F(rdx); a.rdy = F(rdy); a.dts = F(dts); a.epssm = F(epssm);
+  a.k1 = g->kte - 1;
+  const int j0 = imax(g->jts, g->jds + 1), j1 = imin(g->jte, g->jde - 2);
+  const int i0 = imax(g->its, g->ids + 1), i1 = imin(g->ite, g->ide - 2);
+  if (j0 < 1 || i0 < 1 || j1 + 1 > g->ny || i1 + 1 > g->nx) return PYWRF_E_RANGE;
+  Box b = mkbox(0, 1, j0, j1, i0, i1);
+  PW_LAUNCH_COLS(k_advance_mu_t_adj, b, s, dims_of(g), a);
+  PW_RET();
+}

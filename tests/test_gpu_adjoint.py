@@ -333,6 +333,50 @@ def test_advance_uv_adjoint_matches_oracle_autograd(config):
         assert err <= 1e-4, name
 
 
+@pytest.mark.parametrize("config", ["config1c", "bdy24"])
+def test_advance_mu_t_adjoint_matches_oracle_differences(config):
+    """advance_mu_t's gradient with respect to each of u, v, ww, mu, muave,
+    muts, mudf, t, t_ave, ft and mu_tend, at the arguments of the step's first
+    acoustic substep, against FP64 differences of the oracle's advance_mu_t:
+    for a seeded direction d in one input, sum_o <lam_o, f_o(x + d) - f_o(x)>
+    (exact up to FP64 rounding: the map is affine in these inputs) against
+    <grad, d>.  Autograd through the oracle cannot serve here: like the
+    reference it updates the column masses under no_grad.  Bound: 1e-4
+    per input, relative to the larger of |difference| and the 2-norm of the
+    summands grad_i d_i (a sum over random signs can cancel far below the
+    size of its terms, and the FP32 gradient is exact only to the latter)."""
+    import oracle
+    import pywrf_amd
+    from pywrf_amd import adjoint
+    from pywrf_amd.gen import synthetic_state
+    from pywrf_amd.step import rk3_step, RunConfig
+    s = synthetic_state(config, device="cuda")
+    rec = _FirstCall(pywrf_amd.ops(), "advance_mu_t")
+    rk3_step(s, rec, RunConfig())
+    args = rec.args
+    D, OUT = adjoint.AdvanceMuT.DIFF, adjoint.AdvanceMuT.OUT
+    gen = torch.Generator().manual_seed(13)
+    lams = [torch.randn(tuple(args[n].shape), generator=gen, dtype=torch.float64) for n in (0, 24, 26, 8, 9, 12, 6)]
+    x = [args[n].clone().requires_grad_() for n in D]
+    outs = adjoint.AdvanceMuT.apply(*x, args)
+    sum((o * l.float().cuda()).sum() for o, l in zip(outs, lams)).backward()
+    a64 = [a.detach().double().cpu() if isinstance(a, torch.Tensor) else a for a in args]
+    base = oracle.ops().advance_mu_t(*a64)
+    names = ("u", "v", "ww", "mu", "muave", "muts", "mudf", "t", "t_ave", "ft", "mu_tend")
+    for name, pos, mine in zip(names, D, x):
+        d = torch.randn(tuple(a64[pos].shape), generator=gen, dtype=torch.float64)
+        b = list(a64)
+        b[pos] = a64[pos] + d
+        moved = oracle.ops().advance_mu_t(*b)
+        ref = sum(float(((moved[n] - base[n]) * l).sum()) for n, l in zip(OUT, lams))
+        terms = mine.grad.double().cpu() * d
+        got = float(terms.sum())
+        scale = max(abs(ref), float(terms.norm()))      # a random-direction sum can cancel far below its terms
+        err = abs(got - ref) / scale
+        print("d/d%s: <grad, d> %.6e, difference %.6e, rel %.2e" % (name, got, ref, err))
+        assert err <= 1e-4, name
+
+
 def test_adjoints_match_the_reference_autograd_gradients():
     """Against the reference's own functions differentiated by torch autograd
     (tests/golden/adjoint_golden.npz, make_adjoint_golden.py): advance_w's
