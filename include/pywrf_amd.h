@@ -536,6 +536,12 @@ int pywrf_wsm6_columns(const pywrf_grid_t* g, float* th, float* q, float* qc, fl
  * counterpart in the reference (the surrogate is new). */
 int pywrf_mlp_gemm(int epi, int M, int N, int K, const void* A, int lda, const void* B, int ldb, void* C, int ldc,
                    const float* bias, const void* H, int ldh, int ksplit, pywrf_stream_t s);
+/* W [N x K] (BF16, row-major, ldw) rewritten in the B-operand fragment order
+ * of v_mfma_f32_32x32x16_bf16: for row tile nt of 32 rows and k-step ks of 16
+ * columns, lane l's 8 values W[32 nt + l % 32][16 ks + 8 (l / 32) + 0..7]
+ * at out[((nt (K / 16) + ks) 64 + l) 8]; rows at or beyond N are zero.  out
+ * holds ceil(N / 32) 32 K BF16.  K a multiple of 16. */
+int pywrf_mlp_frag_pack(int N, int K, const void* W, int ldw, void* out, pywrf_stream_t s);
 /* out[c][r] = in[r][c], BF16. */
 int pywrf_mlp_transpose(int R, int C, const void* in, int ldi, void* out, int ldo, pywrf_stream_t s);
 /* out[r] = sum_{c < C} in[r][c] (BF16 in, FP32 out): bias gradients. */
@@ -581,6 +587,11 @@ typedef struct pywrf_surrogate {
   const float* in_hi;
   const float* out_absmax;
   float qtol, wtol, envelope, dtmax;
+  /* w1, w2, w3 in the matrix cores' fragment order (pywrf_mlp_frag_pack):
+   * what the inline network reads */
+  const void* w1f;
+  const void* w2f;
+  const void* w3f;
 } pywrf_surrogate_t;
 
 /* Bytes of workspace pywrf_wsm6_surrogate needs for grid g (its physics tile)
@@ -601,10 +612,23 @@ int pywrf_wsm6_surrogate(const pywrf_grid_t* g, const pywrf_surrogate_t* net, fl
                          float* snowncv, float* sr, float* graupel, float* graupelncv, void* workspace,
                          long long workspace_bytes, int* nfallback, pywrf_stream_t s);
 
+/* The surrogate network's standardised outputs for every column of g's
+ * physics tile (no checks, nothing applied): Y[f * ldy + c], f < 288, c the
+ * column (row-major over the tile).  fused = 1: one launch that keeps the
+ * inputs and hidden layers on chip (the form pywrf_wsm6_surrogate runs;
+ * ldy a multiple of 4, Y 16-byte aligned); fused = 0: the inputs packed to
+ * memory and three GEMM launches, through `work` (2816 bytes per column,
+ * columns rounded up to 128).  Both give the same bits.  No counterpart in
+ * the reference. */
+int pywrf_wsm6_surrogate_outputs(const pywrf_grid_t* g, const pywrf_surrogate_t* net, const float* th,
+                                 const float* rho, const float* pii, const float* p, const float* qv, const float* qc,
+                                 const float* qr, const float* qi, const float* qs, const float* qg, float* Y, int ldy,
+                                 int fused, void* work, long long work_bytes, pywrf_stream_t s);
+
 /* pywrf_wsm6_surrogate without a workspace: no counterpart in the reference.
  * For callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid allocation): the
- * buffers are a static device array of 65.3 MB held on every GPU the library
+ * buffers are a static device array of 38.0 MB held on every GPU the library
  * runs on until the process ends.  Limit: at most 65536 tile columns (the
  * 240 x 240 domain has 51529); more return PYWRF_E_RANGE.  It calls no
  * allocator.  Calls that overlap on one GPU race (they share its array). */

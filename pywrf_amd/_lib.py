@@ -33,11 +33,12 @@ _GridPtr = ctypes.POINTER(Grid)
 
 class Surrogate(ctypes.Structure):
     """pywrf_surrogate_t: device pointers of the surrogate network and its
-    standardisation, and the fallback tolerances."""
+    standardisation, the fallback tolerances, and the weights in fragment
+    order."""
     _fields_ = [(n, ctypes.c_void_p) for n in
                 "w1 w2 w3 b1 b2 b3 in_mean in_rstd out_mean out_std in_lo in_hi out_absmax".split()] + [
                ("qtol", ctypes.c_float), ("wtol", ctypes.c_float), ("envelope", ctypes.c_float),
-               ("dtmax", ctypes.c_float)]
+               ("dtmax", ctypes.c_float)] + [(n, ctypes.c_void_p) for n in "w1f w2f w3f".split()]
 
 
 _CTYPES = {

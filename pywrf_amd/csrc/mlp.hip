@@ -302,6 +302,36 @@ extern "C" int pywrf_mlp_gemm(int epi, int M, int N, int K, const void* A, int l
   PW_RET();
 }
 
+// ------------------------------------------------------------------ fragment order
+// W [N x K] (row-major, ldw) rewritten in the order the B operand of
+// v_mfma_f32_32x32x16_bf16 is read: for row tile nt (32 rows) and k-step ks
+// (16 columns), lane l's 8 values W[32 nt + l % 32][16 ks + 8 (l / 32) + ..]
+// at out[((nt (K / 16) + ks) 64 + l) 8], so that a wave's fragment is one
+// contiguous 1 KB read.  Rows at or beyond N are zero.
+__global__ void k_mlp_frag_pack(int N, int K, const bf16* W, int ldw, bf16* out, long total) {
+  const int nks = K / 16;
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (long)gridDim.x * blockDim.x) {
+    const int l = (int)(q % 64);
+    const long fr = q / 64;
+    const int ks = (int)(fr % nks), nt = (int)(fr / nks);
+    const int row = 32 * nt + (l & 31), col = 16 * ks + 8 * (l >> 5);
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    if (row < N) v = *(const uint4*)(W + (long)row * ldw + col);
+    *(uint4*)(out + q * 8) = v;
+  }
+}
+
+extern "C" int pywrf_mlp_frag_pack(int N, int K, const void* W, int ldw, void* out, pywrf_stream_t s) {
+  if (N < 0 || K < 0 || K % 16 || ldw < K || ldw % 8 || !W || !out) return PYWRF_E_RANGE;
+  if (((uintptr_t)W & 15) || ((uintptr_t)out & 15) || (long)N * ldw >= (1L << 31)) return PYWRF_E_RANGE;
+  const long total = (long)((N + 31) / 32) * (K / 16) * 64;
+  if (total == 0) return 0;
+  const unsigned blocks = (unsigned)imin((int)((total + 255) / 256), 4096);
+  hipLaunchKernelGGL(k_mlp_frag_pack, dim3(blocks), dim3(256), 0, (hipStream_t)s, N, K, (const bf16*)W, ldw,
+                     (bf16*)out, total);
+  PW_RET();
+}
+
 // ------------------------------------------------------------------ transpose
 // out[c][r] = in[r][c] for a bf16 matrix, through 64 x 64 LDS tiles (reads and
 // writes along rows of 64 contiguous elements).  Entries of out's rows beyond

@@ -32,8 +32,9 @@
 // does not split it into snow and graupel, so snowncv = graupelncv = sr = 0).
 //
 // Columns are processed in chunks through a workspace: the fallback list
-// (int [ncol]), then packed inputs X (bf16 [chunk x 384]), hidden layers
-// (bf16 [chunk x 512] x 2) and outputs (f32, feature-major [288 x chunk]).
+// (int [ncol]), then the outputs (f32, feature-major [288 x chunk]); the
+// network runs as one launch per chunk (k_sg_fused: inputs and hidden layers
+// stay in LDS).
 #include <mutex>
 #include "common.h"
 
@@ -221,24 +222,234 @@ __global__ void __launch_bounds__(64 * SA_LG) k_sg_apply(Tile T, Fields f, const
   }
 }
 
+// The network in one launch: a 512-thread workgroup (8 waves) takes SF_M = 128
+// columns, packs their standardised inputs into LDS as BF16 rows (the
+// k_sg_pack values), and runs the three layers out of LDS: each layer's
+// output overwrites its input there (after a barrier) as the next layer's
+// BF16 operand, so the hidden activations never leave the CU.  Only the
+// outputs go to memory, feature-major as the GEMM's transposed epilogue
+// writes them.  Layers 1 and 2: wave w owns output features [64w, 64w + 64)
+// of all 128 columns (4 x 2 MFMA tiles of 32 x 32: the A fragments come from
+// LDS, each shared by two MFMAs, the B fragments -- L2-resident weights
+// rewritten in fragment order, so each is one contiguous 1 KB read -- straight
+// from global memory, each shared by four, fetched SF_KD k-steps ahead).
+// Layer 3 (288 = 9 x 32 features): wave w owns feature tile w of the four
+// column tiles, and waves 0..3 also tile 8 of column tile w.  Every 32 x 32
+// output tile is accumulated from zero over k ascending in 16-deep MFMA
+// steps, with the bias, ReLU and BF16 rounding of the GEMM epilogues, so the
+// outputs equal the three-GEMM path's bit for bit.
+#define SF_M 128
+#define SF_LP (SG_H + 8)      // LDS row pitch in bf16: rows 4 banks apart, 16-byte fragment reads conflict-free
+typedef __bf16 sf_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float sf_f32x16 __attribute__((ext_vector_type(16)));
+
+// B fragment of row tile nt, k-step ks from a weight in fragment order
+// (pywrf_mlp_frag_pack): one contiguous 1 KB read per wave
+__device__ inline sf_bf16x8 sf_frag(const bf16* Wf, int nks, int nt, int ks, int lane) {
+  return *(const sf_bf16x8*)(Wf + ((long)(nt * nks + ks) * 64 + lane) * 8);
+}
+
+// acc[t][u] += L[32 t + .., k] . W[n0 + 32 u + .., k] for k in [0, K), K a multiple of 32 SF_KD
+#define SF_KD 2               // k-steps of B fragments fetched ahead (1, 2 and 4 time alike)
+
+template <int NU>
+__device__ inline void sf_layer(const bf16* L, const bf16* Wf, int K, int n0, int lane, sf_f32x16 (&acc)[4][NU]) {
+  const int lr = lane & 31, lk = (lane >> 5) * 8, nks = K / 16, nt0 = n0 / 32;
+  sf_bf16x8 b0[SF_KD][NU], b1[SF_KD][NU];
+  auto fetch = [&](sf_bf16x8 (&b)[SF_KD][NU], int kb) {
+#pragma unroll
+    for (int s = 0; s < SF_KD; ++s)
+#pragma unroll
+      for (int u = 0; u < NU; ++u) b[s][u] = sf_frag(Wf, nks, nt0 + u, kb / 16 + s, lane);
+  };
+  auto run = [&](const sf_bf16x8 (&b)[SF_KD][NU], int kb) {
+#pragma unroll
+    for (int s = 0; s < SF_KD; ++s) {
+      sf_bf16x8 af[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) af[t] = *(const sf_bf16x8*)&L[(32 * t + lr) * SF_LP + kb + 16 * s + lk];
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int u = 0; u < NU; ++u)
+          acc[t][u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[t], b[s][u], acc[t][u], 0, 0, 0);
+    }
+  };
+  constexpr int D = 16 * SF_KD;
+  fetch(b0, 0);
+#pragma unroll 1
+  for (int kb = 0; kb < K; kb += 2 * D) {
+    fetch(b1, kb + D);
+    run(b0, kb);
+    if (kb + 2 * D < K) fetch(b0, kb + 2 * D);
+    run(b1, kb + D);
+  }
+}
+
+// acc -> bf16(relu(acc + bias)) into L[row][n0 + ..] (rows of the 4 column tiles)
+template <int NU>
+__device__ inline void sf_store_hidden(bf16* L, const float* bias, int n0, int lane, const sf_f32x16 (&acc)[4][NU]) {
+  const int lr = lane & 31;
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    const float bv = bias[n0 + 32 * u + lr];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int row = 32 * t + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+        float v = acc[t][u][e] + bv;
+        v = v > 0.0f ? v : 0.0f;
+        L[row * SF_LP + n0 + 32 * u + lr] = (bf16)v;
+      }
+  }
+}
+
+// column tile t's 32 x 32 output tile of features [n0, n0 + 32) + bias into Yt (feature-major)
+__device__ inline void sf_store_out(float* Yt, int ldy, int c0, int mv, const float* bias, int n0, int t, int lane,
+                                    const sf_f32x16& acc) {
+  const int n = n0 + (lane & 31);
+  const float bv = bias[n];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int m = 32 * t + 8 * r + 4 * (lane >> 5);
+    float* dst = Yt + (long)n * ldy + c0 + m;
+    if (m + 4 <= mv) {
+      float4 v;
+      v.x = acc[4 * r] + bv;
+      v.y = acc[4 * r + 1] + bv;
+      v.z = acc[4 * r + 2] + bv;
+      v.w = acc[4 * r + 3] + bv;
+      *(float4*)dst = v;
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if (m + e < mv) dst[e] = acc[4 * r + e] + bv;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(512) k_sg_fused(Tile T, Fields f, const float* mean, const float* rstd,
+                                                   const bf16* w1, const float* b1, const bf16* w2, const float* b2,
+                                                   const bf16* w3, const float* b3, int c_base, int m, float* Yt,
+                                                   int ldy) {
+  __shared__ __attribute__((aligned(16))) bf16 L[SF_M * SF_LP];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int c0 = (int)blockIdx.x * SF_M;                 // chunk-local first column of the tile
+  const int mv = imin(SF_M, m - c0);                     // columns of the tile in the chunk
+  // inputs: thread = (column tid % 128, level group tid / 128), as k_sg_pack
+  {
+    const int cl = tid & (SF_M - 1), grp = tid >> 7;
+    const bool live = cl < mv;
+    const int c = c_base + c0 + (live ? cl : 0);
+    const int j = T.j0 + c / T.ni, i = T.i0 + c % T.ni;
+    bf16* row = L + cl * SF_LP;
+    for (int k = grp; k < SG_LEV; k += 4) {
+      const bool on = live && k < T.km;
+      const long q = T.D.ix(on ? k : 0, j, i);
+      float v[SG_NIN];
+      const float th = f.th[q], pii = f.pii[q];
+      v[1] = f.p[q]; v[2] = f.rho[q]; v[3] = f.qv[q]; v[4] = f.qc[q]; v[5] = f.qr[q];
+      v[6] = f.qi[q]; v[7] = f.qs[q]; v[8] = f.qg[q];
+      v[0] = th * pii;
+#pragma unroll
+      for (int var = 0; var < SG_NIN; ++var) {
+        const int feat = var * SG_LEV + k;
+        row[feat] = (bf16)(on ? (v[var] - mean[feat]) * rstd[feat] : 0.0f);
+      }
+    }
+    for (int feat = SG_NIN * SG_LEV + grp; feat < SG_KIN; feat += 4) row[feat] = (bf16)0.0f;
+  }
+  __syncthreads();
+  sf_f32x16 acc[4][2];
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[t][u][e] = 0.0f;
+  sf_layer<2>(L, w1, SG_KIN, 64 * w, lane, acc);
+  __syncthreads();                                       // every wave is done reading the inputs
+  sf_store_hidden<2>(L, b1, 64 * w, lane, acc);
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[t][u][e] = 0.0f;
+  sf_layer<2>(L, w2, SG_H, 64 * w, lane, acc);
+  __syncthreads();
+  sf_store_hidden<2>(L, b2, 64 * w, lane, acc);
+  __syncthreads();
+  // layer 3: feature tile w of the four column tiles, and for waves 0..3 tile 8 of column tile w
+  sf_f32x16 o[4][1], ox;
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int e = 0; e < 16; ++e) o[t][0][e] = 0.0f;
+#pragma unroll
+  for (int e = 0; e < 16; ++e) ox[e] = 0.0f;
+  {
+    const int lr = lane & 31, lk = (lane >> 5) * 8;
+    const bool extra = w < 4;                            // uniform across the wave
+    const int xr = 32 * (extra ? w : 0) + lr;            // the extra tile's column rows
+#pragma unroll 2
+    for (int ks = 0; ks < SG_H; ks += 16) {
+      const sf_bf16x8 b = sf_frag(w3, SG_H / 16, w, ks / 16, lane);
+      const sf_bf16x8 bx = sf_frag(w3, SG_H / 16, 8, ks / 16, lane);
+      sf_bf16x8 af[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) af[t] = *(const sf_bf16x8*)&L[(32 * t + lr) * SF_LP + ks + lk];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) o[t][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[t], b, o[t][0], 0, 0, 0);
+      if (extra) {
+        const sf_bf16x8 ax = *(const sf_bf16x8*)&L[xr * SF_LP + ks + lk];
+        ox = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ax, bx, ox, 0, 0, 0);
+      }
+    }
+    if (c0 >= m) return;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) sf_store_out(Yt, ldy, c0, mv, b3, 32 * w, t, lane, o[t][0]);
+    if (extra) sf_store_out(Yt, ldy, c0, mv, b3, 8 * 32, w, lane, ox);
+  }
+}
+
 __global__ void k_sg_zero_int(int* p) { *p = 0; }
 
-static long sg_chunk_bytes(long chunk) {
-  return chunk * (SG_KIN * 2L + 2L * SG_H * 2L + SG_NO * 4L);
+// workspace bytes per column of a chunk: the outputs, f32 [288] (the fused
+// kernel keeps inputs and hidden layers on chip)
+static long sg_chunk_bytes(long chunk) { return chunk * (SG_NO * 4L); }
+
+static int sg_tile(const pywrf_grid_t* g, Tile& T, long& ncol) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
+  const int km = (g->kte - 1) - g->kts;
+  if (g->kts != 0 || km < 4 || km > SG_LEV || g->kte > g->nz) return PYWRF_E_RANGE;
+  if (g->jts < 0 || g->its < 0 || g->jte - 1 > g->ny || g->ite - 1 > g->nx) return PYWRF_E_RANGE;
+  const int ni = (g->ite - 1) - g->its, nj = (g->jte - 1) - g->jts;
+  ncol = ni > 0 && nj > 0 ? (long)ni * nj : 0;
+  T.D = dims_of(g); T.j0 = g->jts; T.i0 = g->its; T.ni = ni; T.ncol = (int)ncol; T.km = km;
+  return 0;
+}
+
+static void sg_fused_launch(const Tile& T, const Fields& f, const pywrf_surrogate_t* net, int c_base, int m, float* Y,
+                            int ldy, pywrf_stream_t s) {
+  hipLaunchKernelGGL(k_sg_fused, dim3((unsigned)((m + SF_M - 1) / SF_M)), dim3(512), 0, (hipStream_t)s, T, f,
+                     net->in_mean, net->in_rstd, (const bf16*)net->w1f, net->b1, (const bf16*)net->w2f, net->b2,
+                     (const bf16*)net->w3f, net->b3, c_base, m, Y, ldy);
 }
 
 static int sg_run(const pywrf_grid_t* g, const pywrf_surrogate_t* net, float* th, const float* rho,
                   const float* pii, const float* p, const float* delz, double dt, float* qv, float* qc, float* qr,
                   float* qi, float* qs, float* qg, float* rain, float* rainncv, float* snow, float* snowncv, float* sr,
                   float* graupel, float* graupelncv, char* ws, long ws_bytes, int* nfall, pywrf_stream_t s) {
-  if (!pw_size_ok(g) || !net || !ws || !nfall) return PYWRF_E_RANGE;
-  const int km = (g->kte - 1) - g->kts;
-  if (g->kts != 0 || km < 4 || km > SG_LEV || g->kte > g->nz) return PYWRF_E_RANGE;
-  if (g->jts < 0 || g->its < 0 || g->jte - 1 > g->ny || g->ite - 1 > g->nx) return PYWRF_E_RANGE;
-  const int ni = (g->ite - 1) - g->its, nj = (g->jte - 1) - g->jts;
-  if (ni <= 0 || nj <= 0) return 0;
-  const long ncol = (long)ni * nj;
-  // workspace: fallback list and counter first, then the chunk buffers
+  if (!net || !ws || !nfall) return PYWRF_E_RANGE;
+  if (!net->w1f || !net->w2f || !net->w3f) return PYWRF_E_ARG;
+  Tile T;
+  long ncol;
+  if (int e = sg_tile(g, T, ncol)) return e;
+  if (ncol == 0) return 0;
+  // workspace: fallback list and counter first, then the chunk's outputs
   const long list_bytes = ((ncol + 1) * 4 + 255) / 256 * 256;
   const long avail = ws_bytes - list_bytes;
   long chunk = avail > 0 ? avail / sg_chunk_bytes(1) : 0;
@@ -246,13 +457,7 @@ static int sg_run(const pywrf_grid_t* g, const pywrf_surrogate_t* net, float* th
   if (chunk < 128) return PYWRF_E_RANGE;
   if (chunk > ncol) chunk = (ncol + 127) / 128 * 128;
   int* fall = (int*)ws;
-  char* b = ws + list_bytes;
-  bf16* X = (bf16*)b;
-  bf16* H1 = X + chunk * SG_KIN;
-  bf16* H2 = H1 + chunk * SG_H;
-  float* Y = (float*)(H2 + chunk * SG_H);
-  Tile T;
-  T.D = dims_of(g); T.j0 = g->jts; T.i0 = g->its; T.ni = ni; T.ncol = (int)ncol; T.km = km;
+  float* Y = (float*)(ws + list_bytes);
   Fields f;
   f.th = th; f.qv = qv; f.qc = qc; f.qr = qr; f.qi = qi; f.qs = qs; f.qg = qg;
   f.rho = rho; f.pii = pii; f.p = p; f.dz = delz;
@@ -265,13 +470,8 @@ static int sg_run(const pywrf_grid_t* g, const pywrf_surrogate_t* net, float* th
   hipLaunchKernelGGL(k_sg_zero_int, dim3(1), dim3(1), 0, (hipStream_t)s, nfall);
   for (long c0 = 0; c0 < ncol; c0 += chunk) {
     const int m = (int)(ncol - c0 < chunk ? ncol - c0 : chunk);
-    hipLaunchKernelGGL(k_sg_pack, dim3((unsigned)((m + SG_COLS - 1) / SG_COLS)), dim3(64 * SG_PW), 0, (hipStream_t)s, T, f,
-                       net->in_mean, net->in_rstd, (int)c0, m, X);
-    int e = pywrf_mlp_gemm(1, m, SG_H, SG_KIN, X, SG_KIN, net->w1, SG_KIN, H1, SG_H, net->b1, nullptr, 0, 1, s);
-    if (!e) e = pywrf_mlp_gemm(1, m, SG_H, SG_H, H1, SG_H, net->w2, SG_H, H2, SG_H, net->b2, nullptr, 0, 1, s);
     // the outputs feature-major (Y^T, [288][chunk]) for the column-per-thread update
-    if (!e) e = pywrf_mlp_gemm(4, m, SG_NO, SG_H, H2, SG_H, net->w3, SG_H, Y, (int)chunk, net->b3, nullptr, 0, 1, s);
-    if (e) return e;
+    sg_fused_launch(T, f, net, (int)c0, m, Y, (int)chunk, s);
     hipLaunchKernelGGL(k_sg_apply, dim3((unsigned)((m + 63) / 64)), dim3(64, SA_LG), 0, (hipStream_t)s, T, f, Y,
                        (int)chunk, net->out_mean, net->out_std, env, (float)dt, net->qtol, net->wtol, (int)c0, m,
                        fall, nfall);
@@ -279,6 +479,44 @@ static int sg_run(const pywrf_grid_t* g, const pywrf_surrogate_t* net, float* th
   // the plain scheme on the columns that failed a check
   return pywrf_wsm6_columns(g, th, qv, qc, qr, qi, qs, qg, rho, pii, p, delz, dt, rain, rainncv, snow, snowncv, sr,
                             graupel, graupelncv, fall, nfall, s);
+}
+
+// The network's outputs for every tile column, feature-major (Y[f * ldy + c]),
+// by the fused kernel (fused = 1) or by k_sg_pack and three GEMM launches
+// through `work` (fused = 0: packed inputs and both hidden layers, 2816 bytes
+// a column, columns rounded up to 128); the two give the same bits.
+extern "C" int pywrf_wsm6_surrogate_outputs(const pywrf_grid_t* g, const pywrf_surrogate_t* net, const float* th,
+                                            const float* rho, const float* pii, const float* p, const float* qv,
+                                            const float* qc, const float* qr, const float* qi, const float* qs,
+                                            const float* qg, float* Y, int ldy, int fused, void* work,
+                                            long long work_bytes, pywrf_stream_t s) {
+  if (!net || !Y) return PYWRF_E_RANGE;
+  Tile T;
+  long ncol;
+  if (int e = sg_tile(g, T, ncol)) return e;
+  if (ncol == 0) return 0;
+  if (ldy < ncol || (long)SG_NO * ldy >= (1L << 31)) return PYWRF_E_RANGE;
+  if (fused && (ldy % 4 || ((uintptr_t)Y & 15))) return PYWRF_E_RANGE;   // 16-byte stores of four columns
+  Fields f = {};
+  f.th = (float*)th; f.qv = (float*)qv; f.qc = (float*)qc; f.qr = (float*)qr; f.qi = (float*)qi;
+  f.qs = (float*)qs; f.qg = (float*)qg; f.rho = rho; f.pii = pii; f.p = p;
+  if (fused) {
+    if (!net->w1f || !net->w2f || !net->w3f) return PYWRF_E_ARG;
+    sg_fused_launch(T, f, net, 0, (int)ncol, Y, ldy, s);
+    PW_RET();
+  }
+  const long rows = (ncol + 127) / 128 * 128;
+  if (!work || work_bytes < rows * (SG_KIN * 2L + 2L * SG_H * 2L)) return PYWRF_E_RANGE;
+  bf16* X = (bf16*)work;
+  bf16* H1 = X + rows * SG_KIN;
+  bf16* H2 = H1 + rows * SG_H;
+  const int m = (int)ncol;
+  hipLaunchKernelGGL(k_sg_pack, dim3((unsigned)((m + SG_COLS - 1) / SG_COLS)), dim3(64 * SG_PW), 0, (hipStream_t)s,
+                     T, f, net->in_mean, net->in_rstd, 0, m, X);
+  int e = pywrf_mlp_gemm(1, m, SG_H, SG_KIN, X, SG_KIN, net->w1, SG_KIN, H1, SG_H, net->b1, nullptr, 0, 1, s);
+  if (!e) e = pywrf_mlp_gemm(1, m, SG_H, SG_H, H1, SG_H, net->w2, SG_H, H2, SG_H, net->b2, nullptr, 0, 1, s);
+  if (!e) e = pywrf_mlp_gemm(4, m, SG_NO, SG_H, H2, SG_H, net->w3, SG_H, Y, ldy, net->b3, nullptr, 0, 1, s);
+  return e;
 }
 
 extern "C" int pywrf_wsm6_surrogate_workspace(const pywrf_grid_t* g, int chunk, long long* bytes) {
@@ -303,14 +541,14 @@ extern "C" int pywrf_wsm6_surrogate(const pywrf_grid_t* g, const pywrf_surrogate
 // capture, serving loops): the same computation through a file-scope device
 // array instead of a caller's workspace.  It holds the fallback list of up to
 // SG_NA_MAX_COLS columns and the buffers of one chunk of SG_NA_CHUNK columns
-// (65.3 MB on every GPU the library runs on, for the life of the process).
+// (38.0 MB on every GPU the library runs on, for the life of the process).
 // Every call on a GPU uses that GPU's copy, so calls that overlap on one GPU
 // race; the fallback counter (the caller's *nfallback) is zeroed by a kernel
 // at the start of each call and the list is written before it is read.
 #define SG_NA_MAX_COLS 65536
-#define SG_NA_CHUNK 16384
+#define SG_NA_CHUNK 32768
 #define SG_NA_LIST_BYTES ((((long)SG_NA_MAX_COLS + 1) * 4 + 255) / 256 * 256)
-#define SG_NA_BYTES (SG_NA_LIST_BYTES + (long)SG_NA_CHUNK * (SG_KIN * 2 + 2 * SG_H * 2 + SG_NO * 4))
+#define SG_NA_BYTES (SG_NA_LIST_BYTES + (long)SG_NA_CHUNK * SG_NO * 4)
 static __device__ __align__(256) char scratch[SG_NA_BYTES];
 
 #define SG_NA_MAX_DEVICES 64

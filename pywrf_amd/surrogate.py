@@ -423,9 +423,18 @@ class Surrogate:
         return torch.cat(out)
 
     def struct(self):
+        """pywrf_surrogate_t of the current weights (with their fragment-order
+        copies, which this object keeps alive)."""
         s = _lib.Surrogate()
         net = self.net
         s.w1, s.w2, s.w3 = (t.data_ptr() for t in net.Wb)
+        self._frag = []
+        for W in net.Wb:
+            n, k = W.shape
+            f = torch.empty(((n + 31) // 32 * 32, k), dtype=torch.bfloat16, device=W.device)
+            call("pywrf_mlp_frag_pack", n, k, W.data_ptr(), W.stride(0), f.data_ptr(), stream())
+            self._frag.append(f)
+        s.w1f, s.w2f, s.w3f = (f.data_ptr() for f in self._frag)
         s.b1, s.b2, s.b3 = (t.data_ptr() for t in net.b)
         s.in_mean, s.in_rstd = self.in_mean.data_ptr(), self.in_rstd.data_ptr()
         s.out_mean, s.out_std = self.out_mean.data_ptr(), self.out_std.data_ptr()
