@@ -332,9 +332,10 @@ __device__ inline void sf_store_out(float* Yt, int ldy, int c0, int mv, const fl
 __global__ void __launch_bounds__(512) k_sg_fused(Tile T, Fields f, const float* mean, const float* rstd,
                                                    const bf16* w1, const float* b1, const bf16* w2, const float* b2,
                                                    const bf16* w3, const float* b3, int c_base, int m, float* Yt,
-                                                   int ldy) {
+                                                   int ldy, int* counter) {
   __shared__ __attribute__((aligned(16))) bf16 L[SF_M * SF_LP];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  if (counter && blockIdx.x == 0 && tid == 0) *counter = 0;   // the fallback count, for the apply launch after this one
   const int c0 = (int)blockIdx.x * SF_M;                 // chunk-local first column of the tile
   const int mv = imin(SF_M, m - c0);                     // columns of the tile in the chunk
   // inputs: thread = (column tid % 128, level group tid / 128), as k_sg_pack
@@ -394,10 +395,12 @@ __global__ void __launch_bounds__(512) k_sg_fused(Tile T, Fields f, const float*
     const int lr = lane & 31, lk = (lane >> 5) * 8;
     const bool extra = w < 4;                            // uniform across the wave
     const int xr = 32 * (extra ? w : 0) + lr;            // the extra tile's column rows
+    // the k-step's two B fragments, fetched one k-step ahead
+    sf_bf16x8 b = sf_frag(w3, SG_H / 16, w, 0, lane), bx = sf_frag(w3, SG_H / 16, 8, 0, lane);
 #pragma unroll 2
     for (int ks = 0; ks < SG_H; ks += 16) {
-      const sf_bf16x8 b = sf_frag(w3, SG_H / 16, w, ks / 16, lane);
-      const sf_bf16x8 bx = sf_frag(w3, SG_H / 16, 8, ks / 16, lane);
+      const int kn = ks + 16 < SG_H ? ks + 16 : ks;      // the last k-step fetches its own again (unused)
+      const sf_bf16x8 nb = sf_frag(w3, SG_H / 16, w, kn / 16, lane), nbx = sf_frag(w3, SG_H / 16, 8, kn / 16, lane);
       sf_bf16x8 af[4];
 #pragma unroll
       for (int t = 0; t < 4; ++t) af[t] = *(const sf_bf16x8*)&L[(32 * t + lr) * SF_LP + ks + lk];
@@ -407,6 +410,8 @@ __global__ void __launch_bounds__(512) k_sg_fused(Tile T, Fields f, const float*
         const sf_bf16x8 ax = *(const sf_bf16x8*)&L[xr * SF_LP + ks + lk];
         ox = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ax, bx, ox, 0, 0, 0);
       }
+      b = nb;
+      bx = nbx;
     }
     if (c0 >= m) return;
 #pragma unroll
@@ -414,8 +419,6 @@ __global__ void __launch_bounds__(512) k_sg_fused(Tile T, Fields f, const float*
     if (extra) sf_store_out(Yt, ldy, c0, mv, b3, 8 * 32, w, lane, ox);
   }
 }
-
-__global__ void k_sg_zero_int(int* p) { *p = 0; }
 
 // workspace bytes per column of a chunk: the outputs, f32 [288] (the fused
 // kernel keeps inputs and hidden layers on chip)
@@ -432,11 +435,12 @@ static int sg_tile(const pywrf_grid_t* g, Tile& T, long& ncol) {
   return 0;
 }
 
+// counter: an int set to zero by the launch (null: none)
 static void sg_fused_launch(const Tile& T, const Fields& f, const pywrf_surrogate_t* net, int c_base, int m, float* Y,
-                            int ldy, pywrf_stream_t s) {
+                            int ldy, int* counter, pywrf_stream_t s) {
   hipLaunchKernelGGL(k_sg_fused, dim3((unsigned)((m + SF_M - 1) / SF_M)), dim3(512), 0, (hipStream_t)s, T, f,
                      net->in_mean, net->in_rstd, (const bf16*)net->w1f, net->b1, (const bf16*)net->w2f, net->b2,
-                     (const bf16*)net->w3f, net->b3, c_base, m, Y, ldy);
+                     (const bf16*)net->w3f, net->b3, c_base, m, Y, ldy, counter);
 }
 
 static int sg_run(const pywrf_grid_t* g, const pywrf_surrogate_t* net, float* th, const float* rho,
@@ -467,11 +471,11 @@ static int sg_run(const pywrf_grid_t* g, const pywrf_surrogate_t* net, float* th
   Envelope env;
   env.lo = net->in_lo; env.hi = net->in_hi; env.amax = net->out_absmax; env.scale = net->envelope;
   env.dtmax = net->dtmax;
-  hipLaunchKernelGGL(k_sg_zero_int, dim3(1), dim3(1), 0, (hipStream_t)s, nfall);
   for (long c0 = 0; c0 < ncol; c0 += chunk) {
     const int m = (int)(ncol - c0 < chunk ? ncol - c0 : chunk);
-    // the outputs feature-major (Y^T, [288][chunk]) for the column-per-thread update
-    sg_fused_launch(T, f, net, (int)c0, m, Y, (int)chunk, s);
+    // the outputs feature-major (Y^T, [288][chunk]) for the column-per-thread
+    // update; the first launch also zeroes the fallback count
+    sg_fused_launch(T, f, net, (int)c0, m, Y, (int)chunk, c0 == 0 ? nfall : nullptr, s);
     hipLaunchKernelGGL(k_sg_apply, dim3((unsigned)((m + 63) / 64)), dim3(64, SA_LG), 0, (hipStream_t)s, T, f, Y,
                        (int)chunk, net->out_mean, net->out_std, env, (float)dt, net->qtol, net->wtol, (int)c0, m,
                        fall, nfall);
@@ -502,7 +506,7 @@ extern "C" int pywrf_wsm6_surrogate_outputs(const pywrf_grid_t* g, const pywrf_s
   f.qs = (float*)qs; f.qg = (float*)qg; f.rho = rho; f.pii = pii; f.p = p;
   if (fused) {
     if (!net->w1f || !net->w2f || !net->w3f) return PYWRF_E_ARG;
-    sg_fused_launch(T, f, net, 0, (int)ncol, Y, ldy, s);
+    sg_fused_launch(T, f, net, 0, (int)ncol, Y, ldy, nullptr, s);
     PW_RET();
   }
   const long rows = (ncol + 127) / 128 * 128;
