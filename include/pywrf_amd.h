@@ -595,15 +595,15 @@ typedef struct pywrf_surrogate {
 } pywrf_surrogate_t;
 
 /* Bytes of workspace pywrf_wsm6_surrogate needs for grid g (its physics tile)
- * when it processes `chunk` columns at a time. */
+ * (the fallback list: the network keeps everything else on chip); `chunk` is
+ * checked (>= 128) and otherwise unused. */
 int pywrf_wsm6_surrogate_workspace(const pywrf_grid_t* g, int chunk, long long* bytes);
 /* microphysics_driver / wsm6 (wrf_physics.py:94-134, 137-1214) replaced by the
  * surrogate: same fields, tile and outputs as pywrf_wsm6; every column that
  * fails the checks (negative mixing ratio, water gain, non-finite value, an
  * input or a tendency outside the training envelope) is recomputed by the
  * plain scheme, and their count is written to *nfallback
- * (device int).  workspace: device memory of at least the size above for
- * some chunk >= 128 (larger chunks, fewer launches).  No counterpart in the
+ * (device int).  workspace: device memory of at least the size above.  No counterpart in the
  * reference (its README, README.md:170-173, describes but does not ship the
  * learned coupling). */
 int pywrf_wsm6_surrogate(const pywrf_grid_t* g, const pywrf_surrogate_t* net, float* th, const float* rho,
@@ -628,7 +628,7 @@ int pywrf_wsm6_surrogate_outputs(const pywrf_grid_t* g, const pywrf_surrogate_t*
 /* pywrf_wsm6_surrogate without a workspace: no counterpart in the reference.
  * For callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid allocation): the
- * buffers are a static device array of 38.0 MB held on every GPU the library
+ * buffers are a static device array of 0.26 MB held on every GPU the library
  * runs on until the process ends.  Limit: at most 65536 tile columns (the
  * 240 x 240 domain has 51529); more return PYWRF_E_RANGE.  It calls no
  * allocator.  Calls that overlap on one GPU race (they share its array). */

@@ -31,10 +31,10 @@
 // surface precipitation of the step (rainncv, added to rainnc; the surrogate
 // does not split it into snow and graupel, so snowncv = graupelncv = sr = 0).
 //
-// Columns are processed in chunks through a workspace: the fallback list
-// (int [ncol]), then the outputs (f32, feature-major [288 x chunk]); the
-// network runs as one launch per chunk (k_sg_fused: inputs and hidden layers
-// stay in LDS).
+// The whole surrogate is one launch (k_sg_fused: inputs, hidden layers and
+// outputs stay in LDS, the checks and the update run in the same workgroup);
+// its only workspace is the fallback list (int [ncol + 1]), which the plain
+// scheme then takes.
 #include <mutex>
 #include "common.h"
 
@@ -114,16 +114,13 @@ __global__ void __launch_bounds__(64 * SG_PW) k_sg_pack(Tile T, Fields f, const 
   }
 }
 
-// The network's tendencies applied with the checks, on a workgroup of 64
-// columns x SA_LG level groups (16: three levels a wave; 8 groups ran 35 us
-// longer at 240 x 240 for want of waves in flight): lane = column (every load and store of a
-// level is a row segment of the field), wave = a group of SA_LPG consecutive
-// levels.  Each thread forms the updated values of its levels in registers,
-// runs the per-level checks and sums its levels' column water; the groups'
-// flags and sums meet in LDS (added in group order), and a column that
-// passes every check stores its values, one that fails any is appended to
-// the fallback list.  Yt holds the network's outputs feature-major,
-// [288][ldy].
+// The network's tendencies applied with the checks.  A column's levels are
+// taken in SA_LG groups of SA_LPG consecutive levels; for each group the
+// updated values are formed in registers, the per-level checks run and the
+// group's column water is summed (levels in order); the groups' flags and
+// sums then meet in LDS and are added in group order, and a column that
+// passes every check stores its values, one that fails any is appended to the
+// fallback list.
 struct Envelope {
   const float *lo, *hi, *amax;
   float scale, dtmax;
@@ -135,38 +132,38 @@ __device__ inline bool in_range(const Envelope& e, int feat, float x) {
   return x >= e.lo[feat] - w && x <= e.hi[feat] + w;
 }
 
-#ifndef SA_LG
-#define SA_LG 16                                   // level groups (waves) per workgroup
-#endif
-#define SA_LPG ((SG_LEV + SA_LG - 1) / SA_LG)      // levels per group
-__global__ void __launch_bounds__(64 * SA_LG) k_sg_apply(Tile T, Fields f, const float* Yt, int ldy,
-                                                          const float* omean, const float* ostd, Envelope env,
-                                                          float dt, float qtol, float wtol, int c_base, int m,
-                                                          int* fall, int* nfall) {
-  __shared__ float s_wo[SA_LG][64], s_wn[SA_LG][64];
-  __shared__ int s_bad[SA_LG][64];
-  const int tx = threadIdx.x, lg = threadIdx.y;
-  const int cl = (int)blockIdx.x * 64 + tx;
-  const bool live = cl < m;
-  const int c = c_base + (live ? cl : 0);
-  const int j = T.j0 + c / T.ni, i = T.i0 + c % T.ni;
-  const int k0 = lg * SA_LPG;
+struct SgApply {
+  const float *omean, *ostd;      // de-standardisation of the outputs
+  Envelope env;
+  float dt, qtol, wtol;
+  int *fall, *nfall;              // fallback list and its count
+};
+
+#define SA_LG 16                  // level groups of a column
+#define SA_LPG 3                  // levels per group (16 x 3 >= 41)
+
+// level group g of column (j, i): out(o) is the network's standardised output o = var * 41 + k
+template <class Out>
+__device__ inline void sg_group(const Tile& T, const Fields& f, const SgApply& a, int g, int j, int i, Out out,
+                                bool& bad, float& wo, float& wn, float (&nth)[SA_LPG],
+                                float (&nq)[SA_LPG][SG_NOUT - 1]) {
+  const Envelope& env = a.env;
   auto tend = [&](int v, int k) {
     const int o = v * SG_LEV + k;
-    return Yt[(long)o * ldy + (live ? cl : 0)] * ostd[o] + omean[o];
+    return out(o) * a.ostd[o] + a.omean[o];
   };
-  bool bad = false;
-  float wo = 0.0f, wn = 0.0f;
-  float nth[SA_LPG], nq[SA_LPG][SG_NOUT - 1];
+  bad = false;
+  wo = 0.0f;
+  wn = 0.0f;
 #pragma unroll
   for (int kk = 0; kk < SA_LPG; ++kk) {
-    const int k = k0 + kk;
-    if (k >= T.km) break;                      // uniform across the wave
+    const int k = g * SA_LPG + kk;
+    if (k >= T.km) break;
     const long q = T.D.ix(k, j, i);
     const float th = f.th[q], pii = f.pii[q], rho = f.rho[q], mass = rho * f.dz[q];
     const float old[SG_NOUT - 1] = {f.qv[q], f.qc[q], f.qr[q], f.qi[q], f.qs[q], f.qg[q]};
     const float t0 = th * pii, tt = tend(0, k);
-    const float tn = t0 + dt * tt;
+    const float tn = t0 + a.dt * tt;
     bad |= !(tn - tn == 0.0f) || !(fabsf(tn - t0) <= env.dtmax);   // NaN, infinite or too large
     bad |= !in_range(env, k, t0) || !in_range(env, 2 * SG_LEV + k, rho) || !(fabsf(tt) <= env.scale * env.amax[k]);
     nth[kk] = tn / pii;
@@ -174,8 +171,8 @@ __global__ void __launch_bounds__(64 * SA_LG) k_sg_apply(Tile T, Fields f, const
 #pragma unroll
     for (int v = 1; v < SG_NOUT; ++v) {
       const float tv = tend(v, k);
-      const float x = old[v - 1] + dt * tv;
-      bad |= !(x - x == 0.0f) || x < -qtol;
+      const float x = old[v - 1] + a.dt * tv;
+      bad |= !(x - x == 0.0f) || x < -a.qtol;
       bad |= !in_range(env, (v + 2) * SG_LEV + k, old[v - 1]) || !(fabsf(tv) <= env.scale * env.amax[v * SG_LEV + k]);
       nq[kk][v - 1] = x > 0.0f ? x : 0.0f;
       so += old[v - 1];
@@ -184,43 +181,35 @@ __global__ void __launch_bounds__(64 * SA_LG) k_sg_apply(Tile T, Fields f, const
     wo += mass * so;
     wn += mass * sn;
   }
-  s_wo[lg][tx] = wo;
-  s_wn[lg][tx] = wn;
-  s_bad[lg][tx] = bad ? 1 : 0;
-  __syncthreads();
-  float w_old = 0.0f, w_new = 0.0f;
-  int nbad = 0;
-#pragma unroll
-  for (int gq = 0; gq < SA_LG; ++gq) {
-    w_old += s_wo[gq][tx];
-    w_new += s_wn[gq][tx];
-    nbad += s_bad[gq][tx];
-  }
-  if (!live) return;
-  if (nbad || w_new - w_old > wtol * w_old) {
-    if (lg == 0) fall[atomicAdd(nfall, 1)] = c;
-    return;
-  }
+}
+
+// the stores of a passing column's level group g
+__device__ inline void sg_group_store(const Tile& T, const Fields& f, int g, int j, int i, const float (&nth)[SA_LPG],
+                                      const float (&nq)[SA_LPG][SG_NOUT - 1]) {
   float* dst[SG_NOUT - 1] = {f.qv, f.qc, f.qr, f.qi, f.qs, f.qg};
 #pragma unroll
   for (int kk = 0; kk < SA_LPG; ++kk) {
-    const int k = k0 + kk;
+    const int k = g * SA_LPG + kk;
     if (k >= T.km) break;
     const long q = T.D.ix(k, j, i);
     f.th[q] = nth[kk];
 #pragma unroll
     for (int v = 0; v < SG_NOUT - 1; ++v) dst[v][q] = nq[kk][v];
   }
-  if (lg == 0) {
-    const long q2 = T.D.ix2(j, i);
-    const float pr = w_old - w_new > 0.0f ? w_old - w_new : 0.0f;
-    f.rainncv[q2] = pr;
-    f.rain[q2] = f.rain[q2] + pr;
-    f.snowncv[q2] = 0.0f;
-    f.graupelncv[q2] = 0.0f;
-    f.sr[q2] = 0.0f;
-  }
 }
+
+// the column's surface precipitation of the step: its water loss
+__device__ inline void sg_precip(const Tile& T, const Fields& f, int j, int i, float w_old, float w_new) {
+  const long q2 = T.D.ix2(j, i);
+  const float pr = w_old - w_new > 0.0f ? w_old - w_new : 0.0f;
+  f.rainncv[q2] = pr;
+  f.rain[q2] = f.rain[q2] + pr;
+  f.snowncv[q2] = 0.0f;
+  f.graupelncv[q2] = 0.0f;
+  f.sr[q2] = 0.0f;
+}
+
+__global__ void k_sg_zero_int(int* p) { *p = 0; }
 
 // The network in one launch: a 512-thread workgroup (8 waves) takes SF_M = 128
 // columns, packs their standardised inputs into LDS as BF16 rows (the
@@ -329,13 +318,20 @@ __device__ inline void sf_store_out(float* Yt, int ldy, int c0, int mv, const fl
   }
 }
 
+// APPLY: the outputs go through LDS, 64 columns at a time (FP32, pitch
+// SF_FP), to the checks and the update (sg_group, the column's 16 level groups
+// taken by the 8 waves two each), and nothing but the updated fields, the
+// precipitation and the fallback list leave the CU.  Otherwise the outputs are
+// stored to Yt (feature-major, [288][ldy]).
+#define SF_FP (SG_NO + 1)     // odd pitch: a wave's 64 columns fall in distinct banks
+template <bool APPLY>
 __global__ void __launch_bounds__(512) k_sg_fused(Tile T, Fields f, const float* mean, const float* rstd,
                                                    const bf16* w1, const float* b1, const bf16* w2, const float* b2,
                                                    const bf16* w3, const float* b3, int c_base, int m, float* Yt,
-                                                   int ldy, int* counter) {
+                                                   int ldy, SgApply a) {
   __shared__ __attribute__((aligned(16))) bf16 L[SF_M * SF_LP];
+  static_assert(64 * SF_FP * 4 + 3 * SA_LG * 64 * 4 <= SF_M * SF_LP * 2, "the apply buffers fit in L");
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  if (counter && blockIdx.x == 0 && tid == 0) *counter = 0;   // the fallback count, for the apply launch after this one
   const int c0 = (int)blockIdx.x * SF_M;                 // chunk-local first column of the tile
   const int mv = imin(SF_M, m - c0);                     // columns of the tile in the chunk
   // inputs: thread = (column tid % 128, level group tid / 128), as k_sg_pack
@@ -413,16 +409,78 @@ __global__ void __launch_bounds__(512) k_sg_fused(Tile T, Fields f, const float*
       b = nb;
       bx = nbx;
     }
-    if (c0 >= m) return;
+    if (!APPLY) {
+      if (c0 >= m) return;
 #pragma unroll
-    for (int t = 0; t < 4; ++t) sf_store_out(Yt, ldy, c0, mv, b3, 32 * w, t, lane, o[t][0]);
-    if (extra) sf_store_out(Yt, ldy, c0, mv, b3, 8 * 32, w, lane, ox);
+      for (int t = 0; t < 4; ++t) sf_store_out(Yt, ldy, c0, mv, b3, 32 * w, t, lane, o[t][0]);
+      if (extra) sf_store_out(Yt, ldy, c0, mv, b3, 8 * 32, w, lane, ox);
+      return;
+    }
+  }
+  float* F = (float*)L;                                  // [64][SF_FP] outputs (+ bias) of half the columns
+  float* s_wo = F + 64 * SF_FP;                          // [SA_LG][64] group water before / after, flags
+  float* s_wn = s_wo + SA_LG * 64;
+  int* s_bad = (int*)(s_wn + SA_LG * 64);
+  const int lr = lane & 31;
+  const int col = tid & 63, grp = tid >> 6;             // apply: column of the half, level groups grp, grp + 8
+  const float bw = b3[32 * w + lr], bx = b3[8 * 32 + lr];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    __syncthreads();                                     // layer 3 done with L / the previous half done with F
+#pragma unroll
+    for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int r = 32 * tt + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+        F[r * SF_FP + 32 * w + lr] = o[2 * h + tt][0][e] + bw;
+      }
+    if (w < 4 && (w >> 1) == h)                          // the extra tile (column tile w, features 256..287)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int r = 32 * (w - 2 * h) + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
+        F[r * SF_FP + 8 * 32 + lr] = ox[e] + bx;
+      }
+    __syncthreads();
+    const int cl = 64 * h + col;
+    const bool live = cl < mv;
+    const int c = c_base + c0 + (live ? cl : 0);
+    const int j = T.j0 + c / T.ni, i = T.i0 + c % T.ni;
+    auto out = [&](int oi) { return F[col * SF_FP + oi]; };
+    bool bad0, bad1;
+    float wo0, wn0, wo1, wn1;
+    float nth0[SA_LPG], nq0[SA_LPG][SG_NOUT - 1], nth1[SA_LPG], nq1[SA_LPG][SG_NOUT - 1];
+    sg_group(T, f, a, grp, j, i, out, bad0, wo0, wn0, nth0, nq0);
+    sg_group(T, f, a, grp + 8, j, i, out, bad1, wo1, wn1, nth1, nq1);
+    s_wo[grp * 64 + col] = wo0;
+    s_wn[grp * 64 + col] = wn0;
+    s_bad[grp * 64 + col] = bad0 ? 1 : 0;
+    s_wo[(grp + 8) * 64 + col] = wo1;
+    s_wn[(grp + 8) * 64 + col] = wn1;
+    s_bad[(grp + 8) * 64 + col] = bad1 ? 1 : 0;
+    __syncthreads();
+    float w_old = 0.0f, w_new = 0.0f;
+    int nbad = 0;
+#pragma unroll
+    for (int gq = 0; gq < SA_LG; ++gq) {
+      w_old += s_wo[gq * 64 + col];
+      w_new += s_wn[gq * 64 + col];
+      nbad += s_bad[gq * 64 + col];
+    }
+    if (live) {
+      if (nbad || w_new - w_old > a.wtol * w_old) {
+        if (grp == 0) a.fall[atomicAdd(a.nfall, 1)] = c;
+      } else {
+        sg_group_store(T, f, grp, j, i, nth0, nq0);
+        sg_group_store(T, f, grp + 8, j, i, nth1, nq1);
+        if (grp == 0) sg_precip(T, f, j, i, w_old, w_new);
+      }
+    }
   }
 }
 
-// workspace bytes per column of a chunk: the outputs, f32 [288] (the fused
-// kernel keeps inputs and hidden layers on chip)
-static long sg_chunk_bytes(long chunk) { return chunk * (SG_NO * 4L); }
+// workspace bytes: the fallback list and its count (the fused kernel keeps
+// inputs, hidden layers and outputs on chip)
+static long sg_list_bytes(long ncol) { return ((ncol + 1) * 4 + 255) / 256 * 256; }
 
 static int sg_tile(const pywrf_grid_t* g, Tile& T, long& ncol) {
   if (!pw_size_ok(g)) return PYWRF_E_RANGE;
@@ -435,12 +493,18 @@ static int sg_tile(const pywrf_grid_t* g, Tile& T, long& ncol) {
   return 0;
 }
 
-// counter: an int set to zero by the launch (null: none)
+// apply: run the checks and the update in the same launch (Y unused); else
+// store the outputs to Y
 static void sg_fused_launch(const Tile& T, const Fields& f, const pywrf_surrogate_t* net, int c_base, int m, float* Y,
-                            int ldy, int* counter, pywrf_stream_t s) {
-  hipLaunchKernelGGL(k_sg_fused, dim3((unsigned)((m + SF_M - 1) / SF_M)), dim3(512), 0, (hipStream_t)s, T, f,
-                     net->in_mean, net->in_rstd, (const bf16*)net->w1f, net->b1, (const bf16*)net->w2f, net->b2,
-                     (const bf16*)net->w3f, net->b3, c_base, m, Y, ldy, counter);
+                            int ldy, const SgApply* apply, pywrf_stream_t s) {
+  const dim3 grid((unsigned)((m + SF_M - 1) / SF_M)), block(512);
+  const bf16 *w1 = (const bf16*)net->w1f, *w2 = (const bf16*)net->w2f, *w3 = (const bf16*)net->w3f;
+  if (apply)
+    hipLaunchKernelGGL(k_sg_fused<true>, grid, block, 0, (hipStream_t)s, T, f, net->in_mean, net->in_rstd, w1, net->b1,
+                       w2, net->b2, w3, net->b3, c_base, m, Y, ldy, *apply);
+  else
+    hipLaunchKernelGGL(k_sg_fused<false>, grid, block, 0, (hipStream_t)s, T, f, net->in_mean, net->in_rstd, w1,
+                       net->b1, w2, net->b2, w3, net->b3, c_base, m, Y, ldy, SgApply{});
 }
 
 static int sg_run(const pywrf_grid_t* g, const pywrf_surrogate_t* net, float* th, const float* rho,
@@ -453,33 +517,21 @@ static int sg_run(const pywrf_grid_t* g, const pywrf_surrogate_t* net, float* th
   long ncol;
   if (int e = sg_tile(g, T, ncol)) return e;
   if (ncol == 0) return 0;
-  // workspace: fallback list and counter first, then the chunk's outputs
-  const long list_bytes = ((ncol + 1) * 4 + 255) / 256 * 256;
-  const long avail = ws_bytes - list_bytes;
-  long chunk = avail > 0 ? avail / sg_chunk_bytes(1) : 0;
-  chunk = chunk / 128 * 128;
-  if (chunk < 128) return PYWRF_E_RANGE;
-  if (chunk > ncol) chunk = (ncol + 127) / 128 * 128;
+  if (ws_bytes < sg_list_bytes(ncol)) return PYWRF_E_RANGE;
   int* fall = (int*)ws;
-  float* Y = (float*)(ws + list_bytes);
   Fields f;
   f.th = th; f.qv = qv; f.qc = qc; f.qr = qr; f.qi = qi; f.qs = qs; f.qg = qg;
   f.rho = rho; f.pii = pii; f.p = p; f.dz = delz;
   f.rain = rain; f.rainncv = rainncv; f.snowncv = snowncv; f.sr = sr; f.graupelncv = graupelncv;
   (void)snow; (void)graupel;
   if (!net->in_lo || !net->in_hi || !net->out_absmax || !(net->envelope >= 1.0f)) return PYWRF_E_ARG;
-  Envelope env;
-  env.lo = net->in_lo; env.hi = net->in_hi; env.amax = net->out_absmax; env.scale = net->envelope;
-  env.dtmax = net->dtmax;
-  for (long c0 = 0; c0 < ncol; c0 += chunk) {
-    const int m = (int)(ncol - c0 < chunk ? ncol - c0 : chunk);
-    // the outputs feature-major (Y^T, [288][chunk]) for the column-per-thread
-    // update; the first launch also zeroes the fallback count
-    sg_fused_launch(T, f, net, (int)c0, m, Y, (int)chunk, c0 == 0 ? nfall : nullptr, s);
-    hipLaunchKernelGGL(k_sg_apply, dim3((unsigned)((m + 63) / 64)), dim3(64, SA_LG), 0, (hipStream_t)s, T, f, Y,
-                       (int)chunk, net->out_mean, net->out_std, env, (float)dt, net->qtol, net->wtol, (int)c0, m,
-                       fall, nfall);
-  }
+  SgApply a;
+  a.omean = net->out_mean; a.ostd = net->out_std;
+  a.env.lo = net->in_lo; a.env.hi = net->in_hi; a.env.amax = net->out_absmax; a.env.scale = net->envelope;
+  a.env.dtmax = net->dtmax;
+  a.dt = (float)dt; a.qtol = net->qtol; a.wtol = net->wtol; a.fall = fall; a.nfall = nfall;
+  hipLaunchKernelGGL(k_sg_zero_int, dim3(1), dim3(1), 0, (hipStream_t)s, nfall);
+  sg_fused_launch(T, f, net, 0, (int)ncol, nullptr, 0, &a, s);
   // the plain scheme on the columns that failed a check
   return pywrf_wsm6_columns(g, th, qv, qc, qr, qi, qs, qg, rho, pii, p, delz, dt, rain, rainncv, snow, snowncv, sr,
                             graupel, graupelncv, fall, nfall, s);
@@ -526,7 +578,7 @@ extern "C" int pywrf_wsm6_surrogate_outputs(const pywrf_grid_t* g, const pywrf_s
 extern "C" int pywrf_wsm6_surrogate_workspace(const pywrf_grid_t* g, int chunk, long long* bytes) {
   if (!g || chunk < 128 || !bytes) return PYWRF_E_ARG;
   const long ncol = (long)imax((g->ite - 1) - g->its, 0) * imax((g->jte - 1) - g->jts, 0);
-  *bytes = ((ncol + 1) * 4 + 255) / 256 * 256 + sg_chunk_bytes(chunk / 128 * 128);
+  *bytes = sg_list_bytes(ncol);
   return 0;
 }
 
@@ -544,15 +596,13 @@ extern "C" int pywrf_wsm6_surrogate(const pywrf_grid_t* g, const pywrf_surrogate
 // For callers that cannot allocate device memory (code under HIP graph
 // capture, serving loops): the same computation through a file-scope device
 // array instead of a caller's workspace.  It holds the fallback list of up to
-// SG_NA_MAX_COLS columns and the buffers of one chunk of SG_NA_CHUNK columns
-// (38.0 MB on every GPU the library runs on, for the life of the process).
+// SG_NA_MAX_COLS columns (0.26 MB on every GPU the library runs on, for the
+// life of the process).
 // Every call on a GPU uses that GPU's copy, so calls that overlap on one GPU
 // race; the fallback counter (the caller's *nfallback) is zeroed by a kernel
 // at the start of each call and the list is written before it is read.
 #define SG_NA_MAX_COLS 65536
-#define SG_NA_CHUNK 32768
-#define SG_NA_LIST_BYTES ((((long)SG_NA_MAX_COLS + 1) * 4 + 255) / 256 * 256)
-#define SG_NA_BYTES (SG_NA_LIST_BYTES + (long)SG_NA_CHUNK * SG_NO * 4)
+#define SG_NA_BYTES ((((long)SG_NA_MAX_COLS + 1) * 4 + 255) / 256 * 256)
 static __device__ __align__(256) char scratch[SG_NA_BYTES];
 
 #define SG_NA_MAX_DEVICES 64
