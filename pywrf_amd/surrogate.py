@@ -584,10 +584,16 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--out", required=True)
+    ap.add_argument("--out", help="where to save the trained network")
+    ap.add_argument("--load", help="a trained network: skip training, run only the comparison")
     ap.add_argument("--compare", type=int, default=60, help="steps of the 240x240x41 comparison (0: none)")
     a = ap.parse_args(argv)
     log = lambda *m: print(*m, file=sys.stderr, flush=True)
+    if a.load:
+        print(json.dumps({"compare_240": compare_runs(Surrogate.load(a.load), nsteps=a.compare, log=log)}), flush=True)
+        return
+    if not a.out:
+        ap.error("--out is required when training")
     data = collect_pairs(a.runs, a.steps, log=log)
     nval = max(a.batch, len(data) // 50)
     perm = torch.randperm(len(data), device=data.X.device, generator=torch.Generator(device="cuda").manual_seed(1))
