@@ -651,6 +651,13 @@ int pywrf_wsm6_surrogate_noalloc_limits(int* max_columns, long long* scratch_byt
  * host-side check, not of a function. */
 int pywrf_nan_scan(const pywrf_grid_t* g, const float* field, int code, int* flag, pywrf_stream_t s);
 int pywrf_nan_flag_clear(int* flag, pywrf_stream_t s);
+/* The boundary clock of a captured step (solver.py:457-458, 713, where the
+ * host advances it): step (int64) += 1; it_bdy (int64) = floor(step /
+ * bdy_interval) when step mod bdy_interval == 1; dtbc (float) = (step -
+ * it_bdy * bdy_interval) * dt formed in double, rounded once.  One-value
+ * device arrays, advanced by a kernel so that a graph of steps replays it.
+ * Not a function of the reference: its driver keeps these on the host. */
+int pywrf_clock_advance(void* step, void* it_bdy, float* dtbc, long long bdy_interval, double dt, pywrf_stream_t s);
 
 /* ---------------------------------------------------------------- adjoints
  * Transposes of linear parts of the step, for gradients through it.  The

@@ -359,3 +359,33 @@ extern "C" int pywrf_nan_flag_clear(int* flag, pywrf_stream_t s) {
   hipLaunchKernelGGL(k_nan_flag_clear, dim3(1), dim3(64), 0, (hipStream_t)s, flag);
   PW_RET();
 }
+
+// ------------------------------------------------------------------ boundary clock
+// The driver's step counter, boundary record index and boundary time
+// (solver.py:457-458, 713) advanced on the device at the start of a step, so
+// that a captured graph of steps forms each replay's dtbc itself: step += 1;
+// it_bdy = floor(step / bdy_interval) when step mod bdy_interval == 1 (mod and
+// division rounding towards minus infinity, as the host's Python integers);
+// dtbc = (step - it_bdy bdy_interval) dt formed in double and rounded to
+// float once, as the kernels take the host's Python float.
+__global__ void k_clock_advance(long long* step, long long* it_bdy, float* dtbc, long long n, double dt) {
+  if (threadIdx.x != 0) return;
+  const long long s = *step + 1;
+  long long q = s / n, r = s % n;
+  if (r != 0 && ((r < 0) != (n < 0))) {
+    r += n;
+    --q;
+  }
+  const long long it = r == 1 ? q : *it_bdy;
+  *step = s;
+  *it_bdy = it;
+  *dtbc = (float)((double)(s - it * n) * dt);
+}
+
+extern "C" int pywrf_clock_advance(void* step, void* it_bdy, float* dtbc, long long bdy_interval, double dt,
+                                   pywrf_stream_t s) {
+  if (!step || !it_bdy || !dtbc || bdy_interval <= 0) return PYWRF_E_ARG;
+  hipLaunchKernelGGL(k_clock_advance, dim3(1), dim3(64), 0, (hipStream_t)s, (long long*)step, (long long*)it_bdy,
+                     dtbc, bdy_interval, dt);
+  PW_RET();
+}

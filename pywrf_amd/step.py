@@ -90,8 +90,9 @@ class Grid:
 class DeviceClock:
     """The step counter, the boundary record index it_bdy and the boundary
     time dtbc = dt (step - it_bdy bdy_interval) (solver.py:457-458, 713) held
-    as device tensors and advanced by device operations at the start of each
-    step, so that a captured graph of steps forms each replay's dtbc itself.
+    as device tensors and advanced by one kernel (pywrf_clock_advance) at the
+    start of each step, so that a captured graph of steps forms each replay's
+    dtbc itself.
     dtbc is formed in double and rounded to float once, the value the
     kernels take for the host's Python float."""
 
@@ -107,11 +108,9 @@ class DeviceClock:
         self.it_bdy.fill_(s.it_bdy)
 
     def advance(self, cfg):
-        import torch
-        self.step.add_(1)
-        rec = torch.div(self.step, cfg.bdy_interval, rounding_mode="floor")
-        self.it_bdy.copy_(torch.where(torch.remainder(self.step, cfg.bdy_interval) == 1, rec, self.it_bdy))
-        self.dtbc.copy_(((self.step - self.it_bdy * cfg.bdy_interval).double() * cfg.dt).float())
+        from ._lib import call, stream
+        call("pywrf_clock_advance", self.step.data_ptr(), self.it_bdy.data_ptr(), self.dtbc.data_ptr(),
+             int(cfg.bdy_interval), float(cfg.dt), stream())
 
 
 def _nan_check(p, where):

@@ -60,3 +60,29 @@ def test_two_step_graph_matches_eager(config):
         for name in b:
             assert torch.equal(a[name], b[name]), (n, name)
     assert s.step == ref.step
+
+
+@pytest.mark.parametrize("interval", [1, 2, 3, 360])
+def test_device_clock_matches_the_host_counters(interval):
+    """pywrf_clock_advance against the driver's own host arithmetic
+    (solver.py:457-458, 713: itimestep += 1; it_bdy = int(itimestep /
+    bdy_interval) when itimestep % bdy_interval == 1; dtbc = dt * (itimestep -
+    it_bdy * bdy_interval), taken by the kernels as float32) for 12 steps from
+    a non-zero start."""
+    import numpy as np
+    from pywrf_amd.step import DeviceClock, RunConfig
+
+    class Host:
+        step, it_bdy = 5, 1
+
+    cfg = RunConfig(dt=54.0, bdy_interval=interval)
+    clock = DeviceClock(Host)
+    step, it_bdy = Host.step, Host.it_bdy
+    for _ in range(12):
+        step += 1
+        if step % interval == 1:
+            it_bdy = int(step / interval)
+        dtbc = np.float32(cfg.dt * (step - it_bdy * interval))
+        clock.advance(cfg)
+        assert int(clock.step) == step and int(clock.it_bdy) == it_bdy
+        assert np.float32(clock.dtbc.item()) == dtbc, (step, float(clock.dtbc), float(dtbc))
