@@ -9,7 +9,7 @@ transpose of a linear part of the step:
 * AdvectScalar -- advect_scalar (wrf_dynamics.py:1841-2028) as a function of
   the field, the mass fluxes held fixed.  Backward: pywrf_advect_scalar_adjoint.
 * AdvanceW -- the vertical acoustic step advance_w (wrf_dynamics.py:3641-3796)
-  as a function of w, ph, t_2 and t_2ave.  Backward: pywrf_advance_w_adjoint
+  as a function of w, ph, t_2, t_2ave, u, v, muts and muave.  Backward: pywrf_advance_w_adjoint
   (the damping, the transposed column solve, the explicit predictor and the
   geopotential right-hand side, in reverse).
 * AdvanceUV -- the horizontal acoustic step advance_uv
@@ -133,39 +133,50 @@ class TridiagSolve(torch.autograd.Function):
 # ----------------------------------------------------------------------- acoustic step: advance_w
 class AdvanceW(torch.autograd.Function):
     """(t_2ave, w, ph) = advance_w(...) (wrf_dynamics.py:3641-3796) with the
-    gradient with respect to w, ph, t_2 and t_2ave (the prognostic inputs;
-    every other argument is a coefficient held fixed).  args: advance_w's
+    gradient with respect to w, ph, t_2, t_2ave, u, v, muts and muave (every
+    other argument is a coefficient held fixed).  args: advance_w's
     positional arguments in the reference's order."""
 
     # positions in advance_w's signature
-    W, T2AVE, T2, PH = 0, 18, 19, 21
+    W, U, V, MUAVE, MUTS, T2AVE, T2, PH = 0, 4, 5, 8, 9, 18, 19, 21
+    DIFF = (W, PH, T2, T2AVE, U, V, MUTS, MUAVE)
 
     @staticmethod
-    def forward(ctx, w, ph, t_2, t_2ave, args):
+    def forward(ctx, w, ph, t_2, t_2ave, u, v, muts, muave, args):
         from . import wrf_dynamics
         args = list(args)
-        args[AdvanceW.W], args[AdvanceW.PH], args[AdvanceW.T2], args[AdvanceW.T2AVE] = (
-            _f32(w), _f32(ph), _f32(t_2), _f32(t_2ave))
+        for pos, x in zip(AdvanceW.DIFF, (w, ph, t_2, t_2ave, u, v, muts, muave)):
+            args[pos] = _f32(x)
         ctx.args = args
         t2a, wo, pho = wrf_dynamics.advance_w(*args)
+        ctx.outs = (t2a, wo)
         return t2a, wo, pho
 
     @staticmethod
     def backward(ctx, lT, lW, lP):
+        from .wrf_dynamics import _cf
         a = ctx.args
         g = grid(a[0].shape, tuple(a[-18:]))
         z = lambda t: torch.zeros_like(a[0]) if t is None else _f32(t)
         lT, lW, lP = z(lT), z(lW), z(lP)
-        out = [torch.empty_like(a[0]) for _ in range(6)]
+        cf, _keep = _cf(a[43], a[44], a[45])
+        out = [torch.empty_like(a[0]) for _ in range(4)]
+        uv = [torch.empty_like(a[4]), torch.empty_like(a[5])]
+        planes = [torch.empty_like(a[9]), torch.empty_like(a[8])]
+        work = [torch.empty_like(a[0]) for _ in range(2)]
+        t2a_out, w_out = ctx.outs
         # advance_w(w, rw_tend, ww, w_save, u, v, mu1, mut, muave, muts, c1h, c2h, c1f, c2f, c3h, c4h, c3f, c4f,
         #           t_2ave, t_2, t_1, ph, ph_1, phb, ph_tend, ht, c2a, cqw, alt, alb, a, alpha, gamma,
         #           rdx, rdy, dts, t0, epssm, dnw, fnm, fnp, rdnw, rdn, cf1, cf2, cf3, msftx, msfty, top_lid, idx)
         call("pywrf_advance_w_adjoint", g, ptr(lT), ptr(lW), ptr(lP), ptr(a[7]), ptr(a[9]), ptr(a[20]),
              ptr(a[22]), ptr(a[23]), ptr(a[26]), ptr(a[27]), ptr(a[28]), ptr(a[30]), ptr(a[31]), ptr(a[32]),
-             float(a[35]), float(a[36]), float(a[37]), ptr(a[41]), ptr(a[42]), ptr(a[47]), *[ptr(t) for t in out],
-             stream())
-        lw, lph, lt2, lt2a = out[:4]
-        return lw, lph, lt2, lt2a, None
+             float(a[35]), float(a[36]), float(a[37]), ptr(a[41]), ptr(a[42]), ptr(a[47]), ptr(t2a_out),
+             ptr(w_out), ptr(a[25]), ptr(a[46]), cf, float(a[33]), float(a[34]), *[ptr(t) for t in out],
+             *[ptr(t) for t in uv], *[ptr(t) for t in planes], *[ptr(t) for t in work], stream())
+        lw, lph, lt2, lt2a = out
+        lu, lv = uv
+        lmuts, lmuave = planes
+        return lw, lph, lt2, lt2a, lu, lv, lmuts, lmuave, None
 
 
 class AdvanceUV(torch.autograd.Function):

@@ -303,25 +303,37 @@ extern "C" int pywrf_tridiag_solve_adjoint(const pywrf_grid_t* g, const float* a
   PW_RET();
 }
 
+// adds v to *p (atomically: neighbouring columns share points); zero adds are skipped
+__device__ inline void atomic_add(float* p, float v) { if (v != 0.0f) atomicAdd(p, v); }
+
+__global__ void k_zero_n(float* p, long n) {
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) p[q] = 0.0f;
+}
+
 // ------------------------------------------------------------------ advance_w^T
-// advance_w (wrf_dynamics.py:3641-3796) is linear in its prognostic inputs
-// w, ph, t_2 and t_2ave once the coefficients (column masses, c2a, cqw, alt,
-// the tridiagonal factors, the damping weights from ph_1) are fixed:
+// advance_w (wrf_dynamics.py:3641-3796) with respect to w, ph, t_2, t_2ave,
+// u, v, muts and muave (the coefficients -- mut, c2a, cqw, alt, the
+// tridiagonal factors, the damping weights from ph_1 -- fixed).  It is linear
+// in all but muts, whose derivatives take the forward's outputs t_2ave and w:
 //   t2a[k]  = (0.5 ((1+e) t_2[k] + (1-e) t2a_in[k]) + muave t0) / (muts (t0 + t_1[k])),  k < K
 //   rhs[k]  = ph[k] + msfty dts (ph_tend[k] + 0.5 g (1-e) w[k] - ...) / mut,            1 <= k <= K
 //   wp[k]   = w[k] + ... (rhs, ph at k-1, k, k+1; t2a at k-1, k),                      1 <= k <= K
-//   ws      = the column solve of wp (w[0] from the terrain, fixed),
+//   ws      = the column solve of wp (w[0] from the terrain slopes and u, v),
 //   w_out   = (ws - dampwt mut w_save) / (1 + dampwt),
 //   ph_out  = rhs + msfty 0.5 dts g (1+e) w_out / muts,
 // with K = kte - 1, on the columns [jds+1, jde-2) x [ids+1, ide-2); every
-// other point passes w, ph and t_2ave through.  One thread per column walks
-// these in reverse.  work, work2: 3-D scratch (the sensitivities of w and of
-// rhs along the column).
+// other point passes w, ph and t_2ave through.  w[0] is formed from the
+// terrain slopes and u, v at levels 0..2 (cf1..cf3), and muave enters
+// t2a's numerator and wp[k] as - dts g muave / msfty.  One thread per column
+// walks these in reverse.  work, work2: 3-D scratch (the sensitivities of w
+// and of rhs along the column).
 struct AdjW {
   const float *lT, *lW, *lP;                      // sensitivities of the outputs t_2ave, w, ph
   const float *mut, *muts, *t_1, *ph_1, *phb, *c2a, *cqw, *alt, *a, *alpha, *gamma, *rdnw, *rdn, *msfty;
+  const float *t2a_out, *w_out, *ht, *msftx, *cf;  // the forward's outputs t_2ave, w; terrain, map factor, cf1..3
   float *lw_in, *lph_in, *lt2, *lt2a_in, *work, *work2;
-  float dts, t0, ep1, em1, dampmag, hpi, hdepth;
+  float *lu, *lv, *lmuts, *lmuave;                 // zeroed by the caller; u, v by atomic adds
+  float dts, t0, ep1, em1, dampmag, hpi, hdepth, rdx, rdy;
   int K, j0, j1, i0, i1;
 };
 
@@ -359,6 +371,30 @@ __global__ void k_advance_w_adj(Box all, Dims D, AdjW a) {
     const float v = AT(a.work, k, j, i) * AT(a.alpha, k, j, i);
     AT(a.work, k, j, i) = v;
     if (k - 1 >= 1) AT(a.work, k - 1, j, i) -= AT(a.a, k, j, i) * v;
+  }
+  // the surface w (from the terrain and u, v at levels 0..2) enters the solve
+  // at level 1 and is an output itself
+  {
+    const float lw0 = AT(a.lW, 0, j, i) - AT(a.a, 1, j, i) * AT(a.work, 1, j, i);
+    const float htc = AT2(a.ht, j, i), mx = AT2(a.msftx, j, i);
+    const float cy = my * 0.5f * a.rdy * lw0, cx = mx * 0.5f * a.rdx * lw0;
+    const float vn = cy * (AT2(a.ht, j + 1, i) - htc), vs = cy * (htc - AT2(a.ht, j - 1, i));
+    const float ue = cx * (AT2(a.ht, j, i + 1) - htc), uw = cx * (htc - AT2(a.ht, j, i - 1));
+    for (int c = 0; c < 3; ++c) {
+      atomic_add(&AT(a.lv, c, j + 1, i), a.cf[c] * vn);
+      atomic_add(&AT(a.lv, c, j, i), a.cf[c] * vs);
+      atomic_add(&AT(a.lu, c, j, i + 1), a.cf[c] * ue);
+      atomic_add(&AT(a.lu, c, j, i), a.cf[c] * uw);
+    }
+  }
+  // muave in the predictor (- dts g muave / msfty at every level 1..K) and muts in ph_out
+  float lmuave = 0.0f, lmuts = 0.0f;
+  {
+    const float cp = my * 0.5f * a.dts * G * a.ep1;
+    for (int k = 1; k <= K; ++k) {
+      lmuave -= a.dts * G * mi * AT(a.work, k, j, i);
+      lmuts -= cp * AT(a.w_out, k, j, i) / (muts * muts) * AT(a.lP, k, j, i);
+    }
   }
   // accumulators: rhs (work2), ph_in, t2a (in lt2a_in), w_in
   for (int k = 0; k < nz; ++k) {
@@ -400,17 +436,21 @@ __global__ void k_advance_w_adj(Box all, Dims D, AdjW a) {
     AT(a.lph_in, k, j, i) += lr;
     AT(a.lw_in, k, j, i) += gam * lr;
   }
-  // t2a[k] for k < K from t_2 and t2a_in; levels from K on pass through
+  // t2a[k] for k < K from t_2, t2a_in, muave and muts; levels from K on pass through
   for (int k = 0; k < nz; ++k) {
     const float l = AT(a.lt2a_in, k, j, i);
     if (k < K) {
       const float den = muts * (a.t0 + AT(a.t_1, k, j, i));
       AT(a.lt2, k, j, i) = 0.5f * a.ep1 / den * l;
       AT(a.lt2a_in, k, j, i) = 0.5f * a.em1 / den * l;
+      lmuave += a.t0 / den * l;
+      lmuts -= AT(a.t2a_out, k, j, i) / muts * l;
     } else {
       AT(a.lt2, k, j, i) = 0.0f;
     }
   }
+  AT2(a.lmuave, j, i) = lmuave;
+  AT2(a.lmuts, j, i) = lmuts;
 }
 
 extern "C" int pywrf_advance_w_adjoint(const pywrf_grid_t* g, const float* lam_t2ave_out, const float* lam_w_out,
@@ -418,9 +458,11 @@ extern "C" int pywrf_advance_w_adjoint(const pywrf_grid_t* g, const float* lam_t
                                        const float* t_1, const float* ph_1, const float* phb, const float* c2a,
                                        const float* cqw, const float* alt, const float* a, const float* alpha,
                                        const float* gamma, double dts, double t0, double epssm, const float* rdnw,
-                                       const float* rdn, const float* msfty, float* lam_w, float* lam_ph,
-                                       float* lam_t_2, float* lam_t2ave, float* work, float* work2,
-                                       pywrf_stream_t s) {
+                                       const float* rdn, const float* msfty, const float* t2ave_out,
+                                       const float* w_out, const float* ht, const float* msftx, const float* cf,
+                                       double rdx, double rdy, float* lam_w, float* lam_ph, float* lam_t_2,
+                                       float* lam_t2ave, float* lam_u, float* lam_v, float* lam_muts,
+                                       float* lam_muave, float* work, float* work2, pywrf_stream_t s) {
   if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 1);
   if (g->kts != 0 || g->kte > g->nz || g->kte < 3) return PYWRF_E_RANGE;
@@ -429,11 +471,20 @@ extern "C" int pywrf_advance_w_adjoint(const pywrf_grid_t* g, const float* lam_t
   p.mut = mut; p.muts = muts; p.t_1 = t_1; p.ph_1 = ph_1; p.phb = phb; p.c2a = c2a; p.cqw = cqw; p.alt = alt;
   p.a = a; p.alpha = alpha; p.gamma = gamma; p.rdnw = rdnw; p.rdn = rdn; p.msfty = msfty;
   p.lw_in = lam_w; p.lph_in = lam_ph; p.lt2 = lam_t_2; p.lt2a_in = lam_t2ave; p.work = work; p.work2 = work2;
+  p.t2a_out = t2ave_out; p.w_out = w_out; p.ht = ht; p.msftx = msftx; p.cf = cf;
+  p.lu = lam_u; p.lv = lam_v; p.lmuts = lam_muts; p.lmuave = lam_muave;
+  p.rdx = F(rdx); p.rdy = F(rdy);
   p.dts = F(dts); p.t0 = F(t0); p.ep1 = F(1. + epssm); p.em1 = F(1. - epssm);
   p.dampmag = F(dts * 0.2); p.hpi = F(0.5 * 3.14159265359); p.hdepth = 5000.0f;
   p.K = g->kte - 1;
   p.j0 = imax(g->jts, g->jds + 1); p.j1 = imin(g->jte, g->jde - 2);
   p.i0 = imax(g->its, g->ids + 1); p.i1 = imin(g->ite, g->ide - 2);
+  if (p.j0 < 1 || p.i0 < 1 || p.j1 + 1 > g->ny || p.i1 + 1 > g->nx || g->nz < 3) return PYWRF_E_RANGE;
+  const long n3 = (long)g->nz * g->ny * g->nx, n2 = (long)g->ny * g->nx;
+  hipLaunchKernelGGL(k_zero_n, dim3(1024), dim3(256), 0, (hipStream_t)s, lam_u, n3);
+  hipLaunchKernelGGL(k_zero_n, dim3(1024), dim3(256), 0, (hipStream_t)s, lam_v, n3);
+  hipLaunchKernelGGL(k_zero_n, dim3(256), dim3(256), 0, (hipStream_t)s, lam_muts, n2);
+  hipLaunchKernelGGL(k_zero_n, dim3(256), dim3(256), 0, (hipStream_t)s, lam_muave, n2);
   Box all = whole(g);
   PW_LAUNCH_COLS(k_advance_w_adj, all, s, dims_of(g), p);
   PW_RET();
@@ -462,7 +513,6 @@ struct AdjUV {
   Box bu, bv;                    // the u and v boxes (j, i ranges)
 };
 
-__device__ inline void atomic_add(float* p, float v) { if (v != 0.0f) atomicAdd(p, v); }
 
 template <bool U>
 __global__ void k_advance_uv_adj(Box b, Dims D, AdjUV a) {
@@ -524,9 +574,6 @@ __global__ void k_advance_uv_adj(Box b, Dims D, AdjUV a) {
 // Outputs: lam_u / lam_v (= the sensitivities of the outputs: u and v pass
 // through), lam_p, lam_ph, lam_al, lam_mu, lam_mudf, lam_ru_tend,
 // lam_rv_tend; the library zeroes them first.
-__global__ void k_zero_n(float* p, long n) {
-  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) p[q] = 0.0f;
-}
 
 extern "C" int pywrf_advance_uv_adjoint(const pywrf_grid_t* g, const float* lam_u_out, const float* lam_v_out,
                                         const float* pb, const float* php, const float* alt, const float* muu,

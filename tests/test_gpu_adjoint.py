@@ -265,10 +265,11 @@ class _FirstCall:
 @pytest.mark.parametrize("config", ["config1c", "bdy24"])
 def test_advance_w_adjoint_matches_oracle_autograd(config):
     """Gradients of <lT, t_2ave> + <lW, w> + <lP, ph> after advance_w with
-    respect to w, ph, t_2, t_2ave, at the arguments of the step's first
-    acoustic substep, against torch autograd through the oracle's advance_w
-    in FP64 (the same coefficients).  Bound: 1e-4 normwise per gradient (the
-    coefficients are FP32 values in both; the FP32 adjoint's rounding)."""
+    respect to w, ph, t_2, t_2ave, u, v, muts and muave, at the arguments of
+    the step's first acoustic substep, against torch autograd through the
+    oracle's advance_w in FP64 (the same coefficients).  Bound: 1e-4
+    normwise per gradient (the coefficients are FP32 values in both; the
+    FP32 adjoint's rounding)."""
     import oracle
     import pywrf_amd
     from pywrf_amd import adjoint
@@ -280,16 +281,18 @@ def test_advance_w_adjoint_matches_oracle_autograd(config):
     args = rec.args
     gen = torch.Generator(device="cuda").manual_seed(7)
     lams = [torch.randn(s.w.shape, device="cuda", generator=gen) for _ in range(3)]
-    W, T2A, T2, PH = adjoint.AdvanceW.W, adjoint.AdvanceW.T2AVE, adjoint.AdvanceW.T2, adjoint.AdvanceW.PH
-    x = [args[n].clone().requires_grad_() for n in (W, PH, T2, T2A)]
+    D = adjoint.AdvanceW.DIFF
+    x = [args[n].clone().requires_grad_() for n in D]
     outs = adjoint.AdvanceW.apply(*x, args)
     sum(float(0) + (o * l).sum() for o, l in zip(outs, lams)).backward()
     a64 = [a.detach().double().cpu() if isinstance(a, torch.Tensor) else a for a in args]
-    x64 = [a64[n].clone().requires_grad_() for n in (W, PH, T2, T2A)]
-    a64[W], a64[PH], a64[T2], a64[T2A] = x64
+    x64 = [a64[n].clone().requires_grad_() for n in D]
+    for n, t in zip(D, x64):
+        a64[n] = t
     o64 = oracle.ops().advance_w(*a64)
     sum((o * l.double().cpu()).sum() for o, l in zip(o64, lams)).backward()
-    for name, mine, ref in zip(("w", "ph", "t_2", "t_2ave"), x, x64):
+    for name, mine, ref in zip(("w", "ph", "t_2", "t_2ave", "u", "v", "muts", "muave"), x, x64):
+        assert float(ref.grad.abs().max()) > 0, name
         err = _rel(mine.grad.double().cpu(), ref.grad)
         print("d/d%s: %.2e" % (name, err))
         assert err <= 1e-4, name
@@ -398,11 +401,10 @@ def test_adjoints_match_the_reference_autograd_gradients():
     w_args = rec._ops.args
     g = torch.Generator().manual_seed(7)
     lT, lW, lP = (torch.randn(tuple(s.w.shape), generator=g).cuda() for _ in range(3))
-    W, T2A, T2, PH = adjoint.AdvanceW.W, adjoint.AdvanceW.T2AVE, adjoint.AdvanceW.T2, adjoint.AdvanceW.PH
-    x = [w_args[n].clone().requires_grad_() for n in (W, PH, T2, T2A)]
+    x = [w_args[n].clone().requires_grad_() for n in adjoint.AdvanceW.DIFF]
     outs = adjoint.AdvanceW.apply(*x, w_args)
     ((outs[0] * lT).sum() + (outs[1] * lW).sum() + (outs[2] * lP).sum()).backward()
-    for name, t in zip(("w", "ph", "t_2", "t_2ave"), x):
+    for name, t in zip(("w", "ph", "t_2", "t_2ave"), x[:4]):
         ref = torch.from_numpy(gold["advance_w.d" + name]).double()
         err = _rel(t.grad.double().cpu(), ref)
         print("advance_w d/d%s vs reference autograd: %.2e" % (name, err))
