@@ -434,6 +434,13 @@ int pywrf_spec_bdyupdate_ph(const pywrf_grid_t* g, const float* ph_save, float* 
                             pywrf_stream_t s);
 /* zero_grad_bdy, wrf_boundary.py:361-405 */
 int pywrf_zero_grad_bdy(const pywrf_grid_t* g, float* field, char variable_in, pywrf_stream_t s);
+/* spec_bdyupdate_ph on `field` and zero_grad_bdy on `zg_field` (another
+ * array), as the two calls in that order would leave them, in one launch
+ * (the substep's strip updates after advance_w, solver.py:1458-1468).  No
+ * counterpart in the reference. */
+int pywrf_spec_bdyupdate_ph_zero_grad(const pywrf_grid_t* g, const float* ph_save, float* field,
+                                      const float* field_tend, const float* mu_tend, const float* muts, double dt,
+                                      char variable_in, float* zg_field, char zg_variable, pywrf_stream_t s);
 /* mass_weight, wrf_boundary.py:408-415 (tile = its..ite x jts..jte x kts..kte of g) */
 int pywrf_mass_weight(const pywrf_grid_t* g, const float* field, const float* mut, float* rfield,
                       pywrf_stream_t s);

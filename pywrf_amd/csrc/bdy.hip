@@ -365,50 +365,79 @@ extern "C" int pywrf_spec_bdyupdate_group(const pywrf_grid_t* g3, const pywrf_gr
 }
 
 // ------------------------------------------------------------------ spec_bdyupdate_ph
-__global__ void k_spec_bdyupdate_ph(Box all, int w, Dims D, const float* ph_save, float* f, const float* ft,
-                                    const float* mu_tend, const float* muts, float dt, int jts, int ibs, int ibe,
-                                    int jbs, int jbe) {
-  int k, j, i;
-  if (!edge_point(all, w, k, j, i)) return;
-  bool ir = i >= ibs && i < ibe, jc = j >= jbs + 1 && j < jbe - 1;
-  int n = (j == jts && ir) + (j == jbe - 1 && ir) + (i == ibs && jc) + (i == ibe - 1 && jc);
+struct PhStrip {
+  Box all;
+  int w;
+  const float *ph_save, *ft, *mu_tend, *muts;
+  float* f;
+  float dt;
+  int jts, ibs, ibe, jbs, jbe;
+};
+
+__device__ inline void ph_strip_point(const Dims& D, const PhStrip& a, int k, int j, int i) {
+  bool ir = i >= a.ibs && i < a.ibe, jc = j >= a.jbs + 1 && j < a.jbe - 1;
+  int n = (j == a.jts && ir) + (j == a.jbe - 1 && ir) + (i == a.ibs && jc) + (i == a.ibe - 1 && jc);
   if (!n) return;
   long q = D.ix(k, j, i);
-  float mt = AT2(muts, j, i);
-  float mo = mt - dt * AT2(mu_tend, j, i);
-  float v = f[q];
-  for (int m = 0; m < n; ++m) v = v * mo / mt + dt * ft[q] / mt + ph_save[q] * (mo / mt - 1.0f);
-  f[q] = v;
+  float mt = AT2(a.muts, j, i);
+  float mo = mt - a.dt * AT2(a.mu_tend, j, i);
+  float v = a.f[q];
+  for (int m = 0; m < n; ++m) v = v * mo / mt + a.dt * a.ft[q] / mt + a.ph_save[q] * (mo / mt - 1.0f);
+  a.f[q] = v;
 }
 
-extern "C" int pywrf_spec_bdyupdate_ph(const pywrf_grid_t* g, const float* ph_save, float* field,
-                                       const float* field_tend, const float* mu_tend, const float* muts, double dt,
-                                       char variable_in, pywrf_stream_t s) {
+__global__ void k_spec_bdyupdate_ph(Box all, int w, Dims D, PhStrip a) {
+  int k, j, i;
+  if (!edge_point(all, w, k, j, i)) return;
+  ph_strip_point(D, a, k, j, i);
+}
+
+static int ph_strip_setup(const pywrf_grid_t* g, const float* ph_save, float* field, const float* field_tend,
+                          const float* mu_tend, const float* muts, double dt, char variable_in, PhStrip& a) {
   if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   char v = variable_in;
   if (v == 'U') v = 'u'; else if (v == 'V') v = 'v'; else if (v == 'M') v = 'm'; else if (v == 'H') v = 'h';
   int ibe, jbe, ktf;
   bounds_of(v, g, ibe, jbe, ktf);
   if (ibe > g->nx || jbe > g->ny || ktf > g->nz || g->ids < 0 || g->jds < 0) return PYWRF_E_RANGE;
-  Box all = mkbox(g->kts, ktf, imin(g->jts, g->jds), imax(jbe, g->jts + 1), g->ids, ibe);
-  int w = (g->jts == all.j0 && jbe == all.j1) ? 1 : 0;
-  pw_launch_edges(k_spec_bdyupdate_ph, all, w, s, dims_of(g), ph_save, field, field_tend, mu_tend, muts, F(dt),
-                  g->jts, g->ids, ibe, g->jds, jbe);
+  a.all = mkbox(g->kts, ktf, imin(g->jts, g->jds), imax(jbe, g->jts + 1), g->ids, ibe);
+  a.w = (g->jts == a.all.j0 && jbe == a.all.j1) ? 1 : 0;
+  a.ph_save = ph_save; a.f = field; a.ft = field_tend; a.mu_tend = mu_tend; a.muts = muts; a.dt = F(dt);
+  a.jts = g->jts; a.ibs = g->ids; a.ibe = ibe; a.jbs = g->jds; a.jbe = jbe;
+  return 0;
+}
+
+extern "C" int pywrf_spec_bdyupdate_ph(const pywrf_grid_t* g, const float* ph_save, float* field,
+                                       const float* field_tend, const float* mu_tend, const float* muts, double dt,
+                                       char variable_in, pywrf_stream_t s) {
+  PhStrip a;
+  if (int e = ph_strip_setup(g, ph_save, field, field_tend, mu_tend, muts, dt, variable_in, a)) return e;
+  pw_launch_edges(k_spec_bdyupdate_ph, a.all, a.w, s, dims_of(g), a);
   PW_RET();
 }
 
 // ------------------------------------------------------------------ zero_grad_bdy
-__global__ void k_zero_grad(Box all, int w, Dims D, float* f, int ibs, int ibe, int jbs, int jbe) {
-  int k, j, i;
-  if (!edge_point(all, w, k, j, i)) return;
-  bool ring = (j == jbs || j == jbe - 1) || (i == ibs || i == ibe - 1);
+struct ZgStrip {
+  Box all;
+  float* f;
+  int ibs, ibe, jbs, jbe;
+};
+
+__device__ inline void zero_grad_point(const Dims& D, const ZgStrip& a, int k, int j, int i) {
+  bool ring = (j == a.jbs || j == a.jbe - 1) || (i == a.ibs || i == a.ibe - 1);
   if (!ring) return;
-  int sj = j < jbs + 1 ? jbs + 1 : (j > jbe - 2 ? jbe - 2 : j);
-  int si = i < ibs + 1 ? ibs + 1 : (i > ibe - 2 ? ibe - 2 : i);
-  AT(f, k, j, i) = AT(f, k, sj, si);
+  int sj = j < a.jbs + 1 ? a.jbs + 1 : (j > a.jbe - 2 ? a.jbe - 2 : j);
+  int si = i < a.ibs + 1 ? a.ibs + 1 : (i > a.ibe - 2 ? a.ibe - 2 : i);
+  AT(a.f, k, j, i) = AT(a.f, k, sj, si);
 }
 
-extern "C" int pywrf_zero_grad_bdy(const pywrf_grid_t* g, float* field, char variable_in, pywrf_stream_t s) {
+__global__ void k_zero_grad(Box all, int w, Dims D, ZgStrip a) {
+  int k, j, i;
+  if (!edge_point(all, w, k, j, i)) return;
+  zero_grad_point(D, a, k, j, i);
+}
+
+static int zero_grad_setup(const pywrf_grid_t* g, float* field, char variable_in, ZgStrip& a) {
   if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   char v = variable_in;
   if (v == 'U') v = 'u'; else if (v == 'V') v = 'v';
@@ -417,8 +446,49 @@ extern "C" int pywrf_zero_grad_bdy(const pywrf_grid_t* g, float* field, char var
   if (v == 'v') jbe = g->jde;
   if (v == 'w') ktf = g->kde;
   if (ibe > g->nx || jbe > g->ny || ktf > g->nz || g->ids < 0 || g->jds < 0) return PYWRF_E_RANGE;
-  Box all = mkbox(g->kts, ktf, g->jds, jbe, g->ids, ibe);
-  pw_launch_edges(k_zero_grad, all, 1, s, dims_of(g), field, g->ids, ibe, g->jds, jbe);
+  a.all = mkbox(g->kts, ktf, g->jds, jbe, g->ids, ibe);
+  a.f = field; a.ibs = g->ids; a.ibe = ibe; a.jbs = g->jds; a.jbe = jbe;
+  return 0;
+}
+
+extern "C" int pywrf_zero_grad_bdy(const pywrf_grid_t* g, float* field, char variable_in, pywrf_stream_t s) {
+  ZgStrip a;
+  if (int e = zero_grad_setup(g, field, variable_in, a)) return e;
+  pw_launch_edges(k_zero_grad, a.all, 1, s, dims_of(g), a);
+  PW_RET();
+}
+
+// spec_bdyupdate_ph and zero_grad_bdy (on another array) in one launch: grid
+// row y = 0 takes the first's band, y = 1 the second's (both one cell wide).
+__global__ void k_ph_zero_grad(Dims D, PhStrip p, ZgStrip z) {
+  int k, j, i;
+  if (blockIdx.y == 0) {
+    if ((int)blockIdx.z >= p.all.nk() || !edge_point(p.all, 1, k, j, i)) return;
+    ph_strip_point(D, p, k, j, i);
+  } else {
+    if ((int)blockIdx.z >= z.all.nk() || !edge_point(z.all, 1, k, j, i)) return;
+    zero_grad_point(D, z, k, j, i);
+  }
+}
+
+extern "C" int pywrf_spec_bdyupdate_ph_zero_grad(const pywrf_grid_t* g, const float* ph_save, float* field,
+                                                 const float* field_tend, const float* mu_tend, const float* muts,
+                                                 double dt, char variable_in, float* zg_field, char zg_variable,
+                                                 pywrf_stream_t s) {
+  if (zg_field == field) return PYWRF_E_ARG;
+  PhStrip p;
+  ZgStrip z;
+  if (int e = ph_strip_setup(g, ph_save, field, field_tend, mu_tend, muts, dt, variable_in, p)) return e;
+  if (int e = zero_grad_setup(g, zg_field, zg_variable, z)) return e;
+  const bool band = p.w == 1 && 2 <= p.all.nj() && 2 <= p.all.ni() && 2 <= z.all.nj() && 2 <= z.all.ni();
+  if (!band || p.all.size() <= 0 || z.all.size() <= 0) {          // as the two calls
+    pw_launch_edges(k_spec_bdyupdate_ph, p.all, p.w, s, dims_of(g), p);
+    pw_launch_edges(k_zero_grad, z.all, 1, s, dims_of(g), z);
+    PW_RET();
+  }
+  const long n = imax((int)edge_count(p.all, 1), (int)edge_count(z.all, 1));
+  dim3 grid((unsigned)((n + PW_EDGE_BLOCK - 1) / PW_EDGE_BLOCK), 2, (unsigned)imax(p.all.nk(), z.all.nk()));
+  hipLaunchKernelGGL(k_ph_zero_grad, grid, dim3(PW_EDGE_BLOCK), 0, (hipStream_t)s, dims_of(g), p, z);
   PW_RET();
 }
 

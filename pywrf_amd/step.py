@@ -416,9 +416,12 @@ def rk3_step(s, ops, cfg, check_nan=False, clock=None, nan_flag=None):
                                                s.c1f, s.c2f, s.c3h, s.c4h, s.c3f, s.c4f, s.ru_m, s.rv_m, s.ww_m, EPSSM,
                                                s.msfux, s.msfuy, s.msfvx, s.msfvx_inv, s.msfvy, iteration, nsmall,
                                                *I18)
-            s.ph = o.spec_bdyupdate_ph(s.ph_save, s.ph, s.ph_tend, s.mu_tend, s.muts, s.c1f, s.c2f, dts_rk, 'h',
-                                       SPEC_ZONE, *I24)
-            s.w = o.zero_grad_bdy(s.w, 'w', SPEC_ZONE, *I24)
+            ph_args = (s.ph_save, s.ph, s.ph_tend, s.mu_tend, s.muts, s.c1f, s.c2f, dts_rk, 'h', SPEC_ZONE, *I24)
+            if hasattr(o, "spec_bdyupdate_ph_zero_grad"):
+                s.ph, s.w = o.spec_bdyupdate_ph_zero_grad(ph_args, (s.w, 'w', SPEC_ZONE, *I24))
+            else:
+                s.ph = o.spec_bdyupdate_ph(*ph_args)
+                s.w = o.zero_grad_bdy(s.w, 'w', SPEC_ZONE, *I24)
             s.al, s.p, s.pm1 = cpr(iteration)
             s.ph, s.al, s.p, s.muts, s.mu, s.mudf = bcs((s.ph, 'w', 0), (s.al, 'p', 0), (s.p, 'p', 0),
                                                         (s.muts, 't', 1), (s.mu, 't', 1), (s.mudf, 't', 1))

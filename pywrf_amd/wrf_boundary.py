@@ -134,6 +134,23 @@ def zero_grad_bdy(field,
     return field
 
 
+def spec_bdyupdate_ph_zero_grad(ph_args, zg_args):
+    """spec_bdyupdate_ph then zero_grad_bdy (wrf_boundary.py:287-358,
+    361-405) on two different arrays, each given as the tuple of its
+    positional arguments, in one launch; the index sets must agree.  Returns
+    the two calls' results.  Not part of the reference API."""
+    (ph_save, field, field_tend, mu_tend, muts, c1, c2, dt, variable_in, spec_zone, *ix) = ph_args
+    (zg_field, zg_variable, zg_zone, *zix) = zg_args
+    if tuple(ix) != tuple(zix) or spec_zone != zg_zone:
+        raise ValueError("spec_bdyupdate_ph_zero_grad: the two calls must share the index set and zone")
+    if zg_field is field:
+        raise ValueError("spec_bdyupdate_ph_zero_grad: the two calls must act on different arrays")
+    g = grid(field.shape, _I24(*ix))
+    call("pywrf_spec_bdyupdate_ph_zero_grad", g, ptr(ph_save), ptr(field), ptr(field_tend), ptr(mu_tend), ptr(muts),
+         float(dt), flag(variable_in), ptr(zg_field), flag(zg_variable), stream())
+    return field, zg_field
+
+
 def mass_weight(field, mut, rfield, c1, c2,
                 ids, ide, jds, jde, kds, kde,
                 ims, ime, jms, jme, kms, kme,

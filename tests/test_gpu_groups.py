@@ -385,3 +385,29 @@ def test_boundary_epilogue_equals_the_grouped_calls(on_device):
         assert torch.equal(ta, tc), v
     changed = sum(int((ta != t0).sum()) for (ta, _, _), (t0, _, _) in zip(bcs_a, bcs0))
     assert changed > 0
+
+
+@pytest.mark.parametrize("n", [40, 120])
+def test_spec_bdyupdate_ph_zero_grad_equals_the_two_calls(n):
+    """The paired launch against spec_bdyupdate_ph on ph then zero_grad_bdy on
+    w, bit for bit on both whole arrays, with random fields (positive column
+    masses)."""
+    import pywrf_amd.wrf_boundary as bdy
+    from pywrf_amd.step import Grid
+    g = Grid(41, n, n)
+    torch.manual_seed(4)
+    r3 = lambda: torch.randn(41, n, n, device="cuda")
+    ph_save, ph_tend, w0, ph0 = r3(), r3(), r3(), r3()
+    mu_tend = torch.randn(n, n, device="cuda")
+    muts = torch.rand(n, n, device="cuda") * 1e4 + 5e4
+    c1f, c2f = torch.rand(41, device="cuda"), torch.rand(41, device="cuda")
+    args = lambda ph: (ph_save, ph, ph_tend, mu_tend, muts, c1f, c2f, 13.5, "h", 1, *g.ix24)
+    ph_a, w_a = ph0.clone(), w0.clone()
+    bdy.spec_bdyupdate_ph(*args(ph_a))
+    bdy.zero_grad_bdy(w_a, "w", 1, *g.ix24)
+    ph_b, w_b = ph0.clone(), w0.clone()
+    bdy.spec_bdyupdate_ph_zero_grad(args(ph_b), (w_b, "w", 1, *g.ix24))
+    torch.cuda.synchronize()
+    assert not torch.equal(ph_a, ph0) and not torch.equal(w_a, w0)
+    assert torch.equal(ph_a, ph_b)
+    assert torch.equal(w_a, w_b)
