@@ -215,15 +215,17 @@ __global__ void k_sg_zero_int(int* p) { *p = 0; }
 // columns, packs their standardised inputs into LDS as BF16 rows (the
 // k_sg_pack values), and runs the three layers out of LDS: each layer's
 // output overwrites its input there (after a barrier) as the next layer's
-// BF16 operand, so the hidden activations never leave the CU.  Only the
-// outputs go to memory, feature-major as the GEMM's transposed epilogue
-// writes them.  Layers 1 and 2: wave w owns output features [64w, 64w + 64)
+// BF16 operand, so the hidden activations never leave the CU; the outputs
+// either go through LDS to the checks and the update (APPLY, below) or to
+// memory, feature-major as the GEMM's transposed epilogue writes them.
+// Layers 1 and 2: wave w owns output features [64w, 64w + 64)
 // of all 128 columns (4 x 2 MFMA tiles of 32 x 32: the A fragments come from
 // LDS, each shared by two MFMAs, the B fragments -- L2-resident weights
 // rewritten in fragment order, so each is one contiguous 1 KB read -- straight
 // from global memory, each shared by four, fetched SF_KD k-steps ahead).
 // Layer 3 (288 = 9 x 32 features): wave w owns feature tile w of the four
-// column tiles, and waves 0..3 also tile 8 of column tile w.  Every 32 x 32
+// column tiles, and waves 0..3 also tile 8 of column tile w (fragments fetched
+// one k-step ahead).  Every 32 x 32
 // output tile is accumulated from zero over k ascending in 16-deep MFMA
 // steps, with the bias, ReLU and BF16 rounding of the GEMM epilogues, so the
 // outputs equal the three-GEMM path's bit for bit.
