@@ -691,13 +691,15 @@ int pywrf_rk_update_scalar_adjoint(const pywrf_grid_t* g, const float* lam_new, 
                                    int spec_zone, float* lam_n, float* lam_adv, float* lam_sc, float* lam_prev,
                                    pywrf_stream_t s);
 /* Adjoint of advance_w (wrf_dynamics.py:3641-3796) with respect to w, ph,
- * t_2, t_2ave, u, v, muts and muave, every coefficient (mut, c2a, cqw, alt,
+ * t_2, t_2ave, u, v, ww, muts and muave, every coefficient (mut, c2a, cqw, alt,
  * a, alpha, gamma, the damping from ph_1 and phb) held at the values given:
  * from the sensitivities of the three outputs (t_2ave, w, ph) writes those of
- * w, ph, t_2, t_2ave, u, v (levels 0..2, through the surface w) and of the
- * planes muts, muave on the whole array.  t2ave_out, w_out: the forward
+ * w, ph, t_2, t_2ave, u, v (levels 0..2, through the surface w), ww (through
+ * the vertical advection of ph_1 + phb) and of the planes muts, muave on the
+ * whole array.  t2ave_out, w_out: the forward
  * call's outputs (muts enters by division); ht, msftx, cf, rdx, rdy: its
- * terrain, map factor, surface extrapolation and grid spacing arguments.
+ * terrain, map factor, surface extrapolation, grid spacing and fnm / fnp
+ * arguments.
  * work, work2: 3-D scratch arrays of the grid's size. */
 int pywrf_advance_w_adjoint(const pywrf_grid_t* g, const float* lam_t2ave_out, const float* lam_w_out,
                             const float* lam_ph_out, const float* mut, const float* muts, const float* t_1,
@@ -705,9 +707,10 @@ int pywrf_advance_w_adjoint(const pywrf_grid_t* g, const float* lam_t2ave_out, c
                             const float* alt, const float* a, const float* alpha, const float* gamma, double dts,
                             double t0, double epssm, const float* rdnw, const float* rdn, const float* msfty,
                             const float* t2ave_out, const float* w_out, const float* ht, const float* msftx,
-                            const float* cf, double rdx, double rdy, float* lam_w, float* lam_ph, float* lam_t_2,
-                            float* lam_t2ave, float* lam_u, float* lam_v, float* lam_muts, float* lam_muave,
-                            float* work, float* work2, pywrf_stream_t s);
+                            const float* cf, double rdx, double rdy, const float* fnm, const float* fnp,
+                            float* lam_w, float* lam_ph, float* lam_t_2, float* lam_t2ave, float* lam_u,
+                            float* lam_v, float* lam_muts, float* lam_muave, float* lam_ww, float* work,
+                            float* work2, pywrf_stream_t s);
 /* The adjoint of advance_uv (wrf_dynamics.py:3391-3525) with respect to u,
  * v, ru_tend, rv_tend, p, ph, al, mu and mudf (alt, pb, php, cqu, cqv, muu,
  * muv and the map factors held fixed): lam_u_out / lam_v_out are the

@@ -9,7 +9,7 @@ transpose of a linear part of the step:
 * AdvectScalar -- advect_scalar (wrf_dynamics.py:1841-2028) as a function of
   the field, the mass fluxes held fixed.  Backward: pywrf_advect_scalar_adjoint.
 * AdvanceW -- the vertical acoustic step advance_w (wrf_dynamics.py:3641-3796)
-  as a function of w, ph, t_2, t_2ave, u, v, muts and muave.  Backward: pywrf_advance_w_adjoint
+  as a function of w, ph, t_2, t_2ave, u, v, muts, muave and ww.  Backward: pywrf_advance_w_adjoint
   (the damping, the transposed column solve, the explicit predictor and the
   geopotential right-hand side, in reverse).
 * AdvanceUV -- the horizontal acoustic step advance_uv
@@ -133,19 +133,19 @@ class TridiagSolve(torch.autograd.Function):
 # ----------------------------------------------------------------------- acoustic step: advance_w
 class AdvanceW(torch.autograd.Function):
     """(t_2ave, w, ph) = advance_w(...) (wrf_dynamics.py:3641-3796) with the
-    gradient with respect to w, ph, t_2, t_2ave, u, v, muts and muave (every
+    gradient with respect to w, ph, t_2, t_2ave, u, v, muts, muave and ww (every
     other argument is a coefficient held fixed).  args: advance_w's
     positional arguments in the reference's order."""
 
     # positions in advance_w's signature
-    W, U, V, MUAVE, MUTS, T2AVE, T2, PH = 0, 4, 5, 8, 9, 18, 19, 21
-    DIFF = (W, PH, T2, T2AVE, U, V, MUTS, MUAVE)
+    W, WW, U, V, MUAVE, MUTS, T2AVE, T2, PH = 0, 2, 4, 5, 8, 9, 18, 19, 21
+    DIFF = (W, PH, T2, T2AVE, U, V, MUTS, MUAVE, WW)
 
     @staticmethod
-    def forward(ctx, w, ph, t_2, t_2ave, u, v, muts, muave, args):
+    def forward(ctx, w, ph, t_2, t_2ave, u, v, muts, muave, ww, args):
         from . import wrf_dynamics
         args = list(args)
-        for pos, x in zip(AdvanceW.DIFF, (w, ph, t_2, t_2ave, u, v, muts, muave)):
+        for pos, x in zip(AdvanceW.DIFF, (w, ph, t_2, t_2ave, u, v, muts, muave, ww)):
             args[pos] = _f32(x)
         ctx.args = args
         t2a, wo, pho = wrf_dynamics.advance_w(*args)
@@ -163,6 +163,7 @@ class AdvanceW(torch.autograd.Function):
         out = [torch.empty_like(a[0]) for _ in range(4)]
         uv = [torch.empty_like(a[4]), torch.empty_like(a[5])]
         planes = [torch.empty_like(a[9]), torch.empty_like(a[8])]
+        lww = torch.empty_like(a[2])
         work = [torch.empty_like(a[0]) for _ in range(2)]
         t2a_out, w_out = ctx.outs
         # advance_w(w, rw_tend, ww, w_save, u, v, mu1, mut, muave, muts, c1h, c2h, c1f, c2f, c3h, c4h, c3f, c4f,
@@ -171,12 +172,13 @@ class AdvanceW(torch.autograd.Function):
         call("pywrf_advance_w_adjoint", g, ptr(lT), ptr(lW), ptr(lP), ptr(a[7]), ptr(a[9]), ptr(a[20]),
              ptr(a[22]), ptr(a[23]), ptr(a[26]), ptr(a[27]), ptr(a[28]), ptr(a[30]), ptr(a[31]), ptr(a[32]),
              float(a[35]), float(a[36]), float(a[37]), ptr(a[41]), ptr(a[42]), ptr(a[47]), ptr(t2a_out),
-             ptr(w_out), ptr(a[25]), ptr(a[46]), cf, float(a[33]), float(a[34]), *[ptr(t) for t in out],
-             *[ptr(t) for t in uv], *[ptr(t) for t in planes], *[ptr(t) for t in work], stream())
+             ptr(w_out), ptr(a[25]), ptr(a[46]), cf, float(a[33]), float(a[34]), ptr(a[39]), ptr(a[40]),
+             *[ptr(t) for t in out], *[ptr(t) for t in uv], *[ptr(t) for t in planes], ptr(lww),
+             *[ptr(t) for t in work], stream())
         lw, lph, lt2, lt2a = out
         lu, lv = uv
         lmuts, lmuave = planes
-        return lw, lph, lt2, lt2a, lu, lv, lmuts, lmuave, None
+        return lw, lph, lt2, lt2a, lu, lv, lmuts, lmuave, lww, None
 
 
 class AdvanceUV(torch.autograd.Function):

@@ -312,7 +312,7 @@ __global__ void k_zero_n(float* p, long n) {
 
 // ------------------------------------------------------------------ advance_w^T
 // advance_w (wrf_dynamics.py:3641-3796) with respect to w, ph, t_2, t_2ave,
-// u, v, muts and muave (the coefficients -- mut, c2a, cqw, alt, the
+// u, v, ww, muts and muave (the coefficients -- mut, c2a, cqw, alt, the
 // tridiagonal factors, the damping weights from ph_1 -- fixed).  It is linear
 // in all but muts, whose derivatives take the forward's outputs t_2ave and w:
 //   t2a[k]  = (0.5 ((1+e) t_2[k] + (1-e) t2a_in[k]) + muave t0) / (muts (t0 + t_1[k])),  k < K
@@ -331,8 +331,9 @@ struct AdjW {
   const float *lT, *lW, *lP;                      // sensitivities of the outputs t_2ave, w, ph
   const float *mut, *muts, *t_1, *ph_1, *phb, *c2a, *cqw, *alt, *a, *alpha, *gamma, *rdnw, *rdn, *msfty;
   const float *t2a_out, *w_out, *ht, *msftx, *cf;  // the forward's outputs t_2ave, w; terrain, map factor, cf1..3
+  const float *fnm, *fnp;
   float *lw_in, *lph_in, *lt2, *lt2a_in, *work, *work2;
-  float *lu, *lv, *lmuts, *lmuave;                 // zeroed by the caller; u, v by atomic adds
+  float *lu, *lv, *lmuts, *lmuave, *lww;           // zeroed by the caller; u, v by atomic adds
   float dts, t0, ep1, em1, dampmag, hpi, hdepth, rdx, rdy;
   int K, j0, j1, i0, i1;
 };
@@ -429,12 +430,23 @@ __global__ void k_advance_w_adj(Box all, Dims D, AdjW a) {
       AT(a.lt2a_in, K - 1, j, i) -= mi * a.dts * G * 2.0f * rk * AT(a.c2a, K - 1, j, i) * AT(a.alt, K - 1, j, i) * L;
     }
   }
-  // rhs[k] = ph[k] + msfty dts 0.5 g (1-e) w[k] / mut + (terms without w, ph)
+  // rhs[k] = ph[k] + msfty dts 0.5 g (1-e) w[k] / mut - msfty dts (fnm[k] wup[k] + fnp[k] wdn[k]) / mut
+  // + (terms without w, ph, ww), wup / wdn = 0.5 (ww[k+1] + ww[k]) / (ww[k] + ww[k-1]) times rdnw and the
+  // ph_1 + phb differences above / below level k (k < K)
   const float gam = my * a.dts * 0.5f * G * a.em1 / mut;
   for (int k = 1; k <= K; ++k) {
     const float lr = AT(a.work2, k, j, i);
     AT(a.lph_in, k, j, i) += lr;
     AT(a.lw_in, k, j, i) += gam * lr;
+    if (k < K) {
+      const float li = -a.dts * my / mut * lr;
+      const float d1 = AT(a.ph_1, k + 1, j, i) - AT(a.ph_1, k, j, i) + AT(a.phb, k + 1, j, i) - AT(a.phb, k, j, i);
+      const float d0 = AT(a.ph_1, k, j, i) - AT(a.ph_1, k - 1, j, i) + AT(a.phb, k, j, i) - AT(a.phb, k - 1, j, i);
+      const float cu = 0.5f * a.rdnw[k] * d1 * a.fnm[k] * li, cd = 0.5f * a.rdnw[k - 1] * d0 * a.fnp[k] * li;
+      AT(a.lww, k + 1, j, i) += cu;
+      AT(a.lww, k, j, i) += cu + cd;
+      AT(a.lww, k - 1, j, i) += cd;
+    }
   }
   // t2a[k] for k < K from t_2, t2a_in, muave and muts; levels from K on pass through
   for (int k = 0; k < nz; ++k) {
@@ -460,9 +472,10 @@ extern "C" int pywrf_advance_w_adjoint(const pywrf_grid_t* g, const float* lam_t
                                        const float* gamma, double dts, double t0, double epssm, const float* rdnw,
                                        const float* rdn, const float* msfty, const float* t2ave_out,
                                        const float* w_out, const float* ht, const float* msftx, const float* cf,
-                                       double rdx, double rdy, float* lam_w, float* lam_ph, float* lam_t_2,
-                                       float* lam_t2ave, float* lam_u, float* lam_v, float* lam_muts,
-                                       float* lam_muave, float* work, float* work2, pywrf_stream_t s) {
+                                       double rdx, double rdy, const float* fnm, const float* fnp, float* lam_w,
+                                       float* lam_ph, float* lam_t_2, float* lam_t2ave, float* lam_u,
+                                       float* lam_v, float* lam_muts, float* lam_muave, float* lam_ww,
+                                       float* work, float* work2, pywrf_stream_t s) {
   if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 1);
   if (g->kts != 0 || g->kte > g->nz || g->kte < 3) return PYWRF_E_RANGE;
@@ -472,7 +485,8 @@ extern "C" int pywrf_advance_w_adjoint(const pywrf_grid_t* g, const float* lam_t
   p.a = a; p.alpha = alpha; p.gamma = gamma; p.rdnw = rdnw; p.rdn = rdn; p.msfty = msfty;
   p.lw_in = lam_w; p.lph_in = lam_ph; p.lt2 = lam_t_2; p.lt2a_in = lam_t2ave; p.work = work; p.work2 = work2;
   p.t2a_out = t2ave_out; p.w_out = w_out; p.ht = ht; p.msftx = msftx; p.cf = cf;
-  p.lu = lam_u; p.lv = lam_v; p.lmuts = lam_muts; p.lmuave = lam_muave;
+  p.lu = lam_u; p.lv = lam_v; p.lmuts = lam_muts; p.lmuave = lam_muave; p.lww = lam_ww;
+  p.fnm = fnm; p.fnp = fnp;
   p.rdx = F(rdx); p.rdy = F(rdy);
   p.dts = F(dts); p.t0 = F(t0); p.ep1 = F(1. + epssm); p.em1 = F(1. - epssm);
   p.dampmag = F(dts * 0.2); p.hpi = F(0.5 * 3.14159265359); p.hdepth = 5000.0f;
@@ -483,6 +497,7 @@ extern "C" int pywrf_advance_w_adjoint(const pywrf_grid_t* g, const float* lam_t
   const long n3 = (long)g->nz * g->ny * g->nx, n2 = (long)g->ny * g->nx;
   hipLaunchKernelGGL(k_zero_n, dim3(1024), dim3(256), 0, (hipStream_t)s, lam_u, n3);
   hipLaunchKernelGGL(k_zero_n, dim3(1024), dim3(256), 0, (hipStream_t)s, lam_v, n3);
+  hipLaunchKernelGGL(k_zero_n, dim3(1024), dim3(256), 0, (hipStream_t)s, lam_ww, n3);
   hipLaunchKernelGGL(k_zero_n, dim3(256), dim3(256), 0, (hipStream_t)s, lam_muts, n2);
   hipLaunchKernelGGL(k_zero_n, dim3(256), dim3(256), 0, (hipStream_t)s, lam_muave, n2);
   Box all = whole(g);

@@ -265,7 +265,7 @@ class _FirstCall:
 @pytest.mark.parametrize("config", ["config1c", "bdy24"])
 def test_advance_w_adjoint_matches_oracle_autograd(config):
     """Gradients of <lT, t_2ave> + <lW, w> + <lP, ph> after advance_w with
-    respect to w, ph, t_2, t_2ave, u, v, muts and muave, at the arguments of
+    respect to w, ph, t_2, t_2ave, u, v, muts, muave and ww, at the arguments of
     the step's first acoustic substep, against torch autograd through the
     oracle's advance_w in FP64 (the same coefficients).  Bound: 1e-4
     normwise per gradient (the coefficients are FP32 values in both; the
@@ -291,7 +291,7 @@ def test_advance_w_adjoint_matches_oracle_autograd(config):
         a64[n] = t
     o64 = oracle.ops().advance_w(*a64)
     sum((o * l.double().cpu()).sum() for o, l in zip(o64, lams)).backward()
-    for name, mine, ref in zip(("w", "ph", "t_2", "t_2ave", "u", "v", "muts", "muave"), x, x64):
+    for name, mine, ref in zip(("w", "ph", "t_2", "t_2ave", "u", "v", "muts", "muave", "ww"), x, x64):
         assert float(ref.grad.abs().max()) > 0, name
         err = _rel(mine.grad.double().cpu(), ref.grad)
         print("d/d%s: %.2e" % (name, err))
@@ -416,3 +416,117 @@ def test_adjoints_match_the_reference_autograd_gradients():
     err = _rel(gq.double().cpu(), torch.from_numpy(gold["advect_scalar.dfield"]).double())
     print("advect_scalar d/dfield vs reference autograd: %.2e" % err)
     assert err <= 1e-4
+
+
+class _Substep:
+    """The step's functions, keeping clones of the arguments of every call
+    from the first advance_uv through the first advance_w."""
+
+    def __init__(self, ops):
+        self._ops, self.calls, self.on, self.done = ops, [], False, False
+
+    def __getattr__(self, n):
+        f = getattr(self._ops, n)
+
+        def rec(*args):
+            self.on = self.on or n == "advance_uv"
+            if self.on and not self.done:
+                clone = lambda a: a.clone() if isinstance(a, torch.Tensor) else (
+                    [tuple(clone(x) for x in it) for it in a] if isinstance(a, list) else a)
+                self.calls.append((n, [clone(a) for a in args]))
+                self.done = n == "advance_w"
+            return f(*args)
+        return rec
+
+
+@pytest.mark.parametrize("config", ["config1c", "bdy24"])
+def test_acoustic_adjoints_chain_through_a_substep(config):
+    """The three acoustic adjoints chained by autograd through the first
+    acoustic substep's advance_uv, the zone update of u and v, advance_mu_t,
+    the zone update of t, mu and muts, and advance_w (the zone updates add
+    fixed tendencies on the outer ring: their Jacobian is the identity),
+    against FP64 central differences of the same sequence through the
+    oracle: for each of u, v, p, ph, al, mu, mudf, ww, muave, muts, t, t_ave
+    and w a seeded direction d, <grad, d> against the difference of
+    sum_o <lam_o, out_o>, relative to the larger of the difference and the
+    2-norm of grad_i d_i; bound 1e-4."""
+    import oracle
+    import pywrf_amd
+    from pywrf_amd import adjoint
+    from pywrf_amd.gen import synthetic_state
+    from pywrf_amd.step import rk3_step, RunConfig
+    s = synthetic_state(config, device="cuda")
+    rec = _Substep(pywrf_amd.ops())
+    rk3_step(s, rec, RunConfig())
+    names = [n for n, _ in rec.calls]
+    assert names == ["advance_uv", "spec_bdyupdate_group", "advance_mu_t", "spec_bdyupdate_group", "advance_w"], names
+    uv, z1, mt, z2, wa = (a for _, a in rec.calls)
+    o = pywrf_amd.ops()
+
+    def zone(items, dt, ix24, ix2d, fields):
+        """the increments the zone update adds to `fields` (HIP, on clones)"""
+        outs = [f.detach().clone() for f in fields]
+        o.spec_bdyupdate_group([(f, it[1], it[2], it[3]) for f, it in zip(outs, items)], dt, ix24, ix2d)
+        return [a - f.detach() for a, f in zip(outs, fields)]
+
+    # inputs: advance_uv's u, v, p, ph, al, mu, mudf; advance_mu_t's ww, muave, muts, t, t_ave; advance_w's w
+    src = {"u": (uv, 0), "v": (uv, 2), "p": (uv, 4), "ph": (uv, 6), "al": (uv, 9), "mu": (uv, 10),
+           "mudf": (uv, 15), "ww": (mt, 0), "muave": (mt, 8), "muts": (mt, 9), "t": (mt, 24), "t_ave": (mt, 26),
+           "w": (wa, 0)}
+    x = {n: a[p].clone().requires_grad_() for n, (a, p) in src.items()}
+    u1, v1 = adjoint.AdvanceUV.apply(x["u"], uv[1], x["v"], uv[3], x["p"], x["ph"], x["al"], x["mu"], x["mudf"], uv)
+    du, dv = zone(z1[0], z1[1], z1[2], z1[3], [u1, v1])
+    u2, v2 = u1 + du, v1 + dv
+    ww_o, t_o, tave_o, muave_o, muts_o, mudf_o, mu_o = adjoint.AdvanceMuT.apply(
+        u2, v2, x["ww"], x["mu"], x["muave"], x["muts"], x["mudf"], x["t"], x["t_ave"], mt[27], mt[28], mt)
+    dt_, dmu, dmuts = zone(z2[0], z2[1], z2[2], z2[3], [t_o, mu_o, muts_o])
+    t2, mu2, muts2 = t_o + dt_, mu_o + dmu, muts_o + dmuts
+    t2a_w, w_o, ph_o = adjoint.AdvanceW.apply(x["w"], x["ph"], t2, tave_o, u2, v2, muts2, muave_o, ww_o, wa)
+    outs = [u2, v2, ww_o, t2, mu2, muts2, mudf_o, muave_o, t2a_w, w_o, ph_o]
+    gen = torch.Generator().manual_seed(17)
+    lams = [torch.randn(tuple(t.shape), generator=gen, dtype=torch.float64) for t in outs]
+    sum((t * l.float().cuda()).sum() for t, l in zip(outs, lams)).backward()
+
+    # the same sequence through the oracle in FP64
+    ro = oracle.ops()
+    d64 = lambda args: [a.detach().double().cpu() if isinstance(a, torch.Tensor) else a for a in args]
+    uv64, mt64, wa64 = d64(uv), d64(mt), d64(wa)
+    z164 = [(it[0].double().cpu(), it[1].double().cpu(), it[2], it[3]) for it in z1[0]]
+    z264 = [(it[0].double().cpu(), it[1].double().cpu(), it[2], it[3]) for it in z2[0]]
+
+    def ozone(items, dt, ix24, ix2d, fields):
+        return [ro.spec_bdyupdate(f.clone(), it[1], dt, it[2], 1, *(ix2d if it[3] else ix24))
+                for f, it in zip(fields, items)]
+
+    def chain(v):
+        a = list(uv64)
+        for n in ("u", "v", "p", "ph", "al", "mu", "mudf"):
+            a[src[n][1]] = v[n]
+        u1_, v1_ = ro.advance_uv(*a)
+        u2_, v2_ = ozone(z164, z1[1], z1[2], z1[3], [u1_, v1_])
+        m = list(mt64)
+        m[2], m[4] = u2_, v2_
+        # advance_mu_t's positions of its inputs (mu and mudf are advance_uv's too)
+        for n, pos in (("ww", 0), ("mu", 6), ("muave", 8), ("muts", 9), ("mudf", 12), ("t", 24), ("t_ave", 26)):
+            m[pos] = v[n]
+        r = ro.advance_mu_t(*m)
+        t2_, mu2_, muts2_ = ozone(z264, z2[1], z2[2], z2[3], [r[2], r[10], r[8]])
+        w_ = list(wa64)
+        w_[0], w_[2], w_[4], w_[5], w_[8], w_[9], w_[18], w_[19], w_[21] = (
+            v["w"], r[0], u2_, v2_, r[7], muts2_, r[3], t2_, v["ph"])
+        t2a_, wo_, pho_ = ro.advance_w(*w_)
+        res = [u2_, v2_, r[0], t2_, mu2_, muts2_, r[9], r[7], t2a_, wo_, pho_]
+        return sum(float((q * l).sum()) for q, l in zip(res, lams))
+
+    base = {n: x[n].detach().double().cpu() for n in src}
+    for n in src:
+        d = torch.randn(tuple(base[n].shape), generator=gen, dtype=torch.float64)
+        eps = 1e-4 * float(base[n].abs().max() + 1.0) / float(d.abs().max())
+        vp, vm = dict(base), dict(base)
+        vp[n], vm[n] = base[n] + eps * d, base[n] - eps * d
+        ref = (chain(vp) - chain(vm)) / (2.0 * eps)
+        terms = x[n].grad.double().cpu() * d
+        got = float(terms.sum())
+        err = abs(got - ref) / max(abs(ref), float(terms.norm()))
+        print("d/d%s: <grad, d> %.6e  difference %.6e  rel %.2e" % (n, got, ref, err))
+        assert err <= 1e-4, n
