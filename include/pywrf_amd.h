@@ -738,6 +738,44 @@ int pywrf_advance_mu_t_adjoint(const pywrf_grid_t* g, const float* lam_ww, const
                                float* lam_ww_ori, float* lam_u, float* lam_v, float* lam_mu_ori,
                                float* lam_muave_ori, float* lam_muts_ori, float* lam_mudf_ori, float* lam_t_ori,
                                float* lam_t_ave_ori, float* lam_ft, float* lam_mu_tend, pywrf_stream_t s);
+/* The adjoint of calc_p_rho (wrf_dynamics.py:3269-3313) with respect to al,
+ * p, pm1, ph, t_2, mu and mut (alt, t_1 and c2a held fixed): lam_al, lam_p,
+ * lam_pm1 are the sensitivities of the returned al, p, pm1 (pm1 being the
+ * copy of p at step 0); the seven outputs are overwritten, lam_mu and
+ * lam_mut being planes. */
+int pywrf_calc_p_rho_adjoint(const pywrf_grid_t* g, const float* lam_al, const float* lam_p, const float* lam_pm1,
+                             const float* ph, const float* alt, const float* t_2, const float* t_1, const float* c2a,
+                             const float* mu, const float* mut, double t0, const float* rdnw, double smdiv, int step,
+                             float* lam_al_ori, float* lam_p_ori, float* lam_pm1_in, float* lam_ph, float* lam_t_2,
+                             float* lam_mu, float* lam_mut, pywrf_stream_t s);
+/* The adjoint of sumflux (wrf_dynamics.py:3799-3839), linear in its fluxes
+ * and accumulators: lam_ru_m .. lam_ww_m are the sensitivities of the
+ * accumulators after the call; the six outputs (those of ru, rv, ww and of
+ * the accumulators before the call) are overwritten. */
+int pywrf_sumflux_adjoint(const pywrf_grid_t* g, const float* lam_ru_m, const float* lam_rv_m,
+                          const float* lam_ww_m, int iteration, int number_of_small_timesteps, float* lam_ru,
+                          float* lam_rv, float* lam_ww, float* lam_ru_m_in, float* lam_rv_m_in, float* lam_ww_m_in,
+                          pywrf_stream_t s);
+/* The adjoint of a run of open-boundary fills (set_physical_bc3d / bc2d,
+ * wrf_dynamics.py:164-278), the same arguments as
+ * pywrf_set_physical_bc_group with each field replaced by its sensitivity
+ * after the fill, which is turned in place into the sensitivity before it. */
+int pywrf_set_physical_bc_group_adjoint(const pywrf_grid_t* g3, const pywrf_grid_t* g2, int n,
+                                        void* const* lam_fields, const char* variables, const int* is2d,
+                                        pywrf_stream_t s);
+/* The adjoint of spec_bdyupdate_ph (wrf_boundary.py:287-358) with respect to
+ * the field and muts (ph_save, field_tend and mu_tend held fixed): field_in
+ * is the field before the update; lam_field holds the sensitivity of the
+ * updated field on entry and that of field_in on return; lam_muts is written
+ * on the strips' columns only (the caller clears it). */
+int pywrf_spec_bdyupdate_ph_adjoint(const pywrf_grid_t* g, const float* ph_save, const float* field_in,
+                                    const float* field_tend, const float* mu_tend, const float* muts, double dt,
+                                    char variable_in, float* lam_field, float* lam_muts, pywrf_stream_t s);
+/* The adjoint of zero_grad_bdy (wrf_boundary.py:361-405): lam, the
+ * sensitivity of the field after the copy, gives lam_in (another array), that
+ * of the field before it. */
+int pywrf_zero_grad_bdy_adjoint(const pywrf_grid_t* g, const float* lam, char variable_in, float* lam_in,
+                                pywrf_stream_t s);
 /* The column tridiagonal solve of advance_w (wrf_dynamics.py:3758-3765), in
  * place on w over advance_w's columns, levels 1..kte-1 (level 0 is the lower
  * boundary value): the same operations in the same order. */

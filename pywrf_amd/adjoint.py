@@ -20,6 +20,13 @@ transpose of a linear part of the step:
   (wrf_dynamics.py:3528-3638) as a function of u, v, ww, mu, muave, muts,
   mudf, t, t_ave, ft and mu_tend.  Backward: pywrf_advance_mu_t_adjoint (the
   eta-velocity recurrence as suffix sums down each column).
+* SumFlux, SpecBdyUpdatePh, ZeroGradBdy, CalcPRho, OpenBoundaryFill -- the
+  calls that close an acoustic substep: sumflux (wrf_dynamics.py:3799-3839),
+  spec_bdyupdate_ph (wrf_boundary.py:287-358), zero_grad_bdy (361-405),
+  calc_p_rho (wrf_dynamics.py:3269-3313) and a run of open-boundary fills
+  (164-278), each on copies of its in-place operands.  With the three above
+  and the zone updates (fixed increments) they chain into the gradient of the
+  whole acoustic loop.
 * TridiagSolve -- the column tridiagonal solve of advance_w
   (wrf_dynamics.py:3758-3765) as a function of its coefficients a, alpha,
   gamma and right-hand side.  Backward: pywrf_tridiag_solve_adjoint
@@ -45,7 +52,7 @@ transpose of a linear part of the step:
 """
 import torch
 
-from ._lib import call, grid, ptr, stream
+from ._lib import call, flag, grid, ptr, stream
 from . import surrogate as _sg
 
 
@@ -256,6 +263,153 @@ class AdvanceMuT(torch.autograd.Function):
              float(a[32]), *[ptr(x) for x in out], stream())
         gww, gu, gv, gmu, gmuave, gmuts, gmudf, gt, gtave, gft, gmut = out
         return gu, gv, gww, gmu, gmuave, gmuts, gmudf, gt, gtave, gft, gmut, None
+
+
+class CalcPRho(torch.autograd.Function):
+    """(al, p, pm1) = calc_p_rho(...) (wrf_dynamics.py:3269-3313) with the
+    gradient with respect to al, p, ph, t_2, pm1, mu and mut (alt, t_1 and c2a
+    held fixed).  The incoming pm1 is left as it is (the call's in-place update
+    goes to a copy).  args: calc_p_rho's positional arguments in the
+    reference's order."""
+
+    # positions in calc_p_rho's signature
+    DIFF = (0, 1, 2, 4, 7, 8, 9)
+
+    @staticmethod
+    def forward(ctx, al, p, ph, t_2, pm1, mu, mut, args):
+        from . import wrf_dynamics
+        args = list(args)
+        for pos, x in zip(CalcPRho.DIFF, (al, p, ph, t_2, pm1, mu, mut)):
+            args[pos] = _f32(x)
+        args[7] = args[7].clone()
+        ctx.args = args
+        return wrf_dynamics.calc_p_rho(*args)
+
+    @staticmethod
+    def backward(ctx, lal, lp, lpm1):
+        a = ctx.args
+        g = grid(a[1].shape, tuple(a[-18:]))
+        lal, lp, lpm1 = [torch.zeros_like(a[1]) if l is None else _f32(l) for l in (lal, lp, lpm1)]
+        out = [torch.empty_like(a[1]) for _ in range(5)] + [torch.empty_like(a[8]), torch.empty_like(a[9])]
+        # (al_ori, p_ori, ph, alt, t_2, t_1, c2a, pm1, mu, mut, c1h, c2h, c1f, c2f, c3h, c4h, c3f, c4f, znu, t0,
+        #  rdnw, dnw, smdiv, non_hydrostatic, step, idx)
+        call("pywrf_calc_p_rho_adjoint", g, ptr(lal), ptr(lp), ptr(lpm1), ptr(a[2]), ptr(a[3]), ptr(a[4]),
+             ptr(a[5]), ptr(a[6]), ptr(a[8]), ptr(a[9]), float(a[19]), ptr(a[20]), float(a[22]), int(a[24]),
+             *[ptr(t) for t in out], stream())
+        gal, gp, gpm1, gph, gt2, gmu, gmut = out
+        return gal, gp, gph, gt2, gpm1, gmu, gmut, None
+
+
+class SumFlux(torch.autograd.Function):
+    """(ru_m, rv_m, ww_m) = sumflux(...) (wrf_dynamics.py:3799-3839) with the
+    gradient with respect to ru, rv, ww and the three accumulators (the call's
+    in-place update goes to copies).  args: sumflux's positional arguments in
+    the reference's order."""
+
+    # positions in sumflux's signature
+    DIFF = (0, 1, 2, 16, 17, 18)
+
+    @staticmethod
+    def forward(ctx, ru, rv, ww, ru_m, rv_m, ww_m, args):
+        from . import wrf_dynamics
+        args = list(args)
+        for pos, x in zip(SumFlux.DIFF, (ru, rv, ww, ru_m, rv_m, ww_m)):
+            args[pos] = _f32(x)
+        for pos in (16, 17, 18):
+            args[pos] = args[pos].clone()
+        ctx.args = args
+        return wrf_dynamics.sumflux(*args)
+
+    @staticmethod
+    def backward(ctx, l1, l2, l3):
+        a = ctx.args
+        g = grid(a[0].shape, tuple(a[-18:]))
+        lam = [torch.zeros_like(a[n]) if l is None else _f32(l) for l, n in zip((l1, l2, l3), (16, 17, 18))]
+        out = [torch.empty_like(a[n]) for n in (0, 1, 2, 16, 17, 18)]
+        call("pywrf_sumflux_adjoint", g, *[ptr(x) for x in lam], int(a[25]), int(a[26]),
+             *[ptr(x) for x in out], stream())
+        return (*out, None)
+
+
+class SpecBdyUpdatePh(torch.autograd.Function):
+    """spec_bdyupdate_ph (wrf_boundary.py:287-358) with the gradient with
+    respect to the field and muts (ph_save, field_tend and mu_tend held
+    fixed); the update goes to a copy of the field.  args: its positional
+    arguments in the reference's order."""
+
+    @staticmethod
+    def forward(ctx, field, muts, args):
+        from . import wrf_boundary
+        args = list(args)
+        ctx.f_in = _f32(field)
+        args[1], args[4] = ctx.f_in.clone(), _f32(muts)
+        ctx.args = args
+        return wrf_boundary.spec_bdyupdate_ph(*args)
+
+    @staticmethod
+    def backward(ctx, lf):
+        a = ctx.args
+        g = grid(a[1].shape, tuple(a[-24:]))
+        lam = torch.zeros_like(a[1]) if lf is None else _f32(lf).clone()
+        lmuts = torch.zeros_like(a[4])
+        call("pywrf_spec_bdyupdate_ph_adjoint", g, ptr(a[0]), ptr(ctx.f_in), ptr(a[2]), ptr(a[3]), ptr(a[4]),
+             float(a[7]), flag(a[8]), ptr(lam), ptr(lmuts), stream())
+        return lam, lmuts, None
+
+
+class ZeroGradBdy(torch.autograd.Function):
+    """zero_grad_bdy (wrf_boundary.py:361-405) on a copy of the field, with
+    its gradient.  args: its positional arguments in the reference's order."""
+
+    @staticmethod
+    def forward(ctx, field, args):
+        from . import wrf_boundary
+        args = list(args)
+        args[0] = _f32(field).clone()
+        ctx.args = args
+        return wrf_boundary.zero_grad_bdy(*args)
+
+    @staticmethod
+    def backward(ctx, lf):
+        a = ctx.args
+        g = grid(a[0].shape, tuple(a[-24:]))
+        lam = torch.zeros_like(a[0]) if lf is None else _f32(lf)
+        out = torch.empty_like(a[0])
+        call("pywrf_zero_grad_bdy_adjoint", g, ptr(lam), flag(a[1]), ptr(out), stream())
+        return out, None
+
+
+class OpenBoundaryFill(torch.autograd.Function):
+    """A run of set_physical_bc3d / set_physical_bc2d calls
+    (wrf_dynamics.py:164-278) on copies of distinct fields, with the gradient
+    with respect to each.  kinds: [(variable, is_2d)] per field; idx24 /
+    idx2d: the index tuples the individual calls would take."""
+
+    @staticmethod
+    def forward(ctx, kinds, idx24, idx2d, *fields):
+        from . import wrf_dynamics
+        outs = [_f32(f).clone() for f in fields]
+        wrf_dynamics.set_physical_bc_group([(o, v, two) for o, (v, two) in zip(outs, kinds)], idx24, idx2d)
+        ctx.kinds, ctx.idx = kinds, (idx24, idx2d)
+        ctx.shapes = [o.shape for o in outs]
+        ctx.like = outs[0]
+        return tuple(outs)
+
+    @staticmethod
+    def backward(ctx, *lams):
+        import ctypes
+        kinds, (idx24, idx2d) = ctx.kinds, ctx.idx
+        lam = [torch.zeros(sh, dtype=torch.float32, device=ctx.like.device) if l is None else _f32(l).clone()
+               for l, sh in zip(lams, ctx.shapes)]
+        sh3 = next((sh for sh, (_, two) in zip(ctx.shapes, kinds) if not two), (idx24[5],) + tuple(ctx.shapes[0][-2:]))
+        sh2 = next((sh for sh, (_, two) in zip(ctx.shapes, kinds) if two), tuple(ctx.shapes[0][-2:]))
+        n = len(lam)
+        ptrs = (ctypes.c_void_p * n)(*[ptr(t).value for t in lam])
+        flags = b"".join(v.encode() for v, _ in kinds)
+        two = (ctypes.c_int * n)(*[1 if t else 0 for _, t in kinds])
+        call("pywrf_set_physical_bc_group_adjoint", grid(sh3, idx24), grid(sh2, idx2d), n, ptrs, flags, two,
+             stream())
+        return (None, None, None, *lam)
 
 
 # ----------------------------------------------------------------------- tracer transport

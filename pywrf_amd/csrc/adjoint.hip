@@ -17,6 +17,7 @@
 #include "common.h"
 #include "consts.h"
 #include "diff.h"
+#include "openbc.h"
 
 int pw_check_grid(const pywrf_grid_t* g, int r);
 #define CHECK(g, r) do { int _e = pw_check_grid((g), (r)); if (_e) return _e; } while (0)
@@ -742,5 +743,219 @@ extern "C" int pywrf_advance_mu_t_adjoint(const pywrf_grid_t* g, const float* la
   if (j0 < 1 || i0 < 1 || j1 + 1 > g->ny || i1 + 1 > g->nx) return PYWRF_E_RANGE;
   Box b = mkbox(0, 1, j0, j1, i0, i1);
   PW_LAUNCH_COLS(k_advance_mu_t_adj, b, s, dims_of(g), a);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ calc_p_rho^T
+// calc_p_rho (wrf_dynamics.py:3269-3313) in its box (levels kts..min(kte,
+// kde-1), the interior columns):
+//   al = -(alt mu + rdnw[k] (ph[k+1] - ph[k])) / mut,
+//   p  = c2a (alt (t_2 - mu t_1) / (mut (t0 + t_1)) - al),
+// then, on a call with step != 0, p' = p + smdiv (p - pm1) and pm1' = p (in
+// the box); at step 0 pm1' is a copy of the whole of p.  Outside the box al and
+// p keep their incoming values.  alt, t_1 and c2a are held fixed.  One thread
+// per column walks the levels: ph[k]'s sensitivity gathers al[k]'s and
+// al[k-1]'s, those of mu and mut are summed down the column.
+struct AdjPRho {
+  const float *lal, *lp, *lpm1;                          // sensitivities of the results
+  const float *ph, *alt, *t_2, *t_1, *c2a, *mu, *mut, *rdnw;
+  float *gal, *gp, *gpm1, *gph, *gt2, *gmu, *gmut;     // sensitivities of the inputs
+  float t0, smdiv;
+  int step;
+};
+
+__global__ void k_calc_p_rho_adj(Box all, Dims D, Box b, AdjPRho a) {
+  int j, i;
+  if (!box_column(all, j, i)) return;
+  const bool col = b.has2(j, i);
+  const float mut = col ? AT2(a.mut, j, i) : 1.0f, mu = col ? AT2(a.mu, j, i) : 0.0f;
+  float gmu = 0.0f, gmut = 0.0f, prev = 0.0f;   // prev: rdnw[k-1] (al[k-1]'s sensitivity) / mut
+  for (int k = 0; k < D.nz; ++k) {
+    const long q = D.ix(k, j, i);
+    const float lal = a.lal[q], lp = a.lp[q], lpm = a.lpm1[q];
+    float r = 0.0f;
+    if (col && k >= b.k0 && k < b.k1) {
+      const float ph0 = a.ph[q], ph1 = AT(a.ph, k + 1, j, i);
+      const float altq = a.alt[q], t1 = a.t_1[q], t2 = a.t_2[q], c2 = a.c2a[q], rd = a.rdnw[k];
+      const float al = -1.0f / mut * (altq * mu + rd * (ph1 - ph0));
+      const float den = mut * (a.t0 + t1);
+      const float pa = altq * (t2 - mu * t1) / den;
+      float lpr;                                 // sensitivity of p before the damping
+      if (a.step != 0) {
+        lpr = lp * (1.0f + a.smdiv) + lpm;
+        a.gpm1[q] = -a.smdiv * lp;
+      } else {
+        lpr = lp + lpm;
+        a.gpm1[q] = 0.0f;
+      }
+      const float cl = c2 * lpr;
+      const float L = lal - cl;                  // sensitivity of al, through p as well
+      a.gt2[q] = cl * altq / den;
+      gmu += cl * (-(altq * t1) / den) + L * (-altq / mut);
+      gmut += cl * (-pa / mut) + L * (-al / mut);
+      r = rd * L / mut;
+      a.gal[q] = 0.0f;
+      a.gp[q] = 0.0f;
+    } else {
+      a.gal[q] = lal;
+      a.gp[q] = a.step != 0 ? lp : lp + lpm;
+      a.gpm1[q] = a.step != 0 ? lpm : 0.0f;
+      a.gt2[q] = 0.0f;
+    }
+    a.gph[q] = r - prev;
+    prev = r;
+  }
+  AT2(a.gmu, j, i) = gmu;
+  AT2(a.gmut, j, i) = gmut;
+}
+
+extern "C" int pywrf_calc_p_rho_adjoint(const pywrf_grid_t* g, const float* lam_al, const float* lam_p,
+                                        const float* lam_pm1, const float* ph, const float* alt, const float* t_2,
+                                        const float* t_1, const float* c2a, const float* mu, const float* mut,
+                                        double t0, const float* rdnw, double smdiv, int step, float* lam_al_ori,
+                                        float* lam_p_ori, float* lam_pm1_in, float* lam_ph, float* lam_t_2,
+                                        float* lam_mu, float* lam_mut, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
+  CHECK(g, 0);
+  Box b = mkbox(g->kts, imin(g->kte, g->kde - 1), g->jts, imin(g->jte, g->jde - 1), g->its,
+                imin(g->ite, g->ide - 1));
+  if (b.k1 + 1 > g->nz) return PYWRF_E_RANGE;
+  AdjPRho a = {lam_al, lam_p, lam_pm1, ph, alt, t_2, t_1, c2a, mu, mut, rdnw,
+               lam_al_ori, lam_p_ori, lam_pm1_in, lam_ph, lam_t_2, lam_mu, lam_mut, F(t0), F(smdiv), step};
+  PW_LAUNCH_COLS(k_calc_p_rho_adj, whole(g), s, dims_of(g), b, a);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ sumflux^T
+// sumflux (wrf_dynamics.py:3799-3839) is linear: on its first call the three
+// accumulators are zeroed over the tile, each then adds its flux over its box,
+// and on the last call the sum is divided by the number of calls (the u_lin
+// terms are fixed).  One thread per point.
+__global__ void k_sumflux_adj(Box all, Dims D, const float* lru_m, const float* lrv_m, const float* lww_m,
+                              float* gru, float* grv, float* gww, float* gru_m, float* grv_m, float* gww_m, Box b,
+                              Box bu, Box bv, Box bw, int first, int last, float n) {
+  int k, j, i;
+  if (!box_point(all, k, j, i)) return;
+  const long q = D.ix(k, j, i);
+  const bool zeroed = first && b.has(k, j, i);
+  const float* lam[3] = {lru_m, lrv_m, lww_m};
+  float* gf[3] = {gru, grv, gww};
+  float* ga[3] = {gru_m, grv_m, gww_m};
+  const Box* bx[3] = {&bu, &bv, &bw};
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    const float l = lam[f][q];
+    const bool in = bx[f]->has(k, j, i);
+    const float s = last ? l / n : l;
+    gf[f][q] = in ? s : 0.0f;
+    ga[f][q] = zeroed ? 0.0f : (in ? s : l);
+  }
+}
+
+extern "C" int pywrf_sumflux_adjoint(const pywrf_grid_t* g, const float* lam_ru_m, const float* lam_rv_m,
+                                     const float* lam_ww_m, int iteration, int number_of_small_timesteps,
+                                     float* lam_ru, float* lam_rv, float* lam_ww, float* lam_ru_m_in,
+                                     float* lam_rv_m_in, float* lam_ww_m_in, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
+  CHECK(g, 0);
+  if (g->kte > g->nz || number_of_small_timesteps == 0) return PYWRF_E_RANGE;
+  int mini = imin(g->ide - 1, g->ite), minj = imin(g->jde - 1, g->jte), mink = imin(g->kde - 1, g->kte);
+  Box bu = mkbox(g->kts, mink, g->jts, minj, g->its, g->ite);
+  Box bv = mkbox(g->kts, mink, g->jts, g->jte, g->its, mini);
+  Box bw = mkbox(g->kts, g->kte, g->jts, minj, g->its, mini);
+  Box b = mkbox(g->kts, g->kte, g->jts, g->jte, g->its, g->ite);
+  PW_LAUNCH_BOX(k_sumflux_adj, whole(g), s, dims_of(g), lam_ru_m, lam_rv_m, lam_ww_m, lam_ru, lam_rv, lam_ww,
+                lam_ru_m_in, lam_rv_m_in, lam_ww_m_in, b, bu, bv, bw, iteration == 1,
+                iteration == number_of_small_timesteps, (float)number_of_small_timesteps);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ open-boundary fill^T
+// The fill of one plane (set_physical_bc3d / bc2d, wrf_dynamics.py:164-278) is
+// a run of copies: the x-halos from the first / last interior column (over
+// rows that include the y-halos), then the y-halos from the first / last
+// interior row.  Its transpose runs the two in the opposite order, each copy
+// target handing its sensitivity to its source and keeping none: first the
+// y-copies (one thread per column and side, its three targets in turn), a
+// barrier, then the x-copies (one thread per row and side).  No two threads
+// of one phase share a source or a target, so no atomics are needed; an
+// x-copy's source in a y-halo row then receives the sensitivity of the value
+// it held before the y-copies, which is what the x-copies read.
+__device__ inline void open_bc_plane_adj(float* a, int nx, const BcSpec& c) {
+  const int ny_items = 2 * (c.yc1 - c.yc0);
+  for (int t = (int)threadIdx.x; t < ny_items; t += (int)blockDim.x) {
+    const int col = c.yc0 + t / 2, side = t & 1;
+    int srow, d0;
+    if (side == 0) { srow = c.jds; d0 = c.jds - 3; }
+    else if (c.vstag) { srow = c.jde - 1; d0 = c.jde; }
+    else { srow = c.jde - 2; d0 = c.jde - 1; }
+    float sum = 0.0f;
+    for (int h = 0; h < 3; ++h) {
+      float* p = a + (long)(d0 + h) * nx + col;
+      sum += *p;
+      *p = 0.0f;
+    }
+    a[(long)srow * nx + col] += sum;
+  }
+  __syncthreads();
+  const int r0 = c.ustag ? c.ur0 : c.xr0, r1 = c.ustag ? c.ur1 : c.xr1;
+  const int nstart = c.xr1 - c.xr0, nx_items = nstart + (r1 - r0);
+  for (int t = (int)threadIdx.x; t < nx_items; t += (int)blockDim.x) {
+    int row, src, d0;
+    if (t < nstart) { row = c.xr0 + t; src = c.ids; d0 = c.ids - 3; }
+    else { row = r0 + (t - nstart); src = c.ustag ? c.ide - 1 : c.ide - 2; d0 = c.ustag ? c.ide : c.ide - 1; }
+    float* base = a + (long)row * nx;
+    float sum = 0.0f;
+    for (int h = 0; h < 3; ++h) {
+      sum += base[d0 + h];
+      base[d0 + h] = 0.0f;
+    }
+    base[src] += sum;
+  }
+}
+
+#define PW_BC_ADJ_MAX 32
+struct BcAdjJobs {
+  int n;
+  BcJob job[PW_BC_ADJ_MAX];
+};
+
+__global__ void k_open_bc_adj_group(long plane, int nx, BcAdjJobs J) {
+  if ((int)blockIdx.y >= J.n) return;
+  const BcJob& jb = J.job[blockIdx.y];
+  if ((int)blockIdx.x >= jb.nk) return;
+  open_bc_plane_adj(jb.p + (long)(jb.k0 + (int)blockIdx.x) * plane, nx, jb.c);
+}
+
+extern "C" int pywrf_set_physical_bc_group_adjoint(const pywrf_grid_t* g3, const pywrf_grid_t* g2, int n,
+                                                   void* const* lam_fields, const char* variables,
+                                                   const int* is2d, pywrf_stream_t s) {
+  if (!pw_size_ok(g3) || !pw_size_ok(g2)) return PYWRF_E_RANGE;
+  if (n < 0 || n > PW_BC_ADJ_MAX || g3->ny != g2->ny || g3->nx != g2->nx) return PYWRF_E_ARG;
+  BcAdjJobs J;
+  J.n = n;
+  int maxk = 1;
+  for (int f = 0; f < n; ++f) {
+    BcJob& jb = J.job[f];
+    jb.p = (float*)lam_fields[f];
+    int e, k1 = 1;
+    if (is2d[f]) {
+      e = bc_spec2(g2, variables[f], jb.c);
+      jb.k0 = 0;
+      if (!e && !bc_spec_ok(g2->ny, g2->nx, 1, 0, 1, jb.c)) e = PYWRF_E_RANGE;
+    } else {
+      e = bc_spec3(g3, variables[f], jb.c, k1);
+      jb.k0 = g3->kts;
+      if (!e && !bc_spec_ok(g3->ny, g3->nx, g3->nz, g3->kts, k1, jb.c)) e = PYWRF_E_RANGE;
+    }
+    if (e) return e;
+    // the copies' sources must lie outside their targets (interior rows and columns)
+    if (jb.c.jde - 2 <= jb.c.jds || jb.c.ide - 2 <= jb.c.ids) return PYWRF_E_RANGE;
+    jb.nk = k1 - jb.k0 > 0 ? k1 - jb.k0 : 0;
+    maxk = imax(maxk, jb.nk);
+  }
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_open_bc_adj_group, dim3(maxk, n), dim3(BC_THREADS), 0, (hipStream_t)s,
+                     (long)g3->ny * g3->nx, g3->nx, J);
   PW_RET();
 }

@@ -492,6 +492,93 @@ extern "C" int pywrf_spec_bdyupdate_ph_zero_grad(const pywrf_grid_t* g, const fl
   PW_RET();
 }
 
+// ------------------------------------------------------------------ spec_bdyupdate_ph^T, zero_grad_bdy^T
+// On its strips spec_bdyupdate_ph sets f' = f r + dt ft / muts + ph_save (r - 1)
+// with r = (muts - dt mu_tend) / muts (applied once per strip holding the
+// point).  Its transpose with ph_save, ft and mu_tend fixed: f's sensitivity
+// is scaled by r per application, and muts's gathers, down the column, the
+// derivative of f' with respect to muts, carried through the applications in
+// forward mode.  One thread per column of the band; lam holds f''s
+// sensitivity on entry and f's on return; lam_muts is written on the band's
+// columns only.
+__global__ void k_spec_bdyupdate_ph_adj(Box cols, Dims D, PhStrip a, const float* f_in, float* lam,
+                                        float* lam_muts) {
+  int j, i;
+  if (!box_column(cols, j, i)) return;
+  bool ir = i >= a.ibs && i < a.ibe, jc = j >= a.jbs + 1 && j < a.jbe - 1;
+  int n = (j == a.jts && ir) + (j == a.jbe - 1 && ir) + (i == a.ibs && jc) + (i == a.ibe - 1 && jc);
+  if (!n) return;
+  const float mt = AT2(a.muts, j, i), mtend = AT2(a.mu_tend, j, i);
+  const float mo = mt - a.dt * mtend, r = mo / mt, dr = a.dt * mtend / (mt * mt);
+  float gm = 0.0f;
+  for (int k = a.all.k0; k < a.all.k1; ++k) {
+    const long q = D.ix(k, j, i);
+    const float ft = a.ft[q], ps = a.ph_save[q];
+    const float dc = -a.dt * ft / (mt * mt) + ps * dr;
+    float v = f_in[q], dv = 0.0f, s = 1.0f;
+    for (int m = 0; m < n; ++m) {
+      dv = dv * r + v * dr + dc;
+      v = v * mo / mt + a.dt * ft / mt + ps * (mo / mt - 1.0f);
+      s *= r;
+    }
+    const float l = lam[q];
+    gm += l * dv;
+    lam[q] = l * s;
+  }
+  AT2(lam_muts, j, i) = gm;
+}
+
+extern "C" int pywrf_spec_bdyupdate_ph_adjoint(const pywrf_grid_t* g, const float* ph_save, const float* field_in,
+                                               const float* field_tend, const float* mu_tend, const float* muts,
+                                               double dt, char variable_in, float* lam_field, float* lam_muts,
+                                               pywrf_stream_t s) {
+  PhStrip a;
+  if (int e = ph_strip_setup(g, ph_save, nullptr, field_tend, mu_tend, muts, dt, variable_in, a)) return e;
+  Box cols = mkbox(0, 1, a.all.j0, a.all.j1, a.all.i0, a.all.i1);
+  PW_LAUNCH_COLS(k_spec_bdyupdate_ph_adj, cols, s, dims_of(g), a, field_in, lam_field, lam_muts);
+  PW_RET();
+}
+
+// zero_grad_bdy copies, on every ring point of its box, the nearest point of
+// the next ring in.  Its transpose, out of place over the whole array: a ring
+// point keeps no sensitivity, a point of the next ring in gathers those of
+// the ring points that copy it (its outward neighbours in j, in i and on the
+// diagonal), every other point passes its own through.
+__global__ void k_zero_grad_adj(Box all, Dims D, ZgStrip a, const float* lam, float* out) {
+  int k, j, i;
+  if (!box_point(all, k, j, i)) return;
+  const long q = D.ix(k, j, i);
+  float v = lam[q];
+  if (a.all.has(k, j, i)) {
+    const bool ring = (j == a.jbs || j == a.jbe - 1) || (i == a.ibs || i == a.ibe - 1);
+    if (ring) {
+      v = 0.0f;
+    } else {
+      // ring rows / columns whose clamped source row / column is j / i
+      int rj[3] = {j, -1, -1}, ri[3] = {i, -1, -1};
+      if (j == a.jbs + 1) rj[1] = a.jbs;
+      if (j == a.jbe - 2) rj[2] = a.jbe - 1;
+      if (i == a.ibs + 1) ri[1] = a.ibs;
+      if (i == a.ibe - 2) ri[2] = a.ibe - 1;
+      for (int x = 0; x < 3; ++x)
+        for (int y = 0; y < 3; ++y)
+          if ((x || y) && rj[x] >= 0 && ri[y] >= 0) v += AT(lam, k, rj[x], ri[y]);
+    }
+  }
+  out[q] = v;
+}
+
+extern "C" int pywrf_zero_grad_bdy_adjoint(const pywrf_grid_t* g, const float* lam, char variable_in, float* lam_in,
+                                           pywrf_stream_t s) {
+  ZgStrip a;
+  if (int e = zero_grad_setup(g, nullptr, variable_in, a)) return e;
+  // every ring point's source must be an interior point
+  if (a.jbe - a.jbs < 3 || a.ibe - a.ibs < 3) return PYWRF_E_RANGE;
+  if (lam == lam_in) return PYWRF_E_ARG;
+  PW_LAUNCH_BOX(k_zero_grad_adj, whole(g), s, dims_of(g), a, lam, lam_in);
+  PW_RET();
+}
+
 // ------------------------------------------------------------------ mass_weight
 __global__ void k_mass_weight(Box b, Dims D, const float* field, const float* mut, float* rfield) {
   int k, j, i;

@@ -530,3 +530,182 @@ def test_acoustic_adjoints_chain_through_a_substep(config):
         err = abs(got - ref) / max(abs(ref), float(terms.norm()))
         print("d/d%s: <grad, d> %.6e  difference %.6e  rel %.2e" % (n, got, ref, err))
         assert err <= 1e-4, n
+
+
+def _clone(a):
+    if isinstance(a, torch.Tensor):
+        return a.clone()
+    if isinstance(a, list):
+        return [_clone(x) for x in a]
+    if isinstance(a, tuple):
+        return tuple(_clone(x) for x in a)
+    return a
+
+
+def _double(a):
+    if isinstance(a, torch.Tensor):
+        return a.detach().double().cpu()
+    if isinstance(a, list):
+        return [_double(x) for x in a]
+    if isinstance(a, tuple):
+        return tuple(_double(x) for x in a)
+    return a
+
+
+class _Stage:
+    """The step's functions, keeping clones of the arguments of every call of
+    the acoustic loop that starts at the n-th call of advance_uv, up to the
+    loop's end (the next calc_mu_uv_1)."""
+
+    def __init__(self, ops, n):
+        self._ops, self._n, self.seen, self.calls, self.done = ops, n, 0, [], False
+
+    def __getattr__(self, name):
+        f = getattr(self._ops, name)
+
+        def rec(*args):
+            if name == "advance_uv":
+                self.seen += 1
+            if name == "calc_mu_uv_1" and self.seen >= self._n:
+                self.done = True
+            if self.seen >= self._n and not self.done:
+                self.calls.append((name, _clone(list(args))))
+            return f(*args)
+        return rec
+
+
+SUBSTEP = ["advance_uv", "spec_bdyupdate_group", "advance_mu_t", "spec_bdyupdate_group", "advance_w", "sumflux",
+           "spec_bdyupdate_ph_zero_grad", "calc_p_rho", "set_physical_bc_group"]
+STATE = ("u", "v", "p", "ph", "al", "mu", "mudf", "ww", "muave", "muts", "t", "t_ave", "w", "pm1", "ru_m", "rv_m",
+         "ww_m")
+
+
+@pytest.mark.parametrize("config", ["config1c", "bdy24"])
+def test_acoustic_loop_adjoint_matches_fp64_differences(config):
+    """The gradient of the second RK3 stage's whole acoustic loop (two
+    substeps, each advance_uv, the u/v zone update, advance_mu_t, the
+    t/mu/muts zone update, advance_w, sumflux, spec_bdyupdate_ph,
+    zero_grad_bdy, calc_p_rho and the six open-boundary fills) chained by
+    autograd from the hand-written adjoints, with respect to all 17 fields
+    the loop carries (u, v, p, ph, al, mu, mudf, ww, muave, muts, t, t_ave,
+    w, pm1 and the three flux accumulators), against FP64 central
+    differences of the same sequence through the oracle: for each field a
+    seeded direction d, <grad, d> against the difference of
+    sum_o <lam_o, out_o> over the 17 results, relative to the larger of the
+    difference and the 2-norm of grad_i d_i; bound 1e-4.  The two substeps
+    take sumflux's first and last branches and calc_p_rho's damped one."""
+    import oracle
+    import pywrf_amd
+    from pywrf_amd import adjoint
+    from pywrf_amd.gen import synthetic_state
+    from pywrf_amd.step import rk3_step, RunConfig
+    s = synthetic_state(config, device="cuda")
+    rec = _Stage(pywrf_amd.ops(), 2)
+    rk3_step(s, rec, RunConfig())
+    names = [n for n, _ in rec.calls]
+    assert names == SUBSTEP * 2, names
+    subs = [[a for _, a in rec.calls[9 * m:9 * m + 9]] for m in range(2)]
+    o = pywrf_amd.ops()
+
+    def zone(call, fields):
+        """the zone update on copies (its Jacobian is the identity: it adds fixed tendencies)"""
+        items, dt, ix24, ix2d = call
+        outs = [f.detach().clone() for f in fields]
+        o.spec_bdyupdate_group([(f, it[1], it[2], it[3]) for f, it in zip(outs, items)], dt, ix24, ix2d)
+        return [f + (a - f.detach()) for a, f in zip(outs, fields)]
+
+    uv, _, mt, _, wa, _, _, cp, _ = subs[0]
+    first = {"u": uv[0], "v": uv[2], "p": uv[4], "ph": uv[6], "al": uv[9], "mu": uv[10], "mudf": uv[15],
+             "ww": mt[0], "muave": mt[8], "muts": mt[9], "t": mt[24], "t_ave": mt[26], "w": wa[0], "pm1": cp[7],
+             "ru_m": mt[21], "rv_m": mt[22], "ww_m": mt[23]}
+    x = {n: first[n].clone().requires_grad_() for n in STATE}
+
+    st = dict(x)
+    for uv, z1, mt, z2, wa, sf, pz, cp, bc in subs:
+        u1, v1 = adjoint.AdvanceUV.apply(st["u"], uv[1], st["v"], uv[3], st["p"], st["ph"], st["al"], st["mu"],
+                                         st["mudf"], uv)
+        u2, v2 = zone(z1, [u1, v1])
+        ww_o, t_o, tave_o, muave_o, muts_o, mudf_o, mu_o = adjoint.AdvanceMuT.apply(
+            u2, v2, st["ww"], st["mu"], st["muave"], st["muts"], st["mudf"], st["t"], st["t_ave"], mt[27], mt[28],
+            mt)
+        t2, mu2, muts2 = zone(z2, [t_o, mu_o, muts_o])
+        t2a, w_o, ph_o = adjoint.AdvanceW.apply(st["w"], st["ph"], t2, tave_o, u2, v2, muts2, muave_o, ww_o, wa)
+        ru_m, rv_m, ww_m = adjoint.SumFlux.apply(u2, v2, ww_o, st["ru_m"], st["rv_m"], st["ww_m"], sf)
+        ph_s = adjoint.SpecBdyUpdatePh.apply(ph_o, muts2, pz[0])
+        w_z = adjoint.ZeroGradBdy.apply(w_o, pz[1])
+        al, p, pm1 = adjoint.CalcPRho.apply(st["al"], st["p"], ph_s, t2, st["pm1"], mu2, muts2, cp)
+        items, ix24, ix2d = bc
+        ph_f, al_f, p_f, muts_f, mu_f, mudf_f = adjoint.OpenBoundaryFill.apply(
+            [(v, two) for _, v, two in items], ix24, ix2d, ph_s, al, p, muts2, mu2, mudf_o)
+        st = {"u": u2, "v": v2, "p": p_f, "ph": ph_f, "al": al_f, "mu": mu_f, "mudf": mudf_f, "ww": ww_o,
+              "muave": muave_o, "muts": muts_f, "t": t2, "t_ave": t2a, "w": w_z, "pm1": pm1, "ru_m": ru_m,
+              "rv_m": rv_m, "ww_m": ww_m}
+    gen = torch.Generator().manual_seed(23)
+    lams = {n: torch.randn(tuple(st[n].shape), generator=gen, dtype=torch.float64) for n in STATE}
+    sum((st[n] * lams[n].float().cuda()).sum() for n in STATE).backward()
+
+    # the same sequence through the oracle in FP64
+    ro = oracle.ops()
+    subs64 = [_double(c) for c in subs]
+
+    def ozone(call, fields):
+        items, dt, ix24, ix2d = call
+        return [ro.spec_bdyupdate(f.clone(), it[1], dt, it[2], 1, *(ix2d if it[3] else ix24))
+                for f, it in zip(fields, items)]
+
+    def loop(v):
+        st = {n: v[n].clone() for n in STATE}
+        for uv, z1, mt, z2, wa, sf, pz, cp, bc in subs64:
+            a = list(uv)
+            a[0], a[2], a[4], a[6], a[9], a[10], a[15] = (st["u"], st["v"], st["p"], st["ph"], st["al"], st["mu"],
+                                                          st["mudf"])
+            u2, v2 = ozone(z1, ro.advance_uv(*[_clone(q) for q in a]))
+            m = _clone(list(mt))
+            m[0], m[2], m[4], m[6], m[8], m[9], m[12], m[24], m[26] = (
+                st["ww"], u2, v2, st["mu"], st["muave"], st["muts"], st["mudf"], st["t"], st["t_ave"])
+            m = [_clone(q) for q in m]
+            r = ro.advance_mu_t(*m)
+            t2, mu2, muts2 = ozone(z2, [r[2], r[10], r[8]])
+            w_ = _clone(list(wa))
+            w_[0], w_[2], w_[4], w_[5], w_[8], w_[9], w_[18], w_[19], w_[21] = (
+                st["w"].clone(), r[0], u2, v2, r[7], muts2, r[3].clone(), t2, st["ph"].clone())
+            t2a, wo, pho = ro.advance_w(*w_)
+            f = _clone(list(sf))
+            f[0], f[1], f[2], f[16], f[17], f[18] = u2, v2, r[0], st["ru_m"].clone(), st["rv_m"].clone(), \
+                st["ww_m"].clone()
+            ru_m, rv_m, ww_m = ro.sumflux(*f)
+            pa, za = list(_clone(pz[0])), list(_clone(pz[1]))
+            pa[1], pa[4] = pho.clone(), muts2
+            ph_s = ro.spec_bdyupdate_ph(*pa)
+            za[0] = wo.clone()
+            w_z = ro.zero_grad_bdy(*za)
+            c = _clone(list(cp))
+            c[0], c[1], c[2], c[4], c[7], c[8], c[9] = st["al"], st["p"], ph_s, t2, st["pm1"].clone(), mu2, muts2
+            al, p, pm1 = ro.calc_p_rho(*c)
+            items, ix24, ix2d = bc
+            filled = []
+            for q, (_, var, two) in zip((ph_s, al, p, muts2, mu2, r[9]), items):
+                q = q.clone()
+                filled.append(ro.set_physical_bc2d(q, var, *ix2d) if two else ro.set_physical_bc3d(q, var, *ix24))
+            ph_f, al_f, p_f, muts_f, mu_f, mudf_f = filled
+            st = {"u": u2, "v": v2, "p": p_f, "ph": ph_f, "al": al_f, "mu": mu_f, "mudf": mudf_f, "ww": r[0],
+                  "muave": r[7], "muts": muts_f, "t": t2, "t_ave": t2a, "w": w_z, "pm1": pm1, "ru_m": ru_m,
+                  "rv_m": rv_m, "ww_m": ww_m}
+        return sum(float((st[n] * lams[n]).sum()) for n in STATE)
+
+    base = {n: x[n].detach().double().cpu() for n in STATE}
+    worst = 0.0
+    for n in STATE:
+        d = torch.randn(tuple(base[n].shape), generator=gen, dtype=torch.float64)
+        eps = 1e-4 * float(base[n].abs().max() + 1.0) / float(d.abs().max())
+        vp, vm = dict(base), dict(base)
+        vp[n], vm[n] = base[n] + eps * d, base[n] - eps * d
+        ref = (loop(vp) - loop(vm)) / (2.0 * eps)
+        terms = x[n].grad.double().cpu() * d
+        got = float(terms.sum())
+        err = abs(got - ref) / max(abs(ref), float(terms.norm()), 1e-300)
+        worst = max(worst, err)
+        print("d/d%s: <grad, d> %.6e  difference %.6e  rel %.2e" % (n, got, ref, err), flush=True)
+    for n in STATE:
+        assert x[n].grad is not None, n
+    assert worst <= 1e-4
