@@ -412,6 +412,150 @@ class OpenBoundaryFill(torch.autograd.Function):
         return (None, None, None, *lam)
 
 
+class SpecBdyUpdateGroup(torch.autograd.Function):
+    """A run of spec_bdyupdate calls (wrf_boundary.py:228-284) on copies of
+    distinct fields: each adds its fixed tendency times dt on the outer ring,
+    so the Jacobian with respect to each field is the identity.  call: the
+    grouped call's arguments (items, dt, idx24, idx2d) as the step makes it,
+    the items' fields being replaced by `fields`."""
+
+    @staticmethod
+    def forward(ctx, call, *fields):
+        from . import wrf_boundary
+        items, dt, ix24, ix2d = call
+        outs = [_f32(f).clone() for f in fields]
+        wrf_boundary.spec_bdyupdate_group([(o, it[1], it[2], it[3]) for o, it in zip(outs, items)], dt, ix24, ix2d)
+        return tuple(outs)
+
+    @staticmethod
+    def backward(ctx, *lams):
+        return (None, *lams)
+
+
+# ----------------------------------------------------------------------- the acoustic loop
+# The 17 fields an RK stage's acoustic loop (solver.py:1349-1493) carries from
+# one substep to the next; everything else its calls read is fixed for the
+# stage (the large-step tendencies, the saved *_save / *_1 fields, the
+# coefficients a, alpha, gamma, c2a, the map factors).
+ACOUSTIC_STATE = ("u", "v", "p", "ph", "al", "mu", "mudf", "ww", "muave", "muts", "t", "t_ave", "w", "pm1",
+                  "ru_m", "rv_m", "ww_m")
+SUBSTEP_CALLS = ("advance_uv", "spec_bdyupdate_group", "advance_mu_t", "spec_bdyupdate_group", "advance_w",
+                 "sumflux", "spec_bdyupdate_ph_zero_grad", "calc_p_rho", "set_physical_bc_group")
+
+
+class _LoopRecorder:
+    """The step's functions, keeping copies of the arguments of every call of
+    the acoustic loop that starts at the n-th call of advance_uv, up to the
+    loop's end (the next calc_mu_uv_1)."""
+
+    def __init__(self, ops, n):
+        self._ops, self._n, self._seen, self.calls, self._done = ops, n, 0, [], False
+
+    @staticmethod
+    def _copy(a):
+        if isinstance(a, torch.Tensor):
+            return a.clone()
+        if isinstance(a, (list, tuple)):
+            return type(a)(_LoopRecorder._copy(x) for x in a)
+        return a
+
+    def __getattr__(self, name):
+        f = getattr(self._ops, name)
+
+        def rec(*args):
+            if name == "advance_uv":
+                self._seen += 1
+            if name == "calc_mu_uv_1" and self._seen >= self._n:
+                self._done = True
+            if self._seen >= self._n and not self._done:
+                self.calls.append((name, self._copy(list(args))))
+            return f(*args)
+        return rec
+
+
+def record_acoustic_loop(s, ops, cfg, stage=2):
+    """Run one RK3 step on state s and keep the acoustic loop of RK stage
+    `stage` (1, 2 or 3): returns (x, substeps) with x the 17 carried fields at
+    the loop's start (ACOUSTIC_STATE order) and substeps one list of the nine
+    calls' arguments per substep.  The step must take the grouped calls (the
+    library's own ops do)."""
+    from .step import rk3_step
+    rec = _LoopRecorder(ops, {1: 1, 2: 2, 3: 2 + cfg.soundsteps // 2}[stage])
+    rk3_step(s, rec, cfg)
+    names = [n for n, _ in rec.calls]
+    m = len(names) // len(SUBSTEP_CALLS)
+    if m == 0 or names != list(SUBSTEP_CALLS) * m:
+        raise RuntimeError("record_acoustic_loop: unexpected call sequence %s" % names[:12])
+    subs = [[a for _, a in rec.calls[9 * q:9 * q + 9]] for q in range(m)]
+    uv, _, mt, _, wa, _, _, cp, _ = subs[0]
+    first = {"u": uv[0], "v": uv[2], "p": uv[4], "ph": uv[6], "al": uv[9], "mu": uv[10], "mudf": uv[15],
+             "ww": mt[0], "muave": mt[8], "muts": mt[9], "t": mt[24], "t_ave": mt[26], "w": wa[0], "pm1": cp[7],
+             "ru_m": mt[21], "rv_m": mt[22], "ww_m": mt[23]}
+    return [first[n] for n in ACOUSTIC_STATE], subs
+
+
+def acoustic_substep(x, calls):
+    """One acoustic substep (solver.py:1349-1493) as autograd Functions:
+    advance_uv, the u/v zone update, advance_mu_t, the t/mu/muts zone update,
+    advance_w, sumflux, spec_bdyupdate_ph, zero_grad_bdy, calc_p_rho and the
+    six open-boundary fills.  x: the 17 carried fields (ACOUSTIC_STATE
+    order); calls: the nine recorded calls' arguments.  Returns the 17 fields
+    after the substep."""
+    st = dict(zip(ACOUSTIC_STATE, x))
+    uv, z1, mt, z2, wa, sf, pz, cp, bc = calls
+    u1, v1 = AdvanceUV.apply(st["u"], uv[1], st["v"], uv[3], st["p"], st["ph"], st["al"], st["mu"], st["mudf"],
+                             uv)
+    u2, v2 = SpecBdyUpdateGroup.apply(z1, u1, v1)
+    ww, t, t_ave, muave, muts, mudf, mu = AdvanceMuT.apply(
+        u2, v2, st["ww"], st["mu"], st["muave"], st["muts"], st["mudf"], st["t"], st["t_ave"], mt[27], mt[28], mt)
+    t2, mu2, muts2 = SpecBdyUpdateGroup.apply(z2, t, mu, muts)
+    t2a, w, ph = AdvanceW.apply(st["w"], st["ph"], t2, t_ave, u2, v2, muts2, muave, ww, wa)
+    ru_m, rv_m, ww_m = SumFlux.apply(u2, v2, ww, st["ru_m"], st["rv_m"], st["ww_m"], sf)
+    ph = SpecBdyUpdatePh.apply(ph, muts2, pz[0])
+    w = ZeroGradBdy.apply(w, pz[1])
+    al, p, pm1 = CalcPRho.apply(st["al"], st["p"], ph, t2, st["pm1"], mu2, muts2, cp)
+    items, ix24, ix2d = bc
+    ph, al, p, muts2, mu2, mudf = OpenBoundaryFill.apply([(v, two) for _, v, two in items], ix24, ix2d, ph, al, p,
+                                                         muts2, mu2, mudf)
+    new = {"u": u2, "v": v2, "p": p, "ph": ph, "al": al, "mu": mu2, "mudf": mudf, "ww": ww, "muave": muave,
+           "muts": muts2, "t": t2, "t_ave": t2a, "w": w, "pm1": pm1, "ru_m": ru_m, "rv_m": rv_m, "ww_m": ww_m}
+    return [new[n] for n in ACOUSTIC_STATE]
+
+
+class AcousticLoop(torch.autograd.Function):
+    """A stage's whole acoustic loop, x -> the 17 carried fields after its
+    last substep, with one checkpoint: the forward keeps only the loop's
+    input fields; the backward runs the substeps again from them (storing
+    each substep's operands for that pass only) and returns the gradient
+    through the hand-written adjoints.  substeps: from record_acoustic_loop.
+
+        x0, subs = adjoint.record_acoustic_loop(state, ops, cfg)
+        x = [f.clone().requires_grad_() for f in x0]
+        out = adjoint.AcousticLoop.apply(subs, *x)
+    """
+
+    @staticmethod
+    def forward(ctx, substeps, *x):
+        ctx.substeps = substeps
+        ctx.save_for_backward(*x)
+        with torch.no_grad():
+            y = list(x)
+            for calls in substeps:
+                y = acoustic_substep(y, calls)
+        return tuple(y)
+
+    @staticmethod
+    def backward(ctx, *lams):
+        x = [t.detach().requires_grad_() for t in ctx.saved_tensors]
+        with torch.enable_grad():
+            y = x
+            for calls in ctx.substeps:
+                y = acoustic_substep(y, calls)
+            pairs = [(o, l) for o, l in zip(y, lams) if l is not None]
+            grads = torch.autograd.grad([o for o, _ in pairs], x, [l for _, l in pairs], allow_unused=True)
+        return (None, *grads)
+
+
 # ----------------------------------------------------------------------- tracer transport
 class _TransportRecorder:
     """The step's functions, recording for each RK stage the arguments the
@@ -533,3 +677,74 @@ class SurrogateNet(torch.autograd.Function):
         dX = torch.empty(M, _sg.KIN, dtype=torch.float32, device=dY.device)
         _sg.gemm(0, ws.dH1, net.Wt[0], dX)                  # dH1 W1
         return dX[:, :_sg.FIN].clone(), None
+
+
+def main(argv=None):
+    """Time the acoustic loop's gradient on a seeded state: the loop forward,
+    the checkpointed forward + backward (AcousticLoop) and the same with every
+    operand kept, with the device memory each holds.
+
+        python -m pywrf_amd.adjoint --config config2 --stage 3 --repeat 5
+    """
+    import argparse
+    import json
+    import time
+    from . import ops
+    from .gen import synthetic_state
+    from .step import RunConfig
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--config", default="config2")
+    ap.add_argument("--stage", type=int, default=3)
+    ap.add_argument("--repeat", type=int, default=5)
+    a = ap.parse_args(argv)
+    s = synthetic_state(a.config, device="cuda")
+    x0, subs = record_acoustic_loop(s, ops(), RunConfig(), stage=a.stage)
+    del s
+    lams = [torch.randn_like(f) for f in x0]
+    fields = sum(f.numel() * f.element_size() for f in x0)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(a.repeat):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / a.repeat
+
+    def fwd():
+        with torch.no_grad():
+            y = list(x0)
+            for calls in subs:
+                y = acoustic_substep(y, calls)
+
+    held = {}
+
+    def grad(kept):
+        def run():
+            torch.cuda.synchronize()
+            base = torch.cuda.memory_allocated()
+            torch.cuda.reset_peak_memory_stats()
+            x = [f.clone().requires_grad_() for f in x0]
+            if kept:
+                y = x
+                for calls in subs:
+                    y = acoustic_substep(y, calls)
+            else:
+                y = AcousticLoop.apply(subs, *x)
+            torch.cuda.synchronize()
+            held["kept" if kept else "checkpoint"] = torch.cuda.memory_allocated() - base
+            torch.autograd.backward(y, lams)
+            torch.cuda.synchronize()
+            held[("kept" if kept else "checkpoint") + "_peak"] = torch.cuda.max_memory_allocated() - base
+        return run
+
+    r = {"config": a.config, "stage": a.stage, "substeps": len(subs), "fields_MB": fields / 1e6,
+         "forward_ms": 1e3 * timed(fwd), "checkpoint_grad_ms": 1e3 * timed(grad(False)),
+         "kept_grad_ms": 1e3 * timed(grad(True))}
+    r.update({k + "_MB": v / 1e6 for k, v in held.items()})
+    print(json.dumps(r))
+
+
+if __name__ == "__main__":
+    main()

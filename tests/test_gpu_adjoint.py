@@ -552,94 +552,33 @@ def _double(a):
     return a
 
 
-class _Stage:
-    """The step's functions, keeping clones of the arguments of every call of
-    the acoustic loop that starts at the n-th call of advance_uv, up to the
-    loop's end (the next calc_mu_uv_1)."""
-
-    def __init__(self, ops, n):
-        self._ops, self._n, self.seen, self.calls, self.done = ops, n, 0, [], False
-
-    def __getattr__(self, name):
-        f = getattr(self._ops, name)
-
-        def rec(*args):
-            if name == "advance_uv":
-                self.seen += 1
-            if name == "calc_mu_uv_1" and self.seen >= self._n:
-                self.done = True
-            if self.seen >= self._n and not self.done:
-                self.calls.append((name, _clone(list(args))))
-            return f(*args)
-        return rec
-
-
-SUBSTEP = ["advance_uv", "spec_bdyupdate_group", "advance_mu_t", "spec_bdyupdate_group", "advance_w", "sumflux",
-           "spec_bdyupdate_ph_zero_grad", "calc_p_rho", "set_physical_bc_group"]
-STATE = ("u", "v", "p", "ph", "al", "mu", "mudf", "ww", "muave", "muts", "t", "t_ave", "w", "pm1", "ru_m", "rv_m",
-         "ww_m")
-
-
-@pytest.mark.parametrize("config", ["config1c", "bdy24"])
+@pytest.mark.parametrize("config", ["config1c", "bdy24", "config4"])
 def test_acoustic_loop_adjoint_matches_fp64_differences(config):
     """The gradient of the second RK3 stage's whole acoustic loop (two
     substeps, each advance_uv, the u/v zone update, advance_mu_t, the
     t/mu/muts zone update, advance_w, sumflux, spec_bdyupdate_ph,
-    zero_grad_bdy, calc_p_rho and the six open-boundary fills) chained by
-    autograd from the hand-written adjoints, with respect to all 17 fields
-    the loop carries (u, v, p, ph, al, mu, mudf, ww, muave, muts, t, t_ave,
-    w, pm1 and the three flux accumulators), against FP64 central
-    differences of the same sequence through the oracle: for each field a
-    seeded direction d, <grad, d> against the difference of
-    sum_o <lam_o, out_o> over the 17 results, relative to the larger of the
-    difference and the 2-norm of grad_i d_i; bound 1e-4.  The two substeps
-    take sumflux's first and last branches and calc_p_rho's damped one."""
+    zero_grad_bdy, calc_p_rho and the six open-boundary fills), taken by
+    AcousticLoop (one checkpoint, the substeps run again in its backward)
+    from the hand-written adjoints, with respect to all 17 fields the loop
+    carries (u, v, p, ph, al, mu, mudf, ww, muave, muts, t, t_ave, w, pm1 and
+    the three flux accumulators), against FP64 central differences of the
+    same sequence through the oracle: for each field a seeded direction d,
+    <grad, d> against the difference of sum_o <lam_o, out_o> over the 17
+    results, relative to the larger of the difference and the 2-norm of
+    grad_i d_i; bound 1e-4.  The two substeps take sumflux's first and last
+    branches and calc_p_rho's damped one."""
     import oracle
     import pywrf_amd
     from pywrf_amd import adjoint
+    from pywrf_amd.adjoint import ACOUSTIC_STATE as STATE
     from pywrf_amd.gen import synthetic_state
-    from pywrf_amd.step import rk3_step, RunConfig
+    from pywrf_amd.step import RunConfig
     s = synthetic_state(config, device="cuda")
-    rec = _Stage(pywrf_amd.ops(), 2)
-    rk3_step(s, rec, RunConfig())
-    names = [n for n, _ in rec.calls]
-    assert names == SUBSTEP * 2, names
-    subs = [[a for _, a in rec.calls[9 * m:9 * m + 9]] for m in range(2)]
-    o = pywrf_amd.ops()
-
-    def zone(call, fields):
-        """the zone update on copies (its Jacobian is the identity: it adds fixed tendencies)"""
-        items, dt, ix24, ix2d = call
-        outs = [f.detach().clone() for f in fields]
-        o.spec_bdyupdate_group([(f, it[1], it[2], it[3]) for f, it in zip(outs, items)], dt, ix24, ix2d)
-        return [f + (a - f.detach()) for a, f in zip(outs, fields)]
-
-    uv, _, mt, _, wa, _, _, cp, _ = subs[0]
-    first = {"u": uv[0], "v": uv[2], "p": uv[4], "ph": uv[6], "al": uv[9], "mu": uv[10], "mudf": uv[15],
-             "ww": mt[0], "muave": mt[8], "muts": mt[9], "t": mt[24], "t_ave": mt[26], "w": wa[0], "pm1": cp[7],
-             "ru_m": mt[21], "rv_m": mt[22], "ww_m": mt[23]}
-    x = {n: first[n].clone().requires_grad_() for n in STATE}
-
-    st = dict(x)
-    for uv, z1, mt, z2, wa, sf, pz, cp, bc in subs:
-        u1, v1 = adjoint.AdvanceUV.apply(st["u"], uv[1], st["v"], uv[3], st["p"], st["ph"], st["al"], st["mu"],
-                                         st["mudf"], uv)
-        u2, v2 = zone(z1, [u1, v1])
-        ww_o, t_o, tave_o, muave_o, muts_o, mudf_o, mu_o = adjoint.AdvanceMuT.apply(
-            u2, v2, st["ww"], st["mu"], st["muave"], st["muts"], st["mudf"], st["t"], st["t_ave"], mt[27], mt[28],
-            mt)
-        t2, mu2, muts2 = zone(z2, [t_o, mu_o, muts_o])
-        t2a, w_o, ph_o = adjoint.AdvanceW.apply(st["w"], st["ph"], t2, tave_o, u2, v2, muts2, muave_o, ww_o, wa)
-        ru_m, rv_m, ww_m = adjoint.SumFlux.apply(u2, v2, ww_o, st["ru_m"], st["rv_m"], st["ww_m"], sf)
-        ph_s = adjoint.SpecBdyUpdatePh.apply(ph_o, muts2, pz[0])
-        w_z = adjoint.ZeroGradBdy.apply(w_o, pz[1])
-        al, p, pm1 = adjoint.CalcPRho.apply(st["al"], st["p"], ph_s, t2, st["pm1"], mu2, muts2, cp)
-        items, ix24, ix2d = bc
-        ph_f, al_f, p_f, muts_f, mu_f, mudf_f = adjoint.OpenBoundaryFill.apply(
-            [(v, two) for _, v, two in items], ix24, ix2d, ph_s, al, p, muts2, mu2, mudf_o)
-        st = {"u": u2, "v": v2, "p": p_f, "ph": ph_f, "al": al_f, "mu": mu_f, "mudf": mudf_f, "ww": ww_o,
-              "muave": muave_o, "muts": muts_f, "t": t2, "t_ave": t2a, "w": w_z, "pm1": pm1, "ru_m": ru_m,
-              "rv_m": rv_m, "ww_m": ww_m}
+    x0, subs = adjoint.record_acoustic_loop(s, pywrf_amd.ops(), RunConfig(), stage=2)
+    assert len(subs) == 2
+    x = {n: f.clone().requires_grad_() for n, f in zip(STATE, x0)}
+    out = adjoint.AcousticLoop.apply(subs, *[x[n] for n in STATE])
+    st = dict(zip(STATE, out))
     gen = torch.Generator().manual_seed(23)
     lams = {n: torch.randn(tuple(st[n].shape), generator=gen, dtype=torch.float64) for n in STATE}
     sum((st[n] * lams[n].float().cuda()).sum() for n in STATE).backward()
@@ -709,3 +648,55 @@ def test_acoustic_loop_adjoint_matches_fp64_differences(config):
     for n in STATE:
         assert x[n].grad is not None, n
     assert worst <= 1e-4
+
+
+def test_acoustic_loop_checkpoint_equals_the_stored_chain():
+    """AcousticLoop (one checkpoint: the forward keeps the loop's input, the
+    backward runs the substeps again) against the same substeps chained with
+    every operand kept (acoustic_substep under autograd), on the third RK3
+    stage's four substeps at 120 x 120 x 41: the gradients of all 17 inputs
+    agree within 1e-5 relative, and after the forward the checkpointed form
+    holds only its
+    inputs and results (at most 2.1 times the 17 fields' bytes) while the
+    kept chain holds more than twice that.  Both run the same kernels on the
+    same operands; the adjoints' atomic adds sum in no fixed order, which
+    over four substeps moves the column-summed gradients (mu, muts) by about
+    1e-6 of their norm, so the bound is 1e-5."""
+    import pywrf_amd
+    from pywrf_amd import adjoint
+    from pywrf_amd.gen import synthetic_state
+    from pywrf_amd.step import RunConfig
+    s = synthetic_state("config4", device="cuda")
+    x0, subs = adjoint.record_acoustic_loop(s, pywrf_amd.ops(), RunConfig(), stage=3)
+    assert len(subs) == 4
+    gen = torch.Generator().manual_seed(5)
+    lams = [torch.randn(tuple(f.shape), generator=gen).cuda() for f in x0]
+    fields = sum(f.numel() * f.element_size() for f in x0)
+
+    torch.cuda.synchronize()
+    base = torch.cuda.memory_allocated()
+    x = [f.clone().requires_grad_() for f in x0]
+    out = adjoint.AcousticLoop.apply(subs, *x)
+    torch.cuda.synchronize()
+    held_ck = torch.cuda.memory_allocated() - base
+    torch.autograd.backward(out, lams)
+    g_ck = [t.grad.clone() for t in x]
+    del out, x
+
+    torch.cuda.synchronize()
+    base = torch.cuda.memory_allocated()
+    x = [f.clone().requires_grad_() for f in x0]
+    y = x
+    for calls in subs:
+        y = adjoint.acoustic_substep(y, calls)
+    torch.cuda.synchronize()
+    held_kept = torch.cuda.memory_allocated() - base
+    torch.autograd.backward(y, lams)
+    print("held after the forward: checkpoint %.1f MB, kept chain %.1f MB, 17 fields %.1f MB"
+          % (held_ck / 1e6, held_kept / 1e6, fields / 1e6))
+    for n, a, t in zip(adjoint.ACOUSTIC_STATE, g_ck, x):
+        rel = float((a - t.grad).norm() / max(float(t.grad.norm()), 1e-30))
+        print("d/d%s: checkpoint vs kept chain %.2e" % (n, rel))
+        assert rel <= 1e-5, n
+    assert held_ck <= 2.1 * fields
+    assert held_kept > 2.0 * held_ck
