@@ -297,9 +297,10 @@ def drift(config, nsteps, cfg):
 
 def surrogate_leg(args):
     """The learned WSM6 replacement against the plain scheme: steps/s of the
-    whole step with each (HIP graphs), 60-step qv / theta RMS difference, the
-    fraction of columns that fell back, and the matrix-core roofline of the
-    network's three layers over all 227^2 columns."""
+    whole step with each (HIP graphs, replays interleaved), the microphysics
+    call's own time inside a step of each, 60-step qv / theta RMS difference,
+    the fraction of columns that fell back, and the matrix-core roofline of
+    the network's three layers over all 227^2 columns."""
     import torch
     from pywrf_amd import surrogate as sg
     if not os.path.exists(args.surrogate):
@@ -315,6 +316,16 @@ def surrogate_leg(args):
     from pywrf_amd import _lib
     from pywrf_amd.gen import synthetic_state
     from pywrf_amd.step import rk3_step, RunConfig
+    # the microphysics call alone inside an eager step of each scheme (HIP
+    # events, as in_step_times), after one warm-up step on the same state
+    mp = {}
+    for name, mops in (("wsm6", pywrf_amd.ops()), ("surrogate", sg.surrogate_ops(sur))):
+        s = synthetic_state("config2", device="cuda")
+        rk3_step(s, mops, RunConfig())
+        t = in_step_times(s, mops, RunConfig())
+        mp[name] = sum(x for _, x in t["microphysics_driver"]) * 1e6
+    rec["microphysics_us_in_step"] = {k: round(v, 1) for k, v in mp.items()}
+    rec["microphysics_speedup"] = mp["wsm6"] / mp["surrogate"]
     s = synthetic_state("config2", device="cuda")
     rk3_step(s, pywrf_amd.ops(), RunConfig())
     g = _lib.grid(s.th_phy.shape, s.grid.phys18())

@@ -83,7 +83,9 @@ __global__ void k_ssprep_3d(Box all, Dims D, SsPrep3 a) {
   // store, and every operand (all at this point) is loaded before the
   // branches, so that no branch waits on its loads
   const float u2 = a.u_2[q], v2 = a.v_2[q], t2 = a.t_2[q], w2 = a.w_2[q], p2 = a.ph_2[q];
-  const float u1l = a.u_1[q], v1l = a.v_1[q], t1l = a.t_1[q], w1l = a.w_1[q], p1l = a.ph_1[q];
+  // the *_1 fields are read only past the first stage (rk1 is uniform)
+  const float u1l = a.rk1 ? 0.0f : a.u_1[q], v1l = a.rk1 ? 0.0f : a.v_1[q], t1l = a.rk1 ? 0.0f : a.t_1[q];
+  const float w1l = a.rk1 ? 0.0f : a.w_1[q], p1l = a.rk1 ? 0.0f : a.ph_1[q];
   const float pbq = a.pb[q], pq = a.p[q], altq = a.alt[q], wwq = a.ww[q];
   const float muus = a.muus[q2], muu = a.muu[q2], msfuy = a.msfuy[q2], muvs = a.muvs[q2], muv = a.muv[q2];
   const float msfvx_inv = a.msfvx_inv[q2], muts = a.muts[q2], mut = a.mut[q2], msfty = a.msfty[q2];
@@ -181,14 +183,17 @@ __global__ void k_calc_p_rho(Box all, Dims D, Box b, PRho a) {
   int k, j, i;
   if (!box_point(all, k, j, i)) return;
   long q = D.ix(k, j, i);
-  // every operand is loaded before the branch (a load under a branch is waited
-  // for before the branch ends); the level above is clamped into the array,
-  // its value being used only inside the box, where k + 1 is a level
-  const float al_in = a.al_in[q], p_in = a.p_in[q], ph0 = a.ph[q], ph1 = AT(a.ph, imin(k + 1, D.nz - 1), j, i);
-  const float altq = a.alt[q], t1 = a.t_1[q], t2 = a.t_2[q], c2 = a.c2a[q], pm = a.pm1[q];
+  // the box's operands are loaded before the branch (a load under a branch is
+  // waited for before the branch ends); the level above is clamped into the
+  // array, its value being used only inside the box, where k + 1 is a level.
+  // The incoming al and p are read only outside the box (a wave inside it
+  // skips them as one), pm1 only past step 0 (uniform)
+  const bool in = b.has(k, j, i);
+  const float al_in = in ? 0.0f : a.al_in[q], p_in = in ? 0.0f : a.p_in[q];
+  const float ph0 = a.ph[q], ph1 = AT(a.ph, imin(k + 1, D.nz - 1), j, i);
+  const float altq = a.alt[q], t1 = a.t_1[q], t2 = a.t_2[q], c2 = a.c2a[q], pm = a.step != 0 ? a.pm1[q] : 0.0f;
   const float mut = AT2(a.mut, j, i), mu = AT2(a.mu, j, i), rd = a.rdnw[k];
   float al = al_in, p = p_in, pmo = 0.0f;
-  const bool in = b.has(k, j, i);
   if (in) {
     al = -1.0f / mut * (altq * mu + rd * (ph1 - ph0));
     p = c2 * (altq * (t2 - mu * t1) / (mut * (a.t0 + t1)) - al);
