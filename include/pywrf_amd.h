@@ -699,7 +699,9 @@ int pywrf_rk_update_scalar_adjoint(const pywrf_grid_t* g, const float* lam_new, 
  * whole array.  t2ave_out, w_out: the forward
  * call's outputs (muts enters by division); ht, msftx, cf, rdx, rdy: its
  * terrain, map factor, surface extrapolation, grid spacing and fnm / fnp
- * arguments.
+ * arguments.  Also the sensitivities of the forcing rw_tend and ph_tend and
+ * of w_save (the damping's reference), zero outside levels 1..kte-1 of the
+ * columns advanced.
  * work, work2: 3-D scratch arrays of the grid's size. */
 int pywrf_advance_w_adjoint(const pywrf_grid_t* g, const float* lam_t2ave_out, const float* lam_w_out,
                             const float* lam_ph_out, const float* mut, const float* muts, const float* t_1,
@@ -709,7 +711,8 @@ int pywrf_advance_w_adjoint(const pywrf_grid_t* g, const float* lam_t2ave_out, c
                             const float* t2ave_out, const float* w_out, const float* ht, const float* msftx,
                             const float* cf, double rdx, double rdy, const float* fnm, const float* fnp,
                             float* lam_w, float* lam_ph, float* lam_t_2, float* lam_t2ave, float* lam_u,
-                            float* lam_v, float* lam_muts, float* lam_muave, float* lam_ww, float* work,
+                            float* lam_v, float* lam_muts, float* lam_muave, float* lam_ww,
+                            float* lam_rw_tend, float* lam_ph_tend, float* lam_w_save, float* work,
                             float* work2, pywrf_stream_t s);
 /* The adjoint of advance_uv (wrf_dynamics.py:3391-3525) with respect to u,
  * v, ru_tend, rv_tend, p, ph, al, mu and mudf (alt, pb, php, cqu, cqv, muu,
@@ -764,13 +767,15 @@ int pywrf_set_physical_bc_group_adjoint(const pywrf_grid_t* g3, const pywrf_grid
                                         void* const* lam_fields, const char* variables, const int* is2d,
                                         pywrf_stream_t s);
 /* The adjoint of spec_bdyupdate_ph (wrf_boundary.py:287-358) with respect to
- * the field and muts (ph_save, field_tend and mu_tend held fixed): field_in
- * is the field before the update; lam_field holds the sensitivity of the
- * updated field on entry and that of field_in on return; lam_muts is written
- * on the strips' columns only (the caller clears it). */
+ * the field, muts, field_tend and mu_tend (ph_save held fixed): field_in is
+ * the field before the update; lam_field holds the sensitivity of the
+ * updated field on entry and that of field_in on return; lam_muts and
+ * lam_mu_tend are written on the strips' columns only, lam_field_tend on the
+ * strips' points only (the caller clears them). */
 int pywrf_spec_bdyupdate_ph_adjoint(const pywrf_grid_t* g, const float* ph_save, const float* field_in,
                                     const float* field_tend, const float* mu_tend, const float* muts, double dt,
-                                    char variable_in, float* lam_field, float* lam_muts, pywrf_stream_t s);
+                                    char variable_in, float* lam_field, float* lam_muts, float* lam_field_tend,
+                                    float* lam_mu_tend, pywrf_stream_t s);
 /* The adjoint of zero_grad_bdy (wrf_boundary.py:361-405): lam, the
  * sensitivity of the field after the copy, gives lam_in (another array), that
  * of the field before it. */

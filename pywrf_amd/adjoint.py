@@ -140,19 +140,25 @@ class TridiagSolve(torch.autograd.Function):
 # ----------------------------------------------------------------------- acoustic step: advance_w
 class AdvanceW(torch.autograd.Function):
     """(t_2ave, w, ph) = advance_w(...) (wrf_dynamics.py:3641-3796) with the
-    gradient with respect to w, ph, t_2, t_2ave, u, v, muts, muave and ww (every
-    other argument is a coefficient held fixed).  args: advance_w's
-    positional arguments in the reference's order."""
+    gradient with respect to w, ph, t_2, t_2ave, u, v, muts, muave, ww, the
+    forcing rw_tend and ph_tend and w_save (every other argument is a
+    coefficient held fixed).  args: advance_w's positional arguments in the
+    reference's order; rw_tend, ph_tend, w_save None: those of args."""
 
     # positions in advance_w's signature
-    W, WW, U, V, MUAVE, MUTS, T2AVE, T2, PH = 0, 2, 4, 5, 8, 9, 18, 19, 21
-    DIFF = (W, PH, T2, T2AVE, U, V, MUTS, MUAVE, WW)
+    W, RWT, WW, WSAVE, U, V, MUAVE, MUTS, T2AVE, T2, PH, PHT = 0, 1, 2, 3, 4, 5, 8, 9, 18, 19, 21, 24
+    DIFF = (W, PH, T2, T2AVE, U, V, MUTS, MUAVE, WW)        # the inputs before args
+    EXTRA = (RWT, PHT, WSAVE)                               # the optional ones after it
 
     @staticmethod
-    def forward(ctx, w, ph, t_2, t_2ave, u, v, muts, muave, ww, args):
+    def forward(ctx, w, ph, t_2, t_2ave, u, v, muts, muave, ww, args, rw_tend=None, ph_tend=None, w_save=None):
         from . import wrf_dynamics
         args = list(args)
-        for pos, x in zip(AdvanceW.DIFF, (w, ph, t_2, t_2ave, u, v, muts, muave, ww)):
+        rw_tend = args[AdvanceW.RWT] if rw_tend is None else rw_tend
+        ph_tend = args[AdvanceW.PHT] if ph_tend is None else ph_tend
+        w_save = args[AdvanceW.WSAVE] if w_save is None else w_save
+        for pos, x in zip(AdvanceW.DIFF + AdvanceW.EXTRA,
+                          (w, ph, t_2, t_2ave, u, v, muts, muave, ww, rw_tend, ph_tend, w_save)):
             args[pos] = _f32(x)
         ctx.args = args
         t2a, wo, pho = wrf_dynamics.advance_w(*args)
@@ -171,6 +177,7 @@ class AdvanceW(torch.autograd.Function):
         uv = [torch.empty_like(a[4]), torch.empty_like(a[5])]
         planes = [torch.empty_like(a[9]), torch.empty_like(a[8])]
         lww = torch.empty_like(a[2])
+        forcing = [torch.empty_like(a[0]) for _ in range(3)]       # rw_tend, ph_tend, w_save
         work = [torch.empty_like(a[0]) for _ in range(2)]
         t2a_out, w_out = ctx.outs
         # advance_w(w, rw_tend, ww, w_save, u, v, mu1, mut, muave, muts, c1h, c2h, c1f, c2f, c3h, c4h, c3f, c4f,
@@ -181,11 +188,13 @@ class AdvanceW(torch.autograd.Function):
              float(a[35]), float(a[36]), float(a[37]), ptr(a[41]), ptr(a[42]), ptr(a[47]), ptr(t2a_out),
              ptr(w_out), ptr(a[25]), ptr(a[46]), cf, float(a[33]), float(a[34]), ptr(a[39]), ptr(a[40]),
              *[ptr(t) for t in out], *[ptr(t) for t in uv], *[ptr(t) for t in planes], ptr(lww),
-             *[ptr(t) for t in work], stream())
+             *[ptr(t) for t in forcing], *[ptr(t) for t in work], stream())
         lw, lph, lt2, lt2a = out
         lu, lv = uv
         lmuts, lmuave = planes
-        return lw, lph, lt2, lt2a, lu, lv, lmuts, lmuave, lww, None
+        res = (lw, lph, lt2, lt2a, lu, lv, lmuts, lmuave, lww, None, *forcing)
+        need = ctx.needs_input_grad          # the optional forcing inputs: only those given as tensors
+        return tuple(r if (n < 10 or need[n]) else None for n, r in enumerate(res[:len(need)]))
 
 
 class AdvanceUV(torch.autograd.Function):
@@ -333,16 +342,20 @@ class SumFlux(torch.autograd.Function):
 
 class SpecBdyUpdatePh(torch.autograd.Function):
     """spec_bdyupdate_ph (wrf_boundary.py:287-358) with the gradient with
-    respect to the field and muts (ph_save, field_tend and mu_tend held
-    fixed); the update goes to a copy of the field.  args: its positional
-    arguments in the reference's order."""
+    respect to the field, muts, field_tend and mu_tend (ph_save held fixed);
+    the update goes to a copy of the field.  args: its positional arguments
+    in the reference's order; field_tend, mu_tend None: those of args."""
 
     @staticmethod
-    def forward(ctx, field, muts, args):
+    def forward(ctx, field, muts, args, field_tend=None, mu_tend=None):
         from . import wrf_boundary
         args = list(args)
         ctx.f_in = _f32(field)
         args[1], args[4] = ctx.f_in.clone(), _f32(muts)
+        if field_tend is not None:
+            args[2] = _f32(field_tend)
+        if mu_tend is not None:
+            args[3] = _f32(mu_tend)
         ctx.args = args
         return wrf_boundary.spec_bdyupdate_ph(*args)
 
@@ -351,10 +364,11 @@ class SpecBdyUpdatePh(torch.autograd.Function):
         a = ctx.args
         g = grid(a[1].shape, tuple(a[-24:]))
         lam = torch.zeros_like(a[1]) if lf is None else _f32(lf).clone()
-        lmuts = torch.zeros_like(a[4])
+        lmuts, lft, lmt = torch.zeros_like(a[4]), torch.zeros_like(a[2]), torch.zeros_like(a[3])
         call("pywrf_spec_bdyupdate_ph_adjoint", g, ptr(a[0]), ptr(ctx.f_in), ptr(a[2]), ptr(a[3]), ptr(a[4]),
-             float(a[7]), flag(a[8]), ptr(lam), ptr(lmuts), stream())
-        return lam, lmuts, None
+             float(a[7]), flag(a[8]), ptr(lam), ptr(lmuts), ptr(lft), ptr(lmt), stream())
+        need = ctx.needs_input_grad          # the optional tendencies: only those given as tensors
+        return tuple(r if (n < 3 or need[n]) else None for n, r in enumerate((lam, lmuts, None, lft, lmt)[:len(need)]))
 
 
 class ZeroGradBdy(torch.autograd.Function):
@@ -414,22 +428,40 @@ class OpenBoundaryFill(torch.autograd.Function):
 
 class SpecBdyUpdateGroup(torch.autograd.Function):
     """A run of spec_bdyupdate calls (wrf_boundary.py:228-284) on copies of
-    distinct fields: each adds its fixed tendency times dt on the outer ring,
-    so the Jacobian with respect to each field is the identity.  call: the
-    grouped call's arguments (items, dt, idx24, idx2d) as the step makes it,
-    the items' fields being replaced by `fields`."""
+    distinct fields: each adds its tendency times dt on the outer ring, so the
+    Jacobian with respect to each field is the identity and that with respect
+    to each tendency dt on the ring (the same update applied to zeros).  call:
+    the grouped call's arguments (items, dt, idx24, idx2d) as the step makes
+    it; args: the n fields that replace the items' fields, then n tendencies
+    (None: the item's own, held fixed).
+
+        u2, v2 = SpecBdyUpdateGroup.apply(call, u, v, None, None)
+    """
 
     @staticmethod
-    def forward(ctx, call, *fields):
+    def forward(ctx, call, *args):
         from . import wrf_boundary
         items, dt, ix24, ix2d = call
+        n = len(items)
+        fields, tends = args[:n], args[n:2 * n]
+        tends = [it[1] if t is None else _f32(t) for t, it in zip(tends, items)]
         outs = [_f32(f).clone() for f in fields]
-        wrf_boundary.spec_bdyupdate_group([(o, it[1], it[2], it[3]) for o, it in zip(outs, items)], dt, ix24, ix2d)
+        wrf_boundary.spec_bdyupdate_group([(o, t, it[2], it[3]) for o, t, it in zip(outs, tends, items)], dt, ix24,
+                                          ix2d)
+        ctx.call, ctx.live = call, [t is not None for t in args[n:2 * n]]
+        ctx.tshapes = [t for t in tends]
         return tuple(outs)
 
     @staticmethod
     def backward(ctx, *lams):
-        return (None, *lams)
+        from . import wrf_boundary
+        items, dt, ix24, ix2d = ctx.call
+        lt = [torch.zeros_like(t) if (on and l is not None) else None
+              for on, l, t in zip(ctx.live, lams, ctx.tshapes)]
+        run = [(z, _f32(l), it[2], it[3]) for z, l, it in zip(lt, lams, items) if z is not None]
+        if run:
+            wrf_boundary.spec_bdyupdate_group(run, dt, ix24, ix2d)
+        return (None, *lams, *lt)
 
 
 # ----------------------------------------------------------------------- the acoustic loop
@@ -494,24 +526,41 @@ def record_acoustic_loop(s, ops, cfg, stage=2):
     return [first[n] for n in ACOUSTIC_STATE], subs
 
 
-def acoustic_substep(x, calls):
+FORCING = ("ru_tend", "rv_tend", "rw_tend", "t_tend", "ph_tend", "mu_tend")
+
+
+def loop_forcing(substeps):
+    """The six large-step tendencies (FORCING order) a recorded loop reads."""
+    uv, _, mt, _, wa = substeps[0][:5]
+    return [uv[1], uv[3], wa[1], mt[27], wa[24], mt[28]]
+
+
+def acoustic_substep(x, calls, forcing=None):
     """One acoustic substep (solver.py:1349-1493) as autograd Functions:
     advance_uv, the u/v zone update, advance_mu_t, the t/mu/muts zone update,
     advance_w, sumflux, spec_bdyupdate_ph, zero_grad_bdy, calc_p_rho and the
     six open-boundary fills.  x: the 17 carried fields (ACOUSTIC_STATE
-    order); calls: the nine recorded calls' arguments.  Returns the 17 fields
-    after the substep."""
+    order); calls: the nine recorded calls' arguments; forcing: None, or the
+    six large-step tendencies (FORCING order) to use, and differentiate, in
+    place of the recorded ones wherever the substep reads them.  Returns the
+    17 fields after the substep."""
     st = dict(zip(ACOUSTIC_STATE, x))
     uv, z1, mt, z2, wa, sf, pz, cp, bc = calls
-    u1, v1 = AdvanceUV.apply(st["u"], uv[1], st["v"], uv[3], st["p"], st["ph"], st["al"], st["mu"], st["mudf"],
-                             uv)
-    u2, v2 = SpecBdyUpdateGroup.apply(z1, u1, v1)
+    fc = dict(zip(FORCING, forcing)) if forcing is not None else {
+        "ru_tend": uv[1], "rv_tend": uv[3], "rw_tend": None, "t_tend": mt[27], "ph_tend": None, "mu_tend": mt[28]}
+    given = forcing is not None
+    u1, v1 = AdvanceUV.apply(st["u"], fc["ru_tend"], st["v"], fc["rv_tend"], st["p"], st["ph"], st["al"], st["mu"],
+                             st["mudf"], uv)
+    u2, v2 = SpecBdyUpdateGroup.apply(z1, u1, v1, *((fc["ru_tend"], fc["rv_tend"]) if given else (None, None)))
     ww, t, t_ave, muave, muts, mudf, mu = AdvanceMuT.apply(
-        u2, v2, st["ww"], st["mu"], st["muave"], st["muts"], st["mudf"], st["t"], st["t_ave"], mt[27], mt[28], mt)
-    t2, mu2, muts2 = SpecBdyUpdateGroup.apply(z2, t, mu, muts)
-    t2a, w, ph = AdvanceW.apply(st["w"], st["ph"], t2, t_ave, u2, v2, muts2, muave, ww, wa)
+        u2, v2, st["ww"], st["mu"], st["muave"], st["muts"], st["mudf"], st["t"], st["t_ave"], fc["t_tend"],
+        fc["mu_tend"], mt)
+    t2, mu2, muts2 = SpecBdyUpdateGroup.apply(z2, t, mu, muts, *((fc["t_tend"], fc["mu_tend"], fc["mu_tend"])
+                                                                 if given else (None, None, None)))
+    t2a, w, ph = AdvanceW.apply(st["w"], st["ph"], t2, t_ave, u2, v2, muts2, muave, ww, wa, fc["rw_tend"],
+                                fc["ph_tend"], None)
     ru_m, rv_m, ww_m = SumFlux.apply(u2, v2, ww, st["ru_m"], st["rv_m"], st["ww_m"], sf)
-    ph = SpecBdyUpdatePh.apply(ph, muts2, pz[0])
+    ph = SpecBdyUpdatePh.apply(ph, muts2, pz[0], fc["ph_tend"], fc["mu_tend"] if given else None)
     w = ZeroGradBdy.apply(w, pz[1])
     al, p, pm1 = CalcPRho.apply(st["al"], st["p"], ph, t2, st["pm1"], mu2, muts2, cp)
     items, ix24, ix2d = bc
@@ -525,9 +574,12 @@ def acoustic_substep(x, calls):
 class AcousticLoop(torch.autograd.Function):
     """A stage's whole acoustic loop, x -> the 17 carried fields after its
     last substep, with one checkpoint: the forward keeps only the loop's
-    input fields; the backward runs the substeps again from them (storing
-    each substep's operands for that pass only) and returns the gradient
-    through the hand-written adjoints.  substeps: from record_acoustic_loop.
+    inputs; the backward runs the substeps again from them (storing each
+    substep's operands for that pass only) and returns the gradient through
+    the hand-written adjoints.  substeps: from record_acoustic_loop; args:
+    the 17 fields (ACOUSTIC_STATE order), then optionally the six large-step
+    tendencies (FORCING order), which then replace the recorded ones in every
+    substep and get gradients too.
 
         x0, subs = adjoint.record_acoustic_loop(state, ops, cfg)
         x = [f.clone().requires_grad_() for f in x0]
@@ -535,22 +587,27 @@ class AcousticLoop(torch.autograd.Function):
     """
 
     @staticmethod
-    def forward(ctx, substeps, *x):
+    def _run(substeps, inputs):
+        x, forcing = list(inputs[:len(ACOUSTIC_STATE)]), inputs[len(ACOUSTIC_STATE):] or None
+        for calls in substeps:
+            x = acoustic_substep(x, calls, forcing)
+        return x
+
+    @staticmethod
+    def forward(ctx, substeps, *args):
+        if len(args) not in (len(ACOUSTIC_STATE), len(ACOUSTIC_STATE) + len(FORCING)):
+            raise ValueError("AcousticLoop: expected the 17 carried fields, optionally then the 6 tendencies")
         ctx.substeps = substeps
-        ctx.save_for_backward(*x)
+        ctx.save_for_backward(*args)
         with torch.no_grad():
-            y = list(x)
-            for calls in substeps:
-                y = acoustic_substep(y, calls)
+            y = AcousticLoop._run(substeps, args)
         return tuple(y)
 
     @staticmethod
     def backward(ctx, *lams):
         x = [t.detach().requires_grad_() for t in ctx.saved_tensors]
         with torch.enable_grad():
-            y = x
-            for calls in ctx.substeps:
-                y = acoustic_substep(y, calls)
+            y = AcousticLoop._run(ctx.substeps, x)
             pairs = [(o, l) for o, l in zip(y, lams) if l is not None]
             grads = torch.autograd.grad([o for o, _ in pairs], x, [l for _, l in pairs], allow_unused=True)
         return (None, *grads)

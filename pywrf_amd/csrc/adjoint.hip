@@ -335,6 +335,7 @@ struct AdjW {
   const float *fnm, *fnp;
   float *lw_in, *lph_in, *lt2, *lt2a_in, *work, *work2;
   float *lu, *lv, *lmuts, *lmuave, *lww;           // zeroed by the caller; u, v by atomic adds
+  float *lrw, *lpht, *lws;                         // rw_tend, ph_tend, w_save (zeroed by the caller)
   float dts, t0, ep1, em1, dampmag, hpi, hdepth, rdx, rdy;
   int K, j0, j1, i0, i1;
 };
@@ -366,6 +367,7 @@ __global__ void k_advance_w_adj(Box all, Dims D, AdjW a) {
     }
     const float lwd = AT(a.lW, k, j, i) + my * 0.5f * a.dts * G * a.ep1 / muts * AT(a.lP, k, j, i);
     AT(a.work, k, j, i) = lwd / (1.0f + dw);
+    AT(a.lws, k, j, i) = -dw * mut / (1.0f + dw) * lwd;   // the damping's pull towards w_save
   }
   // the column solve, transposed: back substitution upwards, elimination downwards
   for (int k = 1; k <= K - 1; ++k) AT(a.work, k + 1, j, i) -= AT(a.gamma, k, j, i) * AT(a.work, k, j, i);
@@ -389,6 +391,8 @@ __global__ void k_advance_w_adj(Box all, Dims D, AdjW a) {
       atomic_add(&AT(a.lu, c, j, i), a.cf[c] * uw);
     }
   }
+  // rw_tend enters the predictor as dts rw_tend
+  for (int k = 1; k <= K; ++k) AT(a.lrw, k, j, i) = a.dts * AT(a.work, k, j, i);
   // muave in the predictor (- dts g muave / msfty at every level 1..K) and muts in ph_out
   float lmuave = 0.0f, lmuts = 0.0f;
   {
@@ -439,6 +443,7 @@ __global__ void k_advance_w_adj(Box all, Dims D, AdjW a) {
     const float lr = AT(a.work2, k, j, i);
     AT(a.lph_in, k, j, i) += lr;
     AT(a.lw_in, k, j, i) += gam * lr;
+    AT(a.lpht, k, j, i) = my * a.dts / mut * lr;       // rhs holds msfty dts ph_tend / mut
     if (k < K) {
       const float li = -a.dts * my / mut * lr;
       const float d1 = AT(a.ph_1, k + 1, j, i) - AT(a.ph_1, k, j, i) + AT(a.phb, k + 1, j, i) - AT(a.phb, k, j, i);
@@ -476,7 +481,8 @@ extern "C" int pywrf_advance_w_adjoint(const pywrf_grid_t* g, const float* lam_t
                                        double rdx, double rdy, const float* fnm, const float* fnp, float* lam_w,
                                        float* lam_ph, float* lam_t_2, float* lam_t2ave, float* lam_u,
                                        float* lam_v, float* lam_muts, float* lam_muave, float* lam_ww,
-                                       float* work, float* work2, pywrf_stream_t s) {
+                                       float* lam_rw_tend, float* lam_ph_tend, float* lam_w_save, float* work,
+                                       float* work2, pywrf_stream_t s) {
   if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 1);
   if (g->kts != 0 || g->kte > g->nz || g->kte < 3) return PYWRF_E_RANGE;
@@ -487,6 +493,7 @@ extern "C" int pywrf_advance_w_adjoint(const pywrf_grid_t* g, const float* lam_t
   p.lw_in = lam_w; p.lph_in = lam_ph; p.lt2 = lam_t_2; p.lt2a_in = lam_t2ave; p.work = work; p.work2 = work2;
   p.t2a_out = t2ave_out; p.w_out = w_out; p.ht = ht; p.msftx = msftx; p.cf = cf;
   p.lu = lam_u; p.lv = lam_v; p.lmuts = lam_muts; p.lmuave = lam_muave; p.lww = lam_ww;
+  p.lrw = lam_rw_tend; p.lpht = lam_ph_tend; p.lws = lam_w_save;
   p.fnm = fnm; p.fnp = fnp;
   p.rdx = F(rdx); p.rdy = F(rdy);
   p.dts = F(dts); p.t0 = F(t0); p.ep1 = F(1. + epssm); p.em1 = F(1. - epssm);
@@ -499,6 +506,9 @@ extern "C" int pywrf_advance_w_adjoint(const pywrf_grid_t* g, const float* lam_t
   hipLaunchKernelGGL(k_zero_n, dim3(1024), dim3(256), 0, (hipStream_t)s, lam_u, n3);
   hipLaunchKernelGGL(k_zero_n, dim3(1024), dim3(256), 0, (hipStream_t)s, lam_v, n3);
   hipLaunchKernelGGL(k_zero_n, dim3(1024), dim3(256), 0, (hipStream_t)s, lam_ww, n3);
+  hipLaunchKernelGGL(k_zero_n, dim3(1024), dim3(256), 0, (hipStream_t)s, lam_rw_tend, n3);
+  hipLaunchKernelGGL(k_zero_n, dim3(1024), dim3(256), 0, (hipStream_t)s, lam_ph_tend, n3);
+  hipLaunchKernelGGL(k_zero_n, dim3(1024), dim3(256), 0, (hipStream_t)s, lam_w_save, n3);
   hipLaunchKernelGGL(k_zero_n, dim3(256), dim3(256), 0, (hipStream_t)s, lam_muts, n2);
   hipLaunchKernelGGL(k_zero_n, dim3(256), dim3(256), 0, (hipStream_t)s, lam_muave, n2);
   Box all = whole(g);

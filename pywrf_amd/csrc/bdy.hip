@@ -498,44 +498,51 @@ extern "C" int pywrf_spec_bdyupdate_ph_zero_grad(const pywrf_grid_t* g, const fl
 // point).  Its transpose with ph_save, ft and mu_tend fixed: f's sensitivity
 // is scaled by r per application, and muts's gathers, down the column, the
 // derivative of f' with respect to muts, carried through the applications in
-// forward mode.  One thread per column of the band; lam holds f''s
-// sensitivity on entry and f's on return; lam_muts is written on the band's
-// columns only.
+// forward mode, and so do those of mu_tend (through r) and of ft (dt / muts
+// per application).  One thread per column of the band; lam holds f''s
+// sensitivity on entry and f's on return; lam_muts and lam_mu_tend are
+// written on the band's columns only, lam_ft on the strips' points only.
 __global__ void k_spec_bdyupdate_ph_adj(Box cols, Dims D, PhStrip a, const float* f_in, float* lam,
-                                        float* lam_muts) {
+                                        float* lam_muts, float* lam_ft, float* lam_mu_tend) {
   int j, i;
   if (!box_column(cols, j, i)) return;
   bool ir = i >= a.ibs && i < a.ibe, jc = j >= a.jbs + 1 && j < a.jbe - 1;
   int n = (j == a.jts && ir) + (j == a.jbe - 1 && ir) + (i == a.ibs && jc) + (i == a.ibe - 1 && jc);
   if (!n) return;
   const float mt = AT2(a.muts, j, i), mtend = AT2(a.mu_tend, j, i);
-  const float mo = mt - a.dt * mtend, r = mo / mt, dr = a.dt * mtend / (mt * mt);
-  float gm = 0.0f;
+  const float mo = mt - a.dt * mtend, r = mo / mt, dr = a.dt * mtend / (mt * mt), drt = -a.dt / mt;
+  float gm = 0.0f, gt = 0.0f;
   for (int k = a.all.k0; k < a.all.k1; ++k) {
     const long q = D.ix(k, j, i);
     const float ft = a.ft[q], ps = a.ph_save[q];
     const float dc = -a.dt * ft / (mt * mt) + ps * dr;
-    float v = f_in[q], dv = 0.0f, s = 1.0f;
+    float v = f_in[q], dv = 0.0f, dft = 0.0f, dtn = 0.0f, s = 1.0f;
     for (int m = 0; m < n; ++m) {
       dv = dv * r + v * dr + dc;
+      dft = dft * r + a.dt / mt;
+      dtn = dtn * r + (v + ps) * drt;
       v = v * mo / mt + a.dt * ft / mt + ps * (mo / mt - 1.0f);
       s *= r;
     }
     const float l = lam[q];
     gm += l * dv;
+    gt += l * dtn;
+    lam_ft[q] = l * dft;
     lam[q] = l * s;
   }
   AT2(lam_muts, j, i) = gm;
+  AT2(lam_mu_tend, j, i) = gt;
 }
 
 extern "C" int pywrf_spec_bdyupdate_ph_adjoint(const pywrf_grid_t* g, const float* ph_save, const float* field_in,
                                                const float* field_tend, const float* mu_tend, const float* muts,
                                                double dt, char variable_in, float* lam_field, float* lam_muts,
-                                               pywrf_stream_t s) {
+                                               float* lam_field_tend, float* lam_mu_tend, pywrf_stream_t s) {
   PhStrip a;
   if (int e = ph_strip_setup(g, ph_save, nullptr, field_tend, mu_tend, muts, dt, variable_in, a)) return e;
   Box cols = mkbox(0, 1, a.all.j0, a.all.j1, a.all.i0, a.all.i1);
-  PW_LAUNCH_COLS(k_spec_bdyupdate_ph_adj, cols, s, dims_of(g), a, field_in, lam_field, lam_muts);
+  PW_LAUNCH_COLS(k_spec_bdyupdate_ph_adj, cols, s, dims_of(g), a, field_in, lam_field, lam_muts, lam_field_tend,
+                 lam_mu_tend);
   PW_RET();
 }
 

@@ -265,11 +265,11 @@ class _FirstCall:
 @pytest.mark.parametrize("config", ["config1c", "bdy24"])
 def test_advance_w_adjoint_matches_oracle_autograd(config):
     """Gradients of <lT, t_2ave> + <lW, w> + <lP, ph> after advance_w with
-    respect to w, ph, t_2, t_2ave, u, v, muts, muave and ww, at the arguments of
-    the step's first acoustic substep, against torch autograd through the
-    oracle's advance_w in FP64 (the same coefficients).  Bound: 1e-4
-    normwise per gradient (the coefficients are FP32 values in both; the
-    FP32 adjoint's rounding)."""
+    respect to w, ph, t_2, t_2ave, u, v, muts, muave, ww and the forcing
+    rw_tend, ph_tend and w_save, at the arguments of the step's first
+    acoustic substep, against torch autograd through the oracle's advance_w
+    in FP64 (the same coefficients).  Bound: 1e-4 normwise per gradient (the
+    coefficients are FP32 values in both; the FP32 adjoint's rounding)."""
     import oracle
     import pywrf_amd
     from pywrf_amd import adjoint
@@ -281,9 +281,9 @@ def test_advance_w_adjoint_matches_oracle_autograd(config):
     args = rec.args
     gen = torch.Generator(device="cuda").manual_seed(7)
     lams = [torch.randn(s.w.shape, device="cuda", generator=gen) for _ in range(3)]
-    D = adjoint.AdvanceW.DIFF
+    D = adjoint.AdvanceW.DIFF + adjoint.AdvanceW.EXTRA
     x = [args[n].clone().requires_grad_() for n in D]
-    outs = adjoint.AdvanceW.apply(*x, args)
+    outs = adjoint.AdvanceW.apply(*x[:9], args, *x[9:])
     sum(float(0) + (o * l).sum() for o, l in zip(outs, lams)).backward()
     a64 = [a.detach().double().cpu() if isinstance(a, torch.Tensor) else a for a in args]
     x64 = [a64[n].clone().requires_grad_() for n in D]
@@ -291,7 +291,8 @@ def test_advance_w_adjoint_matches_oracle_autograd(config):
         a64[n] = t
     o64 = oracle.ops().advance_w(*a64)
     sum((o * l.double().cpu()).sum() for o, l in zip(o64, lams)).backward()
-    for name, mine, ref in zip(("w", "ph", "t_2", "t_2ave", "u", "v", "muts", "muave", "ww"), x, x64):
+    for name, mine, ref in zip(("w", "ph", "t_2", "t_2ave", "u", "v", "muts", "muave", "ww", "rw_tend", "ph_tend",
+                                "w_save"), x, x64):
         assert float(ref.grad.abs().max()) > 0, name
         err = _rel(mine.grad.double().cpu(), ref.grad)
         print("d/d%s: %.2e" % (name, err))
@@ -561,23 +562,26 @@ def test_acoustic_loop_adjoint_matches_fp64_differences(config):
     AcousticLoop (one checkpoint, the substeps run again in its backward)
     from the hand-written adjoints, with respect to all 17 fields the loop
     carries (u, v, p, ph, al, mu, mudf, ww, muave, muts, t, t_ave, w, pm1 and
-    the three flux accumulators), against FP64 central differences of the
-    same sequence through the oracle: for each field a seeded direction d,
-    <grad, d> against the difference of sum_o <lam_o, out_o> over the 17
-    results, relative to the larger of the difference and the 2-norm of
-    grad_i d_i; bound 1e-4.  The two substeps take sumflux's first and last
-    branches and calc_p_rho's damped one."""
+    the three flux accumulators) and the six large-step tendencies it reads
+    (ru_tend, rv_tend, rw_tend, t_tend, ph_tend, mu_tend, wherever a call
+    reads them), against FP64 central differences of the same sequence
+    through the oracle: for each input a seeded direction d, <grad, d>
+    against the difference of sum_o <lam_o, out_o> over the 17 results,
+    relative to the larger of the difference and the 2-norm of grad_i d_i;
+    bound 1e-4.  The two substeps take sumflux's first and last branches and
+    calc_p_rho's damped one."""
     import oracle
     import pywrf_amd
     from pywrf_amd import adjoint
-    from pywrf_amd.adjoint import ACOUSTIC_STATE as STATE
+    from pywrf_amd.adjoint import ACOUSTIC_STATE as STATE, FORCING
     from pywrf_amd.gen import synthetic_state
     from pywrf_amd.step import RunConfig
     s = synthetic_state(config, device="cuda")
     x0, subs = adjoint.record_acoustic_loop(s, pywrf_amd.ops(), RunConfig(), stage=2)
     assert len(subs) == 2
-    x = {n: f.clone().requires_grad_() for n, f in zip(STATE, x0)}
-    out = adjoint.AcousticLoop.apply(subs, *[x[n] for n in STATE])
+    names = STATE + FORCING
+    x = {n: f.clone().requires_grad_() for n, f in zip(names, list(x0) + adjoint.loop_forcing(subs))}
+    out = adjoint.AcousticLoop.apply(subs, *[x[n] for n in names])
     st = dict(zip(STATE, out))
     gen = torch.Generator().manual_seed(23)
     lams = {n: torch.randn(tuple(st[n].shape), generator=gen, dtype=torch.float64) for n in STATE}
@@ -587,34 +591,37 @@ def test_acoustic_loop_adjoint_matches_fp64_differences(config):
     ro = oracle.ops()
     subs64 = [_double(c) for c in subs]
 
-    def ozone(call, fields):
+    def ozone(call, fields, tends):
         items, dt, ix24, ix2d = call
-        return [ro.spec_bdyupdate(f.clone(), it[1], dt, it[2], 1, *(ix2d if it[3] else ix24))
-                for f, it in zip(fields, items)]
+        return [ro.spec_bdyupdate(f.clone(), td, dt, it[2], 1, *(ix2d if it[3] else ix24))
+                for f, td, it in zip(fields, tends, items)]
 
     def loop(v):
         st = {n: v[n].clone() for n in STATE}
+        fc = {n: v[n] for n in FORCING}
         for uv, z1, mt, z2, wa, sf, pz, cp, bc in subs64:
             a = list(uv)
-            a[0], a[2], a[4], a[6], a[9], a[10], a[15] = (st["u"], st["v"], st["p"], st["ph"], st["al"], st["mu"],
-                                                          st["mudf"])
-            u2, v2 = ozone(z1, ro.advance_uv(*[_clone(q) for q in a]))
+            a[0], a[1], a[2], a[3], a[4], a[6], a[9], a[10], a[15] = (
+                st["u"], fc["ru_tend"], st["v"], fc["rv_tend"], st["p"], st["ph"], st["al"], st["mu"], st["mudf"])
+            u2, v2 = ozone(z1, ro.advance_uv(*[_clone(q) for q in a]), (fc["ru_tend"], fc["rv_tend"]))
             m = _clone(list(mt))
-            m[0], m[2], m[4], m[6], m[8], m[9], m[12], m[24], m[26] = (
-                st["ww"], u2, v2, st["mu"], st["muave"], st["muts"], st["mudf"], st["t"], st["t_ave"])
+            m[0], m[2], m[4], m[6], m[8], m[9], m[12], m[24], m[26], m[27], m[28] = (
+                st["ww"], u2, v2, st["mu"], st["muave"], st["muts"], st["mudf"], st["t"], st["t_ave"],
+                fc["t_tend"], fc["mu_tend"])
             m = [_clone(q) for q in m]
             r = ro.advance_mu_t(*m)
-            t2, mu2, muts2 = ozone(z2, [r[2], r[10], r[8]])
+            t2, mu2, muts2 = ozone(z2, [r[2], r[10], r[8]], (fc["t_tend"], fc["mu_tend"], fc["mu_tend"]))
             w_ = _clone(list(wa))
-            w_[0], w_[2], w_[4], w_[5], w_[8], w_[9], w_[18], w_[19], w_[21] = (
-                st["w"].clone(), r[0], u2, v2, r[7], muts2, r[3].clone(), t2, st["ph"].clone())
+            w_[0], w_[1], w_[2], w_[4], w_[5], w_[8], w_[9], w_[18], w_[19], w_[21], w_[24] = (
+                st["w"].clone(), fc["rw_tend"], r[0], u2, v2, r[7], muts2, r[3].clone(), t2, st["ph"].clone(),
+                fc["ph_tend"])
             t2a, wo, pho = ro.advance_w(*w_)
             f = _clone(list(sf))
             f[0], f[1], f[2], f[16], f[17], f[18] = u2, v2, r[0], st["ru_m"].clone(), st["rv_m"].clone(), \
                 st["ww_m"].clone()
             ru_m, rv_m, ww_m = ro.sumflux(*f)
             pa, za = list(_clone(pz[0])), list(_clone(pz[1]))
-            pa[1], pa[4] = pho.clone(), muts2
+            pa[1], pa[2], pa[3], pa[4] = pho.clone(), fc["ph_tend"], fc["mu_tend"], muts2
             ph_s = ro.spec_bdyupdate_ph(*pa)
             za[0] = wo.clone()
             w_z = ro.zero_grad_bdy(*za)
@@ -632,9 +639,9 @@ def test_acoustic_loop_adjoint_matches_fp64_differences(config):
                   "rv_m": rv_m, "ww_m": ww_m}
         return sum(float((st[n] * lams[n]).sum()) for n in STATE)
 
-    base = {n: x[n].detach().double().cpu() for n in STATE}
+    base = {n: x[n].detach().double().cpu() for n in names}
     worst = 0.0
-    for n in STATE:
+    for n in names:
         d = torch.randn(tuple(base[n].shape), generator=gen, dtype=torch.float64)
         eps = 1e-4 * float(base[n].abs().max() + 1.0) / float(d.abs().max())
         vp, vm = dict(base), dict(base)
@@ -645,7 +652,7 @@ def test_acoustic_loop_adjoint_matches_fp64_differences(config):
         err = abs(got - ref) / max(abs(ref), float(terms.norm()), 1e-300)
         worst = max(worst, err)
         print("d/d%s: <grad, d> %.6e  difference %.6e  rel %.2e" % (n, got, ref, err), flush=True)
-    for n in STATE:
+    for n in names:
         assert x[n].grad is not None, n
     assert worst <= 1e-4
 
