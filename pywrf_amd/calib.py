@@ -1,4 +1,5 @@
-"""Truncated-backprop calibration of a water-vapour source bias.
+"""Truncated-backprop calibration: a water-vapour source bias through the
+transport adjoint, and a theta-forcing bias through the acoustic loop's.
 
 The model's water vapour is transported by the step's RK3 scalar transport
 (rk_scalar_tend's advection and rk_update_scalar, wrf_dynamics.py:3982-4119)
@@ -134,6 +135,64 @@ def calibrate(config="config2", nsteps=36, window=6, epochs=30, lr=2e-7, log=Non
     return {"config": config, "steps": nsteps, "window": window, "epochs": epochs, "history": hist,
             "transport_steps_per_s": fwd, "gradient_steps_per_s": grad,
             "peak_hbm_gb": torch.cuda.max_memory_allocated() / 1e9}
+
+
+def fit_forcing_bias(config="config4", stage=3, iters=60, lr=None, scale=0.1, seed=3, log=None):
+    """A theta-forcing bias fitted through an RK stage's acoustic loop.
+
+    The loop (solver.py:1349-1493: the stage's acoustic substeps, recorded
+    from a model run by adjoint.record_acoustic_loop) is run from its
+    recorded start with the large-step theta tendency t_tend + b_true: the
+    truth.  The model starts from b = 0 and fits b to the truth's theta at
+    the loop's end through adjoint.AcousticLoop (one checkpoint; the
+    substeps run again in its backward, through the hand-written adjoints of
+    advance_uv, advance_mu_t, advance_w and the calls around them), Adam.
+    b_true: a smooth field of `scale` times max |t_tend| on the mass points.
+    Returns the history of the misfit (relative to the truth's departure from
+    the unbiased run) and of the relative error of b."""
+    import pywrf_amd
+    from .gen import synthetic_state
+    from .step import RunConfig
+    log = log or (lambda *a: None)
+    s = synthetic_state(config, device="cuda")
+    x0, subs = adjoint.record_acoustic_loop(s, pywrf_amd.ops(), RunConfig(), stage=stage)
+    forcing = adjoint.loop_forcing(subs)
+    it = adjoint.FORCING.index("t_tend")
+    tt = forcing[it]
+    mask = interior_mask(s.grid, device=tt.device)
+    b_true = smooth_field(tuple(tt.shape), scale * float(tt.abs().max()), seed, device=tt.device) * mask
+    ti = adjoint.ACOUSTIC_STATE.index("t")
+
+    def theta_end(b):
+        f = list(forcing)
+        f[it] = tt + b
+        return adjoint.AcousticLoop.apply(subs, *x0, *f)[ti]
+
+    with torch.no_grad():
+        truth, plain = theta_end(b_true), theta_end(torch.zeros_like(tt))
+    norm = float(((truth - plain) ** 2).sum())
+    b = torch.zeros_like(tt, requires_grad=True)
+    lr = lr if lr is not None else 0.05 * float(b_true.abs().max())
+    opt = torch.optim.Adam([b], lr=lr)
+    sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, iters)
+    hist = []
+    t0 = time.perf_counter()
+    for n in range(iters + 1):
+        opt.zero_grad()
+        loss = ((theta_end(b) - truth) ** 2).sum() / norm
+        err = float((b.detach() - b_true).norm() / b_true.norm())
+        lv = float(loss.detach())
+        hist.append({"iter": n, "loss": lv, "bias_error": err})
+        if n % 10 == 0 or n == iters:
+            log("forcing fit %3d: misfit %.3e  bias error %.3e" % (n, lv, err))
+        if n == iters:
+            break
+        loss.backward()
+        opt.step()
+        sched.step()
+    torch.cuda.synchronize()
+    return {"config": config, "stage": stage, "substeps": len(subs), "iters": iters,
+            "seconds": time.perf_counter() - t0, "history": hist}
 
 
 def main(argv=None):
