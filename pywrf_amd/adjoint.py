@@ -449,7 +449,7 @@ class SpecBdyUpdateGroup(torch.autograd.Function):
         wrf_boundary.spec_bdyupdate_group([(o, t, it[2], it[3]) for o, t, it in zip(outs, tends, items)], dt, ix24,
                                           ix2d)
         ctx.call, ctx.live = call, [t is not None for t in args[n:2 * n]]
-        ctx.tshapes = [t for t in tends]
+        ctx.tends = tends
         return tuple(outs)
 
     @staticmethod
@@ -457,7 +457,7 @@ class SpecBdyUpdateGroup(torch.autograd.Function):
         from . import wrf_boundary
         items, dt, ix24, ix2d = ctx.call
         lt = [torch.zeros_like(t) if (on and l is not None) else None
-              for on, l, t in zip(ctx.live, lams, ctx.tshapes)]
+              for on, l, t in zip(ctx.live, lams, ctx.tends)]
         run = [(z, _f32(l), it[2], it[3]) for z, l, it in zip(lt, lams, items) if z is not None]
         if run:
             wrf_boundary.spec_bdyupdate_group(run, dt, ix24, ix2d)
@@ -466,9 +466,10 @@ class SpecBdyUpdateGroup(torch.autograd.Function):
 
 # ----------------------------------------------------------------------- the acoustic loop
 # The 17 fields an RK stage's acoustic loop (solver.py:1349-1493) carries from
-# one substep to the next; everything else its calls read is fixed for the
-# stage (the large-step tendencies, the saved *_save / *_1 fields, the
-# coefficients a, alpha, gamma, c2a, the map factors).
+# one substep to the next.  Everything else its calls read is fixed for the
+# stage: the large-step tendencies (which can also be given as inputs, FORCING
+# below), the saved *_save / *_1 fields, the coefficients a, alpha, gamma,
+# c2a, alt, php, cq*, mut and the map factors.
 ACOUSTIC_STATE = ("u", "v", "p", "ph", "al", "mu", "mudf", "ww", "muave", "muts", "t", "t_ave", "w", "pm1",
                   "ru_m", "rv_m", "ww_m")
 SUBSTEP_CALLS = ("advance_uv", "spec_bdyupdate_group", "advance_mu_t", "spec_bdyupdate_group", "advance_w",
