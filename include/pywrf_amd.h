@@ -699,8 +699,10 @@ int pywrf_rk_update_scalar_adjoint(const pywrf_grid_t* g, const float* lam_new, 
  * whole array.  t2ave_out, w_out: the forward
  * call's outputs (muts enters by division); ht, msftx, cf, rdx, rdy: its
  * terrain, map factor, surface extrapolation, grid spacing and fnm / fnp
- * arguments.  Also the sensitivities of the forcing rw_tend and ph_tend and
- * of w_save (the damping's reference), zero outside levels 1..kte-1 of the
+ * arguments.  Also the sensitivities of the forcing rw_tend and ph_tend, of
+ * w_save (the damping's reference), of t_1 (in the theta average) and of
+ * ph_1 (in the vertical advection of the geopotential, with the call's ww,
+ * and in the damping weights, with its w_save), zero outside the levels and
  * columns advanced.
  * work, work2: 3-D scratch arrays of the grid's size. */
 int pywrf_advance_w_adjoint(const pywrf_grid_t* g, const float* lam_t2ave_out, const float* lam_w_out,
@@ -712,8 +714,9 @@ int pywrf_advance_w_adjoint(const pywrf_grid_t* g, const float* lam_t2ave_out, c
                             const float* cf, double rdx, double rdy, const float* fnm, const float* fnp,
                             float* lam_w, float* lam_ph, float* lam_t_2, float* lam_t2ave, float* lam_u,
                             float* lam_v, float* lam_muts, float* lam_muave, float* lam_ww,
-                            float* lam_rw_tend, float* lam_ph_tend, float* lam_w_save, float* work,
-                            float* work2, pywrf_stream_t s);
+                            float* lam_rw_tend, float* lam_ph_tend, float* lam_w_save, float* lam_t_1,
+                            const float* ww, const float* w_save, float* lam_ph_1, float* work, float* work2,
+                            pywrf_stream_t s);
 /* The adjoint of advance_uv (wrf_dynamics.py:3391-3525) with respect to u,
  * v, ru_tend, rv_tend, p, ph, al, mu and mudf (alt, pb, php, cqu, cqv, muu,
  * muv and the map factors held fixed): lam_u_out / lam_v_out are the
@@ -729,10 +732,11 @@ int pywrf_advance_uv_adjoint(const pywrf_grid_t* g, const float* lam_u_out, cons
                              float* lam_v, float* lam_p, float* lam_ph, float* lam_al, float* lam_mu,
                              float* lam_mudf, float* lam_ru_tend, float* lam_rv_tend, pywrf_stream_t s);
 /* The adjoint of advance_mu_t (wrf_dynamics.py:3528-3638) with respect to
- * u, v, ww, mu, muave, muts, mudf, t, t_ave, ft and mu_tend (t_1 and the map
- * factors held fixed; u_1, v_1, ww_1 enter additively): lam_ww .. lam_mu are
- * the sensitivities of the returned ww, t, t_ave, muave, muts, mudf and mu;
- * the eleven outputs are overwritten (the mu-like ones are planes). */
+ * u, v, ww, mu, muave, muts, mudf, t, t_ave, ft, mu_tend and the large-step
+ * u_1, v_1, ww_1, t_1 (mut, muu, muv and the map factors held fixed): lam_ww
+ * .. lam_mu are the sensitivities of the returned ww, t, t_ave, muave, muts,
+ * mudf and mu; ww_out is the returned ww, u and v the call's; the fifteen
+ * outputs are overwritten (the mu-like ones are planes). */
 int pywrf_advance_mu_t_adjoint(const pywrf_grid_t* g, const float* lam_ww, const float* lam_t,
                                const float* lam_t_ave, const float* lam_muave, const float* lam_muts,
                                const float* lam_mudf, const float* lam_mu, const float* t_1, const float* msftx,
@@ -740,25 +744,31 @@ int pywrf_advance_mu_t_adjoint(const pywrf_grid_t* g, const float* lam_ww, const
                                const float* rdnw, double rdx, double rdy, double dts, double epssm,
                                float* lam_ww_ori, float* lam_u, float* lam_v, float* lam_mu_ori,
                                float* lam_muave_ori, float* lam_muts_ori, float* lam_mudf_ori, float* lam_t_ori,
-                               float* lam_t_ave_ori, float* lam_ft, float* lam_mu_tend, pywrf_stream_t s);
+                               float* lam_t_ave_ori, float* lam_ft, float* lam_mu_tend, const float* ww_out,
+                               const float* u, const float* v, const float* muu, const float* muv,
+                               const float* msfuy, const float* msfvx_inv, float* lam_u_1, float* lam_v_1,
+                               float* lam_ww_1, float* lam_t_1, pywrf_stream_t s);
 /* The adjoint of calc_p_rho (wrf_dynamics.py:3269-3313) with respect to al,
- * p, pm1, ph, t_2, mu and mut (alt, t_1 and c2a held fixed): lam_al, lam_p,
+ * p, pm1, ph, t_2, mu, mut and t_1 (alt and c2a held fixed): lam_al, lam_p,
  * lam_pm1 are the sensitivities of the returned al, p, pm1 (pm1 being the
- * copy of p at step 0); the seven outputs are overwritten, lam_mu and
- * lam_mut being planes. */
+ * copy of p at step 0); the outputs are overwritten, lam_mu and lam_mut
+ * being planes; lam_t_1 may be null. */
 int pywrf_calc_p_rho_adjoint(const pywrf_grid_t* g, const float* lam_al, const float* lam_p, const float* lam_pm1,
                              const float* ph, const float* alt, const float* t_2, const float* t_1, const float* c2a,
                              const float* mu, const float* mut, double t0, const float* rdnw, double smdiv, int step,
                              float* lam_al_ori, float* lam_p_ori, float* lam_pm1_in, float* lam_ph, float* lam_t_2,
-                             float* lam_mu, float* lam_mut, pywrf_stream_t s);
-/* The adjoint of sumflux (wrf_dynamics.py:3799-3839), linear in its fluxes
- * and accumulators: lam_ru_m .. lam_ww_m are the sensitivities of the
- * accumulators after the call; the six outputs (those of ru, rv, ww and of
- * the accumulators before the call) are overwritten. */
+                             float* lam_mu, float* lam_mut, float* lam_t_1, pywrf_stream_t s);
+/* The adjoint of sumflux (wrf_dynamics.py:3799-3839), linear in its fluxes,
+ * accumulators and u_lin, v_lin, ww_lin: lam_ru_m .. lam_ww_m are the
+ * sensitivities of the accumulators after the call; the outputs (those of
+ * ru, rv, ww, of the accumulators before the call and, when lam_u_lin is not
+ * null, of u_lin, v_lin, ww_lin, which take muu, muv, msfuy, msfvx_inv) are
+ * overwritten. */
 int pywrf_sumflux_adjoint(const pywrf_grid_t* g, const float* lam_ru_m, const float* lam_rv_m,
                           const float* lam_ww_m, int iteration, int number_of_small_timesteps, float* lam_ru,
                           float* lam_rv, float* lam_ww, float* lam_ru_m_in, float* lam_rv_m_in, float* lam_ww_m_in,
-                          pywrf_stream_t s);
+                          const float* muu, const float* muv, const float* msfuy, const float* msfvx_inv,
+                          float* lam_u_lin, float* lam_v_lin, float* lam_ww_lin, pywrf_stream_t s);
 /* The adjoint of a run of open-boundary fills (set_physical_bc3d / bc2d,
  * wrf_dynamics.py:164-278), the same arguments as
  * pywrf_set_physical_bc_group with each field replaced by its sensitivity
@@ -770,12 +780,13 @@ int pywrf_set_physical_bc_group_adjoint(const pywrf_grid_t* g3, const pywrf_grid
  * the field, muts, field_tend and mu_tend (ph_save held fixed): field_in is
  * the field before the update; lam_field holds the sensitivity of the
  * updated field on entry and that of field_in on return; lam_muts and
- * lam_mu_tend are written on the strips' columns only, lam_field_tend on the
- * strips' points only (the caller clears them). */
+ * lam_mu_tend are written on the strips' columns only, lam_field_tend and
+ * lam_ph_save (which may be null) on the strips' points only (the caller
+ * clears them). */
 int pywrf_spec_bdyupdate_ph_adjoint(const pywrf_grid_t* g, const float* ph_save, const float* field_in,
                                     const float* field_tend, const float* mu_tend, const float* muts, double dt,
                                     char variable_in, float* lam_field, float* lam_muts, float* lam_field_tend,
-                                    float* lam_mu_tend, pywrf_stream_t s);
+                                    float* lam_mu_tend, float* lam_ph_save, pywrf_stream_t s);
 /* The adjoint of zero_grad_bdy (wrf_boundary.py:361-405): lam, the
  * sensitivity of the field after the copy, gives lam_in (another array), that
  * of the field before it. */

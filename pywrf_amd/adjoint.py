@@ -146,19 +146,22 @@ class AdvanceW(torch.autograd.Function):
     reference's order; rw_tend, ph_tend, w_save None: those of args."""
 
     # positions in advance_w's signature
-    W, RWT, WW, WSAVE, U, V, MUAVE, MUTS, T2AVE, T2, PH, PHT = 0, 1, 2, 3, 4, 5, 8, 9, 18, 19, 21, 24
+    W, RWT, WW, WSAVE, U, V, MUAVE, MUTS, T2AVE, T2, T1, PH, PH1, PHT = 0, 1, 2, 3, 4, 5, 8, 9, 18, 19, 20, 21, 22, 24
     DIFF = (W, PH, T2, T2AVE, U, V, MUTS, MUAVE, WW)        # the inputs before args
-    EXTRA = (RWT, PHT, WSAVE)                               # the optional ones after it
+    EXTRA = (RWT, PHT, WSAVE, T1, PH1)                      # the optional ones after it
 
     @staticmethod
-    def forward(ctx, w, ph, t_2, t_2ave, u, v, muts, muave, ww, args, rw_tend=None, ph_tend=None, w_save=None):
+    def forward(ctx, w, ph, t_2, t_2ave, u, v, muts, muave, ww, args, rw_tend=None, ph_tend=None, w_save=None,
+                t_1=None, ph_1=None):
         from . import wrf_dynamics
         args = list(args)
         rw_tend = args[AdvanceW.RWT] if rw_tend is None else rw_tend
         ph_tend = args[AdvanceW.PHT] if ph_tend is None else ph_tend
         w_save = args[AdvanceW.WSAVE] if w_save is None else w_save
+        t_1 = args[AdvanceW.T1] if t_1 is None else t_1
+        ph_1 = args[AdvanceW.PH1] if ph_1 is None else ph_1
         for pos, x in zip(AdvanceW.DIFF + AdvanceW.EXTRA,
-                          (w, ph, t_2, t_2ave, u, v, muts, muave, ww, rw_tend, ph_tend, w_save)):
+                          (w, ph, t_2, t_2ave, u, v, muts, muave, ww, rw_tend, ph_tend, w_save, t_1, ph_1)):
             args[pos] = _f32(x)
         ctx.args = args
         t2a, wo, pho = wrf_dynamics.advance_w(*args)
@@ -177,7 +180,8 @@ class AdvanceW(torch.autograd.Function):
         uv = [torch.empty_like(a[4]), torch.empty_like(a[5])]
         planes = [torch.empty_like(a[9]), torch.empty_like(a[8])]
         lww = torch.empty_like(a[2])
-        forcing = [torch.empty_like(a[0]) for _ in range(3)]       # rw_tend, ph_tend, w_save
+        forcing = [torch.empty_like(a[0]) for _ in range(4)]       # rw_tend, ph_tend, w_save, t_1
+        lph1 = torch.empty_like(a[0])
         work = [torch.empty_like(a[0]) for _ in range(2)]
         t2a_out, w_out = ctx.outs
         # advance_w(w, rw_tend, ww, w_save, u, v, mu1, mut, muave, muts, c1h, c2h, c1f, c2f, c3h, c4h, c3f, c4f,
@@ -188,11 +192,11 @@ class AdvanceW(torch.autograd.Function):
              float(a[35]), float(a[36]), float(a[37]), ptr(a[41]), ptr(a[42]), ptr(a[47]), ptr(t2a_out),
              ptr(w_out), ptr(a[25]), ptr(a[46]), cf, float(a[33]), float(a[34]), ptr(a[39]), ptr(a[40]),
              *[ptr(t) for t in out], *[ptr(t) for t in uv], *[ptr(t) for t in planes], ptr(lww),
-             *[ptr(t) for t in forcing], *[ptr(t) for t in work], stream())
+             *[ptr(t) for t in forcing], ptr(a[2]), ptr(a[3]), ptr(lph1), *[ptr(t) for t in work], stream())
         lw, lph, lt2, lt2a = out
         lu, lv = uv
         lmuts, lmuave = planes
-        res = (lw, lph, lt2, lt2a, lu, lv, lmuts, lmuave, lww, None, *forcing)
+        res = (lw, lph, lt2, lt2a, lu, lv, lmuts, lmuave, lww, None, *forcing, lph1)
         need = ctx.needs_input_grad          # the optional forcing inputs: only those given as tensors
         return tuple(r if (n < 10 or need[n]) else None for n, r in enumerate(res[:len(need)]))
 
@@ -237,23 +241,30 @@ class AdvanceMuT(torch.autograd.Function):
     """The seven fresh results of advance_mu_t (wrf_dynamics.py:3528-3638):
     (ww, t, t_ave, muave, muts, mudf, mu), with the gradient with respect to
     the inputs at DIFF (u, v, ww, mu, muave, muts, mudf, t, t_ave, ft,
-    mu_tend); t_1, u_1, v_1, ww_1, mut and the map factors are held fixed.
-    The reference runs the mass updates under no_grad (its autograd would
-    leave them out); this is the full derivative.  args: advance_mu_t's
-    positional arguments in the reference's order."""
+    mu_tend) and, given after args, the large-step u_1, v_1, ww_1 and t_1;
+    mut, muu, muv and the map factors are held fixed.  The reference runs the
+    mass updates under no_grad (its autograd would leave them out); this is
+    the full derivative.  args: advance_mu_t's positional arguments in the
+    reference's order; u_1 .. t_1 None: those of args."""
 
     # positions in advance_mu_t's signature
     DIFF = (2, 4, 0, 6, 8, 9, 12, 24, 26, 27, 28)
+    EXTRA = (3, 5, 1, 25)                   # u_1, v_1, ww_1, t_1
     OUT = (0, 2, 3, 7, 8, 9, 10)            # positions of the fresh results in its 11-tuple
 
     @staticmethod
-    def forward(ctx, u, v, ww, mu, muave, muts, mudf, t, t_ave, ft, mu_tend, args):
+    def forward(ctx, u, v, ww, mu, muave, muts, mudf, t, t_ave, ft, mu_tend, args, u_1=None, v_1=None, ww_1=None,
+                t_1=None):
         from . import wrf_dynamics
         args = list(args)
         for pos, x in zip(AdvanceMuT.DIFF, (u, v, ww, mu, muave, muts, mudf, t, t_ave, ft, mu_tend)):
             args[pos] = _f32(x)
+        for pos, x in zip(AdvanceMuT.EXTRA, (u_1, v_1, ww_1, t_1)):
+            if x is not None:
+                args[pos] = _f32(x)
         ctx.args = args
         r = wrf_dynamics.advance_mu_t(*args)
+        ctx.wwo = r[0]
         return tuple(r[n] for n in AdvanceMuT.OUT)
 
     @staticmethod
@@ -267,11 +278,15 @@ class AdvanceMuT(torch.autograd.Function):
         #  c3h, c4h, c3f, c4f, uam, vam, wwam, t_ori, t_1, t_ave_ori, ft, mu_tend, rdx, rdy, dts, epssm,
         #  dnw, fnm, fnp, rdnw, msfux, msfuy, msfvx, msfvx_inv, msfvy, msftx, msfty, step, idx)
         out = [torch.empty_like(a[n]) for n in (0, 2, 4, 6, 8, 9, 12, 24, 26, 27, 28)]
+        big = [torch.empty_like(a[n]) for n in (3, 5, 1, 25)]
         call("pywrf_advance_mu_t_adjoint", g, *[ptr(x) for x in lam], ptr(a[25]), ptr(a[42]), ptr(a[43]),
              ptr(a[33]), ptr(a[34]), ptr(a[35]), ptr(a[36]), float(a[29]), float(a[30]), float(a[31]),
-             float(a[32]), *[ptr(x) for x in out], stream())
+             float(a[32]), *[ptr(x) for x in out], ptr(ctx.wwo), ptr(a[2]), ptr(a[4]), ptr(a[10]), ptr(a[11]),
+             ptr(a[38]), ptr(a[40]), *[ptr(x) for x in big], stream())
         gww, gu, gv, gmu, gmuave, gmuts, gmudf, gt, gtave, gft, gmut = out
-        return gu, gv, gww, gmu, gmuave, gmuts, gmudf, gt, gtave, gft, gmut, None
+        res = (gu, gv, gww, gmu, gmuave, gmuts, gmudf, gt, gtave, gft, gmut, None, *big)
+        need = ctx.needs_input_grad          # the optional inputs: only those given as tensors
+        return tuple(r if (n < 12 or need[n]) else None for n, r in enumerate(res[:len(need)]))
 
 
 class CalcPRho(torch.autograd.Function):
@@ -285,11 +300,13 @@ class CalcPRho(torch.autograd.Function):
     DIFF = (0, 1, 2, 4, 7, 8, 9)
 
     @staticmethod
-    def forward(ctx, al, p, ph, t_2, pm1, mu, mut, args):
+    def forward(ctx, al, p, ph, t_2, pm1, mu, mut, args, t_1=None):
         from . import wrf_dynamics
         args = list(args)
         for pos, x in zip(CalcPRho.DIFF, (al, p, ph, t_2, pm1, mu, mut)):
             args[pos] = _f32(x)
+        if t_1 is not None:
+            args[5] = _f32(t_1)
         args[7] = args[7].clone()
         ctx.args = args
         return wrf_dynamics.calc_p_rho(*args)
@@ -299,14 +316,16 @@ class CalcPRho(torch.autograd.Function):
         a = ctx.args
         g = grid(a[1].shape, tuple(a[-18:]))
         lal, lp, lpm1 = [torch.zeros_like(a[1]) if l is None else _f32(l) for l in (lal, lp, lpm1)]
-        out = [torch.empty_like(a[1]) for _ in range(5)] + [torch.empty_like(a[8]), torch.empty_like(a[9])]
+        out = [torch.empty_like(a[1]) for _ in range(5)] + [torch.empty_like(a[8]), torch.empty_like(a[9]),
+                                                             torch.empty_like(a[1])]
         # (al_ori, p_ori, ph, alt, t_2, t_1, c2a, pm1, mu, mut, c1h, c2h, c1f, c2f, c3h, c4h, c3f, c4f, znu, t0,
         #  rdnw, dnw, smdiv, non_hydrostatic, step, idx)
         call("pywrf_calc_p_rho_adjoint", g, ptr(lal), ptr(lp), ptr(lpm1), ptr(a[2]), ptr(a[3]), ptr(a[4]),
              ptr(a[5]), ptr(a[6]), ptr(a[8]), ptr(a[9]), float(a[19]), ptr(a[20]), float(a[22]), int(a[24]),
              *[ptr(t) for t in out], stream())
-        gal, gp, gpm1, gph, gt2, gmu, gmut = out
-        return gal, gp, gph, gt2, gpm1, gmu, gmut, None
+        gal, gp, gpm1, gph, gt2, gmu, gmut, gt1 = out
+        need = ctx.needs_input_grad
+        return (gal, gp, gph, gt2, gpm1, gmu, gmut, None, gt1 if need[-1] and len(need) > 8 else None)[:len(need)]
 
 
 class SumFlux(torch.autograd.Function):
@@ -319,11 +338,14 @@ class SumFlux(torch.autograd.Function):
     DIFF = (0, 1, 2, 16, 17, 18)
 
     @staticmethod
-    def forward(ctx, ru, rv, ww, ru_m, rv_m, ww_m, args):
+    def forward(ctx, ru, rv, ww, ru_m, rv_m, ww_m, args, u_lin=None, v_lin=None, ww_lin=None):
         from . import wrf_dynamics
         args = list(args)
         for pos, x in zip(SumFlux.DIFF, (ru, rv, ww, ru_m, rv_m, ww_m)):
             args[pos] = _f32(x)
+        for pos, x in ((3, u_lin), (4, v_lin), (5, ww_lin)):
+            if x is not None:
+                args[pos] = _f32(x)
         for pos in (16, 17, 18):
             args[pos] = args[pos].clone()
         ctx.args = args
@@ -335,9 +357,12 @@ class SumFlux(torch.autograd.Function):
         g = grid(a[0].shape, tuple(a[-18:]))
         lam = [torch.zeros_like(a[n]) if l is None else _f32(l) for l, n in zip((l1, l2, l3), (16, 17, 18))]
         out = [torch.empty_like(a[n]) for n in (0, 1, 2, 16, 17, 18)]
+        lin = [torch.empty_like(a[n]) for n in (3, 4, 5)]
         call("pywrf_sumflux_adjoint", g, *[ptr(x) for x in lam], int(a[25]), int(a[26]),
-             *[ptr(x) for x in out], stream())
-        return (*out, None)
+             *[ptr(x) for x in out], ptr(a[6]), ptr(a[7]), ptr(a[21]), ptr(a[23]), *[ptr(x) for x in lin],
+             stream())
+        need = ctx.needs_input_grad
+        return tuple(r if (n < 7 or need[n]) else None for n, r in enumerate((*out, None, *lin)[:len(need)]))
 
 
 class SpecBdyUpdatePh(torch.autograd.Function):
@@ -347,11 +372,13 @@ class SpecBdyUpdatePh(torch.autograd.Function):
     in the reference's order; field_tend, mu_tend None: those of args."""
 
     @staticmethod
-    def forward(ctx, field, muts, args, field_tend=None, mu_tend=None):
+    def forward(ctx, field, muts, args, field_tend=None, mu_tend=None, ph_save=None):
         from . import wrf_boundary
         args = list(args)
         ctx.f_in = _f32(field)
         args[1], args[4] = ctx.f_in.clone(), _f32(muts)
+        if ph_save is not None:
+            args[0] = _f32(ph_save)
         if field_tend is not None:
             args[2] = _f32(field_tend)
         if mu_tend is not None:
@@ -365,10 +392,12 @@ class SpecBdyUpdatePh(torch.autograd.Function):
         g = grid(a[1].shape, tuple(a[-24:]))
         lam = torch.zeros_like(a[1]) if lf is None else _f32(lf).clone()
         lmuts, lft, lmt = torch.zeros_like(a[4]), torch.zeros_like(a[2]), torch.zeros_like(a[3])
+        lps = torch.zeros_like(a[0])
         call("pywrf_spec_bdyupdate_ph_adjoint", g, ptr(a[0]), ptr(ctx.f_in), ptr(a[2]), ptr(a[3]), ptr(a[4]),
-             float(a[7]), flag(a[8]), ptr(lam), ptr(lmuts), ptr(lft), ptr(lmt), stream())
-        need = ctx.needs_input_grad          # the optional tendencies: only those given as tensors
-        return tuple(r if (n < 3 or need[n]) else None for n, r in enumerate((lam, lmuts, None, lft, lmt)[:len(need)]))
+             float(a[7]), flag(a[8]), ptr(lam), ptr(lmuts), ptr(lft), ptr(lmt), ptr(lps), stream())
+        need = ctx.needs_input_grad          # the optional inputs: only those given as tensors
+        return tuple(r if (n < 3 or need[n]) else None
+                     for n, r in enumerate((lam, lmuts, None, lft, lmt, lps)[:len(need)]))
 
 
 class ZeroGradBdy(torch.autograd.Function):
@@ -530,40 +559,57 @@ def record_acoustic_loop(s, ops, cfg, stage=2):
 FORCING = ("ru_tend", "rv_tend", "rw_tend", "t_tend", "ph_tend", "mu_tend")
 
 
+SAVED = ("u_save", "v_save", "w_save", "t_save", "ph_save", "ww1")
+
+
 def loop_forcing(substeps):
     """The six large-step tendencies (FORCING order) a recorded loop reads."""
     uv, _, mt, _, wa = substeps[0][:5]
     return [uv[1], uv[3], wa[1], mt[27], wa[24], mt[28]]
 
 
-def acoustic_substep(x, calls, forcing=None):
+def loop_saved(substeps):
+    """The six fields small_step_prep saved for the stage (SAVED order) that a
+    recorded loop reads: u_save, v_save (advance_mu_t's u_1, v_1, sumflux's
+    u_lin, v_lin), w_save (advance_w's damping), t_save (the t_1 of
+    advance_mu_t, advance_w and calc_p_rho), ph_save (advance_w's ph_1,
+    spec_bdyupdate_ph's ph_save) and ww1 (advance_mu_t's ww_1, sumflux's
+    ww_lin)."""
+    _, _, mt, _, wa = substeps[0][:5]
+    return [mt[3], mt[5], wa[3], mt[25], wa[22], mt[1]]
+
+
+def acoustic_substep(x, calls, forcing=None, saved=None):
     """One acoustic substep (solver.py:1349-1493) as autograd Functions:
     advance_uv, the u/v zone update, advance_mu_t, the t/mu/muts zone update,
     advance_w, sumflux, spec_bdyupdate_ph, zero_grad_bdy, calc_p_rho and the
     six open-boundary fills.  x: the 17 carried fields (ACOUSTIC_STATE
     order); calls: the nine recorded calls' arguments; forcing: None, or the
     six large-step tendencies (FORCING order) to use, and differentiate, in
-    place of the recorded ones wherever the substep reads them.  Returns the
-    17 fields after the substep."""
+    place of the recorded ones wherever the substep reads them; saved: the
+    same for the six saved fields (SAVED order).  Returns the 17 fields
+    after the substep."""
     st = dict(zip(ACOUSTIC_STATE, x))
     uv, z1, mt, z2, wa, sf, pz, cp, bc = calls
     fc = dict(zip(FORCING, forcing)) if forcing is not None else {
         "ru_tend": uv[1], "rv_tend": uv[3], "rw_tend": None, "t_tend": mt[27], "ph_tend": None, "mu_tend": mt[28]}
     given = forcing is not None
+    sv = dict(zip(SAVED, saved)) if saved is not None else dict.fromkeys(SAVED)
     u1, v1 = AdvanceUV.apply(st["u"], fc["ru_tend"], st["v"], fc["rv_tend"], st["p"], st["ph"], st["al"], st["mu"],
                              st["mudf"], uv)
     u2, v2 = SpecBdyUpdateGroup.apply(z1, u1, v1, *((fc["ru_tend"], fc["rv_tend"]) if given else (None, None)))
     ww, t, t_ave, muave, muts, mudf, mu = AdvanceMuT.apply(
         u2, v2, st["ww"], st["mu"], st["muave"], st["muts"], st["mudf"], st["t"], st["t_ave"], fc["t_tend"],
-        fc["mu_tend"], mt)
+        fc["mu_tend"], mt, sv["u_save"], sv["v_save"], sv["ww1"], sv["t_save"])
     t2, mu2, muts2 = SpecBdyUpdateGroup.apply(z2, t, mu, muts, *((fc["t_tend"], fc["mu_tend"], fc["mu_tend"])
                                                                  if given else (None, None, None)))
     t2a, w, ph = AdvanceW.apply(st["w"], st["ph"], t2, t_ave, u2, v2, muts2, muave, ww, wa, fc["rw_tend"],
-                                fc["ph_tend"], None)
-    ru_m, rv_m, ww_m = SumFlux.apply(u2, v2, ww, st["ru_m"], st["rv_m"], st["ww_m"], sf)
-    ph = SpecBdyUpdatePh.apply(ph, muts2, pz[0], fc["ph_tend"], fc["mu_tend"] if given else None)
+                                fc["ph_tend"], sv["w_save"], sv["t_save"], sv["ph_save"])
+    ru_m, rv_m, ww_m = SumFlux.apply(u2, v2, ww, st["ru_m"], st["rv_m"], st["ww_m"], sf, sv["u_save"],
+                                     sv["v_save"], sv["ww1"])
+    ph = SpecBdyUpdatePh.apply(ph, muts2, pz[0], fc["ph_tend"], fc["mu_tend"] if given else None, sv["ph_save"])
     w = ZeroGradBdy.apply(w, pz[1])
-    al, p, pm1 = CalcPRho.apply(st["al"], st["p"], ph, t2, st["pm1"], mu2, muts2, cp)
+    al, p, pm1 = CalcPRho.apply(st["al"], st["p"], ph, t2, st["pm1"], mu2, muts2, cp, sv["t_save"])
     items, ix24, ix2d = bc
     ph, al, p, muts2, mu2, mudf = OpenBoundaryFill.apply([(v, two) for _, v, two in items], ix24, ix2d, ph, al, p,
                                                          muts2, mu2, mudf)
@@ -579,7 +625,8 @@ class AcousticLoop(torch.autograd.Function):
     substep's operands for that pass only) and returns the gradient through
     the hand-written adjoints.  substeps: from record_acoustic_loop; args:
     the 17 fields (ACOUSTIC_STATE order), then optionally the six large-step
-    tendencies (FORCING order), which then replace the recorded ones in every
+    tendencies (FORCING order) and after them optionally the six saved
+    fields (SAVED order), which then replace the recorded ones in every
     substep and get gradients too.
 
         x0, subs = adjoint.record_acoustic_loop(state, ops, cfg)
@@ -589,15 +636,18 @@ class AcousticLoop(torch.autograd.Function):
 
     @staticmethod
     def _run(substeps, inputs):
-        x, forcing = list(inputs[:len(ACOUSTIC_STATE)]), inputs[len(ACOUSTIC_STATE):] or None
+        n, m = len(ACOUSTIC_STATE), len(ACOUSTIC_STATE) + len(FORCING)
+        x, forcing, saved = list(inputs[:n]), inputs[n:m] or None, inputs[m:] or None
         for calls in substeps:
-            x = acoustic_substep(x, calls, forcing)
+            x = acoustic_substep(x, calls, forcing, saved)
         return x
 
     @staticmethod
     def forward(ctx, substeps, *args):
-        if len(args) not in (len(ACOUSTIC_STATE), len(ACOUSTIC_STATE) + len(FORCING)):
-            raise ValueError("AcousticLoop: expected the 17 carried fields, optionally then the 6 tendencies")
+        n = len(ACOUSTIC_STATE)
+        if len(args) not in (n, n + len(FORCING), n + len(FORCING) + len(SAVED)):
+            raise ValueError("AcousticLoop: expected the 17 carried fields, optionally then the 6 tendencies "
+                             "and the 6 saved fields")
         ctx.substeps = substeps
         ctx.save_for_backward(*args)
         with torch.no_grad():

@@ -336,6 +336,9 @@ struct AdjW {
   float *lw_in, *lph_in, *lt2, *lt2a_in, *work, *work2;
   float *lu, *lv, *lmuts, *lmuave, *lww;           // zeroed by the caller; u, v by atomic adds
   float *lrw, *lpht, *lws;                         // rw_tend, ph_tend, w_save (zeroed by the caller)
+  float* lt1;                                      // t_1 (zeroed by the caller)
+  const float *ww, *wsave;                         // the call's ww and w_save
+  float* lph1;                                     // ph_1: vertical advection and damping (zeroed)
   float dts, t0, ep1, em1, dampmag, hpi, hdepth, rdx, rdy;
   int K, j0, j1, i0, i1;
 };
@@ -368,6 +371,15 @@ __global__ void k_advance_w_adj(Box all, Dims D, AdjW a) {
     const float lwd = AT(a.lW, k, j, i) + my * 0.5f * a.dts * G * a.ep1 / muts * AT(a.lP, k, j, i);
     AT(a.work, k, j, i) = lwd / (1.0f + dw);
     AT(a.lws, k, j, i) = -dw * mut / (1.0f + dw) * lwd;   // the damping's pull towards w_save
+    // the damping weight's dependence on ph_1: through hk at this level and
+    // hbot (from level K); at level K the two cancel
+    if (hk >= hbot && k < K) {
+      const float th = a.hpi * (hk - hbot) / a.hdepth;
+      const float ddw = a.dampmag * 2.0f * sinf(th) * cosf(th) * a.hpi / a.hdepth;
+      const float ldw = lwd * (-mut * AT(a.wsave, k, j, i) - AT(a.w_out, k, j, i)) / (1.0f + dw);
+      AT(a.lph1, k, j, i) += ldw * ddw / G;
+      AT(a.lph1, K, j, i) -= ldw * ddw / G;
+    }
   }
   // the column solve, transposed: back substitution upwards, elimination downwards
   for (int k = 1; k <= K - 1; ++k) AT(a.work, k + 1, j, i) -= AT(a.gamma, k, j, i) * AT(a.work, k, j, i);
@@ -452,6 +464,12 @@ __global__ void k_advance_w_adj(Box all, Dims D, AdjW a) {
       AT(a.lww, k + 1, j, i) += cu;
       AT(a.lww, k, j, i) += cu + cd;
       AT(a.lww, k - 1, j, i) += cd;
+      // the same wdwn terms through ph_1
+      const float pu = a.fnm[k] * li * 0.5f * (AT(a.ww, k + 1, j, i) + AT(a.ww, k, j, i)) * a.rdnw[k];
+      const float pd = a.fnp[k] * li * 0.5f * (AT(a.ww, k, j, i) + AT(a.ww, k - 1, j, i)) * a.rdnw[k - 1];
+      AT(a.lph1, k + 1, j, i) += pu;
+      AT(a.lph1, k, j, i) += pd - pu;
+      AT(a.lph1, k - 1, j, i) -= pd;
     }
   }
   // t2a[k] for k < K from t_2, t2a_in, muave and muts; levels from K on pass through
@@ -463,6 +481,7 @@ __global__ void k_advance_w_adj(Box all, Dims D, AdjW a) {
       AT(a.lt2a_in, k, j, i) = 0.5f * a.em1 / den * l;
       lmuave += a.t0 / den * l;
       lmuts -= AT(a.t2a_out, k, j, i) / muts * l;
+      AT(a.lt1, k, j, i) = -AT(a.t2a_out, k, j, i) / (a.t0 + AT(a.t_1, k, j, i)) * l;
     } else {
       AT(a.lt2, k, j, i) = 0.0f;
     }
@@ -481,7 +500,8 @@ extern "C" int pywrf_advance_w_adjoint(const pywrf_grid_t* g, const float* lam_t
                                        double rdx, double rdy, const float* fnm, const float* fnp, float* lam_w,
                                        float* lam_ph, float* lam_t_2, float* lam_t2ave, float* lam_u,
                                        float* lam_v, float* lam_muts, float* lam_muave, float* lam_ww,
-                                       float* lam_rw_tend, float* lam_ph_tend, float* lam_w_save, float* work,
+                                       float* lam_rw_tend, float* lam_ph_tend, float* lam_w_save, float* lam_t_1,
+                                       const float* ww, const float* w_save, float* lam_ph_1, float* work,
                                        float* work2, pywrf_stream_t s) {
   if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 1);
@@ -493,7 +513,8 @@ extern "C" int pywrf_advance_w_adjoint(const pywrf_grid_t* g, const float* lam_t
   p.lw_in = lam_w; p.lph_in = lam_ph; p.lt2 = lam_t_2; p.lt2a_in = lam_t2ave; p.work = work; p.work2 = work2;
   p.t2a_out = t2ave_out; p.w_out = w_out; p.ht = ht; p.msftx = msftx; p.cf = cf;
   p.lu = lam_u; p.lv = lam_v; p.lmuts = lam_muts; p.lmuave = lam_muave; p.lww = lam_ww;
-  p.lrw = lam_rw_tend; p.lpht = lam_ph_tend; p.lws = lam_w_save;
+  p.lrw = lam_rw_tend; p.lpht = lam_ph_tend; p.lws = lam_w_save; p.lt1 = lam_t_1;
+  p.ww = ww; p.wsave = w_save; p.lph1 = lam_ph_1;
   p.fnm = fnm; p.fnp = fnp;
   p.rdx = F(rdx); p.rdy = F(rdy);
   p.dts = F(dts); p.t0 = F(t0); p.ep1 = F(1. + epssm); p.em1 = F(1. - epssm);
@@ -509,6 +530,8 @@ extern "C" int pywrf_advance_w_adjoint(const pywrf_grid_t* g, const float* lam_t
   hipLaunchKernelGGL(k_zero_n, dim3(1024), dim3(256), 0, (hipStream_t)s, lam_rw_tend, n3);
   hipLaunchKernelGGL(k_zero_n, dim3(1024), dim3(256), 0, (hipStream_t)s, lam_ph_tend, n3);
   hipLaunchKernelGGL(k_zero_n, dim3(1024), dim3(256), 0, (hipStream_t)s, lam_w_save, n3);
+  hipLaunchKernelGGL(k_zero_n, dim3(1024), dim3(256), 0, (hipStream_t)s, lam_t_1, n3);
+  hipLaunchKernelGGL(k_zero_n, dim3(1024), dim3(256), 0, (hipStream_t)s, lam_ph_1, n3);
   hipLaunchKernelGGL(k_zero_n, dim3(256), dim3(256), 0, (hipStream_t)s, lam_muts, n2);
   hipLaunchKernelGGL(k_zero_n, dim3(256), dim3(256), 0, (hipStream_t)s, lam_muave, n2);
   Box all = whole(g);
@@ -661,6 +684,10 @@ struct AdjMuT {
   const float *lww, *lt, *ltave, *lmuave, *lmuts, *lmudf, *lmu;
   const float *t1, *msftx, *msfty, *dnw, *fnm, *fnp, *rdnw;
   float *gww, *gu, *gv, *gmu, *gmuave, *gmuts, *gmudf, *gt, *gtave, *gft, *gmut;
+  // the large-step fields: u_1, v_1 in the divergence (times muu / msfuy, muv msfvx_inv), ww_1
+  // subtracted from ww, t_1 in the theta fluxes (with u, v and the forward's ww)
+  const float *u, *v, *wwo, *muu, *muv, *msfuy, *msfvx_inv;
+  float *gu1, *gv1, *gww1, *gt1;                   // zeroed by the caller
   float rdx, rdy, dts, epssm;
   int k1;
 };
@@ -698,7 +725,9 @@ __global__ void k_advance_mu_t_adj(Box b, Dims D, AdjMuT a) {
   R = 0.0f;
   for (int k = a.k1 - 1; k >= 0; --k) {
     const float ldv = a.dnw[k] * lS - a.dnw[k] / mfy * R;
-    R += mut_lw(a, D, k, j, i, c);
+    const float lwk = mut_lw(a, D, k, j, i, c);
+    R += lwk;
+    AT(a.gww1, k, j, i) = -lwk;                  // ww' = ww - ww_1
     const float lt = AT(a.lt, k, j, i);
     AT(a.gt, k, j, i) = lt + AT(a.ltave, k, j, i);
     AT(a.gtave, k, j, i) = 0.0f;
@@ -710,6 +739,25 @@ __global__ void k_advance_mu_t_adj(Box b, Dims D, AdjMuT a) {
     atomic_add(&AT(a.gu, k, j, i), -du - q * 0.5f * a.rdx * (t0 + AT(a.t1, k, j, i - 1)));
     atomic_add(&AT(a.gv, k, j + 1, i), dv + q * 0.5f * a.rdy * (AT(a.t1, k, j + 1, i) + t0));
     atomic_add(&AT(a.gv, k, j, i), -dv - q * 0.5f * a.rdy * (t0 + AT(a.t1, k, j - 1, i)));
+    atomic_add(&AT(a.gu1, k, j, i + 1), du * (AT2(a.muu, j, i + 1) / AT2(a.msfuy, j, i + 1)));
+    atomic_add(&AT(a.gu1, k, j, i), -du * (AT2(a.muu, j, i) / AT2(a.msfuy, j, i)));
+    atomic_add(&AT(a.gv1, k, j + 1, i), dv * (AT2(a.muv, j + 1, i) * AT2(a.msfvx_inv, j + 1, i)));
+    atomic_add(&AT(a.gv1, k, j, i), -dv * (AT2(a.muv, j, i) * AT2(a.msfvx_inv, j, i)));
+    // t_1 in the horizontal theta fluxes
+    const float ue = AT(a.u, k, j, i + 1), uw = AT(a.u, k, j, i), vn = AT(a.v, k, j + 1, i), vs = AT(a.v, k, j, i);
+    const float hx = q * 0.5f * a.rdx, hy = q * 0.5f * a.rdy;
+    atomic_add(&AT(a.gt1, k, j, i + 1), hx * ue);
+    atomic_add(&AT(a.gt1, k, j, i - 1), -hx * uw);
+    atomic_add(&AT(a.gt1, k, j + 1, i), hy * vn);
+    atomic_add(&AT(a.gt1, k, j - 1, i), -hy * vs);
+    atomic_add(&AT(a.gt1, k, j, i), hx * (ue - uw) + hy * (vn - vs));
+    // and in the vertical flux wdtn[k] = ww'[k] (fnm[k] t_1[k] + fnp[k] t_1[k-1]), 0 < k < k1
+    if (k >= 1) {
+      const float lwdtn = c * (a.rdnw[k] * AT(a.lt, k, j, i) - a.rdnw[k - 1] * AT(a.lt, k - 1, j, i));
+      const float wk = AT(a.wwo, k, j, i);
+      atomic_add(&AT(a.gt1, k, j, i), lwdtn * wk * a.fnm[k]);
+      atomic_add(&AT(a.gt1, k - 1, j, i), lwdtn * wk * a.fnp[k]);
+    }
   }
 }
 
@@ -722,6 +770,9 @@ extern "C" int pywrf_advance_mu_t_adjoint(const pywrf_grid_t* g, const float* la
                                           float* lam_v, float* lam_mu_ori, float* lam_muave_ori,
                                           float* lam_muts_ori, float* lam_mudf_ori, float* lam_t_ori,
                                           float* lam_t_ave_ori, float* lam_ft, float* lam_mu_tend,
+                                          const float* ww_out, const float* u, const float* v, const float* muu,
+                                          const float* muv, const float* msfuy, const float* msfvx_inv,
+                                          float* lam_u_1, float* lam_v_1, float* lam_ww_1, float* lam_t_1,
                                           pywrf_stream_t s) {
   if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 1);
@@ -739,6 +790,8 @@ extern "C" int pywrf_advance_mu_t_adjoint(const pywrf_grid_t* g, const float* la
   hipLaunchKernelGGL(k_zero_n, dim3(1024), dim3(256), 0, st, lam_v, n3);
   hipLaunchKernelGGL(k_zero_n, dim3(1024), dim3(256), 0, st, lam_ft, n3);
   hipLaunchKernelGGL(k_zero_n, dim3(256), dim3(256), 0, st, lam_mu_tend, n2);
+  for (float* z : {lam_u_1, lam_v_1, lam_ww_1, lam_t_1})
+    hipLaunchKernelGGL(k_zero_n, dim3(1024), dim3(256), 0, st, z, n3);
   AdjMuT a;
   a.lww = lam_ww; a.lt = lam_t; a.ltave = lam_t_ave; a.lmuave = lam_muave; a.lmuts = lam_muts;
   a.lmudf = lam_mudf; a.lmu = lam_mu; a.t1 = t_1; a.msftx = msftx; a.msfty = msfty; a.dnw = dnw;
@@ -746,6 +799,8 @@ extern "C" int pywrf_advance_mu_t_adjoint(const pywrf_grid_t* g, const float* la
   a.gww = lam_ww_ori; a.gu = lam_u; a.gv = lam_v; a.gmu = lam_mu_ori; a.gmuave = lam_muave_ori;
   a.gmuts = lam_muts_ori; a.gmudf = lam_mudf_ori; a.gt = lam_t_ori; a.gtave = lam_t_ave_ori; a.gft = lam_ft;
   a.gmut = lam_mu_tend;
+  a.u = u; a.v = v; a.wwo = ww_out; a.muu = muu; a.muv = muv; a.msfuy = msfuy; a.msfvx_inv = msfvx_inv;
+  a.gu1 = lam_u_1; a.gv1 = lam_v_1; a.gww1 = lam_ww_1; a.gt1 = lam_t_1;
   a.rdx = F(rdx); a.rdy = F(rdy); a.dts = F(dts); a.epssm = F(epssm);
   a.k1 = g->kte - 1;
   const int j0 = imax(g->jts, g->jds + 1), j1 = imin(g->jte, g->jde - 2);
@@ -770,6 +825,7 @@ struct AdjPRho {
   const float *lal, *lp, *lpm1;                          // sensitivities of the results
   const float *ph, *alt, *t_2, *t_1, *c2a, *mu, *mut, *rdnw;
   float *gal, *gp, *gpm1, *gph, *gt2, *gmu, *gmut;     // sensitivities of the inputs
+  float* gt1;                                           // t_1 (null: not wanted)
   float t0, smdiv;
   int step;
 };
@@ -801,6 +857,7 @@ __global__ void k_calc_p_rho_adj(Box all, Dims D, Box b, AdjPRho a) {
       const float cl = c2 * lpr;
       const float L = lal - cl;                  // sensitivity of al, through p as well
       a.gt2[q] = cl * altq / den;
+      if (a.gt1) a.gt1[q] = cl * (-altq * mu / den - pa / (a.t0 + t1));
       gmu += cl * (-(altq * t1) / den) + L * (-altq / mut);
       gmut += cl * (-pa / mut) + L * (-al / mut);
       r = rd * L / mut;
@@ -811,6 +868,7 @@ __global__ void k_calc_p_rho_adj(Box all, Dims D, Box b, AdjPRho a) {
       a.gp[q] = a.step != 0 ? lp : lp + lpm;
       a.gpm1[q] = a.step != 0 ? lpm : 0.0f;
       a.gt2[q] = 0.0f;
+      if (a.gt1) a.gt1[q] = 0.0f;
     }
     a.gph[q] = r - prev;
     prev = r;
@@ -824,14 +882,15 @@ extern "C" int pywrf_calc_p_rho_adjoint(const pywrf_grid_t* g, const float* lam_
                                         const float* t_1, const float* c2a, const float* mu, const float* mut,
                                         double t0, const float* rdnw, double smdiv, int step, float* lam_al_ori,
                                         float* lam_p_ori, float* lam_pm1_in, float* lam_ph, float* lam_t_2,
-                                        float* lam_mu, float* lam_mut, pywrf_stream_t s) {
+                                        float* lam_mu, float* lam_mut, float* lam_t_1, pywrf_stream_t s) {
   if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 0);
   Box b = mkbox(g->kts, imin(g->kte, g->kde - 1), g->jts, imin(g->jte, g->jde - 1), g->its,
                 imin(g->ite, g->ide - 1));
   if (b.k1 + 1 > g->nz) return PYWRF_E_RANGE;
   AdjPRho a = {lam_al, lam_p, lam_pm1, ph, alt, t_2, t_1, c2a, mu, mut, rdnw,
-               lam_al_ori, lam_p_ori, lam_pm1_in, lam_ph, lam_t_2, lam_mu, lam_mut, F(t0), F(smdiv), step};
+               lam_al_ori, lam_p_ori, lam_pm1_in, lam_ph, lam_t_2, lam_mu, lam_mut, lam_t_1, F(t0), F(smdiv),
+               step};
   PW_LAUNCH_COLS(k_calc_p_rho_adj, whole(g), s, dims_of(g), b, a);
   PW_RET();
 }
@@ -841,9 +900,14 @@ extern "C" int pywrf_calc_p_rho_adjoint(const pywrf_grid_t* g, const float* lam_
 // accumulators are zeroed over the tile, each then adds its flux over its box,
 // and on the last call the sum is divided by the number of calls (the u_lin
 // terms are fixed).  One thread per point.
+struct SumLin {
+  const float *muu, *muv, *msfuy, *msfvx_inv;
+  float *gul, *gvl, *gwl;                               // u_lin, v_lin, ww_lin (null: not wanted)
+};
+
 __global__ void k_sumflux_adj(Box all, Dims D, const float* lru_m, const float* lrv_m, const float* lww_m,
                               float* gru, float* grv, float* gww, float* gru_m, float* grv_m, float* gww_m, Box b,
-                              Box bu, Box bv, Box bw, int first, int last, float n) {
+                              Box bu, Box bv, Box bw, int first, int last, float n, SumLin sl) {
   int k, j, i;
   if (!box_point(all, k, j, i)) return;
   const long q = D.ix(k, j, i);
@@ -860,12 +924,21 @@ __global__ void k_sumflux_adj(Box all, Dims D, const float* lru_m, const float* 
     gf[f][q] = in ? s : 0.0f;
     ga[f][q] = zeroed ? 0.0f : (in ? s : l);
   }
+  // the last call adds muu u_lin / msfuy, muv v_lin msfvx_inv and ww_lin
+  if (sl.gul) {
+    const bool iu = last && bu.has(k, j, i), iv = last && bv.has(k, j, i), iw = last && bw.has(k, j, i);
+    sl.gul[q] = iu ? lru_m[q] * (AT2(sl.muu, j, i) / AT2(sl.msfuy, j, i)) : 0.0f;
+    sl.gvl[q] = iv ? lrv_m[q] * (AT2(sl.muv, j, i) * AT2(sl.msfvx_inv, j, i)) : 0.0f;
+    sl.gwl[q] = iw ? lww_m[q] : 0.0f;
+  }
 }
 
 extern "C" int pywrf_sumflux_adjoint(const pywrf_grid_t* g, const float* lam_ru_m, const float* lam_rv_m,
                                      const float* lam_ww_m, int iteration, int number_of_small_timesteps,
                                      float* lam_ru, float* lam_rv, float* lam_ww, float* lam_ru_m_in,
-                                     float* lam_rv_m_in, float* lam_ww_m_in, pywrf_stream_t s) {
+                                     float* lam_rv_m_in, float* lam_ww_m_in, const float* muu, const float* muv,
+                                     const float* msfuy, const float* msfvx_inv, float* lam_u_lin,
+                                     float* lam_v_lin, float* lam_ww_lin, pywrf_stream_t s) {
   if (!pw_size_ok(g)) return PYWRF_E_RANGE;
   CHECK(g, 0);
   if (g->kte > g->nz || number_of_small_timesteps == 0) return PYWRF_E_RANGE;
@@ -874,9 +947,11 @@ extern "C" int pywrf_sumflux_adjoint(const pywrf_grid_t* g, const float* lam_ru_
   Box bv = mkbox(g->kts, mink, g->jts, g->jte, g->its, mini);
   Box bw = mkbox(g->kts, g->kte, g->jts, minj, g->its, mini);
   Box b = mkbox(g->kts, g->kte, g->jts, g->jte, g->its, g->ite);
+  if (lam_u_lin && (!lam_v_lin || !lam_ww_lin || !muu || !muv || !msfuy || !msfvx_inv)) return PYWRF_E_ARG;
+  SumLin sl = {muu, muv, msfuy, msfvx_inv, lam_u_lin, lam_v_lin, lam_ww_lin};
   PW_LAUNCH_BOX(k_sumflux_adj, whole(g), s, dims_of(g), lam_ru_m, lam_rv_m, lam_ww_m, lam_ru, lam_rv, lam_ww,
                 lam_ru_m_in, lam_rv_m_in, lam_ww_m_in, b, bu, bv, bw, iteration == 1,
-                iteration == number_of_small_timesteps, (float)number_of_small_timesteps);
+                iteration == number_of_small_timesteps, (float)number_of_small_timesteps, sl);
   PW_RET();
 }
 

@@ -503,7 +503,7 @@ extern "C" int pywrf_spec_bdyupdate_ph_zero_grad(const pywrf_grid_t* g, const fl
 // sensitivity on entry and f's on return; lam_muts and lam_mu_tend are
 // written on the band's columns only, lam_ft on the strips' points only.
 __global__ void k_spec_bdyupdate_ph_adj(Box cols, Dims D, PhStrip a, const float* f_in, float* lam,
-                                        float* lam_muts, float* lam_ft, float* lam_mu_tend) {
+                                        float* lam_muts, float* lam_ft, float* lam_mu_tend, float* lam_ph_save) {
   int j, i;
   if (!box_column(cols, j, i)) return;
   bool ir = i >= a.ibs && i < a.ibe, jc = j >= a.jbs + 1 && j < a.jbe - 1;
@@ -516,10 +516,11 @@ __global__ void k_spec_bdyupdate_ph_adj(Box cols, Dims D, PhStrip a, const float
     const long q = D.ix(k, j, i);
     const float ft = a.ft[q], ps = a.ph_save[q];
     const float dc = -a.dt * ft / (mt * mt) + ps * dr;
-    float v = f_in[q], dv = 0.0f, dft = 0.0f, dtn = 0.0f, s = 1.0f;
+    float v = f_in[q], dv = 0.0f, dft = 0.0f, dtn = 0.0f, dps = 0.0f, s = 1.0f;
     for (int m = 0; m < n; ++m) {
       dv = dv * r + v * dr + dc;
       dft = dft * r + a.dt / mt;
+      dps = dps * r + (mo / mt - 1.0f);
       dtn = dtn * r + (v + ps) * drt;
       v = v * mo / mt + a.dt * ft / mt + ps * (mo / mt - 1.0f);
       s *= r;
@@ -528,6 +529,7 @@ __global__ void k_spec_bdyupdate_ph_adj(Box cols, Dims D, PhStrip a, const float
     gm += l * dv;
     gt += l * dtn;
     lam_ft[q] = l * dft;
+    if (lam_ph_save) lam_ph_save[q] = l * dps;
     lam[q] = l * s;
   }
   AT2(lam_muts, j, i) = gm;
@@ -537,12 +539,13 @@ __global__ void k_spec_bdyupdate_ph_adj(Box cols, Dims D, PhStrip a, const float
 extern "C" int pywrf_spec_bdyupdate_ph_adjoint(const pywrf_grid_t* g, const float* ph_save, const float* field_in,
                                                const float* field_tend, const float* mu_tend, const float* muts,
                                                double dt, char variable_in, float* lam_field, float* lam_muts,
-                                               float* lam_field_tend, float* lam_mu_tend, pywrf_stream_t s) {
+                                               float* lam_field_tend, float* lam_mu_tend, float* lam_ph_save,
+                                               pywrf_stream_t s) {
   PhStrip a;
   if (int e = ph_strip_setup(g, ph_save, nullptr, field_tend, mu_tend, muts, dt, variable_in, a)) return e;
   Box cols = mkbox(0, 1, a.all.j0, a.all.j1, a.all.i0, a.all.i1);
   PW_LAUNCH_COLS(k_spec_bdyupdate_ph_adj, cols, s, dims_of(g), a, field_in, lam_field, lam_muts, lam_field_tend,
-                 lam_mu_tend);
+                 lam_mu_tend, lam_ph_save);
   PW_RET();
 }
 

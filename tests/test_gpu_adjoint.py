@@ -265,9 +265,10 @@ class _FirstCall:
 @pytest.mark.parametrize("config", ["config1c", "bdy24"])
 def test_advance_w_adjoint_matches_oracle_autograd(config):
     """Gradients of <lT, t_2ave> + <lW, w> + <lP, ph> after advance_w with
-    respect to w, ph, t_2, t_2ave, u, v, muts, muave, ww and the forcing
-    rw_tend, ph_tend and w_save, at the arguments of the step's first
-    acoustic substep, against torch autograd through the oracle's advance_w
+    respect to w, ph, t_2, t_2ave, u, v, muts, muave, ww, the forcing
+    rw_tend and ph_tend and the saved w_save, t_1 and ph_1 (the last through
+    the vertical advection and the damping weights), at the arguments of the
+    step's first acoustic substep, against torch autograd through the oracle's advance_w
     in FP64 (the same coefficients).  Bound: 1e-4 normwise per gradient (the
     coefficients are FP32 values in both; the FP32 adjoint's rounding)."""
     import oracle
@@ -291,8 +292,10 @@ def test_advance_w_adjoint_matches_oracle_autograd(config):
         a64[n] = t
     o64 = oracle.ops().advance_w(*a64)
     sum((o * l.double().cpu()).sum() for o, l in zip(o64, lams)).backward()
-    for name, mine, ref in zip(("w", "ph", "t_2", "t_2ave", "u", "v", "muts", "muave", "ww", "rw_tend", "ph_tend",
-                                "w_save"), x, x64):
+    names = ("w", "ph", "t_2", "t_2ave", "u", "v", "muts", "muave", "ww", "rw_tend", "ph_tend", "w_save", "t_1",
+             "ph_1")
+    assert len(names) == len(D)
+    for name, mine, ref in zip(names, x, x64):
         assert float(ref.grad.abs().max()) > 0, name
         err = _rel(mine.grad.double().cpu(), ref.grad)
         print("d/d%s: %.2e" % (name, err))
@@ -340,10 +343,11 @@ def test_advance_uv_adjoint_matches_oracle_autograd(config):
 @pytest.mark.parametrize("config", ["config1c", "bdy24"])
 def test_advance_mu_t_adjoint_matches_oracle_differences(config):
     """advance_mu_t's gradient with respect to each of u, v, ww, mu, muave,
-    muts, mudf, t, t_ave, ft and mu_tend, at the arguments of the step's first
-    acoustic substep, against FP64 differences of the oracle's advance_mu_t:
+    muts, mudf, t, t_ave, ft, mu_tend and the large-step u_1, v_1, ww_1, t_1,
+    at the arguments of the step's first acoustic substep, against FP64
+    differences of the oracle's advance_mu_t:
     for a seeded direction d in one input, sum_o <lam_o, f_o(x + d) - f_o(x)>
-    (exact up to FP64 rounding: the map is affine in these inputs) against
+    (exact up to FP64 rounding: the map is affine in each of these inputs) against
     <grad, d>.  Autograd through the oracle cannot serve here: like the
     reference it updates the column masses under no_grad.  Bound: 1e-4
     per input, relative to the larger of |difference| and the 2-norm of the
@@ -358,15 +362,17 @@ def test_advance_mu_t_adjoint_matches_oracle_differences(config):
     rec = _FirstCall(pywrf_amd.ops(), "advance_mu_t")
     rk3_step(s, rec, RunConfig())
     args = rec.args
-    D, OUT = adjoint.AdvanceMuT.DIFF, adjoint.AdvanceMuT.OUT
+    D, OUT = adjoint.AdvanceMuT.DIFF + adjoint.AdvanceMuT.EXTRA, adjoint.AdvanceMuT.OUT
     gen = torch.Generator().manual_seed(13)
     lams = [torch.randn(tuple(args[n].shape), generator=gen, dtype=torch.float64) for n in (0, 24, 26, 8, 9, 12, 6)]
     x = [args[n].clone().requires_grad_() for n in D]
-    outs = adjoint.AdvanceMuT.apply(*x, args)
+    outs = adjoint.AdvanceMuT.apply(*x[:11], args, *x[11:])
     sum((o * l.float().cuda()).sum() for o, l in zip(outs, lams)).backward()
     a64 = [a.detach().double().cpu() if isinstance(a, torch.Tensor) else a for a in args]
     base = oracle.ops().advance_mu_t(*a64)
-    names = ("u", "v", "ww", "mu", "muave", "muts", "mudf", "t", "t_ave", "ft", "mu_tend")
+    names = ("u", "v", "ww", "mu", "muave", "muts", "mudf", "t", "t_ave", "ft", "mu_tend", "u_1", "v_1", "ww_1",
+             "t_1")
+    assert len(names) == len(D)
     for name, pos, mine in zip(names, D, x):
         d = torch.randn(tuple(a64[pos].shape), generator=gen, dtype=torch.float64)
         b = list(a64)
@@ -562,9 +568,11 @@ def test_acoustic_loop_adjoint_matches_fp64_differences(config):
     AcousticLoop (one checkpoint, the substeps run again in its backward)
     from the hand-written adjoints, with respect to all 17 fields the loop
     carries (u, v, p, ph, al, mu, mudf, ww, muave, muts, t, t_ave, w, pm1 and
-    the three flux accumulators) and the six large-step tendencies it reads
-    (ru_tend, rv_tend, rw_tend, t_tend, ph_tend, mu_tend, wherever a call
-    reads them), against FP64 central differences of the same sequence
+    the three flux accumulators), the six large-step tendencies it reads
+    (ru_tend, rv_tend, rw_tend, t_tend, ph_tend, mu_tend) and the six fields
+    small_step_prep saved for the stage (u_save, v_save, w_save, t_save,
+    ph_save, ww1), each wherever a call reads it, against FP64 central
+    differences of the same sequence
     through the oracle: for each input a seeded direction d, <grad, d>
     against the difference of sum_o <lam_o, out_o> over the 17 results,
     relative to the larger of the difference and the 2-norm of grad_i d_i;
@@ -573,14 +581,15 @@ def test_acoustic_loop_adjoint_matches_fp64_differences(config):
     import oracle
     import pywrf_amd
     from pywrf_amd import adjoint
-    from pywrf_amd.adjoint import ACOUSTIC_STATE as STATE, FORCING
+    from pywrf_amd.adjoint import ACOUSTIC_STATE as STATE, FORCING, SAVED
     from pywrf_amd.gen import synthetic_state
     from pywrf_amd.step import RunConfig
     s = synthetic_state(config, device="cuda")
     x0, subs = adjoint.record_acoustic_loop(s, pywrf_amd.ops(), RunConfig(), stage=2)
     assert len(subs) == 2
-    names = STATE + FORCING
-    x = {n: f.clone().requires_grad_() for n, f in zip(names, list(x0) + adjoint.loop_forcing(subs))}
+    names = STATE + FORCING + SAVED
+    x = {n: f.clone().requires_grad_()
+         for n, f in zip(names, list(x0) + adjoint.loop_forcing(subs) + adjoint.loop_saved(subs))}
     out = adjoint.AcousticLoop.apply(subs, *[x[n] for n in names])
     st = dict(zip(STATE, out))
     gen = torch.Generator().manual_seed(23)
@@ -599,6 +608,7 @@ def test_acoustic_loop_adjoint_matches_fp64_differences(config):
     def loop(v):
         st = {n: v[n].clone() for n in STATE}
         fc = {n: v[n] for n in FORCING}
+        sv = {n: v[n] for n in SAVED}
         for uv, z1, mt, z2, wa, sf, pz, cp, bc in subs64:
             a = list(uv)
             a[0], a[1], a[2], a[3], a[4], a[6], a[9], a[10], a[15] = (
@@ -608,6 +618,7 @@ def test_acoustic_loop_adjoint_matches_fp64_differences(config):
             m[0], m[2], m[4], m[6], m[8], m[9], m[12], m[24], m[26], m[27], m[28] = (
                 st["ww"], u2, v2, st["mu"], st["muave"], st["muts"], st["mudf"], st["t"], st["t_ave"],
                 fc["t_tend"], fc["mu_tend"])
+            m[1], m[3], m[5], m[25] = sv["ww1"], sv["u_save"], sv["v_save"], sv["t_save"]
             m = [_clone(q) for q in m]
             r = ro.advance_mu_t(*m)
             t2, mu2, muts2 = ozone(z2, [r[2], r[10], r[8]], (fc["t_tend"], fc["mu_tend"], fc["mu_tend"]))
@@ -615,18 +626,21 @@ def test_acoustic_loop_adjoint_matches_fp64_differences(config):
             w_[0], w_[1], w_[2], w_[4], w_[5], w_[8], w_[9], w_[18], w_[19], w_[21], w_[24] = (
                 st["w"].clone(), fc["rw_tend"], r[0], u2, v2, r[7], muts2, r[3].clone(), t2, st["ph"].clone(),
                 fc["ph_tend"])
+            w_[3], w_[20], w_[22] = sv["w_save"], sv["t_save"], sv["ph_save"]
             t2a, wo, pho = ro.advance_w(*w_)
             f = _clone(list(sf))
             f[0], f[1], f[2], f[16], f[17], f[18] = u2, v2, r[0], st["ru_m"].clone(), st["rv_m"].clone(), \
                 st["ww_m"].clone()
+            f[3], f[4], f[5] = sv["u_save"], sv["v_save"], sv["ww1"]
             ru_m, rv_m, ww_m = ro.sumflux(*f)
             pa, za = list(_clone(pz[0])), list(_clone(pz[1]))
-            pa[1], pa[2], pa[3], pa[4] = pho.clone(), fc["ph_tend"], fc["mu_tend"], muts2
+            pa[0], pa[1], pa[2], pa[3], pa[4] = sv["ph_save"], pho.clone(), fc["ph_tend"], fc["mu_tend"], muts2
             ph_s = ro.spec_bdyupdate_ph(*pa)
             za[0] = wo.clone()
             w_z = ro.zero_grad_bdy(*za)
             c = _clone(list(cp))
             c[0], c[1], c[2], c[4], c[7], c[8], c[9] = st["al"], st["p"], ph_s, t2, st["pm1"].clone(), mu2, muts2
+            c[5] = sv["t_save"]
             al, p, pm1 = ro.calc_p_rho(*c)
             items, ix24, ix2d = bc
             filled = []
