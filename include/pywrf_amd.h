@@ -792,6 +792,42 @@ int pywrf_spec_bdyupdate_ph_adjoint(const pywrf_grid_t* g, const float* ph_save,
  * of the field before it. */
 int pywrf_zero_grad_bdy_adjoint(const pywrf_grid_t* g, const float* lam, char variable_in, float* lam_in,
                                 pywrf_stream_t s);
+/* The adjoint of small_step_prep (wrf_dynamics.py:3152-3266) past the first
+ * RK stage (rk_step > 1; PYWRF_E_ARG otherwise), with pb, p and alt (so c2a)
+ * held fixed: from the sensitivities of the saved u..ph and ww, of the
+ * acoustic u, v, w, t, ph and of the planes mu_2, muts, muus, muvs, mu_save
+ * it writes those of u_2 .. ph_2, u_1 .. ph_1 and ww, and of the planes
+ * mu_1 and mu_2.  u_1 .. t_1 and muus, muvs, muts: the call's inputs and
+ * results.  work2d: three planes of scratch. */
+int pywrf_small_step_prep_adjoint(
+    const pywrf_grid_t* g, int rk_step, const float* lam_u_save, const float* lam_v_save, const float* lam_w_save,
+    const float* lam_t_save, const float* lam_ph_save, const float* lam_ww_save, const float* lam_u,
+    const float* lam_v, const float* lam_w, const float* lam_t, const float* lam_ph, const float* lam_mu_2,
+    const float* lam_muts, const float* lam_muus, const float* lam_muvs, const float* lam_mu_save,
+    const float* u_1, const float* v_1, const float* w_1, const float* t_1, const float* muus, const float* muvs,
+    const float* muts, const float* muu, const float* muv, const float* mut, const float* msfuy,
+    const float* msfvx_inv, const float* msfty, float* lam_u_2_in, float* lam_v_2_in, float* lam_w_2_in,
+    float* lam_t_2_in, float* lam_ph_2_in, float* lam_u_1, float* lam_v_1, float* lam_w_1, float* lam_t_1,
+    float* lam_ph_1, float* lam_ww, float* lam_mu_1, float* lam_mu_2_in, float* work2d, pywrf_stream_t s);
+/* The adjoint of calc_mu_uv_1 (wrf_dynamics.py:3844-3860): from the
+ * sensitivities of the returned muu, muv, those of mu and of the incoming
+ * muu, muv (planes, overwritten). */
+int pywrf_calc_mu_uv_1_adjoint(const pywrf_grid_t* g, const float* lam_muu, const float* lam_muv, float* lam_mu,
+                               float* lam_muu_in, float* lam_muv_in, pywrf_stream_t s);
+/* The adjoint of small_step_finish (wrf_dynamics.py:3863-3978) with mut,
+ * muu, muv, the map factors and h_diabatic held fixed: from the
+ * sensitivities of the returned u, v, w, ph, ww, t and mu_2 (u_out .. t_out
+ * the forward's results) those of the acoustic u_2, v_2, w_2, ph_2, ww, t_2,
+ * mu_2, of the saved fields and of the planes muus, muvs, muts, mu_save. */
+int pywrf_small_step_finish_adjoint(
+    const pywrf_grid_t* g, const float* lam_u, const float* lam_v, const float* lam_w, const float* lam_ph,
+    const float* lam_ww, const float* lam_t, const float* lam_mu_2, const float* u_out, const float* v_out,
+    const float* w_out, const float* t_out, const float* muu, const float* muus, const float* muv,
+    const float* muvs, const float* mut, const float* muts, const float* msfuy, const float* msfvx,
+    const float* msfty, float* lam_u_2, float* lam_v_2, float* lam_w_2, float* lam_ph_2, float* lam_ww_in,
+    float* lam_t_2, float* lam_u_save, float* lam_v_save, float* lam_w_save, float* lam_ph_save, float* lam_ww1,
+    float* lam_t_save, float* lam_muus, float* lam_muvs, float* lam_muts, float* lam_mu_2_in, float* lam_mu_save,
+    pywrf_stream_t s);
 /* The column tridiagonal solve of advance_w (wrf_dynamics.py:3758-3765), in
  * place on w over advance_w's columns, levels 1..kte-1 (level 0 is the lower
  * boundary value): the same operations in the same order. */

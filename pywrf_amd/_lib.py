@@ -125,7 +125,8 @@ PLANE_PARAMS = frozenset((
     "muave_out mubf mudf mudf_in mudf_out muf muts muts_in muts_out muuf muus muvf muvs msf msft msftx "
     "msfty msfu msfux msfuy msfv msfvx msfvx_inv msfvy ht f e sina cosa rain rainnc rainncv snow snownc "
     "snowncv graupel graupelnc graupelncv sr mu_base lam_mu lam_mudf lam_muave lam_muts lam_mu_ori "
-    "lam_muave_ori lam_muts_ori lam_mudf_ori lam_mu_tend lam_mut").split())
+    "lam_muave_ori lam_muts_ori lam_mudf_ori lam_mu_tend lam_mut lam_mu_1 lam_mu_2 lam_mu_2_in lam_muus lam_muvs "
+    "lam_muu lam_muv lam_muu_in lam_muv_in lam_mu_save work2d").split())
 LEVEL_PARAMS = dict.fromkeys("c1h dn dnw fnm fnp fzm fzp rdn rdnw rdzu u_base v_base base".split())
 LEVEL_PARAMS.update(fcx=5, gcx=5, cf=3, cfn=1, cfn1=1, dtbc=1)
 # names whose role depends on the function: the advection operators' rdzw is

@@ -493,6 +493,117 @@ class SpecBdyUpdateGroup(torch.autograd.Function):
         return (None, *lams, *lt)
 
 
+class SmallStepPrep(torch.autograd.Function):
+    """small_step_prep (wrf_dynamics.py:3152-3266) past the first RK stage as a
+    function of u_1, u_2, v_1, v_2, w_1, w_2, t_1, t_2, ph_1, ph_2, mu_1, mu_2
+    and ww (pb, p, alt -- so c2a -- and the column masses mub, muu, muv, mut
+    held fixed), on copies of its in-place operands.  Returns (u_save,
+    v_save, w_save, t_save, ph_save, ww_save, u, v, w, t, ph, mu_2, muts,
+    muus, muvs, mu_save).  args: its positional arguments in the reference's
+    order."""
+
+    POS = (0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 11, 12, 34)
+    INPLACE = (11, 12, 14, 16, 18, 19, 28, 29, 30, 31, 32, 33, 35, 36)
+    OUT = (5, 6, 7, 8, 9, 16, 10, 11, 12, 13, 14, 18, 21, 19, 20, 23)   # positions in its 24 results
+
+    @staticmethod
+    def forward(ctx, u_1, u_2, v_1, v_2, w_1, w_2, t_1, t_2, ph_1, ph_2, mu_1, mu_2, ww, args):
+        from . import wrf_dynamics
+        args = list(args)
+        if int(args[49]) == 1:
+            raise ValueError("SmallStepPrep: the first RK stage (copies) is not differentiated")
+        for pos, x in zip(SmallStepPrep.POS, (u_1, u_2, v_1, v_2, w_1, w_2, t_1, t_2, ph_1, ph_2, mu_1, mu_2, ww)):
+            args[pos] = _f32(x)
+        for pos in SmallStepPrep.INPLACE:
+            args[pos] = args[pos].clone()
+        ctx.args = args
+        r = wrf_dynamics.small_step_prep(*args)
+        ctx.res = (r[19], r[20], r[21])          # muus, muvs, muts
+        return tuple(r[n] for n in SmallStepPrep.OUT)
+
+    @staticmethod
+    def backward(ctx, *lams):
+        a = ctx.args
+        g = grid(a[1].shape, tuple(a[-18:]))
+        like = [a[28], a[29], a[30], a[31], a[32], a[35], a[1], a[3], a[5], a[7], a[9], a[12], a[18], a[14], a[16],
+                a[33]]
+        lam = [torch.zeros_like(x) if l is None else _f32(l) for l, x in zip(lams, like)]
+        out3 = [torch.empty_like(a[n]) for n in (1, 3, 5, 7, 9, 0, 2, 4, 6, 8, 34)]
+        out2 = [torch.empty_like(a[11]), torch.empty_like(a[12])]
+        work = torch.empty((3,) + tuple(a[12].shape), device=a[12].device, dtype=torch.float32)
+        muus, muvs, muts = ctx.res
+        call("pywrf_small_step_prep_adjoint", g, int(a[49]), *[ptr(x) for x in lam], ptr(a[0]), ptr(a[2]),
+             ptr(a[4]), ptr(a[6]), ptr(muus), ptr(muvs), ptr(muts), ptr(a[13]), ptr(a[15]), ptr(a[17]), ptr(a[41]),
+             ptr(a[43]), ptr(a[46]), *[ptr(x) for x in out3], *[ptr(x) for x in out2], ptr(work), stream())
+        gu2, gv2, gw2, gt2, gph2, gu1, gv1, gw1, gt1, gph1, gww = out3
+        gmu1, gmu2 = out2
+        return gu1, gu2, gv1, gv2, gw1, gw2, gt1, gt2, gph1, gph2, gmu1, gmu2, gww, None
+
+
+class CalcMuUV1(torch.autograd.Function):
+    """(muu, muv) = calc_mu_uv_1(mu, muu, muv, ...) (wrf_dynamics.py:3844-3860)
+    on copies of muu, muv, with the gradient with respect to the three."""
+
+    @staticmethod
+    def forward(ctx, mu, muu, muv, args):
+        from . import wrf_dynamics
+        args = list(args)
+        args[0], args[1], args[2] = _f32(mu), _f32(muu).clone(), _f32(muv).clone()
+        ctx.args = args
+        return wrf_dynamics.calc_mu_uv_1(*args)
+
+    @staticmethod
+    def backward(ctx, lu, lv):
+        a = ctx.args
+        g = grid(a[0].shape, tuple(a[-18:]))
+        lu = torch.zeros_like(a[1]) if lu is None else _f32(lu)
+        lv = torch.zeros_like(a[2]) if lv is None else _f32(lv)
+        gmu, gu, gv = torch.empty_like(a[0]), torch.empty_like(a[1]), torch.empty_like(a[2])
+        call("pywrf_calc_mu_uv_1_adjoint", g, ptr(lu), ptr(lv), ptr(gmu), ptr(gu), ptr(gv), stream())
+        return gmu, gu, gv, None
+
+
+class SmallStepFinish(torch.autograd.Function):
+    """(u, v, w, ph, ww, t, mu_2) = small_step_finish(...)
+    (wrf_dynamics.py:3863-3978) as a function of the acoustic u, v, w, t, ph,
+    ww, mu_2, the saved u_save .. ph_save, ww1, mu_save and the column masses
+    muus, muvs, muts (mut, muu, muv, the map factors and h_diabatic held
+    fixed), on copies of its in-place operands.  args: its positional
+    arguments in the reference's order."""
+
+    POS = (0, 2, 4, 6, 8, 10, 12, 28, 29, 30, 31, 32, 11, 33, 17, 19, 15)
+
+    @staticmethod
+    def forward(ctx, u, v, w, t, ph, ww, mu, u_save, v_save, w_save, t_save, ph_save, ww1, mu_save, muus, muvs,
+                muts, args):
+        from . import wrf_dynamics
+        args = list(args)
+        for pos, x in zip(SmallStepFinish.POS, (u, v, w, t, ph, ww, mu, u_save, v_save, w_save, t_save, ph_save,
+                                                ww1, mu_save, muus, muvs, muts)):
+            args[pos] = _f32(x)
+        args[10], args[12] = args[10].clone(), args[12].clone()
+        ctx.args = args
+        r = wrf_dynamics.small_step_finish(*args)
+        ctx.res = (r[0], r[1], r[2], r[5])       # u, v, w, t
+        return r
+
+    @staticmethod
+    def backward(ctx, lu, lv, lw, lph, lww, lt, lmu):
+        a = ctx.args
+        g = grid(a[0].shape, tuple(a[-18:]))
+        like = (a[0], a[2], a[4], a[8], a[10], a[6], a[12])
+        lam = [torch.zeros_like(x) if l is None else _f32(l) for l, x in zip((lu, lv, lw, lph, lww, lt, lmu), like)]
+        out3 = [torch.empty_like(a[n]) for n in (0, 2, 4, 8, 10, 6, 28, 29, 30, 32, 11, 31)]
+        out2 = [torch.empty_like(a[n]) for n in (17, 19, 15, 12, 33)]
+        uo, vo, wo, to = ctx.res
+        call("pywrf_small_step_finish_adjoint", g, *[ptr(x) for x in lam], ptr(uo), ptr(vo), ptr(wo), ptr(to),
+             ptr(a[16]), ptr(a[17]), ptr(a[18]), ptr(a[19]), ptr(a[14]), ptr(a[15]), ptr(a[35]), ptr(a[36]),
+             ptr(a[39]), *[ptr(x) for x in out3], *[ptr(x) for x in out2], stream())
+        gu, gv, gw, gph, gww, gt, gus, gvs, gws, gps, gww1, gts = out3
+        gmuus, gmuvs, gmuts, gmu, gmusv = out2
+        return gu, gv, gw, gt, gph, gww, gmu, gus, gvs, gws, gts, gps, gww1, gmusv, gmuus, gmuvs, gmuts, None
+
+
 # ----------------------------------------------------------------------- the acoustic loop
 # The 17 fields an RK stage's acoustic loop (solver.py:1349-1493) carries from
 # one substep to the next.  Everything else its calls read is fixed for the
@@ -616,6 +727,89 @@ def acoustic_substep(x, calls, forcing=None, saved=None):
     new = {"u": u2, "v": v2, "p": p, "ph": ph, "al": al, "mu": mu2, "mudf": mudf, "ww": ww, "muave": muave,
            "muts": muts2, "t": t2, "t_ave": t2a, "w": w, "pm1": pm1, "ru_m": ru_m, "rv_m": rv_m, "ww_m": ww_m}
     return [new[n] for n in ACOUSTIC_STATE]
+
+
+STAGE_IN = ("u_2", "v_2", "w_2", "t_2", "ph_2", "mu_2", "u_1", "v_1", "w_1", "t_1", "ph_1", "mu_1", "ww", "mudf",
+            "muave", "t_ave", "ru_m", "rv_m", "ww_m")
+STAGE_OUT = ("u", "v", "w", "ph", "ww", "t", "mu", "p", "al", "pm1", "mudf", "muave", "muts", "t_ave", "ru_m",
+             "rv_m", "ww_m")
+
+
+class _StageRecorder(_LoopRecorder):
+    """The step's functions, keeping copies of the arguments of every call
+    from the n-th small_step_prep through the next small_step_finish."""
+
+    def __getattr__(self, name):
+        f = getattr(self._ops, name)
+
+        def rec(*args):
+            if name == "small_step_prep":
+                self._seen += 1
+            on = self._seen == self._n and not self._done
+            if on:
+                self.calls.append((name, self._copy(list(args))))
+            if on and name == "small_step_finish":
+                self._done = True
+            return f(*args)
+        return rec
+
+
+def record_acoustic_stage(s, ops, cfg, stage=2):
+    """Run one RK3 step on state s and keep the acoustic part of RK stage
+    `stage` (2 or 3): small_step_prep, the pressure and density of calc_p_rho,
+    the open-boundary fills, the acoustic loop, calc_mu_uv_1 and
+    small_step_finish (solver.py:1258-1543).  Returns (x, calls): x the
+    stage's inputs in STAGE_IN order, calls the recorded calls' arguments."""
+    from .step import rk3_step
+    if stage not in (2, 3):
+        raise ValueError("record_acoustic_stage: stage 2 or 3 (the first stage's preparation copies)")
+    rec = _StageRecorder(ops, stage)
+    rk3_step(s, rec, cfg)
+    names = [n for n, _ in rec.calls]
+    m = (len(names) - 6) // len(SUBSTEP_CALLS)
+    head = ["small_step_prep", "calc_p_rho", "calc_coef_w", "set_physical_bc_group"]
+    if m < 1 or names != head + list(SUBSTEP_CALLS) * m + ["calc_mu_uv_1", "small_step_finish"]:
+        raise RuntimeError("record_acoustic_stage: unexpected call sequence %s" % names[:8])
+    calls = [a for _, a in rec.calls]
+    pr, mt = calls[0], calls[4 + 2]
+    x = [pr[1], pr[3], pr[5], pr[7], pr[9], pr[12], pr[0], pr[2], pr[4], pr[6], pr[8], pr[11], pr[34], pr[19],
+         mt[8], mt[26], mt[21], mt[22], mt[23]]
+    return x, calls
+
+
+def acoustic_stage(x, calls):
+    """The acoustic part of an RK stage past the first, as autograd Functions
+    (small_step_prep, calc_p_rho at step 0, the open-boundary fills, the
+    acoustic substeps, calc_mu_uv_1, small_step_finish); x: STAGE_IN, calls:
+    from record_acoustic_stage.  Returns the STAGE_OUT fields.  pb, p, al
+    and alt enter only through fixed coefficients (c2a, and al and p outside
+    calc_p_rho's box), the large-step tendencies and masses are fixed."""
+    v = dict(zip(STAGE_IN, x))
+    pr, cp, _, bc = calls[:4]
+    subs = [calls[4 + 9 * q:4 + 9 * q + 9] for q in range((len(calls) - 6) // 9)]
+    mc, fin = calls[-2], calls[-1]
+    (us, vs, ws, ts, ps, ww1, u, vv, w, t, ph, mu, muts, muus, muvs, musv) = SmallStepPrep.apply(
+        v["u_1"], v["u_2"], v["v_1"], v["v_2"], v["w_1"], v["w_2"], v["t_1"], v["t_2"], v["ph_1"], v["ph_2"],
+        v["mu_1"], v["mu_2"], v["ww"], pr)
+    al, p, pm1 = CalcPRho.apply(cp[0], cp[1], ph, t, cp[7], mu, muts, cp, ts)
+    items, ix24, ix2d = bc
+    _, _, ph, al, p, _, ts, _, mu, mudf = OpenBoundaryFill.apply(
+        [(k, two) for _, k, two in items], ix24, ix2d, items[0][0], items[1][0], ph, al, p, v["t_1"], ts,
+        v["mu_1"], mu, v["mudf"])
+    st = [u, vv, p, ph, al, mu, mudf, v["ww"], v["muave"], muts, t, v["t_ave"], w, pm1, v["ru_m"], v["rv_m"],
+          v["ww_m"]]
+    saved = [us, vs, ws, ts, ps, ww1]
+    for c in subs:
+        st = acoustic_substep(st, c, None, saved)
+    st = dict(zip(ACOUSTIC_STATE, st))
+    muus, muvs = CalcMuUV1.apply(st["muts"], muus, muvs, mc)
+    uo, vo, wo, pho, wwo, to, muo = SmallStepFinish.apply(
+        st["u"], st["v"], st["w"], st["t"], st["ph"], st["ww"], st["mu"], us, vs, ws, ts, ps, ww1, musv, muus, muvs,
+        st["muts"], fin)
+    out = {"u": uo, "v": vo, "w": wo, "ph": pho, "ww": wwo, "t": to, "mu": muo}
+    for n in ("p", "al", "pm1", "mudf", "muave", "muts", "t_ave", "ru_m", "rv_m", "ww_m"):
+        out[n] = st[n]
+    return [out[n] for n in STAGE_OUT]
 
 
 class AcousticLoop(torch.autograd.Function):

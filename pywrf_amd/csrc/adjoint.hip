@@ -1044,3 +1044,276 @@ extern "C" int pywrf_set_physical_bc_group_adjoint(const pywrf_grid_t* g3, const
                      (long)g3->ny * g3->nx, g3->nx, J);
   PW_RET();
 }
+
+// ------------------------------------------------------------------ small_step_prep^T (past the first stage)
+// small_step_prep (wrf_dynamics.py:3152-3266) past the first RK stage, with
+// pb, p, alt (so c2a) held fixed: on its boxes it saves u_2 .. ph_2 and ww,
+// and forms the acoustic variables
+//   u = (muus u_1 - muu u_2) / msfuy,  v = (muvs v_1 - muv v_2) msfvx_inv,
+//   t = muts t_1 - mut t_2,  w = (muts w_1 - mut w_2) / msfty,  ph = ph_1 - ph_2,
+// with muts = mub + mu_1, muus / muvs the averages of mub + mu_1 onto the u / v
+// points, mu_save = mu_2 and mu_2' = mu_1 - mu_2.  Pass 1 (one thread per
+// column) gives the 3-D sensitivities and sums down the column those of
+// muus, muvs and muts; pass 2 (one thread per point of the plane) gathers
+// the averages' and the masses' shares into mu_1 and mu_2.
+struct PrepAdj {
+  const float *lus, *lvs, *lws, *lts, *lps, *lwws, *luo, *lvo, *lwo, *lto, *lpo;   // 3-D results
+  const float *lmu2, *lmuts, *lmuus, *lmuvs, *lmusave;                             // plane results
+  const float *u_1, *v_1, *w_1, *t_1, *muus, *muvs, *muts, *muu, *muv, *mut, *msfuy, *msfvx_inv, *msfty;
+  float *gu2, *gv2, *gw2, *gt2, *gph2, *gu1, *gv1, *gw1, *gt1, *gph1, *gww;
+  float *smuus, *smuvs, *smuts;                                                    // plane scratch
+  float *gmu1, *gmu2;
+  int k1, kde, j0, j1, i0, i1, i1u, j1v;
+};
+
+__global__ void k_ssprep_adj(Box all, Dims D, PrepAdj a) {
+  int j, i;
+  if (!box_column(all, j, i)) return;
+  const bool h = j >= a.j0 && j < a.j1 && i >= a.i0 && i < a.i1;
+  const bool ur = j >= a.j0 && j < a.j1 && i >= a.i0 && i < a.i1u;
+  const bool vr = j >= a.j0 && j < a.j1v && i >= a.i0 && i < a.i1;
+  const bool vsr = j >= a.j0 && j < a.j1v && i >= a.i0 && i < a.i1u;   // the muvs box (i-range i1u)
+  const float muus = AT2(a.muus, j, i), muvs = AT2(a.muvs, j, i), muts = AT2(a.muts, j, i);
+  const float muu = AT2(a.muu, j, i), muv = AT2(a.muv, j, i), mut = AT2(a.mut, j, i);
+  const float my = AT2(a.msfuy, j, i), mvx = AT2(a.msfvx_inv, j, i), mty = AT2(a.msfty, j, i);
+  float su = 0.0f, sv = 0.0f, st = 0.0f;
+  for (int k = 0; k < D.nz; ++k) {
+    const long q = D.ix(k, j, i);
+    const bool bu = k < a.k1 && ur, bv = k < a.k1 && vr, bm = k < a.k1 && h, bw = k < a.kde && h;
+    const float luo = a.luo[q], lvo = a.lvo[q], lwo = a.lwo[q], lto = a.lto[q], lpo = a.lpo[q];
+    if (bu) {
+      a.gu2[q] = a.lus[q] - luo * muu / my;
+      a.gu1[q] = luo * muus / my;
+      su += luo * a.u_1[q] / my;
+    } else {
+      a.gu2[q] = luo;
+      a.gu1[q] = 0.0f;
+    }
+    if (bv) {
+      a.gv2[q] = a.lvs[q] - lvo * muv * mvx;
+      a.gv1[q] = lvo * muvs * mvx;
+      sv += lvo * a.v_1[q] * mvx;
+    } else {
+      a.gv2[q] = lvo;
+      a.gv1[q] = 0.0f;
+    }
+    if (bm) {
+      a.gt2[q] = a.lts[q] - lto * mut;
+      a.gt1[q] = lto * muts;
+      st += lto * a.t_1[q];
+    } else {
+      a.gt2[q] = lto;
+      a.gt1[q] = 0.0f;
+    }
+    if (bw) {
+      a.gw2[q] = a.lws[q] - lwo * mut / mty;
+      a.gw1[q] = lwo * muts / mty;
+      st += lwo * a.w_1[q] / mty;
+      a.gph2[q] = a.lps[q] - lpo;
+      a.gph1[q] = lpo;
+      a.gww[q] = a.lwws[q];
+    } else {
+      a.gw2[q] = lwo;
+      a.gw1[q] = 0.0f;
+      a.gph2[q] = lpo;
+      a.gph1[q] = 0.0f;
+      a.gww[q] = 0.0f;
+    }
+  }
+  AT2(a.smuus, j, i) = ur ? AT2(a.lmuus, j, i) + su : 0.0f;
+  AT2(a.smuvs, j, i) = vsr ? AT2(a.lmuvs, j, i) + sv : 0.0f;
+  AT2(a.smuts, j, i) = h ? AT2(a.lmuts, j, i) + st : 0.0f;
+}
+
+__global__ void k_ssprep_adj_2d(Box all, Dims D, PrepAdj a) {
+  int j, i;
+  if (!box_column(all, j, i)) return;
+  const bool h = j >= a.j0 && j < a.j1 && i >= a.i0 && i < a.i1;
+  const float lmu2 = AT2(a.lmu2, j, i);
+  // muus[j][i] and muus[j][i+1] average mu_1[j][i]; muvs[j][i] and muvs[j+1][i] too
+  float g1 = 0.5f * (AT2(a.smuus, j, i) + (i + 1 < D.nx ? AT2(a.smuus, j, i + 1) : 0.0f)) +
+             0.5f * (AT2(a.smuvs, j, i) + (j + 1 < D.ny ? AT2(a.smuvs, j + 1, i) : 0.0f));
+  float g2 = lmu2;                                        // outside the box mu_2 passes through
+  if (h) {
+    g1 += AT2(a.smuts, j, i) + lmu2;                      // muts = mub + mu_1, mu_2' = mu_1 - mu_2
+    g2 = AT2(a.lmusave, j, i) - lmu2;
+  }
+  AT2(a.gmu1, j, i) = g1;
+  AT2(a.gmu2, j, i) = g2;
+}
+
+extern "C" int pywrf_small_step_prep_adjoint(
+    const pywrf_grid_t* g, int rk_step, const float* lam_u_save, const float* lam_v_save, const float* lam_w_save,
+    const float* lam_t_save, const float* lam_ph_save, const float* lam_ww_save, const float* lam_u,
+    const float* lam_v, const float* lam_w, const float* lam_t, const float* lam_ph, const float* lam_mu_2,
+    const float* lam_muts, const float* lam_muus, const float* lam_muvs, const float* lam_mu_save,
+    const float* u_1, const float* v_1, const float* w_1, const float* t_1, const float* muus, const float* muvs,
+    const float* muts, const float* muu, const float* muv, const float* mut, const float* msfuy,
+    const float* msfvx_inv, const float* msfty, float* lam_u_2_in, float* lam_v_2_in, float* lam_w_2_in,
+    float* lam_t_2_in, float* lam_ph_2_in, float* lam_u_1, float* lam_v_1, float* lam_w_1, float* lam_t_1,
+    float* lam_ph_1, float* lam_ww, float* lam_mu_1, float* lam_mu_2_in, float* work2d, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
+  CHECK(g, 1);
+  if (rk_step == 1) return PYWRF_E_ARG;            // the first stage copies: not differentiated here
+  if (g->kde > g->nz || g->kts != 0) return PYWRF_E_RANGE;
+  PrepAdj a = {lam_u_save, lam_v_save, lam_w_save, lam_t_save, lam_ph_save, lam_ww_save, lam_u, lam_v, lam_w,
+               lam_t, lam_ph, lam_mu_2, lam_muts, lam_muus, lam_muvs, lam_mu_save, u_1, v_1, w_1, t_1, muus, muvs,
+               muts, muu, muv, mut, msfuy, msfvx_inv, msfty, lam_u_2_in, lam_v_2_in, lam_w_2_in, lam_t_2_in,
+               lam_ph_2_in, lam_u_1, lam_v_1, lam_w_1, lam_t_1, lam_ph_1, lam_ww, nullptr, nullptr, nullptr,
+               lam_mu_1, lam_mu_2_in, imin(g->kte, g->kde - 1), g->kde, g->jts, imin(g->jte, g->jde - 1), g->its,
+               imin(g->ite, g->ide - 1), g->ite, g->jte};
+  const long n2 = (long)g->ny * g->nx;
+  a.smuus = work2d; a.smuvs = work2d + n2; a.smuts = work2d + 2 * n2;
+  if (a.j0 < 1 || a.i0 < 1) return PYWRF_E_RANGE;   // the averages read j-1 / i-1
+  Box pl = mkbox(0, 1, 0, g->ny, 0, g->nx);
+  PW_LAUNCH_COLS(k_ssprep_adj, pl, s, dims_of(g), a);
+  PW_LAUNCH_COLS(k_ssprep_adj_2d, pl, s, dims_of(g), a);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ calc_mu_uv_1^T
+// calc_mu_uv_1 (wrf_dynamics.py:3844-3860) sets muu / muv from mu: the
+// two-point average inside, mu itself on the first row / column and the
+// second last point's value on the last; elsewhere muu / muv keep theirs.
+// Transpose: each written point hands its sensitivity to the mu points it
+// read (atomic adds onto the zeroed lam_mu) and keeps none.
+__global__ void k_calc_mu_uv_1_adj(Box b, Dims D, const float* lmuu, const float* lmuv, float* gmu, float* gmuu,
+                                   float* gmuv, int jts, int jtf, int jte, int its, int itf, int ite) {
+  int j, i;
+  if (!box_column(b, j, i)) return;
+  float lu = AT2(lmuu, j, i), lv = AT2(lmuv, j, i);
+  bool wu = false, wv = false;
+  if (j < jtf) {
+    if (i == ite - 1) { atomic_add(&AT2(gmu, j, ite - 2), lu); wu = true; }
+    else if (i >= its + 1) { atomic_add(&AT2(gmu, j, i), 0.5f * lu); atomic_add(&AT2(gmu, j, i - 1), 0.5f * lu); wu = true; }
+    else if (i == its) { atomic_add(&AT2(gmu, j, its), lu); wu = true; }
+  }
+  if (i < itf) {
+    if (j == jte - 1) { atomic_add(&AT2(gmu, jte - 2, i), lv); wv = true; }
+    else if (j >= jts + 1) { atomic_add(&AT2(gmu, j, i), 0.5f * lv); atomic_add(&AT2(gmu, j - 1, i), 0.5f * lv); wv = true; }
+    else if (j == jts) { atomic_add(&AT2(gmu, jts, i), lv); wv = true; }
+  }
+  AT2(gmuu, j, i) = wu ? 0.0f : lu;
+  AT2(gmuv, j, i) = wv ? 0.0f : lv;
+}
+
+__global__ void k_copy_n(const float* src, float* dst, long n) {
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) dst[t] = src[t];
+}
+
+extern "C" int pywrf_calc_mu_uv_1_adjoint(const pywrf_grid_t* g, const float* lam_muu, const float* lam_muv,
+                                          float* lam_mu, float* lam_muu_in, float* lam_muv_in, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
+  CHECK(g, 1);
+  const int jtf = imin(g->jte, g->jde - 1), itf = imin(g->ite, g->ide - 1);
+  if (g->ite - 2 < 0 || g->jte - 2 < 0) return PYWRF_E_RANGE;
+  const long n2 = (long)g->ny * g->nx;
+  hipLaunchKernelGGL(k_zero_n, dim3(256), dim3(256), 0, (hipStream_t)s, lam_mu, n2);
+  // outside the box muu / muv pass through
+  hipLaunchKernelGGL(k_copy_n, dim3(256), dim3(256), 0, (hipStream_t)s, lam_muu, lam_muu_in, n2);
+  hipLaunchKernelGGL(k_copy_n, dim3(256), dim3(256), 0, (hipStream_t)s, lam_muv, lam_muv_in, n2);
+  Box b = mkbox(0, 1, g->jts, g->jte, g->its, g->ite);
+  PW_LAUNCH_COLS(k_calc_mu_uv_1_adj, b, s, dims_of(g), lam_muu, lam_muv, lam_mu, lam_muu_in, lam_muv_in, g->jts, jtf,
+                 g->jte, g->its, itf, g->ite);
+  PW_RET();
+}
+
+// ------------------------------------------------------------------ small_step_finish^T
+// small_step_finish (wrf_dynamics.py:3863-3978) couples the acoustic
+// variables back on its boxes:
+//   u = (msfuy u + u_save muu) / muus,  v = (msfvx v + v_save muv) / muvs,
+//   w = (msfty w + w_save mut) / muts,  ph = ph + ph_save,  ww = ww + ww1,
+//   t = (t [- dts n mut h_diabatic] + t_save mut) / muts,  mu_2 = mu_2 + mu_save;
+// elsewhere the fields pass.  mut, muu, muv, the map factors and h_diabatic
+// are held fixed.  One thread per column; the sensitivities of muus, muvs and
+// muts are summed down it (muts by division: the results are inputs).
+struct FinAdj {
+  const float *lu, *lv, *lw, *lph, *lww, *lt, *lmu;                                // results
+  const float *uo, *vo, *wo, *to;                                                  // forward results
+  const float *muu, *muus, *muv, *muvs, *mut, *muts, *msfuy, *msfvx, *msfty;
+  float *gu, *gv, *gw, *gph, *gww, *gt, *gus, *gvs, *gws, *gps, *gww1, *gts;
+  float *gmuus, *gmuvs, *gmuts, *gmu2, *gmusave;
+  int kde, j0, j1, i0, i1, jte, ite;
+};
+
+__global__ void k_ssfinish_adj(Box all, Dims D, FinAdj a) {
+  int j, i;
+  if (!box_column(all, j, i)) return;
+  const bool h = j >= a.j0 && j < a.j1 && i >= a.i0 && i < a.i1;
+  const bool vr = j >= a.j0 && j < a.jte && i >= a.i0 && i < a.i1;
+  const bool ur = j >= a.j0 && j < a.j1 && i >= a.i0 && i < a.ite;
+  const float muus = AT2(a.muus, j, i), muvs = AT2(a.muvs, j, i), muts = AT2(a.muts, j, i);
+  const float muu = AT2(a.muu, j, i), muv = AT2(a.muv, j, i), mut = AT2(a.mut, j, i);
+  const float my = AT2(a.msfuy, j, i), mx = AT2(a.msfvx, j, i), mty = AT2(a.msfty, j, i);
+  float su = 0.0f, sv = 0.0f, st = 0.0f;
+  for (int k = 0; k < D.nz; ++k) {
+    const long q = D.ix(k, j, i);
+    const float lv = a.lv[q], lu = a.lu[q], lw = a.lw[q], lph = a.lph[q], lww = a.lww[q], lt = a.lt[q];
+    if (k < a.kde - 1 && vr) {
+      a.gv[q] = lv * mx / muvs;
+      a.gvs[q] = lv * muv / muvs;
+      sv -= lv * a.vo[q] / muvs;
+    } else {
+      a.gv[q] = lv;
+      a.gvs[q] = 0.0f;
+    }
+    if (k < a.kde - 1 && ur) {
+      a.gu[q] = lu * my / muus;
+      a.gus[q] = lu * muu / muus;
+      su -= lu * a.uo[q] / muus;
+    } else {
+      a.gu[q] = lu;
+      a.gus[q] = 0.0f;
+    }
+    if (k < a.kde && h) {
+      a.gw[q] = lw * mty / muts;
+      a.gws[q] = lw * mut / muts;
+      st -= lw * a.wo[q] / muts;
+      a.gps[q] = lph;
+      a.gww1[q] = lww;
+    } else {
+      a.gw[q] = lw;
+      a.gws[q] = 0.0f;
+      a.gps[q] = 0.0f;
+      a.gww1[q] = 0.0f;
+    }
+    a.gph[q] = lph;
+    a.gww[q] = lww;
+    if (k < a.kde - 1 && h) {
+      a.gt[q] = lt / muts;
+      a.gts[q] = lt * mut / muts;
+      st -= lt * a.to[q] / muts;
+    } else {
+      a.gt[q] = lt;
+      a.gts[q] = 0.0f;
+    }
+  }
+  AT2(a.gmuus, j, i) = su;
+  AT2(a.gmuvs, j, i) = sv;
+  AT2(a.gmuts, j, i) = st;
+  const float lmu = AT2(a.lmu, j, i);
+  AT2(a.gmu2, j, i) = lmu;
+  AT2(a.gmusave, j, i) = h ? lmu : 0.0f;
+}
+
+extern "C" int pywrf_small_step_finish_adjoint(
+    const pywrf_grid_t* g, const float* lam_u, const float* lam_v, const float* lam_w, const float* lam_ph,
+    const float* lam_ww, const float* lam_t, const float* lam_mu_2, const float* u_out, const float* v_out,
+    const float* w_out, const float* t_out, const float* muu, const float* muus, const float* muv,
+    const float* muvs, const float* mut, const float* muts, const float* msfuy, const float* msfvx,
+    const float* msfty, float* lam_u_2, float* lam_v_2, float* lam_w_2, float* lam_ph_2, float* lam_ww_in,
+    float* lam_t_2, float* lam_u_save, float* lam_v_save, float* lam_w_save, float* lam_ph_save, float* lam_ww1,
+    float* lam_t_save, float* lam_muus, float* lam_muvs, float* lam_muts, float* lam_mu_2_in, float* lam_mu_save,
+    pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
+  CHECK(g, 0);
+  if (g->kde > g->nz || g->kds != 0) return PYWRF_E_RANGE;
+  FinAdj a = {lam_u, lam_v, lam_w, lam_ph, lam_ww, lam_t, lam_mu_2, u_out, v_out, w_out, t_out, muu, muus, muv,
+              muvs, mut, muts, msfuy, msfvx, msfty, lam_u_2, lam_v_2, lam_w_2, lam_ph_2, lam_ww_in, lam_t_2,
+              lam_u_save, lam_v_save, lam_w_save, lam_ph_save, lam_ww1, lam_t_save, lam_muus, lam_muvs, lam_muts,
+              lam_mu_2_in, lam_mu_save, g->kde, g->jts, imin(g->jte, g->jde - 1), g->its, imin(g->ite, g->ide - 1),
+              g->jte, g->ite};
+  Box pl = mkbox(0, 1, 0, g->ny, 0, g->nx);
+  PW_LAUNCH_COLS(k_ssfinish_adj, pl, s, dims_of(g), a);
+  PW_RET();
+}

@@ -721,3 +721,130 @@ def test_acoustic_loop_checkpoint_equals_the_stored_chain():
         assert rel <= 1e-5, n
     assert held_ck <= 2.1 * fields
     assert held_kept > 2.0 * held_ck
+
+
+def _oracle_substep(ro, st, sv, calls):
+    """One acoustic substep through the oracle (FP64 operands), the recorded
+    tendencies held, the carried fields st and the saved fields sv given."""
+    uv, z1, mt, z2, wa, sf, pz, cp, bc = calls
+
+    def ozone(call, fields):
+        items, dt, ix24, ix2d = call
+        return [ro.spec_bdyupdate(f.clone(), it[1], dt, it[2], 1, *(ix2d if it[3] else ix24))
+                for f, it in zip(fields, items)]
+    a = _clone(list(uv))
+    a[0], a[2], a[4], a[6], a[9], a[10], a[15] = (st["u"], st["v"], st["p"], st["ph"], st["al"], st["mu"],
+                                                  st["mudf"])
+    u2, v2 = ozone(z1, ro.advance_uv(*[_clone(q) for q in a]))
+    m = _clone(list(mt))
+    m[0], m[2], m[4], m[6], m[8], m[9], m[12], m[24], m[26] = (
+        st["ww"], u2, v2, st["mu"], st["muave"], st["muts"], st["mudf"], st["t"], st["t_ave"])
+    m[1], m[3], m[5], m[25] = sv["ww1"], sv["u_save"], sv["v_save"], sv["t_save"]
+    r = ro.advance_mu_t(*[_clone(q) for q in m])
+    t2, mu2, muts2 = ozone(z2, [r[2], r[10], r[8]])
+    w_ = _clone(list(wa))
+    w_[0], w_[2], w_[3], w_[4], w_[5], w_[8], w_[9], w_[18], w_[19], w_[20], w_[21], w_[22] = (
+        st["w"].clone(), r[0], sv["w_save"], u2, v2, r[7], muts2, r[3].clone(), t2, sv["t_save"], st["ph"].clone(),
+        sv["ph_save"])
+    t2a, wo, pho = ro.advance_w(*w_)
+    f = _clone(list(sf))
+    f[0], f[1], f[2], f[3], f[4], f[5] = u2, v2, r[0], sv["u_save"], sv["v_save"], sv["ww1"]
+    f[16], f[17], f[18] = st["ru_m"].clone(), st["rv_m"].clone(), st["ww_m"].clone()
+    ru_m, rv_m, ww_m = ro.sumflux(*f)
+    pa, za = list(_clone(pz[0])), list(_clone(pz[1]))
+    pa[0], pa[1], pa[4] = sv["ph_save"], pho.clone(), muts2
+    ph_s = ro.spec_bdyupdate_ph(*pa)
+    za[0] = wo.clone()
+    w_z = ro.zero_grad_bdy(*za)
+    c = _clone(list(cp))
+    c[0], c[1], c[2], c[4], c[5], c[7], c[8], c[9] = (st["al"], st["p"], ph_s, t2, sv["t_save"], st["pm1"].clone(),
+                                                      mu2, muts2)
+    al, p, pm1 = ro.calc_p_rho(*c)
+    items, ix24, ix2d = bc
+    filled = [ro.set_physical_bc2d(q.clone(), var, *ix2d) if two else ro.set_physical_bc3d(q.clone(), var, *ix24)
+              for q, (_, var, two) in zip((ph_s, al, p, muts2, mu2, r[9]), items)]
+    ph_f, al_f, p_f, muts_f, mu_f, mudf_f = filled
+    return {"u": u2, "v": v2, "p": p_f, "ph": ph_f, "al": al_f, "mu": mu_f, "mudf": mudf_f, "ww": r[0],
+            "muave": r[7], "muts": muts_f, "t": t2, "t_ave": t2a, "w": w_z, "pm1": pm1, "ru_m": ru_m, "rv_m": rv_m,
+            "ww_m": ww_m}
+
+
+@pytest.mark.parametrize("config", ["config1c", "bdy24"])
+def test_acoustic_stage_adjoint_matches_fp64_differences(config):
+    """The gradient of the second RK stage's whole acoustic part --
+    small_step_prep, calc_p_rho at step 0, the open-boundary fills, the two
+    acoustic substeps, calc_mu_uv_1 and small_step_finish -- from the
+    hand-written adjoints chained by autograd, with respect to the stage's
+    19 inputs (u_2, v_2, w_2, t_2, ph_2, mu_2, u_1 .. mu_1, ww, mudf, muave,
+    t_ave and the flux accumulators), against FP64 central differences of the
+    same sequence through the oracle, over the 17 results; bound 1e-4 as in
+    the loop's test."""
+    import oracle
+    import pywrf_amd
+    from pywrf_amd import adjoint
+    from pywrf_amd.adjoint import ACOUSTIC_STATE, STAGE_IN, STAGE_OUT
+    from pywrf_amd.gen import synthetic_state
+    from pywrf_amd.step import RunConfig
+    s = synthetic_state(config, device="cuda")
+    x0, calls = adjoint.record_acoustic_stage(s, pywrf_amd.ops(), RunConfig(), stage=2)
+    x = {n: f.clone().requires_grad_() for n, f in zip(STAGE_IN, x0)}
+    out = dict(zip(STAGE_OUT, adjoint.acoustic_stage([x[n] for n in STAGE_IN], calls)))
+    gen = torch.Generator().manual_seed(29)
+    lams = {n: torch.randn(tuple(out[n].shape), generator=gen, dtype=torch.float64) for n in STAGE_OUT}
+    sum((out[n] * lams[n].float().cuda()).sum() for n in STAGE_OUT).backward()
+
+    ro = oracle.ops()
+    c64 = [_double(c) for c in calls]
+    pr, cp, _, bc = c64[:4]
+    subs = [c64[4 + 9 * q:4 + 9 * q + 9] for q in range((len(c64) - 6) // 9)]
+    mc, fin = c64[-2], c64[-1]
+
+    def stage(v):
+        a = _clone(list(pr))
+        for n, pos in (("u_1", 0), ("u_2", 1), ("v_1", 2), ("v_2", 3), ("w_1", 4), ("w_2", 5), ("t_1", 6),
+                       ("t_2", 7), ("ph_1", 8), ("ph_2", 9), ("mu_1", 11), ("mu_2", 12), ("ww", 34)):
+            a[pos] = v[n].clone()
+        r = ro.small_step_prep(*a)
+        us, vs, ws, ts, ps, u, vv, w, t, ph, ww1, mu, muus, muvs, muts, musv = (
+            r[5], r[6], r[7], r[8], r[9], r[10], r[11], r[12], r[13], r[14], r[16], r[18], r[19], r[20], r[21], r[23])
+        c = _clone(list(cp))
+        c[2], c[4], c[5], c[8], c[9] = ph, t, ts, mu, muts
+        al, p, pm1 = ro.calc_p_rho(*c)
+        items, ix24, ix2d = bc
+        src = (items[0][0], items[1][0], ph, al, p, v["t_1"], ts, v["mu_1"], mu, v["mudf"])
+        filled = [ro.set_physical_bc2d(q.clone(), var, *ix2d) if two else ro.set_physical_bc3d(q.clone(), var, *ix24)
+                  for q, (_, var, two) in zip(src, items)]
+        _, _, ph, al, p, _, ts, _, mu, mudf = filled
+        st = {"u": u, "v": vv, "p": p, "ph": ph, "al": al, "mu": mu, "mudf": mudf, "ww": v["ww"].clone(),
+              "muave": v["muave"], "muts": muts, "t": t, "t_ave": v["t_ave"], "w": w, "pm1": pm1,
+              "ru_m": v["ru_m"], "rv_m": v["rv_m"], "ww_m": v["ww_m"]}
+        sv = {"u_save": us, "v_save": vs, "w_save": ws, "t_save": ts, "ph_save": ps, "ww1": ww1}
+        for cs in subs:
+            st = _oracle_substep(ro, st, sv, cs)
+        m = list(mc)
+        muus, muvs = ro.calc_mu_uv_1(st["muts"], muus.clone(), muvs.clone(), *m[3:])
+        f = _clone(list(fin))
+        for n, pos in (("u", 0), ("v", 2), ("w", 4), ("t", 6), ("ph", 8), ("ww", 10), ("mu", 12), ("muts", 15)):
+            f[pos] = st[n].clone()
+        f[11], f[17], f[19], f[28], f[29], f[30], f[31], f[32], f[33] = ww1, muus, muvs, us, vs, ws, ts, ps, musv
+        uo, vo, wo, pho, wwo, to, muo = ro.small_step_finish(*f)
+        res = {"u": uo, "v": vo, "w": wo, "ph": pho, "ww": wwo, "t": to, "mu": muo}
+        for n in ("p", "al", "pm1", "mudf", "muave", "muts", "t_ave", "ru_m", "rv_m", "ww_m"):
+            res[n] = st[n]
+        return sum(float((res[n] * lams[n]).sum()) for n in STAGE_OUT)
+
+    base = {n: x[n].detach().double().cpu() for n in STAGE_IN}
+    worst = 0.0
+    for n in STAGE_IN:
+        assert x[n].grad is not None, n
+        d = torch.randn(tuple(base[n].shape), generator=gen, dtype=torch.float64)
+        eps = 1e-4 * float(base[n].abs().max() + 1.0) / float(d.abs().max())
+        vp, vm = dict(base), dict(base)
+        vp[n], vm[n] = base[n] + eps * d, base[n] - eps * d
+        ref = (stage(vp) - stage(vm)) / (2.0 * eps)
+        terms = x[n].grad.double().cpu() * d
+        got = float(terms.sum())
+        err = abs(got - ref) / max(abs(ref), float(terms.norm()), 1e-300)
+        worst = max(worst, err)
+        print("d/d%s: <grad, d> %.6e  difference %.6e  rel %.2e" % (n, got, ref, err), flush=True)
+    assert worst <= 1e-4
