@@ -27,6 +27,10 @@ transpose of a linear part of the step:
   (164-278), each on copies of its in-place operands.  With the three above
   and the zone updates (fixed increments) they chain into the gradient of the
   whole acoustic loop.
+* SmallStepPrep, CalcMuUV1, SmallStepFinish -- small_step_prep past the
+  first stage (wrf_dynamics.py:3152-3266), calc_mu_uv_1 (3844-3860) and
+  small_step_finish (3863-3978); with the loop they make acoustic_stage, an
+  RK stage's whole acoustic part (record_acoustic_stage records it).
 * TridiagSolve -- the column tridiagonal solve of advance_w
   (wrf_dynamics.py:3758-3765) as a function of its coefficients a, alpha,
   gamma and right-hand side.  Backward: pywrf_tridiag_solve_adjoint

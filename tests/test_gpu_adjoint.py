@@ -769,9 +769,10 @@ def _oracle_substep(ro, st, sv, calls):
             "ww_m": ww_m}
 
 
-@pytest.mark.parametrize("config", ["config1c", "bdy24"])
-def test_acoustic_stage_adjoint_matches_fp64_differences(config):
-    """The gradient of the second RK stage's whole acoustic part --
+@pytest.mark.parametrize("config,stage", [("config1c", 2), ("bdy24", 2), ("config1c", 3), ("config4", 2)])
+def test_acoustic_stage_adjoint_matches_fp64_differences(config, stage):
+    """The gradient of the second (or third: four substeps, the diabatic term
+    of the last stage) RK stage's whole acoustic part --
     small_step_prep, calc_p_rho at step 0, the open-boundary fills, the two
     acoustic substeps, calc_mu_uv_1 and small_step_finish -- from the
     hand-written adjoints chained by autograd, with respect to the stage's
@@ -786,7 +787,7 @@ def test_acoustic_stage_adjoint_matches_fp64_differences(config):
     from pywrf_amd.gen import synthetic_state
     from pywrf_amd.step import RunConfig
     s = synthetic_state(config, device="cuda")
-    x0, calls = adjoint.record_acoustic_stage(s, pywrf_amd.ops(), RunConfig(), stage=2)
+    x0, calls = adjoint.record_acoustic_stage(s, pywrf_amd.ops(), RunConfig(), stage=stage)
     x = {n: f.clone().requires_grad_() for n, f in zip(STAGE_IN, x0)}
     out = dict(zip(STAGE_OUT, adjoint.acoustic_stage([x[n] for n in STAGE_IN], calls)))
     gen = torch.Generator().manual_seed(29)
