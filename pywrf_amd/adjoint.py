@@ -816,6 +816,13 @@ def acoustic_stage(x, calls):
     return [out[n] for n in STAGE_OUT]
 
 
+def acoustic_substeps(x, substeps):
+    """The substeps of a recorded loop in turn (acoustic_substep)."""
+    for calls in substeps:
+        x = acoustic_substep(x, calls)
+    return x
+
+
 class AcousticLoop(torch.autograd.Function):
     """A stage's whole acoustic loop, x -> the 17 carried fields after its
     last substep, with one checkpoint: the forward keeps only the loop's
@@ -985,12 +992,43 @@ class SurrogateNet(torch.autograd.Function):
         return dX[:, :_sg.FIN].clone(), None
 
 
-def main(argv=None):
-    """Time the acoustic loop's gradient on a seeded state: the loop forward,
-    the checkpointed forward + backward (AcousticLoop) and the same with every
-    operand kept, with the device memory each holds.
+class AcousticStage(torch.autograd.Function):
+    """An RK stage's whole acoustic part (acoustic_stage) with one
+    checkpoint: the forward keeps only the stage's 19 inputs (STAGE_IN), the
+    backward runs the stage again from them and returns the gradient through
+    the hand-written adjoints.  calls: from record_acoustic_stage.
 
-        python -m pywrf_amd.adjoint --config config2 --stage 3 --repeat 5
+        x0, calls = adjoint.record_acoustic_stage(state, ops, cfg, stage=2)
+        out = adjoint.AcousticStage.apply(calls, *[f.clone().requires_grad_() for f in x0])
+    """
+
+    @staticmethod
+    def forward(ctx, calls, *x):
+        if len(x) != len(STAGE_IN):
+            raise ValueError("AcousticStage: expected the %d stage inputs" % len(STAGE_IN))
+        ctx.calls = calls
+        ctx.save_for_backward(*x)
+        with torch.no_grad():
+            y = acoustic_stage(list(x), calls)
+        return tuple(y)
+
+    @staticmethod
+    def backward(ctx, *lams):
+        x = [t.detach().requires_grad_() for t in ctx.saved_tensors]
+        with torch.enable_grad():
+            y = acoustic_stage(x, ctx.calls)
+            pairs = [(o, l) for o, l in zip(y, lams) if l is not None]
+            grads = torch.autograd.grad([o for o, _ in pairs], x, [l for _, l in pairs], allow_unused=True)
+        return (None, *grads)
+
+
+def main(argv=None):
+    """Time the gradient of an RK stage's acoustic loop (--what loop) or of its
+    whole acoustic part (--what stage) on a seeded state: the forward, the
+    checkpointed forward + backward (AcousticLoop / AcousticStage) and the
+    same with every operand kept, with the device memory each holds.
+
+        python -m pywrf_amd.adjoint --config config2 --stage 3 --repeat 5 [--what stage]
     """
     import argparse
     import json
@@ -1002,11 +1040,18 @@ def main(argv=None):
     ap.add_argument("--config", default="config2")
     ap.add_argument("--stage", type=int, default=3)
     ap.add_argument("--repeat", type=int, default=5)
+    ap.add_argument("--what", choices=("loop", "stage"), default="loop")
     a = ap.parse_args(argv)
     s = synthetic_state(a.config, device="cuda")
-    x0, subs = record_acoustic_loop(s, ops(), RunConfig(), stage=a.stage)
+    if a.what == "loop":
+        x0, subs = record_acoustic_loop(s, ops(), RunConfig(), stage=a.stage)
+        chain, ckpt = (lambda x: acoustic_substeps(x, subs)), (lambda x: AcousticLoop.apply(subs, *x))
+    else:
+        x0, subs = record_acoustic_stage(s, ops(), RunConfig(), stage=a.stage)
+        chain, ckpt = (lambda x: acoustic_stage(x, subs)), (lambda x: AcousticStage.apply(subs, *x))
     del s
-    lams = [torch.randn_like(f) for f in x0]
+    with torch.no_grad():
+        lams = [torch.randn_like(o) for o in chain(list(x0))]
     fields = sum(f.numel() * f.element_size() for f in x0)
 
     def timed(fn):
@@ -1020,9 +1065,7 @@ def main(argv=None):
 
     def fwd():
         with torch.no_grad():
-            y = list(x0)
-            for calls in subs:
-                y = acoustic_substep(y, calls)
+            chain(list(x0))
 
     held = {}
 
@@ -1032,12 +1075,7 @@ def main(argv=None):
             base = torch.cuda.memory_allocated()
             torch.cuda.reset_peak_memory_stats()
             x = [f.clone().requires_grad_() for f in x0]
-            if kept:
-                y = x
-                for calls in subs:
-                    y = acoustic_substep(y, calls)
-            else:
-                y = AcousticLoop.apply(subs, *x)
+            y = chain(x) if kept else ckpt(x)
             torch.cuda.synchronize()
             held["kept" if kept else "checkpoint"] = torch.cuda.memory_allocated() - base
             torch.autograd.backward(y, lams)
@@ -1045,7 +1083,7 @@ def main(argv=None):
             held[("kept" if kept else "checkpoint") + "_peak"] = torch.cuda.max_memory_allocated() - base
         return run
 
-    r = {"config": a.config, "stage": a.stage, "substeps": len(subs), "fields_MB": fields / 1e6,
+    r = {"what": a.what, "config": a.config, "stage": a.stage, "fields_MB": fields / 1e6,
          "forward_ms": 1e3 * timed(fwd), "checkpoint_grad_ms": 1e3 * timed(grad(False)),
          "kept_grad_ms": 1e3 * timed(grad(True))}
     r.update({k + "_MB": v / 1e6 for k, v in held.items()})

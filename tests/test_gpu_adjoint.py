@@ -775,7 +775,8 @@ def test_acoustic_stage_adjoint_matches_fp64_differences(config, stage):
     of the last stage) RK stage's whole acoustic part --
     small_step_prep, calc_p_rho at step 0, the open-boundary fills, the two
     acoustic substeps, calc_mu_uv_1 and small_step_finish -- from the
-    hand-written adjoints chained by autograd, with respect to the stage's
+    hand-written adjoints chained by autograd (AcousticStage: one checkpoint,
+    the stage run again in its backward), with respect to the stage's
     19 inputs (u_2, v_2, w_2, t_2, ph_2, mu_2, u_1 .. mu_1, ww, mudf, muave,
     t_ave and the flux accumulators), against FP64 central differences of the
     same sequence through the oracle, over the 17 results; bound 1e-4 as in
@@ -789,7 +790,7 @@ def test_acoustic_stage_adjoint_matches_fp64_differences(config, stage):
     s = synthetic_state(config, device="cuda")
     x0, calls = adjoint.record_acoustic_stage(s, pywrf_amd.ops(), RunConfig(), stage=stage)
     x = {n: f.clone().requires_grad_() for n, f in zip(STAGE_IN, x0)}
-    out = dict(zip(STAGE_OUT, adjoint.acoustic_stage([x[n] for n in STAGE_IN], calls)))
+    out = dict(zip(STAGE_OUT, adjoint.AcousticStage.apply(calls, *[x[n] for n in STAGE_IN])))
     gen = torch.Generator().manual_seed(29)
     lams = {n: torch.randn(tuple(out[n].shape), generator=gen, dtype=torch.float64) for n in STAGE_OUT}
     sum((out[n] * lams[n].float().cuda()).sum() for n in STAGE_OUT).backward()
