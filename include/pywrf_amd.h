@@ -828,6 +828,13 @@ int pywrf_small_step_finish_adjoint(
     float* lam_t_2, float* lam_u_save, float* lam_v_save, float* lam_w_save, float* lam_ph_save, float* lam_ww1,
     float* lam_t_save, float* lam_muus, float* lam_muvs, float* lam_muts, float* lam_mu_2_in, float* lam_mu_save,
     pywrf_stream_t s);
+/* The adjoint of advect_u (wrf_dynamics.py:1112-1307) with respect to u,
+ * the mass fluxes held fixed (the edge x-faces' upwind choice by the sign of
+ * u taken as in the forward with this u): lam_tendency, the sensitivity of
+ * the tendency advect_u adds, gives lam_u (overwritten). */
+int pywrf_advect_u_adjoint(const pywrf_grid_t* g, const float* lam_tendency, const float* u, const float* ru,
+                           const float* rv, const float* rom, const float* msfux, const float* fzm, const float* fzp,
+                           double rdx, double rdy, const float* rdzw, float* lam_u, pywrf_stream_t s);
 /* The column tridiagonal solve of advance_w (wrf_dynamics.py:3758-3765), in
  * place on w over advance_w's columns, levels 1..kte-1 (level 0 is the lower
  * boundary value): the same operations in the same order. */

@@ -1317,3 +1317,105 @@ extern "C" int pywrf_small_step_finish_adjoint(
   PW_LAUNCH_COLS(k_ssfinish_adj, pl, s, dims_of(g), a);
   PW_RET();
 }
+
+// ------------------------------------------------------------------ advect_u^T
+// advect_u (wrf_dynamics.py:1112-1307) with the mass fluxes fixed is linear in
+// u except at the two x-faces next to the edges, whose upwind value is chosen
+// by the sign of u itself (piecewise linear: the forward's branch is taken).
+// Scatter form: each tendency point hands its sensitivity, through the
+// coefficients of its six faces' rules, to the u points those faces read
+// (atomic adds onto the zeroed output).
+struct AdjAdvU {
+  const float *lam, *u, *ru, *rv, *rom, *msfux, *fzm, *fzp, *rdzw;
+  float* gu;
+  float rdx, rdy;
+  int ids, ide, jds, jde, ktf;
+  Box b, by;
+};
+
+// the y-face jf of u column (k, i): weight w times d(face)/d(u at row jf + o)
+__device__ inline void u_yface_adj(const AdjAdvU& a, const Dims& D, int k, int jf, int i, float rvs, float w) {
+  auto put = [&](int o, float c) { atomic_add(&AT(a.gu, k, jf + o, i), w * c); };
+  if (jf >= a.jds + 3 && jf < a.jde - 3) {
+    const float vel = 0.5f * rvs, s = sgn1(vel);
+    for (int o = -3; o <= 2; ++o) put(o, vel * c5(o, s));
+  } else if (jf == a.jds + 1 || jf == a.jde - 2) {
+    put(0, 0.25f * rvs);
+    put(-1, 0.25f * rvs);
+  } else if (jf == a.jds + 2 || jf == a.jde - 3) {
+    const float vel = 0.5f * rvs, s = jf == a.jds + 2 ? sgn1(vel) : sgn1(0.0f);
+    for (int o = -2; o <= 1; ++o) put(o, vel * c3(o, s));
+  }
+}
+
+__device__ inline void u_xface_adj(const AdjAdvU& a, const Dims& D, int k, int j, int f, float rus, float w) {
+  auto put = [&](int o, float c) { atomic_add(&AT(a.gu, k, j, f + o), w * c); };
+  if (f >= a.ids + 3 && f < a.ide - 2) {
+    const float vel = 0.5f * rus, s = sgn1(vel);
+    for (int o = -3; o <= 2; ++o) put(o, vel * c5(o, s));
+  } else if (f == a.ids + 1) {
+    if (AT(a.u, k, j, f) < 0.0f) put(0, 0.5f * rus);
+    else { put(0, 0.25f * rus); put(-1, 0.25f * rus); }
+  } else if (f == a.ids + 2 || f == a.ide - 2) {
+    const float vel = 0.5f * rus, s = sgn1(vel);
+    for (int o = -2; o <= 1; ++o) put(o, vel * c3(o, s));
+  } else if (f == a.ide - 1) {
+    if (AT(a.u, k, j, f - 1) > 0.0f) put(-1, 0.5f * rus);
+    else { put(-1, 0.25f * rus); put(0, 0.25f * rus); }
+  }
+}
+
+__device__ inline void u_zface_adj(const AdjAdvU& a, const Dims& D, int kf, int j, int i, float vel, float w) {
+  if (kf < 1 || kf >= a.ktf) return;
+  auto put = [&](int o, float c) { atomic_add(&AT(a.gu, kf + o, j, i), w * c); };
+  if (kf >= 3 && kf < a.ktf - 2) {
+    const float s = sgn1(-vel);
+    for (int o = -3; o <= 2; ++o) put(o, vel * c5(o, s));
+  } else if (kf == 1 || kf == a.ktf - 1) {
+    put(0, vel * a.fzm[kf]);
+    put(-1, vel * a.fzp[kf]);
+  } else {
+    const float s = sgn1(-vel);
+    for (int o = -2; o <= 1; ++o) put(o, vel * c3(o, s));
+  }
+}
+
+__global__ void k_advect_u_adj(Box all, Dims D, AdjAdvU a) {
+  int k, j, i;
+  if (!box_point(all, k, j, i)) return;
+  const float lam = AT(a.lam, k, j, i);
+  if (lam == 0.0f) return;
+  const float msf = AT2(a.msfux, j, i), rdz = a.rdzw[k];
+  if (a.by.has(k, j, i)) {
+    const float wy = msf * a.rdy * lam;
+    u_yface_adj(a, D, k, j + 1, i, AT(a.rv, k, j + 1, i) + AT(a.rv, k, j + 1, i - 1), -wy);
+    u_yface_adj(a, D, k, j, i, AT(a.rv, k, j, i) + AT(a.rv, k, j, i - 1), wy);
+  }
+  const float wx = msf * a.rdx * lam;
+  u_xface_adj(a, D, k, j, i + 1, AT(a.ru, k, j, i + 1) + AT(a.ru, k, j, i), -wx);
+  u_xface_adj(a, D, k, j, i, AT(a.ru, k, j, i) + AT(a.ru, k, j, i - 1), wx);
+  const float wz = rdz * lam;
+  u_zface_adj(a, D, k + 1, j, i, 0.5f * (AT(a.rom, k + 1, j, i - 1) + AT(a.rom, k + 1, j, i)), -wz);
+  u_zface_adj(a, D, k, j, i, 0.5f * (AT(a.rom, k, j, i - 1) + AT(a.rom, k, j, i)), wz);
+}
+
+extern "C" int pywrf_advect_u_adjoint(const pywrf_grid_t* g, const float* lam_tendency, const float* u,
+                                      const float* ru, const float* rv, const float* rom, const float* msfux,
+                                      const float* fzm, const float* fzp, double rdx, double rdy,
+                                      const float* rdzw, float* lam_u, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
+  CHECK(g, 4);
+  const int ktf = imin(g->kte, g->kde - 1);
+  if (g->kts != 0 || ktf + 1 > g->nz) return PYWRF_E_RANGE;
+  const int i0 = imax(g->ids + 1, g->its), i1 = imin(g->ide - 1, g->ite);
+  const int j0 = g->jts, j1 = imin(g->jte, g->jde - 1);
+  AdjAdvU a;
+  a.lam = lam_tendency; a.u = u; a.ru = ru; a.rv = rv; a.rom = rom; a.msfux = msfux; a.fzm = fzm; a.fzp = fzp;
+  a.rdzw = rdzw; a.gu = lam_u; a.rdx = F(rdx); a.rdy = F(rdy);
+  a.ids = g->ids; a.ide = g->ide; a.jds = g->jds; a.jde = g->jde; a.ktf = ktf;
+  a.b = mkbox(g->kts, ktf, j0, j1, i0, i1);
+  a.by = mkbox(g->kts, ktf, imax(g->jts, g->jds + 1), imin(g->jte, g->jde - 2), i0, i1);
+  hipLaunchKernelGGL(k_zero_n, dim3(1024), dim3(256), 0, (hipStream_t)s, lam_u, (long)g->nz * g->ny * g->nx);
+  PW_LAUNCH_BOX(k_advect_u_adj, a.b, s, dims_of(g), a);
+  PW_RET();
+}

@@ -850,3 +850,54 @@ def test_acoustic_stage_adjoint_matches_fp64_differences(config, stage):
         worst = max(worst, err)
         print("d/d%s: <grad, d> %.6e  difference %.6e  rel %.2e" % (n, got, ref, err), flush=True)
     assert worst <= 1e-4
+
+
+@pytest.mark.parametrize("config", ["config1c", "bdy24", "config4"])
+def test_advect_u_adjoint_matches_oracle_differences(config):
+    """advect_u's gradient with respect to u (the mass fluxes of the step's
+    first call held fixed), A^T lam by the scatter-form adjoint kernel,
+    against FP64 central differences of the oracle's advect_u (whose
+    in-place face updates autograd cannot follow) along three seeded
+    directions, relative to the larger of the difference and the 2-norm of
+    grad_i d_i; bound 1e-5.  The map is linear in u but for the edge
+    x-faces' upwind choice, which the small steps do not flip."""
+    import oracle
+    import pywrf_amd
+    from pywrf_amd import adjoint
+    from pywrf_amd.gen import synthetic_state
+    from pywrf_amd.step import rk3_step, RunConfig
+    s = synthetic_state(config, device="cuda")
+    name = "advect_u_fresh" if hasattr(pywrf_amd.ops(), "advect_u_fresh") else "advect_u"
+    rec = _FirstCall(pywrf_amd.ops(), name)
+    rk3_step(s, rec, RunConfig())
+    args = rec.args
+    x = args[0].clone().requires_grad_()
+    T = adjoint.AdvectU.apply(x, args)
+    gen = torch.Generator().manual_seed(31)
+    lam = torch.randn(tuple(T.shape), generator=gen, dtype=torch.float64)
+    (T * lam.float().cuda()).sum().backward()
+    a64 = [a.detach().double().cpu() if isinstance(a, torch.Tensor) else a for a in args]
+    u0 = a64[0]
+
+    def A(u):
+        b = list(a64)
+        b[0], b[1], b[2] = u, u, torch.zeros_like(a64[2])
+        return oracle.ops().advect_u(*b)
+    ferr = _rel(T.detach().double().cpu(), A(u0.clone()))
+    print("advect_u forward vs oracle (FP64): %.2e" % ferr)
+    assert ferr <= 1e-5
+    ids, ide = int(args[-18]), int(args[-17])
+    for n in range(3):
+        d = torch.randn(tuple(u0.shape), generator=gen, dtype=torch.float64)
+        eps = 1e-4 * float(u0.abs().max() + 1.0) / float(d.abs().max())
+        # no step across the upwind switch: the u that decides the edge faces
+        # (columns ids+1 and ide-2) is not moved where it is near zero
+        for c in (ids + 1, ide - 2):
+            near = u0[..., c].abs() < 10.0 * eps * d[..., c].abs()
+            d[..., c][near] = 0.0
+        ref = float(((A(u0 + eps * d) - A(u0 - eps * d)) * lam).sum()) / (2.0 * eps)
+        terms = x.grad.double().cpu() * d
+        got = float(terms.sum())
+        err = abs(got - ref) / max(abs(ref), float(terms.norm()))
+        print("direction %d: <A^T lam, d> %.6e  difference %.6e  rel %.2e" % (n, got, ref, err))
+        assert err <= 1e-5
