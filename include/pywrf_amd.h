@@ -835,6 +835,12 @@ int pywrf_small_step_finish_adjoint(
 int pywrf_advect_u_adjoint(const pywrf_grid_t* g, const float* lam_tendency, const float* u, const float* ru,
                            const float* rv, const float* rom, const float* msfux, const float* fzm, const float* fzp,
                            double rdx, double rdy, const float* rdzw, float* lam_u, pywrf_stream_t s);
+/* The adjoint of advect_v (wrf_dynamics.py:1332-1532) with respect to v, as
+ * pywrf_advect_u_adjoint (the edge y-faces' upwind choice by the sign of v). */
+int pywrf_advect_v_adjoint(const pywrf_grid_t* g, const float* lam_tendency, const float* v, const float* ru,
+                           const float* rv, const float* rom, const float* msfvx, const float* msfvy,
+                           const float* fzm, const float* fzp, double rdx, double rdy, const float* rdzw,
+                           float* lam_v, pywrf_stream_t s);
 /* The column tridiagonal solve of advance_w (wrf_dynamics.py:3758-3765), in
  * place on w over advance_w's columns, levels 1..kte-1 (level 0 is the lower
  * boundary value): the same operations in the same order. */

@@ -1419,3 +1419,102 @@ extern "C" int pywrf_advect_u_adjoint(const pywrf_grid_t* g, const float* lam_te
   PW_LAUNCH_BOX(k_advect_u_adj, a.b, s, dims_of(g), a);
   PW_RET();
 }
+
+// ------------------------------------------------------------------ advect_v^T
+// advect_v (wrf_dynamics.py:1332-1532), as advect_u's transpose with the roles
+// of x and y exchanged: the upwind choice by the sign of v at the edge
+// y-faces, the symmetric rules at the x-faces, the z term scaled by
+// msfvy / msfvx, and the reference's +v sign at the z-face ktf-2.
+struct AdjAdvV {
+  const float *lam, *v, *ru, *rv, *rom, *msfvx, *msfvy, *fzm, *fzp, *rdzw;
+  float* gv;
+  float rdx, rdy;
+  int ids, ide, jds, jde, ktf;
+  Box b, bx;
+};
+
+__device__ inline void v_yface_adj(const AdjAdvV& a, const Dims& D, int k, int f, int i, float rvs, float w) {
+  auto put = [&](int o, float c) { atomic_add(&AT(a.gv, k, f + o, i), w * c); };
+  if (f >= a.jds + 3 && f < a.jde - 2) {
+    const float vel = 0.5f * rvs, s = sgn1(vel);
+    for (int o = -3; o <= 2; ++o) put(o, vel * c5(o, s));
+  } else if (f == a.jds + 1) {
+    if (AT(a.v, k, f, i) < 0.0f) put(0, 0.5f * rvs);
+    else { put(0, 0.25f * rvs); put(-1, 0.25f * rvs); }
+  } else if (f == a.jds + 2 || f == a.jde - 2) {
+    const float vel = 0.5f * rvs, s = sgn1(vel);
+    for (int o = -2; o <= 1; ++o) put(o, vel * c3(o, s));
+  } else if (f == a.jde - 1) {
+    if (AT(a.v, k, f - 1, i) > 0.0f) put(-1, 0.5f * rvs);
+    else { put(-1, 0.25f * rvs); put(0, 0.25f * rvs); }
+  }
+}
+
+__device__ inline void v_xface_adj(const AdjAdvV& a, const Dims& D, int k, int j, int f, float rus, float w) {
+  auto put = [&](int o, float c) { atomic_add(&AT(a.gv, k, j, f + o), w * c); };
+  if (f >= a.ids + 3 && f < a.ide - 3) {
+    const float vel = 0.5f * rus, s = sgn1(vel);
+    for (int o = -3; o <= 2; ++o) put(o, vel * c5(o, s));
+  } else if (f == a.ids + 1 || f == a.ide - 2) {
+    put(0, 0.25f * rus);
+    put(-1, 0.25f * rus);
+  } else if (f == a.ids + 2 || f == a.ide - 3) {
+    const float vel = 0.5f * rus, s = sgn1(vel);
+    for (int o = -2; o <= 1; ++o) put(o, vel * c3(o, s));
+  }
+}
+
+__device__ inline void v_zface_adj(const AdjAdvV& a, const Dims& D, int kf, int j, int i, float vel, float w) {
+  if (kf < 1 || kf >= a.ktf) return;
+  auto put = [&](int o, float c) { atomic_add(&AT(a.gv, kf + o, j, i), w * c); };
+  if (kf >= 3 && kf < a.ktf - 2) {
+    const float s = sgn1(-vel);
+    for (int o = -3; o <= 2; ++o) put(o, vel * c5(o, s));
+  } else if (kf == 1 || kf == a.ktf - 1) {
+    put(0, vel * a.fzm[kf]);
+    put(-1, vel * a.fzp[kf]);
+  } else {
+    const float s = kf == 2 ? sgn1(-vel) : sgn1(vel);
+    for (int o = -2; o <= 1; ++o) put(o, vel * c3(o, s));
+  }
+}
+
+__global__ void k_advect_v_adj(Box all, Dims D, AdjAdvV a) {
+  int k, j, i;
+  if (!box_point(all, k, j, i)) return;
+  const float lam = AT(a.lam, k, j, i);
+  if (lam == 0.0f) return;
+  const float my = AT2(a.msfvy, j, i), mvx = AT2(a.msfvx, j, i), rdz = a.rdzw[k];
+  const float wy = my * a.rdy * lam;
+  v_yface_adj(a, D, k, j + 1, i, AT(a.rv, k, j + 1, i) + AT(a.rv, k, j, i), -wy);
+  v_yface_adj(a, D, k, j, i, AT(a.rv, k, j, i) + AT(a.rv, k, j - 1, i), wy);
+  if (a.bx.has(k, j, i)) {
+    const float wx = my * a.rdx * lam;
+    v_xface_adj(a, D, k, j, i + 1, AT(a.ru, k, j, i + 1) + AT(a.ru, k, j - 1, i + 1), -wx);
+    v_xface_adj(a, D, k, j, i, AT(a.ru, k, j, i) + AT(a.ru, k, j - 1, i), wx);
+  }
+  const float wz = (my / mvx) * rdz * lam;
+  v_zface_adj(a, D, k + 1, j, i, 0.5f * (AT(a.rom, k + 1, j, i) + AT(a.rom, k + 1, j - 1, i)), -wz);
+  v_zface_adj(a, D, k, j, i, 0.5f * (AT(a.rom, k, j, i) + AT(a.rom, k, j - 1, i)), wz);
+}
+
+extern "C" int pywrf_advect_v_adjoint(const pywrf_grid_t* g, const float* lam_tendency, const float* v,
+                                      const float* ru, const float* rv, const float* rom, const float* msfvx,
+                                      const float* msfvy, const float* fzm, const float* fzp, double rdx, double rdy,
+                                      const float* rdzw, float* lam_v, pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
+  CHECK(g, 4);
+  const int ktf = imin(g->kte, g->kde - 1);
+  if (g->kts != 0 || ktf + 1 > g->nz) return PYWRF_E_RANGE;
+  const int i0 = g->its, i1 = imin(g->ite, g->ide - 1);
+  const int j0 = imax(g->jts, g->jds + 1), j1 = imin(g->jte, g->jde - 1);
+  AdjAdvV a;
+  a.lam = lam_tendency; a.v = v; a.ru = ru; a.rv = rv; a.rom = rom; a.msfvx = msfvx; a.msfvy = msfvy;
+  a.fzm = fzm; a.fzp = fzp; a.rdzw = rdzw; a.gv = lam_v; a.rdx = F(rdx); a.rdy = F(rdy);
+  a.ids = g->ids; a.ide = g->ide; a.jds = g->jds; a.jde = g->jde; a.ktf = ktf;
+  a.b = mkbox(g->kts, ktf, j0, j1, i0, i1);
+  a.bx = mkbox(g->kts, ktf, j0, j1, imax(g->ids + 1, g->its), imin(g->ide - 2, g->ite));
+  hipLaunchKernelGGL(k_zero_n, dim3(1024), dim3(256), 0, (hipStream_t)s, lam_v, (long)g->nz * g->ny * g->nx);
+  PW_LAUNCH_BOX(k_advect_v_adj, a.b, s, dims_of(g), a);
+  PW_RET();
+}

@@ -852,10 +852,11 @@ def test_acoustic_stage_adjoint_matches_fp64_differences(config, stage):
     assert worst <= 1e-4
 
 
+@pytest.mark.parametrize("which", ["u", "v"])
 @pytest.mark.parametrize("config", ["config1c", "bdy24", "config4"])
-def test_advect_u_adjoint_matches_oracle_differences(config):
-    """advect_u's gradient with respect to u (the mass fluxes of the step's
-    first call held fixed), A^T lam by the scatter-form adjoint kernel,
+def test_advect_u_adjoint_matches_oracle_differences(config, which):
+    """advect_u's (advect_v's) gradient with respect to u (v) (the mass fluxes
+    of the step's first call held fixed), A^T lam by the scatter-form adjoint kernel,
     against FP64 central differences of the oracle's advect_u (whose
     in-place face updates autograd cannot follow) along three seeded
     directions, relative to the larger of the difference and the 2-norm of
@@ -867,12 +868,12 @@ def test_advect_u_adjoint_matches_oracle_differences(config):
     from pywrf_amd.gen import synthetic_state
     from pywrf_amd.step import rk3_step, RunConfig
     s = synthetic_state(config, device="cuda")
-    name = "advect_u_fresh" if hasattr(pywrf_amd.ops(), "advect_u_fresh") else "advect_u"
+    name = "advect_%s_fresh" % which if hasattr(pywrf_amd.ops(), "advect_u_fresh") else "advect_" + which
     rec = _FirstCall(pywrf_amd.ops(), name)
     rk3_step(s, rec, RunConfig())
     args = rec.args
     x = args[0].clone().requires_grad_()
-    T = adjoint.AdvectU.apply(x, args)
+    T = (adjoint.AdvectU if which == "u" else adjoint.AdvectV).apply(x, args)
     gen = torch.Generator().manual_seed(31)
     lam = torch.randn(tuple(T.shape), generator=gen, dtype=torch.float64)
     (T * lam.float().cuda()).sum().backward()
@@ -882,9 +883,9 @@ def test_advect_u_adjoint_matches_oracle_differences(config):
     def A(u):
         b = list(a64)
         b[0], b[1], b[2] = u, u, torch.zeros_like(a64[2])
-        return oracle.ops().advect_u(*b)
+        return getattr(oracle.ops(), "advect_" + which)(*b)
     ferr = _rel(T.detach().double().cpu(), A(u0.clone()))
-    print("advect_u forward vs oracle (FP64): %.2e" % ferr)
+    print("advect_%s forward vs oracle (FP64): %.2e" % (which, ferr))
     assert ferr <= 1e-5
     ids, ide = int(args[-18]), int(args[-17])
     for n in range(3):
@@ -892,9 +893,15 @@ def test_advect_u_adjoint_matches_oracle_differences(config):
         eps = 1e-4 * float(u0.abs().max() + 1.0) / float(d.abs().max())
         # no step across the upwind switch: the u that decides the edge faces
         # (columns ids+1 and ide-2) is not moved where it is near zero
-        for c in (ids + 1, ide - 2):
-            near = u0[..., c].abs() < 10.0 * eps * d[..., c].abs()
-            d[..., c][near] = 0.0
+        jds, jde = int(args[-16]), int(args[-15])
+        if which == "u":
+            for c in (ids + 1, ide - 2):
+                near = u0[..., c].abs() < 10.0 * eps * d[..., c].abs()
+                d[..., c][near] = 0.0
+        else:
+            for r in (jds + 1, jde - 2):
+                near = u0[..., r, :].abs() < 10.0 * eps * d[..., r, :].abs()
+                d[..., r, :][near] = 0.0
         ref = float(((A(u0 + eps * d) - A(u0 - eps * d)) * lam).sum()) / (2.0 * eps)
         terms = x.grad.double().cpu() * d
         got = float(terms.sum())
