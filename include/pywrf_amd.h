@@ -841,6 +841,11 @@ int pywrf_advect_v_adjoint(const pywrf_grid_t* g, const float* lam_tendency, con
                            const float* rv, const float* rom, const float* msfvx, const float* msfvy,
                            const float* fzm, const float* fzp, double rdx, double rdy, const float* rdzw,
                            float* lam_v, pywrf_stream_t s);
+/* The adjoint of advect_w (wrf_dynamics.py:1535-1838) with respect to w, the
+ * mass fluxes held fixed: lam_tendency gives lam_w (overwritten). */
+int pywrf_advect_w_adjoint(const pywrf_grid_t* g, const float* lam_tendency, const float* ru, const float* rv,
+                           const float* rom, const float* msftx, const float* fzm, const float* fzp, double rdx,
+                           double rdy, const float* rdzu, float* lam_w, pywrf_stream_t s);
 /* The column tridiagonal solve of advance_w (wrf_dynamics.py:3758-3765), in
  * place on w over advance_w's columns, levels 1..kte-1 (level 0 is the lower
  * boundary value): the same operations in the same order. */

@@ -10,7 +10,7 @@ transpose of a linear part of the step:
   the field, the mass fluxes held fixed.  Backward: pywrf_advect_scalar_adjoint.
 * AdvectU -- advect_u (wrf_dynamics.py:1112-1307) as a function of u, the
   mass fluxes held fixed.  Backward: pywrf_advect_u_adjoint (scatter form).
-* AdvectV -- advect_v (1332-1532) likewise.
+* AdvectV, AdvectW -- advect_v (1332-1532) and advect_w (1535-1838) likewise.
 * AdvanceW -- the vertical acoustic step advance_w (wrf_dynamics.py:3641-3796)
   as a function of w, ph, t_2, t_2ave, u, v, muts, muave and ww.  Backward: pywrf_advance_w_adjoint
   (the damping, the transposed column solve, the explicit predictor and the
@@ -160,6 +160,31 @@ class AdvectV(torch.autograd.Function):
         out = torch.empty_like(a[0])
         call("pywrf_advect_v_adjoint", g, ptr(_f32(lam)), ptr(a[0]), ptr(a[3]), ptr(a[4]), ptr(a[5]), ptr(a[12]),
              ptr(a[13]), ptr(a[16]), ptr(a[17]), float(a[18]), float(a[19]), ptr(a[20]), ptr(out), stream())
+        return out, None
+
+
+class AdvectW(torch.autograd.Function):
+    """The tendency advect_w (wrf_dynamics.py:1535-1838) adds to a zeroed
+    array, as a function of w (the mass fluxes held fixed).  args: advect_w's
+    positional arguments in the reference's order.  Backward:
+    pywrf_advect_w_adjoint."""
+
+    @staticmethod
+    def forward(ctx, w, args):
+        from . import wrf_dynamics
+        args = list(args)
+        args[0] = _f32(w)
+        args[2] = torch.empty_like(args[0])
+        ctx.args = args
+        return wrf_dynamics.advect_w_fresh(*args)
+
+    @staticmethod
+    def backward(ctx, lam):
+        a = ctx.args
+        g = grid(a[0].shape, tuple(a[-18:]))
+        out = torch.empty_like(a[0])
+        call("pywrf_advect_w_adjoint", g, ptr(_f32(lam)), ptr(a[3]), ptr(a[4]), ptr(a[5]), ptr(a[14]), ptr(a[16]),
+             ptr(a[17]), float(a[18]), float(a[19]), ptr(a[20]), ptr(out), stream())
         return out, None
 
 

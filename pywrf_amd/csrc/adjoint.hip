@@ -1518,3 +1518,109 @@ extern "C" int pywrf_advect_v_adjoint(const pywrf_grid_t* g, const float* lam_te
   PW_LAUNCH_BOX(k_advect_v_adj, a.b, s, dims_of(g), a);
   PW_RET();
 }
+
+// ------------------------------------------------------------------ advect_w^T
+// advect_w (wrf_dynamics.py:1535-1838) is linear in w with the mass fluxes
+// fixed (the horizontal ones interpolated to w levels, extrapolated at the
+// top; the vertical faces' rules from 2nd order at the ends to 5th inside,
+// with the reference's +w sign at the face ktf-1; at the top level only the
+// face below, doubled).  Scatter form as advect_u's.
+struct AdjAdvW {
+  const float *lam, *ru, *rv, *rom, *msftx, *fzm, *fzp, *rdzu;
+  float* gw;
+  float rdx, rdy;
+  int ids, ide, jds, jde, ktf;
+  Box b, by, bx;
+};
+
+__device__ inline float w_level_flux_a(const AdjAdvW& a, int k, float rA, float rB) {
+  if (k < a.ktf) return a.fzm[k] * rA + a.fzp[k] * rB;
+  return (2.0f - a.fzm[a.ktf - 1]) * rA - a.fzp[a.ktf - 1] * rB;
+}
+
+// the horizontal face f of a row or column [lo, hi): d(face) / d(w at f + o), times w
+template <class Put>
+__device__ inline void w_hface_adj(int f, int lo, int hi, float vel, float w, Put put) {
+  if (f >= lo + 3 && f < hi - 3) {
+    const float s = sgn1(vel);
+    for (int o = -3; o <= 2; ++o) put(o, w * vel * c5(o, s));
+  } else if (f == lo + 1 || f == hi - 2) {
+    put(0, w * 0.5f * vel);
+    put(-1, w * 0.5f * vel);
+  } else if (f == lo + 2 || f == hi - 3) {
+    const float s = sgn1(vel);
+    for (int o = -2; o <= 1; ++o) put(o, w * vel * c3(o, s));
+  }
+}
+
+__device__ inline void w_zface_adj(const AdjAdvW& a, const Dims& D, int kf, int j, int i, float rs, float w) {
+  if (kf < 1 || kf > a.ktf) return;
+  auto put = [&](int o, float c) { atomic_add(&AT(a.gw, kf + o, j, i), w * c); };
+  if (kf == 1 || kf == a.ktf) {
+    put(0, 0.25f * rs);
+    put(-1, 0.25f * rs);
+    return;
+  }
+  const float vel = 0.5f * rs;
+  if (kf >= 3 && kf < a.ktf - 1) {
+    const float s = sgn1(-vel);
+    for (int o = -3; o <= 2; ++o) put(o, vel * c5(o, s));
+  } else {
+    const float s = kf == 2 ? sgn1(-vel) : sgn1(vel);
+    for (int o = -2; o <= 1; ++o) put(o, vel * c3(o, s));
+  }
+}
+
+__global__ void k_advect_w_adj(Box all, Dims D, AdjAdvW a) {
+  int k, j, i;
+  if (!box_point(all, k, j, i)) return;
+  const float lam = AT(a.lam, k, j, i);
+  if (lam == 0.0f) return;
+  const int ktf = a.ktf, top = k >= ktf;
+  const int kA = top ? ktf - 1 : k, kB = imax(top ? ktf - 2 : k - 1, 0);
+  const float mx = AT2(a.msftx, j, i), rdz = a.rdzu[top ? ktf - 1 : k];
+  if (a.by.has(k, j, i)) {
+    const float wy = mx * a.rdy * lam;
+    auto py = [&](int jf) { return [&, jf](int o, float c) { atomic_add(&AT(a.gw, k, jf + o, i), c); }; };
+    w_hface_adj(j + 1, a.jds, a.jde, w_level_flux_a(a, k, AT(a.rv, kA, j + 1, i), AT(a.rv, kB, j + 1, i)), -wy,
+                py(j + 1));
+    w_hface_adj(j, a.jds, a.jde, w_level_flux_a(a, k, AT(a.rv, kA, j, i), AT(a.rv, kB, j, i)), wy, py(j));
+  }
+  if (a.bx.has(k, j, i)) {
+    const float wx = mx * a.rdx * lam;
+    auto px = [&](int f) { return [&, f](int o, float c) { atomic_add(&AT(a.gw, k, j, f + o), c); }; };
+    w_hface_adj(i + 1, a.ids, a.ide, w_level_flux_a(a, k, AT(a.ru, kA, j, i + 1), AT(a.ru, kB, j, i + 1)), -wx,
+                px(i + 1));
+    w_hface_adj(i, a.ids, a.ide, w_level_flux_a(a, k, AT(a.ru, kA, j, i), AT(a.ru, kB, j, i)), wx, px(i));
+  }
+  const float rm0 = AT(a.rom, k, j, i), rmb = AT(a.rom, imax(k - 1, 0), j, i);
+  if (!top) {
+    const float rmn = AT(a.rom, imin(k + 1, D.nz - 1), j, i);
+    w_zface_adj(a, D, k + 1, j, i, rmn + rm0, -rdz * lam);
+    w_zface_adj(a, D, k, j, i, rm0 + rmb, rdz * lam);
+  } else {
+    w_zface_adj(a, D, k, j, i, rm0 + rmb, 2.0f * rdz * lam);
+  }
+}
+
+extern "C" int pywrf_advect_w_adjoint(const pywrf_grid_t* g, const float* lam_tendency, const float* ru,
+                                      const float* rv, const float* rom, const float* msftx, const float* fzm,
+                                      const float* fzp, double rdx, double rdy, const float* rdzu, float* lam_w,
+                                      pywrf_stream_t s) {
+  if (!pw_size_ok(g)) return PYWRF_E_RANGE;
+  CHECK(g, 4);
+  const int ktf = imin(g->kte, g->kde - 1);
+  if (g->kts != 0 || ktf + 1 > g->nz || ktf < 3) return PYWRF_E_RANGE;
+  const int i0 = g->its, i1 = imin(g->ite, g->ide - 1);
+  const int j0 = g->jts, j1 = imin(g->jte, g->jde - 1);
+  AdjAdvW a;
+  a.lam = lam_tendency; a.ru = ru; a.rv = rv; a.rom = rom; a.msftx = msftx; a.fzm = fzm; a.fzp = fzp;
+  a.rdzu = rdzu; a.gw = lam_w; a.rdx = F(rdx); a.rdy = F(rdy);
+  a.ids = g->ids; a.ide = g->ide; a.jds = g->jds; a.jde = g->jde; a.ktf = ktf;
+  a.b = mkbox(g->kts + 1, ktf + 1, j0, j1, i0, i1);
+  a.by = mkbox(g->kts + 1, ktf + 1, imax(g->jts, g->jds + 1), imin(g->jte, g->jde - 2), i0, i1);
+  a.bx = mkbox(g->kts + 1, ktf + 1, j0, j1, imax(g->ids + 1, g->its), imin(g->ide - 2, g->ite));
+  hipLaunchKernelGGL(k_zero_n, dim3(1024), dim3(256), 0, (hipStream_t)s, lam_w, (long)g->nz * g->ny * g->nx);
+  PW_LAUNCH_BOX(k_advect_w_adj, a.b, s, dims_of(g), a);
+  PW_RET();
+}

@@ -852,10 +852,10 @@ def test_acoustic_stage_adjoint_matches_fp64_differences(config, stage):
     assert worst <= 1e-4
 
 
-@pytest.mark.parametrize("which", ["u", "v"])
+@pytest.mark.parametrize("which", ["u", "v", "w"])
 @pytest.mark.parametrize("config", ["config1c", "bdy24", "config4"])
 def test_advect_u_adjoint_matches_oracle_differences(config, which):
-    """advect_u's (advect_v's) gradient with respect to u (v) (the mass fluxes
+    """advect_u's (advect_v's, advect_w's) gradient with respect to u (v, w) (the mass fluxes
     of the step's first call held fixed), A^T lam by the scatter-form adjoint kernel,
     against FP64 central differences of the oracle's advect_u (whose
     in-place face updates autograd cannot follow) along three seeded
@@ -872,8 +872,14 @@ def test_advect_u_adjoint_matches_oracle_differences(config, which):
     rec = _FirstCall(pywrf_amd.ops(), name)
     rk3_step(s, rec, RunConfig())
     args = rec.args
+    if which == "w":
+        # w is zero at the step's start in these states (advect_w's tendency
+        # would be 0): a seeded field of 0.5 m/s stands in for it
+        args = list(args)
+        w0 = torch.randn(tuple(args[0].shape), generator=torch.Generator().manual_seed(5))
+        args[0] = (0.5 * w0).to(args[0].device)
     x = args[0].clone().requires_grad_()
-    T = (adjoint.AdvectU if which == "u" else adjoint.AdvectV).apply(x, args)
+    T = {"u": adjoint.AdvectU, "v": adjoint.AdvectV, "w": adjoint.AdvectW}[which].apply(x, args)
     gen = torch.Generator().manual_seed(31)
     lam = torch.randn(tuple(T.shape), generator=gen, dtype=torch.float64)
     (T * lam.float().cuda()).sum().backward()
@@ -898,7 +904,7 @@ def test_advect_u_adjoint_matches_oracle_differences(config, which):
             for c in (ids + 1, ide - 2):
                 near = u0[..., c].abs() < 10.0 * eps * d[..., c].abs()
                 d[..., c][near] = 0.0
-        else:
+        elif which == "v":
             for r in (jds + 1, jde - 2):
                 near = u0[..., r, :].abs() < 10.0 * eps * d[..., r, :].abs()
                 d[..., r, :][near] = 0.0
