@@ -1102,13 +1102,66 @@ class AcousticStage(torch.autograd.Function):
         return (None, *grads)
 
 
+def time_advection_adjoints(s, repeat=20):
+    """advect_u, advect_v and advect_w (the tendency into a zeroed array) and
+    their adjoints (AdvectU / AdvectV / AdvectW's backward) on the arguments
+    of their first calls in one RK3 step of the state s, each timed over
+    `repeat` calls between HIP events on the current stream.  bytes: the 3D
+    arrays each adjoint entry reads (lam, the field whose sign picks the edge
+    faces' upwind side -- u and v only --, ru, rv, rom) and writes (its
+    gradient) once each, FP32."""
+    from . import ops
+    from .step import rk3_step, RunConfig
+    base, first = ops(), {}
+
+    class _First:
+        def __getattr__(self, n):
+            f = getattr(base, n)
+            w = n[7:-6] if n.startswith("advect_") and n.endswith("_fresh") else None
+            if w not in ("u", "v", "w"):
+                return f
+
+            def run(*args):
+                first.setdefault(w, [x.clone() if isinstance(x, torch.Tensor) else x for x in args])
+                return f(*args)
+            return run
+
+    rk3_step(s, _First(), RunConfig())
+    fns, arrays = {"u": AdvectU, "v": AdvectV, "w": AdvectW}, {"u": 6, "v": 6, "w": 5}
+    out = {}
+    for w in ("u", "v", "w"):
+        args = first[w]
+        x = args[0].clone().requires_grad_()
+        y = fns[w].apply(x, args)
+        lam = torch.randn_like(y)
+
+        def per_call(fn):
+            fn()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(repeat):
+                fn()
+            e1.record()
+            e1.synchronize()
+            return 1e3 * e0.elapsed_time(e1) / repeat
+
+        with torch.no_grad():
+            fwd = per_call(lambda: fns[w].apply(x, args))
+        adj = per_call(lambda: torch.autograd.grad(y, x, lam, retain_graph=True))
+        nbytes = arrays[w] * x.numel() * 4
+        out["advect_" + w] = {"forward_us": fwd, "adjoint_us": adj, "adjoint_bytes": nbytes,
+                              "adjoint_GBs": nbytes / adj / 1e3}
+    return out
+
+
 def main(argv=None):
     """Time the gradient of an RK stage's acoustic loop (--what loop) or of its
     whole acoustic part (--what stage) on a seeded state: the forward, the
     checkpointed forward + backward (AcousticLoop / AcousticStage) and the
-    same with every operand kept, with the device memory each holds.
+    same with every operand kept, with the device memory each holds; or
+    (--what advect) the momentum advection adjoints per call.
 
-        python -m pywrf_amd.adjoint --config config2 --stage 3 --repeat 5 [--what stage]
+        python -m pywrf_amd.adjoint --config config2 --stage 3 --repeat 5 [--what stage|advect]
     """
     import argparse
     import json
@@ -1120,9 +1173,14 @@ def main(argv=None):
     ap.add_argument("--config", default="config2")
     ap.add_argument("--stage", type=int, default=3)
     ap.add_argument("--repeat", type=int, default=5)
-    ap.add_argument("--what", choices=("loop", "stage"), default="loop")
+    ap.add_argument("--what", choices=("loop", "stage", "advect"), default="loop")
     a = ap.parse_args(argv)
     s = synthetic_state(a.config, device="cuda")
+    if a.what == "advect":
+        r = {"what": a.what, "config": a.config, "repeat": a.repeat}
+        r.update(time_advection_adjoints(s, a.repeat))
+        print(json.dumps(r))
+        return
     if a.what == "loop":
         x0, subs = record_acoustic_loop(s, ops(), RunConfig(), stage=a.stage)
         chain, ckpt = (lambda x: acoustic_substeps(x, subs)), (lambda x: AcousticLoop.apply(subs, *x))
