@@ -1334,8 +1334,8 @@ struct AdjAdvU {
 };
 
 // the y-face jf of u column (k, i): weight w times d(face)/d(u at row jf + o)
-__device__ inline void u_yface_adj(const AdjAdvU& a, const Dims& D, int k, int jf, int i, float rvs, float w) {
-  auto put = [&](int o, float c) { atomic_add(&AT(a.gu, k, jf + o, i), w * c); };
+__device__ inline void u_yface_adj(const AdjAdvU& a, int jf, float rvs, float w, float* acc) {
+  auto put = [&](int o, float c) { acc[o] += w * c; };
   if (jf >= a.jds + 3 && jf < a.jde - 3) {
     const float vel = 0.5f * rvs, s = sgn1(vel);
     for (int o = -3; o <= 2; ++o) put(o, vel * c5(o, s));
@@ -1348,8 +1348,8 @@ __device__ inline void u_yface_adj(const AdjAdvU& a, const Dims& D, int k, int j
   }
 }
 
-__device__ inline void u_xface_adj(const AdjAdvU& a, const Dims& D, int k, int j, int f, float rus, float w) {
-  auto put = [&](int o, float c) { atomic_add(&AT(a.gu, k, j, f + o), w * c); };
+__device__ inline void u_xface_adj(const AdjAdvU& a, const Dims& D, int k, int j, int f, float rus, float w, float* acc) {
+  auto put = [&](int o, float c) { acc[o] += w * c; };
   if (f >= a.ids + 3 && f < a.ide - 2) {
     const float vel = 0.5f * rus, s = sgn1(vel);
     for (int o = -3; o <= 2; ++o) put(o, vel * c5(o, s));
@@ -1365,9 +1365,9 @@ __device__ inline void u_xface_adj(const AdjAdvU& a, const Dims& D, int k, int j
   }
 }
 
-__device__ inline void u_zface_adj(const AdjAdvU& a, const Dims& D, int kf, int j, int i, float vel, float w) {
+__device__ inline void u_zface_adj(const AdjAdvU& a, int kf, float vel, float w, float* acc) {
   if (kf < 1 || kf >= a.ktf) return;
-  auto put = [&](int o, float c) { atomic_add(&AT(a.gu, kf + o, j, i), w * c); };
+  auto put = [&](int o, float c) { acc[o] += w * c; };
   if (kf >= 3 && kf < a.ktf - 2) {
     const float s = sgn1(-vel);
     for (int o = -3; o <= 2; ++o) put(o, vel * c5(o, s));
@@ -1386,17 +1386,25 @@ __global__ void k_advect_u_adj(Box all, Dims D, AdjAdvU a) {
   const float lam = AT(a.lam, k, j, i);
   if (lam == 0.0f) return;
   const float msf = AT2(a.msfux, j, i), rdz = a.rdzw[k];
+  // the two faces of each direction read overlapping stencils: their shares
+  // are summed per target (offsets -3..3 from the point) before the atomics
+  float ay[7] = {}, ax[7] = {}, az[7] = {};
   if (a.by.has(k, j, i)) {
     const float wy = msf * a.rdy * lam;
-    u_yface_adj(a, D, k, j + 1, i, AT(a.rv, k, j + 1, i) + AT(a.rv, k, j + 1, i - 1), -wy);
-    u_yface_adj(a, D, k, j, i, AT(a.rv, k, j, i) + AT(a.rv, k, j, i - 1), wy);
+    u_yface_adj(a, j + 1, AT(a.rv, k, j + 1, i) + AT(a.rv, k, j + 1, i - 1), -wy, ay + 4);
+    u_yface_adj(a, j, AT(a.rv, k, j, i) + AT(a.rv, k, j, i - 1), wy, ay + 3);
   }
   const float wx = msf * a.rdx * lam;
-  u_xface_adj(a, D, k, j, i + 1, AT(a.ru, k, j, i + 1) + AT(a.ru, k, j, i), -wx);
-  u_xface_adj(a, D, k, j, i, AT(a.ru, k, j, i) + AT(a.ru, k, j, i - 1), wx);
+  u_xface_adj(a, D, k, j, i + 1, AT(a.ru, k, j, i + 1) + AT(a.ru, k, j, i), -wx, ax + 4);
+  u_xface_adj(a, D, k, j, i, AT(a.ru, k, j, i) + AT(a.ru, k, j, i - 1), wx, ax + 3);
   const float wz = rdz * lam;
-  u_zface_adj(a, D, k + 1, j, i, 0.5f * (AT(a.rom, k + 1, j, i - 1) + AT(a.rom, k + 1, j, i)), -wz);
-  u_zface_adj(a, D, k, j, i, 0.5f * (AT(a.rom, k, j, i - 1) + AT(a.rom, k, j, i)), wz);
+  u_zface_adj(a, k + 1, 0.5f * (AT(a.rom, k + 1, j, i - 1) + AT(a.rom, k + 1, j, i)), -wz, az + 4);
+  u_zface_adj(a, k, 0.5f * (AT(a.rom, k, j, i - 1) + AT(a.rom, k, j, i)), wz, az + 3);
+  for (int m = 0; m < 7; ++m) {          // a nonzero share is one a face stencil read: in bounds
+    atomic_add(&AT(a.gu, k, j + m - 3, i), ay[m]);
+    atomic_add(&AT(a.gu, k, j, i + m - 3), ax[m]);
+    atomic_add(&AT(a.gu, k + m - 3, j, i), az[m]);
+  }
 }
 
 extern "C" int pywrf_advect_u_adjoint(const pywrf_grid_t* g, const float* lam_tendency, const float* u,
@@ -1433,8 +1441,8 @@ struct AdjAdvV {
   Box b, bx;
 };
 
-__device__ inline void v_yface_adj(const AdjAdvV& a, const Dims& D, int k, int f, int i, float rvs, float w) {
-  auto put = [&](int o, float c) { atomic_add(&AT(a.gv, k, f + o, i), w * c); };
+__device__ inline void v_yface_adj(const AdjAdvV& a, const Dims& D, int k, int f, int i, float rvs, float w, float* acc) {
+  auto put = [&](int o, float c) { acc[o] += w * c; };
   if (f >= a.jds + 3 && f < a.jde - 2) {
     const float vel = 0.5f * rvs, s = sgn1(vel);
     for (int o = -3; o <= 2; ++o) put(o, vel * c5(o, s));
@@ -1450,8 +1458,8 @@ __device__ inline void v_yface_adj(const AdjAdvV& a, const Dims& D, int k, int f
   }
 }
 
-__device__ inline void v_xface_adj(const AdjAdvV& a, const Dims& D, int k, int j, int f, float rus, float w) {
-  auto put = [&](int o, float c) { atomic_add(&AT(a.gv, k, j, f + o), w * c); };
+__device__ inline void v_xface_adj(const AdjAdvV& a, const Dims& D, int k, int j, int f, float rus, float w, float* acc) {
+  auto put = [&](int o, float c) { acc[o] += w * c; };
   if (f >= a.ids + 3 && f < a.ide - 3) {
     const float vel = 0.5f * rus, s = sgn1(vel);
     for (int o = -3; o <= 2; ++o) put(o, vel * c5(o, s));
@@ -1464,9 +1472,9 @@ __device__ inline void v_xface_adj(const AdjAdvV& a, const Dims& D, int k, int j
   }
 }
 
-__device__ inline void v_zface_adj(const AdjAdvV& a, const Dims& D, int kf, int j, int i, float vel, float w) {
+__device__ inline void v_zface_adj(const AdjAdvV& a, const Dims& D, int kf, int j, int i, float vel, float w, float* acc) {
   if (kf < 1 || kf >= a.ktf) return;
-  auto put = [&](int o, float c) { atomic_add(&AT(a.gv, kf + o, j, i), w * c); };
+  auto put = [&](int o, float c) { acc[o] += w * c; };
   if (kf >= 3 && kf < a.ktf - 2) {
     const float s = sgn1(-vel);
     for (int o = -3; o <= 2; ++o) put(o, vel * c5(o, s));
@@ -1485,17 +1493,24 @@ __global__ void k_advect_v_adj(Box all, Dims D, AdjAdvV a) {
   const float lam = AT(a.lam, k, j, i);
   if (lam == 0.0f) return;
   const float my = AT2(a.msfvy, j, i), mvx = AT2(a.msfvx, j, i), rdz = a.rdzw[k];
+  // shares summed per target (offsets -3..3 from the point) before the atomics
+  float ay[7] = {}, ax[7] = {}, az[7] = {};
   const float wy = my * a.rdy * lam;
-  v_yface_adj(a, D, k, j + 1, i, AT(a.rv, k, j + 1, i) + AT(a.rv, k, j, i), -wy);
-  v_yface_adj(a, D, k, j, i, AT(a.rv, k, j, i) + AT(a.rv, k, j - 1, i), wy);
+  v_yface_adj(a, D, k, j + 1, i, AT(a.rv, k, j + 1, i) + AT(a.rv, k, j, i), -wy, ay + 4);
+  v_yface_adj(a, D, k, j, i, AT(a.rv, k, j, i) + AT(a.rv, k, j - 1, i), wy, ay + 3);
   if (a.bx.has(k, j, i)) {
     const float wx = my * a.rdx * lam;
-    v_xface_adj(a, D, k, j, i + 1, AT(a.ru, k, j, i + 1) + AT(a.ru, k, j - 1, i + 1), -wx);
-    v_xface_adj(a, D, k, j, i, AT(a.ru, k, j, i) + AT(a.ru, k, j - 1, i), wx);
+    v_xface_adj(a, D, k, j, i + 1, AT(a.ru, k, j, i + 1) + AT(a.ru, k, j - 1, i + 1), -wx, ax + 4);
+    v_xface_adj(a, D, k, j, i, AT(a.ru, k, j, i) + AT(a.ru, k, j - 1, i), wx, ax + 3);
   }
   const float wz = (my / mvx) * rdz * lam;
-  v_zface_adj(a, D, k + 1, j, i, 0.5f * (AT(a.rom, k + 1, j, i) + AT(a.rom, k + 1, j - 1, i)), -wz);
-  v_zface_adj(a, D, k, j, i, 0.5f * (AT(a.rom, k, j, i) + AT(a.rom, k, j - 1, i)), wz);
+  v_zface_adj(a, D, k + 1, j, i, 0.5f * (AT(a.rom, k + 1, j, i) + AT(a.rom, k + 1, j - 1, i)), -wz, az + 4);
+  v_zface_adj(a, D, k, j, i, 0.5f * (AT(a.rom, k, j, i) + AT(a.rom, k, j - 1, i)), wz, az + 3);
+  for (int m = 0; m < 7; ++m) {          // a nonzero share is one a face stencil read: in bounds
+    atomic_add(&AT(a.gv, k, j + m - 3, i), ay[m]);
+    atomic_add(&AT(a.gv, k, j, i + m - 3), ax[m]);
+    atomic_add(&AT(a.gv, k + m - 3, j, i), az[m]);
+  }
 }
 
 extern "C" int pywrf_advect_v_adjoint(const pywrf_grid_t* g, const float* lam_tendency, const float* v,
