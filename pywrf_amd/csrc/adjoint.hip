@@ -1568,9 +1568,9 @@ __device__ inline void w_hface_adj(int f, int lo, int hi, float vel, float w, Pu
   }
 }
 
-__device__ inline void w_zface_adj(const AdjAdvW& a, const Dims& D, int kf, int j, int i, float rs, float w) {
+__device__ inline void w_zface_adj(const AdjAdvW& a, int kf, float rs, float w, float* acc) {
   if (kf < 1 || kf > a.ktf) return;
-  auto put = [&](int o, float c) { atomic_add(&AT(a.gw, kf + o, j, i), w * c); };
+  auto put = [&](int o, float c) { acc[o] += w * c; };
   if (kf == 1 || kf == a.ktf) {
     put(0, 0.25f * rs);
     put(-1, 0.25f * rs);
@@ -1594,27 +1594,33 @@ __global__ void k_advect_w_adj(Box all, Dims D, AdjAdvW a) {
   const int ktf = a.ktf, top = k >= ktf;
   const int kA = top ? ktf - 1 : k, kB = imax(top ? ktf - 2 : k - 1, 0);
   const float mx = AT2(a.msftx, j, i), rdz = a.rdzu[top ? ktf - 1 : k];
+  // shares summed per target (offsets -3..3 from the point) before the atomics
+  float ay[7] = {}, ax[7] = {}, az[7] = {};
+  auto into = [](float* acc) { return [acc](int o, float c) { acc[o] += c; }; };
   if (a.by.has(k, j, i)) {
     const float wy = mx * a.rdy * lam;
-    auto py = [&](int jf) { return [&, jf](int o, float c) { atomic_add(&AT(a.gw, k, jf + o, i), c); }; };
     w_hface_adj(j + 1, a.jds, a.jde, w_level_flux_a(a, k, AT(a.rv, kA, j + 1, i), AT(a.rv, kB, j + 1, i)), -wy,
-                py(j + 1));
-    w_hface_adj(j, a.jds, a.jde, w_level_flux_a(a, k, AT(a.rv, kA, j, i), AT(a.rv, kB, j, i)), wy, py(j));
+                into(ay + 4));
+    w_hface_adj(j, a.jds, a.jde, w_level_flux_a(a, k, AT(a.rv, kA, j, i), AT(a.rv, kB, j, i)), wy, into(ay + 3));
   }
   if (a.bx.has(k, j, i)) {
     const float wx = mx * a.rdx * lam;
-    auto px = [&](int f) { return [&, f](int o, float c) { atomic_add(&AT(a.gw, k, j, f + o), c); }; };
     w_hface_adj(i + 1, a.ids, a.ide, w_level_flux_a(a, k, AT(a.ru, kA, j, i + 1), AT(a.ru, kB, j, i + 1)), -wx,
-                px(i + 1));
-    w_hface_adj(i, a.ids, a.ide, w_level_flux_a(a, k, AT(a.ru, kA, j, i), AT(a.ru, kB, j, i)), wx, px(i));
+                into(ax + 4));
+    w_hface_adj(i, a.ids, a.ide, w_level_flux_a(a, k, AT(a.ru, kA, j, i), AT(a.ru, kB, j, i)), wx, into(ax + 3));
   }
   const float rm0 = AT(a.rom, k, j, i), rmb = AT(a.rom, imax(k - 1, 0), j, i);
   if (!top) {
     const float rmn = AT(a.rom, imin(k + 1, D.nz - 1), j, i);
-    w_zface_adj(a, D, k + 1, j, i, rmn + rm0, -rdz * lam);
-    w_zface_adj(a, D, k, j, i, rm0 + rmb, rdz * lam);
+    w_zface_adj(a, k + 1, rmn + rm0, -rdz * lam, az + 4);
+    w_zface_adj(a, k, rm0 + rmb, rdz * lam, az + 3);
   } else {
-    w_zface_adj(a, D, k, j, i, rm0 + rmb, 2.0f * rdz * lam);
+    w_zface_adj(a, k, rm0 + rmb, 2.0f * rdz * lam, az + 3);
+  }
+  for (int m = 0; m < 7; ++m) {          // a nonzero share is one a face stencil read: in bounds
+    atomic_add(&AT(a.gw, k, j + m - 3, i), ay[m]);
+    atomic_add(&AT(a.gw, k, j, i + m - 3), ax[m]);
+    atomic_add(&AT(a.gw, k + m - 3, j, i), az[m]);
   }
 }
 
